@@ -1,0 +1,127 @@
+"""ctypes binding of the C ABI in include/atdf.h (libatdf.so).
+
+Pointers are device addresses (``tensor.data_ptr()``); the stream is torch's
+current HIP stream unless one is given.  Every call raises ``AtdfError`` on a
+non-zero status, with the library's message.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+from . import LIB_PATH
+
+OK = 0
+ERR_INVALID_ARGUMENT = 1
+ERR_CAPACITY = 2
+ERR_TOO_MANY_INTERSECTIONS = 3
+ERR_ORDER = 4
+ERR_HIP = 5
+ERR_UNSUPPORTED = 6
+
+GRID_F32 = 0
+GRID_BF16 = 1
+
+
+class AtdfError(RuntimeError):
+    def __init__(self, status, msg):
+        super().__init__(f"atdf status {status}: {msg}")
+        self.status = status
+
+
+class Tree(C.Structure):
+    _fields_ = [("num_nodes", C.c_int64), ("node_position_min", C.c_void_p), ("node_position_max", C.c_void_p),
+                ("node_children", C.c_void_p), ("node_parent", C.c_void_p), ("node_active", C.c_void_p)]
+
+
+class Sampling(C.Structure):
+    _fields_ = [("max_samples_per_node", C.c_int32), ("max_intersections", C.c_int32), ("jitter_mode", C.c_int32),
+                ("reserved", C.c_int32), ("jitter_seed", C.c_uint64), ("jitter", C.c_void_p)]
+
+
+class Intervals(C.Structure):
+    _fields_ = [("node", C.c_void_p), ("tmin", C.c_void_p), ("tmax", C.c_void_p), ("count", C.c_void_p),
+                ("first", C.c_void_p), ("ray", C.c_void_p)]
+
+
+class Samples(C.Structure):
+    _fields_ = [("global_coordinate", C.c_void_p), ("weight", C.c_void_p), ("ray_t", C.c_void_p),
+                ("ray_index", C.c_void_p), ("local_index_in_ray", C.c_void_p), ("node_id", C.c_void_p)]
+
+
+class Field(C.Structure):
+    _fields_ = [("C", C.c_int32), ("H", C.c_int32), ("G", C.c_int32), ("grid_dtype", C.c_int32),
+                ("softplus_beta", C.c_float), ("rays_per_wave", C.c_int32), ("grid", C.c_void_p),
+                ("num_grids", C.c_int64), ("node_to_grid", C.c_void_p), ("mlp", C.c_void_p)]
+
+
+_P, _I32, _I64, _F, _SZ, _U64 = C.c_void_p, C.c_int32, C.c_int64, C.c_float, C.c_size_t, C.c_uint64
+
+# name -> (restype, argtypes)
+_SIGNATURES = {
+    "atdf_last_error": (C.c_char_p, []),
+    "atdf_version": (C.c_char_p, []),
+    "atdf_tree_packed_bytes": (_SZ, [_I64]),
+    "atdf_tree_pack": (_I32, [C.POINTER(Tree), _P, _P]),
+    "atdf_traverse_count": (_I32, [_P, _I64, _P, _P, _I64, C.POINTER(Sampling), _P, _P, _P, _P]),
+    "atdf_scan_workspace_bytes": (_SZ, [_I64]),
+    "atdf_exclusive_scan_i32": (_I32, [_P, _I64, _P, _P, _SZ, _P]),
+    "atdf_traverse_write": (_I32, [_P, _I64, _P, _P, _I64, C.POINTER(Sampling), _P, _P, C.POINTER(Intervals),
+                                   C.POINTER(Samples), _P]),
+    "atdf_local_samples": (_I32, [_P, _P, _P, _P, _I64, _P, _P]),
+    "atdf_grid_sample3d_fwd": (_I32, [_P, _I64, _I32, _I32, _I32, _I32, _P, _P, _I64, _P, _P]),
+    "atdf_grid_sample3d_bwd": (_I32, [_P, _I64, _I32, _I32, _I32, _I32, _P, _P, _I64, _P, _P, _P, _P]),
+    "atdf_integrate_xray": (_I32, [_P, _P, _P, _I64, _I32, _I64, _P, _P]),
+    "atdf_accumulate_loss_per_node": (_I32, [_P, _I32, _I64, _P, _P, _P, _I64, _I64, _P, _P, _P]),
+    "atdf_project_fwd": (_I32, [_P, _I64, _P, _P, _I64, C.POINTER(Sampling), _P, C.POINTER(Intervals),
+                                C.POINTER(Field), _P, _P]),
+    "atdf_project_bwd_workspace_bytes": (_SZ, [_I64, C.POINTER(Field)]),
+    "atdf_project_bwd": (_I32, [_P, _I64, _P, _P, _I64, C.POINTER(Sampling), _P, C.POINTER(Intervals),
+                                C.POINTER(Field), _P, _P, _P, _P, _SZ, _P]),
+    "atdf_mse_workspace_bytes": (_SZ, [_I64]),
+    "atdf_mse_loss": (_I32, [_P, _P, _I64, _F, _P, _P, _P, _SZ, _P]),
+    "atdf_adam_step": (_I32, [_P, _P, _P, _P, _I64, _F, _F, _F, _F, _I64, _I32, _P]),
+    "atdf_adam_step_bf16": (_I32, [_P, _P, _P, _P, _P, _I64, _F, _F, _F, _F, _I64, _I32, _P]),
+    "atdf_grid_layout": (_I32, [_P, _P, _I64, _I32, _I32, _I32, _P]),
+    "atdf_f32_to_bf16": (_I32, [_P, _P, _I64, _P]),
+}
+
+_lib = None
+
+
+def lib():
+    """Load libatdf.so and declare every exported entry point."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"{LIB_PATH} is not built; run `python -m adaptivetensordf_amd.build`")
+        handle = C.CDLL(LIB_PATH)
+        for name, (res, args) in _SIGNATURES.items():
+            fn = getattr(handle, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = handle
+    return _lib
+
+
+def exported_symbols():
+    return list(_SIGNATURES)
+
+
+def stream_ptr(stream=None):
+    import torch
+
+    s = stream if stream is not None else torch.cuda.current_stream()
+    return C.c_void_p(s.cuda_stream)
+
+
+def ptr(t):
+    """Device pointer of a tensor (None stays NULL)."""
+    return None if t is None else C.c_void_p(t.data_ptr())
+
+
+def call(name, *args):
+    st = getattr(lib(), name)(*args)
+    if st != OK:
+        raise AtdfError(st, lib().atdf_last_error().decode())
+    return st

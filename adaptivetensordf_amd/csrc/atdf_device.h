@@ -1,0 +1,211 @@
+// atdf_device.h -- device-side building blocks shared by the gfx950 kernels.
+//
+// Everything here that feeds traversal, sample placement, local coordinates
+// or the trilinear weights is evaluated in the written order with no FMA
+// contraction (the library is compiled with -ffp-contract=off), so those
+// values are bit-identical to a C restatement compiled the same way.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace atdf {
+
+// Packed node used by the traversal kernels (48 B, three 16-byte loads).
+struct alignas(16) PNode {
+    float mn[3];
+    int32_t first_child;  // -1 for a leaf
+    float mx[3];
+    int32_t parent;       // -1 for the root
+    int32_t flags;        // bit 0: active, bit 1: has an active descendant
+    int32_t pad[3];
+};
+static_assert(sizeof(PNode) == 48, "PNode must stay 48 bytes");
+
+constexpr int kFlagActive = 1;
+constexpr int kFlagActiveBelow = 2;
+
+// Slab test of structure/HyperTree.h:338-380: float divisions, running
+// near/far values kept in double, the parallel-axis test asymmetric
+// (o < min or o >= max misses).
+__device__ __forceinline__ bool intersect_box_ray(const float* bmin, const float* bmax, const float* o,
+                                                  const float* d, float& tmin, float& tmax)
+{
+    double t_near = -236436436;
+    double t_far = 43637575;
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+        if (d[i] == 0) {
+            if ((o[i] < bmin[i]) || (o[i] >= bmax[i])) {
+                tmin = 0;
+                tmax = 0;
+                return false;
+            }
+        } else {
+            float T_1 = (bmin[i] - o[i]) / d[i];
+            float T_2 = (bmax[i] - o[i]) / d[i];
+            if (T_1 > T_2) {
+                float tmp = T_1;
+                T_1 = T_2;
+                T_2 = tmp;
+            }
+            if (T_1 > t_near) t_near = T_1;
+            if (T_2 < t_far) t_far = T_2;
+            if ((t_near > t_far) || (t_far < 0)) {
+                tmin = 0;
+                tmax = 0;
+                return false;
+            }
+        }
+    }
+    tmin = (float)t_near;
+    tmax = (float)t_far;
+    return true;
+}
+
+__device__ __forceinline__ int iceil_f(float v)
+{
+    int iv = (int)v;
+    return (v > (float)iv) ? iv + 1 : iv;
+}
+
+// Samples of one hit node: n = ceil((tmax - tmin) / |max - min| * K)
+// (structure/HyperTree.cu:350-356).
+__device__ __forceinline__ int node_sample_count(const float* bmin, const float* bmax, float tmin, float tmax, int K)
+{
+    float dx = bmax[0] - bmin[0];
+    float dy = bmax[1] - bmin[1];
+    float dz = bmax[2] - bmin[2];
+    float diag = sqrtf(dx * dx + dy * dy + dz * dz);
+    float dis = tmax - tmin;
+    float rel = dis / diag;
+    return iceil_f(rel * (float)K);
+}
+
+__device__ __forceinline__ uint64_t splitmix64(uint64_t x)
+{
+    x += 0x9E3779B97F4A7C15ull;
+    x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+    x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+    return x ^ (x >> 31);
+}
+
+// Stratified jitter, 24-bit uniform in [0,1), keyed by (seed, ray, index in ray).
+__device__ __forceinline__ float hash_jitter(uint64_t seed, int64_t ray, int64_t local_index)
+{
+    uint64_t key = ((uint64_t)ray << 24) ^ (uint64_t)local_index;
+    uint64_t h = splitmix64(seed ^ splitmix64(key));
+    return (float)(h >> 40) * (1.0f / 16777216.0f);
+}
+
+// t of sample j in [tmin, tmax] split into n strata (structure/HyperTree.cu:466-474).
+__device__ __forceinline__ float sample_t(float tmin, float step, int j, float a)
+{
+    float t1 = tmin + (float)j * step;
+    float t2 = tmin + (float)(j + 1) * step;
+    return t1 * (1 - a) + t2 * a;
+}
+
+// Midpoint-rule weight of sample j (structure/HyperTree.cu:495-512).
+__device__ __forceinline__ float sample_weight(float tmin, float tmax, int n, int j, float t_prev, float t,
+                                               float t_next)
+{
+    float lo = (j == 0) ? tmin : (t_prev + t) * 0.5f;
+    float hi = (j == n - 1) ? tmax : (t_next + t) * 0.5f;
+    return hi - lo;
+}
+
+__device__ __forceinline__ int clampi(int v, int lim)
+{
+    int a = v > 0 ? v : 0;
+    return a < (lim - 1) ? a : (lim - 1);
+}
+
+// Trilinear setup with align_corners=True and clamped corners
+// (modules/IndirectGridSample2D.cu:40-91 extended to 3-D).  Weights are in
+// torch grid_sample order tnw tne tsw tse bnw bne bsw bse (t/b = z0/z1,
+// n/s = y0/y1, w/e = x0/x1), computed from the unclamped corner indices.
+struct Tri {
+    int x0, x1, y0, y1, z0, z1;
+    float w[8];
+};
+
+__device__ __forceinline__ void tri_setup(float u, float v, float s, int Z, int Y, int X, Tri& t)
+{
+    float ix = ((u + 1) / 2) * (float)(X - 1);
+    float iy = ((v + 1) / 2) * (float)(Y - 1);
+    float iz = ((s + 1) / 2) * (float)(Z - 1);
+    int ix0 = (int)floorf(ix), iy0 = (int)floorf(iy), iz0 = (int)floorf(iz);
+    int ix1 = ix0 + 1, iy1 = iy0 + 1, iz1 = iz0 + 1;
+    float fx1 = (float)ix1 - ix, fx0 = ix - (float)ix0;
+    float fy1 = (float)iy1 - iy, fy0 = iy - (float)iy0;
+    float fz1 = (float)iz1 - iz, fz0 = iz - (float)iz0;
+    t.w[0] = fx1 * fy1 * fz1;
+    t.w[1] = fx0 * fy1 * fz1;
+    t.w[2] = fx1 * fy0 * fz1;
+    t.w[3] = fx0 * fy0 * fz1;
+    t.w[4] = fx1 * fy1 * fz0;
+    t.w[5] = fx0 * fy1 * fz0;
+    t.w[6] = fx1 * fy0 * fz0;
+    t.w[7] = fx0 * fy0 * fz0;
+    t.x0 = clampi(ix0, X);
+    t.x1 = clampi(ix1, X);
+    t.y0 = clampi(iy0, Y);
+    t.y1 = clampi(iy1, Y);
+    t.z0 = clampi(iz0, Z);
+    t.z1 = clampi(iz1, Z);
+}
+
+// Partial derivatives of the 8 weights w.r.t. (ix, iy, iz).
+__device__ __forceinline__ void tri_dweights(float u, float v, float s, int Z, int Y, int X, float dw[8][3])
+{
+    float ix = ((u + 1) / 2) * (float)(X - 1);
+    float iy = ((v + 1) / 2) * (float)(Y - 1);
+    float iz = ((s + 1) / 2) * (float)(Z - 1);
+    int ix0 = (int)floorf(ix), iy0 = (int)floorf(iy), iz0 = (int)floorf(iz);
+    float fx[2] = {(float)(ix0 + 1) - ix, ix - (float)ix0};
+    float fy[2] = {(float)(iy0 + 1) - iy, iy - (float)iy0};
+    float fz[2] = {(float)(iz0 + 1) - iz, iz - (float)iz0};
+#pragma unroll
+    for (int c = 0; c < 8; ++c) {
+        int bx = c & 1, by = (c >> 1) & 1, bz = (c >> 2) & 1;
+        float sx = bx ? 1.f : -1.f, sy = by ? 1.f : -1.f, sz = bz ? 1.f : -1.f;
+        dw[c][0] = sx * fy[by] * fz[bz];
+        dw[c][1] = fx[bx] * sy * fz[bz];
+        dw[c][2] = fx[bx] * fy[by] * sz;
+    }
+}
+
+// softplus(y, beta) with torch's threshold 20 (geometry/geometry.cpp:210-213).
+__device__ __forceinline__ float softplus_f(float y, float beta)
+{
+    float yb = y * beta;
+    if (yb > 20.0f) return y;
+    return log1pf(expf(yb)) / beta;
+}
+
+__device__ __forceinline__ float softplus_grad_f(float y, float beta)
+{
+    float yb = y * beta;
+    if (yb > 20.0f) return 1.0f;
+    float z = expf(yb);
+    return z / (z + 1.0f);
+}
+
+__device__ __forceinline__ float bf16_to_f32(uint16_t h)
+{
+    return __uint_as_float(((uint32_t)h) << 16);
+}
+
+__device__ __forceinline__ uint16_t f32_to_bf16_rne(float f)
+{
+    uint32_t u = __float_as_uint(f);
+    if ((u & 0x7f800000u) == 0x7f800000u) {  // inf / nan
+        return (uint16_t)((u >> 16) | ((u & 0xffffu) ? 0x40u : 0u));
+    }
+    uint32_t lsb = (u >> 16) & 1u;
+    u += 0x7fffu + lsb;
+    return (uint16_t)(u >> 16);
+}
+
+}  // namespace atdf

@@ -1,0 +1,393 @@
+// atdf_ops.hip -- the per-sample operators of the reference API (local
+// coordinates, indirect 3-D gather and its backward, line integral, per-node
+// FP64 loss), the MSE loss, Adam, and grid layout conversion.
+#include <hip/hip_runtime.h>
+
+#include "atdf_device.h"
+#include "atdf_internal.h"
+
+namespace atdf {
+
+__global__ __launch_bounds__(256) void local_samples_kernel(const float* __restrict__ pmin, const float* __restrict__ pmax,
+                                                            const float* __restrict__ g,
+                                                            const int64_t* __restrict__ node, int64_t S,
+                                                            float* __restrict__ local)
+{
+    int64_t s = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (s >= S) return;
+    int64_t nd = node[s];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        float mn = pmin[nd * 3 + k];
+        float mx = pmax[nd * 3 + k];
+        float size = mx - mn;
+        local[s * 3 + k] = ((g[s * 3 + k] - mn) / size) * 2 - 1;
+    }
+}
+
+// One thread per sample, grid [N, C, Z, Y, X] (reference layout).
+__global__ __launch_bounds__(256) void grid_sample3d_fwd_kernel(const float* __restrict__ grid, int C, int Z, int Y,
+                                                                int X, const int64_t* __restrict__ index,
+                                                                const float* __restrict__ uv, int64_t S,
+                                                                float* __restrict__ out)
+{
+    int64_t s = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (s >= S) return;
+    Tri t;
+    tri_setup(uv[s * 3 + 0], uv[s * 3 + 1], uv[s * 3 + 2], Z, Y, X, t);
+    const int64_t plane = (int64_t)Z * Y * X;
+    const float* g = grid + index[s] * C * plane;
+    const int64_t off[8] = {((int64_t)t.z0 * Y + t.y0) * X + t.x0, ((int64_t)t.z0 * Y + t.y0) * X + t.x1,
+                            ((int64_t)t.z0 * Y + t.y1) * X + t.x0, ((int64_t)t.z0 * Y + t.y1) * X + t.x1,
+                            ((int64_t)t.z1 * Y + t.y0) * X + t.x0, ((int64_t)t.z1 * Y + t.y0) * X + t.x1,
+                            ((int64_t)t.z1 * Y + t.y1) * X + t.x0, ((int64_t)t.z1 * Y + t.y1) * X + t.x1};
+    for (int c = 0; c < C; ++c) {
+        const float* gc = g + c * plane;
+        float acc = 0;
+#pragma unroll
+        for (int k = 0; k < 8; ++k) acc += gc[off[k]] * t.w[k];
+        out[s * C + c] = acc;
+    }
+}
+
+__global__ __launch_bounds__(256) void grid_sample3d_bwd_kernel(const float* __restrict__ grid, int C, int Z, int Y,
+                                                                int X, const int64_t* __restrict__ index,
+                                                                const float* __restrict__ uv, int64_t S,
+                                                                const float* __restrict__ grad_out,
+                                                                float* __restrict__ grad_grid,
+                                                                float* __restrict__ grad_uv)
+{
+    int64_t s = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (s >= S) return;
+    const float u = uv[s * 3 + 0], v = uv[s * 3 + 1], w = uv[s * 3 + 2];
+    Tri t;
+    tri_setup(u, v, w, Z, Y, X, t);
+    const int64_t plane = (int64_t)Z * Y * X;
+    const int64_t base = index[s] * C * plane;
+    const int64_t off[8] = {((int64_t)t.z0 * Y + t.y0) * X + t.x0, ((int64_t)t.z0 * Y + t.y0) * X + t.x1,
+                            ((int64_t)t.z0 * Y + t.y1) * X + t.x0, ((int64_t)t.z0 * Y + t.y1) * X + t.x1,
+                            ((int64_t)t.z1 * Y + t.y0) * X + t.x0, ((int64_t)t.z1 * Y + t.y0) * X + t.x1,
+                            ((int64_t)t.z1 * Y + t.y1) * X + t.x0, ((int64_t)t.z1 * Y + t.y1) * X + t.x1};
+    float dw[8][3];
+    if (grad_uv) tri_dweights(u, v, w, Z, Y, X, dw);
+    float gu[3] = {0.f, 0.f, 0.f};
+    for (int c = 0; c < C; ++c) {
+        float go = grad_out[s * C + c];
+        float* gg = grad_grid + base + c * plane;
+#pragma unroll
+        for (int k = 0; k < 8; ++k) unsafeAtomicAdd(gg + off[k], t.w[k] * go);
+        if (grad_uv) {
+            const float* gc = grid + base + c * plane;
+#pragma unroll
+            for (int k = 0; k < 8; ++k) {
+                float val = gc[off[k]] * go;
+                gu[0] += val * dw[k][0];
+                gu[1] += val * dw[k][1];
+                gu[2] += val * dw[k][2];
+            }
+        }
+    }
+    if (grad_uv) {
+        grad_uv[s * 3 + 0] += gu[0] * ((float)(X - 1) / 2);
+        grad_uv[s * 3 + 1] += gu[1] * ((float)(Y - 1) / 2);
+        grad_uv[s * 3 + 2] += gu[2] * ((float)(Z - 1) / 2);
+    }
+}
+
+__global__ __launch_bounds__(256) void integrate_kernel(const float* __restrict__ v, const float* __restrict__ w,
+                                                        const int64_t* __restrict__ ray, int64_t S, int C, int64_t R,
+                                                        float* __restrict__ out)
+{
+    int64_t s = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (s >= S) return;
+    int64_t r = ray[s];
+    float ws = w[s];
+    for (int c = 0; c < C; ++c) unsafeAtomicAdd(out + c * R + r, v[s * C + c] * ws);
+}
+
+__global__ __launch_bounds__(256) void loss_per_node_kernel(const float* __restrict__ prl, int C, int64_t R,
+                                                            const int64_t* __restrict__ ray,
+                                                            const float* __restrict__ w,
+                                                            const int64_t* __restrict__ node, int64_t S,
+                                                            double* __restrict__ loss_sum,
+                                                            double* __restrict__ weight_sum)
+{
+    int64_t s = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (s >= S) return;
+    int64_t r = ray[s];
+    double l = 0;
+    for (int c = 0; c < C; ++c) l += (double)prl[c * R + r];
+    l /= C;
+    double ws = (double)w[s];
+    unsafeAtomicAdd(loss_sum + node[s], l * ws);
+    unsafeAtomicAdd(weight_sum + node[s], ws);
+}
+
+// ---------------------------------------------------------------------------
+// MSE: per-block FP64 partial sums, then one block reduces them in a fixed
+// order (deterministic).
+// ---------------------------------------------------------------------------
+constexpr int kRedThreads = 256;
+
+__device__ __forceinline__ double block_sum_d(double v, double* smem)
+{
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (lane == 0) smem[wave] = v;
+    __syncthreads();
+    double t = 0;
+    if (threadIdx.x == 0)
+        for (int w = 0; w < kRedThreads / 64; ++w) t += smem[w];
+    __syncthreads();
+    return t;
+}
+
+__global__ __launch_bounds__(kRedThreads) void mse_partial_kernel(const float* __restrict__ P,
+                                                                  const float* __restrict__ T, int64_t R,
+                                                                  float scale, float* __restrict__ dLdP,
+                                                                  double* __restrict__ partial)
+{
+    __shared__ double smem[kRedThreads / 64];
+    double acc = 0;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < R; i += (int64_t)gridDim.x * blockDim.x) {
+        float diff = P[i] - T[i];
+        acc += (double)diff * (double)diff;
+        if (dLdP) dLdP[i] = scale * (2.0f * diff / (float)R);
+    }
+    double t = block_sum_d(acc, smem);
+    if (threadIdx.x == 0) partial[blockIdx.x] = t;
+}
+
+__global__ __launch_bounds__(kRedThreads) void mse_final_kernel(const double* __restrict__ partial, int nb, int64_t R,
+                                                                float scale, double* __restrict__ loss)
+{
+    __shared__ double smem[kRedThreads / 64];
+    double acc = 0;
+    for (int i = threadIdx.x; i < nb; i += blockDim.x) acc += partial[i];
+    double t = block_sum_d(acc, smem);
+    if (threadIdx.x == 0) *loss = (double)scale * t / (double)R;
+}
+
+// ---------------------------------------------------------------------------
+// Adam (torch::optim::Adam defaults: no weight decay, no amsgrad)
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void adam_kernel(float* __restrict__ p, float* __restrict__ g,
+                                                   float* __restrict__ m, float* __restrict__ v, int64_t n, float lr,
+                                                   float b1, float b2, float eps, float bc1, float bc2_sqrt,
+                                                   int zero_grad)
+{
+    int64_t i4 = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) * 4;
+    if (i4 + 3 < n) {
+        float4 gg = *reinterpret_cast<float4*>(g + i4);
+        float4 mm = *reinterpret_cast<float4*>(m + i4);
+        float4 vv = *reinterpret_cast<float4*>(v + i4);
+        float4 pp = *reinterpret_cast<float4*>(p + i4);
+        float* gp = &gg.x;
+        float* mp = &mm.x;
+        float* vp = &vv.x;
+        float* pq = &pp.x;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            mp[k] = b1 * mp[k] + (1.f - b1) * gp[k];
+            vp[k] = b2 * vp[k] + (1.f - b2) * gp[k] * gp[k];
+            float denom = sqrtf(vp[k]) / bc2_sqrt + eps;
+            pq[k] = pq[k] - (lr / bc1) * (mp[k] / denom);
+        }
+        *reinterpret_cast<float4*>(m + i4) = mm;
+        *reinterpret_cast<float4*>(v + i4) = vv;
+        *reinterpret_cast<float4*>(p + i4) = pp;
+        if (zero_grad) *reinterpret_cast<float4*>(g + i4) = make_float4(0.f, 0.f, 0.f, 0.f);
+    } else {
+        for (int64_t i = i4; i < n; ++i) {
+            float gi = g[i];
+            m[i] = b1 * m[i] + (1.f - b1) * gi;
+            v[i] = b2 * v[i] + (1.f - b2) * gi * gi;
+            float denom = sqrtf(v[i]) / bc2_sqrt + eps;
+            p[i] = p[i] - (lr / bc1) * (m[i] / denom);
+            if (zero_grad) g[i] = 0.f;
+        }
+    }
+}
+
+__global__ __launch_bounds__(256) void adam_bf16_kernel(float* __restrict__ p, uint16_t* __restrict__ pb,
+                                                        float* __restrict__ g, float* __restrict__ m,
+                                                        float* __restrict__ v, int64_t n, float lr, float b1,
+                                                        float b2, float eps, float bc1, float bc2_sqrt,
+                                                        int zero_grad)
+{
+    int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float gi = g[i];
+    float mi = b1 * m[i] + (1.f - b1) * gi;
+    float vi = b2 * v[i] + (1.f - b2) * gi * gi;
+    float denom = sqrtf(vi) / bc2_sqrt + eps;
+    float pi = p[i] - (lr / bc1) * (mi / denom);
+    m[i] = mi;
+    v[i] = vi;
+    p[i] = pi;
+    pb[i] = f32_to_bf16_rne(pi);
+    if (zero_grad) g[i] = 0.f;
+}
+
+__global__ __launch_bounds__(256) void layout_kernel(const float* __restrict__ src, float* __restrict__ dst, int64_t N,
+                                                     int C, int G, int direction)
+{
+    int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    int64_t cells = (int64_t)G * G * G;
+    if (i >= N * C * cells) return;
+    // i indexes the channels-last destination / source: [n][cell][c]
+    int64_t c = i % C;
+    int64_t cell = (i / C) % cells;
+    int64_t n = i / (C * cells);
+    int64_t ncz = (n * C + c) * cells + cell;
+    if (direction == 0)
+        dst[i] = src[ncz];
+    else
+        dst[ncz] = src[i];
+}
+
+__global__ __launch_bounds__(256) void to_bf16_kernel(const float* __restrict__ src, uint16_t* __restrict__ dst,
+                                                      int64_t n)
+{
+    int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i < n) dst[i] = f32_to_bf16_rne(src[i]);
+}
+
+}  // namespace atdf
+
+using namespace atdf;
+
+static unsigned blocks_for(int64_t n, int threads = 256) { return (unsigned)atdf::div_up(n, threads); }
+
+extern "C" {
+
+atdf_status atdf_local_samples(const float* pmin, const float* pmax, const float* g, const int64_t* node, int64_t S,
+                               float* local, atdf_stream_t stream)
+{
+    ATDF_REQUIRE(S >= 0, "atdf_local_samples: negative size");
+    if (S == 0) return ATDF_OK;
+    ATDF_REQUIRE(pmin && pmax && g && node && local, "atdf_local_samples: null buffer");
+    hipLaunchKernelGGL(local_samples_kernel, dim3(blocks_for(S)), dim3(256), 0, to_hip(stream), pmin, pmax, g, node, S,
+                       local);
+    return check_launch("atdf_local_samples");
+}
+
+atdf_status atdf_grid_sample3d_fwd(const float* grid, int64_t N, int32_t C, int32_t Z, int32_t Y, int32_t X,
+                                   const int64_t* index, const float* uv, int64_t S, float* out, atdf_stream_t stream)
+{
+    ATDF_REQUIRE(N > 0 && C > 0 && Z > 0 && Y > 0 && X > 0 && S >= 0, "atdf_grid_sample3d_fwd: bad shape");
+    if (S == 0) return ATDF_OK;
+    ATDF_REQUIRE(grid && index && uv && out, "atdf_grid_sample3d_fwd: null buffer");
+    hipLaunchKernelGGL(grid_sample3d_fwd_kernel, dim3(blocks_for(S)), dim3(256), 0, to_hip(stream), grid, C, Z, Y, X,
+                       index, uv, S, out);
+    return check_launch("atdf_grid_sample3d_fwd");
+}
+
+atdf_status atdf_grid_sample3d_bwd(const float* grid, int64_t N, int32_t C, int32_t Z, int32_t Y, int32_t X,
+                                   const int64_t* index, const float* uv, int64_t S, const float* grad_out,
+                                   float* grad_grid, float* grad_uv, atdf_stream_t stream)
+{
+    ATDF_REQUIRE(N > 0 && C > 0 && Z > 0 && Y > 0 && X > 0 && S >= 0, "atdf_grid_sample3d_bwd: bad shape");
+    if (S == 0) return ATDF_OK;
+    ATDF_REQUIRE(grid && index && uv && grad_out && grad_grid, "atdf_grid_sample3d_bwd: null buffer");
+    hipLaunchKernelGGL(grid_sample3d_bwd_kernel, dim3(blocks_for(S)), dim3(256), 0, to_hip(stream), grid, C, Z, Y, X,
+                       index, uv, S, grad_out, grad_grid, grad_uv);
+    return check_launch("atdf_grid_sample3d_bwd");
+}
+
+atdf_status atdf_integrate_xray(const float* values, const float* weight, const int64_t* ray_index, int64_t S,
+                                int32_t C, int64_t R, float* out, atdf_stream_t stream)
+{
+    ATDF_REQUIRE(S >= 0 && C > 0 && R >= 0, "atdf_integrate_xray: bad shape");
+    if (S == 0) return ATDF_OK;
+    ATDF_REQUIRE(values && weight && ray_index && out, "atdf_integrate_xray: null buffer");
+    hipLaunchKernelGGL(integrate_kernel, dim3(blocks_for(S)), dim3(256), 0, to_hip(stream), values, weight, ray_index,
+                       S, C, R, out);
+    return check_launch("atdf_integrate_xray");
+}
+
+atdf_status atdf_accumulate_loss_per_node(const float* per_ray_loss, int32_t C, int64_t R, const int64_t* ray_index,
+                                          const float* weight, const int64_t* node_id, int64_t S, int64_t num_nodes,
+                                          double* loss_sum, double* weight_sum, atdf_stream_t stream)
+{
+    ATDF_REQUIRE(S >= 0 && C > 0 && num_nodes > 0, "atdf_accumulate_loss_per_node: bad shape");
+    if (S == 0) return ATDF_OK;
+    ATDF_REQUIRE(per_ray_loss && ray_index && weight && node_id && loss_sum && weight_sum,
+                 "atdf_accumulate_loss_per_node: null buffer");
+    hipLaunchKernelGGL(loss_per_node_kernel, dim3(blocks_for(S)), dim3(256), 0, to_hip(stream), per_ray_loss, C, R,
+                       ray_index, weight, node_id, S, loss_sum, weight_sum);
+    return check_launch("atdf_accumulate_loss_per_node");
+}
+
+static int mse_blocks(int64_t R)
+{
+    int64_t b = atdf::div_up(R > 0 ? R : 1, kRedThreads * 4);
+    return (int)(b < 1024 ? b : 1024);
+}
+
+size_t atdf_mse_workspace_bytes(int64_t num_rays) { return (size_t)mse_blocks(num_rays) * sizeof(double); }
+
+atdf_status atdf_mse_loss(const float* P, const float* T, int64_t R, float scale, float* dLdP, double* loss,
+                          void* workspace, size_t workspace_bytes, atdf_stream_t stream)
+{
+    ATDF_REQUIRE(R > 0 && P && T && loss && workspace, "atdf_mse_loss: bad arguments");
+    ATDF_REQUIRE(workspace_bytes >= atdf_mse_workspace_bytes(R), "atdf_mse_loss: workspace too small");
+    int nb = mse_blocks(R);
+    hipStream_t s = to_hip(stream);
+    double* partial = reinterpret_cast<double*>(workspace);
+    hipLaunchKernelGGL(mse_partial_kernel, dim3(nb), dim3(kRedThreads), 0, s, P, T, R, scale, dLdP, partial);
+    hipLaunchKernelGGL(mse_final_kernel, dim3(1), dim3(kRedThreads), 0, s, partial, nb, R, scale, loss);
+    return check_launch("atdf_mse_loss");
+}
+
+atdf_status atdf_adam_step(float* param, float* grad, float* exp_avg, float* exp_avg_sq, int64_t n, float lr,
+                           float beta1, float beta2, float eps, int64_t step, int32_t zero_grad, atdf_stream_t stream)
+{
+    ATDF_REQUIRE(n >= 0 && step >= 1, "atdf_adam_step: bad arguments");
+    if (n == 0) return ATDF_OK;
+    ATDF_REQUIRE(param && grad && exp_avg && exp_avg_sq, "atdf_adam_step: null buffer");
+    ATDF_REQUIRE(((uintptr_t)param | (uintptr_t)grad | (uintptr_t)exp_avg | (uintptr_t)exp_avg_sq) % 16 == 0,
+                 "atdf_adam_step: buffers must be 16-byte aligned");
+    float bc1 = 1.0f - (float)pow((double)beta1, (double)step);
+    float bc2 = 1.0f - (float)pow((double)beta2, (double)step);
+    hipLaunchKernelGGL(adam_kernel, dim3(blocks_for(atdf::div_up(n, 4))), dim3(256), 0, to_hip(stream), param, grad,
+                       exp_avg, exp_avg_sq, n, lr, beta1, beta2, eps, bc1, sqrtf(bc2), zero_grad);
+    return check_launch("atdf_adam_step");
+}
+
+atdf_status atdf_adam_step_bf16(float* master, uint16_t* param_bf16, float* grad, float* exp_avg, float* exp_avg_sq,
+                                int64_t n, float lr, float beta1, float beta2, float eps, int64_t step,
+                                int32_t zero_grad, atdf_stream_t stream)
+{
+    ATDF_REQUIRE(n >= 0 && step >= 1, "atdf_adam_step_bf16: bad arguments");
+    if (n == 0) return ATDF_OK;
+    ATDF_REQUIRE(master && param_bf16 && grad && exp_avg && exp_avg_sq, "atdf_adam_step_bf16: null buffer");
+    float bc1 = 1.0f - (float)pow((double)beta1, (double)step);
+    float bc2 = 1.0f - (float)pow((double)beta2, (double)step);
+    hipLaunchKernelGGL(adam_bf16_kernel, dim3(blocks_for(n)), dim3(256), 0, to_hip(stream), master, param_bf16, grad,
+                       exp_avg, exp_avg_sq, n, lr, beta1, beta2, eps, bc1, sqrtf(bc2), zero_grad);
+    return check_launch("atdf_adam_step_bf16");
+}
+
+atdf_status atdf_grid_layout(const float* src, float* dst, int64_t N, int32_t C, int32_t G, int32_t direction,
+                             atdf_stream_t stream)
+{
+    ATDF_REQUIRE(N >= 0 && C > 0 && G > 0 && (direction == 0 || direction == 1), "atdf_grid_layout: bad arguments");
+    int64_t total = N * C * (int64_t)G * G * G;
+    if (total == 0) return ATDF_OK;
+    ATDF_REQUIRE(src && dst && src != dst, "atdf_grid_layout: null or aliased buffers");
+    hipLaunchKernelGGL(layout_kernel, dim3(blocks_for(total)), dim3(256), 0, to_hip(stream), src, dst, N, C, G,
+                       direction);
+    return check_launch("atdf_grid_layout");
+}
+
+atdf_status atdf_f32_to_bf16(const float* src, uint16_t* dst, int64_t n, atdf_stream_t stream)
+{
+    ATDF_REQUIRE(n >= 0, "atdf_f32_to_bf16: bad size");
+    if (n == 0) return ATDF_OK;
+    ATDF_REQUIRE(src && dst, "atdf_f32_to_bf16: null buffer");
+    hipLaunchKernelGGL(to_bf16_kernel, dim3(blocks_for(n)), dim3(256), 0, to_hip(stream), src, dst, n);
+    return check_launch("atdf_f32_to_bf16");
+}
+
+}  // extern "C"

@@ -1,0 +1,643 @@
+// atdf_project.hip -- fused per-ray projection and its adjoint.
+//
+// Work unit: one wavefront owns `rays_per_wave` consecutive rays and walks
+// each ray's samples in tiles of 16.  Samples are recomputed from the ray's
+// node intervals (never stored): lane (g, j) = (lane / 16, lane % 16) handles
+// sample j of the tile and channels [CPL*g, CPL*g + CPL) of its gather, so
+// the four lanes of a column read one contiguous channels-last corner record.
+//
+// Decoder C -> 64 -> 64 -> 1 on v_mfma_f32_16x16x4f32 with activations kept
+// in the MFMA result layout: samples are the N (column) index, hidden units
+// the M (row) index.  The result layout of one layer is directly the B
+// operand of the next: lane (g, j) holds rows 4g + r of column j, and the
+// K-step kk of a 16-row block uses rows {4g + kk}, so no data moves between
+// layers.  Weights are the A operands, read from LDS.
+//
+// The adjoint recomputes the forward per tile and accumulates
+//   dW2 += dH2 . H1^T and dW1 += dH1 . X^T
+// on MFMA with the tile's samples as the K index (operands transposed
+// through a per-wave LDS buffer), dX = W1^T . dH1 on MFMA, and scatters
+// w_corner * dX into the channels-last grid gradient with float atomics.
+// Per-wave parameter-gradient partials go to a workspace and are summed in a
+// fixed order by a second kernel (deterministic decoder gradients).
+#include <hip/hip_runtime.h>
+
+#include "atdf_device.h"
+#include "atdf_internal.h"
+
+namespace atdf {
+
+typedef float v4f __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ v4f mfma4(float a, float b, v4f c)
+{
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+}
+
+// Compiler + wavefront ordering point for LDS data exchanged between lanes
+// of one wave (LDS executes a wave's operations in issue order).
+__device__ __forceinline__ void wave_lds_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+}
+
+constexpr int kH = 64;        // hidden width
+constexpr int kTile = 16;     // samples per tile
+constexpr int kWaves = 4;     // waves per block
+constexpr int kW2S = kH + 4;  // padded LDS row stride of W2
+constexpr int kTS = kTile + 4;  // padded row stride of the transpose buffers
+
+struct ProjArgs {
+    const PNode* nodes;
+    const float* O;
+    const float* D;
+    int64_t R;
+    int jitter_mode;
+    uint64_t seed;
+    const int64_t* iv_off;
+    const int32_t* iv_node;
+    const float* iv_tmin;
+    const float* iv_tmax;
+    const int32_t* iv_count;
+    const int32_t* iv_first;
+    const void* grid;
+    const int32_t* node_to_grid;
+    const float* mlp;
+    int G;
+    float beta;
+    int rpw;
+};
+
+template <int C>
+struct MlpSmem {
+    static constexpr int W1S = C + 4;
+    static constexpr int kW1 = 0;
+    static constexpr int kW2 = kW1 + kH * W1S;
+    static constexpr int kB1 = kW2 + kH * kW2S;
+    static constexpr int kB2 = kB1 + kH;
+    static constexpr int kW3 = kB2 + kH;
+    static constexpr int kB3 = kW3 + kH;
+    static constexpr int kTotal = kB3 + 4;
+};
+
+template <int C>
+__device__ __forceinline__ void load_mlp_to_lds(const float* __restrict__ mlp, float* s)
+{
+    using L = MlpSmem<C>;
+    const float* W1 = mlp;
+    const float* b1 = W1 + kH * C;
+    const float* W2 = b1 + kH;
+    const float* b2 = W2 + kH * kH;
+    const float* W3 = b2 + kH;
+    const float* b3 = W3 + kH;
+    for (int i = threadIdx.x; i < kH * C; i += blockDim.x) s[L::kW1 + (i / C) * L::W1S + (i % C)] = W1[i];
+    for (int i = threadIdx.x; i < kH * kH; i += blockDim.x) s[L::kW2 + (i / kH) * kW2S + (i % kH)] = W2[i];
+    for (int i = threadIdx.x; i < kH; i += blockDim.x) {
+        s[L::kB1 + i] = b1[i];
+        s[L::kB2 + i] = b2[i];
+        s[L::kW3 + i] = W3[i];
+    }
+    if (threadIdx.x == 0) s[L::kB3] = b3[0];
+}
+
+// Per-lane sample state of one tile.
+struct SampleCtx {
+    bool valid;
+    float w;       // integration weight
+    Tri tri;
+    int64_t gbase; // element offset of the node grid, -1 if none
+};
+
+// Recompute sample `s_local` of ray r (lane-private interval cursor `cur`).
+__device__ __forceinline__ void make_sample(const ProjArgs& a, int64_t r, const float* o, const float* d, int ns,
+                                            int s_local, int64_t h1, int64_t& cur, int C, SampleCtx& sc)
+{
+    sc.valid = s_local < ns;
+    sc.w = 0.f;
+    sc.gbase = -1;
+    if (!sc.valid) return;
+    while (cur + 1 < h1 && a.iv_first[cur] + a.iv_count[cur] <= s_local) ++cur;
+    const int node = a.iv_node[cur];
+    const float tmin = a.iv_tmin[cur];
+    const float tmax = a.iv_tmax[cur];
+    const int n = a.iv_count[cur];
+    const int jj = s_local - a.iv_first[cur];
+    const float step = (tmax - tmin) / (float)n;
+    float aj = 0.5f, ap = 0.5f, an = 0.5f;
+    if (a.jitter_mode == 2) {
+        aj = hash_jitter(a.seed, r, s_local);
+        if (jj > 0) ap = hash_jitter(a.seed, r, s_local - 1);
+        if (jj < n - 1) an = hash_jitter(a.seed, r, s_local + 1);
+    }
+    const float t = sample_t(tmin, step, jj, aj);
+    const float tp = (jj > 0) ? sample_t(tmin, step, jj - 1, ap) : 0.f;
+    const float tn = (jj < n - 1) ? sample_t(tmin, step, jj + 1, an) : 0.f;
+    sc.w = sample_weight(tmin, tmax, n, jj, tp, t, tn);
+    const PNode nd = a.nodes[node];
+    float loc[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        float pos = o[k] + d[k] * t;
+        float size = nd.mx[k] - nd.mn[k];
+        loc[k] = ((pos - nd.mn[k]) / size) * 2 - 1;
+    }
+    tri_setup(loc[0], loc[1], loc[2], a.G, a.G, a.G, sc.tri);
+    const int slot = a.node_to_grid[node];
+    if (slot >= 0) sc.gbase = (int64_t)slot * a.G * a.G * a.G * C;
+}
+
+__device__ __forceinline__ int64_t corner_cell(const Tri& t, int k, int G)
+{
+    int x = (k & 1) ? t.x1 : t.x0;
+    int y = (k & 2) ? t.y1 : t.y0;
+    int z = (k & 4) ? t.z1 : t.z0;
+    return ((int64_t)z * G + y) * G + x;
+}
+
+// Gather CPL channels starting at c0 (canonical corner order per channel).
+template <int C, bool kBF16>
+__device__ __forceinline__ void gather_channels(const ProjArgs& a, const SampleCtx& sc, int c0, float* x)
+{
+    constexpr int CPL = C / 4;
+#pragma unroll
+    for (int q = 0; q < CPL; ++q) x[q] = 0.f;
+    if (!sc.valid || sc.gbase < 0) return;
+    float v[8][CPL];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        int64_t off = sc.gbase + corner_cell(sc.tri, k, a.G) * C + c0;
+        if constexpr (!kBF16) {
+            const float* g = reinterpret_cast<const float*>(a.grid) + off;
+            if constexpr (CPL == 4) {
+                float4 f = *reinterpret_cast<const float4*>(g);
+                v[k][0] = f.x;
+                v[k][1] = f.y;
+                v[k][2] = f.z;
+                v[k][3] = f.w;
+            } else {
+                float2 f = *reinterpret_cast<const float2*>(g);
+                v[k][0] = f.x;
+                v[k][1] = f.y;
+            }
+        } else {
+            const uint16_t* g = reinterpret_cast<const uint16_t*>(a.grid) + off;
+            if constexpr (CPL == 4) {
+                uint2 u = *reinterpret_cast<const uint2*>(g);
+                v[k][0] = __uint_as_float(u.x << 16);
+                v[k][1] = __uint_as_float(u.x & 0xffff0000u);
+                v[k][2] = __uint_as_float(u.y << 16);
+                v[k][3] = __uint_as_float(u.y & 0xffff0000u);
+            } else {
+                uint32_t u = *reinterpret_cast<const uint32_t*>(g);
+                v[k][0] = __uint_as_float(u << 16);
+                v[k][1] = __uint_as_float(u & 0xffff0000u);
+            }
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < CPL; ++q) {
+        float acc = 0.f;
+#pragma unroll
+        for (int k = 0; k < 8; ++k) acc += v[k][q] * sc.tri.w[k];
+        x[q] = acc;
+    }
+}
+
+// Forward decoder for one tile.  x: this lane's CPL features of sample j.
+// Returns y (pre-softplus) of sample j on every lane of column j; h1/h2 are
+// the post-ReLU activations in result layout.
+template <int C>
+__device__ __forceinline__ float decoder_fwd(const float* s, const float* x, int g, int j, v4f* h1, v4f* h2)
+{
+    using L = MlpSmem<C>;
+    constexpr int CPL = C / 4;
+#pragma unroll
+    for (int rb = 0; rb < 4; ++rb) {
+        v4f acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int kk = 0; kk < CPL; ++kk) acc = mfma4(s[L::kW1 + (16 * rb + j) * L::W1S + CPL * g + kk], x[kk], acc);
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            float v = acc[r] + s[L::kB1 + 16 * rb + 4 * g + r];
+            h1[rb][r] = v > 0.f ? v : 0.f;
+        }
+    }
+#pragma unroll
+    for (int rb2 = 0; rb2 < 4; ++rb2) {
+        v4f acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int rb = 0; rb < 4; ++rb) {
+            const float* wrow = s + L::kW2 + (16 * rb2 + j) * kW2S + 16 * rb + 4 * g;
+            float4 wv = *reinterpret_cast<const float4*>(wrow);
+            acc = mfma4(wv.x, h1[rb][0], acc);
+            acc = mfma4(wv.y, h1[rb][1], acc);
+            acc = mfma4(wv.z, h1[rb][2], acc);
+            acc = mfma4(wv.w, h1[rb][3], acc);
+        }
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            float v = acc[r] + s[L::kB2 + 16 * rb2 + 4 * g + r];
+            h2[rb2][r] = v > 0.f ? v : 0.f;
+        }
+    }
+    float part = 0.f;
+#pragma unroll
+    for (int rb2 = 0; rb2 < 4; ++rb2)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) part += s[L::kW3 + 16 * rb2 + 4 * g + r] * h2[rb2][r];
+    part += __shfl_xor(part, 16, 64);
+    part += __shfl_xor(part, 32, 64);
+    return part + s[L::kB3];
+}
+
+template <int C, bool kBF16>
+__global__ __launch_bounds__(64 * kWaves) void project_fwd_kernel(ProjArgs a, float* __restrict__ P)
+{
+    using L = MlpSmem<C>;
+    constexpr int CPL = C / 4;
+    __shared__ __attribute__((aligned(16))) float s[L::kTotal];
+    load_mlp_to_lds<C>(a.mlp, s);
+    __syncthreads();
+
+    const int lane = threadIdx.x & 63;
+    const int g = lane >> 4, j = lane & 15;
+    const int64_t wave = (int64_t)blockIdx.x * kWaves + (threadIdx.x >> 6);
+    const int64_t r0 = wave * a.rpw;
+    const int64_t r1 = (r0 + a.rpw < a.R) ? r0 + a.rpw : a.R;
+    for (int64_t r = r0; r < r1; ++r) {
+        const int64_t h0 = a.iv_off[r], h1e = a.iv_off[r + 1];
+        if (h0 == h1e) {
+            if (lane == 0) P[r] = 0.f;
+            continue;
+        }
+        const int ns = a.iv_first[h1e - 1] + a.iv_count[h1e - 1];
+        float o[3], d[3];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            o[k] = a.O[r * 3 + k];
+            d[k] = a.D[r * 3 + k];
+        }
+        int64_t cur = h0;
+        float acc = 0.f;
+        for (int base = 0; base < ns; base += kTile) {
+            SampleCtx sc;
+            make_sample(a, r, o, d, ns, base + j, h1e, cur, C, sc);
+            float x[CPL];
+            gather_channels<C, kBF16>(a, sc, CPL * g, x);
+            v4f h1[4], h2[4];
+            float y = decoder_fwd<C>(s, x, g, j, h1, h2);
+            if (sc.valid) acc += softplus_f(y, a.beta) * sc.w;
+        }
+        acc += __shfl_xor(acc, 1, 64);
+        acc += __shfl_xor(acc, 2, 64);
+        acc += __shfl_xor(acc, 4, 64);
+        acc += __shfl_xor(acc, 8, 64);
+        if (lane == 0) P[r] = acc;
+    }
+}
+
+template <int C>
+constexpr int mlp_numel()
+{
+    return kH * C + kH + kH * kH + kH + kH + 1;
+}
+
+template <int C, bool kBF16>
+__global__ __launch_bounds__(64 * kWaves) void project_bwd_kernel(ProjArgs a, const float* __restrict__ dLdP,
+                                                                  float* __restrict__ grad_grid,
+                                                                  float* __restrict__ ws)
+{
+    using L = MlpSmem<C>;
+    constexpr int CPL = C / 4;
+    constexpr int NP = mlp_numel<C>();
+    __shared__ __attribute__((aligned(16))) float s[L::kTotal];
+    __shared__ __attribute__((aligned(16))) float tbuf[kWaves][2][kH * kTS];
+    load_mlp_to_lds<C>(a.mlp, s);
+    __syncthreads();
+
+    const int lane = threadIdx.x & 63;
+    const int g = lane >> 4, j = lane & 15;
+    const int wl = threadIdx.x >> 6;
+    float* bufA = tbuf[wl][0];
+    float* bufB = tbuf[wl][1];
+    const int64_t wave = (int64_t)blockIdx.x * kWaves + wl;
+    const int64_t r0 = wave * a.rpw;
+    const int64_t r1 = (r0 + a.rpw < a.R) ? r0 + a.rpw : a.R;
+
+    v4f accW2[4][4];
+    v4f accW1[4];
+#pragma unroll
+    for (int p = 0; p < 4; ++p) {
+        accW1[p] = (v4f){0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int q = 0; q < 4; ++q) accW2[p][q] = (v4f){0.f, 0.f, 0.f, 0.f};
+    }
+    float gW3[4][4], gB2[4][4], gB1[4][4];
+#pragma unroll
+    for (int p = 0; p < 4; ++p)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) gW3[p][q] = gB2[p][q] = gB1[p][q] = 0.f;
+    float gB3 = 0.f;
+
+    for (int64_t r = r0; r < r1; ++r) {
+        const int64_t h0 = a.iv_off[r], h1e = a.iv_off[r + 1];
+        const float dp = dLdP[r];
+        if (h0 == h1e || dp == 0.f) continue;
+        const int ns = a.iv_first[h1e - 1] + a.iv_count[h1e - 1];
+        float o[3], d[3];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            o[k] = a.O[r * 3 + k];
+            d[k] = a.D[r * 3 + k];
+        }
+        int64_t cur = h0;
+        for (int base = 0; base < ns; base += kTile) {
+            SampleCtx sc;
+            make_sample(a, r, o, d, ns, base + j, h1e, cur, C, sc);
+            float x[CPL];
+            gather_channels<C, kBF16>(a, sc, CPL * g, x);
+            v4f h1[4], h2[4];
+            float y = decoder_fwd<C>(s, x, g, j, h1, h2);
+            const float dy = sc.valid ? dp * sc.w * softplus_grad_f(y, a.beta) : 0.f;
+            if (g == 0) gB3 += dy;
+
+            // dH2 = dy * W3 masked by ReLU; dW3, db2 partials
+            v4f dh2[4];
+#pragma unroll
+            for (int rb2 = 0; rb2 < 4; ++rb2)
+#pragma unroll
+                for (int r4 = 0; r4 < 4; ++r4) {
+                    const int row = 16 * rb2 + 4 * g + r4;
+                    gW3[rb2][r4] += dy * h2[rb2][r4];
+                    float v = (h2[rb2][r4] > 0.f) ? dy * s[L::kW3 + row] : 0.f;
+                    dh2[rb2][r4] = v;
+                    gB2[rb2][r4] += v;
+                }
+
+            // dW2 += dH2 . H1^T  (K = the tile's 16 samples)
+#pragma unroll
+            for (int rb = 0; rb < 4; ++rb)
+#pragma unroll
+                for (int r4 = 0; r4 < 4; ++r4) {
+                    bufA[(16 * rb + 4 * g + r4) * kTS + j] = h1[rb][r4];
+                    bufB[(16 * rb + 4 * g + r4) * kTS + j] = dh2[rb][r4];
+                }
+            wave_lds_sync();
+#pragma unroll
+            for (int rb2 = 0; rb2 < 4; ++rb2) {
+                float4 av = *reinterpret_cast<const float4*>(bufB + (16 * rb2 + j) * kTS + 4 * g);
+#pragma unroll
+                for (int rb = 0; rb < 4; ++rb) {
+                    float4 bv = *reinterpret_cast<const float4*>(bufA + (16 * rb + j) * kTS + 4 * g);
+                    v4f acc = accW2[rb2][rb];
+                    acc = mfma4(av.x, bv.x, acc);
+                    acc = mfma4(av.y, bv.y, acc);
+                    acc = mfma4(av.z, bv.z, acc);
+                    acc = mfma4(av.w, bv.w, acc);
+                    accW2[rb2][rb] = acc;
+                }
+            }
+            wave_lds_sync();
+
+            // dH1 = W2^T . dH2, masked by ReLU
+            v4f dh1[4];
+#pragma unroll
+            for (int rb = 0; rb < 4; ++rb) {
+                v4f acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+                for (int rb2 = 0; rb2 < 4; ++rb2)
+#pragma unroll
+                    for (int kk = 0; kk < 4; ++kk)
+                        acc = mfma4(s[L::kW2 + (16 * rb2 + 4 * g + kk) * kW2S + 16 * rb + j], dh2[rb2][kk], acc);
+#pragma unroll
+                for (int r4 = 0; r4 < 4; ++r4) {
+                    float v = (h1[rb][r4] > 0.f) ? acc[r4] : 0.f;
+                    dh1[rb][r4] = v;
+                    gB1[rb][r4] += v;
+                }
+            }
+
+            // dX = W1^T . dH1 (rows = channels, padded to 16)
+            v4f dx = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+            for (int rb = 0; rb < 4; ++rb)
+#pragma unroll
+                for (int kk = 0; kk < 4; ++kk) {
+                    float wv = (j < C) ? s[L::kW1 + (16 * rb + 4 * g + kk) * L::W1S + j] : 0.f;
+                    dx = mfma4(wv, dh1[rb][kk], dx);
+                }
+
+            // scatter w_corner * dX into the channels-last grid gradient
+            if (sc.valid && sc.gbase >= 0 && 4 * g < C) {
+#pragma unroll
+                for (int k = 0; k < 8; ++k) {
+                    float* gp = grad_grid + sc.gbase + corner_cell(sc.tri, k, a.G) * C + 4 * g;
+                    const float wk = sc.tri.w[k];
+#pragma unroll
+                    for (int r4 = 0; r4 < 4; ++r4) unsafeAtomicAdd(gp + r4, wk * dx[r4]);
+                }
+            }
+
+            // dW1 += dH1 . X^T
+#pragma unroll
+            for (int rb = 0; rb < 4; ++rb)
+#pragma unroll
+                for (int r4 = 0; r4 < 4; ++r4) bufA[(16 * rb + 4 * g + r4) * kTS + j] = dh1[rb][r4];
+#pragma unroll
+            for (int q = 0; q < CPL; ++q) bufB[(CPL * g + q) * kTS + j] = x[q];
+            wave_lds_sync();
+            {
+                float4 bv = make_float4(0.f, 0.f, 0.f, 0.f);
+                if (j < C) bv = *reinterpret_cast<const float4*>(bufB + j * kTS + 4 * g);
+#pragma unroll
+                for (int rb = 0; rb < 4; ++rb) {
+                    float4 av = *reinterpret_cast<const float4*>(bufA + (16 * rb + j) * kTS + 4 * g);
+                    v4f acc = accW1[rb];
+                    acc = mfma4(av.x, bv.x, acc);
+                    acc = mfma4(av.y, bv.y, acc);
+                    acc = mfma4(av.z, bv.z, acc);
+                    acc = mfma4(av.w, bv.w, acc);
+                    accW1[rb] = acc;
+                }
+            }
+            wave_lds_sync();
+        }
+    }
+
+    // per-wave partials -> workspace row `wave` (every entry written)
+    float* out = ws + wave * (int64_t)NP;
+    constexpr int oW1 = 0, oB1 = kH * C, oW2 = oB1 + kH, oB2 = oW2 + kH * kH, oW3 = oB2 + kH, oB3 = oW3 + kH;
+#pragma unroll
+    for (int rb = 0; rb < 4; ++rb)
+#pragma unroll
+        for (int r4 = 0; r4 < 4; ++r4) {
+            if (j < C) out[oW1 + (16 * rb + 4 * g + r4) * C + j] = accW1[rb][r4];
+        }
+#pragma unroll
+    for (int rb2 = 0; rb2 < 4; ++rb2)
+#pragma unroll
+        for (int rb = 0; rb < 4; ++rb)
+#pragma unroll
+            for (int r4 = 0; r4 < 4; ++r4) out[oW2 + (16 * rb2 + 4 * g + r4) * kH + 16 * rb + j] = accW2[rb2][rb][r4];
+#pragma unroll
+    for (int p = 0; p < 4; ++p)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            float v1 = gB1[p][q], v2 = gB2[p][q], v3 = gW3[p][q];
+#pragma unroll
+            for (int off = 1; off < 16; off <<= 1) {
+                v1 += __shfl_xor(v1, off, 64);
+                v2 += __shfl_xor(v2, off, 64);
+                v3 += __shfl_xor(v3, off, 64);
+            }
+            if (j == 0) {
+                const int row = 16 * p + 4 * g + q;
+                out[oB1 + row] = v1;
+                out[oB2 + row] = v2;
+                out[oW3 + row] = v3;
+            }
+        }
+#pragma unroll
+    for (int off = 1; off < 16; off <<= 1) gB3 += __shfl_xor(gB3, off, 64);
+    if (lane == 0) out[oB3] = gB3;
+}
+
+// grad_mlp[i] += sum_w ws[w][i], summed in wave order.
+__global__ __launch_bounds__(256) void reduce_partials_kernel(const float* __restrict__ ws, int64_t nwaves, int np,
+                                                              float* __restrict__ grad)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= np) return;
+    float acc = 0.f;
+    for (int64_t w = 0; w < nwaves; ++w) acc += ws[w * np + i];
+    grad[i] += acc;
+}
+
+}  // namespace atdf
+
+using namespace atdf;
+
+static int64_t rays_per_wave(const atdf_field* f) { return f->rays_per_wave > 0 ? f->rays_per_wave : 64; }
+
+static atdf_status check_project_args(const void* packed_tree, int64_t num_nodes, const float* O, const float* D,
+                                      int64_t R, const atdf_sampling* sp, const int64_t* iv_off,
+                                      const atdf_intervals* iv, const atdf_field* f)
+{
+    ATDF_REQUIRE(packed_tree && num_nodes > 0, "project: empty tree");
+    ATDF_REQUIRE(R >= 0, "project: negative ray count");
+    ATDF_REQUIRE(sp && (sp->jitter_mode == 0 || sp->jitter_mode == 2),
+                 "project: jitter_mode must be 0 (midpoint) or 2 (hash)");
+    ATDF_REQUIRE(f, "project: null field");
+    ATDF_REQUIRE(f->H == 64, "project: hidden width %d not built (64 is)", f->H);
+    ATDF_REQUIRE(f->C == 16 || f->C == 8, "project: %d channels not built (16 and 8 are)", f->C);
+    ATDF_REQUIRE(f->G >= 2, "project: grid edge must be >= 2");
+    ATDF_REQUIRE(f->grid_dtype == ATDF_GRID_F32 || f->grid_dtype == ATDF_GRID_BF16, "project: unknown grid dtype");
+    ATDF_REQUIRE(f->grid && f->node_to_grid && f->mlp && f->num_grids > 0, "project: incomplete field");
+    if (R > 0) {
+        ATDF_REQUIRE(O && D && iv_off && iv && iv->node && iv->tmin && iv->tmax && iv->count && iv->first,
+                     "project: null ray or interval buffer");
+    }
+    return ATDF_OK;
+}
+
+static ProjArgs make_args(const void* packed_tree, const float* O, const float* D, int64_t R, const atdf_sampling* sp,
+                          const int64_t* iv_off, const atdf_intervals* iv, const atdf_field* f)
+{
+    ProjArgs a{};
+    a.nodes = reinterpret_cast<const PNode*>(packed_tree);
+    a.O = O;
+    a.D = D;
+    a.R = R;
+    a.jitter_mode = sp->jitter_mode;
+    a.seed = sp->jitter_seed;
+    a.iv_off = iv_off;
+    a.iv_node = iv->node;
+    a.iv_tmin = iv->tmin;
+    a.iv_tmax = iv->tmax;
+    a.iv_count = iv->count;
+    a.iv_first = iv->first;
+    a.grid = f->grid;
+    a.node_to_grid = f->node_to_grid;
+    a.mlp = f->mlp;
+    a.G = f->G;
+    a.beta = f->softplus_beta;
+    a.rpw = (int)rays_per_wave(f);
+    return a;
+}
+
+extern "C" {
+
+atdf_status atdf_project_fwd(const void* packed_tree, int64_t num_nodes, const float* O, const float* D, int64_t R,
+                             const atdf_sampling* sp, const int64_t* iv_off, const atdf_intervals* iv,
+                             const atdf_field* f, float* P, atdf_stream_t stream)
+{
+    atdf_status st = check_project_args(packed_tree, num_nodes, O, D, R, sp, iv_off, iv, f);
+    if (st) return st;
+    if (R == 0) return ATDF_OK;
+    ATDF_REQUIRE(P, "atdf_project_fwd: null projection");
+    ProjArgs a = make_args(packed_tree, O, D, R, sp, iv_off, iv, f);
+    int64_t nwaves = div_up(R, a.rpw);
+    dim3 grid((unsigned)div_up(nwaves, kWaves)), block(64 * kWaves);
+    hipStream_t s = to_hip(stream);
+    const bool bf = f->grid_dtype == ATDF_GRID_BF16;
+    if (f->C == 16) {
+        if (bf)
+            hipLaunchKernelGGL((project_fwd_kernel<16, true>), grid, block, 0, s, a, P);
+        else
+            hipLaunchKernelGGL((project_fwd_kernel<16, false>), grid, block, 0, s, a, P);
+    } else {
+        if (bf)
+            hipLaunchKernelGGL((project_fwd_kernel<8, true>), grid, block, 0, s, a, P);
+        else
+            hipLaunchKernelGGL((project_fwd_kernel<8, false>), grid, block, 0, s, a, P);
+    }
+    return check_launch("atdf_project_fwd");
+}
+
+size_t atdf_project_bwd_workspace_bytes(int64_t R, const atdf_field* f)
+{
+    if (!f || R <= 0) return 0;
+    int64_t np = (int64_t)f->H * f->C + f->H + (int64_t)f->H * f->H + f->H + f->H + 1;
+    int64_t nwaves = div_up(R, rays_per_wave(f));
+    int64_t nw_pad = div_up(nwaves, kWaves) * kWaves;  // every launched wave writes a row
+    return (size_t)(nw_pad * np) * sizeof(float);
+}
+
+atdf_status atdf_project_bwd(const void* packed_tree, int64_t num_nodes, const float* O, const float* D, int64_t R,
+                             const atdf_sampling* sp, const int64_t* iv_off, const atdf_intervals* iv,
+                             const atdf_field* f, const float* dLdP, float* grad_grid, float* grad_mlp,
+                             void* workspace, size_t workspace_bytes, atdf_stream_t stream)
+{
+    atdf_status st = check_project_args(packed_tree, num_nodes, O, D, R, sp, iv_off, iv, f);
+    if (st) return st;
+    if (R == 0) return ATDF_OK;
+    ATDF_REQUIRE(dLdP && grad_grid && grad_mlp && workspace, "atdf_project_bwd: null buffer");
+    ATDF_REQUIRE(workspace_bytes >= atdf_project_bwd_workspace_bytes(R, f), "atdf_project_bwd: workspace too small");
+    ProjArgs a = make_args(packed_tree, O, D, R, sp, iv_off, iv, f);
+    int64_t nwaves = div_up(R, a.rpw);
+    int64_t nblocks = div_up(nwaves, kWaves);
+    dim3 grid((unsigned)nblocks), block(64 * kWaves);
+    hipStream_t s = to_hip(stream);
+    float* ws = reinterpret_cast<float*>(workspace);
+    const bool bf = f->grid_dtype == ATDF_GRID_BF16;
+    int np;
+    if (f->C == 16) {
+        np = mlp_numel<16>();
+        if (bf)
+            hipLaunchKernelGGL((project_bwd_kernel<16, true>), grid, block, 0, s, a, dLdP, grad_grid, ws);
+        else
+            hipLaunchKernelGGL((project_bwd_kernel<16, false>), grid, block, 0, s, a, dLdP, grad_grid, ws);
+    } else {
+        np = mlp_numel<8>();
+        if (bf)
+            hipLaunchKernelGGL((project_bwd_kernel<8, true>), grid, block, 0, s, a, dLdP, grad_grid, ws);
+        else
+            hipLaunchKernelGGL((project_bwd_kernel<8, false>), grid, block, 0, s, a, dLdP, grad_grid, ws);
+    }
+    hipLaunchKernelGGL(reduce_partials_kernel, dim3((unsigned)div_up(np, 256)), dim3(256), 0, s, ws,
+                       nblocks * kWaves, np, grad_mlp);
+    return check_launch("atdf_project_bwd");
+}
+
+}  // extern "C"

@@ -1,0 +1,415 @@
+// atdf_tree.hip -- octree packing, ray traversal with per-node sample
+// placement, and the exclusive scan that turns per-ray counts into offsets.
+//
+// Traversal is one lane per ray walking the octree depth-first in
+// front-to-back child order (child c is visited at position c ^ mask, mask
+// = sign bits of the direction), stackless: the return path uses the
+// parent links and the contiguous child numbering.  A child box can only
+// be hit when its parent box is hit (the rounded slab values of a child
+// are nested inside the parent's), so pruning on the parent test visits
+// exactly the active nodes the reference's brute-force loop
+// (structure/HyperTree.cu:320-359) would hit.  Front-to-back order emits
+// hits in increasing tmin; the kernel checks that order for every ray and
+// reports a violation instead of silently emitting a different order.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+
+#include "atdf_device.h"
+#include "atdf_internal.h"
+
+namespace atdf {
+
+static thread_local char g_last_error[512] = "";
+
+void set_error(const char* fmt, ...)
+{
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_last_error, sizeof(g_last_error), fmt, ap);
+    va_end(ap);
+}
+
+// ---------------------------------------------------------------------------
+// tree packing
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void pack_nodes_kernel(const float* __restrict__ pmin, const float* __restrict__ pmax,
+                                                         const int32_t* __restrict__ children,
+                                                         const int32_t* __restrict__ parent,
+                                                         const int32_t* __restrict__ active, int64_t n, PNode* out)
+{
+    int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    PNode p;
+    for (int k = 0; k < 3; ++k) {
+        p.mn[k] = pmin[i * 3 + k];
+        p.mx[k] = pmax[i * 3 + k];
+    }
+    p.first_child = children[i * 8];
+    p.parent = parent[i];
+    p.flags = active[i] ? kFlagActive : 0;
+    p.pad[0] = p.pad[1] = p.pad[2] = 0;
+    out[i] = p;
+}
+
+__global__ __launch_bounds__(256) void mark_active_below_kernel(const int32_t* __restrict__ parent,
+                                                                const int32_t* __restrict__ active, int64_t n,
+                                                                PNode* out)
+{
+    int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= n || !active[i]) return;
+    int32_t p = parent[i];
+    while (p >= 0) {
+        int old = atomicOr(&out[p].flags, kFlagActiveBelow);
+        if (old & kFlagActiveBelow) break;  // whoever set it walks further up
+        p = parent[p];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// traversal
+// ---------------------------------------------------------------------------
+struct TraverseArgs {
+    const PNode* nodes;
+    const float* O;
+    const float* D;
+    int64_t R;
+    int K;
+    int max_inter;
+    int jitter_mode;
+    uint64_t seed;
+    const float* jitter;
+    int32_t* ray_num_intervals;
+    int32_t* ray_num_samples;
+    int32_t* status;
+    const int64_t* iv_off;
+    const int64_t* smp_off;
+    atdf_intervals iv;
+    atdf_samples smp;
+};
+
+__device__ __forceinline__ float jitter_of(const TraverseArgs& a, int64_t slot, int64_t ray, int64_t loc)
+{
+    if (a.jitter_mode == 1) return a.jitter[slot];
+    if (a.jitter_mode == 2) return hash_jitter(a.seed, ray, loc);
+    return 0.5f;
+}
+
+template <bool kWrite>
+__global__ __launch_bounds__(256) void traverse_kernel(TraverseArgs a)
+{
+    int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (r >= a.R) return;
+    float o[3], d[3];
+    for (int k = 0; k < 3; ++k) {
+        o[k] = a.O[r * 3 + k];
+        d[k] = a.D[r * 3 + k];
+    }
+    const int mask = (d[0] < 0 ? 1 : 0) | (d[1] < 0 ? 2 : 0) | (d[2] < 0 ? 4 : 0);
+
+    int nh = 0, ns = 0;
+    float last_t = -INFINITY;
+    int last_node = -1;
+    bool order_bad = false;
+    const int64_t ivb = kWrite ? a.iv_off[r] : 0;
+    const int64_t sb = kWrite ? a.smp_off[r] : 0;
+
+    int node = 0;
+    while (true) {
+        const PNode nd = a.nodes[node];
+        float tmin, tmax;
+        bool hit = intersect_box_ray(nd.mn, nd.mx, o, d, tmin, tmax);
+        if (hit && (nd.flags & kFlagActive) && !(tmax - tmin < 1e-5)) {
+            int n = node_sample_count(nd.mn, nd.mx, tmin, tmax, a.K);
+            if (tmin < last_t || (tmin == last_t && node < last_node)) order_bad = true;
+            last_t = tmin;
+            last_node = node;
+            if (kWrite) {
+                if (a.iv.node) {
+                    int64_t q = ivb + nh;
+                    a.iv.node[q] = node;
+                    a.iv.tmin[q] = tmin;
+                    a.iv.tmax[q] = tmax;
+                    a.iv.count[q] = n;
+                    a.iv.first[q] = ns;
+                    a.iv.ray[q] = (int32_t)r;
+                }
+                if (a.smp.weight && n > 0) {
+                    float step = (tmax - tmin) / (float)n;
+                    int64_t slot0 = sb + ns;
+                    float t_prev = 0.f;
+                    float t_cur = sample_t(tmin, step, 0, jitter_of(a, slot0, r, ns));
+                    for (int j = 0; j < n; ++j) {
+                        float t_next = 0.f;
+                        if (j < n - 1) t_next = sample_t(tmin, step, j + 1, jitter_of(a, slot0 + j + 1, r, ns + j + 1));
+                        int64_t slot = slot0 + j;
+                        a.smp.ray_t[slot] = t_cur;
+                        a.smp.weight[slot] = sample_weight(tmin, tmax, n, j, t_prev, t_cur, t_next);
+                        for (int k = 0; k < 3; ++k) a.smp.global_coordinate[slot * 3 + k] = o[k] + d[k] * t_cur;
+                        a.smp.ray_index[slot] = r;
+                        a.smp.local_index_in_ray[slot] = ns + j;
+                        a.smp.node_id[slot] = node;
+                        t_prev = t_cur;
+                        t_cur = t_next;
+                    }
+                }
+            }
+            nh++;
+            ns += n;
+        }
+        if (hit && (nd.flags & kFlagActiveBelow) && nd.first_child >= 0) {
+            node = nd.first_child + mask;  // first child in front-to-back order
+            continue;
+        }
+        // next sibling in front-to-back order, ascending while exhausted
+        int cur = node;
+        int parent = nd.parent;
+        while (true) {
+            if (parent < 0) goto done;
+            int first = a.nodes[parent].first_child;
+            int k = (cur - first) ^ mask;
+            if (k < 7) {
+                node = first + ((k + 1) ^ mask);
+                break;
+            }
+            cur = parent;
+            parent = a.nodes[parent].parent;
+        }
+    }
+done:
+    if (!kWrite) {
+        a.ray_num_intervals[r] = nh;
+        a.ray_num_samples[r] = ns;
+        if (nh > a.max_inter) atomicAdd(&a.status[0], 1);
+        if (order_bad) atomicAdd(&a.status[1], 1);
+        atomicMax(&a.status[2], ns);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// exclusive scan, int32 -> int64 (n + 1 outputs)
+// ---------------------------------------------------------------------------
+constexpr int kScanThreads = 256;
+constexpr int kScanPerThread = 4;
+constexpr int kScanTile = kScanThreads * kScanPerThread;
+
+__device__ __forceinline__ int64_t wave_inclusive_scan(int64_t v)
+{
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        int64_t u = __shfl_up(v, off, 64);
+        if (lane >= off) v += u;
+    }
+    return v;
+}
+
+// Inclusive scan of one value per thread across the block; returns the
+// thread's inclusive prefix and writes the block total to *total.
+__device__ __forceinline__ int64_t block_inclusive_scan(int64_t v, int64_t* smem, int64_t* total)
+{
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    int64_t inc = wave_inclusive_scan(v);
+    if (lane == 63) smem[wave] = inc;
+    __syncthreads();
+    int64_t off = 0;
+    int64_t tot = 0;
+    for (int w = 0; w < kScanThreads / 64; ++w) {
+        if (w < wave) off += smem[w];
+        tot += smem[w];
+    }
+    __syncthreads();
+    *total = tot;
+    return inc + off;
+}
+
+__global__ __launch_bounds__(kScanThreads) void scan_tile_sums_kernel(const int32_t* __restrict__ in, int64_t n,
+                                                                      int64_t* __restrict__ tile_sums)
+{
+    __shared__ int64_t smem[kScanThreads / 64];
+    int64_t base = (int64_t)blockIdx.x * kScanTile + (int64_t)threadIdx.x * kScanPerThread;
+    int64_t s = 0;
+    for (int k = 0; k < kScanPerThread; ++k)
+        if (base + k < n) s += in[base + k];
+    int64_t tot;
+    block_inclusive_scan(s, smem, &tot);
+    if (threadIdx.x == 0) tile_sums[blockIdx.x] = tot;
+}
+
+// Single block: in-place exclusive scan of the tile sums.
+__global__ __launch_bounds__(kScanThreads) void scan_tile_offsets_kernel(int64_t* __restrict__ tile_sums, int64_t nt)
+{
+    __shared__ int64_t smem[kScanThreads / 64];
+    int64_t carry = 0;
+    for (int64_t base = 0; base < nt; base += kScanThreads) {
+        int64_t i = base + threadIdx.x;
+        int64_t v = (i < nt) ? tile_sums[i] : 0;
+        int64_t tot;
+        int64_t inc = block_inclusive_scan(v, smem, &tot);
+        if (i < nt) tile_sums[i] = carry + inc - v;
+        carry += tot;
+        __syncthreads();
+    }
+}
+
+__global__ __launch_bounds__(kScanThreads) void scan_apply_kernel(const int32_t* __restrict__ in, int64_t n,
+                                                                  const int64_t* __restrict__ tile_offsets,
+                                                                  int64_t* __restrict__ out)
+{
+    __shared__ int64_t smem[kScanThreads / 64];
+    int64_t base = (int64_t)blockIdx.x * kScanTile + (int64_t)threadIdx.x * kScanPerThread;
+    int64_t v[kScanPerThread];
+    int64_t s = 0;
+    for (int k = 0; k < kScanPerThread; ++k) {
+        v[k] = (base + k < n) ? in[base + k] : 0;
+        s += v[k];
+    }
+    int64_t tot;
+    int64_t inc = block_inclusive_scan(s, smem, &tot);
+    int64_t run = tile_offsets[blockIdx.x] + inc - s;
+    for (int k = 0; k < kScanPerThread; ++k) {
+        run += v[k];
+        if (base + k < n) out[base + k + 1] = run;
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) out[0] = 0;
+}
+
+}  // namespace atdf
+
+using namespace atdf;
+
+extern "C" {
+
+const char* atdf_last_error(void) { return g_last_error; }
+const char* atdf_version(void) { return "atdf 0.1 gfx950"; }
+
+size_t atdf_tree_packed_bytes(int64_t num_nodes) { return (size_t)(num_nodes > 0 ? num_nodes : 0) * sizeof(PNode); }
+
+atdf_status atdf_tree_pack(const atdf_tree* t, void* packed, atdf_stream_t stream)
+{
+    ATDF_REQUIRE(t && packed, "atdf_tree_pack: null argument");
+    ATDF_REQUIRE(t->num_nodes > 0 && t->num_nodes < (int64_t)INT32_MAX, "atdf_tree_pack: num_nodes %lld out of range",
+                 (long long)t->num_nodes);
+    ATDF_REQUIRE(t->node_position_min && t->node_position_max && t->node_children && t->node_parent &&
+                     t->node_active,
+                 "atdf_tree_pack: null tree tensor");
+    hipStream_t s = to_hip(stream);
+    int64_t n = t->num_nodes;
+    dim3 grid((unsigned)div_up(n, 256));
+    PNode* out = reinterpret_cast<PNode*>(packed);
+    hipLaunchKernelGGL(pack_nodes_kernel, grid, dim3(256), 0, s, t->node_position_min, t->node_position_max,
+                       t->node_children, t->node_parent, t->node_active, n, out);
+    hipLaunchKernelGGL(mark_active_below_kernel, grid, dim3(256), 0, s, t->node_parent, t->node_active, n, out);
+    return check_launch("atdf_tree_pack");
+}
+
+static atdf_status check_sampling(const atdf_sampling* sp)
+{
+    ATDF_REQUIRE(sp, "sampling parameters are null");
+    ATDF_REQUIRE(sp->max_samples_per_node >= 1, "max_samples_per_node must be >= 1 (got %d)", sp->max_samples_per_node);
+    ATDF_REQUIRE(sp->max_intersections >= 1, "max_intersections must be >= 1");
+    ATDF_REQUIRE(sp->jitter_mode >= 0 && sp->jitter_mode <= 2, "jitter_mode must be 0, 1 or 2");
+    return ATDF_OK;
+}
+
+atdf_status atdf_traverse_count(const void* packed_tree, int64_t num_nodes, const float* ray_origin,
+                                const float* ray_direction, int64_t num_rays, const atdf_sampling* sampling,
+                                int32_t* ray_num_intervals, int32_t* ray_num_samples, int32_t* status,
+                                atdf_stream_t stream)
+{
+    atdf_status st = check_sampling(sampling);
+    if (st) return st;
+    ATDF_REQUIRE(packed_tree && num_nodes > 0, "atdf_traverse_count: empty tree");
+    ATDF_REQUIRE(num_rays >= 0, "atdf_traverse_count: negative ray count");
+    if (num_rays == 0) return ATDF_OK;
+    ATDF_REQUIRE(ray_origin && ray_direction && ray_num_intervals && ray_num_samples && status,
+                 "atdf_traverse_count: null buffer");
+    TraverseArgs a{};
+    a.nodes = reinterpret_cast<const PNode*>(packed_tree);
+    a.O = ray_origin;
+    a.D = ray_direction;
+    a.R = num_rays;
+    a.K = sampling->max_samples_per_node;
+    a.max_inter = sampling->max_intersections;
+    a.jitter_mode = sampling->jitter_mode;
+    a.seed = sampling->jitter_seed;
+    a.jitter = sampling->jitter;
+    a.ray_num_intervals = ray_num_intervals;
+    a.ray_num_samples = ray_num_samples;
+    a.status = status;
+    hipLaunchKernelGGL(traverse_kernel<false>, dim3((unsigned)div_up(num_rays, 256)), dim3(256), 0, to_hip(stream), a);
+    return check_launch("atdf_traverse_count");
+}
+
+atdf_status atdf_traverse_write(const void* packed_tree, int64_t num_nodes, const float* ray_origin,
+                                const float* ray_direction, int64_t num_rays, const atdf_sampling* sampling,
+                                const int64_t* ray_interval_offset, const int64_t* ray_sample_offset,
+                                const atdf_intervals* intervals, const atdf_samples* samples,
+                                atdf_stream_t stream)
+{
+    atdf_status st = check_sampling(sampling);
+    if (st) return st;
+    ATDF_REQUIRE(packed_tree && num_nodes > 0, "atdf_traverse_write: empty tree");
+    if (num_rays == 0) return ATDF_OK;
+    ATDF_REQUIRE(ray_origin && ray_direction, "atdf_traverse_write: null rays");
+    ATDF_REQUIRE(intervals || samples, "atdf_traverse_write: nothing to write");
+    if (intervals) {
+        ATDF_REQUIRE(ray_interval_offset && intervals->node && intervals->tmin && intervals->tmax &&
+                         intervals->count && intervals->first && intervals->ray,
+                     "atdf_traverse_write: incomplete interval buffers");
+    }
+    if (samples) {
+        ATDF_REQUIRE(ray_sample_offset && samples->global_coordinate && samples->weight && samples->ray_t &&
+                         samples->ray_index && samples->local_index_in_ray && samples->node_id,
+                     "atdf_traverse_write: incomplete sample buffers");
+    }
+    ATDF_REQUIRE(sampling->jitter_mode != 1 || (samples && sampling->jitter),
+                 "atdf_traverse_write: jitter_mode 1 needs sample output and a jitter array");
+    TraverseArgs a{};
+    a.nodes = reinterpret_cast<const PNode*>(packed_tree);
+    a.O = ray_origin;
+    a.D = ray_direction;
+    a.R = num_rays;
+    a.K = sampling->max_samples_per_node;
+    a.max_inter = sampling->max_intersections;
+    a.jitter_mode = sampling->jitter_mode;
+    a.seed = sampling->jitter_seed;
+    a.jitter = sampling->jitter;
+    a.iv_off = intervals ? ray_interval_offset : ray_sample_offset;
+    a.smp_off = samples ? ray_sample_offset : ray_interval_offset;
+    if (intervals) a.iv = *intervals;
+    if (samples) a.smp = *samples;
+    hipLaunchKernelGGL(traverse_kernel<true>, dim3((unsigned)div_up(num_rays, 256)), dim3(256), 0, to_hip(stream), a);
+    return check_launch("atdf_traverse_write");
+}
+
+size_t atdf_scan_workspace_bytes(int64_t n)
+{
+    int64_t nt = n > 0 ? div_up(n, kScanTile) : 1;
+    return (size_t)nt * sizeof(int64_t);
+}
+
+atdf_status atdf_exclusive_scan_i32(const int32_t* in, int64_t n, int64_t* out, void* workspace,
+                                    size_t workspace_bytes, atdf_stream_t stream)
+{
+    ATDF_REQUIRE(n >= 0 && out, "atdf_exclusive_scan_i32: bad arguments");
+    hipStream_t s = to_hip(stream);
+    if (n == 0) {
+        hipMemsetAsync(out, 0, sizeof(int64_t), s);
+        return check_launch("atdf_exclusive_scan_i32");
+    }
+    ATDF_REQUIRE(in && workspace, "atdf_exclusive_scan_i32: null buffer");
+    ATDF_REQUIRE(workspace_bytes >= atdf_scan_workspace_bytes(n), "atdf_exclusive_scan_i32: workspace too small");
+    int64_t nt = div_up(n, kScanTile);
+    int64_t* tiles = reinterpret_cast<int64_t*>(workspace);
+    hipLaunchKernelGGL(scan_tile_sums_kernel, dim3((unsigned)nt), dim3(kScanThreads), 0, s, in, n, tiles);
+    hipLaunchKernelGGL(scan_tile_offsets_kernel, dim3(1), dim3(kScanThreads), 0, s, tiles, nt);
+    hipLaunchKernelGGL(scan_apply_kernel, dim3((unsigned)nt), dim3(kScanThreads), 0, s, in, n, tiles, out);
+    return check_launch("atdf_exclusive_scan_i32");
+}
+
+}  // extern "C"

@@ -1,0 +1,105 @@
+// atdf_bindings.cpp -- Python bindings of the libtorch host layer.
+#include <torch/extension.h>
+
+#include "atdf_torch.h"
+
+namespace py = pybind11;
+using namespace atdf_torch;
+
+PYBIND11_MODULE(TORCH_EXTENSION_NAME, m)
+{
+    m.doc() = "libtorch host layer of the MI355X reconstruction hot path (calls libatdf.so)";
+
+    py::class_<RayList>(m, "RayList")
+        .def(py::init<>())
+        .def(py::init([](torch::Tensor o, torch::Tensor d) { return RayList{o, d}; }))
+        .def_readwrite("origin", &RayList::origin)
+        .def_readwrite("direction", &RayList::direction)
+        .def("size", &RayList::size);
+
+    py::class_<SampleList>(m, "SampleList")
+        .def(py::init<>())
+        .def_readwrite("global_coordinate", &SampleList::global_coordinate)
+        .def_readwrite("weight", &SampleList::weight)
+        .def_readwrite("ray_t", &SampleList::ray_t)
+        .def_readwrite("ray_index", &SampleList::ray_index)
+        .def_readwrite("local_index_in_ray", &SampleList::local_index_in_ray)
+        .def_readwrite("node_id", &SampleList::node_id)
+        .def_readwrite("max_samples_per_ray", &SampleList::max_samples_per_ray)
+        .def("size", &SampleList::size);
+
+    py::class_<Intervals>(m, "Intervals")
+        .def(py::init<>())
+        .def_readwrite("ray_offset", &Intervals::ray_offset)
+        .def_readwrite("node", &Intervals::node)
+        .def_readwrite("tmin", &Intervals::tmin)
+        .def_readwrite("tmax", &Intervals::tmax)
+        .def_readwrite("count", &Intervals::count)
+        .def_readwrite("first", &Intervals::first)
+        .def_readwrite("ray", &Intervals::ray)
+        .def_readwrite("ray_num_samples", &Intervals::ray_num_samples)
+        .def_readwrite("num_intervals", &Intervals::num_intervals)
+        .def_readwrite("num_samples", &Intervals::num_samples)
+        .def_readwrite("max_samples_per_ray", &Intervals::max_samples_per_ray);
+
+    py::class_<SamplingParams>(m, "SamplingParams")
+        .def(py::init<>())
+        .def_readwrite("max_samples_per_node", &SamplingParams::max_samples_per_node)
+        .def_readwrite("max_intersections", &SamplingParams::max_intersections)
+        .def_readwrite("jitter_mode", &SamplingParams::jitter_mode)
+        .def_readwrite("jitter_seed", &SamplingParams::jitter_seed);
+
+    py::class_<FieldParams>(m, "FieldParams")
+        .def(py::init<>())
+        .def_readwrite("C", &FieldParams::C)
+        .def_readwrite("H", &FieldParams::H)
+        .def_readwrite("G", &FieldParams::G)
+        .def_readwrite("softplus_beta", &FieldParams::softplus_beta)
+        .def_readwrite("rays_per_wave", &FieldParams::rays_per_wave);
+
+    py::class_<HyperTreeBase>(m, "HyperTreeBase")
+        .def(py::init<int, int>(), py::arg("d"), py::arg("max_depth"))
+        .def("settree", &HyperTreeBase::settree)
+        .def("SetActive", &HyperTreeBase::SetActive)
+        .def("UpdateActive", &HyperTreeBase::UpdateActive)
+        .def("to", [](HyperTreeBase& t, const std::string& dev) { t.to(torch::Device(dev)); })
+        .def("NumNodes", &HyperTreeBase::NumNodes)
+        .def("NumActiveNodes", &HyperTreeBase::NumActiveNodes)
+        .def("ActiveNodeTensor", &HyperTreeBase::ActiveNodeTensor)
+        .def("CreateSamplesForRays", &HyperTreeBase::CreateSamplesForRays)
+        .def("CreateIntervalsForRays", &HyperTreeBase::CreateIntervalsForRays)
+        .def("ComputeLocalSamples", &HyperTreeBase::ComputeLocalSamples)
+        .def("SetErrorForActiveNodes", &HyperTreeBase::SetErrorForActiveNodes)
+        .def("UpdateCulling", &HyperTreeBase::UpdateCulling)
+        .def_readwrite("node_parent", &HyperTreeBase::node_parent)
+        .def_readwrite("node_children", &HyperTreeBase::node_children)
+        .def_readwrite("node_position_min", &HyperTreeBase::node_position_min)
+        .def_readwrite("node_position_max", &HyperTreeBase::node_position_max)
+        .def_readwrite("node_scale", &HyperTreeBase::node_scale)
+        .def_readwrite("node_error", &HyperTreeBase::node_error)
+        .def_readwrite("node_max_density", &HyperTreeBase::node_max_density)
+        .def_readwrite("node_active", &HyperTreeBase::node_active)
+        .def_readwrite("node_culled", &HyperTreeBase::node_culled)
+        .def_readwrite("node_depth", &HyperTreeBase::node_depth)
+        .def_readwrite("node_active_prefix_sum", &HyperTreeBase::node_active_prefix_sum)
+        .def_readwrite("active_node_ids", &HyperTreeBase::active_node_ids)
+        .def_readwrite("node_to_grid", &HyperTreeBase::node_to_grid);
+
+    m.def("IndirectGridSample3D", &IndirectGridSample3D);
+    m.def("IntegrateSamplesXRay", &IntegrateSamplesXRay);
+    m.def("AccumulateSampleLossPerNode", &AccumulateSampleLossPerNode);
+    m.def("ProjectRays", &ProjectRays);
+
+    py::class_<TrainState>(m, "TrainState")
+        .def_readwrite("grid", &TrainState::grid)
+        .def_readwrite("grid_master", &TrainState::grid_master)
+        .def_readwrite("mlp", &TrainState::mlp)
+        .def_readwrite("grad_grid", &TrainState::grad_grid)
+        .def_readwrite("grad_mlp", &TrainState::grad_mlp)
+        .def_readwrite("loss", &TrainState::loss)
+        .def_readwrite("step", &TrainState::step)
+        .def_readwrite("lr_grid", &TrainState::lr_grid)
+        .def_readwrite("lr_mlp", &TrainState::lr_mlp);
+    m.def("MakeTrainState", &MakeTrainState);
+    m.def("TrainStep", &TrainStep);
+}

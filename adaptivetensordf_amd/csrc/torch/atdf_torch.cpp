@@ -1,0 +1,683 @@
+// atdf_torch.cpp -- libtorch host layer over libatdf.so (see atdf_torch.h).
+#include "atdf_torch.h"
+
+#include <c10/hip/HIPStream.h>
+
+#include <cmath>
+#include <cstring>
+#include <stdexcept>
+
+namespace atdf_torch {
+
+using torch::Tensor;
+
+atdf_stream_t current_stream()
+{
+    return reinterpret_cast<atdf_stream_t>(c10::hip::getCurrentHIPStream().stream());
+}
+
+void check_status(atdf_status s, const char* what)
+{
+    TORCH_CHECK(s == ATDF_OK, what, " failed (status ", s, "): ", atdf_last_error());
+}
+
+static void require_device_f32(const Tensor& t, const char* name)
+{
+    TORCH_CHECK(t.defined(), name, " is undefined");
+    TORCH_CHECK(t.is_cuda(), name, " must be a GPU tensor (the reference path is GPU-only)");
+    TORCH_CHECK(t.scalar_type() == torch::kFloat32, name, " must be float32");
+    TORCH_CHECK(t.is_contiguous(), name, " must be contiguous");
+}
+
+atdf_intervals Intervals::view() const
+{
+    atdf_intervals v{};
+    if (num_intervals > 0) {
+        v.node = node.data_ptr<int32_t>();
+        v.tmin = tmin.data_ptr<float>();
+        v.tmax = tmax.data_ptr<float>();
+        v.count = count.data_ptr<int32_t>();
+        v.first = first.data_ptr<int32_t>();
+        v.ray = ray.data_ptr<int32_t>();
+    }
+    return v;
+}
+
+atdf_sampling SamplingParams::view(const float* jitter) const
+{
+    atdf_sampling s{};
+    s.max_samples_per_node = max_samples_per_node;
+    s.max_intersections = max_intersections;
+    s.jitter_mode = jitter_mode;
+    s.jitter_seed = jitter_seed;
+    s.jitter = jitter;
+    return s;
+}
+
+// ---------------------------------------------------------------------------
+// HyperTreeBase
+// ---------------------------------------------------------------------------
+HyperTreeBase::HyperTreeBase(int d_, int max_depth_) : d(d_), max_depth(max_depth_)
+{
+    TORCH_CHECK(d == 3, "HyperTreeBase: only D = 3 (octree) is built");
+    TORCH_CHECK(max_depth >= 0 && max_depth <= 8, "HyperTreeBase: max_depth must be in [0, 8]");
+}
+
+void HyperTreeBase::settree()
+{
+    const int NS = 8;
+    int64_t num_nodes = 0, p = 1;
+    for (int i = 0; i <= max_depth; ++i) {
+        num_nodes += p;
+        p *= NS;
+    }
+    node_parent = torch::empty({num_nodes}, torch::kInt32);
+    node_children = -torch::ones({num_nodes, NS}, torch::kInt32);
+    node_position_min = torch::empty({num_nodes, 3}, torch::kFloat32);
+    node_position_max = torch::empty({num_nodes, 3}, torch::kFloat32);
+    node_scale = torch::empty({num_nodes}, torch::kFloat32);
+    node_error = -torch::ones({num_nodes}, torch::kFloat32);
+    node_max_density = -torch::ones({num_nodes}, torch::kFloat32);
+    node_active = torch::zeros({num_nodes}, torch::kInt32);
+    node_culled = torch::zeros({num_nodes}, torch::kInt32);
+    node_depth = torch::zeros({num_nodes}, torch::kInt32);
+
+    float* pmin = node_position_min.data_ptr<float>();
+    float* pmax = node_position_max.data_ptr<float>();
+    int32_t* ch = node_children.data_ptr<int32_t>();
+    int32_t* par = node_parent.data_ptr<int32_t>();
+    int32_t* dep = node_depth.data_ptr<int32_t>();
+    float* scl = node_scale.data_ptr<float>();
+    auto set_node = [&](int64_t id, int64_t parent, const float* mn, const float* mx, int depth) {
+        float scale = max_depth == 0 ? 0.f : ((float)depth / (float)max_depth) * 2 - 1;
+        for (int k = 0; k < 3; ++k) {
+            pmin[id * 3 + k] = mn[k];
+            pmax[id * 3 + k] = mx[k];
+        }
+        dep[id] = depth;
+        par[id] = (int32_t)parent;
+        scl[id] = scale;
+    };
+    const float rmin[3] = {-1.f, -1.f, -1.f}, rmax[3] = {1.f, 1.f, 1.f};
+    int64_t cur = 0;
+    set_node(cur++, -1, rmin, rmax, 0);
+    int64_t start_layer = 0;
+    for (int depth = 1; depth <= max_depth; ++depth) {
+        int64_t n = cur - start_layer;
+        for (int64_t j = 0; j < n; ++j) {
+            int64_t id = start_layer + j;
+            float c[3], mn[3], mx[3];
+            for (int k = 0; k < 3; ++k) c[k] = (pmin[id * 3 + k] + pmax[id * 3 + k]) * 0.5f;
+            for (int i = 0; i < NS; ++i) {
+                for (int k = 0; k < 3; ++k) {
+                    if (((i >> k) & 1) == 0) {
+                        mn[k] = pmin[id * 3 + k];
+                        mx[k] = c[k];
+                    } else {
+                        mn[k] = c[k];
+                        mx[k] = pmax[id * 3 + k];
+                    }
+                }
+                ch[id * NS + i] = (int32_t)cur;
+                set_node(cur++, id, mn, mx, depth);
+            }
+        }
+        start_layer = cur - n * NS;
+    }
+    active_node_ids = torch::zeros({1}, torch::kLong);
+    node_active_prefix_sum = torch::zeros({num_nodes}, torch::kLong);
+    SetActive(0);
+}
+
+void HyperTreeBase::SetActive(int depth)
+{
+    node_active = (node_depth == depth).to(torch::kInt32);
+    UpdateActive();
+}
+
+void HyperTreeBase::UpdateActive()
+{
+    auto act = node_active.to(torch::kCPU).contiguous();
+    auto cul = node_culled.to(torch::kCPU).contiguous();
+    TORCH_CHECK((act * cul).sum().item<int64_t>() == 0, "UpdateActive: a culled node is not allowed to be active");
+    const int32_t* a = act.data_ptr<int32_t>();
+    int64_t n = act.size(0);
+    std::vector<int64_t> ids;
+    auto prefix = torch::empty({n}, torch::kLong);
+    int64_t* pp = prefix.data_ptr<int64_t>();
+    int64_t count = 0;
+    for (int64_t i = 0; i < n; ++i) {
+        if (a[i] == 1) {
+            ids.push_back(i);
+            pp[i] = count++;
+        } else {
+            pp[i] = -1;
+        }
+    }
+    auto dev = node_position_min.device();
+    active_node_ids = torch::from_blob(ids.data(), {(int64_t)ids.size()}, torch::kLong).clone().to(dev);
+    node_active_prefix_sum = prefix.to(dev);
+    node_to_grid = prefix.to(torch::kInt32).to(dev);
+    node_active = node_active.to(dev);
+    if (dev.is_cuda()) repack();
+}
+
+void HyperTreeBase::to(torch::Device device)
+{
+    node_parent = node_parent.to(device);
+    node_children = node_children.to(device);
+    node_position_min = node_position_min.to(device);
+    node_position_max = node_position_max.to(device);
+    node_scale = node_scale.to(device);
+    node_error = node_error.to(device);
+    node_max_density = node_max_density.to(device);
+    node_active = node_active.to(device);
+    node_culled = node_culled.to(device);
+    node_depth = node_depth.to(device);
+    node_active_prefix_sum = node_active_prefix_sum.to(device);
+    active_node_ids = active_node_ids.to(device);
+    node_to_grid = node_to_grid.to(device);
+    if (device.is_cuda()) repack();
+}
+
+void HyperTreeBase::repack()
+{
+    int64_t n = NumNodes();
+    packed_ = torch::empty({(int64_t)atdf_tree_packed_bytes(n)},
+                           torch::TensorOptions().dtype(torch::kUInt8).device(node_position_min.device()));
+    atdf_tree t{};
+    t.num_nodes = n;
+    t.node_position_min = node_position_min.data_ptr<float>();
+    t.node_position_max = node_position_max.data_ptr<float>();
+    t.node_children = node_children.data_ptr<int32_t>();
+    t.node_parent = node_parent.data_ptr<int32_t>();
+    t.node_active = node_active.data_ptr<int32_t>();
+    check_status(atdf_tree_pack(&t, packed_.data_ptr(), current_stream()), "atdf_tree_pack");
+}
+
+namespace {
+struct TraversalCounts {
+    Tensor num_iv, num_s, iv_off, smp_off;
+    int64_t total_iv = 0, total_s = 0;
+    int max_per_ray = 0;
+};
+
+TraversalCounts count_and_scan(HyperTreeBase& tree, const RayList& rays, const atdf_sampling& s)
+{
+    TraversalCounts tc;
+    const int64_t R = rays.size();
+    auto dev = tree.device();
+    auto i32 = torch::TensorOptions().dtype(torch::kInt32).device(dev);
+    auto i64 = torch::TensorOptions().dtype(torch::kLong).device(dev);
+    tc.num_iv = torch::empty({R}, i32);
+    tc.num_s = torch::empty({R}, i32);
+    tc.iv_off = torch::empty({R + 1}, i64);
+    tc.smp_off = torch::empty({R + 1}, i64);
+    auto status = torch::zeros({4}, i32);
+    auto st = current_stream();
+    if (R > 0) {
+        check_status(atdf_traverse_count(tree.packed(), tree.NumNodes(), rays.origin.data_ptr<float>(),
+                                         rays.direction.data_ptr<float>(), R, &s, tc.num_iv.data_ptr<int32_t>(),
+                                         tc.num_s.data_ptr<int32_t>(), status.data_ptr<int32_t>(), st),
+                     "atdf_traverse_count");
+    }
+    auto ws = torch::empty({(int64_t)atdf_scan_workspace_bytes(R)},
+                           torch::TensorOptions().dtype(torch::kUInt8).device(dev));
+    check_status(atdf_exclusive_scan_i32(tc.num_iv.data_ptr<int32_t>(), R, tc.iv_off.data_ptr<int64_t>(),
+                                         ws.data_ptr(), ws.numel(), st),
+                 "atdf_exclusive_scan_i32");
+    check_status(atdf_exclusive_scan_i32(tc.num_s.data_ptr<int32_t>(), R, tc.smp_off.data_ptr<int64_t>(),
+                                         ws.data_ptr(), ws.numel(), st),
+                 "atdf_exclusive_scan_i32");
+    // one device -> host copy: status, totals (the reference reads its sample
+    // count back the same way, HyperTree.cu:550)
+    auto info = torch::cat({status.to(torch::kLong), tc.iv_off.slice(0, R, R + 1), tc.smp_off.slice(0, R, R + 1)})
+                    .to(torch::kCPU);
+    const int64_t* ip = info.data_ptr<int64_t>();
+    TORCH_CHECK(ip[0] == 0, "CreateSamplesForRays: ", ip[0], " ray(s) hit more than ", s.max_intersections,
+                " nodes (max_intersections)");
+    TORCH_CHECK(ip[1] == 0, "CreateSamplesForRays: traversal order check failed on ", ip[1], " ray(s)");
+    tc.max_per_ray = (int)ip[2];
+    tc.total_iv = ip[4];
+    tc.total_s = ip[5];
+    return tc;
+}
+
+void check_rays(const RayList& rays)
+{
+    require_device_f32(rays.origin, "rays.origin");
+    require_device_f32(rays.direction, "rays.direction");
+    TORCH_CHECK(rays.origin.dim() == 2 && rays.origin.size(1) == 3, "rays.origin must be [R, 3]");
+    TORCH_CHECK(rays.direction.sizes() == rays.origin.sizes(), "rays.direction must match rays.origin");
+}
+}  // namespace
+
+Intervals HyperTreeBase::CreateIntervalsForRays(const RayList& rays, const SamplingParams& sp)
+{
+    check_rays(rays);
+    TORCH_CHECK(sp.jitter_mode != 1, "CreateIntervalsForRays: jitter_mode 1 needs materialised samples");
+    atdf_sampling s = sp.view();
+    auto tc = count_and_scan(*this, rays, s);
+    Intervals iv;
+    auto dev = device();
+    auto i32 = torch::TensorOptions().dtype(torch::kInt32).device(dev);
+    auto f32 = torch::TensorOptions().dtype(torch::kFloat32).device(dev);
+    iv.ray_offset = tc.iv_off;
+    iv.ray_num_samples = tc.num_s;
+    iv.num_intervals = tc.total_iv;
+    iv.num_samples = tc.total_s;
+    iv.max_samples_per_ray = tc.max_per_ray;
+    const int64_t H = tc.total_iv;
+    iv.node = torch::empty({H}, i32);
+    iv.tmin = torch::empty({H}, f32);
+    iv.tmax = torch::empty({H}, f32);
+    iv.count = torch::empty({H}, i32);
+    iv.first = torch::empty({H}, i32);
+    iv.ray = torch::empty({H}, i32);
+    if (H > 0) {
+        atdf_intervals v = iv.view();
+        check_status(atdf_traverse_write(packed(), NumNodes(), rays.origin.data_ptr<float>(),
+                                         rays.direction.data_ptr<float>(), rays.size(), &s,
+                                         iv.ray_offset.data_ptr<int64_t>(), tc.smp_off.data_ptr<int64_t>(), &v,
+                                         nullptr, current_stream()),
+                     "atdf_traverse_write");
+    }
+    return iv;
+}
+
+SampleList HyperTreeBase::CreateSamplesForRays(const RayList& rays, int max_samples_per_node, bool interval_jitter)
+{
+    check_rays(rays);
+    SamplingParams sp;
+    sp.max_samples_per_node = max_samples_per_node;
+    sp.jitter_mode = 0;
+    atdf_sampling s = sp.view();
+    auto tc = count_and_scan(*this, rays, s);
+    const int64_t S = tc.total_s;
+    auto dev = device();
+    auto f32 = torch::TensorOptions().dtype(torch::kFloat32).device(dev);
+    auto i64 = torch::TensorOptions().dtype(torch::kLong).device(dev);
+    SampleList list;
+    list.global_coordinate = torch::empty({S, 3}, f32);
+    list.weight = torch::empty({S}, f32);
+    list.ray_t = torch::empty({S}, f32);
+    list.ray_index = torch::empty({S}, i64);
+    list.local_index_in_ray = torch::empty({S}, i64);
+    list.node_id = torch::empty({S}, i64);
+    list.max_samples_per_ray = tc.max_per_ray;
+    Tensor rnd;
+    if (interval_jitter) {
+        // the reference draws one U[0,1) per sample slot (HyperTree.cu:533)
+        rnd = torch::rand({S}, f32);
+        s.jitter_mode = 1;
+        s.jitter = rnd.data_ptr<float>();
+    }
+    if (S > 0) {
+        atdf_samples v{list.global_coordinate.data_ptr<float>(), list.weight.data_ptr<float>(),
+                       list.ray_t.data_ptr<float>(),              list.ray_index.data_ptr<int64_t>(),
+                       list.local_index_in_ray.data_ptr<int64_t>(), list.node_id.data_ptr<int64_t>()};
+        check_status(atdf_traverse_write(packed(), NumNodes(), rays.origin.data_ptr<float>(),
+                                         rays.direction.data_ptr<float>(), rays.size(), &s, tc.iv_off.data_ptr<int64_t>(),
+                                         tc.smp_off.data_ptr<int64_t>(), nullptr, &v, current_stream()),
+                     "atdf_traverse_write");
+    }
+    return list;
+}
+
+Tensor HyperTreeBase::ComputeLocalSamples(Tensor global_samples, Tensor node_indices)
+{
+    require_device_f32(global_samples, "global_samples");
+    TORCH_CHECK(node_indices.is_cuda() && node_indices.scalar_type() == torch::kLong, "node_indices must be GPU int64");
+    auto g = global_samples.reshape({-1, 3}).contiguous();
+    auto nd = node_indices.reshape({-1}).contiguous();
+    TORCH_CHECK(g.size(0) == nd.size(0), "ComputeLocalSamples: ", g.size(0), " samples vs ", nd.size(0), " node ids");
+    auto local = torch::empty_like(g);
+    check_status(atdf_local_samples(node_position_min.data_ptr<float>(), node_position_max.data_ptr<float>(),
+                                    g.data_ptr<float>(), nd.data_ptr<int64_t>(), g.size(0), local.data_ptr<float>(),
+                                    current_stream()),
+                 "atdf_local_samples");
+    return local.reshape(global_samples.sizes());
+}
+
+void HyperTreeBase::SetErrorForActiveNodes(Tensor error, const std::string& strategy)
+{
+    TORCH_CHECK(error.sizes() == node_error.sizes(), "SetErrorForActiveNodes: size mismatch");
+    TORCH_CHECK(error.device() == node_error.device(), "SetErrorForActiveNodes: device mismatch");
+    auto active_float = node_active.to(torch::kFloat32);
+    auto inactive = 1 - active_float;
+    auto invalid_errors = (node_error < 0).to(torch::kFloat32);
+    error = active_float * error.to(torch::kFloat32);
+    if (strategy == "override") {
+        node_error = node_error * inactive + error;
+    } else if (strategy == "min") {
+        error = error + inactive * 1000000;
+        auto filtered = node_error + (invalid_errors * active_float) * 1000000;
+        node_error = torch::min(filtered, error);
+    } else {
+        TORCH_CHECK(false, "SetErrorForActiveNodes: unknown strategy ", strategy);
+    }
+}
+
+void HyperTreeBase::UpdateCulling()
+{
+    auto culled = node_culled.to(torch::kCPU).contiguous();
+    auto children = node_children.to(torch::kCPU).contiguous();
+    auto active = node_active.to(torch::kCPU).contiguous();
+    int32_t* cp = culled.data_ptr<int32_t>();
+    const int32_t* chp = children.data_ptr<int32_t>();
+    const int32_t* ap = active.data_ptr<int32_t>();
+    bool changed = true;
+    while (changed) {
+        changed = false;
+        for (int64_t i = 0; i < culled.size(0); ++i) {
+            if (cp[i]) continue;
+            if (chp[i * 8] < 0) continue;  // leaves have no children to inherit culling from
+            bool all = true;
+            for (int c = 0; c < 8; ++c)
+                if (!cp[chp[i * 8 + c]]) all = false;
+            if (all) {
+                TORCH_CHECK(!ap[i], "UpdateCulling: active node with all children culled");
+                cp[i] = 1;
+                changed = true;
+            }
+        }
+    }
+    node_culled = culled.to(node_culled.device());
+    node_error = node_error * (1 - node_culled);
+}
+
+// ---------------------------------------------------------------------------
+// IndirectGridSample3D
+// ---------------------------------------------------------------------------
+namespace {
+struct IndirectGridSample3DFn : public torch::autograd::Function<IndirectGridSample3DFn> {
+    static Tensor forward(torch::autograd::AutogradContext* ctx, Tensor grid, Tensor index, Tensor uv)
+    {
+        require_device_f32(grid, "multi_grid");
+        require_device_f32(uv, "uv");
+        TORCH_CHECK(grid.dim() == 5, "multi_grid must be [N, C, Z, Y, X]");
+        TORCH_CHECK(index.dim() == 1 && index.scalar_type() == torch::kLong && index.is_cuda(), "index must be GPU int64 [S]");
+        TORCH_CHECK(uv.dim() == 2 && uv.size(1) == 3 && uv.size(0) == index.size(0), "uv must be [S, 3]");
+        auto idx = index.contiguous();
+        const int64_t S = idx.size(0), C = grid.size(1);
+        auto out = torch::empty({S, C}, grid.options());
+        check_status(atdf_grid_sample3d_fwd(grid.data_ptr<float>(), grid.size(0), (int)C, (int)grid.size(2),
+                                            (int)grid.size(3), (int)grid.size(4), idx.data_ptr<int64_t>(),
+                                            uv.data_ptr<float>(), S, out.data_ptr<float>(), current_stream()),
+                     "atdf_grid_sample3d_fwd");
+        ctx->save_for_backward({grid, idx, uv});
+        return out;
+    }
+    static torch::autograd::variable_list backward(torch::autograd::AutogradContext* ctx,
+                                                   torch::autograd::variable_list grad_output)
+    {
+        auto saved = ctx->get_saved_variables();
+        auto grid = saved[0], idx = saved[1], uv = saved[2];
+        auto go = grad_output[0].contiguous();
+        auto grad_grid = torch::zeros_like(grid);
+        auto grad_uv = torch::zeros_like(uv);
+        check_status(atdf_grid_sample3d_bwd(grid.data_ptr<float>(), grid.size(0), (int)grid.size(1), (int)grid.size(2),
+                                            (int)grid.size(3), (int)grid.size(4), idx.data_ptr<int64_t>(),
+                                            uv.data_ptr<float>(), idx.size(0), go.data_ptr<float>(),
+                                            grad_grid.data_ptr<float>(), grad_uv.data_ptr<float>(), current_stream()),
+                     "atdf_grid_sample3d_bwd");
+        return {grad_grid, Tensor(), grad_uv};
+    }
+};
+
+struct IntegrateFn : public torch::autograd::Function<IntegrateFn> {
+    static Tensor forward(torch::autograd::AutogradContext* ctx, Tensor values, Tensor weight, Tensor ray_index,
+                          int64_t C, int64_t R)
+    {
+        auto v = values.reshape({-1, C}).contiguous();
+        auto w = weight.reshape({-1}).contiguous();
+        auto ri = ray_index.reshape({-1}).contiguous();
+        require_device_f32(v, "sample_values");
+        require_device_f32(w, "integration_weight");
+        TORCH_CHECK(ri.scalar_type() == torch::kLong && ri.is_cuda(), "ray_index must be GPU int64");
+        TORCH_CHECK(v.size(0) == w.size(0) && w.size(0) == ri.size(0), "IntegrateSamplesXRay: size mismatch");
+        auto out = torch::zeros({C, R}, v.options());
+        check_status(atdf_integrate_xray(v.data_ptr<float>(), w.data_ptr<float>(), ri.data_ptr<int64_t>(), v.size(0),
+                                         (int)C, R, out.data_ptr<float>(), current_stream()),
+                     "atdf_integrate_xray");
+        ctx->save_for_backward({v, w, ri});
+        ctx->saved_data["vshape"] = values.sizes().vec();
+        ctx->saved_data["wshape"] = weight.sizes().vec();
+        return out;
+    }
+    static torch::autograd::variable_list backward(torch::autograd::AutogradContext* ctx,
+                                                   torch::autograd::variable_list grad_output)
+    {
+        auto saved = ctx->get_saved_variables();
+        auto v = saved[0], w = saved[1], ri = saved[2];
+        // d out[c, ray[s]] / d v[s, c] = w[s];  / d w[s] = sum_c v[s, c] * g
+        auto g = grad_output[0].t().index_select(0, ri);  // [S, C]
+        auto gv = (g * w.unsqueeze(1)).reshape(ctx->saved_data["vshape"].toIntVector());
+        auto gw = (g * v).sum(1).reshape(ctx->saved_data["wshape"].toIntVector());
+        return {gv, gw, Tensor(), Tensor(), Tensor()};
+    }
+};
+}  // namespace
+
+Tensor IndirectGridSample3D(Tensor multi_grid, Tensor index, Tensor uv)
+{
+    return IndirectGridSample3DFn::apply(multi_grid, index, uv);
+}
+
+Tensor IntegrateSamplesXRay(Tensor sample_values, Tensor integration_weight, Tensor ray_index, int num_channels,
+                            int64_t num_rays)
+{
+    if (sample_values.numel() == 0)
+        return torch::zeros({num_channels, num_rays}, sample_values.options());
+    return IntegrateFn::apply(sample_values, integration_weight, ray_index, (int64_t)num_channels, num_rays);
+}
+
+std::pair<Tensor, Tensor> AccumulateSampleLossPerNode(const SampleList& samples, Tensor per_ray_loss,
+                                                      int64_t num_nodes)
+{
+    auto prl = per_ray_loss.to(torch::kFloat32).contiguous();
+    TORCH_CHECK(prl.dim() == 2, "per_ray_loss must be [C, R]");
+    auto opts = torch::TensorOptions().dtype(torch::kFloat64).device(prl.device());
+    auto loss_sum = torch::zeros({num_nodes}, opts);
+    auto weight_sum = torch::zeros({num_nodes}, opts);
+    check_status(atdf_accumulate_loss_per_node(prl.data_ptr<float>(), (int)prl.size(0), prl.size(1),
+                                               samples.ray_index.data_ptr<int64_t>(), samples.weight.data_ptr<float>(),
+                                               samples.node_id.data_ptr<int64_t>(), samples.size(), num_nodes,
+                                               loss_sum.data_ptr<double>(), weight_sum.data_ptr<double>(),
+                                               current_stream()),
+                 "atdf_accumulate_loss_per_node");
+    return {loss_sum, weight_sum};
+}
+
+// ---------------------------------------------------------------------------
+// fused projection
+// ---------------------------------------------------------------------------
+namespace {
+atdf_field make_field(const HyperTreeBase& tree, const Tensor& grid, const Tensor& mlp, const FieldParams& fp)
+{
+    TORCH_CHECK(grid.is_cuda() && grid.is_contiguous(), "grid must be a contiguous GPU tensor");
+    TORCH_CHECK(grid.dim() == 5 && grid.size(1) == fp.G && grid.size(2) == fp.G && grid.size(3) == fp.G &&
+                    grid.size(4) == fp.C,
+                "grid must be [N, G, G, G, C] = [N, ", fp.G, ", ", fp.G, ", ", fp.G, ", ", fp.C, "]");
+    TORCH_CHECK(grid.scalar_type() == torch::kFloat32 || grid.scalar_type() == torch::kBFloat16,
+                "grid must be float32 or bfloat16");
+    require_device_f32(mlp, "mlp");
+    const int64_t np = (int64_t)fp.H * fp.C + fp.H + (int64_t)fp.H * fp.H + fp.H + fp.H + 1;
+    TORCH_CHECK(mlp.numel() == np, "mlp has ", mlp.numel(), " parameters, expected ", np);
+    TORCH_CHECK(grid.size(0) >= tree.NumActiveNodes(), "grid has ", grid.size(0), " node grids for ",
+                tree.NumActiveNodes(), " active nodes");
+    atdf_field f{};
+    f.C = fp.C;
+    f.H = fp.H;
+    f.G = fp.G;
+    f.grid_dtype = grid.scalar_type() == torch::kBFloat16 ? ATDF_GRID_BF16 : ATDF_GRID_F32;
+    f.softplus_beta = fp.softplus_beta;
+    f.rays_per_wave = fp.rays_per_wave;
+    f.grid = grid.data_ptr();
+    f.num_grids = grid.size(0);
+    f.node_to_grid = tree.node_to_grid.data_ptr<int32_t>();
+    f.mlp = mlp.data_ptr<float>();
+    return f;
+}
+
+struct ProjectRaysFn : public torch::autograd::Function<ProjectRaysFn> {
+    static Tensor forward(torch::autograd::AutogradContext* ctx, Tensor grid, Tensor mlp, Tensor origin,
+                          Tensor direction, HyperTreeBase* tree, SamplingParams sp, FieldParams fp)
+    {
+        RayList rays{origin, direction};
+        Intervals iv = tree->CreateIntervalsForRays(rays, sp);
+        atdf_field f = make_field(*tree, grid, mlp, fp);
+        atdf_sampling s = sp.view();
+        auto P = torch::empty({rays.size()}, origin.options());
+        atdf_intervals v = iv.view();
+        check_status(atdf_project_fwd(tree->packed(), tree->NumNodes(), origin.data_ptr<float>(),
+                                      direction.data_ptr<float>(), rays.size(), &s, iv.ray_offset.data_ptr<int64_t>(),
+                                      &v, &f, P.data_ptr<float>(), current_stream()),
+                     "atdf_project_fwd");
+        ctx->save_for_backward({grid, mlp, origin, direction, iv.ray_offset, iv.node, iv.tmin, iv.tmax, iv.count,
+                                iv.first, iv.ray});
+        ctx->saved_data["tree"] = (int64_t)reinterpret_cast<intptr_t>(tree);
+        ctx->saved_data["K"] = (int64_t)sp.max_samples_per_node;
+        ctx->saved_data["maxi"] = (int64_t)sp.max_intersections;
+        ctx->saved_data["jm"] = (int64_t)sp.jitter_mode;
+        ctx->saved_data["seed"] = (int64_t)sp.jitter_seed;
+        ctx->saved_data["C"] = (int64_t)fp.C;
+        ctx->saved_data["H"] = (int64_t)fp.H;
+        ctx->saved_data["G"] = (int64_t)fp.G;
+        ctx->saved_data["beta"] = (double)fp.softplus_beta;
+        ctx->saved_data["rpw"] = (int64_t)fp.rays_per_wave;
+        ctx->saved_data["H_iv"] = iv.num_intervals;
+        return P;
+    }
+
+    static torch::autograd::variable_list backward(torch::autograd::AutogradContext* ctx,
+                                                   torch::autograd::variable_list grad_output)
+    {
+        auto sv = ctx->get_saved_variables();
+        auto grid = sv[0], mlp = sv[1], origin = sv[2], direction = sv[3];
+        Intervals iv;
+        iv.ray_offset = sv[4];
+        iv.node = sv[5];
+        iv.tmin = sv[6];
+        iv.tmax = sv[7];
+        iv.count = sv[8];
+        iv.first = sv[9];
+        iv.ray = sv[10];
+        iv.num_intervals = ctx->saved_data["H_iv"].toInt();
+        auto* tree = reinterpret_cast<HyperTreeBase*>((intptr_t)ctx->saved_data["tree"].toInt());
+        SamplingParams sp;
+        sp.max_samples_per_node = (int)ctx->saved_data["K"].toInt();
+        sp.max_intersections = (int)ctx->saved_data["maxi"].toInt();
+        sp.jitter_mode = (int)ctx->saved_data["jm"].toInt();
+        sp.jitter_seed = (uint64_t)ctx->saved_data["seed"].toInt();
+        FieldParams fp;
+        fp.C = (int)ctx->saved_data["C"].toInt();
+        fp.H = (int)ctx->saved_data["H"].toInt();
+        fp.G = (int)ctx->saved_data["G"].toInt();
+        fp.softplus_beta = (float)ctx->saved_data["beta"].toDouble();
+        fp.rays_per_wave = (int)ctx->saved_data["rpw"].toInt();
+        atdf_field f = make_field(*tree, grid, mlp, fp);
+        atdf_sampling s = sp.view();
+        auto dP = grad_output[0].to(torch::kFloat32).contiguous();
+        const int64_t R = origin.size(0);
+        auto grad_grid = torch::zeros(grid.sizes(), grid.options().dtype(torch::kFloat32));
+        auto grad_mlp = torch::zeros_like(mlp);
+        auto ws = torch::empty({(int64_t)atdf_project_bwd_workspace_bytes(R, &f)},
+                               torch::TensorOptions().dtype(torch::kUInt8).device(grid.device()));
+        atdf_intervals v = iv.view();
+        check_status(atdf_project_bwd(tree->packed(), tree->NumNodes(), origin.data_ptr<float>(),
+                                      direction.data_ptr<float>(), R, &s, iv.ray_offset.data_ptr<int64_t>(), &v, &f,
+                                      dP.data_ptr<float>(), grad_grid.data_ptr<float>(), grad_mlp.data_ptr<float>(),
+                                      ws.data_ptr(), ws.numel(), current_stream()),
+                     "atdf_project_bwd");
+        if (grid.scalar_type() != torch::kFloat32) grad_grid = grad_grid.to(grid.scalar_type());
+        return {grad_grid, grad_mlp, Tensor(), Tensor(), Tensor(), Tensor(), Tensor()};
+    }
+};
+}  // namespace
+
+Tensor ProjectRays(HyperTreeBase& tree, const RayList& rays, Tensor grid, Tensor mlp, const SamplingParams& sp,
+                   const FieldParams& fp)
+{
+    check_rays(rays);
+    return ProjectRaysFn::apply(grid, mlp, rays.origin, rays.direction, &tree, sp, fp);
+}
+
+TrainState MakeTrainState(Tensor grid, Tensor mlp, bool bf16_grid, float lr_grid, float lr_mlp)
+{
+    require_device_f32(grid, "grid");
+    require_device_f32(mlp, "mlp");
+    TrainState st;
+    if (bf16_grid) {
+        st.grid_master = grid.clone();
+        st.grid = torch::empty(grid.sizes(), grid.options().dtype(torch::kBFloat16));
+        check_status(atdf_f32_to_bf16(st.grid_master.data_ptr<float>(),
+                                      reinterpret_cast<uint16_t*>(st.grid.data_ptr()), grid.numel(), current_stream()),
+                     "atdf_f32_to_bf16");
+    } else {
+        st.grid = grid.clone();
+    }
+    st.mlp = mlp.clone();
+    st.grad_grid = torch::zeros(grid.sizes(), grid.options());
+    st.grad_mlp = torch::zeros_like(mlp);
+    st.m_grid = torch::zeros(grid.sizes(), grid.options());
+    st.v_grid = torch::zeros(grid.sizes(), grid.options());
+    st.m_mlp = torch::zeros_like(mlp);
+    st.v_mlp = torch::zeros_like(mlp);
+    st.loss = torch::zeros({1}, grid.options().dtype(torch::kFloat64));
+    st.lr_grid = lr_grid;
+    st.lr_mlp = lr_mlp;
+    return st;
+}
+
+Tensor TrainStep(HyperTreeBase& tree, TrainState& st, const RayList& rays, Tensor target, const SamplingParams& sp,
+                 const FieldParams& fp)
+{
+    check_rays(rays);
+    require_device_f32(target, "target");
+    const int64_t R = rays.size();
+    TORCH_CHECK(target.numel() == R, "target must hold one value per ray");
+    auto stream = current_stream();
+    Intervals iv = tree.CreateIntervalsForRays(rays, sp);
+    atdf_field f = make_field(tree, st.grid, st.mlp, fp);
+    atdf_sampling s = sp.view();
+    atdf_intervals v = iv.view();
+    auto P = torch::empty({R}, rays.origin.options());
+    check_status(atdf_project_fwd(tree.packed(), tree.NumNodes(), rays.origin.data_ptr<float>(),
+                                  rays.direction.data_ptr<float>(), R, &s, iv.ray_offset.data_ptr<int64_t>(), &v, &f,
+                                  P.data_ptr<float>(), stream),
+                 "atdf_project_fwd");
+    auto dP = torch::empty({R}, rays.origin.options());
+    auto mse_ws = torch::empty({(int64_t)atdf_mse_workspace_bytes(R)},
+                               torch::TensorOptions().dtype(torch::kUInt8).device(rays.origin.device()));
+    check_status(atdf_mse_loss(P.data_ptr<float>(), target.data_ptr<float>(), R, 1.0f, dP.data_ptr<float>(),
+                               st.loss.data_ptr<double>(), mse_ws.data_ptr(), mse_ws.numel(), stream),
+                 "atdf_mse_loss");
+    auto ws = torch::empty({(int64_t)atdf_project_bwd_workspace_bytes(R, &f)},
+                           torch::TensorOptions().dtype(torch::kUInt8).device(rays.origin.device()));
+    check_status(atdf_project_bwd(tree.packed(), tree.NumNodes(), rays.origin.data_ptr<float>(),
+                                  rays.direction.data_ptr<float>(), R, &s, iv.ray_offset.data_ptr<int64_t>(), &v, &f,
+                                  dP.data_ptr<float>(), st.grad_grid.data_ptr<float>(), st.grad_mlp.data_ptr<float>(),
+                                  ws.data_ptr(), ws.numel(), stream),
+                 "atdf_project_bwd");
+    st.step += 1;
+    const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
+    if (st.grid_master.defined()) {
+        check_status(atdf_adam_step_bf16(st.grid_master.data_ptr<float>(),
+                                         reinterpret_cast<uint16_t*>(st.grid.data_ptr()), st.grad_grid.data_ptr<float>(),
+                                         st.m_grid.data_ptr<float>(), st.v_grid.data_ptr<float>(), st.grid.numel(),
+                                         st.lr_grid, b1, b2, eps, st.step, 1, stream),
+                     "atdf_adam_step_bf16");
+    } else {
+        check_status(atdf_adam_step(st.grid.data_ptr<float>(), st.grad_grid.data_ptr<float>(),
+                                    st.m_grid.data_ptr<float>(), st.v_grid.data_ptr<float>(), st.grid.numel(),
+                                    st.lr_grid, b1, b2, eps, st.step, 1, stream),
+                     "atdf_adam_step");
+    }
+    check_status(atdf_adam_step(st.mlp.data_ptr<float>(), st.grad_mlp.data_ptr<float>(), st.m_mlp.data_ptr<float>(),
+                                st.v_mlp.data_ptr<float>(), st.mlp.numel(), st.lr_mlp, b1, b2, eps, st.step, 1, stream),
+                 "atdf_adam_step");
+    return st.loss;
+}
+
+}  // namespace atdf_torch

@@ -1,0 +1,151 @@
+// atdf_torch.h -- libtorch host layer over the C ABI (include/atdf.h).
+//
+// Recreates the reference's C++ operator API for the reconstruction hot
+// path (yuanhaowang1213/adaptivetensordf, hyperacorn/csrc) so callers of
+// HyperTreeBaseImpl / IndirectGridSample3D / IntegrateSamplesXRay /
+// AccumulateSampleLossPerNode can switch over, and adds the fused
+// projection with its hand-written adjoint behind a
+// torch::autograd::Function.  All device work goes through libatdf.so; this
+// layer only allocates tensors, checks shapes and orders the calls on the
+// current HIP stream.
+#pragma once
+
+#include <torch/torch.h>
+
+#include <string>
+#include <tuple>
+#include <utility>
+#include <vector>
+
+#include "../../../include/atdf.h"
+
+namespace atdf_torch {
+
+atdf_stream_t current_stream();
+void check_status(atdf_status s, const char* what);
+
+// structure/HelperStructs.h:103-175
+struct RayList {
+    torch::Tensor origin;     // [R, 3] f32
+    torch::Tensor direction;  // [R, 3] f32
+    int64_t size() const { return origin.defined() ? origin.size(0) : 0; }
+};
+
+// structure/HelperStructs.h:231-297
+struct SampleList {
+    torch::Tensor global_coordinate;   // [S, 3] f32
+    torch::Tensor weight;              // [S] f32
+    torch::Tensor ray_t;               // [S] f32
+    torch::Tensor ray_index;           // [S] i64
+    torch::Tensor local_index_in_ray;  // [S] i64
+    torch::Tensor node_id;             // [S] i64
+    int max_samples_per_ray = 0;
+    int64_t size() const { return weight.defined() ? weight.size(0) : 0; }
+};
+
+// Per-ray node intervals: the compact traversal result the fused kernels
+// expand into samples on the fly.
+struct Intervals {
+    torch::Tensor ray_offset;  // [R + 1] i64
+    torch::Tensor node;        // [H] i32
+    torch::Tensor tmin;        // [H] f32
+    torch::Tensor tmax;        // [H] f32
+    torch::Tensor count;       // [H] i32
+    torch::Tensor first;       // [H] i32
+    torch::Tensor ray;         // [H] i32
+    torch::Tensor ray_num_samples;  // [R] i32
+    int64_t num_intervals = 0;
+    int64_t num_samples = 0;
+    int max_samples_per_ray = 0;
+    atdf_intervals view() const;
+};
+
+struct SamplingParams {
+    int max_samples_per_node = 32;  // Settings.h:258
+    int max_intersections = 256;    // HyperTree.cu:280
+    int jitter_mode = 0;            // 0 midpoint, 1 array, 2 hash
+    uint64_t jitter_seed = 0;
+    atdf_sampling view(const float* jitter = nullptr) const;
+};
+
+// HyperTreeBaseImpl (structure/HyperTree.h:395-563) for D = 3, octree.
+class HyperTreeBase {
+   public:
+    HyperTreeBase(int d, int max_depth);
+    void settree();                      // HyperTree.cpp:298-525
+    void SetActive(int depth);           // HyperTree.cpp:544-549
+    void UpdateActive();                 // HyperTree.cpp:550-588
+    void to(torch::Device device);
+    int64_t NumNodes() const { return node_position_min.size(0); }
+    int64_t NumActiveNodes() const { return active_node_ids.size(0); }
+    torch::Tensor ActiveNodeTensor() const { return active_node_ids; }
+    torch::Device device() const { return node_position_min.device(); }
+
+    // HyperTree.cu:516-610
+    SampleList CreateSamplesForRays(const RayList& rays, int max_samples_per_node, bool interval_jitter);
+    Intervals CreateIntervalsForRays(const RayList& rays, const SamplingParams& sp);
+    // HyperTree.cu:237-259
+    torch::Tensor ComputeLocalSamples(torch::Tensor global_samples, torch::Tensor node_indices);
+    // HyperTree.cpp:607-647
+    void SetErrorForActiveNodes(torch::Tensor error, const std::string& strategy);
+    // HyperTree.cpp:666-705
+    void UpdateCulling();
+
+    const void* packed() const { return packed_.data_ptr(); }
+
+    int d;
+    int max_depth;
+    torch::Tensor node_parent, node_children, node_position_min, node_position_max, node_scale, node_error,
+        node_max_density, node_active, node_culled, node_depth, node_active_prefix_sum, active_node_ids;
+    torch::Tensor node_to_grid;  // i32 copy of node_active_prefix_sum for the kernels
+
+   private:
+    void repack();
+    torch::Tensor packed_;
+};
+
+// IndirectGridSample3D(multi_grid, index, uv): grid [N, C, Z, Y, X]
+torch::Tensor IndirectGridSample3D(torch::Tensor multi_grid, torch::Tensor index, torch::Tensor uv);
+
+// NeuralGeometry::IntegrateSamplesXRay (geometry.cpp:977-1019)
+torch::Tensor IntegrateSamplesXRay(torch::Tensor sample_values, torch::Tensor integration_weight,
+                                   torch::Tensor ray_index, int num_channels, int64_t num_rays);
+
+// HierarchicalNeuralGeometry::AccumulateSampleLossPerNode (geometry.cpp:33-53),
+// FP64 sums.
+std::pair<torch::Tensor, torch::Tensor> AccumulateSampleLossPerNode(const SampleList& samples,
+                                                                    torch::Tensor per_ray_loss, int64_t num_nodes);
+
+// Field parameters of the fused path.
+struct FieldParams {
+    int C = 16;
+    int H = 64;
+    int G = 11;
+    float softplus_beta = 2.0f;
+    int rays_per_wave = 0;
+};
+
+// Fused projection with hand-written adjoint (torch::autograd::Function).
+// grid: [num_grids, G, G, G, C] f32 (or bf16 for inference-only forward),
+// mlp: flat decoder parameters.  Returns the projection [R].
+torch::Tensor ProjectRays(HyperTreeBase& tree, const RayList& rays, torch::Tensor grid, torch::Tensor mlp,
+                          const SamplingParams& sp, const FieldParams& fp);
+
+// One training step of the plain method entirely on the device:
+// intervals -> fused forward -> MSE -> fused adjoint -> Adam on grid and mlp.
+// bf16 grids keep an f32 master copy (grid_master) and a bf16 copy (grid).
+struct TrainState {
+    torch::Tensor grid;          // [N, G, G, G, C] f32 or bf16
+    torch::Tensor grid_master;   // f32 master when grid is bf16 (else undefined)
+    torch::Tensor mlp;           // [P] f32
+    torch::Tensor grad_grid, grad_mlp;
+    torch::Tensor m_grid, v_grid, m_mlp, v_mlp;
+    torch::Tensor loss;          // [1] f64 on device
+    int64_t step = 0;
+    float lr_grid = 1e-3f, lr_mlp = 1e-3f;
+};
+TrainState MakeTrainState(torch::Tensor grid, torch::Tensor mlp, bool bf16_grid, float lr_grid, float lr_mlp);
+torch::Tensor TrainStep(HyperTreeBase& tree, TrainState& st, const RayList& rays, torch::Tensor target,
+                        const SamplingParams& sp, const FieldParams& fp);
+
+}  // namespace atdf_torch

@@ -1,0 +1,235 @@
+/*
+ * atdf.h -- C ABI of libatdf.so, the MI355X (gfx950) reconstruction hot path.
+ *
+ * Plain pointers and sizes only.  Every pointer argument is DEVICE memory
+ * unless stated otherwise; the caller owns every buffer (the libtorch layer
+ * allocates them from torch's caching allocator).  Every launch goes on the
+ * caller's stream; no entry point synchronises the device.  Status codes
+ * replace the reference's glog CHECK aborts; atdf_last_error() returns the
+ * message of the last failing call on the calling host thread.
+ *
+ * Reference interfaces (yuanhaowang1213/adaptivetensordf, hyperacorn/csrc)
+ * replaced, per entry point, are cited below as file:line.
+ */
+#ifndef ATDF_H
+#define ATDF_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+struct ihipStream_t;
+typedef struct ihipStream_t* atdf_stream_t; /* == hipStream_t */
+
+typedef int32_t atdf_status;
+#define ATDF_OK 0
+#define ATDF_ERR_INVALID_ARGUMENT 1       /* bad shape / size / null pointer     */
+#define ATDF_ERR_CAPACITY 2               /* output larger than caller capacity  */
+#define ATDF_ERR_TOO_MANY_INTERSECTIONS 3 /* a ray hit > max_intersections nodes */
+#define ATDF_ERR_ORDER 4                  /* traversal order check failed         */
+#define ATDF_ERR_HIP 5                    /* HIP runtime error                    */
+#define ATDF_ERR_UNSUPPORTED 6            /* combination not built                */
+
+const char* atdf_last_error(void);
+const char* atdf_version(void);
+
+/* ======================================================================= */
+/* Octree                                                                   */
+/* ======================================================================= */
+/* The reference tree tensors (structure/HyperTree.h:522-560), device side.
+ * Children of a node are contiguous ids (BFS numbering of
+ * structure/HyperTree.cpp:453-497); child c takes the upper half on axis k
+ * iff bit k of c is set. */
+typedef struct {
+    int64_t num_nodes;
+    const float* node_position_min; /* [num_nodes, 3] */
+    const float* node_position_max; /* [num_nodes, 3] */
+    const int32_t* node_children;   /* [num_nodes, 8], -1 = leaf */
+    const int32_t* node_parent;     /* [num_nodes], -1 = root */
+    const int32_t* node_active;     /* [num_nodes], 0 / 1 */
+} atdf_tree;
+
+/* Packed traversal form of the tree (48 B per node), rebuilt after every
+ * change of node_active.  Replaces the DeviceHyperTree view
+ * (structure/HyperTree.cu:23-55). */
+size_t atdf_tree_packed_bytes(int64_t num_nodes);
+atdf_status atdf_tree_pack(const atdf_tree* tree, void* packed, atdf_stream_t stream);
+
+/* ======================================================================= */
+/* Ray traversal + per-node stratified sample placement                     */
+/* ======================================================================= */
+/* Replaces HyperTreeBaseImpl::CreateSamplesForRays (structure/HyperTree.h:463,
+ * structure/HyperTree.cu:516-610) and its kernel ComputeRaySamples
+ * (structure/HyperTree.cu:266-514).  Canonical output order: rays in input
+ * order, within a ray the hit nodes sorted by (tmin, node id), samples of a
+ * node in increasing t. */
+typedef struct {
+    int32_t max_samples_per_node; /* K: n = ceil((tmax-tmin)/diag * K)         */
+    int32_t max_intersections;    /* 256 in the reference (HyperTree.cu:280)    */
+    int32_t jitter_mode;          /* 0 midpoint, 1 array, 2 hash(seed,ray,idx)  */
+    int32_t reserved;
+    uint64_t jitter_seed;         /* mode 2                                     */
+    const float* jitter;          /* mode 1: U[0,1) per canonical sample slot   */
+} atdf_sampling;
+
+/* status[0] += rays over max_intersections, status[1] += order-check
+ * failures, status[2] = max(samples per ray).  status is device int32[4],
+ * zeroed by the caller. */
+atdf_status atdf_traverse_count(const void* packed_tree, int64_t num_nodes, const float* ray_origin,
+                                const float* ray_direction, int64_t num_rays, const atdf_sampling* sampling,
+                                int32_t* ray_num_intervals, int32_t* ray_num_samples, int32_t* status,
+                                atdf_stream_t stream);
+
+/* out[0] = 0, out[i+1] = out[i] + in[i] for i < n (n+1 outputs). */
+size_t atdf_scan_workspace_bytes(int64_t n);
+atdf_status atdf_exclusive_scan_i32(const int32_t* in, int64_t n, int64_t* out, void* workspace,
+                                    size_t workspace_bytes, atdf_stream_t stream);
+
+/* Per-node ray intervals, indexed by interval id (ray r owns
+ * [ray_interval_offset[r], ray_interval_offset[r+1])). */
+typedef struct {
+    int32_t* node;  /* hit node id                             */
+    float* tmin;    /* entry t                                 */
+    float* tmax;    /* exit t                                  */
+    int32_t* count; /* samples in this node                    */
+    int32_t* first; /* index of its first sample within the ray */
+    int32_t* ray;   /* owning ray                              */
+} atdf_intervals;
+
+/* The reference SampleList (structure/HelperStructs.h:231-297). */
+typedef struct {
+    float* global_coordinate;   /* [S, 3] = origin + direction * t        */
+    float* weight;              /* [S]    midpoint-rule interval weight   */
+    float* ray_t;               /* [S]                                    */
+    int64_t* ray_index;         /* [S]                                    */
+    int64_t* local_index_in_ray;/* [S]                                    */
+    int64_t* node_id;           /* [S]                                    */
+} atdf_samples;
+
+/* Second pass: writes intervals (if `intervals` is non-null) and samples
+ * (if `samples` is non-null) at the scanned offsets. */
+atdf_status atdf_traverse_write(const void* packed_tree, int64_t num_nodes, const float* ray_origin,
+                                const float* ray_direction, int64_t num_rays, const atdf_sampling* sampling,
+                                const int64_t* ray_interval_offset, const int64_t* ray_sample_offset,
+                                const atdf_intervals* intervals, const atdf_samples* samples,
+                                atdf_stream_t stream);
+
+/* ======================================================================= */
+/* Per-sample ops (the reference's operator API)                            */
+/* ======================================================================= */
+/* HyperTreeBaseImpl::ComputeLocalSamples (structure/HyperTree.cu:237-259):
+ * local = ((g - min[node]) / (max[node] - min[node])) * 2 - 1 */
+atdf_status atdf_local_samples(const float* node_position_min, const float* node_position_max,
+                               const float* global_coordinate, const int64_t* node_id, int64_t num_samples,
+                               float* local, atdf_stream_t stream);
+
+/* IndirectGridSample3D(multi_grid, index, uv) (called at
+ * geometry/geometry_tensor_qo.cpp:347; 2-D form modules/IndirectGridSample2D.cu:22-162):
+ * out[s, c] = trilinear(grid[index[s], c], uv[s]) with align_corners=True
+ * mapping and border clamping.  grid is [N, C, Z, Y, X]. */
+atdf_status atdf_grid_sample3d_fwd(const float* grid, int64_t N, int32_t C, int32_t Z, int32_t Y, int32_t X,
+                                   const int64_t* index, const float* uv, int64_t num_samples, float* out,
+                                   atdf_stream_t stream);
+/* Backward (modules/IndirectGridSample2D.cu:175-393 pattern): grad_grid +=
+ * scatter of grad_out; grad_uv (nullable) += analytic d/duv. */
+atdf_status atdf_grid_sample3d_bwd(const float* grid, int64_t N, int32_t C, int32_t Z, int32_t Y, int32_t X,
+                                   const int64_t* index, const float* uv, int64_t num_samples,
+                                   const float* grad_out, float* grad_grid, float* grad_uv,
+                                   atdf_stream_t stream);
+
+/* NeuralGeometry::IntegrateSamplesXRay (geometry/geometry.h:269-270,
+ * geometry/geometry.cpp:977-1019): out[c, ray[s]] += values[s, c] * weight[s]. */
+atdf_status atdf_integrate_xray(const float* values, const float* weight, const int64_t* ray_index,
+                                int64_t num_samples, int32_t C, int64_t num_rays, float* out,
+                                atdf_stream_t stream);
+
+/* HierarchicalNeuralGeometry::AccumulateSampleLossPerNode
+ * (geometry/geometry.h:326-327, geometry/geometry.cpp:33-53), reduced in FP64:
+ * loss_sum[node[s]] += mean_c(per_ray_loss[c, ray[s]]) * weight[s];
+ * weight_sum[node[s]] += weight[s]. */
+atdf_status atdf_accumulate_loss_per_node(const float* per_ray_loss, int32_t C, int64_t num_rays,
+                                          const int64_t* ray_index, const float* weight, const int64_t* node_id,
+                                          int64_t num_samples, int64_t num_nodes, double* loss_sum,
+                                          double* weight_sum, atdf_stream_t stream);
+
+/* ======================================================================= */
+/* Fused projection (forward) and its hand-written adjoint                  */
+/* ======================================================================= */
+/* The field: per-node feature grids, channels-last [num_grids, G, G, G, C]
+ * (the reference ExplicitFeatureGrid stores [N, C, G, G, G],
+ * geometry/ImplicitNet.h:66-137; atdf_grid_layout converts), decoded by the
+ * shared MLP C -> H -> H -> 1 with ReLU hidden activations and
+ * softplus(beta) output (geometry/Modules.h:23-145, geometry/geometry.cpp:176-253).
+ * mlp = [W1 (H x C), b1 (H), W2 (H x H), b2 (H), W3 (1 x H), b3 (1)]. */
+#define ATDF_GRID_F32 0
+#define ATDF_GRID_BF16 1
+typedef struct {
+    int32_t C;                   /* channels; 16 and 8 are built              */
+    int32_t H;                   /* hidden width; 64 is built                 */
+    int32_t G;                   /* grid edge, e.g. 11                        */
+    int32_t grid_dtype;          /* ATDF_GRID_F32 / ATDF_GRID_BF16            */
+    float softplus_beta;         /* 2 in the reference (Settings.h:164)       */
+    int32_t rays_per_wave;       /* work unit of one wavefront (0 = default)  */
+    const void* grid;            /* [num_grids, G, G, G, C]                   */
+    int64_t num_grids;
+    const int32_t* node_to_grid; /* [num_nodes] -> grid slot (-1 = none)      */
+    const float* mlp;            /* packed decoder parameters (f32)          */
+} atdf_field;
+
+/* Forward: projection[r] = sum over the ray's samples of
+ * softplus(MLP(trilinear(grid))) * weight -- the composition
+ * ComputeImage -> SampleVolumeIndirect -> DecodeFeatures ->
+ * IntegrateSamplesXRay (geometry/geometry.cpp:828-914) for one channel,
+ * with samples recomputed from the intervals and never stored. */
+atdf_status atdf_project_fwd(const void* packed_tree, int64_t num_nodes, const float* ray_origin,
+                             const float* ray_direction, int64_t num_rays, const atdf_sampling* sampling,
+                             const int64_t* ray_interval_offset, const atdf_intervals* intervals,
+                             const atdf_field* field, float* projection, atdf_stream_t stream);
+
+/* Adjoint of atdf_project_fwd with per-ray recomputation: given dL/dP per
+ * ray, grad_grid (f32, channels-last, same shape as the grid) += dL/dgrid and
+ * grad_mlp += dL/dmlp.  Replaces the autograd replay of
+ * IndirectGridSample*::backward (modules/IndirectGridSample2D.cu:441-477),
+ * the FCBlock backward and the index_add_ transpose. */
+size_t atdf_project_bwd_workspace_bytes(int64_t num_rays, const atdf_field* field);
+atdf_status atdf_project_bwd(const void* packed_tree, int64_t num_nodes, const float* ray_origin,
+                             const float* ray_direction, int64_t num_rays, const atdf_sampling* sampling,
+                             const int64_t* ray_interval_offset, const atdf_intervals* intervals,
+                             const atdf_field* field, const float* dLdP, float* grad_grid, float* grad_mlp,
+                             void* workspace, size_t workspace_bytes, atdf_stream_t stream);
+
+/* ======================================================================= */
+/* Loss, optimizer, layout                                                  */
+/* ======================================================================= */
+/* MSE of TrainStep (samples/hyper_train.cpp:1015-1062, loss_weighted < 0):
+ * loss = scale * mean((P - T)^2) in FP64; dLdP = scale * 2 (P - T) / R.
+ * loss (device double[1]) is overwritten; dLdP is nullable. */
+size_t atdf_mse_workspace_bytes(int64_t num_rays);
+atdf_status atdf_mse_loss(const float* projection, const float* target, int64_t num_rays, float scale,
+                          float* dLdP, double* loss, void* workspace, size_t workspace_bytes,
+                          atdf_stream_t stream);
+
+/* torch::optim::Adam step (used by OptimizerStep, geometry/geometry.h:203-225),
+ * one fused pass; zero_grad != 0 also clears grad (optimizer zero_grad). */
+atdf_status atdf_adam_step(float* param, float* grad, float* exp_avg, float* exp_avg_sq, int64_t n,
+                           float lr, float beta1, float beta2, float eps, int64_t step, int32_t zero_grad,
+                           atdf_stream_t stream);
+/* Same, for bf16 parameters with an f32 master copy: master is updated and
+ * rounded to param_bf16 (round to nearest even). */
+atdf_status atdf_adam_step_bf16(float* master, uint16_t* param_bf16, float* grad, float* exp_avg,
+                                float* exp_avg_sq, int64_t n, float lr, float beta1, float beta2, float eps,
+                                int64_t step, int32_t zero_grad, atdf_stream_t stream);
+
+/* [N, C, G, G, G] <-> [N, G, G, G, C] (direction 0: to channels-last). */
+atdf_status atdf_grid_layout(const float* src, float* dst, int64_t N, int32_t C, int32_t G, int32_t direction,
+                             atdf_stream_t stream);
+/* f32 -> bf16 (round to nearest even). */
+atdf_status atdf_f32_to_bf16(const float* src, uint16_t* dst, int64_t n, atdf_stream_t stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,139 @@
+/*
+ * atdf_oracle.h -- CPU oracle for the reconstruction hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker: tests/, the
+ * smoke() entry and the CPU-baseline leg of bench.py load it; the product
+ * path (libatdf.so + the libtorch layer) never links, loads or calls it.
+ *
+ * It restates, in plain C, the arithmetic of the reference path
+ * (yuanhaowang1213/adaptivetensordf, hyperacorn/csrc):
+ *   tree build            structure/HyperTree.cpp:298-525
+ *   active ids            structure/HyperTree.cpp:550-588
+ *   ray/box slab test     structure/HyperTree.h:338-380
+ *   ray sample placement  structure/HyperTree.cu:266-514, 516-610
+ *   local coordinates     structure/HyperTree.cu:237-259
+ *   3-D indirect gather   modules/IndirectGridSample2D.cu:22-162 (2-D
+ *                         pattern, extended to 3-D with torch grid_sample's
+ *                         corner order; the 3-D source is not shipped)
+ *   decoder MLP           geometry/Modules.h:23-145, geometry.cpp:176-253
+ *   line integral         geometry/geometry.cpp:977-1019
+ *   per-node loss         geometry/geometry.cpp:33-53
+ * Every float expression is evaluated in the order written with no FMA
+ * contraction (the Makefile passes -ffp-contract=off); those orders are
+ * the canonical ones the HIP kernels reproduce bit for bit where the
+ * kernels promise bit-exactness (traversal, sample placement, local
+ * coordinates, trilinear gather).
+ */
+#ifndef ATDF_ORACLE_H
+#define ATDF_ORACLE_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- tree ------------------------------------------------------------- */
+int64_t atdfo_tree_num_nodes(int max_depth);
+/* BFS octree over [-1,1]^3: node 0 = root, children of node i contiguous,
+ * child c takes the upper half on axis k iff bit k of c is set. */
+void atdfo_tree_build(int max_depth, float* node_min, float* node_max, int32_t* children,
+                      int32_t* parent, int32_t* depth);
+
+/* ---- slab test -------------------------------------------------------- */
+int atdfo_intersect_box_ray(const float* bmin, const float* bmax, const float* o, const float* d,
+                            float* tmin, float* tmax);
+
+/* ---- jitter hash (shared definition with the HIP kernels) ------------- */
+float atdfo_jitter(uint64_t seed, int64_t ray, int64_t local_index);
+
+/* ---- traversal + per-node sample placement ----------------------------
+ * Brute force over active_ids (as the reference kernel does), hits sorted
+ * by (tmin, node id).  jitter_mode: 0 = midpoint (a = 0.5), 1 = explicit
+ * per-slot array `jitter` indexed by the canonical global sample slot,
+ * 2 = hash(seed, ray, local index).
+ * Pass 1 (samples == NULL): fills ray_num_samples / ray_num_intervals.
+ * Pass 2: fills the sample arrays at ray_sample_offset[r] (exclusive scan).
+ * Returns total samples, or -1 if a ray has more than max_intersections hits. */
+int64_t atdfo_traverse(const float* node_min, const float* node_max, const int64_t* active_ids,
+                       int64_t num_active, const float* ray_o, const float* ray_d, int64_t num_rays,
+                       int K, int max_intersections, int jitter_mode, const float* jitter, uint64_t seed,
+                       int32_t* ray_num_samples, int32_t* ray_num_intervals, const int64_t* ray_sample_offset,
+                       float* global_coordinate, float* weight, float* ray_t, int64_t* ray_index,
+                       int64_t* local_index_in_ray, int64_t* node_id,
+                       /* optional interval outputs, indexed by ray_interval_offset */
+                       const int64_t* ray_interval_offset, int32_t* iv_node, float* iv_tmin, float* iv_tmax,
+                       int32_t* iv_count, int32_t* iv_first);
+
+/* local = ((g - min) / (max - min)) * 2 - 1 */
+void atdfo_local_samples(const float* node_min, const float* node_max, const float* g,
+                         const int64_t* node, int64_t S, float* local);
+
+/* ---- indirect 3-D trilinear gather, grid [N,C,Z,Y,X] ------------------ */
+void atdfo_grid_sample3d_fwd(const float* grid, int64_t N, int C, int Z, int Y, int X,
+                             const int64_t* index, const float* uv, int64_t S, float* out);
+/* grad_grid accumulated in double, samples in index order */
+void atdfo_grid_sample3d_bwd(const float* grid, int64_t N, int C, int Z, int Y, int X,
+                             const int64_t* index, const float* uv, int64_t S, const float* grad_out,
+                             double* grad_grid, double* grad_uv);
+
+/* ---- decoder: C -> H -> H -> 1, hidden activation relu, softplus(beta)
+ * params = [W1 (H x C), b1 (H), W2 (H x H), b2 (H), W3 (1 x H), b3 (1)]
+ * (torch Linear weight layout [out, in]).  Sums run over the input index
+ * in increasing order, starting from the bias-free product, bias added last. */
+int64_t atdfo_mlp_num_params(int C, int H);
+void atdfo_mlp_fwd(const float* params, int C, int H, float beta, const float* x, int64_t S,
+                   float* y_pre, float* density);
+
+/* ---- line integral: P[c, ray[s]] += v[s,c] * w[s], samples in order --- */
+void atdfo_integrate(const float* v, const float* w, const int64_t* ray, int64_t S, int C,
+                     int64_t R, float* P);
+
+/* ---- per-node loss / weight sums in FP64 ------------------------------ */
+void atdfo_accumulate_loss_per_node(const float* per_ray_loss, int C, int64_t R,
+                                    const int64_t* ray_index, const float* weight,
+                                    const int64_t* node_id, int64_t S, int64_t num_nodes,
+                                    double* loss_sum, double* weight_sum);
+
+/* ---- fused per-ray projection (forward) and its adjoint ----------------
+ * The whole hot path for one batch of rays, per ray: traversal, sample
+ * placement, local coords, gather from channels-last grids
+ * grid[g, z, y, x, c] with g = node_to_grid[node], decoder, line integral.
+ * precision 0: FP32 restatement (reference arithmetic, sequential sums).
+ * precision 1: same inputs, every operation after sample placement in FP64.
+ * Returns -1 on too many intersections. */
+typedef struct {
+    const float* node_min;
+    const float* node_max;
+    const int64_t* active_ids;
+    int64_t num_active;
+    const int32_t* node_to_grid;
+    const float* grid;      /* [num_grids, G, G, G, C] channels-last */
+    int G;
+    int C;
+    int H;
+    const float* mlp;       /* atdfo_mlp layout */
+    float beta;
+    int K;
+    int max_intersections;
+    int jitter_mode;        /* 0 or 2 */
+    uint64_t seed;
+} atdfo_model;
+
+int atdfo_project_fwd(const atdfo_model* m, const float* ray_o, const float* ray_d, int64_t R,
+                      int precision, double* P, int num_threads);
+/* dLdP given per ray; gradients accumulated (+=) in FP64. */
+int atdfo_project_bwd(const atdfo_model* m, const float* ray_o, const float* ray_d, int64_t R,
+                      const double* dLdP, double* grad_grid, double* grad_mlp);
+
+/* FP32 training-step baseline (reference structure on the CPU, OpenMP over
+ * rays): fwd, MSE grad, bwd with FP32 accumulation, Adam on grid + mlp.
+ * Used only as the timed CPU baseline.  Returns the MSE loss. */
+double atdfo_train_step_f32(atdfo_model* m, float* grid_mut, float* mlp_mut, int64_t grid_numel,
+                            const float* ray_o, const float* ray_d, const float* target, int64_t R,
+                            float* grad_grid, float* grad_mlp, float* adam_m_grid, float* adam_v_grid,
+                            float* adam_m_mlp, float* adam_v_mlp, int step, float lr, int num_threads);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,257 @@
+"""numpy front end of the CPU oracle (liboracle.so, built from atdf_oracle.c).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, by the smoke() check and by
+bench.py's cpu_baseline leg.  The product path never imports this module.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "liboracle.so")
+
+_lib = None
+
+_P = C.c_void_p
+
+
+def build():
+    subprocess.run(["make", "-C", HERE, "-B"], check=True, stdout=subprocess.DEVNULL)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB):
+            build()
+        _lib = C.CDLL(LIB)
+        _lib.atdfo_tree_num_nodes.restype = C.c_int64
+        _lib.atdfo_tree_num_nodes.argtypes = [C.c_int]
+        _lib.atdfo_intersect_box_ray.restype = C.c_int
+        _lib.atdfo_jitter.restype = C.c_float
+        _lib.atdfo_jitter.argtypes = [C.c_uint64, C.c_int64, C.c_int64]
+        _lib.atdfo_traverse.restype = C.c_int64
+        _lib.atdfo_mlp_num_params.restype = C.c_int64
+        _lib.atdfo_project_fwd.restype = C.c_int
+        _lib.atdfo_project_bwd.restype = C.c_int
+        _lib.atdfo_train_step_f32.restype = C.c_double
+    return _lib
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(_P)
+
+
+def _c(a, dtype):
+    return np.ascontiguousarray(a, dtype=dtype)
+
+
+class Model(C.Structure):
+    _fields_ = [("node_min", _P), ("node_max", _P), ("active_ids", _P), ("num_active", C.c_int64),
+                ("node_to_grid", _P), ("grid", _P), ("G", C.c_int), ("C", C.c_int), ("H", C.c_int),
+                ("mlp", _P), ("beta", C.c_float), ("K", C.c_int), ("max_intersections", C.c_int),
+                ("jitter_mode", C.c_int), ("seed", C.c_uint64)]
+
+
+# ------------------------------------------------------------------ tree
+def tree_build(max_depth: int):
+    n = lib().atdfo_tree_num_nodes(max_depth)
+    nmin = np.zeros((n, 3), np.float32)
+    nmax = np.zeros((n, 3), np.float32)
+    ch = np.zeros((n, 8), np.int32)
+    par = np.zeros(n, np.int32)
+    dep = np.zeros(n, np.int32)
+    lib().atdfo_tree_build(C.c_int(max_depth), _p(nmin), _p(nmax), _p(ch), _p(par), _p(dep))
+    return dict(node_min=nmin, node_max=nmax, children=ch, parent=par, depth=dep)
+
+
+def active_ids(active: np.ndarray) -> np.ndarray:
+    return np.nonzero(active)[0].astype(np.int64)
+
+
+def prefix_sum(active: np.ndarray) -> np.ndarray:
+    out = np.full(active.shape[0], -1, np.int32)
+    ids = np.nonzero(active)[0]
+    out[ids] = np.arange(ids.shape[0], dtype=np.int32)
+    return out
+
+
+def intersect(bmin, bmax, o, d):
+    tmin = C.c_float()
+    tmax = C.c_float()
+    arrs = [_c(x, np.float32) for x in (bmin, bmax, o, d)]
+    hit = lib().atdfo_intersect_box_ray(*[_p(a) for a in arrs], C.byref(tmin), C.byref(tmax))
+    return bool(hit), tmin.value, tmax.value
+
+
+def jitter(seed: int, ray: int, local: int) -> float:
+    return lib().atdfo_jitter(C.c_uint64(seed), C.c_int64(ray), C.c_int64(local))
+
+
+# ------------------------------------------------------------- traversal
+def traverse(node_min, node_max, act_ids, o, d, K, max_intersections=256, jitter_mode=0, jitter_arr=None, seed=0):
+    """Canonical samples and intervals of a ray batch.  Returns a dict, or
+    raises ValueError when a ray exceeds max_intersections."""
+    L = lib()
+    node_min, node_max = _c(node_min, np.float32), _c(node_max, np.float32)
+    act_ids = _c(act_ids, np.int64)
+    o, d = _c(o, np.float32), _c(d, np.float32)
+    R = o.shape[0]
+    ns = np.zeros(R, np.int32)
+    nh = np.zeros(R, np.int32)
+    common = [_p(node_min), _p(node_max), _p(act_ids), C.c_int64(act_ids.shape[0]), _p(o), _p(d), C.c_int64(R),
+              C.c_int(K), C.c_int(max_intersections), C.c_int(jitter_mode),
+              _p(None if jitter_arr is None else _c(jitter_arr, np.float32)), C.c_uint64(seed)]
+    jarr = None if jitter_arr is None else _c(jitter_arr, np.float32)
+    common[10] = _p(jarr)
+    tot = L.atdfo_traverse(*common, _p(ns), _p(nh), *([None] * 7), *([None] * 6))
+    if tot < 0:
+        raise ValueError("too many intersections")
+    soff = np.zeros(R + 1, np.int64)
+    soff[1:] = np.cumsum(ns)
+    ioff = np.zeros(R + 1, np.int64)
+    ioff[1:] = np.cumsum(nh)
+    S, H = int(soff[-1]), int(ioff[-1])
+    out = dict(ray_num_samples=ns, ray_num_intervals=nh, ray_sample_offset=soff, ray_interval_offset=ioff,
+               global_coordinate=np.zeros((S, 3), np.float32), weight=np.zeros(S, np.float32),
+               ray_t=np.zeros(S, np.float32), ray_index=np.zeros(S, np.int64),
+               local_index_in_ray=np.zeros(S, np.int64), node_id=np.zeros(S, np.int64),
+               iv_node=np.zeros(H, np.int32), iv_tmin=np.zeros(H, np.float32), iv_tmax=np.zeros(H, np.float32),
+               iv_count=np.zeros(H, np.int32), iv_first=np.zeros(H, np.int32))
+    L.atdfo_traverse(*common, None, None, _p(soff), _p(out["global_coordinate"]), _p(out["weight"]),
+                     _p(out["ray_t"]), _p(out["ray_index"]), _p(out["local_index_in_ray"]), _p(out["node_id"]),
+                     _p(ioff), _p(out["iv_node"]), _p(out["iv_tmin"]), _p(out["iv_tmax"]), _p(out["iv_count"]),
+                     _p(out["iv_first"]))
+    return out
+
+
+def local_samples(node_min, node_max, g, node):
+    g = _c(g, np.float32)
+    node = _c(node, np.int64)
+    out = np.zeros_like(g)
+    lib().atdfo_local_samples(_p(_c(node_min, np.float32)), _p(_c(node_max, np.float32)), _p(g), _p(node),
+                              C.c_int64(node.shape[0]), _p(out))
+    return out
+
+
+# ---------------------------------------------------------------- gather
+def grid_sample3d_fwd(grid, index, uv):
+    grid = _c(grid, np.float32)
+    N, Cc, Z, Y, X = grid.shape
+    index = _c(index, np.int64)
+    uv = _c(uv, np.float32)
+    out = np.zeros((index.shape[0], Cc), np.float32)
+    lib().atdfo_grid_sample3d_fwd(_p(grid), C.c_int64(N), C.c_int(Cc), C.c_int(Z), C.c_int(Y), C.c_int(X),
+                                  _p(index), _p(uv), C.c_int64(index.shape[0]), _p(out))
+    return out
+
+
+def grid_sample3d_bwd(grid, index, uv, grad_out):
+    grid = _c(grid, np.float32)
+    N, Cc, Z, Y, X = grid.shape
+    index = _c(index, np.int64)
+    uv = _c(uv, np.float32)
+    go = _c(grad_out, np.float32)
+    gg = np.zeros(grid.shape, np.float64)
+    gu = np.zeros(uv.shape, np.float64)
+    lib().atdfo_grid_sample3d_bwd(_p(grid), C.c_int64(N), C.c_int(Cc), C.c_int(Z), C.c_int(Y), C.c_int(X),
+                                  _p(index), _p(uv), C.c_int64(index.shape[0]), _p(go), _p(gg), _p(gu))
+    return gg, gu
+
+
+# ------------------------------------------------------------------- mlp
+def mlp_fwd(params, Cc, H, beta, x):
+    params = _c(params, np.float32)
+    x = _c(x, np.float32)
+    S = x.shape[0]
+    y = np.zeros(S, np.float32)
+    dens = np.zeros(S, np.float32)
+    lib().atdfo_mlp_fwd(_p(params), C.c_int(Cc), C.c_int(H), C.c_float(beta), _p(x), C.c_int64(S), _p(y), _p(dens))
+    return y, dens
+
+
+def integrate(v, w, ray, R):
+    v = _c(v, np.float32)
+    Cc = v.shape[1]
+    P = np.zeros((Cc, R), np.float32)
+    lib().atdfo_integrate(_p(v), _p(_c(w, np.float32)), _p(_c(ray, np.int64)), C.c_int64(v.shape[0]),
+                          C.c_int(Cc), C.c_int64(R), _p(P))
+    return P
+
+
+def loss_per_node(per_ray_loss, ray_index, weight, node_id, num_nodes):
+    prl = _c(per_ray_loss, np.float32)
+    ls = np.zeros(num_nodes, np.float64)
+    wsum = np.zeros(num_nodes, np.float64)
+    lib().atdfo_accumulate_loss_per_node(_p(prl), C.c_int(prl.shape[0]), C.c_int64(prl.shape[1]),
+                                         _p(_c(ray_index, np.int64)), _p(_c(weight, np.float32)),
+                                         _p(_c(node_id, np.int64)), C.c_int64(len(ray_index)),
+                                         C.c_int64(num_nodes), _p(ls), _p(wsum))
+    return ls, wsum
+
+
+# ----------------------------------------------------------- fused path
+class FusedModel:
+    """Holds the arrays the fused oracle reads (keeps them alive)."""
+
+    def __init__(self, node_min, node_max, active, grid_cl, mlp, C_, H=64, beta=2.0, K=8, max_intersections=256,
+                 jitter_mode=0, seed=0):
+        self.node_min = _c(node_min, np.float32)
+        self.node_max = _c(node_max, np.float32)
+        self.active_ids = active_ids(active)
+        self.node_to_grid = prefix_sum(active)
+        self.grid = _c(grid_cl, np.float32)
+        self.mlp = _c(mlp, np.float32)
+        self.G = self.grid.shape[1]
+        self.m = Model(_p(self.node_min), _p(self.node_max), _p(self.active_ids), self.active_ids.shape[0],
+                       _p(self.node_to_grid), _p(self.grid), self.G, C_, H, _p(self.mlp), beta, K,
+                       max_intersections, jitter_mode, seed)
+
+    def project(self, o, d, precision=0, threads=0):
+        o, d = _c(o, np.float32), _c(d, np.float32)
+        P = np.zeros(o.shape[0], np.float64)
+        st = lib().atdfo_project_fwd(C.byref(self.m), _p(o), _p(d), C.c_int64(o.shape[0]), C.c_int(precision),
+                                     _p(P), C.c_int(threads))
+        if st != 0:
+            raise ValueError("too many intersections")
+        return P
+
+    def project_bwd(self, o, d, dLdP):
+        o, d = _c(o, np.float32), _c(d, np.float32)
+        dLdP = _c(dLdP, np.float64)
+        gg = np.zeros(self.grid.shape, np.float64)
+        gm = np.zeros(self.mlp.shape, np.float64)
+        st = lib().atdfo_project_bwd(C.byref(self.m), _p(o), _p(d), C.c_int64(o.shape[0]), _p(dLdP), _p(gg), _p(gm))
+        if st != 0:
+            raise ValueError("too many intersections")
+        return gg, gm
+
+
+class TrainBaseline:
+    """FP32 CPU training step (the timed CPU baseline)."""
+
+    def __init__(self, fused: FusedModel, lr=1e-3):
+        self.f = fused
+        self.grid = fused.grid.copy()
+        self.mlp = fused.mlp.copy()
+        self.gg = np.zeros_like(self.grid)
+        self.gm = np.zeros_like(self.mlp)
+        self.mg = np.zeros_like(self.grid)
+        self.vg = np.zeros_like(self.grid)
+        self.mm = np.zeros_like(self.mlp)
+        self.vm = np.zeros_like(self.mlp)
+        self.step = 0
+        self.lr = lr
+
+    def train_step(self, o, d, target, threads=0):
+        o, d = _c(o, np.float32), _c(d, np.float32)
+        t = _c(target, np.float32)
+        self.step += 1
+        return lib().atdfo_train_step_f32(C.byref(self.f.m), _p(self.grid), _p(self.mlp),
+                                          C.c_int64(self.grid.size), _p(o), _p(d), _p(t), C.c_int64(o.shape[0]),
+                                          _p(self.gg), _p(self.gm), _p(self.mg), _p(self.vg), _p(self.mm),
+                                          _p(self.vm), C.c_int(self.step), C.c_float(self.lr), C.c_int(threads))
