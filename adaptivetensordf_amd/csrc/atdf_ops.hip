@@ -150,9 +150,9 @@ __global__ __launch_bounds__(kRedThreads) void mse_partial_kernel(const float* _
     __shared__ double smem[kRedThreads / 64];
     double acc = 0;
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < R; i += (int64_t)gridDim.x * blockDim.x) {
-        float diff = P[i] - T[i];
-        acc += (double)diff * (double)diff;
-        if (dLdP) dLdP[i] = scale * (2.0f * diff / (float)R);
+        double diff = (double)P[i] - (double)T[i];
+        acc += diff * diff;
+        if (dLdP) dLdP[i] = scale * (2.0f * (P[i] - T[i]) / (float)R);
     }
     double t = block_sum_d(acc, smem);
     if (threadIdx.x == 0) partial[blockIdx.x] = t;

@@ -784,7 +784,8 @@ static void adam_f32(float* p, float* g, float* mm, float* vv, int64_t n, int st
 double atdfo_train_step_f32(atdfo_model* m, float* grid_mut, float* mlp_mut, int64_t grid_numel,
                             const float* ray_o, const float* ray_d, const float* target, int64_t R,
                             float* grad_grid, float* grad_mlp, float* adam_m_grid, float* adam_v_grid,
-                            float* adam_m_mlp, float* adam_v_mlp, int step, float lr, int num_threads)
+                            float* adam_m_mlp, float* adam_v_mlp, int step, float lr, int num_threads,
+                            int do_adam)
 {
     int C = m->C, H = m->H, G = m->G;
     int64_t nmlp = atdfo_mlp_num_params(C, H);
@@ -887,7 +888,9 @@ double atdfo_train_step_f32(atdfo_model* m, float* grid_mut, float* mlp_mut, int
         free(dx);
         free(gm);
     }
-    adam_f32(grid_mut, grad_grid, adam_m_grid, adam_v_grid, grid_numel, step, lr);
-    adam_f32(mlp_mut, grad_mlp, adam_m_mlp, adam_v_mlp, nmlp, step, lr);
+    if (do_adam) {
+        adam_f32(grid_mut, grad_grid, adam_m_grid, adam_v_grid, grid_numel, step, lr);
+        adam_f32(mlp_mut, grad_mlp, adam_m_mlp, adam_v_mlp, nmlp, step, lr);
+    }
     return loss / (double)R;
 }

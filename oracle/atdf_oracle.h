@@ -126,12 +126,14 @@ int atdfo_project_bwd(const atdfo_model* m, const float* ray_o, const float* ray
                       const double* dLdP, double* grad_grid, double* grad_mlp);
 
 /* FP32 training-step baseline (reference structure on the CPU, OpenMP over
- * rays): fwd, MSE grad, bwd with FP32 accumulation, Adam on grid + mlp.
+ * rays): fwd, MSE grad, bwd with FP32 accumulation, then (do_adam) Adam on
+ * grid + mlp.
  * Used only as the timed CPU baseline.  Returns the MSE loss. */
 double atdfo_train_step_f32(atdfo_model* m, float* grid_mut, float* mlp_mut, int64_t grid_numel,
                             const float* ray_o, const float* ray_d, const float* target, int64_t R,
                             float* grad_grid, float* grad_mlp, float* adam_m_grid, float* adam_v_grid,
-                            float* adam_m_mlp, float* adam_v_mlp, int step, float lr, int num_threads);
+                            float* adam_m_mlp, float* adam_v_mlp, int step, float lr, int num_threads,
+                            int do_adam);
 
 #ifdef __cplusplus
 }

@@ -247,11 +247,12 @@ class TrainBaseline:
         self.step = 0
         self.lr = lr
 
-    def train_step(self, o, d, target, threads=0):
+    def train_step(self, o, d, target, threads=0, do_adam=True):
         o, d = _c(o, np.float32), _c(d, np.float32)
         t = _c(target, np.float32)
         self.step += 1
         return lib().atdfo_train_step_f32(C.byref(self.f.m), _p(self.grid), _p(self.mlp),
                                           C.c_int64(self.grid.size), _p(o), _p(d), _p(t), C.c_int64(o.shape[0]),
                                           _p(self.gg), _p(self.gm), _p(self.mg), _p(self.vg), _p(self.mm),
-                                          _p(self.vm), C.c_int(self.step), C.c_float(self.lr), C.c_int(threads))
+                                          _p(self.vm), C.c_int(self.step), C.c_float(self.lr), C.c_int(threads),
+                                          C.c_int(1 if do_adam else 0))
