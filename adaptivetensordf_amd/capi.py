@@ -21,6 +21,8 @@ ERR_UNSUPPORTED = 6
 
 GRID_F32 = 0
 GRID_BF16 = 1
+DECODER_F32 = 0
+DECODER_BF16 = 1
 
 
 class AtdfError(RuntimeError):
@@ -52,7 +54,8 @@ class Samples(C.Structure):
 class Field(C.Structure):
     _fields_ = [("C", C.c_int32), ("H", C.c_int32), ("G", C.c_int32), ("grid_dtype", C.c_int32),
                 ("softplus_beta", C.c_float), ("rays_per_wave", C.c_int32), ("grid", C.c_void_p),
-                ("num_grids", C.c_int64), ("node_to_grid", C.c_void_p), ("mlp", C.c_void_p)]
+                ("num_grids", C.c_int64), ("node_to_grid", C.c_void_p), ("mlp", C.c_void_p),
+                ("decoder_dtype", C.c_int32), ("reserved", C.c_int32)]
 
 
 _P, _I32, _I64, _F, _SZ, _U64 = C.c_void_p, C.c_int32, C.c_int64, C.c_float, C.c_size_t, C.c_uint64

@@ -20,6 +20,10 @@
 // w_corner * dX into the channels-last grid gradient with float atomics.
 // Per-wave parameter-gradient partials go to a workspace and are summed in a
 // fixed order by a second kernel (deterministic decoder gradients).
+//
+// The BF16 decoder variant runs the same contractions on
+// v_mfma_f32_16x16x16_bf16 / v_mfma_f32_16x16x32_bf16 with FP32 accumulation
+// (a K = 64 contraction is two 16x16x32 steps; a K = 16 one is one 16x16x16).
 #include <hip/hip_runtime.h>
 
 #include "atdf_device.h"
@@ -32,6 +36,34 @@ typedef float v4f __attribute__((ext_vector_type(4)));
 __device__ __forceinline__ v4f mfma4(float a, float b, v4f c)
 {
     return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+}
+
+typedef short s4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
+
+__device__ __forceinline__ s4 pack_bf4(float a, float b, float c, float d)
+{
+    bf4 v = {(__bf16)a, (__bf16)b, (__bf16)c, (__bf16)d};
+    return __builtin_bit_cast(s4, v);
+}
+
+__device__ __forceinline__ bf8 pack_bf8(const v4f& lo, const v4f& hi)
+{
+    bf8 v = {(__bf16)lo[0], (__bf16)lo[1], (__bf16)lo[2], (__bf16)lo[3],
+             (__bf16)hi[0], (__bf16)hi[1], (__bf16)hi[2], (__bf16)hi[3]};
+    return v;
+}
+
+// BF16 inputs, FP32 accumulation: K = 16 (4 per lane) and K = 32 (8 per lane).
+__device__ __forceinline__ v4f mfma_bf16_k16(s4 a, s4 b, v4f c)
+{
+    return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
+}
+
+__device__ __forceinline__ v4f mfma_bf16_k32(bf8 a, bf8 b, v4f c)
+{
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
 }
 
 // Compiler + wavefront ordering point for LDS data exchanged between lanes
@@ -251,13 +283,124 @@ __device__ __forceinline__ float decoder_fwd(const float* s, const float* x, int
     return part + s[L::kB3];
 }
 
-template <int C, bool kBF16>
+// ---------------------------------------------------------------------------
+// BF16 decoder.  Weights are stored in LDS as bf16, pre-permuted so each
+// lane's MFMA operand is one contiguous 8- or 16-byte read.  For a K = 64
+// contraction (hidden -> hidden) K-step s and lane group g use the rows
+//   {32 s + 4 g + r} U {32 s + 16 + 4 g + r},  r = 0..3,
+// i.e. the result-layout registers of row blocks 2s and 2s+1, so activations
+// chain between layers without moving between lanes.
+// ---------------------------------------------------------------------------
+constexpr int kBW = 72;  // padded row stride (bf16 elements) of the K = 64 tables
+constexpr int kTB = 24;  // padded row stride (bf16 elements) of the transpose buffers
+
+template <int C>
+struct BSmem {
+    // bf16 element offsets
+    static constexpr int kW1b = 0;                 // [64][16]  W1 rows, lane-group order
+    static constexpr int kW2b = kW1b + kH * 16;    // [64][72]  W2 rows, permuted K
+    static constexpr int kW2Tb = kW2b + kH * kBW;  // [64][72]  W2^T rows, permuted K
+    static constexpr int kW1Tb = kW2Tb + kH * kBW; // [16][72]  W1^T rows (channel), permuted K
+    static constexpr int kBf16Total = kW1Tb + 16 * kBW;
+    // f32 element offsets (after the bf16 region)
+    static constexpr int kB1 = 0, kB2 = kH, kW3 = 2 * kH, kB3 = 3 * kH, kF32Total = 3 * kH + 4;
+};
+
+__device__ __forceinline__ int permuted_k(int p)  // p in [0, 64): (s, g, q) -> hidden index
+{
+    int s = p >> 5, g = (p >> 3) & 3, q = p & 7;
+    return 32 * s + 16 * (q >> 2) + 4 * g + (q & 3);
+}
+
+template <int C>
+__device__ __forceinline__ void load_mlp_bf16_to_lds(const float* __restrict__ mlp, __bf16* sb, float* sf)
+{
+    using L = BSmem<C>;
+    const float* W1 = mlp;
+    const float* b1 = W1 + kH * C;
+    const float* W2 = b1 + kH;
+    const float* b2 = W2 + kH * kH;
+    const float* W3 = b2 + kH;
+    const float* b3 = W3 + kH;
+    for (int i = threadIdx.x; i < kH * 16; i += blockDim.x) {
+        int row = i >> 4, p = i & 15, g = p >> 2, q = p & 3;
+        int c = (C == 16) ? 4 * g + q : (q < 2 ? 2 * g + q : -1);
+        sb[L::kW1b + i] = (__bf16)(c >= 0 ? W1[row * C + c] : 0.f);
+    }
+    for (int i = threadIdx.x; i < kH * kH; i += blockDim.x) {
+        int row = i >> 6, p = i & 63, k = permuted_k(p);
+        sb[L::kW2b + row * kBW + p] = (__bf16)W2[row * kH + k];
+        sb[L::kW2Tb + row * kBW + p] = (__bf16)W2[k * kH + row];
+    }
+    for (int i = threadIdx.x; i < 16 * kH; i += blockDim.x) {
+        int c = i >> 6, p = i & 63, k = permuted_k(p);
+        sb[L::kW1Tb + c * kBW + p] = (__bf16)(c < C ? W1[k * C + c] : 0.f);
+    }
+    for (int i = threadIdx.x; i < kH; i += blockDim.x) {
+        sf[L::kB1 + i] = b1[i];
+        sf[L::kB2 + i] = b2[i];
+        sf[L::kW3 + i] = W3[i];
+    }
+    if (threadIdx.x == 0) sf[L::kB3] = b3[0];
+}
+
+template <int C>
+__device__ __forceinline__ float decoder_fwd_bf16(const __bf16* sb, const float* sf, const float* x, int g, int j,
+                                                  v4f* h1, v4f* h2)
+{
+    using L = BSmem<C>;
+    const s4 xb = (C == 16) ? pack_bf4(x[0], x[1], x[2], x[3]) : pack_bf4(x[0], x[1], 0.f, 0.f);
+#pragma unroll
+    for (int rb = 0; rb < 4; ++rb) {
+        v4f acc = {0.f, 0.f, 0.f, 0.f};
+        acc = mfma_bf16_k16(*reinterpret_cast<const s4*>(sb + L::kW1b + (16 * rb + j) * 16 + 4 * g), xb, acc);
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            float v = acc[r] + sf[L::kB1 + 16 * rb + 4 * g + r];
+            h1[rb][r] = v > 0.f ? v : 0.f;
+        }
+    }
+#pragma unroll
+    for (int rb2 = 0; rb2 < 4; ++rb2) {
+        v4f acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int st = 0; st < 2; ++st) {
+            bf8 av = *reinterpret_cast<const bf8*>(sb + L::kW2b + (16 * rb2 + j) * kBW + 32 * st + 8 * g);
+            acc = mfma_bf16_k32(av, pack_bf8(h1[2 * st], h1[2 * st + 1]), acc);
+        }
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            float v = acc[r] + sf[L::kB2 + 16 * rb2 + 4 * g + r];
+            h2[rb2][r] = v > 0.f ? v : 0.f;
+        }
+    }
+    float part = 0.f;
+#pragma unroll
+    for (int rb2 = 0; rb2 < 4; ++rb2)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) part += sf[L::kW3 + 16 * rb2 + 4 * g + r] * h2[rb2][r];
+    part += __shfl_xor(part, 16, 64);
+    part += __shfl_xor(part, 32, 64);
+    return part + sf[L::kB3];
+}
+
+template <int C, bool kDecBF16>
+struct DecoderSmem {
+    static constexpr int kF32 = kDecBF16 ? BSmem<C>::kF32Total : MlpSmem<C>::kTotal;
+    static constexpr int kBF16 = kDecBF16 ? BSmem<C>::kBf16Total : 8;
+};
+
+template <int C, bool kBF16, bool kDecBF16>
 __global__ __launch_bounds__(64 * kWaves) void project_fwd_kernel(ProjArgs a, float* __restrict__ P)
 {
-    using L = MlpSmem<C>;
+    using SM = DecoderSmem<C, kDecBF16>;
     constexpr int CPL = C / 4;
-    __shared__ __attribute__((aligned(16))) float s[L::kTotal];
-    load_mlp_to_lds<C>(a.mlp, s);
+    __shared__ __attribute__((aligned(16))) float s[SM::kF32];
+    __shared__ __attribute__((aligned(16))) __bf16 sb[SM::kBF16];
+    if constexpr (kDecBF16)
+        load_mlp_bf16_to_lds<C>(a.mlp, sb, s);
+    else
+        load_mlp_to_lds<C>(a.mlp, s);
     __syncthreads();
 
     const int lane = threadIdx.x & 63;
@@ -286,7 +429,11 @@ __global__ __launch_bounds__(64 * kWaves) void project_fwd_kernel(ProjArgs a, fl
             float x[CPL];
             gather_channels<C, kBF16>(a, sc, CPL * g, x);
             v4f h1[4], h2[4];
-            float y = decoder_fwd<C>(s, x, g, j, h1, h2);
+            float y;
+            if constexpr (kDecBF16)
+                y = decoder_fwd_bf16<C>(sb, s, x, g, j, h1, h2);
+            else
+                y = decoder_fwd<C>(s, x, g, j, h1, h2);
             if (sc.valid) acc += softplus_f(y, a.beta) * sc.w;
         }
         acc += __shfl_xor(acc, 1, 64);
@@ -303,17 +450,26 @@ constexpr int mlp_numel()
     return kH * C + kH + kH * kH + kH + kH + 1;
 }
 
-template <int C, bool kBF16>
+template <int C, bool kBF16, bool kDecBF16>
 __global__ __launch_bounds__(64 * kWaves) void project_bwd_kernel(ProjArgs a, const float* __restrict__ dLdP,
                                                                   float* __restrict__ grad_grid,
                                                                   float* __restrict__ ws)
 {
     using L = MlpSmem<C>;
+    using LB = BSmem<C>;
+    using SM = DecoderSmem<C, kDecBF16>;
     constexpr int CPL = C / 4;
     constexpr int NP = mlp_numel<C>();
-    __shared__ __attribute__((aligned(16))) float s[L::kTotal];
-    __shared__ __attribute__((aligned(16))) float tbuf[kWaves][2][kH * kTS];
-    load_mlp_to_lds<C>(a.mlp, s);
+    constexpr int kTF = kDecBF16 ? 4 : kH * kTS;  // f32 transpose buffer (FP32 decoder)
+    constexpr int kTBb = kDecBF16 ? kH * kTB : 8; // bf16 transpose buffer (BF16 decoder)
+    __shared__ __attribute__((aligned(16))) float s[SM::kF32];
+    __shared__ __attribute__((aligned(16))) __bf16 sb[SM::kBF16];
+    __shared__ __attribute__((aligned(16))) float tbuf[kWaves][2][kTF];
+    __shared__ __attribute__((aligned(16))) __bf16 tbufb[kWaves][2][kTBb];
+    if constexpr (kDecBF16)
+        load_mlp_bf16_to_lds<C>(a.mlp, sb, s);
+    else
+        load_mlp_to_lds<C>(a.mlp, s);
     __syncthreads();
 
     const int lane = threadIdx.x & 63;
@@ -321,6 +477,9 @@ __global__ __launch_bounds__(64 * kWaves) void project_bwd_kernel(ProjArgs a, co
     const int wl = threadIdx.x >> 6;
     float* bufA = tbuf[wl][0];
     float* bufB = tbuf[wl][1];
+    __bf16* bufAb = tbufb[wl][0];
+    __bf16* bufBb = tbufb[wl][1];
+    const float* sW3 = kDecBF16 ? s + LB::kW3 : s + L::kW3;
     const int64_t wave = (int64_t)blockIdx.x * kWaves + wl;
     const int64_t r0 = wave * a.rpw;
     const int64_t r1 = (r0 + a.rpw < a.R) ? r0 + a.rpw : a.R;
@@ -358,7 +517,11 @@ __global__ __launch_bounds__(64 * kWaves) void project_bwd_kernel(ProjArgs a, co
             float x[CPL];
             gather_channels<C, kBF16>(a, sc, CPL * g, x);
             v4f h1[4], h2[4];
-            float y = decoder_fwd<C>(s, x, g, j, h1, h2);
+            float y;
+            if constexpr (kDecBF16)
+                y = decoder_fwd_bf16<C>(sb, s, x, g, j, h1, h2);
+            else
+                y = decoder_fwd<C>(s, x, g, j, h1, h2);
             const float dy = sc.valid ? dp * sc.w * softplus_grad_f(y, a.beta) : 0.f;
             if (g == 0) gB3 += dy;
 
@@ -370,63 +533,109 @@ __global__ __launch_bounds__(64 * kWaves) void project_bwd_kernel(ProjArgs a, co
                 for (int r4 = 0; r4 < 4; ++r4) {
                     const int row = 16 * rb2 + 4 * g + r4;
                     gW3[rb2][r4] += dy * h2[rb2][r4];
-                    float v = (h2[rb2][r4] > 0.f) ? dy * s[L::kW3 + row] : 0.f;
+                    float v = (h2[rb2][r4] > 0.f) ? dy * sW3[row] : 0.f;
                     dh2[rb2][r4] = v;
                     gB2[rb2][r4] += v;
                 }
 
-            // dW2 += dH2 . H1^T  (K = the tile's 16 samples)
+            v4f dh1[4];
+            v4f dx = {0.f, 0.f, 0.f, 0.f};
+            if constexpr (!kDecBF16) {
+                // dW2 += dH2 . H1^T  (K = the tile's 16 samples)
 #pragma unroll
-            for (int rb = 0; rb < 4; ++rb)
+                for (int rb = 0; rb < 4; ++rb)
 #pragma unroll
-                for (int r4 = 0; r4 < 4; ++r4) {
-                    bufA[(16 * rb + 4 * g + r4) * kTS + j] = h1[rb][r4];
-                    bufB[(16 * rb + 4 * g + r4) * kTS + j] = dh2[rb][r4];
+                    for (int r4 = 0; r4 < 4; ++r4) {
+                        bufA[(16 * rb + 4 * g + r4) * kTS + j] = h1[rb][r4];
+                        bufB[(16 * rb + 4 * g + r4) * kTS + j] = dh2[rb][r4];
+                    }
+                wave_lds_sync();
+#pragma unroll
+                for (int rb2 = 0; rb2 < 4; ++rb2) {
+                    float4 av = *reinterpret_cast<const float4*>(bufB + (16 * rb2 + j) * kTS + 4 * g);
+#pragma unroll
+                    for (int rb = 0; rb < 4; ++rb) {
+                        float4 bv = *reinterpret_cast<const float4*>(bufA + (16 * rb + j) * kTS + 4 * g);
+                        v4f acc = accW2[rb2][rb];
+                        acc = mfma4(av.x, bv.x, acc);
+                        acc = mfma4(av.y, bv.y, acc);
+                        acc = mfma4(av.z, bv.z, acc);
+                        acc = mfma4(av.w, bv.w, acc);
+                        accW2[rb2][rb] = acc;
+                    }
                 }
-            wave_lds_sync();
-#pragma unroll
-            for (int rb2 = 0; rb2 < 4; ++rb2) {
-                float4 av = *reinterpret_cast<const float4*>(bufB + (16 * rb2 + j) * kTS + 4 * g);
+                wave_lds_sync();
+
+                // dH1 = W2^T . dH2, masked by ReLU
 #pragma unroll
                 for (int rb = 0; rb < 4; ++rb) {
-                    float4 bv = *reinterpret_cast<const float4*>(bufA + (16 * rb + j) * kTS + 4 * g);
-                    v4f acc = accW2[rb2][rb];
-                    acc = mfma4(av.x, bv.x, acc);
-                    acc = mfma4(av.y, bv.y, acc);
-                    acc = mfma4(av.z, bv.z, acc);
-                    acc = mfma4(av.w, bv.w, acc);
-                    accW2[rb2][rb] = acc;
+                    v4f acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+                    for (int rb2 = 0; rb2 < 4; ++rb2)
+#pragma unroll
+                        for (int kk = 0; kk < 4; ++kk)
+                            acc = mfma4(s[L::kW2 + (16 * rb2 + 4 * g + kk) * kW2S + 16 * rb + j], dh2[rb2][kk], acc);
+#pragma unroll
+                    for (int r4 = 0; r4 < 4; ++r4) {
+                        float v = (h1[rb][r4] > 0.f) ? acc[r4] : 0.f;
+                        dh1[rb][r4] = v;
+                        gB1[rb][r4] += v;
+                    }
+                }
+
+                // dX = W1^T . dH1 (rows = channels, padded to 16)
+#pragma unroll
+                for (int rb = 0; rb < 4; ++rb)
+#pragma unroll
+                    for (int kk = 0; kk < 4; ++kk) {
+                        float wv = (j < C) ? s[L::kW1 + (16 * rb + 4 * g + kk) * L::W1S + j] : 0.f;
+                        dx = mfma4(wv, dh1[rb][kk], dx);
+                    }
+            } else {
+                // dW2 += dH2 . H1^T on bf16 MFMA (K = the tile's 16 samples)
+#pragma unroll
+                for (int rb = 0; rb < 4; ++rb)
+#pragma unroll
+                    for (int r4 = 0; r4 < 4; ++r4) {
+                        bufAb[(16 * rb + 4 * g + r4) * kTB + j] = (__bf16)h1[rb][r4];
+                        bufBb[(16 * rb + 4 * g + r4) * kTB + j] = (__bf16)dh2[rb][r4];
+                    }
+                wave_lds_sync();
+#pragma unroll
+                for (int rb2 = 0; rb2 < 4; ++rb2) {
+                    s4 av = *reinterpret_cast<const s4*>(bufBb + (16 * rb2 + j) * kTB + 4 * g);
+#pragma unroll
+                    for (int rb = 0; rb < 4; ++rb) {
+                        s4 bv = *reinterpret_cast<const s4*>(bufAb + (16 * rb + j) * kTB + 4 * g);
+                        accW2[rb2][rb] = mfma_bf16_k16(av, bv, accW2[rb2][rb]);
+                    }
+                }
+                wave_lds_sync();
+
+                // dH1 = W2^T . dH2 (K = 64 in two bf16 steps), masked by ReLU
+#pragma unroll
+                for (int rb = 0; rb < 4; ++rb) {
+                    v4f acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+                    for (int st = 0; st < 2; ++st) {
+                        bf8 av = *reinterpret_cast<const bf8*>(sb + LB::kW2Tb + (16 * rb + j) * kBW + 32 * st + 8 * g);
+                        acc = mfma_bf16_k32(av, pack_bf8(dh2[2 * st], dh2[2 * st + 1]), acc);
+                    }
+#pragma unroll
+                    for (int r4 = 0; r4 < 4; ++r4) {
+                        float v = (h1[rb][r4] > 0.f) ? acc[r4] : 0.f;
+                        dh1[rb][r4] = v;
+                        gB1[rb][r4] += v;
+                    }
+                }
+
+                // dX = W1^T . dH1 (rows = channels, padded to 16)
+#pragma unroll
+                for (int st = 0; st < 2; ++st) {
+                    bf8 av = *reinterpret_cast<const bf8*>(sb + LB::kW1Tb + j * kBW + 32 * st + 8 * g);
+                    dx = mfma_bf16_k32(av, pack_bf8(dh1[2 * st], dh1[2 * st + 1]), dx);
                 }
             }
-            wave_lds_sync();
-
-            // dH1 = W2^T . dH2, masked by ReLU
-            v4f dh1[4];
-#pragma unroll
-            for (int rb = 0; rb < 4; ++rb) {
-                v4f acc = {0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-                for (int rb2 = 0; rb2 < 4; ++rb2)
-#pragma unroll
-                    for (int kk = 0; kk < 4; ++kk)
-                        acc = mfma4(s[L::kW2 + (16 * rb2 + 4 * g + kk) * kW2S + 16 * rb + j], dh2[rb2][kk], acc);
-#pragma unroll
-                for (int r4 = 0; r4 < 4; ++r4) {
-                    float v = (h1[rb][r4] > 0.f) ? acc[r4] : 0.f;
-                    dh1[rb][r4] = v;
-                    gB1[rb][r4] += v;
-                }
-            }
-
-            // dX = W1^T . dH1 (rows = channels, padded to 16)
-            v4f dx = {0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-            for (int rb = 0; rb < 4; ++rb)
-#pragma unroll
-                for (int kk = 0; kk < 4; ++kk) {
-                    float wv = (j < C) ? s[L::kW1 + (16 * rb + 4 * g + kk) * L::W1S + j] : 0.f;
-                    dx = mfma4(wv, dh1[rb][kk], dx);
-                }
 
             // scatter w_corner * dX into the channels-last grid gradient
             if (sc.valid && sc.gbase >= 0 && 4 * g < C) {
@@ -440,14 +649,14 @@ __global__ __launch_bounds__(64 * kWaves) void project_bwd_kernel(ProjArgs a, co
             }
 
             // dW1 += dH1 . X^T
+            if constexpr (!kDecBF16) {
 #pragma unroll
-            for (int rb = 0; rb < 4; ++rb)
+                for (int rb = 0; rb < 4; ++rb)
 #pragma unroll
-                for (int r4 = 0; r4 < 4; ++r4) bufA[(16 * rb + 4 * g + r4) * kTS + j] = dh1[rb][r4];
+                    for (int r4 = 0; r4 < 4; ++r4) bufA[(16 * rb + 4 * g + r4) * kTS + j] = dh1[rb][r4];
 #pragma unroll
-            for (int q = 0; q < CPL; ++q) bufB[(CPL * g + q) * kTS + j] = x[q];
-            wave_lds_sync();
-            {
+                for (int q = 0; q < CPL; ++q) bufB[(CPL * g + q) * kTS + j] = x[q];
+                wave_lds_sync();
                 float4 bv = make_float4(0.f, 0.f, 0.f, 0.f);
                 if (j < C) bv = *reinterpret_cast<const float4*>(bufB + j * kTS + 4 * g);
 #pragma unroll
@@ -460,8 +669,24 @@ __global__ __launch_bounds__(64 * kWaves) void project_bwd_kernel(ProjArgs a, co
                     acc = mfma4(av.w, bv.w, acc);
                     accW1[rb] = acc;
                 }
+                wave_lds_sync();
+            } else {
+#pragma unroll
+                for (int rb = 0; rb < 4; ++rb)
+#pragma unroll
+                    for (int r4 = 0; r4 < 4; ++r4) bufAb[(16 * rb + 4 * g + r4) * kTB + j] = (__bf16)dh1[rb][r4];
+#pragma unroll
+                for (int q = 0; q < CPL; ++q) bufBb[(CPL * g + q) * kTB + j] = (__bf16)x[q];
+                wave_lds_sync();
+                s4 bv = {0, 0, 0, 0};
+                if (j < C) bv = *reinterpret_cast<const s4*>(bufBb + j * kTB + 4 * g);
+#pragma unroll
+                for (int rb = 0; rb < 4; ++rb) {
+                    s4 av = *reinterpret_cast<const s4*>(bufAb + (16 * rb + j) * kTB + 4 * g);
+                    accW1[rb] = mfma_bf16_k16(av, bv, accW1[rb]);
+                }
+                wave_lds_sync();
             }
-            wave_lds_sync();
         }
     }
 
@@ -533,6 +758,8 @@ static atdf_status check_project_args(const void* packed_tree, int64_t num_nodes
     ATDF_REQUIRE(f->C == 16 || f->C == 8, "project: %d channels not built (16 and 8 are)", f->C);
     ATDF_REQUIRE(f->G >= 2, "project: grid edge must be >= 2");
     ATDF_REQUIRE(f->grid_dtype == ATDF_GRID_F32 || f->grid_dtype == ATDF_GRID_BF16, "project: unknown grid dtype");
+    ATDF_REQUIRE(f->decoder_dtype == ATDF_DECODER_F32 || f->decoder_dtype == ATDF_DECODER_BF16,
+                 "project: unknown decoder dtype");
     ATDF_REQUIRE(f->grid && f->node_to_grid && f->mlp && f->num_grids > 0, "project: incomplete field");
     if (R > 0) {
         ATDF_REQUIRE(O && D && iv_off && iv && iv->node && iv->tmin && iv->tmax && iv->count && iv->first,
@@ -581,17 +808,16 @@ atdf_status atdf_project_fwd(const void* packed_tree, int64_t num_nodes, const f
     dim3 grid((unsigned)div_up(nwaves, kWaves)), block(64 * kWaves);
     hipStream_t s = to_hip(stream);
     const bool bf = f->grid_dtype == ATDF_GRID_BF16;
+    const bool bd = f->decoder_dtype == ATDF_DECODER_BF16;
+#define ATDF_FWD(CC, BF, BD) hipLaunchKernelGGL((project_fwd_kernel<CC, BF, BD>), grid, block, 0, s, a, P)
     if (f->C == 16) {
-        if (bf)
-            hipLaunchKernelGGL((project_fwd_kernel<16, true>), grid, block, 0, s, a, P);
-        else
-            hipLaunchKernelGGL((project_fwd_kernel<16, false>), grid, block, 0, s, a, P);
+        if (bf) { if (bd) ATDF_FWD(16, true, true); else ATDF_FWD(16, true, false); }
+        else    { if (bd) ATDF_FWD(16, false, true); else ATDF_FWD(16, false, false); }
     } else {
-        if (bf)
-            hipLaunchKernelGGL((project_fwd_kernel<8, true>), grid, block, 0, s, a, P);
-        else
-            hipLaunchKernelGGL((project_fwd_kernel<8, false>), grid, block, 0, s, a, P);
+        if (bf) { if (bd) ATDF_FWD(8, true, true); else ATDF_FWD(8, true, false); }
+        else    { if (bd) ATDF_FWD(8, false, true); else ATDF_FWD(8, false, false); }
     }
+#undef ATDF_FWD
     return check_launch("atdf_project_fwd");
 }
 
@@ -621,20 +847,18 @@ atdf_status atdf_project_bwd(const void* packed_tree, int64_t num_nodes, const f
     hipStream_t s = to_hip(stream);
     float* ws = reinterpret_cast<float*>(workspace);
     const bool bf = f->grid_dtype == ATDF_GRID_BF16;
-    int np;
+    const bool bd = f->decoder_dtype == ATDF_DECODER_BF16;
+    const int np = f->C == 16 ? mlp_numel<16>() : mlp_numel<8>();
+#define ATDF_BWD(CC, BF, BD) \
+    hipLaunchKernelGGL((project_bwd_kernel<CC, BF, BD>), grid, block, 0, s, a, dLdP, grad_grid, ws)
     if (f->C == 16) {
-        np = mlp_numel<16>();
-        if (bf)
-            hipLaunchKernelGGL((project_bwd_kernel<16, true>), grid, block, 0, s, a, dLdP, grad_grid, ws);
-        else
-            hipLaunchKernelGGL((project_bwd_kernel<16, false>), grid, block, 0, s, a, dLdP, grad_grid, ws);
+        if (bf) { if (bd) ATDF_BWD(16, true, true); else ATDF_BWD(16, true, false); }
+        else    { if (bd) ATDF_BWD(16, false, true); else ATDF_BWD(16, false, false); }
     } else {
-        np = mlp_numel<8>();
-        if (bf)
-            hipLaunchKernelGGL((project_bwd_kernel<8, true>), grid, block, 0, s, a, dLdP, grad_grid, ws);
-        else
-            hipLaunchKernelGGL((project_bwd_kernel<8, false>), grid, block, 0, s, a, dLdP, grad_grid, ws);
+        if (bf) { if (bd) ATDF_BWD(8, true, true); else ATDF_BWD(8, true, false); }
+        else    { if (bd) ATDF_BWD(8, false, true); else ATDF_BWD(8, false, false); }
     }
+#undef ATDF_BWD
     hipLaunchKernelGGL(reduce_partials_kernel, dim3((unsigned)div_up(np, 256)), dim3(256), 0, s, ws,
                        nblocks * kWaves, np, grad_mlp);
     return check_launch("atdf_project_bwd");

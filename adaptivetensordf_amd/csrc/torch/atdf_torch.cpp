@@ -513,6 +513,7 @@ atdf_field make_field(const HyperTreeBase& tree, const Tensor& grid, const Tenso
     f.grid_dtype = grid.scalar_type() == torch::kBFloat16 ? ATDF_GRID_BF16 : ATDF_GRID_F32;
     f.softplus_beta = fp.softplus_beta;
     f.rays_per_wave = fp.rays_per_wave;
+    f.decoder_dtype = fp.bf16_decoder ? ATDF_DECODER_BF16 : ATDF_DECODER_F32;
     f.grid = grid.data_ptr();
     f.num_grids = grid.size(0);
     f.node_to_grid = tree.node_to_grid.data_ptr<int32_t>();
@@ -546,6 +547,7 @@ struct ProjectRaysFn : public torch::autograd::Function<ProjectRaysFn> {
         ctx->saved_data["G"] = (int64_t)fp.G;
         ctx->saved_data["beta"] = (double)fp.softplus_beta;
         ctx->saved_data["rpw"] = (int64_t)fp.rays_per_wave;
+        ctx->saved_data["bfdec"] = (bool)fp.bf16_decoder;
         ctx->saved_data["H_iv"] = iv.num_intervals;
         return P;
     }
@@ -576,6 +578,7 @@ struct ProjectRaysFn : public torch::autograd::Function<ProjectRaysFn> {
         fp.G = (int)ctx->saved_data["G"].toInt();
         fp.softplus_beta = (float)ctx->saved_data["beta"].toDouble();
         fp.rays_per_wave = (int)ctx->saved_data["rpw"].toInt();
+        fp.bf16_decoder = ctx->saved_data["bfdec"].toBool();
         atdf_field f = make_field(*tree, grid, mlp, fp);
         atdf_sampling s = sp.view();
         auto dP = grad_output[0].to(torch::kFloat32).contiguous();

@@ -123,6 +123,7 @@ struct FieldParams {
     int G = 11;
     float softplus_beta = 2.0f;
     int rays_per_wave = 0;
+    bool bf16_decoder = false;  // BF16 MFMA inputs, FP32 accumulation
 };
 
 // Fused projection with hand-written adjoint (torch::autograd::Function).

@@ -166,6 +166,10 @@ atdf_status atdf_accumulate_loss_per_node(const float* per_ray_loss, int32_t C, 
  * mlp = [W1 (H x C), b1 (H), W2 (H x H), b2 (H), W3 (1 x H), b3 (1)]. */
 #define ATDF_GRID_F32 0
 #define ATDF_GRID_BF16 1
+/* Decoder arithmetic: FP32 MFMA (v_mfma_f32_16x16x4_f32), or BF16 MFMA inputs
+ * with FP32 accumulation (v_mfma_f32_16x16x16_bf16 / 16x16x32_bf16). */
+#define ATDF_DECODER_F32 0
+#define ATDF_DECODER_BF16 1
 typedef struct {
     int32_t C;                   /* channels; 16 and 8 are built              */
     int32_t H;                   /* hidden width; 64 is built                 */
@@ -177,6 +181,8 @@ typedef struct {
     int64_t num_grids;
     const int32_t* node_to_grid; /* [num_nodes] -> grid slot (-1 = none)      */
     const float* mlp;            /* packed decoder parameters (f32)          */
+    int32_t decoder_dtype;       /* ATDF_DECODER_F32 / ATDF_DECODER_BF16      */
+    int32_t reserved;
 } atdf_field;
 
 /* Forward: projection[r] = sum over the ray's samples of

@@ -435,6 +435,22 @@ static double softplus_grad_d(double y, double beta)
     return z / (z + 1.0);
 }
 
+/* round to bfloat16 (round to nearest even), returned as double */
+static double bf16r(double v)
+{
+    float f = (float)v;
+    uint32_t u;
+    memcpy(&u, &f, 4);
+    if ((u & 0x7f800000u) != 0x7f800000u) {
+        u += 0x7fffu + ((u >> 16) & 1u);
+        u &= 0xffff0000u;
+    }
+    memcpy(&f, &u, 4);
+    return (double)f;
+}
+
+static double rb(int on, double v) { return on ? bf16r(v) : v; }
+
 static float mlp_fwd_one_f(const float* P, int C, int H, float beta, const float* x, float* h1, float* h2,
                            float* y_pre)
 {
@@ -600,7 +616,7 @@ static void gather_cl_d(const atdfo_model* m, const sample_t_* s, double* feat, 
 }
 
 static double mlp_fwd_one_d(const float* P, int C, int H, double beta, const double* x, double* h1,
-                            double* h2, double* y_pre)
+                            double* h2, double* y_pre, int bf)
 {
     const float* W1 = P;
     const float* b1 = W1 + H * C;
@@ -610,13 +626,13 @@ static double mlp_fwd_one_d(const float* P, int C, int H, double beta, const dou
     const float* b3 = W3 + H;
     for (int i = 0; i < H; ++i) {
         double acc = 0;
-        for (int k = 0; k < C; ++k) acc += (double)W1[i * C + k] * x[k];
+        for (int k = 0; k < C; ++k) acc += rb(bf, W1[i * C + k]) * rb(bf, x[k]);
         acc += b1[i];
         h1[i] = acc > 0 ? acc : 0;
     }
     for (int i = 0; i < H; ++i) {
         double acc = 0;
-        for (int k = 0; k < H; ++k) acc += (double)W2[i * H + k] * h1[k];
+        for (int k = 0; k < H; ++k) acc += rb(bf, W2[i * H + k]) * rb(bf, h1[k]);
         acc += b2[i];
         h2[i] = acc > 0 ? acc : 0;
     }
@@ -667,9 +683,15 @@ int atdfo_project_fwd(const atdfo_model* m, const float* ray_o, const float* ray
                     tri_d t;
                     int64_t base;
                     double yp;
-                    gather_cl_d(m, &smp[s], featd, &t, &base);
+                    if (m->decoder_bf16) {
+                        /* the BF16 decoder rounds the FP32 gather result */
+                        gather_cl_f(m, &smp[s], featf);
+                        for (int c = 0; c < m->C; ++c) featd[c] = featf[c];
+                    } else {
+                        gather_cl_d(m, &smp[s], featd, &t, &base);
+                    }
                     double dn = mlp_fwd_one_d(m->mlp, m->C, m->H, m->beta, featd, featd + m->C,
-                                              featd + m->C + m->H, &yp);
+                                              featd + m->C + m->H, &yp, m->decoder_bf16);
                     acc += dn * (double)smp[s].w;
                 }
                 P[r] = acc;
@@ -716,8 +738,14 @@ int atdfo_project_bwd(const atdfo_model* m, const float* ray_o, const float* ray
             tri_d t;
             int64_t base;
             double yp;
+            const int bf = m->decoder_bf16;
             gather_cl_d(m, &smp[s], x, &t, &base);
-            mlp_fwd_one_d(m->mlp, C, H, m->beta, x, h1, h2, &yp);
+            if (bf) {
+                float xf[64];
+                gather_cl_f(m, &smp[s], xf);
+                for (int c = 0; c < C; ++c) x[c] = xf[c];
+            }
+            mlp_fwd_one_d(m->mlp, C, H, m->beta, x, h1, h2, &yp, bf);
             double dy = dLdP[r] * (double)smp[s].w * softplus_grad_d(yp, m->beta);
             for (int i = 0; i < H; ++i) {
                 gW3[i] += dy * h2[i];
@@ -729,8 +757,8 @@ int atdfo_project_bwd(const atdfo_model* m, const float* ray_o, const float* ray
                 if (dh2[i] == 0) continue;
                 gb2[i] += dh2[i];
                 for (int k = 0; k < H; ++k) {
-                    gW2[i * H + k] += dh2[i] * h1[k];
-                    dh1[k] += dh2[i] * (double)W2[i * H + k];
+                    gW2[i * H + k] += rb(bf, dh2[i]) * rb(bf, h1[k]);
+                    dh1[k] += rb(bf, dh2[i]) * rb(bf, W2[i * H + k]);
                 }
             }
             for (int c = 0; c < C; ++c) dx[c] = 0;
@@ -738,8 +766,8 @@ int atdfo_project_bwd(const atdfo_model* m, const float* ray_o, const float* ray
                 if (!(h1[i] > 0)) continue;
                 gb1[i] += dh1[i];
                 for (int k = 0; k < C; ++k) {
-                    gW1[i * C + k] += dh1[i] * x[k];
-                    dx[k] += dh1[i] * (double)W1[i * C + k];
+                    gW1[i * C + k] += rb(bf, dh1[i]) * rb(bf, x[k]);
+                    dx[k] += rb(bf, dh1[i]) * rb(bf, W1[i * C + k]);
                 }
             }
             for (int k = 0; k < 8; ++k) {

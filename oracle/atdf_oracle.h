@@ -117,6 +117,8 @@ typedef struct {
     int max_intersections;
     int jitter_mode;        /* 0 or 2 */
     uint64_t seed;
+    int decoder_bf16;       /* 1: MFMA inputs rounded to bf16 (RNE) in the
+                               FP64 path, as the BF16 decoder kernels do */
 } atdfo_model;
 
 int atdfo_project_fwd(const atdfo_model* m, const float* ray_o, const float* ray_d, int64_t R,

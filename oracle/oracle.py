@@ -54,7 +54,7 @@ class Model(C.Structure):
     _fields_ = [("node_min", _P), ("node_max", _P), ("active_ids", _P), ("num_active", C.c_int64),
                 ("node_to_grid", _P), ("grid", _P), ("G", C.c_int), ("C", C.c_int), ("H", C.c_int),
                 ("mlp", _P), ("beta", C.c_float), ("K", C.c_int), ("max_intersections", C.c_int),
-                ("jitter_mode", C.c_int), ("seed", C.c_uint64)]
+                ("jitter_mode", C.c_int), ("seed", C.c_uint64), ("decoder_bf16", C.c_int)]
 
 
 # ------------------------------------------------------------------ tree
@@ -199,7 +199,7 @@ class FusedModel:
     """Holds the arrays the fused oracle reads (keeps them alive)."""
 
     def __init__(self, node_min, node_max, active, grid_cl, mlp, C_, H=64, beta=2.0, K=8, max_intersections=256,
-                 jitter_mode=0, seed=0):
+                 jitter_mode=0, seed=0, decoder_bf16=False):
         self.node_min = _c(node_min, np.float32)
         self.node_max = _c(node_max, np.float32)
         self.active_ids = active_ids(active)
@@ -209,7 +209,7 @@ class FusedModel:
         self.G = self.grid.shape[1]
         self.m = Model(_p(self.node_min), _p(self.node_max), _p(self.active_ids), self.active_ids.shape[0],
                        _p(self.node_to_grid), _p(self.grid), self.G, C_, H, _p(self.mlp), beta, K,
-                       max_intersections, jitter_mode, seed)
+                       max_intersections, jitter_mode, seed, 1 if decoder_bf16 else 0)
 
     def project(self, o, d, precision=0, threads=0):
         o, d = _c(o, np.float32), _c(d, np.float32)
