@@ -43,7 +43,7 @@ class Sampling(C.Structure):
 
 class Intervals(C.Structure):
     _fields_ = [("node", C.c_void_p), ("tmin", C.c_void_p), ("tmax", C.c_void_p), ("count", C.c_void_p),
-                ("first", C.c_void_p), ("ray", C.c_void_p)]
+                ("first", C.c_void_p), ("ray", C.c_void_p), ("total", C.c_int64)]
 
 
 class Samples(C.Structure):
@@ -78,7 +78,7 @@ _SIGNATURES = {
     "atdf_accumulate_loss_per_node": (_I32, [_P, _I32, _I64, _P, _P, _P, _I64, _I64, _P, _P, _P]),
     "atdf_project_fwd": (_I32, [_P, _I64, _P, _P, _I64, C.POINTER(Sampling), _P, C.POINTER(Intervals),
                                 C.POINTER(Field), _P, _P]),
-    "atdf_project_bwd_workspace_bytes": (_SZ, [_I64, C.POINTER(Field)]),
+    "atdf_project_bwd_workspace_bytes": (_SZ, [_I64, _I64, C.POINTER(Field)]),
     "atdf_project_bwd": (_I32, [_P, _I64, _P, _P, _I64, C.POINTER(Sampling), _P, C.POINTER(Intervals),
                                 C.POINTER(Field), _P, _P, _P, _P, _SZ, _P]),
     "atdf_mse_workspace_bytes": (_SZ, [_I64]),

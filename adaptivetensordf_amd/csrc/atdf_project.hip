@@ -141,20 +141,11 @@ struct SampleCtx {
     int64_t gbase; // element offset of the node grid, -1 if none
 };
 
-// Recompute sample `s_local` of ray r (lane-private interval cursor `cur`).
-__device__ __forceinline__ void make_sample(const ProjArgs& a, int64_t r, const float* o, const float* d, int ns,
-                                            int s_local, int64_t h1, int64_t& cur, int C, SampleCtx& sc)
+// Recompute sample jj of a node interval (ray r, sample index s_local in the ray).
+__device__ __forceinline__ void make_sample_iv(const ProjArgs& a, int64_t r, const float* o, const float* d, int node,
+                                               float tmin, float tmax, int n, int jj, int s_local, int C,
+                                               SampleCtx& sc)
 {
-    sc.valid = s_local < ns;
-    sc.w = 0.f;
-    sc.gbase = -1;
-    if (!sc.valid) return;
-    while (cur + 1 < h1 && a.iv_first[cur] + a.iv_count[cur] <= s_local) ++cur;
-    const int node = a.iv_node[cur];
-    const float tmin = a.iv_tmin[cur];
-    const float tmax = a.iv_tmax[cur];
-    const int n = a.iv_count[cur];
-    const int jj = s_local - a.iv_first[cur];
     const float step = (tmax - tmin) / (float)n;
     float aj = 0.5f, ap = 0.5f, an = 0.5f;
     if (a.jitter_mode == 2) {
@@ -176,7 +167,20 @@ __device__ __forceinline__ void make_sample(const ProjArgs& a, int64_t r, const 
     }
     tri_setup(loc[0], loc[1], loc[2], a.G, a.G, a.G, sc.tri);
     const int slot = a.node_to_grid[node];
-    if (slot >= 0) sc.gbase = (int64_t)slot * a.G * a.G * a.G * C;
+    sc.gbase = slot >= 0 ? (int64_t)slot * a.G * a.G * a.G * C : -1;
+}
+
+// Recompute sample `s_local` of ray r (lane-private interval cursor `cur`).
+__device__ __forceinline__ void make_sample(const ProjArgs& a, int64_t r, const float* o, const float* d, int ns,
+                                            int s_local, int64_t h1, int64_t& cur, int C, SampleCtx& sc)
+{
+    sc.valid = s_local < ns;
+    sc.w = 0.f;
+    sc.gbase = -1;
+    if (!sc.valid) return;
+    while (cur + 1 < h1 && a.iv_first[cur] + a.iv_count[cur] <= s_local) ++cur;
+    make_sample_iv(a, r, o, d, a.iv_node[cur], a.iv_tmin[cur], a.iv_tmax[cur], a.iv_count[cur],
+                   s_local - a.iv_first[cur], s_local, C, sc);
 }
 
 __device__ __forceinline__ int64_t corner_cell(const Tri& t, int k, int G)
@@ -450,70 +454,341 @@ constexpr int mlp_numel()
     return kH * C + kH + kH * kH + kH + kH + 1;
 }
 
-template <int C, bool kBF16, bool kDecBF16>
-__global__ __launch_bounds__(64 * kWaves) void project_bwd_kernel(ProjArgs a, const float* __restrict__ dLdP,
-                                                                  float* __restrict__ grad_grid,
-                                                                  float* __restrict__ ws)
+// Decoder-parameter gradient accumulators of one wave.
+struct DecGrads {
+    v4f W2[4][4];
+    v4f W1[4];
+    float W3[4][4], B2[4][4], B1[4][4];
+    float B3;
+};
+
+__device__ __forceinline__ void zero_grads(DecGrads& q)
+{
+#pragma unroll
+    for (int p = 0; p < 4; ++p) {
+        q.W1[p] = (v4f){0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            q.W2[p][r] = (v4f){0.f, 0.f, 0.f, 0.f};
+            q.W3[p][r] = q.B2[p][r] = q.B1[p][r] = 0.f;
+        }
+    }
+    q.B3 = 0.f;
+}
+
+// Backward of the decoder for one tile given dL/dy (dy) of this lane's
+// sample; accumulates the parameter gradients and returns dX (channels
+// 4g .. 4g+3 of sample j, result layout).  bufA/bufB (f32) or bufAb/bufBb
+// (bf16) are this wave's transpose buffers.
+template <int C, bool kDecBF16>
+__device__ __forceinline__ v4f tile_backward(const float* s, const __bf16* sb, float* bufA, float* bufB, __bf16* bufAb,
+                                             __bf16* bufBb, const float* x, const v4f* h1, const v4f* h2, float dy,
+                                             int g, int j, DecGrads& q)
 {
     using L = MlpSmem<C>;
     using LB = BSmem<C>;
-    using SM = DecoderSmem<C, kDecBF16>;
+    constexpr int CPL = C / 4;
+    const float* sW3 = kDecBF16 ? s + LB::kW3 : s + L::kW3;
+    if (g == 0) q.B3 += dy;
+    v4f dh2[4];
+#pragma unroll
+    for (int rb2 = 0; rb2 < 4; ++rb2)
+#pragma unroll
+        for (int r4 = 0; r4 < 4; ++r4) {
+            const int row = 16 * rb2 + 4 * g + r4;
+            q.W3[rb2][r4] += dy * h2[rb2][r4];
+            float v = (h2[rb2][r4] > 0.f) ? dy * sW3[row] : 0.f;
+            dh2[rb2][r4] = v;
+            q.B2[rb2][r4] += v;
+        }
+    v4f dh1[4];
+    v4f dx = {0.f, 0.f, 0.f, 0.f};
+    if constexpr (!kDecBF16) {
+        // dW2 += dH2 . H1^T (K = the tile's 16 samples)
+#pragma unroll
+        for (int rb = 0; rb < 4; ++rb)
+#pragma unroll
+            for (int r4 = 0; r4 < 4; ++r4) {
+                bufA[(16 * rb + 4 * g + r4) * kTS + j] = h1[rb][r4];
+                bufB[(16 * rb + 4 * g + r4) * kTS + j] = dh2[rb][r4];
+            }
+        wave_lds_sync();
+#pragma unroll
+        for (int rb2 = 0; rb2 < 4; ++rb2) {
+            float4 av = *reinterpret_cast<const float4*>(bufB + (16 * rb2 + j) * kTS + 4 * g);
+#pragma unroll
+            for (int rb = 0; rb < 4; ++rb) {
+                float4 bv = *reinterpret_cast<const float4*>(bufA + (16 * rb + j) * kTS + 4 * g);
+                v4f acc = q.W2[rb2][rb];
+                acc = mfma4(av.x, bv.x, acc);
+                acc = mfma4(av.y, bv.y, acc);
+                acc = mfma4(av.z, bv.z, acc);
+                acc = mfma4(av.w, bv.w, acc);
+                q.W2[rb2][rb] = acc;
+            }
+        }
+        wave_lds_sync();
+        // dH1 = W2^T . dH2, masked by ReLU
+#pragma unroll
+        for (int rb = 0; rb < 4; ++rb) {
+            v4f acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+            for (int rb2 = 0; rb2 < 4; ++rb2)
+#pragma unroll
+                for (int kk = 0; kk < 4; ++kk)
+                    acc = mfma4(s[L::kW2 + (16 * rb2 + 4 * g + kk) * kW2S + 16 * rb + j], dh2[rb2][kk], acc);
+#pragma unroll
+            for (int r4 = 0; r4 < 4; ++r4) {
+                float v = (h1[rb][r4] > 0.f) ? acc[r4] : 0.f;
+                dh1[rb][r4] = v;
+                q.B1[rb][r4] += v;
+            }
+        }
+        // dX = W1^T . dH1 (rows = channels, padded to 16)
+#pragma unroll
+        for (int rb = 0; rb < 4; ++rb)
+#pragma unroll
+            for (int kk = 0; kk < 4; ++kk) {
+                float wv = (j < C) ? s[L::kW1 + (16 * rb + 4 * g + kk) * L::W1S + j] : 0.f;
+                dx = mfma4(wv, dh1[rb][kk], dx);
+            }
+        // dW1 += dH1 . X^T
+#pragma unroll
+        for (int rb = 0; rb < 4; ++rb)
+#pragma unroll
+            for (int r4 = 0; r4 < 4; ++r4) bufA[(16 * rb + 4 * g + r4) * kTS + j] = dh1[rb][r4];
+#pragma unroll
+        for (int qq = 0; qq < CPL; ++qq) bufB[(CPL * g + qq) * kTS + j] = x[qq];
+        wave_lds_sync();
+        float4 bv = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (j < C) bv = *reinterpret_cast<const float4*>(bufB + j * kTS + 4 * g);
+#pragma unroll
+        for (int rb = 0; rb < 4; ++rb) {
+            float4 av = *reinterpret_cast<const float4*>(bufA + (16 * rb + j) * kTS + 4 * g);
+            v4f acc = q.W1[rb];
+            acc = mfma4(av.x, bv.x, acc);
+            acc = mfma4(av.y, bv.y, acc);
+            acc = mfma4(av.z, bv.z, acc);
+            acc = mfma4(av.w, bv.w, acc);
+            q.W1[rb] = acc;
+        }
+        wave_lds_sync();
+    } else {
+        // dW2 += dH2 . H1^T on bf16 MFMA (K = the tile's 16 samples)
+#pragma unroll
+        for (int rb = 0; rb < 4; ++rb)
+#pragma unroll
+            for (int r4 = 0; r4 < 4; ++r4) {
+                bufAb[(16 * rb + 4 * g + r4) * kTB + j] = (__bf16)h1[rb][r4];
+                bufBb[(16 * rb + 4 * g + r4) * kTB + j] = (__bf16)dh2[rb][r4];
+            }
+        wave_lds_sync();
+#pragma unroll
+        for (int rb2 = 0; rb2 < 4; ++rb2) {
+            s4 av = *reinterpret_cast<const s4*>(bufBb + (16 * rb2 + j) * kTB + 4 * g);
+#pragma unroll
+            for (int rb = 0; rb < 4; ++rb) {
+                s4 bv = *reinterpret_cast<const s4*>(bufAb + (16 * rb + j) * kTB + 4 * g);
+                q.W2[rb2][rb] = mfma_bf16_k16(av, bv, q.W2[rb2][rb]);
+            }
+        }
+        wave_lds_sync();
+        // dH1 = W2^T . dH2 (K = 64 in two bf16 steps), masked by ReLU
+#pragma unroll
+        for (int rb = 0; rb < 4; ++rb) {
+            v4f acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+            for (int st = 0; st < 2; ++st) {
+                bf8 av = *reinterpret_cast<const bf8*>(sb + LB::kW2Tb + (16 * rb + j) * kBW + 32 * st + 8 * g);
+                acc = mfma_bf16_k32(av, pack_bf8(dh2[2 * st], dh2[2 * st + 1]), acc);
+            }
+#pragma unroll
+            for (int r4 = 0; r4 < 4; ++r4) {
+                float v = (h1[rb][r4] > 0.f) ? acc[r4] : 0.f;
+                dh1[rb][r4] = v;
+                q.B1[rb][r4] += v;
+            }
+        }
+        // dX = W1^T . dH1 (rows = channels, padded to 16)
+#pragma unroll
+        for (int st = 0; st < 2; ++st) {
+            bf8 av = *reinterpret_cast<const bf8*>(sb + LB::kW1Tb + j * kBW + 32 * st + 8 * g);
+            dx = mfma_bf16_k32(av, pack_bf8(dh1[2 * st], dh1[2 * st + 1]), dx);
+        }
+        // dW1 += dH1 . X^T
+#pragma unroll
+        for (int rb = 0; rb < 4; ++rb)
+#pragma unroll
+            for (int r4 = 0; r4 < 4; ++r4) bufAb[(16 * rb + 4 * g + r4) * kTB + j] = (__bf16)dh1[rb][r4];
+#pragma unroll
+        for (int qq = 0; qq < CPL; ++qq) bufBb[(CPL * g + qq) * kTB + j] = (__bf16)x[qq];
+        wave_lds_sync();
+        s4 bv = {0, 0, 0, 0};
+        if (j < C) bv = *reinterpret_cast<const s4*>(bufBb + j * kTB + 4 * g);
+#pragma unroll
+        for (int rb = 0; rb < 4; ++rb) {
+            s4 av = *reinterpret_cast<const s4*>(bufAb + (16 * rb + j) * kTB + 4 * g);
+            q.W1[rb] = mfma_bf16_k16(av, bv, q.W1[rb]);
+        }
+        wave_lds_sync();
+    }
+    return dx;
+}
+
+// Writes this wave's decoder-gradient partials to row `out` (every entry).
+template <int C>
+__device__ __forceinline__ void store_partials(DecGrads& q, float* out, int g, int j, int lane)
+{
+    constexpr int oW1 = 0, oB1 = kH * C, oW2 = oB1 + kH, oB2 = oW2 + kH * kH, oW3 = oB2 + kH, oB3 = oW3 + kH;
+#pragma unroll
+    for (int rb = 0; rb < 4; ++rb)
+#pragma unroll
+        for (int r4 = 0; r4 < 4; ++r4)
+            if (j < C) out[oW1 + (16 * rb + 4 * g + r4) * C + j] = q.W1[rb][r4];
+#pragma unroll
+    for (int rb2 = 0; rb2 < 4; ++rb2)
+#pragma unroll
+        for (int rb = 0; rb < 4; ++rb)
+#pragma unroll
+            for (int r4 = 0; r4 < 4; ++r4) out[oW2 + (16 * rb2 + 4 * g + r4) * kH + 16 * rb + j] = q.W2[rb2][rb][r4];
+#pragma unroll
+    for (int p = 0; p < 4; ++p)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            float v1 = q.B1[p][r], v2 = q.B2[p][r], v3 = q.W3[p][r];
+#pragma unroll
+            for (int off = 1; off < 16; off <<= 1) {
+                v1 += __shfl_xor(v1, off, 64);
+                v2 += __shfl_xor(v2, off, 64);
+                v3 += __shfl_xor(v3, off, 64);
+            }
+            if (j == 0) {
+                const int row = 16 * p + 4 * g + r;
+                out[oB1 + row] = v1;
+                out[oB2 + row] = v2;
+                out[oW3 + row] = v3;
+            }
+        }
+    float b3 = q.B3;
+#pragma unroll
+    for (int off = 1; off < 16; off <<= 1) b3 += __shfl_xor(b3, off, 64);
+    if (lane == 0) out[oB3] = b3;
+}
+
+// ---------------------------------------------------------------------------
+// Node-major adjoint.  Intervals are counting-sorted by grid slot and cut
+// into chunks of at most kChunk intervals of one node.  A persistent
+// workgroup (one per CU: the node's gradient lives in LDS) takes chunks
+// from a global counter, splits the chunk's intervals over its 4 waves,
+// recomputes every sample (interval -> t, weight, position, gather, decoder),
+// runs the decoder backward on MFMA, and scatters w_corner * dX into the
+// LDS copy of the node's grid gradient with LDS float atomics.  At the end
+// of the chunk the LDS gradient is added to the global gradient once:
+// a plain read-add-write when the chunk covers the whole node, a global
+// atomic add when the node was split over several chunks.
+// ---------------------------------------------------------------------------
+constexpr int kChunk = 2048;
+
+struct NodeBwdArgs {
+    ProjArgs a;
+    const float* dLdP;
+    const int32_t* sorted;    // interval ids, grouped by grid slot
+    const int4* chunks;       // (slot, begin, end, split)
+    const int64_t* chunk_off; // [num_slots + 1]; chunk_off[num_slots] = chunk count
+    int64_t num_slots;
+    int* counter;             // zeroed before the launch
+    const int32_t* iv_ray;
+    float* grad_grid;
+    float* ws;                // decoder-gradient partial rows
+    int lds_f32_off;          // float offset of the weight region in dynamic LDS
+    int lds_bf16_off;         // byte offset of the bf16 weight region
+    int lds_tb_off;           // byte offset of the transpose buffers
+};
+
+template <int C, bool kDecBF16>
+__host__ __device__ constexpr int weights_f32_floats()
+{
+    return kDecBF16 ? BSmem<C>::kF32Total : MlpSmem<C>::kTotal;
+}
+
+template <int C, bool kBF16, bool kDecBF16>
+__global__ __launch_bounds__(64 * kWaves) void project_bwd_node_kernel(NodeBwdArgs b)
+{
+    extern __shared__ __attribute__((aligned(16))) unsigned char dsm[];
+    __shared__ int s_chunk;
+    const ProjArgs& a = b.a;
     constexpr int CPL = C / 4;
     constexpr int NP = mlp_numel<C>();
-    constexpr int kTF = kDecBF16 ? 4 : kH * kTS;  // f32 transpose buffer (FP32 decoder)
-    constexpr int kTBb = kDecBF16 ? kH * kTB : 8; // bf16 transpose buffer (BF16 decoder)
-    __shared__ __attribute__((aligned(16))) float s[SM::kF32];
-    __shared__ __attribute__((aligned(16))) __bf16 sb[SM::kBF16];
-    __shared__ __attribute__((aligned(16))) float tbuf[kWaves][2][kTF];
-    __shared__ __attribute__((aligned(16))) __bf16 tbufb[kWaves][2][kTBb];
+    const int G = a.G;
+    const int gsz = G * G * G * C;
+    float* lgrad = reinterpret_cast<float*>(dsm);
+    float* s = reinterpret_cast<float*>(dsm) + b.lds_f32_off;
+    __bf16* sb = reinterpret_cast<__bf16*>(dsm + b.lds_bf16_off);
+    const int lane = threadIdx.x & 63;
+    const int g = lane >> 4, j = lane & 15;
+    const int wl = threadIdx.x >> 6;
+    float* bufA = reinterpret_cast<float*>(dsm + b.lds_tb_off) + wl * 2 * kH * kTS;
+    float* bufB = bufA + kH * kTS;
+    __bf16* bufAb = reinterpret_cast<__bf16*>(dsm + b.lds_tb_off) + wl * 2 * kH * kTB;
+    __bf16* bufBb = bufAb + kH * kTB;
+
     if constexpr (kDecBF16)
         load_mlp_bf16_to_lds<C>(a.mlp, sb, s);
     else
         load_mlp_to_lds<C>(a.mlp, s);
+    for (int i = threadIdx.x; i < gsz; i += blockDim.x) lgrad[i] = 0.f;
+
+    DecGrads q;
+    zero_grads(q);
+    const int64_t total_chunks = b.chunk_off[b.num_slots];
     __syncthreads();
 
-    const int lane = threadIdx.x & 63;
-    const int g = lane >> 4, j = lane & 15;
-    const int wl = threadIdx.x >> 6;
-    float* bufA = tbuf[wl][0];
-    float* bufB = tbuf[wl][1];
-    __bf16* bufAb = tbufb[wl][0];
-    __bf16* bufBb = tbufb[wl][1];
-    const float* sW3 = kDecBF16 ? s + LB::kW3 : s + L::kW3;
-    const int64_t wave = (int64_t)blockIdx.x * kWaves + wl;
-    const int64_t r0 = wave * a.rpw;
-    const int64_t r1 = (r0 + a.rpw < a.R) ? r0 + a.rpw : a.R;
+    while (true) {
+        if (threadIdx.x == 0) s_chunk = atomicAdd(b.counter, 1);
+        __syncthreads();
+        const int64_t c = s_chunk;
+        __syncthreads();
+        if (c >= total_chunks) break;
+        const int4 ch = b.chunks[c];
+        const int64_t gbase = (int64_t)ch.x * gsz;
+        const int n_iv = ch.z - ch.y;
+        const int per = (n_iv + kWaves - 1) / kWaves;
+        const int ib = ch.y + wl * per;
+        const int ie = min(ib + per, ch.z);
+        // samples of this wave's intervals
+        int tot = 0;
+        for (int k = ib + lane; k < ie; k += 64) tot += a.iv_count[b.sorted[k]];
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) tot += __shfl_xor(tot, off, 64);
 
-    v4f accW2[4][4];
-    v4f accW1[4];
-#pragma unroll
-    for (int p = 0; p < 4; ++p) {
-        accW1[p] = (v4f){0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-        for (int q = 0; q < 4; ++q) accW2[p][q] = (v4f){0.f, 0.f, 0.f, 0.f};
-    }
-    float gW3[4][4], gB2[4][4], gB1[4][4];
-#pragma unroll
-    for (int p = 0; p < 4; ++p)
-#pragma unroll
-        for (int q = 0; q < 4; ++q) gW3[p][q] = gB2[p][q] = gB1[p][q] = 0.f;
-    float gB3 = 0.f;
-
-    for (int64_t r = r0; r < r1; ++r) {
-        const int64_t h0 = a.iv_off[r], h1e = a.iv_off[r + 1];
-        const float dp = dLdP[r];
-        if (h0 == h1e || dp == 0.f) continue;
-        const int ns = a.iv_first[h1e - 1] + a.iv_count[h1e - 1];
-        float o[3], d[3];
-#pragma unroll
-        for (int k = 0; k < 3; ++k) {
-            o[k] = a.O[r * 3 + k];
-            d[k] = a.D[r * 3 + k];
-        }
-        int64_t cur = h0;
-        for (int base = 0; base < ns; base += kTile) {
+        int cur = ib, cum = 0;
+        int cur_cnt = (ib < ie) ? a.iv_count[b.sorted[ib]] : 0;
+        for (int base = 0; base < tot; base += kTile) {
+            const int t = base + j;
             SampleCtx sc;
-            make_sample(a, r, o, d, ns, base + j, h1e, cur, C, sc);
+            sc.valid = t < tot;
+            sc.w = 0.f;
+            sc.gbase = -1;
+            float dp = 0.f;
+            if (sc.valid) {
+                while (cum + cur_cnt <= t) {
+                    cum += cur_cnt;
+                    ++cur;
+                    cur_cnt = a.iv_count[b.sorted[cur]];
+                }
+                const int iv = b.sorted[cur];
+                const int64_t r = b.iv_ray[iv];
+                float o[3], d[3];
+#pragma unroll
+                for (int k = 0; k < 3; ++k) {
+                    o[k] = a.O[r * 3 + k];
+                    d[k] = a.D[r * 3 + k];
+                }
+                const int jj = t - cum;
+                make_sample_iv(a, r, o, d, a.iv_node[iv], a.iv_tmin[iv], a.iv_tmax[iv], cur_cnt, jj,
+                               a.iv_first[iv] + jj, C, sc);
+                dp = b.dLdP[r];
+            }
             float x[CPL];
             gather_channels<C, kBF16>(a, sc, CPL * g, x);
             v4f h1[4], h2[4];
@@ -523,209 +798,90 @@ __global__ __launch_bounds__(64 * kWaves) void project_bwd_kernel(ProjArgs a, co
             else
                 y = decoder_fwd<C>(s, x, g, j, h1, h2);
             const float dy = sc.valid ? dp * sc.w * softplus_grad_f(y, a.beta) : 0.f;
-            if (g == 0) gB3 += dy;
-
-            // dH2 = dy * W3 masked by ReLU; dW3, db2 partials
-            v4f dh2[4];
-#pragma unroll
-            for (int rb2 = 0; rb2 < 4; ++rb2)
-#pragma unroll
-                for (int r4 = 0; r4 < 4; ++r4) {
-                    const int row = 16 * rb2 + 4 * g + r4;
-                    gW3[rb2][r4] += dy * h2[rb2][r4];
-                    float v = (h2[rb2][r4] > 0.f) ? dy * sW3[row] : 0.f;
-                    dh2[rb2][r4] = v;
-                    gB2[rb2][r4] += v;
-                }
-
-            v4f dh1[4];
-            v4f dx = {0.f, 0.f, 0.f, 0.f};
-            if constexpr (!kDecBF16) {
-                // dW2 += dH2 . H1^T  (K = the tile's 16 samples)
-#pragma unroll
-                for (int rb = 0; rb < 4; ++rb)
-#pragma unroll
-                    for (int r4 = 0; r4 < 4; ++r4) {
-                        bufA[(16 * rb + 4 * g + r4) * kTS + j] = h1[rb][r4];
-                        bufB[(16 * rb + 4 * g + r4) * kTS + j] = dh2[rb][r4];
-                    }
-                wave_lds_sync();
-#pragma unroll
-                for (int rb2 = 0; rb2 < 4; ++rb2) {
-                    float4 av = *reinterpret_cast<const float4*>(bufB + (16 * rb2 + j) * kTS + 4 * g);
-#pragma unroll
-                    for (int rb = 0; rb < 4; ++rb) {
-                        float4 bv = *reinterpret_cast<const float4*>(bufA + (16 * rb + j) * kTS + 4 * g);
-                        v4f acc = accW2[rb2][rb];
-                        acc = mfma4(av.x, bv.x, acc);
-                        acc = mfma4(av.y, bv.y, acc);
-                        acc = mfma4(av.z, bv.z, acc);
-                        acc = mfma4(av.w, bv.w, acc);
-                        accW2[rb2][rb] = acc;
-                    }
-                }
-                wave_lds_sync();
-
-                // dH1 = W2^T . dH2, masked by ReLU
-#pragma unroll
-                for (int rb = 0; rb < 4; ++rb) {
-                    v4f acc = {0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-                    for (int rb2 = 0; rb2 < 4; ++rb2)
-#pragma unroll
-                        for (int kk = 0; kk < 4; ++kk)
-                            acc = mfma4(s[L::kW2 + (16 * rb2 + 4 * g + kk) * kW2S + 16 * rb + j], dh2[rb2][kk], acc);
-#pragma unroll
-                    for (int r4 = 0; r4 < 4; ++r4) {
-                        float v = (h1[rb][r4] > 0.f) ? acc[r4] : 0.f;
-                        dh1[rb][r4] = v;
-                        gB1[rb][r4] += v;
-                    }
-                }
-
-                // dX = W1^T . dH1 (rows = channels, padded to 16)
-#pragma unroll
-                for (int rb = 0; rb < 4; ++rb)
-#pragma unroll
-                    for (int kk = 0; kk < 4; ++kk) {
-                        float wv = (j < C) ? s[L::kW1 + (16 * rb + 4 * g + kk) * L::W1S + j] : 0.f;
-                        dx = mfma4(wv, dh1[rb][kk], dx);
-                    }
-            } else {
-                // dW2 += dH2 . H1^T on bf16 MFMA (K = the tile's 16 samples)
-#pragma unroll
-                for (int rb = 0; rb < 4; ++rb)
-#pragma unroll
-                    for (int r4 = 0; r4 < 4; ++r4) {
-                        bufAb[(16 * rb + 4 * g + r4) * kTB + j] = (__bf16)h1[rb][r4];
-                        bufBb[(16 * rb + 4 * g + r4) * kTB + j] = (__bf16)dh2[rb][r4];
-                    }
-                wave_lds_sync();
-#pragma unroll
-                for (int rb2 = 0; rb2 < 4; ++rb2) {
-                    s4 av = *reinterpret_cast<const s4*>(bufBb + (16 * rb2 + j) * kTB + 4 * g);
-#pragma unroll
-                    for (int rb = 0; rb < 4; ++rb) {
-                        s4 bv = *reinterpret_cast<const s4*>(bufAb + (16 * rb + j) * kTB + 4 * g);
-                        accW2[rb2][rb] = mfma_bf16_k16(av, bv, accW2[rb2][rb]);
-                    }
-                }
-                wave_lds_sync();
-
-                // dH1 = W2^T . dH2 (K = 64 in two bf16 steps), masked by ReLU
-#pragma unroll
-                for (int rb = 0; rb < 4; ++rb) {
-                    v4f acc = {0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-                    for (int st = 0; st < 2; ++st) {
-                        bf8 av = *reinterpret_cast<const bf8*>(sb + LB::kW2Tb + (16 * rb + j) * kBW + 32 * st + 8 * g);
-                        acc = mfma_bf16_k32(av, pack_bf8(dh2[2 * st], dh2[2 * st + 1]), acc);
-                    }
-#pragma unroll
-                    for (int r4 = 0; r4 < 4; ++r4) {
-                        float v = (h1[rb][r4] > 0.f) ? acc[r4] : 0.f;
-                        dh1[rb][r4] = v;
-                        gB1[rb][r4] += v;
-                    }
-                }
-
-                // dX = W1^T . dH1 (rows = channels, padded to 16)
-#pragma unroll
-                for (int st = 0; st < 2; ++st) {
-                    bf8 av = *reinterpret_cast<const bf8*>(sb + LB::kW1Tb + j * kBW + 32 * st + 8 * g);
-                    dx = mfma_bf16_k32(av, pack_bf8(dh1[2 * st], dh1[2 * st + 1]), dx);
-                }
-            }
-
-            // scatter w_corner * dX into the channels-last grid gradient
-            if (sc.valid && sc.gbase >= 0 && 4 * g < C) {
+            v4f dx = tile_backward<C, kDecBF16>(s, sb, bufA, bufB, bufAb, bufBb, x, h1, h2, dy, g, j, q);
+            if (sc.valid && sc.gbase >= 0 && 4 * g < C && dy != 0.f) {
 #pragma unroll
                 for (int k = 0; k < 8; ++k) {
-                    float* gp = grad_grid + sc.gbase + corner_cell(sc.tri, k, a.G) * C + 4 * g;
+                    float* gp = lgrad + corner_cell(sc.tri, k, G) * C + 4 * g;
                     const float wk = sc.tri.w[k];
 #pragma unroll
-                    for (int r4 = 0; r4 < 4; ++r4) unsafeAtomicAdd(gp + r4, wk * dx[r4]);
+                    for (int r4 = 0; r4 < 4; ++r4) atomicAdd(gp + r4, wk * dx[r4]);
                 }
-            }
-
-            // dW1 += dH1 . X^T
-            if constexpr (!kDecBF16) {
-#pragma unroll
-                for (int rb = 0; rb < 4; ++rb)
-#pragma unroll
-                    for (int r4 = 0; r4 < 4; ++r4) bufA[(16 * rb + 4 * g + r4) * kTS + j] = dh1[rb][r4];
-#pragma unroll
-                for (int q = 0; q < CPL; ++q) bufB[(CPL * g + q) * kTS + j] = x[q];
-                wave_lds_sync();
-                float4 bv = make_float4(0.f, 0.f, 0.f, 0.f);
-                if (j < C) bv = *reinterpret_cast<const float4*>(bufB + j * kTS + 4 * g);
-#pragma unroll
-                for (int rb = 0; rb < 4; ++rb) {
-                    float4 av = *reinterpret_cast<const float4*>(bufA + (16 * rb + j) * kTS + 4 * g);
-                    v4f acc = accW1[rb];
-                    acc = mfma4(av.x, bv.x, acc);
-                    acc = mfma4(av.y, bv.y, acc);
-                    acc = mfma4(av.z, bv.z, acc);
-                    acc = mfma4(av.w, bv.w, acc);
-                    accW1[rb] = acc;
-                }
-                wave_lds_sync();
-            } else {
-#pragma unroll
-                for (int rb = 0; rb < 4; ++rb)
-#pragma unroll
-                    for (int r4 = 0; r4 < 4; ++r4) bufAb[(16 * rb + 4 * g + r4) * kTB + j] = (__bf16)dh1[rb][r4];
-#pragma unroll
-                for (int q = 0; q < CPL; ++q) bufBb[(CPL * g + q) * kTB + j] = (__bf16)x[q];
-                wave_lds_sync();
-                s4 bv = {0, 0, 0, 0};
-                if (j < C) bv = *reinterpret_cast<const s4*>(bufBb + j * kTB + 4 * g);
-#pragma unroll
-                for (int rb = 0; rb < 4; ++rb) {
-                    s4 av = *reinterpret_cast<const s4*>(bufAb + (16 * rb + j) * kTB + 4 * g);
-                    accW1[rb] = mfma_bf16_k16(av, bv, accW1[rb]);
-                }
-                wave_lds_sync();
             }
         }
+        __syncthreads();
+        // flush the node gradient once, clear LDS for the next chunk
+        float* gg = b.grad_grid + gbase;
+        if (ch.w) {
+            for (int i = threadIdx.x; i < gsz; i += blockDim.x) {
+                float v = lgrad[i];
+                lgrad[i] = 0.f;
+                if (v != 0.f) unsafeAtomicAdd(gg + i, v);
+            }
+        } else {
+            for (int i = threadIdx.x; i < gsz; i += blockDim.x) {
+                float v = lgrad[i];
+                lgrad[i] = 0.f;
+                if (v != 0.f) gg[i] += v;
+            }
+        }
+        __syncthreads();
     }
+    store_partials<C>(q, b.ws + ((int64_t)blockIdx.x * kWaves + wl) * NP, g, j, lane);
+}
 
-    // per-wave partials -> workspace row `wave` (every entry written)
-    float* out = ws + wave * (int64_t)NP;
-    constexpr int oW1 = 0, oB1 = kH * C, oW2 = oB1 + kH, oB2 = oW2 + kH * kH, oW3 = oB2 + kH, oB3 = oW3 + kH;
-#pragma unroll
-    for (int rb = 0; rb < 4; ++rb)
-#pragma unroll
-        for (int r4 = 0; r4 < 4; ++r4) {
-            if (j < C) out[oW1 + (16 * rb + 4 * g + r4) * C + j] = accW1[rb][r4];
-        }
-#pragma unroll
-    for (int rb2 = 0; rb2 < 4; ++rb2)
-#pragma unroll
-        for (int rb = 0; rb < 4; ++rb)
-#pragma unroll
-            for (int r4 = 0; r4 < 4; ++r4) out[oW2 + (16 * rb2 + 4 * g + r4) * kH + 16 * rb + j] = accW2[rb2][rb][r4];
-#pragma unroll
-    for (int p = 0; p < 4; ++p)
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            float v1 = gB1[p][q], v2 = gB2[p][q], v3 = gW3[p][q];
-#pragma unroll
-            for (int off = 1; off < 16; off <<= 1) {
-                v1 += __shfl_xor(v1, off, 64);
-                v2 += __shfl_xor(v2, off, 64);
-                v3 += __shfl_xor(v3, off, 64);
-            }
-            if (j == 0) {
-                const int row = 16 * p + 4 * g + q;
-                out[oB1 + row] = v1;
-                out[oB2 + row] = v2;
-                out[oW3 + row] = v3;
-            }
-        }
-#pragma unroll
-    for (int off = 1; off < 16; off <<= 1) gB3 += __shfl_xor(gB3, off, 64);
-    if (lane == 0) out[oB3] = gB3;
+// ---- counting sort of intervals by grid slot, chunk table ----
+__global__ __launch_bounds__(256) void fill_i32_kernel(int32_t* p, int64_t n, int32_t v)
+{
+    int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+__global__ __launch_bounds__(256) void slot_hist_kernel(const int32_t* __restrict__ iv_node,
+                                                        const int32_t* __restrict__ node_to_grid, int64_t H,
+                                                        int32_t* __restrict__ cnt)
+{
+    int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= H) return;
+    int slot = node_to_grid[iv_node[i]];
+    if (slot >= 0) atomicAdd(&cnt[slot], 1);
+}
+
+__global__ __launch_bounds__(256) void slot_scatter_kernel(const int32_t* __restrict__ iv_node,
+                                                           const int32_t* __restrict__ node_to_grid, int64_t H,
+                                                           const int64_t* __restrict__ slot_off,
+                                                           int32_t* __restrict__ cursor, int32_t* __restrict__ sorted)
+{
+    int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= H) return;
+    int slot = node_to_grid[iv_node[i]];
+    if (slot < 0) return;
+    int pos = atomicAdd(&cursor[slot], 1);
+    sorted[slot_off[slot] + pos] = (int32_t)i;
+}
+
+__global__ __launch_bounds__(256) void chunk_count_kernel(const int64_t* __restrict__ slot_off, int64_t NA,
+                                                          int32_t* __restrict__ chunk_cnt)
+{
+    int64_t s = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (s >= NA) return;
+    int64_t n = slot_off[s + 1] - slot_off[s];
+    chunk_cnt[s] = (int32_t)((n + kChunk - 1) / kChunk);
+}
+
+__global__ __launch_bounds__(256) void chunk_desc_kernel(const int64_t* __restrict__ slot_off,
+                                                         const int64_t* __restrict__ chunk_off, int64_t NA,
+                                                         int4* __restrict__ desc)
+{
+    int64_t s = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (s >= NA) return;
+    int64_t b0 = slot_off[s], e0 = slot_off[s + 1];
+    int64_t c0 = chunk_off[s], nch = chunk_off[s + 1] - c0;
+    for (int64_t k = 0; k < nch; ++k) {
+        int64_t b = b0 + k * kChunk;
+        int64_t e = b + kChunk < e0 ? b + kChunk : e0;
+        desc[c0 + k] = make_int4((int)s, (int)b, (int)e, nch > 1 ? 1 : 0);
+    }
 }
 
 // grad_mlp[i] += sum_w ws[w][i], summed in wave order.
@@ -821,13 +977,93 @@ atdf_status atdf_project_fwd(const void* packed_tree, int64_t num_nodes, const f
     return check_launch("atdf_project_fwd");
 }
 
-size_t atdf_project_bwd_workspace_bytes(int64_t R, const atdf_field* f)
+}  // extern "C"
+
+namespace {
+struct BwdWorkspace {
+    size_t cnt, slot_off, sorted, chunk_cnt, chunk_off, desc, counter, scan, partials, total;
+};
+
+size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+int persistent_blocks()
 {
-    if (!f || R <= 0) return 0;
-    int64_t np = (int64_t)f->H * f->C + f->H + (int64_t)f->H * f->H + f->H + f->H + 1;
-    int64_t nwaves = div_up(R, rays_per_wave(f));
-    int64_t nw_pad = div_up(nwaves, kWaves) * kWaves;  // every launched wave writes a row
-    return (size_t)(nw_pad * np) * sizeof(float);
+    static int n = 0;
+    if (n == 0) {
+        int dev = 0, cus = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
+            cus = 256;
+        n = cus;
+    }
+    return n;
+}
+
+BwdWorkspace bwd_layout(int64_t H, int64_t NA, int np)
+{
+    BwdWorkspace w{};
+    size_t o = 0;
+    auto take = [&](size_t bytes) { size_t at = o; o = align256(o + bytes); return at; };
+    w.cnt = take(sizeof(int32_t) * (NA + 1));
+    w.slot_off = take(sizeof(int64_t) * (NA + 1));
+    w.sorted = take(sizeof(int32_t) * (H > 0 ? H : 1));
+    w.chunk_cnt = take(sizeof(int32_t) * (NA + 1));
+    w.chunk_off = take(sizeof(int64_t) * (NA + 1));
+    w.desc = take(sizeof(int4) * (H / kChunk + NA + 1));
+    w.counter = take(256);
+    w.scan = take(atdf_scan_workspace_bytes(NA));
+    w.partials = take(sizeof(float) * (size_t)persistent_blocks() * kWaves * np);
+    w.total = o;
+    return w;
+}
+
+template <int C, bool kDecBF16>
+void lds_layout(int G, int& f32_off, int& bf16_off, int& tb_off, int& total)
+{
+    const int gsz = G * G * G * C;
+    f32_off = (gsz + 3) & ~3;
+    int after_f32 = (f32_off + weights_f32_floats<C, kDecBF16>()) * 4;
+    bf16_off = (after_f32 + 15) & ~15;
+    int after_bf = bf16_off + (kDecBF16 ? BSmem<C>::kBf16Total * 2 : 0);
+    tb_off = (after_bf + 15) & ~15;
+    total = tb_off + (kDecBF16 ? kWaves * 2 * kH * kTB * 2 : kWaves * 2 * kH * kTS * 4);
+}
+
+template <int C, bool kBF16, bool kDecBF16>
+atdf_status launch_node_bwd(NodeBwdArgs args, hipStream_t s)
+{
+    int f32_off, bf16_off, tb_off, total;
+    lds_layout<C, kDecBF16>(args.a.G, f32_off, bf16_off, tb_off, total);
+    args.lds_f32_off = f32_off;
+    args.lds_bf16_off = bf16_off;
+    args.lds_tb_off = tb_off;
+    static int max_lds = 0;
+    if (max_lds == 0) {
+        int dev = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess) {
+            (void)hipGetLastError();
+            max_lds = 64 * 1024;
+        }
+    }
+    if (total > max_lds) {
+        set_error("atdf_project_bwd: node gradient (G=%d, C=%d) needs %d bytes of LDS per workgroup, the device "
+                  "allows %d", args.a.G, C, total, max_lds);
+        return ATDF_ERR_UNSUPPORTED;
+    }
+    hipLaunchKernelGGL((project_bwd_node_kernel<C, kBF16, kDecBF16>), dim3(persistent_blocks()),
+                       dim3(64 * kWaves), total, s, args);
+    return ATDF_OK;
+}
+}  // namespace
+
+extern "C" {
+
+size_t atdf_project_bwd_workspace_bytes(int64_t R, int64_t H, const atdf_field* f)
+{
+    if (!f || R <= 0 || H < 0 || f->num_grids <= 0) return 0;
+    int np = (int)((int64_t)f->H * f->C + f->H + (int64_t)f->H * f->H + f->H + f->H + 1);
+    return bwd_layout(H, f->num_grids, np).total;
 }
 
 atdf_status atdf_project_bwd(const void* packed_tree, int64_t num_nodes, const float* O, const float* D, int64_t R,
@@ -839,18 +1075,58 @@ atdf_status atdf_project_bwd(const void* packed_tree, int64_t num_nodes, const f
     if (st) return st;
     if (R == 0) return ATDF_OK;
     ATDF_REQUIRE(dLdP && grad_grid && grad_mlp && workspace, "atdf_project_bwd: null buffer");
-    ATDF_REQUIRE(workspace_bytes >= atdf_project_bwd_workspace_bytes(R, f), "atdf_project_bwd: workspace too small");
-    ProjArgs a = make_args(packed_tree, O, D, R, sp, iv_off, iv, f);
-    int64_t nwaves = div_up(R, a.rpw);
-    int64_t nblocks = div_up(nwaves, kWaves);
-    dim3 grid((unsigned)nblocks), block(64 * kWaves);
+    ATDF_REQUIRE(iv->total >= 0 && iv->ray, "atdf_project_bwd: intervals need total and ray");
+    const int64_t H = iv->total;
+    const int64_t NA = f->num_grids;
+    ATDF_REQUIRE(H < (int64_t)INT32_MAX, "atdf_project_bwd: too many intervals");
+    ATDF_REQUIRE(workspace_bytes >= atdf_project_bwd_workspace_bytes(R, H, f),
+                 "atdf_project_bwd: workspace too small");
+    const int np = f->C == 16 ? mlp_numel<16>() : mlp_numel<8>();
+    BwdWorkspace L = bwd_layout(H, NA, np);
+    unsigned char* w = reinterpret_cast<unsigned char*>(workspace);
+    int32_t* cnt = reinterpret_cast<int32_t*>(w + L.cnt);
+    int64_t* slot_off = reinterpret_cast<int64_t*>(w + L.slot_off);
+    int32_t* sorted = reinterpret_cast<int32_t*>(w + L.sorted);
+    int32_t* chunk_cnt = reinterpret_cast<int32_t*>(w + L.chunk_cnt);
+    int64_t* chunk_off = reinterpret_cast<int64_t*>(w + L.chunk_off);
+    int4* desc = reinterpret_cast<int4*>(w + L.desc);
+    int* counter = reinterpret_cast<int*>(w + L.counter);
+    void* scan_ws = w + L.scan;
+    float* partials = reinterpret_cast<float*>(w + L.partials);
     hipStream_t s = to_hip(stream);
-    float* ws = reinterpret_cast<float*>(workspace);
+    const size_t scan_bytes = atdf_scan_workspace_bytes(NA);
+
+    hipLaunchKernelGGL(fill_i32_kernel, dim3((unsigned)div_up(NA, 256)), dim3(256), 0, s, cnt, NA, 0);
+    hipLaunchKernelGGL(fill_i32_kernel, dim3(1), dim3(256), 0, s, counter, 1, 0);
+    if (H > 0)
+        hipLaunchKernelGGL(slot_hist_kernel, dim3((unsigned)div_up(H, 256)), dim3(256), 0, s, iv->node,
+                           f->node_to_grid, H, cnt);
+    st = atdf_exclusive_scan_i32(cnt, NA, slot_off, scan_ws, scan_bytes, stream);
+    if (st) return st;
+    hipLaunchKernelGGL(fill_i32_kernel, dim3((unsigned)div_up(NA, 256)), dim3(256), 0, s, cnt, NA, 0);
+    if (H > 0)
+        hipLaunchKernelGGL(slot_scatter_kernel, dim3((unsigned)div_up(H, 256)), dim3(256), 0, s, iv->node,
+                           f->node_to_grid, H, slot_off, cnt, sorted);
+    hipLaunchKernelGGL(chunk_count_kernel, dim3((unsigned)div_up(NA, 256)), dim3(256), 0, s, slot_off, NA, chunk_cnt);
+    st = atdf_exclusive_scan_i32(chunk_cnt, NA, chunk_off, scan_ws, scan_bytes, stream);
+    if (st) return st;
+    hipLaunchKernelGGL(chunk_desc_kernel, dim3((unsigned)div_up(NA, 256)), dim3(256), 0, s, slot_off, chunk_off, NA,
+                       desc);
+
+    NodeBwdArgs args{};
+    args.a = make_args(packed_tree, O, D, R, sp, iv_off, iv, f);
+    args.dLdP = dLdP;
+    args.sorted = sorted;
+    args.chunks = desc;
+    args.chunk_off = chunk_off;
+    args.num_slots = NA;
+    args.counter = counter;
+    args.iv_ray = iv->ray;
+    args.grad_grid = grad_grid;
+    args.ws = partials;
     const bool bf = f->grid_dtype == ATDF_GRID_BF16;
     const bool bd = f->decoder_dtype == ATDF_DECODER_BF16;
-    const int np = f->C == 16 ? mlp_numel<16>() : mlp_numel<8>();
-#define ATDF_BWD(CC, BF, BD) \
-    hipLaunchKernelGGL((project_bwd_kernel<CC, BF, BD>), grid, block, 0, s, a, dLdP, grad_grid, ws)
+#define ATDF_BWD(CC, BF, BD) st = launch_node_bwd<CC, BF, BD>(args, s)
     if (f->C == 16) {
         if (bf) { if (bd) ATDF_BWD(16, true, true); else ATDF_BWD(16, true, false); }
         else    { if (bd) ATDF_BWD(16, false, true); else ATDF_BWD(16, false, false); }
@@ -859,8 +1135,9 @@ atdf_status atdf_project_bwd(const void* packed_tree, int64_t num_nodes, const f
         else    { if (bd) ATDF_BWD(8, false, true); else ATDF_BWD(8, false, false); }
     }
 #undef ATDF_BWD
-    hipLaunchKernelGGL(reduce_partials_kernel, dim3((unsigned)div_up(np, 256)), dim3(256), 0, s, ws,
-                       nblocks * kWaves, np, grad_mlp);
+    if (st) return st;
+    hipLaunchKernelGGL(reduce_partials_kernel, dim3((unsigned)div_up(np, 256)), dim3(256), 0, s, partials,
+                       (int64_t)persistent_blocks() * kWaves, np, grad_mlp);
     return check_launch("atdf_project_bwd");
 }
 

@@ -278,6 +278,11 @@ __global__ __launch_bounds__(kScanThreads) void scan_apply_kernel(const int32_t*
     if (blockIdx.x == 0 && threadIdx.x == 0) out[0] = 0;
 }
 
+__global__ void zero_i64_kernel(int64_t* p)
+{
+    if (threadIdx.x == 0) p[0] = 0;
+}
+
 }  // namespace atdf
 
 using namespace atdf;
@@ -399,7 +404,7 @@ atdf_status atdf_exclusive_scan_i32(const int32_t* in, int64_t n, int64_t* out, 
     ATDF_REQUIRE(n >= 0 && out, "atdf_exclusive_scan_i32: bad arguments");
     hipStream_t s = to_hip(stream);
     if (n == 0) {
-        hipMemsetAsync(out, 0, sizeof(int64_t), s);
+        hipLaunchKernelGGL(zero_i64_kernel, dim3(1), dim3(64), 0, s, out);
         return check_launch("atdf_exclusive_scan_i32");
     }
     ATDF_REQUIRE(in && workspace, "atdf_exclusive_scan_i32: null buffer");

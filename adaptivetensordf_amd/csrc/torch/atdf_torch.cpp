@@ -40,6 +40,7 @@ atdf_intervals Intervals::view() const
         v.first = first.data_ptr<int32_t>();
         v.ray = ray.data_ptr<int32_t>();
     }
+    v.total = num_intervals;
     return v;
 }
 
@@ -585,7 +586,7 @@ struct ProjectRaysFn : public torch::autograd::Function<ProjectRaysFn> {
         const int64_t R = origin.size(0);
         auto grad_grid = torch::zeros(grid.sizes(), grid.options().dtype(torch::kFloat32));
         auto grad_mlp = torch::zeros_like(mlp);
-        auto ws = torch::empty({(int64_t)atdf_project_bwd_workspace_bytes(R, &f)},
+        auto ws = torch::empty({(int64_t)atdf_project_bwd_workspace_bytes(R, iv.num_intervals, &f)},
                                torch::TensorOptions().dtype(torch::kUInt8).device(grid.device()));
         atdf_intervals v = iv.view();
         check_status(atdf_project_bwd(tree->packed(), tree->NumNodes(), origin.data_ptr<float>(),
@@ -656,7 +657,7 @@ Tensor TrainStep(HyperTreeBase& tree, TrainState& st, const RayList& rays, Tenso
     check_status(atdf_mse_loss(P.data_ptr<float>(), target.data_ptr<float>(), R, 1.0f, dP.data_ptr<float>(),
                                st.loss.data_ptr<double>(), mse_ws.data_ptr(), mse_ws.numel(), stream),
                  "atdf_mse_loss");
-    auto ws = torch::empty({(int64_t)atdf_project_bwd_workspace_bytes(R, &f)},
+    auto ws = torch::empty({(int64_t)atdf_project_bwd_workspace_bytes(R, iv.num_intervals, &f)},
                            torch::TensorOptions().dtype(torch::kUInt8).device(rays.origin.device()));
     check_status(atdf_project_bwd(tree.packed(), tree.NumNodes(), rays.origin.data_ptr<float>(),
                                   rays.direction.data_ptr<float>(), R, &s, iv.ray_offset.data_ptr<int64_t>(), &v, &f,

@@ -46,6 +46,8 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--profile-steps", type=int, default=0, help="only run this many steps (profiling)")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only copy a 512 MiB buffer 3 times (scale factor for HBM byte counters)")
     return ap.parse_args()
 
 
@@ -61,6 +63,16 @@ def main():
         sys.exit(2)
 
     import torch
+
+    if args.copy_calibration:
+        x = torch.empty(1 << 27, dtype=torch.float32, device="cuda:0").fill_(1.0)
+        y = torch.empty_like(x)
+        torch.cuda.synchronize()
+        for _ in range(3):
+            y.copy_(x)
+        torch.cuda.synchronize()
+        log("copied 3 x 512 MiB (read 512 MiB + write 512 MiB per copy)")
+        return
 
     from adaptivetensordf_amd import capi, native, phantom
 
@@ -106,6 +118,7 @@ def main():
     sp.max_samples_per_node = K
     fp = N.FieldParams()
     fp.C, fp.G, fp.H = C, G, H
+    fp.bf16_decoder = args.mode == "bf16"
     grid0 = phantom.init_grid(num_active, G, C, std=0.01, seed=0, device=dev)
     mlp0 = phantom.init_mlp(C, H, seed=0).to(dev)
     st = N.MakeTrainState(grid0, mlp0, args.mode == "bf16", 1e-3, 1e-3)
@@ -143,10 +156,10 @@ def main():
 
     grid_dtype = capi.GRID_BF16 if args.mode == "bf16" else capi.GRID_F32
     f = capi.Field(C, H, G, grid_dtype, 2.0, 0, st.grid.data_ptr(), st.grid.shape[0], tree.node_to_grid.data_ptr(),
-                   st.mlp.data_ptr())
+                   st.mlp.data_ptr(), capi.DECODER_BF16 if args.mode == "bf16" else capi.DECODER_F32, 0)
     csp = capi.Sampling(K, 256, 2, 0, 7, None)
     civ = capi.Intervals(iv.node.data_ptr(), iv.tmin.data_ptr(), iv.tmax.data_ptr(), iv.count.data_ptr(),
-                         iv.first.data_ptr(), iv.ray.data_ptr())
+                         iv.first.data_ptr(), iv.ray.data_ptr(), iv.num_intervals)
     packed = torch.empty(capi.lib().atdf_tree_packed_bytes(tree.NumNodes()), dtype=torch.uint8, device=dev)
     tdesc = capi.Tree(tree.NumNodes(), tree.node_position_min.data_ptr(), tree.node_position_max.data_ptr(),
                       tree.node_children.data_ptr(), tree.node_parent.data_ptr(), tree.node_active.data_ptr())
@@ -157,7 +170,7 @@ def main():
     dP = torch.randn(R, device=dev) * 1e-6
     gg = torch.zeros(st.grid.shape, dtype=torch.float32, device=dev)
     gm = torch.zeros_like(st.mlp)
-    ws = torch.empty(capi.lib().atdf_project_bwd_workspace_bytes(R, Ct.byref(f)), dtype=torch.uint8, device=dev)
+    ws = torch.empty(capi.lib().atdf_project_bwd_workspace_bytes(R, iv.num_intervals, Ct.byref(f)), dtype=torch.uint8, device=dev)
 
     def fwd():
         capi.call("atdf_project_fwd", capi.ptr(packed), tree.NumNodes(), capi.ptr(rays.origin),
@@ -196,7 +209,7 @@ def main():
     t_dom, bytes_dom, flop_dom = (t_bwd, bwd_bytes, bwd_flop) if dom == "bwd" else (t_fwd, fwd_bytes, fwd_flop)
     achieved_gbs = bytes_dom / t_dom / 1e9
     achieved_tflops = flop_dom / t_dom / 1e12
-    mfma_peak = BF16_PEAK_TFLOPS if args.mode == "bf16" and False else FP32_PEAK_TFLOPS
+    mfma_peak = BF16_PEAK_TFLOPS if args.mode == "bf16" else FP32_PEAK_TFLOPS
     log(f"project_fwd {t_fwd * 1e3:.2f} ms, project_bwd {t_bwd * 1e3:.2f} ms; dominant {dom}: "
         f"{achieved_gbs:.0f} GB/s algorithmic, {achieved_tflops:.1f} TFLOP/s decoder")
     # whole-step model of BASELINE.md: bytes/ray = 60 + S (16 C b + 32 C)

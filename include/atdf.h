@@ -97,6 +97,8 @@ typedef struct {
     int32_t* count; /* samples in this node                    */
     int32_t* first; /* index of its first sample within the ray */
     int32_t* ray;   /* owning ray                              */
+    int64_t total;  /* number of intervals (read by the adjoint; the write
+                       pass ignores it)                         */
 } atdf_intervals;
 
 /* The reference SampleList (structure/HelperStructs.h:231-297). */
@@ -195,12 +197,15 @@ atdf_status atdf_project_fwd(const void* packed_tree, int64_t num_nodes, const f
                              const int64_t* ray_interval_offset, const atdf_intervals* intervals,
                              const atdf_field* field, float* projection, atdf_stream_t stream);
 
-/* Adjoint of atdf_project_fwd with per-ray recomputation: given dL/dP per
+/* Adjoint of atdf_project_fwd with per-sample recomputation: given dL/dP per
  * ray, grad_grid (f32, channels-last, same shape as the grid) += dL/dgrid and
  * grad_mlp += dL/dmlp.  Replaces the autograd replay of
  * IndirectGridSample*::backward (modules/IndirectGridSample2D.cu:441-477),
- * the FCBlock backward and the index_add_ transpose. */
-size_t atdf_project_bwd_workspace_bytes(int64_t num_rays, const atdf_field* field);
+ * the FCBlock backward and the index_add_ transpose.  Work is node-major:
+ * intervals are counting-sorted by node and each node's grid gradient is
+ * accumulated in LDS and written once (no per-sample global atomics).
+ * intervals->total must hold the interval count. */
+size_t atdf_project_bwd_workspace_bytes(int64_t num_rays, int64_t num_intervals, const atdf_field* field);
 atdf_status atdf_project_bwd(const void* packed_tree, int64_t num_nodes, const float* ray_origin,
                              const float* ray_direction, int64_t num_rays, const atdf_sampling* sampling,
                              const int64_t* ray_interval_offset, const atdf_intervals* intervals,

@@ -108,3 +108,12 @@ def test_invalid_arguments_are_reported(cuda):
     with pytest.raises(capi.AtdfError) as e:
         capi.call("atdf_grid_sample3d_fwd", None, 0, 16, 11, 11, 11, None, None, 10, None, capi.stream_ptr())
     assert e.value.status == capi.ERR_INVALID_ARGUMENT
+
+
+def test_device_limits_for_node_major_adjoint(cuda):
+    """The node-major adjoint keeps a node's 11^3 x 16 f32 gradient in LDS."""
+    props = torch.cuda.get_device_properties(0)
+    lds = getattr(props, "shared_memory_per_block", None)
+    print(f"{props.name} {props.gcnArchName}: max LDS per workgroup {lds} bytes, {props.multi_processor_count} CUs",
+          flush=True)
+    assert lds is None or lds >= 150000

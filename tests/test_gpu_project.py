@@ -203,3 +203,46 @@ def test_bf16_grid_forward_error(oracle, cuda):
     e_bf16 = rel_close(P, P64, 1e-2)
     print(f"bf16 grids: rel err vs FP64(bf16 grid) {e_fp32:.2e}, vs FP64(f32 grid) {e_bf16:.2e}", flush=True)
     assert e_fp32 <= 1e-5 and e_bf16 <= 2e-2
+
+
+@pytest.mark.parametrize("C_,grid_bf16", [(16, False), (16, True), (8, True)])
+def test_bf16_decoder_matches_emulating_oracle(oracle, cuda, C_, grid_bf16):
+    """BF16 MFMA decoder (FP32 accumulation) against the oracle's FP64 path
+    with the same bf16 input rounding; agreement is limited only by
+    accumulation order and by values that land on a bf16 rounding boundary."""
+    from adaptivetensordf_amd import native
+
+    cs = make_case(oracle, C_=C_, seed=31 + C_, jitter_mode=2)
+    o_t, d_t = rays_to_device(cs["o"], cs["d"])
+    rays = native().RayList(o_t, d_t)
+    fp = cs["fp"]
+    fp.bf16_decoder = True
+    g = cs["grid"].cuda()
+    if grid_bf16:
+        g = g.to(torch.bfloat16)
+    grid_host = g.float().cpu().numpy()
+    gt = g.float().requires_grad_() if not grid_bf16 else g
+    mt = cs["mlp"].cuda().requires_grad_()
+    P = native().ProjectRays(cs["tree"], rays, gt if not grid_bf16 else g, mt, cs["sp"], fp)
+    fmb = oracle.FusedModel(cs["t"]["node_min"], cs["t"]["node_max"], cs["act"], grid_host, cs["mlp"].numpy(), C_,
+                            K=6, jitter_mode=2, seed=1234 + 31 + C_, decoder_bf16=True)
+    Pb = fmb.project(cs["o"], cs["d"], precision=1)
+    fm64 = oracle.FusedModel(cs["t"]["node_min"], cs["t"]["node_max"], cs["act"], grid_host, cs["mlp"].numpy(), C_,
+                             K=6, jitter_mode=2, seed=1234 + 31 + C_)
+    P64 = fm64.project(cs["o"], cs["d"], precision=1)
+    Pg = P.detach().cpu().double().numpy()
+    e_emul = rel_close(Pg, Pb, 1e-4)
+    e_64 = rel_close(Pg, P64, 1e-2)
+    print(f"C={C_} bf16 grid={grid_bf16}: rel err vs bf16-emulating FP64 {e_emul:.2e}, vs FP64 {e_64:.2e}", flush=True)
+    assert e_emul <= 2e-4 and e_64 <= 5e-2
+    dLdP = np.random.default_rng(C_).normal(size=P.shape[0])
+    wrt = [mt] if grid_bf16 else [gt, mt]
+    grads = torch.autograd.grad(P, wrt, torch.from_numpy(dLdP).float().cuda())
+    ref_g, ref_m = fmb.project_bwd(cs["o"], cs["d"], dLdP)
+    em = np.abs(grads[-1].cpu().double().numpy() - ref_m).max() / np.abs(ref_m).max()
+    print(f"  decoder grad rel err {em:.2e}", flush=True)
+    assert em <= 2e-3
+    if not grid_bf16:
+        eg = np.abs(grads[0].cpu().double().numpy() - ref_g).max() / np.abs(ref_g).max()
+        print(f"  grid grad rel err {eg:.2e}", flush=True)
+        assert eg <= 2e-3
