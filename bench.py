@@ -212,6 +212,7 @@ def main():
     mfma_peak = BF16_PEAK_TFLOPS if args.mode == "bf16" else FP32_PEAK_TFLOPS
     log(f"project_fwd {t_fwd * 1e3:.2f} ms, project_bwd {t_bwd * 1e3:.2f} ms; dominant {dom}: "
         f"{achieved_gbs:.0f} GB/s algorithmic, {achieved_tflops:.1f} TFLOP/s decoder")
+    traffic = measured_traffic(args.mode, f"project_{dom}")
     # whole-step model of BASELINE.md: bytes/ray = 60 + S (16 C b + 32 C)
     step_bytes_per_ray = 60 + s_eff * (16 * C * b + 32 * C)
 
@@ -247,13 +248,28 @@ def main():
         },
         "roofline": {
             "bound": "hbm", "kernel": f"project_{dom}", "achieved": achieved_gbs, "peak": HBM_PEAK_GBS,
-            "unit": "GB/s", "frac": achieved_gbs / HBM_PEAK_GBS, "traffic": None,
+            "unit": "GB/s", "frac": achieved_gbs / HBM_PEAK_GBS, "traffic": traffic,
             "mfma": {"achieved": achieved_tflops, "peak": mfma_peak, "unit": "TFLOP/s",
                      "frac": achieved_tflops / mfma_peak},
         },
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
+
+
+def measured_traffic(mode, kernel):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 PMC
+    summary of this workload (FETCH_SIZE / WRITE_SIZE, copy-calibrated), or
+    None when no summary for this mode is committed."""
+    path = os.path.join(HERE, "profiles", f"traffic_{'plain_fp32' if mode == 'plain' else 'bf16'}_512cube.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as fh:
+        data = json.load(fh)
+    for name, rec in data.get("kernels", {}).items():
+        if kernel + "_node_kernel" in name or (kernel + "_kernel") in name:
+            return rec["hbm_bytes_per_launch"]
+    return None
 
 
 def cpu_baseline(tree, active, st, grid0, mlp0, rays, tgt, K, C, G, seconds):
