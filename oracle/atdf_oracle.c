@@ -841,7 +841,7 @@ double atdfo_train_step_f32(atdfo_model* m, float* grid_mut, float* mlp_mut, int
 #endif
         for (int64_t r = 0; r < R; ++r) {
             int ns = ray_samples(m, r, ray_o + r * 3, ray_d + r * 3, hits, smp, ORACLE_SAMPLE_CAP);
-            if (ns <= 0) continue;
+            if (ns < 0) continue;
             float P = 0;
             for (int s = 0; s < ns; ++s) {
                 gather_cl_f(m, &smp[s], x);

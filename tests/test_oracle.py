@@ -388,3 +388,24 @@ def test_fused_backward_matches_torch_autograd(oracle):
     scale_m = np.abs(mt.grad.numpy()).max()
     assert np.abs(gg - gt.grad.numpy()).max() <= 1e-6 * scale_g
     assert np.abs(gm - mt.grad.numpy()).max() <= 1e-6 * scale_m
+
+
+def test_train_step_baseline_gradients(oracle):
+    """The FP32 CPU training step (timed baseline): loss = MSE of the FP32
+    projection, gradients = the FP64 adjoint with dL/dP = 2 (P - T) / R."""
+    t, act, grid, mlp, m = _small_model(oracle, C=8, depth=2, K=5, G=5)
+    o, d = random_rays(64, 14)
+    target = np.random.default_rng(2).uniform(0, 1, 64).astype(np.float32)
+    tb = oracle.TrainBaseline(m)
+    loss = tb.train_step(o, d, target, threads=2, do_adam=False)
+    P = m.project(o, d, precision=0)
+    assert abs(loss - np.mean((P - target) ** 2)) <= 1e-6 * loss
+    gg, gm = m.project_bwd(o, d, 2 * (P - target) / 64)
+    assert np.abs(tb.gg - gg).max() <= 1e-4 * np.abs(gg).max()
+    assert np.abs(tb.gm - gm).max() <= 1e-4 * np.abs(gm).max()
+    # with Adam: parameters move against the gradient sign on the first step
+    tb2 = oracle.TrainBaseline(m, lr=1e-3)
+    tb2.train_step(o, d, target, threads=2, do_adam=True)
+    moved = tb2.mlp - mlp
+    big = np.abs(gm) > 1e-3 * np.abs(gm).max()
+    assert np.all(np.sign(moved[big]) == -np.sign(gm[big]))
