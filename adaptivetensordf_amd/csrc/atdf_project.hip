@@ -687,7 +687,10 @@ __device__ __forceinline__ void store_partials(DecGrads& q, float* out, int g, i
 // a plain read-add-write when the chunk covers the whole node, a global
 // atomic add when the node was split over several chunks.
 // ---------------------------------------------------------------------------
-constexpr int kChunk = 2048;
+#ifndef ATDF_NODE_CHUNK
+#define ATDF_NODE_CHUNK 512
+#endif
+constexpr int kChunk = ATDF_NODE_CHUNK;  // intervals per node chunk
 
 struct NodeBwdArgs {
     ProjArgs a;
@@ -716,6 +719,7 @@ __global__ __launch_bounds__(64 * kWaves) void project_bwd_node_kernel(NodeBwdAr
 {
     extern __shared__ __attribute__((aligned(16))) unsigned char dsm[];
     __shared__ int s_chunk;
+    __shared__ int s_tiles[kWaves];
     const ProjArgs& a = b.a;
     constexpr int CPL = C / 4;
     constexpr int NP = mlp_numel<C>();
@@ -761,9 +765,17 @@ __global__ __launch_bounds__(64 * kWaves) void project_bwd_node_kernel(NodeBwdAr
 #pragma unroll
         for (int off = 32; off > 0; off >>= 1) tot += __shfl_xor(tot, off, 64);
 
+        // every wave runs the same number of tiles: the scatter below is phased
+        if (lane == 0) s_tiles[wl] = (tot + kTile - 1) / kTile;
+        __syncthreads();
+        int ntiles = 0;
+#pragma unroll
+        for (int w = 0; w < kWaves; ++w) ntiles = max(ntiles, s_tiles[w]);
+
         int cur = ib, cum = 0;
         int cur_cnt = (ib < ie) ? a.iv_count[b.sorted[ib]] : 0;
-        for (int base = 0; base < tot; base += kTile) {
+        for (int ti = 0; ti < ntiles; ++ti) {
+            const int base = ti * kTile;
             const int t = base + j;
             SampleCtx sc;
             sc.valid = t < tot;
@@ -799,30 +811,85 @@ __global__ __launch_bounds__(64 * kWaves) void project_bwd_node_kernel(NodeBwdAr
                 y = decoder_fwd<C>(s, x, g, j, h1, h2);
             const float dy = sc.valid ? dp * sc.w * softplus_grad_f(y, a.beta) : 0.f;
             v4f dx = tile_backward<C, kDecBF16>(s, sb, bufA, bufB, bufAb, bufBb, x, h1, h2, dy, g, j, q);
-            if (sc.valid && sc.gbase >= 0 && 4 * g < C && dy != 0.f) {
+
+            // Scatter w_corner * dX into the LDS node gradient.  The waves take
+            // turns (one phase each), so no two waves touch it at once; inside
+            // a wave one instruction can only collide between samples of the
+            // same floor cell, so samples with a unique cell in the tile use a
+            // plain float4 read-add-write and the rest (shared cell, or a
+            // clamped corner at the grid border) use LDS atomics.
+            const bool live = sc.valid && sc.gbase >= 0 && dy != 0.f;
+            const bool border = sc.tri.x0 == sc.tri.x1 || sc.tri.y0 == sc.tri.y1 || sc.tri.z0 == sc.tri.z1;
+            const int key = live ? (border ? -1 : (sc.tri.z0 * G + sc.tri.y0) * G + sc.tri.x0) : -2 - j;
+            bool dup = false;
 #pragma unroll
-                for (int k = 0; k < 8; ++k) {
-                    float* gp = lgrad + corner_cell(sc.tri, k, G) * C + 4 * g;
-                    const float wk = sc.tri.w[k];
+            for (int off = 1; off < kTile; ++off) {
+                const int other = __shfl(key, (lane & 48) | ((j + off) & 15), 64);
+                dup |= (other == key);
+            }
+            const bool plain = live && !border && !dup && 4 * g < C;
+            const bool atomic = live && (border || dup) && 4 * g < C;
+#pragma unroll 1
+            for (int p = 0; p < kWaves; ++p) {
+                if (wl == p) {
+                    if (plain) {
 #pragma unroll
-                    for (int r4 = 0; r4 < 4; ++r4) atomicAdd(gp + r4, wk * dx[r4]);
+                        for (int k = 0; k < 8; ++k) {
+                            float4* gp = reinterpret_cast<float4*>(lgrad + corner_cell(sc.tri, k, G) * C + 4 * g);
+                            const float wk = sc.tri.w[k];
+                            float4 v = *gp;
+                            v.x += wk * dx[0];
+                            v.y += wk * dx[1];
+                            v.z += wk * dx[2];
+                            v.w += wk * dx[3];
+                            *gp = v;
+                        }
+                    }
+                    wave_lds_sync();
+                    if (atomic) {
+#pragma unroll
+                        for (int k = 0; k < 8; ++k) {
+                            float* gp = lgrad + corner_cell(sc.tri, k, G) * C + 4 * g;
+                            const float wk = sc.tri.w[k];
+#pragma unroll
+                            for (int r4 = 0; r4 < 4; ++r4) atomicAdd(gp + r4, wk * dx[r4]);
+                        }
+                    }
                 }
+                __syncthreads();
             }
         }
         __syncthreads();
         // flush the node gradient once, clear LDS for the next chunk
-        float* gg = b.grad_grid + gbase;
-        if (ch.w) {
-            for (int i = threadIdx.x; i < gsz; i += blockDim.x) {
-                float v = lgrad[i];
-                lgrad[i] = 0.f;
-                if (v != 0.f) unsafeAtomicAdd(gg + i, v);
+        // (float4 accesses, four in flight per thread; G^3 C is a multiple of 4 for C in {8, 16})
+        float4* gg4 = reinterpret_cast<float4*>(b.grad_grid + gbase);
+        float4* lg4 = reinterpret_cast<float4*>(lgrad);
+        const int n4 = gsz >> 2;
+        const float4 z4 = make_float4(0.f, 0.f, 0.f, 0.f);
+        for (int i0 = threadIdx.x; i0 < n4; i0 += 4 * blockDim.x) {
+            float4 v[4], o[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const int i = i0 + u * blockDim.x;
+                if (i < n4) {
+                    v[u] = lg4[i];
+                    lg4[i] = z4;
+                    if (!ch.w) o[u] = gg4[i];
+                }
             }
-        } else {
-            for (int i = threadIdx.x; i < gsz; i += blockDim.x) {
-                float v = lgrad[i];
-                lgrad[i] = 0.f;
-                if (v != 0.f) gg[i] += v;
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const int i = i0 + u * blockDim.x;
+                if (i >= n4) continue;
+                if (ch.w) {
+                    float* dst = reinterpret_cast<float*>(gg4 + i);
+                    if (v[u].x != 0.f) unsafeAtomicAdd(dst + 0, v[u].x);
+                    if (v[u].y != 0.f) unsafeAtomicAdd(dst + 1, v[u].y);
+                    if (v[u].z != 0.f) unsafeAtomicAdd(dst + 2, v[u].z);
+                    if (v[u].w != 0.f) unsafeAtomicAdd(dst + 3, v[u].w);
+                } else {
+                    gg4[i] = make_float4(o[u].x + v[u].x, o[u].y + v[u].y, o[u].z + v[u].z, o[u].w + v[u].w);
+                }
             }
         }
         __syncthreads();
