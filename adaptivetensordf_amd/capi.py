@@ -81,6 +81,8 @@ _SIGNATURES = {
     "atdf_project_bwd_workspace_bytes": (_SZ, [_I64, _I64, C.POINTER(Field)]),
     "atdf_project_bwd": (_I32, [_P, _I64, _P, _P, _I64, C.POINTER(Sampling), _P, C.POINTER(Intervals),
                                 C.POINTER(Field), _P, _P, _P, _P, _SZ, _P]),
+    "atdf_node_density_stats": (_I32, [_P, _I64, _P, _I64, C.POINTER(Field), _I32, _I32, _I32, _U64, _P, _P, _P,
+                                        _P]),
     "atdf_mse_workspace_bytes": (_SZ, [_I64]),
     "atdf_mse_loss": (_I32, [_P, _P, _I64, _F, _P, _P, _P, _SZ, _P]),
     "atdf_adam_step": (_I32, [_P, _P, _P, _P, _I64, _F, _F, _F, _F, _I64, _I32, _P]),

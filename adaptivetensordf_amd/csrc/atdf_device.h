@@ -98,6 +98,14 @@ __device__ __forceinline__ float hash_jitter(uint64_t seed, int64_t ray, int64_t
     return (float)(h >> 40) * (1.0f / 16777216.0f);
 }
 
+// Uniform in [0,1) for point `point` of node `node` (random node samples).
+__device__ __forceinline__ float point_uniform(uint64_t seed, int64_t node, int64_t point, int axis)
+{
+    uint64_t key = ((uint64_t)node << 34) ^ ((uint64_t)point << 2) ^ (uint64_t)axis;
+    uint64_t h = splitmix64(seed ^ splitmix64(key ^ 0x5bd1e995ull));
+    return (float)(h >> 40) * (1.0f / 16777216.0f);
+}
+
 // t of sample j in [tmin, tmax] split into n strata (structure/HyperTree.cu:466-474).
 __device__ __forceinline__ float sample_t(float tmin, float step, int j, float a)
 {

@@ -951,6 +951,118 @@ __global__ __launch_bounds__(256) void chunk_desc_kernel(const int64_t* __restri
     }
 }
 
+// ---------------------------------------------------------------------------
+// Per-node density statistics: one workgroup per active node, tiles of 16
+// points per wave, FP64 sum / sum of squares and FP32 max reduced in a fixed
+// order (lane, then wave).
+// ---------------------------------------------------------------------------
+struct StatsArgs {
+    ProjArgs a;
+    const int64_t* active_ids;
+    int mode, n, rounds;
+    uint64_t seed;
+    float* node_max;
+    double* node_sum;
+    double* node_sumsq;
+};
+
+template <int C, bool kBF16, bool kDecBF16>
+__global__ __launch_bounds__(64 * kWaves) void node_stats_kernel(StatsArgs sa)
+{
+    using SM = DecoderSmem<C, kDecBF16>;
+    constexpr int CPL = C / 4;
+    __shared__ __attribute__((aligned(16))) float s[SM::kF32];
+    __shared__ __attribute__((aligned(16))) __bf16 sb[SM::kBF16];
+    __shared__ double s_sum[kWaves], s_sq[kWaves];
+    __shared__ float s_max[kWaves];
+    const ProjArgs& a = sa.a;
+    if constexpr (kDecBF16)
+        load_mlp_bf16_to_lds<C>(a.mlp, sb, s);
+    else
+        load_mlp_to_lds<C>(a.mlp, s);
+    __syncthreads();
+
+    const int lane = threadIdx.x & 63;
+    const int g = lane >> 4, j = lane & 15;
+    const int wl = threadIdx.x >> 6;
+    const int64_t act = blockIdx.x;
+    const int node = (int)sa.active_ids[act];
+    const PNode nd = a.nodes[node];
+    float size[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) size[k] = nd.mx[k] - nd.mn[k];
+    const int n = sa.n;
+    const int64_t per = (int64_t)n * n * n * (sa.mode == 1 ? sa.rounds : 1);
+    const int slot = a.node_to_grid[node];
+
+    float dmax = -INFINITY;
+    double s1 = 0.0, s2 = 0.0;
+    for (int64_t base = (int64_t)wl * kTile; base < per; base += (int64_t)kWaves * kTile) {
+        const int64_t p = base + j;
+        SampleCtx sc;
+        sc.valid = p < per;
+        sc.w = 1.f;
+        sc.gbase = -1;
+        if (sc.valid) {
+            float pos[3];
+            if (sa.mode == 0) {
+                const int64_t idx[3] = {p % n, (p / n) % n, p / ((int64_t)n * n)};
+#pragma unroll
+                for (int k = 0; k < 3; ++k) {
+                    float lp = (float)idx[k] / (float)(n - 1);
+                    pos[k] = lp * size[k] + nd.mn[k];
+                }
+            } else {
+#pragma unroll
+                for (int k = 0; k < 3; ++k) pos[k] = nd.mn[k] + point_uniform(sa.seed, node, p, k) * size[k];
+            }
+            float loc[3];
+#pragma unroll
+            for (int k = 0; k < 3; ++k) loc[k] = ((pos[k] - nd.mn[k]) / size[k]) * 2 - 1;
+            tri_setup(loc[0], loc[1], loc[2], a.G, a.G, a.G, sc.tri);
+            if (slot >= 0) sc.gbase = (int64_t)slot * a.G * a.G * a.G * C;
+        }
+        float x[CPL];
+        gather_channels<C, kBF16>(a, sc, CPL * g, x);
+        v4f h1[4], h2[4];
+        float y;
+        if constexpr (kDecBF16)
+            y = decoder_fwd_bf16<C>(sb, s, x, g, j, h1, h2);
+        else
+            y = decoder_fwd<C>(s, x, g, j, h1, h2);
+        if (sc.valid && g == 0) {
+            const float dn = softplus_f(y, a.beta);
+            dmax = fmaxf(dmax, dn);
+            s1 += (double)dn;
+            s2 += (double)dn * (double)dn;
+        }
+    }
+#pragma unroll
+    for (int off = 1; off < 16; off <<= 1) {
+        s1 += __shfl_xor(s1, off, 64);
+        s2 += __shfl_xor(s2, off, 64);
+        dmax = fmaxf(dmax, __shfl_xor(dmax, off, 64));
+    }
+    if (lane == 0) {
+        s_sum[wl] = s1;
+        s_sq[wl] = s2;
+        s_max[wl] = dmax;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t1 = 0.0, t2 = 0.0;
+        float m = -INFINITY;
+        for (int w = 0; w < kWaves; ++w) {
+            t1 += s_sum[w];
+            t2 += s_sq[w];
+            m = fmaxf(m, s_max[w]);
+        }
+        sa.node_max[act] = m;
+        sa.node_sum[act] = t1;
+        sa.node_sumsq[act] = t2;
+    }
+}
+
 // grad_mlp[i] += sum_w ws[w][i], summed in wave order.
 __global__ __launch_bounds__(256) void reduce_partials_kernel(const float* __restrict__ ws, int64_t nwaves, int np,
                                                               float* __restrict__ grad)
@@ -1206,6 +1318,48 @@ atdf_status atdf_project_bwd(const void* packed_tree, int64_t num_nodes, const f
     hipLaunchKernelGGL(reduce_partials_kernel, dim3((unsigned)div_up(np, 256)), dim3(256), 0, s, partials,
                        (int64_t)persistent_blocks() * kWaves, np, grad_mlp);
     return check_launch("atdf_project_bwd");
+}
+
+
+atdf_status atdf_node_density_stats(const void* packed_tree, int64_t num_nodes, const int64_t* active_ids,
+                                    int64_t num_active, const atdf_field* f, int32_t mode, int32_t n, int32_t rounds,
+                                    uint64_t seed, float* node_max, double* node_sum, double* node_sumsq,
+                                    atdf_stream_t stream)
+{
+    ATDF_REQUIRE(packed_tree && num_nodes > 0 && f, "atdf_node_density_stats: empty tree or field");
+    ATDF_REQUIRE(f->H == 64 && (f->C == 16 || f->C == 8), "atdf_node_density_stats: field shape not built");
+    ATDF_REQUIRE(f->grid && f->node_to_grid && f->mlp, "atdf_node_density_stats: incomplete field");
+    ATDF_REQUIRE(mode == 0 || mode == 1, "atdf_node_density_stats: mode must be 0 (lattice) or 1 (random)");
+    ATDF_REQUIRE(mode == 1 || n >= 2, "atdf_node_density_stats: lattice needs n >= 2");
+    ATDF_REQUIRE(n >= 1 && (mode == 0 || rounds >= 1), "atdf_node_density_stats: bad point count");
+    if (num_active == 0) return ATDF_OK;
+    ATDF_REQUIRE(active_ids && node_max && node_sum && node_sumsq, "atdf_node_density_stats: null buffer");
+    StatsArgs sa{};
+    atdf_sampling sp{};
+    atdf_intervals none{};
+    sa.a = make_args(packed_tree, nullptr, nullptr, 0, &sp, nullptr, &none, f);
+    sa.active_ids = active_ids;
+    sa.mode = mode;
+    sa.n = n;
+    sa.rounds = rounds;
+    sa.seed = seed;
+    sa.node_max = node_max;
+    sa.node_sum = node_sum;
+    sa.node_sumsq = node_sumsq;
+    dim3 grid((unsigned)num_active), block(64 * kWaves);
+    hipStream_t s = to_hip(stream);
+    const bool bf = f->grid_dtype == ATDF_GRID_BF16;
+    const bool bd = f->decoder_dtype == ATDF_DECODER_BF16;
+#define ATDF_ST(CC, BF, BD) hipLaunchKernelGGL((node_stats_kernel<CC, BF, BD>), grid, block, 0, s, sa)
+    if (f->C == 16) {
+        if (bf) { if (bd) ATDF_ST(16, true, true); else ATDF_ST(16, true, false); }
+        else    { if (bd) ATDF_ST(16, false, true); else ATDF_ST(16, false, false); }
+    } else {
+        if (bf) { if (bd) ATDF_ST(8, true, true); else ATDF_ST(8, true, false); }
+        else    { if (bd) ATDF_ST(8, false, true); else ATDF_ST(8, false, false); }
+    }
+#undef ATDF_ST
+    return check_launch("atdf_node_density_stats");
 }
 
 }  // extern "C"

@@ -90,6 +90,8 @@ PYBIND11_MODULE(TORCH_EXTENSION_NAME, m)
     m.def("IntegrateSamplesXRay", &IntegrateSamplesXRay);
     m.def("AccumulateSampleLossPerNode", &AccumulateSampleLossPerNode);
     m.def("ProjectRays", &ProjectRays);
+    m.def("ComputeMaxDensity", &ComputeMaxDensity);
+    m.def("ComputeStd", &ComputeStd);
 
     py::class_<TrainState>(m, "TrainState")
         .def_readwrite("grid", &TrainState::grid)

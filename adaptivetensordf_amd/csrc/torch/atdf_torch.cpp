@@ -607,6 +607,45 @@ Tensor ProjectRays(HyperTreeBase& tree, const RayList& rays, Tensor grid, Tensor
     return ProjectRaysFn::apply(grid, mlp, rays.origin, rays.direction, &tree, sp, fp);
 }
 
+namespace {
+std::tuple<Tensor, Tensor, Tensor> node_stats(HyperTreeBase& tree, const Tensor& grid, const Tensor& mlp,
+                                              const FieldParams& fp, int mode, int n, int rounds, uint64_t seed)
+{
+    atdf_field f = make_field(tree, grid, mlp, fp);
+    const int64_t NA = tree.NumActiveNodes();
+    auto dev = grid.device();
+    auto mx = torch::empty({NA}, torch::TensorOptions().dtype(torch::kFloat32).device(dev));
+    auto s1 = torch::empty({NA}, torch::TensorOptions().dtype(torch::kFloat64).device(dev));
+    auto s2 = torch::empty({NA}, torch::TensorOptions().dtype(torch::kFloat64).device(dev));
+    auto ids = tree.active_node_ids.contiguous();
+    check_status(atdf_node_density_stats(tree.packed(), tree.NumNodes(), ids.data_ptr<int64_t>(), NA, &f, mode, n,
+                                         rounds, seed, mx.data_ptr<float>(), s1.data_ptr<double>(),
+                                         s2.data_ptr<double>(), current_stream()),
+                 "atdf_node_density_stats");
+    return {mx, s1, s2};
+}
+}  // namespace
+
+void ComputeMaxDensity(HyperTreeBase& tree, Tensor grid, Tensor mlp, const FieldParams& fp, int n)
+{
+    auto [mx, s1, s2] = node_stats(tree, grid, mlp, fp, 0, n, 1, 0);
+    (void)s1;
+    (void)s2;
+    tree.node_max_density = tree.node_max_density.to(mx.device()).index_copy(0, tree.active_node_ids, mx);
+}
+
+Tensor ComputeStd(HyperTreeBase& tree, Tensor grid, Tensor mlp, const FieldParams& fp, int n, int rounds,
+                  uint64_t seed)
+{
+    auto [mx, s1, s2] = node_stats(tree, grid, mlp, fp, 1, n, rounds, seed);
+    (void)mx;
+    const double cnt = (double)n * n * n * rounds;
+    auto var = (s2 - s1 * s1 / cnt) / (cnt - 1.0);
+    auto sd = torch::sqrt(torch::clamp_min(var, 0.0));
+    auto out = torch::zeros({tree.NumNodes()}, sd.options());
+    return out.index_copy(0, tree.active_node_ids, sd);
+}
+
 TrainState MakeTrainState(Tensor grid, Tensor mlp, bool bf16_grid, float lr_grid, float lr_mlp)
 {
     require_device_f32(grid, "grid");

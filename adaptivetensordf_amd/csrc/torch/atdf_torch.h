@@ -132,6 +132,15 @@ struct FieldParams {
 torch::Tensor ProjectRays(HyperTreeBase& tree, const RayList& rays, torch::Tensor grid, torch::Tensor mlp,
                           const SamplingParams& sp, const FieldParams& fp);
 
+// ComputeMaxDensity (samples/hyper_train.cpp:662-707): node_max_density of the
+// active nodes = max density on an n^3 lattice per node (UniformGlobalSamples).
+void ComputeMaxDensity(HyperTreeBase& tree, torch::Tensor grid, torch::Tensor mlp, const FieldParams& fp, int n);
+// ComputeStd (samples/hyper_train.cpp:709-759): per-node standard deviation
+// (correction 1) of the density at rounds x n^3 uniform random points of each
+// active node, reduced in FP64.  Returns [num_nodes] float64, 0 for inactive.
+torch::Tensor ComputeStd(HyperTreeBase& tree, torch::Tensor grid, torch::Tensor mlp, const FieldParams& fp, int n,
+                         int rounds, uint64_t seed);
+
 // One training step of the plain method entirely on the device:
 // intervals -> fused forward -> MSE -> fused adjoint -> Adam on grid and mlp.
 // bf16 grids keep an f32 master copy (grid_master) and a bf16 copy (grid).

@@ -212,6 +212,19 @@ atdf_status atdf_project_bwd(const void* packed_tree, int64_t num_nodes, const f
                              const atdf_field* field, const float* dLdP, float* grad_grid, float* grad_mlp,
                              void* workspace, size_t workspace_bytes, atdf_stream_t stream);
 
+/* Per-node density statistics for the octree optimizer, reduced in FP64:
+ * for active node a (node id active_ids[a]) the field's density is evaluated
+ *   mode 0: on the n^3 lattice of UniformGlobalSamples (structure/HyperTree.cu:1618-1653),
+ *           as ComputeMaxDensity does (samples/hyper_train.cpp:662-707);
+ *   mode 1: at rounds x n^3 uniform points of CreateSamplesRandomly
+ *           (structure/HyperTree.cu:887-920), as ComputeStd does (hyper_train.cpp:709-759);
+ * node_max[a] = max density (f32), node_sum[a], node_sumsq[a] = sum and sum
+ * of squares (f64).  One workgroup per node, no atomics (deterministic). */
+atdf_status atdf_node_density_stats(const void* packed_tree, int64_t num_nodes, const int64_t* active_ids,
+                                    int64_t num_active, const atdf_field* field, int32_t mode, int32_t n,
+                                    int32_t rounds, uint64_t seed, float* node_max, double* node_sum,
+                                    double* node_sumsq, atdf_stream_t stream);
+
 /* ======================================================================= */
 /* Loss, optimizer, layout                                                  */
 /* ======================================================================= */

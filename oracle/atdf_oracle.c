@@ -789,6 +789,73 @@ int atdfo_project_bwd(const atdfo_model* m, const float* ray_o, const float* ray
 }
 
 /* ------------------------------------------------------------------------ */
+/* per-node density statistics                                               */
+/* ------------------------------------------------------------------------ */
+float atdfo_point_uniform(uint64_t seed, int64_t node, int64_t point, int axis)
+{
+    uint64_t key = ((uint64_t)node << 34) ^ ((uint64_t)point << 2) ^ (uint64_t)axis;
+    uint64_t h = splitmix64(seed ^ splitmix64(key ^ 0x5bd1e995ull));
+    return (float)(h >> 40) * (1.0f / 16777216.0f);
+}
+
+void atdfo_node_density_stats(const atdfo_model* m, int mode, int n, int rounds, uint64_t seed, int precision,
+                              float* node_max, double* node_sum, double* node_sumsq, float* densities)
+{
+    int C = m->C, H = m->H;
+    int64_t per = (int64_t)n * n * n * (mode == 1 ? rounds : 1);
+    float* featf = (float*)malloc(sizeof(float) * (C + 2 * H));
+    double* featd = (double*)malloc(sizeof(double) * (C + 2 * H));
+    for (int64_t a = 0; a < m->num_active; ++a) {
+        int64_t node = m->active_ids[a];
+        const float* mn = m->node_min + node * 3;
+        const float* mx = m->node_max + node * 3;
+        float size[3];
+        for (int k = 0; k < 3; ++k) size[k] = mx[k] - mn[k];
+        float dmax = -INFINITY;
+        double s1 = 0, s2 = 0;
+        for (int64_t p = 0; p < per; ++p) {
+            sample_t_ smp;
+            smp.node = (int32_t)node;
+            if (mode == 0) {
+                int64_t idx[3] = {p % n, (p / n) % n, p / ((int64_t)n * n)};
+                for (int k = 0; k < 3; ++k) {
+                    float lp = (float)idx[k] / (float)(n - 1);
+                    smp.pos[k] = lp * size[k] + mn[k];
+                }
+            } else {
+                for (int k = 0; k < 3; ++k) smp.pos[k] = mn[k] + atdfo_point_uniform(seed, node, p, k) * size[k];
+            }
+            float dn;
+            if (precision == 0) {
+                gather_cl_f(m, &smp, featf);
+                dn = mlp_fwd_one_f(m->mlp, C, H, m->beta, featf, featf + C, featf + C + H, NULL);
+            } else {
+                tri_d t;
+                int64_t base;
+                double yp;
+                if (m->decoder_bf16) {
+                    gather_cl_f(m, &smp, featf);
+                    for (int c = 0; c < C; ++c) featd[c] = featf[c];
+                } else {
+                    gather_cl_d(m, &smp, featd, &t, &base);
+                }
+                dn = (float)mlp_fwd_one_d(m->mlp, C, H, m->beta, featd, featd + C, featd + C + H, &yp,
+                                          m->decoder_bf16);
+            }
+            if (densities) densities[a * per + p] = dn;
+            if (dn > dmax) dmax = dn;
+            s1 += (double)dn;
+            s2 += (double)dn * (double)dn;
+        }
+        node_max[a] = dmax;
+        node_sum[a] = s1;
+        node_sumsq[a] = s2;
+    }
+    free(featf);
+    free(featd);
+}
+
+/* ------------------------------------------------------------------------ */
 /* FP32 CPU training step (timed baseline)                                   */
 /* ------------------------------------------------------------------------ */
 static void adam_f32(float* p, float* g, float* mm, float* vv, int64_t n, int step, float lr)

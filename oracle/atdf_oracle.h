@@ -127,6 +127,19 @@ int atdfo_project_fwd(const atdfo_model* m, const float* ray_o, const float* ray
 int atdfo_project_bwd(const atdfo_model* m, const float* ray_o, const float* ray_d, int64_t R,
                       const double* dLdP, double* grad_grid, double* grad_mlp);
 
+/* ---- per-node density statistics (FP64 reductions) -----------------------
+ * For active node a (id active_ids[a]) evaluate the field's density at
+ *   mode 0: the n^3 lattice of UniformGlobalSamples (HyperTree.cu:1618-1653):
+ *           point (i,j,k) = (idx / (n-1)) * size + min, x fastest;
+ *   mode 1: rounds x n^3 uniform points of CreateSamplesRandomly
+ *           (HyperTree.cu:887-920): min + u * size, u = atdfo_point_uniform.
+ * and reduce max (FP32), sum and sum of squares (FP64).  Densities are the
+ * FP32 path (precision 0) or the FP64 path (precision 1).  Feeds
+ * ComputeMaxDensity (hyper_train.cpp:662-707) and ComputeStd (709-759). */
+float atdfo_point_uniform(uint64_t seed, int64_t node, int64_t point, int axis);
+void atdfo_node_density_stats(const atdfo_model* m, int mode, int n, int rounds, uint64_t seed, int precision,
+                              float* node_max, double* node_sum, double* node_sumsq, float* densities);
+
 /* FP32 training-step baseline (reference structure on the CPU, OpenMP over
  * rays): fwd, MSE grad, bwd with FP32 accumulation, then (do_adam) Adam on
  * grid + mlp.

@@ -231,6 +231,25 @@ class FusedModel:
         return gg, gm
 
 
+def point_uniform(seed, node, point, axis):
+    f = lib().atdfo_point_uniform
+    f.restype = C.c_float
+    f.argtypes = [C.c_uint64, C.c_int64, C.c_int64, C.c_int]
+    return f(seed, node, point, axis)
+
+
+def node_density_stats(fm, mode, n, rounds=1, seed=0, precision=0, want_densities=False):
+    na = fm.active_ids.shape[0]
+    per = n ** 3 * (rounds if mode == 1 else 1)
+    mx = np.zeros(na, np.float32)
+    s1 = np.zeros(na, np.float64)
+    s2 = np.zeros(na, np.float64)
+    dens = np.zeros(na * per, np.float32) if want_densities else None
+    lib().atdfo_node_density_stats(C.byref(fm.m), C.c_int(mode), C.c_int(n), C.c_int(rounds), C.c_uint64(seed),
+                                   C.c_int(precision), _p(mx), _p(s1), _p(s2), _p(dens))
+    return (mx, s1, s2, dens.reshape(na, per) if want_densities else None)
+
+
 class TrainBaseline:
     """FP32 CPU training step (the timed CPU baseline)."""
 

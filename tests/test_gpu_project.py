@@ -246,3 +246,34 @@ def test_bf16_decoder_matches_emulating_oracle(oracle, cuda, C_, grid_bf16):
         eg = np.abs(grads[0].cpu().double().numpy() - ref_g).max() / np.abs(ref_g).max()
         print(f"  grid grad rel err {eg:.2e}", flush=True)
         assert eg <= 2e-3
+
+
+@pytest.mark.parametrize("bf16", [False, True])
+def test_node_density_stats_match_oracle(oracle, cuda, bf16):
+    """ComputeMaxDensity (20^3 lattice) and ComputeStd (rounds x 16^3 random
+    points) per active node, FP64 reductions, against the oracle."""
+    import math
+
+    from adaptivetensordf_amd import capi, native
+
+    cs = make_case(oracle, C_=16, seed=41, budget=200)
+    fp = cs["fp"]
+    fp.bf16_decoder = bf16
+    grid = cs["grid"].cuda()
+    mlp = cs["mlp"].cuda()
+    tree = cs["tree"]
+    native().ComputeMaxDensity(tree, grid, mlp, fp, 20)
+    gmax = tree.node_max_density.cpu().numpy()[oracle.active_ids(cs["act"])]
+    fm = oracle.FusedModel(cs["t"]["node_min"], cs["t"]["node_max"], cs["act"], cs["grid"].numpy(),
+                           cs["mlp"].numpy(), 16, decoder_bf16=bf16)
+    omax, _, _, _ = oracle.node_density_stats(fm, 0, 20, precision=1 if bf16 else 0)
+    e_max = np.abs(gmax - omax).max() / np.abs(omax).max()
+    sd = native().ComputeStd(tree, grid, mlp, fp, 16, 3, 77).cpu().numpy()
+    _, s1, s2, _ = oracle.node_density_stats(fm, 1, 16, rounds=3, seed=77, precision=1 if bf16 else 0)
+    cnt = 16 ** 3 * 3
+    osd = np.sqrt(np.maximum((s2 - s1 ** 2 / cnt) / (cnt - 1), 0))
+    gsd = sd[oracle.active_ids(cs["act"])]
+    e_sd = np.abs(gsd - osd).max() / np.abs(osd).max()
+    print(f"bf16={bf16}: node max rel err {e_max:.2e}, node std rel err {e_sd:.2e}", flush=True)
+    assert sd.dtype == np.float64 and np.all(sd[cs["act"] == 0] == 0)
+    assert e_max <= (1e-3 if bf16 else 1e-5) and e_sd <= (1e-2 if bf16 else 1e-4)

@@ -409,3 +409,31 @@ def test_train_step_baseline_gradients(oracle):
     moved = tb2.mlp - mlp
     big = np.abs(gm) > 1e-3 * np.abs(gm).max()
     assert np.all(np.sign(moved[big]) == -np.sign(gm[big]))
+
+
+def test_node_density_stats(oracle):
+    """Lattice positions, random points inside the boxes, FP64 sums against
+    numpy over the returned per-point densities, constant-field closed form."""
+    t, act, grid, mlp, m = _small_model(oracle, C=8, depth=2, K=5, G=5)
+    mx, s1, s2, dens = oracle.node_density_stats(m, 0, 4, want_densities=True)
+    assert dens.shape == (64, 64)
+    assert np.allclose(mx, dens.max(1)) and np.allclose(s1, dens.astype(np.float64).sum(1), rtol=1e-12)
+    assert np.allclose(s2, (dens.astype(np.float64) ** 2).sum(1), rtol=1e-12)
+    # the lattice covers the node corners: density at the min corner equals a direct gather at local (-1,-1,-1)
+    ids = oracle.active_ids(act)
+    _, d0 = oracle.mlp_fwd(mlp, 8, 64, 2.0, grid[0, 0, 0, 0][None, :])
+    assert abs(dens[0, 0] - d0[0]) <= 1e-6
+    # random points: uniforms in [0,1) with mean ~0.5
+    u = np.array([oracle.point_uniform(3, int(ids[0]), p, k) for p in range(2000) for k in range(3)])
+    assert u.min() >= 0 and u.max() < 1 and abs(u.mean() - 0.5) < 0.02
+    mx, s1, s2, dens = oracle.node_density_stats(m, 1, 3, rounds=5, seed=3, want_densities=True)
+    sd = np.sqrt((s2 - s1 ** 2 / 135) / 134)
+    assert np.allclose(sd, dens.astype(np.float64).std(1, ddof=1), rtol=1e-6)
+    # constant field: max = mean = softplus_2(c), std = 0
+    mlp2 = mlp.copy()
+    mlp2[-65:-1] = 0.0
+    mlp2[-1] = 0.25
+    m2 = oracle.FusedModel(t["node_min"], t["node_max"], act, grid, mlp2, 8, K=5)
+    mx, s1, s2, _ = oracle.node_density_stats(m2, 1, 3, rounds=2, seed=1, precision=1)
+    c = math.log1p(math.exp(0.5)) / 2
+    assert np.allclose(mx, c, rtol=1e-6) and np.allclose(s1 / 54, c, rtol=1e-6)
