@@ -219,6 +219,29 @@ def init_mlp(C: int, H: int = 64, seed: int = 0):
     return torch.cat(parts)
 
 
+PROPOSAL_BINS, PROPOSAL_INPUTS, PROPOSAL_HIDDEN = 32, 38, 64
+
+
+def proposal_num_params() -> int:
+    n, h, b = PROPOSAL_INPUTS, PROPOSAL_HIDDEN, PROPOSAL_BINS
+    return h * n + h + 2 * (h * h + h) + b * h + b
+
+
+def init_proposal(seed: int = 0):
+    """Proposal MLP parameters [W1, b1, W2, b2, W3, b3, W4, b4] for
+    38 -> 64 -> 64 -> 64 -> 32, initialised like the decoder."""
+    import torch
+
+    g = torch.Generator().manual_seed(seed)
+    n, h, b = PROPOSAL_INPUTS, PROPOSAL_HIDDEN, PROPOSAL_BINS
+    parts = []
+    for fan_in, fan_out in ((n, h), (h, h), (h, h), (h, b)):
+        w = torch.randn(fan_out, fan_in, generator=g, dtype=torch.float32) * math.sqrt(2.0 / fan_in)
+        bb = (torch.rand(fan_out, generator=g, dtype=torch.float32) * 2 - 1) / math.sqrt(fan_in)
+        parts += [w.reshape(-1), bb]
+    return torch.cat(parts)
+
+
 def init_grid(num_grids: int, G: int, C: int, std: float = 0.01, seed: int = 0, device="cpu"):
     """Channels-last node grids [N, G, G, G, C] ~ N(0, std^2)."""
     import torch

@@ -533,9 +533,22 @@ typedef struct {
 static int ray_samples(const atdfo_model* m, int64_t r, const float* o, const float* d, hit_t* hits,
                        sample_t_* smp, int smp_cap)
 {
-    int nh = ray_hits(m->node_min, m->node_max, m->active_ids, m->num_active, o, d, m->K, hits,
+    int nh;
+    if (m->iv_off) {
+        nh = (int)(m->iv_off[r + 1] - m->iv_off[r]);
+        if (nh > ORACLE_HIT_CAP) return -1;
+        for (int h = 0; h < nh; ++h) {
+            int64_t q = m->iv_off[r] + h;
+            hits[h].node = m->iv_node[q];
+            hits[h].tmin = m->iv_tmin[q];
+            hits[h].tmax = m->iv_tmax[q];
+            hits[h].n = m->iv_count[q];
+        }
+    } else {
+        nh = ray_hits(m->node_min, m->node_max, m->active_ids, m->num_active, o, d, m->K, hits,
                       ORACLE_HIT_CAP);
-    if (nh > m->max_intersections || nh > ORACLE_HIT_CAP) return -1;
+        if (nh > m->max_intersections || nh > ORACLE_HIT_CAP) return -1;
+    }
     int ns = 0;
     for (int h = 0; h < nh; ++h) {
         float tmin = hits[h].tmin, tmax = hits[h].tmax;
@@ -645,8 +658,25 @@ static double mlp_fwd_one_d(const float* P, int C, int H, double beta, const dou
 
 #define ORACLE_SAMPLE_CAP 65536
 
+static int sample_bin(float t, float t0, float t1, int nb)
+{
+    float span = t1 - t0;
+    int b = 0;
+    if (span > 0) {
+        float rel = (t - t0) / span;
+        b = (int)(rel * (float)nb);
+    }
+    return b < 0 ? 0 : (b > nb - 1 ? nb - 1 : b);
+}
+
 int atdfo_project_fwd(const atdfo_model* m, const float* ray_o, const float* ray_d, int64_t R,
                       int precision, double* P, int num_threads)
+{
+    return atdfo_project_fwd_bins(m, ray_o, ray_d, R, precision, P, NULL, 0, num_threads);
+}
+
+int atdfo_project_fwd_bins(const atdfo_model* m, const float* ray_o, const float* ray_d, int64_t R,
+                           int precision, double* P, double* bins, int nb, int num_threads)
 {
     int err = 0;
 #ifdef _OPENMP
@@ -668,6 +698,11 @@ int atdfo_project_fwd(const atdfo_model* m, const float* ray_o, const float* ray
                 P[r] = 0;
                 continue;
             }
+            float t0 = 0, t1 = 0;
+            if (bins) {
+                for (int b = 0; b < nb; ++b) bins[r * nb + b] = 0;
+                atdfo_intersect_box_ray(m->node_min, m->node_max, ray_o + r * 3, ray_d + r * 3, &t0, &t1);
+            }
             if (precision == 0) {
                 float acc = 0;
                 for (int s = 0; s < ns; ++s) {
@@ -675,6 +710,7 @@ int atdfo_project_fwd(const atdfo_model* m, const float* ray_o, const float* ray
                     float dn = mlp_fwd_one_f(m->mlp, m->C, m->H, m->beta, featf, featf + m->C,
                                              featf + m->C + m->H, NULL);
                     acc += dn * smp[s].w;
+                    if (bins) bins[r * nb + sample_bin(smp[s].t, t0, t1, nb)] += (double)(dn * smp[s].w);
                 }
                 P[r] = acc;
             } else {
@@ -693,6 +729,7 @@ int atdfo_project_fwd(const atdfo_model* m, const float* ray_o, const float* ray
                     double dn = mlp_fwd_one_d(m->mlp, m->C, m->H, m->beta, featd, featd + m->C,
                                               featd + m->C + m->H, &yp, m->decoder_bf16);
                     acc += dn * (double)smp[s].w;
+                    if (bins) bins[r * nb + sample_bin(smp[s].t, t0, t1, nb)] += dn * (double)smp[s].w;
                 }
                 P[r] = acc;
             }

@@ -119,10 +119,26 @@ typedef struct {
     uint64_t seed;
     int decoder_bf16;       /* 1: MFMA inputs rounded to bf16 (RNE) in the
                                FP64 path, as the BF16 decoder kernels do */
+    /* Optional explicit intervals (e.g. the learned proposal's): when iv_off
+     * is non-null, ray r's samples come from intervals
+     * [iv_off[r], iv_off[r+1]) -- iv_count[h] stratified samples in
+     * [iv_tmin[h], iv_tmax[h]] of node iv_node[h] -- instead of the
+     * traversal. */
+    const int64_t* iv_off;
+    const int32_t* iv_node;
+    const float* iv_tmin;
+    const float* iv_tmax;
+    const int32_t* iv_count;
 } atdfo_model;
 
 int atdfo_project_fwd(const atdfo_model* m, const float* ray_o, const float* ray_d, int64_t R,
                       int precision, double* P, int num_threads);
+/* Same, plus (bins non-null) the per-ray mass histogram bins[r, b] = sum of
+ * density * weight over the samples whose t falls in bin b of the ray's
+ * chord [t0, t1] through the root box: b = (int)((t - t0) / (t1 - t0) * nb)
+ * clamped to [0, nb - 1] (0 when t1 <= t0). */
+int atdfo_project_fwd_bins(const atdfo_model* m, const float* ray_o, const float* ray_d, int64_t R,
+                           int precision, double* P, double* bins, int nb, int num_threads);
 /* dLdP given per ray; gradients accumulated (+=) in FP64. */
 int atdfo_project_bwd(const atdfo_model* m, const float* ray_o, const float* ray_d, int64_t R,
                       const double* dLdP, double* grad_grid, double* grad_mlp);
@@ -149,6 +165,57 @@ double atdfo_train_step_f32(atdfo_model* m, float* grid_mut, float* mlp_mut, int
                             float* grad_grid, float* grad_mlp, float* adam_m_grid, float* adam_v_grid,
                             float* adam_m_mlp, float* adam_v_mlp, int step, float lr, int num_threads,
                             int do_adam);
+
+/* ---- learned sample proposal (atdf_oracle_proposal.c) -----------------
+ * MLP 38 -> 64 -> 64 -> 64 -> 32 (ReLU hidden), inputs ray origin (3),
+ * direction (3) and the coarse occupancy of 32 equal bins of the ray's chord
+ * through the root box; softmax over the 32 logits is a piecewise-constant
+ * pdf over the bins.  params (torch Linear layout [out, in]):
+ *   W1 [64,38] b1 [64] W2 [64,64] b2 [64] W3 [64,64] b3 [64] W4 [32,64] b4 [32]
+ * Sums run over the input index in increasing order from zero, bias added
+ * last, in FP32 with no contraction: the HIP kernels reproduce the logits,
+ * probabilities, KL and every sample placement bit for bit. */
+#define ATDFO_PROP_BINS 32
+#define ATDFO_PROP_IN 38
+#define ATDFO_PROP_H 64
+int64_t atdfo_prop_num_params(void);
+/* exp / log used by the proposal (Cody-Waite + degree-7 Taylor; atanh series) */
+float atdfo_prop_exp(float x);
+float atdfo_prop_log(float x);
+
+/* Active node id (or -1) of every cell of the (2^depth)^3 lattice over the
+ * root box (x fastest): descend from the root through the child whose half
+ * holds the cell centre until an active node or a leaf. */
+void atdfo_leaf_table(const float* node_min, const float* node_max, const int32_t* children,
+                      const int32_t* active, int depth, int32_t* table);
+int32_t atdfo_locate(const float* root_min, const float* root_max, const int32_t* table, int depth,
+                     const float* p);
+
+typedef struct {
+    const float* root_min;   /* root box */
+    const float* root_max;
+    const int32_t* table;    /* atdfo_leaf_table */
+    int depth;
+    const float* coarse;     /* [num_nodes] coarse density of the active nodes */
+    const float* params;
+    float mix;               /* uniform mixing weight of the sampling pdf */
+    float kl_threshold;      /* fall back when KL(coarse || pdf) exceeds it */
+} atdfo_proposal;
+
+/* features [R, 38]; chord [R, 2] (t0, t1; zeros when the chord is shorter
+ * than 1e-5 or missing). */
+void atdfo_prop_features(const atdfo_proposal* p, const float* o, const float* d, int64_t R, float* feat,
+                         float* chord);
+void atdfo_prop_forward(const float* params, const float* feat, int64_t R, float* logits, float* probs);
+/* Per ray: fallback flag, KL, and (learned rays) up to 32 count-1 intervals,
+ * slot-major: node[i * R + r], tmin[i * R + r], tmax[i * R + r], i <
+ * n_learned[r]. */
+void atdfo_prop_sample(const atdfo_proposal* p, const float* o, const float* d, int64_t R, int32_t* fallback,
+                       int32_t* n_learned, int32_t* node, float* tmin, float* tmax, float* kl);
+/* FP64 cross-entropy of the proposal pdf against the normalised target
+ * masses bins [R, 32] (rays with zero mass skipped), divided by R; grad
+ * (overwritten, FP64) of that loss w.r.t. params.  Returns the loss. */
+double atdfo_prop_grad(const float* params, const float* feat, const float* bins, int64_t R, double* grad);
 
 #ifdef __cplusplus
 }

@@ -39,6 +39,14 @@ def lib():
         _lib.atdfo_project_fwd.restype = C.c_int
         _lib.atdfo_project_bwd.restype = C.c_int
         _lib.atdfo_train_step_f32.restype = C.c_double
+        _lib.atdfo_project_fwd_bins.restype = C.c_int
+        _lib.atdfo_prop_num_params.restype = C.c_int64
+        _lib.atdfo_prop_exp.restype = C.c_float
+        _lib.atdfo_prop_exp.argtypes = [C.c_float]
+        _lib.atdfo_prop_log.restype = C.c_float
+        _lib.atdfo_prop_log.argtypes = [C.c_float]
+        _lib.atdfo_locate.restype = C.c_int32
+        _lib.atdfo_prop_grad.restype = C.c_double
     return _lib
 
 
@@ -54,7 +62,8 @@ class Model(C.Structure):
     _fields_ = [("node_min", _P), ("node_max", _P), ("active_ids", _P), ("num_active", C.c_int64),
                 ("node_to_grid", _P), ("grid", _P), ("G", C.c_int), ("C", C.c_int), ("H", C.c_int),
                 ("mlp", _P), ("beta", C.c_float), ("K", C.c_int), ("max_intersections", C.c_int),
-                ("jitter_mode", C.c_int), ("seed", C.c_uint64), ("decoder_bf16", C.c_int)]
+                ("jitter_mode", C.c_int), ("seed", C.c_uint64), ("decoder_bf16", C.c_int),
+                ("iv_off", _P), ("iv_node", _P), ("iv_tmin", _P), ("iv_tmax", _P), ("iv_count", _P)]
 
 
 # ------------------------------------------------------------------ tree
@@ -211,14 +220,27 @@ class FusedModel:
                        _p(self.node_to_grid), _p(self.grid), self.G, C_, H, _p(self.mlp), beta, K,
                        max_intersections, jitter_mode, seed, 1 if decoder_bf16 else 0)
 
-    def project(self, o, d, precision=0, threads=0):
+    def set_intervals(self, iv_off=None, node=None, tmin=None, tmax=None, count=None):
+        """Use explicit per-ray intervals instead of the traversal (None resets)."""
+        if iv_off is None:
+            self._iv = None
+            self.m.iv_off = self.m.iv_node = self.m.iv_tmin = self.m.iv_tmax = self.m.iv_count = None
+            return
+        self._iv = (_c(iv_off, np.int64), _c(node, np.int32), _c(tmin, np.float32), _c(tmax, np.float32),
+                    _c(count, np.int32))
+        self.m.iv_off, self.m.iv_node, self.m.iv_tmin, self.m.iv_tmax, self.m.iv_count = [_p(a) for a in self._iv]
+
+    def project(self, o, d, precision=0, threads=0, bins=0):
+        """Projection [R] (float64 array); with bins > 0 also the per-ray mass
+        histogram [R, bins] over the root chord."""
         o, d = _c(o, np.float32), _c(d, np.float32)
         P = np.zeros(o.shape[0], np.float64)
-        st = lib().atdfo_project_fwd(C.byref(self.m), _p(o), _p(d), C.c_int64(o.shape[0]), C.c_int(precision),
-                                     _p(P), C.c_int(threads))
+        B = np.zeros((o.shape[0], bins), np.float64) if bins else None
+        st = lib().atdfo_project_fwd_bins(C.byref(self.m), _p(o), _p(d), C.c_int64(o.shape[0]), C.c_int(precision),
+                                          _p(P), _p(B), C.c_int(bins), C.c_int(threads))
         if st != 0:
             raise ValueError("too many intersections")
-        return P
+        return (P, B) if bins else P
 
     def project_bwd(self, o, d, dLdP):
         o, d = _c(o, np.float32), _c(d, np.float32)
@@ -275,3 +297,86 @@ class TrainBaseline:
                                           _p(self.gg), _p(self.gm), _p(self.mg), _p(self.vg), _p(self.mm),
                                           _p(self.vm), C.c_int(self.step), C.c_float(self.lr), C.c_int(threads),
                                           C.c_int(1 if do_adam else 0))
+
+
+# ------------------------------------------------------ learned proposal
+PROP_BINS, PROP_IN, PROP_H = 32, 38, 64
+
+
+class PropStruct(C.Structure):
+    _fields_ = [("root_min", _P), ("root_max", _P), ("table", _P), ("depth", C.c_int), ("coarse", _P),
+                ("params", _P), ("mix", C.c_float), ("kl_threshold", C.c_float)]
+
+
+def prop_num_params():
+    return int(lib().atdfo_prop_num_params())
+
+
+def prop_exp(x):
+    return lib().atdfo_prop_exp(float(x))
+
+
+def prop_log(x):
+    return lib().atdfo_prop_log(float(x))
+
+
+def leaf_table(node_min, node_max, children, active, depth):
+    res = 1 << depth
+    table = np.zeros(res ** 3, np.int32)
+    lib().atdfo_leaf_table(_p(_c(node_min, np.float32)), _p(_c(node_max, np.float32)), _p(_c(children, np.int32)),
+                           _p(_c(active, np.int32)), C.c_int(depth), _p(table))
+    return table
+
+
+def locate(node_min, node_max, table, depth, p):
+    return int(lib().atdfo_locate(_p(_c(node_min[0], np.float32)), _p(_c(node_max[0], np.float32)),
+                                  _p(_c(table, np.int32)), C.c_int(depth), _p(_c(p, np.float32))))
+
+
+class Proposal:
+    """Holds the arrays the proposal oracle reads."""
+
+    def __init__(self, node_min, node_max, table, depth, coarse, params, mix=0.25, kl_threshold=0.5):
+        self.root_min = _c(np.asarray(node_min)[0], np.float32)
+        self.root_max = _c(np.asarray(node_max)[0], np.float32)
+        self.table = _c(table, np.int32)
+        self.coarse = _c(coarse, np.float32)
+        self.params = _c(params, np.float32)
+        self.s = PropStruct(_p(self.root_min), _p(self.root_max), _p(self.table), depth, _p(self.coarse),
+                            _p(self.params), mix, kl_threshold)
+
+    def features(self, o, d):
+        o, d = _c(o, np.float32), _c(d, np.float32)
+        R = o.shape[0]
+        f = np.zeros((R, PROP_IN), np.float32)
+        ch = np.zeros((R, 2), np.float32)
+        lib().atdfo_prop_features(C.byref(self.s), _p(o), _p(d), C.c_int64(R), _p(f), _p(ch))
+        return f, ch
+
+    def forward(self, feat):
+        feat = _c(feat, np.float32)
+        R = feat.shape[0]
+        z = np.zeros((R, PROP_BINS), np.float32)
+        p = np.zeros((R, PROP_BINS), np.float32)
+        lib().atdfo_prop_forward(_p(self.params), _p(feat), C.c_int64(R), _p(z), _p(p))
+        return z, p
+
+    def sample(self, o, d):
+        o, d = _c(o, np.float32), _c(d, np.float32)
+        R = o.shape[0]
+        out = dict(fallback=np.zeros(R, np.int32), n_learned=np.zeros(R, np.int32),
+                   node=np.full((PROP_BINS, R), -7, np.int32), tmin=np.zeros((PROP_BINS, R), np.float32),
+                   tmax=np.zeros((PROP_BINS, R), np.float32), kl=np.zeros(R, np.float32))
+        lib().atdfo_prop_sample(C.byref(self.s), _p(o), _p(d), C.c_int64(R), _p(out["fallback"]),
+                                _p(out["n_learned"]), _p(out["node"]), _p(out["tmin"]), _p(out["tmax"]),
+                                _p(out["kl"]))
+        return out
+
+
+def prop_grad(params, feat, bins):
+    params = _c(params, np.float32)
+    feat = _c(feat, np.float32)
+    bins = _c(bins, np.float32)
+    g = np.zeros(params.shape[0], np.float64)
+    loss = lib().atdfo_prop_grad(_p(params), _p(feat), _p(bins), C.c_int64(feat.shape[0]), _p(g))
+    return loss, g
