@@ -38,12 +38,19 @@ class Tree(C.Structure):
 
 class Sampling(C.Structure):
     _fields_ = [("max_samples_per_node", C.c_int32), ("max_intersections", C.c_int32), ("jitter_mode", C.c_int32),
-                ("reserved", C.c_int32), ("jitter_seed", C.c_uint64), ("jitter", C.c_void_p)]
+                ("reserved", C.c_int32), ("jitter_seed", C.c_uint64), ("jitter", C.c_void_p),
+                ("ray_select", C.c_void_p)]
 
 
 class Intervals(C.Structure):
     _fields_ = [("node", C.c_void_p), ("tmin", C.c_void_p), ("tmax", C.c_void_p), ("count", C.c_void_p),
                 ("first", C.c_void_p), ("ray", C.c_void_p), ("total", C.c_int64)]
+
+
+class Proposal(C.Structure):
+    _fields_ = [("leaf_table", C.c_void_p), ("table_depth", C.c_int32), ("mix", C.c_float),
+                ("kl_threshold", C.c_float), ("jitter_mode", C.c_int32), ("jitter_seed", C.c_uint64),
+                ("coarse", C.c_void_p), ("params", C.c_void_p)]
 
 
 class Samples(C.Structure):
@@ -87,6 +94,17 @@ _SIGNATURES = {
     "atdf_mse_loss": (_I32, [_P, _P, _I64, _F, _P, _P, _P, _SZ, _P]),
     "atdf_adam_step": (_I32, [_P, _P, _P, _P, _I64, _F, _F, _F, _F, _I64, _I32, _P]),
     "atdf_adam_step_bf16": (_I32, [_P, _P, _P, _P, _P, _I64, _F, _F, _F, _F, _I64, _I32, _P]),
+    "atdf_project_fwd_bins": (_I32, [_P, _I64, _P, _P, _I64, C.POINTER(Sampling), _P, C.POINTER(Intervals),
+                                     C.POINTER(Field), _P, _P, _P]),
+    "atdf_proposal_num_params": (_I64, []),
+    "atdf_leaf_table_numel": (_I64, [_I32]),
+    "atdf_leaf_table": (_I32, [_P, _I64, _I32, _P, _P]),
+    "atdf_proposal_features": (_I32, [_P, _I64, _P, _P, _I64, C.POINTER(Proposal), _P, _P, _P]),
+    "atdf_proposal_sample": (_I32, [_P, _I64, _P, _P, _I64, C.POINTER(Proposal), _P, _P, _P, _P, _P, _P, _P]),
+    "atdf_proposal_merge_counts": (_I32, [_I64, _P, _P, _P, _P, _P]),
+    "atdf_proposal_write": (_I32, [_I64, _P, _P, _P, _P, _P, _P, C.POINTER(Intervals), _P]),
+    "atdf_proposal_grad_workspace_bytes": (_SZ, [_I64]),
+    "atdf_proposal_grad": (_I32, [_P, _P, _P, _I64, _F, _P, _P, _P, _SZ, _P]),
     "atdf_grid_layout": (_I32, [_P, _P, _I64, _I32, _I32, _I32, _P]),
     "atdf_f32_to_bf16": (_I32, [_P, _P, _I64, _P]),
 }

@@ -99,6 +99,7 @@ struct ProjArgs {
     int G;
     float beta;
     int rpw;
+    float* bins;  // [R, 32] mass histogram (binned forward only)
 };
 
 template <int C>
@@ -136,6 +137,7 @@ __device__ __forceinline__ void load_mlp_to_lds(const float* __restrict__ mlp, f
 // Per-lane sample state of one tile.
 struct SampleCtx {
     bool valid;
+    float t;       // ray parameter
     float w;       // integration weight
     Tri tri;
     int64_t gbase; // element offset of the node grid, -1 if none
@@ -157,6 +159,7 @@ __device__ __forceinline__ void make_sample_iv(const ProjArgs& a, int64_t r, con
     const float tp = (jj > 0) ? sample_t(tmin, step, jj - 1, ap) : 0.f;
     const float tn = (jj < n - 1) ? sample_t(tmin, step, jj + 1, an) : 0.f;
     sc.w = sample_weight(tmin, tmax, n, jj, tp, t, tn);
+    sc.t = t;
     const PNode nd = a.nodes[node];
     float loc[3];
 #pragma unroll
@@ -394,7 +397,19 @@ struct DecoderSmem {
     static constexpr int kBF16 = kDecBF16 ? BSmem<C>::kBf16Total : 8;
 };
 
-template <int C, bool kBF16, bool kDecBF16>
+// Bin of the mass histogram: b = (int)((t - t0) / (t1 - t0) * 32), clamped.
+__device__ __forceinline__ int sample_bin(float t, float t0, float t1)
+{
+    const float span = t1 - t0;
+    int b = 0;
+    if (span > 0.f) {
+        const float rel = (t - t0) / span;
+        b = (int)(rel * 32.0f);
+    }
+    return b < 0 ? 0 : (b > 31 ? 31 : b);
+}
+
+template <int C, bool kBF16, bool kDecBF16, bool kBins>
 __global__ __launch_bounds__(64 * kWaves) void project_fwd_kernel(ProjArgs a, float* __restrict__ P)
 {
     using SM = DecoderSmem<C, kDecBF16>;
@@ -416,6 +431,12 @@ __global__ __launch_bounds__(64 * kWaves) void project_fwd_kernel(ProjArgs a, fl
         const int64_t h0 = a.iv_off[r], h1e = a.iv_off[r + 1];
         if (h0 == h1e) {
             if (lane == 0) P[r] = 0.f;
+            if constexpr (kBins) {
+                if (j == 0) {
+#pragma unroll
+                    for (int q = 0; q < 8; ++q) a.bins[r * 32 + 8 * g + q] = 0.f;
+                }
+            }
             continue;
         }
         const int ns = a.iv_first[h1e - 1] + a.iv_count[h1e - 1];
@@ -427,6 +448,15 @@ __global__ __launch_bounds__(64 * kWaves) void project_fwd_kernel(ProjArgs a, fl
         }
         int64_t cur = h0;
         float acc = 0.f;
+        // binned forward: lane group g owns bins 8g .. 8g+7
+        float bacc[8];
+        float bt0 = 0.f, bt1 = 0.f;
+        if constexpr (kBins) {
+#pragma unroll
+            for (int q = 0; q < 8; ++q) bacc[q] = 0.f;
+            const PNode root = a.nodes[0];
+            intersect_box_ray(root.mn, root.mx, o, d, bt0, bt1);
+        }
         for (int base = 0; base < ns; base += kTile) {
             SampleCtx sc;
             make_sample(a, r, o, d, ns, base + j, h1e, cur, C, sc);
@@ -438,13 +468,37 @@ __global__ __launch_bounds__(64 * kWaves) void project_fwd_kernel(ProjArgs a, fl
                 y = decoder_fwd_bf16<C>(sb, s, x, g, j, h1, h2);
             else
                 y = decoder_fwd<C>(s, x, g, j, h1, h2);
-            if (sc.valid) acc += softplus_f(y, a.beta) * sc.w;
+            if (sc.valid) {
+                const float v = softplus_f(y, a.beta) * sc.w;
+                acc += v;
+                if constexpr (kBins) {
+                    const int b = sample_bin(sc.t, bt0, bt1);
+#pragma unroll
+                    for (int q = 0; q < 8; ++q)
+                        if (b == 8 * g + q) bacc[q] += v;
+                }
+            }
         }
         acc += __shfl_xor(acc, 1, 64);
         acc += __shfl_xor(acc, 2, 64);
         acc += __shfl_xor(acc, 4, 64);
         acc += __shfl_xor(acc, 8, 64);
         if (lane == 0) P[r] = acc;
+        if constexpr (kBins) {
+#pragma unroll
+            for (int q = 0; q < 8; ++q) {
+                float v = bacc[q];
+                v += __shfl_xor(v, 1, 64);
+                v += __shfl_xor(v, 2, 64);
+                v += __shfl_xor(v, 4, 64);
+                v += __shfl_xor(v, 8, 64);
+                bacc[q] = v;
+            }
+            if (j == 0) {
+#pragma unroll
+                for (int q = 0; q < 8; ++q) a.bins[r * 32 + 8 * g + q] = bacc[q];
+            }
+        }
     }
 }
 
@@ -1130,21 +1184,29 @@ static ProjArgs make_args(const void* packed_tree, const float* O, const float* 
 
 extern "C" {
 
-atdf_status atdf_project_fwd(const void* packed_tree, int64_t num_nodes, const float* O, const float* D, int64_t R,
-                             const atdf_sampling* sp, const int64_t* iv_off, const atdf_intervals* iv,
-                             const atdf_field* f, float* P, atdf_stream_t stream)
+static atdf_status project_fwd_impl(const void* packed_tree, int64_t num_nodes, const float* O, const float* D,
+                                    int64_t R, const atdf_sampling* sp, const int64_t* iv_off,
+                                    const atdf_intervals* iv, const atdf_field* f, float* P, float* bins,
+                                    atdf_stream_t stream, const char* who)
 {
     atdf_status st = check_project_args(packed_tree, num_nodes, O, D, R, sp, iv_off, iv, f);
     if (st) return st;
     if (R == 0) return ATDF_OK;
-    ATDF_REQUIRE(P, "atdf_project_fwd: null projection");
+    ATDF_REQUIRE(P, "%s: null projection", who);
     ProjArgs a = make_args(packed_tree, O, D, R, sp, iv_off, iv, f);
+    a.bins = bins;
     int64_t nwaves = div_up(R, a.rpw);
     dim3 grid((unsigned)div_up(nwaves, kWaves)), block(64 * kWaves);
     hipStream_t s = to_hip(stream);
     const bool bf = f->grid_dtype == ATDF_GRID_BF16;
     const bool bd = f->decoder_dtype == ATDF_DECODER_BF16;
-#define ATDF_FWD(CC, BF, BD) hipLaunchKernelGGL((project_fwd_kernel<CC, BF, BD>), grid, block, 0, s, a, P)
+#define ATDF_FWD(CC, BF, BD)                                                                              \
+    do {                                                                                                  \
+        if (bins)                                                                                         \
+            hipLaunchKernelGGL((project_fwd_kernel<CC, BF, BD, true>), grid, block, 0, s, a, P);          \
+        else                                                                                              \
+            hipLaunchKernelGGL((project_fwd_kernel<CC, BF, BD, false>), grid, block, 0, s, a, P);         \
+    } while (0)
     if (f->C == 16) {
         if (bf) { if (bd) ATDF_FWD(16, true, true); else ATDF_FWD(16, true, false); }
         else    { if (bd) ATDF_FWD(16, false, true); else ATDF_FWD(16, false, false); }
@@ -1153,7 +1215,25 @@ atdf_status atdf_project_fwd(const void* packed_tree, int64_t num_nodes, const f
         else    { if (bd) ATDF_FWD(8, false, true); else ATDF_FWD(8, false, false); }
     }
 #undef ATDF_FWD
-    return check_launch("atdf_project_fwd");
+    return check_launch(who);
+}
+
+atdf_status atdf_project_fwd(const void* packed_tree, int64_t num_nodes, const float* O, const float* D, int64_t R,
+                             const atdf_sampling* sp, const int64_t* iv_off, const atdf_intervals* iv,
+                             const atdf_field* f, float* P, atdf_stream_t stream)
+{
+    return project_fwd_impl(packed_tree, num_nodes, O, D, R, sp, iv_off, iv, f, P, nullptr, stream,
+                            "atdf_project_fwd");
+}
+
+atdf_status atdf_project_fwd_bins(const void* packed_tree, int64_t num_nodes, const float* O, const float* D,
+                                  int64_t R, const atdf_sampling* sp, const int64_t* iv_off,
+                                  const atdf_intervals* iv, const atdf_field* f, float* P, float* bins,
+                                  atdf_stream_t stream)
+{
+    if (R > 0) ATDF_REQUIRE(bins, "atdf_project_fwd_bins: null bins");
+    return project_fwd_impl(packed_tree, num_nodes, O, D, R, sp, iv_off, iv, f, P, bins, stream,
+                            "atdf_project_fwd_bins");
 }
 
 }  // extern "C"

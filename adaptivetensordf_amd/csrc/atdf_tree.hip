@@ -82,6 +82,7 @@ struct TraverseArgs {
     int jitter_mode;
     uint64_t seed;
     const float* jitter;
+    const int32_t* select;
     int32_t* ray_num_intervals;
     int32_t* ray_num_samples;
     int32_t* status;
@@ -103,6 +104,13 @@ __global__ __launch_bounds__(256) void traverse_kernel(TraverseArgs a)
 {
     int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (r >= a.R) return;
+    if (a.select && a.select[r] == 0) {
+        if (!kWrite) {
+            a.ray_num_intervals[r] = 0;
+            a.ray_num_samples[r] = 0;
+        }
+        return;
+    }
     float o[3], d[3];
     for (int k = 0; k < 3; ++k) {
         o[k] = a.O[r * 3 + k];
@@ -343,6 +351,7 @@ atdf_status atdf_traverse_count(const void* packed_tree, int64_t num_nodes, cons
     a.jitter_mode = sampling->jitter_mode;
     a.seed = sampling->jitter_seed;
     a.jitter = sampling->jitter;
+    a.select = sampling->ray_select;
     a.ray_num_intervals = ray_num_intervals;
     a.ray_num_samples = ray_num_samples;
     a.status = status;
@@ -384,6 +393,7 @@ atdf_status atdf_traverse_write(const void* packed_tree, int64_t num_nodes, cons
     a.jitter_mode = sampling->jitter_mode;
     a.seed = sampling->jitter_seed;
     a.jitter = sampling->jitter;
+    a.select = sampling->ray_select;
     a.iv_off = intervals ? ray_interval_offset : ray_sample_offset;
     a.smp_off = samples ? ray_sample_offset : ray_interval_offset;
     if (intervals) a.iv = *intervals;

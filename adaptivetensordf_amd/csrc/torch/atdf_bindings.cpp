@@ -72,6 +72,7 @@ PYBIND11_MODULE(TORCH_EXTENSION_NAME, m)
         .def("ComputeLocalSamples", &HyperTreeBase::ComputeLocalSamples)
         .def("SetErrorForActiveNodes", &HyperTreeBase::SetErrorForActiveNodes)
         .def("UpdateCulling", &HyperTreeBase::UpdateCulling)
+        .def("LeafTable", &HyperTreeBase::LeafTable)
         .def_readwrite("node_parent", &HyperTreeBase::node_parent)
         .def_readwrite("node_children", &HyperTreeBase::node_children)
         .def_readwrite("node_position_min", &HyperTreeBase::node_position_min)
@@ -105,4 +106,37 @@ PYBIND11_MODULE(TORCH_EXTENSION_NAME, m)
         .def_readwrite("lr_mlp", &TrainState::lr_mlp);
     m.def("MakeTrainState", &MakeTrainState);
     m.def("TrainStep", &TrainStep);
+    m.def("TrainStepIntervals", &TrainStepIntervals);
+    m.def("ProjectIntervals", &ProjectIntervals);
+    m.def("ProjectIntervalsBackward", &ProjectIntervalsBackward);
+    m.def("ProjectRaysBins", &ProjectRaysBins);
+
+    py::class_<ProposalParams>(m, "ProposalParams")
+        .def(py::init<>())
+        .def_readwrite("mix", &ProposalParams::mix)
+        .def_readwrite("kl_threshold", &ProposalParams::kl_threshold)
+        .def_readwrite("coarse_lattice", &ProposalParams::coarse_lattice)
+        .def_readwrite("target_power", &ProposalParams::target_power);
+    py::class_<ProposalState>(m, "ProposalState")
+        .def_readwrite("params", &ProposalState::params)
+        .def_readwrite("grad", &ProposalState::grad)
+        .def_readwrite("loss", &ProposalState::loss)
+        .def_readwrite("coarse", &ProposalState::coarse)
+        .def_readwrite("step", &ProposalState::step)
+        .def_readwrite("lr", &ProposalState::lr)
+        .def_readwrite("pp", &ProposalState::pp)
+        .def_readwrite("ready", &ProposalState::ready);
+    m.def("MakeProposalState", &MakeProposalState);
+    m.def("ProposalRefreshCoarse", &ProposalRefreshCoarse);
+    m.def("ProposalFeatures", &ProposalFeatures);
+    m.def("ProposalSample", &ProposalSample);
+    m.def("CreateProposalIntervals",
+          [](HyperTreeBase& tree, const ProposalState& ps, const RayList& rays, const SamplingParams& sp) {
+              torch::Tensor fb;
+              Intervals iv = CreateProposalIntervals(tree, ps, rays, sp, &fb);
+              return std::make_pair(iv, fb);
+          });
+    m.def("ProposalGrad", &ProposalGrad);
+    m.def("ProposalTrain", &ProposalTrain);
+    m.def("TrainStepLearned", &TrainStepLearned);
 }

@@ -194,6 +194,11 @@ void HyperTreeBase::repack()
     t.node_parent = node_parent.data_ptr<int32_t>();
     t.node_active = node_active.data_ptr<int32_t>();
     check_status(atdf_tree_pack(&t, packed_.data_ptr(), current_stream()), "atdf_tree_pack");
+    const int depth = leaf_depth();
+    leaf_table_ = torch::empty({atdf_leaf_table_numel(depth)},
+                               torch::TensorOptions().dtype(torch::kInt32).device(node_position_min.device()));
+    check_status(atdf_leaf_table(packed_.data_ptr(), n, depth, leaf_table_.data_ptr<int32_t>(), current_stream()),
+                 "atdf_leaf_table");
 }
 
 namespace {
@@ -677,11 +682,18 @@ Tensor TrainStep(HyperTreeBase& tree, TrainState& st, const RayList& rays, Tenso
                  const FieldParams& fp)
 {
     check_rays(rays);
+    Intervals iv = tree.CreateIntervalsForRays(rays, sp);
+    return TrainStepIntervals(tree, st, rays, target, sp, fp, iv);
+}
+
+Tensor TrainStepIntervals(HyperTreeBase& tree, TrainState& st, const RayList& rays, Tensor target,
+                          const SamplingParams& sp, const FieldParams& fp, const Intervals& iv)
+{
+    check_rays(rays);
     require_device_f32(target, "target");
     const int64_t R = rays.size();
     TORCH_CHECK(target.numel() == R, "target must hold one value per ray");
     auto stream = current_stream();
-    Intervals iv = tree.CreateIntervalsForRays(rays, sp);
     atdf_field f = make_field(tree, st.grid, st.mlp, fp);
     atdf_sampling s = sp.view();
     atdf_intervals v = iv.view();
@@ -721,6 +733,273 @@ Tensor TrainStep(HyperTreeBase& tree, TrainState& st, const RayList& rays, Tenso
                                 st.v_mlp.data_ptr<float>(), st.mlp.numel(), st.lr_mlp, b1, b2, eps, st.step, 1, stream),
                  "atdf_adam_step");
     return st.loss;
+}
+
+namespace {
+Tensor project_intervals(HyperTreeBase& tree, const RayList& rays, const Intervals& iv, const Tensor& grid,
+                         const Tensor& mlp, const SamplingParams& sp, const FieldParams& fp, Tensor* bins)
+{
+    check_rays(rays);
+    const int64_t R = rays.size();
+    atdf_field f = make_field(tree, grid, mlp, fp);
+    atdf_sampling s = sp.view();
+    atdf_intervals v = iv.view();
+    auto P = torch::empty({R}, rays.origin.options());
+    if (bins) {
+        *bins = torch::empty({R, ATDF_PROPOSAL_BINS}, rays.origin.options());
+        check_status(atdf_project_fwd_bins(tree.packed(), tree.NumNodes(), rays.origin.data_ptr<float>(),
+                                           rays.direction.data_ptr<float>(), R, &s, iv.ray_offset.data_ptr<int64_t>(),
+                                           &v, &f, P.data_ptr<float>(), bins->data_ptr<float>(), current_stream()),
+                     "atdf_project_fwd_bins");
+    } else {
+        check_status(atdf_project_fwd(tree.packed(), tree.NumNodes(), rays.origin.data_ptr<float>(),
+                                      rays.direction.data_ptr<float>(), R, &s, iv.ray_offset.data_ptr<int64_t>(), &v,
+                                      &f, P.data_ptr<float>(), current_stream()),
+                     "atdf_project_fwd");
+    }
+    return P;
+}
+
+atdf_proposal proposal_view(const HyperTreeBase& tree, const ProposalState& ps, const SamplingParams* sp = nullptr)
+{
+    TORCH_CHECK(ps.params.defined() && ps.params.is_cuda(), "proposal parameters must be on the device");
+    TORCH_CHECK(ps.coarse.defined() && ps.coarse.numel() == tree.NumNodes(),
+                "proposal coarse densities missing (run ProposalRefreshCoarse)");
+    atdf_proposal p{};
+    p.leaf_table = tree.leaf_table();
+    p.table_depth = tree.leaf_depth();
+    p.mix = ps.pp.mix;
+    p.kl_threshold = ps.pp.kl_threshold;
+    p.coarse = ps.coarse.data_ptr<float>();
+    p.params = ps.params.data_ptr<float>();
+    if (sp) {
+        p.jitter_mode = sp->jitter_mode;
+        p.jitter_seed = sp->jitter_seed;
+    }
+    return p;
+}
+}  // namespace
+
+Tensor ProjectIntervals(HyperTreeBase& tree, const RayList& rays, const Intervals& iv, Tensor grid, Tensor mlp,
+                        const SamplingParams& sp, const FieldParams& fp)
+{
+    return project_intervals(tree, rays, iv, grid, mlp, sp, fp, nullptr);
+}
+
+std::pair<Tensor, Tensor> ProjectIntervalsBackward(HyperTreeBase& tree, const RayList& rays, const Intervals& iv,
+                                                   Tensor grid, Tensor mlp, const SamplingParams& sp,
+                                                   const FieldParams& fp, Tensor dLdP)
+{
+    check_rays(rays);
+    const int64_t R = rays.size();
+    require_device_f32(dLdP, "dLdP");
+    TORCH_CHECK(dLdP.numel() == R, "dLdP must hold one value per ray");
+    atdf_field f = make_field(tree, grid, mlp, fp);
+    atdf_sampling s = sp.view();
+    atdf_intervals v = iv.view();
+    auto grad_grid = torch::zeros(grid.sizes(), grid.options().dtype(torch::kFloat32));
+    auto grad_mlp = torch::zeros_like(mlp);
+    auto ws = torch::empty({(int64_t)atdf_project_bwd_workspace_bytes(R, iv.num_intervals, &f)},
+                           torch::TensorOptions().dtype(torch::kUInt8).device(grid.device()));
+    check_status(atdf_project_bwd(tree.packed(), tree.NumNodes(), rays.origin.data_ptr<float>(),
+                                  rays.direction.data_ptr<float>(), R, &s, iv.ray_offset.data_ptr<int64_t>(), &v, &f,
+                                  dLdP.data_ptr<float>(), grad_grid.data_ptr<float>(), grad_mlp.data_ptr<float>(),
+                                  ws.data_ptr(), ws.numel(), current_stream()),
+                 "atdf_project_bwd");
+    return {grad_grid, grad_mlp};
+}
+
+std::pair<Tensor, Tensor> ProjectRaysBins(HyperTreeBase& tree, const RayList& rays, Tensor grid, Tensor mlp,
+                                          const SamplingParams& sp, const FieldParams& fp)
+{
+    Intervals iv = tree.CreateIntervalsForRays(rays, sp);
+    Tensor bins;
+    Tensor P = project_intervals(tree, rays, iv, grid, mlp, sp, fp, &bins);
+    return {P, bins};
+}
+
+ProposalState MakeProposalState(Tensor params, float lr, const ProposalParams& pp)
+{
+    require_device_f32(params, "proposal params");
+    TORCH_CHECK(params.numel() == atdf_proposal_num_params(), "proposal params must hold ",
+                atdf_proposal_num_params(), " values");
+    ProposalState ps;
+    ps.params = params.clone();
+    ps.grad = torch::zeros_like(params);
+    ps.m = torch::zeros_like(params);
+    ps.v = torch::zeros_like(params);
+    ps.loss = torch::zeros({1}, params.options().dtype(torch::kFloat64));
+    ps.lr = lr;
+    ps.pp = pp;
+    return ps;
+}
+
+void ProposalRefreshCoarse(HyperTreeBase& tree, ProposalState& ps, Tensor grid, Tensor mlp, const FieldParams& fp)
+{
+    const int n = ps.pp.coarse_lattice;
+    TORCH_CHECK(n >= 2, "coarse_lattice must be >= 2");
+    auto [mx, s1, s2] = node_stats(tree, grid, mlp, fp, 0, n, 1, 0);
+    (void)mx;
+    (void)s2;
+    auto mean = (s1 / (double)((int64_t)n * n * n)).to(torch::kFloat32);
+    ps.coarse = torch::zeros({tree.NumNodes()}, mean.options()).index_copy(0, tree.active_node_ids, mean);
+}
+
+std::pair<Tensor, Tensor> ProposalFeatures(HyperTreeBase& tree, const ProposalState& ps, const RayList& rays)
+{
+    check_rays(rays);
+    const int64_t R = rays.size();
+    atdf_proposal p = proposal_view(tree, ps);
+    auto f = torch::empty({R, ATDF_PROPOSAL_INPUTS}, rays.origin.options());
+    auto ch = torch::empty({R, 2}, rays.origin.options());
+    check_status(atdf_proposal_features(tree.packed(), tree.NumNodes(), rays.origin.data_ptr<float>(),
+                                        rays.direction.data_ptr<float>(), R, &p, f.data_ptr<float>(),
+                                        ch.data_ptr<float>(), current_stream()),
+                 "atdf_proposal_features");
+    return {f, ch};
+}
+
+std::vector<Tensor> ProposalSample(HyperTreeBase& tree, const ProposalState& ps, const RayList& rays,
+                                   const SamplingParams& sp)
+{
+    check_rays(rays);
+    TORCH_CHECK(sp.jitter_mode == 0 || sp.jitter_mode == 2, "ProposalSample: jitter_mode must be 0 or 2");
+    const int64_t R = rays.size();
+    atdf_proposal p = proposal_view(tree, ps, &sp);
+    auto i32 = rays.origin.options().dtype(torch::kInt32);
+    auto f32 = rays.origin.options();
+    auto fb = torch::empty({R}, i32), nl = torch::empty({R}, i32);
+    auto node = torch::empty({ATDF_PROPOSAL_BINS, R}, i32);
+    auto lo = torch::empty({ATDF_PROPOSAL_BINS, R}, f32), hi = torch::empty({ATDF_PROPOSAL_BINS, R}, f32);
+    auto kl = torch::empty({R}, f32);
+    check_status(atdf_proposal_sample(tree.packed(), tree.NumNodes(), rays.origin.data_ptr<float>(),
+                                      rays.direction.data_ptr<float>(), R, &p, fb.data_ptr<int32_t>(),
+                                      nl.data_ptr<int32_t>(), node.data_ptr<int32_t>(), lo.data_ptr<float>(),
+                                      hi.data_ptr<float>(), kl.data_ptr<float>(), current_stream()),
+                 "atdf_proposal_sample");
+    return {fb, nl, node, lo, hi, kl};
+}
+
+Intervals CreateProposalIntervals(HyperTreeBase& tree, const ProposalState& ps, const RayList& rays,
+                                  const SamplingParams& fallback_sp, Tensor* ray_fallback)
+{
+    check_rays(rays);
+    TORCH_CHECK(fallback_sp.jitter_mode != 1, "CreateProposalIntervals: jitter_mode 1 needs materialised samples");
+    const int64_t R = rays.size();
+    auto v = ProposalSample(tree, ps, rays, fallback_sp);
+    Tensor fb = v[0], nl = v[1];
+    atdf_sampling s = fallback_sp.view();
+    s.ray_select = fb.data_ptr<int32_t>();
+    auto dev = tree.device();
+    auto i32 = torch::TensorOptions().dtype(torch::kInt32).device(dev);
+    auto i64 = torch::TensorOptions().dtype(torch::kLong).device(dev);
+    auto f32 = torch::TensorOptions().dtype(torch::kFloat32).device(dev);
+    auto st = current_stream();
+    auto num_iv = torch::empty({R}, i32), num_s = torch::empty({R}, i32);
+    auto status = torch::zeros({4}, i32);
+    Intervals iv;
+    iv.ray_offset = torch::empty({R + 1}, i64);
+    auto smp_off = torch::empty({R + 1}, i64);
+    if (R > 0) {
+        check_status(atdf_traverse_count(tree.packed(), tree.NumNodes(), rays.origin.data_ptr<float>(),
+                                         rays.direction.data_ptr<float>(), R, &s, num_iv.data_ptr<int32_t>(),
+                                         num_s.data_ptr<int32_t>(), status.data_ptr<int32_t>(), st),
+                     "atdf_traverse_count");
+        check_status(atdf_proposal_merge_counts(R, fb.data_ptr<int32_t>(), nl.data_ptr<int32_t>(),
+                                                num_iv.data_ptr<int32_t>(), num_s.data_ptr<int32_t>(), st),
+                     "atdf_proposal_merge_counts");
+    }
+    auto ws = torch::empty({(int64_t)atdf_scan_workspace_bytes(R)}, torch::TensorOptions().dtype(torch::kUInt8).device(dev));
+    check_status(atdf_exclusive_scan_i32(num_iv.data_ptr<int32_t>(), R, iv.ray_offset.data_ptr<int64_t>(),
+                                         ws.data_ptr(), ws.numel(), st),
+                 "atdf_exclusive_scan_i32");
+    check_status(atdf_exclusive_scan_i32(num_s.data_ptr<int32_t>(), R, smp_off.data_ptr<int64_t>(), ws.data_ptr(),
+                                         ws.numel(), st),
+                 "atdf_exclusive_scan_i32");
+    auto max_s = num_s.numel() > 0 ? num_s.max().to(torch::kLong).reshape({1}) : torch::zeros({1}, i64);
+    auto info = torch::cat({status.to(torch::kLong), iv.ray_offset.slice(0, R, R + 1), smp_off.slice(0, R, R + 1),
+                            max_s, fb.sum().to(torch::kLong).reshape({1})})
+                    .to(torch::kCPU);
+    const int64_t* ip = info.data_ptr<int64_t>();
+    TORCH_CHECK(ip[0] == 0, "CreateProposalIntervals: ", ip[0], " fallback ray(s) hit more than ",
+                s.max_intersections, " nodes");
+    TORCH_CHECK(ip[1] == 0, "CreateProposalIntervals: traversal order check failed on ", ip[1], " ray(s)");
+    iv.ray_num_samples = num_s;
+    iv.num_intervals = ip[4];
+    iv.num_samples = ip[5];
+    iv.max_samples_per_ray = (int)ip[6];
+    const int64_t H = iv.num_intervals;
+    iv.node = torch::empty({H}, i32);
+    iv.tmin = torch::empty({H}, f32);
+    iv.tmax = torch::empty({H}, f32);
+    iv.count = torch::empty({H}, i32);
+    iv.first = torch::empty({H}, i32);
+    iv.ray = torch::empty({H}, i32);
+    if (H > 0) {
+        atdf_intervals w = iv.view();
+        if (ip[7] > 0)
+            check_status(atdf_traverse_write(tree.packed(), tree.NumNodes(), rays.origin.data_ptr<float>(),
+                                             rays.direction.data_ptr<float>(), R, &s, iv.ray_offset.data_ptr<int64_t>(),
+                                             smp_off.data_ptr<int64_t>(), &w, nullptr, st),
+                         "atdf_traverse_write");
+        check_status(atdf_proposal_write(R, fb.data_ptr<int32_t>(), nl.data_ptr<int32_t>(), v[2].data_ptr<int32_t>(),
+                                         v[3].data_ptr<float>(), v[4].data_ptr<float>(),
+                                         iv.ray_offset.data_ptr<int64_t>(), &w, st),
+                     "atdf_proposal_write");
+    }
+    if (ray_fallback) *ray_fallback = fb;
+    return iv;
+}
+
+Tensor ProposalGrad(ProposalState& ps, Tensor features, Tensor target_bins)
+{
+    require_device_f32(features, "features");
+    require_device_f32(target_bins, "target_bins");
+    const int64_t R = features.size(0);
+    TORCH_CHECK(features.dim() == 2 && features.size(1) == ATDF_PROPOSAL_INPUTS, "features must be [R, 38]");
+    TORCH_CHECK(target_bins.dim() == 2 && target_bins.size(0) == R && target_bins.size(1) == ATDF_PROPOSAL_BINS,
+                "target_bins must be [R, 32]");
+    auto ws = torch::empty({(int64_t)atdf_proposal_grad_workspace_bytes(R)},
+                           torch::TensorOptions().dtype(torch::kUInt8).device(features.device()));
+    check_status(atdf_proposal_grad(ps.params.data_ptr<float>(), features.data_ptr<float>(),
+                                    target_bins.data_ptr<float>(), R, ps.pp.target_power, ps.grad.data_ptr<float>(),
+                                    ps.loss.data_ptr<double>(), ws.data_ptr(), ws.numel(), current_stream()),
+                 "atdf_proposal_grad");
+    return ps.loss;
+}
+
+double ProposalTrain(HyperTreeBase& tree, ProposalState& ps, const RayList& rays, const SamplingParams& sp,
+                     Tensor grid, Tensor mlp, const FieldParams& fp, int steps)
+{
+    TORCH_CHECK(steps >= 1, "ProposalTrain: steps must be >= 1");
+    ProposalRefreshCoarse(tree, ps, grid, mlp, fp);
+    auto pb = ProjectRaysBins(tree, rays, grid, mlp, sp, fp);
+    auto feat = ProposalFeatures(tree, ps, rays).first;
+    const int64_t R = rays.size();
+    auto ws = torch::empty({(int64_t)atdf_proposal_grad_workspace_bytes(R)},
+                           torch::TensorOptions().dtype(torch::kUInt8).device(feat.device()));
+    auto st = current_stream();
+    for (int i = 0; i < steps; ++i) {
+        check_status(atdf_proposal_grad(ps.params.data_ptr<float>(), feat.data_ptr<float>(),
+                                        pb.second.data_ptr<float>(), R, ps.pp.target_power, ps.grad.data_ptr<float>(),
+                                        ps.loss.data_ptr<double>(), ws.data_ptr(), ws.numel(), st),
+                     "atdf_proposal_grad");
+        ps.step += 1;
+        check_status(atdf_adam_step(ps.params.data_ptr<float>(), ps.grad.data_ptr<float>(), ps.m.data_ptr<float>(),
+                                    ps.v.data_ptr<float>(), ps.params.numel(), ps.lr, 0.9f, 0.999f, 1e-8f, ps.step, 1,
+                                    st),
+                     "atdf_adam_step");
+    }
+    ps.ready = true;
+    return ps.loss.item<double>();
+}
+
+Tensor TrainStepLearned(HyperTreeBase& tree, TrainState& st, ProposalState& ps, const RayList& rays, Tensor target,
+                        const SamplingParams& sp, const FieldParams& fp)
+{
+    if (!ps.ready) return TrainStep(tree, st, rays, target, sp, fp);
+    Intervals iv = CreateProposalIntervals(tree, ps, rays, sp);
+    return TrainStepIntervals(tree, st, rays, target, sp, fp, iv);
 }
 
 }  // namespace atdf_torch

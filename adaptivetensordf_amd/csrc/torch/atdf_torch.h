@@ -92,6 +92,10 @@ class HyperTreeBase {
     void UpdateCulling();
 
     const void* packed() const { return packed_.data_ptr(); }
+    // point-location table of the active set (rebuilt with the packed tree)
+    const int32_t* leaf_table() const { return leaf_table_.data_ptr<int32_t>(); }
+    int leaf_depth() const { return max_depth < 1 ? 1 : max_depth; }
+    torch::Tensor LeafTable() const { return leaf_table_; }
 
     int d;
     int max_depth;
@@ -102,6 +106,7 @@ class HyperTreeBase {
    private:
     void repack();
     torch::Tensor packed_;
+    torch::Tensor leaf_table_;
 };
 
 // IndirectGridSample3D(multi_grid, index, uv): grid [N, C, Z, Y, X]
@@ -157,5 +162,64 @@ struct TrainState {
 TrainState MakeTrainState(torch::Tensor grid, torch::Tensor mlp, bool bf16_grid, float lr_grid, float lr_mlp);
 torch::Tensor TrainStep(HyperTreeBase& tree, TrainState& st, const RayList& rays, torch::Tensor target,
                         const SamplingParams& sp, const FieldParams& fp);
+// The same step over a given interval list (e.g. the learned proposal's).
+torch::Tensor TrainStepIntervals(HyperTreeBase& tree, TrainState& st, const RayList& rays, torch::Tensor target,
+                                 const SamplingParams& sp, const FieldParams& fp, const Intervals& iv);
+
+// Projection over a given interval list (no autograd), and, with bins,
+// the per-ray mass histogram [R, 32] over the root chord.
+torch::Tensor ProjectIntervals(HyperTreeBase& tree, const RayList& rays, const Intervals& iv, torch::Tensor grid,
+                               torch::Tensor mlp, const SamplingParams& sp, const FieldParams& fp);
+// Adjoint over a given interval list: (dL/dgrid f32, dL/dmlp) for dL/dP.
+std::pair<torch::Tensor, torch::Tensor> ProjectIntervalsBackward(HyperTreeBase& tree, const RayList& rays,
+                                                                 const Intervals& iv, torch::Tensor grid,
+                                                                 torch::Tensor mlp, const SamplingParams& sp,
+                                                                 const FieldParams& fp, torch::Tensor dLdP);
+std::pair<torch::Tensor, torch::Tensor> ProjectRaysBins(HyperTreeBase& tree, const RayList& rays, torch::Tensor grid,
+                                                        torch::Tensor mlp, const SamplingParams& sp,
+                                                        const FieldParams& fp);
+
+// ---- learned sample proposal (no reference counterpart; the stratified
+// sampler of HyperTree.cu:266-514 is its fallback) ----
+struct ProposalParams {
+    float mix = 0.25f;          // uniform mixing weight of the sampling pdf
+    float kl_threshold = 0.5f;  // fallback when KL(coarse || pdf) exceeds it
+    int coarse_lattice = 4;     // n^3 lattice per node for the coarse density
+    float target_power = 0.5f;  // training target q_b ~ (binned density mass)^power
+};
+struct ProposalState {
+    torch::Tensor params, grad, m, v;  // [P] f32
+    torch::Tensor loss;                // [1] f64 (device)
+    torch::Tensor coarse;              // [num_nodes] f32 coarse density
+    int64_t step = 0;
+    float lr = 1e-3f;
+    ProposalParams pp;
+    bool ready = false;                // trained at least once
+};
+ProposalState MakeProposalState(torch::Tensor params, float lr, const ProposalParams& pp);
+// coarse[node] = mean density on the n^3 lattice of each active node (0 elsewhere)
+void ProposalRefreshCoarse(HyperTreeBase& tree, ProposalState& ps, torch::Tensor grid, torch::Tensor mlp,
+                           const FieldParams& fp);
+// network inputs [R, 38] and root chords [R, 2]
+std::pair<torch::Tensor, torch::Tensor> ProposalFeatures(HyperTreeBase& tree, const ProposalState& ps,
+                                                         const RayList& rays);
+// placement: {fallback [R], n_learned [R], node [32, R], tmin [32, R], tmax [32, R], kl [R]}
+// for intervals integrated with the jitter of sp
+std::vector<torch::Tensor> ProposalSample(HyperTreeBase& tree, const ProposalState& ps, const RayList& rays,
+                                          const SamplingParams& sp);
+// learned one-sample intervals for the learned rays, stratified traversal
+// (fallback_sp) for the fallback rays, one list in ray order
+Intervals CreateProposalIntervals(HyperTreeBase& tree, const ProposalState& ps, const RayList& rays,
+                                  const SamplingParams& fallback_sp, torch::Tensor* ray_fallback = nullptr);
+// loss (device [1] f64) and ps.grad for given features / target masses
+torch::Tensor ProposalGrad(ProposalState& ps, torch::Tensor features, torch::Tensor target_bins);
+// Online training on densities of the current reconstruction: target masses
+// from the stratified binned forward over `rays`, then `steps` Adam steps.
+// Refreshes the coarse densities first.  Returns the last loss.
+double ProposalTrain(HyperTreeBase& tree, ProposalState& ps, const RayList& rays, const SamplingParams& sp,
+                     torch::Tensor grid, torch::Tensor mlp, const FieldParams& fp, int steps);
+// Training step sampling with the proposal once it is ready (stratified before).
+torch::Tensor TrainStepLearned(HyperTreeBase& tree, TrainState& st, ProposalState& ps, const RayList& rays,
+                               torch::Tensor target, const SamplingParams& sp, const FieldParams& fp);
 
 }  // namespace atdf_torch

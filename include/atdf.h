@@ -73,6 +73,8 @@ typedef struct {
     int32_t reserved;
     uint64_t jitter_seed;         /* mode 2                                     */
     const float* jitter;          /* mode 1: U[0,1) per canonical sample slot   */
+    const int32_t* ray_select;    /* nullable [R]: rays with 0 get no intervals
+                                     (the learned proposal's fallback mask)     */
 } atdf_sampling;
 
 /* status[0] += rays over max_intersections, status[1] += order-check
@@ -212,6 +214,18 @@ atdf_status atdf_project_bwd(const void* packed_tree, int64_t num_nodes, const f
                              const atdf_field* field, const float* dLdP, float* grad_grid, float* grad_mlp,
                              void* workspace, size_t workspace_bytes, atdf_stream_t stream);
 
+/* atdf_project_fwd plus the per-ray mass histogram ray_bins[r, b] (f32,
+ * [num_rays, 32]): the sum of density * weight over the samples of ray r
+ * whose t falls in bin b = (int)((t - t0) / (t1 - t0) * 32), clamped to
+ * [0, 31] (0 when t1 <= t0), of the ray's chord [t0, t1] through the root
+ * box.  These are the learned proposal's training targets: "densities the
+ * reconstruction itself produces", binned along each ray. */
+atdf_status atdf_project_fwd_bins(const void* packed_tree, int64_t num_nodes, const float* ray_origin,
+                                  const float* ray_direction, int64_t num_rays, const atdf_sampling* sampling,
+                                  const int64_t* ray_interval_offset, const atdf_intervals* intervals,
+                                  const atdf_field* field, float* projection, float* ray_bins,
+                                  atdf_stream_t stream);
+
 /* Per-node density statistics for the octree optimizer, reduced in FP64:
  * for active node a (node id active_ids[a]) the field's density is evaluated
  *   mode 0: on the n^3 lattice of UniformGlobalSamples (structure/HyperTree.cu:1618-1653),
@@ -224,6 +238,97 @@ atdf_status atdf_node_density_stats(const void* packed_tree, int64_t num_nodes, 
                                     int64_t num_active, const atdf_field* field, int32_t mode, int32_t n,
                                     int32_t rounds, uint64_t seed, float* node_max, double* node_sum,
                                     double* node_sumsq, atdf_stream_t stream);
+
+/* ======================================================================= */
+/* Learned sample proposal                                                  */
+/* ======================================================================= */
+/* No counterpart in the reference: its sampler is the stratified per-node
+ * placement of ComputeRaySamples (structure/HyperTree.cu:266-514), which
+ * stays as the fallback.  A small MLP 38 -> 64 -> 64 -> 64 -> 32 (ReLU)
+ * reads the ray origin (3), direction (3) and the coarse occupancy of 32
+ * equal bins of the ray's chord through the root box (the coarse density of
+ * the active node at each bin midpoint), and its softmax is a
+ * piecewise-constant pdf over the bins.  Sampling pdf = (1 - mix) * softmax
+ * + mix / 32; the inverse CDF at i / 32 (i = 1..31) cuts the chord into 32
+ * equal-probability cells, narrow where the pdf is high, and each cell
+ * becomes a one-sample interval (node located at the sample the fused
+ * kernels will place: the midpoint, or the jittered point) that the fused
+ * kernels integrate like any other interval.  The cells tile the chord, so
+ * with jitter the estimate is unbiased for any pdf.  A ray falls
+ * back to the stratified traversal when KL(coarse occupancy || pdf) exceeds
+ * kl_threshold.  Every FP32 operation runs in a fixed order with no
+ * contraction, so logits, KL, fallback decisions and sample placement are
+ * reproducible bit for bit on the CPU.
+ * params (torch Linear layout [out, in]): W1 [64, 38] b1 [64] W2 [64, 64]
+ * b2 [64] W3 [64, 64] b3 [64] W4 [32, 64] b4 [32]. */
+#define ATDF_PROPOSAL_BINS 32
+#define ATDF_PROPOSAL_INPUTS 38
+#define ATDF_PROPOSAL_HIDDEN 64
+int64_t atdf_proposal_num_params(void);
+
+/* Point location table: table[(z * res + y) * res + x], res = 2^depth, holds
+ * the active node containing the centre of lattice cell (x, y, z) of the
+ * root box, or -1 (no active node there, e.g. a culled region).  Rebuilt
+ * after every change of the active set; depth = the tree's max depth
+ * (1..8).  A point p maps to the cell floor((p - min) / (max - min) * res),
+ * clamped.  Plays the role of NodeIdForPositionGPU
+ * (structure/HyperTree.cu:1339-1525) for the proposal. */
+int64_t atdf_leaf_table_numel(int32_t depth);
+atdf_status atdf_leaf_table(const void* packed_tree, int64_t num_nodes, int32_t depth, int32_t* table,
+                            atdf_stream_t stream);
+
+typedef struct {
+    const int32_t* leaf_table; /* atdf_leaf_table output                       */
+    int32_t table_depth;
+    float mix;                 /* uniform mixing weight in [0, 1]              */
+    float kl_threshold;        /* fallback when KL(coarse || pdf) > threshold  */
+    int32_t jitter_mode;       /* 0 / 2, as the sampling the intervals are
+                                  integrated with: each cell's node is the one
+                                  holding the sample the fused kernels place */
+    uint64_t jitter_seed;
+    const float* coarse;       /* [num_nodes] coarse density per node          */
+    const float* params;       /* [atdf_proposal_num_params()]                 */
+} atdf_proposal;
+
+/* Network inputs [num_rays, 38] and the root chord [num_rays, 2] (t0, t1;
+ * zero when the chord misses or is shorter than 1e-5). */
+atdf_status atdf_proposal_features(const void* packed_tree, int64_t num_nodes, const float* ray_origin,
+                                   const float* ray_direction, int64_t num_rays, const atdf_proposal* proposal,
+                                   float* features, float* chord, atdf_stream_t stream);
+
+/* Inference + placement.  Per ray: ray_fallback (0/1), ray_num_learned
+ * (0 for fallback rays) and the learned one-sample intervals, slot-major:
+ * learned_node / learned_tmin / learned_tmax [32, num_rays], slot i < n of
+ * ray r at i * num_rays + r.  ray_kl (nullable) = KL(coarse || pdf). */
+atdf_status atdf_proposal_sample(const void* packed_tree, int64_t num_nodes, const float* ray_origin,
+                                 const float* ray_direction, int64_t num_rays, const atdf_proposal* proposal,
+                                 int32_t* ray_fallback, int32_t* ray_num_learned, int32_t* learned_node,
+                                 float* learned_tmin, float* learned_tmax, float* ray_kl, atdf_stream_t stream);
+
+/* Mixed interval list: run atdf_traverse_count with sampling.ray_select =
+ * ray_fallback, then this replaces the counts of the learned rays by
+ * ray_num_learned (in place); after atdf_exclusive_scan_i32 and
+ * atdf_traverse_write (same ray_select) over the combined offsets,
+ * atdf_proposal_write fills in the learned rays' intervals (count 1, first =
+ * slot, ray = r). */
+atdf_status atdf_proposal_merge_counts(int64_t num_rays, const int32_t* ray_fallback,
+                                       const int32_t* ray_num_learned, int32_t* ray_num_intervals,
+                                       int32_t* ray_num_samples, atdf_stream_t stream);
+atdf_status atdf_proposal_write(int64_t num_rays, const int32_t* ray_fallback, const int32_t* ray_num_learned,
+                                const int32_t* learned_node, const float* learned_tmin,
+                                const float* learned_tmax, const int64_t* ray_interval_offset,
+                                const atdf_intervals* intervals, atdf_stream_t stream);
+
+/* Training: loss = (1/R) sum over rays with positive target mass of the
+ * cross-entropy -sum_b q_b log softmax_b, q_b = target_bins_b^target_power
+ * normalised per ray (target_bins from atdf_project_fwd_bins), and grad
+ * (f32, overwritten) its gradient w.r.t. params; loss (device double[1])
+ * overwritten.  Fixed-order reductions: the result does not depend on
+ * scheduling. */
+size_t atdf_proposal_grad_workspace_bytes(int64_t num_rays);
+atdf_status atdf_proposal_grad(const float* params, const float* features, const float* target_bins,
+                               int64_t num_rays, float target_power, float* grad, double* loss, void* workspace,
+                               size_t workspace_bytes, atdf_stream_t stream);
 
 /* ======================================================================= */
 /* Loss, optimizer, layout                                                  */

@@ -742,6 +742,58 @@ int atdfo_project_fwd_bins(const atdfo_model* m, const float* ray_o, const float
     return err ? -1 : 0;
 }
 
+/* Per ray, the smallest |pre-activation| of the decoder's hidden units over
+ * its samples (FP64 path).  Where it is tiny, the sign of that unit -- and
+ * with it the ReLU mask and the sample's gradient direction -- can differ
+ * between FP32 and FP64 evaluation; gradient comparisons leave such rays out. */
+void atdfo_project_min_preact(const atdfo_model* m, const float* ray_o, const float* ray_d, int64_t R,
+                              double* out)
+{
+    int C = m->C, H = m->H;
+    hit_t* hits = (hit_t*)malloc(sizeof(hit_t) * ORACLE_HIT_CAP);
+    sample_t_* smp = (sample_t_*)malloc(sizeof(sample_t_) * ORACLE_SAMPLE_CAP);
+    double* x = (double*)malloc(sizeof(double) * C);
+    double* h1 = (double*)malloc(sizeof(double) * H);
+    float* xf = (float*)malloc(sizeof(float) * C);
+    const float* W1 = m->mlp;
+    const float* b1 = W1 + H * C;
+    const float* W2 = b1 + H;
+    const float* b2 = W2 + H * H;
+    for (int64_t r = 0; r < R; ++r) {
+        int ns = ray_samples(m, r, ray_o + r * 3, ray_d + r * 3, hits, smp, ORACLE_SAMPLE_CAP);
+        double mn = INFINITY;
+        for (int s = 0; s < ns; ++s) {
+            tri_d t;
+            int64_t base;
+            if (m->decoder_bf16) {
+                gather_cl_f(m, &smp[s], xf);
+                for (int c = 0; c < C; ++c) x[c] = xf[c];
+            } else {
+                gather_cl_d(m, &smp[s], x, &t, &base);
+            }
+            for (int i = 0; i < H; ++i) {
+                double acc = 0;
+                for (int k = 0; k < C; ++k) acc += rb(m->decoder_bf16, W1[i * C + k]) * rb(m->decoder_bf16, x[k]);
+                acc += b1[i];
+                if (fabs(acc) < mn) mn = fabs(acc);
+                h1[i] = acc > 0 ? acc : 0;
+            }
+            for (int i = 0; i < H; ++i) {
+                double acc = 0;
+                for (int k = 0; k < H; ++k) acc += rb(m->decoder_bf16, W2[i * H + k]) * rb(m->decoder_bf16, h1[k]);
+                acc += b2[i];
+                if (fabs(acc) < mn) mn = fabs(acc);
+            }
+        }
+        out[r] = mn;
+    }
+    free(hits);
+    free(smp);
+    free(x);
+    free(h1);
+    free(xf);
+}
+
 /* FP64 adjoint of atdfo_project_fwd(precision=1), sequential over rays. */
 int atdfo_project_bwd(const atdfo_model* m, const float* ray_o, const float* ray_d, int64_t R,
                       const double* dLdP, double* grad_grid, double* grad_mlp)

@@ -139,6 +139,11 @@ int atdfo_project_fwd(const atdfo_model* m, const float* ray_o, const float* ray
  * clamped to [0, nb - 1] (0 when t1 <= t0). */
 int atdfo_project_fwd_bins(const atdfo_model* m, const float* ray_o, const float* ray_d, int64_t R,
                            int precision, double* P, double* bins, int nb, int num_threads);
+/* Per-ray smallest |hidden pre-activation| over the ray's samples (FP64):
+ * rays where it is tiny sit on a ReLU kink, where FP32 and FP64 may pick
+ * different masks. */
+void atdfo_project_min_preact(const atdfo_model* m, const float* ray_o, const float* ray_d, int64_t R,
+                              double* out);
 /* dLdP given per ray; gradients accumulated (+=) in FP64. */
 int atdfo_project_bwd(const atdfo_model* m, const float* ray_o, const float* ray_d, int64_t R,
                       const double* dLdP, double* grad_grid, double* grad_mlp);
@@ -200,6 +205,9 @@ typedef struct {
     const float* params;
     float mix;               /* uniform mixing weight of the sampling pdf */
     float kl_threshold;      /* fall back when KL(coarse || pdf) exceeds it */
+    int jitter_mode;         /* 0 midpoint, 2 hash: where the fused kernels put
+                                each cell's sample (its node is located there) */
+    uint64_t jitter_seed;
 } atdfo_proposal;
 
 /* features [R, 38]; chord [R, 2] (t0, t1; zeros when the chord is shorter
@@ -209,13 +217,17 @@ void atdfo_prop_features(const atdfo_proposal* p, const float* o, const float* d
 void atdfo_prop_forward(const float* params, const float* feat, int64_t R, float* logits, float* probs);
 /* Per ray: fallback flag, KL, and (learned rays) up to 32 count-1 intervals,
  * slot-major: node[i * R + r], tmin[i * R + r], tmax[i * R + r], i <
- * n_learned[r]. */
+ * n_learned[r].  The cells are [T_i, T_i+1] with T_0 = t0, T_32 = t1 and
+ * T_i (0 < i < 32) the inverse CDF of the sampling pdf at i / 32; empty
+ * cells and cells whose sample lies where no active node is are dropped. */
 void atdfo_prop_sample(const atdfo_proposal* p, const float* o, const float* d, int64_t R, int32_t* fallback,
                        int32_t* n_learned, int32_t* node, float* tmin, float* tmax, float* kl);
-/* FP64 cross-entropy of the proposal pdf against the normalised target
- * masses bins [R, 32] (rays with zero mass skipped), divided by R; grad
- * (overwritten, FP64) of that loss w.r.t. params.  Returns the loss. */
-double atdfo_prop_grad(const float* params, const float* feat, const float* bins, int64_t R, double* grad);
+/* FP64 cross-entropy of the proposal pdf against the target q_b ~
+ * bins_b^power normalised per ray (bins [R, 32] the binned masses; rays with
+ * zero mass skipped), divided by R; grad (overwritten, FP64) of that loss
+ * w.r.t. params.  Returns the loss. */
+double atdfo_prop_grad(const float* params, const float* feat, const float* bins, int64_t R, float power,
+                       double* grad);
 
 #ifdef __cplusplus
 }

@@ -3,9 +3,9 @@
  *
  * TEST INFRASTRUCTURE ONLY (see atdf_oracle.h).  The reference has no
  * learned proposal; this file defines the method the HIP kernels implement
- * (ray features -> MLP -> 32-bin pdf -> systematic allocation of 32 samples
- * -> count-1 node intervals, KL gate against the coarse occupancy) and its
- * training loss.  The FP32 functions fix the rounding order the kernels
+ * (ray features -> MLP -> 32-bin pdf -> 32 equal-probability cells of the
+ * chord, one sample each -> count-1 node intervals, KL gate against the
+ * coarse occupancy) and its training loss.  The FP32 functions fix the rounding order the kernels
  * follow (compiled with -ffp-contract=off); atdfo_prop_grad is the FP64
  * restatement the training kernels are checked against.
  *
@@ -224,14 +224,15 @@ void atdfo_prop_forward(const float* params, const float* feat, int64_t R, float
     }
 }
 
-/* number of systematic points (i + 1/2) / 32, i = 0..31, below x */
-static int points_below(float x)
+/* Jitter of a one-sample interval: the fused kernels place sample s_local
+ * of ray r at lo (1 - a) + (lo + (hi - lo)) a with a = 0.5 (mode 0) or
+ * atdfo_jitter(seed, r, s_local) (mode 2). */
+static float cell_sample_t(float lo, float hi, float a)
 {
-    float v = x * (float)NB - 0.5f;
-    if (!(v > 0.0f)) return 0;
-    int iv = (int)v;
-    int k = (v > (float)iv) ? iv + 1 : iv;
-    return k > NB ? NB : k;
+    float step = (hi - lo) / 1.0f;
+    float t1 = lo + 0.0f * step;
+    float t2 = lo + 1.0f * step;
+    return t1 * (1 - a) + t2 * a;
 }
 
 void atdfo_prop_sample(const atdfo_proposal* p, const float* o, const float* d, int64_t R, int32_t* fallback,
@@ -272,37 +273,49 @@ void atdfo_prop_sample(const atdfo_proposal* p, const float* o, const float* d, 
             fallback[r] = 1;
             continue;
         }
-        /* systematic allocation of 32 samples over the bins */
-        int N[NB + 1];
-        float c = 0;
-        N[0] = 0;
-        for (int b = 0; b < NB; ++b) {
-            c = c + pm[b];
-            N[b + 1] = (b == NB - 1) ? NB : points_below(c);
-        }
+        /* cells between the inverse-CDF images of u = i / 32: one sample each */
         float len = (t1 - t0) / (float)NB;
-        int i = 0;
-        for (int b = 0; b < NB; ++b) {
-            int n = N[b + 1] - N[b];
-            if (n <= 0) continue;
-            float lo = bin_edge(t0, t1, len, b);
-            float hi = bin_edge(t0, t1, len, b + 1);
-            float step = (hi - lo) / (float)n;
-            for (int j = 0; j < n; ++j) {
-                float clo = lo + (float)j * step;
-                float chi = (j == n - 1) ? hi : lo + (float)(j + 1) * step;
-                float mid = (clo + chi) * 0.5f;
-                float pos[3];
-                for (int k = 0; k < 3; ++k) pos[k] = ro[k] + rd[k] * mid;
-                int32_t nd = atdfo_locate(p->root_min, p->root_max, p->table, p->depth, pos);
-                if (nd < 0) continue;
-                node[(int64_t)i * R + r] = nd;
-                tmin[(int64_t)i * R + r] = clo;
-                tmax[(int64_t)i * R + r] = chi;
+        float c = 0, prev = t0;
+        int i = 1, emitted = 0;
+        for (int b = 0; b <= NB; ++b) {
+            float T;
+            int last = (b == NB);
+            float c_hi = 0;
+            if (!last) c_hi = (b == NB - 1) ? 2.0f : c + pm[b];
+            while (1) {
+                if (last) {
+                    if (i > NB) break;
+                    T = t1;
+                } else {
+                    if (!(i < NB && (float)i * (1.0f / (float)NB) < c_hi)) break;
+                    float u = (float)i * (1.0f / (float)NB);
+                    float frac = (u - c) / pm[b];
+                    if (!(frac >= 0.0f)) frac = 0.0f;
+                    if (!(frac <= 1.0f)) frac = 1.0f;
+                    float lo = bin_edge(t0, t1, len, b);
+                    float hi = bin_edge(t0, t1, len, b + 1);
+                    T = lo + frac * (hi - lo);
+                }
                 ++i;
+                if (T > prev) {
+                    float a = 0.5f;
+                    if (p->jitter_mode == 2) a = atdfo_jitter(p->jitter_seed, r, emitted);
+                    float ts = cell_sample_t(prev, T, a);
+                    float pos[3];
+                    for (int k = 0; k < 3; ++k) pos[k] = ro[k] + rd[k] * ts;
+                    int32_t nd = atdfo_locate(p->root_min, p->root_max, p->table, p->depth, pos);
+                    if (nd >= 0) {
+                        node[(int64_t)emitted * R + r] = nd;
+                        tmin[(int64_t)emitted * R + r] = prev;
+                        tmax[(int64_t)emitted * R + r] = T;
+                        ++emitted;
+                    }
+                    prev = T;
+                }
             }
+            if (!last) c = c + pm[b];
         }
-        n_learned[r] = i;
+        n_learned[r] = emitted;
     }
 }
 
@@ -335,15 +348,20 @@ static void dense_back_d(const float* W, int nin, int nout, const double* x, con
     }
 }
 
-double atdfo_prop_grad(const float* params, const float* feat, const float* bins, int64_t R, double* grad)
+double atdfo_prop_grad(const float* params, const float* feat, const float* bins, int64_t R, float power,
+                       double* grad)
 {
     const float* P = params;
     memset(grad, 0, sizeof(double) * NPARAM);
     double loss = 0;
     for (int64_t r = 0; r < R; ++r) {
         const float* bm = bins + r * NB;
+        double qv[NB];
         double Q = 0;
-        for (int b = 0; b < NB; ++b) Q += bm[b] > 0 ? bm[b] : 0;
+        for (int b = 0; b < NB; ++b) {
+            qv[b] = bm[b] > 0 ? pow((double)bm[b], (double)power) : 0;
+            Q += qv[b];
+        }
         if (!(Q > 0)) continue;
         double x[NI], h1[NH], h2[NH], h3[NH], z[NB];
         for (int k = 0; k < NI; ++k) x[k] = feat[r * NI + k];
@@ -358,7 +376,7 @@ double atdfo_prop_grad(const float* params, const float* feat, const float* bins
         double lse = m + log(S);
         double dz[NB];
         for (int b = 0; b < NB; ++b) {
-            double q = (bm[b] > 0 ? bm[b] : 0) / Q;
+            double q = qv[b] / Q;
             double p = exp(z[b] - lse);
             loss += -q * (z[b] - lse);
             dz[b] = (p - q) / (double)R;

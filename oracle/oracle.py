@@ -242,6 +242,14 @@ class FusedModel:
             raise ValueError("too many intersections")
         return (P, B) if bins else P
 
+    def min_preact(self, o, d):
+        """Per-ray smallest |hidden pre-activation| (FP64); rays below ~1e-5
+        sit on a ReLU kink where FP32 and FP64 gradients may differ."""
+        o, d = _c(o, np.float32), _c(d, np.float32)
+        out = np.zeros(o.shape[0], np.float64)
+        lib().atdfo_project_min_preact(C.byref(self.m), _p(o), _p(d), C.c_int64(o.shape[0]), _p(out))
+        return out
+
     def project_bwd(self, o, d, dLdP):
         o, d = _c(o, np.float32), _c(d, np.float32)
         dLdP = _c(dLdP, np.float64)
@@ -305,7 +313,8 @@ PROP_BINS, PROP_IN, PROP_H = 32, 38, 64
 
 class PropStruct(C.Structure):
     _fields_ = [("root_min", _P), ("root_max", _P), ("table", _P), ("depth", C.c_int), ("coarse", _P),
-                ("params", _P), ("mix", C.c_float), ("kl_threshold", C.c_float)]
+                ("params", _P), ("mix", C.c_float), ("kl_threshold", C.c_float), ("jitter_mode", C.c_int),
+                ("jitter_seed", C.c_uint64)]
 
 
 def prop_num_params():
@@ -336,14 +345,15 @@ def locate(node_min, node_max, table, depth, p):
 class Proposal:
     """Holds the arrays the proposal oracle reads."""
 
-    def __init__(self, node_min, node_max, table, depth, coarse, params, mix=0.25, kl_threshold=0.5):
+    def __init__(self, node_min, node_max, table, depth, coarse, params, mix=0.25, kl_threshold=0.5, jitter_mode=0,
+                 jitter_seed=0):
         self.root_min = _c(np.asarray(node_min)[0], np.float32)
         self.root_max = _c(np.asarray(node_max)[0], np.float32)
         self.table = _c(table, np.int32)
         self.coarse = _c(coarse, np.float32)
         self.params = _c(params, np.float32)
         self.s = PropStruct(_p(self.root_min), _p(self.root_max), _p(self.table), depth, _p(self.coarse),
-                            _p(self.params), mix, kl_threshold)
+                            _p(self.params), mix, kl_threshold, jitter_mode, jitter_seed)
 
     def features(self, o, d):
         o, d = _c(o, np.float32), _c(d, np.float32)
@@ -373,10 +383,10 @@ class Proposal:
         return out
 
 
-def prop_grad(params, feat, bins):
+def prop_grad(params, feat, bins, power=1.0):
     params = _c(params, np.float32)
     feat = _c(feat, np.float32)
     bins = _c(bins, np.float32)
     g = np.zeros(params.shape[0], np.float64)
-    loss = lib().atdfo_prop_grad(_p(params), _p(feat), _p(bins), C.c_int64(feat.shape[0]), _p(g))
+    loss = lib().atdfo_prop_grad(_p(params), _p(feat), _p(bins), C.c_int64(feat.shape[0]), C.c_float(power), _p(g))
     return loss, g

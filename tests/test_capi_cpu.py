@@ -48,6 +48,26 @@ def test_validation_without_device_work():
     assert lib.atdf_scan_workspace_bytes(1024) == 8
 
 
+def test_proposal_boundary_without_device_work(oracle):
+    import ctypes as C
+
+    from adaptivetensordf_amd import capi, phantom
+
+    lib = capi.lib()
+    assert lib.atdf_proposal_num_params() == oracle.prop_num_params() == phantom.proposal_num_params()
+    assert lib.atdf_leaf_table_numel(5) == 32 ** 3 and lib.atdf_leaf_table_numel(0) == 0
+    assert lib.atdf_leaf_table(C.c_void_p(1), 9, 9, C.c_void_p(1), None) == capi.ERR_INVALID_ARGUMENT
+    assert b"depth" in lib.atdf_last_error()
+    p = capi.Proposal(None, 5, 0.25, 0.5, 0, 0, None, None)
+    st = lib.atdf_proposal_sample(C.c_void_p(1), 9, None, None, 4, C.byref(p), *([None] * 7))
+    assert st == capi.ERR_INVALID_ARGUMENT and b"incomplete proposal" in lib.atdf_last_error()
+    p = capi.Proposal(C.c_void_p(1), 5, 1.5, 0.5, 0, 0, C.c_void_p(1), C.c_void_p(1))
+    st = lib.atdf_proposal_features(C.c_void_p(1), 9, None, None, 4, C.byref(p), None, None, None)
+    assert st == capi.ERR_INVALID_ARGUMENT and b"mix" in lib.atdf_last_error()
+    assert lib.atdf_proposal_grad(None, None, None, 0, 1.0, None, None, None, 0, None) == capi.ERR_INVALID_ARGUMENT
+    assert lib.atdf_proposal_grad_workspace_bytes(64) > 64 * 455 * 4
+
+
 def test_host_tree_matches_oracle(oracle, native):
     t = native.HyperTreeBase(3, 3)
     t.settree()

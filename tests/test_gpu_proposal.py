@@ -1,0 +1,363 @@
+"""Learned sample proposal on the GPU against the oracle.
+
+Bit-exact: the point-location table, ray features, logits-driven placement
+(fallback flags, KL, per-ray learned intervals) and the mixed interval list.
+Within tolerance: projections over learned intervals (1e-5 relative to the
+FP64 oracle), the binned forward (1e-5), the training gradient (1e-5 of
+max|g| against the FP64 oracle).  The gain test trains a reconstruction and
+the proposal on the GPU and shows the trained proposal's 32 samples per ray
+beat both 32 stratified samples and the untrained proposal against a
+512-sample stratified judge.
+"""
+import numpy as np
+import pytest
+import torch
+
+from conftest import edge_case_rays, random_rays
+from scene_util import adaptive_active, device_tree, rays_to_device, tilt_rays
+
+pytestmark = pytest.mark.gpu
+NB = 32
+
+
+def _scene(oracle, native, depth=4, budget=300, seed=0, holes=3):
+    t = oracle.tree_build(depth)
+    act = adaptive_active(t, num_ellipsoids=20, seed=seed, budget=budget, start_depth=1).astype(np.int32)
+    # culled regions: drop a few active nodes (their cells have no active node)
+    ids = np.nonzero(act)[0]
+    rng = np.random.default_rng(seed)
+    act[rng.choice(ids, holes, replace=False)] = 0
+    tree = device_tree(depth, act)
+    return t, act, tree
+
+
+def _rays(n, seed):
+    o1, d1 = random_rays(n // 2, 50 + seed, spread=1.0)
+    o2, d2 = tilt_rays(n - n // 2, 60 + seed)
+    o3, d3 = edge_case_rays()
+    return np.concatenate([o1, o2, o3]), np.concatenate([d1, d2, d3])
+
+
+def _state(native, oracle, t, act, mix, tau, seed=0, jitter_mode=0, jitter_seed=0):
+    from adaptivetensordf_amd import phantom
+
+    pp = native.ProposalParams()
+    pp.mix = mix
+    pp.kl_threshold = tau
+    params = phantom.init_proposal(seed)
+    ps = native.MakeProposalState(params.cuda(), 1e-3, pp)
+    coarse = np.random.default_rng(seed + 1).uniform(0.05, 1.0, t["node_min"].shape[0]).astype(np.float32)
+    coarse[act == 0] = 0
+    coarse[np.random.default_rng(seed + 2).random(coarse.shape[0]) < 0.1] = 0  # some empty active nodes
+    ps.coarse = torch.from_numpy(coarse).cuda()
+    depth = int(t["depth"].max())
+    table = oracle.leaf_table(t["node_min"], t["node_max"], t["children"], act, depth)
+    prop = oracle.Proposal(t["node_min"], t["node_max"], table, depth, coarse, params.numpy(), mix, tau, jitter_mode,
+                           jitter_seed)
+    return ps, prop, table
+
+
+def _sp(native, K=8, jitter_mode=0, seed=0):
+    sp = native.SamplingParams()
+    sp.max_samples_per_node = K
+    sp.jitter_mode = jitter_mode
+    sp.jitter_seed = seed
+    return sp
+
+
+def test_leaf_table_bit_exact(oracle, native, cuda):
+    for depth, budget in ((3, 120), (5, 900)):
+        t, act, tree = _scene(oracle, native, depth=depth, budget=budget)
+        ref = oracle.leaf_table(t["node_min"], t["node_max"], t["children"], act, depth)
+        got = tree.LeafTable().cpu().numpy()
+        print(f"depth {depth}: {int(act.sum())} active, {np.count_nonzero(ref < 0)} empty cells", flush=True)
+        assert np.array_equal(got, ref)
+
+
+def test_features_and_placement_bit_exact(oracle, native, cuda):
+    t, act, tree = _scene(oracle, native)
+    o, d = _rays(4000, 0)
+    rays = native.RayList(*rays_to_device(o, d))
+    ps, prop, _ = _state(native, oracle, t, act, mix=0.3, tau=1e30)
+    f_ref, ch_ref = prop.features(o, d)
+    f, ch = native.ProposalFeatures(tree, ps, rays)
+    assert np.array_equal(f.cpu().numpy(), f_ref) and np.array_equal(ch.cpu().numpy(), ch_ref)
+    # threshold at the median KL so both branches run
+    kl_all = prop.sample(o, d)["kl"]
+    tau = float(np.median(kl_all[ch_ref[:, 1] > ch_ref[:, 0]]))
+    for mix, jm in ((0.3, 0), (0.0, 2), (1.0, 0), (0.5, 2)):
+        ps, prop, _ = _state(native, oracle, t, act, mix=mix, tau=tau, jitter_mode=jm, jitter_seed=77)
+        ref = prop.sample(o, d)
+        fb, nl, node, lo, hi, kl = [x.cpu().numpy() for x in native.ProposalSample(tree, ps, rays,
+                                                                                   _sp(native, 8, jm, 77))]
+        print(f"mix {mix} jitter {jm}: fallback {fb.sum()} / {fb.size}, learned samples {nl.sum()}", flush=True)
+        assert np.array_equal(fb, ref["fallback"]) and np.array_equal(nl, ref["n_learned"])
+        assert np.array_equal(kl.view(np.uint32), ref["kl"].view(np.uint32))
+        assert 0 < fb.sum() < fb.size or mix == 1.0
+        slot = np.arange(NB)[:, None] < nl[None, :]
+        assert np.array_equal(node[slot], ref["node"][slot])
+        assert np.array_equal(lo[slot].view(np.uint32), ref["tmin"][slot].view(np.uint32))
+        assert np.array_equal(hi[slot].view(np.uint32), ref["tmax"][slot].view(np.uint32))
+
+
+def _oracle_mixed_list(oracle, t, act, prop, o, d, K, jitter_mode, seed):
+    """The mixed interval list built on the CPU: learned rays from the
+    proposal oracle, fallback rays from the traversal oracle."""
+    s = prop.sample(o, d)
+    tr = oracle.traverse(t["node_min"], t["node_max"], oracle.active_ids(act), o, d, K, jitter_mode=jitter_mode,
+                         seed=seed)
+    R = o.shape[0]
+    node, lo, hi, cnt, first, ray, off = [], [], [], [], [], [], [0]
+    for r in range(R):
+        if s["fallback"][r]:
+            a, b = tr["ray_interval_offset"][r], tr["ray_interval_offset"][r + 1]
+            node += list(tr["iv_node"][a:b])
+            lo += list(tr["iv_tmin"][a:b])
+            hi += list(tr["iv_tmax"][a:b])
+            cnt += list(tr["iv_count"][a:b])
+            first += list(tr["iv_first"][a:b])
+            ray += [r] * (b - a)
+        else:
+            n = s["n_learned"][r]
+            node += list(s["node"][:n, r])
+            lo += list(s["tmin"][:n, r])
+            hi += list(s["tmax"][:n, r])
+            cnt += [1] * n
+            first += list(range(n))
+            ray += [r] * n
+        off.append(len(node))
+    return (np.array(off, np.int64), np.array(node, np.int32), np.array(lo, np.float32), np.array(hi, np.float32),
+            np.array(cnt, np.int32), np.array(first, np.int32), np.array(ray, np.int32), s["fallback"])
+
+
+@pytest.mark.parametrize("jitter_mode", [0, 2])
+def test_mixed_interval_list_and_projection(oracle, native, cuda, jitter_mode):
+    from adaptivetensordf_amd import phantom
+
+    t, act, tree = _scene(oracle, native, seed=1)
+    o, d = _rays(1500, 1)
+    rays = native.RayList(*rays_to_device(o, d))
+    ps, prop, _ = _state(native, oracle, t, act, mix=0.25, tau=1e30, seed=1)
+    kl = prop.sample(o, d)["kl"]
+    tau = float(np.quantile(kl[kl > 0], 0.7))
+    ps, prop, _ = _state(native, oracle, t, act, mix=0.25, tau=tau, seed=1, jitter_mode=jitter_mode, jitter_seed=99)
+    K = 5
+    sp = native.SamplingParams()
+    sp.max_samples_per_node = K
+    sp.jitter_mode = jitter_mode
+    sp.jitter_seed = 99
+    iv, fb = native.CreateProposalIntervals(tree, ps, rays, sp)
+    off, node, lo, hi, cnt, first, ray, fb_ref = _oracle_mixed_list(oracle, t, act, prop, o, d, K, jitter_mode, 99)
+    print(f"jitter {jitter_mode}: {iv.num_intervals} intervals, {int(fb.sum())} fallback rays", flush=True)
+    assert np.array_equal(fb.cpu().numpy(), fb_ref)
+    assert iv.num_intervals == node.shape[0] and np.array_equal(iv.ray_offset.cpu().numpy(), off)
+    assert np.array_equal(iv.node.cpu().numpy(), node) and np.array_equal(iv.count.cpu().numpy(), cnt)
+    assert np.array_equal(iv.tmin.cpu().numpy().view(np.uint32), lo.view(np.uint32))
+    assert np.array_equal(iv.tmax.cpu().numpy().view(np.uint32), hi.view(np.uint32))
+    assert np.array_equal(iv.first.cpu().numpy(), first) and np.array_equal(iv.ray.cpu().numpy(), ray)
+    assert iv.num_samples == int(cnt.sum())
+    # projection over the mixed list against the oracle over the same list
+    C_, G = 16, 7
+    na = int(act.sum())
+    grid = phantom.init_grid(na, G, C_, std=0.3, seed=3)
+    mlp = phantom.init_mlp(C_, 64, seed=3)
+    fp = native.FieldParams()
+    fp.C, fp.G = C_, G
+    P = native.ProjectIntervals(tree, rays, iv, grid.cuda(), mlp.cuda(), sp, fp).cpu().numpy().astype(np.float64)
+    fm = oracle.FusedModel(t["node_min"], t["node_max"], act, grid.numpy(), mlp.numpy(), C_, K=K,
+                           jitter_mode=jitter_mode, seed=99)
+    fm.set_intervals(off, node, lo, hi, cnt)
+    P64 = fm.project(o, d, precision=1)
+    scale = np.maximum(np.abs(P64), 1e-3 * np.abs(P64).max())
+    err = np.max(np.abs(P - P64) / scale)
+    print(f"projection over the mixed list: max rel err {err:.2e}", flush=True)
+    assert err <= 1e-5
+    # adjoint over the mixed list against the FP64 oracle adjoint, on the rays
+    # whose samples sit clear of ReLU kinks (|pre-activation| >= 3e-6, above
+    # the FP32 rounding of a 64-term dot product)
+    clear = fm.min_preact(o, d) >= 3e-6
+    print(f"rays on a ReLU kink (left out of the gradient check): {int((~clear).sum())}", flush=True)
+    assert clear.mean() > 0.9
+    dP = (2 * P / o.shape[0] * clear).astype(np.float32)
+    g_g, g_m = native.ProjectIntervalsBackward(tree, rays, iv, grid.cuda(), mlp.cuda(), sp, fp,
+                                               torch.from_numpy(dP).cuda())
+    gg, gm = fm.project_bwd(o, d, dP.astype(np.float64))
+    e_g = np.abs(g_g.cpu().numpy() - gg).max() / np.abs(gg).max()
+    e_m = np.abs(g_m.cpu().numpy() - gm).max() / np.abs(gm).max()
+    print(f"adjoint over the mixed list: grid {e_g:.2e}, mlp {e_m:.2e}", flush=True)
+    assert e_g <= 1e-5 and e_m <= 1e-5
+
+
+def test_binned_forward(oracle, native, cuda):
+    from adaptivetensordf_amd import phantom
+
+    t, act, tree = _scene(oracle, native, seed=2, holes=0)
+    o, d = _rays(1200, 2)
+    rays = native.RayList(*rays_to_device(o, d))
+    C_, G, K = 8, 6, 4
+    na = int(act.sum())
+    grid = phantom.init_grid(na, G, C_, std=0.4, seed=4)
+    mlp = phantom.init_mlp(C_, 64, seed=4)
+    fp = native.FieldParams()
+    fp.C, fp.G = C_, G
+    sp = native.SamplingParams()
+    sp.max_samples_per_node = K
+    sp.jitter_mode = 2
+    sp.jitter_seed = 5
+    P, B = native.ProjectRaysBins(tree, rays, grid.cuda(), mlp.cuda(), sp, fp)
+    P, B = P.cpu().numpy().astype(np.float64), B.cpu().numpy().astype(np.float64)
+    fm = oracle.FusedModel(t["node_min"], t["node_max"], act, grid.numpy(), mlp.numpy(), C_, K=K, jitter_mode=2,
+                           seed=5)
+    P64, B64 = fm.project(o, d, precision=1, bins=NB)
+    scale = np.maximum(np.abs(P64), 1e-3 * np.abs(P64).max())[:, None]
+    err_b = np.max(np.abs(B - B64) / scale)
+    err_p = np.max(np.abs(P - P64) / scale[:, 0])
+    print(f"binned forward: P {err_p:.2e}, bins {err_b:.2e}", flush=True)
+    assert err_p <= 1e-5 and err_b <= 1e-5
+    assert np.allclose(B.sum(1), P, rtol=1e-5, atol=1e-6)
+
+
+def test_training_gradient_matches_oracle(oracle, native, cuda):
+    from adaptivetensordf_amd import phantom
+
+    rng = np.random.default_rng(7)
+    for R in (1, 200, 3001):
+        feat = rng.normal(size=(R, 38)).astype(np.float32)
+        feat[:, 6:] = np.abs(feat[:, 6:])
+        bins = (rng.uniform(0, 1, (R, NB)) ** 4).astype(np.float32)
+        bins[::7] = 0  # rays without mass
+        params = phantom.init_proposal(R)
+        pp = native.ProposalParams()
+        ps = native.MakeProposalState(params.cuda(), 1e-3, pp)
+        pp.target_power = 0.5 if R != 200 else 1.0
+        ps = native.MakeProposalState(params.cuda(), 1e-3, pp)
+        loss = native.ProposalGrad(ps, torch.from_numpy(feat).cuda(), torch.from_numpy(bins).cuda()).item()
+        l_ref, g_ref = oracle.prop_grad(params.numpy(), feat, bins, pp.target_power)
+        g = ps.grad.cpu().numpy().astype(np.float64)
+        err = np.abs(g - g_ref).max() / max(np.abs(g_ref).max(), 1e-30)
+        print(f"R={R}: loss {loss:.6f} vs {l_ref:.6f}, grad rel err {err:.2e}", flush=True)
+        assert abs(loss - l_ref) <= 1e-5 * max(abs(l_ref), 1e-6)
+        assert err <= 1e-5
+
+
+def _train_scene(native, oracle, seed=0, depth=4, budget=500, W=64, steps=150, nrays=1 << 14):
+    """A reconstruction trained on the GPU for `steps` plain steps on analytic
+    phantom projections (a field with structure for the proposal to learn)."""
+    from adaptivetensordf_amd import phantom
+
+    t = oracle.tree_build(depth)
+    ell = phantom.ellipsoids(20, seed)
+    act = phantom.adaptive_active_mask(t["node_min"], t["node_max"], t["children"], t["depth"], ell, budget, 1,
+                                       depth).astype(np.int32)
+    tree = device_tree(depth, act)
+    C_, G = 16, 7
+    grid = phantom.init_grid(int(act.sum()), G, C_, std=0.01, seed=seed)
+    mlp = phantom.init_mlp(C_, 64, seed=seed)
+    fp = native.FieldParams()
+    fp.C, fp.G = C_, G
+    sp = native.SamplingParams()
+    sp.max_samples_per_node = 8
+    sp.jitter_mode = 2
+    ang = phantom.tilt_angles(61)
+    batches = []
+    for b in range(4):
+        tl, px, py = phantom.sample_pixels(nrays, 61, W, W, 100 + b)
+        o, d = phantom.rays_for_pixels(tl, px, py, W, W, ang)
+        tgt = phantom.analytic_projection(o.astype(np.float64), d.astype(np.float64), ell).astype(np.float32)
+        batches.append((native.RayList(*rays_to_device(o, d)), torch.from_numpy(tgt).cuda()))
+    st = native.MakeTrainState(grid.cuda(), mlp.cuda(), False, 1e-2, 1e-3)
+    for i in range(steps):
+        sp.jitter_seed = i
+        native.TrainStep(tree, st, batches[i % 4][0], batches[i % 4][1], sp, fp)
+    return dict(t=t, act=act, tree=tree, st=st, fp=fp, sp=sp, batches=batches, ell=ell, W=W)
+
+
+def _k_for(native, tree, rays, target):
+    best = None
+    for K in range(1, 64):
+        sp = native.SamplingParams()
+        sp.max_samples_per_node = K
+        iv = tree.CreateIntervalsForRays(rays, sp)
+        mean = iv.num_samples / rays.size()
+        if best is None or abs(mean - target) < abs(best[1] - target):
+            best = (K, mean)
+        if mean > target:
+            break
+    return best
+
+
+def test_trained_proposal_gains(oracle, native, cuda):
+    """What training the proposal buys, on a reconstruction trained on the
+    GPU: projection error of 32 learned cells per ray against a ~250-sample
+    stratified judge, for the trained network, the same network untrained,
+    and the uniform pdf (mix = 1); stratified sampling at ~32 and ~128
+    samples per ray is printed beside it."""
+    from adaptivetensordf_amd import phantom
+
+    sc = _train_scene(native, oracle)
+    tree, st, fp = sc["tree"], sc["st"], sc["fp"]
+    ang = phantom.tilt_angles(61)
+    tl, px, py = phantom.sample_pixels(4096, 61, sc["W"], sc["W"], 999)
+    o, d = phantom.rays_for_pixels(tl, px, py, sc["W"], sc["W"], ang)
+    rays = native.RayList(*rays_to_device(o, d))
+
+    def strat(target):
+        K, s = _k_for(native, tree, rays, target)
+        sp = _sp(native, K)
+        return native.ProjectIntervals(tree, rays, tree.CreateIntervalsForRays(rays, sp), st.grid, st.mlp, sp, fp), s
+
+    P_ref, s_ref = strat(512)
+    P_s32, s32 = strat(32)
+    P_s128, s128 = strat(128)
+    pp = native.ProposalParams()
+    pp.mix = 0.3
+    pp.kl_threshold = 1e30
+    ps = native.MakeProposalState(phantom.init_proposal(0).cuda(), 3e-3, pp)
+    native.ProposalRefreshCoarse(tree, ps, st.grid, st.mlp, fp)
+    sp0 = _sp(native)
+
+    def learned_P(state):
+        iv, fb = native.CreateProposalIntervals(tree, state, rays, sp0)
+        assert int(fb.sum()) == 0
+        return native.ProjectIntervals(tree, rays, iv, st.grid, st.mlp, sp0, fp), iv.num_samples / rays.size()
+
+    P_un, n_un = learned_P(ps)
+    pp_u = native.ProposalParams()
+    pp_u.mix = 1.0
+    pp_u.kl_threshold = 1e30
+    ps_u = native.MakeProposalState(phantom.init_proposal(0).cuda(), 3e-3, pp_u)
+    native.ProposalRefreshCoarse(tree, ps_u, st.grid, st.mlp, fp)
+    P_uni, n_uni = learned_P(ps_u)
+    sp_t = _sp(native, _k_for(native, tree, sc["batches"][0][0], 128)[0], 2, 5)
+    losses = [native.ProposalTrain(tree, ps, sc["batches"][0][0], sp_t, st.grid, st.mlp, fp, 1)]
+    losses.append(native.ProposalTrain(tree, ps, sc["batches"][1][0], sp_t, st.grid, st.mlp, fp, 300))
+    P_tr, n_tr = learned_P(ps)
+    mse = lambda P: float(((P.double() - P_ref.double()) ** 2).mean().item())  # noqa: E731
+    e = dict(strat32=mse(P_s32), strat128=mse(P_s128), untrained=mse(P_un), uniform=mse(P_uni), trained=mse(P_tr))
+    print(f"judge {s_ref:.1f} samples/ray; MSE vs judge: stratified {s32:.1f}/ray {e['strat32']:.3e}, "
+          f"stratified {s128:.1f}/ray {e['strat128']:.3e}, uniform cells ({n_uni:.1f}/ray) {e['uniform']:.3e}, "
+          f"untrained ({n_un:.1f}/ray) {e['untrained']:.3e}, trained ({n_tr:.1f}/ray) {e['trained']:.3e}; "
+          f"proposal loss {losses[0]:.4f} -> {losses[1]:.4f}", flush=True)
+    assert losses[1] < losses[0]
+    assert n_tr <= 32
+    assert e["trained"] < e["untrained"] and e["trained"] < e["uniform"]
+
+
+def test_learned_training_step_reduces_loss(oracle, native, cuda):
+    sc = _train_scene(native, oracle, steps=20)
+    tree, st, fp = sc["tree"], sc["st"], sc["fp"]
+    pp = native.ProposalParams()
+    pp.mix = 0.25
+    pp.kl_threshold = 0.5
+    from adaptivetensordf_amd import phantom
+
+    ps = native.MakeProposalState(phantom.init_proposal(1).cuda(), 3e-3, pp)
+    sp = _sp(native, 8, 2, 0)
+    native.ProposalTrain(tree, ps, sc["batches"][0][0], sp, st.grid, st.mlp, fp, 100)
+    losses = []
+    for i in range(60):
+        sp.jitter_seed = 1000 + i
+        rays, tgt = sc["batches"][i % 4]
+        losses.append(native.TrainStepLearned(tree, st, ps, rays, tgt, sp, fp).item())
+    print(f"learned-sampling training loss {losses[0]:.4e} -> {losses[-1]:.4e}", flush=True)
+    assert np.mean(losses[-8:]) < 0.8 * np.mean(losses[:8])

@@ -142,56 +142,58 @@ def test_forward_matches_torch(oracle):
 
 
 def test_sample_placement(oracle):
+    """Learned rays: at most 32 cells, ordered, tiling the root chord, with
+    boundaries at multiples of 1/32 of the FP64 CDF of the mixed pdf; each
+    cell's (jittered) sample lies in its node.  KL and the fallback gate
+    against numpy."""
     depth = 3
     t, act = _scene(oracle, depth)
     mix, tau = 0.3, 0.8
-    table, coarse, params, prop = _proposal(oracle, t, act, depth, mix=mix, tau=tau)
     o, d = random_rays(400, 5, spread=1.0)
-    f, ch = prop.features(o, d)
-    _, p = prop.forward(f)
-    s = prop.sample(o, d)
-    pm = p.astype(np.float64) * (1 - mix) + mix / NB
-    n_learned_rays = 0
-    for r in range(400):
-        t0, t1 = float(ch[r, 0]), float(ch[r, 1])
-        if not t1 > t0:
-            assert s["n_learned"][r] == 0 and s["fallback"][r] == 0
-            continue
-        occ = f[r, 6:].astype(np.float64)
-        q = occ / occ.sum()
-        nz = occ > 0
-        kl = float(np.sum(q[nz] * np.log(q[nz] / pm[r, nz])))
-        assert abs(s["kl"][r] - kl) <= 1e-5 * max(1.0, kl)
-        if abs(kl - tau) > 1e-4:
-            assert s["fallback"][r] == (1 if kl > tau else 0)
-        if s["fallback"][r]:
-            assert s["n_learned"][r] == 0
-            continue
-        n_learned_rays += 1
-        cdf = np.concatenate([[0.0], np.cumsum(pm[r])])
-        pts = (np.arange(NB) + 0.5) / NB
-        near = np.min(np.abs(cdf[1:-1, None] - pts[None, :])) < 1e-5
-        cnt = np.array([np.sum((pts >= cdf[b]) & (pts < cdf[b + 1])) for b in range(NB)])
-        cnt[-1] += np.sum(pts >= cdf[-1])
-        n = s["n_learned"][r]
-        assert n <= NB
-        lo, hi, nd = s["tmin"][:n, r], s["tmax"][:n, r], s["node"][:n, r]
-        assert np.all(hi > lo) and np.all(np.diff(lo) > 0)
-        # every sample's node box holds its cell midpoint
-        mid = o[r].astype(np.float64) + d[r].astype(np.float64) * (0.5 * (lo.astype(np.float64) + hi))[:, None]
-        assert np.all(mid >= t["node_min"][nd] - 1e-6) and np.all(mid <= t["node_max"][nd] + 1e-6)
-        assert np.all(act[nd] == 1)
-        # cells tile their bins: per bin, count and total width
-        edges = np.float32(t0) + np.arange(NB + 1, dtype=np.float32) * np.float32((t1 - t0) / np.float32(NB))
-        edges[-1] = t1
-        b_of = np.clip(np.searchsorted(edges, 0.5 * (lo + hi), side="right") - 1, 0, NB - 1)
-        got = np.bincount(b_of, minlength=NB)
-        if not near:
-            assert np.array_equal(got, cnt), (r, got, cnt)
-        for b in np.nonzero(got)[0]:
-            w = (hi[b_of == b].astype(np.float64) - lo[b_of == b]).sum()
-            assert abs(w - (edges[b + 1] - edges[b])) <= 1e-6
-    assert n_learned_rays > 100
+    for jitter_mode, seed in ((0, 0), (2, 17)):
+        table, coarse, params, prop = _proposal(oracle, t, act, depth, mix=mix, tau=tau)
+        prop = oracle.Proposal(t["node_min"], t["node_max"], table, depth, coarse, params, mix, tau, jitter_mode, seed)
+        f, ch = prop.features(o, d)
+        _, p = prop.forward(f)
+        s = prop.sample(o, d)
+        pm = p.astype(np.float64) * (1 - mix) + mix / NB
+        n_learned_rays = 0
+        for r in range(400):
+            t0, t1 = float(ch[r, 0]), float(ch[r, 1])
+            if not t1 > t0:
+                assert s["n_learned"][r] == 0 and s["fallback"][r] == 0
+                continue
+            occ = f[r, 6:].astype(np.float64)
+            q = occ / occ.sum()
+            nz = occ > 0
+            kl = float(np.sum(q[nz] * np.log(q[nz] / pm[r, nz])))
+            assert abs(s["kl"][r] - kl) <= 1e-5 * max(1.0, kl)
+            if abs(kl - tau) > 1e-4:
+                assert s["fallback"][r] == (1 if kl > tau else 0)
+            if s["fallback"][r]:
+                assert s["n_learned"][r] == 0
+                continue
+            n_learned_rays += 1
+            n = s["n_learned"][r]
+            assert 0 < n <= NB
+            lo, hi, nd = s["tmin"][:n, r], s["tmax"][:n, r], s["node"][:n, r]
+            # the tree has no holes: the cells tile [t0, t1]
+            assert lo[0] == np.float32(t0) and hi[-1] == np.float32(t1)
+            assert np.array_equal(lo[1:], hi[:-1]) and np.all(hi > lo)
+            # equal-probability cells: CDF at each inner boundary is k/32
+            lenb = (t1 - t0) / NB
+            cdf = np.concatenate([[0.0], np.cumsum(pm[r])])
+            b = np.clip(((hi[:-1].astype(np.float64) - t0) / lenb).astype(int), 0, NB - 1)
+            F = cdf[b] + pm[r, b] * (hi[:-1] - (t0 + b * lenb)) / lenb
+            assert np.all(np.abs(F * NB - np.round(F * NB)) < 1e-3)  # float rounding of T
+            assert np.all(np.diff(np.round(F * NB)) >= 1)
+            # each cell's sample (midpoint or jittered) lies in its node
+            a = np.full(n, 0.5) if jitter_mode == 0 else np.array([oracle.jitter(seed, r, i) for i in range(n)])
+            ts = lo.astype(np.float64) * (1 - a) + hi * a
+            pos = o[r].astype(np.float64) + d[r].astype(np.float64) * ts[:, None]
+            assert np.all(pos >= t["node_min"][nd] - 1e-6) and np.all(pos <= t["node_max"][nd] + 1e-6)
+            assert np.all(act[nd] == 1)
+        assert n_learned_rays > 100
 
 
 def test_uniform_pdf_closed_form(oracle):
@@ -278,7 +280,8 @@ def test_training_gradient(oracle):
     rng = np.random.default_rng(0)
     bins = rng.uniform(0, 1, (50, NB)).astype(np.float32) ** 3
     bins[3] = 0
-    loss, g = oracle.prop_grad(params, f, bins)
+    power = 0.5
+    loss, g = oracle.prop_grad(params, f, bins, power)
     P = torch.from_numpy(params.astype(np.float64)).requires_grad_()
     # rebuild the MLP on the differentiable parameter vector
     h = torch.from_numpy(f.astype(np.float64))
@@ -290,7 +293,7 @@ def test_training_gradient(oracle):
         h = h @ W.T + b
         if i < 3:
             h = torch.relu(h)
-    q = torch.from_numpy(bins.astype(np.float64))
+    q = torch.from_numpy(bins.astype(np.float64)) ** power
     mass = q.sum(1, keepdim=True)
     valid = (mass[:, 0] > 0)
     qn = torch.where(mass > 0, q / mass.clamp_min(1e-300), torch.zeros_like(q))
@@ -307,7 +310,7 @@ def test_training_gradient(oracle):
         pp[idx] += e
         pm_[idx] -= e
         e_eff = (np.float64(pp[idx]) - np.float64(pm_[idx])) / 2
-        lp, _ = oracle.prop_grad(pp, f, bins)
-        lm, _ = oracle.prop_grad(pm_, f, bins)
+        lp, _ = oracle.prop_grad(pp, f, bins, power)
+        lm, _ = oracle.prop_grad(pm_, f, bins, power)
         fd = (lp - lm) / (2 * e_eff)
         assert abs(fd - g[idx]) <= 1e-4 * max(1e-3, abs(g[idx])) + 1e-7, (idx, fd, g[idx])
