@@ -2,13 +2,23 @@
 """Benchmark: fwd+bwd rays/s of one training step of the reconstruction hot
 path on one MI355X.
 
-Workload (default, the 512^3 configuration): 200-ellipsoid phantom (seed 1),
-61 tilts from -60 to +60 degrees, 512 x 512 detector, adaptive octree up to
-depth 5, 11^3 x 16-channel node grids, batches of 2^18 rays, stratified
-samples with K chosen so the mean is ~128 per ray.  One step = traversal ->
+Workload (the 512^3 configuration): 200-ellipsoid phantom (seed 1), 61 tilts
+from -60 to +60 degrees, 512 x 512 detector, adaptive octree up to depth 5,
+11^3 x 16-channel node grids, batches of 2^18 rays.  One step = sampling ->
 fused forward -> MSE -> fused adjoint -> Adam on every grid and decoder
 parameter, all on the device.  Inputs (rays, targets) are resident in HBM
 before the timed region.
+
+Modes:
+  plain    FP32 grids and decoder, stratified samples (~128 per ray)
+  bf16     BF16 grids (FP32 masters) and BF16-MFMA decoder, stratified samples
+  learned  (default) BF16 as above with the learned proposal: 32 samples per
+           ray, stratified ~128 on the KL fallback rays, the proposal trained
+           online every 100 iterations inside the timed loop.  The plain FP32
+           leg runs in the same process; both legs train the same number of
+           iterations from the same initial field and are then judged on
+           held-out rays with the stratified ~128-sample projection against
+           the analytic targets ("projection MSE").
 
 Prints one JSON line (see the keys below).
 """
@@ -35,9 +45,14 @@ BF16_PEAK_TFLOPS = 2500.0
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--steps", type=int, default=100)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--mode", choices=["plain", "bf16"], default="plain")
+    ap.add_argument("--mode", choices=["plain", "bf16", "learned"], default="learned")
+    ap.add_argument("--train-iters", type=int, default=300,
+                    help="learned mode: iterations each leg trains before the projection-MSE check")
+    ap.add_argument("--mix", type=float, default=0.3, help="learned mode: uniform mixing of the proposal pdf")
+    ap.add_argument("--kl-threshold", type=float, default=0.3, help="learned mode: fallback threshold")
+    ap.add_argument("--proposal-rays", type=int, default=1 << 15, help="rays per proposal training pass")
     ap.add_argument("--volume", type=int, default=512)
     ap.add_argument("--rays", type=int, default=1 << 18)
     ap.add_argument("--batches", type=int, default=4)
@@ -74,7 +89,7 @@ def main():
         log("copied 3 x 512 MiB (read 512 MiB + write 512 MiB per copy)")
         return
 
-    from adaptivetensordf_amd import capi, native, phantom
+    from adaptivetensordf_amd import native, phantom
 
     N = native()
     dev = torch.device("cuda:0")
@@ -95,13 +110,15 @@ def main():
     tree.to("cuda")
     num_active = int(tree.NumActiveNodes())
     angles = phantom.tilt_angles(61)
-    batches = []
-    for b in range(args.batches):
-        tl, px, py = phantom.sample_pixels(args.rays, 61, n, n, 100 + b)
+
+    def make_batch(nrays, seed):
+        tl, px, py = phantom.sample_pixels(nrays, 61, n, n, seed)
         o, d = phantom.rays_for_pixels(tl, px, py, n, n, angles)
         o_t, d_t = torch.from_numpy(o).to(dev), torch.from_numpy(d).to(dev)
         tgt = phantom.analytic_projection_torch(o_t, d_t, ell).float()
-        batches.append((N.RayList(o_t, d_t), tgt.contiguous()))
+        return N.RayList(o_t, d_t), tgt.contiguous()
+
+    batches = [make_batch(args.rays, 100 + b) for b in range(args.batches)]
     # K so that the mean number of samples per ray is ~samples_per_ray
     sp = N.SamplingParams()
     sp.jitter_mode = 2
@@ -116,48 +133,227 @@ def main():
             break
     K, s_eff = best
     sp.max_samples_per_node = K
-    fp = N.FieldParams()
-    fp.C, fp.G, fp.H = C, G, H
-    fp.bf16_decoder = args.mode == "bf16"
     grid0 = phantom.init_grid(num_active, G, C, std=0.01, seed=0, device=dev)
     mlp0 = phantom.init_mlp(C, H, seed=0).to(dev)
-    st = N.MakeTrainState(grid0, mlp0, args.mode == "bf16", 1e-3, 1e-3)
     log(f"setup {time.time() - t0:.1f}s: active nodes {num_active}, K {K}, mean samples/ray {s_eff:.1f}")
 
-    # ------------------------------------------------------------ timed steps
-    def step(i):
-        rays, tgt = batches[i % len(batches)]
-        return N.TrainStep(tree, st, rays, tgt, sp, fp)
+    def field(bf16):
+        fp = N.FieldParams()
+        fp.C, fp.G, fp.H = C, G, H
+        fp.bf16_decoder = bf16
+        return fp
 
-    for i in range(args.warmup):
-        step(i)
-    torch.cuda.synchronize()
     steps = args.profile_steps if args.profile_steps > 0 else args.steps
+
+    if args.mode != "learned":
+        bf16 = args.mode == "bf16"
+        leg = run_leg(N, tree, batches, sp, field(bf16), grid0, mlp0, bf16, None, args, steps)
+        if args.profile_steps > 0:
+            return
+        out_leg = leg
+        kern = kernel_times(N, tree, batches[0][0], sp, field(bf16), leg["state"], bf16, None)
+        plain_summary = None
+        mse = None
+        workload = (f"{'plain FP32' if not bf16 else 'BF16 grids'} training step, {n}^3 phantom (200 ellipsoids), "
+                    f"61 tilts, {n}x{n} detector, adaptive octree depth<=5")
+        dtype = "f32" if not bf16 else "bf16 grids / f32 accumulate"
+    else:
+        plain = run_leg(N, tree, batches, sp, field(False), grid0, mlp0, False, None, args, steps)
+        pp = N.ProposalParams()
+        pp.mix = args.mix
+        pp.kl_threshold = args.kl_threshold
+        ps = N.MakeProposalState(phantom.init_proposal(0).to(dev), 3e-3, pp)
+        learned = run_leg(N, tree, batches, sp, field(True), grid0, mlp0, True, ps, args, steps)
+        if args.profile_steps > 0:
+            return
+        out_leg = learned
+        kern = kernel_times(N, tree, batches[0][0], sp, field(True), learned["state"], True, ps)
+        kern_plain = kernel_times(N, tree, batches[0][0], sp, field(False), plain["state"], False, None)
+        # projection MSE on held-out rays, judged by the stratified sampler
+        ev_rays, ev_tgt = make_batch(1 << 16, 999)
+        sp_judge = N.SamplingParams()
+        sp_judge.max_samples_per_node = K
+        iv_j = tree.CreateIntervalsForRays(ev_rays, sp_judge)
+
+        def judge(leg, fp):
+            P = N.ProjectIntervals(tree, ev_rays, iv_j, leg["state"].grid, leg["state"].mlp, sp_judge, fp)
+            return float(((P.double() - ev_tgt.double()) ** 2).mean().item())
+
+        ivl, fb = N.CreateProposalIntervals(tree, ps, ev_rays, sp_judge)
+        P_own = N.ProjectIntervals(tree, ev_rays, ivl, learned["state"].grid, learned["state"].mlp, sp_judge,
+                                   field(True))
+        mse = {
+            "judge": f"stratified K={K} (~{s_eff:.0f} samples/ray, midpoints) on {ev_rays.size()} held-out rays "
+                     f"vs analytic targets, after {args.train_iters} training iterations of each leg",
+            "plain": judge(plain, field(False)),
+            "learned_bf16": judge(learned, field(True)),
+            "learned_bf16_own_sampler": float(((P_own.double() - ev_tgt.double()) ** 2).mean().item()),
+            "learned_own_sampler_samples_per_ray": ivl.num_samples / ev_rays.size(),
+            "learned_own_sampler_fallback_frac": float(fb.float().mean().item()),
+        }
+        mse["ratio_learned_over_plain"] = mse["learned_bf16"] / mse["plain"]
+        # "equal projection MSE (within 1%)": the learned leg is no worse than 1% above plain
+        mse["learned_within_1pct_or_better"] = mse["ratio_learned_over_plain"] <= 1.01
+        log(f"projection MSE (judge): plain {mse['plain']:.5e}, learned+bf16 {mse['learned_bf16']:.5e} "
+            f"(ratio {mse['ratio_learned_over_plain']:.4f}); learned own sampler {mse['learned_bf16_own_sampler']:.5e}")
+        plain_summary = {
+            "value": plain["rays_per_s"], "ms_per_step": plain["ms_per_step"],
+            "speedup_learned_over_plain": learned["rays_per_s"] / plain["rays_per_s"],
+            "kernel_ms": kern_plain["kernel_ms"], "roofline": kern_plain["roofline"],
+            "loss_first_last": plain["loss_first_last"],
+        }
+        workload = (f"learned proposal + BF16 grids training step (proposal trained online every 100 iterations "
+                    f"inside the timed loop), {n}^3 phantom (200 ellipsoids), 61 tilts, {n}x{n} detector, "
+                    f"adaptive octree depth<=5; plain FP32 leg timed in the same run")
+        dtype = "bf16 grids / f32 accumulate"
+
+    rays_per_s = out_leg["rays_per_s"]
+    cpu = None
+    if not args.no_cpu_baseline:
+        rays0, tgt0 = batches[0]
+        cpu = cpu_baseline(tree, active, grid0, mlp0, rays0, tgt0, K, C, G, args.cpu_seconds)
+    b = 2 if args.mode != "plain" else 4
+    out = {
+        "metric": METRIC,
+        "value": rays_per_s,
+        "unit": "rays/s",
+        "n_gpus": 1,
+        "steps": steps,
+        "warmup": args.warmup,
+        "ms_per_step": out_leg["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": dtype,
+        "data": "synthetic (analytic ellipsoid phantom projections, random-init field)",
+        "config": {
+            "workload": workload,
+            "rays_per_batch": args.rays, "stratified_samples_per_ray": round(s_eff, 2), "K": K,
+            "samples_per_ray": round(out_leg["samples_per_ray"], 2),
+            "active_nodes": num_active, "channels": C, "grid": G, "decoder": f"{C}-{H}-{H}-1",
+            "parallelism": "single GPU",
+            "step": "sampling + fused fwd + MSE + fused adjoint + Adam(grids, decoder)",
+            "loss_first_last": out_leg["loss_first_last"],
+            "step_model_bytes_per_ray": 60 + out_leg["samples_per_ray"] * (16 * C * b + 32 * C),
+            "kernel_ms": kern["kernel_ms"],
+        },
+        "roofline": kern["roofline"],
+        "cpu_baseline": cpu,
+    }
+    if args.mode == "learned":
+        out["config"]["proposal"] = learned["proposal"]
+        out["plain_fp32"] = plain_summary
+        out["projection_mse"] = mse
+    print(json.dumps(out), flush=True)
+
+
+def run_leg(N, tree, batches, sp0, fp, grid0, mlp0, bf16, ps, args, steps):
+    """Train one leg: warmup, `steps` timed steps, then (learned mode) more
+    untimed steps up to --train-iters.  With a proposal state, the proposal is
+    trained every 100 iterations (the first time at iteration 100, before
+    which the leg samples stratified)."""
+    import torch
+
+    st = N.MakeTrainState(grid0, mlp0, bf16, 1e-3, 1e-3)
+    prop_rays = {}
+    stats = {"updates": 0, "update_ms": [], "loss": [], "fallback_frac": None}
+    sp = N.SamplingParams()
+    sp.max_samples_per_node = sp0.max_samples_per_node
+    sp.jitter_mode = sp0.jitter_mode
+
+    def sub(b):
+        if b not in prop_rays:
+            rays, _ = batches[b]
+            R = rays.size()
+            stride = max(1, R // args.proposal_rays)
+            idx = torch.arange(0, R, stride, device=rays.origin.device)[: args.proposal_rays]
+            prop_rays[b] = N.RayList(rays.origin.index_select(0, idx).contiguous(),
+                                     rays.direction.index_select(0, idx).contiguous())
+        return prop_rays[b]
+
+    def step(i):
+        b = i % len(batches)
+        rays, tgt = batches[b]
+        sp.jitter_seed = 7 + i
+        if ps is None:
+            return N.TrainStep(tree, st, rays, tgt, sp, fp)
+        if i > 0 and i % 100 == 0:
+            t0 = time.perf_counter()
+            loss = N.ProposalTrain(tree, ps, sub(b), sp, st.grid, st.mlp, fp, 200 if stats["updates"] == 0 else 50)
+            stats["update_ms"].append((time.perf_counter() - t0) * 1e3)
+            stats["loss"].append(loss)
+            stats["updates"] += 1
+        return N.TrainStepLearned(tree, st, ps, rays, tgt, sp, fp)
+
+    # learned legs start timing once the proposal is in use (iteration 100)
+    start = 100 if ps is not None else 0
+    it = 0
+    while it < start + args.warmup:
+        step(it)
+        it += 1
+    torch.cuda.synchronize()
     first_loss = float(st.loss.item())
     torch.cuda.synchronize()
     t_start = time.perf_counter()
-    for i in range(steps):
-        step(args.warmup + i)
+    for _ in range(steps):
+        step(it)
+        it += 1
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t_start
     last_loss = float(st.loss.item())
-    ms_per_step = elapsed * 1e3 / steps
-    rays_per_s = args.rays * steps / elapsed
-    log(f"{steps} steps in {elapsed:.3f}s: {ms_per_step:.2f} ms/step, {rays_per_s / 1e6:.3f} M rays/s, "
-        f"loss {first_loss:.5f} -> {last_loss:.5f}")
-    if args.profile_steps > 0:
-        return
+    ms = elapsed * 1e3 / steps
+    rps = batches[0][0].size() * steps / elapsed
+    # samples per ray of the leg's own sampler on batch 0
+    rays0 = batches[0][0]
+    if ps is not None:
+        iv, fb = N.CreateProposalIntervals(tree, ps, rays0, sp)
+        stats["fallback_frac"] = float(fb.float().mean().item())
+    else:
+        iv = tree.CreateIntervalsForRays(rays0, sp)
+    spr = iv.num_samples / rays0.size()
+    log(f"{'learned+bf16' if ps is not None else ('bf16' if bf16 else 'plain')}: {steps} steps in {elapsed:.3f}s: "
+        f"{ms:.2f} ms/step, {rps / 1e6:.3f} M rays/s, {spr:.1f} samples/ray, loss {first_loss:.5f} -> {last_loss:.5f}"
+        + (f", fallback {stats['fallback_frac']:.3f}, proposal updates {stats['updates']} "
+           f"({', '.join(f'{x:.0f}' for x in stats['update_ms'])} ms)" if ps is not None else ""))
+    if args.mode == "learned":
+        while it < args.train_iters:
+            step(it)
+            it += 1
+        torch.cuda.synchronize()
+    leg = {"state": st, "rays_per_s": rps, "ms_per_step": ms, "loss_first_last": [first_loss, last_loss],
+           "samples_per_ray": spr}
+    if ps is not None:
+        leg["proposal"] = {"mix": args.mix, "kl_threshold": args.kl_threshold, "bins": 32, "samples": 32,
+                           "updates_in_timed_region": sum(1 for j in range(start + args.warmup,
+                                                                           start + args.warmup + steps)
+                                                          if j > 0 and j % 100 == 0),
+                           "update_ms": stats["update_ms"], "proposal_loss": stats["loss"],
+                           "fallback_frac": stats["fallback_frac"], "train_rays": args.proposal_rays}
+    return leg
 
-    # ------------------------------------------------- per-kernel durations
-    rays, tgt = batches[0]
-    iv = tree.CreateIntervalsForRays(rays, sp)
-    S_total = iv.num_samples
+
+def kernel_times(N, tree, rays, sp, fp, st, bf16, ps):
+    """HIP-event durations of the fused forward and adjoint over one batch's
+    interval list (the leg's own sampler), on the stream they run on, and the
+    roofline of the dominant one."""
     import ctypes as Ct
 
-    grid_dtype = capi.GRID_BF16 if args.mode == "bf16" else capi.GRID_F32
-    f = capi.Field(C, H, G, grid_dtype, 2.0, 0, st.grid.data_ptr(), st.grid.shape[0], tree.node_to_grid.data_ptr(),
-                   st.mlp.data_ptr(), capi.DECODER_BF16 if args.mode == "bf16" else capi.DECODER_F32, 0)
-    csp = capi.Sampling(K, 256, 2, 0, 7, None)
+    import torch
+
+    from adaptivetensordf_amd import capi
+
+    if ps is not None:
+        iv, _ = N.CreateProposalIntervals(tree, ps, rays, sp)
+    else:
+        iv = tree.CreateIntervalsForRays(rays, sp)
+    dev = rays.origin.device
+    C, G, H = fp.C, fp.G, fp.H
+    S_total = iv.num_samples
+    R = rays.size()
+    f = capi.Field(C, H, G, capi.GRID_BF16 if bf16 else capi.GRID_F32, 2.0, 0, st.grid.data_ptr(),
+                   st.grid.shape[0], tree.node_to_grid.data_ptr(), st.mlp.data_ptr(),
+                   capi.DECODER_BF16 if bf16 else capi.DECODER_F32, 0)
+    csp = capi.Sampling(sp.max_samples_per_node, 256, sp.jitter_mode, 0, sp.jitter_seed, None, None)
     civ = capi.Intervals(iv.node.data_ptr(), iv.tmin.data_ptr(), iv.tmax.data_ptr(), iv.count.data_ptr(),
                          iv.first.data_ptr(), iv.ray.data_ptr(), iv.num_intervals)
     packed = torch.empty(capi.lib().atdf_tree_packed_bytes(tree.NumNodes()), dtype=torch.uint8, device=dev)
@@ -165,12 +361,12 @@ def main():
                       tree.node_children.data_ptr(), tree.node_parent.data_ptr(), tree.node_active.data_ptr())
     s = capi.stream_ptr()
     capi.call("atdf_tree_pack", Ct.byref(tdesc), capi.ptr(packed), s)
-    R = rays.origin.shape[0]
     P = torch.empty(R, device=dev)
     dP = torch.randn(R, device=dev) * 1e-6
     gg = torch.zeros(st.grid.shape, dtype=torch.float32, device=dev)
     gm = torch.zeros_like(st.mlp)
-    ws = torch.empty(capi.lib().atdf_project_bwd_workspace_bytes(R, iv.num_intervals, Ct.byref(f)), dtype=torch.uint8, device=dev)
+    ws = torch.empty(capi.lib().atdf_project_bwd_workspace_bytes(R, iv.num_intervals, Ct.byref(f)),
+                     dtype=torch.uint8, device=dev)
 
     def fwd():
         capi.call("atdf_project_fwd", capi.ptr(packed), tree.NumNodes(), capi.ptr(rays.origin),
@@ -196,7 +392,7 @@ def main():
 
     t_fwd = timed(fwd)
     t_bwd = timed(bwd)
-    b = 2 if args.mode == "bf16" else 4
+    b = 2 if bf16 else 4
     # algorithmic bytes per launch (no cache reuse): per sample the gather of
     # 8 corners x C channels (fwd), plus the scatter of 8 x C f32 (bwd);
     # per ray origin + direction (24 B) and the projection or dL/dP (4 B)
@@ -209,59 +405,30 @@ def main():
     t_dom, bytes_dom, flop_dom = (t_bwd, bwd_bytes, bwd_flop) if dom == "bwd" else (t_fwd, fwd_bytes, fwd_flop)
     achieved_gbs = bytes_dom / t_dom / 1e9
     achieved_tflops = flop_dom / t_dom / 1e12
-    mfma_peak = BF16_PEAK_TFLOPS if args.mode == "bf16" else FP32_PEAK_TFLOPS
-    log(f"project_fwd {t_fwd * 1e3:.2f} ms, project_bwd {t_bwd * 1e3:.2f} ms; dominant {dom}: "
-        f"{achieved_gbs:.0f} GB/s algorithmic, {achieved_tflops:.1f} TFLOP/s decoder")
-    traffic = measured_traffic(args.mode, f"project_{dom}")
-    # whole-step model of BASELINE.md: bytes/ray = 60 + S (16 C b + 32 C)
-    step_bytes_per_ray = 60 + s_eff * (16 * C * b + 32 * C)
-
-    # ------------------------------------------------------------ CPU baseline
-    cpu = None
-    if not args.no_cpu_baseline:
-        cpu = cpu_baseline(tree, active, st, grid0, mlp0, rays, tgt, K, C, G, args.cpu_seconds)
-
-    out = {
-        "metric": METRIC,
-        "value": rays_per_s,
-        "unit": "rays/s",
-        "n_gpus": 1,
-        "steps": steps,
-        "warmup": args.warmup,
-        "ms_per_step": ms_per_step,
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f32" if args.mode == "plain" else "bf16 grids / f32 accumulate",
-        "data": "synthetic (analytic ellipsoid phantom projections, random-init field)",
-        "config": {
-            "workload": f"{'plain FP32' if args.mode == 'plain' else 'BF16 grids'} training step, {n}^3 phantom "
-                        f"(200 ellipsoids), 61 tilts, {n}x{n} detector, adaptive octree depth<=5",
-            "rays_per_batch": args.rays, "samples_per_ray": round(s_eff, 2), "K": K,
-            "active_nodes": num_active, "channels": C, "grid": G, "decoder": f"{C}-{H}-{H}-1",
-            "parallelism": "single GPU",
-            "step": "traverse + fused fwd + MSE + fused adjoint + Adam(grids, decoder)",
-            "loss_first_last": [first_loss, last_loss],
-            "step_model_bytes_per_ray": step_bytes_per_ray,
-            "step_model_hbm_frac": step_bytes_per_ray * rays_per_s / (HBM_PEAK_GBS * 1e9),
-            "kernel_ms": {"project_fwd": t_fwd * 1e3, "project_bwd": t_bwd * 1e3},
-        },
+    mfma_peak = BF16_PEAK_TFLOPS if bf16 else FP32_PEAK_TFLOPS
+    log(f"  project_fwd {t_fwd * 1e3:.2f} ms, project_bwd {t_bwd * 1e3:.2f} ms ({S_total / R:.1f} samples/ray, "
+        f"{iv.num_intervals} intervals); dominant {dom}: {achieved_gbs:.0f} GB/s algorithmic, "
+        f"{achieved_tflops:.1f} TFLOP/s decoder")
+    mode = ("learned" if ps is not None else "bf16") if bf16 else "plain"
+    traffic = measured_traffic(mode, f"project_{dom}")
+    return {
+        "kernel_ms": {"project_fwd": t_fwd * 1e3, "project_bwd": t_bwd * 1e3},
         "roofline": {
             "bound": "hbm", "kernel": f"project_{dom}", "achieved": achieved_gbs, "peak": HBM_PEAK_GBS,
             "unit": "GB/s", "frac": achieved_gbs / HBM_PEAK_GBS, "traffic": traffic,
+            "algorithmic_bytes_per_launch": bytes_dom, "samples_per_launch": S_total,
             "mfma": {"achieved": achieved_tflops, "peak": mfma_peak, "unit": "TFLOP/s",
                      "frac": achieved_tflops / mfma_peak},
         },
-        "cpu_baseline": cpu,
     }
-    print(json.dumps(out), flush=True)
 
 
 def measured_traffic(mode, kernel):
     """HBM bytes per launch of `kernel` from the committed rocprofv3 PMC
     summary of this workload (FETCH_SIZE / WRITE_SIZE, copy-calibrated), or
     None when no summary for this mode is committed."""
-    path = os.path.join(HERE, "profiles", f"traffic_{'plain_fp32' if mode == 'plain' else 'bf16'}_512cube.json")
+    tag = {"plain": "plain_fp32", "bf16": "bf16", "learned": "learned_bf16"}[mode]
+    path = os.path.join(HERE, "profiles", f"traffic_{tag}_512cube.json")
     if not os.path.exists(path):
         return None
     with open(path) as fh:
@@ -272,7 +439,7 @@ def measured_traffic(mode, kernel):
     return None
 
 
-def cpu_baseline(tree, active, st, grid0, mlp0, rays, tgt, K, C, G, seconds):
+def cpu_baseline(tree, active, grid0, mlp0, rays, tgt, K, C, G, seconds):
     """The oracle's FP32 training step (OpenMP over rays, dense Adam) on the
     same workload, timed on this host for about `seconds`."""
     import torch
