@@ -173,6 +173,23 @@ __device__ __forceinline__ void make_sample_iv(const ProjArgs& a, int64_t r, con
     sc.gbase = slot >= 0 ? (int64_t)slot * a.G * a.G * a.G * C : -1;
 }
 
+// Index (0..16) of the lane group's candidate interval holding sample t:
+// the number of candidates whose exclusive sample end is <= t.  `ends` is
+// nondecreasing over the 16 lanes of the group (lanes past the list hold
+// INT_MAX); a branch-free binary search with five cross-lane reads.
+__device__ __forceinline__ int group_search(int ends, int t, int grp)
+{
+    // four halving steps count the first 15 candidates, the last read the 16th
+    int pos = 0;
+#pragma unroll
+    for (int s = 8; s > 0; s >>= 1) {
+        const int e = __shfl(ends, grp | (pos + s - 1), 64);
+        if (e <= t) pos += s;
+    }
+    const int e15 = __shfl(ends, grp | 15, 64);
+    return e15 <= t ? 16 : pos;
+}
+
 // Recompute sample `s_local` of ray r (lane-private interval cursor `cur`).
 __device__ __forceinline__ void make_sample(const ProjArgs& a, int64_t r, const float* o, const float* d, int ns,
                                             int s_local, int64_t h1, int64_t& cur, int C, SampleCtx& sc)
@@ -458,8 +475,45 @@ __global__ __launch_bounds__(64 * kWaves) void project_fwd_kernel(ProjArgs a, fl
             intersect_box_ray(root.mn, root.mx, o, d, bt0, bt1);
         }
         for (int base = 0; base < ns; base += kTile) {
+            // the tile's 16 samples lie in the 16 intervals from `cur` on
+            // (every interval holds at least one sample): load them once,
+            // then each lane finds its own by a cross-lane binary search
             SampleCtx sc;
-            make_sample(a, r, o, d, ns, base + j, h1e, cur, C, sc);
+            const int64_t cand = cur + j;
+            int c_end = INT_MAX, c_cnt = 0, c_node = 0;
+            float c_lo = 0.f, c_hi = 0.f;
+            if (cand < h1e) {
+                c_cnt = a.iv_count[cand];
+                c_end = a.iv_first[cand] + c_cnt;
+                c_node = a.iv_node[cand];
+                c_lo = a.iv_tmin[cand];
+                c_hi = a.iv_tmax[cand];
+            }
+            const int t = base + j;
+            const int grp = lane & 48;
+            int pos = group_search(c_end, t, grp);
+            const int posn = group_search(c_end, base + kTile, grp);
+            const int src = grp | (pos < kTile ? pos : kTile - 1);
+            const int s_cnt = __shfl(c_cnt, src, 64);
+            const int s_end = __shfl(c_end, src, 64);
+            const int s_node = __shfl(c_node, src, 64);
+            const float s_lo = __shfl(c_lo, src, 64);
+            const float s_hi = __shfl(c_hi, src, 64);
+            sc.valid = t < ns;
+            sc.w = 0.f;
+            sc.gbase = -1;
+            if (sc.valid) {
+                if (pos < kTile) {
+                    make_sample_iv(a, r, o, d, s_node, s_lo, s_hi, s_cnt, t - (s_end - s_cnt), t, C, sc);
+                } else {
+                    // only with zero-sample intervals in the list: walk on
+                    int64_t q = cur + kTile - 1;
+                    while (q + 1 < h1e && a.iv_first[q] + a.iv_count[q] <= t) ++q;
+                    make_sample_iv(a, r, o, d, a.iv_node[q], a.iv_tmin[q], a.iv_tmax[q], a.iv_count[q],
+                                   t - a.iv_first[q], t, C, sc);
+                }
+            }
+            cur += posn;
             float x[CPL];
             gather_channels<C, kBF16>(a, sc, CPL * g, x);
             v4f h1[4], h2[4];
@@ -826,23 +880,55 @@ __global__ __launch_bounds__(64 * kWaves) void project_bwd_node_kernel(NodeBwdAr
 #pragma unroll
         for (int w = 0; w < kWaves; ++w) ntiles = max(ntiles, s_tiles[w]);
 
+        // wave-uniform cursor: `cur` holds the tile's first sample, `cum`
+        // samples of this wave's list precede it
         int cur = ib, cum = 0;
-        int cur_cnt = (ib < ie) ? a.iv_count[b.sorted[ib]] : 0;
+        const int grp = lane & 48;
         for (int ti = 0; ti < ntiles; ++ti) {
             const int base = ti * kTile;
             const int t = base + j;
+            // the tile's samples lie in the 16 intervals from `cur` on (every
+            // interval holds at least one sample): load them once, prefix-sum
+            // their counts across the lane group, binary-search per lane
+            const int cand = cur + j;
+            int c_iv = 0, c_cnt = 0;
+            if (cand < ie) {
+                c_iv = b.sorted[cand];
+                c_cnt = a.iv_count[c_iv];
+            }
+            int c_end = c_cnt;
+#pragma unroll
+            for (int off = 1; off < kTile; off <<= 1) {
+                const int v = __shfl_up(c_end, off, kTile);
+                if (j >= off) c_end += v;
+            }
+            c_end = cand < ie ? c_end + cum : INT_MAX;
+            const int pos = group_search(c_end, t, grp);
+            const int posn = group_search(c_end, base + kTile, grp);
+            const int src = grp | (pos < kTile ? pos : kTile - 1);
+            const int s_iv = __shfl(c_iv, src, 64);
+            const int s_cnt = __shfl(c_cnt, src, 64);
+            const int s_end = __shfl(c_end, src, 64);
+            const int last_end = __shfl(c_end, grp | (kTile - 1), 64);
             SampleCtx sc;
             sc.valid = t < tot;
             sc.w = 0.f;
             sc.gbase = -1;
             float dp = 0.f;
             if (sc.valid) {
-                while (cum + cur_cnt <= t) {
-                    cum += cur_cnt;
-                    ++cur;
-                    cur_cnt = a.iv_count[b.sorted[cur]];
+                int iv = s_iv, n_iv = s_cnt, start = s_end - s_cnt;
+                if (pos >= kTile) {
+                    // only with zero-sample intervals in the list: walk on
+                    int q = cur + kTile;
+                    start = last_end;
+                    n_iv = a.iv_count[b.sorted[q]];
+                    while (start + n_iv <= t) {
+                        start += n_iv;
+                        ++q;
+                        n_iv = a.iv_count[b.sorted[q]];
+                    }
+                    iv = b.sorted[q];
                 }
-                const int iv = b.sorted[cur];
                 const int64_t r = b.iv_ray[iv];
                 float o[3], d[3];
 #pragma unroll
@@ -850,11 +936,13 @@ __global__ __launch_bounds__(64 * kWaves) void project_bwd_node_kernel(NodeBwdAr
                     o[k] = a.O[r * 3 + k];
                     d[k] = a.D[r * 3 + k];
                 }
-                const int jj = t - cum;
-                make_sample_iv(a, r, o, d, a.iv_node[iv], a.iv_tmin[iv], a.iv_tmax[iv], cur_cnt, jj,
+                const int jj = t - start;
+                make_sample_iv(a, r, o, d, a.iv_node[iv], a.iv_tmin[iv], a.iv_tmax[iv], n_iv, jj,
                                a.iv_first[iv] + jj, C, sc);
                 dp = b.dLdP[r];
             }
+            if (posn > 0) cum = __shfl(c_end, grp | (posn - 1), 64);
+            cur += posn;
             float x[CPL];
             gather_channels<C, kBF16>(a, sc, CPL * g, x);
             v4f h1[4], h2[4];
