@@ -1069,6 +1069,62 @@ __global__ __launch_bounds__(256) void slot_scatter_kernel(const int32_t* __rest
     sorted[slot_off[slot] + pos] = (int32_t)i;
 }
 
+// Block-aggregated counting sort for lists with many intervals per slot
+// (learned one-sample intervals put ~1000 on each node): each workgroup
+// counts kSortItems consecutive intervals in an LDS histogram, so the global
+// counters see one atomic per distinct slot per workgroup instead of one per
+// interval.  The scatter pass reserves the workgroup's range of every slot
+// with one atomic and ranks its intervals with LDS atomics.
+constexpr int kSortItems = 8192;
+
+__device__ __forceinline__ void block_slot_hist(const int32_t* __restrict__ iv_node,
+                                                const int32_t* __restrict__ node_to_grid, int64_t b0, int64_t e0,
+                                                int64_t NA, int32_t* lh)
+{
+    for (int64_t s = threadIdx.x; s < NA; s += blockDim.x) lh[s] = 0;
+    __syncthreads();
+    for (int64_t i = b0 + threadIdx.x; i < e0; i += blockDim.x) {
+        const int slot = node_to_grid[iv_node[i]];
+        if (slot >= 0) atomicAdd(&lh[slot], 1);
+    }
+    __syncthreads();
+}
+
+__global__ __launch_bounds__(256) void slot_hist_lds_kernel(const int32_t* __restrict__ iv_node,
+                                                            const int32_t* __restrict__ node_to_grid, int64_t H,
+                                                            int64_t NA, int32_t* __restrict__ cnt)
+{
+    extern __shared__ int32_t lh[];
+    const int64_t b0 = blockIdx.x * (int64_t)kSortItems;
+    const int64_t e0 = b0 + kSortItems < H ? b0 + kSortItems : H;
+    block_slot_hist(iv_node, node_to_grid, b0, e0, NA, lh);
+    for (int64_t s = threadIdx.x; s < NA; s += blockDim.x) {
+        const int v = lh[s];
+        if (v) atomicAdd(&cnt[s], v);
+    }
+}
+
+__global__ __launch_bounds__(256) void slot_scatter_lds_kernel(const int32_t* __restrict__ iv_node,
+                                                               const int32_t* __restrict__ node_to_grid, int64_t H,
+                                                               int64_t NA, const int64_t* __restrict__ slot_off,
+                                                               int32_t* __restrict__ cursor,
+                                                               int32_t* __restrict__ sorted)
+{
+    extern __shared__ int32_t lh[];
+    const int64_t b0 = blockIdx.x * (int64_t)kSortItems;
+    const int64_t e0 = b0 + kSortItems < H ? b0 + kSortItems : H;
+    block_slot_hist(iv_node, node_to_grid, b0, e0, NA, lh);
+    for (int64_t s = threadIdx.x; s < NA; s += blockDim.x) {
+        const int v = lh[s];
+        lh[s] = v ? (int)(slot_off[s] + atomicAdd(&cursor[s], v)) : 0;
+    }
+    __syncthreads();
+    for (int64_t i = b0 + threadIdx.x; i < e0; i += blockDim.x) {
+        const int slot = node_to_grid[iv_node[i]];
+        if (slot >= 0) sorted[atomicAdd(&lh[slot], 1)] = (int32_t)i;
+    }
+}
+
 __global__ __launch_bounds__(256) void chunk_count_kernel(const int64_t* __restrict__ slot_off, int64_t NA,
                                                           int32_t* __restrict__ chunk_cnt)
 {
@@ -1376,6 +1432,21 @@ void lds_layout(int G, int& f32_off, int& bf16_off, int& tb_off, int& total)
     total = tb_off + (kDecBF16 ? kWaves * 2 * kH * kTB * 2 : kWaves * 2 * kH * kTS * 4);
 }
 
+int max_lds_per_block()
+{
+    static int max_lds = 0;
+    if (max_lds == 0) {
+        int dev = 0, v = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess || v <= 0) {
+            (void)hipGetLastError();
+            v = 64 * 1024;
+        }
+        max_lds = v;
+    }
+    return max_lds;
+}
+
 template <int C, bool kBF16, bool kDecBF16>
 atdf_status launch_node_bwd(NodeBwdArgs args, hipStream_t s)
 {
@@ -1384,15 +1455,7 @@ atdf_status launch_node_bwd(NodeBwdArgs args, hipStream_t s)
     args.lds_f32_off = f32_off;
     args.lds_bf16_off = bf16_off;
     args.lds_tb_off = tb_off;
-    static int max_lds = 0;
-    if (max_lds == 0) {
-        int dev = 0;
-        if (hipGetDevice(&dev) != hipSuccess ||
-            hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess) {
-            (void)hipGetLastError();
-            max_lds = 64 * 1024;
-        }
-    }
+    const int max_lds = max_lds_per_block();
     if (total > max_lds) {
         set_error("atdf_project_bwd: node gradient (G=%d, C=%d) needs %d bytes of LDS per workgroup, the device "
                   "allows %d", args.a.G, C, total, max_lds);
@@ -1445,15 +1508,29 @@ atdf_status atdf_project_bwd(const void* packed_tree, int64_t num_nodes, const f
 
     hipLaunchKernelGGL(fill_i32_kernel, dim3((unsigned)div_up(NA, 256)), dim3(256), 0, s, cnt, NA, 0);
     hipLaunchKernelGGL(fill_i32_kernel, dim3(1), dim3(256), 0, s, counter, 1, 0);
-    if (H > 0)
-        hipLaunchKernelGGL(slot_hist_kernel, dim3((unsigned)div_up(H, 256)), dim3(256), 0, s, iv->node,
-                           f->node_to_grid, H, cnt);
+    // LDS-aggregated counting sort when a slot histogram fits in LDS
+    const bool lds_sort = (int64_t)sizeof(int32_t) * NA <= (int64_t)max_lds_per_block();
+    const size_t hist_lds = sizeof(int32_t) * (size_t)NA;
+    const unsigned sort_blocks = (unsigned)div_up(H, kSortItems);
+    if (H > 0) {
+        if (lds_sort)
+            hipLaunchKernelGGL(slot_hist_lds_kernel, dim3(sort_blocks), dim3(256), hist_lds, s, iv->node,
+                               f->node_to_grid, H, NA, cnt);
+        else
+            hipLaunchKernelGGL(slot_hist_kernel, dim3((unsigned)div_up(H, 256)), dim3(256), 0, s, iv->node,
+                               f->node_to_grid, H, cnt);
+    }
     st = atdf_exclusive_scan_i32(cnt, NA, slot_off, scan_ws, scan_bytes, stream);
     if (st) return st;
     hipLaunchKernelGGL(fill_i32_kernel, dim3((unsigned)div_up(NA, 256)), dim3(256), 0, s, cnt, NA, 0);
-    if (H > 0)
-        hipLaunchKernelGGL(slot_scatter_kernel, dim3((unsigned)div_up(H, 256)), dim3(256), 0, s, iv->node,
-                           f->node_to_grid, H, slot_off, cnt, sorted);
+    if (H > 0) {
+        if (lds_sort)
+            hipLaunchKernelGGL(slot_scatter_lds_kernel, dim3(sort_blocks), dim3(256), hist_lds, s, iv->node,
+                               f->node_to_grid, H, NA, slot_off, cnt, sorted);
+        else
+            hipLaunchKernelGGL(slot_scatter_kernel, dim3((unsigned)div_up(H, 256)), dim3(256), 0, s, iv->node,
+                               f->node_to_grid, H, slot_off, cnt, sorted);
+    }
     hipLaunchKernelGGL(chunk_count_kernel, dim3((unsigned)div_up(NA, 256)), dim3(256), 0, s, slot_off, NA, chunk_cnt);
     st = atdf_exclusive_scan_i32(chunk_cnt, NA, chunk_off, scan_ws, scan_bytes, stream);
     if (st) return st;
