@@ -424,17 +424,19 @@ def kernel_times(N, tree, rays, sp, fp, st, bf16, ps):
 
 
 def measured_traffic(mode, kernel):
-    """HBM bytes per launch of `kernel` from the committed rocprofv3 PMC
-    summary of this workload (FETCH_SIZE / WRITE_SIZE, copy-calibrated), or
-    None when no summary for this mode is committed."""
-    tag = {"plain": "plain_fp32", "bf16": "bf16", "learned": "learned_bf16"}[mode]
-    path = os.path.join(HERE, "profiles", f"traffic_{tag}_512cube.json")
-    if not os.path.exists(path):
+    """HBM bytes per launch of the main kernel behind `kernel` (project_fwd or
+    project_bwd) for this mode's interval lists, from the committed rocprofv3
+    PMC summary of the 512^3 workload (FETCH_SIZE + WRITE_SIZE,
+    copy-calibrated), or None when that combination was not measured."""
+    path = os.path.join(HERE, "profiles", "traffic_512cube.json")
+    template = {"plain": "<16, false, false", "learned": "<16, true, true"}.get(mode)
+    if template is None or not os.path.exists(path):
         return None
     with open(path) as fh:
         data = json.load(fh)
+    stem = "project_bwd_node_kernel" if kernel == "project_bwd" else "project_fwd_kernel"
     for name, rec in data.get("kernels", {}).items():
-        if kernel + "_node_kernel" in name or (kernel + "_kernel") in name:
+        if stem in name and template in name:
             return rec["hbm_bytes_per_launch"]
     return None
 
