@@ -26,6 +26,8 @@
 // (a K = 64 contraction is two 16x16x32 steps; a K = 16 one is one 16x16x16).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "atdf_device.h"
 #include "atdf_internal.h"
 
@@ -785,20 +787,34 @@ __device__ __forceinline__ void store_partials(DecGrads& q, float* out, int g, i
 
 // ---------------------------------------------------------------------------
 // Node-major adjoint.  Intervals are counting-sorted by grid slot and cut
-// into chunks of at most kChunk intervals of one node.  A persistent
-// workgroup (one per CU: the node's gradient lives in LDS) takes chunks
-// from a global counter, splits the chunk's intervals over its 4 waves,
-// recomputes every sample (interval -> t, weight, position, gather, decoder),
-// runs the decoder backward on MFMA, and scatters w_corner * dX into the
-// LDS copy of the node's grid gradient with LDS float atomics.  At the end
-// of the chunk the LDS gradient is added to the global gradient once:
-// a plain read-add-write when the chunk covers the whole node, a global
-// atomic add when the node was split over several chunks.
+// into chunks of one node.  A persistent workgroup (one per CU: the node's
+// gradient lives in LDS) takes chunks from a global counter, splits the
+// chunk's intervals over its 4 waves, recomputes every sample (interval ->
+// t, weight, position, gather, decoder), runs the decoder backward on MFMA,
+// and scatters w_corner * dX into the LDS copy of the node's grid gradient
+// (waves take turns; see the scatter below).  At the end of the chunk the
+// LDS gradient is added to the global gradient once: a plain read-add-write
+// when the chunk covers the whole node, a global atomic add when the node
+// was split over several chunks.
+//
+// A chunk holds about kChunkSamples samples: its interval count is
+// kChunkSamples / (mean samples per interval of the list), so stratified
+// lists (~6 samples per interval) and learned one-sample lists both give
+// chunks of similar work, and a node is split only when it carries more
+// samples than that.
 // ---------------------------------------------------------------------------
-#ifndef ATDF_NODE_CHUNK
-#define ATDF_NODE_CHUNK 512
+#ifndef ATDF_NODE_CHUNK_SAMPLES
+#define ATDF_NODE_CHUNK_SAMPLES 3072
 #endif
-constexpr int kChunk = ATDF_NODE_CHUNK;  // intervals per node chunk
+constexpr int64_t kChunkSamples = ATDF_NODE_CHUNK_SAMPLES;
+constexpr int kMinChunk = 64;     // intervals
+constexpr int kMaxChunk = 16384;  // intervals
+
+__device__ __forceinline__ int64_t chunk_intervals(int64_t H, int64_t S)
+{
+    int64_t c = S > 0 ? (kChunkSamples * H + S - 1) / S : kMaxChunk;
+    return c < kMinChunk ? kMinChunk : (c > kMaxChunk ? kMaxChunk : c);
+}
 
 struct NodeBwdArgs {
     ProjArgs a;
@@ -1125,26 +1141,45 @@ __global__ __launch_bounds__(256) void slot_scatter_lds_kernel(const int32_t* __
     }
 }
 
+// Total samples of the interval list (for the chunk size).
+__global__ __launch_bounds__(256) void sum_counts_kernel(const int32_t* __restrict__ count, int64_t H,
+                                                         unsigned long long* __restrict__ total)
+{
+    __shared__ unsigned long long part[4];
+    unsigned long long v = 0;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < H; i += (int64_t)gridDim.x * blockDim.x)
+        v += (unsigned long long)count[i];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) atomicAdd(total, part[0] + part[1] + part[2] + part[3]);
+}
+
 __global__ __launch_bounds__(256) void chunk_count_kernel(const int64_t* __restrict__ slot_off, int64_t NA,
+                                                          int64_t H, const unsigned long long* __restrict__ total,
                                                           int32_t* __restrict__ chunk_cnt)
 {
     int64_t s = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (s >= NA) return;
+    const int64_t ch = chunk_intervals(H, (int64_t)*total);
     int64_t n = slot_off[s + 1] - slot_off[s];
-    chunk_cnt[s] = (int32_t)((n + kChunk - 1) / kChunk);
+    chunk_cnt[s] = (int32_t)((n + ch - 1) / ch);
 }
 
 __global__ __launch_bounds__(256) void chunk_desc_kernel(const int64_t* __restrict__ slot_off,
                                                          const int64_t* __restrict__ chunk_off, int64_t NA,
+                                                         int64_t H, const unsigned long long* __restrict__ total,
                                                          int4* __restrict__ desc)
 {
     int64_t s = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (s >= NA) return;
+    const int64_t ch = chunk_intervals(H, (int64_t)*total);
     int64_t b0 = slot_off[s], e0 = slot_off[s + 1];
     int64_t c0 = chunk_off[s], nch = chunk_off[s + 1] - c0;
     for (int64_t k = 0; k < nch; ++k) {
-        int64_t b = b0 + k * kChunk;
-        int64_t e = b + kChunk < e0 ? b + kChunk : e0;
+        int64_t b = b0 + k * ch;
+        int64_t e = b + ch < e0 ? b + ch : e0;
         desc[c0 + k] = make_int4((int)s, (int)b, (int)e, nch > 1 ? 1 : 0);
     }
 }
@@ -1384,7 +1419,7 @@ atdf_status atdf_project_fwd_bins(const void* packed_tree, int64_t num_nodes, co
 
 namespace {
 struct BwdWorkspace {
-    size_t cnt, slot_off, sorted, chunk_cnt, chunk_off, desc, counter, scan, partials, total;
+    size_t cnt, slot_off, sorted, chunk_cnt, chunk_off, desc, counter, stotal, scan, partials, total;
 };
 
 size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
@@ -1412,8 +1447,9 @@ BwdWorkspace bwd_layout(int64_t H, int64_t NA, int np)
     w.sorted = take(sizeof(int32_t) * (H > 0 ? H : 1));
     w.chunk_cnt = take(sizeof(int32_t) * (NA + 1));
     w.chunk_off = take(sizeof(int64_t) * (NA + 1));
-    w.desc = take(sizeof(int4) * (H / kChunk + NA + 1));
+    w.desc = take(sizeof(int4) * (H / kMinChunk + NA + 1));
     w.counter = take(256);
+    w.stotal = take(sizeof(unsigned long long));
     w.scan = take(atdf_scan_workspace_bytes(NA));
     w.partials = take(sizeof(float) * (size_t)persistent_blocks() * kWaves * np);
     w.total = o;
@@ -1531,11 +1567,17 @@ atdf_status atdf_project_bwd(const void* packed_tree, int64_t num_nodes, const f
             hipLaunchKernelGGL(slot_scatter_kernel, dim3((unsigned)div_up(H, 256)), dim3(256), 0, s, iv->node,
                                f->node_to_grid, H, slot_off, cnt, sorted);
     }
-    hipLaunchKernelGGL(chunk_count_kernel, dim3((unsigned)div_up(NA, 256)), dim3(256), 0, s, slot_off, NA, chunk_cnt);
+    unsigned long long* stotal = reinterpret_cast<unsigned long long*>(w + L.stotal);
+    hipLaunchKernelGGL(fill_i32_kernel, dim3(1), dim3(256), 0, s, reinterpret_cast<int32_t*>(stotal), 2, 0);
+    if (H > 0)
+        hipLaunchKernelGGL(sum_counts_kernel, dim3((unsigned)std::min<int64_t>(div_up(H, 256), 1024)), dim3(256), 0,
+                           s, iv->count, H, stotal);
+    hipLaunchKernelGGL(chunk_count_kernel, dim3((unsigned)div_up(NA, 256)), dim3(256), 0, s, slot_off, NA, H, stotal,
+                       chunk_cnt);
     st = atdf_exclusive_scan_i32(chunk_cnt, NA, chunk_off, scan_ws, scan_bytes, stream);
     if (st) return st;
     hipLaunchKernelGGL(chunk_desc_kernel, dim3((unsigned)div_up(NA, 256)), dim3(256), 0, s, slot_off, chunk_off, NA,
-                       desc);
+                       H, stotal, desc);
 
     NodeBwdArgs args{};
     args.a = make_args(packed_tree, O, D, R, sp, iv_off, iv, f);

@@ -402,6 +402,11 @@ __global__ __launch_bounds__(256) void merge_counts_kernel(int64_t R, const int3
     ns[r] = n_learned[r];
 }
 
+// Copies the learned cells of 64 rays from the slot-major staging buffers
+// into the mixed interval list: coalesced reads along rays into LDS, then
+// coalesced writes along each ray's contiguous run of intervals.
+constexpr int kWriteRays = 64;
+
 __global__ __launch_bounds__(256) void proposal_write_kernel(int64_t R, const int32_t* __restrict__ fallback,
                                                              const int32_t* __restrict__ n_learned,
                                                              const int32_t* __restrict__ lnode,
@@ -409,19 +414,39 @@ __global__ __launch_bounds__(256) void proposal_write_kernel(int64_t R, const in
                                                              const float* __restrict__ ltmax,
                                                              const int64_t* __restrict__ iv_off, atdf_intervals iv)
 {
-    // thread = (slot i, ray r), ray fastest: the slot-major source reads coalesce
-    const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    if (t >= R * NB) return;
-    const int64_t r = t % R;
-    const int i = (int)(t / R);
-    if (fallback[r] || i >= n_learned[r]) return;
-    const int64_t q = iv_off[r] + i;
-    iv.node[q] = lnode[t];
-    iv.tmin[q] = ltmin[t];
-    iv.tmax[q] = ltmax[t];
-    iv.count[q] = 1;
-    iv.first[q] = i;
-    iv.ray[q] = (int32_t)r;
+    __shared__ int32_t s_node[NB * kWriteRays];
+    __shared__ float s_lo[NB * kWriteRays];
+    __shared__ float s_hi[NB * kWriteRays];
+    __shared__ int32_t s_n[kWriteRays];
+    __shared__ int64_t s_off[kWriteRays];
+    const int64_t r0 = blockIdx.x * (int64_t)kWriteRays;
+    for (int t = threadIdx.x; t < NB * kWriteRays; t += blockDim.x) {
+        const int i = t / kWriteRays, rl = t - i * kWriteRays;
+        const int64_t r = r0 + rl;
+        if (r < R) {
+            const int64_t q = (int64_t)i * R + r;
+            s_node[rl * NB + i] = lnode[q];
+            s_lo[rl * NB + i] = ltmin[q];
+            s_hi[rl * NB + i] = ltmax[q];
+        }
+    }
+    for (int rl = threadIdx.x; rl < kWriteRays; rl += blockDim.x) {
+        const int64_t r = r0 + rl;
+        s_n[rl] = (r < R && !fallback[r]) ? n_learned[r] : 0;
+        s_off[rl] = r < R ? iv_off[r] : 0;
+    }
+    __syncthreads();
+    for (int t = threadIdx.x; t < NB * kWriteRays; t += blockDim.x) {
+        const int rl = t / NB, i = t - rl * NB;
+        if (i >= s_n[rl]) continue;
+        const int64_t q = s_off[rl] + i;
+        iv.node[q] = s_node[t];
+        iv.tmin[q] = s_lo[t];
+        iv.tmax[q] = s_hi[t];
+        iv.count[q] = 1;
+        iv.first[q] = i;
+        iv.ray[q] = (int32_t)(r0 + rl);
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -766,8 +791,8 @@ atdf_status atdf_proposal_write(int64_t num_rays, const int32_t* ray_fallback, c
     ATDF_REQUIRE(intervals->node && intervals->tmin && intervals->tmax && intervals->count && intervals->first &&
                      intervals->ray,
                  "atdf_proposal_write: incomplete interval buffers");
-    const int64_t n = num_rays * NB;
-    hipLaunchKernelGGL(proposal_write_kernel, dim3((unsigned)div_up(n, 256)), dim3(256), 0, to_hip(stream),
+    hipLaunchKernelGGL(proposal_write_kernel, dim3((unsigned)div_up(num_rays, kWriteRays)), dim3(256), 0,
+                       to_hip(stream),
                        num_rays, ray_fallback, ray_num_learned, learned_node, learned_tmin, learned_tmax,
                        ray_interval_offset, *intervals);
     return check_launch("atdf_proposal_write");
