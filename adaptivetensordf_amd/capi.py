@@ -81,6 +81,12 @@ _SIGNATURES = {
     "atdf_local_samples": (_I32, [_P, _P, _P, _P, _I64, _P, _P]),
     "atdf_grid_sample3d_fwd": (_I32, [_P, _I64, _I32, _I32, _I32, _I32, _P, _P, _I64, _P, _P]),
     "atdf_grid_sample3d_bwd": (_I32, [_P, _I64, _I32, _I32, _I32, _I32, _P, _P, _I64, _P, _P, _P, _P]),
+    "atdf_grid_sample2d_fwd": (_I32, [_P, _I64, _I32, _I32, _I32, _P, _P, _I64, _P, _P]),
+    "atdf_grid_sample2d_bwd": (_I32, [_P, _I64, _I32, _I32, _I32, _P, _P, _I64, _P, _P, _P, _P]),
+    "atdf_grid_sample1d_fwd": (_I32, [_P, _I64, _I32, _I32, _P, _P, _I64, _P, _P]),
+    "atdf_grid_sample1d_bwd": (_I32, [_P, _I64, _I32, _I32, _P, _P, _I64, _P, _P, _P, _P]),
+    "atdf_gradient_correction_bwd": (_I32, [_P, _P, _I64, _P, _P, _P]),
+    "atdf_compute_bound_weight": (_I32, [_P, _I64, _P, _P, _P]),
     "atdf_integrate_xray": (_I32, [_P, _P, _P, _I64, _I32, _I64, _P, _P]),
     "atdf_accumulate_loss_per_node": (_I32, [_P, _I32, _I64, _P, _P, _P, _I64, _I64, _P, _P, _P]),
     "atdf_project_fwd": (_I32, [_P, _I64, _P, _P, _I64, C.POINTER(Sampling), _P, C.POINTER(Intervals),

@@ -94,6 +94,175 @@ __global__ __launch_bounds__(256) void grid_sample3d_bwd_kernel(const float* __r
     }
 }
 
+// 2-D indirect gather, grid [N, C, H, W] (modules/IndirectGridSample2D.cu:22-162):
+// corners tnw tne tsw tse from floor(ix), floor(iy), weights from the
+// unclipped corners, indices clipped (border padding), summed in that order.
+struct Bil {
+    int x0, x1, y0, y1;
+    float w[4];
+};
+
+__device__ __forceinline__ void bil_setup(float u, float v, int H, int W, Bil& b)
+{
+    const float ix = ((u + 1) / 2) * (float)(W - 1);
+    const float iy = ((v + 1) / 2) * (float)(H - 1);
+    const int x0 = (int)floorf(ix), y0 = (int)floorf(iy);
+    const int x1 = x0 + 1, y1 = y0 + 1;
+    const float fx1 = (float)x1 - ix, fx0 = ix - (float)x0;
+    const float fy1 = (float)y1 - iy, fy0 = iy - (float)y0;
+    b.w[0] = fx1 * fy1;
+    b.w[1] = fx0 * fy1;
+    b.w[2] = fx1 * fy0;
+    b.w[3] = fx0 * fy0;
+    b.x0 = clampi(x0, W);
+    b.x1 = clampi(x1, W);
+    b.y0 = clampi(y0, H);
+    b.y1 = clampi(y1, H);
+}
+
+__global__ __launch_bounds__(256) void grid_sample2d_fwd_kernel(const float* __restrict__ grid, int C, int H, int W,
+                                                                const int64_t* __restrict__ index,
+                                                                const float* __restrict__ uv, int64_t S,
+                                                                float* __restrict__ out)
+{
+    const int64_t s = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (s >= S) return;
+    Bil b;
+    bil_setup(uv[s * 2], uv[s * 2 + 1], H, W, b);
+    const int64_t plane = (int64_t)H * W;
+    const float* g = grid + index[s] * C * plane;
+    const int64_t off[4] = {(int64_t)b.y0 * W + b.x0, (int64_t)b.y0 * W + b.x1, (int64_t)b.y1 * W + b.x0,
+                            (int64_t)b.y1 * W + b.x1};
+    for (int c = 0; c < C; ++c) {
+        const float* gc = g + c * plane;
+        float acc = 0.f;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) acc += gc[off[k]] * b.w[k];
+        out[s * C + c] = acc;
+    }
+}
+
+// Backward: atomics into the clipped corners (IndirectGridSample2D.cu:175-393);
+// grad_uv (nullable) += the analytic derivative of the forward.
+__global__ __launch_bounds__(256) void grid_sample2d_bwd_kernel(const float* __restrict__ grid, int C, int H, int W,
+                                                                const int64_t* __restrict__ index,
+                                                                const float* __restrict__ uv, int64_t S,
+                                                                const float* __restrict__ grad_out,
+                                                                float* __restrict__ grad_grid,
+                                                                float* __restrict__ grad_uv)
+{
+    const int64_t s = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (s >= S) return;
+    const float u = uv[s * 2], v = uv[s * 2 + 1];
+    Bil b;
+    bil_setup(u, v, H, W, b);
+    const int64_t plane = (int64_t)H * W;
+    const int64_t base = index[s] * C * plane;
+    const int64_t off[4] = {(int64_t)b.y0 * W + b.x0, (int64_t)b.y0 * W + b.x1, (int64_t)b.y1 * W + b.x0,
+                            (int64_t)b.y1 * W + b.x1};
+    const float ix = ((u + 1) / 2) * (float)(W - 1);
+    const float iy = ((v + 1) / 2) * (float)(H - 1);
+    const float fx[2] = {(float)((int)floorf(ix) + 1) - ix, ix - (float)(int)floorf(ix)};
+    const float fy[2] = {(float)((int)floorf(iy) + 1) - iy, iy - (float)(int)floorf(iy)};
+    float gx = 0.f, gy = 0.f;
+    for (int c = 0; c < C; ++c) {
+        const float go = grad_out[s * C + c];
+        float* gg = grad_grid + base + c * plane;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) unsafeAtomicAdd(gg + off[k], b.w[k] * go);
+        if (grad_uv) {
+            const float* gc = grid + base + c * plane;
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                const int bx = k & 1, by = k >> 1;
+                const float val = gc[off[k]] * go;
+                gx += val * (bx ? fy[by] : -fy[by]);
+                gy += val * (by ? fx[bx] : -fx[bx]);
+            }
+        }
+    }
+    if (grad_uv) {
+        grad_uv[s * 2] += gx * ((float)(W - 1) / 2);
+        grad_uv[s * 2 + 1] += gy * ((float)(H - 1) / 2);
+    }
+}
+
+// 1-D indirect gather, grid [N, C, W, 1] (modules/IndirectGridSample1D.cu:20-140).
+__global__ __launch_bounds__(256) void grid_sample1d_fwd_kernel(const float* __restrict__ grid, int C, int W,
+                                                                const int64_t* __restrict__ index,
+                                                                const float* __restrict__ uv, int64_t S,
+                                                                float* __restrict__ out)
+{
+    const int64_t s = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (s >= S) return;
+    const float ix = ((uv[s] + 1) / 2) * (float)(W - 1);
+    const int x0 = (int)floorf(ix), x1 = x0 + 1;
+    const float w0 = (float)x1 - ix, w1 = ix - (float)x0;
+    const int c0 = clampi(x0, W), c1 = clampi(x1, W);
+    const float* g = grid + index[s] * C * W;
+    for (int c = 0; c < C; ++c) {
+        float acc = 0.f;
+        acc += g[c * W + c0] * w0;
+        acc += g[c * W + c1] * w1;
+        out[s * C + c] = acc;
+    }
+}
+
+__global__ __launch_bounds__(256) void grid_sample1d_bwd_kernel(const float* __restrict__ grid, int C, int W,
+                                                                const int64_t* __restrict__ index,
+                                                                const float* __restrict__ uv, int64_t S,
+                                                                const float* __restrict__ grad_out,
+                                                                float* __restrict__ grad_grid,
+                                                                float* __restrict__ grad_uv)
+{
+    const int64_t s = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (s >= S) return;
+    const float ix = ((uv[s] + 1) / 2) * (float)(W - 1);
+    const int x0 = (int)floorf(ix), x1 = x0 + 1;
+    const float w0 = (float)x1 - ix, w1 = ix - (float)x0;
+    const int c0 = clampi(x0, W), c1 = clampi(x1, W);
+    const int64_t base = index[s] * C * W;
+    float gx = 0.f;
+    for (int c = 0; c < C; ++c) {
+        const float go = grad_out[s * C + c];
+        unsafeAtomicAdd(grad_grid + base + c * W + c0, w0 * go);
+        unsafeAtomicAdd(grad_grid + base + c * W + c1, w1 * go);
+        if (grad_uv) gx += (grid[base + c * W + c1] - grid[base + c * W + c0]) * go;
+    }
+    if (grad_uv) grad_uv[s] += gx * ((float)(W - 1) / 2);
+}
+
+// GradientCorrection backward (modules/GradientCorrection.cu:22-124):
+// g - (g . d) d with d the direction of the sample's ray.
+__global__ __launch_bounds__(256) void gradient_correction_kernel(const float* __restrict__ dir,
+                                                                  const int64_t* __restrict__ ray, int64_t S,
+                                                                  const float* __restrict__ grad_out,
+                                                                  float* __restrict__ grad_pos)
+{
+    const int64_t s = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (s >= S) return;
+    const int64_t r = ray[s];
+    const float d0 = dir[r * 3], d1 = dir[r * 3 + 1], d2 = dir[r * 3 + 2];
+    const float g0 = grad_out[s * 3], g1 = grad_out[s * 3 + 1], g2 = grad_out[s * 3 + 2];
+    float dot = g0 * d0 + g1 * d1;
+    dot = dot + g2 * d2;
+    grad_pos[s * 3] = g0 - dot * d0;
+    grad_pos[s * 3 + 1] = g1 - dot * d1;
+    grad_pos[s * 3 + 2] = g2 - dot * d2;
+}
+
+// ComputeBoundWeight (structure/HyperTree.cu:962-993) with an O(1) node flag
+// instead of a scan over the in-ROI list: out[s] = s if in_roi_flag[node[s]]
+// else -1.
+__global__ __launch_bounds__(256) void bound_weight_kernel(const int64_t* __restrict__ node,
+                                                           const int32_t* __restrict__ flag, int64_t S,
+                                                           int64_t* __restrict__ out)
+{
+    const int64_t s = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (s >= S) return;
+    out[s] = flag[node[s]] ? s : -1;
+}
+
 __global__ __launch_bounds__(256) void integrate_kernel(const float* __restrict__ v, const float* __restrict__ w,
                                                         const int64_t* __restrict__ ray, int64_t S, int C, int64_t R,
                                                         float* __restrict__ out)
@@ -293,6 +462,74 @@ atdf_status atdf_grid_sample3d_bwd(const float* grid, int64_t N, int32_t C, int3
     hipLaunchKernelGGL(grid_sample3d_bwd_kernel, dim3(blocks_for(S)), dim3(256), 0, to_hip(stream), grid, C, Z, Y, X,
                        index, uv, S, grad_out, grad_grid, grad_uv);
     return check_launch("atdf_grid_sample3d_bwd");
+}
+
+atdf_status atdf_grid_sample2d_fwd(const float* grid, int64_t N, int32_t C, int32_t H, int32_t W,
+                                   const int64_t* index, const float* uv, int64_t S, float* out, atdf_stream_t stream)
+{
+    ATDF_REQUIRE(N > 0 && C > 0 && H > 0 && W > 0 && S >= 0, "atdf_grid_sample2d_fwd: bad shape");
+    if (S == 0) return ATDF_OK;
+    ATDF_REQUIRE(grid && index && uv && out, "atdf_grid_sample2d_fwd: null buffer");
+    hipLaunchKernelGGL(grid_sample2d_fwd_kernel, dim3(blocks_for(S)), dim3(256), 0, to_hip(stream), grid, C, H, W,
+                       index, uv, S, out);
+    return check_launch("atdf_grid_sample2d_fwd");
+}
+
+atdf_status atdf_grid_sample2d_bwd(const float* grid, int64_t N, int32_t C, int32_t H, int32_t W,
+                                   const int64_t* index, const float* uv, int64_t S, const float* grad_out,
+                                   float* grad_grid, float* grad_uv, atdf_stream_t stream)
+{
+    ATDF_REQUIRE(N > 0 && C > 0 && H > 0 && W > 0 && S >= 0, "atdf_grid_sample2d_bwd: bad shape");
+    if (S == 0) return ATDF_OK;
+    ATDF_REQUIRE(grid && index && uv && grad_out && grad_grid, "atdf_grid_sample2d_bwd: null buffer");
+    hipLaunchKernelGGL(grid_sample2d_bwd_kernel, dim3(blocks_for(S)), dim3(256), 0, to_hip(stream), grid, C, H, W,
+                       index, uv, S, grad_out, grad_grid, grad_uv);
+    return check_launch("atdf_grid_sample2d_bwd");
+}
+
+atdf_status atdf_grid_sample1d_fwd(const float* grid, int64_t N, int32_t C, int32_t W, const int64_t* index,
+                                   const float* uv, int64_t S, float* out, atdf_stream_t stream)
+{
+    ATDF_REQUIRE(N > 0 && C > 0 && W > 0 && S >= 0, "atdf_grid_sample1d_fwd: bad shape");
+    if (S == 0) return ATDF_OK;
+    ATDF_REQUIRE(grid && index && uv && out, "atdf_grid_sample1d_fwd: null buffer");
+    hipLaunchKernelGGL(grid_sample1d_fwd_kernel, dim3(blocks_for(S)), dim3(256), 0, to_hip(stream), grid, C, W, index,
+                       uv, S, out);
+    return check_launch("atdf_grid_sample1d_fwd");
+}
+
+atdf_status atdf_grid_sample1d_bwd(const float* grid, int64_t N, int32_t C, int32_t W, const int64_t* index,
+                                   const float* uv, int64_t S, const float* grad_out, float* grad_grid,
+                                   float* grad_uv, atdf_stream_t stream)
+{
+    ATDF_REQUIRE(N > 0 && C > 0 && W > 0 && S >= 0, "atdf_grid_sample1d_bwd: bad shape");
+    if (S == 0) return ATDF_OK;
+    ATDF_REQUIRE(grid && index && uv && grad_out && grad_grid, "atdf_grid_sample1d_bwd: null buffer");
+    hipLaunchKernelGGL(grid_sample1d_bwd_kernel, dim3(blocks_for(S)), dim3(256), 0, to_hip(stream), grid, C, W, index,
+                       uv, S, grad_out, grad_grid, grad_uv);
+    return check_launch("atdf_grid_sample1d_bwd");
+}
+
+atdf_status atdf_gradient_correction_bwd(const float* ray_direction, const int64_t* ray_index, int64_t S,
+                                         const float* grad_out, float* grad_position, atdf_stream_t stream)
+{
+    ATDF_REQUIRE(S >= 0, "atdf_gradient_correction_bwd: negative sample count");
+    if (S == 0) return ATDF_OK;
+    ATDF_REQUIRE(ray_direction && ray_index && grad_out && grad_position, "atdf_gradient_correction_bwd: null buffer");
+    hipLaunchKernelGGL(gradient_correction_kernel, dim3(blocks_for(S)), dim3(256), 0, to_hip(stream), ray_direction,
+                       ray_index, S, grad_out, grad_position);
+    return check_launch("atdf_gradient_correction_bwd");
+}
+
+atdf_status atdf_compute_bound_weight(const int64_t* node_id, int64_t S, const int32_t* in_roi_flag,
+                                      int64_t* out, atdf_stream_t stream)
+{
+    ATDF_REQUIRE(S >= 0, "atdf_compute_bound_weight: negative sample count");
+    if (S == 0) return ATDF_OK;
+    ATDF_REQUIRE(node_id && in_roi_flag && out, "atdf_compute_bound_weight: null buffer");
+    hipLaunchKernelGGL(bound_weight_kernel, dim3(blocks_for(S)), dim3(256), 0, to_hip(stream), node_id, in_roi_flag,
+                       S, out);
+    return check_launch("atdf_compute_bound_weight");
 }
 
 atdf_status atdf_integrate_xray(const float* values, const float* weight, const int64_t* ray_index, int64_t S,

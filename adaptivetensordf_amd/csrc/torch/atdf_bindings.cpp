@@ -73,6 +73,7 @@ PYBIND11_MODULE(TORCH_EXTENSION_NAME, m)
         .def("SetErrorForActiveNodes", &HyperTreeBase::SetErrorForActiveNodes)
         .def("UpdateCulling", &HyperTreeBase::UpdateCulling)
         .def("LeafTable", &HyperTreeBase::LeafTable)
+        .def("RoiSplit", &HyperTreeBase::RoiSplit)
         .def_readwrite("node_parent", &HyperTreeBase::node_parent)
         .def_readwrite("node_children", &HyperTreeBase::node_children)
         .def_readwrite("node_position_min", &HyperTreeBase::node_position_min)
@@ -87,7 +88,30 @@ PYBIND11_MODULE(TORCH_EXTENSION_NAME, m)
         .def_readwrite("active_node_ids", &HyperTreeBase::active_node_ids)
         .def_readwrite("node_to_grid", &HyperTreeBase::node_to_grid);
 
+    py::class_<RoiMaps>(m, "RoiMaps")
+        .def_readwrite("in_roi_node_index", &RoiMaps::in_roi_node_index)
+        .def_readwrite("out_roi_node_index", &RoiMaps::out_roi_node_index)
+        .def_readwrite("prefix_inroi", &RoiMaps::prefix_inroi)
+        .def_readwrite("prefix_outroi", &RoiMaps::prefix_outroi)
+        .def_readwrite("in_roi_flag", &RoiMaps::in_roi_flag);
+    py::class_<VMFields>(m, "VMFields")
+        .def(py::init<>())
+        .def_readwrite("plane_yz", &VMFields::plane_yz)
+        .def_readwrite("plane_xz", &VMFields::plane_xz)
+        .def_readwrite("plane_xy", &VMFields::plane_xy)
+        .def_readwrite("line_x", &VMFields::line_x)
+        .def_readwrite("line_y", &VMFields::line_y)
+        .def_readwrite("line_z", &VMFields::line_z)
+        .def_readwrite("grid_out", &VMFields::grid_out)
+        .def_readwrite("tensor_op", &VMFields::tensor_op)
+        .def_readwrite("roi_min", &VMFields::roi_min)
+        .def_readwrite("roi_max", &VMFields::roi_max);
     m.def("IndirectGridSample3D", &IndirectGridSample3D);
+    m.def("IndirectGridSample2D", &IndirectGridSample2D);
+    m.def("IndirectGridSample1D", &IndirectGridSample1D);
+    m.def("GradientCorrection", &GradientCorrection);
+    m.def("ComputeBoundWeight", &ComputeBoundWeight);
+    m.def("SampleVolumeIndirectFeature", &SampleVolumeIndirectFeature);
     m.def("IntegrateSamplesXRay", &IntegrateSamplesXRay);
     m.def("AccumulateSampleLossPerNode", &AccumulateSampleLossPerNode);
     m.def("ProjectRays", &ProjectRays);

@@ -431,6 +431,104 @@ struct IndirectGridSample3DFn : public torch::autograd::Function<IndirectGridSam
     }
 };
 
+struct IndirectGridSample2DFn : public torch::autograd::Function<IndirectGridSample2DFn> {
+    static Tensor forward(torch::autograd::AutogradContext* ctx, Tensor grid, Tensor index, Tensor uv)
+    {
+        require_device_f32(grid.contiguous(), "multi_grid");
+        require_device_f32(uv.contiguous(), "uv");
+        TORCH_CHECK(grid.dim() == 4, "multi_grid must be [N, C, H, W]");
+        TORCH_CHECK(index.dim() == 1 && index.scalar_type() == torch::kLong && index.is_cuda(), "index must be GPU int64 [S]");
+        TORCH_CHECK(uv.dim() == 2 && uv.size(1) == 2 && uv.size(0) == index.size(0), "uv must be [S, 2]");
+        auto g = grid.contiguous();
+        auto idx = index.contiguous();
+        auto u = uv.contiguous();
+        const int64_t S = idx.size(0), C = g.size(1);
+        auto out = torch::empty({S, C}, g.options());
+        check_status(atdf_grid_sample2d_fwd(g.data_ptr<float>(), g.size(0), (int)C, (int)g.size(2), (int)g.size(3),
+                                            idx.data_ptr<int64_t>(), u.data_ptr<float>(), S, out.data_ptr<float>(),
+                                            current_stream()),
+                     "atdf_grid_sample2d_fwd");
+        ctx->save_for_backward({g, idx, u});
+        return out;
+    }
+    static torch::autograd::variable_list backward(torch::autograd::AutogradContext* ctx,
+                                                   torch::autograd::variable_list grad_output)
+    {
+        auto saved = ctx->get_saved_variables();
+        auto g = saved[0], idx = saved[1], u = saved[2];
+        auto go = grad_output[0].contiguous();
+        auto grad_grid = torch::zeros_like(g);
+        auto grad_uv = torch::zeros_like(u);
+        check_status(atdf_grid_sample2d_bwd(g.data_ptr<float>(), g.size(0), (int)g.size(1), (int)g.size(2),
+                                            (int)g.size(3), idx.data_ptr<int64_t>(), u.data_ptr<float>(), idx.size(0),
+                                            go.data_ptr<float>(), grad_grid.data_ptr<float>(),
+                                            grad_uv.data_ptr<float>(), current_stream()),
+                     "atdf_grid_sample2d_bwd");
+        return {grad_grid, Tensor(), grad_uv};
+    }
+};
+
+struct IndirectGridSample1DFn : public torch::autograd::Function<IndirectGridSample1DFn> {
+    static Tensor forward(torch::autograd::AutogradContext* ctx, Tensor grid, Tensor index, Tensor uv)
+    {
+        require_device_f32(grid.contiguous(), "multi_grid");
+        require_device_f32(uv.contiguous(), "uv");
+        TORCH_CHECK(grid.dim() == 4 && grid.size(3) == 1, "multi_grid must be [N, C, W, 1]");
+        TORCH_CHECK(index.dim() == 1 && index.scalar_type() == torch::kLong && index.is_cuda(), "index must be GPU int64 [S]");
+        TORCH_CHECK(uv.numel() == index.size(0), "uv must hold one coordinate per sample");
+        auto g = grid.contiguous();
+        auto idx = index.contiguous();
+        auto u = uv.contiguous();
+        const int64_t S = idx.size(0), C = g.size(1);
+        auto out = torch::empty({S, C}, g.options());
+        check_status(atdf_grid_sample1d_fwd(g.data_ptr<float>(), g.size(0), (int)C, (int)g.size(2),
+                                            idx.data_ptr<int64_t>(), u.data_ptr<float>(), S, out.data_ptr<float>(),
+                                            current_stream()),
+                     "atdf_grid_sample1d_fwd");
+        ctx->save_for_backward({g, idx, u});
+        return out;
+    }
+    static torch::autograd::variable_list backward(torch::autograd::AutogradContext* ctx,
+                                                   torch::autograd::variable_list grad_output)
+    {
+        auto saved = ctx->get_saved_variables();
+        auto g = saved[0], idx = saved[1], u = saved[2];
+        auto go = grad_output[0].contiguous();
+        auto grad_grid = torch::zeros_like(g);
+        auto grad_uv = torch::zeros_like(u);
+        check_status(atdf_grid_sample1d_bwd(g.data_ptr<float>(), g.size(0), (int)g.size(1), (int)g.size(2),
+                                            idx.data_ptr<int64_t>(), u.data_ptr<float>(), idx.size(0),
+                                            go.data_ptr<float>(), grad_grid.data_ptr<float>(),
+                                            grad_uv.data_ptr<float>(), current_stream()),
+                     "atdf_grid_sample1d_bwd");
+        return {grad_grid, Tensor(), grad_uv};
+    }
+};
+
+struct GradientCorrectionFn : public torch::autograd::Function<GradientCorrectionFn> {
+    static Tensor forward(torch::autograd::AutogradContext* ctx, Tensor sample_position, Tensor ray_direction,
+                          Tensor index)
+    {
+        require_device_f32(sample_position, "sample_position");
+        require_device_f32(ray_direction, "ray_direction");
+        TORCH_CHECK(index.scalar_type() == torch::kLong && index.is_cuda(), "index must be GPU int64");
+        ctx->save_for_backward({ray_direction.contiguous(), index.contiguous()});
+        return sample_position.view_as(sample_position);
+    }
+    static torch::autograd::variable_list backward(torch::autograd::AutogradContext* ctx,
+                                                   torch::autograd::variable_list grad_output)
+    {
+        auto saved = ctx->get_saved_variables();
+        auto dir = saved[0], idx = saved[1];
+        auto go = grad_output[0].contiguous();
+        auto grad = torch::empty_like(go);
+        check_status(atdf_gradient_correction_bwd(dir.data_ptr<float>(), idx.data_ptr<int64_t>(), idx.numel(),
+                                                  go.data_ptr<float>(), grad.data_ptr<float>(), current_stream()),
+                     "atdf_gradient_correction_bwd");
+        return {grad, Tensor(), Tensor()};
+    }
+};
+
 struct IntegrateFn : public torch::autograd::Function<IntegrateFn> {
     static Tensor forward(torch::autograd::AutogradContext* ctx, Tensor values, Tensor weight, Tensor ray_index,
                           int64_t C, int64_t R)
@@ -468,6 +566,148 @@ struct IntegrateFn : public torch::autograd::Function<IntegrateFn> {
 Tensor IndirectGridSample3D(Tensor multi_grid, Tensor index, Tensor uv)
 {
     return IndirectGridSample3DFn::apply(multi_grid, index, uv);
+}
+
+Tensor IndirectGridSample2D(Tensor multi_grid, Tensor index, Tensor uv)
+{
+    return IndirectGridSample2DFn::apply(multi_grid, index, uv);
+}
+
+Tensor IndirectGridSample1D(Tensor multi_grid, Tensor index, Tensor uv)
+{
+    return IndirectGridSample1DFn::apply(multi_grid, index, uv);
+}
+
+Tensor GradientCorrection(Tensor sample_position, Tensor ray_direction, Tensor index)
+{
+    return GradientCorrectionFn::apply(sample_position, ray_direction, index);
+}
+
+Tensor ComputeBoundWeight(Tensor node_id, const RoiMaps& roi)
+{
+    TORCH_CHECK(node_id.scalar_type() == torch::kLong && node_id.is_cuda(), "node_id must be GPU int64");
+    auto nid = node_id.reshape({-1}).contiguous();
+    auto out = torch::empty({nid.size(0)}, nid.options());
+    check_status(atdf_compute_bound_weight(nid.data_ptr<int64_t>(), nid.size(0), roi.in_roi_flag.data_ptr<int32_t>(),
+                                           out.data_ptr<int64_t>(), current_stream()),
+                 "atdf_compute_bound_weight");
+    return out;
+}
+
+namespace {
+// geometry/geometry.h:426-449: a node is in the ROI when, on every axis, its
+// min, its max or its "mid" (min + max, as the reference computes it) lies
+// strictly inside the ROI interval; the y interval is widened by epsilon.
+bool node_in_roi(const float* mn, const float* mx, const float* roi_min, const float* roi_max, float eps)
+{
+    for (int k = 0; k < 3; ++k) {
+        const float lo = k == 1 ? roi_min[k] - eps : roi_min[k];
+        const float hi = k == 1 ? roi_max[k] + eps : roi_max[k];
+        const float mid = mn[k] + mx[k];
+        if (!((mn[k] > lo && mn[k] < hi) || (mx[k] > lo && mx[k] < hi) || (mid > lo && mid < hi))) return false;
+    }
+    return true;
+}
+}  // namespace
+
+RoiMaps HyperTreeBase::RoiSplit(std::vector<float> roi_min, std::vector<float> roi_max, float epsilon) const
+{
+    TORCH_CHECK(roi_min.size() == 3 && roi_max.size() == 3, "RoiSplit: ROI bounds must have 3 entries");
+    auto pmin = node_position_min.to(torch::kCPU).contiguous();
+    auto pmax = node_position_max.to(torch::kCPU).contiguous();
+    auto act = node_active.to(torch::kCPU).contiguous();
+    const float* mn = pmin.data_ptr<float>();
+    const float* mx = pmax.data_ptr<float>();
+    const int32_t* a = act.data_ptr<int32_t>();
+    const int64_t n = act.size(0);
+    std::vector<int64_t> in_ids, out_ids, pin(n, -1), pout(n, -1);
+    std::vector<int32_t> flag(n, 0);
+    for (int64_t i = 0; i < n; ++i) {
+        if (a[i] != 1) continue;
+        if (node_in_roi(mn + 3 * i, mx + 3 * i, roi_min.data(), roi_max.data(), epsilon)) {
+            pin[i] = (int64_t)in_ids.size();
+            in_ids.push_back(i);
+            flag[i] = 1;
+        } else {
+            pout[i] = (int64_t)out_ids.size();
+            out_ids.push_back(i);
+        }
+    }
+    auto dev = node_position_min.device();
+    auto to_t = [&](std::vector<int64_t>& v) {
+        return torch::from_blob(v.data(), {(int64_t)v.size()}, torch::kLong).clone().to(dev);
+    };
+    RoiMaps m;
+    m.in_roi_node_index = to_t(in_ids);
+    m.out_roi_node_index = to_t(out_ids);
+    m.prefix_inroi = to_t(pin);
+    m.prefix_outroi = to_t(pout);
+    m.in_roi_flag = torch::from_blob(flag.data(), {n}, torch::kInt32).clone().to(dev);
+    m.roi_min = roi_min;
+    m.roi_max = roi_max;
+    return m;
+}
+
+Tensor SampleVolumeIndirectFeature(HyperTreeBase& tree, const RoiMaps& roi, const VMFields& f,
+                                   Tensor global_coordinate, Tensor node_id)
+{
+    if (global_coordinate.numel() == 0) return global_coordinate;
+    auto g = global_coordinate.reshape({-1, 3});
+    auto nid = node_id.reshape({-1}).contiguous();
+    const int64_t S = nid.size(0);
+    auto dev = g.device();
+    // route samples by the ROI membership of their node
+    auto bound = ComputeBoundWeight(nid, roi);
+    auto all = torch::arange(S, torch::TensorOptions().dtype(torch::kLong).device(dev));
+    auto in_idx = torch::masked_select(all, bound >= 0);
+    auto out_idx = torch::masked_select(all, bound < 0);
+    auto nid_in = nid.index_select(0, in_idx);
+    auto nid_out = nid.index_select(0, out_idx);
+    auto g_in = g.index_select(0, in_idx);
+    auto g_out = g.index_select(0, out_idx);
+    Tensor feat_in;
+    {
+        // in-ROI nodes: plane x line features (geometry_tensor_qo.cpp:279-327)
+        auto local = tree.ComputeLocalSamples(g_in, nid_in);
+        auto slot = roi.prefix_inroi.index_select(0, nid_in).contiguous();
+        auto lx = local.slice(1, 0, 1), ly = local.slice(1, 1, 2), lz = local.slice(1, 2, 3);
+        auto plane_yz = torch::cat({ly, lz}, 1);
+        auto plane_xz = torch::cat({lx, lz}, 1);
+        auto plane_xy = torch::cat({lx, ly}, 1);
+        auto vec_id = torch::zeros({local.size(0)}, torch::TensorOptions().dtype(torch::kLong).device(dev));
+        auto line_x = 2. * (g_in.slice(1, 0, 1) - f.roi_min[0]) / (f.roi_max[0] - f.roi_min[0]) - 1;
+        auto line_y = ly;
+        auto line_z = g_in.slice(1, 2, 3);
+        if (f.tensor_op == "cat") {
+            auto mat = torch::cat({IndirectGridSample2D(f.plane_yz, slot, plane_yz),
+                                   IndirectGridSample2D(f.plane_xz, slot, plane_xz),
+                                   IndirectGridSample2D(f.plane_xy, slot, plane_xy)},
+                                  1);
+            auto vec = torch::cat({IndirectGridSample1D(f.line_x, vec_id, line_x),
+                                   IndirectGridSample1D(f.line_y, vec_id, line_y),
+                                   IndirectGridSample1D(f.line_z, vec_id, line_z)},
+                                  1);
+            feat_in = mat * vec;
+        } else {
+            TORCH_CHECK(f.tensor_op == "add", "tensor_op must be add or cat");
+            feat_in = IndirectGridSample2D(f.plane_yz, slot, plane_yz) * IndirectGridSample1D(f.line_x, vec_id, line_x);
+            feat_in = feat_in + IndirectGridSample2D(f.plane_xz, slot, plane_xz) *
+                                    IndirectGridSample1D(f.line_y, vec_id, line_y);
+            feat_in = feat_in + IndirectGridSample2D(f.plane_xy, slot, plane_xy) *
+                                    IndirectGridSample1D(f.line_z, vec_id, line_z);
+        }
+    }
+    Tensor feat_out;
+    if (f.grid_out.defined()) {
+        // out-of-ROI nodes: explicit 3-D grids (geometry_tensor_qo.cpp:336-347)
+        auto local = tree.ComputeLocalSamples(g_out, nid_out);
+        auto slot = roi.prefix_outroi.index_select(0, nid_out).contiguous();
+        feat_out = IndirectGridSample3D(f.grid_out, slot, local);
+    }
+    auto feats = torch::zeros({S, feat_in.size(1)}, feat_in.options());
+    feats = feats.index_add(0, in_idx, feat_in);
+    if (feat_out.defined()) feats = feats.index_add(0, out_idx, feat_out);
+    return feats;
 }
 
 Tensor IntegrateSamplesXRay(Tensor sample_values, Tensor integration_weight, Tensor ray_index, int num_channels,

@@ -68,6 +68,17 @@ struct SamplingParams {
     atdf_sampling view(const float* jitter = nullptr) const;
 };
 
+// Split of the active nodes by the reconstruction ROI (tensorial VM
+// features inside, explicit grids outside), geometry_tensor_qo.cpp:940-1030.
+struct RoiMaps {
+    torch::Tensor in_roi_node_index;   // [N_in] i64 node ids
+    torch::Tensor out_roi_node_index;  // [N_out] i64
+    torch::Tensor prefix_inroi;        // [Nn] i64 slot in the VM planes, -1 elsewhere
+    torch::Tensor prefix_outroi;       // [Nn] i64 slot in the explicit grids, -1 elsewhere
+    torch::Tensor in_roi_flag;         // [Nn] i32 0/1
+    std::vector<float> roi_min, roi_max;
+};
+
 // HyperTreeBaseImpl (structure/HyperTree.h:395-563) for D = 3, octree.
 class HyperTreeBase {
    public:
@@ -90,6 +101,8 @@ class HyperTreeBase {
     void SetErrorForActiveNodes(torch::Tensor error, const std::string& strategy);
     // HyperTree.cpp:666-705
     void UpdateCulling();
+    // in_roi test of geometry/geometry.h:426-449 over the active nodes
+    RoiMaps RoiSplit(std::vector<float> roi_min, std::vector<float> roi_max, float epsilon) const;
 
     const void* packed() const { return packed_.data_ptr(); }
     // point-location table of the active set (rebuilt with the packed tree)
@@ -111,6 +124,32 @@ class HyperTreeBase {
 
 // IndirectGridSample3D(multi_grid, index, uv): grid [N, C, Z, Y, X]
 torch::Tensor IndirectGridSample3D(torch::Tensor multi_grid, torch::Tensor index, torch::Tensor uv);
+
+// IndirectGridSample2D / 1D (modules/IndirectGridSample2D.h:10-24,
+// IndirectGridSample1D.cu): grid [N, C, H, W] with uv [S, 2]; grid
+// [N, C, W, 1] with uv [S, 1].
+torch::Tensor IndirectGridSample2D(torch::Tensor multi_grid, torch::Tensor index, torch::Tensor uv);
+torch::Tensor IndirectGridSample1D(torch::Tensor multi_grid, torch::Tensor index, torch::Tensor uv);
+// GradientCorrection (modules/GradientCorrection.h:11-25): identity forward,
+// backward removes the gradient component along the sample's ray.
+torch::Tensor GradientCorrection(torch::Tensor sample_position, torch::Tensor ray_direction, torch::Tensor index);
+// HyperTreeBaseImpl::ComputeBoundWeight(node_id, in_roi_node, false) (HyperTree.cu:962-993)
+torch::Tensor ComputeBoundWeight(torch::Tensor node_id, const RoiMaps& roi);
+
+// Tensorial (VM) fields of the in-ROI nodes and explicit grids of the rest
+// (GeometryTensorQO, geometry_tensor_qo.cpp:257-380).
+struct VMFields {
+    torch::Tensor plane_yz, plane_xz, plane_xy;  // [N_in, C, G, G] (u -> last axis)
+    torch::Tensor line_x, line_y, line_z;        // [1, C, G, 1]
+    torch::Tensor grid_out;                      // [N_out, C', G, G, G] (may be undefined)
+    std::string tensor_op = "add";               // "add" or "cat"
+    std::vector<float> roi_min{-1.f, -1.f, -1.f}, roi_max{1.f, 1.f, 1.f};
+};
+// GeometryTensorQO::SampleVolumeIndirect_feature: features [S, C] of the
+// samples, VM plane x line products for in-ROI nodes, 3-D gathers for the
+// others, merged back in sample order (autograd through every part).
+torch::Tensor SampleVolumeIndirectFeature(HyperTreeBase& tree, const RoiMaps& roi, const VMFields& fields,
+                                          torch::Tensor global_coordinate, torch::Tensor node_id);
 
 // NeuralGeometry::IntegrateSamplesXRay (geometry.cpp:977-1019)
 torch::Tensor IntegrateSamplesXRay(torch::Tensor sample_values, torch::Tensor integration_weight,

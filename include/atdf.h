@@ -144,6 +144,42 @@ atdf_status atdf_grid_sample3d_bwd(const float* grid, int64_t N, int32_t C, int3
                                    const float* grad_out, float* grad_grid, float* grad_uv,
                                    atdf_stream_t stream);
 
+/* IndirectGridSample2D(multi_grid, index, uv) (modules/IndirectGridSample2D.h:10-24,
+ * IndirectGridSample2D.cu:22-492; called for the in-ROI planes at
+ * geometry/geometry_tensor_qo.cpp:306-327): grid [N, C, H, W], uv [S, 2]
+ * (u -> W, v -> H), corners tnw tne tsw tse, clipped indices.  Backward:
+ * grad_grid += scatter of grad_out; grad_uv (nullable) += the analytic
+ * derivative (the reference leaves it at zero). */
+atdf_status atdf_grid_sample2d_fwd(const float* grid, int64_t N, int32_t C, int32_t H, int32_t W,
+                                   const int64_t* index, const float* uv, int64_t num_samples, float* out,
+                                   atdf_stream_t stream);
+atdf_status atdf_grid_sample2d_bwd(const float* grid, int64_t N, int32_t C, int32_t H, int32_t W,
+                                   const int64_t* index, const float* uv, int64_t num_samples,
+                                   const float* grad_out, float* grad_grid, float* grad_uv, atdf_stream_t stream);
+/* IndirectGridSample1D (modules/IndirectGridSample1D.cu:20-221; the in-ROI
+ * lines at geometry_tensor_qo.cpp:306-327): grid [N, C, W, 1], uv [S];
+ * grad_uv (nullable) += the analytic derivative (the reference sums a
+ * different expression, IndirectGridSample1D.cu:128-134). */
+atdf_status atdf_grid_sample1d_fwd(const float* grid, int64_t N, int32_t C, int32_t W, const int64_t* index,
+                                   const float* uv, int64_t num_samples, float* out, atdf_stream_t stream);
+atdf_status atdf_grid_sample1d_bwd(const float* grid, int64_t N, int32_t C, int32_t W, const int64_t* index,
+                                   const float* uv, int64_t num_samples, const float* grad_out, float* grad_grid,
+                                   float* grad_uv, atdf_stream_t stream);
+
+/* GradientCorrection(sample_position, ray_direction, index) backward
+ * (modules/GradientCorrection.h:11-25, GradientCorrection.cu:22-124): the
+ * forward is the identity; grad_position = g - (g . d) d with d =
+ * ray_direction[ray_index[s]]. */
+atdf_status atdf_gradient_correction_bwd(const float* ray_direction, const int64_t* ray_index, int64_t num_samples,
+                                         const float* grad_out, float* grad_position, atdf_stream_t stream);
+
+/* HyperTreeBaseImpl::ComputeBoundWeight(node_id, in_roi_node, false)
+ * (structure/HyperTree.cu:962-993): out[s] = s when node_id[s] is an in-ROI
+ * node, else -1.  in_roi_flag is [num_nodes] 0/1 (one lookup per sample
+ * instead of the reference's scan over the in-ROI list). */
+atdf_status atdf_compute_bound_weight(const int64_t* node_id, int64_t num_samples, const int32_t* in_roi_flag,
+                                      int64_t* out, atdf_stream_t stream);
+
 /* NeuralGeometry::IntegrateSamplesXRay (geometry/geometry.h:269-270,
  * geometry/geometry.cpp:977-1019): out[c, ray[s]] += values[s, c] * weight[s]. */
 atdf_status atdf_integrate_xray(const float* values, const float* weight, const int64_t* ray_index,

@@ -171,6 +171,24 @@ double atdfo_train_step_f32(atdfo_model* m, float* grid_mut, float* mlp_mut, int
                             float* adam_m_mlp, float* adam_v_mlp, int step, float lr, int num_threads,
                             int do_adam);
 
+/* ---- tensorial (VM) features and helpers (atdf_oracle_vm.c) -----------
+ * 2-D gather: grid [N, C, H, W], uv [S, 2] (u -> W, v -> H); 1-D gather:
+ * grid [N, C, W, 1], uv [S]; align_corners mapping, clipped corners. */
+void atdfo_grid_sample2d_fwd(const float* grid, int64_t N, int C, int H, int W, const int64_t* index,
+                             const float* uv, int64_t S, float* out);
+void atdfo_grid_sample2d_bwd(const float* grid, int64_t N, int C, int H, int W, const int64_t* index,
+                             const float* uv, int64_t S, const float* grad_out, double* grad_grid, double* grad_uv);
+void atdfo_grid_sample1d_fwd(const float* grid, int64_t N, int C, int W, const int64_t* index, const float* uv,
+                             int64_t S, float* out);
+void atdfo_grid_sample1d_bwd(const float* grid, int64_t N, int C, int W, const int64_t* index, const float* uv,
+                             int64_t S, const float* grad_out, double* grad_grid, double* grad_uv);
+/* GradientCorrection backward: g - (g . d) d with d the sample's ray direction */
+void atdfo_gradient_correction(const float* ray_direction, const int64_t* ray_index, const float* grad_out,
+                               int64_t S, float* grad_pos);
+/* ComputeBoundWeight: out[s] = s if node_id[s] is in in_roi_node, else -1 */
+void atdfo_compute_bound_weight(const int64_t* node_id, int64_t S, const int64_t* in_roi_node, int64_t n_in,
+                                int64_t* out);
+
 /* ---- learned sample proposal (atdf_oracle_proposal.c) -----------------
  * MLP 38 -> 64 -> 64 -> 64 -> 32 (ReLU hidden), inputs ray origin (3),
  * direction (3) and the coarse occupancy of 32 equal bins of the ray's chord

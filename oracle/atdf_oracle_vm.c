@@ -1,0 +1,142 @@
+/*
+ * atdf_oracle_vm.c -- CPU oracle of the 2-D / 1-D indirect gathers (the
+ * tensorial plane and line features of in-ROI nodes), of GradientCorrection
+ * and of the per-sample ROI test.
+ *
+ * TEST INFRASTRUCTURE ONLY (see atdf_oracle.h).  Follows
+ *   IndirectGridSample2D  modules/IndirectGridSample2D.cu:22-162 (forward),
+ *                         175-393 (backward: atomics into clipped corners)
+ *   IndirectGridSample1D  modules/IndirectGridSample1D.cu:20-140
+ *   GradientCorrection    modules/GradientCorrection.cu:22-124
+ *   ComputeBoundWeight    structure/HyperTree.cu:962-993
+ * Forward values are evaluated in the reference's order with no FMA
+ * contraction (bit-exact with the kernels); gradients accumulate in FP64.
+ * grad_uv is the analytic derivative of the forward (the reference leaves it
+ * at zero in 2-D and sums a different expression in 1-D).
+ */
+#include <math.h>
+#include <stdint.h>
+
+#include "atdf_oracle.h"
+
+static int clip(int v, int lim)
+{
+    int a = v > 0 ? v : 0;
+    return a < lim - 1 ? a : lim - 1;
+}
+
+void atdfo_grid_sample2d_fwd(const float* grid, int64_t N, int C, int H, int W, const int64_t* index,
+                             const float* uv, int64_t S, float* out)
+{
+    (void)N;
+    const int64_t plane = (int64_t)H * W;
+    for (int64_t s = 0; s < S; ++s) {
+        float ix = ((uv[s * 2] + 1) / 2) * (float)(W - 1);
+        float iy = ((uv[s * 2 + 1] + 1) / 2) * (float)(H - 1);
+        int x0 = (int)floorf(ix), y0 = (int)floorf(iy);
+        int x1 = x0 + 1, y1 = y0 + 1;
+        float fx1 = (float)x1 - ix, fx0 = ix - (float)x0;
+        float fy1 = (float)y1 - iy, fy0 = iy - (float)y0;
+        float w[4] = {fx1 * fy1, fx0 * fy1, fx1 * fy0, fx0 * fy0}; /* tnw tne tsw tse */
+        int cx[2] = {clip(x0, W), clip(x1, W)}, cy[2] = {clip(y0, H), clip(y1, H)};
+        const float* g = grid + index[s] * C * plane;
+        for (int c = 0; c < C; ++c) {
+            float acc = 0;
+            for (int k = 0; k < 4; ++k) acc += g[c * plane + (int64_t)cy[k >> 1] * W + cx[k & 1]] * w[k];
+            out[s * C + c] = acc;
+        }
+    }
+}
+
+void atdfo_grid_sample2d_bwd(const float* grid, int64_t N, int C, int H, int W, const int64_t* index,
+                             const float* uv, int64_t S, const float* grad_out, double* grad_grid, double* grad_uv)
+{
+    (void)N;
+    const int64_t plane = (int64_t)H * W;
+    for (int64_t s = 0; s < S; ++s) {
+        double ix = ((uv[s * 2] + 1.0) / 2) * (W - 1);
+        double iy = ((uv[s * 2 + 1] + 1.0) / 2) * (H - 1);
+        int x0 = (int)floor(ix), y0 = (int)floor(iy);
+        double fx[2] = {x0 + 1 - ix, ix - x0}, fy[2] = {y0 + 1 - iy, iy - y0};
+        int cx[2] = {clip(x0, W), clip(x0 + 1, W)}, cy[2] = {clip(y0, H), clip(y0 + 1, H)};
+        double gx = 0, gy = 0;
+        for (int c = 0; c < C; ++c) {
+            double go = grad_out[s * C + c];
+            for (int k = 0; k < 4; ++k) {
+                int bx = k & 1, by = k >> 1;
+                int64_t off = (index[s] * C + c) * plane + (int64_t)cy[by] * W + cx[bx];
+                grad_grid[off] += fx[bx] * fy[by] * go;
+                double v = grid[off];
+                gx += v * (bx ? 1.0 : -1.0) * fy[by] * go;
+                gy += v * fx[bx] * (by ? 1.0 : -1.0) * go;
+            }
+        }
+        if (grad_uv) {
+            grad_uv[s * 2] += gx * (W - 1) / 2.0;
+            grad_uv[s * 2 + 1] += gy * (H - 1) / 2.0;
+        }
+    }
+}
+
+void atdfo_grid_sample1d_fwd(const float* grid, int64_t N, int C, int W, const int64_t* index, const float* uv,
+                             int64_t S, float* out)
+{
+    (void)N;
+    for (int64_t s = 0; s < S; ++s) {
+        float ix = ((uv[s] + 1) / 2) * (float)(W - 1);
+        int x0 = (int)floorf(ix);
+        int x1 = x0 + 1;
+        float w0 = (float)x1 - ix, w1 = ix - (float)x0;
+        int c0 = clip(x0, W), c1 = clip(x1, W);
+        const float* g = grid + index[s] * C * W;
+        for (int c = 0; c < C; ++c) {
+            float acc = 0;
+            acc += g[c * W + c0] * w0;
+            acc += g[c * W + c1] * w1;
+            out[s * C + c] = acc;
+        }
+    }
+}
+
+void atdfo_grid_sample1d_bwd(const float* grid, int64_t N, int C, int W, const int64_t* index, const float* uv,
+                             int64_t S, const float* grad_out, double* grad_grid, double* grad_uv)
+{
+    (void)N;
+    for (int64_t s = 0; s < S; ++s) {
+        double ix = ((uv[s] + 1.0) / 2) * (W - 1);
+        int x0 = (int)floor(ix);
+        double w0 = x0 + 1 - ix, w1 = ix - x0;
+        int c0 = clip(x0, W), c1 = clip(x0 + 1, W);
+        double gx = 0;
+        for (int c = 0; c < C; ++c) {
+            double go = grad_out[s * C + c];
+            int64_t base = (index[s] * C + c) * W;
+            grad_grid[base + c0] += w0 * go;
+            grad_grid[base + c1] += w1 * go;
+            gx += (grid[base + c1] - grid[base + c0]) * go;
+        }
+        if (grad_uv) grad_uv[s] += gx * (W - 1) / 2.0;
+    }
+}
+
+void atdfo_gradient_correction(const float* ray_direction, const int64_t* ray_index, const float* grad_out,
+                               int64_t S, float* grad_pos)
+{
+    for (int64_t s = 0; s < S; ++s) {
+        const float* d = ray_direction + ray_index[s] * 3;
+        const float* g = grad_out + s * 3;
+        float dot = g[0] * d[0] + g[1] * d[1];
+        dot = dot + g[2] * d[2];
+        for (int k = 0; k < 3; ++k) grad_pos[s * 3 + k] = g[k] - dot * d[k];
+    }
+}
+
+void atdfo_compute_bound_weight(const int64_t* node_id, int64_t S, const int64_t* in_roi_node, int64_t n_in,
+                                int64_t* out)
+{
+    for (int64_t s = 0; s < S; ++s) {
+        out[s] = -1;
+        for (int64_t i = 0; i < n_in; ++i)
+            if (node_id[s] == in_roi_node[i]) out[s] = s;
+    }
+}

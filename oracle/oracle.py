@@ -172,6 +172,63 @@ def grid_sample3d_bwd(grid, index, uv, grad_out):
     return gg, gu
 
 
+def grid_sample2d_fwd(grid, index, uv):
+    grid = _c(grid, np.float32)
+    N, Cc, H, W = grid.shape
+    index, uv = _c(index, np.int64), _c(uv, np.float32)
+    out = np.zeros((index.shape[0], Cc), np.float32)
+    lib().atdfo_grid_sample2d_fwd(_p(grid), C.c_int64(N), C.c_int(Cc), C.c_int(H), C.c_int(W), _p(index), _p(uv),
+                                  C.c_int64(index.shape[0]), _p(out))
+    return out
+
+
+def grid_sample2d_bwd(grid, index, uv, grad_out):
+    grid = _c(grid, np.float32)
+    N, Cc, H, W = grid.shape
+    index, uv, go = _c(index, np.int64), _c(uv, np.float32), _c(grad_out, np.float32)
+    gg = np.zeros(grid.shape, np.float64)
+    gu = np.zeros(uv.shape, np.float64)
+    lib().atdfo_grid_sample2d_bwd(_p(grid), C.c_int64(N), C.c_int(Cc), C.c_int(H), C.c_int(W), _p(index), _p(uv),
+                                  C.c_int64(index.shape[0]), _p(go), _p(gg), _p(gu))
+    return gg, gu
+
+
+def grid_sample1d_fwd(grid, index, uv):
+    grid = _c(grid, np.float32)
+    N, Cc, W = grid.shape[:3]
+    index, uv = _c(index, np.int64), _c(uv, np.float32).reshape(-1)
+    out = np.zeros((index.shape[0], Cc), np.float32)
+    lib().atdfo_grid_sample1d_fwd(_p(grid), C.c_int64(N), C.c_int(Cc), C.c_int(W), _p(index), _p(uv),
+                                  C.c_int64(index.shape[0]), _p(out))
+    return out
+
+
+def grid_sample1d_bwd(grid, index, uv, grad_out):
+    grid = _c(grid, np.float32)
+    N, Cc, W = grid.shape[:3]
+    index, uv, go = _c(index, np.int64), _c(uv, np.float32).reshape(-1), _c(grad_out, np.float32)
+    gg = np.zeros(grid.shape, np.float64)
+    gu = np.zeros(uv.shape, np.float64)
+    lib().atdfo_grid_sample1d_bwd(_p(grid), C.c_int64(N), C.c_int(Cc), C.c_int(W), _p(index), _p(uv),
+                                  C.c_int64(index.shape[0]), _p(go), _p(gg), _p(gu))
+    return gg, gu
+
+
+def gradient_correction(direction, ray_index, grad_out):
+    direction, ray_index, go = _c(direction, np.float32), _c(ray_index, np.int64), _c(grad_out, np.float32)
+    out = np.zeros_like(go)
+    lib().atdfo_gradient_correction(_p(direction), _p(ray_index), _p(go), C.c_int64(go.shape[0]), _p(out))
+    return out
+
+
+def compute_bound_weight(node_id, in_roi_node):
+    node_id, in_roi_node = _c(node_id, np.int64), _c(in_roi_node, np.int64)
+    out = np.zeros(node_id.shape[0], np.int64)
+    lib().atdfo_compute_bound_weight(_p(node_id), C.c_int64(node_id.shape[0]), _p(in_roi_node),
+                                     C.c_int64(in_roi_node.shape[0]), _p(out))
+    return out
+
+
 # ------------------------------------------------------------------- mlp
 def mlp_fwd(params, Cc, H, beta, x):
     params = _c(params, np.float32)

@@ -108,3 +108,43 @@ def test_host_error_and_culling_semantics(native):
     assert t.node_error[t.node_children[1].long()].abs().sum().item() == 0
     with pytest.raises(RuntimeError):
         t.SetErrorForActiveNodes(torch.zeros(3), "override")
+
+
+def _np_in_roi(mn, mx, lo, hi, eps):
+    """geometry/geometry.h:426-449 restated: on every axis the min, the max or
+    min + max (the reference's "mid") lies strictly inside the interval; the
+    y interval widened by eps."""
+    lo = np.array(lo, np.float32).copy()
+    hi = np.array(hi, np.float32).copy()
+    lo[1] -= np.float32(eps)
+    hi[1] += np.float32(eps)
+    mid = mn + mx
+    ok = ((mn > lo) & (mn < hi)) | ((mx > lo) & (mx < hi)) | ((mid > lo) & (mid < hi))
+    return ok.all(axis=1)
+
+
+def test_roi_split_host(native):
+    t = native.HyperTreeBase(3, 3)
+    t.settree()
+    t.SetActive(2)
+    lo, hi = [-0.5, -0.6, -1.0], [0.5, 0.2, 0.1]
+    m = t.RoiSplit(lo, hi, 0.0)
+    act = t.node_active.numpy().astype(bool)
+    mn, mx = t.node_position_min.numpy(), t.node_position_max.numpy()
+    ref = _np_in_roi(mn, mx, lo, hi, 0.0) & act
+    assert np.array_equal(m.in_roi_flag.numpy().astype(bool), ref)
+    assert np.array_equal(m.in_roi_node_index.numpy(), np.nonzero(ref)[0])
+    assert np.array_equal(m.out_roi_node_index.numpy(), np.nonzero(act & ~ref)[0])
+    pin = m.prefix_inroi.numpy()
+    assert np.array_equal(pin[ref], np.arange(ref.sum())) and np.all(pin[~ref] == -1)
+    pout = m.prefix_outroi.numpy()
+    assert np.array_equal(pout[act & ~ref], np.arange((act & ~ref).sum())) and np.all(pout[~(act & ~ref)] == -1)
+    # hand-checked nodes: the depth-2 node [-0.5, 0]^3 is inside; [0.5, 1] on x is not (min 0.5 is not < 0.5,
+    # max 1 is outside, and min + max = 1.5 is outside)
+    ids = np.nonzero(act)[0]
+    inside = [i for i in ids if np.allclose(mn[i], [-0.5, -0.5, -0.5])][0]
+    outside = [i for i in ids if np.isclose(mn[i][0], 0.5)][0]
+    assert ref[inside] and not ref[outside]
+    # the widened y interval admits a node whose y-min sits on the ROI bound
+    m2 = t.RoiSplit([-1, 0.5, -1], [1, 1, 1], 1e-6)
+    assert np.array_equal(m2.in_roi_flag.numpy().astype(bool), _np_in_roi(mn, mx, [-1, 0.5, -1], [1, 1, 1], 1e-6) & act)

@@ -1,0 +1,168 @@
+"""Tensorial (VM) feature path on the GPU against the oracle: 2-D / 1-D
+indirect gathers (forward bit-exact, backward within 1e-5 of FP64),
+GradientCorrection, ComputeBoundWeight, and the ROI-routed feature
+composition SampleVolumeIndirectFeature ("add" and "cat") with its
+autograd gradients."""
+import numpy as np
+import pytest
+import torch
+
+from conftest import random_rays
+from scene_util import adaptive_active, device_tree
+
+pytestmark = pytest.mark.gpu
+
+
+def _rel(a, b):
+    return np.abs(a - b).max() / max(np.abs(b).max(), 1e-30)
+
+
+def test_grid_sample2d_1d(oracle, native, cuda):
+    rng = np.random.default_rng(0)
+    for C_, H, W in ((16, 11, 9), (8, 5, 13)):
+        grid = rng.normal(size=(30, C_, H, W)).astype(np.float32)
+        S = 20000
+        index = rng.integers(0, 30, S)
+        uv = rng.uniform(-1.05, 1.05, (S, 2)).astype(np.float32)
+        uv[:5] = [[-1, -1], [1, 1], [1, -1], [0, 0], [1.0000001, -1.0000001]]
+        g_t = torch.from_numpy(grid).cuda().requires_grad_()
+        u_t = torch.from_numpy(uv).cuda().requires_grad_()
+        out = native.IndirectGridSample2D(g_t, torch.from_numpy(index).cuda(), u_t)
+        ref = oracle.grid_sample2d_fwd(grid, index, uv)
+        assert np.array_equal(out.detach().cpu().numpy().view(np.uint32), ref.view(np.uint32))
+        go = rng.normal(size=(S, C_)).astype(np.float32)
+        gg, gu = torch.autograd.grad(out, [g_t, u_t], torch.from_numpy(go).cuda())
+        rg, ru = oracle.grid_sample2d_bwd(grid, index, uv, go)
+        print(f"2-D C={C_} {H}x{W}: grad_grid {_rel(gg.cpu().numpy(), rg):.2e}, grad_uv {_rel(gu.cpu().numpy(), ru):.2e}",
+              flush=True)
+        assert _rel(gg.cpu().double().numpy(), rg) <= 1e-5 and _rel(gu.cpu().double().numpy(), ru) <= 1e-5
+    line = rng.normal(size=(2, 16, 23, 1)).astype(np.float32)
+    S = 30000
+    index = rng.integers(0, 2, S)
+    u = rng.uniform(-1.1, 1.1, (S, 1)).astype(np.float32)
+    l_t = torch.from_numpy(line).cuda().requires_grad_()
+    u_t = torch.from_numpy(u).cuda().requires_grad_()
+    out = native.IndirectGridSample1D(l_t, torch.from_numpy(index).cuda(), u_t)
+    ref = oracle.grid_sample1d_fwd(line, index, u)
+    assert np.array_equal(out.detach().cpu().numpy().view(np.uint32), ref.view(np.uint32))
+    go = rng.normal(size=(S, 16)).astype(np.float32)
+    gg, gu = torch.autograd.grad(out, [l_t, u_t], torch.from_numpy(go).cuda())
+    rg, ru = oracle.grid_sample1d_bwd(line, index, u, go)
+    print(f"1-D: grad_grid {_rel(gg.cpu().numpy(), rg):.2e}, grad_uv {_rel(gu.cpu().numpy().ravel(), ru):.2e}",
+          flush=True)
+    assert _rel(gg.cpu().double().numpy(), rg) <= 1e-5 and _rel(gu.cpu().double().numpy().ravel(), ru) <= 1e-5
+
+
+def test_gradient_correction_and_bound_weight(oracle, native, cuda):
+    rng = np.random.default_rng(1)
+    d = rng.normal(size=(100, 3))
+    d = (d / np.linalg.norm(d, axis=1, keepdims=True)).astype(np.float32)
+    ray = rng.integers(0, 100, 5000)
+    pos = torch.from_numpy(rng.normal(size=(5000, 3)).astype(np.float32)).cuda().requires_grad_()
+    out = native.GradientCorrection(pos, torch.from_numpy(d).cuda(), torch.from_numpy(ray).cuda())
+    assert torch.equal(out, pos)
+    go = rng.normal(size=(5000, 3)).astype(np.float32)
+    (g,) = torch.autograd.grad(out, pos, torch.from_numpy(go).cuda())
+    assert np.array_equal(g.cpu().numpy(), oracle.gradient_correction(d, ray, go))
+    # bound weight against the reference's scan
+    t = oracle.tree_build(2)
+    tree = native.HyperTreeBase(3, 2)
+    tree.settree()
+    tree.SetActive(2)
+    tree.to("cuda")
+    roi = tree.RoiSplit([-0.5, -0.5, -0.5], [0.5, 0.9, 0.2], 0.0)
+    nodes = np.nonzero(t["depth"] == 2)[0]
+    node_id = rng.choice(nodes, 7000)
+    bw = native.ComputeBoundWeight(torch.from_numpy(node_id).cuda(), roi).cpu().numpy()
+    assert np.array_equal(bw, oracle.compute_bound_weight(node_id, roi.in_roi_node_index.cpu().numpy()))
+
+
+def _vm_setup(native, oracle, op, seed=0):
+    rng = np.random.default_rng(seed)
+    depth = 3
+    t = oracle.tree_build(depth)
+    act = adaptive_active(t, budget=150, seed=seed)
+    tree = device_tree(depth, act)
+    lo, hi = [-0.55, -0.45, -0.6], [0.5, 0.6, 0.4]
+    roi = tree.RoiSplit(lo, hi, 0.0)
+    n_in = int(roi.in_roi_node_index.numel())
+    n_out = int(roi.out_roi_node_index.numel())
+    C_, G, Gl = 8, 9, 17
+    Cg = C_ * (3 if op == "cat" else 1)
+    f = native.VMFields()
+    arr = {}
+    for name, shape in (("plane_yz", (n_in, C_, G, G)), ("plane_xz", (n_in, C_, G, G)),
+                        ("plane_xy", (n_in, C_, G, G)), ("line_x", (1, C_, Gl, 1)), ("line_y", (1, C_, Gl, 1)),
+                        ("line_z", (1, C_, Gl, 1)), ("grid_out", (n_out, Cg, 5, 5, 5))):
+        arr[name] = rng.uniform(-0.5, 0.5, shape).astype(np.float32)
+        setattr(f, name, torch.from_numpy(arr[name]).cuda().requires_grad_())
+    f.tensor_op = op
+    f.roi_min, f.roi_max = lo, hi
+    o, d = random_rays(300, 40 + seed, spread=0.9)
+    tr = oracle.traverse(t["node_min"], t["node_max"], oracle.active_ids(act), o, d, 4)
+    return t, act, tree, roi, f, arr, tr, (lo, hi)
+
+
+def _vm_oracle(oracle, t, act, roi, arr, tr, lohi, op):
+    """The same composition from the oracle's gathers, in the kernels' order."""
+    g = tr["global_coordinate"]
+    nid = tr["node_id"]
+    flag = roi.in_roi_flag.cpu().numpy()[nid].astype(bool)
+    pin, pout = roi.prefix_inroi.cpu().numpy(), roi.prefix_outroi.cpu().numpy()
+    loc = oracle.local_samples(t["node_min"], t["node_max"], g, nid)
+    lo, hi = np.float32(lohi[0][0]), np.float32(lohi[1][0])
+    li = loc[flag]
+    gi = g[flag]
+    slot = pin[nid[flag]]
+    pl = {"plane_yz": li[:, [1, 2]], "plane_xz": li[:, [0, 2]], "plane_xy": li[:, [0, 1]]}
+    ln = {"line_x": np.float32(2.0) * (gi[:, 0] - lo) / (hi - lo) - np.float32(1.0), "line_y": li[:, 1],
+          "line_z": gi[:, 2]}
+    zero = np.zeros(li.shape[0], np.int64)
+    P = {k: oracle.grid_sample2d_fwd(arr[k], slot, v) for k, v in pl.items()}
+    L = {k: oracle.grid_sample1d_fwd(arr[k], zero, v.astype(np.float32)) for k, v in ln.items()}
+    pairs = (("plane_yz", "line_x"), ("plane_xz", "line_y"), ("plane_xy", "line_z"))
+    if op == "add":
+        fin = P["plane_yz"] * L["line_x"]
+        fin = fin + P["plane_xz"] * L["line_y"]
+        fin = fin + P["plane_xy"] * L["line_z"]
+    else:
+        fin = np.concatenate([P[a] for a, _ in pairs], 1) * np.concatenate([L[b] for _, b in pairs], 1)
+    out_slot = pout[nid[~flag]]
+    grid_ncz = arr["grid_out"]
+    fout = oracle.grid_sample3d_fwd(grid_ncz, out_slot, loc[~flag])
+    feats = np.zeros((g.shape[0], fin.shape[1]), np.float32)
+    feats[flag] = fin
+    feats[~flag] = fout
+    return feats, dict(flag=flag, slot=slot, out_slot=out_slot, loc=loc, pl=pl, ln=ln, P=P, L=L, pairs=pairs)
+
+
+@pytest.mark.parametrize("op", ["add", "cat"])
+def test_sample_volume_indirect_feature(oracle, native, cuda, op):
+    t, act, tree, roi, f, arr, tr, lohi = _vm_setup(native, oracle, op)
+    g = torch.from_numpy(tr["global_coordinate"]).cuda()
+    nid = torch.from_numpy(tr["node_id"]).cuda()
+    feats = native.SampleVolumeIndirectFeature(tree, roi, f, g, nid)
+    ref, aux = _vm_oracle(oracle, t, act, roi, arr, tr, lohi, op)
+    flag = aux["flag"]
+    print(f"{op}: {int(flag.sum())} in-ROI / {int((~flag).sum())} out-of-ROI samples", flush=True)
+    assert 0 < flag.sum() < flag.size
+    assert np.abs(feats.detach().cpu().numpy() - ref).max() <= 1e-6 * np.abs(ref).max()
+    # gradients of every field against the FP64 oracle backward of each part
+    rng = np.random.default_rng(5)
+    go = rng.normal(size=ref.shape).astype(np.float32)
+    names = ["plane_yz", "plane_xz", "plane_xy", "line_x", "line_y", "line_z", "grid_out"]
+    grads = torch.autograd.grad(feats, [getattr(f, k) for k in names], torch.from_numpy(go).cuda())
+    got = dict(zip(names, [x.cpu().double().numpy() for x in grads]))
+    gin = go[flag]
+    C_ = arr["line_x"].shape[1]
+    zero = np.zeros(gin.shape[0], np.int64)
+    for i, (pk, lk) in enumerate(aux["pairs"]):
+        gi = gin if op == "add" else gin[:, i * C_:(i + 1) * C_]
+        dP = (gi * aux["L"][lk]).astype(np.float32)
+        dL = (gi * aux["P"][pk]).astype(np.float32)
+        rp, _ = oracle.grid_sample2d_bwd(arr[pk], aux["slot"], aux["pl"][pk], dP)
+        rl, _ = oracle.grid_sample1d_bwd(arr[lk], zero, aux["ln"][lk].astype(np.float32), dL)
+        assert _rel(got[pk], rp) <= 1e-5, pk
+        assert _rel(got[lk], rl) <= 1e-5, lk
+    rgo, _ = oracle.grid_sample3d_bwd(arr["grid_out"], aux["out_slot"], aux["loc"][~flag], go[~flag])
+    assert _rel(got["grid_out"], rgo) <= 1e-5
