@@ -106,6 +106,29 @@ PYBIND11_MODULE(TORCH_EXTENSION_NAME, m)
         .def_readwrite("tensor_op", &VMFields::tensor_op)
         .def_readwrite("roi_min", &VMFields::roi_min)
         .def_readwrite("roi_max", &VMFields::roi_max);
+    py::class_<TreeOptimizerParams>(m, "TreeOptimizerParams")
+        .def(py::init<>())
+        .def_readwrite("max_active_nodes", &TreeOptimizerParams::max_active_nodes)
+        .def_readwrite("error_merge_factor", &TreeOptimizerParams::error_merge_factor)
+        .def_readwrite("error_split_factor", &TreeOptimizerParams::error_split_factor)
+        .def_readwrite("use_saved_errors", &TreeOptimizerParams::use_saved_errors)
+        .def_readwrite("tree_edge_roi_min", &TreeOptimizerParams::tree_edge_roi_min)
+        .def_readwrite("tree_edge_roi_max", &TreeOptimizerParams::tree_edge_roi_max);
+    py::class_<StructureDecision>(m, "StructureDecision")
+        .def_readonly("choice", &StructureDecision::choice)
+        .def_readonly("objective", &StructureDecision::objective)
+        .def_readonly("active_weight", &StructureDecision::active_weight)
+        .def_readonly("num_none", &StructureDecision::num_none)
+        .def_readonly("num_split", &StructureDecision::num_split)
+        .def_readonly("num_grp", &StructureDecision::num_grp)
+        .def_readonly("feasible", &StructureDecision::feasible);
+    m.def("SolveTreeStructure", &SolveTreeStructure);
+    m.def("OptimizeTreeStructure", [](HyperTreeBase& tree, const TreeOptimizerParams& p) {
+        StructureDecision d;
+        bool changed = OptimizeTreeStructure(tree, p, &d);
+        return std::make_pair(changed, d);
+    });
+    m.def("NodeCulling", &NodeCulling);
     m.def("IndirectGridSample3D", &IndirectGridSample3D);
     m.def("IndirectGridSample2D", &IndirectGridSample2D);
     m.def("IndirectGridSample1D", &IndirectGridSample1D);

@@ -122,6 +122,30 @@ class HyperTreeBase {
     torch::Tensor leaf_table_;
 };
 
+// ---- octree structure optimizer (structure/HyperTreeStructureOptimizer.{h,cpp}) ----
+struct TreeOptimizerParams {           // HyperTreeStructureOptimizer.h:6-30
+    int max_active_nodes = 512;
+    double error_merge_factor = 1.1;
+    double error_split_factor = 0.75;
+    bool use_saved_errors = true;
+    std::vector<float> tree_edge_roi_min{-1.f, -1.f, -1.f}, tree_edge_roi_max{1.f, 1.f, 1.f};
+};
+struct StructureDecision {
+    std::vector<int> choice;           // per node: -1 not active, 0 none, 1 split, 2 merge
+    double objective = 0;
+    int64_t active_weight = 0;         // budget used
+    int num_none = 0, num_split = 0, num_grp = 0;
+    bool feasible = false;
+};
+// Exact optimum of the reference's structure MIP (multiple-choice knapsack DP).
+StructureDecision SolveTreeStructure(const HyperTreeBase& tree, const TreeOptimizerParams& params);
+// Solve, apply the split / merge decisions (ApplySplit) and UpdateActive;
+// returns whether the active set changed (OptimizeTree).
+bool OptimizeTreeStructure(HyperTreeBase& tree, const TreeOptimizerParams& params, StructureDecision* out = nullptr);
+// NodeCulling (samples/hyper_train.cpp:761-830).
+void NodeCulling(HyperTreeBase& tree, float culling_threshold, std::vector<float> roi_min, std::vector<float> roi_max,
+                 float epsilon);
+
 // IndirectGridSample3D(multi_grid, index, uv): grid [N, C, Z, Y, X]
 torch::Tensor IndirectGridSample3D(torch::Tensor multi_grid, torch::Tensor index, torch::Tensor uv);
 
