@@ -111,6 +111,8 @@ _SIGNATURES = {
     "atdf_proposal_write": (_I32, [_I64, _P, _P, _P, _P, _P, _P, C.POINTER(Intervals), _P]),
     "atdf_proposal_grad_workspace_bytes": (_SZ, [_I64]),
     "atdf_proposal_grad": (_I32, [_P, _P, _P, _I64, _F, _P, _P, _P, _SZ, _P]),
+    "atdf_node_id_for_position": (_I32, [_P, _I64, _P, _P, _I64, _P, _P, _P]),
+    "atdf_grid_transfer": (_I32, [_P, _I64, _P, _P, _I64, _P, _I64, _P, _P, _I64, _I32, _I32, _I32, _P]),
     "atdf_grid_layout": (_I32, [_P, _P, _I64, _I32, _I32, _I32, _P]),
     "atdf_f32_to_bf16": (_I32, [_P, _P, _I64, _P]),
 }

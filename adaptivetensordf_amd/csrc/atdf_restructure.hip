@@ -1,0 +1,189 @@
+// atdf_restructure.hip -- point location in the active set and the transfer
+// of node grids across a change of the active set.
+//
+// Point location follows ActiveNodeIdForGlobalPosition
+// (structure/HyperTree.cu:1339-1399): from the root, while the node is not
+// active, step into the first child (in child order) whose closed box holds
+// the point; no such child, or a leaf, means "not in an active node".  The
+// literal box tests are kept (instead of a mid-plane rule) so the result
+// does not depend on how the child boxes were rounded.
+//
+// The grid transfer is the nearest-lattice copy of
+// InterpolateGridForInactiveNodes (structure/HyperTree.cu:1527-1616): every
+// lattice point (x, y, z) / (G - 1) of a destination node is mapped to the
+// global position, located among the source active nodes, and takes the
+// whole channel record of the nearest lattice point of that node
+// (round half away from zero).  A destination node that is active in the
+// source set keeps its own grid, as the reference's kernel leaves active
+// nodes untouched.  Positions and local coordinates are
+// evaluated in the reference's order without FMA contraction, so the chosen
+// lattice points are bit-identical to a C restatement.
+#include <hip/hip_runtime.h>
+
+#include "atdf_device.h"
+#include "atdf_internal.h"
+
+namespace atdf {
+
+// Active node holding p, or -1.
+__device__ __forceinline__ int32_t active_node_for_position(const PNode* __restrict__ nodes, const float* p)
+{
+    int32_t node = 0;
+    while (true) {
+        const PNode nd = nodes[node];
+        if (nd.flags & kFlagActive) return node;
+        if (nd.first_child < 0) return -1;
+        int32_t next = -1;
+        for (int c = 0; c < 8; ++c) {
+            const PNode ch = nodes[nd.first_child + c];
+            if (ch.mn[0] <= p[0] && ch.mx[0] >= p[0] && ch.mn[1] <= p[1] && ch.mx[1] >= p[1] && ch.mn[2] <= p[2] &&
+                ch.mx[2] >= p[2]) {
+                next = nd.first_child + c;
+                break;
+            }
+        }
+        if (next < 0) return -1;
+        node = next;
+    }
+}
+
+// ComputeNodeId (structure/HyperTree.cu:1466-1485): a position outside every
+// active node gets the first active node and mask 0.
+__global__ __launch_bounds__(256) void node_id_kernel(const PNode* __restrict__ nodes,
+                                                      const int64_t* __restrict__ active_ids,
+                                                      const float* __restrict__ pos, int64_t n,
+                                                      int64_t* __restrict__ node_id, float* __restrict__ mask)
+{
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float p[3] = {pos[i * 3], pos[i * 3 + 1], pos[i * 3 + 2]};
+    const int32_t nd = active_node_for_position(nodes, p);
+    node_id[i] = nd < 0 ? active_ids[0] : (int64_t)nd;
+    mask[i] = nd < 0 ? 0.f : 1.f;
+}
+
+struct TransferArgs {
+    const PNode* src_nodes;      // source active set (boxes shared with the destination)
+    const int32_t* src_slot;     // [num_nodes] node -> source grid slot, null = identity
+    const float* src_grid;
+    int64_t src_slots;
+    const int64_t* dst_nodes;    // [n_dst]
+    const int32_t* dst_slot;     // [num_nodes] node -> destination slot, null = identity
+    float* dst_grid;
+    int64_t dst_slots;
+    int64_t num_nodes;
+    int C, G;
+    int layout;                  // 0: [slots, G, G, G, C]; 1: [slots, C, G, G, G]
+};
+
+__device__ __forceinline__ int64_t grid_offset(int layout, int64_t slot, int C, int G, int x, int y, int z, int c)
+{
+    const int64_t g3 = (int64_t)G * G * G;
+    const int64_t cell = ((int64_t)z * G + y) * G + x;
+    return layout == 0 ? (slot * g3 + cell) * C + c : (slot * C + c) * g3 + cell;
+}
+
+// One workgroup per destination node, one lane per lattice point.
+__global__ __launch_bounds__(256) void grid_transfer_kernel(TransferArgs a)
+{
+    const int64_t node = a.dst_nodes[blockIdx.x];
+    if (node < 0 || node >= a.num_nodes) return;
+    const int64_t dslot = a.dst_slot ? (int64_t)a.dst_slot[node] : node;
+    if (dslot < 0 || dslot >= a.dst_slots) return;
+    const PNode box = a.src_nodes[node];
+    const int G = a.G;
+    if (box.flags & kFlagActive) {
+        // active in the source set: the node keeps its own grid
+        const int64_t sslot = a.src_slot ? (int64_t)a.src_slot[node] : node;
+        if (sslot < 0 || sslot >= a.src_slots) return;
+        const int64_t rec = (int64_t)G * G * G * a.C;
+        for (int64_t i = threadIdx.x; i < rec; i += blockDim.x)
+            a.dst_grid[dslot * rec + i] = a.src_grid[sslot * rec + i];
+        return;
+    }
+    const float gm1 = (float)(G - 1);
+    float bsize[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) bsize[k] = box.mx[k] - box.mn[k];
+    const int cells = G * G * G;
+    for (int cell = threadIdx.x; cell < cells; cell += blockDim.x) {
+        const int res[3] = {cell % G, (cell / G) % G, cell / (G * G)};
+        float g[3];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            float local = (float)res[k] / gm1;
+            float c = local * bsize[k];
+            g[k] = c + box.mn[k];
+        }
+        const int32_t target = active_node_for_position(a.src_nodes, g);
+        if (target < 0) continue;  // no source node here: the destination keeps its values
+        const int64_t sslot = a.src_slot ? (int64_t)a.src_slot[target] : (int64_t)target;
+        if (sslot < 0 || sslot >= a.src_slots) continue;
+        const PNode tb = a.src_nodes[target];
+        int fi[3];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            float tsize = tb.mx[k] - tb.mn[k];
+            float c = g[k] - tb.mn[k];
+            c = c / tsize;
+            float f = c * gm1;
+            int r = (int)roundf(f);
+            fi[k] = r < 0 ? 0 : (r > G - 1 ? G - 1 : r);
+        }
+        for (int ch = 0; ch < a.C; ++ch)
+            a.dst_grid[grid_offset(a.layout, dslot, a.C, G, res[0], res[1], res[2], ch)] =
+                a.src_grid[grid_offset(a.layout, sslot, a.C, G, fi[0], fi[1], fi[2], ch)];
+    }
+}
+
+}  // namespace atdf
+
+using namespace atdf;
+
+extern "C" {
+
+atdf_status atdf_node_id_for_position(const void* packed_tree, int64_t num_nodes, const int64_t* active_node_ids,
+                                      const float* positions, int64_t num_positions, int64_t* node_id, float* mask,
+                                      atdf_stream_t stream)
+{
+    ATDF_REQUIRE(packed_tree && num_nodes > 0, "atdf_node_id_for_position: null tree");
+    ATDF_REQUIRE(num_positions >= 0, "atdf_node_id_for_position: negative count");
+    if (num_positions == 0) return ATDF_OK;
+    ATDF_REQUIRE(active_node_ids && positions && node_id && mask, "atdf_node_id_for_position: null buffer");
+    hipLaunchKernelGGL(node_id_kernel, dim3((unsigned)div_up(num_positions, 256)), dim3(256), 0, to_hip(stream),
+                       reinterpret_cast<const PNode*>(packed_tree), active_node_ids, positions, num_positions, node_id,
+                       mask);
+    return check_launch("atdf_node_id_for_position");
+}
+
+atdf_status atdf_grid_transfer(const void* src_packed_tree, int64_t num_nodes, const int32_t* src_slot,
+                               const float* src_grid, int64_t src_slots, const int64_t* dst_nodes, int64_t num_dst,
+                               const int32_t* dst_slot, float* dst_grid, int64_t dst_slots, int32_t C, int32_t G,
+                               int32_t layout, atdf_stream_t stream)
+{
+    ATDF_REQUIRE(src_packed_tree && num_nodes > 0, "atdf_grid_transfer: null tree");
+    ATDF_REQUIRE(C >= 1 && G >= 2 && (layout == 0 || layout == 1), "atdf_grid_transfer: bad C %d, G %d or layout %d",
+                 C, G, layout);
+    ATDF_REQUIRE(num_dst >= 0 && src_slots >= 0 && dst_slots >= 0, "atdf_grid_transfer: negative size");
+    ATDF_REQUIRE(num_dst < (int64_t)INT32_MAX, "atdf_grid_transfer: too many destination nodes");
+    if (num_dst == 0 || dst_slots == 0 || src_slots == 0) return ATDF_OK;
+    ATDF_REQUIRE(src_grid && dst_grid && dst_nodes, "atdf_grid_transfer: null buffer");
+    ATDF_REQUIRE(src_grid != dst_grid, "atdf_grid_transfer: source and destination grids alias");
+    TransferArgs a;
+    a.src_nodes = reinterpret_cast<const PNode*>(src_packed_tree);
+    a.src_slot = src_slot;
+    a.src_grid = src_grid;
+    a.src_slots = src_slots;
+    a.dst_nodes = dst_nodes;
+    a.dst_slot = dst_slot;
+    a.dst_grid = dst_grid;
+    a.dst_slots = dst_slots;
+    a.num_nodes = num_nodes;
+    a.C = C;
+    a.G = G;
+    a.layout = layout;
+    hipLaunchKernelGGL(grid_transfer_kernel, dim3((unsigned)num_dst), dim3(256), 0, to_hip(stream), a);
+    return check_launch("atdf_grid_transfer");
+}
+
+}  // extern "C"

@@ -74,6 +74,9 @@ PYBIND11_MODULE(TORCH_EXTENSION_NAME, m)
         .def("UpdateCulling", &HyperTreeBase::UpdateCulling)
         .def("LeafTable", &HyperTreeBase::LeafTable)
         .def("RoiSplit", &HyperTreeBase::RoiSplit)
+        .def("NodeIdForPositionGPU", &HyperTreeBase::NodeIdForPositionGPU)
+        .def("InterpolateGridForInactiveNodes", &HyperTreeBase::InterpolateGridForInactiveNodes)
+        .def("PackedTree", &HyperTreeBase::PackedTree)
         .def_readwrite("node_parent", &HyperTreeBase::node_parent)
         .def_readwrite("node_children", &HyperTreeBase::node_children)
         .def_readwrite("node_position_min", &HyperTreeBase::node_position_min)
@@ -129,6 +132,19 @@ PYBIND11_MODULE(TORCH_EXTENSION_NAME, m)
         return std::make_pair(changed, d);
     });
     m.def("NodeCulling", &NodeCulling);
+    m.def(
+        "TransferNodeGrids",
+        [](const HyperTreeBase& tree, torch::Tensor old_packed, torch::Tensor old_node_to_grid, torch::Tensor old_grid,
+           std::optional<torch::Tensor> init) {
+            return TransferNodeGrids(tree, old_packed, old_node_to_grid, old_grid, init ? *init : torch::Tensor());
+        },
+        py::arg("tree"), py::arg("old_packed"), py::arg("old_node_to_grid"), py::arg("old_grid"),
+        py::arg("init") = py::none());
+    m.def("RestructureTrainState", [](HyperTreeBase& tree, TrainState& st, const TreeOptimizerParams& p) {
+        StructureDecision d;
+        bool changed = RestructureTrainState(tree, st, p, &d);
+        return std::make_pair(changed, d);
+    });
     m.def("IndirectGridSample3D", &IndirectGridSample3D);
     m.def("IndirectGridSample2D", &IndirectGridSample2D);
     m.def("IndirectGridSample1D", &IndirectGridSample1D);
@@ -147,6 +163,10 @@ PYBIND11_MODULE(TORCH_EXTENSION_NAME, m)
         .def_readwrite("mlp", &TrainState::mlp)
         .def_readwrite("grad_grid", &TrainState::grad_grid)
         .def_readwrite("grad_mlp", &TrainState::grad_mlp)
+        .def_readwrite("m_grid", &TrainState::m_grid)
+        .def_readwrite("v_grid", &TrainState::v_grid)
+        .def_readwrite("m_mlp", &TrainState::m_mlp)
+        .def_readwrite("v_mlp", &TrainState::v_mlp)
         .def_readwrite("loss", &TrainState::loss)
         .def_readwrite("step", &TrainState::step)
         .def_readwrite("lr_grid", &TrainState::lr_grid)

@@ -103,6 +103,16 @@ class HyperTreeBase {
     void UpdateCulling();
     // in_roi test of geometry/geometry.h:426-449 over the active nodes
     RoiMaps RoiSplit(std::vector<float> roi_min, std::vector<float> roi_max, float epsilon) const;
+    // NodeIdForPositionGPU (HyperTree.cu:1487-1525): (node_id [P] i64, mask [P] f32)
+    // for positions [..., 3]; outside every active node: first active node, 0
+    std::pair<torch::Tensor, torch::Tensor> NodeIdForPositionGPU(torch::Tensor global_samples) const;
+    // InterpolateGridForInactiveNodes (HyperTree.cu:1603-1616): a copy of
+    // active_grid [num_nodes, C, G, G, G] whose inactive nodes take the
+    // nearest lattice values of the active nodes
+    torch::Tensor InterpolateGridForInactiveNodes(torch::Tensor active_grid) const;
+    // packed traversal form of the current active set (replaced, not
+    // modified, on every change: holding it keeps a snapshot)
+    torch::Tensor PackedTree() const { return packed_; }
 
     const void* packed() const { return packed_.data_ptr(); }
     // point-location table of the active set (rebuilt with the packed tree)
@@ -145,6 +155,14 @@ bool OptimizeTreeStructure(HyperTreeBase& tree, const TreeOptimizerParams& param
 // NodeCulling (samples/hyper_train.cpp:761-830).
 void NodeCulling(HyperTreeBase& tree, float culling_threshold, std::vector<float> roi_min, std::vector<float> roi_max,
                  float epsilon);
+// Grids [N_active, G, G, G, C] f32 of the tree's current active set from the
+// grids of an earlier active set (old_packed = PackedTree() and old_node_to_grid
+// = node_to_grid taken before the change): nodes active in both keep their
+// grid, new nodes take the nearest lattice values of the old active nodes
+// (InterpolateGridForInactiveNodes' rule).  Lattice points of a new node
+// that no old active node covers keep init's values (zeros if undefined).
+torch::Tensor TransferNodeGrids(const HyperTreeBase& tree, torch::Tensor old_packed, torch::Tensor old_node_to_grid,
+                                torch::Tensor old_grid, torch::Tensor init = {});
 
 // IndirectGridSample3D(multi_grid, index, uv): grid [N, C, Z, Y, X]
 torch::Tensor IndirectGridSample3D(torch::Tensor multi_grid, torch::Tensor index, torch::Tensor uv);
@@ -223,6 +241,13 @@ struct TrainState {
     float lr_grid = 1e-3f, lr_mlp = 1e-3f;
 };
 TrainState MakeTrainState(torch::Tensor grid, torch::Tensor mlp, bool bf16_grid, float lr_grid, float lr_mlp);
+// Structure step of training (samples/hyper_train.cpp:1887-1910): optimize
+// the tree structure; if the active set changed, move the grids to the new
+// active set (TransferNodeGrids, through the f32 master for bf16 grids) and
+// reset the Adam state of grids and decoder (ResetGeometryOptimizer,
+// geometry/geometry.h:183-195).  Returns whether the structure changed.
+bool RestructureTrainState(HyperTreeBase& tree, TrainState& st, const TreeOptimizerParams& p,
+                           StructureDecision* out = nullptr);
 torch::Tensor TrainStep(HyperTreeBase& tree, TrainState& st, const RayList& rays, torch::Tensor target,
                         const SamplingParams& sp, const FieldParams& fp);
 // The same step over a given interval list (e.g. the learned proposal's).

@@ -337,4 +337,98 @@ void NodeCulling(HyperTreeBase& tree, float culling_threshold, std::vector<float
     tree.UpdateCulling();
 }
 
+std::pair<torch::Tensor, torch::Tensor> HyperTreeBase::NodeIdForPositionGPU(torch::Tensor global_samples) const
+{
+    TORCH_CHECK(global_samples.is_cuda() && node_position_min.is_cuda(), "NodeIdForPositionGPU: GPU tensors required");
+    TORCH_CHECK(global_samples.size(-1) == 3, "NodeIdForPositionGPU: positions must end in 3");
+    TORCH_CHECK(NumActiveNodes() > 0, "NodeIdForPositionGPU: no active node");
+    auto pos = global_samples.reshape({-1, 3}).to(torch::kFloat32).contiguous();
+    const int64_t n = pos.size(0);
+    auto node_id = torch::zeros({n}, pos.options().dtype(torch::kLong));
+    auto mask = torch::zeros({n}, pos.options());
+    check_status(atdf_node_id_for_position(packed(), NumNodes(), active_node_ids.data_ptr<int64_t>(),
+                                           pos.data_ptr<float>(), n, node_id.data_ptr<int64_t>(),
+                                           mask.data_ptr<float>(), current_stream()),
+                 "atdf_node_id_for_position");
+    return {node_id, mask};
+}
+
+torch::Tensor HyperTreeBase::InterpolateGridForInactiveNodes(torch::Tensor active_grid) const
+{
+    TORCH_CHECK(active_grid.is_cuda() && active_grid.scalar_type() == torch::kFloat32 && active_grid.dim() == 5,
+                "InterpolateGridForInactiveNodes: float32 GPU grid [num_nodes, C, G, G, G] required");
+    TORCH_CHECK(active_grid.size(0) == NumNodes(), "InterpolateGridForInactiveNodes: one grid per node required");
+    const int64_t G = active_grid.size(2);
+    TORCH_CHECK(active_grid.size(3) == G && active_grid.size(4) == G && G >= 2,
+                "InterpolateGridForInactiveNodes: cubic grids of at least 2^3 required");
+    auto src = active_grid.contiguous();
+    auto out = src.clone();
+    auto inactive = torch::nonzero(node_active == 0).reshape({-1}).contiguous();
+    check_status(atdf_grid_transfer(packed(), NumNodes(), nullptr, src.data_ptr<float>(), NumNodes(),
+                                    inactive.data_ptr<int64_t>(), inactive.numel(), nullptr, out.data_ptr<float>(),
+                                    NumNodes(), (int32_t)src.size(1), (int32_t)G, 1, current_stream()),
+                 "atdf_grid_transfer");
+    return out;
+}
+
+torch::Tensor TransferNodeGrids(const HyperTreeBase& tree, torch::Tensor old_packed, torch::Tensor old_node_to_grid,
+                                torch::Tensor old_grid, torch::Tensor init)
+{
+    TORCH_CHECK(old_grid.is_cuda() && old_grid.scalar_type() == torch::kFloat32 && old_grid.dim() == 5 &&
+                    old_grid.is_contiguous(),
+                "TransferNodeGrids: contiguous float32 GPU grid [N, G, G, G, C] required");
+    TORCH_CHECK(old_packed.defined() && old_packed.is_cuda() &&
+                    old_packed.numel() == (int64_t)atdf_tree_packed_bytes(tree.NumNodes()),
+                "TransferNodeGrids: old_packed must be a PackedTree() of this tree");
+    TORCH_CHECK(old_node_to_grid.scalar_type() == torch::kInt32 && old_node_to_grid.numel() == tree.NumNodes() &&
+                    old_node_to_grid.is_cuda(),
+                "TransferNodeGrids: old_node_to_grid must be the tree's int32 node_to_grid before the change");
+    const int64_t G = old_grid.size(1), C = old_grid.size(4);
+    TORCH_CHECK(old_grid.size(2) == G && old_grid.size(3) == G && G >= 2, "TransferNodeGrids: cubic grids required");
+    const int64_t na = tree.NumActiveNodes();
+    torch::Tensor out;
+    if (init.defined()) {
+        TORCH_CHECK(init.sizes() == torch::IntArrayRef({na, G, G, G, C}) && init.scalar_type() == torch::kFloat32,
+                    "TransferNodeGrids: init must be float32 [N_active, G, G, G, C]");
+        out = init.to(old_grid.device()).contiguous().clone();
+    } else {
+        out = torch::zeros({na, G, G, G, C}, old_grid.options());
+    }
+    auto ids = tree.active_node_ids.contiguous();
+    auto old_slot = old_node_to_grid.contiguous();
+    check_status(atdf_grid_transfer(old_packed.data_ptr(), tree.NumNodes(), old_slot.data_ptr<int32_t>(),
+                                    old_grid.data_ptr<float>(), old_grid.size(0), ids.data_ptr<int64_t>(), na,
+                                    tree.node_to_grid.data_ptr<int32_t>(), out.data_ptr<float>(), na, (int32_t)C,
+                                    (int32_t)G, 0, current_stream()),
+                 "atdf_grid_transfer");
+    return out;
+}
+
+bool RestructureTrainState(HyperTreeBase& tree, TrainState& st, const TreeOptimizerParams& p, StructureDecision* out)
+{
+    auto old_packed = tree.PackedTree();
+    auto old_slot = tree.node_to_grid;
+    const bool changed = OptimizeTreeStructure(tree, p, out);
+    if (!changed) return false;
+    const bool bf16 = st.grid_master.defined();
+    auto moved = TransferNodeGrids(tree, old_packed, old_slot, bf16 ? st.grid_master : st.grid);
+    if (bf16) {
+        st.grid_master = moved;
+        st.grid = torch::empty(moved.sizes(), moved.options().dtype(torch::kBFloat16));
+        check_status(atdf_f32_to_bf16(moved.data_ptr<float>(), reinterpret_cast<uint16_t*>(st.grid.data_ptr()),
+                                      moved.numel(), current_stream()),
+                     "atdf_f32_to_bf16");
+    } else {
+        st.grid = moved;
+    }
+    st.grad_grid = torch::zeros(moved.sizes(), moved.options());
+    st.m_grid = torch::zeros(moved.sizes(), moved.options());
+    st.v_grid = torch::zeros(moved.sizes(), moved.options());
+    st.m_mlp.zero_();
+    st.v_mlp.zero_();
+    st.grad_mlp.zero_();
+    st.step = 0;
+    return true;
+}
+
 }  // namespace atdf_torch

@@ -122,6 +122,39 @@ atdf_status atdf_traverse_write(const void* packed_tree, int64_t num_nodes, cons
                                 atdf_stream_t stream);
 
 /* ======================================================================= */
+/* Point location and grid transfer across a change of the active set      */
+/* ======================================================================= */
+/* HyperTreeBaseImpl::NodeIdForPositionGPU (structure/HyperTree.cu:1466-1525,
+ * descent of ActiveNodeIdForGlobalPosition, 1339-1399): node_id[i] = the
+ * active node reached from the root by stepping into the first child whose
+ * closed box holds positions[i]; mask[i] = 1.  Where the descent ends
+ * without one (a culled region, outside the root), node_id[i] =
+ * active_node_ids[0] and mask[i] = 0.  positions [n, 3] f32. */
+atdf_status atdf_node_id_for_position(const void* packed_tree, int64_t num_nodes, const int64_t* active_node_ids,
+                                      const float* positions, int64_t num_positions, int64_t* node_id, float* mask,
+                                      atdf_stream_t stream);
+
+/* Nearest-lattice transfer of node grids, the copy rule of
+ * HyperTreeBaseImpl::InterpolateGridForInactiveNodes
+ * (structure/HyperTree.cu:1527-1616): for each destination node and each
+ * lattice point (x, y, z) of its G^3 grid, the position
+ * min + (x, y, z) / (G - 1) * (max - min) is located among the active nodes
+ * of src_packed_tree (as atdf_node_id_for_position) and the destination
+ * record takes all C channels of the nearest lattice point of that node.
+ * A destination node that is active in the source set keeps its own grid
+ * (the reference leaves active nodes untouched).  Lattice points outside
+ * every source active node leave the destination unchanged (the reference
+ * reads node -1 there).  Source and destination
+ * share the tree topology; only the active sets differ.  src_slot /
+ * dst_slot map node ids to grid slots (null: slot = node id).  layout 0:
+ * [slots, G, G, G, C] (the fused path's channels-last grids); layout 1:
+ * [slots, C, G, G, G] (the reference's [num_nodes, C, z, y, x]). */
+atdf_status atdf_grid_transfer(const void* src_packed_tree, int64_t num_nodes, const int32_t* src_slot,
+                               const float* src_grid, int64_t src_slots, const int64_t* dst_nodes, int64_t num_dst,
+                               const int32_t* dst_slot, float* dst_grid, int64_t dst_slots, int32_t C, int32_t G,
+                               int32_t layout, atdf_stream_t stream);
+
+/* ======================================================================= */
 /* Per-sample ops (the reference's operator API)                            */
 /* ======================================================================= */
 /* HyperTreeBaseImpl::ComputeLocalSamples (structure/HyperTree.cu:237-259):

@@ -247,6 +247,23 @@ void atdfo_prop_sample(const atdfo_proposal* p, const float* o, const float* d, 
 double atdfo_prop_grad(const float* params, const float* feat, const float* bins, int64_t R, float power,
                        double* grad);
 
+/* Point location (structure/HyperTree.cu:1339-1399): the active node reached
+ * from the root through the first child whose closed box holds p, or -1. */
+int32_t atdfo_active_node_for_position(const float* node_min, const float* node_max, const int32_t* children,
+                                       const int32_t* active, const float* p);
+/* NodeIdForPositionGPU: -1 becomes first_active with mask 0. */
+void atdfo_node_id_for_position(const float* node_min, const float* node_max, const int32_t* children,
+                                const int32_t* active, int64_t first_active, const float* pos, int64_t n,
+                                int64_t* node_id, float* mask);
+/* Nearest-lattice grid transfer (InterpolateGridForInactiveNodes copy rule)
+ * into the dst_nodes' grids (a node active in the source set keeps its own
+ * grid); slots null = identity; layout 0
+ * [slots, G, G, G, C], layout 1 [slots, C, G, G, G]. */
+void atdfo_grid_transfer(const float* node_min, const float* node_max, const int32_t* children,
+                         const int32_t* src_active, const int32_t* src_slot, const float* src_grid,
+                         const int64_t* dst_nodes, int64_t n_dst, const int32_t* dst_slot, float* dst_grid, int C,
+                         int G, int layout);
+
 #ifdef __cplusplus
 }
 #endif

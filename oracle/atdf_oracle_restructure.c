@@ -1,0 +1,105 @@
+/*
+ * atdf_oracle_restructure.c -- CPU oracle of point location in the active
+ * set and of the nearest-lattice grid transfer.
+ *
+ * TEST INFRASTRUCTURE ONLY (see atdf_oracle.h).  Follows
+ *   ActiveNodeIdForGlobalPosition  structure/HyperTree.cu:1339-1399
+ *   ComputeNodeId / NodeIdForPositionGPU  structure/HyperTree.cu:1466-1525
+ *   InterpolateGridForInactiveNodes  structure/HyperTree.cu:1527-1616
+ * Lattice positions and target coordinates are evaluated in the reference's
+ * order with no FMA contraction (bit-exact with the kernels).  Where the
+ * reference would read node -1 (a lattice point outside every active node)
+ * the destination is left unchanged.  A destination node active in the
+ * source set keeps its own grid (the reference skips active nodes).
+ */
+#include <math.h>
+#include <stdint.h>
+
+#include "atdf_oracle.h"
+
+int32_t atdfo_active_node_for_position(const float* node_min, const float* node_max, const int32_t* children,
+                                       const int32_t* active, const float* p)
+{
+    int32_t node = 0;
+    for (;;) {
+        if (active[node] > 0) return node;
+        int32_t next = -1;
+        for (int c = 0; c < 8; ++c) {
+            int32_t ch = children[node * 8 + c];
+            if (ch < 0) return -1;
+            const float* a = node_min + (int64_t)ch * 3;
+            const float* b = node_max + (int64_t)ch * 3;
+            if (a[0] <= p[0] && b[0] >= p[0] && a[1] <= p[1] && b[1] >= p[1] && a[2] <= p[2] && b[2] >= p[2]) {
+                next = ch;
+                break;
+            }
+        }
+        if (next < 0) return -1;
+        node = next;
+    }
+}
+
+void atdfo_node_id_for_position(const float* node_min, const float* node_max, const int32_t* children,
+                                const int32_t* active, int64_t first_active, const float* pos, int64_t n,
+                                int64_t* node_id, float* mask)
+{
+    for (int64_t i = 0; i < n; ++i) {
+        int32_t nd = atdfo_active_node_for_position(node_min, node_max, children, active, pos + i * 3);
+        node_id[i] = nd < 0 ? first_active : nd;
+        mask[i] = nd < 0 ? 0.f : 1.f;
+    }
+}
+
+static int64_t offset(int layout, int64_t slot, int C, int G, int x, int y, int z, int c)
+{
+    int64_t g3 = (int64_t)G * G * G;
+    int64_t cell = ((int64_t)z * G + y) * G + x;
+    return layout == 0 ? (slot * g3 + cell) * C + c : (slot * C + c) * g3 + cell;
+}
+
+void atdfo_grid_transfer(const float* node_min, const float* node_max, const int32_t* children,
+                         const int32_t* src_active, const int32_t* src_slot, const float* src_grid,
+                         const int64_t* dst_nodes, int64_t n_dst, const int32_t* dst_slot, float* dst_grid, int C,
+                         int G, int layout)
+{
+    const float gm1 = (float)(G - 1);
+    for (int64_t j = 0; j < n_dst; ++j) {
+        int64_t node = dst_nodes[j];
+        int64_t ds = dst_slot ? dst_slot[node] : node;
+        const float* mn = node_min + node * 3;
+        const float* mx = node_max + node * 3;
+        if (src_active[node] > 0) { /* stays active: its own grid */
+            int64_t ss = src_slot ? src_slot[node] : node;
+            int64_t rec = (int64_t)G * G * G * C;
+            for (int64_t i = 0; i < rec; ++i) dst_grid[ds * rec + i] = src_grid[ss * rec + i];
+            continue;
+        }
+        for (int z = 0; z < G; ++z)
+            for (int y = 0; y < G; ++y)
+                for (int x = 0; x < G; ++x) {
+                    int res[3] = {x, y, z};
+                    float g[3];
+                    for (int k = 0; k < 3; ++k) {
+                        float size = mx[k] - mn[k];
+                        float local = (float)res[k] / gm1;
+                        float c = local * size;
+                        g[k] = c + mn[k];
+                    }
+                    int32_t t = atdfo_active_node_for_position(node_min, node_max, children, src_active, g);
+                    if (t < 0) continue;
+                    int64_t ss = src_slot ? src_slot[t] : t;
+                    int fi[3];
+                    for (int k = 0; k < 3; ++k) {
+                        float tsize = node_max[t * 3 + k] - node_min[t * 3 + k];
+                        float c = g[k] - node_min[t * 3 + k];
+                        c = c / tsize;
+                        float f = c * gm1;
+                        int r = (int)roundf(f);
+                        fi[k] = r < 0 ? 0 : (r > G - 1 ? G - 1 : r);
+                    }
+                    for (int ch = 0; ch < C; ++ch)
+                        dst_grid[offset(layout, ds, C, G, x, y, z, ch)] =
+                            src_grid[offset(layout, ss, C, G, fi[0], fi[1], fi[2], ch)];
+                }
+    }
+}

@@ -447,3 +447,44 @@ def prop_grad(params, feat, bins, power=1.0):
     g = np.zeros(params.shape[0], np.float64)
     loss = lib().atdfo_prop_grad(_p(params), _p(feat), _p(bins), C.c_int64(feat.shape[0]), C.c_float(power), _p(g))
     return loss, g
+
+
+# ------------------------------------------------------- point location / grid transfer
+def node_for_position(node_min, node_max, children, active, p):
+    return int(lib().atdfo_active_node_for_position(_p(_c(node_min, np.float32)), _p(_c(node_max, np.float32)),
+                                                    _p(_c(children, np.int32)), _p(_c(active, np.int32)),
+                                                    _p(_c(p, np.float32))))
+
+
+def node_id_for_position(node_min, node_max, children, active, pos):
+    """(node_id i64, mask f32) of NodeIdForPositionGPU."""
+    active = _c(active, np.int32)
+    pos = _c(pos, np.float32).reshape(-1, 3)
+    n = pos.shape[0]
+    node_id = np.zeros(n, np.int64)
+    mask = np.zeros(n, np.float32)
+    first = int(np.nonzero(active)[0][0])
+    lib().atdfo_node_id_for_position(_p(_c(node_min, np.float32)), _p(_c(node_max, np.float32)),
+                                     _p(_c(children, np.int32)), _p(active), C.c_int64(first), _p(pos),
+                                     C.c_int64(n), _p(node_id), _p(mask))
+    return node_id, mask
+
+
+def grid_transfer(node_min, node_max, children, src_active, src_slot, src_grid, dst_nodes, dst_slot, dst_grid,
+                  layout):
+    """Writes into dst_grid (a contiguous f32 array) and returns it.  Slots
+    None = identity.  layout 0: [slots, G, G, G, C]; 1: [slots, C, G, G, G]."""
+    src_grid = _c(src_grid, np.float32)
+    assert dst_grid.dtype == np.float32 and dst_grid.flags.c_contiguous
+    if layout == 0:
+        G, Cc = src_grid.shape[1], src_grid.shape[4]
+    else:
+        Cc, G = src_grid.shape[1], src_grid.shape[2]
+    dst_nodes = _c(dst_nodes, np.int64)
+    ss = None if src_slot is None else _c(src_slot, np.int32)
+    ds = None if dst_slot is None else _c(dst_slot, np.int32)
+    lib().atdfo_grid_transfer(_p(_c(node_min, np.float32)), _p(_c(node_max, np.float32)), _p(_c(children, np.int32)),
+                              _p(_c(src_active, np.int32)), None if ss is None else _p(ss), _p(src_grid),
+                              _p(dst_nodes), C.c_int64(dst_nodes.shape[0]), None if ds is None else _p(ds),
+                              _p(dst_grid), C.c_int(Cc), C.c_int(G), C.c_int(layout))
+    return dst_grid

@@ -68,6 +68,23 @@ def test_proposal_boundary_without_device_work(oracle):
     assert lib.atdf_proposal_grad_workspace_bytes(64) > 64 * 455 * 4
 
 
+def test_restructure_boundary_without_device_work():
+    import ctypes as C
+
+    from adaptivetensordf_amd import capi
+
+    lib = capi.lib()
+    one = C.c_void_p(1)
+    st = lib.atdf_grid_transfer(one, 9, None, one, 9, one, 3, None, C.c_void_p(2), 9, 8, 1, 0, None)
+    assert st == capi.ERR_INVALID_ARGUMENT and b"bad C" in lib.atdf_last_error()
+    st = lib.atdf_grid_transfer(one, 9, None, one, 9, one, 3, None, one, 9, 8, 11, 0, None)
+    assert st == capi.ERR_INVALID_ARGUMENT and b"alias" in lib.atdf_last_error()
+    assert lib.atdf_grid_transfer(one, 9, None, None, 9, None, 0, None, None, 9, 8, 11, 1, None) == capi.OK
+    assert lib.atdf_node_id_for_position(one, 9, None, None, 0, None, None, None) == capi.OK
+    st = lib.atdf_node_id_for_position(one, 9, None, one, 5, one, one, None)
+    assert st == capi.ERR_INVALID_ARGUMENT and b"null buffer" in lib.atdf_last_error()
+
+
 def test_host_tree_matches_oracle(oracle, native):
     t = native.HyperTreeBase(3, 3)
     t.settree()
