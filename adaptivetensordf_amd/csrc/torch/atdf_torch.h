@@ -241,7 +241,7 @@ struct TrainState {
     float lr_grid = 1e-3f, lr_mlp = 1e-3f;
 };
 TrainState MakeTrainState(torch::Tensor grid, torch::Tensor mlp, bool bf16_grid, float lr_grid, float lr_mlp);
-// Structure step of training (samples/hyper_train.cpp:1887-1910): optimize
+// Structure step of training (samples/hyper_train.cpp:1886-1911): optimize
 // the tree structure; if the active set changed, move the grids to the new
 // active set (TransferNodeGrids, through the f32 master for bf16 grids) and
 // reset the Adam state of grids and decoder (ResetGeometryOptimizer,
