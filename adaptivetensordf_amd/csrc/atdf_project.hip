@@ -17,7 +17,8 @@
 //   dW2 += dH2 . H1^T and dW1 += dH1 . X^T
 // on MFMA with the tile's samples as the K index (operands transposed
 // through a per-wave LDS buffer), dX = W1^T . dH1 on MFMA, and scatters
-// w_corner * dX into the channels-last grid gradient with float atomics.
+// w_corner * dX into an LDS copy of the node's channels-last grid gradient
+// (node-major order, below), which is added to HBM once per chunk.
 // Per-wave parameter-gradient partials go to a workspace and are summed in a
 // fixed order by a second kernel (deterministic decoder gradients).
 //
@@ -792,10 +793,16 @@ __device__ __forceinline__ void store_partials(DecGrads& q, float* out, int g, i
 // chunk's intervals over its 4 waves, recomputes every sample (interval ->
 // t, weight, position, gather, decoder), runs the decoder backward on MFMA,
 // and scatters w_corner * dX into the LDS copy of the node's grid gradient
-// (waves take turns; see the scatter below).  At the end of the chunk the
-// LDS gradient is added to the global gradient once: a plain read-add-write
-// when the chunk covers the whole node, a global atomic add when the node
-// was split over several chunks.
+// under a workgroup lock (see the scatter below).  At the end of the chunk
+// the LDS gradient is added to the global gradient once: a plain
+// read-add-write when the chunk covers the whole node, a global atomic add
+// when the node was split over several chunks.
+//
+// Before the node kernel, pack_intervals_kernel writes every interval's
+// fields, its ray's origin / direction and dL/dP into one 48-byte record in
+// sorted order, so the node kernel reads its candidate intervals as
+// contiguous records instead of through three levels of indirection
+// (sorted id -> interval -> ray).
 //
 // A chunk holds about kChunkSamples samples: its interval count is
 // kChunkSamples / (mean samples per interval of the list), so stratified
@@ -825,6 +832,7 @@ struct NodeBwdArgs {
     int64_t num_slots;
     int* counter;             // zeroed before the launch
     const int32_t* iv_ray;
+    const float4* rec;        // [H][3] sorted interval records (pack_intervals_kernel)
     float* grad_grid;
     float* ws;                // decoder-gradient partial rows
     int lds_f32_off;          // float offset of the weight region in dynamic LDS
@@ -838,13 +846,87 @@ __host__ __device__ constexpr int weights_f32_floats()
     return kDecBF16 ? BSmem<C>::kF32Total : MlpSmem<C>::kTotal;
 }
 
+// Sorted interval record k (three float4): {tmin, tmax, count, first sample
+// in the ray}, {origin, direction.x}, {direction.y, direction.z, dL/dP, ray}.
+__global__ __launch_bounds__(256) void pack_intervals_kernel(ProjArgs a, const int32_t* __restrict__ sorted,
+                                                             const int32_t* __restrict__ iv_ray,
+                                                             const float* __restrict__ dLdP, int64_t H,
+                                                             float4* __restrict__ out)
+{
+    const int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (k >= H) return;
+    const int iv = sorted[k];
+    const int64_t r = iv_ray[iv];
+    out[k * 3] = make_float4(a.iv_tmin[iv], a.iv_tmax[iv], __int_as_float(a.iv_count[iv]),
+                             __int_as_float(a.iv_first[iv]));
+    out[k * 3 + 1] = make_float4(a.O[r * 3], a.O[r * 3 + 1], a.O[r * 3 + 2], a.D[r * 3]);
+    out[k * 3 + 2] = make_float4(a.D[r * 3 + 1], a.D[r * 3 + 2], dLdP[r], __int_as_float((int)r));
+}
+
+// make_sample_iv for a node whose box and grid offset are already known.
+__device__ __forceinline__ void make_sample_node(const ProjArgs& a, const PNode& nd, int64_t gbase, int64_t r,
+                                                 const float* o, const float* d, float tmin, float tmax, int n,
+                                                 int jj, int s_local, SampleCtx& sc)
+{
+    const float step = (tmax - tmin) / (float)n;
+    float aj = 0.5f, ap = 0.5f, an = 0.5f;
+    if (a.jitter_mode == 2) {
+        aj = hash_jitter(a.seed, r, s_local);
+        if (jj > 0) ap = hash_jitter(a.seed, r, s_local - 1);
+        if (jj < n - 1) an = hash_jitter(a.seed, r, s_local + 1);
+    }
+    const float t = sample_t(tmin, step, jj, aj);
+    const float tp = (jj > 0) ? sample_t(tmin, step, jj - 1, ap) : 0.f;
+    const float tn = (jj < n - 1) ? sample_t(tmin, step, jj + 1, an) : 0.f;
+    sc.w = sample_weight(tmin, tmax, n, jj, tp, t, tn);
+    sc.t = t;
+    float loc[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        float pos = o[k] + d[k] * t;
+        float size = nd.mx[k] - nd.mn[k];
+        loc[k] = ((pos - nd.mn[k]) / size) * 2 - 1;
+    }
+    tri_setup(loc[0], loc[1], loc[2], a.G, a.G, a.G, sc.tri);
+    sc.gbase = gbase;
+}
+
+// Workgroup lock in LDS guarding the node gradient.  Lane 0 polls with a
+// compare-and-swap; the holder runs a bounded scatter and releases, so the
+// wait ends.  Returns 0 only if the lock was not obtained within 2^22 polls.
+constexpr int kLockPolls = 1 << 22;
+
+__device__ __forceinline__ int lds_lock(int* lock, int lane)
+{
+    int got = 0;
+    if (lane == 0) {
+        for (int it = 0; it < kLockPolls; ++it) {
+            int expected = 0;
+            if (__hip_atomic_compare_exchange_strong(lock, &expected, 1, __ATOMIC_ACQUIRE, __ATOMIC_RELAXED,
+                                                     __HIP_MEMORY_SCOPE_WORKGROUP)) {
+                got = 1;
+                break;
+            }
+            __builtin_amdgcn_s_sleep(1);
+        }
+    }
+    return __shfl(got, 0, 64);
+}
+
+__device__ __forceinline__ void lds_unlock(int* lock, int lane)
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    if (lane == 0) __hip_atomic_store(lock, 0, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
 template <int C, bool kBF16, bool kDecBF16>
 __global__ __launch_bounds__(64 * kWaves) void project_bwd_node_kernel(NodeBwdArgs b)
 {
     extern __shared__ __attribute__((aligned(16))) unsigned char dsm[];
     __shared__ int s_chunk;
-    __shared__ int s_tiles[kWaves];
+    __shared__ int s_lock;
     const ProjArgs& a = b.a;
+    const float4* __restrict__ rec = b.rec;
     constexpr int CPL = C / 4;
     constexpr int NP = mlp_numel<C>();
     const int G = a.G;
@@ -865,6 +947,7 @@ __global__ __launch_bounds__(64 * kWaves) void project_bwd_node_kernel(NodeBwdAr
     else
         load_mlp_to_lds<C>(a.mlp, s);
     for (int i = threadIdx.x; i < gsz; i += blockDim.x) lgrad[i] = 0.f;
+    if (threadIdx.x == 0) s_lock = 0;
 
     DecGrads q;
     zero_grads(q);
@@ -879,22 +962,17 @@ __global__ __launch_bounds__(64 * kWaves) void project_bwd_node_kernel(NodeBwdAr
         if (c >= total_chunks) break;
         const int4 ch = b.chunks[c];
         const int64_t gbase = (int64_t)ch.x * gsz;
+        const PNode nd = a.nodes[a.iv_node[b.sorted[ch.y]]];  // one node per chunk
         const int n_iv = ch.z - ch.y;
         const int per = (n_iv + kWaves - 1) / kWaves;
         const int ib = ch.y + wl * per;
         const int ie = min(ib + per, ch.z);
         // samples of this wave's intervals
         int tot = 0;
-        for (int k = ib + lane; k < ie; k += 64) tot += a.iv_count[b.sorted[k]];
+        for (int k = ib + lane; k < ie; k += 64) tot += __float_as_int(rec[3 * (int64_t)k].z);
 #pragma unroll
         for (int off = 32; off > 0; off >>= 1) tot += __shfl_xor(tot, off, 64);
-
-        // every wave runs the same number of tiles: the scatter below is phased
-        if (lane == 0) s_tiles[wl] = (tot + kTile - 1) / kTile;
-        __syncthreads();
-        int ntiles = 0;
-#pragma unroll
-        for (int w = 0; w < kWaves; ++w) ntiles = max(ntiles, s_tiles[w]);
+        const int ntiles = (tot + kTile - 1) / kTile;
 
         // wave-uniform cursor: `cur` holds the tile's first sample, `cum`
         // samples of this wave's list precede it
@@ -904,14 +982,10 @@ __global__ __launch_bounds__(64 * kWaves) void project_bwd_node_kernel(NodeBwdAr
             const int base = ti * kTile;
             const int t = base + j;
             // the tile's samples lie in the 16 intervals from `cur` on (every
-            // interval holds at least one sample): load them once, prefix-sum
-            // their counts across the lane group, binary-search per lane
+            // interval holds at least one sample): read their counts once,
+            // prefix-sum them across the lane group, binary-search per lane
             const int cand = cur + j;
-            int c_iv = 0, c_cnt = 0;
-            if (cand < ie) {
-                c_iv = b.sorted[cand];
-                c_cnt = a.iv_count[c_iv];
-            }
+            const int c_cnt = cand < ie ? __float_as_int(rec[3 * (int64_t)cand].z) : 0;
             int c_end = c_cnt;
 #pragma unroll
             for (int off = 1; off < kTile; off <<= 1) {
@@ -922,7 +996,6 @@ __global__ __launch_bounds__(64 * kWaves) void project_bwd_node_kernel(NodeBwdAr
             const int pos = group_search(c_end, t, grp);
             const int posn = group_search(c_end, base + kTile, grp);
             const int src = grp | (pos < kTile ? pos : kTile - 1);
-            const int s_iv = __shfl(c_iv, src, 64);
             const int s_cnt = __shfl(c_cnt, src, 64);
             const int s_end = __shfl(c_end, src, 64);
             const int last_end = __shfl(c_end, grp | (kTile - 1), 64);
@@ -932,30 +1005,27 @@ __global__ __launch_bounds__(64 * kWaves) void project_bwd_node_kernel(NodeBwdAr
             sc.gbase = -1;
             float dp = 0.f;
             if (sc.valid) {
-                int iv = s_iv, n_iv = s_cnt, start = s_end - s_cnt;
+                int k = cur + pos, n_s = s_cnt, start = s_end - s_cnt;
                 if (pos >= kTile) {
                     // only with zero-sample intervals in the list: walk on
-                    int q = cur + kTile;
+                    k = cur + kTile;
                     start = last_end;
-                    n_iv = a.iv_count[b.sorted[q]];
-                    while (start + n_iv <= t) {
-                        start += n_iv;
-                        ++q;
-                        n_iv = a.iv_count[b.sorted[q]];
+                    n_s = __float_as_int(rec[3 * (int64_t)k].z);
+                    while (start + n_s <= t) {
+                        start += n_s;
+                        ++k;
+                        n_s = __float_as_int(rec[3 * (int64_t)k].z);
                     }
-                    iv = b.sorted[q];
                 }
-                const int64_t r = b.iv_ray[iv];
-                float o[3], d[3];
-#pragma unroll
-                for (int k = 0; k < 3; ++k) {
-                    o[k] = a.O[r * 3 + k];
-                    d[k] = a.D[r * 3 + k];
-                }
+                const float4 A = rec[3 * (int64_t)k];
+                const float4 B = rec[3 * (int64_t)k + 1];
+                const float4 D = rec[3 * (int64_t)k + 2];
+                const float o[3] = {B.x, B.y, B.z};
+                const float d[3] = {B.w, D.x, D.y};
                 const int jj = t - start;
-                make_sample_iv(a, r, o, d, a.iv_node[iv], a.iv_tmin[iv], a.iv_tmax[iv], n_iv, jj,
-                               a.iv_first[iv] + jj, C, sc);
-                dp = b.dLdP[r];
+                make_sample_node(a, nd, gbase, __float_as_int(D.w), o, d, A.x, A.y, n_s, jj,
+                                 __float_as_int(A.w) + jj, sc);
+                dp = D.z;
             }
             if (posn > 0) cum = __shfl(c_end, grp | (posn - 1), 64);
             cur += posn;
@@ -970,13 +1040,15 @@ __global__ __launch_bounds__(64 * kWaves) void project_bwd_node_kernel(NodeBwdAr
             const float dy = sc.valid ? dp * sc.w * softplus_grad_f(y, a.beta) : 0.f;
             v4f dx = tile_backward<C, kDecBF16>(s, sb, bufA, bufB, bufAb, bufBb, x, h1, h2, dy, g, j, q);
 
-            // Scatter w_corner * dX into the LDS node gradient.  The waves take
-            // turns (one phase each), so no two waves touch it at once; inside
-            // a wave one instruction can only collide between samples of the
-            // same floor cell, so samples with a unique cell in the tile use a
-            // plain float4 read-add-write and the rest (shared cell, or a
-            // clamped corner at the grid border) use LDS atomics.
-            const bool live = sc.valid && sc.gbase >= 0 && dy != 0.f;
+            // Scatter w_corner * dX into the LDS node gradient.  One wave at a
+            // time holds the lock, so waves never interleave their updates,
+            // and the others keep computing meanwhile.  Inside a wave one
+            // instruction can only collide between samples of the same floor
+            // cell (LDS runs a wave's operations in order), so samples with a
+            // unique cell in the tile use a plain float4 read-add-write and the
+            // rest (shared cell, or a clamped corner at the grid border) use
+            // LDS atomics.
+            const bool live = sc.valid && dy != 0.f;
             const bool border = sc.tri.x0 == sc.tri.x1 || sc.tri.y0 == sc.tri.y1 || sc.tri.z0 == sc.tri.z1;
             const int key = live ? (border ? -1 : (sc.tri.z0 * G + sc.tri.y0) * G + sc.tri.x0) : -2 - j;
             bool dup = false;
@@ -987,47 +1059,48 @@ __global__ __launch_bounds__(64 * kWaves) void project_bwd_node_kernel(NodeBwdAr
             }
             const bool plain = live && !border && !dup && 4 * g < C;
             const bool atomic = live && (border || dup) && 4 * g < C;
-#pragma unroll 1
-            for (int p = 0; p < kWaves; ++p) {
-                if (wl == p) {
-                    if (plain) {
+            if (__any(plain || atomic)) {
+                const int got = lds_lock(&s_lock, lane);
+                if (got && plain) {
 #pragma unroll
-                        for (int k = 0; k < 8; ++k) {
-                            float4* gp = reinterpret_cast<float4*>(lgrad + corner_cell(sc.tri, k, G) * C + 4 * g);
-                            const float wk = sc.tri.w[k];
-                            float4 v = *gp;
-                            v.x += wk * dx[0];
-                            v.y += wk * dx[1];
-                            v.z += wk * dx[2];
-                            v.w += wk * dx[3];
-                            *gp = v;
-                        }
-                    }
-                    wave_lds_sync();
-                    if (atomic) {
-#pragma unroll
-                        for (int k = 0; k < 8; ++k) {
-                            float* gp = lgrad + corner_cell(sc.tri, k, G) * C + 4 * g;
-                            const float wk = sc.tri.w[k];
-#pragma unroll
-                            for (int r4 = 0; r4 < 4; ++r4) atomicAdd(gp + r4, wk * dx[r4]);
-                        }
+                    for (int k = 0; k < 8; ++k) {
+                        float4* gp = reinterpret_cast<float4*>(lgrad + corner_cell(sc.tri, k, G) * C + 4 * g);
+                        const float wk = sc.tri.w[k];
+                        float4 v = *gp;
+                        v.x += wk * dx[0];
+                        v.y += wk * dx[1];
+                        v.z += wk * dx[2];
+                        v.w += wk * dx[3];
+                        *gp = v;
                     }
                 }
-                __syncthreads();
+                wave_lds_sync();
+                // without the lock (never seen: 2^22 polls) every update is atomic
+                if (atomic || (!got && plain)) {
+#pragma unroll
+                    for (int k = 0; k < 8; ++k) {
+                        float* gp = lgrad + corner_cell(sc.tri, k, G) * C + 4 * g;
+                        const float wk = sc.tri.w[k];
+#pragma unroll
+                        for (int r4 = 0; r4 < 4; ++r4) atomicAdd(gp + r4, wk * dx[r4]);
+                    }
+                }
+                if (got) lds_unlock(&s_lock, lane);
             }
         }
         __syncthreads();
-        // flush the node gradient once, clear LDS for the next chunk
-        // (float4 accesses, four in flight per thread; G^3 C is a multiple of 4 for C in {8, 16})
+        // flush the node gradient once, clear LDS for the next chunk (float4
+        // accesses, kU in flight per thread; G^3 C is a multiple of 4 for C in
+        // {8, 16})
+        constexpr int kU = kDecBF16 ? 16 : 4;
         float4* gg4 = reinterpret_cast<float4*>(b.grad_grid + gbase);
         float4* lg4 = reinterpret_cast<float4*>(lgrad);
         const int n4 = gsz >> 2;
         const float4 z4 = make_float4(0.f, 0.f, 0.f, 0.f);
-        for (int i0 = threadIdx.x; i0 < n4; i0 += 4 * blockDim.x) {
-            float4 v[4], o[4];
+        for (int i0 = threadIdx.x; i0 < n4; i0 += kU * blockDim.x) {
+            float4 v[kU], o[kU];
 #pragma unroll
-            for (int u = 0; u < 4; ++u) {
+            for (int u = 0; u < kU; ++u) {
                 const int i = i0 + u * blockDim.x;
                 if (i < n4) {
                     v[u] = lg4[i];
@@ -1036,7 +1109,7 @@ __global__ __launch_bounds__(64 * kWaves) void project_bwd_node_kernel(NodeBwdAr
                 }
             }
 #pragma unroll
-            for (int u = 0; u < 4; ++u) {
+            for (int u = 0; u < kU; ++u) {
                 const int i = i0 + u * blockDim.x;
                 if (i >= n4) continue;
                 if (ch.w) {
@@ -1419,7 +1492,7 @@ atdf_status atdf_project_fwd_bins(const void* packed_tree, int64_t num_nodes, co
 
 namespace {
 struct BwdWorkspace {
-    size_t cnt, slot_off, sorted, chunk_cnt, chunk_off, desc, counter, stotal, scan, partials, total;
+    size_t cnt, slot_off, sorted, chunk_cnt, chunk_off, desc, counter, stotal, scan, partials, rec, total;
 };
 
 size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
@@ -1452,6 +1525,7 @@ BwdWorkspace bwd_layout(int64_t H, int64_t NA, int np)
     w.stotal = take(sizeof(unsigned long long));
     w.scan = take(atdf_scan_workspace_bytes(NA));
     w.partials = take(sizeof(float) * (size_t)persistent_blocks() * kWaves * np);
+    w.rec = take(3 * sizeof(float4) * (size_t)(H > 0 ? H : 1));
     w.total = o;
     return w;
 }
@@ -1588,8 +1662,12 @@ atdf_status atdf_project_bwd(const void* packed_tree, int64_t num_nodes, const f
     args.num_slots = NA;
     args.counter = counter;
     args.iv_ray = iv->ray;
+    args.rec = reinterpret_cast<const float4*>(w + L.rec);
     args.grad_grid = grad_grid;
     args.ws = partials;
+    if (H > 0)
+        hipLaunchKernelGGL(pack_intervals_kernel, dim3((unsigned)div_up(H, 256)), dim3(256), 0, s, args.a, sorted,
+                           iv->ray, dLdP, H, reinterpret_cast<float4*>(w + L.rec));
     const bool bf = f->grid_dtype == ATDF_GRID_BF16;
     const bool bd = f->decoder_dtype == ATDF_DECODER_BF16;
 #define ATDF_BWD(CC, BF, BD) st = launch_node_bwd<CC, BF, BD>(args, s)
