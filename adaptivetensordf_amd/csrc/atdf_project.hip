@@ -193,6 +193,29 @@ __device__ __forceinline__ int group_search(int ends, int t, int grp)
     return e15 <= t ? 16 : pos;
 }
 
+// Inclusive prefix sum over each 16-lane row (DPP row shifts; lanes shifted
+// in from outside the row read 0).
+__device__ __forceinline__ int scan16_inclusive(int v)
+{
+    v += __builtin_amdgcn_update_dpp(0, v, 0x111, 0xF, 0xF, true);  // row_shr:1
+    v += __builtin_amdgcn_update_dpp(0, v, 0x112, 0xF, 0xF, true);  // row_shr:2
+    v += __builtin_amdgcn_update_dpp(0, v, 0x114, 0xF, 0xF, true);  // row_shr:4
+    v += __builtin_amdgcn_update_dpp(0, v, 0x118, 0xF, 0xF, true);  // row_shr:8
+    return v;
+}
+
+// Whether another lane of the same 16-lane row holds `key` (DPP row
+// rotations 1..15, no LDS traffic).
+__device__ __forceinline__ bool dup16(int key)
+{
+    bool dup = false;
+#define ATDF_ROR(n) dup |= (__builtin_amdgcn_update_dpp(0, key, 0x120 + (n), 0xF, 0xF, false) == key)
+    ATDF_ROR(1); ATDF_ROR(2); ATDF_ROR(3); ATDF_ROR(4); ATDF_ROR(5); ATDF_ROR(6); ATDF_ROR(7); ATDF_ROR(8);
+    ATDF_ROR(9); ATDF_ROR(10); ATDF_ROR(11); ATDF_ROR(12); ATDF_ROR(13); ATDF_ROR(14); ATDF_ROR(15);
+#undef ATDF_ROR
+    return dup;
+}
+
 // Recompute sample `s_local` of ray r (lane-private interval cursor `cur`).
 __device__ __forceinline__ void make_sample(const ProjArgs& a, int64_t r, const float* o, const float* d, int ns,
                                             int s_local, int64_t h1, int64_t& cur, int C, SampleCtx& sc)
@@ -494,14 +517,28 @@ __global__ __launch_bounds__(64 * kWaves) void project_fwd_kernel(ProjArgs a, fl
             }
             const int t = base + j;
             const int grp = lane & 48;
-            int pos = group_search(c_end, t, grp);
-            const int posn = group_search(c_end, base + kTile, grp);
-            const int src = grp | (pos < kTile ? pos : kTile - 1);
-            const int s_cnt = __shfl(c_cnt, src, 64);
-            const int s_end = __shfl(c_end, src, 64);
-            const int s_node = __shfl(c_node, src, 64);
-            const float s_lo = __shfl(c_lo, src, 64);
-            const float s_hi = __shfl(c_hi, src, 64);
+            // a window of one-sample intervals maps sample j to candidate j
+            const bool ones = __all(cand >= h1e || c_cnt == 1);
+            int pos, posn, s_cnt, s_end, s_node;
+            float s_lo, s_hi;
+            if (ones) {
+                pos = j;
+                posn = (int)min((int64_t)kTile, h1e - cur);
+                s_cnt = c_cnt;
+                s_end = c_end;
+                s_node = c_node;
+                s_lo = c_lo;
+                s_hi = c_hi;
+            } else {
+                pos = group_search(c_end, t, grp);
+                posn = group_search(c_end, base + kTile, grp);
+                const int src = grp | (pos < kTile ? pos : kTile - 1);
+                s_cnt = __shfl(c_cnt, src, 64);
+                s_end = __shfl(c_end, src, 64);
+                s_node = __shfl(c_node, src, 64);
+                s_lo = __shfl(c_lo, src, 64);
+                s_hi = __shfl(c_hi, src, 64);
+            }
             sc.valid = t < ns;
             sc.w = 0.f;
             sc.gbase = -1;
@@ -984,21 +1021,28 @@ __global__ __launch_bounds__(64 * kWaves) void project_bwd_node_kernel(NodeBwdAr
             // the tile's samples lie in the 16 intervals from `cur` on (every
             // interval holds at least one sample): read their counts once,
             // prefix-sum them across the lane group, binary-search per lane
+            // (a window of one-sample intervals, as learned lists have, maps
+            // sample j to candidate j directly)
             const int cand = cur + j;
             const int c_cnt = cand < ie ? __float_as_int(rec[3 * (int64_t)cand].z) : 0;
+            const bool ones = __all(cand >= ie || c_cnt == 1);
             int c_end = c_cnt;
-#pragma unroll
-            for (int off = 1; off < kTile; off <<= 1) {
-                const int v = __shfl_up(c_end, off, kTile);
-                if (j >= off) c_end += v;
+            int pos, posn, s_cnt, s_end, last_end = 0;
+            if (ones) {
+                pos = j;
+                posn = min(kTile, ie - cur);
+                s_cnt = 1;
+                s_end = cum + j + 1;
+            } else {
+                c_end = scan16_inclusive(c_end);
+                c_end = cand < ie ? c_end + cum : INT_MAX;
+                pos = group_search(c_end, t, grp);
+                posn = group_search(c_end, base + kTile, grp);
+                const int src = grp | (pos < kTile ? pos : kTile - 1);
+                s_cnt = __shfl(c_cnt, src, 64);
+                s_end = __shfl(c_end, src, 64);
+                last_end = __shfl(c_end, grp | (kTile - 1), 64);
             }
-            c_end = cand < ie ? c_end + cum : INT_MAX;
-            const int pos = group_search(c_end, t, grp);
-            const int posn = group_search(c_end, base + kTile, grp);
-            const int src = grp | (pos < kTile ? pos : kTile - 1);
-            const int s_cnt = __shfl(c_cnt, src, 64);
-            const int s_end = __shfl(c_end, src, 64);
-            const int last_end = __shfl(c_end, grp | (kTile - 1), 64);
             SampleCtx sc;
             sc.valid = t < tot;
             sc.w = 0.f;
@@ -1027,7 +1071,10 @@ __global__ __launch_bounds__(64 * kWaves) void project_bwd_node_kernel(NodeBwdAr
                                  __float_as_int(A.w) + jj, sc);
                 dp = D.z;
             }
-            if (posn > 0) cum = __shfl(c_end, grp | (posn - 1), 64);
+            if (ones)
+                cum += posn;
+            else if (posn > 0)
+                cum = __shfl(c_end, grp | (posn - 1), 64);
             cur += posn;
             float x[CPL];
             gather_channels<C, kBF16>(a, sc, CPL * g, x);
@@ -1051,12 +1098,7 @@ __global__ __launch_bounds__(64 * kWaves) void project_bwd_node_kernel(NodeBwdAr
             const bool live = sc.valid && dy != 0.f;
             const bool border = sc.tri.x0 == sc.tri.x1 || sc.tri.y0 == sc.tri.y1 || sc.tri.z0 == sc.tri.z1;
             const int key = live ? (border ? -1 : (sc.tri.z0 * G + sc.tri.y0) * G + sc.tri.x0) : -2 - j;
-            bool dup = false;
-#pragma unroll
-            for (int off = 1; off < kTile; ++off) {
-                const int other = __shfl(key, (lane & 48) | ((j + off) & 15), 64);
-                dup |= (other == key);
-            }
+            const bool dup = dup16(key);
             const bool plain = live && !border && !dup && 4 * g < C;
             const bool atomic = live && (border || dup) && 4 * g < C;
             if (__any(plain || atomic)) {
