@@ -39,7 +39,7 @@ class Tree(C.Structure):
 class Sampling(C.Structure):
     _fields_ = [("max_samples_per_node", C.c_int32), ("max_intersections", C.c_int32), ("jitter_mode", C.c_int32),
                 ("reserved", C.c_int32), ("jitter_seed", C.c_uint64), ("jitter", C.c_void_p),
-                ("ray_select", C.c_void_p)]
+                ("ray_select", C.c_void_p), ("ray_list", C.c_void_p)]
 
 
 class Intervals(C.Structure):
@@ -111,6 +111,7 @@ _SIGNATURES = {
     "atdf_proposal_write": (_I32, [_I64, _P, _P, _P, _P, _P, _P, C.POINTER(Intervals), _P]),
     "atdf_proposal_grad_workspace_bytes": (_SZ, [_I64]),
     "atdf_proposal_grad": (_I32, [_P, _P, _P, _I64, _F, _P, _P, _P, _SZ, _P]),
+    "atdf_compact_rays": (_I32, [_P, _I64, _P, _P, _P]),
     "atdf_node_id_for_position": (_I32, [_P, _I64, _P, _P, _I64, _P, _P, _P]),
     "atdf_grid_transfer": (_I32, [_P, _I64, _P, _P, _I64, _P, _I64, _P, _P, _I64, _I32, _I32, _I32, _P]),
     "atdf_grid_layout": (_I32, [_P, _P, _I64, _I32, _I32, _I32, _P]),

@@ -835,11 +835,10 @@ __device__ __forceinline__ void store_partials(DecGrads& q, float* out, int g, i
 // read-add-write when the chunk covers the whole node, a global atomic add
 // when the node was split over several chunks.
 //
-// Before the node kernel, pack_intervals_kernel writes every interval's
-// fields, its ray's origin / direction and dL/dP into one 48-byte record in
-// sorted order, so the node kernel reads its candidate intervals as
-// contiguous records instead of through three levels of indirection
-// (sorted id -> interval -> ray).
+// The counting sort writes every interval's fields, its ray's origin /
+// direction and dL/dP into one 48-byte record at its sorted position, so the
+// node kernel reads its candidate intervals as contiguous records instead of
+// through three levels of indirection (sorted id -> interval -> ray).
 //
 // A chunk holds about kChunkSamples samples: its interval count is
 // kChunkSamples / (mean samples per interval of the list), so stratified
@@ -869,7 +868,7 @@ struct NodeBwdArgs {
     int64_t num_slots;
     int* counter;             // zeroed before the launch
     const int32_t* iv_ray;
-    const float4* rec;        // [H][3] sorted interval records (pack_intervals_kernel)
+    const float4* rec;        // [H][3] sorted interval records (write_record)
     float* grad_grid;
     float* ws;                // decoder-gradient partial rows
     int lds_f32_off;          // float offset of the weight region in dynamic LDS
@@ -885,19 +884,26 @@ __host__ __device__ constexpr int weights_f32_floats()
 
 // Sorted interval record k (three float4): {tmin, tmax, count, first sample
 // in the ray}, {origin, direction.x}, {direction.y, direction.z, dL/dP, ray}.
-__global__ __launch_bounds__(256) void pack_intervals_kernel(ProjArgs a, const int32_t* __restrict__ sorted,
-                                                             const int32_t* __restrict__ iv_ray,
-                                                             const float* __restrict__ dLdP, int64_t H,
-                                                             float4* __restrict__ out)
+// Written by the counting sort's scatter pass, which reads the intervals in
+// list order (coalesced) and their rays (mostly the same few lines).
+struct RecordSrc {
+    const float* tmin;
+    const float* tmax;
+    const int32_t* count;
+    const int32_t* first;
+    const int32_t* ray;
+    const float* O;
+    const float* D;
+    const float* dLdP;
+    float4* rec;
+};
+
+__device__ __forceinline__ void write_record(const RecordSrc& rs, int64_t i, int64_t k)
 {
-    const int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    if (k >= H) return;
-    const int iv = sorted[k];
-    const int64_t r = iv_ray[iv];
-    out[k * 3] = make_float4(a.iv_tmin[iv], a.iv_tmax[iv], __int_as_float(a.iv_count[iv]),
-                             __int_as_float(a.iv_first[iv]));
-    out[k * 3 + 1] = make_float4(a.O[r * 3], a.O[r * 3 + 1], a.O[r * 3 + 2], a.D[r * 3]);
-    out[k * 3 + 2] = make_float4(a.D[r * 3 + 1], a.D[r * 3 + 2], dLdP[r], __int_as_float((int)r));
+    const int64_t r = rs.ray[i];
+    rs.rec[k * 3] = make_float4(rs.tmin[i], rs.tmax[i], __int_as_float(rs.count[i]), __int_as_float(rs.first[i]));
+    rs.rec[k * 3 + 1] = make_float4(rs.O[r * 3], rs.O[r * 3 + 1], rs.O[r * 3 + 2], rs.D[r * 3]);
+    rs.rec[k * 3 + 2] = make_float4(rs.D[r * 3 + 1], rs.D[r * 3 + 2], rs.dLdP[r], __int_as_float((int)r));
 }
 
 // make_sample_iv for a node whose box and grid offset are already known.
@@ -1190,14 +1196,17 @@ __global__ __launch_bounds__(256) void slot_hist_kernel(const int32_t* __restric
 __global__ __launch_bounds__(256) void slot_scatter_kernel(const int32_t* __restrict__ iv_node,
                                                            const int32_t* __restrict__ node_to_grid, int64_t H,
                                                            const int64_t* __restrict__ slot_off,
-                                                           int32_t* __restrict__ cursor, int32_t* __restrict__ sorted)
+                                                           int32_t* __restrict__ cursor, int32_t* __restrict__ sorted,
+                                                           RecordSrc rs)
 {
     int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (i >= H) return;
     int slot = node_to_grid[iv_node[i]];
     if (slot < 0) return;
     int pos = atomicAdd(&cursor[slot], 1);
-    sorted[slot_off[slot] + pos] = (int32_t)i;
+    const int64_t k = slot_off[slot] + pos;
+    sorted[k] = (int32_t)i;
+    write_record(rs, i, k);
 }
 
 // Block-aggregated counting sort for lists with many intervals per slot
@@ -1239,7 +1248,7 @@ __global__ __launch_bounds__(256) void slot_scatter_lds_kernel(const int32_t* __
                                                                const int32_t* __restrict__ node_to_grid, int64_t H,
                                                                int64_t NA, const int64_t* __restrict__ slot_off,
                                                                int32_t* __restrict__ cursor,
-                                                               int32_t* __restrict__ sorted)
+                                                               int32_t* __restrict__ sorted, RecordSrc rs)
 {
     extern __shared__ int32_t lh[];
     const int64_t b0 = blockIdx.x * (int64_t)kSortItems;
@@ -1252,7 +1261,11 @@ __global__ __launch_bounds__(256) void slot_scatter_lds_kernel(const int32_t* __
     __syncthreads();
     for (int64_t i = b0 + threadIdx.x; i < e0; i += blockDim.x) {
         const int slot = node_to_grid[iv_node[i]];
-        if (slot >= 0) sorted[atomicAdd(&lh[slot], 1)] = (int32_t)i;
+        if (slot >= 0) {
+            const int64_t k = atomicAdd(&lh[slot], 1);
+            sorted[k] = (int32_t)i;
+            write_record(rs, i, k);
+        }
     }
 }
 
@@ -1412,14 +1425,28 @@ __global__ __launch_bounds__(64 * kWaves) void node_stats_kernel(StatsArgs sa)
 }
 
 // grad_mlp[i] += sum_w ws[w][i], summed in wave order.
+// grad[i] += sum over the partial rows, in a fixed order: a workgroup owns 16
+// parameters, 16 row groups (rows w = g, g + 16, ...) sum in registers, and
+// the 16 group sums are added in group order.
+constexpr int kReduceParams = 16;
+
 __global__ __launch_bounds__(256) void reduce_partials_kernel(const float* __restrict__ ws, int64_t nwaves, int np,
                                                               float* __restrict__ grad)
 {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= np) return;
+    __shared__ float part[16][kReduceParams + 1];
+    const int pl = threadIdx.x % kReduceParams, rg = threadIdx.x / kReduceParams;
+    const int i = blockIdx.x * kReduceParams + pl;
     float acc = 0.f;
-    for (int64_t w = 0; w < nwaves; ++w) acc += ws[w * np + i];
-    grad[i] += acc;
+    if (i < np)
+        for (int64_t w = rg; w < nwaves; w += 16) acc += ws[w * np + i];
+    part[rg][pl] = acc;
+    __syncthreads();
+    if (rg == 0 && i < np) {
+        float t = 0.f;
+#pragma unroll
+        for (int k = 0; k < 16; ++k) t += part[k][pl];
+        grad[i] += t;
+    }
 }
 
 }  // namespace atdf
@@ -1675,13 +1702,15 @@ atdf_status atdf_project_bwd(const void* packed_tree, int64_t num_nodes, const f
     st = atdf_exclusive_scan_i32(cnt, NA, slot_off, scan_ws, scan_bytes, stream);
     if (st) return st;
     hipLaunchKernelGGL(fill_i32_kernel, dim3((unsigned)div_up(NA, 256)), dim3(256), 0, s, cnt, NA, 0);
+    float4* rec = reinterpret_cast<float4*>(w + L.rec);
+    const RecordSrc rs{iv->tmin, iv->tmax, iv->count, iv->first, iv->ray, O, D, dLdP, rec};
     if (H > 0) {
         if (lds_sort)
             hipLaunchKernelGGL(slot_scatter_lds_kernel, dim3(sort_blocks), dim3(256), hist_lds, s, iv->node,
-                               f->node_to_grid, H, NA, slot_off, cnt, sorted);
+                               f->node_to_grid, H, NA, slot_off, cnt, sorted, rs);
         else
             hipLaunchKernelGGL(slot_scatter_kernel, dim3((unsigned)div_up(H, 256)), dim3(256), 0, s, iv->node,
-                               f->node_to_grid, H, slot_off, cnt, sorted);
+                               f->node_to_grid, H, slot_off, cnt, sorted, rs);
     }
     unsigned long long* stotal = reinterpret_cast<unsigned long long*>(w + L.stotal);
     hipLaunchKernelGGL(fill_i32_kernel, dim3(1), dim3(256), 0, s, reinterpret_cast<int32_t*>(stotal), 2, 0);
@@ -1704,12 +1733,9 @@ atdf_status atdf_project_bwd(const void* packed_tree, int64_t num_nodes, const f
     args.num_slots = NA;
     args.counter = counter;
     args.iv_ray = iv->ray;
-    args.rec = reinterpret_cast<const float4*>(w + L.rec);
+    args.rec = rec;
     args.grad_grid = grad_grid;
     args.ws = partials;
-    if (H > 0)
-        hipLaunchKernelGGL(pack_intervals_kernel, dim3((unsigned)div_up(H, 256)), dim3(256), 0, s, args.a, sorted,
-                           iv->ray, dLdP, H, reinterpret_cast<float4*>(w + L.rec));
     const bool bf = f->grid_dtype == ATDF_GRID_BF16;
     const bool bd = f->decoder_dtype == ATDF_DECODER_BF16;
 #define ATDF_BWD(CC, BF, BD) st = launch_node_bwd<CC, BF, BD>(args, s)
@@ -1722,7 +1748,7 @@ atdf_status atdf_project_bwd(const void* packed_tree, int64_t num_nodes, const f
     }
 #undef ATDF_BWD
     if (st) return st;
-    hipLaunchKernelGGL(reduce_partials_kernel, dim3((unsigned)div_up(np, 256)), dim3(256), 0, s, partials,
+    hipLaunchKernelGGL(reduce_partials_kernel, dim3((unsigned)div_up(np, kReduceParams)), dim3(256), 0, s, partials,
                        (int64_t)persistent_blocks() * kWaves, np, grad_mlp);
     return check_launch("atdf_project_bwd");
 }

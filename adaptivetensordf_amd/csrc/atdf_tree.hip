@@ -83,6 +83,7 @@ struct TraverseArgs {
     uint64_t seed;
     const float* jitter;
     const int32_t* select;
+    const int32_t* list;
     int32_t* ray_num_intervals;
     int32_t* ray_num_samples;
     int32_t* status;
@@ -104,6 +105,10 @@ __global__ __launch_bounds__(256) void traverse_kernel(TraverseArgs a)
 {
     int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (r >= a.R) return;
+    if (a.list) {
+        r = a.list[r];
+        if (r < 0 || r >= a.R) return;
+    }
     if (a.select && a.select[r] == 0) {
         if (!kWrite) {
             a.ray_num_intervals[r] = 0;
@@ -291,6 +296,30 @@ __global__ void zero_i64_kernel(int64_t* p)
     if (threadIdx.x == 0) p[0] = 0;
 }
 
+// ---------------------------------------------------------------------------
+// ray-list compaction: one counter atomic per wave
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void fill_list_kernel(int32_t* __restrict__ list, int64_t n,
+                                                        int32_t* __restrict__ count)
+{
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i == 0) *count = 0;
+    if (i < n) list[i] = -1;
+}
+
+__global__ __launch_bounds__(256) void compact_kernel(const int32_t* __restrict__ flags, int64_t n,
+                                                      int32_t* __restrict__ list, int32_t* __restrict__ count)
+{
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    const bool f = i < n && flags[i] != 0;
+    const unsigned long long m = __ballot(f);
+    const int lane = threadIdx.x & 63;
+    int base = 0;
+    if (lane == 0 && m) base = atomicAdd(count, __popcll(m));
+    base = __shfl(base, 0, 64);
+    if (f) list[base + __popcll(m & ((1ull << lane) - 1ull))] = (int32_t)i;
+}
+
 }  // namespace atdf
 
 using namespace atdf;
@@ -352,6 +381,7 @@ atdf_status atdf_traverse_count(const void* packed_tree, int64_t num_nodes, cons
     a.seed = sampling->jitter_seed;
     a.jitter = sampling->jitter;
     a.select = sampling->ray_select;
+    a.list = sampling->ray_list;
     a.ray_num_intervals = ray_num_intervals;
     a.ray_num_samples = ray_num_samples;
     a.status = status;
@@ -394,6 +424,7 @@ atdf_status atdf_traverse_write(const void* packed_tree, int64_t num_nodes, cons
     a.seed = sampling->jitter_seed;
     a.jitter = sampling->jitter;
     a.select = sampling->ray_select;
+    a.list = sampling->ray_list;
     a.iv_off = intervals ? ray_interval_offset : ray_sample_offset;
     a.smp_off = samples ? ray_sample_offset : ray_interval_offset;
     if (intervals) a.iv = *intervals;
@@ -406,6 +437,17 @@ size_t atdf_scan_workspace_bytes(int64_t n)
 {
     int64_t nt = n > 0 ? div_up(n, kScanTile) : 1;
     return (size_t)nt * sizeof(int64_t);
+}
+
+atdf_status atdf_compact_rays(const int32_t* flags, int64_t n, int32_t* list, int32_t* count, atdf_stream_t stream)
+{
+    ATDF_REQUIRE(n >= 0 && n < (int64_t)INT32_MAX && count, "atdf_compact_rays: bad arguments");
+    ATDF_REQUIRE(n == 0 || (flags && list), "atdf_compact_rays: null buffer");
+    hipStream_t s = to_hip(stream);
+    const unsigned blocks = (unsigned)(n > 0 ? div_up(n, 256) : 1);
+    hipLaunchKernelGGL(fill_list_kernel, dim3(blocks), dim3(256), 0, s, list, n, count);
+    if (n > 0) hipLaunchKernelGGL(compact_kernel, dim3(blocks), dim3(256), 0, s, flags, n, list, count);
+    return check_launch("atdf_compact_rays");
 }
 
 atdf_status atdf_exclusive_scan_i32(const int32_t* in, int64_t n, int64_t* out, void* workspace,

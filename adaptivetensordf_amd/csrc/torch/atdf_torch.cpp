@@ -1135,6 +1135,12 @@ Intervals CreateProposalIntervals(HyperTreeBase& tree, const ProposalState& ps, 
     auto i64 = torch::TensorOptions().dtype(torch::kLong).device(dev);
     auto f32 = torch::TensorOptions().dtype(torch::kFloat32).device(dev);
     auto st = current_stream();
+    // the fallback rays (a few percent) traverse in dense waves
+    auto list = torch::empty({R}, i32), list_count = torch::empty({1}, i32);
+    check_status(atdf_compact_rays(fb.data_ptr<int32_t>(), R, list.data_ptr<int32_t>(), list_count.data_ptr<int32_t>(),
+                                   st),
+                 "atdf_compact_rays");
+    s.ray_list = list.data_ptr<int32_t>();
     auto num_iv = torch::empty({R}, i32), num_s = torch::empty({R}, i32);
     auto status = torch::zeros({4}, i32);
     Intervals iv;

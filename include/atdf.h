@@ -75,6 +75,11 @@ typedef struct {
     const float* jitter;          /* mode 1: U[0,1) per canonical sample slot   */
     const int32_t* ray_select;    /* nullable [R]: rays with 0 get no intervals
                                      (the learned proposal's fallback mask)     */
+    const int32_t* ray_list;      /* nullable [R]: lane i traverses ray
+                                     ray_list[i] (-1: none), so a sparse
+                                     selection runs in dense waves; per-ray
+                                     outputs of unlisted rays are not written
+                                     (atdf_compact_rays builds the list)        */
 } atdf_sampling;
 
 /* status[0] += rays over max_intersections, status[1] += order-check
@@ -84,6 +89,10 @@ atdf_status atdf_traverse_count(const void* packed_tree, int64_t num_nodes, cons
                                 const float* ray_direction, int64_t num_rays, const atdf_sampling* sampling,
                                 int32_t* ray_num_intervals, int32_t* ray_num_samples, int32_t* status,
                                 atdf_stream_t stream);
+
+/* list[0..m) = the indices i < n with flags[i] != 0 (in no fixed order),
+ * list[m..n) = -1, *count = m (device int32, written by the call). */
+atdf_status atdf_compact_rays(const int32_t* flags, int64_t n, int32_t* list, int32_t* count, atdf_stream_t stream);
 
 /* out[0] = 0, out[i+1] = out[i] + in[i] for i < n (n+1 outputs). */
 size_t atdf_scan_workspace_bytes(int64_t n);

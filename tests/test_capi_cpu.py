@@ -81,6 +81,8 @@ def test_restructure_boundary_without_device_work():
     assert st == capi.ERR_INVALID_ARGUMENT and b"alias" in lib.atdf_last_error()
     assert lib.atdf_grid_transfer(one, 9, None, None, 9, None, 0, None, None, 9, 8, 11, 1, None) == capi.OK
     assert lib.atdf_node_id_for_position(one, 9, None, None, 0, None, None, None) == capi.OK
+    assert lib.atdf_compact_rays(None, -1, None, one, None) == capi.ERR_INVALID_ARGUMENT
+    assert lib.atdf_compact_rays(None, 5, None, one, None) == capi.ERR_INVALID_ARGUMENT
     st = lib.atdf_node_id_for_position(one, 9, None, one, 5, one, one, None)
     assert st == capi.ERR_INVALID_ARGUMENT and b"null buffer" in lib.atdf_last_error()
 

@@ -65,6 +65,47 @@ def _sp(native, K=8, jitter_mode=0, seed=0):
     return sp
 
 
+def test_compact_rays_and_listed_traversal(oracle, native, cuda):
+    """atdf_compact_rays lists exactly the flagged rays (then -1), and a
+    traversal over the list gives the listed rays the counts of a full one."""
+    import ctypes as Ct
+
+    from adaptivetensordf_amd import capi
+
+    rng = np.random.default_rng(9)
+    for n in (1, 63, 64, 1000, 70001):
+        flags = torch.from_numpy((rng.random(n) < 0.05).astype(np.int32)).cuda()
+        lst = torch.empty(n, dtype=torch.int32, device="cuda")
+        cnt = torch.empty(1, dtype=torch.int32, device="cuda")
+        capi.call("atdf_compact_rays", capi.ptr(flags), n, capi.ptr(lst), capi.ptr(cnt), capi.stream_ptr())
+        m = int(cnt.item())
+        f = flags.cpu().numpy()
+        got = lst.cpu().numpy()
+        assert m == int(f.sum())
+        assert sorted(got[:m].tolist()) == np.nonzero(f)[0].tolist() and np.all(got[m:] == -1)
+    t = oracle.tree_build(4)
+    tree = device_tree(4, adaptive_active(t, budget=300, seed=2))
+    o, d = random_rays(5000, 4)
+    o_t, d_t = rays_to_device(o, d)
+    sel = torch.from_numpy((rng.random(5000) < 0.1).astype(np.int32)).cuda()
+    lst = torch.empty(5000, dtype=torch.int32, device="cuda")
+    cnt = torch.empty(1, dtype=torch.int32, device="cuda")
+    capi.call("atdf_compact_rays", capi.ptr(sel), 5000, capi.ptr(lst), capi.ptr(cnt), capi.stream_ptr())
+    out = []
+    for use_list in (False, True):
+        sp = capi.Sampling(6, 256, 2, 0, 3, None, sel.data_ptr(), lst.data_ptr() if use_list else None)
+        niv = torch.full((5000,), -7, dtype=torch.int32, device="cuda")
+        ns = torch.full((5000,), -7, dtype=torch.int32, device="cuda")
+        status = torch.zeros(4, dtype=torch.int32, device="cuda")
+        capi.call("atdf_traverse_count", capi.ptr(tree.PackedTree()), tree.NumNodes(), capi.ptr(o_t), capi.ptr(d_t),
+                  5000, Ct.byref(sp), capi.ptr(niv), capi.ptr(ns), capi.ptr(status), capi.stream_ptr())
+        out.append((niv.cpu().numpy(), ns.cpu().numpy(), status.cpu().numpy()))
+    s_ = sel.cpu().numpy().astype(bool)
+    assert np.array_equal(out[0][0][s_], out[1][0][s_]) and np.array_equal(out[0][1][s_], out[1][1][s_])
+    assert np.all(out[1][0][~s_] == -7)  # unlisted rays are not written
+    assert np.array_equal(out[0][2], out[1][2])
+
+
 def test_leaf_table_bit_exact(oracle, native, cuda):
     for depth, budget in ((3, 120), (5, 900)):
         t, act, tree = _scene(oracle, native, depth=depth, budget=budget)
