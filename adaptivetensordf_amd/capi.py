@@ -42,6 +42,12 @@ class Sampling(C.Structure):
                 ("ray_select", C.c_void_p), ("ray_list", C.c_void_p)]
 
 
+class GridAdam(C.Structure):
+    _fields_ = [("param", C.c_void_p), ("param_bf16", C.c_void_p), ("exp_avg", C.c_void_p),
+                ("exp_avg_sq", C.c_void_p), ("lr", C.c_float), ("beta1", C.c_float), ("beta2", C.c_float),
+                ("eps", C.c_float), ("step", C.c_int64)]
+
+
 class Intervals(C.Structure):
     _fields_ = [("node", C.c_void_p), ("tmin", C.c_void_p), ("tmax", C.c_void_p), ("count", C.c_void_p),
                 ("first", C.c_void_p), ("ray", C.c_void_p), ("total", C.c_int64)]
@@ -100,6 +106,8 @@ _SIGNATURES = {
     "atdf_mse_loss": (_I32, [_P, _P, _I64, _F, _P, _P, _P, _SZ, _P]),
     "atdf_adam_step": (_I32, [_P, _P, _P, _P, _I64, _F, _F, _F, _F, _I64, _I32, _P]),
     "atdf_adam_step_bf16": (_I32, [_P, _P, _P, _P, _P, _I64, _F, _F, _F, _F, _I64, _I32, _P]),
+    "atdf_project_bwd_adam": (_I32, [_P, _I64, _P, _P, _I64, C.POINTER(Sampling), _P, C.POINTER(Intervals),
+                                     C.POINTER(Field), _P, _P, _P, C.POINTER(GridAdam), _P, _SZ, _P]),
     "atdf_project_fwd_bins": (_I32, [_P, _I64, _P, _P, _I64, C.POINTER(Sampling), _P, C.POINTER(Intervals),
                                      C.POINTER(Field), _P, _P, _P]),
     "atdf_proposal_num_params": (_I64, []),

@@ -205,6 +205,17 @@ __device__ __forceinline__ float bf16_to_f32(uint16_t h)
     return __uint_as_float(((uint32_t)h) << 16);
 }
 
+// One torch::optim::Adam update of a parameter (bias corrections bc1 and
+// sqrt(bc2) precomputed); m and v are updated in place, returns the new value.
+__device__ __forceinline__ float adam_update(float p, float g, float& m, float& v, float lr, float b1, float b2,
+                                             float eps, float bc1, float bc2_sqrt)
+{
+    m = b1 * m + (1.f - b1) * g;
+    v = b2 * v + (1.f - b2) * g * g;
+    const float denom = sqrtf(v) / bc2_sqrt + eps;
+    return p - (lr / bc1) * (m / denom);
+}
+
 __device__ __forceinline__ uint16_t f32_to_bf16_rne(float f)
 {
     uint32_t u = __float_as_uint(f);

@@ -356,23 +356,17 @@ __global__ __launch_bounds__(256) void adam_kernel(float* __restrict__ p, float*
         float* vp = &vv.x;
         float* pq = &pp.x;
 #pragma unroll
-        for (int k = 0; k < 4; ++k) {
-            mp[k] = b1 * mp[k] + (1.f - b1) * gp[k];
-            vp[k] = b2 * vp[k] + (1.f - b2) * gp[k] * gp[k];
-            float denom = sqrtf(vp[k]) / bc2_sqrt + eps;
-            pq[k] = pq[k] - (lr / bc1) * (mp[k] / denom);
-        }
+        for (int k = 0; k < 4; ++k) pq[k] = adam_update(pq[k], gp[k], mp[k], vp[k], lr, b1, b2, eps, bc1, bc2_sqrt);
         *reinterpret_cast<float4*>(m + i4) = mm;
         *reinterpret_cast<float4*>(v + i4) = vv;
         *reinterpret_cast<float4*>(p + i4) = pp;
         if (zero_grad) *reinterpret_cast<float4*>(g + i4) = make_float4(0.f, 0.f, 0.f, 0.f);
     } else {
         for (int64_t i = i4; i < n; ++i) {
-            float gi = g[i];
-            m[i] = b1 * m[i] + (1.f - b1) * gi;
-            v[i] = b2 * v[i] + (1.f - b2) * gi * gi;
-            float denom = sqrtf(v[i]) / bc2_sqrt + eps;
-            p[i] = p[i] - (lr / bc1) * (m[i] / denom);
+            float mi = m[i], vi = v[i];
+            p[i] = adam_update(p[i], g[i], mi, vi, lr, b1, b2, eps, bc1, bc2_sqrt);
+            m[i] = mi;
+            v[i] = vi;
             if (zero_grad) g[i] = 0.f;
         }
     }
@@ -386,11 +380,8 @@ __global__ __launch_bounds__(256) void adam_bf16_kernel(float* __restrict__ p, u
 {
     int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (i >= n) return;
-    float gi = g[i];
-    float mi = b1 * m[i] + (1.f - b1) * gi;
-    float vi = b2 * v[i] + (1.f - b2) * gi * gi;
-    float denom = sqrtf(vi) / bc2_sqrt + eps;
-    float pi = p[i] - (lr / bc1) * (mi / denom);
+    float mi = m[i], vi = v[i];
+    const float pi = adam_update(p[i], g[i], mi, vi, lr, b1, b2, eps, bc1, bc2_sqrt);
     m[i] = mi;
     v[i] = vi;
     p[i] = pi;

@@ -866,8 +866,19 @@ __device__ __forceinline__ int64_t chunk_intervals(int64_t H, int64_t S)
     return c < kMinChunk ? kMinChunk : (c > kMaxChunk ? kMaxChunk : c);
 }
 
+// Adam on the node grids, applied by the adjoint itself (atdf_project_bwd_adam).
+struct GridAdam {
+    float* p;      // f32 grids (BF16 field: the master copy)
+    uint16_t* pb;  // BF16 grid copy or null
+    float* m;
+    float* v;
+    float lr, b1, b2, eps, bc1, bc2s;
+    int enabled;
+};
+
 struct NodeBwdArgs {
     ProjArgs a;
+    GridAdam ad;
     const float* dLdP;
     const int32_t* sorted;    // interval ids, grouped by grid slot
     const int4* chunks;       // (slot, begin, end, split)
@@ -1152,6 +1163,54 @@ __global__ __launch_bounds__(64 * kWaves) void project_bwd_node_kernel(NodeBwdAr
         float4* lg4 = reinterpret_cast<float4*>(lgrad);
         const int n4 = gsz >> 2;
         const float4 z4 = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (b.ad.enabled && !ch.w) {
+            // the chunk holds all of this node's samples: its gradient is
+            // complete, so apply Adam here instead of going through grad_grid
+            const GridAdam& ad = b.ad;
+            float4* p4 = reinterpret_cast<float4*>(ad.p + gbase);
+            float4* m4 = reinterpret_cast<float4*>(ad.m + gbase);
+            float4* v4 = reinterpret_cast<float4*>(ad.v + gbase);
+            constexpr int kA = 8;
+            for (int i0 = threadIdx.x; i0 < n4; i0 += kA * blockDim.x) {
+                float4 gv[kA], pv[kA], mv[kA], vv[kA];
+#pragma unroll
+                for (int u = 0; u < kA; ++u) {
+                    const int i = i0 + u * blockDim.x;
+                    if (i < n4) {
+                        gv[u] = lg4[i];
+                        lg4[i] = z4;
+                        pv[u] = p4[i];
+                        mv[u] = m4[i];
+                        vv[u] = v4[i];
+                    }
+                }
+#pragma unroll
+                for (int u = 0; u < kA; ++u) {
+                    const int i = i0 + u * blockDim.x;
+                    if (i >= n4) continue;
+                    float* gp = &gv[u].x;
+                    float* pp = &pv[u].x;
+                    float* mp = &mv[u].x;
+                    float* vp = &vv[u].x;
+#pragma unroll
+                    for (int k = 0; k < 4; ++k)
+                        pp[k] = adam_update(pp[k], gp[k], mp[k], vp[k], ad.lr, ad.b1, ad.b2, ad.eps, ad.bc1, ad.bc2s);
+                    p4[i] = pv[u];
+                    m4[i] = mv[u];
+                    v4[i] = vv[u];
+                    if (ad.pb) {
+                        ushort4 h;
+                        h.x = f32_to_bf16_rne(pp[0]);
+                        h.y = f32_to_bf16_rne(pp[1]);
+                        h.z = f32_to_bf16_rne(pp[2]);
+                        h.w = f32_to_bf16_rne(pp[3]);
+                        reinterpret_cast<ushort4*>(ad.pb + gbase)[i] = h;
+                    }
+                }
+            }
+            __syncthreads();
+            continue;
+        }
         for (int i0 = threadIdx.x; i0 < n4; i0 += kU * blockDim.x) {
             float4 v[kU], o[kU];
 #pragma unroll
@@ -1456,6 +1515,42 @@ __global__ __launch_bounds__(256) void reduce_partials_kernel(const float* __res
     }
 }
 
+// Adam for the grid slots the adjoint did not update itself: nodes split over
+// several chunks (gradient in grad_grid) and nodes without samples (zero
+// gradient).  Clears grad_grid of those slots.
+__global__ __launch_bounds__(256) void grid_adam_rest_kernel(GridAdam ad, float* __restrict__ grad,
+                                                             const int32_t* __restrict__ chunk_cnt, int blocks_per_slot,
+                                                             int n4)
+{
+    const int64_t slot = blockIdx.x / blocks_per_slot;
+    if (chunk_cnt[slot] == 1) return;
+    const int i = (int)(blockIdx.x % blocks_per_slot) * blockDim.x + threadIdx.x;
+    if (i >= n4) return;
+    const int64_t e = slot * (int64_t)n4 + i;
+    float4 gv = reinterpret_cast<float4*>(grad)[e];
+    float4 pv = reinterpret_cast<float4*>(ad.p)[e];
+    float4 mv = reinterpret_cast<float4*>(ad.m)[e];
+    float4 vv = reinterpret_cast<float4*>(ad.v)[e];
+    float* gp = &gv.x;
+    float* pp = &pv.x;
+    float* mp = &mv.x;
+    float* vp = &vv.x;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) pp[k] = adam_update(pp[k], gp[k], mp[k], vp[k], ad.lr, ad.b1, ad.b2, ad.eps, ad.bc1, ad.bc2s);
+    reinterpret_cast<float4*>(ad.p)[e] = pv;
+    reinterpret_cast<float4*>(ad.m)[e] = mv;
+    reinterpret_cast<float4*>(ad.v)[e] = vv;
+    reinterpret_cast<float4*>(grad)[e] = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (ad.pb) {
+        ushort4 h;
+        h.x = f32_to_bf16_rne(pp[0]);
+        h.y = f32_to_bf16_rne(pp[1]);
+        h.z = f32_to_bf16_rne(pp[2]);
+        h.w = f32_to_bf16_rne(pp[3]);
+        reinterpret_cast<ushort4*>(ad.pb)[e] = h;
+    }
+}
+
 }  // namespace atdf
 
 using namespace atdf;
@@ -1662,13 +1757,35 @@ size_t atdf_project_bwd_workspace_bytes(int64_t R, int64_t H, const atdf_field* 
     return bwd_layout(H, f->num_grids, np).total;
 }
 
-atdf_status atdf_project_bwd(const void* packed_tree, int64_t num_nodes, const float* O, const float* D, int64_t R,
-                             const atdf_sampling* sp, const int64_t* iv_off, const atdf_intervals* iv,
-                             const atdf_field* f, const float* dLdP, float* grad_grid, float* grad_mlp,
-                             void* workspace, size_t workspace_bytes, atdf_stream_t stream)
+}  // extern "C"
+
+static atdf_status project_bwd_impl(const void* packed_tree, int64_t num_nodes, const float* O, const float* D,
+                                    int64_t R, const atdf_sampling* sp, const int64_t* iv_off,
+                                    const atdf_intervals* iv, const atdf_field* f, const float* dLdP,
+                                    float* grad_grid, float* grad_mlp, const atdf_grid_adam* adam, void* workspace,
+                                    size_t workspace_bytes, atdf_stream_t stream)
 {
     atdf_status st = check_project_args(packed_tree, num_nodes, O, D, R, sp, iv_off, iv, f);
     if (st) return st;
+    const int64_t n_grid = f->num_grids * (int64_t)f->G * f->G * f->G * f->C;
+    if (adam) {
+        ATDF_REQUIRE(adam->param && adam->exp_avg && adam->exp_avg_sq && adam->step >= 1,
+                     "atdf_project_bwd_adam: incomplete Adam state");
+        const bool bf16 = f->grid_dtype == ATDF_GRID_BF16;
+        ATDF_REQUIRE(bf16 ? (adam->param_bf16 && (const void*)adam->param_bf16 == f->grid)
+                          : (!adam->param_bf16 && (const void*)adam->param == f->grid),
+                     "atdf_project_bwd_adam: the field's grid must be adam->param (f32) or adam->param_bf16 (bf16)");
+        ATDF_REQUIRE(((uintptr_t)adam->param | (uintptr_t)adam->exp_avg | (uintptr_t)adam->exp_avg_sq |
+                      (uintptr_t)grad_grid) % 16 == 0,
+                     "atdf_project_bwd_adam: grid buffers must be 16-byte aligned");
+        if (R == 0) {  // no samples: every grid takes a zero-gradient step
+            return bf16 ? atdf_adam_step_bf16(adam->param, adam->param_bf16, grad_grid, adam->exp_avg,
+                                              adam->exp_avg_sq, n_grid, adam->lr, adam->beta1, adam->beta2,
+                                              adam->eps, adam->step, 1, stream)
+                        : atdf_adam_step(adam->param, grad_grid, adam->exp_avg, adam->exp_avg_sq, n_grid, adam->lr,
+                                         adam->beta1, adam->beta2, adam->eps, adam->step, 1, stream);
+        }
+    }
     if (R == 0) return ATDF_OK;
     ATDF_REQUIRE(dLdP && grad_grid && grad_mlp && workspace, "atdf_project_bwd: null buffer");
     ATDF_REQUIRE(iv->total >= 0 && iv->ray, "atdf_project_bwd: intervals need total and ray");
@@ -1743,6 +1860,21 @@ atdf_status atdf_project_bwd(const void* packed_tree, int64_t num_nodes, const f
     args.rec = rec;
     args.grad_grid = grad_grid;
     args.ws = partials;
+    args.ad = GridAdam{};
+    if (adam) {
+        args.ad.p = adam->param;
+        args.ad.pb = adam->param_bf16;
+        args.ad.m = adam->exp_avg;
+        args.ad.v = adam->exp_avg_sq;
+        args.ad.lr = adam->lr;
+        args.ad.b1 = adam->beta1;
+        args.ad.b2 = adam->beta2;
+        args.ad.eps = adam->eps;
+        // the bias corrections of atdf_adam_step
+        args.ad.bc1 = 1.0f - (float)pow((double)adam->beta1, (double)adam->step);
+        args.ad.bc2s = sqrtf(1.0f - (float)pow((double)adam->beta2, (double)adam->step));
+        args.ad.enabled = 1;
+    }
     const bool bf = f->grid_dtype == ATDF_GRID_BF16;
     const bool bd = f->decoder_dtype == ATDF_DECODER_BF16;
 #define ATDF_BWD(CC, BF, BD) st = launch_node_bwd<CC, BF, BD>(args, s)
@@ -1755,9 +1887,37 @@ atdf_status atdf_project_bwd(const void* packed_tree, int64_t num_nodes, const f
     }
 #undef ATDF_BWD
     if (st) return st;
+    if (adam) {
+        const int n4 = (int)(n_grid / NA / 4);
+        const int bps = (int)div_up(n4, 256);
+        hipLaunchKernelGGL(grid_adam_rest_kernel, dim3((unsigned)(NA * bps)), dim3(256), 0, s, args.ad, grad_grid,
+                           chunk_cnt, bps, n4);
+    }
     hipLaunchKernelGGL(reduce_partials_kernel, dim3((unsigned)div_up(np, kReduceParams)), dim3(256), 0, s, partials,
                        (int64_t)persistent_blocks() * kWaves, np, grad_mlp);
     return check_launch("atdf_project_bwd");
+}
+
+extern "C" {
+
+atdf_status atdf_project_bwd(const void* packed_tree, int64_t num_nodes, const float* O, const float* D, int64_t R,
+                             const atdf_sampling* sp, const int64_t* iv_off, const atdf_intervals* iv,
+                             const atdf_field* f, const float* dLdP, float* grad_grid, float* grad_mlp,
+                             void* workspace, size_t workspace_bytes, atdf_stream_t stream)
+{
+    return project_bwd_impl(packed_tree, num_nodes, O, D, R, sp, iv_off, iv, f, dLdP, grad_grid, grad_mlp, nullptr,
+                            workspace, workspace_bytes, stream);
+}
+
+atdf_status atdf_project_bwd_adam(const void* packed_tree, int64_t num_nodes, const float* O, const float* D,
+                                  int64_t R, const atdf_sampling* sp, const int64_t* iv_off, const atdf_intervals* iv,
+                                  const atdf_field* f, const float* dLdP, float* grad_grid, float* grad_mlp,
+                                  const atdf_grid_adam* adam, void* workspace, size_t workspace_bytes,
+                                  atdf_stream_t stream)
+{
+    ATDF_REQUIRE(adam, "atdf_project_bwd_adam: null Adam state");
+    return project_bwd_impl(packed_tree, num_nodes, O, D, R, sp, iv_off, iv, f, dLdP, grad_grid, grad_mlp, adam,
+                            workspace, workspace_bytes, stream);
 }
 
 

@@ -950,25 +950,25 @@ Tensor TrainStepIntervals(HyperTreeBase& tree, TrainState& st, const RayList& ra
                  "atdf_mse_loss");
     auto ws = torch::empty({(int64_t)atdf_project_bwd_workspace_bytes(R, iv.num_intervals, &f)},
                            torch::TensorOptions().dtype(torch::kUInt8).device(rays.origin.device()));
-    check_status(atdf_project_bwd(tree.packed(), tree.NumNodes(), rays.origin.data_ptr<float>(),
-                                  rays.direction.data_ptr<float>(), R, &s, iv.ray_offset.data_ptr<int64_t>(), &v, &f,
-                                  dP.data_ptr<float>(), st.grad_grid.data_ptr<float>(), st.grad_mlp.data_ptr<float>(),
-                                  ws.data_ptr(), ws.numel(), stream),
-                 "atdf_project_bwd");
     st.step += 1;
     const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
-    if (st.grid_master.defined()) {
-        check_status(atdf_adam_step_bf16(st.grid_master.data_ptr<float>(),
-                                         reinterpret_cast<uint16_t*>(st.grid.data_ptr()), st.grad_grid.data_ptr<float>(),
-                                         st.m_grid.data_ptr<float>(), st.v_grid.data_ptr<float>(), st.grid.numel(),
-                                         st.lr_grid, b1, b2, eps, st.step, 1, stream),
-                     "atdf_adam_step_bf16");
-    } else {
-        check_status(atdf_adam_step(st.grid.data_ptr<float>(), st.grad_grid.data_ptr<float>(),
-                                    st.m_grid.data_ptr<float>(), st.v_grid.data_ptr<float>(), st.grid.numel(),
-                                    st.lr_grid, b1, b2, eps, st.step, 1, stream),
-                     "atdf_adam_step");
-    }
+    // adjoint and the grids' Adam step in one pass (grad_grid stays zero)
+    atdf_grid_adam ga{};
+    const bool bf16 = st.grid_master.defined();
+    ga.param = bf16 ? st.grid_master.data_ptr<float>() : st.grid.data_ptr<float>();
+    ga.param_bf16 = bf16 ? reinterpret_cast<uint16_t*>(st.grid.data_ptr()) : nullptr;
+    ga.exp_avg = st.m_grid.data_ptr<float>();
+    ga.exp_avg_sq = st.v_grid.data_ptr<float>();
+    ga.lr = st.lr_grid;
+    ga.beta1 = b1;
+    ga.beta2 = b2;
+    ga.eps = eps;
+    ga.step = st.step;
+    check_status(atdf_project_bwd_adam(tree.packed(), tree.NumNodes(), rays.origin.data_ptr<float>(),
+                                       rays.direction.data_ptr<float>(), R, &s, iv.ray_offset.data_ptr<int64_t>(), &v,
+                                       &f, dP.data_ptr<float>(), st.grad_grid.data_ptr<float>(),
+                                       st.grad_mlp.data_ptr<float>(), &ga, ws.data_ptr(), ws.numel(), stream),
+                 "atdf_project_bwd_adam");
     check_status(atdf_adam_step(st.mlp.data_ptr<float>(), st.grad_mlp.data_ptr<float>(), st.m_mlp.data_ptr<float>(),
                                 st.v_mlp.data_ptr<float>(), st.mlp.numel(), st.lr_mlp, b1, b2, eps, st.step, 1, stream),
                  "atdf_adam_step");

@@ -286,11 +286,38 @@ atdf_status atdf_project_fwd(const void* packed_tree, int64_t num_nodes, const f
  * accumulated in LDS and written once (no per-sample global atomics).
  * intervals->total must hold the interval count. */
 size_t atdf_project_bwd_workspace_bytes(int64_t num_rays, int64_t num_intervals, const atdf_field* field);
+
 atdf_status atdf_project_bwd(const void* packed_tree, int64_t num_nodes, const float* ray_origin,
                              const float* ray_direction, int64_t num_rays, const atdf_sampling* sampling,
                              const int64_t* ray_interval_offset, const atdf_intervals* intervals,
                              const atdf_field* field, const float* dLdP, float* grad_grid, float* grad_mlp,
                              void* workspace, size_t workspace_bytes, atdf_stream_t stream);
+
+/* Adam state of the node grids for atdf_project_bwd_adam. */
+typedef struct {
+    float* param;          /* [num_grids * G^3 * C] f32 grids; the master copy when the field is BF16 */
+    uint16_t* param_bf16;  /* the field's BF16 grid (rounded from param), or null for an F32 field    */
+    float* exp_avg;        /* Adam m, same shape as param                                             */
+    float* exp_avg_sq;     /* Adam v                                                                  */
+    float lr, beta1, beta2, eps;
+    int64_t step;          /* 1-based step count of this update                                       */
+} atdf_grid_adam;
+
+/* atdf_project_bwd followed by one atdf_adam_step[_bf16] on the grids, fused:
+ * a node whose samples fit one chunk gets its Adam update straight from the
+ * LDS gradient at the end of its chunk (no gradient round trip through HBM);
+ * the other nodes (split over chunks, or without samples) are updated from
+ * grad_grid afterwards.  grad_grid must be all zero on entry and is all zero
+ * on return; grad_mlp += dL/dmlp (the decoder's Adam stays with the caller).
+ * The field's grid must be the memory adam->param (F32) or adam->param_bf16
+ * (BF16).  No reference counterpart: the fusion of the adjoint with
+ * OptimizerStep's grid update (geometry/geometry.h:203-225). */
+atdf_status atdf_project_bwd_adam(const void* packed_tree, int64_t num_nodes, const float* ray_origin,
+                                  const float* ray_direction, int64_t num_rays, const atdf_sampling* sampling,
+                                  const int64_t* ray_interval_offset, const atdf_intervals* intervals,
+                                  const atdf_field* field, const float* dLdP, float* grad_grid, float* grad_mlp,
+                                  const atdf_grid_adam* adam, void* workspace, size_t workspace_bytes,
+                                  atdf_stream_t stream);
 
 /* atdf_project_fwd plus the per-ray mass histogram ray_bins[r, b] (f32,
  * [num_rays, 32]): the sum of density * weight over the samples of ray r

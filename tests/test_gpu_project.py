@@ -277,3 +277,87 @@ def test_node_density_stats_match_oracle(oracle, cuda, bf16):
     print(f"bf16={bf16}: node max rel err {e_max:.2e}, node std rel err {e_sd:.2e}", flush=True)
     assert sd.dtype == np.float64 and np.all(sd[cs["act"] == 0] == 0)
     assert e_max <= (1e-3 if bf16 else 1e-5) and e_sd <= (1e-2 if bf16 else 1e-4)
+
+
+@pytest.mark.parametrize("bf16", [False, True])
+def test_fused_adjoint_adam_matches_separate(oracle, cuda, bf16):
+    """atdf_project_bwd_adam against atdf_project_bwd + atdf_adam_step[_bf16]
+    on a list where some nodes split over chunks (> 3072 samples), some fit
+    one chunk and most see no sample: the same parameters, Adam moments and
+    BF16 copy (to the rounding of the LDS accumulation order), grad_grid
+    left at zero, the same decoder gradient."""
+    from adaptivetensordf_amd import capi, phantom
+
+    rng = np.random.default_rng(7)
+    t = oracle.tree_build(3)
+    act = uniform_active(t, 3)
+    tree = device_tree(3, act)
+    n = 20000
+    u = rng.random((n, 2)) ** 3  # rays crowd the y = z = -1 edge: node sample counts span 50x
+    o = np.stack([np.full(n, -3.0), -1.0 + u[:, 0], -1.0 + u[:, 1]], 1).astype(np.float32)
+    d = np.tile(np.array([[1.0, 0.0, 0.0]], np.float32), (n, 1))
+    o_t, d_t = rays_to_device(o, d)
+    from adaptivetensordf_amd import native
+
+    sp_n = native().SamplingParams()
+    sp_n.max_samples_per_node = 8
+    sp_n.jitter_mode = 2
+    sp_n.jitter_seed = 5
+    iv = tree.CreateIntervalsForRays(native().RayList(o_t, d_t), sp_n)
+    counts = torch.bincount(iv.node.long(), weights=iv.count.double(), minlength=t["depth"].size).cpu().numpy()
+    used = counts[act == 1]
+    print(f"{iv.num_samples} samples; nodes with samples: {(used > 0).sum()} of {used.size}, max {used.max():.0f}, "
+          f"min>0 {used[used > 0].min():.0f}", flush=True)
+    assert (used > 3072 * 2).any() and ((used > 0) & (used < 3072)).any() and (used == 0).any()
+    na = int(act.sum())
+    C_ = 16
+    master = (0.3 * torch.randn(na, 11, 11, 11, C_, generator=torch.Generator().manual_seed(1))).cuda()
+    mlp = phantom.init_mlp(C_, 64, seed=3).cuda()
+    m0 = (0.01 * torch.randn_like(master)).abs()
+    v0 = (1e-4 * torch.rand_like(master))
+    dP = torch.from_numpy(rng.normal(size=n).astype(np.float32) * 1e-2).cuda()
+    sp = capi.Sampling(8, 256, 2, 0, 5, None, None, None)
+    civ = capi.Intervals(iv.node.data_ptr(), iv.tmin.data_ptr(), iv.tmax.data_ptr(), iv.count.data_ptr(),
+                         iv.first.data_ptr(), iv.ray.data_ptr(), iv.num_intervals)
+    s = capi.stream_ptr()
+    results = []
+    for fused in (False, True):
+        p, m, v = master.clone(), m0.clone(), v0.clone()
+        pb = p.to(torch.bfloat16) if bf16 else None
+        grid = pb if bf16 else p
+        f = capi.Field(C_, 64, 11, capi.GRID_BF16 if bf16 else capi.GRID_F32, 2.0, 0, grid.data_ptr(), na,
+                       tree.node_to_grid.data_ptr(), mlp.data_ptr(), capi.DECODER_BF16 if bf16 else capi.DECODER_F32,
+                       0)
+        gg = torch.zeros_like(master)
+        gm = torch.zeros_like(mlp)
+        ws = torch.empty(capi.lib().atdf_project_bwd_workspace_bytes(n, iv.num_intervals, C.byref(f)),
+                         dtype=torch.uint8, device="cuda")
+        args = (capi.ptr(tree.PackedTree()), tree.NumNodes(), capi.ptr(o_t), capi.ptr(d_t), n, C.byref(sp),
+                capi.ptr(iv.ray_offset), C.byref(civ), C.byref(f), capi.ptr(dP), capi.ptr(gg), capi.ptr(gm))
+        if fused:
+            ad = capi.GridAdam(p.data_ptr(), pb.data_ptr() if bf16 else None, m.data_ptr(), v.data_ptr(), 1e-2, 0.9,
+                               0.999, 1e-8, 3)
+            capi.call("atdf_project_bwd_adam", *args, C.byref(ad), capi.ptr(ws), ws.numel(), s)
+            assert float(gg.abs().max()) == 0.0
+        else:
+            capi.call("atdf_project_bwd", *args, capi.ptr(ws), ws.numel(), s)
+            if bf16:
+                capi.call("atdf_adam_step_bf16", capi.ptr(p), capi.ptr(pb), capi.ptr(gg), capi.ptr(m), capi.ptr(v),
+                          p.numel(), C.c_float(1e-2), C.c_float(0.9), C.c_float(0.999), C.c_float(1e-8), 3, 1, s)
+            else:
+                capi.call("atdf_adam_step", capi.ptr(p), capi.ptr(gg), capi.ptr(m), capi.ptr(v), p.numel(),
+                          C.c_float(1e-2), C.c_float(0.9), C.c_float(0.999), C.c_float(1e-8), 3, 1, s)
+        torch.cuda.synchronize()
+        results.append((p, m, v, pb, gm))
+    (p0, m0_, v0_, pb0, gm0), (p1, m1, v1, pb1, gm1) = results
+    moved = (p0 - master).abs().max().item()
+    dp = (p1 - p0).abs().max().item()
+    dm = ((m1 - m0_).abs().max() / m0_.abs().max()).item()
+    dv = ((v1 - v0_).abs().max() / v0_.abs().max()).item()
+    dg = ((gm1 - gm0).abs().max() / gm0.abs().max()).item()
+    print(f"bf16={bf16}: max update {moved:.3e}, |dp| {dp:.2e}, m rel {dm:.2e}, v rel {dv:.2e}, grad_mlp rel {dg:.2e}",
+          flush=True)
+    assert moved > 1e-3
+    assert dp <= 1e-5 * moved and dm <= 1e-5 and dv <= 1e-5 and dg <= 1e-5
+    if bf16:
+        assert torch.equal(pb1, p1.to(torch.bfloat16))
