@@ -1170,7 +1170,7 @@ __global__ __launch_bounds__(64 * kWaves) void project_bwd_node_kernel(NodeBwdAr
             float4* p4 = reinterpret_cast<float4*>(ad.p + gbase);
             float4* m4 = reinterpret_cast<float4*>(ad.m + gbase);
             float4* v4 = reinterpret_cast<float4*>(ad.v + gbase);
-            constexpr int kA = 8;
+            constexpr int kA = 4;
             for (int i0 = threadIdx.x; i0 < n4; i0 += kA * blockDim.x) {
                 float4 gv[kA], pv[kA], mv[kA], vv[kA];
 #pragma unroll

@@ -4,10 +4,11 @@
 // with the KL fallback gate, and the merge of learned and stratified
 // intervals into one list for the fused projection kernels.
 //
-// Inference runs one ray per lane in FP32 VALU with the weights in LDS
-// (broadcast reads) and a per-wave activation buffer; every sum runs over the
-// input index in increasing order with no contraction, so logits, KL,
-// fallback decisions and placements match the C restatement bit for bit.
+// Inference runs one ray per lane in FP32 VALU with the weights read as
+// wave-uniform scalar loads (SGPR operands) and a per-wave activation buffer
+// in LDS; every sum runs over the input index in increasing order with no
+// contraction, so logits, KL, fallback decisions and placements match the C
+// restatement bit for bit.  (Training keeps the weights in LDS.)
 // The network is tiny (25 kFLOP per ray) next to the 32 decoder evaluations
 // it replaces 96 of; the kernels are bound by the point-location gathers,
 // not by arithmetic.
@@ -171,11 +172,44 @@ __device__ __forceinline__ void dense_lane(const float* W, const float* b, const
     }
 }
 
+// Same as dense_lane with the weights read from the packed global parameters
+// (rows of NIN floats): the addresses are wave-uniform, so they become scalar
+// loads, and the LDS holds only the activations.
+template <int NIN, int NOUT, bool RELU>
+__device__ __forceinline__ void dense_lane_s(const float* __restrict__ W, const float* __restrict__ b, const float* x,
+                                             float* buf, int lane)
+{
+    for (int o = 0; o < NOUT; ++o) {
+        const float* wr = W + o * NIN;
+        float acc = 0.f;
+#pragma unroll
+        for (int k = 0; k < NIN; ++k) acc = acc + wr[k] * x[k];
+        acc = acc + b[o];
+        buf[o * 64 + lane] = RELU ? (acc > 0.f ? acc : 0.f) : acc;
+    }
+}
+
 template <int N>
 __device__ __forceinline__ void load_column(const float* buf, int lane, float* v)
 {
 #pragma unroll
     for (int k = 0; k < N; ++k) v[k] = buf[k * 64 + lane];
+}
+
+// Logits z[32] of one ray from the global parameters P (buf = this wave's
+// activation buffer).
+__device__ __forceinline__ void mlp_forward_s(const float* __restrict__ P, float* buf, int lane, const float* x,
+                                              float* z)
+{
+    float h[NH];
+    dense_lane_s<NI, NH, true>(P + OW1, P + OB1, x, buf, lane);
+    load_column<NH>(buf, lane, h);
+    dense_lane_s<NH, NH, true>(P + OW2, P + OB2, h, buf, lane);
+    load_column<NH>(buf, lane, h);
+    dense_lane_s<NH, NH, true>(P + OW3, P + OB3, h, buf, lane);
+    load_column<NH>(buf, lane, h);
+    dense_lane_s<NH, NB, false>(P + OW4, P + OB4, h, buf, lane);
+    load_column<NB>(buf, lane, z);
 }
 
 // Logits z[32] of one ray (s = LDS parameters, buf = this wave's buffer).
@@ -287,13 +321,10 @@ __global__ __launch_bounds__(256) void features_kernel(PropArgs a)
 
 // One ray per lane: features, MLP, pdf, KL gate, systematic allocation and
 // the one-sample intervals of a learned ray.
-__global__ __launch_bounds__(kThreads) void sample_kernel(PropArgs a)
+__global__ __launch_bounds__(kThreads) void sample_kernel(PropArgs a, const float* __restrict__ params)
 {
     extern __shared__ __attribute__((aligned(16))) float sm[];
-    float* s = sm;
-    float* buf = sm + LTOTAL + (threadIdx.x >> 6) * kActFloats;
-    load_params(a.params, s);
-    __syncthreads();
+    float* buf = sm + (threadIdx.x >> 6) * kActFloats;
     const int lane = threadIdx.x & 63;
     const int64_t r = blockIdx.x * (int64_t)kThreads + threadIdx.x;
     if (r >= a.R) return;  // no barrier below
@@ -306,7 +337,7 @@ __global__ __launch_bounds__(kThreads) void sample_kernel(PropArgs a)
     }
     const bool ok = ray_features(root, a.table, a.depth, a.coarse, o, d, x, t0, t1);
     float z[NB];
-    mlp_forward(s, buf, lane, x, z);
+    mlp_forward_s(params, buf, lane, x, z);
     a.fallback[r] = 0;
     a.n_learned[r] = 0;
     if (a.kl) a.kl[r] = 0.f;
@@ -667,6 +698,7 @@ int max_lds_bytes()
 }
 
 constexpr size_t kMlpLds = sizeof(float) * ((size_t)LTOTAL + (size_t)kWaves * kActFloats);
+constexpr size_t kSampleLds = sizeof(float) * (size_t)kWaves * kActFloats;
 
 atdf_status check_proposal(const void* packed_tree, int64_t num_nodes, const atdf_proposal* p, const char* who)
 {
@@ -751,8 +783,8 @@ atdf_status atdf_proposal_sample(const void* packed_tree, int64_t num_nodes, con
     ATDF_REQUIRE(ray_origin && ray_direction && ray_fallback && ray_num_learned && learned_node && learned_tmin &&
                      learned_tmax,
                  "atdf_proposal_sample: null buffer");
-    ATDF_REQUIRE((int)kMlpLds <= max_lds_bytes(), "atdf_proposal_sample: needs %d bytes of LDS per workgroup",
-                 (int)kMlpLds);
+    ATDF_REQUIRE((int)kSampleLds <= max_lds_bytes(), "atdf_proposal_sample: needs %d bytes of LDS per workgroup",
+                 (int)kSampleLds);
     PropArgs a = make_args(packed_tree, ray_origin, ray_direction, num_rays, proposal);
     a.fallback = ray_fallback;
     a.n_learned = ray_num_learned;
@@ -760,8 +792,8 @@ atdf_status atdf_proposal_sample(const void* packed_tree, int64_t num_nodes, con
     a.ltmin = learned_tmin;
     a.ltmax = learned_tmax;
     a.kl = ray_kl;
-    hipLaunchKernelGGL(sample_kernel, dim3((unsigned)div_up(num_rays, kThreads)), dim3(kThreads), kMlpLds,
-                       to_hip(stream), a);
+    hipLaunchKernelGGL(sample_kernel, dim3((unsigned)div_up(num_rays, kThreads)), dim3(kThreads), kSampleLds,
+                       to_hip(stream), a, proposal->params);
     return check_launch("atdf_proposal_sample");
 }
 
