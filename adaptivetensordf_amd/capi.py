@@ -48,6 +48,11 @@ class GridAdam(C.Structure):
                 ("eps", C.c_float), ("step", C.c_int64)]
 
 
+class Lattice(C.Structure):
+    _fields_ = [("size", C.c_int32 * 3), ("mode", C.c_int32), ("swap_xy", C.c_int32), ("roi_min", C.c_float * 3),
+                ("step", C.c_float * 3)]
+
+
 class Intervals(C.Structure):
     _fields_ = [("node", C.c_void_p), ("tmin", C.c_void_p), ("tmax", C.c_void_p), ("count", C.c_void_p),
                 ("first", C.c_void_p), ("ray", C.c_void_p), ("total", C.c_int64)]
@@ -119,6 +124,7 @@ _SIGNATURES = {
     "atdf_proposal_write": (_I32, [_I64, _P, _P, _P, _P, _P, _P, C.POINTER(Intervals), _P]),
     "atdf_proposal_grad_workspace_bytes": (_SZ, [_I64]),
     "atdf_proposal_grad": (_I32, [_P, _P, _P, _I64, _F, _P, _P, _P, _SZ, _P]),
+    "atdf_sample_volume": (_I32, [_P, _I64, _P, C.POINTER(Field), C.POINTER(Lattice), _P, _P, _P, _P, _P]),
     "atdf_compact_rays": (_I32, [_P, _I64, _P, _P, _P]),
     "atdf_node_id_for_position": (_I32, [_P, _I64, _P, _P, _I64, _P, _P, _P]),
     "atdf_grid_transfer": (_I32, [_P, _I64, _P, _P, _I64, _P, _I64, _P, _P, _I64, _I32, _I32, _I32, _P]),

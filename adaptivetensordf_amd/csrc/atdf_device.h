@@ -205,6 +205,30 @@ __device__ __forceinline__ float bf16_to_f32(uint16_t h)
     return __uint_as_float(((uint32_t)h) << 16);
 }
 
+// Active node holding p, or -1: from the root, step into the first child (in
+// child order) whose closed box holds p (ActiveNodeIdForGlobalPosition,
+// structure/HyperTree.cu:1339-1399).
+__device__ __forceinline__ int32_t active_node_for_position(const PNode* __restrict__ nodes, const float* p)
+{
+    int32_t node = 0;
+    while (true) {
+        const PNode nd = nodes[node];
+        if (nd.flags & kFlagActive) return node;
+        if (nd.first_child < 0) return -1;
+        int32_t next = -1;
+        for (int c = 0; c < 8; ++c) {
+            const PNode ch = nodes[nd.first_child + c];
+            if (ch.mn[0] <= p[0] && ch.mx[0] >= p[0] && ch.mn[1] <= p[1] && ch.mx[1] >= p[1] && ch.mn[2] <= p[2] &&
+                ch.mx[2] >= p[2]) {
+                next = nd.first_child + c;
+                break;
+            }
+        }
+        if (next < 0) return -1;
+        node = next;
+    }
+}
+
 // One torch::optim::Adam update of a parameter (bias corrections bc1 and
 // sqrt(bc2) precomputed); m and v are updated in place, returns the new value.
 __device__ __forceinline__ float adam_update(float p, float g, float& m, float& v, float lr, float b1, float b2,

@@ -1490,6 +1490,131 @@ __global__ __launch_bounds__(64 * kWaves) void node_stats_kernel(StatsArgs sa)
     }
 }
 
+
+// ---------------------------------------------------------------------------
+// Field on a regular lattice (volume export): a wave takes tiles of 16
+// consecutive lattice points, locates each one in the active set, gathers
+// and decodes as the projection does, and writes density * mask.
+// ---------------------------------------------------------------------------
+struct VolumeArgs {
+    ProjArgs a;
+    const int64_t* active_ids;
+    int mode, swap;
+    int size[3];
+    float roi_min[3], step[3];
+    int64_t total;
+    float* density;
+    int64_t* node_id;
+    float* valid;
+    float* position;
+};
+
+// Lattice point p: index i (axis 0 fastest, or axis 2 with swap) and position
+// (mode 0: i / (size - 1) * 2 - 1; mode 1: roi_min + step * i).
+__device__ __forceinline__ void volume_point(const VolumeArgs& v, int64_t p, float* pos)
+{
+    int c[3];
+    int64_t tmp = p;
+    if (v.swap) {
+        for (int d = 2; d >= 0; --d) {
+            c[d] = (int)(tmp % v.size[d]);
+            tmp /= v.size[d];
+        }
+    } else {
+        for (int d = 0; d < 3; ++d) {
+            c[d] = (int)(tmp % v.size[d]);
+            tmp /= v.size[d];
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        if (v.mode == 0) {
+            const float q = (float)c[k] / ((float)v.size[k] - 1.0f);
+            pos[k] = q * 2 - 1.0f;
+        } else {
+            pos[k] = v.roi_min[k] + v.step[k] * (float)c[k];
+        }
+    }
+}
+
+__global__ __launch_bounds__(256) void volume_points_kernel(VolumeArgs v)
+{
+    const int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (p >= v.total) return;
+    float pos[3];
+    volume_point(v, p, pos);
+    const int32_t nd = active_node_for_position(v.a.nodes, pos);
+    if (v.position) {
+#pragma unroll
+        for (int k = 0; k < 3; ++k) v.position[p * 3 + k] = pos[k];
+    }
+    if (v.node_id) v.node_id[p] = nd < 0 ? v.active_ids[0] : (int64_t)nd;
+    if (v.valid) v.valid[p] = nd < 0 ? 0.f : 1.f;
+}
+
+template <int C, bool kBF16, bool kDecBF16>
+__global__ __launch_bounds__(64 * kWaves, 4) void volume_kernel(VolumeArgs v)
+{
+    using SM = DecoderSmem<C, kDecBF16>;
+    constexpr int CPL = C / 4;
+    __shared__ __attribute__((aligned(16))) float s[SM::kF32];
+    __shared__ __attribute__((aligned(16))) __bf16 sb[SM::kBF16];
+    const ProjArgs& a = v.a;
+    if constexpr (kDecBF16)
+        load_mlp_bf16_to_lds<C>(a.mlp, sb, s);
+    else
+        load_mlp_to_lds<C>(a.mlp, s);
+    __syncthreads();
+    const int lane = threadIdx.x & 63;
+    const int g = lane >> 4, j = lane & 15;
+    const int64_t wave = (int64_t)blockIdx.x * kWaves + (threadIdx.x >> 6);
+    const int64_t nwaves = (int64_t)gridDim.x * kWaves;
+    for (int64_t base = wave * kTile; base < v.total; base += nwaves * kTile) {
+        const int64_t p = base + j;
+        SampleCtx sc;
+        sc.valid = false;
+        sc.w = 1.f;
+        sc.gbase = -1;
+        int32_t nd = -1;
+        float pos[3] = {0.f, 0.f, 0.f};
+        if (p < v.total) {
+            volume_point(v, p, pos);
+            nd = active_node_for_position(a.nodes, pos);
+            if (nd >= 0) {
+                const PNode box = a.nodes[nd];
+                float loc[3];
+#pragma unroll
+                for (int k = 0; k < 3; ++k) loc[k] = ((pos[k] - box.mn[k]) / (box.mx[k] - box.mn[k])) * 2 - 1;
+                tri_setup(loc[0], loc[1], loc[2], a.G, a.G, a.G, sc.tri);
+                const int slot = a.node_to_grid[nd];
+                if (slot >= 0) {
+                    sc.gbase = (int64_t)slot * a.G * a.G * a.G * C;
+                    sc.valid = true;
+                }
+            }
+        }
+        float x[CPL];
+        gather_channels<C, kBF16>(a, sc, CPL * g, x);
+        v4f h1[4], h2[4];
+        float y;
+        int opq = 0;
+        asm volatile("" : "+v"(opq));  // no hoisting of the weight reads
+        if constexpr (kDecBF16)
+            y = decoder_fwd_bf16<C>(sb + opq, s + opq, x, g, j, h1, h2);
+        else
+            y = decoder_fwd<C>(s + opq, x, g, j, h1, h2);
+        if (g == 0 && p < v.total) {
+            v.density[p] = sc.valid ? softplus_f(y, a.beta) : 0.f;
+            if (v.node_id) v.node_id[p] = nd < 0 ? v.active_ids[0] : (int64_t)nd;
+            if (v.valid) v.valid[p] = nd < 0 ? 0.f : 1.f;
+            if (v.position) {
+#pragma unroll
+                for (int k = 0; k < 3; ++k) v.position[p * 3 + k] = pos[k];
+            }
+        }
+    }
+}
+
 // grad_mlp[i] += sum_w ws[w][i], summed in wave order.
 // grad[i] += sum over the partial rows, in a fixed order: a workgroup owns 16
 // parameters, 16 row groups (rows w = g, g + 16, ...) sum in registers, and
@@ -1920,6 +2045,60 @@ atdf_status atdf_project_bwd_adam(const void* packed_tree, int64_t num_nodes, co
                             workspace, workspace_bytes, stream);
 }
 
+
+
+atdf_status atdf_sample_volume(const void* packed_tree, int64_t num_nodes, const int64_t* active_node_ids,
+                               const atdf_field* f, const atdf_lattice* lattice, float* density, int64_t* node_id,
+                               float* valid, float* position, atdf_stream_t stream)
+{
+    ATDF_REQUIRE(packed_tree && num_nodes > 0 && lattice && active_node_ids, "atdf_sample_volume: null argument");
+    ATDF_REQUIRE(lattice->mode == 0 || lattice->mode == 1, "atdf_sample_volume: mode must be 0 or 1");
+    int64_t total = 1;
+    for (int k = 0; k < 3; ++k) {
+        ATDF_REQUIRE(lattice->size[k] >= (lattice->mode == 0 ? 2 : 1), "atdf_sample_volume: bad size[%d] = %d", k,
+                     lattice->size[k]);
+        total *= lattice->size[k];
+    }
+    VolumeArgs v{};
+    v.active_ids = active_node_ids;
+    v.mode = lattice->mode;
+    v.swap = lattice->swap_xy != 0;
+    for (int k = 0; k < 3; ++k) {
+        v.size[k] = lattice->size[k];
+        v.roi_min[k] = lattice->roi_min[k];
+        v.step[k] = lattice->step[k];
+    }
+    v.total = total;
+    v.density = density;
+    v.node_id = node_id;
+    v.valid = valid;
+    v.position = position;
+    hipStream_t s = to_hip(stream);
+    if (!density) {  // positions / point location only
+        v.a.nodes = reinterpret_cast<const PNode*>(packed_tree);
+        hipLaunchKernelGGL(volume_points_kernel, dim3((unsigned)div_up(total, 256)), dim3(256), 0, s, v);
+        return check_launch("atdf_sample_volume");
+    }
+    ATDF_REQUIRE(f && f->H == 64 && (f->C == 16 || f->C == 8), "atdf_sample_volume: field shape not built");
+    ATDF_REQUIRE(f->grid && f->node_to_grid && f->mlp, "atdf_sample_volume: incomplete field");
+    atdf_sampling sp{};
+    atdf_intervals none{};
+    v.a = make_args(packed_tree, nullptr, nullptr, 0, &sp, nullptr, &none, f);
+    const int64_t tiles = div_up(total, kTile);
+    const unsigned blocks = (unsigned)std::min<int64_t>(div_up(tiles, kWaves), 65536);
+    const bool bf = f->grid_dtype == ATDF_GRID_BF16;
+    const bool bd = f->decoder_dtype == ATDF_DECODER_BF16;
+#define ATDF_VOL(CC, BF, BD) hipLaunchKernelGGL((volume_kernel<CC, BF, BD>), dim3(blocks), dim3(64 * kWaves), 0, s, v)
+    if (f->C == 16) {
+        if (bf) { if (bd) ATDF_VOL(16, true, true); else ATDF_VOL(16, true, false); }
+        else    { if (bd) ATDF_VOL(16, false, true); else ATDF_VOL(16, false, false); }
+    } else {
+        if (bf) { if (bd) ATDF_VOL(8, true, true); else ATDF_VOL(8, true, false); }
+        else    { if (bd) ATDF_VOL(8, false, true); else ATDF_VOL(8, false, false); }
+    }
+#undef ATDF_VOL
+    return check_launch("atdf_sample_volume");
+}
 
 atdf_status atdf_node_density_stats(const void* packed_tree, int64_t num_nodes, const int64_t* active_ids,
                                     int64_t num_active, const atdf_field* f, int32_t mode, int32_t n, int32_t rounds,

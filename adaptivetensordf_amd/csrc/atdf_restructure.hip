@@ -25,28 +25,6 @@
 
 namespace atdf {
 
-// Active node holding p, or -1.
-__device__ __forceinline__ int32_t active_node_for_position(const PNode* __restrict__ nodes, const float* p)
-{
-    int32_t node = 0;
-    while (true) {
-        const PNode nd = nodes[node];
-        if (nd.flags & kFlagActive) return node;
-        if (nd.first_child < 0) return -1;
-        int32_t next = -1;
-        for (int c = 0; c < 8; ++c) {
-            const PNode ch = nodes[nd.first_child + c];
-            if (ch.mn[0] <= p[0] && ch.mx[0] >= p[0] && ch.mn[1] <= p[1] && ch.mx[1] >= p[1] && ch.mn[2] <= p[2] &&
-                ch.mx[2] >= p[2]) {
-                next = nd.first_child + c;
-                break;
-            }
-        }
-        if (next < 0) return -1;
-        node = next;
-    }
-}
-
 // ComputeNodeId (structure/HyperTree.cu:1466-1485): a position outside every
 // active node gets the first active node and mask 0.
 __global__ __launch_bounds__(256) void node_id_kernel(const PNode* __restrict__ nodes,

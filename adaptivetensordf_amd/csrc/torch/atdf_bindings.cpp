@@ -75,6 +75,8 @@ PYBIND11_MODULE(TORCH_EXTENSION_NAME, m)
         .def("LeafTable", &HyperTreeBase::LeafTable)
         .def("RoiSplit", &HyperTreeBase::RoiSplit)
         .def("NodeIdForPositionGPU", &HyperTreeBase::NodeIdForPositionGPU)
+        .def("UniformPhantomSamplesGPU", &HyperTreeBase::UniformPhantomSamplesGPU)
+        .def("UniformPhantomSamplesGPUbySlice", &HyperTreeBase::UniformPhantomSamplesGPUbySlice)
         .def("InterpolateGridForInactiveNodes", &HyperTreeBase::InterpolateGridForInactiveNodes)
         .def("PackedTree", &HyperTreeBase::PackedTree)
         .def_readwrite("node_parent", &HyperTreeBase::node_parent)
@@ -132,6 +134,10 @@ PYBIND11_MODULE(TORCH_EXTENSION_NAME, m)
         return std::make_pair(changed, d);
     });
     m.def("NodeCulling", &NodeCulling);
+    m.def("UniformSampledVolume", &UniformSampledVolume);
+    m.def("SampleVolumeBySlice", &SampleVolumeBySlice);
+    m.def("SaveHDRImageTensor", &SaveHDRImageTensor);
+    m.def("LoadHDRImageTensor", &LoadHDRImageTensor);
     m.def(
         "TransferNodeGrids",
         [](const HyperTreeBase& tree, torch::Tensor old_packed, torch::Tensor old_node_to_grid, torch::Tensor old_grid,

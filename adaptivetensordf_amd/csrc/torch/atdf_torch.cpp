@@ -738,7 +738,6 @@ std::pair<Tensor, Tensor> AccumulateSampleLossPerNode(const SampleList& samples,
 // ---------------------------------------------------------------------------
 // fused projection
 // ---------------------------------------------------------------------------
-namespace {
 atdf_field make_field(const HyperTreeBase& tree, const Tensor& grid, const Tensor& mlp, const FieldParams& fp)
 {
     TORCH_CHECK(grid.is_cuda() && grid.is_contiguous(), "grid must be a contiguous GPU tensor");
@@ -766,6 +765,8 @@ atdf_field make_field(const HyperTreeBase& tree, const Tensor& grid, const Tenso
     f.mlp = mlp.data_ptr<float>();
     return f;
 }
+
+namespace {
 
 struct ProjectRaysFn : public torch::autograd::Function<ProjectRaysFn> {
     static Tensor forward(torch::autograd::AutogradContext* ctx, Tensor grid, Tensor mlp, Tensor origin,

@@ -110,6 +110,13 @@ class HyperTreeBase {
     // active_grid [num_nodes, C, G, G, G] whose inactive nodes take the
     // nearest lattice values of the active nodes
     torch::Tensor InterpolateGridForInactiveNodes(torch::Tensor active_grid) const;
+    // UniformPhantomSamplesGPU / UniformPhantomSamplesGPUbySlice (HyperTree.cu:
+    // 1158-1293): lattice points (axis 0 fastest, axis 2 with swap_xy) as a
+    // SampleList: global_coordinate, ray_index = point index, node_id and
+    // weight = NodeIdForPositionGPU's (node, mask)
+    SampleList UniformPhantomSamplesGPU(std::vector<int> size, bool swap_xy) const;
+    SampleList UniformPhantomSamplesGPUbySlice(std::vector<int> size, bool swap_xy, std::vector<float> roi_min,
+                                               std::vector<float> step_size) const;
     // packed traversal form of the current active set (replaced, not
     // modified, on every change: holding it keeps a snapshot)
     torch::Tensor PackedTree() const { return packed_; }
@@ -212,11 +219,33 @@ struct FieldParams {
     bool bf16_decoder = false;  // BF16 MFMA inputs, FP32 accumulation
 };
 
+// C-ABI view of a field (checks shapes and dtypes).
+atdf_field make_field(const HyperTreeBase& tree, const torch::Tensor& grid, const torch::Tensor& mlp,
+                      const FieldParams& fp);
+
 // Fused projection with hand-written adjoint (torch::autograd::Function).
 // grid: [num_grids, G, G, G, C] f32 (or bf16 for inference-only forward),
 // mlp: flat decoder parameters.  Returns the projection [R].
 torch::Tensor ProjectRays(HyperTreeBase& tree, const RayList& rays, torch::Tensor grid, torch::Tensor mlp,
                           const SamplingParams& sp, const FieldParams& fp);
+
+// HierarchicalNeuralGeometry::UniformSampledVolume (geometry/geometry.cpp:
+// 347-660, non-final path) for the fused field: density on the shape[0] x
+// shape[1] x shape[2] lattice over the unit cube, returned as
+// (volume [1, shape...] f32 holding point i at flat index i (axis 0
+// fastest), node_id [shape...] i64, valid [shape...] i64).
+std::tuple<torch::Tensor, torch::Tensor, torch::Tensor> UniformSampledVolume(HyperTreeBase& tree, torch::Tensor grid,
+                                                                             torch::Tensor mlp, const FieldParams& fp,
+                                                                             std::vector<int> shape);
+// The same on the lattice roi_min + step * i (the by-slice sampler).
+torch::Tensor SampleVolumeBySlice(HyperTreeBase& tree, torch::Tensor grid, torch::Tensor mlp, const FieldParams& fp,
+                                  std::vector<int> shape, std::vector<float> roi_min, std::vector<float> step);
+// SaveHDRImageTensor / LoadHDRImageTensor (utils/cimg_wrapper.cpp:12-52):
+// a 4-D f32 volume [s0, s1, s2, s3] as an Analyze 7.5 pair (path ".hdr"
+// header, ".img" raw little-endian float32, s3 fastest; dim = {4, s3, s2,
+// s1, s0}); the loader returns [1, depth, height, width] of the first frame.
+void SaveHDRImageTensor(torch::Tensor volume, const std::string& path);
+torch::Tensor LoadHDRImageTensor(const std::string& path);
 
 // ComputeMaxDensity (samples/hyper_train.cpp:662-707): node_max_density of the
 // active nodes = max density on an n^3 lattice per node (UniformGlobalSamples).

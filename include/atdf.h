@@ -344,6 +344,34 @@ atdf_status atdf_node_density_stats(const void* packed_tree, int64_t num_nodes, 
                                     int32_t rounds, uint64_t seed, float* node_max, double* node_sum,
                                     double* node_sumsq, atdf_stream_t stream);
 
+/* Regular lattice of points for volume export.  Point i = (i0, i1, i2) with
+ * i0 fastest (swap_xy: i2 fastest), index i0 + size0 (i1 + size1 i2):
+ *   mode 0: p = (i / (size - 1)) * 2 - 1 over the unit cube
+ *           (UniformPhantomSamplesGPU, structure/HyperTree.cu:1158-1219);
+ *   mode 1: p = roi_min + step * i
+ *           (UniformPhantomSamplesGPUbySlice, structure/HyperTree.cu:1221-1293).
+ * Host memory, like atdf_sampling. */
+typedef struct {
+    int32_t size[3];
+    int32_t mode;
+    int32_t swap_xy;
+    float roi_min[3];
+    float step[3];
+} atdf_lattice;
+
+/* The field on the lattice: point location as atdf_node_id_for_position
+ * (node_id: the active node, or active_node_ids[0]; valid: 1 / 0), then the
+ * projection's gather and decoder: density[i] = softplus(decoder) for points
+ * inside an active node and 0 outside (the reference multiplies by the
+ * mask).  Replaces the evaluation loop of
+ * HierarchicalNeuralGeometry::UniformSampledVolume (geometry/geometry.cpp:
+ * 347-660, non-final path).  Every output is nullable; with density null
+ * the field is not evaluated (positions / point location only, f may be
+ * null).  position: [n, 3] f32. */
+atdf_status atdf_sample_volume(const void* packed_tree, int64_t num_nodes, const int64_t* active_node_ids,
+                               const atdf_field* field, const atdf_lattice* lattice, float* density,
+                               int64_t* node_id, float* valid, float* position, atdf_stream_t stream);
+
 /* ======================================================================= */
 /* Learned sample proposal                                                  */
 /* ======================================================================= */

@@ -263,6 +263,10 @@ void atdfo_grid_transfer(const float* node_min, const float* node_max, const int
                          const int32_t* src_active, const int32_t* src_slot, const float* src_grid,
                          const int64_t* dst_nodes, int64_t n_dst, const int32_t* dst_slot, float* dst_grid, int C,
                          int G, int layout);
+/* Lattice points of volume export: mode 0 (i / (size - 1)) * 2 - 1, mode 1
+ * roi_min + step * i; axis 0 fastest (axis 2 with swap_xy); pos [n, 3]. */
+void atdfo_volume_points(int mode, int swap_xy, const int* size, const float* roi_min, const float* step,
+                         float* pos);
 
 #ifdef __cplusplus
 }

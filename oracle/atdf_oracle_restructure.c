@@ -103,3 +103,34 @@ void atdfo_grid_transfer(const float* node_min, const float* node_max, const int
                 }
     }
 }
+
+void atdfo_volume_points(int mode, int swap_xy, const int* size, const float* roi_min, const float* step,
+                         float* pos)
+{
+    /* UniformPhantomSamplesGPU / ...bySlice (structure/HyperTree.cu:1158-1293) */
+    int64_t n = (int64_t)size[0] * size[1] * size[2];
+    for (int64_t p = 0; p < n; ++p) {
+        int c[3];
+        int64_t tmp = p;
+        if (swap_xy) {
+            for (int d = 2; d >= 0; --d) {
+                c[d] = (int)(tmp % size[d]);
+                tmp /= size[d];
+            }
+        } else {
+            for (int d = 0; d < 3; ++d) {
+                c[d] = (int)(tmp % size[d]);
+                tmp /= size[d];
+            }
+        }
+        for (int k = 0; k < 3; ++k) {
+            if (mode == 0) {
+                float q = (float)c[k] / ((float)size[k] - 1.0f);
+                pos[p * 3 + k] = q * 2 - 1.0f;
+            } else {
+                float s = step[k] * (float)c[k];
+                pos[p * 3 + k] = roi_min[k] + s;
+            }
+        }
+    }
+}

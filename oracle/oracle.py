@@ -488,3 +488,12 @@ def grid_transfer(node_min, node_max, children, src_active, src_slot, src_grid, 
                               _p(dst_nodes), C.c_int64(dst_nodes.shape[0]), None if ds is None else _p(ds),
                               _p(dst_grid), C.c_int(Cc), C.c_int(G), C.c_int(layout))
     return dst_grid
+
+
+def volume_points(mode, size, roi_min=(0, 0, 0), step=(0, 0, 0), swap_xy=False):
+    size = np.asarray(size, np.int32)
+    n = int(np.prod(size.astype(np.int64)))
+    pos = np.zeros((n, 3), np.float32)
+    lib().atdfo_volume_points(C.c_int(mode), C.c_int(int(swap_xy)), _p(_c(size, np.int32)),
+                              _p(_c(roi_min, np.float32)), _p(_c(step, np.float32)), _p(pos))
+    return pos
