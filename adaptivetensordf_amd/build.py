@@ -55,7 +55,7 @@ def _torch_flags():
               "-Wno-deprecated-declarations"] + abi + [f"-I{p}" for p in inc]
     ldflags = [f"-L{LIB_DIR}", "-latdf", "-Wl,-rpath,$ORIGIN/lib", f"-L{torch_lib}", "-lc10", "-ltorch",
                "-ltorch_cpu", "-ltorch_python", "-lc10_hip", "-ltorch_hip", f"-Wl,-rpath,{torch_lib}",
-               f"-L{ROCM}/lib", "-lamdhip64"]
+               f"-L{ROCM}/lib", "-lamdhip64", "-lz"]
     return cflags, ldflags
 
 
@@ -63,7 +63,8 @@ def build_torch_layer() -> str:
     """Compile and link atdf_torch.so (always recompiles)."""
     cflags, ldflags = _torch_flags()
     os.makedirs(OBJ_DIR, exist_ok=True)
-    srcs = [os.path.join(HERE, "csrc", "torch", f) for f in ("atdf_torch.cpp", "atdf_tree_opt.cpp", "atdf_volume.cpp", "atdf_bindings.cpp")]
+    srcs = [os.path.join(HERE, "csrc", "torch", f) for f in ("atdf_torch.cpp", "atdf_tree_opt.cpp", "atdf_volume.cpp", "atdf_scene.cpp",
+                                                                    "atdf_bindings.cpp")]
     objs = [os.path.join(OBJ_DIR, os.path.basename(s) + ".o") for s in srcs]
     cxx = os.environ.get("CXX", "g++")
     with ThreadPoolExecutor(max_workers=2) as ex:

@@ -53,6 +53,17 @@ class Lattice(C.Structure):
                 ("step", C.c_float * 3)]
 
 
+class CameraRig(C.Structure):
+    _fields_ = [("camera_model", C.c_int32), ("w", C.c_int32), ("h", C.c_int32), ("align_corners", C.c_int32),
+                ("num_cameras", C.c_int64), ("num_images", C.c_int64), ("intrinsics", C.c_void_p),
+                ("rotation", C.c_void_p), ("translation", C.c_void_p), ("volume_translation", C.c_float * 3)]
+
+
+class ProjectionSet(C.Structure):
+    _fields_ = [("projection", C.c_void_p), ("mask", C.c_void_p), ("w", C.c_int32), ("h", C.c_int32),
+                ("num_images", C.c_int64), ("align_corners", C.c_int32)]
+
+
 class Intervals(C.Structure):
     _fields_ = [("node", C.c_void_p), ("tmin", C.c_void_p), ("tmax", C.c_void_p), ("count", C.c_void_p),
                 ("first", C.c_void_p), ("ray", C.c_void_p), ("total", C.c_int64)]
@@ -128,6 +139,11 @@ _SIGNATURES = {
     "atdf_compact_rays": (_I32, [_P, _I64, _P, _P, _P]),
     "atdf_node_id_for_position": (_I32, [_P, _I64, _P, _P, _I64, _P, _P, _P]),
     "atdf_grid_transfer": (_I32, [_P, _I64, _P, _P, _I64, _P, _I64, _P, _P, _I64, _I32, _I32, _I32, _P]),
+    "atdf_generate_rays": (_I32, [C.POINTER(CameraRig), _P, _P, _P, _I64, _P, _P, _P]),
+    "atdf_log_convert": (_I32, [_P, _P, _I64, C.c_double, C.c_double, C.c_double, _I32, _I32, _P, _P]),
+    "atdf_sample_projection": (_I32, [C.POINTER(ProjectionSet), _P, _P, _I64, _P, _P, _P]),
+    "atdf_random_ray_batch": (_I32, [C.POINTER(CameraRig), C.POINTER(ProjectionSet), _P, _I64, _P, _I32, _I32,
+                                     _I32, _I32, C.c_uint64, _I64, _P, _P, _P, _P, _P, _P, _P, _P]),
     "atdf_grid_layout": (_I32, [_P, _P, _I64, _I32, _I32, _I32, _P]),
     "atdf_f32_to_bf16": (_I32, [_P, _P, _I64, _P]),
 }

@@ -136,6 +136,71 @@ PYBIND11_MODULE(TORCH_EXTENSION_NAME, m)
     m.def("NodeCulling", &NodeCulling);
     m.def("UniformSampledVolume", &UniformSampledVolume);
     m.def("SampleVolumeBySlice", &SampleVolumeBySlice);
+    py::class_<DatasetParams>(m, "DatasetParams")
+        .def(py::init<>())
+        .def("Load", &DatasetParams::Load)
+        .def_readwrite("camera_model", &DatasetParams::camera_model)
+        .def_readwrite("image_dir", &DatasetParams::image_dir)
+        .def_readwrite("mask_dir", &DatasetParams::mask_dir)
+        .def_readwrite("volume_file", &DatasetParams::volume_file)
+        .def_readwrite("projection_factor", &DatasetParams::projection_factor)
+        .def_readwrite("vis_volume_intensity_factor", &DatasetParams::vis_volume_intensity_factor)
+        .def_readwrite("scene_scale", &DatasetParams::scene_scale)
+        .def_readwrite("xray_min", &DatasetParams::xray_min)
+        .def_readwrite("xray_max", &DatasetParams::xray_max)
+        .def_readwrite("log_space_input", &DatasetParams::log_space_input)
+        .def_readwrite("use_log10_conversion", &DatasetParams::use_log10_conversion)
+        .def_readwrite("roi_min", &DatasetParams::roi_min)
+        .def_readwrite("roi_max", &DatasetParams::roi_max);
+    py::class_<CameraBase>(m, "CameraBase")
+        .def(py::init<>())
+        .def("Load", &CameraBase::Load)
+        .def_readwrite("w", &CameraBase::w)
+        .def_readwrite("h", &CameraBase::h)
+        .def_readwrite("K", &CameraBase::K);
+    py::class_<UnifiedImage>(m, "UnifiedImage")
+        .def_readonly("camera_id", &UnifiedImage::camera_id)
+        .def_readonly("rotation", &UnifiedImage::rotation)
+        .def_readonly("translation", &UnifiedImage::translation)
+        .def_readonly("image_file", &UnifiedImage::image_file)
+        .def_readonly("mask_file", &UnifiedImage::mask_file)
+        .def_readonly("loaded", &UnifiedImage::loaded);
+    py::class_<PixelList>(m, "PixelList")
+        .def_readonly("uv", &PixelList::uv)
+        .def_readonly("image_id", &PixelList::image_id)
+        .def_readonly("camera_id", &PixelList::camera_id)
+        .def_readonly("target", &PixelList::target)
+        .def_readonly("target_mask", &PixelList::target_mask)
+        .def_readonly("rays", &PixelList::rays);
+    py::class_<SceneBase>(m, "SceneBase")
+        .def(py::init<const std::string&>())
+        .def_readonly("scene_path", &SceneBase::scene_path)
+        .def_readonly("scene_name", &SceneBase::scene_name)
+        .def_readonly("dataset_params", &SceneBase::dataset_params)
+        .def_readonly("cameras", &SceneBase::cameras)
+        .def_readonly("frames", &SceneBase::frames)
+        .def_readwrite("train_indices", &SceneBase::train_indices)
+        .def_readwrite("test_indices", &SceneBase::test_indices)
+        .def_readonly("volume_translation", &SceneBase::volume_translation)
+        .def_readonly("intrinsics", &SceneBase::intrinsics)
+        .def_readonly("rotation", &SceneBase::rotation)
+        .def_readonly("translation", &SceneBase::translation)
+        .def_readonly("projections", &SceneBase::projections)
+        .def_readonly("masks", &SceneBase::masks)
+        .def_readwrite("align_corners", &SceneBase::align_corners)
+        .def("NumImages", &SceneBase::NumImages)
+        .def("CameraModelId", &SceneBase::CameraModelId)
+        .def("LoadSplit", &SceneBase::LoadSplit)
+        .def("Finalize", &SceneBase::Finalize, py::arg("device"),
+             py::arg("volume_translation") = std::vector<float>{0, 0, 0})
+        .def("LoadImagesCT", &SceneBase::LoadImagesCT)
+        .def("GetRays", &SceneBase::GetRays)
+        .def("SampleProjection", &SceneBase::SampleProjection)
+        .def("RandomRayBatch", &SceneBase::RandomRayBatch, py::arg("indices"), py::arg("num_rays"), py::arg("seed"),
+             py::arg("interpolate") = true)
+        .def_static("ReadIndexFile", &SceneBase::ReadIndexFile);
+    m.def("LoadNumpy", &LoadNumpy);
+    m.def("LoadNumpyNpz", &LoadNumpyNpz);
     m.def("SaveHDRImageTensor", &SaveHDRImageTensor);
     m.def("LoadHDRImageTensor", &LoadHDRImageTensor);
     m.def(

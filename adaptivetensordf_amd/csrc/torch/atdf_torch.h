@@ -339,4 +339,75 @@ double ProposalTrain(HyperTreeBase& tree, ProposalState& ps, const RayList& rays
 torch::Tensor TrainStepLearned(HyperTreeBase& tree, TrainState& st, ProposalState& ps, const RayList& rays,
                                torch::Tensor target, const SamplingParams& sp, const FieldParams& fp);
 
+// ---- scene data (data/SceneBase.h, data/SceneBase.cpp, Settings.h:271-345) ----
+// dataset.ini, section [DatasetParams] (Settings.h:271-345)
+struct DatasetParams {
+    std::string camera_model = "pinhole";  // "pinhole" (cone beam) or "orthographic" (parallel beam)
+    std::string image_dir, mask_dir, volume_file;
+    double projection_factor = 1, vis_volume_intensity_factor = 1, scene_scale = 1;
+    double xray_min = 0, xray_max = 65535;
+    bool log_space_input = false, use_log10_conversion = true;
+    std::vector<float> roi_min{-1, -1, -1}, roi_max{1, 1, 1};
+    void Load(const std::string& file);
+};
+// camera.ini, section [CameraBase] (SceneBase.h:84-127): detector size and
+// K = fx fy cx cy s.
+struct CameraBase {
+    int w = 0, h = 0;
+    std::vector<double> K{1, 1, 0, 0, 0};
+    void Load(const std::string& file);
+};
+// One projection image (UnifiedImage, SceneBase.h:35-80).
+struct UnifiedImage {
+    int camera_id = 0;
+    std::vector<double> rotation{0, 0, 0, 1};  // unit quaternion x y z w
+    std::vector<double> translation{0, 0, 0};  // pose translation * scene_scale
+    std::string image_file, mask_file;
+    bool loaded = false;
+};
+// A random ray batch (PixelList of data/Dataloader.cpp:9-48) with its rays.
+struct PixelList {
+    torch::Tensor uv, image_id, camera_id, target, target_mask;
+    RayList rays;
+};
+// The scene directory: dataset.ini, camera.ini, poses.txt ("qx qy qz qw tx ty
+// tz" per image), images.txt, masks.txt, camera_indices.txt; projections as
+// .npy / .npz (array "proj") files under image_dir.  Projections and masks
+// live on the device as [num_images, h, w] f32 after LoadImagesCT.
+class SceneBase {
+   public:
+    explicit SceneBase(const std::string& scene_dir);
+    std::string scene_path, scene_name;
+    DatasetParams dataset_params;
+    std::vector<CameraBase> cameras;
+    std::vector<UnifiedImage> frames;
+    std::vector<int> train_indices, test_indices;
+    std::vector<float> volume_translation{0, 0, 0};
+    torch::Device device{torch::kCPU};
+    torch::Tensor intrinsics, rotation, translation;  // f64 [cams, 5], [images, 4], [images, 3]
+    torch::Tensor projections, masks;                 // f32 [images, h, w]; masks undefined without mask files
+    bool align_corners = true;                        // uv_align_corners (SceneBase.h:24)
+
+    // <scene>/<split_name>/train.txt and eval.txt (samples/hyper_train.cpp:93-97), sorted
+    void LoadSplit(const std::string& split_name);
+    // the pose and camera tensors on `device` (SceneBase::Finalize, SceneBase.cpp:98-140):
+    // translation = pose translation - volume_translation
+    void Finalize(const std::string& device, std::vector<float> volume_translation);
+    // read, log-convert and mask the projections of `indices` (SceneBase.cpp:904-990)
+    void LoadImagesCT(std::vector<int> indices);
+    RayList GetRays(torch::Tensor uv, torch::Tensor image_id, torch::Tensor camera_id) const;
+    std::pair<torch::Tensor, torch::Tensor> SampleProjection(torch::Tensor uv, torch::Tensor image_id) const;
+    // RandomRaybasedSampleDataset::get_batch over `indices`, with the batch's rays
+    PixelList RandomRayBatch(torch::Tensor indices, int64_t num_rays, uint64_t seed, bool interpolate) const;
+    int NumImages() const { return (int)frames.size(); }
+    int CameraModelId() const;
+    static std::vector<int> ReadIndexFile(const std::string& file);  // TrainParameters.h:207-221
+    atdf_camera_rig rig() const;
+    atdf_projection_set projection_set() const;
+};
+// NPY / NPZ readers (the cnpy loaders of utils/utils.h:47-83): little- or
+// big-endian float, integer and bool arrays in their own dtype; C order only.
+torch::Tensor LoadNumpy(const std::string& path);
+torch::Tensor LoadNumpyNpz(const std::string& path, const std::string& varname);
+
 }  // namespace atdf_torch

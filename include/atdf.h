@@ -373,6 +373,106 @@ atdf_status atdf_sample_volume(const void* packed_tree, int64_t num_nodes, const
                                int64_t* node_id, float* valid, float* position, atdf_stream_t stream);
 
 /* ======================================================================= */
+/* Scene data: detector rays, projections, random ray batches              */
+/* ======================================================================= */
+/* Cameras and poses of a scene (host struct, device buffers):
+ *   camera_model 0 pinhole (cone beam), 1 orthographic (parallel beam)
+ *     (DatasetParams::camera_model, Settings.h:300);
+ *   w, h: detector size; align_corners: the uv convention of UVToPixel
+ *     (1 in the reference, SceneBase.h:24);
+ *   intrinsics [num_cameras, 5] f64 fx fy cx cy s (CameraBase::K,
+ *     SceneBase.h:84-127);
+ *   rotation [num_images, 4] f64 unit quaternion (x, y, z, w), translation
+ *     [num_images, 3] f64 (CameraPoseModuleImpl, modules/CameraPose.cu:
+ *     250-279): the pose translation times scene_scale minus
+ *     volume_translation (SceneBase.cpp:98-117, 840); read by the pinhole
+ *     model only;
+ *   volume_translation: TrainParams::volume_translation (SceneBase.cpp:
+ *     112-117), added by the orthographic model. */
+typedef struct {
+    int32_t camera_model;
+    int32_t w, h;
+    int32_t align_corners;
+    int64_t num_cameras, num_images;
+    const double* intrinsics;
+    const double* rotation;
+    const double* translation;
+    float volume_translation[3];
+} atdf_camera_rig;
+
+/* SceneBase::GetRays (data/SceneBase.cpp:369-449) for num_rays detector
+ * points: uv [n, 2] f32 in [0, 1], image_id / camera_id [n] i32 ->
+ * ray_origin, ray_direction [n, 3] f32.  UVToPixel in f32
+ * (modules/CameraPose.h:14-30), then in f64: u = ((px - cx) / fx,
+ * (py - cy) / fy, 1) (CameraModelModuleImpl::Unproject, modules/
+ * CameraPose.cu:469-481) and the rotation v' = v + w t + q x t with
+ * t = 2 (q x v) (RotatePointGPUk, modules/CameraPose.cu:27-40).
+ *   pinhole:      origin = translation, direction = R u / |u|;
+ *   orthographic: direction = R (0, 0, 1),
+ *                 origin = R (u_x, u_y, 0) + volume_translation - sqrt(2) direction.
+ * Cast to f32 at the end.  An image or camera index outside the rig gives
+ * NaN rays (nothing is read out of bounds). */
+atdf_status atdf_generate_rays(const atdf_camera_rig* rig, const float* uv, const int32_t* image_id,
+                               const int32_t* camera_id, int64_t num_rays, float* ray_origin, float* ray_direction,
+                               atdf_stream_t stream);
+
+/* X-ray log conversion of SceneBase::LoadImagesCT (data/SceneBase.cpp:
+ * 939-983) over n pixels:
+ *   log_space_input:  out = raw / xray_max
+ *   otherwise:        out = -log(max(raw, 1e-5) / xray_max)
+ *                           * (projection_factor / -log(xray_min / xray_max))
+ * with log10 (log10_conversion != 0) or the natural log; the scale is
+ * formed in f64 and applied in f32, like a libtorch f32 tensor times a
+ * double scalar.  mask (nullable, [n] f32 in [0, 1]) multiplies the result
+ * (SceneBase.cpp:976-984). */
+atdf_status atdf_log_convert(const float* raw, const float* mask, int64_t n, double xray_min, double xray_max,
+                             double projection_factor, int32_t log10_conversion, int32_t log_space_input,
+                             float* out, atdf_stream_t stream);
+
+/* Projections of a scene resident in device memory: projection and mask
+ * (nullable) [num_images, h, w] f32, one channel (the CT case of
+ * UnifiedImage::projection, SceneBase.h:58-64).  Host struct. */
+typedef struct {
+    const float* projection;
+    const float* mask;
+    int32_t w, h;
+    int64_t num_images;
+    int32_t align_corners;
+} atdf_projection_set;
+
+/* UnifiedImage::SampleProjection (data/SceneBase.cpp:76-97): the
+ * projection (and mask) of image image_id[i] at uv[i], bilinear with border
+ * clamping (libtorch grid_sample, padding_mode border, on g = uv * 2 - 1):
+ *   x = (g + 1) * ((w - 1) / 2)  (align_corners)  or  (g + 1) * (w / 2) - 0.5,
+ *   clamped to [0, w - 1]; the four neighbours weighted (1-a)(1-b), a(1-b),
+ *   (1-a)b, ab and summed in that order, neighbours past the last pixel read 0.
+ * target [n] f32; target_mask [n] f32 (nullable; 1 where the set has no mask).
+ * An image index outside the set gives NaN. */
+atdf_status atdf_sample_projection(const atdf_projection_set* set, const float* uv, const int32_t* image_id,
+                                   int64_t n, float* target, float* target_mask, atdf_stream_t stream);
+
+/* Random ray batch (RandomRaybasedSampleDataset::get_batch, data/
+ * Dataloader.cpp:9-48): ray i draws an entry k uniformly from
+ * image_indices[num_indices] (image_id = image_indices[k], camera_id =
+ * image_camera[k], 0 with image_camera null) and its detector point
+ *   interpolate != 0: uv uniform in [0, 1)^2 (UnifiedImage::CoordinatesRandom,
+ *                     SceneBase.h:52);
+ *   interpolate == 0: a uniform pixel (x, y) mapped by PixelToUV
+ *                     (CoordinatesRandomNoInterpolate, SceneBase.cpp:38-45;
+ *                     modules/CameraPose.h:32-48).
+ * Draws come from a counter-based hash of (seed, i), so a batch is a pure
+ * function of its arguments (the reference draws from libtorch's generator).
+ * Then, in the same launch, the batch's targets (atdf_sample_projection;
+ * set nullable) and rays (atdf_generate_rays; rig nullable).  Every output
+ * except uv, image_id and camera_id is nullable. */
+atdf_status atdf_random_ray_batch(const atdf_camera_rig* rig, const atdf_projection_set* set,
+                                  const int32_t* image_indices, int64_t num_indices, const int32_t* image_camera,
+                                  int32_t w, int32_t h, int32_t align_corners, int32_t interpolate, uint64_t seed,
+                                  int64_t num_rays, float* uv, int32_t* image_id, int32_t* camera_id, float* target,
+                                  float* target_mask, float* ray_origin, float* ray_direction,
+                                  atdf_stream_t stream);
+
+/* ======================================================================= */
 /* Learned sample proposal                                                  */
 /* ======================================================================= */
 /* No counterpart in the reference: its sampler is the stratified per-node

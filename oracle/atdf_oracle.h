@@ -268,6 +268,25 @@ void atdfo_grid_transfer(const float* node_min, const float* node_max, const int
 void atdfo_volume_points(int mode, int swap_xy, const int* size, const float* roi_min, const float* step,
                          float* pos);
 
+/* Scene data path.  Quaternions (x, y, z, w); rotation v' = (v + w t) +
+ * q x t with t = 2 q x v, in f64. */
+void atdfo_rotate_point(const double* q, const double* v, double* out);
+/* SceneBase::GetRays: camera_model 0 pinhole, 1 orthographic; outputs f32. */
+void atdfo_generate_rays(int camera_model, int w, int h, int align_corners, const double* intrinsics,
+                         const double* rotation, const double* translation, const float* vol_translate,
+                         const float* uv, const int32_t* image_id, const int32_t* camera_id, int64_t n,
+                         float* origin, float* direction);
+/* LoadImagesCT log conversion; mask nullable. */
+void atdfo_log_convert(const float* raw, const float* mask, int64_t n, double xray_min, double xray_max,
+                       double projection_factor, int log10_conversion, int log_space_input, float* out);
+/* grid_sample bilinear / border of [images, h, w] projections at uv. */
+void atdfo_sample_projection(const float* projection, const float* mask, int w, int h, int align_corners,
+                             const float* uv, const int32_t* image_id, int64_t n, float* target, float* target_mask);
+/* Random batch draws (image entry, uv) of the batch kernel. */
+void atdfo_random_pixels(const int32_t* indices, int64_t num_indices, const int32_t* image_camera, int w, int h,
+                         int align_corners, int interpolate, uint64_t seed, int64_t n, float* uv, int32_t* image_id,
+                         int32_t* camera_id);
+
 #ifdef __cplusplus
 }
 #endif

@@ -497,3 +497,61 @@ def volume_points(mode, size, roi_min=(0, 0, 0), step=(0, 0, 0), swap_xy=False):
     lib().atdfo_volume_points(C.c_int(mode), C.c_int(int(swap_xy)), _p(_c(size, np.int32)),
                               _p(_c(roi_min, np.float32)), _p(_c(step, np.float32)), _p(pos))
     return pos
+
+
+# --------------------------------------------------------------------------
+# scene data path
+# --------------------------------------------------------------------------
+def rotate_point(q, v):
+    out = np.zeros(3, np.float64)
+    lib().atdfo_rotate_point(_p(_c(q, np.float64)), _p(_c(v, np.float64)), _p(out))
+    return out
+
+
+def generate_rays(camera_model, w, h, intrinsics, rotation, translation, uv, image_id, camera_id,
+                  vol_translate=(0.0, 0.0, 0.0), align_corners=True):
+    uv = _c(uv, np.float32).reshape(-1, 2)
+    n = uv.shape[0]
+    o = np.zeros((n, 3), np.float32)
+    d = np.zeros((n, 3), np.float32)
+    tr = None if translation is None else _c(translation, np.float64)
+    lib().atdfo_generate_rays(C.c_int(camera_model), C.c_int(w), C.c_int(h), C.c_int(int(align_corners)),
+                              _p(_c(intrinsics, np.float64)), _p(_c(rotation, np.float64)), _p(tr),
+                              _p(_c(vol_translate, np.float32)), _p(uv), _p(_c(image_id, np.int32)),
+                              _p(_c(camera_id, np.int32)), C.c_int64(n), _p(o), _p(d))
+    return o, d
+
+
+def log_convert(raw, xray_min, xray_max, projection_factor=1.0, log10=True, log_space_input=False, mask=None):
+    raw = _c(raw, np.float32)
+    out = np.zeros(raw.shape, np.float32)
+    m = None if mask is None else _c(mask, np.float32)
+    lib().atdfo_log_convert(_p(raw), _p(m), C.c_int64(raw.size), C.c_double(xray_min), C.c_double(xray_max),
+                            C.c_double(projection_factor), C.c_int(int(log10)), C.c_int(int(log_space_input)),
+                            _p(out))
+    return out
+
+
+def sample_projection(projection, uv, image_id, mask=None, align_corners=True):
+    projection = _c(projection, np.float32)
+    h, w = projection.shape[-2:]
+    uv = _c(uv, np.float32).reshape(-1, 2)
+    n = uv.shape[0]
+    t = np.zeros(n, np.float32)
+    tm = np.zeros(n, np.float32)
+    m = None if mask is None else _c(mask, np.float32)
+    lib().atdfo_sample_projection(_p(projection), _p(m), C.c_int(w), C.c_int(h), C.c_int(int(align_corners)),
+                                  _p(uv), _p(_c(image_id, np.int32)), C.c_int64(n), _p(t), _p(tm))
+    return t, tm
+
+
+def random_pixels(indices, image_camera, w, h, n, seed, interpolate=True, align_corners=True):
+    indices = _c(indices, np.int32)
+    uv = np.zeros((n, 2), np.float32)
+    img = np.zeros(n, np.int32)
+    cam = np.zeros(n, np.int32)
+    ic = None if image_camera is None else _c(image_camera, np.int32)
+    lib().atdfo_random_pixels(_p(indices), C.c_int64(indices.size), _p(ic), C.c_int(w), C.c_int(h),
+                              C.c_int(int(align_corners)), C.c_int(int(interpolate)), C.c_uint64(seed),
+                              C.c_int64(n), _p(uv), _p(img), _p(cam))
+    return uv, img, cam
