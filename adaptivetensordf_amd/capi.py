@@ -144,6 +144,10 @@ _SIGNATURES = {
     "atdf_sample_projection": (_I32, [C.POINTER(ProjectionSet), _P, _P, _I64, _P, _P, _P]),
     "atdf_random_ray_batch": (_I32, [C.POINTER(CameraRig), C.POINTER(ProjectionSet), _P, _I64, _P, _I32, _I32,
                                      _I32, _I32, C.c_uint64, _I64, _P, _P, _P, _P, _P, _P, _P, _P]),
+    "atdf_point_in_any_image": (_I32, [C.POINTER(CameraRig), _P, _P, _P, _I64, _P, _P]),
+    "atdf_node_lattice_points": (_I32, [_P, _I64, _P, _I64, _I32, _P, _P]),
+    "atdf_node_density_stats_masked": (_I32, [_P, _I64, _P, _I64, C.POINTER(Field), _I32, _I32, _I32, C.c_uint64,
+                                              _P, _P, _P, _P, _P]),
     "atdf_grid_layout": (_I32, [_P, _P, _I64, _I32, _I32, _I32, _P]),
     "atdf_f32_to_bf16": (_I32, [_P, _P, _I64, _P]),
 }

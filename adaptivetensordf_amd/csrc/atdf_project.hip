@@ -1391,7 +1391,38 @@ struct StatsArgs {
     float* node_max;
     double* node_sum;
     double* node_sumsq;
+    const float* point_mask;  // nullable [num_active, points per node]
 };
+
+// Point p of the n^3 lattice of a node (UniformGlobalSamples,
+// structure/HyperTree.cu:1618-1653): (idx / (n - 1)) * size + min, x fastest.
+__device__ __forceinline__ void lattice_position(const PNode& nd, const float* size, int64_t p, int n, float* pos)
+{
+    const int64_t idx[3] = {p % n, (p / n) % n, p / ((int64_t)n * n)};
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        float lp = (float)idx[k] / (float)(n - 1);
+        pos[k] = lp * size[k] + nd.mn[k];
+    }
+}
+
+__global__ __launch_bounds__(256) void lattice_points_kernel(const PNode* __restrict__ nodes,
+                                                             const int64_t* __restrict__ active_ids, int n,
+                                                             float* __restrict__ out)
+{
+    const PNode nd = nodes[active_ids[blockIdx.x]];
+    float size[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) size[k] = nd.mx[k] - nd.mn[k];
+    const int64_t per = (int64_t)n * n * n;
+    float* o = out + (int64_t)blockIdx.x * per * 3;
+    for (int64_t p = threadIdx.x; p < per; p += blockDim.x) {
+        float pos[3];
+        lattice_position(nd, size, p, n, pos);
+#pragma unroll
+        for (int k = 0; k < 3; ++k) o[p * 3 + k] = pos[k];
+    }
+}
 
 template <int C, bool kBF16, bool kDecBF16>
 __global__ __launch_bounds__(64 * kWaves) void node_stats_kernel(StatsArgs sa)
@@ -1433,12 +1464,7 @@ __global__ __launch_bounds__(64 * kWaves) void node_stats_kernel(StatsArgs sa)
         if (sc.valid) {
             float pos[3];
             if (sa.mode == 0) {
-                const int64_t idx[3] = {p % n, (p / n) % n, p / ((int64_t)n * n)};
-#pragma unroll
-                for (int k = 0; k < 3; ++k) {
-                    float lp = (float)idx[k] / (float)(n - 1);
-                    pos[k] = lp * size[k] + nd.mn[k];
-                }
+                lattice_position(nd, size, p, n, pos);
             } else {
 #pragma unroll
                 for (int k = 0; k < 3; ++k) pos[k] = nd.mn[k] + point_uniform(sa.seed, node, p, k) * size[k];
@@ -1458,7 +1484,8 @@ __global__ __launch_bounds__(64 * kWaves) void node_stats_kernel(StatsArgs sa)
         else
             y = decoder_fwd<C>(s, x, g, j, h1, h2);
         if (sc.valid && g == 0) {
-            const float dn = softplus_f(y, a.beta);
+            float dn = softplus_f(y, a.beta);
+            if (sa.point_mask) dn = dn * sa.point_mask[act * per + p];
             dmax = fmaxf(dmax, dn);
             s1 += (double)dn;
             s2 += (double)dn * (double)dn;
@@ -2100,10 +2127,32 @@ atdf_status atdf_sample_volume(const void* packed_tree, int64_t num_nodes, const
     return check_launch("atdf_sample_volume");
 }
 
+atdf_status atdf_node_lattice_points(const void* packed_tree, int64_t num_nodes, const int64_t* active_ids,
+                                     int64_t num_active, int32_t n, float* positions, atdf_stream_t stream)
+{
+    ATDF_REQUIRE(packed_tree && num_nodes > 0, "atdf_node_lattice_points: empty tree");
+    ATDF_REQUIRE(n >= 2 && n <= 1024, "atdf_node_lattice_points: lattice size %d outside [2, 1024]", n);
+    ATDF_REQUIRE(num_active >= 0 && num_active < (int64_t)INT32_MAX, "atdf_node_lattice_points: bad node count");
+    if (num_active == 0) return ATDF_OK;
+    ATDF_REQUIRE(active_ids && positions, "atdf_node_lattice_points: null buffer");
+    hipLaunchKernelGGL(lattice_points_kernel, dim3((unsigned)num_active), dim3(256), 0, to_hip(stream),
+                       reinterpret_cast<const PNode*>(packed_tree), active_ids, n, positions);
+    return check_launch("atdf_node_lattice_points");
+}
+
 atdf_status atdf_node_density_stats(const void* packed_tree, int64_t num_nodes, const int64_t* active_ids,
                                     int64_t num_active, const atdf_field* f, int32_t mode, int32_t n, int32_t rounds,
                                     uint64_t seed, float* node_max, double* node_sum, double* node_sumsq,
                                     atdf_stream_t stream)
+{
+    return atdf_node_density_stats_masked(packed_tree, num_nodes, active_ids, num_active, f, mode, n, rounds, seed,
+                                          nullptr, node_max, node_sum, node_sumsq, stream);
+}
+
+atdf_status atdf_node_density_stats_masked(const void* packed_tree, int64_t num_nodes, const int64_t* active_ids,
+                                           int64_t num_active, const atdf_field* f, int32_t mode, int32_t n,
+                                           int32_t rounds, uint64_t seed, const float* point_mask, float* node_max,
+                                           double* node_sum, double* node_sumsq, atdf_stream_t stream)
 {
     ATDF_REQUIRE(packed_tree && num_nodes > 0 && f, "atdf_node_density_stats: empty tree or field");
     ATDF_REQUIRE(f->H == 64 && (f->C == 16 || f->C == 8), "atdf_node_density_stats: field shape not built");
@@ -2125,6 +2174,7 @@ atdf_status atdf_node_density_stats(const void* packed_tree, int64_t num_nodes, 
     sa.node_max = node_max;
     sa.node_sum = node_sum;
     sa.node_sumsq = node_sumsq;
+    sa.point_mask = point_mask;
     dim3 grid((unsigned)num_active), block(64 * kWaves);
     hipStream_t s = to_hip(stream);
     const bool bf = f->grid_dtype == ATDF_GRID_BF16;

@@ -287,6 +287,62 @@ __global__ __launch_bounds__(256) void random_batch_kernel(BatchArgs a)
     }
 }
 
+struct VisArgs {
+    RigDev rig;
+    float roi_min[3], roi_max[3];
+    const float* points;
+    int64_t n;
+    float* mask;
+};
+
+// One lane per point; the images are visited in order until one sees the
+// point (the reference launches a block per image and sets the flag
+// concurrently; the result is the same "any").
+__global__ __launch_bounds__(256) void point_visible_kernel(VisArgs a)
+{
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= a.n) return;
+    const float pf[3] = {a.points[i * 3], a.points[i * 3 + 1], a.points[i * 3 + 2]};
+    bool inside = true;
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
+        if (a.roi_min[k] > pf[k] || a.roi_max[k] < pf[k]) inside = false;
+    float vis = 0.f;
+    if (inside) {
+        const RigDev& r = a.rig;
+        const double p[3] = {(double)pf[0], (double)pf[1], (double)pf[2]};
+        const double fx = r.intrinsics[0], fy = r.intrinsics[1], cx = r.intrinsics[2], cy = r.intrinsics[3];
+        const double sk = 0.0;
+        for (int64_t img = 0; img < r.num_images; ++img) {
+            const double* q = r.rotation + img * 4;
+            const double* t = r.translation + img * 3;
+            const double qi[4] = {-q[0], -q[1], -q[2], q[3]};
+            const double mt[3] = {-t[0], -t[1], -t[2]};
+            double a1[3], a2[3], v[3];
+            rotate_point(qi, p, a1);
+            rotate_point(qi, mt, a2);
+#pragma unroll
+            for (int k = 0; k < 3; ++k) v[k] = a1[k] + a2[k];
+            double ix, iy;
+            if (!r.orthographic) {
+                const double invz = 1.0 / v[2];
+                const double x = v[0] * invz, y = v[1] * invz;
+                ix = (fx * x + sk * y) + cx;
+                iy = fy * y + cy;
+            } else {
+                ix = (fx * v[0] + sk * v[1]) + cx;
+                iy = fy * v[1] + cy;
+            }
+            const double rx = round(ix), ry = round(iy);
+            if (rx >= 0.0 && rx < (double)r.w && ry >= 0.0 && ry < (double)r.h) {
+                vis = 1.f;
+                break;
+            }
+        }
+    }
+    a.mask[i] = vis;
+}
+
 atdf_status make_rig(const atdf_camera_rig* rig, RigDev* out, const char* who)
 {
     ATDF_REQUIRE(rig->camera_model == 0 || rig->camera_model == 1,
@@ -433,6 +489,27 @@ atdf_status atdf_random_ray_batch(const atdf_camera_rig* rig, const atdf_project
     a.direction = ray_direction;
     hipLaunchKernelGGL(random_batch_kernel, dim3((unsigned)div_up(num_rays, 256)), dim3(256), 0, to_hip(stream), a);
     return check_launch("atdf_random_ray_batch");
+}
+
+atdf_status atdf_point_in_any_image(const atdf_camera_rig* rig, const float* roi_min, const float* roi_max,
+                                    const float* points, int64_t n, float* mask, atdf_stream_t stream)
+{
+    ATDF_REQUIRE(rig && roi_min && roi_max && n >= 0, "atdf_point_in_any_image: bad arguments");
+    VisArgs a{};
+    atdf_status st = make_rig(rig, &a.rig, "atdf_point_in_any_image");
+    if (st != ATDF_OK) return st;
+    ATDF_REQUIRE(rig->translation, "atdf_point_in_any_image: the rig needs translations");
+    if (n == 0) return ATDF_OK;
+    ATDF_REQUIRE(points && mask, "atdf_point_in_any_image: null buffer");
+    for (int k = 0; k < 3; ++k) {
+        a.roi_min[k] = roi_min[k];
+        a.roi_max[k] = roi_max[k];
+    }
+    a.points = points;
+    a.n = n;
+    a.mask = mask;
+    hipLaunchKernelGGL(point_visible_kernel, dim3((unsigned)div_up(n, 256)), dim3(256), 0, to_hip(stream), a);
+    return check_launch("atdf_point_in_any_image");
 }
 
 }  // extern "C"

@@ -75,6 +75,7 @@ PYBIND11_MODULE(TORCH_EXTENSION_NAME, m)
         .def("LeafTable", &HyperTreeBase::LeafTable)
         .def("RoiSplit", &HyperTreeBase::RoiSplit)
         .def("NodeIdForPositionGPU", &HyperTreeBase::NodeIdForPositionGPU)
+        .def("UniformGlobalSamples", &HyperTreeBase::UniformGlobalSamples)
         .def("UniformPhantomSamplesGPU", &HyperTreeBase::UniformPhantomSamplesGPU)
         .def("UniformPhantomSamplesGPUbySlice", &HyperTreeBase::UniformPhantomSamplesGPUbySlice)
         .def("InterpolateGridForInactiveNodes", &HyperTreeBase::InterpolateGridForInactiveNodes)
@@ -189,6 +190,7 @@ PYBIND11_MODULE(TORCH_EXTENSION_NAME, m)
         .def_readonly("masks", &SceneBase::masks)
         .def_readwrite("align_corners", &SceneBase::align_corners)
         .def("NumImages", &SceneBase::NumImages)
+        .def("PointInAnyImage", &SceneBase::PointInAnyImage)
         .def("CameraModelId", &SceneBase::CameraModelId)
         .def("LoadSplit", &SceneBase::LoadSplit)
         .def("Finalize", &SceneBase::Finalize, py::arg("device"),
@@ -225,7 +227,11 @@ PYBIND11_MODULE(TORCH_EXTENSION_NAME, m)
     m.def("IntegrateSamplesXRay", &IntegrateSamplesXRay);
     m.def("AccumulateSampleLossPerNode", &AccumulateSampleLossPerNode);
     m.def("ProjectRays", &ProjectRays);
-    m.def("ComputeMaxDensity", &ComputeMaxDensity);
+    m.def(
+        "ComputeMaxDensity",
+        [](HyperTreeBase& tree, torch::Tensor grid, torch::Tensor mlp, const FieldParams& fp, int n,
+           const SceneBase* scene) { ComputeMaxDensity(tree, grid, mlp, fp, n, scene); },
+        py::arg("tree"), py::arg("grid"), py::arg("mlp"), py::arg("fp"), py::arg("n"), py::arg("scene") = nullptr);
     m.def("ComputeStd", &ComputeStd);
 
     py::class_<TrainState>(m, "TrainState")

@@ -532,6 +532,21 @@ void check_ids(const Tensor& ids, int64_t bound, const char* what)
 }
 }  // namespace
 
+Tensor SceneBase::PointInAnyImage(Tensor points) const
+{
+    TORCH_CHECK(points.is_cuda() && points.size(-1) == 3, "PointInAnyImage: [..., 3] device points required");
+    auto sizes = points.sizes().vec();
+    sizes.pop_back();
+    auto p = points.reshape({-1, 3}).to(torch::kFloat32).contiguous();
+    auto mask = torch::empty({p.size(0)}, p.options());
+    atdf_camera_rig r = rig();
+    const auto& dp = dataset_params;
+    check_status(atdf_point_in_any_image(&r, dp.roi_min.data(), dp.roi_max.data(), p.data_ptr<float>(), p.size(0),
+                                         mask.data_ptr<float>(), current_stream()),
+                 "atdf_point_in_any_image");
+    return mask.reshape(sizes);
+}
+
 RayList SceneBase::GetRays(Tensor uv, Tensor image_id, Tensor camera_id) const
 {
     TORCH_CHECK(uv.is_cuda() && uv.scalar_type() == torch::kFloat32 && uv.dim() == 2 && uv.size(1) == 2,

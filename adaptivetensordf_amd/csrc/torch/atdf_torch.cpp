@@ -855,7 +855,8 @@ Tensor ProjectRays(HyperTreeBase& tree, const RayList& rays, Tensor grid, Tensor
 
 namespace {
 std::tuple<Tensor, Tensor, Tensor> node_stats(HyperTreeBase& tree, const Tensor& grid, const Tensor& mlp,
-                                              const FieldParams& fp, int mode, int n, int rounds, uint64_t seed)
+                                              const FieldParams& fp, int mode, int n, int rounds, uint64_t seed,
+                                              const Tensor& point_mask = Tensor())
 {
     atdf_field f = make_field(tree, grid, mlp, fp);
     const int64_t NA = tree.NumActiveNodes();
@@ -864,17 +865,24 @@ std::tuple<Tensor, Tensor, Tensor> node_stats(HyperTreeBase& tree, const Tensor&
     auto s1 = torch::empty({NA}, torch::TensorOptions().dtype(torch::kFloat64).device(dev));
     auto s2 = torch::empty({NA}, torch::TensorOptions().dtype(torch::kFloat64).device(dev));
     auto ids = tree.active_node_ids.contiguous();
-    check_status(atdf_node_density_stats(tree.packed(), tree.NumNodes(), ids.data_ptr<int64_t>(), NA, &f, mode, n,
-                                         rounds, seed, mx.data_ptr<float>(), s1.data_ptr<double>(),
-                                         s2.data_ptr<double>(), current_stream()),
-                 "atdf_node_density_stats");
+    check_status(atdf_node_density_stats_masked(
+                     tree.packed(), tree.NumNodes(), ids.data_ptr<int64_t>(), NA, &f, mode, n, rounds, seed,
+                     point_mask.defined() ? point_mask.data_ptr<float>() : nullptr, mx.data_ptr<float>(),
+                     s1.data_ptr<double>(), s2.data_ptr<double>(), current_stream()),
+                 "atdf_node_density_stats_masked");
     return {mx, s1, s2};
 }
 }  // namespace
 
-void ComputeMaxDensity(HyperTreeBase& tree, Tensor grid, Tensor mlp, const FieldParams& fp, int n)
+void ComputeMaxDensity(HyperTreeBase& tree, Tensor grid, Tensor mlp, const FieldParams& fp, int n,
+                       const SceneBase* scene)
 {
-    auto [mx, s1, s2] = node_stats(tree, grid, mlp, fp, 0, n, 1, 0);
+    Tensor mask;
+    if (scene) {
+        auto pos = tree.UniformGlobalSamples(tree.active_node_ids, n);
+        mask = scene->PointInAnyImage(pos).reshape({-1}).contiguous();
+    }
+    auto [mx, s1, s2] = node_stats(tree, grid, mlp, fp, 0, n, 1, 0, mask);
     (void)s1;
     (void)s2;
     tree.node_max_density = tree.node_max_density.to(mx.device()).index_copy(0, tree.active_node_ids, mx);

@@ -106,6 +106,9 @@ class HyperTreeBase {
     // NodeIdForPositionGPU (HyperTree.cu:1487-1525): (node_id [P] i64, mask [P] f32)
     // for positions [..., 3]; outside every active node: first active node, 0
     std::pair<torch::Tensor, torch::Tensor> NodeIdForPositionGPU(torch::Tensor global_samples) const;
+    // UniformGlobalSamples (HyperTree.cu:1618-1665): [N, n, n, n, 3] lattice
+    // positions of the listed nodes (x fastest)
+    torch::Tensor UniformGlobalSamples(torch::Tensor node_id, int n) const;
     // InterpolateGridForInactiveNodes (HyperTree.cu:1603-1616): a copy of
     // active_grid [num_nodes, C, G, G, G] whose inactive nodes take the
     // nearest lattice values of the active nodes
@@ -248,8 +251,12 @@ void SaveHDRImageTensor(torch::Tensor volume, const std::string& path);
 torch::Tensor LoadHDRImageTensor(const std::string& path);
 
 // ComputeMaxDensity (samples/hyper_train.cpp:662-707): node_max_density of the
-// active nodes = max density on an n^3 lattice per node (UniformGlobalSamples).
-void ComputeMaxDensity(HyperTreeBase& tree, torch::Tensor grid, torch::Tensor mlp, const FieldParams& fp, int n);
+// active nodes = max density on an n^3 lattice per node (UniformGlobalSamples),
+// each point's density times its visibility in the scene's images
+// (SceneBase::PointInAnyImage) when a scene is given.
+class SceneBase;
+void ComputeMaxDensity(HyperTreeBase& tree, torch::Tensor grid, torch::Tensor mlp, const FieldParams& fp, int n,
+                       const SceneBase* scene = nullptr);
 // ComputeStd (samples/hyper_train.cpp:709-759): per-node standard deviation
 // (correction 1) of the density at rounds x n^3 uniform random points of each
 // active node, reduced in FP64.  Returns [num_nodes] float64, 0 for inactive.
@@ -400,6 +407,8 @@ class SceneBase {
     // RandomRaybasedSampleDataset::get_batch over `indices`, with the batch's rays
     PixelList RandomRayBatch(torch::Tensor indices, int64_t num_rays, uint64_t seed, bool interpolate) const;
     int NumImages() const { return (int)frames.size(); }
+    // PointInAnyImage (data/SceneBase.cu:67-95): [..., 3] points -> [...] f32 0 / 1
+    torch::Tensor PointInAnyImage(torch::Tensor points) const;
     int CameraModelId() const;
     static std::vector<int> ReadIndexFile(const std::string& file);  // TrainParameters.h:207-221
     atdf_camera_rig rig() const;

@@ -353,6 +353,23 @@ std::pair<torch::Tensor, torch::Tensor> HyperTreeBase::NodeIdForPositionGPU(torc
     return {node_id, mask};
 }
 
+torch::Tensor HyperTreeBase::UniformGlobalSamples(torch::Tensor node_id, int n) const
+{
+    TORCH_CHECK(node_position_min.is_cuda(), "UniformGlobalSamples: the tree must be on the GPU");
+    TORCH_CHECK(node_id.dim() == 1, "UniformGlobalSamples: a 1-D node id list is required");
+    auto ids = node_id.to(node_position_min.device()).to(torch::kLong).contiguous();
+    const int64_t N = ids.size(0);
+    auto out = torch::empty({N, n, n, n, 3}, node_position_min.options());
+    if (N == 0) return out;
+    auto mm = torch::aminmax(ids);
+    TORCH_CHECK(std::get<0>(mm).item<int64_t>() >= 0 && std::get<1>(mm).item<int64_t>() < NumNodes(),
+                "UniformGlobalSamples: node id out of range");
+    check_status(atdf_node_lattice_points(packed(), NumNodes(), ids.data_ptr<int64_t>(), N, n, out.data_ptr<float>(),
+                                          current_stream()),
+                 "atdf_node_lattice_points");
+    return out;
+}
+
 torch::Tensor HyperTreeBase::InterpolateGridForInactiveNodes(torch::Tensor active_grid) const
 {
     TORCH_CHECK(active_grid.is_cuda() && active_grid.scalar_type() == torch::kFloat32 && active_grid.dim() == 5,

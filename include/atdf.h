@@ -344,6 +344,23 @@ atdf_status atdf_node_density_stats(const void* packed_tree, int64_t num_nodes, 
                                     int32_t rounds, uint64_t seed, float* node_max, double* node_sum,
                                     double* node_sumsq, atdf_stream_t stream);
 
+/* The same with a per-point multiplier: density = density * point_mask[a *
+ * points_per_node + p] before the reductions, as SampleVolumeBatched
+ * multiplies by its sample mask (geometry/geometry.h:353-424) when
+ * ComputeMaxDensity passes the visibility of PointInAnyImage
+ * (hyper_train.cpp:686-691).  point_mask nullable. */
+atdf_status atdf_node_density_stats_masked(const void* packed_tree, int64_t num_nodes, const int64_t* active_ids,
+                                           int64_t num_active, const atdf_field* field, int32_t mode, int32_t n,
+                                           int32_t rounds, uint64_t seed, const float* point_mask, float* node_max,
+                                           double* node_sum, double* node_sumsq, atdf_stream_t stream);
+
+/* HyperTreeBaseImpl::UniformGlobalSamples (structure/HyperTree.cu:1618-1665):
+ * positions [num_active, n, n, n, 3] f32, point (x, y, z) of node a =
+ * ((x, y, z) / (n - 1)) * (max - min) + min, x fastest -- the points of
+ * atdf_node_density_stats mode 0, in the same order. */
+atdf_status atdf_node_lattice_points(const void* packed_tree, int64_t num_nodes, const int64_t* active_ids,
+                                     int64_t num_active, int32_t n, float* positions, atdf_stream_t stream);
+
 /* Regular lattice of points for volume export.  Point i = (i0, i1, i2) with
  * i0 fastest (swap_xy: i2 fastest), index i0 + size0 (i1 + size1 i2):
  *   mode 0: p = (i / (size - 1)) * 2 - 1 over the unit cube
@@ -471,6 +488,19 @@ atdf_status atdf_random_ray_batch(const atdf_camera_rig* rig, const atdf_project
                                   int64_t num_rays, float* uv, int32_t* image_id, int32_t* camera_id, float* target,
                                   float* target_mask, float* ray_origin, float* ray_direction,
                                   atdf_stream_t stream);
+
+/* SceneBase::PointInAnyImage (data/SceneBase.cu:13-95): mask[i] = 1 if
+ * points[i] lies in the closed box [roi_min, roi_max] (host arrays) and
+ * projects into the detector of at least one image, else 0.  In f64: view =
+ * R^T p + R^T (-t) (the inverse pose, t = rig translation, read for both
+ * camera models like the reference), then with camera 0's intrinsics and
+ * skew 0
+ *   pinhole:      (fx (x / z) + cx, fy (y / z) + cy)   (division as x * (1 / z))
+ *   orthographic: (fx x + cx, fy y + cy)
+ * rounded half away from zero; visible when 0 <= px < w and 0 <= py < h.
+ * points [n, 3] f32 (the reference casts its float positions to f64). */
+atdf_status atdf_point_in_any_image(const atdf_camera_rig* rig, const float* roi_min, const float* roi_max,
+                                    const float* points, int64_t n, float* mask, atdf_stream_t stream);
 
 /* ======================================================================= */
 /* Learned sample proposal                                                  */

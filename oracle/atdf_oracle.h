@@ -282,6 +282,14 @@ void atdfo_log_convert(const float* raw, const float* mask, int64_t n, double xr
 /* grid_sample bilinear / border of [images, h, w] projections at uv. */
 void atdfo_sample_projection(const float* projection, const float* mask, int w, int h, int align_corners,
                              const float* uv, const int32_t* image_id, int64_t n, float* target, float* target_mask);
+/* PointInAnyImage visibility of points [n, 3] against a rig (camera 0). */
+void atdfo_point_in_any_image(int camera_model, int w, int h, const double* intrinsics, const double* rotation,
+                              const double* translation, int64_t num_images, const float* roi_min,
+                              const float* roi_max, const float* points, int64_t n, float* mask);
+/* UniformGlobalSamples: the n^3 lattice of each listed node, x fastest
+ * (the points of atdfo_node_density_stats mode 0); pos [num_active, n^3, 3]. */
+void atdfo_node_lattice_points(const float* node_min, const float* node_max, const int64_t* active_ids,
+                               int64_t num_active, int n, float* pos);
 /* Random batch draws (image entry, uv) of the batch kernel. */
 void atdfo_random_pixels(const int32_t* indices, int64_t num_indices, const int32_t* image_camera, int w, int h,
                          int align_corners, int interpolate, uint64_t seed, int64_t n, float* uv, int32_t* image_id,

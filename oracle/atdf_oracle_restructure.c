@@ -134,3 +134,22 @@ void atdfo_volume_points(int mode, int swap_xy, const int* size, const float* ro
         }
     }
 }
+
+void atdfo_node_lattice_points(const float* node_min, const float* node_max, const int64_t* active_ids,
+                               int64_t num_active, int n, float* pos)
+{
+    int64_t per = (int64_t)n * n * n;
+    for (int64_t a = 0; a < num_active; ++a) {
+        const float* mn = node_min + active_ids[a] * 3;
+        const float* mx = node_max + active_ids[a] * 3;
+        float size[3];
+        for (int k = 0; k < 3; ++k) size[k] = mx[k] - mn[k];
+        for (int64_t p = 0; p < per; ++p) {
+            int64_t idx[3] = {p % n, (p / n) % n, p / ((int64_t)n * n)};
+            for (int k = 0; k < 3; ++k) {
+                float lp = (float)idx[k] / (float)(n - 1);
+                pos[(a * per + p) * 3 + k] = lp * size[k] + mn[k];
+            }
+        }
+    }
+}

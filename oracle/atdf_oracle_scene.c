@@ -197,3 +197,47 @@ void atdfo_random_pixels(const int32_t* indices, int64_t num_indices, const int3
         camera_id[i] = image_camera ? image_camera[k] : 0;
     }
 }
+
+/* SceneBase::PointInAnyImage (data/SceneBase.cu:13-95): inside the closed
+ * ROI box and projected (inverse pose R^T p + R^T (-t), camera 0, skew 0,
+ * pinhole through x * (1 / z)) into some detector after rounding half away
+ * from zero.  The pinhole projection is Saiga's IntrinsicsPinhole::project
+ * (not vendored), restated as the perspective division it publishes. */
+void atdfo_point_in_any_image(int camera_model, int w, int h, const double* intrinsics, const double* rotation,
+                              const double* translation, int64_t num_images, const float* roi_min,
+                              const float* roi_max, const float* points, int64_t n, float* mask)
+{
+    double fx = intrinsics[0], fy = intrinsics[1], cx = intrinsics[2], cy = intrinsics[3], sk = 0.0;
+    for (int64_t i = 0; i < n; ++i) {
+        const float* pf = points + i * 3;
+        int inside = 1;
+        for (int k = 0; k < 3; ++k)
+            if (roi_min[k] > pf[k] || roi_max[k] < pf[k]) inside = 0;
+        float vis = 0.f;
+        if (inside) {
+            double p[3] = {pf[0], pf[1], pf[2]};
+            for (int64_t img = 0; img < num_images && vis == 0.f; ++img) {
+                const double* q = rotation + img * 4;
+                const double* t = translation + img * 3;
+                double qi[4] = {-q[0], -q[1], -q[2], q[3]};
+                double mt[3] = {-t[0], -t[1], -t[2]};
+                double a1[3], a2[3], v[3], ix, iy;
+                atdfo_rotate_point(qi, p, a1);
+                atdfo_rotate_point(qi, mt, a2);
+                for (int k = 0; k < 3; ++k) v[k] = a1[k] + a2[k];
+                if (camera_model == 0) {
+                    double invz = 1.0 / v[2];
+                    double x = v[0] * invz, y = v[1] * invz;
+                    ix = (fx * x + sk * y) + cx;
+                    iy = fy * y + cy;
+                } else {
+                    ix = (fx * v[0] + sk * v[1]) + cx;
+                    iy = fy * v[1] + cy;
+                }
+                double rx = round(ix), ry = round(iy);
+                if (rx >= 0.0 && rx < (double)w && ry >= 0.0 && ry < (double)h) vis = 1.f;
+            }
+        }
+        mask[i] = vis;
+    }
+}

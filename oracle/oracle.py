@@ -555,3 +555,22 @@ def random_pixels(indices, image_camera, w, h, n, seed, interpolate=True, align_
                               C.c_int(int(align_corners)), C.c_int(int(interpolate)), C.c_uint64(seed),
                               C.c_int64(n), _p(uv), _p(img), _p(cam))
     return uv, img, cam
+
+
+def point_in_any_image(camera_model, w, h, intrinsics, rotation, translation, roi_min, roi_max, points):
+    points = _c(points, np.float32).reshape(-1, 3)
+    rotation = _c(rotation, np.float64)
+    mask = np.zeros(points.shape[0], np.float32)
+    lib().atdfo_point_in_any_image(C.c_int(camera_model), C.c_int(w), C.c_int(h), _p(_c(intrinsics, np.float64)),
+                                   _p(rotation), _p(_c(translation, np.float64)), C.c_int64(rotation.shape[0]),
+                                   _p(_c(roi_min, np.float32)), _p(_c(roi_max, np.float32)), _p(points),
+                                   C.c_int64(points.shape[0]), _p(mask))
+    return mask
+
+
+def node_lattice_points(node_min, node_max, active_ids, n):
+    ids = _c(active_ids, np.int64)
+    pos = np.zeros((ids.size, n * n * n, 3), np.float32)
+    lib().atdfo_node_lattice_points(_p(_c(node_min, np.float32)), _p(_c(node_max, np.float32)), _p(ids),
+                                    C.c_int64(ids.size), C.c_int(n), _p(pos))
+    return pos

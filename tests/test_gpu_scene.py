@@ -217,3 +217,69 @@ def test_scene_directory_end_to_end(oracle, native, cuda, tmp_path):
     t2, m2 = s.SampleProjection(b.uv, b.image_id)
     assert torch.equal(t2, b.target) and torch.equal(m2, b.target_mask)
     assert math.isclose(float(b.uv[:, 0].mul(w - 1).round().max()), w - 1)
+
+
+def _visibility_scene(tmp_path, model):
+    # a narrow detector (visible |x| <~ 0.5 in view space) and the ROI's
+    # z limit of +-0.5 leave part of the cube unseen
+    w, h = 24, 40
+    K = (w - 1.0, (h - 1) / 2.0, (w - 1) / 2.0, (h - 1) / 2.0, 0.3) if model == "orthographic" else \
+        (20.0, 20.0, (w - 1) / 2.0, (h - 1) / 2.0, 0.3)
+    root, _, _ = write_scene(str(tmp_path / f"vis_{model}"), camera_model=model, w=w, h=h, num_images=3, K=K)
+    return root
+
+
+@pytest.mark.parametrize("model", ["orthographic", "pinhole"])
+def test_point_in_any_image_bit_exact(oracle, native, cuda, tmp_path, model):
+    from scene_util import adaptive_active, device_tree
+
+    s = native.SceneBase(_visibility_scene(tmp_path, model))
+    s.Finalize("cuda:0", [0.1, 0.0, -0.05])
+    t = oracle.tree_build(3)
+    act = adaptive_active(t, budget=200, seed=3)
+    tree = device_tree(3, act)
+    ids = oracle.active_ids(act)
+    pos = tree.UniformGlobalSamples(torch.from_numpy(ids).to(cuda), 7)
+    want_pos = oracle.node_lattice_points(t["node_min"], t["node_max"], ids, 7)
+    assert tuple(pos.shape) == (ids.size, 7, 7, 7, 3)
+    assert np.array_equal(pos.reshape(ids.size, -1, 3).cpu().numpy(), want_pos)
+    mask = s.PointInAnyImage(pos)
+    assert tuple(mask.shape) == (ids.size, 7, 7, 7)
+    dp = s.dataset_params
+    want = oracle.point_in_any_image(s.CameraModelId(), 24, 40, s.intrinsics.cpu().numpy(), s.rotation.cpu().numpy(),
+                                     s.translation.cpu().numpy(), dp.roi_min, dp.roi_max, want_pos.reshape(-1, 3))
+    got = mask.reshape(-1).cpu().numpy()
+    print(f"{model}: visible {got.mean():.3f}", flush=True)
+    assert np.array_equal(got, want)
+    assert 0.05 < got.mean() < 0.95
+
+
+def test_compute_max_density_with_visibility(oracle, native, cuda, tmp_path):
+    from test_gpu_project import make_case
+
+    cs = make_case(oracle, C_=16, seed=41, budget=200)
+    s = native.SceneBase(_visibility_scene(tmp_path, "orthographic"))
+    s.Finalize("cuda:0", [0.0, 0.0, 0.0])
+    grid, mlp, tree = cs["grid"].cuda(), cs["mlp"].cuda(), cs["tree"]
+    n = 9
+    native.ComputeMaxDensity(tree, grid, mlp, cs["fp"], n, s)
+    ids = oracle.active_ids(cs["act"])
+    gmax = tree.node_max_density.cpu().numpy()[ids]
+    fm = oracle.FusedModel(cs["t"]["node_min"], cs["t"]["node_max"], cs["act"], cs["grid"].numpy(),
+                           cs["mlp"].numpy(), 16)
+    _, _, _, dens = oracle.node_density_stats(fm, 0, n, want_densities=True)
+    pts = oracle.node_lattice_points(cs["t"]["node_min"], cs["t"]["node_max"], ids, n).reshape(-1, 3)
+    dp = s.dataset_params
+    vis = oracle.point_in_any_image(1, 24, 40, s.intrinsics.cpu().numpy(), s.rotation.cpu().numpy(),
+                                    s.translation.cpu().numpy(), dp.roi_min, dp.roi_max, pts)
+    omax = (dens.reshape(ids.size, -1) * vis.reshape(ids.size, -1)).max(1)
+    unseen = vis.reshape(ids.size, -1).max(1) == 0
+    print(f"nodes {ids.size}, unseen {unseen.sum()}, partly seen "
+          f"{((vis.reshape(ids.size, -1).min(1) == 0) & ~unseen).sum()}", flush=True)
+    assert unseen.any() and not unseen.all()
+    assert np.all(gmax[unseen] == 0)
+    assert np.abs(gmax - omax).max() <= 1e-5 * np.abs(omax).max()
+    # without a scene the same call takes every lattice point
+    native.ComputeMaxDensity(tree, grid, mlp, cs["fp"], n)
+    full = tree.node_max_density.cpu().numpy()[ids]
+    assert np.all(full >= gmax) and np.any(full > gmax)

@@ -315,3 +315,58 @@ def test_scene_boundary_without_device_work():
     st = lib.atdf_random_ray_batch(None, None, one, 2, None, 8, 8, 1, 1, 0, 4, one, one, one, one, None, None,
                                    None, None)
     assert st == capi.ERR_INVALID_ARGUMENT and b"projection set" in lib.atdf_last_error()
+
+
+def test_point_in_any_image_closed_forms(oracle):
+    """Visibility against the projection written with the rotation matrix:
+    R^T (p - t) mapped through K and rounded half away from zero."""
+    rng = np.random.default_rng(12)
+    w, h = 30, 20
+    roi_min, roi_max = np.float32([-1, -1, -1]), np.float32([1, 1, 1])
+    q = _unit_quats(1, 13)
+    R = _rotmat(q[0])
+    # pinhole: camera 4 units back along its own z axis
+    t = -4.0 * R[:, 2]
+    K = np.array([[60.0, 58.0, 14.5, 9.5, 0.7]])  # skew ignored
+    # pixel targets away from rounding ties, some inside, some outside
+    px = np.concatenate([rng.uniform(-3, w + 3, 400), [-0.6, -0.4, w - 0.6, w - 0.4]])
+    py = np.concatenate([rng.uniform(-3, h + 3, 400), [5.2, 5.2, 5.2, 5.2]])
+    z = rng.uniform(3.5, 4.5, px.size)
+    view = np.stack([(px - K[0, 2]) / K[0, 0] * z, (py - K[0, 3]) / K[0, 1] * z, z], 1)
+    pts = (view @ R.T + t).astype(np.float32)
+    inside_roi = np.all(np.abs(pts) <= 1, 1)
+    vis = oracle.point_in_any_image(0, w, h, K, q, t[None], roi_min, roi_max, pts)
+    # recompute the pixel from the float32 points, then round half away from zero
+    v = (pts.astype(np.float64) - t) @ R
+    ix, iy = K[0, 0] * v[:, 0] / v[:, 2] + K[0, 2], K[0, 1] * v[:, 1] / v[:, 2] + K[0, 3]
+    rx, ry = np.sign(ix) * np.floor(np.abs(ix) + 0.5), np.sign(iy) * np.floor(np.abs(iy) + 0.5)
+    want = inside_roi & (rx >= 0) & (rx < w) & (ry >= 0) & (ry < h)
+    assert np.array_equal(vis, want.astype(np.float32))
+    assert 0.15 < want.mean() < 0.85
+    assert list(vis[-4:]) == [0.0, 1.0, 1.0, 0.0] or not inside_roi[-4:].all()
+    # orthographic: the same with (fx x + cx, fy y + cy); "any" over images
+    Ko = np.array([[8.0, 8.0, 14.5, 9.5, 0.0]])
+    q2 = np.stack([tilt_quaternion(0.0), tilt_quaternion(math.pi / 2)])
+    t2 = np.zeros((2, 3))
+    p = np.float32([[1.5, 0.0, 0.0], [0.0, 0.0, 0.5], [0.9, 0.0, 0.9], [0.1, -0.2, 0.3], [0.2, 1.3, 0.0]])
+    # image 0 sees -1.875 < x < 1.875 (pixels 0..29 after rounding), image 1
+    # (90 degrees about y) sees z in the same range
+    vis = oracle.point_in_any_image(1, w, h, Ko, q2, t2, np.float32([-2] * 3), np.float32([2] * 3), p)
+    assert list(vis) == [1.0, 1.0, 1.0, 1.0, 0.0]
+    vis = oracle.point_in_any_image(1, w, h, Ko, q2[:1], t2[:1], np.float32([-2] * 3), np.float32([2] * 3),
+                                    np.float32([[1.9, 0, 0], [-1.9, 0, 0], [1.8, 0, 0]]))
+    assert list(vis) == [0.0, 0.0, 1.0]
+    # the ROI box clips first
+    vis = oracle.point_in_any_image(1, w, h, Ko, q2, t2, np.float32([-0.5] * 3), np.float32([0.5] * 3), p)
+    assert list(vis) == [0.0, 1.0, 0.0, 1.0, 0.0]
+
+
+def test_node_lattice_points_closed_form(oracle):
+    t = oracle.tree_build(2)
+    ids = np.int64([0, 1, 12])
+    pos = oracle.node_lattice_points(t["node_min"], t["node_max"], ids, 3)
+    for a, node in enumerate(ids):
+        mn, mx = t["node_min"][node].astype(np.float64), t["node_max"][node].astype(np.float64)
+        for p in range(27):
+            idx = np.array([p % 3, (p // 3) % 3, p // 9])
+            assert np.array_equal(pos[a, p], (mn + idx / 2.0 * (mx - mn)).astype(np.float32))
