@@ -64,6 +64,11 @@ class ProjectionSet(C.Structure):
                 ("num_images", C.c_int64), ("align_corners", C.c_int32)]
 
 
+class TVShape(C.Structure):
+    _fields_ = [("B", C.c_int64), ("C", C.c_int64), ("D", C.c_int32), ("n", C.c_int32 * 3), ("stride_b", C.c_int64),
+                ("stride_c", C.c_int64), ("stride_s", C.c_int64 * 3)]
+
+
 class Intervals(C.Structure):
     _fields_ = [("node", C.c_void_p), ("tmin", C.c_void_p), ("tmax", C.c_void_p), ("count", C.c_void_p),
                 ("first", C.c_void_p), ("ray", C.c_void_p), ("total", C.c_int64)]
@@ -148,6 +153,8 @@ _SIGNATURES = {
     "atdf_node_lattice_points": (_I32, [_P, _I64, _P, _I64, _I32, _P, _P]),
     "atdf_node_density_stats_masked": (_I32, [_P, _I64, _P, _I64, C.POINTER(Field), _I32, _I32, _I32, C.c_uint64,
                                               _P, _P, _P, _P, _P]),
+    "atdf_tv_workspace_bytes": (_SZ, [_I64]),
+    "atdf_tv_loss": (_I32, [_P, C.POINTER(TVShape), _P, C.c_double, _P, _P, _P, _SZ, _P]),
     "atdf_grid_layout": (_I32, [_P, _P, _I64, _I32, _I32, _I32, _P]),
     "atdf_f32_to_bf16": (_I32, [_P, _P, _I64, _P]),
 }

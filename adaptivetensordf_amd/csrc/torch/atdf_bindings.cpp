@@ -224,6 +224,24 @@ PYBIND11_MODULE(TORCH_EXTENSION_NAME, m)
     m.def("GradientCorrection", &GradientCorrection);
     m.def("ComputeBoundWeight", &ComputeBoundWeight);
     m.def("SampleVolumeIndirectFeature", &SampleVolumeIndirectFeature);
+    m.def(
+        "TVLoss",
+        [](torch::Tensor grid, std::optional<torch::Tensor> weight, bool channels_last) {
+            return TVLoss(grid, weight ? *weight : torch::Tensor(), channels_last);
+        },
+        py::arg("grid"), py::arg("weight") = py::none(), py::arg("channels_last") = false);
+    py::class_<RegularizerParams>(m, "RegularizerParams")
+        .def(py::init<>())
+        .def_readwrite("loss_tv", &RegularizerParams::loss_tv)
+        .def_readwrite("loss_l2", &RegularizerParams::loss_l2)
+        .def_readwrite("test_line_x", &RegularizerParams::test_line_x)
+        .def_readwrite("test_line_y", &RegularizerParams::test_line_y)
+        .def_readwrite("test_line_z", &RegularizerParams::test_line_z)
+        .def_readwrite("plane_vec_only", &RegularizerParams::plane_vec_only)
+        .def_readwrite("use_loss_tv_3d", &RegularizerParams::use_loss_tv_3d)
+        .def_readwrite("test_para", &RegularizerParams::test_para)
+        .def_readwrite("tv_size", &RegularizerParams::tv_size);
+    m.def("VolumeRegularizer", &VolumeRegularizer);
     m.def("IntegrateSamplesXRay", &IntegrateSamplesXRay);
     m.def("AccumulateSampleLossPerNode", &AccumulateSampleLossPerNode);
     m.def("ProjectRays", &ProjectRays);

@@ -203,6 +203,27 @@ struct VMFields {
 torch::Tensor SampleVolumeIndirectFeature(HyperTreeBase& tree, const RoiMaps& roi, const VMFields& fields,
                                           torch::Tensor global_coordinate, torch::Tensor node_id);
 
+// TVLoss::forward (geometry/geometry.h:31-81) on grid [B, C, n...] (1 to 3
+// spatial dims; channels_last: [B, n..., C]) with an optional per-batch weight,
+// as an autograd Function (loss and gradient from one atdf_tv_loss launch).
+torch::Tensor TVLoss(torch::Tensor grid, torch::Tensor weight = {}, bool channels_last = false);
+// The regulariser switches of TrainParams (Settings.h:429-607).
+struct RegularizerParams {
+    double loss_tv = 0.0005;
+    double loss_l2 = -0.001;
+    float test_line_x = 1, test_line_y = 1, test_line_z = 1;
+    bool plane_vec_only = true;
+    bool use_loss_tv_3d = true;
+    float test_para = -1;
+    std::vector<int64_t> tv_size{16, 16, 16};  // lattice of the 3-D TV: vec_grid_size * loss_tv_scale
+};
+// GeometryTensorQO::VolumeRegularizer (geometry_tensor_qo.cpp:527-622 and
+// 913-922): the TV terms (3-D TV of the features on a lattice of each in-ROI
+// node, or TV of the planes, lines and explicit grids) and the L2 terms.
+// The edge and Fourier terms are not built.  Undefined when no term is on.
+torch::Tensor VolumeRegularizer(HyperTreeBase& tree, const RoiMaps& roi, const VMFields& fields,
+                                const RegularizerParams& p);
+
 // NeuralGeometry::IntegrateSamplesXRay (geometry.cpp:977-1019)
 torch::Tensor IntegrateSamplesXRay(torch::Tensor sample_values, torch::Tensor integration_weight,
                                    torch::Tensor ray_index, int num_channels, int64_t num_rays);

@@ -615,6 +615,31 @@ atdf_status atdf_adam_step_bf16(float* master, uint16_t* param_bf16, float* grad
                                 float* exp_avg_sq, int64_t n, float lr, float beta1, float beta2, float eps,
                                 int64_t step, int32_t zero_grad, atdf_stream_t stream);
 
+/* ======================================================================= */
+/* Regulariser                                                              */
+/* ======================================================================= */
+/* A batch of D-dimensional (D = 1..3) feature grids x[b, c, s_0 .. s_{D-1}]
+ * at element offset b * stride_b + c * stride_c + sum_d s_d * stride_s[d]
+ * (channels-first [B, C, ...] and channels-last [B, ..., C] alike). */
+typedef struct {
+    int64_t B, C;
+    int32_t D;
+    int32_t n[3];
+    int64_t stride_b, stride_c;
+    int64_t stride_s[3];
+} atdf_tv_shape;
+
+/* TVLoss::forward (geometry/geometry.h:31-81), as VolumeRegularizer applies
+ * it to VM planes, lines and explicit grids (geometry_tensor_qo.cpp:
+ * 527-607):
+ *   loss[0] = scale * C * sum_d mean_b( w_b * mean_{c,s} |x[s] - x[s + e_d]| )
+ * in FP64 (weight [B] nullable = 1; every n_d >= 2).  With grad non-null
+ * the gradient (subgradient 0 at equal neighbours) is ADDED to grad, same
+ * layout as x.  Deterministic (fixed-order reductions). */
+size_t atdf_tv_workspace_bytes(int64_t num_elements);
+atdf_status atdf_tv_loss(const float* x, const atdf_tv_shape* shape, const float* weight, double scale, double* loss,
+                         float* grad, void* workspace, size_t workspace_bytes, atdf_stream_t stream);
+
 /* [N, C, G, G, G] <-> [N, G, G, G, C] (direction 0: to channels-last). */
 atdf_status atdf_grid_layout(const float* src, float* dst, int64_t N, int32_t C, int32_t G, int32_t direction,
                              atdf_stream_t stream);

@@ -290,6 +290,10 @@ void atdfo_point_in_any_image(int camera_model, int w, int h, const double* intr
  * (the points of atdfo_node_density_stats mode 0); pos [num_active, n^3, 3]. */
 void atdfo_node_lattice_points(const float* node_min, const float* node_max, const int64_t* active_ids,
                                int64_t num_active, int n, float* pos);
+/* TVLoss of a strided [B, C, n...] grid (D = 1..3): returns the FP64 loss,
+ * adds the gradient to grad (nullable). */
+double atdfo_tv_loss(const float* x, int64_t B, int64_t C, int D, const int* n, int64_t sb, int64_t sc,
+                     const int64_t* ss, const float* weight, double scale, float* grad);
 /* Random batch draws (image entry, uv) of the batch kernel. */
 void atdfo_random_pixels(const int32_t* indices, int64_t num_indices, const int32_t* image_camera, int w, int h,
                          int align_corners, int interpolate, uint64_t seed, int64_t n, float* uv, int32_t* image_id,

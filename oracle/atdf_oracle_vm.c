@@ -140,3 +140,55 @@ void atdfo_compute_bound_weight(const int64_t* node_id, int64_t S, const int64_t
             if (node_id[s] == in_roi_node[i]) out[s] = s;
     }
 }
+
+/* ---- TVLoss::forward (geometry/geometry.h:31-81) -----------------------
+ * x[b, c, s...] at b*sb + c*sc + sum s_d*ss[d]; loss (FP64, sequential sum)
+ * = scale * sum over elements and dimensions of w_b * coef_d * |x[s] -
+ * x[s + e_d]|, coef_d = C / (B * C * (n_d - 1) * prod_{k != d} n_k); grad
+ * (nullable, added) gathered per element as sum_d (float)(scale * coef_d) *
+ * (sgn(x[s] - x[s + e_d]) - sgn(x[s - e_d] - x[s])) * w_b. */
+static float sgnf_(float v) { return v > 0.f ? 1.f : (v < 0.f ? -1.f : 0.f); }
+
+double atdfo_tv_loss(const float* x, int64_t B, int64_t C, int D, const int* n, int64_t sb, int64_t sc,
+                     const int64_t* ss, const float* weight, double scale, float* grad)
+{
+    double coef[3];
+    float gcoef[3];
+    int64_t per = 1;
+    for (int d = 0; d < D; ++d) per *= n[d];
+    for (int d = 0; d < D; ++d) {
+        double count = (double)C * (double)(n[d] - 1);
+        for (int k = 0; k < D; ++k)
+            if (k != d) count *= (double)n[k];
+        coef[d] = (double)C / ((double)B * count);
+        gcoef[d] = (float)(scale * coef[d]);
+    }
+    double acc = 0.0;
+    for (int64_t b = 0; b < B; ++b)
+        for (int64_t c = 0; c < C; ++c)
+            for (int64_t p = 0; p < per; ++p) {
+                int s[3] = {0, 0, 0};
+                int64_t r = p;
+                for (int d = D - 1; d >= 0; --d) {
+                    s[d] = (int)(r % n[d]);
+                    r /= n[d];
+                }
+                int64_t addr = b * sb + c * sc;
+                for (int d = 0; d < D; ++d) addr += (int64_t)s[d] * ss[d];
+                float v = x[addr];
+                float wb = weight ? weight[b] : 1.f;
+                float g = 0.f;
+                for (int d = 0; d < D; ++d) {
+                    float up = 0.f, dn = 0.f;
+                    if (s[d] + 1 < n[d]) {
+                        float diff = v - x[addr + ss[d]];
+                        acc += (double)fabsf(diff) * coef[d] * (double)wb;
+                        up = sgnf_(diff);
+                    }
+                    if (s[d] > 0) dn = sgnf_(x[addr - ss[d]] - v);
+                    g = g + gcoef[d] * (up - dn);
+                }
+                if (grad) grad[addr] = grad[addr] + g * wb;
+            }
+    return acc * scale;
+}

@@ -574,3 +574,27 @@ def node_lattice_points(node_min, node_max, active_ids, n):
     lib().atdfo_node_lattice_points(_p(_c(node_min, np.float32)), _p(_c(node_max, np.float32)), _p(ids),
                                     C.c_int64(ids.size), C.c_int(n), _p(pos))
     return pos
+
+
+def tv_loss(x, channels_last=False, weight=None, scale=1.0, grad=None):
+    """TVLoss of x [B, C, n...] (channels_last: [B, n..., C]); returns
+    (loss, grad) with the gradient added to `grad` (zeros by default)."""
+    x = _c(x, np.float32)
+    if channels_last:
+        nb, nc = x.shape[0], x.shape[-1]
+        n = list(x.shape[1:-1])
+        st = [s // 4 for s in x.strides]
+        sb, sc, ss = st[0], st[-1], st[1:-1]
+    else:
+        nb, nc = x.shape[0], x.shape[1]
+        n = list(x.shape[2:])
+        st = [s // 4 for s in x.strides]
+        sb, sc, ss = st[0], st[1], st[2:]
+    D = len(n)
+    g = np.zeros_like(x) if grad is None else _c(grad, np.float32).copy()
+    w = None if weight is None else _c(weight, np.float32)
+    fn = lib().atdfo_tv_loss
+    fn.restype = C.c_double
+    loss = fn(_p(x), C.c_int64(nb), C.c_int64(nc), C.c_int(D), _p(_c(n + [1] * (3 - D), np.int32)), C.c_int64(sb),
+              C.c_int64(sc), _p(_c(list(ss) + [0] * (3 - D), np.int64)), _p(w), C.c_double(scale), _p(g))
+    return loss, g

@@ -166,3 +166,105 @@ def test_sample_volume_indirect_feature(oracle, native, cuda, op):
         assert _rel(got[lk], rl) <= 1e-5, lk
     rgo, _ = oracle.grid_sample3d_bwd(arr["grid_out"], aux["out_slot"], aux["loc"][~flag], go[~flag])
     assert _rel(got["grid_out"], rgo) <= 1e-5
+
+
+@pytest.mark.parametrize("shape,cl", [((3, 8, 33), False), ((40, 8, 9, 9), False), ((30, 8, 5, 5, 5), False),
+                                      ((500, 11, 11, 11, 16), True)])
+def test_tv_loss_matches_oracle(oracle, cuda, shape, cl):
+    import ctypes as C
+
+    from adaptivetensordf_amd import capi
+
+    rng = np.random.default_rng(sum(shape))
+    x = rng.normal(size=shape).astype(np.float32)
+    x[..., 0] = x[..., 1]  # ties
+    X = torch.from_numpy(x).to(cuda)
+    nd = len(shape) - 2
+    if cl:
+        n, st = shape[1:-1], [s // 4 for s in x.strides]
+        sh = capi.TVShape(shape[0], shape[-1], nd, (C.c_int32 * 3)(*(list(n) + [1] * (3 - nd))), st[0], 1,
+                          (C.c_int64 * 3)(*(st[1:-1] + [0] * (3 - nd))))
+    else:
+        n, st = shape[2:], [s // 4 for s in x.strides]
+        sh = capi.TVShape(shape[0], shape[1], nd, (C.c_int32 * 3)(*(list(n) + [1] * (3 - nd))), st[0], st[1],
+                          (C.c_int64 * 3)(*(st[2:] + [0] * (3 - nd))))
+    for weight in (None, rng.uniform(0.5, 2.0, size=shape[0]).astype(np.float32)):
+        W = None if weight is None else torch.from_numpy(weight).to(cuda)
+        g0 = rng.normal(size=shape).astype(np.float32)
+        G = torch.from_numpy(g0).to(cuda)
+        loss = torch.zeros(1, dtype=torch.float64, device=cuda)
+        ws = torch.empty(capi.lib().atdf_tv_workspace_bytes(x.size), dtype=torch.uint8, device=cuda)
+        print(f"tv {shape} cl={cl} weight={weight is not None}", flush=True)
+        capi.call("atdf_tv_loss", capi.ptr(X), C.byref(sh), capi.ptr(W), 0.25, capi.ptr(loss), capi.ptr(G),
+                  capi.ptr(ws), ws.numel(), capi.stream_ptr())
+        want, wg = oracle.tv_loss(x, channels_last=cl, weight=weight, scale=0.25, grad=g0)
+        assert abs(loss.item() - want) <= 1e-12 * abs(want)
+        assert np.array_equal(G.cpu().numpy(), wg)
+
+
+def test_tv_loss_linear_field_full_size(native, cuda):
+    """Explicit grids at the bench's node count (8191 x 11^3 x 16,
+    channels-last): a linear field's loss is C * sum_d |a_d| whatever the
+    size, and the gradient vanishes inside every node."""
+    N, G, C_ = 8191, 11, 16
+    a = (0.5, -0.25, 0.125)
+    s = torch.arange(G, dtype=torch.float32, device=cuda)
+    field = a[0] * s.view(G, 1, 1) + a[1] * s.view(1, G, 1) + a[2] * s.view(1, 1, G)
+    x = (field.view(1, G, G, G, 1) + torch.arange(C_, device=cuda).view(1, 1, 1, 1, C_)).expand(N, -1, -1, -1, -1)
+    x = x.contiguous().requires_grad_()
+    loss = native.TVLoss(x, channels_last=True)
+    assert abs(loss.item() - C_ * sum(abs(v) for v in a)) <= 1e-6
+    loss.backward()
+    assert torch.all(x.grad[:, 1:-1, 1:-1, 1:-1, :] == 0) and torch.any(x.grad != 0)
+
+
+def _torch_tv(t):
+    nc, D = t.shape[1], t.dim() - 2
+    total = 0
+    for i in range(D):
+        d = i + 2
+        n = t.shape[d]
+        total = total + (t.narrow(d, 0, n - 1) - t.narrow(d, 1, n - 1)).abs().mean()
+    return total * nc
+
+
+@pytest.mark.parametrize("tv3d", [True, False])
+def test_volume_regularizer(oracle, native, cuda, tv3d):
+    t, act, tree, roi, f, arr, tr, lohi = _vm_setup(native, oracle, "add", seed=3)
+    p = native.RegularizerParams()
+    p.loss_tv, p.use_loss_tv_3d, p.tv_size = 0.01, tv3d, [7, 13, 7]
+    p.plane_vec_only, p.test_para, p.loss_l2 = False, 1.0, 0.02
+    p.test_line_x, p.test_line_y, p.test_line_z = 1.0, 0.5, 2.0
+    names = ["plane_yz", "plane_xz", "plane_xy", "line_x", "line_y", "line_z", "grid_out"]
+    fields = [getattr(f, k) for k in names]
+    loss = native.VolumeRegularizer(tree, roi, f, p)
+    got = torch.autograd.grad(loss, fields, allow_unused=True)
+    # the same terms written with libtorch ops
+    if tv3d:
+        ids = roi.in_roi_node_index
+        N = ids.numel()
+        lin = [torch.linspace(-1, 1, k, device=cuda) for k in p.tv_size]
+        mesh = torch.meshgrid(*lin, indexing="ij")
+        cell = torch.stack(mesh, -1)[None]
+        mn = tree.node_position_min.index_select(0, ids).view(N, 1, 1, 1, 3)
+        sz = tree.node_position_max.index_select(0, ids).view(N, 1, 1, 1, 3) - mn
+        pos = ((cell + 1) / 2 * sz + mn).reshape(-1, 3)
+        nid = ids.view(N, 1, 1, 1).expand(N, *p.tv_size).reshape(-1)
+        feat = native.SampleVolumeIndirectFeature(tree, roi, f, pos, nid).reshape(N, *p.tv_size, -1)
+        ref = _torch_tv(feat.movedim(-1, 1)) * p.loss_tv
+    else:
+        ref = sum(_torch_tv(getattr(f, k)) * p.loss_tv for k in ("plane_xz", "plane_yz", "plane_xy"))
+        for k, w in (("line_x", 1.0), ("line_y", 0.5), ("line_z", 2.0)):
+            ref = ref + _torch_tv(getattr(f, k).squeeze(-1)) * (p.loss_tv * w)
+        ref = ref + _torch_tv(f.grid_out) * p.loss_tv
+    for k in ("plane_xz", "plane_yz", "plane_xy", "line_x", "line_y", "line_z"):
+        ref = ref + getattr(f, k).square().mean() * p.loss_l2
+    ref = ref + f.grid_out.square().mean()
+    want = torch.autograd.grad(ref, fields, allow_unused=True)
+    print(f"tv3d={tv3d}: loss {loss.item():.6e} torch {ref.item():.6e}", flush=True)
+    assert abs(loss.item() - ref.item()) <= 1e-5 * abs(ref.item())
+    for k, a, b in zip(names, got, want):
+        if b is None:
+            assert a is None or torch.all(a == 0), k
+            continue
+        assert _rel(a.cpu().double().numpy(), b.cpu().double().numpy()) <= 1e-4, k

@@ -1,0 +1,74 @@
+"""TV regulariser on the CPU: the oracle against libtorch's own composition
+of TVLoss::forward (slices, abs, means, autograd; float32 differences,
+float64 means), closed forms (constant and linear fields), the
+subgradient at ties, and the C ABI's argument checks."""
+import ctypes as C
+
+import numpy as np
+import pytest
+import torch
+
+
+def _torch_tv(x, weight=None, channels_last=False):
+    t = torch.from_numpy(x).requires_grad_()
+    g = t.movedim(-1, 1) if channels_last else t
+    nc, D = g.shape[1], g.dim() - 2
+    total = 0
+    for i in range(D):
+        d = i + 2
+        n = g.shape[d]
+        diff = (g.narrow(d, 0, n - 1) - g.narrow(d, 1, n - 1)).abs().double()
+        loss = diff.mean(tuple(range(1, g.dim())))
+        if weight is not None:
+            loss = loss * torch.from_numpy(weight).double()
+        total = total + loss.mean()
+    total = total * nc
+    total.backward()
+    return total.item(), t.grad.numpy()
+
+
+@pytest.mark.parametrize("shape,cl", [((2, 3, 17), False), ((4, 5, 9, 7), False), ((3, 2, 5, 6, 4), False),
+                                      ((3, 5, 6, 4, 8), True), ((2, 9, 9, 16), True)])
+def test_tv_oracle_against_torch(oracle, shape, cl):
+    rng = np.random.default_rng(len(shape) * 10 + cl)
+    x = rng.normal(size=shape).astype(np.float32)
+    for weight in (None, rng.uniform(0.5, 2.0, size=shape[0]).astype(np.float32)):
+        loss, g = oracle.tv_loss(x, channels_last=cl, weight=weight, scale=0.37)
+        tl, tg = _torch_tv(x, weight, cl)
+        assert abs(loss - 0.37 * tl) <= 1e-12 * abs(tl)
+        assert np.abs(g - 0.37 * tg).max() <= 1e-6 * np.abs(tg).max()
+
+
+def test_tv_closed_forms(oracle):
+    # constant field: no loss, no gradient
+    loss, g = oracle.tv_loss(np.full((2, 3, 5, 4), 1.25, np.float32))
+    assert loss == 0.0 and np.all(g == 0)
+    # linear field a_d * s_d: every difference is |a_d|, loss = C * sum_d |a_d|;
+    # interior gradients cancel, only the faces keep one term
+    a = (0.5, -2.0, 0.25)
+    s = np.meshgrid(*[np.arange(n) for n in (4, 5, 6)], indexing="ij")
+    field = sum(ai * si for ai, si in zip(a, s)).astype(np.float32)
+    x = np.stack([field, field + 3.0])[None]  # B = 1, C = 2
+    loss, g = oracle.tv_loss(x)
+    assert loss == pytest.approx(2 * sum(abs(v) for v in a), rel=1e-15)
+    assert np.all(g[0, :, 1:-1, 1:-1, 1:-1] == 0) and np.any(g != 0)
+    # a tie takes the zero subgradient
+    loss, g = oracle.tv_loss(np.float32([[[1.0, 1.0, 3.0]]]))
+    assert loss == pytest.approx(1.0) and g[0, 0, 0] == 0.0
+
+
+def test_tv_boundary_without_device_work():
+    from adaptivetensordf_amd import capi
+
+    lib = capi.lib()
+    one = C.c_void_p(1)
+    sh = capi.TVShape(2, 3, 4, (C.c_int32 * 3)(5, 5, 5), 375, 125, (C.c_int64 * 3)(25, 5, 1))
+    st = lib.atdf_tv_loss(one, C.byref(sh), None, 1.0, one, None, one, 1 << 20, None)
+    assert st == capi.ERR_INVALID_ARGUMENT and b"D must be" in lib.atdf_last_error()
+    sh = capi.TVShape(2, 3, 2, (C.c_int32 * 3)(5, 1, 1), 15, 5, (C.c_int64 * 3)(1, 1, 0))
+    st = lib.atdf_tv_loss(one, C.byref(sh), None, 1.0, one, None, one, 1 << 20, None)
+    assert st == capi.ERR_INVALID_ARGUMENT and b"spatial size 1" in lib.atdf_last_error()
+    sh = capi.TVShape(2, 3, 1, (C.c_int32 * 3)(5, 1, 1), 15, 5, (C.c_int64 * 3)(1, 0, 0))
+    st = lib.atdf_tv_loss(one, C.byref(sh), None, 1.0, one, None, one, 0, None)
+    assert st == capi.ERR_INVALID_ARGUMENT and b"workspace" in lib.atdf_last_error()
+    assert lib.atdf_tv_workspace_bytes(0) == 0 and lib.atdf_tv_workspace_bytes(1000) == 8 * 4
