@@ -129,6 +129,12 @@ _SIGNATURES = {
     "atdf_adam_step_bf16": (_I32, [_P, _P, _P, _P, _P, _I64, _F, _F, _F, _F, _I64, _I32, _P]),
     "atdf_project_bwd_adam": (_I32, [_P, _I64, _P, _P, _I64, C.POINTER(Sampling), _P, C.POINTER(Intervals),
                                      C.POINTER(Field), _P, _P, _P, C.POINTER(GridAdam), _P, _SZ, _P]),
+    "atdf_project_prepare": (_I32, [_P, _I64, _P, _P, _I64, C.POINTER(Sampling), _P, C.POINTER(Intervals),
+                                    C.POINTER(Field), _P, _SZ, _P]),
+    "atdf_project_fwd_prepared": (_I32, [_P, _I64, _P, _P, _I64, C.POINTER(Sampling), _P, C.POINTER(Intervals),
+                                         C.POINTER(Field), _P, _P, _SZ, _P]),
+    "atdf_project_bwd_prepared": (_I32, [_P, _I64, _P, _P, _I64, C.POINTER(Sampling), _P, C.POINTER(Intervals),
+                                         C.POINTER(Field), _P, _P, _P, C.POINTER(GridAdam), _P, _SZ, _P]),
     "atdf_project_fwd_bins": (_I32, [_P, _I64, _P, _P, _I64, C.POINTER(Sampling), _P, C.POINTER(Intervals),
                                      C.POINTER(Field), _P, _P, _P]),
     "atdf_proposal_num_params": (_I64, []),

@@ -901,7 +901,8 @@ __host__ __device__ constexpr int weights_f32_floats()
 }
 
 // Sorted interval record k (three float4): {tmin, tmax, count, first sample
-// in the ray}, {origin, direction.x}, {direction.y, direction.z, dL/dP, ray}.
+// in the ray}, {origin, direction.x}, {direction.y, direction.z, interval id,
+// ray}.
 // Written by the counting sort's scatter pass, which reads the intervals in
 // list order (coalesced) and their rays (mostly the same few lines).
 struct RecordSrc {
@@ -912,7 +913,6 @@ struct RecordSrc {
     const int32_t* ray;
     const float* O;
     const float* D;
-    const float* dLdP;
     float4* rec;
 };
 
@@ -921,7 +921,7 @@ __device__ __forceinline__ void write_record(const RecordSrc& rs, int64_t i, int
     const int64_t r = rs.ray[i];
     rs.rec[k * 3] = make_float4(rs.tmin[i], rs.tmax[i], __int_as_float(rs.count[i]), __int_as_float(rs.first[i]));
     rs.rec[k * 3 + 1] = make_float4(rs.O[r * 3], rs.O[r * 3 + 1], rs.O[r * 3 + 2], rs.D[r * 3]);
-    rs.rec[k * 3 + 2] = make_float4(rs.D[r * 3 + 1], rs.D[r * 3 + 2], rs.dLdP[r], __int_as_float((int)r));
+    rs.rec[k * 3 + 2] = make_float4(rs.D[r * 3 + 1], rs.D[r * 3 + 2], __int_as_float((int)i), __int_as_float((int)r));
 }
 
 // make_sample_iv for a node whose box and grid offset are already known.
@@ -1093,7 +1093,7 @@ __global__ __launch_bounds__(64 * kWaves) void project_bwd_node_kernel(NodeBwdAr
                 const int jj = t - start;
                 make_sample_node(a, nd, gbase, __float_as_int(D.w), o, d, A.x, A.y, n_s, jj,
                                  __float_as_int(A.w) + jj, sc);
-                dp = D.z;
+                dp = b.dLdP[__float_as_int(D.w)];
             }
             if (ones)
                 cum += posn;
@@ -1240,6 +1240,172 @@ __global__ __launch_bounds__(64 * kWaves) void project_bwd_node_kernel(NodeBwdAr
         __syncthreads();
     }
     store_partials<C>(q, b.ws + ((int64_t)blockIdx.x * kWaves + wl) * NP, g, j, lane);
+}
+
+// ---------------------------------------------------------------------------
+// Node-major forward over the sorted interval records of the adjoint's
+// counting sort.  Workgroups stride over the chunk table; a wave maps its
+// tile samples to intervals exactly as the adjoint does, gathers (the node's
+// grid stays in L2 for the whole chunk), runs the decoder, and sums
+// w * softplus(y) per interval with a segmented scan over the tile's 16
+// samples.  An interval belongs to one wave and its samples come in order,
+// so each interval's sum is formed in a fixed order and stored once:
+// ivsum[interval id].  ray_sum_kernel then adds a ray's interval sums in list
+// order.  Deterministic; the ray's samples are summed per interval first, so
+// the rounding differs from the ray-major forward by a few ulp.
+// ---------------------------------------------------------------------------
+struct NodeFwdArgs {
+    ProjArgs a;
+    const int32_t* sorted;
+    const int4* chunks;
+    const int64_t* chunk_off;
+    int64_t num_slots;
+    const float4* rec;
+    float* ivsum;
+};
+
+template <int C, bool kBF16, bool kDecBF16>
+__global__ __launch_bounds__(64 * kWaves, 4) void project_fwd_node_kernel(NodeFwdArgs b)
+{
+    using SM = DecoderSmem<C, kDecBF16>;
+    constexpr int CPL = C / 4;
+    __shared__ __attribute__((aligned(16))) float s[SM::kF32];
+    __shared__ __attribute__((aligned(16))) __bf16 sb[SM::kBF16];
+    const ProjArgs& a = b.a;
+    const float4* __restrict__ rec = b.rec;
+    if constexpr (kDecBF16)
+        load_mlp_bf16_to_lds<C>(a.mlp, sb, s);
+    else
+        load_mlp_to_lds<C>(a.mlp, s);
+    __syncthreads();
+    const int G = a.G;
+    const int gsz = G * G * G * C;
+    const int lane = threadIdx.x & 63;
+    const int g = lane >> 4, j = lane & 15;
+    const int wl = threadIdx.x >> 6;
+    const int64_t total_chunks = b.chunk_off[b.num_slots];
+    for (int64_t c = blockIdx.x; c < total_chunks; c += gridDim.x) {
+        const int4 ch = b.chunks[c];
+        const int64_t gbase = (int64_t)ch.x * gsz;
+        const PNode nd = a.nodes[a.iv_node[b.sorted[ch.y]]];
+        const int n_iv = ch.z - ch.y;
+        const int per = (n_iv + kWaves - 1) / kWaves;
+        const int ib = ch.y + wl * per;
+        const int ie = min(ib + per, ch.z);
+        int tot = 0;
+        for (int k = ib + lane; k < ie; k += 64) tot += __float_as_int(rec[3 * (int64_t)k].z);
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) tot += __shfl_xor(tot, off, 64);
+        const int ntiles = (tot + kTile - 1) / kTile;
+        int cur = ib, cum = 0;
+        const int grp = lane & 48;
+        // the wave's open interval (continuing into the next tile) and its sum
+        int carry_id = -1;
+        float carry = 0.f;
+        for (int ti = 0; ti < ntiles; ++ti) {
+            const int base = ti * kTile;
+            const int t = base + j;
+            const int cand = cur + j;
+            const int c_cnt = cand < ie ? __float_as_int(rec[3 * (int64_t)cand].z) : 0;
+            const bool ones = __all(cand >= ie || c_cnt == 1);
+            int c_end = c_cnt;
+            int pos, posn, s_cnt, s_end, last_end = 0;
+            if (ones) {
+                pos = j;
+                posn = min(kTile, ie - cur);
+                s_cnt = 1;
+                s_end = cum + j + 1;
+            } else {
+                c_end = scan16_inclusive(c_end);
+                c_end = cand < ie ? c_end + cum : INT_MAX;
+                pos = group_search(c_end, t, grp);
+                posn = group_search(c_end, base + kTile, grp);
+                const int src = grp | (pos < kTile ? pos : kTile - 1);
+                s_cnt = __shfl(c_cnt, src, 64);
+                s_end = __shfl(c_end, src, 64);
+                last_end = __shfl(c_end, grp | (kTile - 1), 64);
+            }
+            SampleCtx sc;
+            sc.valid = t < tot;
+            sc.w = 0.f;
+            sc.gbase = -1;
+            int iv_id = -1;
+            if (sc.valid) {
+                int k = cur + pos, n_s = s_cnt, start = s_end - s_cnt;
+                if (pos >= kTile) {
+                    k = cur + kTile;
+                    start = last_end;
+                    n_s = __float_as_int(rec[3 * (int64_t)k].z);
+                    while (start + n_s <= t) {
+                        start += n_s;
+                        ++k;
+                        n_s = __float_as_int(rec[3 * (int64_t)k].z);
+                    }
+                }
+                const float4 A = rec[3 * (int64_t)k];
+                const float4 B = rec[3 * (int64_t)k + 1];
+                const float4 D = rec[3 * (int64_t)k + 2];
+                const float o[3] = {B.x, B.y, B.z};
+                const float d[3] = {B.w, D.x, D.y};
+                const int jj = t - start;
+                make_sample_node(a, nd, gbase, __float_as_int(D.w), o, d, A.x, A.y, n_s, jj,
+                                 __float_as_int(A.w) + jj, sc);
+                iv_id = __float_as_int(D.z);
+            }
+            if (ones)
+                cum += posn;
+            else if (posn > 0)
+                cum = __shfl(c_end, grp | (posn - 1), 64);
+            cur += posn;
+            float x[CPL];
+            gather_channels<C, kBF16>(a, sc, CPL * g, x);
+            v4f h1[4], h2[4];
+            float y;
+            int opq = 0;
+            asm volatile("" : "+v"(opq));  // no hoisting of the weight reads
+            const float* so = s + opq;
+            const __bf16* sbo = sb + opq;
+            if constexpr (kDecBF16)
+                y = decoder_fwd_bf16<C>(sbo, so, x, g, j, h1, h2);
+            else
+                y = decoder_fwd<C>(so, x, g, j, h1, h2);
+            // segmented inclusive scan of w * density over the tile (lanes 0..15)
+            float v = sc.valid ? softplus_f(y, a.beta) * sc.w : 0.f;
+#pragma unroll
+            for (int off = 1; off < kTile; off <<= 1) {
+                const float ov = __shfl_up(v, off, kTile);
+                const int oid = __shfl_up(iv_id, off, kTile);
+                if (j >= off && oid == iv_id) v += ov;
+            }
+            const int next_id = __shfl_down(iv_id, 1, kTile);
+            const bool seg_end = sc.valid && (j == kTile - 1 || next_id != iv_id);
+            // the tile's first interval may continue the previous tile's last one
+            const int first_id = __shfl(iv_id, 0, 64);
+            if (carry_id >= 0 && carry_id != first_id && lane == 0) b.ivsum[carry_id] = carry;
+            if (seg_end && iv_id == carry_id) v += carry;
+            // the last valid segment stays open unless this is the wave's last tile
+            const int last_valid = min(kTile, tot - base) - 1;
+            const int last_id = __shfl(iv_id, last_valid, 64);
+            const float last_v = __shfl(v, last_valid, 64);
+            const bool final_tile = ti == ntiles - 1;
+            if (g == 0 && seg_end && (iv_id != last_id || final_tile)) b.ivsum[iv_id] = v;
+            carry_id = final_tile ? -1 : last_id;
+            carry = last_v;
+        }
+    }
+}
+
+__global__ __launch_bounds__(256) void ray_sum_kernel(const int64_t* __restrict__ iv_off,
+                                                      const int32_t* __restrict__ count,
+                                                      const float* __restrict__ ivsum, int64_t R,
+                                                      float* __restrict__ P)
+{
+    const int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (r >= R) return;
+    float acc = 0.f;
+    for (int64_t h = iv_off[r]; h < iv_off[r + 1]; ++h)
+        if (count[h] > 0) acc += ivsum[h];
+    P[r] = acc;
 }
 
 // ---- counting sort of intervals by grid slot, chunk table ----
@@ -1815,7 +1981,7 @@ atdf_status atdf_project_fwd_bins(const void* packed_tree, int64_t num_nodes, co
 
 namespace {
 struct BwdWorkspace {
-    size_t cnt, slot_off, sorted, chunk_cnt, chunk_off, desc, counter, stotal, scan, partials, rec, total;
+    size_t cnt, slot_off, sorted, chunk_cnt, chunk_off, desc, counter, stotal, scan, partials, rec, ivsum, total;
 };
 
 size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
@@ -1849,6 +2015,7 @@ BwdWorkspace bwd_layout(int64_t H, int64_t NA, int np)
     w.scan = take(atdf_scan_workspace_bytes(NA));
     w.partials = take(sizeof(float) * (size_t)persistent_blocks() * kWaves * np);
     w.rec = take(3 * sizeof(float4) * (size_t)(H > 0 ? H : 1));
+    w.ivsum = take(sizeof(float) * (size_t)(H > 0 ? H : 1));
     w.total = o;
     return w;
 }
@@ -1911,58 +2078,53 @@ size_t atdf_project_bwd_workspace_bytes(int64_t R, int64_t H, const atdf_field* 
 
 }  // extern "C"
 
-static atdf_status project_bwd_impl(const void* packed_tree, int64_t num_nodes, const float* O, const float* D,
-                                    int64_t R, const atdf_sampling* sp, const int64_t* iv_off,
-                                    const atdf_intervals* iv, const atdf_field* f, const float* dLdP,
-                                    float* grad_grid, float* grad_mlp, const atdf_grid_adam* adam, void* workspace,
-                                    size_t workspace_bytes, atdf_stream_t stream)
+// The counting sort of the intervals by grid slot, their records and the
+// chunk table, in the caller's workspace (shared by the node-major forward
+// and the adjoint).
+struct Prepared {
+    BwdWorkspace L;
+    unsigned char* w;
+    int np;
+    int64_t H, NA;
+    int32_t* chunk_cnt() const { return reinterpret_cast<int32_t*>(w + L.chunk_cnt); }
+    int32_t* sorted() const { return reinterpret_cast<int32_t*>(w + L.sorted); }
+    int64_t* chunk_off() const { return reinterpret_cast<int64_t*>(w + L.chunk_off); }
+    int4* desc() const { return reinterpret_cast<int4*>(w + L.desc); }
+    float4* rec() const { return reinterpret_cast<float4*>(w + L.rec); }
+};
+
+static atdf_status prepared_view(const atdf_intervals* iv, const atdf_field* f, void* workspace,
+                                 size_t workspace_bytes, int64_t R, Prepared& P, const char* who)
 {
-    atdf_status st = check_project_args(packed_tree, num_nodes, O, D, R, sp, iv_off, iv, f);
-    if (st) return st;
-    const int64_t n_grid = f->num_grids * (int64_t)f->G * f->G * f->G * f->C;
-    if (adam) {
-        ATDF_REQUIRE(adam->param && adam->exp_avg && adam->exp_avg_sq && adam->step >= 1,
-                     "atdf_project_bwd_adam: incomplete Adam state");
-        const bool bf16 = f->grid_dtype == ATDF_GRID_BF16;
-        ATDF_REQUIRE(bf16 ? (adam->param_bf16 && (const void*)adam->param_bf16 == f->grid)
-                          : (!adam->param_bf16 && (const void*)adam->param == f->grid),
-                     "atdf_project_bwd_adam: the field's grid must be adam->param (f32) or adam->param_bf16 (bf16)");
-        ATDF_REQUIRE(((uintptr_t)adam->param | (uintptr_t)adam->exp_avg | (uintptr_t)adam->exp_avg_sq |
-                      (uintptr_t)grad_grid) % 16 == 0,
-                     "atdf_project_bwd_adam: grid buffers must be 16-byte aligned");
-        if (R == 0) {  // no samples: every grid takes a zero-gradient step
-            return bf16 ? atdf_adam_step_bf16(adam->param, adam->param_bf16, grad_grid, adam->exp_avg,
-                                              adam->exp_avg_sq, n_grid, adam->lr, adam->beta1, adam->beta2,
-                                              adam->eps, adam->step, 1, stream)
-                        : atdf_adam_step(adam->param, grad_grid, adam->exp_avg, adam->exp_avg_sq, n_grid, adam->lr,
-                                         adam->beta1, adam->beta2, adam->eps, adam->step, 1, stream);
-        }
-    }
-    if (R == 0) return ATDF_OK;
-    ATDF_REQUIRE(dLdP && grad_grid && grad_mlp && workspace, "atdf_project_bwd: null buffer");
-    ATDF_REQUIRE(iv->total >= 0 && iv->ray, "atdf_project_bwd: intervals need total and ray");
-    const int64_t H = iv->total;
-    const int64_t NA = f->num_grids;
-    ATDF_REQUIRE(H < (int64_t)INT32_MAX, "atdf_project_bwd: too many intervals");
-    ATDF_REQUIRE(workspace_bytes >= atdf_project_bwd_workspace_bytes(R, H, f),
-                 "atdf_project_bwd: workspace too small");
-    const int np = f->C == 16 ? mlp_numel<16>() : mlp_numel<8>();
-    BwdWorkspace L = bwd_layout(H, NA, np);
-    unsigned char* w = reinterpret_cast<unsigned char*>(workspace);
+    ATDF_REQUIRE(workspace, "%s: null workspace", who);
+    ATDF_REQUIRE(iv->total >= 0 && iv->ray, "%s: intervals need total and ray", who);
+    ATDF_REQUIRE(iv->total < (int64_t)INT32_MAX, "%s: too many intervals", who);
+    ATDF_REQUIRE(workspace_bytes >= atdf_project_bwd_workspace_bytes(R, iv->total, f), "%s: workspace too small", who);
+    P.H = iv->total;
+    P.NA = f->num_grids;
+    P.np = f->C == 16 ? mlp_numel<16>() : mlp_numel<8>();
+    P.L = bwd_layout(P.H, P.NA, P.np);
+    P.w = reinterpret_cast<unsigned char*>(workspace);
+    return ATDF_OK;
+}
+
+static atdf_status prepare_impl(const Prepared& P, const float* O, const float* D, const atdf_intervals* iv,
+                                const atdf_field* f, atdf_stream_t stream)
+{
+    const int64_t H = P.H, NA = P.NA;
+    unsigned char* w = P.w;
+    const BwdWorkspace& L = P.L;
     int32_t* cnt = reinterpret_cast<int32_t*>(w + L.cnt);
     int64_t* slot_off = reinterpret_cast<int64_t*>(w + L.slot_off);
-    int32_t* sorted = reinterpret_cast<int32_t*>(w + L.sorted);
-    int32_t* chunk_cnt = reinterpret_cast<int32_t*>(w + L.chunk_cnt);
-    int64_t* chunk_off = reinterpret_cast<int64_t*>(w + L.chunk_off);
-    int4* desc = reinterpret_cast<int4*>(w + L.desc);
-    int* counter = reinterpret_cast<int*>(w + L.counter);
+    int32_t* sorted = P.sorted();
+    int32_t* chunk_cnt = P.chunk_cnt();
+    int64_t* chunk_off = P.chunk_off();
+    int4* desc = P.desc();
     void* scan_ws = w + L.scan;
-    float* partials = reinterpret_cast<float*>(w + L.partials);
     hipStream_t s = to_hip(stream);
     const size_t scan_bytes = atdf_scan_workspace_bytes(NA);
 
     hipLaunchKernelGGL(fill_i32_kernel, dim3((unsigned)div_up(NA, 256)), dim3(256), 0, s, cnt, NA, 0);
-    hipLaunchKernelGGL(fill_i32_kernel, dim3(1), dim3(256), 0, s, counter, 1, 0);
     // LDS-aggregated counting sort when a slot histogram fits in LDS
     const bool lds_sort = (int64_t)sizeof(int32_t) * NA <= (int64_t)max_lds_per_block();
     const size_t hist_lds = sizeof(int32_t) * (size_t)NA;
@@ -1975,11 +2137,10 @@ static atdf_status project_bwd_impl(const void* packed_tree, int64_t num_nodes, 
             hipLaunchKernelGGL(slot_hist_kernel, dim3((unsigned)div_up(H, 256)), dim3(256), 0, s, iv->node,
                                f->node_to_grid, H, cnt);
     }
-    st = atdf_exclusive_scan_i32(cnt, NA, slot_off, scan_ws, scan_bytes, stream);
+    atdf_status st = atdf_exclusive_scan_i32(cnt, NA, slot_off, scan_ws, scan_bytes, stream);
     if (st) return st;
     hipLaunchKernelGGL(fill_i32_kernel, dim3((unsigned)div_up(NA, 256)), dim3(256), 0, s, cnt, NA, 0);
-    float4* rec = reinterpret_cast<float4*>(w + L.rec);
-    const RecordSrc rs{iv->tmin, iv->tmax, iv->count, iv->first, iv->ray, O, D, dLdP, rec};
+    const RecordSrc rs{iv->tmin, iv->tmax, iv->count, iv->first, iv->ray, O, D, P.rec()};
     if (H > 0) {
         if (lds_sort)
             hipLaunchKernelGGL(slot_scatter_lds_kernel, dim3(sort_blocks), dim3(256), hist_lds, s, iv->node,
@@ -1999,17 +2160,31 @@ static atdf_status project_bwd_impl(const void* packed_tree, int64_t num_nodes, 
     if (st) return st;
     hipLaunchKernelGGL(chunk_desc_kernel, dim3((unsigned)div_up(NA, 256)), dim3(256), 0, s, slot_off, chunk_off, NA,
                        H, stotal, desc);
+    return check_launch("atdf_project_prepare");
+}
 
+static atdf_status bwd_node_impl(const Prepared& P, const void* packed_tree, const float* O, const float* D,
+                                 int64_t R, const atdf_sampling* sp, const int64_t* iv_off, const atdf_intervals* iv,
+                                 const atdf_field* f, const float* dLdP, float* grad_grid, float* grad_mlp,
+                                 const atdf_grid_adam* adam, atdf_stream_t stream)
+{
+    const int64_t NA = P.NA;
+    const int np = P.np;
+    const int64_t n_grid = f->num_grids * (int64_t)f->G * f->G * f->G * f->C;
+    hipStream_t s = to_hip(stream);
+    int* counter = reinterpret_cast<int*>(P.w + P.L.counter);
+    float* partials = reinterpret_cast<float*>(P.w + P.L.partials);
+    hipLaunchKernelGGL(fill_i32_kernel, dim3(1), dim3(256), 0, s, counter, 1, 0);
     NodeBwdArgs args{};
     args.a = make_args(packed_tree, O, D, R, sp, iv_off, iv, f);
     args.dLdP = dLdP;
-    args.sorted = sorted;
-    args.chunks = desc;
-    args.chunk_off = chunk_off;
+    args.sorted = P.sorted();
+    args.chunks = P.desc();
+    args.chunk_off = P.chunk_off();
     args.num_slots = NA;
     args.counter = counter;
     args.iv_ray = iv->ray;
-    args.rec = rec;
+    args.rec = P.rec();
     args.grad_grid = grad_grid;
     args.ws = partials;
     args.ad = GridAdam{};
@@ -2027,6 +2202,7 @@ static atdf_status project_bwd_impl(const void* packed_tree, int64_t num_nodes, 
         args.ad.bc2s = sqrtf(1.0f - (float)pow((double)adam->beta2, (double)adam->step));
         args.ad.enabled = 1;
     }
+    atdf_status st = ATDF_OK;
     const bool bf = f->grid_dtype == ATDF_GRID_BF16;
     const bool bd = f->decoder_dtype == ATDF_DECODER_BF16;
 #define ATDF_BWD(CC, BF, BD) st = launch_node_bwd<CC, BF, BD>(args, s)
@@ -2043,11 +2219,97 @@ static atdf_status project_bwd_impl(const void* packed_tree, int64_t num_nodes, 
         const int n4 = (int)(n_grid / NA / 4);
         const int bps = (int)div_up(n4, 256);
         hipLaunchKernelGGL(grid_adam_rest_kernel, dim3((unsigned)(NA * bps)), dim3(256), 0, s, args.ad, grad_grid,
-                           chunk_cnt, bps, n4);
+                           P.chunk_cnt(), bps, n4);
     }
     hipLaunchKernelGGL(reduce_partials_kernel, dim3((unsigned)div_up(np, kReduceParams)), dim3(256), 0, s, partials,
                        (int64_t)persistent_blocks() * kWaves, np, grad_mlp);
     return check_launch("atdf_project_bwd");
+}
+
+static atdf_status fwd_node_impl(const Prepared& P, const void* packed_tree, const float* O, const float* D,
+                                 int64_t R, const atdf_sampling* sp, const int64_t* iv_off, const atdf_intervals* iv,
+                                 const atdf_field* f, float* out, atdf_stream_t stream)
+{
+    hipStream_t s = to_hip(stream);
+    NodeFwdArgs args{};
+    args.a = make_args(packed_tree, O, D, R, sp, iv_off, iv, f);
+    args.sorted = P.sorted();
+    args.chunks = P.desc();
+    args.chunk_off = P.chunk_off();
+    args.num_slots = P.NA;
+    args.rec = P.rec();
+    args.ivsum = reinterpret_cast<float*>(P.w + P.L.ivsum);
+    // chunks are similar in work; a fixed grid strides over them
+    const int64_t max_chunks = P.H / kMinChunk + P.NA + 1;
+    const unsigned blocks = (unsigned)std::max<int64_t>(1, std::min<int64_t>(max_chunks, (int64_t)persistent_blocks() * 8));
+    dim3 grid(blocks), block(64 * kWaves);
+    const bool bf = f->grid_dtype == ATDF_GRID_BF16;
+    const bool bd = f->decoder_dtype == ATDF_DECODER_BF16;
+    if (P.H > 0) {
+#define ATDF_FN(CC, BF, BD) hipLaunchKernelGGL((project_fwd_node_kernel<CC, BF, BD>), grid, block, 0, s, args)
+        if (f->C == 16) {
+            if (bf) { if (bd) ATDF_FN(16, true, true); else ATDF_FN(16, true, false); }
+            else    { if (bd) ATDF_FN(16, false, true); else ATDF_FN(16, false, false); }
+        } else {
+            if (bf) { if (bd) ATDF_FN(8, true, true); else ATDF_FN(8, true, false); }
+            else    { if (bd) ATDF_FN(8, false, true); else ATDF_FN(8, false, false); }
+        }
+#undef ATDF_FN
+    }
+    hipLaunchKernelGGL(ray_sum_kernel, dim3((unsigned)div_up(R, 256)), dim3(256), 0, s, iv_off, iv->count,
+                       args.ivsum, R, out);
+    return check_launch("atdf_project_fwd_prepared");
+}
+
+static atdf_status check_adam(const atdf_field* f, const atdf_grid_adam* adam, const float* grad_grid)
+{
+    const bool bf16 = f->grid_dtype == ATDF_GRID_BF16;
+    ATDF_REQUIRE(adam->param && adam->exp_avg && adam->exp_avg_sq && adam->step >= 1,
+                 "atdf_project_bwd_adam: incomplete Adam state");
+    ATDF_REQUIRE(bf16 ? (adam->param_bf16 && (const void*)adam->param_bf16 == f->grid)
+                      : (!adam->param_bf16 && (const void*)adam->param == f->grid),
+                 "atdf_project_bwd_adam: the field's grid must be adam->param (f32) or adam->param_bf16 (bf16)");
+    ATDF_REQUIRE(((uintptr_t)adam->param | (uintptr_t)adam->exp_avg | (uintptr_t)adam->exp_avg_sq |
+                  (uintptr_t)grad_grid) % 16 == 0,
+                 "atdf_project_bwd_adam: grid buffers must be 16-byte aligned");
+    return ATDF_OK;
+}
+
+// no samples: every grid takes a zero-gradient step
+static atdf_status adam_without_samples(const atdf_field* f, const atdf_grid_adam* adam, float* grad_grid,
+                                        atdf_stream_t stream)
+{
+    const int64_t n_grid = f->num_grids * (int64_t)f->G * f->G * f->G * f->C;
+    return f->grid_dtype == ATDF_GRID_BF16
+               ? atdf_adam_step_bf16(adam->param, adam->param_bf16, grad_grid, adam->exp_avg, adam->exp_avg_sq,
+                                     n_grid, adam->lr, adam->beta1, adam->beta2, adam->eps, adam->step, 1, stream)
+               : atdf_adam_step(adam->param, grad_grid, adam->exp_avg, adam->exp_avg_sq, n_grid, adam->lr,
+                                adam->beta1, adam->beta2, adam->eps, adam->step, 1, stream);
+}
+
+static atdf_status project_bwd_impl(const void* packed_tree, int64_t num_nodes, const float* O, const float* D,
+                                    int64_t R, const atdf_sampling* sp, const int64_t* iv_off,
+                                    const atdf_intervals* iv, const atdf_field* f, const float* dLdP,
+                                    float* grad_grid, float* grad_mlp, const atdf_grid_adam* adam, void* workspace,
+                                    size_t workspace_bytes, bool prepared, atdf_stream_t stream)
+{
+    atdf_status st = check_project_args(packed_tree, num_nodes, O, D, R, sp, iv_off, iv, f);
+    if (st) return st;
+    if (adam) {
+        st = check_adam(f, adam, grad_grid);
+        if (st) return st;
+        if (R == 0) return adam_without_samples(f, adam, grad_grid, stream);
+    }
+    if (R == 0) return ATDF_OK;
+    ATDF_REQUIRE(dLdP && grad_grid && grad_mlp, "atdf_project_bwd: null buffer");
+    Prepared P;
+    st = prepared_view(iv, f, workspace, workspace_bytes, R, P, "atdf_project_bwd");
+    if (st) return st;
+    if (!prepared) {
+        st = prepare_impl(P, O, D, iv, f, stream);
+        if (st) return st;
+    }
+    return bwd_node_impl(P, packed_tree, O, D, R, sp, iv_off, iv, f, dLdP, grad_grid, grad_mlp, adam, stream);
 }
 
 extern "C" {
@@ -2058,7 +2320,7 @@ atdf_status atdf_project_bwd(const void* packed_tree, int64_t num_nodes, const f
                              void* workspace, size_t workspace_bytes, atdf_stream_t stream)
 {
     return project_bwd_impl(packed_tree, num_nodes, O, D, R, sp, iv_off, iv, f, dLdP, grad_grid, grad_mlp, nullptr,
-                            workspace, workspace_bytes, stream);
+                            workspace, workspace_bytes, false, stream);
 }
 
 atdf_status atdf_project_bwd_adam(const void* packed_tree, int64_t num_nodes, const float* O, const float* D,
@@ -2069,7 +2331,45 @@ atdf_status atdf_project_bwd_adam(const void* packed_tree, int64_t num_nodes, co
 {
     ATDF_REQUIRE(adam, "atdf_project_bwd_adam: null Adam state");
     return project_bwd_impl(packed_tree, num_nodes, O, D, R, sp, iv_off, iv, f, dLdP, grad_grid, grad_mlp, adam,
-                            workspace, workspace_bytes, stream);
+                            workspace, workspace_bytes, false, stream);
+}
+
+atdf_status atdf_project_prepare(const void* packed_tree, int64_t num_nodes, const float* O, const float* D,
+                                 int64_t R, const atdf_sampling* sp, const int64_t* iv_off, const atdf_intervals* iv,
+                                 const atdf_field* f, void* workspace, size_t workspace_bytes, atdf_stream_t stream)
+{
+    atdf_status st = check_project_args(packed_tree, num_nodes, O, D, R, sp, iv_off, iv, f);
+    if (st) return st;
+    if (R == 0) return ATDF_OK;
+    Prepared P;
+    st = prepared_view(iv, f, workspace, workspace_bytes, R, P, "atdf_project_prepare");
+    if (st) return st;
+    return prepare_impl(P, O, D, iv, f, stream);
+}
+
+atdf_status atdf_project_fwd_prepared(const void* packed_tree, int64_t num_nodes, const float* O, const float* D,
+                                      int64_t R, const atdf_sampling* sp, const int64_t* iv_off,
+                                      const atdf_intervals* iv, const atdf_field* f, float* P_out, void* workspace,
+                                      size_t workspace_bytes, atdf_stream_t stream)
+{
+    atdf_status st = check_project_args(packed_tree, num_nodes, O, D, R, sp, iv_off, iv, f);
+    if (st) return st;
+    if (R == 0) return ATDF_OK;
+    ATDF_REQUIRE(P_out, "atdf_project_fwd_prepared: null projection");
+    Prepared P;
+    st = prepared_view(iv, f, workspace, workspace_bytes, R, P, "atdf_project_fwd_prepared");
+    if (st) return st;
+    return fwd_node_impl(P, packed_tree, O, D, R, sp, iv_off, iv, f, P_out, stream);
+}
+
+atdf_status atdf_project_bwd_prepared(const void* packed_tree, int64_t num_nodes, const float* O, const float* D,
+                                      int64_t R, const atdf_sampling* sp, const int64_t* iv_off,
+                                      const atdf_intervals* iv, const atdf_field* f, const float* dLdP,
+                                      float* grad_grid, float* grad_mlp, const atdf_grid_adam* adam, void* workspace,
+                                      size_t workspace_bytes, atdf_stream_t stream)
+{
+    return project_bwd_impl(packed_tree, num_nodes, O, D, R, sp, iv_off, iv, f, dLdP, grad_grid, grad_mlp, adam,
+                            workspace, workspace_bytes, true, stream);
 }
 
 

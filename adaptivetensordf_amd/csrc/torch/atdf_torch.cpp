@@ -947,18 +947,29 @@ Tensor TrainStepIntervals(HyperTreeBase& tree, TrainState& st, const RayList& ra
     atdf_sampling s = sp.view();
     atdf_intervals v = iv.view();
     auto P = torch::empty({R}, rays.origin.options());
-    check_status(atdf_project_fwd(tree.packed(), tree.NumNodes(), rays.origin.data_ptr<float>(),
-                                  rays.direction.data_ptr<float>(), R, &s, iv.ray_offset.data_ptr<int64_t>(), &v, &f,
-                                  P.data_ptr<float>(), stream),
-                 "atdf_project_fwd");
+    auto ws = torch::empty({(int64_t)atdf_project_bwd_workspace_bytes(R, iv.num_intervals, &f)},
+                           torch::TensorOptions().dtype(torch::kUInt8).device(rays.origin.device()));
+    const int64_t* ivo = iv.ray_offset.data_ptr<int64_t>();
+    const float *O = rays.origin.data_ptr<float>(), *D = rays.direction.data_ptr<float>();
+    if (fp.node_forward) {
+        // one counting sort for both directions; node-major forward over it
+        check_status(atdf_project_prepare(tree.packed(), tree.NumNodes(), O, D, R, &s, ivo, &v, &f, ws.data_ptr(),
+                                          ws.numel(), stream),
+                     "atdf_project_prepare");
+        check_status(atdf_project_fwd_prepared(tree.packed(), tree.NumNodes(), O, D, R, &s, ivo, &v, &f,
+                                               P.data_ptr<float>(), ws.data_ptr(), ws.numel(), stream),
+                     "atdf_project_fwd_prepared");
+    } else {
+        check_status(atdf_project_fwd(tree.packed(), tree.NumNodes(), O, D, R, &s, ivo, &v, &f, P.data_ptr<float>(),
+                                      stream),
+                     "atdf_project_fwd");
+    }
     auto dP = torch::empty({R}, rays.origin.options());
     auto mse_ws = torch::empty({(int64_t)atdf_mse_workspace_bytes(R)},
                                torch::TensorOptions().dtype(torch::kUInt8).device(rays.origin.device()));
     check_status(atdf_mse_loss(P.data_ptr<float>(), target.data_ptr<float>(), R, 1.0f, dP.data_ptr<float>(),
                                st.loss.data_ptr<double>(), mse_ws.data_ptr(), mse_ws.numel(), stream),
                  "atdf_mse_loss");
-    auto ws = torch::empty({(int64_t)atdf_project_bwd_workspace_bytes(R, iv.num_intervals, &f)},
-                           torch::TensorOptions().dtype(torch::kUInt8).device(rays.origin.device()));
     st.step += 1;
     const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
     // adjoint and the grids' Adam step in one pass (grad_grid stays zero)
@@ -973,11 +984,16 @@ Tensor TrainStepIntervals(HyperTreeBase& tree, TrainState& st, const RayList& ra
     ga.beta2 = b2;
     ga.eps = eps;
     ga.step = st.step;
-    check_status(atdf_project_bwd_adam(tree.packed(), tree.NumNodes(), rays.origin.data_ptr<float>(),
-                                       rays.direction.data_ptr<float>(), R, &s, iv.ray_offset.data_ptr<int64_t>(), &v,
-                                       &f, dP.data_ptr<float>(), st.grad_grid.data_ptr<float>(),
-                                       st.grad_mlp.data_ptr<float>(), &ga, ws.data_ptr(), ws.numel(), stream),
-                 "atdf_project_bwd_adam");
+    if (fp.node_forward)
+        check_status(atdf_project_bwd_prepared(tree.packed(), tree.NumNodes(), O, D, R, &s, ivo, &v, &f,
+                                               dP.data_ptr<float>(), st.grad_grid.data_ptr<float>(),
+                                               st.grad_mlp.data_ptr<float>(), &ga, ws.data_ptr(), ws.numel(), stream),
+                     "atdf_project_bwd_prepared");
+    else
+        check_status(atdf_project_bwd_adam(tree.packed(), tree.NumNodes(), O, D, R, &s, ivo, &v, &f,
+                                           dP.data_ptr<float>(), st.grad_grid.data_ptr<float>(),
+                                           st.grad_mlp.data_ptr<float>(), &ga, ws.data_ptr(), ws.numel(), stream),
+                     "atdf_project_bwd_adam");
     check_status(atdf_adam_step(st.mlp.data_ptr<float>(), st.grad_mlp.data_ptr<float>(), st.m_mlp.data_ptr<float>(),
                                 st.v_mlp.data_ptr<float>(), st.mlp.numel(), st.lr_mlp, b1, b2, eps, st.step, 1, stream),
                  "atdf_adam_step");

@@ -241,6 +241,7 @@ struct FieldParams {
     float softplus_beta = 2.0f;
     int rays_per_wave = 0;
     bool bf16_decoder = false;  // BF16 MFMA inputs, FP32 accumulation
+    bool node_forward = false;  // training step: node-major forward over the adjoint's sort
 };
 
 // C-ABI view of a field (checks shapes and dtypes).

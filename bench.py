@@ -59,6 +59,8 @@ def parse():
     ap.add_argument("--budget", type=int, default=8192, help="max active octree nodes")
     ap.add_argument("--samples-per-ray", type=float, default=128.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--node-major-forward", action="store_true",
+                    help="training step with the node-major forward over the adjoint's sort")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--profile-steps", type=int, default=0, help="only run this many steps (profiling)")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -141,6 +143,7 @@ def main():
         fp = N.FieldParams()
         fp.C, fp.G, fp.H = C, G, H
         fp.bf16_decoder = bf16
+        fp.node_forward = args.node_major_forward
         return fp
 
     steps = args.profile_steps if args.profile_steps > 0 else args.steps
@@ -368,15 +371,28 @@ def kernel_times(N, tree, rays, sp, fp, st, bf16, ps):
     ws = torch.empty(capi.lib().atdf_project_bwd_workspace_bytes(R, iv.num_intervals, Ct.byref(f)),
                      dtype=torch.uint8, device=dev)
 
+    args = (capi.ptr(packed), tree.NumNodes(), capi.ptr(rays.origin), capi.ptr(rays.direction), R, Ct.byref(csp),
+            capi.ptr(iv.ray_offset), Ct.byref(civ), Ct.byref(f))
+    node_fwd = fp.node_forward
+
+    # the calls the training step makes: with the node-major forward, one
+    # counting sort (prepare) serves the forward and the adjoint
+    def prep():
+        capi.call("atdf_project_prepare", *args, capi.ptr(ws), ws.numel(), s)
+
     def fwd():
-        capi.call("atdf_project_fwd", capi.ptr(packed), tree.NumNodes(), capi.ptr(rays.origin),
-                  capi.ptr(rays.direction), R, Ct.byref(csp), capi.ptr(iv.ray_offset), Ct.byref(civ), Ct.byref(f),
-                  capi.ptr(P), s)
+        if node_fwd:
+            capi.call("atdf_project_fwd_prepared", *args, capi.ptr(P), capi.ptr(ws), ws.numel(), s)
+        else:
+            capi.call("atdf_project_fwd", *args, capi.ptr(P), s)
 
     def bwd():
-        capi.call("atdf_project_bwd", capi.ptr(packed), tree.NumNodes(), capi.ptr(rays.origin),
-                  capi.ptr(rays.direction), R, Ct.byref(csp), capi.ptr(iv.ray_offset), Ct.byref(civ), Ct.byref(f),
-                  capi.ptr(dP), capi.ptr(gg), capi.ptr(gm), capi.ptr(ws), ws.numel(), s)
+        if node_fwd:
+            capi.call("atdf_project_bwd_prepared", *args, capi.ptr(dP), capi.ptr(gg), capi.ptr(gm), None,
+                      capi.ptr(ws), ws.numel(), s)
+        else:
+            capi.call("atdf_project_bwd", *args, capi.ptr(dP), capi.ptr(gg), capi.ptr(gm), capi.ptr(ws), ws.numel(),
+                      s)
 
     def timed(fn, reps=5):
         fn()
@@ -390,6 +406,9 @@ def kernel_times(N, tree, rays, sp, fp, st, bf16, ps):
         e1.synchronize()
         return e0.elapsed_time(e1) / reps / 1e3
 
+    t_prep = timed(prep) if node_fwd else 0.0
+    if node_fwd:
+        prep()
     t_fwd = timed(fwd)
     t_bwd = timed(bwd)
     b = 2 if bf16 else 4
@@ -406,13 +425,17 @@ def kernel_times(N, tree, rays, sp, fp, st, bf16, ps):
     achieved_gbs = bytes_dom / t_dom / 1e9
     achieved_tflops = flop_dom / t_dom / 1e12
     mfma_peak = BF16_PEAK_TFLOPS if bf16 else FP32_PEAK_TFLOPS
-    log(f"  project_fwd {t_fwd * 1e3:.2f} ms, project_bwd {t_bwd * 1e3:.2f} ms ({S_total / R:.1f} samples/ray, "
+    log(f"  {'project_prepare %.2f ms, ' % (t_prep * 1e3) if node_fwd else ''}"
+        f"project_fwd{'_prepared' if node_fwd else ''} {t_fwd * 1e3:.2f} ms, "
+        f"project_bwd{'_prepared' if node_fwd else ''} {t_bwd * 1e3:.2f} ms ({S_total / R:.1f} samples/ray, "
         f"{iv.num_intervals} intervals); dominant {dom}: {achieved_gbs:.0f} GB/s algorithmic, "
         f"{achieved_tflops:.1f} TFLOP/s decoder")
     mode = ("learned" if ps is not None else "bf16") if bf16 else "plain"
     traffic = measured_traffic(mode, f"project_{dom}")
     return {
-        "kernel_ms": {"project_fwd": t_fwd * 1e3, "project_bwd": t_bwd * 1e3},
+        "kernel_ms": ({"project_prepare": t_prep * 1e3, "project_fwd_prepared": t_fwd * 1e3,
+                       "project_bwd_prepared": t_bwd * 1e3} if node_fwd else
+                      {"project_fwd": t_fwd * 1e3, "project_bwd": t_bwd * 1e3}),
         "roofline": {
             "bound": "hbm", "kernel": f"project_{dom}", "achieved": achieved_gbs, "peak": HBM_PEAK_GBS,
             "unit": "GB/s", "frac": achieved_gbs / HBM_PEAK_GBS, "traffic": traffic,

@@ -319,6 +319,35 @@ atdf_status atdf_project_bwd_adam(const void* packed_tree, int64_t num_nodes, co
                                   const atdf_grid_adam* adam, void* workspace, size_t workspace_bytes,
                                   atdf_stream_t stream);
 
+/* The training step's sort shared by forward and adjoint.
+ * atdf_project_prepare counting-sorts the intervals by grid slot and writes
+ * their records and the chunk table into the workspace
+ * (atdf_project_bwd_workspace_bytes); atdf_project_fwd_prepared is then a
+ * node-major forward over that sort (each node's grid stays in L2 while its
+ * chunk is integrated; per-interval sums are added per ray in list order,
+ * so the rounding differs from atdf_project_fwd by a few ulp), and
+ * atdf_project_bwd_prepared is atdf_project_bwd_adam (adam non-null) or
+ * atdf_project_bwd (adam null) without sorting again.  The three calls must
+ * see the same rays, intervals and slot map; the grid and decoder may
+ * change in between.  No reference counterpart: the reference gathers
+ * ray-major in both directions (geometry/geometry.cpp:828-914). */
+atdf_status atdf_project_prepare(const void* packed_tree, int64_t num_nodes, const float* ray_origin,
+                                 const float* ray_direction, int64_t num_rays, const atdf_sampling* sampling,
+                                 const int64_t* ray_interval_offset, const atdf_intervals* intervals,
+                                 const atdf_field* field, void* workspace, size_t workspace_bytes,
+                                 atdf_stream_t stream);
+atdf_status atdf_project_fwd_prepared(const void* packed_tree, int64_t num_nodes, const float* ray_origin,
+                                      const float* ray_direction, int64_t num_rays, const atdf_sampling* sampling,
+                                      const int64_t* ray_interval_offset, const atdf_intervals* intervals,
+                                      const atdf_field* field, float* projection, void* workspace,
+                                      size_t workspace_bytes, atdf_stream_t stream);
+atdf_status atdf_project_bwd_prepared(const void* packed_tree, int64_t num_nodes, const float* ray_origin,
+                                      const float* ray_direction, int64_t num_rays, const atdf_sampling* sampling,
+                                      const int64_t* ray_interval_offset, const atdf_intervals* intervals,
+                                      const atdf_field* field, const float* dLdP, float* grad_grid, float* grad_mlp,
+                                      const atdf_grid_adam* adam, void* workspace, size_t workspace_bytes,
+                                      atdf_stream_t stream);
+
 /* atdf_project_fwd plus the per-ray mass histogram ray_bins[r, b] (f32,
  * [num_rays, 32]): the sum of density * weight over the samples of ray r
  * whose t falls in bin b = (int)((t - t0) / (t1 - t0) * 32), clamped to

@@ -361,3 +361,105 @@ def test_fused_adjoint_adam_matches_separate(oracle, cuda, bf16):
     assert dp <= 1e-5 * moved and dm <= 1e-5 and dv <= 1e-5 and dg <= 1e-5
     if bf16:
         assert torch.equal(pb1, p1.to(torch.bfloat16))
+
+
+@pytest.mark.parametrize("C_,bf16", [(16, False), (16, True), (8, False)])
+def test_node_forward_prepared(oracle, cuda, C_, bf16):
+    """atdf_project_prepare + atdf_project_fwd_prepared (node-major) against
+    the ray-major atdf_project_fwd on the same intervals: per-interval sums
+    change the rounding order only (<= 2e-6 relative to the largest
+    projection), zero for rays without samples; within 1e-5 of the oracle;
+    atdf_project_bwd_prepared after the forward equals atdf_project_bwd_adam
+    to the rounding of the adjoint's LDS accumulation order."""
+    from conftest import edge_case_rays
+    from adaptivetensordf_amd import capi, native, phantom
+
+    cs = make_case(oracle, C_=C_, seed=23, jitter_mode=2, nrays=3000)
+    eo, ed = edge_case_rays()
+    o = np.concatenate([cs["o"], eo]).astype(np.float32)
+    d = np.concatenate([cs["d"], ed]).astype(np.float32)
+    o_t, d_t = rays_to_device(o, d)
+    tree = cs["tree"]
+    iv = tree.CreateIntervalsForRays(native().RayList(o_t, d_t), cs["sp"])
+    zero_iv = int((iv.count == 0).sum())
+    R = o.shape[0]
+    grid = cs["grid"].cuda()
+    mlp = cs["mlp"].cuda()
+    g_use = grid.to(torch.bfloat16) if bf16 else grid
+    dt = capi.GRID_BF16 if bf16 else capi.GRID_F32
+    f = capi.Field(C_, 64, 11, dt, 2.0, 0, g_use.data_ptr(), grid.shape[0], tree.node_to_grid.data_ptr(),
+                   mlp.data_ptr(), capi.DECODER_BF16 if bf16 else capi.DECODER_F32, 0)
+    sp = capi.Sampling(6, 256, 2, 0, 1234 + 23, None, None, None)
+    civ = capi.Intervals(iv.node.data_ptr(), iv.tmin.data_ptr(), iv.tmax.data_ptr(), iv.count.data_ptr(),
+                         iv.first.data_ptr(), iv.ray.data_ptr(), iv.num_intervals)
+    packed = tree.PackedTree()
+    args = (capi.ptr(packed), tree.NumNodes(), capi.ptr(o_t), capi.ptr(d_t), R, C.byref(sp),
+            capi.ptr(iv.ray_offset), C.byref(civ), C.byref(f))
+    P0 = torch.empty(R, device=cuda)
+    capi.call("atdf_project_fwd", *args, capi.ptr(P0), capi.stream_ptr())
+    ws = torch.empty(capi.lib().atdf_project_bwd_workspace_bytes(R, iv.num_intervals, C.byref(f)), dtype=torch.uint8,
+                     device=cuda)
+    P1 = torch.full((R,), float("nan"), device=cuda)
+    print(f"C={C_} bf16={bf16}: {iv.num_intervals} intervals ({zero_iv} without samples)", flush=True)
+    capi.call("atdf_project_prepare", *args, capi.ptr(ws), ws.numel(), capi.stream_ptr())
+    capi.call("atdf_project_fwd_prepared", *args, capi.ptr(P1), capi.ptr(ws), ws.numel(), capi.stream_ptr())
+    p0, p1 = P0.cpu().numpy(), P1.cpu().numpy()
+    empty = (iv.ray_offset[1:] == iv.ray_offset[:-1]).cpu().numpy()
+    assert np.all(p1[empty] == 0) and np.all(np.isfinite(p1))
+    err = np.abs(p1 - p0).max() / np.abs(p0).max()
+    print(f"  node-major vs ray-major: {err:.2e}", flush=True)
+    assert err <= 2e-6
+    if not bf16:
+        ref = cs["fm"].project(o, d)
+        assert np.abs(p1 - ref).max() <= 1e-5 * np.abs(ref).max()
+    # the adjoint over the prepared sort == the self-sorting adjoint (with Adam)
+    dP = torch.from_numpy(np.random.default_rng(3).normal(size=R).astype(np.float32)).to(cuda)
+    outs = []
+    for name in ("atdf_project_bwd_adam", "atdf_project_bwd_prepared"):
+        master = grid.clone()
+        gb = master.to(torch.bfloat16) if bf16 else None
+        f2 = capi.Field(C_, 64, 11, dt, 2.0, 0, (gb if bf16 else master).data_ptr(), grid.shape[0],
+                        tree.node_to_grid.data_ptr(), mlp.data_ptr(), capi.DECODER_BF16 if bf16 else capi.DECODER_F32,
+                        0)
+        m, v = torch.zeros_like(master), torch.zeros_like(master)
+        gg, gm = torch.zeros_like(master), torch.zeros_like(mlp)
+        ad = capi.GridAdam(master.data_ptr(), gb.data_ptr() if bf16 else None, m.data_ptr(), v.data_ptr(), 1e-2, 0.9,
+                           0.999, 1e-8, 3)
+        a2 = args[:-1] + (C.byref(f2),)
+        capi.call(name, *a2, capi.ptr(dP), capi.ptr(gg), capi.ptr(gm), C.byref(ad), capi.ptr(ws), ws.numel(),
+                  capi.stream_ptr())
+        outs.append([t.cpu() for t in (master, m, v, gm)])
+    (p0, m0, v0, gm0), (p1, m1, v1, gm1) = outs
+    assert (p0 - grid.cpu()).abs().max().item() > 1e-3
+    # Adam moments (the gradient) and the decoder gradient; the parameter step
+    # itself is sign-like where a gradient sums to almost zero
+    for a, b in ((m0, m1), (v0, v1), (gm0, gm1)):
+        assert ((a - b).abs().max() / a.abs().max()).item() <= 1e-5
+    big = m0.abs() > 1e-3 * m0.abs().max()
+    assert ((p1 - p0).abs()[big].max() / (p0 - grid.cpu()).abs().max()).item() <= 1e-5
+
+
+@pytest.mark.parametrize("bf16", [False, True])
+def test_train_step_node_forward(oracle, cuda, bf16):
+    """The training step with the node-major forward against the ray-major
+    one: the same losses (to the forward's rounding), and updates that
+    differ only where Adam's first steps see gradients near zero."""
+    from adaptivetensordf_amd import native
+
+    cs = make_case(oracle, C_=16, seed=31, jitter_mode=2, nrays=2000)
+    o_t, d_t = rays_to_device(cs["o"], cs["d"])
+    rays = native().RayList(o_t, d_t)
+    tgt = torch.from_numpy(cs["fm"].project(cs["o"], cs["d"]).astype(np.float32) * 1.1).cuda()
+    res = []
+    for node_fwd in (False, True):
+        st = native().MakeTrainState(cs["grid"].cuda(), cs["mlp"].cuda(), bf16, 1e-3, 1e-3)
+        fp = cs["fp"]
+        fp.bf16_decoder = bf16
+        fp.node_forward = node_fwd
+        losses = [native().TrainStep(cs["tree"], st, rays, tgt, cs["sp"], fp).item() for _ in range(3)]
+        res.append((losses, (st.grid_master if bf16 else st.grid).cpu()))
+    (l0, g0), (l1, g1) = res
+    print(f"bf16={bf16}: losses {l0} vs {l1}", flush=True)
+    assert np.allclose(l0, l1, rtol=1e-5)
+    g_init = cs["grid"]
+    assert (g0 - g1).norm() <= 1e-2 * (g0 - g_init).norm()
