@@ -203,6 +203,138 @@ done:
 }
 
 // ---------------------------------------------------------------------------
+// Cooperative traversal for sparse ray lists (the learned proposal's fallback
+// rays): eight lanes per ray.  The walk is the same front-to-back DFS as
+// traverse_kernel, but the eight children of a node are loaded and
+// slab-tested at once, one per lane, and kept in LDS (one row of eight per
+// tree level), so a ray's chain of dependent node loads is one per visited
+// inner node instead of one per visited node.  Node tests, emission order and
+// every written value are those of traverse_kernel.
+// ---------------------------------------------------------------------------
+constexpr int kCoopThreads = 128;
+constexpr int kCoopGroups = kCoopThreads / 8;
+constexpr int kCoopLevels = 12;
+
+struct CoopChild {
+    float tmin, tmax;
+    int32_t first_child;
+    int32_t flags;  // node flags; bit 8: hit
+};
+
+__device__ __forceinline__ void coop_emit(const TraverseArgs& a, bool write, int gl, int64_t r, int node, float tmin,
+                                          float tmax, const PNode& nd, int64_t ivb, int& nh, int& ns, float& last_t,
+                                          int& last_node, bool& order_bad)
+{
+    const int n = node_sample_count(nd.mn, nd.mx, tmin, tmax, a.K);
+    if (tmin < last_t || (tmin == last_t && node < last_node)) order_bad = true;
+    last_t = tmin;
+    last_node = node;
+    if (write && gl == 0) {
+        const int64_t q = ivb + nh;
+        a.iv.node[q] = node;
+        a.iv.tmin[q] = tmin;
+        a.iv.tmax[q] = tmax;
+        a.iv.count[q] = n;
+        a.iv.first[q] = ns;
+        a.iv.ray[q] = (int32_t)r;
+    }
+    nh++;
+    ns += n;
+}
+
+template <bool kWrite>
+__global__ __launch_bounds__(kCoopThreads) void traverse_coop_kernel(TraverseArgs a)
+{
+    __shared__ CoopChild s_child[kCoopGroups][kCoopLevels][8];
+    __shared__ int32_t s_first[kCoopGroups][kCoopLevels];
+    __shared__ int32_t s_bits[kCoopGroups][kCoopLevels];
+    const int gi = threadIdx.x >> 3, gl = threadIdx.x & 7;
+    int64_t r = (blockIdx.x * (int64_t)kCoopThreads + threadIdx.x) >> 3;
+    if (r >= a.R) return;
+    if (a.list) {
+        r = a.list[r];
+        if (r < 0 || r >= a.R) return;
+    }
+    if (a.select && a.select[r] == 0) {
+        if (!kWrite && gl == 0) {
+            a.ray_num_intervals[r] = 0;
+            a.ray_num_samples[r] = 0;
+        }
+        return;
+    }
+    float o[3], d[3];
+    for (int k = 0; k < 3; ++k) {
+        o[k] = a.O[r * 3 + k];
+        d[k] = a.D[r * 3 + k];
+    }
+    const int mask = (d[0] < 0 ? 1 : 0) | (d[1] < 0 ? 2 : 0) | (d[2] < 0 ? 4 : 0);
+    int nh = 0, ns = 0;
+    float last_t = -INFINITY;
+    int last_node = -1;
+    bool order_bad = false, overflow = false;
+    const int64_t ivb = kWrite ? a.iv_off[r] : 0;
+    CoopChild(*ch)[8] = s_child[gi];
+    int32_t* first = s_first[gi];
+    int32_t* bits = s_bits[gi];
+    int level = -1;
+
+    // push the eight children of a node: lane gl tests the child at
+    // front-to-back position gl
+    auto push = [&](int32_t fc) {
+        if (level + 1 >= kCoopLevels) {
+            overflow = true;
+            return;
+        }
+        ++level;
+        const int child = fc + (gl ^ mask);
+        const PNode c = a.nodes[child];
+        float t0, t1;
+        const bool hit = intersect_box_ray(c.mn, c.mx, o, d, t0, t1);
+        ch[level][gl] = CoopChild{t0, t1, c.first_child, c.flags | (hit ? 256 : 0)};
+        int b = hit ? (1 << gl) : 0;
+        b |= __shfl_xor(b, 1, 8);
+        b |= __shfl_xor(b, 2, 8);
+        b |= __shfl_xor(b, 4, 8);
+        if (gl == 0) {
+            first[level] = fc;
+            bits[level] = b;
+        }
+    };
+
+    {
+        const PNode nd = a.nodes[0];
+        float tmin, tmax;
+        const bool hit = intersect_box_ray(nd.mn, nd.mx, o, d, tmin, tmax);
+        if (hit && (nd.flags & kFlagActive) && !(tmax - tmin < 1e-5))
+            coop_emit(a, kWrite, gl, r, 0, tmin, tmax, nd, ivb, nh, ns, last_t, last_node, order_bad);
+        if (hit && (nd.flags & kFlagActiveBelow) && nd.first_child >= 0) push(nd.first_child);
+    }
+    while (level >= 0 && !overflow) {
+        const int b = bits[level];
+        if (b == 0) {
+            --level;
+            continue;
+        }
+        const int k = __ffs(b) - 1;
+        if (gl == 0) bits[level] = b & (b - 1);
+        const CoopChild c = ch[level][k];
+        const int node = first[level] + (k ^ mask);
+        if ((c.flags & kFlagActive) && !(c.tmax - c.tmin < 1e-5)) {
+            const PNode nd = a.nodes[node];
+            coop_emit(a, kWrite, gl, r, node, c.tmin, c.tmax, nd, ivb, nh, ns, last_t, last_node, order_bad);
+        }
+        if ((c.flags & kFlagActiveBelow) && c.first_child >= 0) push(c.first_child);
+    }
+    if (!kWrite && gl == 0) {
+        a.ray_num_intervals[r] = nh;
+        a.ray_num_samples[r] = ns;
+        if (nh > a.max_inter) atomicAdd(&a.status[0], 1);
+        if (order_bad || overflow) atomicAdd(&a.status[1], 1);
+        atomicMax(&a.status[2], ns);
+    }
+}
+
+// ---------------------------------------------------------------------------
 // exclusive scan, int32 -> int64 (n + 1 outputs)
 // ---------------------------------------------------------------------------
 constexpr int kScanThreads = 256;
@@ -385,7 +517,12 @@ atdf_status atdf_traverse_count(const void* packed_tree, int64_t num_nodes, cons
     a.ray_num_intervals = ray_num_intervals;
     a.ray_num_samples = ray_num_samples;
     a.status = status;
-    hipLaunchKernelGGL(traverse_kernel<false>, dim3((unsigned)div_up(num_rays, 256)), dim3(256), 0, to_hip(stream), a);
+    if (a.list)  // a sparse selection: eight lanes per listed ray
+        hipLaunchKernelGGL(traverse_coop_kernel<false>, dim3((unsigned)div_up(num_rays * 8, kCoopThreads)),
+                           dim3(kCoopThreads), 0, to_hip(stream), a);
+    else
+        hipLaunchKernelGGL(traverse_kernel<false>, dim3((unsigned)div_up(num_rays, 256)), dim3(256), 0,
+                           to_hip(stream), a);
     return check_launch("atdf_traverse_count");
 }
 
@@ -429,7 +566,12 @@ atdf_status atdf_traverse_write(const void* packed_tree, int64_t num_nodes, cons
     a.smp_off = samples ? ray_sample_offset : ray_interval_offset;
     if (intervals) a.iv = *intervals;
     if (samples) a.smp = *samples;
-    hipLaunchKernelGGL(traverse_kernel<true>, dim3((unsigned)div_up(num_rays, 256)), dim3(256), 0, to_hip(stream), a);
+    if (a.list && !samples)  // intervals of a sparse selection: eight lanes per listed ray
+        hipLaunchKernelGGL(traverse_coop_kernel<true>, dim3((unsigned)div_up(num_rays * 8, kCoopThreads)),
+                           dim3(kCoopThreads), 0, to_hip(stream), a);
+    else
+        hipLaunchKernelGGL(traverse_kernel<true>, dim3((unsigned)div_up(num_rays, 256)), dim3(256), 0,
+                           to_hip(stream), a);
     return check_launch("atdf_traverse_write");
 }
 

@@ -196,3 +196,81 @@ def test_traversal_full_size_tree(oracle, cuda):
     per_ray = np.add.reduceat(lens, off[:-1][off[1:] > off[:-1]]) if lens.size else lens
     assert np.all(per_ray > 0) and np.all(per_ray < 2 * np.sqrt(3) + 1e-4)
     assert iv.num_samples == int(iv.ray_num_samples.sum())
+
+
+def _listed_traverse(tree, o_t, d_t, K, lst, seed=99):
+    """Count + scan + interval write with a ray list (the cooperative,
+    eight-lanes-per-ray traversal); unlisted rays get no intervals."""
+    from adaptivetensordf_amd import capi
+
+    dev = o_t.device
+    R = o_t.shape[0]
+    s = capi.stream_ptr()
+    sel = torch.zeros(R, dtype=torch.int32, device=dev)
+    sel[lst[lst >= 0].long()] = 1
+    sp = capi.Sampling(K, 256, 2, 0, seed, None, sel.data_ptr(), lst.data_ptr())
+    packed = tree.PackedTree()
+    niv = torch.zeros(R, dtype=torch.int32, device=dev)
+    ns = torch.zeros(R, dtype=torch.int32, device=dev)
+    status = torch.zeros(4, dtype=torch.int32, device=dev)
+    capi.call("atdf_traverse_count", capi.ptr(packed), tree.NumNodes(), capi.ptr(o_t), capi.ptr(d_t), R,
+              C.byref(sp), capi.ptr(niv), capi.ptr(ns), capi.ptr(status), s)
+    ws = torch.empty(max(8, capi.lib().atdf_scan_workspace_bytes(R)), dtype=torch.uint8, device=dev)
+    ivo = torch.empty(R + 1, dtype=torch.int64, device=dev)
+    capi.call("atdf_exclusive_scan_i32", capi.ptr(niv), R, capi.ptr(ivo), capi.ptr(ws), ws.numel(), s)
+    H = int(ivo[-1])
+    i32 = dict(dtype=torch.int32, device=dev)
+    f32 = dict(dtype=torch.float32, device=dev)
+    iv = dict(node=torch.empty(H, **i32), tmin=torch.empty(H, **f32), tmax=torch.empty(H, **f32),
+              count=torch.empty(H, **i32), first=torch.empty(H, **i32), ray=torch.empty(H, **i32))
+    civ = capi.Intervals(*[iv[k].data_ptr() for k in ("node", "tmin", "tmax", "count", "first", "ray")], H)
+    if H > 0:
+        capi.call("atdf_traverse_write", capi.ptr(packed), tree.NumNodes(), capi.ptr(o_t), capi.ptr(d_t), R,
+                  C.byref(sp), capi.ptr(ivo), None, C.byref(civ), None, s)
+    torch.cuda.synchronize()
+    out = {k: v.cpu().numpy() for k, v in iv.items()}
+    out.update(status=status.cpu().numpy(), niv=niv.cpu().numpy(), ns=ns.cpu().numpy(), ivo=ivo.cpu().numpy())
+    return out
+
+
+@pytest.mark.parametrize("case", ["uniform3", "adaptive4", "adaptive5"])
+def test_listed_traversal_bit_exact(oracle, cuda, case):
+    """Traversal of a ray list (eight lanes per ray, children tested in
+    parallel) against the oracle's DFS: counts, node order, tmin / tmax and
+    sample counts bit-exact for every listed ray, nothing for the others."""
+    from adaptivetensordf_amd import phantom
+
+    depth = {"uniform3": 3, "adaptive4": 4, "adaptive5": 5}[case]
+    t = oracle.tree_build(depth)
+    if case == "adaptive5":
+        ell = phantom.ellipsoids(200, 1)
+        act = phantom.adaptive_active_mask(t["node_min"], t["node_max"], t["children"], t["depth"], ell, 2000, 3, 5)
+    elif case == "adaptive4":
+        act = adaptive_active(t, budget=600)
+    else:
+        act = uniform_active(t, depth)
+    tree = device_tree(depth, act)
+    o, d = random_rays(900, 31)
+    eo, ed = edge_case_rays()
+    to_, td = tilt_rays(500, 7)
+    o, d = np.concatenate([o, eo, to_]), np.concatenate([d, ed, td])
+    R = o.shape[0]
+    rng = np.random.default_rng(depth)
+    listed = np.sort(rng.choice(R, size=R // 3, replace=False))
+    lst = np.full(R, -1, np.int32)
+    lst[:listed.size] = rng.permutation(listed)  # list order is arbitrary
+    o_t, d_t = rays_to_device(o, d)
+    print(f"{case}: R={R}, listed {listed.size}, active {int(act.sum())}", flush=True)
+    g = _listed_traverse(tree, o_t, d_t, 6, torch.from_numpy(lst).cuda())
+    r = oracle.traverse(t["node_min"], t["node_max"], oracle.active_ids(act), o[listed], d[listed], 6)
+    assert g["status"][0] == 0 and g["status"][1] == 0
+    assert np.array_equal(g["niv"][listed], r["ray_num_intervals"])
+    assert np.array_equal(g["ns"][listed], r["ray_num_samples"])
+    mask = np.ones(R, bool)
+    mask[listed] = False
+    assert np.all(g["niv"][mask] == 0)
+    assert np.array_equal(g["node"], r["iv_node"]) and np.array_equal(g["count"], r["iv_count"])
+    assert np.array_equal(g["first"], r["iv_first"])
+    assert np.array_equal(g["tmin"].view(np.uint32), r["iv_tmin"].view(np.uint32))
+    assert np.array_equal(g["tmax"].view(np.uint32), r["iv_tmax"].view(np.uint32))
+    assert np.array_equal(g["ray"], np.repeat(listed, r["ray_num_intervals"]))
