@@ -69,6 +69,11 @@ class TVShape(C.Structure):
                 ("stride_c", C.c_int64), ("stride_s", C.c_int64 * 3)]
 
 
+class NeighborSpec(C.Structure):
+    _fields_ = [("size", C.c_int32 * 3), ("epsilon", C.c_float), ("use_roi", C.c_int32), ("roi_min", C.c_float * 3),
+                ("roi_max", C.c_float * 3)]
+
+
 class Intervals(C.Structure):
     _fields_ = [("node", C.c_void_p), ("tmin", C.c_void_p), ("tmax", C.c_void_p), ("count", C.c_void_p),
                 ("first", C.c_void_p), ("ray", C.c_void_p), ("total", C.c_int64)]
@@ -161,6 +166,9 @@ _SIGNATURES = {
                                               _P, _P, _P, _P, _P]),
     "atdf_tv_workspace_bytes": (_SZ, [_I64]),
     "atdf_tv_loss": (_I32, [_P, C.POINTER(TVShape), _P, C.c_double, _P, _P, _P, _SZ, _P]),
+    "atdf_neighbor_candidates": (_I64, [_I64, C.POINTER(NeighborSpec)]),
+    "atdf_neighbor_flags": (_I32, [_P, _I64, _P, _I64, C.POINTER(NeighborSpec), _P, _P]),
+    "atdf_neighbor_write": (_I32, [_P, _I64, _P, _I64, C.POINTER(NeighborSpec), _P, _P, _P, _P]),
     "atdf_grid_layout": (_I32, [_P, _P, _I64, _I32, _I32, _I32, _P]),
     "atdf_f32_to_bf16": (_I32, [_P, _P, _I64, _P]),
 }

@@ -114,6 +114,103 @@ __global__ __launch_bounds__(256) void grid_transfer_kernel(TransferArgs a)
     }
 }
 
+// NodeNeighborSamples (structure/HyperTree.cu:1720-1811, 2010-2075): for
+// every active node, every face (side s, front / back) and every point of a
+// size[s_a] x size[s_b] lattice on it (s_a = s + 1, s_b = s + 2 mod 3), the
+// point p = local * box + min is located at p + epsilon * (front ? +1 : -1)
+// e_s; a different active node there (and, with the ROI test, both nodes in
+// the ROI) makes a pair: the point evaluated in both nodes.  Candidates are
+// numbered per node as (side, front, lattice index) and pairs are written in
+// that order (the reference appends them with an atomic counter).
+struct NeighborArgs {
+    const PNode* nodes;
+    const int64_t* active_ids;
+    int64_t num_active;
+    int size[3];
+    int per_node;  // candidates per node
+    float eps;
+    int use_roi;
+    float roi_min[3], roi_max[3];
+};
+
+// in_roi (structure/HyperTree.cu:1667-1690), including its mid = min + max
+__device__ __forceinline__ bool node_in_roi(const PNode& n, const float* lo, const float* hi)
+{
+    bool in = true;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const float mid = n.mn[k] + n.mx[k];
+        in = in && ((n.mn[k] > lo[k] && n.mn[k] < hi[k]) || (n.mx[k] > lo[k] && n.mx[k] < hi[k]) ||
+                    (mid > lo[k] && mid < hi[k]));
+    }
+    return in;
+}
+
+// candidate c of node a: returns the partner node (or -1) and the point
+__device__ int neighbor_candidate(const NeighborArgs& a, int64_t act, int c, int* node_out, float* g)
+{
+    int side = 0, front = 0, sid = c;
+    for (side = 0; side < 3; ++side) {
+        const int sa = (side + 1) % 3, sb = (side + 2) % 3;
+        const int n = a.size[sa] * a.size[sb];
+        if (sid < 2 * n) {
+            front = sid >= n ? 1 : 0;
+            sid -= front * n;
+            break;
+        }
+        sid -= 2 * n;
+    }
+    const int sa = (side + 1) % 3, sb = (side + 2) % 3;
+    const int node = (int)a.active_ids[act];
+    const PNode nd = a.nodes[node];
+    float local[3];
+    local[side] = front ? 1.f : 0.f;
+    local[sa] = (float)(sid % a.size[sa]) / ((float)a.size[sa] - 1.f);
+    local[sb] = (float)(sid / a.size[sa]) / ((float)a.size[sb] - 1.f);
+    float q[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const float box = nd.mx[k] - nd.mn[k];
+        g[k] = local[k] * box + nd.mn[k];
+        q[k] = g[k];
+    }
+    q[side] = g[side] + (front ? a.eps : -a.eps);
+    const int other = active_node_for_position(a.nodes, q);
+    *node_out = node;
+    if (other < 0 || other == node) return -1;
+    if (a.use_roi && !(node_in_roi(nd, a.roi_min, a.roi_max) && node_in_roi(a.nodes[other], a.roi_min, a.roi_max)))
+        return -1;
+    return other;
+}
+
+__global__ __launch_bounds__(256) void neighbor_flags_kernel(NeighborArgs a, int32_t* __restrict__ flags)
+{
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= a.num_active * a.per_node) return;
+    int node;
+    float g[3];
+    flags[i] = neighbor_candidate(a, i / a.per_node, (int)(i % a.per_node), &node, g) >= 0 ? 1 : 0;
+}
+
+__global__ __launch_bounds__(256) void neighbor_write_kernel(NeighborArgs a, const int64_t* __restrict__ offsets,
+                                                             float* __restrict__ pos, int64_t* __restrict__ node_id)
+{
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= a.num_active * a.per_node) return;
+    if (offsets[i + 1] == offsets[i]) return;
+    int node;
+    float g[3];
+    const int other = neighbor_candidate(a, i / a.per_node, (int)(i % a.per_node), &node, g);
+    const int64_t o = 2 * offsets[i];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        pos[o * 3 + k] = g[k];
+        pos[(o + 1) * 3 + k] = g[k];
+    }
+    node_id[o] = node;
+    node_id[o + 1] = other;
+}
+
 }  // namespace atdf
 
 using namespace atdf;
@@ -162,6 +259,69 @@ atdf_status atdf_grid_transfer(const void* src_packed_tree, int64_t num_nodes, c
     a.layout = layout;
     hipLaunchKernelGGL(grid_transfer_kernel, dim3((unsigned)num_dst), dim3(256), 0, to_hip(stream), a);
     return check_launch("atdf_grid_transfer");
+}
+
+static atdf_status make_neighbor_args(const void* packed_tree, int64_t num_nodes, const int64_t* active_ids,
+                                      int64_t num_active, const atdf_neighbor_spec* spec, NeighborArgs* a,
+                                      const char* who)
+{
+    ATDF_REQUIRE(packed_tree && num_nodes > 0 && spec, "%s: empty tree or spec", who);
+    ATDF_REQUIRE(num_active >= 0, "%s: negative node count", who);
+    for (int k = 0; k < 3; ++k)
+        ATDF_REQUIRE(spec->size[k] >= 2 && spec->size[k] <= 4096, "%s: face lattice size %d outside [2, 4096]", who,
+                     spec->size[k]);
+    a->nodes = reinterpret_cast<const PNode*>(packed_tree);
+    a->active_ids = active_ids;
+    a->num_active = num_active;
+    for (int k = 0; k < 3; ++k) {
+        a->size[k] = spec->size[k];
+        a->roi_min[k] = spec->roi_min[k];
+        a->roi_max[k] = spec->roi_max[k];
+    }
+    a->per_node = 2 * (spec->size[1] * spec->size[2] + spec->size[2] * spec->size[0] + spec->size[0] * spec->size[1]);
+    a->eps = spec->epsilon;
+    a->use_roi = spec->use_roi != 0;
+    ATDF_REQUIRE(num_active * a->per_node < (int64_t)INT32_MAX, "%s: too many candidates", who);
+    return ATDF_OK;
+}
+
+int64_t atdf_neighbor_candidates(int64_t num_active, const atdf_neighbor_spec* spec)
+{
+    if (!spec || num_active <= 0) return 0;
+    return num_active * 2 *
+           ((int64_t)spec->size[1] * spec->size[2] + (int64_t)spec->size[2] * spec->size[0] +
+            (int64_t)spec->size[0] * spec->size[1]);
+}
+
+atdf_status atdf_neighbor_flags(const void* packed_tree, int64_t num_nodes, const int64_t* active_ids,
+                                int64_t num_active, const atdf_neighbor_spec* spec, int32_t* flags,
+                                atdf_stream_t stream)
+{
+    NeighborArgs a{};
+    atdf_status st = make_neighbor_args(packed_tree, num_nodes, active_ids, num_active, spec, &a,
+                                        "atdf_neighbor_flags");
+    if (st) return st;
+    const int64_t n = num_active * a.per_node;
+    if (n == 0) return ATDF_OK;
+    ATDF_REQUIRE(active_ids && flags, "atdf_neighbor_flags: null buffer");
+    hipLaunchKernelGGL(neighbor_flags_kernel, dim3((unsigned)div_up(n, 256)), dim3(256), 0, to_hip(stream), a, flags);
+    return check_launch("atdf_neighbor_flags");
+}
+
+atdf_status atdf_neighbor_write(const void* packed_tree, int64_t num_nodes, const int64_t* active_ids,
+                                int64_t num_active, const atdf_neighbor_spec* spec, const int64_t* offsets,
+                                float* positions, int64_t* node_id, atdf_stream_t stream)
+{
+    NeighborArgs a{};
+    atdf_status st = make_neighbor_args(packed_tree, num_nodes, active_ids, num_active, spec, &a,
+                                        "atdf_neighbor_write");
+    if (st) return st;
+    const int64_t n = num_active * a.per_node;
+    if (n == 0) return ATDF_OK;
+    ATDF_REQUIRE(active_ids && offsets && positions && node_id, "atdf_neighbor_write: null buffer");
+    hipLaunchKernelGGL(neighbor_write_kernel, dim3((unsigned)div_up(n, 256)), dim3(256), 0, to_hip(stream), a,
+                       offsets, positions, node_id);
+    return check_launch("atdf_neighbor_write");
 }
 
 }  // extern "C"

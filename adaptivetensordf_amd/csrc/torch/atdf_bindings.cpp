@@ -77,6 +77,9 @@ PYBIND11_MODULE(TORCH_EXTENSION_NAME, m)
         .def("RoiSplit", &HyperTreeBase::RoiSplit)
         .def("NodeIdForPositionGPU", &HyperTreeBase::NodeIdForPositionGPU)
         .def("UniformGlobalSamples", &HyperTreeBase::UniformGlobalSamples)
+        .def("NodeNeighborSamples", &HyperTreeBase::NodeNeighborSamples, py::arg("size"), py::arg("epsilon"),
+             py::arg("roi_min") = std::vector<float>{}, py::arg("roi_max") = std::vector<float>{},
+             py::arg("use_roi") = false)
         .def("UniformPhantomSamplesGPU", &HyperTreeBase::UniformPhantomSamplesGPU)
         .def("UniformPhantomSamplesGPUbySlice", &HyperTreeBase::UniformPhantomSamplesGPUbySlice)
         .def("InterpolateGridForInactiveNodes", &HyperTreeBase::InterpolateGridForInactiveNodes)
@@ -241,8 +244,15 @@ PYBIND11_MODULE(TORCH_EXTENSION_NAME, m)
         .def_readwrite("plane_vec_only", &RegularizerParams::plane_vec_only)
         .def_readwrite("use_loss_tv_3d", &RegularizerParams::use_loss_tv_3d)
         .def_readwrite("test_para", &RegularizerParams::test_para)
-        .def_readwrite("tv_size", &RegularizerParams::tv_size);
-    m.def("VolumeRegularizer", &VolumeRegularizer);
+        .def_readwrite("tv_size", &RegularizerParams::tv_size)
+        .def_readwrite("loss_edge", &RegularizerParams::loss_edge)
+        .def_readwrite("use_ground_truth_volume", &RegularizerParams::use_ground_truth_volume);
+    m.def(
+        "VolumeRegularizer",
+        [](HyperTreeBase& tree, const RoiMaps& roi, const VMFields& f, const RegularizerParams& p,
+           std::optional<SampleList> pairs) { return VolumeRegularizer(tree, roi, f, p, pairs ? &*pairs : nullptr); },
+        py::arg("tree"), py::arg("roi"), py::arg("fields"), py::arg("params"), py::arg("neighbor_pairs") = py::none());
+    m.def("EdgeLoss", &EdgeLoss);
     m.def("IntegrateSamplesXRay", &IntegrateSamplesXRay);
     m.def("AccumulateSampleLossPerNode", &AccumulateSampleLossPerNode);
     m.def("ProjectRays", &ProjectRays);

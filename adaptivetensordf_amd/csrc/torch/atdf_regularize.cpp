@@ -67,10 +67,36 @@ Tensor TVLoss(Tensor grid, Tensor weight, bool channels_last)
     return TVLossFn::apply(grid, weight, channels_last);
 }
 
-Tensor VolumeRegularizer(HyperTreeBase& tree, const RoiMaps& roi, const VMFields& f, const RegularizerParams& p)
+Tensor EdgeLoss(HyperTreeBase& tree, const RoiMaps& roi, const VMFields& f, const SampleList& pairs,
+               const RegularizerParams& p)
+{
+    const int64_t n = pairs.node_id.numel();
+    TORCH_CHECK(n % 2 == 0, "EdgeLoss: pairs come as rows 2k, 2k + 1");
+    Tensor loss;
+    if (n == 0) return loss;
+    auto feat = SampleVolumeIndirectFeature(tree, roi, f, pairs.global_coordinate, pairs.node_id);
+    auto t1 = feat.slice(0, 0, n, 2), t2 = feat.slice(0, 1, n, 2);
+    auto edge = (t1 - t2).pow(2).mean(1).sqrt();
+    // the pair's group: ROI membership of its first node
+    auto first = pairs.node_id.slice(0, 0, n, 2).contiguous();
+    auto in = ComputeBoundWeight(first, roi) >= 0;
+    auto add = [&](const Tensor& t) { loss = loss.defined() ? loss + t : t; };
+    const int64_t n_in = in.sum().item<int64_t>();
+    if (n_in > 0) add(edge.masked_select(in).mean() * (p.loss_edge * (double)feat.size(1)));
+    if (!p.use_ground_truth_volume && n_in < n / 2)
+        add(edge.masked_select(in.logical_not()).mean() * (p.loss_edge * (double)feat.size(1)));
+    return loss;
+}
+
+Tensor VolumeRegularizer(HyperTreeBase& tree, const RoiMaps& roi, const VMFields& f, const RegularizerParams& p,
+                         const SampleList* neighbor_pairs)
 {
     Tensor loss;
     auto add = [&](const Tensor& t) { loss = loss.defined() ? loss + t : t; };
+    if (p.loss_edge > 0 && neighbor_pairs) {
+        auto e = EdgeLoss(tree, roi, f, *neighbor_pairs, p);
+        if (e.defined()) add(e);
+    }
     if (p.loss_tv > 0) {
         if (p.use_loss_tv_3d) {
             // features on a tv_size lattice of every in-ROI node, [N, x, y, z, C]

@@ -106,6 +106,11 @@ class HyperTreeBase {
     // NodeIdForPositionGPU (HyperTree.cu:1487-1525): (node_id [P] i64, mask [P] f32)
     // for positions [..., 3]; outside every active node: first active node, 0
     std::pair<torch::Tensor, torch::Tensor> NodeIdForPositionGPU(torch::Tensor global_samples) const;
+    // NodeNeighborSamples (HyperTree.cu:1720-1811, 2010-2075): pairs of samples
+    // (rows 2k, 2k + 1: one point in two face-adjacent active nodes), in
+    // candidate order; weight 1, local_index_in_ray 0 / 1
+    SampleList NodeNeighborSamples(std::vector<int> size, double epsilon, std::vector<float> roi_min,
+                                   std::vector<float> roi_max, bool use_roi) const;
     // UniformGlobalSamples (HyperTree.cu:1618-1665): [N, n, n, n, 3] lattice
     // positions of the listed nodes (x fastest)
     torch::Tensor UniformGlobalSamples(torch::Tensor node_id, int n) const;
@@ -216,13 +221,22 @@ struct RegularizerParams {
     bool use_loss_tv_3d = true;
     float test_para = -1;
     std::vector<int64_t> tv_size{16, 16, 16};  // lattice of the 3-D TV: vec_grid_size * loss_tv_scale
+    double loss_edge = 0.002;
+    bool use_ground_truth_volume = false;  // drops the out-of-ROI edge term
 };
+// The edge term of VolumeRegularizer (geometry_tensor_qo.cpp:630-667): for
+// neighbour pairs (NodeNeighborSamples) the features of both samples, split
+// by the ROI membership of the pair's first node; per group
+// mean_pairs sqrt(mean_c (f1 - f2)^2) * loss_edge * channels.
+torch::Tensor EdgeLoss(HyperTreeBase& tree, const RoiMaps& roi, const VMFields& fields, const SampleList& pairs,
+                       const RegularizerParams& p);
 // GeometryTensorQO::VolumeRegularizer (geometry_tensor_qo.cpp:527-622 and
 // 913-922): the TV terms (3-D TV of the features on a lattice of each in-ROI
-// node, or TV of the planes, lines and explicit grids) and the L2 terms.
-// The edge and Fourier terms are not built.  Undefined when no term is on.
+// node, or TV of the planes, lines and explicit grids), the L2 terms and,
+// given neighbour pairs, the edge term.  The Fourier term is not built.
+// Undefined when no term is on.
 torch::Tensor VolumeRegularizer(HyperTreeBase& tree, const RoiMaps& roi, const VMFields& fields,
-                                const RegularizerParams& p);
+                                const RegularizerParams& p, const SampleList* neighbor_pairs = nullptr);
 
 // NeuralGeometry::IntegrateSamplesXRay (geometry.cpp:977-1019)
 torch::Tensor IntegrateSamplesXRay(torch::Tensor sample_values, torch::Tensor integration_weight,

@@ -353,6 +353,49 @@ std::pair<torch::Tensor, torch::Tensor> HyperTreeBase::NodeIdForPositionGPU(torc
     return {node_id, mask};
 }
 
+SampleList HyperTreeBase::NodeNeighborSamples(std::vector<int> size, double epsilon, std::vector<float> roi_min,
+                                              std::vector<float> roi_max, bool use_roi) const
+{
+    TORCH_CHECK(node_position_min.is_cuda(), "NodeNeighborSamples: the tree must be on the GPU");
+    TORCH_CHECK(size.size() == 3, "NodeNeighborSamples: size needs 3 entries");
+    TORCH_CHECK(!use_roi || (roi_min.size() == 3 && roi_max.size() == 3), "NodeNeighborSamples: ROI needs 3 entries");
+    atdf_neighbor_spec spec{};
+    for (int k = 0; k < 3; ++k) {
+        spec.size[k] = size[k];
+        spec.roi_min[k] = use_roi ? roi_min[k] : 0.f;
+        spec.roi_max[k] = use_roi ? roi_max[k] : 0.f;
+    }
+    spec.epsilon = (float)epsilon;
+    spec.use_roi = use_roi ? 1 : 0;
+    auto dev = node_position_min.device();
+    auto ids = active_node_ids.contiguous();
+    const int64_t NA = ids.size(0);
+    const int64_t nc = atdf_neighbor_candidates(NA, &spec);
+    auto flags = torch::empty({std::max<int64_t>(nc, 1)}, torch::TensorOptions().dtype(torch::kInt32).device(dev));
+    check_status(atdf_neighbor_flags(packed(), NumNodes(), ids.data_ptr<int64_t>(), NA, &spec, flags.data_ptr<int32_t>(),
+                                     current_stream()),
+                 "atdf_neighbor_flags");
+    auto off = torch::empty({nc + 1}, torch::TensorOptions().dtype(torch::kLong).device(dev));
+    auto ws = torch::empty({(int64_t)atdf_scan_workspace_bytes(nc)}, torch::TensorOptions().dtype(torch::kUInt8).device(dev));
+    check_status(atdf_exclusive_scan_i32(flags.data_ptr<int32_t>(), nc, off.data_ptr<int64_t>(), ws.data_ptr(),
+                                         ws.numel(), current_stream()),
+                 "atdf_exclusive_scan_i32");
+    const int64_t P = off[nc].item<int64_t>();
+    SampleList out;
+    out.global_coordinate = torch::empty({2 * P, 3}, torch::TensorOptions().dtype(torch::kFloat32).device(dev));
+    out.node_id = torch::empty({2 * P}, torch::TensorOptions().dtype(torch::kLong).device(dev));
+    if (P > 0)
+        check_status(atdf_neighbor_write(packed(), NumNodes(), ids.data_ptr<int64_t>(), NA, &spec,
+                                         off.data_ptr<int64_t>(), out.global_coordinate.data_ptr<float>(),
+                                         out.node_id.data_ptr<int64_t>(), current_stream()),
+                     "atdf_neighbor_write");
+    out.weight = torch::ones({2 * P}, out.global_coordinate.options());
+    out.ray_t = torch::zeros({2 * P}, out.global_coordinate.options());
+    out.ray_index = torch::arange(2 * P, out.node_id.options());
+    out.local_index_in_ray = torch::arange(2 * P, out.node_id.options()).remainder(2);
+    return out;
+}
+
 torch::Tensor HyperTreeBase::UniformGlobalSamples(torch::Tensor node_id, int n) const
 {
     TORCH_CHECK(node_position_min.is_cuda(), "UniformGlobalSamples: the tree must be on the GPU");

@@ -418,6 +418,41 @@ atdf_status atdf_sample_volume(const void* packed_tree, int64_t num_nodes, const
                                const atdf_field* field, const atdf_lattice* lattice, float* density,
                                int64_t* node_id, float* valid, float* position, atdf_stream_t stream);
 
+/* Pairs of points across the faces between active nodes, for the edge
+ * regulariser: HyperTreeBaseImpl::NodeNeighborSamples (structure/
+ * HyperTree.cu:1720-1811, 2010-2075).  For every active node, face (side s
+ * in 0..2, back then front) and point of a size[s+1] x size[s+2] lattice
+ * (indices mod 3) on it, local[s] = front, local[s+1] = (i % size[s+1]) /
+ * (size[s+1] - 1), local[s+2] = (i / size[s+1]) / (size[s+2] - 1), p =
+ * local * (max - min) + min; p moved by epsilon * (front ? +1 : -1) along s
+ * is located as atdf_node_id_for_position.  A different active node there
+ * (with use_roi: both nodes pass in_roi, HyperTree.cu:1667-1690, whose
+ * "mid" is min + max) gives a pair (p in the node, p in the other node).
+ * Host struct. */
+typedef struct {
+    int32_t size[3];
+    float epsilon;
+    int32_t use_roi;
+    float roi_min[3];
+    float roi_max[3];
+} atdf_neighbor_spec;
+
+/* Candidates: num_active * 2 * (size1 size2 + size2 size0 + size0 size1),
+ * numbered per active node as (side, back / front, lattice index). */
+int64_t atdf_neighbor_candidates(int64_t num_active, const atdf_neighbor_spec* spec);
+/* flags[c] = 1 where candidate c is a pair (device int32). */
+atdf_status atdf_neighbor_flags(const void* packed_tree, int64_t num_nodes, const int64_t* active_ids,
+                                int64_t num_active, const atdf_neighbor_spec* spec, int32_t* flags,
+                                atdf_stream_t stream);
+/* With offsets = the exclusive scan of flags (atdf_exclusive_scan_i32, n + 1
+ * entries), pair k = offsets[c] of candidate c is written to samples 2k
+ * (this node) and 2k + 1 (the other): positions [2P, 3] f32, node_id [2P]
+ * i64 -- candidate order, so the list is deterministic (the reference
+ * appends pairs with an atomic counter). */
+atdf_status atdf_neighbor_write(const void* packed_tree, int64_t num_nodes, const int64_t* active_ids,
+                                int64_t num_active, const atdf_neighbor_spec* spec, const int64_t* offsets,
+                                float* positions, int64_t* node_id, atdf_stream_t stream);
+
 /* ======================================================================= */
 /* Scene data: detector rays, projections, random ray batches              */
 /* ======================================================================= */

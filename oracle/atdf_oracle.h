@@ -294,6 +294,12 @@ void atdfo_node_lattice_points(const float* node_min, const float* node_max, con
  * adds the gradient to grad (nullable). */
 double atdfo_tv_loss(const float* x, int64_t B, int64_t C, int D, const int* n, int64_t sb, int64_t sc,
                      const int64_t* ss, const float* weight, double scale, float* grad);
+/* NodeNeighborSamples pairs in candidate order; returns the pair count and
+ * writes at most cap pairs (pos [2P, 3], node [2P]). */
+int64_t atdfo_neighbor_pairs(const float* node_min, const float* node_max, const int32_t* children,
+                             const int32_t* active, const int64_t* active_ids, int64_t num_active, const int* size,
+                             float eps, int use_roi, const float* roi_min, const float* roi_max, float* pos,
+                             int64_t* node_out, int64_t cap);
 /* Random batch draws (image entry, uv) of the batch kernel. */
 void atdfo_random_pixels(const int32_t* indices, int64_t num_indices, const int32_t* image_camera, int w, int h,
                          int align_corners, int interpolate, uint64_t seed, int64_t n, float* uv, int32_t* image_id,

@@ -153,3 +153,63 @@ void atdfo_node_lattice_points(const float* node_min, const float* node_max, con
         }
     }
 }
+
+/* NodeNeighborSamples (structure/HyperTree.cu:1720-1811): pairs in
+ * candidate order (node, side, back / front, lattice index); in_roi of
+ * HyperTree.cu:1667-1690 with its mid = min + max.  Writes up to `cap`
+ * pairs (pos [2P, 3], node [2P]) and returns the pair count. */
+static int in_roi_(const float* mn, const float* mx, const float* lo, const float* hi)
+{
+    for (int k = 0; k < 3; ++k) {
+        float mid = mn[k] + mx[k];
+        if (!((mn[k] > lo[k] && mn[k] < hi[k]) || (mx[k] > lo[k] && mx[k] < hi[k]) || (mid > lo[k] && mid < hi[k])))
+            return 0;
+    }
+    return 1;
+}
+
+int64_t atdfo_neighbor_pairs(const float* node_min, const float* node_max, const int32_t* children,
+                             const int32_t* active, const int64_t* active_ids, int64_t num_active, const int* size,
+                             float eps, int use_roi, const float* roi_min, const float* roi_max, float* pos,
+                             int64_t* node_out, int64_t cap)
+{
+    int64_t np = 0;
+    for (int64_t a = 0; a < num_active; ++a) {
+        int64_t node = active_ids[a];
+        const float* mn = node_min + node * 3;
+        const float* mx = node_max + node * 3;
+        for (int side = 0; side < 3; ++side) {
+            int sa = (side + 1) % 3, sb = (side + 2) % 3;
+            for (int front = 0; front < 2; ++front) {
+                for (int sid = 0; sid < size[sa] * size[sb]; ++sid) {
+                    float local[3], g[3], q[3];
+                    local[side] = front ? 1.f : 0.f;
+                    local[sa] = (float)(sid % size[sa]) / ((float)size[sa] - 1.f);
+                    local[sb] = (float)(sid / size[sa]) / ((float)size[sb] - 1.f);
+                    for (int k = 0; k < 3; ++k) {
+                        float box = mx[k] - mn[k];
+                        g[k] = local[k] * box + mn[k];
+                        q[k] = g[k];
+                    }
+                    q[side] = g[side] + (front ? eps : -eps);
+                    int32_t other = atdfo_active_node_for_position(node_min, node_max, children, active, q);
+                    if (other < 0 || other == node) continue;
+                    if (use_roi && !(in_roi_(mn, mx, roi_min, roi_max) &&
+                                     in_roi_(node_min + (int64_t)other * 3, node_max + (int64_t)other * 3, roi_min,
+                                             roi_max)))
+                        continue;
+                    if (np < cap) {
+                        for (int k = 0; k < 3; ++k) {
+                            pos[(2 * np) * 3 + k] = g[k];
+                            pos[(2 * np + 1) * 3 + k] = g[k];
+                        }
+                        node_out[2 * np] = node;
+                        node_out[2 * np + 1] = other;
+                    }
+                    ++np;
+                }
+            }
+        }
+    }
+    return np;
+}

@@ -598,3 +598,19 @@ def tv_loss(x, channels_last=False, weight=None, scale=1.0, grad=None):
     loss = fn(_p(x), C.c_int64(nb), C.c_int64(nc), C.c_int(D), _p(_c(n + [1] * (3 - D), np.int32)), C.c_int64(sb),
               C.c_int64(sc), _p(_c(list(ss) + [0] * (3 - D), np.int64)), _p(w), C.c_double(scale), _p(g))
     return loss, g
+
+
+def neighbor_pairs(node_min, node_max, children, active, size, eps=1e-3, roi=None):
+    """NodeNeighborSamples: (pos [2P, 3], node [2P]) in candidate order."""
+    ids = active_ids(np.asarray(active))
+    args = [_p(_c(node_min, np.float32)), _p(_c(node_max, np.float32)), _p(_c(children, np.int32)),
+            _p(_c(active, np.int32)), _p(ids), C.c_int64(ids.size), _p(_c(size, np.int32)), C.c_float(eps),
+            C.c_int(roi is not None), _p(_c(roi[0] if roi else (0, 0, 0), np.float32)),
+            _p(_c(roi[1] if roi else (0, 0, 0), np.float32))]
+    fn = lib().atdfo_neighbor_pairs
+    fn.restype = C.c_int64
+    n = fn(*args, None, None, C.c_int64(0))
+    pos = np.zeros((2 * n, 3), np.float32)
+    node = np.zeros(2 * n, np.int64)
+    fn(*args, _p(pos), _p(node), C.c_int64(n))
+    return pos, node

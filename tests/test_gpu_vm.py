@@ -268,3 +268,49 @@ def test_volume_regularizer(oracle, native, cuda, tv3d):
             assert a is None or torch.all(a == 0), k
             continue
         assert _rel(a.cpu().double().numpy(), b.cpu().double().numpy()) <= 1e-4, k
+
+
+@pytest.mark.parametrize("use_roi", [False, True])
+def test_node_neighbor_samples_bit_exact(oracle, native, cuda, use_roi):
+    for depth, budget in ((3, 150), (4, 500)):
+        t = oracle.tree_build(depth)
+        act = adaptive_active(t, budget=budget, seed=depth)
+        tree = device_tree(depth, act)
+        roi = ((-0.6, -0.7, -0.5), (0.55, 0.65, 0.6)) if use_roi else None
+        s = tree.NodeNeighborSamples([4, 3, 5], 1e-3, list(roi[0]) if roi else [], list(roi[1]) if roi else [],
+                                     use_roi)
+        pos, node = oracle.neighbor_pairs(t["node_min"], t["node_max"], t["children"], act, (4, 3, 5), eps=1e-3,
+                                          roi=roi)
+        print(f"depth {depth} roi={use_roi}: {pos.shape[0] // 2} pairs", flush=True)
+        assert pos.shape[0] > 0
+        assert np.array_equal(s.global_coordinate.cpu().numpy(), pos)
+        assert np.array_equal(s.node_id.cpu().numpy(), node)
+        assert np.array_equal(s.local_index_in_ray.cpu().numpy(), np.arange(pos.shape[0]) % 2)
+
+
+def test_edge_loss_and_regularizer(oracle, native, cuda):
+    t, act, tree, roi, f, arr, tr, lohi = _vm_setup(native, oracle, "add", seed=5)
+    pairs = tree.NodeNeighborSamples([5, 5, 5], 1e-3)
+    p = native.RegularizerParams()
+    p.loss_tv, p.loss_l2, p.loss_edge = 0.0, -1.0, 0.05
+    names = ["plane_yz", "plane_xz", "plane_xy", "line_x", "line_y", "line_z", "grid_out"]
+    fields = [getattr(f, k) for k in names]
+    loss = native.VolumeRegularizer(tree, roi, f, p, pairs)
+    got = torch.autograd.grad(loss, fields, allow_unused=True)
+    # the same with libtorch ops: features of both samples, per-group mean of
+    # sqrt(mean_c (f1 - f2)^2) * loss_edge * C
+    feat = native.SampleVolumeIndirectFeature(tree, roi, f, pairs.global_coordinate, pairs.node_id)
+    e = (feat[0::2] - feat[1::2]).pow(2).mean(1).sqrt()
+    inr = native.ComputeBoundWeight(pairs.node_id[0::2].contiguous(), roi) >= 0
+    print(f"{int(inr.sum())} in-ROI pairs, {int((~inr).sum())} outside", flush=True)
+    assert 0 < int(inr.sum()) < inr.numel()
+    ref = e[inr].mean() * (0.05 * feat.shape[1]) + e[~inr].mean() * (0.05 * feat.shape[1])
+    want = torch.autograd.grad(ref, fields, allow_unused=True)
+    assert abs(loss.item() - ref.item()) <= 1e-6 * abs(ref.item())
+    for k, a, b in zip(names, got, want):
+        if b is None:
+            assert a is None or torch.all(a == 0), k
+            continue
+        assert _rel(a.cpu().double().numpy(), b.cpu().double().numpy()) <= 1e-5, k
+    # without pairs (or with the term off) there is no edge term
+    assert native.VolumeRegularizer(tree, roi, f, p) is None

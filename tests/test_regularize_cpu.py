@@ -72,3 +72,72 @@ def test_tv_boundary_without_device_work():
     st = lib.atdf_tv_loss(one, C.byref(sh), None, 1.0, one, None, one, 0, None)
     assert st == capi.ERR_INVALID_ARGUMENT and b"workspace" in lib.atdf_last_error()
     assert lib.atdf_tv_workspace_bytes(0) == 0 and lib.atdf_tv_workspace_bytes(1000) == 8 * 4
+
+
+# ---------------------------------------------------------------------------
+# neighbour pairs of the edge term (NodeNeighborSamples)
+# ---------------------------------------------------------------------------
+def test_neighbor_pairs_closed_forms(oracle):
+    """Depth-1 tree, eight active octants: every octant has three inner
+    faces and each of their lattice points pairs (24 s^2 pairs; outer faces
+    none).  Points on a face's inner edges pair with the first octant whose
+    closed box holds the moved point, which can be a diagonal neighbour."""
+    t = oracle.tree_build(1)
+    act = np.zeros(t["node_min"].shape[0], np.int32)
+    act[1:9] = 1
+    for s in (2, 3, 5):
+        pos, node = oracle.neighbor_pairs(t["node_min"], t["node_max"], t["children"], act, (s, s, s))
+        assert pos.shape[0] == 2 * 24 * s * s
+        assert np.array_equal(pos[0::2], pos[1::2])
+        a, b = node[0::2], node[1::2]
+        assert np.all(a != b) and np.all((a >= 1) & (a <= 8)) and np.all((b >= 1) & (b <= 8))
+        # every point lies on an inner plane, and on the plane the two octants differ in
+        p = pos[0::2]
+        diff = (a - 1) ^ (b - 1)
+        on_plane = np.zeros(p.shape[0], bool)
+        for ax in range(3):
+            on_plane |= ((diff >> ax) & 1).astype(bool) & (p[:, ax] == 0.0)
+        assert np.all(on_plane)
+        # interior points of a face pair with the octant straight across
+        interior = (np.abs(p) > 0).sum(1) == 2
+        assert np.all(np.isin(diff[interior], [1, 2, 4]))
+        # both boxes hold the point
+        for n_ in (a, b):
+            assert np.all((t["node_min"][n_] <= pos[0::2]) & (pos[0::2] <= t["node_max"][n_]))
+    # a root-only tree has no inner faces
+    act0 = np.zeros_like(act)
+    act0[0] = 1
+    pos, node = oracle.neighbor_pairs(t["node_min"], t["node_max"], t["children"], act0, (3, 3, 3))
+    assert pos.shape[0] == 0
+
+
+def test_neighbor_pairs_roi_rule(oracle):
+    """in_roi is true when, per axis, min, max or min + max lies strictly
+    inside the ROI: (-0.5, 0.5) takes every octant (min or max is 0), while
+    (0.1, 0.9) takes none (an octant that holds the ROI is not in it)."""
+    t = oracle.tree_build(1)
+    act = np.zeros(t["node_min"].shape[0], np.int32)
+    act[1:9] = 1
+    full, _ = oracle.neighbor_pairs(t["node_min"], t["node_max"], t["children"], act, (3, 3, 3))
+    inner, _ = oracle.neighbor_pairs(t["node_min"], t["node_max"], t["children"], act, (3, 3, 3),
+                                     roi=((-0.5, -0.5, -0.5), (0.5, 0.5, 0.5)))
+    none, _ = oracle.neighbor_pairs(t["node_min"], t["node_max"], t["children"], act, (3, 3, 3),
+                                    roi=((0.1, 0.1, 0.1), (0.9, 0.9, 0.9)))
+    assert inner.shape == full.shape and np.array_equal(inner, full) and none.shape[0] == 0
+
+
+def test_neighbor_pairs_adaptive(oracle):
+    """Adaptive tree: every pair point lies on a face of its node, the moved
+    point lies in the partner's closed box, and partners are active."""
+    from scene_util import adaptive_active
+
+    t = oracle.tree_build(3)
+    act = adaptive_active(t, budget=150, seed=4)
+    pos, node = oracle.neighbor_pairs(t["node_min"], t["node_max"], t["children"], act, (4, 3, 5), eps=1e-3)
+    assert pos.shape[0] > 0 and np.all(act[node] == 1)
+    a, b = node[0::2], node[1::2]
+    p = pos[0::2]
+    on_face = np.any((p == t["node_min"][a]) | (p == t["node_max"][a]), axis=1)
+    assert np.all(on_face)
+    lo, hi = t["node_min"][b] - 1e-3, t["node_max"][b] + 1e-3
+    assert np.all((lo <= p) & (p <= hi))
