@@ -253,6 +253,10 @@ PYBIND11_MODULE(TORCH_EXTENSION_NAME, m)
            std::optional<SampleList> pairs) { return VolumeRegularizer(tree, roi, f, p, pairs ? &*pairs : nullptr); },
         py::arg("tree"), py::arg("roi"), py::arg("fields"), py::arg("params"), py::arg("neighbor_pairs") = py::none());
     m.def("EdgeLoss", &EdgeLoss);
+    m.def("InitNodeGrid", &InitNodeGrid, py::arg("num_grids"), py::arg("C"), py::arg("G"), py::arg("init"),
+          py::arg("scale"), py::arg("seed") = 0, py::arg("channels_last") = true, py::arg("device") = "cpu");
+    m.def("InitTensorCoefficients", &InitTensorCoefficients, py::arg("shape"), py::arg("init"), py::arg("scale"),
+          py::arg("seed") = 0, py::arg("device") = "cpu");
     m.def("IntegrateSamplesXRay", &IntegrateSamplesXRay);
     m.def("AccumulateSampleLossPerNode", &AccumulateSampleLossPerNode);
     m.def("ProjectRays", &ProjectRays);

@@ -1,5 +1,7 @@
 // atdf_regularize.cpp -- TVLoss as an autograd Function over atdf_tv_loss,
 // and the plane / line / grid regulariser of the tensorial (VM) geometry.
+#include <ATen/CPUGeneratorImpl.h>
+
 #include "atdf_torch.h"
 
 namespace atdf_torch {
@@ -59,6 +61,43 @@ class TVLossFn : public torch::autograd::Function<TVLossFn> {
     }
 };
 }  // namespace
+
+Tensor InitNodeGrid(int64_t num_grids, int C, int G, const std::string& init, double scale, uint64_t seed,
+                    bool channels_last, const std::string& device)
+{
+    TORCH_CHECK(num_grids >= 0 && C >= 1 && G >= 2, "InitNodeGrid: bad shape");
+    std::vector<int64_t> shape = channels_last ? std::vector<int64_t>{num_grids, G, G, G, C}
+                                               : std::vector<int64_t>{num_grids, C, G, G, G};
+    auto opt = torch::TensorOptions().dtype(torch::kFloat32);
+    auto gen = at::detail::createCPUGenerator(seed);
+    Tensor g;
+    if (init == "uniform" || init == "random")
+        g = torch::empty(shape, opt).uniform_(-scale, scale, gen);
+    else if (init == "normal")
+        g = torch::empty(shape, opt).normal_(0.0, scale, gen);
+    else if (init == "minus")
+        g = -torch::ones(shape, opt);
+    else if (init == "zero")
+        g = torch::zeros(shape, opt);
+    else
+        TORCH_CHECK(false, "InitNodeGrid: unknown init '", init, "' (uniform, random, normal, minus, zero)");
+    return g.to(torch::Device(device));
+}
+
+Tensor InitTensorCoefficients(std::vector<int64_t> shape, const std::string& init, double scale, uint64_t seed,
+                              const std::string& device)
+{
+    auto opt = torch::TensorOptions().dtype(torch::kFloat32);
+    auto gen = at::detail::createCPUGenerator(seed);
+    Tensor t;
+    if (init == "random")
+        t = (torch::rand(shape, gen, opt) - 1.0f) * 2.0f * scale;
+    else if (init == "uniform")
+        t = torch::empty(shape, opt).uniform_(-scale, scale, gen);
+    else
+        TORCH_CHECK(false, "InitTensorCoefficients: unknown init '", init, "' (uniform, random)");
+    return t.to(torch::Device(device));
+}
 
 Tensor TVLoss(Tensor grid, Tensor weight, bool channels_last)
 {

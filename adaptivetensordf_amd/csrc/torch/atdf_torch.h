@@ -208,6 +208,18 @@ struct VMFields {
 torch::Tensor SampleVolumeIndirectFeature(HyperTreeBase& tree, const RoiMaps& roi, const VMFields& fields,
                                           torch::Tensor global_coordinate, torch::Tensor node_id);
 
+// Parameter-store initialisation.  Node grids (ExplicitFeatureGridImpl,
+// geometry/ImplicitNet.h:66-100): "uniform" / "random" U(-s, s), "minus" -1,
+// "zero" 0, plus "normal" N(0, s^2); shape [N, G, G, G, C] (channels_last) or
+// the reference's [N, C, G, G, G].  VM planes / lines (Tensor_PlaneQuadImpl /
+// Tensor_LineQuadImpl, geometry/tensorfield.h:9-127, 218-335): "uniform"
+// U(-s, s) or "random" s (U[0, 1) - 1) 2, which lies in (-2s, 0].  Drawn from
+// libtorch's generator seeded with `seed`, in f32 on `device`.
+torch::Tensor InitNodeGrid(int64_t num_grids, int C, int G, const std::string& init, double scale, uint64_t seed,
+                           bool channels_last, const std::string& device);
+torch::Tensor InitTensorCoefficients(std::vector<int64_t> shape, const std::string& init, double scale, uint64_t seed,
+                                     const std::string& device);
+
 // TVLoss::forward (geometry/geometry.h:31-81) on grid [B, C, n...] (1 to 3
 // spatial dims; channels_last: [B, n..., C]) with an optional per-batch weight,
 // as an autograd Function (loss and gradient from one atdf_tv_loss launch).

@@ -141,3 +141,23 @@ def test_neighbor_pairs_adaptive(oracle):
     assert np.all(on_face)
     lo, hi = t["node_min"][b] - 1e-3, t["node_max"][b] + 1e-3
     assert np.all((lo <= p) & (p <= hi))
+
+
+def test_parameter_store_init_modes(native):
+    g = native.InitNodeGrid(7, 16, 11, "uniform", 0.5, 3)
+    assert tuple(g.shape) == (7, 11, 11, 11, 16) and g.dtype == torch.float32
+    assert g.min() >= -0.5 and g.max() <= 0.5 and abs(g.mean().item()) < 0.01 and g.std().item() > 0.25
+    r = native.InitNodeGrid(7, 16, 11, "random", 0.5, 3, channels_last=False)
+    assert tuple(r.shape) == (7, 16, 11, 11, 11) and r.min() >= -0.5 and r.max() <= 0.5
+    n = native.InitNodeGrid(40, 16, 11, "normal", 0.01, 4)
+    assert abs(n.std().item() - 0.01) < 2e-4 and abs(n.mean().item()) < 1e-4
+    assert torch.all(native.InitNodeGrid(2, 8, 3, "minus", 1.0) == -1)
+    assert torch.all(native.InitNodeGrid(2, 8, 3, "zero", 1.0) == 0)
+    assert torch.equal(native.InitNodeGrid(3, 8, 5, "uniform", 0.1, 9), native.InitNodeGrid(3, 8, 5, "uniform", 0.1, 9))
+    with pytest.raises(RuntimeError, match="unknown init"):
+        native.InitNodeGrid(2, 8, 3, "gaussian", 1.0)
+    # the VM planes' "random" mode lies in (-2s, 0] (s (U[0,1) - 1) 2)
+    p = native.InitTensorCoefficients([20, 8, 9, 9], "random", 0.3, 1)
+    assert p.max() <= 0 and p.min() > -0.6 and abs(p.mean().item() + 0.3) < 0.01
+    u = native.InitTensorCoefficients([20, 8, 9, 9], "uniform", 0.3, 1)
+    assert u.min() >= -0.3 and u.max() <= 0.3
