@@ -150,6 +150,7 @@ __device__ __forceinline__ bool ray_features(const PNode& root, const int32_t* _
 }
 
 // y[o] = act(sum_k W[o][k] x[k] + b[o]) for o < NOUT into this lane's column
+// (the sum as a chain of fused multiply-adds in k order, then + b)
 // of buf ([o][64 lanes]); W rows of `stride` floats in LDS.
 template <int NIN, int STRIDE, int NOUT, bool RELU>
 __device__ __forceinline__ void dense_lane(const float* W, const float* b, const float* x, float* buf, int lane)
@@ -160,13 +161,13 @@ __device__ __forceinline__ void dense_lane(const float* W, const float* b, const
 #pragma unroll
         for (int k = 0; k + 4 <= NIN; k += 4) {
             const float4 w = *reinterpret_cast<const float4*>(wr + k);
-            acc = acc + w.x * x[k];
-            acc = acc + w.y * x[k + 1];
-            acc = acc + w.z * x[k + 2];
-            acc = acc + w.w * x[k + 3];
+            acc = fmaf(w.x, x[k], acc);
+            acc = fmaf(w.y, x[k + 1], acc);
+            acc = fmaf(w.z, x[k + 2], acc);
+            acc = fmaf(w.w, x[k + 3], acc);
         }
 #pragma unroll
-        for (int k = NIN & ~3; k < NIN; ++k) acc = acc + wr[k] * x[k];
+        for (int k = NIN & ~3; k < NIN; ++k) acc = fmaf(wr[k], x[k], acc);
         acc = acc + b[o];
         buf[o * 64 + lane] = RELU ? (acc > 0.f ? acc : 0.f) : acc;
     }
@@ -183,7 +184,7 @@ __device__ __forceinline__ void dense_lane_s(const float* __restrict__ W, const 
         const float* wr = W + o * NIN;
         float acc = 0.f;
 #pragma unroll
-        for (int k = 0; k < NIN; ++k) acc = acc + wr[k] * x[k];
+        for (int k = 0; k < NIN; ++k) acc = fmaf(wr[k], x[k], acc);
         acc = acc + b[o];
         buf[o * 64 + lane] = RELU ? (acc > 0.f ? acc : 0.f) : acc;
     }

@@ -583,8 +583,9 @@ atdf_status atdf_point_in_any_image(const atdf_camera_rig* rig, const float* roi
  * kernels integrate like any other interval.  The cells tile the chord, so
  * with jitter the estimate is unbiased for any pdf.  A ray falls
  * back to the stratified traversal when KL(coarse occupancy || pdf) exceeds
- * kl_threshold.  Every FP32 operation runs in a fixed order with no
- * contraction, so logits, KL, fallback decisions and sample placement are
+ * kl_threshold.  Every FP32 operation runs in a fixed order (the dense
+ * layers' dot products as fused multiply-add chains, no other contraction),
+ * so logits, KL, fallback decisions and sample placement are
  * reproducible bit for bit on the CPU.
  * params (torch Linear layout [out, in]): W1 [64, 38] b1 [64] W2 [64, 64]
  * b2 [64] W3 [64, 64] b3 [64] W4 [32, 64] b4 [32]. */

@@ -6,7 +6,8 @@
  * (ray features -> MLP -> 32-bin pdf -> 32 equal-probability cells of the
  * chord, one sample each -> count-1 node intervals, KL gate against the
  * coarse occupancy) and its training loss.  The FP32 functions fix the rounding order the kernels
- * follow (compiled with -ffp-contract=off); atdfo_prop_grad is the FP64
+ * follow (compiled with -ffp-contract=off; the dense layers' dot products are
+ * explicit fmaf chains, correctly rounded on both sides); atdfo_prop_grad is the FP64
  * restatement the training kernels are checked against.
  *
  * Reference anchors: the slab test of the root chord is
@@ -188,7 +189,7 @@ static void dense_f(const float* W, const float* b, int nin, int nout, const flo
 {
     for (int o = 0; o < nout; ++o) {
         float acc = 0;
-        for (int k = 0; k < nin; ++k) acc = acc + W[o * nin + k] * x[k];
+        for (int k = 0; k < nin; ++k) acc = fmaf(W[o * nin + k], x[k], acc);  /* fused, as the kernels */
         acc = acc + b[o];
         y[o] = relu ? (acc > 0 ? acc : 0) : acc;
     }
