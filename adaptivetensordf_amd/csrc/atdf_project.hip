@@ -237,7 +237,8 @@ __device__ __forceinline__ int64_t corner_cell(const Tri& t, int k, int G)
     return ((int64_t)z * G + y) * G + x;
 }
 
-// Gather CPL channels starting at c0 (canonical corner order per channel).
+// Gather CPL channels starting at c0 (canonical corner order per channel,
+// accumulated as a fused multiply-add chain).
 template <int C, bool kBF16>
 __device__ __forceinline__ void gather_channels(const ProjArgs& a, const SampleCtx& sc, int c0, float* x)
 {
@@ -281,7 +282,7 @@ __device__ __forceinline__ void gather_channels(const ProjArgs& a, const SampleC
     for (int q = 0; q < CPL; ++q) {
         float acc = 0.f;
 #pragma unroll
-        for (int k = 0; k < 8; ++k) acc += v[k][q] * sc.tri.w[k];
+        for (int k = 0; k < 8; ++k) acc = fmaf(v[k][q], sc.tri.w[k], acc);
         x[q] = acc;
     }
 }
@@ -1133,10 +1134,10 @@ __global__ __launch_bounds__(64 * kWaves) void project_bwd_node_kernel(NodeBwdAr
                         float4* gp = reinterpret_cast<float4*>(lgrad + corner_cell(sc.tri, k, G) * C + 4 * g);
                         const float wk = sc.tri.w[k];
                         float4 v = *gp;
-                        v.x += wk * dx[0];
-                        v.y += wk * dx[1];
-                        v.z += wk * dx[2];
-                        v.w += wk * dx[3];
+                        v.x = fmaf(wk, dx[0], v.x);
+                        v.y = fmaf(wk, dx[1], v.y);
+                        v.z = fmaf(wk, dx[2], v.z);
+                        v.w = fmaf(wk, dx[3], v.w);
                         *gp = v;
                     }
                 }

@@ -436,7 +436,9 @@ def test_node_forward_prepared(oracle, cuda, C_, bf16):
     for a, b in ((m0, m1), (v0, v1), (gm0, gm1)):
         assert ((a - b).abs().max() / a.abs().max()).item() <= 1e-5
     big = m0.abs() > 1e-3 * m0.abs().max()
-    assert ((p1 - p0).abs()[big].max() / (p0 - grid.cpu()).abs().max()).item() <= 1e-5
+    # within 1e-5 of the step, plus one ulp of the parameter for the final rounding
+    bound = 1e-5 * (p0 - grid.cpu()).abs().max() + 2 ** -23 * p0.abs()
+    assert torch.all(((p1 - p0).abs() <= bound)[big])
 
 
 @pytest.mark.parametrize("bf16", [False, True])
