@@ -1078,3 +1078,47 @@ double atdfo_train_step_f32(atdfo_model* m, float* grid_mut, float* mlp_mut, int
     }
     return loss / (double)R;
 }
+
+/* ------------------------------------------------------------------------ */
+/* density of the fused field at given points (volume error)                 */
+/* ------------------------------------------------------------------------ */
+/* density[i] = softplus(decoder(trilinear(grid of node[i], pos[i]))), in the
+ * FP32 path (precision 0) or the FP64 path (precision 1); node < 0 gives 0. */
+void atdfo_field_density(const atdfo_model* m, const float* pos, const int64_t* node, int64_t n, int precision,
+                         double* density)
+{
+    int C = m->C, H = m->H;
+#pragma omp parallel
+    {
+        float* featf = (float*)malloc(sizeof(float) * (C + 2 * H));
+        double* featd = (double*)malloc(sizeof(double) * (C + 2 * H));
+#pragma omp for schedule(static)
+        for (int64_t i = 0; i < n; ++i) {
+            if (node[i] < 0) {
+                density[i] = 0.0;
+                continue;
+            }
+            sample_t_ smp;
+            smp.node = (int32_t)node[i];
+            for (int k = 0; k < 3; ++k) smp.pos[k] = pos[i * 3 + k];
+            if (precision == 0) {
+                gather_cl_f(m, &smp, featf);
+                density[i] = mlp_fwd_one_f(m->mlp, C, H, m->beta, featf, featf + C, featf + C + H, NULL);
+            } else {
+                tri_d t;
+                int64_t base;
+                double yp;
+                if (m->decoder_bf16) {
+                    gather_cl_f(m, &smp, featf);
+                    for (int c = 0; c < C; ++c) featd[c] = featf[c];
+                } else {
+                    gather_cl_d(m, &smp, featd, &t, &base);
+                }
+                density[i] = mlp_fwd_one_d(m->mlp, C, H, m->beta, featd, featd + C, featd + C + H, &yp,
+                                           m->decoder_bf16);
+            }
+        }
+        free(featf);
+        free(featd);
+    }
+}

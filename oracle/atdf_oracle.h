@@ -300,6 +300,10 @@ int64_t atdfo_neighbor_pairs(const float* node_min, const float* node_max, const
                              const int32_t* active, const int64_t* active_ids, int64_t num_active, const int* size,
                              float eps, int use_roi, const float* roi_min, const float* roi_max, float* pos,
                              int64_t* node_out, int64_t cap);
+/* Density of the fused field at points pos [n, 3] located in node[n]
+ * (< 0: 0), FP32 path (precision 0) or FP64 path (precision 1). */
+void atdfo_field_density(const atdfo_model* m, const float* pos, const int64_t* node, int64_t n, int precision,
+                         double* density);
 /* Random batch draws (image entry, uv) of the batch kernel. */
 void atdfo_random_pixels(const int32_t* indices, int64_t num_indices, const int32_t* image_camera, int w, int h,
                          int align_corners, int interpolate, uint64_t seed, int64_t n, float* uv, int32_t* image_id,

@@ -614,3 +614,12 @@ def neighbor_pairs(node_min, node_max, children, active, size, eps=1e-3, roi=Non
     node = np.zeros(2 * n, np.int64)
     fn(*args, _p(pos), _p(node), C.c_int64(n))
     return pos, node
+
+
+def field_density(fm, pos, node, precision=1):
+    """Density of the FusedModel field at points pos [n, 3] in node [n]."""
+    pos = _c(pos, np.float32).reshape(-1, 3)
+    node = _c(node, np.int64)
+    out = np.zeros(pos.shape[0], np.float64)
+    lib().atdfo_field_density(C.byref(fm.m), _p(pos), _p(node), C.c_int64(pos.shape[0]), C.c_int(precision), _p(out))
+    return out

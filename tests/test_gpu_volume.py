@@ -86,3 +86,43 @@ def test_volume_through_analyze(oracle, native, cuda, tmp_path):
     native.SaveHDRImageTensor(vol, path)
     back = native.LoadHDRImageTensor(path)
     assert torch.equal(back, vol.cpu())
+
+
+def test_bf16_volume_error_on_crop(oracle, native, cuda):
+    """The BF16 field (BF16 grids, BF16-MFMA decoder, FP32 accumulation) on a
+    64^3 lattice over the crop [-0.5, 0.5]^3 (the centre eighth of the cube,
+    a 256^3 crop of the 512^3 volume sampled every fourth voxel) against the
+    FP64 oracle: of the f32 master grid (the precision cost of BF16, stated
+    and bounded) and of the BF16-rounded grid (the kernel's own arithmetic)."""
+    from scene_util import adaptive_active, device_tree
+    from adaptivetensordf_amd import phantom
+
+    t = oracle.tree_build(4)
+    act = adaptive_active(t, budget=400, seed=2)
+    tree = device_tree(4, act)
+    na = int(act.sum())
+    grid = phantom.init_grid(na, 11, 16, std=0.3, seed=5)
+    mlp = phantom.init_mlp(16, 64, seed=5)
+    fp = native.FieldParams()
+    fp.C, fp.G = 16, 11
+    fp.bf16_decoder = True
+    gb = grid.cuda().to(torch.bfloat16)
+    n = 64
+    roi, step = [-0.5] * 3, [1.0 / (n - 1)] * 3
+    vol = native.SampleVolumeBySlice(tree, gb, mlp.cuda(), fp, [n, n, n], roi, step).cpu().numpy().reshape(-1)
+    pos = oracle.volume_points(1, (n, n, n), roi, step)
+    node, mask = oracle.node_id_for_position(t["node_min"], t["node_max"], t["children"], act, pos)
+    node = np.where(mask > 0, node, -1)
+    f32 = oracle.FusedModel(t["node_min"], t["node_max"], act, grid.numpy(), mlp.numpy(), 16)
+    fbf = oracle.FusedModel(t["node_min"], t["node_max"], act, gb.float().cpu().numpy(), mlp.numpy(), 16,
+                            decoder_bf16=True)
+    d64 = oracle.field_density(f32, pos, node, precision=1)
+    dbf = oracle.field_density(fbf, pos, node, precision=1)
+    scale = np.abs(d64).max()
+    e_bf16 = np.abs(vol - d64).max() / scale
+    rms_bf16 = np.sqrt(np.mean((vol - d64) ** 2)) / np.sqrt(np.mean(d64 ** 2))
+    e_self = np.abs(vol - dbf).max() / scale
+    print(f"64^3 crop: BF16 field vs FP64(f32 grid) max {e_bf16:.2e} rms {rms_bf16:.2e}; "
+          f"vs FP64(bf16 grid, bf16 decoder inputs) max {e_self:.2e}", flush=True)
+    assert np.all(vol[node < 0] == 0)
+    assert e_self <= 1e-3 and e_bf16 <= 3e-2 and rms_bf16 <= 1e-2
