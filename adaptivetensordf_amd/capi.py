@@ -166,6 +166,11 @@ _SIGNATURES = {
                                               _P, _P, _P, _P, _P]),
     "atdf_tv_workspace_bytes": (_SZ, [_I64]),
     "atdf_tv_loss": (_I32, [_P, C.POINTER(TVShape), _P, C.c_double, _P, _P, _P, _SZ, _P]),
+    "atdf_fourier_ring_workspace_bytes": (_SZ, [_I32, _I32]),
+    "atdf_fourier_ring_means": (_I32, [_P, _I64, _I32, _I32, _P, _P, _SZ, _P]),
+    "atdf_decode_features": (_I32, [_P, _I64, C.POINTER(Field), _P, _P]),
+    "atdf_decode_features_bwd_workspace_bytes": (_SZ, [_I64, C.POINTER(Field)]),
+    "atdf_decode_features_bwd": (_I32, [_P, _I64, C.POINTER(Field), _P, _P, _P, _P, _SZ, _P]),
     "atdf_neighbor_candidates": (_I64, [_I64, C.POINTER(NeighborSpec)]),
     "atdf_neighbor_flags": (_I32, [_P, _I64, _P, _I64, C.POINTER(NeighborSpec), _P, _P]),
     "atdf_neighbor_write": (_I32, [_P, _I64, _P, _I64, C.POINTER(NeighborSpec), _P, _P, _P, _P]),

@@ -2493,3 +2493,209 @@ atdf_status atdf_node_density_stats_masked(const void* packed_tree, int64_t num_
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------------------
+// DecodeFeatures on given feature rows (HierarchicalNeuralGeometry::
+// DecodeFeatures, geometry/geometry.cpp:176-253, softplus output):
+// density[i] = softplus(MLP(x_i)), and its adjoint.  Tiles of 16 rows per
+// wave with the fused kernels' lane layout (lane (g, j): row j, channels
+// CPL*g ..), the same decoder code on MFMA, a fixed grid-stride assignment of
+// tiles to waves, and the decoder-parameter gradient through per-wave
+// partial rows and the fixed-order reduction of the projection adjoint, so
+// both directions are deterministic.
+// ---------------------------------------------------------------------------
+namespace atdf {
+
+struct DecodeArgs {
+    const float* mlp;
+    const float* x;   // [n, C]
+    const float* dd;  // [n] dL/ddensity (adjoint)
+    float* density;   // [n] (forward)
+    float* dx;        // [n, C] (adjoint, written)
+    float* partials;  // [blocks * kWaves, NP] (adjoint)
+    int64_t n;
+    float beta;
+};
+
+constexpr int kDecodeMaxBlocks = 1024;
+
+template <int C>
+__device__ __forceinline__ void load_feature_row(const DecodeArgs& a, int64_t row, int g, float* x)
+{
+    constexpr int CPL = C / 4;
+#pragma unroll
+    for (int q = 0; q < CPL; ++q) x[q] = row < a.n ? a.x[row * C + CPL * g + q] : 0.f;
+}
+
+template <int C, bool kDecBF16>
+__global__ __launch_bounds__(64 * kWaves, 4) void decode_fwd_kernel(DecodeArgs a)
+{
+    using SM = DecoderSmem<C, kDecBF16>;
+    constexpr int CPL = C / 4;
+    __shared__ __attribute__((aligned(16))) float s[SM::kF32];
+    __shared__ __attribute__((aligned(16))) __bf16 sb[SM::kBF16];
+    if constexpr (kDecBF16)
+        load_mlp_bf16_to_lds<C>(a.mlp, sb, s);
+    else
+        load_mlp_to_lds<C>(a.mlp, s);
+    __syncthreads();
+    const int lane = threadIdx.x & 63;
+    const int g = lane >> 4, j = lane & 15;
+    const int64_t ntiles = (a.n + kTile - 1) / kTile;
+    for (int64_t tile = (int64_t)blockIdx.x * kWaves + (threadIdx.x >> 6); tile < ntiles;
+         tile += (int64_t)gridDim.x * kWaves) {
+        const int64_t row = tile * kTile + j;
+        float x[CPL];
+        load_feature_row<C>(a, row, g, x);
+        v4f h1[4], h2[4];
+        int opq = 0;
+        asm volatile("" : "+v"(opq));  // no hoisting of the weight reads
+        const float* so = s + opq;
+        const __bf16* sbo = sb + opq;
+        float y;
+        if constexpr (kDecBF16)
+            y = decoder_fwd_bf16<C>(sbo, so, x, g, j, h1, h2);
+        else
+            y = decoder_fwd<C>(so, x, g, j, h1, h2);
+        if (g == 0 && row < a.n) a.density[row] = softplus_f(y, a.beta);
+    }
+}
+
+template <int C, bool kDecBF16>
+__global__ __launch_bounds__(64 * kWaves) void decode_bwd_kernel(DecodeArgs a)
+{
+    using SM = DecoderSmem<C, kDecBF16>;
+    constexpr int CPL = C / 4;
+    constexpr int NP = mlp_numel<C>();
+    __shared__ __attribute__((aligned(16))) float s[SM::kF32];
+    __shared__ __attribute__((aligned(16))) __bf16 sb[SM::kBF16];
+    __shared__ __attribute__((aligned(16))) float tb[kDecBF16 ? 4 : kWaves * 2 * kH * kTS];
+    __shared__ __attribute__((aligned(16))) __bf16 tbb[kDecBF16 ? kWaves * 2 * kH * kTB : 8];
+    if constexpr (kDecBF16)
+        load_mlp_bf16_to_lds<C>(a.mlp, sb, s);
+    else
+        load_mlp_to_lds<C>(a.mlp, s);
+    __syncthreads();
+    const int lane = threadIdx.x & 63;
+    const int g = lane >> 4, j = lane & 15;
+    const int wl = threadIdx.x >> 6;
+    float* bufA = tb + (kDecBF16 ? 0 : wl * 2 * kH * kTS);
+    float* bufB = bufA + (kDecBF16 ? 0 : kH * kTS);
+    __bf16* bufAb = tbb + (kDecBF16 ? wl * 2 * kH * kTB : 0);
+    __bf16* bufBb = bufAb + (kDecBF16 ? kH * kTB : 0);
+    DecGrads q;
+    zero_grads(q);
+    const int64_t ntiles = (a.n + kTile - 1) / kTile;
+    for (int64_t tile = (int64_t)blockIdx.x * kWaves + wl; tile < ntiles; tile += (int64_t)gridDim.x * kWaves) {
+        const int64_t row = tile * kTile + j;
+        float x[CPL];
+        load_feature_row<C>(a, row, g, x);
+        v4f h1[4], h2[4];
+        float y;
+        if constexpr (kDecBF16)
+            y = decoder_fwd_bf16<C>(sb, s, x, g, j, h1, h2);
+        else
+            y = decoder_fwd<C>(s, x, g, j, h1, h2);
+        const float dy = row < a.n ? a.dd[row] * softplus_grad_f(y, a.beta) : 0.f;
+        const v4f dx = tile_backward<C, kDecBF16>(s, sb, bufA, bufB, bufAb, bufBb, x, h1, h2, dy, g, j, q);
+        if (row < a.n && 4 * g < C) {
+#pragma unroll
+            for (int k = 0; k < 4; ++k) a.dx[row * C + 4 * g + k] = dx[k];
+        }
+    }
+    store_partials<C>(q, a.partials + ((int64_t)blockIdx.x * kWaves + wl) * NP, g, j, lane);
+}
+
+int decode_blocks(int64_t n)
+{
+    const int64_t ntiles = div_up(n, kTile);
+    return (int)std::min<int64_t>(div_up(ntiles, kWaves), kDecodeMaxBlocks);
+}
+
+atdf_status check_decoder(const atdf_field* f, const char* who)
+{
+    ATDF_REQUIRE(f, "%s: null field", who);
+    ATDF_REQUIRE(f->H == 64 && (f->C == 16 || f->C == 8), "%s: decoder shape C=%d H=%d not built (C 16 or 8, H 64)",
+                 who, f->C, f->H);
+    ATDF_REQUIRE(f->mlp, "%s: null decoder parameters", who);
+    ATDF_REQUIRE(f->decoder_dtype == ATDF_DECODER_F32 || f->decoder_dtype == ATDF_DECODER_BF16,
+                 "%s: unknown decoder dtype %d", who, f->decoder_dtype);
+    return ATDF_OK;
+}
+
+}  // namespace atdf
+
+extern "C" {
+
+atdf_status atdf_decode_features(const float* features, int64_t n, const atdf_field* f, float* density,
+                                 atdf_stream_t stream)
+{
+    atdf_status st = check_decoder(f, "atdf_decode_features");
+    if (st) return st;
+    ATDF_REQUIRE(n >= 0, "atdf_decode_features: negative row count");
+    if (n == 0) return ATDF_OK;
+    ATDF_REQUIRE(features && density, "atdf_decode_features: null buffer");
+    DecodeArgs a{};
+    a.mlp = f->mlp;
+    a.x = features;
+    a.density = density;
+    a.n = n;
+    a.beta = f->softplus_beta;
+    const dim3 grid((unsigned)std::min<int64_t>(div_up(div_up(n, kTile), kWaves), 65535)), block(64 * kWaves);
+    hipStream_t s = to_hip(stream);
+    const bool bd = f->decoder_dtype == ATDF_DECODER_BF16;
+    if (f->C == 16) {
+        if (bd) hipLaunchKernelGGL((decode_fwd_kernel<16, true>), grid, block, 0, s, a);
+        else    hipLaunchKernelGGL((decode_fwd_kernel<16, false>), grid, block, 0, s, a);
+    } else {
+        if (bd) hipLaunchKernelGGL((decode_fwd_kernel<8, true>), grid, block, 0, s, a);
+        else    hipLaunchKernelGGL((decode_fwd_kernel<8, false>), grid, block, 0, s, a);
+    }
+    return check_launch("atdf_decode_features");
+}
+
+size_t atdf_decode_features_bwd_workspace_bytes(int64_t n, const atdf_field* f)
+{
+    if (!f || n <= 0 || (f->C != 16 && f->C != 8)) return 0;
+    const int64_t np = (int64_t)f->H * f->C + f->H + (int64_t)f->H * f->H + f->H + f->H + 1;
+    return sizeof(float) * (size_t)decode_blocks(n) * kWaves * (size_t)np;
+}
+
+atdf_status atdf_decode_features_bwd(const float* features, int64_t n, const atdf_field* f,
+                                     const float* grad_density, float* grad_features, float* grad_mlp,
+                                     void* workspace, size_t workspace_bytes, atdf_stream_t stream)
+{
+    atdf_status st = check_decoder(f, "atdf_decode_features_bwd");
+    if (st) return st;
+    ATDF_REQUIRE(n >= 0, "atdf_decode_features_bwd: negative row count");
+    if (n == 0) return ATDF_OK;
+    ATDF_REQUIRE(features && grad_density && grad_features && grad_mlp, "atdf_decode_features_bwd: null buffer");
+    const size_t need = atdf_decode_features_bwd_workspace_bytes(n, f);
+    ATDF_REQUIRE(workspace && workspace_bytes >= need, "atdf_decode_features_bwd: workspace of %zu bytes, %zu needed",
+                 workspace_bytes, need);
+    DecodeArgs a{};
+    a.mlp = f->mlp;
+    a.x = features;
+    a.dd = grad_density;
+    a.dx = grad_features;
+    a.partials = static_cast<float*>(workspace);
+    a.n = n;
+    a.beta = f->softplus_beta;
+    const int blocks = decode_blocks(n);
+    const int np = (int)(need / sizeof(float) / ((size_t)blocks * kWaves));
+    hipStream_t s = to_hip(stream);
+    const bool bd = f->decoder_dtype == ATDF_DECODER_BF16;
+    const dim3 grid((unsigned)blocks), block(64 * kWaves);
+    if (f->C == 16) {
+        if (bd) hipLaunchKernelGGL((decode_bwd_kernel<16, true>), grid, block, 0, s, a);
+        else    hipLaunchKernelGGL((decode_bwd_kernel<16, false>), grid, block, 0, s, a);
+    } else {
+        if (bd) hipLaunchKernelGGL((decode_bwd_kernel<8, true>), grid, block, 0, s, a);
+        else    hipLaunchKernelGGL((decode_bwd_kernel<8, false>), grid, block, 0, s, a);
+    }
+    hipLaunchKernelGGL(reduce_partials_kernel, dim3((unsigned)div_up(np, kReduceParams)), dim3(256), 0, s,
+                       a.partials, (int64_t)blocks * kWaves, np, grad_mlp);
+    return check_launch("atdf_decode_features_bwd");
+}
+
+}  // extern "C"

@@ -95,6 +95,150 @@ __global__ __launch_bounds__(kTVBlock) void tv_finish_kernel(const double* __res
 
 int tv_blocks(int64_t total) { return (int)std::min<int64_t>(div_up(total, kTVBlock), kTVMaxBlocks); }
 
+// ---------------------------------------------------------------------------
+// Ring means of the Fourier regulariser (geometry_tensor_qo.cpp:784-808) on
+// img [Y, X, Z] (the shifted FFT magnitudes, one X x Z slice per y): ring i
+// (1 <= i < X / 2) has n = (int)(2 pi i) angles of step s = (float)(360 / n),
+// th_k = k s for k < m = ceil(360 / s) (torch::arange(0, 360, s), so m is n
+// or n + 1), rows r_a = i cos(th_a pi / 180) + X / 2 and columns
+// c_b = i sin(th_b pi / 180) + Z / 2, and its value is the mean over all m^2
+// pairs (a, b) -- torch::meshgrid of the two coordinate lists, not the m
+// points of the circle -- of the bilinear value at (r_a, c_b): grid_sample
+// with align_corners and border padding, both coordinates normalised by
+// X - 1 and mapped back onto X - 1 rows and Z - 1 columns, then clamped.
+// Bilinear weights factor into a row weight and a column weight, so the
+// mean is sum_{row, col} A_i[row] B_i[col] img[y, row, col] / m^2 with
+// A_i[row] = sum_a (weight of row in r_a's interpolation) and B_i likewise:
+// ring_weights_kernel builds A and B (FP64, each row summed over the angles
+// in order) and the index range where each is non-zero, ring_means_kernel
+// forms the bilinear form over that support in FP64 with a fixed-order
+// reduction.  Deterministic.
+// ---------------------------------------------------------------------------
+constexpr int kRingMaxAngles = 8192;
+constexpr int kRingBlock = 256;
+
+struct RingGeom {
+    int X, Z, rings;
+    int64_t Y;
+};
+
+__host__ __device__ inline void ring_angles(int i, float& step, int& m)
+{
+    const int n = (int)(2.0 * 3.14159265358979323846 * (double)i);
+    step = (float)(360.0 / (double)n);
+    m = (int)ceil(360.0 / (double)step);
+}
+
+// Coordinate along one axis after the reference's float operations:
+// th / 180 (times the float reciprocal), * pi, cos or sin, * i, + centre,
+// / (X - 1) (reciprocal), * 2 - 1, then grid_sample's ((g + 1) / 2) * (size - 1)
+// and the border clamp.
+__device__ __forceinline__ float ring_coord(int k, float step, int i, int centre, bool use_sin, int X, int size)
+{
+    const float th = (float)k * step;
+    const float a = (th * (1.0f / 180.0f)) * 3.14159265358979323846f;
+    const float cs = use_sin ? sinf(a) : cosf(a);
+    const float p = (float)i * cs + (float)centre;
+    const float gn = (p * (1.0f / (float)(X - 1))) * 2.0f - 1.0f;
+    float u = ((gn + 1.0f) / 2.0f) * (float)(size - 1);
+    u = fminf((float)(size - 1), fmaxf(u, 0.0f));
+    return u;
+}
+
+// grid: rings x 2 workgroups (axis 0 = rows over X, axis 1 = columns over Z);
+// bounds[4 i + 2 axis + {0, 1}] = first / last index with a non-zero weight
+// (the columns are scaled by (Z - 1) / (X - 1), so the support is not the
+// ring's box around the centre when Z != X)
+__global__ __launch_bounds__(kRingBlock) void ring_weights_kernel(RingGeom gm, double* __restrict__ A,
+                                                                  double* __restrict__ B, int* __restrict__ bounds)
+{
+    __shared__ float coord[kRingMaxAngles];
+    __shared__ int s_lo, s_hi;
+    const int i = blockIdx.x >> 1, axis = blockIdx.x & 1;
+    const int size = axis ? gm.Z : gm.X;
+    double* W = (axis ? B + (int64_t)i * gm.Z : A + (int64_t)i * gm.X);
+    if (i == 0) {
+        for (int r = threadIdx.x; r < size; r += kRingBlock) W[r] = 0.0;
+        if (threadIdx.x == 0) {
+            bounds[4 * i + 2 * axis] = 0;
+            bounds[4 * i + 2 * axis + 1] = -1;
+        }
+        return;
+    }
+    if (threadIdx.x == 0) {
+        s_lo = size;
+        s_hi = -1;
+    }
+    float step;
+    int m;
+    ring_angles(i, step, m);
+    const int centre = axis ? gm.Z / 2 : gm.X / 2;
+    for (int k = threadIdx.x; k < m; k += kRingBlock) coord[k] = ring_coord(k, step, i, centre, axis == 1, gm.X, size);
+    __syncthreads();
+    for (int r = threadIdx.x; r < size; r += kRingBlock) {
+        double w = 0.0;
+        for (int k = 0; k < m; ++k) {
+            const float u = coord[k];
+            const float f0 = floorf(u);
+            const int r0 = (int)f0;
+            if (r == r0) w += (double)((f0 + 1.0f) - u);
+            else if (r == r0 + 1) w += (double)(u - f0);
+        }
+        W[r] = w;
+        if (w != 0.0) {
+            atomicMin(&s_lo, r);
+            atomicMax(&s_hi, r);
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        bounds[4 * i + 2 * axis] = s_lo;
+        bounds[4 * i + 2 * axis + 1] = s_hi;
+    }
+}
+
+// grid: rings * Y workgroups, one per (ring, slice)
+__global__ __launch_bounds__(kRingBlock) void ring_means_kernel(RingGeom gm, const float* __restrict__ img,
+                                                                const double* __restrict__ A,
+                                                                const double* __restrict__ B,
+                                                                const int* __restrict__ bounds, float* __restrict__ out)
+{
+    __shared__ double red[kRingBlock];
+    const int64_t blk = blockIdx.x;
+    const int i = (int)(blk / gm.Y);
+    const int64_t y = blk % gm.Y;
+    if (i == 0) {
+        // the reference leaves this row uninitialised (torch::empty); +inf
+        // keeps its entries out of the masked means
+        if (threadIdx.x == 0) out[y] = INFINITY;
+        return;
+    }
+    const int r_lo = bounds[4 * i], r_hi = bounds[4 * i + 1];
+    const int c_lo = bounds[4 * i + 2], c_hi = bounds[4 * i + 3];
+    const int nc = c_hi - c_lo + 1;
+    const int64_t ne = (r_hi < r_lo || nc <= 0) ? 0 : (int64_t)(r_hi - r_lo + 1) * nc;
+    const double* Ai = A + (int64_t)i * gm.X;
+    const double* Bi = B + (int64_t)i * gm.Z;
+    const float* im = img + y * (int64_t)gm.X * gm.Z;
+    double acc = 0.0;
+    for (int64_t e = threadIdx.x; e < ne; e += kRingBlock) {
+        const int r = r_lo + (int)(e / nc), c = c_lo + (int)(e % nc);
+        acc += Ai[r] * Bi[c] * (double)im[(int64_t)r * gm.Z + c];
+    }
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (int off = kRingBlock / 2; off > 0; off >>= 1) {
+        if ((int)threadIdx.x < off) red[threadIdx.x] = red[threadIdx.x] + red[threadIdx.x + off];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        float step;
+        int m;
+        ring_angles(i, step, m);
+        out[(int64_t)i * gm.Y + y] = (float)(red[0] / ((double)m * (double)m));
+    }
+}
+
 }  // namespace atdf
 
 using namespace atdf;
@@ -149,6 +293,39 @@ atdf_status atdf_tv_loss(const float* x, const atdf_tv_shape* shape, const float
     hipLaunchKernelGGL(tv_kernel, dim3((unsigned)blocks), dim3(kTVBlock), 0, s, a);
     hipLaunchKernelGGL(tv_finish_kernel, dim3(1), dim3(kTVBlock), 0, s, a.partial, blocks, scale, loss);
     return check_launch("atdf_tv_loss");
+}
+
+size_t atdf_fourier_ring_workspace_bytes(int32_t X, int32_t Z)
+{
+    if (X < 2 || Z < 2) return 0;
+    return sizeof(double) * (size_t)(X / 2) * (size_t)(X + Z) + sizeof(int) * 4 * (size_t)(X / 2);
+}
+
+atdf_status atdf_fourier_ring_means(const float* img, int64_t Y, int32_t X, int32_t Z, float* out, void* workspace,
+                                    size_t workspace_bytes, atdf_stream_t stream)
+{
+    ATDF_REQUIRE(Y >= 1 && X >= 2 && Z >= 2, "atdf_fourier_ring_means: image [%lld, %d, %d] too small", (long long)Y,
+                 X, Z);
+    ATDF_REQUIRE(img && out, "atdf_fourier_ring_means: null buffer");
+    const int rings = X / 2;
+    float step;
+    int m = 0;
+    if (rings > 1) ring_angles(rings - 1, step, m);
+    ATDF_REQUIRE(m <= kRingMaxAngles, "atdf_fourier_ring_means: %d angles on the outer ring, at most %d (X <= %d)", m,
+                 kRingMaxAngles, 2 * (int)(kRingMaxAngles / 6.2832));
+    ATDF_REQUIRE((int64_t)rings * Y < (int64_t)INT32_MAX, "atdf_fourier_ring_means: too many slices");
+    const size_t need = atdf_fourier_ring_workspace_bytes(X, Z);
+    ATDF_REQUIRE(workspace && workspace_bytes >= need, "atdf_fourier_ring_means: workspace of %zu bytes, %zu needed",
+                 workspace_bytes, need);
+    RingGeom gm{X, Z, rings, Y};
+    double* A = static_cast<double*>(workspace);
+    double* B = A + (size_t)rings * X;
+    int* bounds = reinterpret_cast<int*>(B + (size_t)rings * Z);
+    hipStream_t s = to_hip(stream);
+    hipLaunchKernelGGL(ring_weights_kernel, dim3((unsigned)(2 * rings)), dim3(kRingBlock), 0, s, gm, A, B, bounds);
+    hipLaunchKernelGGL(ring_means_kernel, dim3((unsigned)((int64_t)rings * Y)), dim3(kRingBlock), 0, s, gm, img, A, B,
+                       bounds, out);
+    return check_launch("atdf_fourier_ring_means");
 }
 
 }  // extern "C"

@@ -234,6 +234,27 @@ PYBIND11_MODULE(TORCH_EXTENSION_NAME, m)
             return TVLoss(grid, weight ? *weight : torch::Tensor(), channels_last);
         },
         py::arg("grid"), py::arg("weight") = py::none(), py::arg("channels_last") = false);
+    py::class_<FourierParams>(m, "FourierParams")
+        .def(py::init<>())
+        .def_readwrite("loss_fourier", &FourierParams::loss_fourier)
+        .def_readwrite("scale", &FourierParams::scale)
+        .def_readwrite("scale_y", &FourierParams::scale_y)
+        .def_readwrite("shift", &FourierParams::shift)
+        .def_readwrite("rate", &FourierParams::rate)
+        .def_readwrite("rate_w", &FourierParams::rate_w);
+    py::class_<FourierGrid>(m, "FourierGrid")
+        .def(py::init<>())
+        .def_readwrite("coords", &FourierGrid::coords)
+        .def_readwrite("node_id", &FourierGrid::node_id)
+        .def_readwrite("X", &FourierGrid::X)
+        .def_readwrite("Y", &FourierGrid::Y)
+        .def_readwrite("Z", &FourierGrid::Z);
+    m.def("MakeFourierGrid", &MakeFourierGrid, py::arg("tree"), py::arg("grid_size"), py::arg("tree_depth"),
+          py::arg("roi_min"), py::arg("roi_max"), py::arg("params"));
+    m.def("DecodeFeatures", &DecodeFeatures);
+    m.def("SampleVolumeIndirect", &SampleVolumeIndirect);
+    m.def("FourierRingMeans", &FourierRingMeans);
+    m.def("FourierLoss", &FourierLoss);
     py::class_<RegularizerParams>(m, "RegularizerParams")
         .def(py::init<>())
         .def_readwrite("loss_tv", &RegularizerParams::loss_tv)
@@ -246,12 +267,18 @@ PYBIND11_MODULE(TORCH_EXTENSION_NAME, m)
         .def_readwrite("test_para", &RegularizerParams::test_para)
         .def_readwrite("tv_size", &RegularizerParams::tv_size)
         .def_readwrite("loss_edge", &RegularizerParams::loss_edge)
-        .def_readwrite("use_ground_truth_volume", &RegularizerParams::use_ground_truth_volume);
+        .def_readwrite("use_ground_truth_volume", &RegularizerParams::use_ground_truth_volume)
+        .def_readwrite("fourier", &RegularizerParams::fourier);
     m.def(
         "VolumeRegularizer",
         [](HyperTreeBase& tree, const RoiMaps& roi, const VMFields& f, const RegularizerParams& p,
-           std::optional<SampleList> pairs) { return VolumeRegularizer(tree, roi, f, p, pairs ? &*pairs : nullptr); },
-        py::arg("tree"), py::arg("roi"), py::arg("fields"), py::arg("params"), py::arg("neighbor_pairs") = py::none());
+           std::optional<SampleList> pairs, std::optional<FourierGrid> fourier, std::optional<torch::Tensor> mlp,
+           std::optional<FieldParams> field) {
+            return VolumeRegularizer(tree, roi, f, p, pairs ? &*pairs : nullptr, fourier ? &*fourier : nullptr,
+                                     mlp ? *mlp : torch::Tensor(), field ? &*field : nullptr);
+        },
+        py::arg("tree"), py::arg("roi"), py::arg("fields"), py::arg("params"), py::arg("neighbor_pairs") = py::none(),
+        py::arg("fourier") = py::none(), py::arg("mlp") = py::none(), py::arg("field") = py::none());
     m.def("EdgeLoss", &EdgeLoss);
     m.def("InitNodeGrid", &InitNodeGrid, py::arg("num_grids"), py::arg("C"), py::arg("G"), py::arg("init"),
           py::arg("scale"), py::arg("seed") = 0, py::arg("channels_last") = true, py::arg("device") = "cpu");

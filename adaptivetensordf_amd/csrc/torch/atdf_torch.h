@@ -224,6 +224,43 @@ torch::Tensor InitTensorCoefficients(std::vector<int64_t> shape, const std::stri
 // spatial dims; channels_last: [B, n..., C]) with an optional per-batch weight,
 // as an autograd Function (loss and gradient from one atdf_tv_loss launch).
 torch::Tensor TVLoss(torch::Tensor grid, torch::Tensor weight = {}, bool channels_last = false);
+struct FieldParams;
+// The Fourier regulariser's switches (TrainParams, Settings.h:473-478,
+// 600-605).  The term is on when loss_fourier > 0.
+struct FourierParams {
+    float loss_fourier = -0.0001f;
+    float scale = 1.0f;    // loss_fourier_scale
+    float scale_y = 1.0f;  // loss_fourier_scale_y
+    float shift = 0.05f;   // loss_fourier_shift
+    float rate = 0.2f;     // loss_fourier_rate: the sigmoid factor's knee, in rings
+    float rate_w = 0.01f;  // loss_fourier_rate_w: half width of the central bands
+};
+// The Fourier term's lattice (geometry_tensor_qo.cpp:1101-1145): X x Y x Z
+// points over the ROI (y inset by `shift`; X and Z made odd), x slowest, and
+// their NodeIdForPositionGPU nodes.
+struct FourierGrid {
+    torch::Tensor coords;   // [X * Y * Z, 3]
+    torch::Tensor node_id;  // [X * Y * Z] i64
+    int64_t X = 0, Y = 0, Z = 0;
+};
+FourierGrid MakeFourierGrid(HyperTreeBase& tree, int grid_size, int tree_depth, std::vector<float> roi_min,
+                            std::vector<float> roi_max, const FourierParams& fp);
+// DecodeFeatures (geometry/geometry.cpp:176-253, softplus last activation) on
+// features [S, C] -> [S, 1], autograd over atdf_decode_features(_bwd).
+torch::Tensor DecodeFeatures(torch::Tensor features, torch::Tensor mlp, const FieldParams& fp);
+// GeometryTensorQO::SampleVolumeIndirect (geometry_tensor_qo.cpp:381-395):
+// DecodeFeatures(SampleVolumeIndirectFeature(...)).
+torch::Tensor SampleVolumeIndirect(HyperTreeBase& tree, const RoiMaps& roi, const VMFields& fields, torch::Tensor mlp,
+                                   const FieldParams& fp, torch::Tensor global_coordinate, torch::Tensor node_id);
+// Ring means [X / 2, Y] of img [Y, X, Z] (atdf_fourier_ring_means; row 0 +inf).
+torch::Tensor FourierRingMeans(torch::Tensor img);
+// The Fourier term on densities [X, Y, Z] (geometry_tensor_qo.cpp:758-850,
+// the masked branch): per y slice the shifted, forward-normalised 2-D FFT
+// magnitude over (x, z); its central band means along x and z against the
+// ring profile (detached), where above it, times a sigmoid factor in the
+// radius; loss_fourier * (mean_x + mean_y).  FFT through torch::fft
+// (rocFFT); ring means on the device kernel.  Requires X == Z.
+torch::Tensor FourierLoss(torch::Tensor densities, const FourierParams& fp);
 // The regulariser switches of TrainParams (Settings.h:429-607).
 struct RegularizerParams {
     double loss_tv = 0.0005;
@@ -235,6 +272,7 @@ struct RegularizerParams {
     std::vector<int64_t> tv_size{16, 16, 16};  // lattice of the 3-D TV: vec_grid_size * loss_tv_scale
     double loss_edge = 0.002;
     bool use_ground_truth_volume = false;  // drops the out-of-ROI edge term
+    FourierParams fourier;
 };
 // The edge term of VolumeRegularizer (geometry_tensor_qo.cpp:630-667): for
 // neighbour pairs (NodeNeighborSamples) the features of both samples, split
@@ -245,10 +283,13 @@ torch::Tensor EdgeLoss(HyperTreeBase& tree, const RoiMaps& roi, const VMFields& 
 // GeometryTensorQO::VolumeRegularizer (geometry_tensor_qo.cpp:527-622 and
 // 913-922): the TV terms (3-D TV of the features on a lattice of each in-ROI
 // node, or TV of the planes, lines and explicit grids), the L2 terms and,
-// given neighbour pairs, the edge term.  The Fourier term is not built.
+// given neighbour pairs, the edge term, and given a FourierGrid with the
+// decoder, the Fourier term on the decoded densities of its lattice.
 // Undefined when no term is on.
 torch::Tensor VolumeRegularizer(HyperTreeBase& tree, const RoiMaps& roi, const VMFields& fields,
-                                const RegularizerParams& p, const SampleList* neighbor_pairs = nullptr);
+                                const RegularizerParams& p, const SampleList* neighbor_pairs = nullptr,
+                                const FourierGrid* fourier = nullptr, torch::Tensor mlp = {},
+                                const FieldParams* field = nullptr);
 
 // NeuralGeometry::IntegrateSamplesXRay (geometry.cpp:977-1019)
 torch::Tensor IntegrateSamplesXRay(torch::Tensor sample_values, torch::Tensor integration_weight,

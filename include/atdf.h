@@ -418,6 +418,21 @@ atdf_status atdf_sample_volume(const void* packed_tree, int64_t num_nodes, const
                                const atdf_field* field, const atdf_lattice* lattice, float* density,
                                int64_t* node_id, float* valid, float* position, atdf_stream_t stream);
 
+/* DecodeFeatures on given feature rows: density[i] = softplus(MLP(x_i)) for
+ * features [n, C] f32 (HierarchicalNeuralGeometry::DecodeFeatures,
+ * geometry/geometry.cpp:176-253, with the softplus last activation; the
+ * FCBlock of geometry/Modules.h:23-145).  Reads field->C, H, softplus_beta,
+ * mlp and decoder_dtype only (the grid may be null).  The adjoint WRITES
+ * grad_features [n, C] and ADDS dL/dmlp to grad_mlp; the decoder-parameter
+ * gradient goes through per-wave partials (workspace) and a fixed-order
+ * reduction, so both directions are deterministic. */
+atdf_status atdf_decode_features(const float* features, int64_t n, const atdf_field* field, float* density,
+                                 atdf_stream_t stream);
+size_t atdf_decode_features_bwd_workspace_bytes(int64_t n, const atdf_field* field);
+atdf_status atdf_decode_features_bwd(const float* features, int64_t n, const atdf_field* field,
+                                     const float* grad_density, float* grad_features, float* grad_mlp,
+                                     void* workspace, size_t workspace_bytes, atdf_stream_t stream);
+
 /* Pairs of points across the faces between active nodes, for the edge
  * regulariser: HyperTreeBaseImpl::NodeNeighborSamples (structure/
  * HyperTree.cu:1720-1811, 2010-2075).  For every active node, face (side s
@@ -704,6 +719,21 @@ typedef struct {
 size_t atdf_tv_workspace_bytes(int64_t num_elements);
 atdf_status atdf_tv_loss(const float* x, const atdf_tv_shape* shape, const float* weight, double scale, double* loss,
                          float* grad, void* workspace, size_t workspace_bytes, atdf_stream_t stream);
+
+/* Ring means of the Fourier regulariser (GeometryTensorQO::VolumeRegularizer,
+ * geometry/geometry_tensor_qo.cpp:784-808) on img [Y, X, Z] f32 (the
+ * shifted FFT magnitudes of the density slices): out [X / 2, Y], row i the
+ * ring of radius i around (X / 2, Z / 2) -- n = (int)(2 pi i) angles of step
+ * s = (float)(360 / n), m = ceil(360 / s) of them (torch::arange), and the
+ * mean over all m^2 (row, column) pairs of torch::meshgrid of the row list
+ * i cos(th) + X / 2 and the column list i sin(th) + Z / 2 of the bilinear
+ * value (grid_sample: align_corners, border; both coordinates normalised by
+ * X - 1).  Row 0, which the reference leaves uninitialised (torch::empty),
+ * is +inf.  FP64 sums, deterministic.  Workspace: the row and column
+ * weight tables of every ring and their non-zero ranges. */
+size_t atdf_fourier_ring_workspace_bytes(int32_t X, int32_t Z);
+atdf_status atdf_fourier_ring_means(const float* img, int64_t Y, int32_t X, int32_t Z, float* out, void* workspace,
+                                    size_t workspace_bytes, atdf_stream_t stream);
 
 /* [N, C, G, G, G] <-> [N, G, G, G, C] (direction 0: to channels-last). */
 atdf_status atdf_grid_layout(const float* src, float* dst, int64_t N, int32_t C, int32_t G, int32_t direction,

@@ -1122,3 +1122,67 @@ void atdfo_field_density(const atdfo_model* m, const float* pos, const int64_t* 
         free(featd);
     }
 }
+
+/* ------------------------------------------------------------------------ */
+/* DecodeFeatures on feature rows and its adjoint (geometry.cpp:176-253,     */
+/* the FCBlock of Modules.h:23-145 with the softplus last activation)        */
+/* ------------------------------------------------------------------------ */
+/* FP64; bf = 1 rounds the MLP inputs and the first two layers' weights to
+ * BF16 as the device's BF16 decoder does.  density, grad_x nullable; dd null
+ * skips the adjoint; grad_mlp is ADDED to. */
+void atdfo_decode_features(const float* params, int C, int H, double beta, const float* x, int64_t S, int bf,
+                           const double* dd, double* density, double* grad_x, double* grad_mlp)
+{
+    double* xd = (double*)malloc(sizeof(double) * C);
+    double* h1 = (double*)malloc(sizeof(double) * H);
+    double* h2 = (double*)malloc(sizeof(double) * H);
+    double* dh2 = (double*)malloc(sizeof(double) * H);
+    double* dh1 = (double*)malloc(sizeof(double) * H);
+    const float* W1 = params;
+    const float* W2 = W1 + H * C + H;
+    const float* W3 = W2 + H * H + H;
+    for (int64_t s = 0; s < S; ++s) {
+        double yp;
+        for (int c = 0; c < C; ++c) xd[c] = x[s * C + c];
+        double dn = mlp_fwd_one_d(params, C, H, beta, xd, h1, h2, &yp, bf);
+        if (density) density[s] = dn;
+        if (!dd) continue;
+        double dy = dd[s] * softplus_grad_d(yp, beta);
+        double* gW1 = grad_mlp;
+        double* gb1 = gW1 + H * C;
+        double* gW2 = gb1 + H;
+        double* gb2 = gW2 + H * H;
+        double* gW3 = gb2 + H;
+        double* gb3 = gW3 + H;
+        for (int i = 0; i < H; ++i) {
+            if (gW3) gW3[i] += dy * h2[i];
+            dh2[i] = (h2[i] > 0) ? dy * (double)W3[i] : 0;
+            dh1[i] = 0;
+        }
+        if (gb3) gb3[0] += dy;
+        for (int i = 0; i < H; ++i) {
+            if (dh2[i] == 0) continue;
+            if (grad_mlp) gb2[i] += dh2[i];
+            for (int k = 0; k < H; ++k) {
+                if (grad_mlp) gW2[i * H + k] += rb(bf, dh2[i]) * rb(bf, h1[k]);
+                dh1[k] += rb(bf, dh2[i]) * rb(bf, W2[i * H + k]);
+            }
+        }
+        double* gx = grad_x ? grad_x + s * C : NULL;
+        if (gx)
+            for (int c = 0; c < C; ++c) gx[c] = 0;
+        for (int i = 0; i < H; ++i) {
+            if (!(h1[i] > 0)) continue;
+            if (grad_mlp) gb1[i] += dh1[i];
+            for (int k = 0; k < C; ++k) {
+                if (grad_mlp) gW1[i * C + k] += rb(bf, dh1[i]) * rb(bf, xd[k]);
+                if (gx) gx[k] += rb(bf, dh1[i]) * rb(bf, W1[i * C + k]);
+            }
+        }
+    }
+    free(xd);
+    free(h1);
+    free(h2);
+    free(dh2);
+    free(dh1);
+}

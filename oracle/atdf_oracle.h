@@ -302,6 +302,9 @@ int64_t atdfo_neighbor_pairs(const float* node_min, const float* node_max, const
                              int64_t* node_out, int64_t cap);
 /* Density of the fused field at points pos [n, 3] located in node[n]
  * (< 0: 0), FP32 path (precision 0) or FP64 path (precision 1). */
+void atdfo_decode_features(const float* params, int C, int H, double beta, const float* x, int64_t S, int bf,
+                           const double* dd, double* density, double* grad_x, double* grad_mlp);
+void atdfo_fourier_ring_means(const float* img, int64_t Y, int X, int Z, float* out);
 void atdfo_field_density(const atdfo_model* m, const float* pos, const int64_t* node, int64_t n, int precision,
                          double* density);
 /* Random batch draws (image entry, uv) of the batch kernel. */

@@ -16,6 +16,7 @@
  */
 #include <math.h>
 #include <stdint.h>
+#include <stdlib.h>
 
 #include "atdf_oracle.h"
 
@@ -191,4 +192,70 @@ double atdfo_tv_loss(const float* x, int64_t B, int64_t C, int D, const int* n, 
                 if (grad) grad[addr] = grad[addr] + g * wb;
             }
     return acc * scale;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Ring means of the Fourier regulariser (geometry_tensor_qo.cpp:784-808),   */
+/* pointwise: every one of the m^2 meshgrid pairs is interpolated with the   */
+/* float corner weights of grid_sample (bilinear, border, align_corners)     */
+/* and the values summed in double.  img [Y, X, Z]; out [X / 2, Y], row 0    */
+/* +inf (left uninitialised by the reference).                               */
+/* ------------------------------------------------------------------------ */
+static float ring_axis_coord(int k, float step, int i, int centre, int use_sin, int X, int size)
+{
+    /* torch on the GPU: th = k * step; th / 180 as th * (1 / 180); * (float)pi;
+     * cos / sin; * i; + centre; / (X - 1) as * (1 / (X - 1)); * 2; - 1; then
+     * grid_sample's ((g + 1) / 2) * (size - 1) and the border clamp */
+    float th = (float)k * step;
+    float a = (th * (1.0f / 180.0f)) * 3.14159265358979323846f;
+    float cs = use_sin ? sinf(a) : cosf(a);
+    float p = (float)i * cs + (float)centre;
+    float gn = (p * (1.0f / (float)(X - 1))) * 2.0f - 1.0f;
+    float u = ((gn + 1.0f) / 2.0f) * (float)(size - 1);
+    if (u < 0.0f) u = 0.0f;
+    if (u > (float)(size - 1)) u = (float)(size - 1);
+    return u;
+}
+
+void atdfo_fourier_ring_means(const float* img, int64_t Y, int X, int Z, float* out)
+{
+    const int rings = X / 2;
+    for (int64_t y = 0; y < Y; ++y) out[y] = INFINITY;
+    for (int i = 1; i < rings; ++i) {
+        const int n = (int)(2.0 * 3.14159265358979323846 * (double)i);
+        const float step = (float)(360.0 / (double)n);
+        const int m = (int)ceil(360.0 / (double)step);
+        float* rr = (float*)malloc(sizeof(float) * m);
+        float* cc = (float*)malloc(sizeof(float) * m);
+        for (int k = 0; k < m; ++k) {
+            rr[k] = ring_axis_coord(k, step, i, X / 2, 0, X, X);
+            cc[k] = ring_axis_coord(k, step, i, Z / 2, 1, X, Z);
+        }
+        for (int64_t y = 0; y < Y; ++y) {
+            const float* im = img + y * (int64_t)X * Z;
+            double acc = 0.0;
+            for (int a = 0; a < m; ++a) {
+                const float iy = rr[a];
+                const float iy_n = floorf(iy), iy_s = iy_n + 1.0f;
+                for (int b = 0; b < m; ++b) {
+                    const float ix = cc[b];
+                    const float ix_w = floorf(ix), ix_e = ix_w + 1.0f;
+                    const float nw = (ix_e - ix) * (iy_s - iy);
+                    const float ne = (ix - ix_w) * (iy_s - iy);
+                    const float sw = (ix_e - ix) * (iy - iy_n);
+                    const float se = (ix - ix_w) * (iy - iy_n);
+                    const int r0 = (int)iy_n, c0 = (int)ix_w;
+                    float v = 0.0f;
+                    if (r0 < X && c0 < Z) v += im[(int64_t)r0 * Z + c0] * nw;
+                    if (r0 < X && c0 + 1 < Z) v += im[(int64_t)r0 * Z + c0 + 1] * ne;
+                    if (r0 + 1 < X && c0 < Z) v += im[(int64_t)(r0 + 1) * Z + c0] * sw;
+                    if (r0 + 1 < X && c0 + 1 < Z) v += im[(int64_t)(r0 + 1) * Z + c0 + 1] * se;
+                    acc += v;
+                }
+            }
+            out[(int64_t)i * Y + y] = (float)(acc / ((double)m * (double)m));
+        }
+        free(rr);
+        free(cc);
+    }
 }

@@ -623,3 +623,58 @@ def field_density(fm, pos, node, precision=1):
     out = np.zeros(pos.shape[0], np.float64)
     lib().atdfo_field_density(C.byref(fm.m), _p(pos), _p(node), C.c_int64(pos.shape[0]), C.c_int(precision), _p(out))
     return out
+
+
+def decode_features(params, Cc, H, beta, x, dd=None, bf=False):
+    """DecodeFeatures in FP64: (density [S], grad_x [S, C], grad_mlp [NP]);
+    the gradients are None without dd (dL/ddensity [S])."""
+    x = _c(x, np.float32).reshape(-1, Cc)
+    S = x.shape[0]
+    dens = np.zeros(S, np.float64)
+    gx = gm = None
+    if dd is not None:
+        dd = _c(dd, np.float64).reshape(S)
+        gx = np.zeros((S, Cc), np.float64)
+        gm = np.zeros(H * Cc + H + H * H + H + H + 1, np.float64)
+    lib().atdfo_decode_features(_p(_c(params, np.float32)), C.c_int(Cc), C.c_int(H), C.c_double(beta), _p(x),
+                                C.c_int64(S), C.c_int(int(bf)), _p(dd), _p(dens), _p(gx), _p(gm))
+    return dens, gx, gm
+
+
+def fourier_ring_means(img):
+    """Ring means of the Fourier regulariser on img [Y, X, Z] f32: [X // 2, Y]
+    (row 0 +inf), every meshgrid pair interpolated pointwise."""
+    img = _c(img, np.float32)
+    Y, X, Z = img.shape
+    out = np.zeros((X // 2, Y), np.float32)
+    lib().atdfo_fourier_ring_means(_p(img), C.c_int64(Y), C.c_int(X), C.c_int(Z), _p(out))
+    return out
+
+
+def fourier_terms(densities, rate=0.2, rate_w=0.01, ring=None):
+    """The Fourier regulariser of GeometryTensorQO::VolumeRegularizer
+    (geometry_tensor_qo.cpp:758-850, the masked branch) on densities
+    [X, Y, Z], unscaled: (fx_term + fy_term, parts).  FP64 FFT (numpy), the
+    band and ring values as the reference forms them.  `ring` [X // 2, Y]
+    replaces the ring means (the reference detaches them: a fixed ring gives
+    the function whose derivative its autograd takes)."""
+    d = np.transpose(np.asarray(densities, np.float64), (1, 0, 2))
+    Y, X, Z = d.shape
+    cx, cy = X // 2, Z // 2
+    rh = max(int(np.float32(rate_w) * np.float32(cy)), 1)
+    rw = max(int(np.float32(rate_w) * np.float32(cx)), 1)
+    F = np.abs(np.fft.fftshift(np.fft.fft2(d, axes=(-2, -1), norm="forward"), axes=(-2, -1)))
+    fx = F[:, cy - rh + 1:cy + rh, :].mean(1)
+    fy = F[:, :, cx - rw + 1:cx + rw].mean(2)
+    total = X // 2
+    if ring is None:
+        ring = fourier_ring_means(F.astype(np.float32))
+    ring = np.asarray(ring, np.float64)
+    circle = np.concatenate([ring[::-1], np.zeros((1, Y)), ring], 0).T
+    xs = np.arange(total, dtype=np.float32)
+    yv = np.float32(1) - np.float32(1) / (np.exp(xs - np.float32(np.float32(rate) * np.float32(total))) + np.float32(1))
+    factor = np.broadcast_to(np.concatenate([yv[::-1], [0.0], yv]).astype(np.float64)[None, :], circle.shape)
+    mx, my = fx > circle, fy > circle
+    tx = ((fx[mx] - circle[mx]) * factor[mx]).mean() if mx.any() else np.nan
+    ty = ((fy[my] - circle[my]) * factor[my]).mean() if my.any() else np.nan
+    return tx + ty, dict(fx=fx, fy=fy, circle=circle, factor=factor, mask_x=mx, mask_y=my, F=F, ring=ring)
