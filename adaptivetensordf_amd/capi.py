@@ -71,7 +71,7 @@ class TVShape(C.Structure):
 
 class NeighborSpec(C.Structure):
     _fields_ = [("size", C.c_int32 * 3), ("epsilon", C.c_float), ("use_roi", C.c_int32), ("roi_min", C.c_float * 3),
-                ("roi_max", C.c_float * 3)]
+                ("roi_max", C.c_float * 3), ("variant", C.c_int32)]
 
 
 class Intervals(C.Structure):

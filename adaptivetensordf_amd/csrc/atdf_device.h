@@ -208,7 +208,11 @@ __device__ __forceinline__ float bf16_to_f32(uint16_t h)
 // Active node holding p, or -1: from the root, step into the first child (in
 // child order) whose closed box holds p (ActiveNodeIdForGlobalPosition,
 // structure/HyperTree.cu:1339-1399).
-__device__ __forceinline__ int32_t active_node_for_position(const PNode* __restrict__ nodes, const float* p)
+// nchild = 4 descends only into the first four children of every node, as
+// ActiveNodeIdForGlobalPosition with use_quad (structure/HyperTree.cu:
+// 1340-1400) does on any tree.
+__device__ __forceinline__ int32_t active_node_for_position(const PNode* __restrict__ nodes, const float* p,
+                                                            int nchild = 8)
 {
     int32_t node = 0;
     while (true) {
@@ -216,7 +220,7 @@ __device__ __forceinline__ int32_t active_node_for_position(const PNode* __restr
         if (nd.flags & kFlagActive) return node;
         if (nd.first_child < 0) return -1;
         int32_t next = -1;
-        for (int c = 0; c < 8; ++c) {
+        for (int c = 0; c < nchild; ++c) {
             const PNode ch = nodes[nd.first_child + c];
             if (ch.mn[0] <= p[0] && ch.mx[0] >= p[0] && ch.mn[1] <= p[1] && ch.mx[1] >= p[1] && ch.mn[2] <= p[2] &&
                 ch.mx[2] >= p[2]) {

@@ -131,17 +131,20 @@ struct NeighborArgs {
     float eps;
     int use_roi;
     float roi_min[3], roi_max[3];
+    int quad;  // NodeNeighborSamplesQuad: sides x and z, in_roi_quad, four-child descent
 };
 
-// in_roi (structure/HyperTree.cu:1667-1690), including its mid = min + max
-__device__ __forceinline__ bool node_in_roi(const PNode& n, const float* lo, const float* hi)
+// in_roi (structure/HyperTree.cu:1667-1690), including its mid = min + max;
+// in_roi_quad (HyperTree.cu:1692-1718) widens the y range by widen_y
+__device__ __forceinline__ bool node_in_roi(const PNode& n, const float* lo_, const float* hi_, float widen_y)
 {
     bool in = true;
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
+        const float lo = k == 1 ? lo_[k] - widen_y : lo_[k];
+        const float hi = k == 1 ? hi_[k] + widen_y : hi_[k];
         const float mid = n.mn[k] + n.mx[k];
-        in = in && ((n.mn[k] > lo[k] && n.mn[k] < hi[k]) || (n.mx[k] > lo[k] && n.mx[k] < hi[k]) ||
-                    (mid > lo[k] && mid < hi[k]));
+        in = in && ((n.mn[k] > lo && n.mn[k] < hi) || (n.mx[k] > lo && n.mx[k] < hi) || (mid > lo && mid < hi));
     }
     return in;
 }
@@ -150,7 +153,7 @@ __device__ __forceinline__ bool node_in_roi(const PNode& n, const float* lo, con
 __device__ int neighbor_candidate(const NeighborArgs& a, int64_t act, int c, int* node_out, float* g)
 {
     int side = 0, front = 0, sid = c;
-    for (side = 0; side < 3; ++side) {
+    for (side = 0; side < 3; side += a.quad ? 2 : 1) {
         const int sa = (side + 1) % 3, sb = (side + 2) % 3;
         const int n = a.size[sa] * a.size[sb];
         if (sid < 2 * n) {
@@ -175,10 +178,12 @@ __device__ int neighbor_candidate(const NeighborArgs& a, int64_t act, int c, int
         q[k] = g[k];
     }
     q[side] = g[side] + (front ? a.eps : -a.eps);
-    const int other = active_node_for_position(a.nodes, q);
+    const int other = active_node_for_position(a.nodes, q, a.quad ? 4 : 8);
     *node_out = node;
     if (other < 0 || other == node) return -1;
-    if (a.use_roi && !(node_in_roi(nd, a.roi_min, a.roi_max) && node_in_roi(a.nodes[other], a.roi_min, a.roi_max)))
+    const float widen = a.quad ? a.eps / 2.f : 0.f;
+    if (a.use_roi &&
+        !(node_in_roi(nd, a.roi_min, a.roi_max, widen) && node_in_roi(a.nodes[other], a.roi_min, a.roi_max, widen)))
         return -1;
     return other;
 }
@@ -278,7 +283,10 @@ static atdf_status make_neighbor_args(const void* packed_tree, int64_t num_nodes
         a->roi_min[k] = spec->roi_min[k];
         a->roi_max[k] = spec->roi_max[k];
     }
-    a->per_node = 2 * (spec->size[1] * spec->size[2] + spec->size[2] * spec->size[0] + spec->size[0] * spec->size[1]);
+    ATDF_REQUIRE(spec->variant == 0 || spec->variant == 1, "%s: variant %d (0 NodeNeighborSamples, 1 "
+                 "NodeNeighborSamplesQuad)", who, spec->variant);
+    a->quad = spec->variant == 1;
+    a->per_node = (int)atdf_neighbor_candidates(1, spec);
     a->eps = spec->epsilon;
     a->use_roi = spec->use_roi != 0;
     ATDF_REQUIRE(num_active * a->per_node < (int64_t)INT32_MAX, "%s: too many candidates", who);
@@ -288,9 +296,9 @@ static atdf_status make_neighbor_args(const void* packed_tree, int64_t num_nodes
 int64_t atdf_neighbor_candidates(int64_t num_active, const atdf_neighbor_spec* spec)
 {
     if (!spec || num_active <= 0) return 0;
-    return num_active * 2 *
-           ((int64_t)spec->size[1] * spec->size[2] + (int64_t)spec->size[2] * spec->size[0] +
-            (int64_t)spec->size[0] * spec->size[1]);
+    const int64_t x = (int64_t)spec->size[1] * spec->size[2], y = (int64_t)spec->size[2] * spec->size[0],
+                  z = (int64_t)spec->size[0] * spec->size[1];
+    return num_active * 2 * (spec->variant == 1 ? x + z : x + y + z);
 }
 
 atdf_status atdf_neighbor_flags(const void* packed_tree, int64_t num_nodes, const int64_t* active_ids,

@@ -80,6 +80,9 @@ PYBIND11_MODULE(TORCH_EXTENSION_NAME, m)
         .def("NodeNeighborSamples", &HyperTreeBase::NodeNeighborSamples, py::arg("size"), py::arg("epsilon"),
              py::arg("roi_min") = std::vector<float>{}, py::arg("roi_max") = std::vector<float>{},
              py::arg("use_roi") = false)
+        .def("NodeNeighborSamplesPlane2D", &HyperTreeBase::NodeNeighborSamplesPlane2D, py::arg("size"),
+             py::arg("epsilon"), py::arg("plane_index") = 0, py::arg("roi_min") = std::vector<float>{},
+             py::arg("roi_max") = std::vector<float>{}, py::arg("use_roi") = false, py::arg("use_quad") = false)
         .def("UniformPhantomSamplesGPU", &HyperTreeBase::UniformPhantomSamplesGPU)
         .def("UniformPhantomSamplesGPUbySlice", &HyperTreeBase::UniformPhantomSamplesGPUbySlice)
         .def("InterpolateGridForInactiveNodes", &HyperTreeBase::InterpolateGridForInactiveNodes)

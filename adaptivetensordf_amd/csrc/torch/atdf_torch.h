@@ -111,6 +111,16 @@ class HyperTreeBase {
     // candidate order; weight 1, local_index_in_ray 0 / 1
     SampleList NodeNeighborSamples(std::vector<int> size, double epsilon, std::vector<float> roi_min,
                                    std::vector<float> roi_max, bool use_roi) const;
+    // NodeNeighborSamplesPlane2D (HyperTree.cu:2076-2160), the pairs the
+    // tensorial geometry's edge term uses (geometry_tensor_qo.cpp:1158-1160):
+    // with use_quad = false the NodeNeighborSamplesQuad kernel -- faces normal
+    // to x and z only, in_roi_quad (y range widened by epsilon / 2), point
+    // location over the first four children of each node, as the reference
+    // does on any tree; plane_index is unused.  use_quad = true (the
+    // quad-tree representation's NodeNeighborSamples2D) is rejected.
+    SampleList NodeNeighborSamplesPlane2D(std::vector<int> size, double epsilon, int plane_index,
+                                          std::vector<float> roi_min, std::vector<float> roi_max, bool use_roi,
+                                          bool use_quad) const;
     // UniformGlobalSamples (HyperTree.cu:1618-1665): [N, n, n, n, 3] lattice
     // positions of the listed nodes (x fastest)
     torch::Tensor UniformGlobalSamples(torch::Tensor node_id, int n) const;
@@ -143,6 +153,8 @@ class HyperTreeBase {
 
    private:
     void repack();
+    SampleList neighbor_samples(std::vector<int> size, double epsilon, std::vector<float> roi_min,
+                                std::vector<float> roi_max, bool use_roi, int variant) const;
     torch::Tensor packed_;
     torch::Tensor leaf_table_;
 };

@@ -356,6 +356,22 @@ std::pair<torch::Tensor, torch::Tensor> HyperTreeBase::NodeIdForPositionGPU(torc
 SampleList HyperTreeBase::NodeNeighborSamples(std::vector<int> size, double epsilon, std::vector<float> roi_min,
                                               std::vector<float> roi_max, bool use_roi) const
 {
+    return neighbor_samples(size, epsilon, roi_min, roi_max, use_roi, 0);
+}
+
+SampleList HyperTreeBase::NodeNeighborSamplesPlane2D(std::vector<int> size, double epsilon, int plane_index,
+                                                     std::vector<float> roi_min, std::vector<float> roi_max,
+                                                     bool use_roi, bool use_quad) const
+{
+    (void)plane_index;  // unused by the reference's kernels as well
+    TORCH_CHECK(!use_quad, "NodeNeighborSamplesPlane2D: use_quad selects NodeNeighborSamples2D, the pairs of the "
+                "quad-tree representation, which is not built (the octree's pairs come from use_quad = false)");
+    return neighbor_samples(size, epsilon, roi_min, roi_max, use_roi, 1);
+}
+
+SampleList HyperTreeBase::neighbor_samples(std::vector<int> size, double epsilon, std::vector<float> roi_min,
+                                           std::vector<float> roi_max, bool use_roi, int variant) const
+{
     TORCH_CHECK(node_position_min.is_cuda(), "NodeNeighborSamples: the tree must be on the GPU");
     TORCH_CHECK(size.size() == 3, "NodeNeighborSamples: size needs 3 entries");
     TORCH_CHECK(!use_roi || (roi_min.size() == 3 && roi_max.size() == 3), "NodeNeighborSamples: ROI needs 3 entries");
@@ -367,6 +383,7 @@ SampleList HyperTreeBase::NodeNeighborSamples(std::vector<int> size, double epsi
     }
     spec.epsilon = (float)epsilon;
     spec.use_roi = use_roi ? 1 : 0;
+    spec.variant = variant;
     auto dev = node_position_min.device();
     auto ids = active_node_ids.contiguous();
     const int64_t NA = ids.size(0);

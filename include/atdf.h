@@ -443,17 +443,24 @@ atdf_status atdf_decode_features_bwd(const float* features, int64_t n, const atd
  * is located as atdf_node_id_for_position.  A different active node there
  * (with use_roi: both nodes pass in_roi, HyperTree.cu:1667-1690, whose
  * "mid" is min + max) gives a pair (p in the node, p in the other node).
- * Host struct. */
+ * variant 1 is NodeNeighborSamplesQuad (HyperTree.cu:1915-2008), which the
+ * tensorial geometry's edge term uses through NodeNeighborSamplesPlane2D
+ * (geometry_tensor_qo.cpp:1158-1160): sides 0 and 2 only, in_roi_quad
+ * (HyperTree.cu:1692-1718: the y range widened by epsilon / 2) and point
+ * location over the first four children of each node (the reference's
+ * use_quad descent, applied to whatever tree it is given).  Host struct. */
 typedef struct {
     int32_t size[3];
     float epsilon;
     int32_t use_roi;
     float roi_min[3];
     float roi_max[3];
+    int32_t variant; /* 0 NodeNeighborSamples; 1 NodeNeighborSamplesQuad */
 } atdf_neighbor_spec;
 
-/* Candidates: num_active * 2 * (size1 size2 + size2 size0 + size0 size1),
- * numbered per active node as (side, back / front, lattice index). */
+/* Candidates: num_active * 2 * (size1 size2 + size2 size0 + size0 size1)
+ * (variant 1: without the size2 size0 term of side 1), numbered per active
+ * node as (side, back / front, lattice index). */
 int64_t atdf_neighbor_candidates(int64_t num_active, const atdf_neighbor_spec* spec);
 /* flags[c] = 1 where candidate c is a pair (device int32). */
 atdf_status atdf_neighbor_flags(const void* packed_tree, int64_t num_nodes, const int64_t* active_ids,

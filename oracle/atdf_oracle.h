@@ -298,8 +298,8 @@ double atdfo_tv_loss(const float* x, int64_t B, int64_t C, int D, const int* n, 
  * writes at most cap pairs (pos [2P, 3], node [2P]). */
 int64_t atdfo_neighbor_pairs(const float* node_min, const float* node_max, const int32_t* children,
                              const int32_t* active, const int64_t* active_ids, int64_t num_active, const int* size,
-                             float eps, int use_roi, const float* roi_min, const float* roi_max, float* pos,
-                             int64_t* node_out, int64_t cap);
+                             float eps, int use_roi, const float* roi_min, const float* roi_max, int variant,
+                             float* pos, int64_t* node_out, int64_t cap);
 /* Density of the fused field at points pos [n, 3] located in node[n]
  * (< 0: 0), FP32 path (precision 0) or FP64 path (precision 1). */
 void atdfo_decode_features(const float* params, int C, int H, double beta, const float* x, int64_t S, int bf,

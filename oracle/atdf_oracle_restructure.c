@@ -17,14 +17,17 @@
 
 #include "atdf_oracle.h"
 
-int32_t atdfo_active_node_for_position(const float* node_min, const float* node_max, const int32_t* children,
-                                       const int32_t* active, const float* p)
+/* descent over the first `nchild` children of every node: 8, or 4 as the
+ * reference's ActiveNodeIdForGlobalPosition with use_quad (HyperTree.cu:
+ * 1340-1400) does on any tree */
+static int32_t locate_(const float* node_min, const float* node_max, const int32_t* children, const int32_t* active,
+                       const float* p, int nchild)
 {
     int32_t node = 0;
     for (;;) {
         if (active[node] > 0) return node;
         int32_t next = -1;
-        for (int c = 0; c < 8; ++c) {
+        for (int c = 0; c < nchild; ++c) {
             int32_t ch = children[node * 8 + c];
             if (ch < 0) return -1;
             const float* a = node_min + (int64_t)ch * 3;
@@ -37,6 +40,12 @@ int32_t atdfo_active_node_for_position(const float* node_min, const float* node_
         if (next < 0) return -1;
         node = next;
     }
+}
+
+int32_t atdfo_active_node_for_position(const float* node_min, const float* node_max, const int32_t* children,
+                                       const int32_t* active, const float* p)
+{
+    return locate_(node_min, node_max, children, active, p, 8);
 }
 
 void atdfo_node_id_for_position(const float* node_min, const float* node_max, const int32_t* children,
@@ -154,31 +163,40 @@ void atdfo_node_lattice_points(const float* node_min, const float* node_max, con
     }
 }
 
-/* NodeNeighborSamples (structure/HyperTree.cu:1720-1811): pairs in
- * candidate order (node, side, back / front, lattice index); in_roi of
- * HyperTree.cu:1667-1690 with its mid = min + max.  Writes up to `cap`
- * pairs (pos [2P, 3], node [2P]) and returns the pair count. */
-static int in_roi_(const float* mn, const float* mx, const float* lo, const float* hi)
+/* NodeNeighborSamples (structure/HyperTree.cu:1720-1811, variant 0): pairs
+ * in candidate order (node, side, back / front, lattice index); in_roi of
+ * HyperTree.cu:1667-1690 with its mid = min + max.  Variant 1 is
+ * NodeNeighborSamplesQuad (HyperTree.cu:1915-2008): sides x and z only,
+ * in_roi_quad (HyperTree.cu:1692-1718: the y range widened by eps / 2) and
+ * the four-child descent.  Writes up to `cap` pairs (pos [2P, 3], node [2P])
+ * and returns the pair count. */
+static int in_roi_(const float* mn, const float* mx, const float* lo_, const float* hi_, float widen_y)
 {
     for (int k = 0; k < 3; ++k) {
+        float lo = lo_[k], hi = hi_[k];
+        if (k == 1) {
+            lo = lo - widen_y;
+            hi = hi + widen_y;
+        }
         float mid = mn[k] + mx[k];
-        if (!((mn[k] > lo[k] && mn[k] < hi[k]) || (mx[k] > lo[k] && mx[k] < hi[k]) || (mid > lo[k] && mid < hi[k])))
-            return 0;
+        if (!((mn[k] > lo && mn[k] < hi) || (mx[k] > lo && mx[k] < hi) || (mid > lo && mid < hi))) return 0;
     }
     return 1;
 }
 
 int64_t atdfo_neighbor_pairs(const float* node_min, const float* node_max, const int32_t* children,
                              const int32_t* active, const int64_t* active_ids, int64_t num_active, const int* size,
-                             float eps, int use_roi, const float* roi_min, const float* roi_max, float* pos,
-                             int64_t* node_out, int64_t cap)
+                             float eps, int use_roi, const float* roi_min, const float* roi_max, int variant,
+                             float* pos, int64_t* node_out, int64_t cap)
 {
     int64_t np = 0;
+    const int quad = variant == 1;
+    const float widen = quad ? eps / 2.f : 0.f;
     for (int64_t a = 0; a < num_active; ++a) {
         int64_t node = active_ids[a];
         const float* mn = node_min + node * 3;
         const float* mx = node_max + node * 3;
-        for (int side = 0; side < 3; ++side) {
+        for (int side = 0; side < 3; side += quad ? 2 : 1) {
             int sa = (side + 1) % 3, sb = (side + 2) % 3;
             for (int front = 0; front < 2; ++front) {
                 for (int sid = 0; sid < size[sa] * size[sb]; ++sid) {
@@ -192,11 +210,11 @@ int64_t atdfo_neighbor_pairs(const float* node_min, const float* node_max, const
                         q[k] = g[k];
                     }
                     q[side] = g[side] + (front ? eps : -eps);
-                    int32_t other = atdfo_active_node_for_position(node_min, node_max, children, active, q);
+                    int32_t other = locate_(node_min, node_max, children, active, q, quad ? 4 : 8);
                     if (other < 0 || other == node) continue;
-                    if (use_roi && !(in_roi_(mn, mx, roi_min, roi_max) &&
+                    if (use_roi && !(in_roi_(mn, mx, roi_min, roi_max, widen) &&
                                      in_roi_(node_min + (int64_t)other * 3, node_max + (int64_t)other * 3, roi_min,
-                                             roi_max)))
+                                             roi_max, widen)))
                         continue;
                     if (np < cap) {
                         for (int k = 0; k < 3; ++k) {

@@ -600,13 +600,14 @@ def tv_loss(x, channels_last=False, weight=None, scale=1.0, grad=None):
     return loss, g
 
 
-def neighbor_pairs(node_min, node_max, children, active, size, eps=1e-3, roi=None):
-    """NodeNeighborSamples: (pos [2P, 3], node [2P]) in candidate order."""
+def neighbor_pairs(node_min, node_max, children, active, size, eps=1e-3, roi=None, quad=False):
+    """NodeNeighborSamples (quad: NodeNeighborSamplesQuad): (pos [2P, 3],
+    node [2P]) in candidate order."""
     ids = active_ids(np.asarray(active))
     args = [_p(_c(node_min, np.float32)), _p(_c(node_max, np.float32)), _p(_c(children, np.int32)),
             _p(_c(active, np.int32)), _p(ids), C.c_int64(ids.size), _p(_c(size, np.int32)), C.c_float(eps),
             C.c_int(roi is not None), _p(_c(roi[0] if roi else (0, 0, 0), np.float32)),
-            _p(_c(roi[1] if roi else (0, 0, 0), np.float32))]
+            _p(_c(roi[1] if roi else (0, 0, 0), np.float32)), C.c_int(1 if quad else 0)]
     fn = lib().atdfo_neighbor_pairs
     fn.restype = C.c_int64
     n = fn(*args, None, None, C.c_int64(0))

@@ -271,21 +271,30 @@ def test_volume_regularizer(oracle, native, cuda, tv3d):
 
 
 @pytest.mark.parametrize("use_roi", [False, True])
-def test_node_neighbor_samples_bit_exact(oracle, native, cuda, use_roi):
+@pytest.mark.parametrize("quad", [False, True])
+def test_node_neighbor_samples_bit_exact(oracle, native, cuda, use_roi, quad):
     for depth, budget in ((3, 150), (4, 500)):
         t = oracle.tree_build(depth)
         act = adaptive_active(t, budget=budget, seed=depth)
         tree = device_tree(depth, act)
-        roi = ((-0.6, -0.7, -0.5), (0.55, 0.65, 0.6)) if use_roi else None
-        s = tree.NodeNeighborSamples([4, 3, 5], 1e-3, list(roi[0]) if roi else [], list(roi[1]) if roi else [],
-                                     use_roi)
+        # the quad variant's four-child descent only reaches the z-lowest
+        # nodes of an octree, so its ROI reaches down to z = -1
+        roi = ((-0.6, -0.7, -1.2 if quad else -0.5), (0.55, 0.65, 0.6)) if use_roi else None
+        lo, hi = (list(roi[0]), list(roi[1])) if roi else ([], [])
+        if quad:
+            s = tree.NodeNeighborSamplesPlane2D([4, 3, 5], 1e-3, 0, lo, hi, use_roi, False)
+        else:
+            s = tree.NodeNeighborSamples([4, 3, 5], 1e-3, lo, hi, use_roi)
         pos, node = oracle.neighbor_pairs(t["node_min"], t["node_max"], t["children"], act, (4, 3, 5), eps=1e-3,
-                                          roi=roi)
-        print(f"depth {depth} roi={use_roi}: {pos.shape[0] // 2} pairs", flush=True)
+                                          roi=roi, quad=quad)
+        print(f"depth {depth} roi={use_roi} quad={quad}: {pos.shape[0] // 2} pairs", flush=True)
         assert pos.shape[0] > 0
         assert np.array_equal(s.global_coordinate.cpu().numpy(), pos)
         assert np.array_equal(s.node_id.cpu().numpy(), node)
         assert np.array_equal(s.local_index_in_ray.cpu().numpy(), np.arange(pos.shape[0]) % 2)
+        if quad:
+            with pytest.raises(RuntimeError, match="quad-tree representation"):
+                tree.NodeNeighborSamplesPlane2D([4, 3, 5], 1e-3, 0, lo, hi, use_roi, True)
 
 
 def test_edge_loss_and_regularizer(oracle, native, cuda):

@@ -143,6 +143,92 @@ def test_neighbor_pairs_adaptive(oracle):
     assert np.all((lo <= p) & (p <= hi))
 
 
+def test_neighbor_pairs_quad_closed_forms(oracle):
+    """NodeNeighborSamplesQuad on the depth-1 octree: faces normal to x and z
+    only, and point location over children 0..3 (the z-lower octants).  A
+    moved point with z > 0 is found nowhere, so: x-faces of the z-lower
+    octants pair fully (4 s1 s2), x-faces of the z-upper octants only in
+    their z = 0 row (4 s1), and the z = 0 faces of the z-upper octants pair
+    downwards (4 s0 s1); the z-lower octants' upward moves find nothing."""
+    t = oracle.tree_build(1)
+    act = np.zeros(t["node_min"].shape[0], np.int32)
+    act[1:9] = 1
+    for size in ((5, 5, 5), (3, 4, 5), (2, 6, 3)):
+        s0, s1, s2 = size
+        pos, node = oracle.neighbor_pairs(t["node_min"], t["node_max"], t["children"], act, size, quad=True)
+        assert pos.shape[0] // 2 == 4 * s1 * s2 + 4 * s1 + 4 * s0 * s1
+        a, b = node[0::2], node[1::2]
+        assert np.all((b >= 1) & (b <= 4)) and np.all(a != b)
+        # no pair across a y face
+        assert not np.any(((a - 1) ^ (b - 1)) == 2)
+    # in_roi_quad widens the y range by eps / 2: a y bound 4e-4 below the
+    # octants' shared face still takes the upper octants (eps = 1e-3), the
+    # plain rule does not
+    roi = ((-2.0, -1.0, -2.0), (2.0, -4e-4, 2.0))
+    full, _ = oracle.neighbor_pairs(t["node_min"], t["node_max"], t["children"], act, (3, 3, 3), quad=True)
+    wide, _ = oracle.neighbor_pairs(t["node_min"], t["node_max"], t["children"], act, (3, 3, 3), roi=roi, quad=True)
+    plain_full, _ = oracle.neighbor_pairs(t["node_min"], t["node_max"], t["children"], act, (3, 3, 3))
+    plain, _ = oracle.neighbor_pairs(t["node_min"], t["node_max"], t["children"], act, (3, 3, 3), roi=roi)
+    assert np.array_equal(wide, full) and plain.shape[0] < plain_full.shape[0]
+    roi2 = ((-2.0, -1.0, -2.0), (2.0, -6e-4, 2.0))
+    narrow, _ = oracle.neighbor_pairs(t["node_min"], t["node_max"], t["children"], act, (3, 3, 3), roi=roi2, quad=True)
+    assert 0 < narrow.shape[0] < full.shape[0]
+
+
+def _brute_quad_pairs(t, act, size, eps):
+    """NodeNeighborSamplesQuad written out directly (float32 arithmetic)."""
+    f = np.float32
+    mn, mx, ch = t["node_min"], t["node_max"], t["children"]
+
+    def locate(q):
+        node = 0
+        while True:
+            if act[node]:
+                return node
+            nxt = -1
+            for c in range(4):
+                k = ch[node, c]
+                if k < 0:
+                    return -1
+                if np.all(mn[k] <= q) and np.all(q <= mx[k]):
+                    nxt = k
+                    break
+            if nxt < 0:
+                return -1
+            node = nxt
+
+    out = []
+    for node in np.flatnonzero(act):
+        box = (mx[node] - mn[node]).astype(f)
+        for side in (0, 2):
+            sa, sb = (side + 1) % 3, (side + 2) % 3
+            for front in (0, 1):
+                for sid in range(size[sa] * size[sb]):
+                    loc = np.zeros(3, f)
+                    loc[side] = f(front)
+                    loc[sa] = f(sid % size[sa]) / (f(size[sa]) - f(1))
+                    loc[sb] = f(sid // size[sa]) / (f(size[sb]) - f(1))
+                    g = (loc * box + mn[node]).astype(f)
+                    q = g.copy()
+                    q[side] = g[side] + (f(eps) if front else -f(eps))
+                    other = locate(q)
+                    if other >= 0 and other != node:
+                        out.append((tuple(g), node, other))
+    return out
+
+
+def test_neighbor_pairs_quad_brute_force(oracle):
+    from scene_util import adaptive_active
+
+    t = oracle.tree_build(2)
+    act = adaptive_active(t, budget=30, seed=2)
+    want = _brute_quad_pairs(t, act, (3, 4, 3), 1e-3)
+    pos, node = oracle.neighbor_pairs(t["node_min"], t["node_max"], t["children"], act, (3, 4, 3), quad=True)
+    assert len(want) == pos.shape[0] // 2 > 0
+    for k, (g, a, b) in enumerate(want):
+        assert tuple(pos[2 * k]) == g and node[2 * k] == a and node[2 * k + 1] == b
+
+
 def test_parameter_store_init_modes(native):
     g = native.InitNodeGrid(7, 16, 11, "uniform", 0.5, 3)
     assert tuple(g.shape) == (7, 11, 11, 11, 16) and g.dtype == torch.float32
