@@ -166,6 +166,9 @@ _SIGNATURES = {
                                               _P, _P, _P, _P, _P]),
     "atdf_tv_workspace_bytes": (_SZ, [_I64]),
     "atdf_tv_loss": (_I32, [_P, C.POINTER(TVShape), _P, C.c_double, _P, _P, _P, _SZ, _P]),
+    "atdf_grid_sample_bwd_workspace_bytes": (_SZ, [_I64, _I64]),
+    "atdf_grid_sample_bwd_max_slot_floats": (_I64, []),
+    "atdf_grid_sample_bwd_slots": (_I32, [_I32, _P, _I64, _I32, _P, _P, _P, _I64, _P, _P, _P, _P, _SZ, _P]),
     "atdf_fourier_ring_workspace_bytes": (_SZ, [_I32, _I32]),
     "atdf_fourier_ring_means": (_I32, [_P, _I64, _I32, _I32, _P, _P, _SZ, _P]),
     "atdf_decode_features": (_I32, [_P, _I64, C.POINTER(Field), _P, _P]),

@@ -619,3 +619,383 @@ atdf_status atdf_f32_to_bf16(const float* src, uint16_t* dst, int64_t n, atdf_st
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Slot-major adjoint of the indirect gathers (the backward of
+// IndirectGridSample3D / 2D / 1D, modules/IndirectGridSample2D.cu:441-477
+// pattern).  The per-sample kernels above add every corner with a global
+// float atomic; the samples of one grid slot fall on the same few hundred
+// cells, and a tensorial field's line takes every sample of the batch, so
+// those atomics serialise.  Here the samples are counting-sorted by slot (LDS
+// histograms and LDS ranks, one global atomic per slot and workgroup), cut
+// into chunks of at most kGsChunk samples of one slot, and a workgroup per
+// chunk accumulates w * grad_out into an LDS copy of the slot's gradient (LDS
+// float atomics), then adds it to grad_grid once: a plain read-add-write when
+// the chunk is the slot's only one, global atomics otherwise.  grad_uv is per
+// sample and added in the same pass, with the per-sample kernels' arithmetic.
+// ---------------------------------------------------------------------------
+namespace atdf {
+
+constexpr int kGsSortItems = 8192;  // samples per sort workgroup
+constexpr int kGsHistMax = 8192;    // slots an LDS histogram holds
+constexpr int kGsChunk = 4096;      // samples per accumulation chunk
+constexpr int kGsBlock = 512;
+
+struct GsSort {
+    const int64_t* index;
+    int64_t S, N;
+    int32_t* cnt;     // [N]
+    int64_t* off;     // [N + 1]
+    int32_t* cursor;  // [N]
+    int32_t* sorted;  // [S]
+    int32_t* ccnt;    // [N]
+    int64_t* coff;    // [N + 1]
+    int4* desc;       // [chunks]: slot, begin, end, single chunk
+};
+
+__global__ __launch_bounds__(256) void gs_zero_kernel(int32_t* __restrict__ a, int32_t* __restrict__ b, int64_t n)
+{
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i < n) {
+        a[i] = 0;
+        b[i] = 0;
+    }
+}
+
+__global__ __launch_bounds__(256) void gs_hist_kernel(GsSort a)
+{
+    __shared__ int h[kGsHistMax];
+    const bool lds = a.N <= kGsHistMax;
+    if (lds)
+        for (int i = threadIdx.x; i < a.N; i += 256) h[i] = 0;
+    __syncthreads();
+    const int64_t b0 = blockIdx.x * (int64_t)kGsSortItems;
+    const int64_t b1 = b0 + kGsSortItems < a.S ? b0 + kGsSortItems : a.S;
+    for (int64_t s = b0 + threadIdx.x; s < b1; s += 256) {
+        const int64_t k = a.index[s];
+        if (lds)
+            atomicAdd(&h[k], 1);
+        else
+            atomicAdd(&a.cnt[k], 1);
+    }
+    __syncthreads();
+    if (lds)
+        for (int i = threadIdx.x; i < a.N; i += 256)
+            if (h[i]) atomicAdd(&a.cnt[i], h[i]);
+}
+
+__global__ __launch_bounds__(256) void gs_scatter_kernel(GsSort a)
+{
+    __shared__ int h[kGsHistMax];
+    const bool lds = a.N <= kGsHistMax;
+    const int64_t b0 = blockIdx.x * (int64_t)kGsSortItems;
+    const int64_t b1 = b0 + kGsSortItems < a.S ? b0 + kGsSortItems : a.S;
+    if (!lds) {
+        for (int64_t s = b0 + threadIdx.x; s < b1; s += 256) {
+            const int64_t k = a.index[s];
+            a.sorted[a.off[k] + atomicAdd(&a.cursor[k], 1)] = (int32_t)s;
+        }
+        return;
+    }
+    for (int i = threadIdx.x; i < a.N; i += 256) h[i] = 0;
+    __syncthreads();
+    for (int64_t s = b0 + threadIdx.x; s < b1; s += 256) atomicAdd(&h[a.index[s]], 1);
+    __syncthreads();
+    // reserve this workgroup's range in every slot it touches
+    for (int i = threadIdx.x; i < a.N; i += 256)
+        if (h[i]) h[i] = atomicAdd(&a.cursor[i], h[i]);
+    __syncthreads();
+    for (int64_t s = b0 + threadIdx.x; s < b1; s += 256) {
+        const int64_t k = a.index[s];
+        a.sorted[a.off[k] + atomicAdd(&h[k], 1)] = (int32_t)s;
+    }
+}
+
+__global__ __launch_bounds__(256) void gs_chunk_count_kernel(GsSort a)
+{
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i < a.N) a.ccnt[i] = (a.cnt[i] + kGsChunk - 1) / kGsChunk;
+}
+
+__global__ __launch_bounds__(256) void gs_chunk_fill_kernel(GsSort a)
+{
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= a.N) return;
+    const int n = a.ccnt[i];
+    const int64_t o = a.off[i], e = a.off[i + 1];
+    for (int k = 0; k < n; ++k) {
+        const int64_t b = o + (int64_t)k * kGsChunk;
+        const int64_t f = b + kGsChunk < e ? b + kGsChunk : e;
+        a.desc[a.coff[i] + k] = make_int4((int)i, (int)b, (int)f, n == 1 ? 1 : 0);
+    }
+}
+
+struct GsAcc {
+    const float* grid;
+    const int64_t* index;
+    const float* uv;
+    const float* grad_out;
+    float* grad_grid;
+    float* grad_uv;
+    const int32_t* sorted;
+    const int4* desc;
+    const int64_t* total_chunks;
+    int C, Z, Y, X;  // D = 2: Y = H, X = W; D = 1: X = W
+    int slot_size;   // C * cells
+};
+
+template <int D>
+__global__ __launch_bounds__(kGsBlock) void gs_accumulate_kernel(GsAcc a)
+{
+    extern __shared__ __attribute__((aligned(16))) float lg[];
+    const int C = a.C, Z = a.Z, Y = a.Y, X = a.X;
+    const int cells = a.slot_size / C;
+    const int64_t total = *a.total_chunks;
+    for (int64_t c = blockIdx.x; c < total; c += gridDim.x) {
+        const int4 ch = a.desc[c];
+        for (int i = threadIdx.x; i < a.slot_size; i += kGsBlock) lg[i] = 0.f;
+        __syncthreads();
+        const int64_t base = (int64_t)ch.x * a.slot_size;
+        const float* gslot = a.grid + base;
+        for (int p = ch.y + threadIdx.x; p < ch.z; p += kGsBlock) {
+            const int64_t s = a.sorted[p];
+            const float* go_s = a.grad_out + s * C;
+            if constexpr (D == 3) {
+                const float u = a.uv[s * 3 + 0], v = a.uv[s * 3 + 1], w = a.uv[s * 3 + 2];
+                Tri t;
+                tri_setup(u, v, w, Z, Y, X, t);
+                const int off[8] = {(t.z0 * Y + t.y0) * X + t.x0, (t.z0 * Y + t.y0) * X + t.x1,
+                                    (t.z0 * Y + t.y1) * X + t.x0, (t.z0 * Y + t.y1) * X + t.x1,
+                                    (t.z1 * Y + t.y0) * X + t.x0, (t.z1 * Y + t.y0) * X + t.x1,
+                                    (t.z1 * Y + t.y1) * X + t.x0, (t.z1 * Y + t.y1) * X + t.x1};
+                float dw[8][3];
+                if (a.grad_uv) tri_dweights(u, v, w, Z, Y, X, dw);
+                float gu[3] = {0.f, 0.f, 0.f};
+                for (int cc = 0; cc < C; ++cc) {
+                    const float go = go_s[cc];
+                    float* l = lg + cc * cells;
+#pragma unroll
+                    for (int k = 0; k < 8; ++k) atomicAdd(l + off[k], t.w[k] * go);
+                    if (a.grad_uv) {
+                        const float* gc = gslot + (int64_t)cc * cells;
+#pragma unroll
+                        for (int k = 0; k < 8; ++k) {
+                            const float val = gc[off[k]] * go;
+                            gu[0] += val * dw[k][0];
+                            gu[1] += val * dw[k][1];
+                            gu[2] += val * dw[k][2];
+                        }
+                    }
+                }
+                if (a.grad_uv) {
+                    a.grad_uv[s * 3 + 0] += gu[0] * ((float)(X - 1) / 2);
+                    a.grad_uv[s * 3 + 1] += gu[1] * ((float)(Y - 1) / 2);
+                    a.grad_uv[s * 3 + 2] += gu[2] * ((float)(Z - 1) / 2);
+                }
+            } else if constexpr (D == 2) {
+                const float u = a.uv[s * 2], v = a.uv[s * 2 + 1];
+                Bil b;
+                bil_setup(u, v, Y, X, b);
+                const int off[4] = {b.y0 * X + b.x0, b.y0 * X + b.x1, b.y1 * X + b.x0, b.y1 * X + b.x1};
+                const float ix = ((u + 1) / 2) * (float)(X - 1);
+                const float iy = ((v + 1) / 2) * (float)(Y - 1);
+                const float fx[2] = {(float)((int)floorf(ix) + 1) - ix, ix - (float)(int)floorf(ix)};
+                const float fy[2] = {(float)((int)floorf(iy) + 1) - iy, iy - (float)(int)floorf(iy)};
+                float gx = 0.f, gy = 0.f;
+                for (int cc = 0; cc < C; ++cc) {
+                    const float go = go_s[cc];
+                    float* l = lg + cc * cells;
+#pragma unroll
+                    for (int k = 0; k < 4; ++k) atomicAdd(l + off[k], b.w[k] * go);
+                    if (a.grad_uv) {
+                        const float* gc = gslot + (int64_t)cc * cells;
+#pragma unroll
+                        for (int k = 0; k < 4; ++k) {
+                            const int bx = k & 1, by = k >> 1;
+                            const float val = gc[off[k]] * go;
+                            gx += val * (bx ? fy[by] : -fy[by]);
+                            gy += val * (by ? fx[bx] : -fx[bx]);
+                        }
+                    }
+                }
+                if (a.grad_uv) {
+                    a.grad_uv[s * 2] += gx * ((float)(X - 1) / 2);
+                    a.grad_uv[s * 2 + 1] += gy * ((float)(Y - 1) / 2);
+                }
+            } else {
+                const float ix = ((a.uv[s] + 1) / 2) * (float)(X - 1);
+                const int x0 = (int)floorf(ix), x1 = x0 + 1;
+                const float w0 = (float)x1 - ix, w1 = ix - (float)x0;
+                const int c0 = clampi(x0, X), c1 = clampi(x1, X);
+                float gx = 0.f;
+                for (int cc = 0; cc < C; ++cc) {
+                    const float go = go_s[cc];
+                    atomicAdd(lg + cc * X + c0, w0 * go);
+                    atomicAdd(lg + cc * X + c1, w1 * go);
+                    if (a.grad_uv) gx += (gslot[cc * X + c1] - gslot[cc * X + c0]) * go;
+                }
+                if (a.grad_uv) a.grad_uv[s] += gx * ((float)(X - 1) / 2);
+            }
+        }
+        __syncthreads();
+        float* gg = a.grad_grid + base;
+        if (ch.w) {
+            for (int i = threadIdx.x; i < a.slot_size; i += kGsBlock) gg[i] += lg[i];
+        } else {
+            for (int i = threadIdx.x; i < a.slot_size; i += kGsBlock)
+                if (lg[i] != 0.f) unsafeAtomicAdd(gg + i, lg[i]);
+        }
+        __syncthreads();
+    }
+}
+
+struct GsLayout {
+    size_t cnt, off, cursor, ccnt, coff, sorted, desc, scan, total;
+};
+
+size_t gs_align(size_t x) { return (x + 255) & ~(size_t)255; }
+
+GsLayout gs_layout(int64_t N, int64_t S)
+{
+    GsLayout L{};
+    size_t p = 0;
+    auto take = [&](size_t bytes) {
+        const size_t at = p;
+        p += gs_align(bytes);
+        return at;
+    };
+    L.cnt = take(sizeof(int32_t) * N);
+    L.off = take(sizeof(int64_t) * (N + 1));
+    L.cursor = take(sizeof(int32_t) * N);
+    L.ccnt = take(sizeof(int32_t) * N);
+    L.coff = take(sizeof(int64_t) * (N + 1));
+    L.sorted = take(sizeof(int32_t) * S);
+    L.desc = take(sizeof(int4) * (size_t)(S / kGsChunk + N + 1));
+    L.scan = take(atdf_scan_workspace_bytes(N));
+    L.total = p;
+    return L;
+}
+
+int gs_max_lds()
+{
+    static int v = 0;
+    if (v == 0) {
+        int dev = 0, x = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&x, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess || x <= 0) {
+            (void)hipGetLastError();
+            x = 64 * 1024;
+        }
+        v = x;
+    }
+    return v;
+}
+
+int gs_blocks()
+{
+    static int n = 0;
+    if (n == 0) {
+        int dev = 0, cus = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0) {
+            (void)hipGetLastError();
+            cus = 256;
+        }
+        n = 2 * cus;
+    }
+    return n;
+}
+
+}  // namespace atdf
+
+extern "C" {
+
+int64_t atdf_grid_sample_bwd_max_slot_floats(void) { return gs_max_lds() / (int)sizeof(float); }
+
+size_t atdf_grid_sample_bwd_workspace_bytes(int64_t num_slots, int64_t num_samples)
+{
+    if (num_slots <= 0 || num_samples <= 0) return 0;
+    return gs_layout(num_slots, num_samples).total;
+}
+
+atdf_status atdf_grid_sample_bwd_slots(int32_t D, const float* grid, int64_t N, int32_t C, const int32_t* size,
+                                       const int64_t* index, const float* uv, int64_t S, const float* grad_out,
+                                       float* grad_grid, float* grad_uv, void* workspace, size_t workspace_bytes,
+                                       atdf_stream_t stream)
+{
+    ATDF_REQUIRE(D >= 1 && D <= 3 && size, "atdf_grid_sample_bwd_slots: D must be 1, 2 or 3");
+    int Z = 1, Y = 1, X = 1;
+    if (D == 3) {
+        Z = size[0];
+        Y = size[1];
+        X = size[2];
+    } else if (D == 2) {
+        Y = size[0];
+        X = size[1];
+    } else {
+        X = size[0];
+    }
+    ATDF_REQUIRE(N > 0 && C > 0 && Z > 0 && Y > 0 && X > 0 && S >= 0, "atdf_grid_sample_bwd_slots: bad shape");
+    if (S == 0) return ATDF_OK;
+    ATDF_REQUIRE(grid && index && uv && grad_out && grad_grid, "atdf_grid_sample_bwd_slots: null buffer");
+    ATDF_REQUIRE(S < (int64_t)INT32_MAX && N < (int64_t)INT32_MAX, "atdf_grid_sample_bwd_slots: %lld samples / %lld "
+                 "slots, at most 2^31 - 1", (long long)S, (long long)N);
+    const int64_t slot = (int64_t)C * Z * Y * X;
+    ATDF_REQUIRE(slot * (int64_t)sizeof(float) <= gs_max_lds(), "atdf_grid_sample_bwd_slots: a slot's gradient "
+                 "(%lld floats) exceeds the %d bytes of LDS a workgroup can hold; use the per-sample backward",
+                 (long long)slot, gs_max_lds());
+    const GsLayout L = gs_layout(N, S);
+    ATDF_REQUIRE(workspace && workspace_bytes >= L.total, "atdf_grid_sample_bwd_slots: workspace of %zu bytes, %zu "
+                 "needed", workspace_bytes, L.total);
+    unsigned char* w = static_cast<unsigned char*>(workspace);
+    GsSort a{};
+    a.index = index;
+    a.S = S;
+    a.N = N;
+    a.cnt = reinterpret_cast<int32_t*>(w + L.cnt);
+    a.off = reinterpret_cast<int64_t*>(w + L.off);
+    a.cursor = reinterpret_cast<int32_t*>(w + L.cursor);
+    a.ccnt = reinterpret_cast<int32_t*>(w + L.ccnt);
+    a.coff = reinterpret_cast<int64_t*>(w + L.coff);
+    a.sorted = reinterpret_cast<int32_t*>(w + L.sorted);
+    a.desc = reinterpret_cast<int4*>(w + L.desc);
+    hipStream_t s = to_hip(stream);
+    const unsigned nb = blocks_for(N), sb = (unsigned)div_up(S, kGsSortItems);
+    hipLaunchKernelGGL(gs_zero_kernel, dim3(nb), dim3(256), 0, s, a.cnt, a.cursor, N);
+    hipLaunchKernelGGL(gs_hist_kernel, dim3(sb), dim3(256), 0, s, a);
+    atdf_status st = atdf_exclusive_scan_i32(a.cnt, N, a.off, w + L.scan, atdf_scan_workspace_bytes(N), stream);
+    if (st) return st;
+    hipLaunchKernelGGL(gs_scatter_kernel, dim3(sb), dim3(256), 0, s, a);
+    hipLaunchKernelGGL(gs_chunk_count_kernel, dim3(nb), dim3(256), 0, s, a);
+    st = atdf_exclusive_scan_i32(a.ccnt, N, a.coff, w + L.scan, atdf_scan_workspace_bytes(N), stream);
+    if (st) return st;
+    hipLaunchKernelGGL(gs_chunk_fill_kernel, dim3(nb), dim3(256), 0, s, a);
+    GsAcc g{};
+    g.grid = grid;
+    g.index = index;
+    g.uv = uv;
+    g.grad_out = grad_out;
+    g.grad_grid = grad_grid;
+    g.grad_uv = grad_uv;
+    g.sorted = a.sorted;
+    g.desc = a.desc;
+    g.total_chunks = a.coff + N;
+    g.C = C;
+    g.Z = Z;
+    g.Y = Y;
+    g.X = X;
+    g.slot_size = (int)slot;
+    const size_t lds = sizeof(float) * (size_t)slot;
+    const dim3 grid_dim((unsigned)gs_blocks()), block(kGsBlock);
+#define ATDF_GS(DD) hipLaunchKernelGGL(gs_accumulate_kernel<DD>, grid_dim, block, lds, s, g)
+    if (D == 3)
+        ATDF_GS(3);
+    else if (D == 2)
+        ATDF_GS(2);
+    else
+        ATDF_GS(1);
+#undef ATDF_GS
+    return check_launch("atdf_grid_sample_bwd_slots");
+}
+
+}  // extern "C"

@@ -256,6 +256,7 @@ PYBIND11_MODULE(TORCH_EXTENSION_NAME, m)
           py::arg("roi_min"), py::arg("roi_max"), py::arg("params"));
     m.def("DecodeFeatures", &DecodeFeatures);
     m.def("SampleVolumeIndirect", &SampleVolumeIndirect);
+    m.def("ComputeImageVM", &ComputeImageVM);
     m.def("FourierRingMeans", &FourierRingMeans);
     m.def("FourierLoss", &FourierLoss);
     py::class_<RegularizerParams>(m, "RegularizerParams")

@@ -203,6 +203,15 @@ Tensor SampleVolumeIndirect(HyperTreeBase& tree, const RoiMaps& roi, const VMFie
     return DecodeFeatures(SampleVolumeIndirectFeature(tree, roi, fields, global_coordinate, node_id), mlp, fp);
 }
 
+Tensor ComputeImageVM(HyperTreeBase& tree, const RoiMaps& roi, const VMFields& fields, Tensor mlp,
+                      const FieldParams& fp, const SampleList& samples, int64_t num_rays)
+{
+    auto dens = SampleVolumeIndirect(tree, roi, fields, mlp, fp, samples.global_coordinate, samples.node_id);
+    if (samples.global_coordinate.numel() == 0)
+        return torch::zeros({1, num_rays}, samples.global_coordinate.options().dtype(torch::kFloat32));
+    return IntegrateSamplesXRay(dens, samples.weight.unsqueeze(1), samples.ray_index, 1, num_rays);
+}
+
 FourierGrid MakeFourierGrid(HyperTreeBase& tree, int grid_size, int tree_depth, std::vector<float> roi_min,
                             std::vector<float> roi_max, const FourierParams& fp)
 {

@@ -396,6 +396,26 @@ void HyperTreeBase::UpdateCulling()
 // IndirectGridSample3D
 // ---------------------------------------------------------------------------
 namespace {
+// The slot-major backward (samples sorted by slot, each slot's gradient in
+// LDS) when a slot's gradient fits one workgroup's LDS; false otherwise, and
+// the caller takes the per-sample backward.
+bool slot_backward(int D, const Tensor& g, const Tensor& idx, const Tensor& u, const Tensor& go, Tensor& grad_grid,
+                   Tensor& grad_uv)
+{
+    const int64_t S = idx.size(0), N = g.size(0);
+    if (S == 0 || N == 0) return true;
+    if (g[0].numel() > atdf_grid_sample_bwd_max_slot_floats()) return false;
+    int32_t size[3] = {1, 1, 1};
+    for (int d = 0; d < D; ++d) size[d] = (int32_t)g.size(2 + d);
+    auto ws = torch::empty({(int64_t)atdf_grid_sample_bwd_workspace_bytes(N, S)}, g.options().dtype(torch::kUInt8));
+    check_status(atdf_grid_sample_bwd_slots(D, g.data_ptr<float>(), N, (int)g.size(1), size, idx.data_ptr<int64_t>(),
+                                            u.data_ptr<float>(), S, go.data_ptr<float>(), grad_grid.data_ptr<float>(),
+                                            grad_uv.data_ptr<float>(), ws.data_ptr(), (size_t)ws.numel(),
+                                            current_stream()),
+                 "atdf_grid_sample_bwd_slots");
+    return true;
+}
+
 struct IndirectGridSample3DFn : public torch::autograd::Function<IndirectGridSample3DFn> {
     static Tensor forward(torch::autograd::AutogradContext* ctx, Tensor grid, Tensor index, Tensor uv)
     {
@@ -422,6 +442,7 @@ struct IndirectGridSample3DFn : public torch::autograd::Function<IndirectGridSam
         auto go = grad_output[0].contiguous();
         auto grad_grid = torch::zeros_like(grid);
         auto grad_uv = torch::zeros_like(uv);
+        if (slot_backward(3, grid, idx, uv.contiguous(), go, grad_grid, grad_uv)) return {grad_grid, Tensor(), grad_uv};
         check_status(atdf_grid_sample3d_bwd(grid.data_ptr<float>(), grid.size(0), (int)grid.size(1), (int)grid.size(2),
                                             (int)grid.size(3), (int)grid.size(4), idx.data_ptr<int64_t>(),
                                             uv.data_ptr<float>(), idx.size(0), go.data_ptr<float>(),
@@ -459,6 +480,7 @@ struct IndirectGridSample2DFn : public torch::autograd::Function<IndirectGridSam
         auto go = grad_output[0].contiguous();
         auto grad_grid = torch::zeros_like(g);
         auto grad_uv = torch::zeros_like(u);
+        if (slot_backward(2, g, idx, u, go, grad_grid, grad_uv)) return {grad_grid, Tensor(), grad_uv};
         check_status(atdf_grid_sample2d_bwd(g.data_ptr<float>(), g.size(0), (int)g.size(1), (int)g.size(2),
                                             (int)g.size(3), idx.data_ptr<int64_t>(), u.data_ptr<float>(), idx.size(0),
                                             go.data_ptr<float>(), grad_grid.data_ptr<float>(),
@@ -496,6 +518,7 @@ struct IndirectGridSample1DFn : public torch::autograd::Function<IndirectGridSam
         auto go = grad_output[0].contiguous();
         auto grad_grid = torch::zeros_like(g);
         auto grad_uv = torch::zeros_like(u);
+        if (slot_backward(1, g, idx, u, go, grad_grid, grad_uv)) return {grad_grid, Tensor(), grad_uv};
         check_status(atdf_grid_sample1d_bwd(g.data_ptr<float>(), g.size(0), (int)g.size(1), (int)g.size(2),
                                             idx.data_ptr<int64_t>(), u.data_ptr<float>(), idx.size(0),
                                             go.data_ptr<float>(), grad_grid.data_ptr<float>(),

@@ -264,6 +264,12 @@ torch::Tensor DecodeFeatures(torch::Tensor features, torch::Tensor mlp, const Fi
 // DecodeFeatures(SampleVolumeIndirectFeature(...)).
 torch::Tensor SampleVolumeIndirect(HyperTreeBase& tree, const RoiMaps& roi, const VMFields& fields, torch::Tensor mlp,
                                    const FieldParams& fp, torch::Tensor global_coordinate, torch::Tensor node_id);
+// HierarchicalNeuralGeometry::ComputeImage (geometry.cpp:828-914) for the
+// tensorial geometry: SampleVolumeIndirect of the samples, then
+// IntegrateSamplesXRay with their weights -> [1, num_rays] (autograd
+// through the VM gathers, the decoder and the integration).
+torch::Tensor ComputeImageVM(HyperTreeBase& tree, const RoiMaps& roi, const VMFields& fields, torch::Tensor mlp,
+                             const FieldParams& fp, const SampleList& samples, int64_t num_rays);
 // Ring means [X / 2, Y] of img [Y, X, Z] (atdf_fourier_ring_means; row 0 +inf).
 torch::Tensor FourierRingMeans(torch::Tensor img);
 // The Fourier term on densities [X, Y, Z] (geometry_tensor_qo.cpp:758-850,

@@ -742,6 +742,24 @@ size_t atdf_fourier_ring_workspace_bytes(int32_t X, int32_t Z);
 atdf_status atdf_fourier_ring_means(const float* img, int64_t Y, int32_t X, int32_t Z, float* out, void* workspace,
                                     size_t workspace_bytes, atdf_stream_t stream);
 
+/* Slot-major backward of the indirect gathers (IndirectGridSample3D / 2D /
+ * 1D backward; same results as atdf_grid_sample{3,2,1}d_bwd up to the order
+ * of the float additions): samples counting-sorted by slot, each slot's
+ * gradient accumulated in LDS and added to grad_grid once per chunk of
+ * samples, instead of one global atomic per corner and channel.  D = 3: size
+ * = {Z, Y, X}, uv [S, 3]; D = 2: size = {H, W}, uv [S, 2]; D = 1: size = {W},
+ * uv [S].  grid [N, C, size...]; index in [0, N); grad_grid and grad_uv
+ * (nullable) are ADDED to.  A slot's gradient (C * cells floats) must fit in
+ * one workgroup's LDS.  Workspace: atdf_grid_sample_bwd_workspace_bytes(N, S). */
+size_t atdf_grid_sample_bwd_workspace_bytes(int64_t num_slots, int64_t num_samples);
+/* The largest slot (C * cells floats) atdf_grid_sample_bwd_slots takes on the
+ * current device. */
+int64_t atdf_grid_sample_bwd_max_slot_floats(void);
+atdf_status atdf_grid_sample_bwd_slots(int32_t D, const float* grid, int64_t N, int32_t C, const int32_t* size,
+                                       const int64_t* index, const float* uv, int64_t S, const float* grad_out,
+                                       float* grad_grid, float* grad_uv, void* workspace, size_t workspace_bytes,
+                                       atdf_stream_t stream);
+
 /* [N, C, G, G, G] <-> [N, G, G, G, C] (direction 0: to channels-last). */
 atdf_status atdf_grid_layout(const float* src, float* dst, int64_t N, int32_t C, int32_t G, int32_t direction,
                              atdf_stream_t stream);

@@ -165,3 +165,35 @@ def test_volume_regularizer_fourier_term(oracle, native, cuda):
         assert gr is not None and torch.isfinite(gr).all() and gr.abs().max() > 0, k
     # the term is off without its lattice or with loss_fourier <= 0
     assert native.VolumeRegularizer(tree, roi, f, p) is None
+
+
+def test_compute_image_vm(oracle, native, cuda):
+    """ComputeImage of the tensorial geometry: traversal samples, VM features,
+    decoder, line integral; against the oracle composition in FP64, and the
+    decoder-parameter gradient against the oracle's adjoint."""
+    from test_gpu_vm import _vm_oracle
+    from conftest import random_rays
+
+    t, act, tree, roi, f, arr, tr, lohi = _vm_setup(native, oracle, "add", seed=6)
+    o, d = random_rays(300, 46, spread=0.9)
+    rays = native.RayList(torch.from_numpy(o).to(cuda), torch.from_numpy(d).to(cuda))
+    samples = tree.CreateSamplesForRays(rays, 4, False)
+    rng = np.random.default_rng(12)
+    mlp_np = _mlp(rng, 8, scale=0.3)
+    mlp = torch.from_numpy(mlp_np).to(cuda).requires_grad_()
+    fp = _fp(native, 8)
+    img = native.ComputeImageVM(tree, roi, f, mlp, fp, samples, 300)
+    assert img.shape == (1, 300)
+    feats, _ = _vm_oracle(oracle, t, act, roi, arr, tr, lohi, "add")
+    assert np.array_equal(samples.node_id.cpu().numpy(), tr["node_id"])
+    dens, _, _ = oracle.decode_features(mlp_np, 8, 64, 2.0, feats)
+    want = np.zeros(300)
+    np.add.at(want, tr["ray_index"], dens * tr["weight"].astype(np.float64))
+    got = img[0].detach().cpu().double().numpy()
+    print(f"VM image: {tr['node_id'].size} samples, rel {_rel(got, want):.2e}", flush=True)
+    assert _rel(got, want) <= 1e-5
+    go = rng.normal(size=300)
+    (gm,) = torch.autograd.grad(img[0], mlp, torch.from_numpy(go.astype(np.float32)).to(cuda))
+    dd = go[tr["ray_index"]] * tr["weight"].astype(np.float64)
+    _, _, ogm = oracle.decode_features(mlp_np, 8, 64, 2.0, feats, dd)
+    assert _rel(gm.cpu().double().numpy(), ogm) <= 1e-5

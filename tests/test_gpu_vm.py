@@ -323,3 +323,57 @@ def test_edge_loss_and_regularizer(oracle, native, cuda):
         assert _rel(a.cpu().double().numpy(), b.cpu().double().numpy()) <= 1e-5, k
     # without pairs (or with the term off) there is no edge term
     assert native.VolumeRegularizer(tree, roi, f, p) is None
+
+
+@pytest.mark.parametrize("D,N,shape,S", [(1, 1, (193,), 100_000), (2, 9000, (11, 11), 60_000),
+                                         (3, 40, (11, 11, 11), 50_000), (3, 9000, (5, 5, 5), 30_000)])
+def test_grid_sample_bwd_slots(oracle, cuda, D, N, shape, S):
+    """The slot-major backward through the raw ABI: one line taking every
+    sample (25 chunks of one slot), more slots than an LDS histogram holds
+    (global-atomic sort), multi-chunk 3-D slots of 85 KB; against the FP64
+    oracle backward."""
+    import ctypes as C
+    from adaptivetensordf_amd import capi
+
+    lib = capi.lib()
+    rng = np.random.default_rng(D * 100 + N)
+    Cc = 16
+    grid = rng.normal(size=(N, Cc) + shape).astype(np.float32)
+    index = np.sort(rng.integers(0, N, S)) if D == 3 and N == 40 else rng.integers(0, N, S)
+    uv = rng.uniform(-1.05, 1.05, (S, D)).astype(np.float32)
+    # keep coordinates off the lattice points, where the derivative of the
+    # interpolation jumps and float32 and FP64 can take different sides
+    while True:
+        bad = np.zeros(S, bool)
+        for d in range(D):
+            n = np.float32(shape[D - 1 - d] - 1)
+            ix = ((uv[:, d] + np.float32(1)) / np.float32(2)) * n
+            bad |= np.abs(ix - np.round(ix)) < 1e-3
+        if not bad.any():
+            break
+        uv[bad] = rng.uniform(-1.05, 1.05, (int(bad.sum()), D)).astype(np.float32)
+    go = rng.normal(size=(S, Cc)).astype(np.float32)
+    print(f"slots D={D} N={N} shape={shape} S={S}", flush=True)
+    t = {k: torch.from_numpy(v).to(cuda) for k, v in (("grid", grid), ("index", index), ("uv", uv), ("go", go))}
+    gg = torch.zeros_like(t["grid"])
+    gu = torch.zeros_like(t["uv"])
+    ws = torch.empty(lib.atdf_grid_sample_bwd_workspace_bytes(N, S), dtype=torch.uint8, device=cuda)
+    size = (C.c_int32 * 3)(*(list(shape) + [1] * (3 - D)))
+    assert lib.atdf_grid_sample_bwd_max_slot_floats() >= Cc * int(np.prod(shape))
+    st = lib.atdf_grid_sample_bwd_slots(D, t["grid"].data_ptr(), N, Cc, size, t["index"].data_ptr(), t["uv"].data_ptr(),
+                                        S, t["go"].data_ptr(), gg.data_ptr(), gu.data_ptr(), ws.data_ptr(), ws.numel(),
+                                        None)
+    assert st == 0, lib.atdf_last_error()
+    torch.cuda.synchronize()
+    if D == 3:
+        rg, ru = oracle.grid_sample3d_bwd(grid, index, uv, go)
+    elif D == 2:
+        rg, ru = oracle.grid_sample2d_bwd(grid, index, uv, go)
+    else:
+        rg, ru = oracle.grid_sample1d_bwd(grid[..., None], index, uv, go)
+        rg = rg.reshape(grid.shape)
+    ru = np.asarray(ru).reshape(uv.shape)
+    print(f"  grad_grid {_rel(gg.cpu().double().numpy(), rg):.2e} grad_uv {_rel(gu.cpu().double().numpy(), ru):.2e}",
+          flush=True)
+    assert _rel(gg.cpu().double().numpy(), rg) <= 1e-5
+    assert _rel(gu.cpu().double().numpy(), ru) <= 1e-5
