@@ -254,6 +254,22 @@ PYBIND11_MODULE(TORCH_EXTENSION_NAME, m)
         .def_readwrite("Z", &FourierGrid::Z);
     m.def("MakeFourierGrid", &MakeFourierGrid, py::arg("tree"), py::arg("grid_size"), py::arg("tree_depth"),
           py::arg("roi_min"), py::arg("roi_max"), py::arg("params"));
+    m.def(
+        "AdamStep",
+        [](torch::Tensor param, torch::Tensor grad, torch::Tensor exp_avg, torch::Tensor exp_avg_sq, int64_t step,
+           float lr, float beta1, float beta2, float eps, bool zero_grad) {
+            for (const auto* t : {&param, &grad, &exp_avg, &exp_avg_sq})
+                TORCH_CHECK(t->is_cuda() && t->scalar_type() == torch::kFloat32 && t->is_contiguous() &&
+                                t->numel() == param.numel(),
+                            "AdamStep: contiguous f32 device tensors of one size required");
+            check_status(atdf_adam_step(param.data_ptr<float>(), grad.data_ptr<float>(), exp_avg.data_ptr<float>(),
+                                        exp_avg_sq.data_ptr<float>(), param.numel(), lr, beta1, beta2, eps, step,
+                                        zero_grad ? 1 : 0, current_stream()),
+                         "atdf_adam_step");
+        },
+        py::arg("param"), py::arg("grad"), py::arg("exp_avg"), py::arg("exp_avg_sq"), py::arg("step"),
+        py::arg("lr") = 1e-3f, py::arg("beta1") = 0.9f, py::arg("beta2") = 0.999f, py::arg("eps") = 1e-8f,
+        py::arg("zero_grad") = true);
     m.def("DecodeFeatures", &DecodeFeatures);
     m.def("SampleVolumeIndirect", &SampleVolumeIndirect);
     m.def("ComputeImageVM", &ComputeImageVM);

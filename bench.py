@@ -47,7 +47,8 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=100)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--mode", choices=["plain", "bf16", "learned"], default="learned")
+    ap.add_argument("--mode", choices=["plain", "bf16", "learned", "vm"], default="learned",
+                    help="vm: the tensorial (plane x line) field through the composed operator path")
     ap.add_argument("--train-iters", type=int, default=300,
                     help="learned mode: iterations each leg trains before the projection-MSE check")
     ap.add_argument("--mix", type=float, default=0.3, help="learned mode: uniform mixing of the proposal pdf")
@@ -148,6 +149,9 @@ def main():
 
     steps = args.profile_steps if args.profile_steps > 0 else args.steps
 
+    if args.mode == "vm":
+        vm_leg(N, tree, batches, K, s_eff, C, G, H, depth, n, args, steps)
+        return
     if args.mode != "learned":
         bf16 = args.mode == "bf16"
         leg = run_leg(N, tree, batches, sp, field(bf16), grid0, mlp0, bf16, None, args, steps)
@@ -248,6 +252,76 @@ def main():
         out["plain_fp32"] = plain_summary
         out["projection_mse"] = mse
     print(json.dumps(out), flush=True)
+
+
+def vm_leg(N, tree, batches, K, s_eff, C, G, H, depth, n, args, steps):
+    """Training step of the tensorial geometry (GeometryTensorQO) through the
+    composed operators: CreateSamplesForRays, ComputeImageVM (VM gathers for
+    in-ROI nodes, explicit grids for the rest, DecodeFeatures, integration),
+    MSE, autograd backward (slot-major gather adjoints, decoder adjoint),
+    Adam on every field.  One JSON line of its own (not the headline)."""
+    import torch
+
+    dev = torch.device("cuda:0")
+    lo, hi = [-0.6, -0.6, -0.6], [0.6, 0.6, 0.6]
+    roi = tree.RoiSplit(lo, hi, 0.0)
+    n_in, n_out = int(roi.in_roi_node_index.numel()), int(roi.out_roi_node_index.numel())
+    line = (G - 1) * (1 << depth) + 1
+    f = N.VMFields()
+    gen = torch.Generator(device="cpu").manual_seed(0)
+
+    def param(*shape):
+        return (torch.randn(*shape, generator=gen) * 0.01).to(dev).requires_grad_()
+
+    f.plane_yz, f.plane_xz, f.plane_xy = (param(n_in, C, G, G) for _ in range(3))
+    f.line_x, f.line_y, f.line_z = (param(1, C, line, 1) for _ in range(3))
+    f.grid_out = param(n_out, C, G, G, G)
+    f.tensor_op = "add"
+    f.roi_min, f.roi_max = lo, hi
+    from adaptivetensordf_amd import phantom
+
+    mlp = phantom.init_mlp(C, H, seed=0).to(dev).requires_grad_()
+    fp = N.FieldParams()
+    fp.C, fp.G, fp.H = C, G, H
+    params = [f.plane_yz, f.plane_xz, f.plane_xy, f.line_x, f.line_y, f.line_z, f.grid_out, mlp]
+    moments = [(torch.zeros_like(p), torch.zeros_like(p)) for p in params]
+    state = {"it": 0, "loss": None}
+
+    def step():
+        rays, tgt = batches[state["it"] % len(batches)]
+        samples = tree.CreateSamplesForRays(rays, K, True)
+        img = N.ComputeImageVM(tree, roi, f, mlp, fp, samples, rays.size())
+        loss = (img[0] - tgt).square().mean()
+        grads = torch.autograd.grad(loss, params)
+        state["it"] += 1
+        for p, g, (m, v) in zip(params, grads, moments):
+            N.AdamStep(p.data, g.contiguous(), m, v, state["it"], 1e-3)
+        state["loss"] = loss.detach()
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    first = float(state["loss"].item()) if state["loss"] is not None else None
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        step()
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    R = batches[0][0].size()
+    log(f"vm: {steps} steps in {el:.3f}s: {el * 1e3 / steps:.2f} ms/step, {R * steps / el / 1e6:.3f} M rays/s, "
+        f"{n_in} VM / {n_out} explicit nodes")
+    print(json.dumps({
+        "metric": "fwd+bwd rays/s, tensorial (VM) field through the composed operators, 512^3, 61 tilts",
+        "value": R * steps / el, "unit": "rays/s", "n_gpus": 1, "steps": steps, "warmup": args.warmup,
+        "ms_per_step": el * 1e3 / steps, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32", "data": "synthetic (analytic ellipsoid phantom projections, random-init field)",
+        "config": {"workload": f"tensorial geometry training step, {n}^3 phantom, 61 tilts, {n}x{n} detector, "
+                               f"adaptive octree depth<=5, ROI {lo}..{hi}: {n_in} plane/line nodes, {n_out} "
+                               f"explicit-grid nodes, lines of {line}",
+                   "rays_per_batch": R, "K": K, "stratified_samples_per_ray": round(s_eff, 2), "channels": C,
+                   "grid": G, "decoder": f"{C}-{H}-{H}-1", "parallelism": "single GPU",
+                   "loss_first_last": [first, float(state["loss"].item())]},
+        "roofline": None, "cpu_baseline": None}), flush=True)
 
 
 def run_leg(N, tree, batches, sp0, fp, grid0, mlp0, bf16, ps, args, steps):
