@@ -31,23 +31,26 @@ __global__ __launch_bounds__(256) void grid_sample3d_fwd_kernel(const float* __r
                                                                 const float* __restrict__ uv, int64_t S,
                                                                 float* __restrict__ out)
 {
-    int64_t s = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    if (s >= S) return;
+    // one lane per (sample, channel): the eight corner loads of a channel
+    // plane are independent of the other channels', so spreading the
+    // channels over lanes keeps C times more loads in flight than a lane
+    // walking all channels (the reference layout puts them a plane apart)
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= S * C) return;
+    const int64_t s = i / C;
+    const int c = (int)(i - s * C);
     Tri t;
     tri_setup(uv[s * 3 + 0], uv[s * 3 + 1], uv[s * 3 + 2], Z, Y, X, t);
     const int64_t plane = (int64_t)Z * Y * X;
-    const float* g = grid + index[s] * C * plane;
+    const float* gc = grid + (index[s] * C + c) * plane;
     const int64_t off[8] = {((int64_t)t.z0 * Y + t.y0) * X + t.x0, ((int64_t)t.z0 * Y + t.y0) * X + t.x1,
                             ((int64_t)t.z0 * Y + t.y1) * X + t.x0, ((int64_t)t.z0 * Y + t.y1) * X + t.x1,
                             ((int64_t)t.z1 * Y + t.y0) * X + t.x0, ((int64_t)t.z1 * Y + t.y0) * X + t.x1,
                             ((int64_t)t.z1 * Y + t.y1) * X + t.x0, ((int64_t)t.z1 * Y + t.y1) * X + t.x1};
-    for (int c = 0; c < C; ++c) {
-        const float* gc = g + c * plane;
-        float acc = 0;
+    float acc = 0;
 #pragma unroll
-        for (int k = 0; k < 8; ++k) acc += gc[off[k]] * t.w[k];
-        out[s * C + c] = acc;
-    }
+    for (int k = 0; k < 8; ++k) acc += gc[off[k]] * t.w[k];
+    out[s * C + c] = acc;
 }
 
 __global__ __launch_bounds__(256) void grid_sample3d_bwd_kernel(const float* __restrict__ grid, int C, int Z, int Y,
@@ -125,21 +128,21 @@ __global__ __launch_bounds__(256) void grid_sample2d_fwd_kernel(const float* __r
                                                                 const float* __restrict__ uv, int64_t S,
                                                                 float* __restrict__ out)
 {
-    const int64_t s = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    if (s >= S) return;
+    // one lane per (sample, channel), as the 3-D forward
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= S * C) return;
+    const int64_t s = i / C;
+    const int c = (int)(i - s * C);
     Bil b;
     bil_setup(uv[s * 2], uv[s * 2 + 1], H, W, b);
     const int64_t plane = (int64_t)H * W;
-    const float* g = grid + index[s] * C * plane;
+    const float* gc = grid + (index[s] * C + c) * plane;
     const int64_t off[4] = {(int64_t)b.y0 * W + b.x0, (int64_t)b.y0 * W + b.x1, (int64_t)b.y1 * W + b.x0,
                             (int64_t)b.y1 * W + b.x1};
-    for (int c = 0; c < C; ++c) {
-        const float* gc = g + c * plane;
-        float acc = 0.f;
+    float acc = 0.f;
 #pragma unroll
-        for (int k = 0; k < 4; ++k) acc += gc[off[k]] * b.w[k];
-        out[s * C + c] = acc;
-    }
+    for (int k = 0; k < 4; ++k) acc += gc[off[k]] * b.w[k];
+    out[s * C + c] = acc;
 }
 
 // Backward: atomics into the clipped corners (IndirectGridSample2D.cu:175-393);
@@ -438,8 +441,8 @@ atdf_status atdf_grid_sample3d_fwd(const float* grid, int64_t N, int32_t C, int3
     ATDF_REQUIRE(N > 0 && C > 0 && Z > 0 && Y > 0 && X > 0 && S >= 0, "atdf_grid_sample3d_fwd: bad shape");
     if (S == 0) return ATDF_OK;
     ATDF_REQUIRE(grid && index && uv && out, "atdf_grid_sample3d_fwd: null buffer");
-    hipLaunchKernelGGL(grid_sample3d_fwd_kernel, dim3(blocks_for(S)), dim3(256), 0, to_hip(stream), grid, C, Z, Y, X,
-                       index, uv, S, out);
+    hipLaunchKernelGGL(grid_sample3d_fwd_kernel, dim3(blocks_for(S * C)), dim3(256), 0, to_hip(stream), grid, C, Z, Y,
+                       X, index, uv, S, out);
     return check_launch("atdf_grid_sample3d_fwd");
 }
 
@@ -461,7 +464,7 @@ atdf_status atdf_grid_sample2d_fwd(const float* grid, int64_t N, int32_t C, int3
     ATDF_REQUIRE(N > 0 && C > 0 && H > 0 && W > 0 && S >= 0, "atdf_grid_sample2d_fwd: bad shape");
     if (S == 0) return ATDF_OK;
     ATDF_REQUIRE(grid && index && uv && out, "atdf_grid_sample2d_fwd: null buffer");
-    hipLaunchKernelGGL(grid_sample2d_fwd_kernel, dim3(blocks_for(S)), dim3(256), 0, to_hip(stream), grid, C, H, W,
+    hipLaunchKernelGGL(grid_sample2d_fwd_kernel, dim3(blocks_for(S * C)), dim3(256), 0, to_hip(stream), grid, C, H, W,
                        index, uv, S, out);
     return check_launch("atdf_grid_sample2d_fwd");
 }
