@@ -1874,7 +1874,11 @@ __global__ __launch_bounds__(256) void grid_adam_rest_kernel(GridAdam ad, float*
 
 using namespace atdf;
 
-static int64_t rays_per_wave(const atdf_field* f) { return f->rays_per_wave > 0 ? f->rays_per_wave : 64; }
+// 16 rays per wave by default: with every wave of a 2^18-ray batch resident
+// at once, 64 rays per wave left the waves holding the longest ray lists
+// running alone at the end (512^3 workload, BF16 forward: 1.30 -> 1.17 ms at
+// ~39 samples/ray, 3.48 -> 3.03 ms at ~128)
+static int64_t rays_per_wave(const atdf_field* f) { return f->rays_per_wave > 0 ? f->rays_per_wave : 16; }
 
 static atdf_status check_project_args(const void* packed_tree, int64_t num_nodes, const float* O, const float* D,
                                       int64_t R, const atdf_sampling* sp, const int64_t* iv_off,

@@ -258,7 +258,7 @@ typedef struct {
     int32_t G;                   /* grid edge, e.g. 11                        */
     int32_t grid_dtype;          /* ATDF_GRID_F32 / ATDF_GRID_BF16            */
     float softplus_beta;         /* 2 in the reference (Settings.h:164)       */
-    int32_t rays_per_wave;       /* work unit of one wavefront (0 = default)  */
+    int32_t rays_per_wave;       /* work unit of one wavefront (0 = 16)       */
     const void* grid;            /* [num_grids, G, G, G, C]                   */
     int64_t num_grids;
     const int32_t* node_to_grid; /* [num_nodes] -> grid slot (-1 = none)      */
