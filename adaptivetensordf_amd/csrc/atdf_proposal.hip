@@ -668,18 +668,33 @@ __global__ __launch_bounds__(256) void grad_partials_kernel(const float* __restr
 }
 
 // grad[p] = sum over chunks (ascending, FP64) of the partials; loss likewise / R.
+// A workgroup owns 16 entries; its 16 row groups (chunks c = g, g + 16, ...)
+// sum in FP64 registers and the group sums are added in group order, so the
+// order is fixed and 16 times more lanes share the chunk loop than with one
+// lane per entry.
+constexpr int kRedEntries = 16;
+
 __global__ __launch_bounds__(256) void grad_reduce_kernel(const float* __restrict__ partials, int64_t nchunks,
                                                           int64_t R, float* __restrict__ grad,
                                                           double* __restrict__ loss)
 {
-    const int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p > NPARAM) return;
+    __shared__ double part[16][kRedEntries + 1];
+    const int pl = threadIdx.x % kRedEntries, rg = threadIdx.x / kRedEntries;
+    const int p = blockIdx.x * kRedEntries + pl;
     double acc = 0.0;
-    for (int64_t c = 0; c < nchunks; ++c) acc += (double)partials[c * (NPARAM + 1) + p];
-    if (p < NPARAM)
-        grad[p] = (float)acc;
-    else
-        loss[0] = acc / (double)R;
+    if (p <= NPARAM)
+        for (int64_t c = rg; c < nchunks; c += 16) acc += (double)partials[c * (NPARAM + 1) + p];
+    part[rg][pl] = acc;
+    __syncthreads();
+    if (rg == 0 && p <= NPARAM) {
+        double t = 0.0;
+#pragma unroll
+        for (int k = 0; k < 16; ++k) t += part[k][pl];
+        if (p < NPARAM)
+            grad[p] = (float)t;
+        else
+            loss[0] = t / (double)R;
+    }
 }
 
 size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
@@ -859,7 +874,7 @@ atdf_status atdf_proposal_grad(const float* params, const float* features, const
     GradArgs a{params, features, target_bins, num_rays, target_power, cols};
     hipLaunchKernelGGL(grad_rays_kernel, dim3((unsigned)div_up(num_rays, kThreads)), dim3(kThreads), kMlpLds, s, a);
     hipLaunchKernelGGL(grad_partials_kernel, dim3((unsigned)nchunks, 4), dim3(256), 0, s, cols, num_rays, partials);
-    hipLaunchKernelGGL(grad_reduce_kernel, dim3((unsigned)div_up(NPARAM + 1, 256)), dim3(256), 0, s, partials,
+    hipLaunchKernelGGL(grad_reduce_kernel, dim3((unsigned)div_up(NPARAM + 1, kRedEntries)), dim3(256), 0, s, partials,
                        nchunks, num_rays, grad, loss);
     return check_launch("atdf_proposal_grad");
 }
