@@ -105,6 +105,7 @@ _SIGNATURES = {
     "atdf_version": (C.c_char_p, []),
     "atdf_tree_packed_bytes": (_SZ, [_I64]),
     "atdf_tree_pack": (_I32, [C.POINTER(Tree), _P, _P]),
+    "atdf_tree_pack_box": (_I32, [C.POINTER(C.c_float), C.POINTER(C.c_float), _P, _P]),
     "atdf_traverse_count": (_I32, [_P, _I64, _P, _P, _I64, C.POINTER(Sampling), _P, _P, _P, _P]),
     "atdf_scan_workspace_bytes": (_SZ, [_I64]),
     "atdf_exclusive_scan_i32": (_I32, [_P, _I64, _P, _P, _SZ, _P]),

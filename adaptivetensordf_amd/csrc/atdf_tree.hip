@@ -481,6 +481,35 @@ atdf_status atdf_tree_pack(const atdf_tree* t, void* packed, atdf_stream_t strea
     return check_launch("atdf_tree_pack");
 }
 
+__global__ void pack_box_kernel(float x0, float y0, float z0, float x1, float y1, float z1, PNode* out)
+{
+    if (threadIdx.x != 0) return;
+    PNode p;
+    p.mn[0] = x0;
+    p.mn[1] = y0;
+    p.mn[2] = z0;
+    p.mx[0] = x1;
+    p.mx[1] = y1;
+    p.mx[2] = z1;
+    p.first_child = -1;
+    p.parent = -1;
+    p.flags = kFlagActive;
+    p.pad[0] = p.pad[1] = p.pad[2] = 0;
+    out[0] = p;
+}
+
+atdf_status atdf_tree_pack_box(const float* roi_min, const float* roi_max, void* packed, atdf_stream_t stream)
+{
+    ATDF_REQUIRE(roi_min && roi_max && packed, "atdf_tree_pack_box: null argument");
+    for (int k = 0; k < 3; ++k)
+        ATDF_REQUIRE(std::isfinite(roi_min[k]) && std::isfinite(roi_max[k]) && roi_min[k] < roi_max[k],
+                     "atdf_tree_pack_box: axis %d: roi_min %g must be below roi_max %g", k, (double)roi_min[k],
+                     (double)roi_max[k]);
+    hipLaunchKernelGGL(pack_box_kernel, dim3(1), dim3(64), 0, to_hip(stream), roi_min[0], roi_min[1], roi_min[2],
+                       roi_max[0], roi_max[1], roi_max[2], reinterpret_cast<PNode*>(packed));
+    return check_launch("atdf_tree_pack_box");
+}
+
 static atdf_status check_sampling(const atdf_sampling* sp)
 {
     ATDF_REQUIRE(sp, "sampling parameters are null");

@@ -70,6 +70,8 @@ PYBIND11_MODULE(TORCH_EXTENSION_NAME, m)
         .def("ActiveNodeTensor", &HyperTreeBase::ActiveNodeTensor)
         .def("CreateSamplesForRays", &HyperTreeBase::CreateSamplesForRays)
         .def("CreateIntervalsForRays", &HyperTreeBase::CreateIntervalsForRays)
+        .def("CreateSamplesForRays_area", &HyperTreeBase::CreateSamplesForRays_area)
+        .def("CreateIntervalsForRays_area", &HyperTreeBase::CreateIntervalsForRays_area)
         .def("ComputeLocalSamples", &HyperTreeBase::ComputeLocalSamples)
         .def("SetErrorForActiveNodes", &HyperTreeBase::SetErrorForActiveNodes)
         .def("UpdateCulling", &HyperTreeBase::UpdateCulling)

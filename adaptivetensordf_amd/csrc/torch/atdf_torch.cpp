@@ -208,11 +208,11 @@ struct TraversalCounts {
     int max_per_ray = 0;
 };
 
-TraversalCounts count_and_scan(HyperTreeBase& tree, const RayList& rays, const atdf_sampling& s)
+TraversalCounts count_and_scan(const void* packed, int64_t num_nodes, torch::Device dev, const RayList& rays,
+                               const atdf_sampling& s)
 {
     TraversalCounts tc;
     const int64_t R = rays.size();
-    auto dev = tree.device();
     auto i32 = torch::TensorOptions().dtype(torch::kInt32).device(dev);
     auto i64 = torch::TensorOptions().dtype(torch::kLong).device(dev);
     tc.num_iv = torch::empty({R}, i32);
@@ -222,7 +222,7 @@ TraversalCounts count_and_scan(HyperTreeBase& tree, const RayList& rays, const a
     auto status = torch::zeros({4}, i32);
     auto st = current_stream();
     if (R > 0) {
-        check_status(atdf_traverse_count(tree.packed(), tree.NumNodes(), rays.origin.data_ptr<float>(),
+        check_status(atdf_traverse_count(packed, num_nodes, rays.origin.data_ptr<float>(),
                                          rays.direction.data_ptr<float>(), R, &s, tc.num_iv.data_ptr<int32_t>(),
                                          tc.num_s.data_ptr<int32_t>(), status.data_ptr<int32_t>(), st),
                      "atdf_traverse_count");
@@ -258,12 +258,13 @@ void check_rays(const RayList& rays)
 }
 }  // namespace
 
-Intervals HyperTreeBase::CreateIntervalsForRays(const RayList& rays, const SamplingParams& sp)
+Intervals HyperTreeBase::intervals_for(const void* packed, int64_t num_nodes, const RayList& rays,
+                                       const SamplingParams& sp) const
 {
     check_rays(rays);
     TORCH_CHECK(sp.jitter_mode != 1, "CreateIntervalsForRays: jitter_mode 1 needs materialised samples");
     atdf_sampling s = sp.view();
-    auto tc = count_and_scan(*this, rays, s);
+    auto tc = count_and_scan(packed, num_nodes, device(), rays, s);
     Intervals iv;
     auto dev = device();
     auto i32 = torch::TensorOptions().dtype(torch::kInt32).device(dev);
@@ -282,7 +283,7 @@ Intervals HyperTreeBase::CreateIntervalsForRays(const RayList& rays, const Sampl
     iv.ray = torch::empty({H}, i32);
     if (H > 0) {
         atdf_intervals v = iv.view();
-        check_status(atdf_traverse_write(packed(), NumNodes(), rays.origin.data_ptr<float>(),
+        check_status(atdf_traverse_write(packed, num_nodes, rays.origin.data_ptr<float>(),
                                          rays.direction.data_ptr<float>(), rays.size(), &s,
                                          iv.ray_offset.data_ptr<int64_t>(), tc.smp_off.data_ptr<int64_t>(), &v,
                                          nullptr, current_stream()),
@@ -291,14 +292,39 @@ Intervals HyperTreeBase::CreateIntervalsForRays(const RayList& rays, const Sampl
     return iv;
 }
 
-SampleList HyperTreeBase::CreateSamplesForRays(const RayList& rays, int max_samples_per_node, bool interval_jitter)
+Intervals HyperTreeBase::CreateIntervalsForRays(const RayList& rays, const SamplingParams& sp)
+{
+    return intervals_for(packed(), NumNodes(), rays, sp);
+}
+
+namespace {
+Tensor pack_box(const std::vector<float>& roi_min, const std::vector<float>& roi_max, torch::Device dev)
+{
+    TORCH_CHECK(roi_min.size() == 3 && roi_max.size() == 3, "roi_min and roi_max need 3 entries");
+    auto packed = torch::empty({(int64_t)atdf_tree_packed_bytes(1)},
+                               torch::TensorOptions().dtype(torch::kUInt8).device(dev));
+    check_status(atdf_tree_pack_box(roi_min.data(), roi_max.data(), packed.data_ptr(), current_stream()),
+                 "atdf_tree_pack_box");
+    return packed;
+}
+}  // namespace
+
+Intervals HyperTreeBase::CreateIntervalsForRays_area(const RayList& rays, const SamplingParams& sp,
+                                                     std::vector<float> roi_min, std::vector<float> roi_max)
+{
+    auto box = pack_box(roi_min, roi_max, device());
+    return intervals_for(box.data_ptr(), 1, rays, sp);
+}
+
+SampleList HyperTreeBase::samples_for(const void* packed, int64_t num_nodes, const RayList& rays,
+                                      int max_samples_per_node, bool interval_jitter) const
 {
     check_rays(rays);
     SamplingParams sp;
     sp.max_samples_per_node = max_samples_per_node;
     sp.jitter_mode = 0;
     atdf_sampling s = sp.view();
-    auto tc = count_and_scan(*this, rays, s);
+    auto tc = count_and_scan(packed, num_nodes, device(), rays, s);
     const int64_t S = tc.total_s;
     auto dev = device();
     auto f32 = torch::TensorOptions().dtype(torch::kFloat32).device(dev);
@@ -322,12 +348,24 @@ SampleList HyperTreeBase::CreateSamplesForRays(const RayList& rays, int max_samp
         atdf_samples v{list.global_coordinate.data_ptr<float>(), list.weight.data_ptr<float>(),
                        list.ray_t.data_ptr<float>(),              list.ray_index.data_ptr<int64_t>(),
                        list.local_index_in_ray.data_ptr<int64_t>(), list.node_id.data_ptr<int64_t>()};
-        check_status(atdf_traverse_write(packed(), NumNodes(), rays.origin.data_ptr<float>(),
+        check_status(atdf_traverse_write(packed, num_nodes, rays.origin.data_ptr<float>(),
                                          rays.direction.data_ptr<float>(), rays.size(), &s, tc.iv_off.data_ptr<int64_t>(),
                                          tc.smp_off.data_ptr<int64_t>(), nullptr, &v, current_stream()),
                      "atdf_traverse_write");
     }
     return list;
+}
+
+SampleList HyperTreeBase::CreateSamplesForRays(const RayList& rays, int max_samples_per_node, bool interval_jitter)
+{
+    return samples_for(packed(), NumNodes(), rays, max_samples_per_node, interval_jitter);
+}
+
+SampleList HyperTreeBase::CreateSamplesForRays_area(const RayList& rays, int max_samples_per_node, bool interval_jitter,
+                                                    std::vector<float> roi_min, std::vector<float> roi_max)
+{
+    auto box = pack_box(roi_min, roi_max, device());
+    return samples_for(box.data_ptr(), 1, rays, max_samples_per_node, interval_jitter);
 }
 
 Tensor HyperTreeBase::ComputeLocalSamples(Tensor global_samples, Tensor node_indices)

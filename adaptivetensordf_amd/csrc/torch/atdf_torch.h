@@ -95,6 +95,13 @@ class HyperTreeBase {
     // HyperTree.cu:516-610
     SampleList CreateSamplesForRays(const RayList& rays, int max_samples_per_node, bool interval_jitter);
     Intervals CreateIntervalsForRays(const RayList& rays, const SamplingParams& sp);
+    // HyperTree.cu:818-883 (ComputeRaySamples_area, 651-816): one box, the
+    // dataset ROI, for depth-0 trees (samples/hyper_train.cpp:921-927); every
+    // sample has node id 0, so local coordinates stay relative to node 0's box
+    SampleList CreateSamplesForRays_area(const RayList& rays, int max_samples_per_node, bool interval_jitter,
+                                         std::vector<float> roi_min, std::vector<float> roi_max);
+    Intervals CreateIntervalsForRays_area(const RayList& rays, const SamplingParams& sp, std::vector<float> roi_min,
+                                          std::vector<float> roi_max);
     // HyperTree.cu:237-259
     torch::Tensor ComputeLocalSamples(torch::Tensor global_samples, torch::Tensor node_indices);
     // HyperTree.cpp:607-647
@@ -153,6 +160,9 @@ class HyperTreeBase {
 
    private:
     void repack();
+    Intervals intervals_for(const void* packed, int64_t num_nodes, const RayList& rays, const SamplingParams& sp) const;
+    SampleList samples_for(const void* packed, int64_t num_nodes, const RayList& rays, int max_samples_per_node,
+                           bool interval_jitter) const;
     SampleList neighbor_samples(std::vector<int> size, double epsilon, std::vector<float> roi_min,
                                 std::vector<float> roi_max, bool use_roi, int variant) const;
     torch::Tensor packed_;

@@ -82,6 +82,18 @@ typedef struct {
                                      (atdf_compact_rays builds the list)        */
 } atdf_sampling;
 
+/* The single box of HyperTreeBaseImpl::CreateSamplesForRays_area /
+ * ComputeRaySamples_area (structure/HyperTree.cu:651-883), used by
+ * hyper_train for depth-0 trees with the dataset ROI
+ * (samples/hyper_train.cpp:921-927): writes a packed one-node tree whose
+ * node 0 is the box [roi_min, roi_max] (host arrays of 3 floats), active,
+ * without children (atdf_tree_packed_bytes(1) bytes).  atdf_traverse_count /
+ * atdf_traverse_write over it with num_nodes = 1 place that kernel's
+ * samples: one interval per ray, node id 0, n = ceil((tmax - tmin) /
+ * |roi_max - roi_min| * K) <= K, the same slab test, grazing-interval drop,
+ * jitter and midpoint weights. */
+atdf_status atdf_tree_pack_box(const float* roi_min, const float* roi_max, void* packed, atdf_stream_t stream);
+
 /* status[0] += rays over max_intersections, status[1] += order-check
  * failures, status[2] = max(samples per ray).  status is device int32[4],
  * zeroed by the caller. */

@@ -794,86 +794,144 @@ void atdfo_project_min_preact(const atdfo_model* m, const float* ray_o, const fl
     free(xf);
 }
 
-/* FP64 adjoint of atdfo_project_fwd(precision=1), sequential over rays. */
-int atdfo_project_bwd(const atdfo_model* m, const float* ray_o, const float* ray_d, int64_t R,
-                      const double* dLdP, double* grad_grid, double* grad_mlp)
+/* FP64 adjoint of one ray: per sample, the grid base offset, the trilinear
+ * record and dL/dfeature (dx, C values), plus the ray's decoder-parameter
+ * gradient (added to gm).  Returns the sample count, or -1 on overflow. */
+typedef struct {
+    int64_t base;
+    tri_d t;
+} bwd_rec_t;
+
+static int bwd_ray(const atdfo_model* m, int64_t r, const float* o, const float* d, double dLdP, hit_t* hits,
+                   sample_t_* smp, bwd_rec_t** rec_buf, double** dx_buf, int* cap, double* gm, double* work)
 {
-    int C = m->C, H = m->H, G = m->G;
-    hit_t* hits = (hit_t*)malloc(sizeof(hit_t) * ORACLE_HIT_CAP);
-    sample_t_* smp = (sample_t_*)malloc(sizeof(sample_t_) * ORACLE_SAMPLE_CAP);
-    double* x = (double*)malloc(sizeof(double) * C);
-    double* h1 = (double*)malloc(sizeof(double) * H);
-    double* h2 = (double*)malloc(sizeof(double) * H);
-    double* dh2 = (double*)malloc(sizeof(double) * H);
-    double* dh1 = (double*)malloc(sizeof(double) * H);
-    double* dx = (double*)malloc(sizeof(double) * C);
+    int C = m->C, H = m->H;
+    int ns = ray_samples(m, r, o, d, hits, smp, ORACLE_SAMPLE_CAP);
+    if (ns < 0) return -1;
+    if (ns > *cap) {
+        *cap = ns;
+        *rec_buf = (bwd_rec_t*)realloc(*rec_buf, sizeof(bwd_rec_t) * (size_t)ns);
+        *dx_buf = (double*)realloc(*dx_buf, sizeof(double) * (size_t)ns * C);
+    }
+    bwd_rec_t* rec = *rec_buf;
+    double* dxs = *dx_buf;
+    double* x = work;
+    double* h1 = x + C;
+    double* h2 = h1 + H;
+    double* dh2 = h2 + H;
+    double* dh1 = dh2 + H;
     const float* W1 = m->mlp;
     const float* W2 = W1 + H * C + H;
     const float* W3 = W2 + H * H + H;
-    double* gW1 = grad_mlp;
+    double* gW1 = gm;
     double* gb1 = gW1 + H * C;
     double* gW2 = gb1 + H;
     double* gb2 = gW2 + H * H;
     double* gW3 = gb2 + H;
     double* gb3 = gW3 + H;
-    int err = 0;
-    for (int64_t r = 0; r < R; ++r) {
-        int ns = ray_samples(m, r, ray_o + r * 3, ray_d + r * 3, hits, smp, ORACLE_SAMPLE_CAP);
-        if (ns < 0) {
-            err = 1;
-            continue;
+    const int bf = m->decoder_bf16;
+    for (int s = 0; s < ns; ++s) {
+        double yp;
+        gather_cl_d(m, &smp[s], x, &rec[s].t, &rec[s].base);
+        if (bf) {
+            float xf[64];
+            gather_cl_f(m, &smp[s], xf);
+            for (int c = 0; c < C; ++c) x[c] = xf[c];
         }
-        for (int s = 0; s < ns; ++s) {
-            tri_d t;
-            int64_t base;
-            double yp;
-            const int bf = m->decoder_bf16;
-            gather_cl_d(m, &smp[s], x, &t, &base);
-            if (bf) {
-                float xf[64];
-                gather_cl_f(m, &smp[s], xf);
-                for (int c = 0; c < C; ++c) x[c] = xf[c];
+        mlp_fwd_one_d(m->mlp, C, H, m->beta, x, h1, h2, &yp, bf);
+        double dy = dLdP * (double)smp[s].w * softplus_grad_d(yp, m->beta);
+        for (int i = 0; i < H; ++i) {
+            gW3[i] += dy * h2[i];
+            dh2[i] = (h2[i] > 0) ? dy * (double)W3[i] : 0;
+        }
+        gb3[0] += dy;
+        for (int i = 0; i < H; ++i) dh1[i] = 0;
+        for (int i = 0; i < H; ++i) {
+            if (dh2[i] == 0) continue;
+            gb2[i] += dh2[i];
+            for (int k = 0; k < H; ++k) {
+                gW2[i * H + k] += rb(bf, dh2[i]) * rb(bf, h1[k]);
+                dh1[k] += rb(bf, dh2[i]) * rb(bf, W2[i * H + k]);
             }
-            mlp_fwd_one_d(m->mlp, C, H, m->beta, x, h1, h2, &yp, bf);
-            double dy = dLdP[r] * (double)smp[s].w * softplus_grad_d(yp, m->beta);
-            for (int i = 0; i < H; ++i) {
-                gW3[i] += dy * h2[i];
-                dh2[i] = (h2[i] > 0) ? dy * (double)W3[i] : 0;
-            }
-            gb3[0] += dy;
-            for (int i = 0; i < H; ++i) dh1[i] = 0;
-            for (int i = 0; i < H; ++i) {
-                if (dh2[i] == 0) continue;
-                gb2[i] += dh2[i];
-                for (int k = 0; k < H; ++k) {
-                    gW2[i * H + k] += rb(bf, dh2[i]) * rb(bf, h1[k]);
-                    dh1[k] += rb(bf, dh2[i]) * rb(bf, W2[i * H + k]);
-                }
-            }
-            for (int c = 0; c < C; ++c) dx[c] = 0;
-            for (int i = 0; i < H; ++i) {
-                if (!(h1[i] > 0)) continue;
-                gb1[i] += dh1[i];
-                for (int k = 0; k < C; ++k) {
-                    gW1[i * C + k] += rb(bf, dh1[i]) * rb(bf, x[k]);
-                    dx[k] += rb(bf, dh1[i]) * rb(bf, W1[i * C + k]);
-                }
-            }
-            for (int k = 0; k < 8; ++k) {
-                int bx = k & 1, by = (k >> 1) & 1, bz = (k >> 2) & 1;
-                int64_t off = base + (((int64_t)t.z[bz] * G + t.y[by]) * G + t.x[bx]) * C;
-                for (int c = 0; c < C; ++c) grad_grid[off + c] += t.w[k] * dx[c];
+        }
+        double* dx = dxs + (int64_t)s * C;
+        for (int c = 0; c < C; ++c) dx[c] = 0;
+        for (int i = 0; i < H; ++i) {
+            if (!(h1[i] > 0)) continue;
+            gb1[i] += dh1[i];
+            for (int k = 0; k < C; ++k) {
+                gW1[i * C + k] += rb(bf, dh1[i]) * rb(bf, x[k]);
+                dx[k] += rb(bf, dh1[i]) * rb(bf, W1[i * C + k]);
             }
         }
     }
-    free(hits);
-    free(smp);
-    free(x);
-    free(h1);
-    free(h2);
-    free(dh2);
-    free(dh1);
-    free(dx);
+    return ns;
+}
+
+/* FP64 adjoint of atdfo_project_fwd(precision=1).  Rays are evaluated in
+ * parallel in blocks; the grid scatter and the decoder-gradient sums run in
+ * ray order afterwards, so the result does not depend on the thread count. */
+#define BWD_BLOCK 64
+int atdfo_project_bwd(const atdfo_model* m, const float* ray_o, const float* ray_d, int64_t R,
+                      const double* dLdP, double* grad_grid, double* grad_mlp)
+{
+    int C = m->C, H = m->H, G = m->G;
+    int64_t NP = (int64_t)H * C + H + (int64_t)H * H + H + H + 1;
+    bwd_rec_t* rec[BWD_BLOCK];
+    double* dxs[BWD_BLOCK];
+    double* gms[BWD_BLOCK];
+    int cnt[BWD_BLOCK], cap[BWD_BLOCK];
+    for (int b = 0; b < BWD_BLOCK; ++b) {
+        cap[b] = 1024;
+        rec[b] = (bwd_rec_t*)malloc(sizeof(bwd_rec_t) * (size_t)cap[b]);
+        dxs[b] = (double*)malloc(sizeof(double) * (size_t)cap[b] * C);
+        gms[b] = (double*)malloc(sizeof(double) * NP);
+    }
+    int err = 0;
+    for (int64_t r0 = 0; r0 < R; r0 += BWD_BLOCK) {
+        int nb = (int)(R - r0 < BWD_BLOCK ? R - r0 : BWD_BLOCK);
+#ifdef _OPENMP
+#pragma omp parallel
+#endif
+        {
+            hit_t* hits = (hit_t*)malloc(sizeof(hit_t) * ORACLE_HIT_CAP);
+            sample_t_* smp = (sample_t_*)malloc(sizeof(sample_t_) * ORACLE_SAMPLE_CAP);
+            double* work = (double*)malloc(sizeof(double) * (C + 4 * H));
+#ifdef _OPENMP
+#pragma omp for schedule(dynamic, 1)
+#endif
+            for (int b = 0; b < nb; ++b) {
+                int64_t r = r0 + b;
+                for (int64_t i = 0; i < NP; ++i) gms[b][i] = 0;
+                cnt[b] = bwd_ray(m, r, ray_o + r * 3, ray_d + r * 3, dLdP[r], hits, smp, &rec[b], &dxs[b], &cap[b],
+                                 gms[b], work);
+            }
+            free(hits);
+            free(smp);
+            free(work);
+        }
+        for (int b = 0; b < nb; ++b) {
+            if (cnt[b] < 0) {
+                err = 1;
+                continue;
+            }
+            for (int64_t i = 0; i < NP; ++i) grad_mlp[i] += gms[b][i];
+            for (int s = 0; s < cnt[b]; ++s) {
+                const bwd_rec_t* q = &rec[b][s];
+                const double* dx = dxs[b] + (int64_t)s * C;
+                for (int k = 0; k < 8; ++k) {
+                    int bx = k & 1, by = (k >> 1) & 1, bz = (k >> 2) & 1;
+                    int64_t off = q->base + (((int64_t)q->t.z[bz] * G + q->t.y[by]) * G + q->t.x[bx]) * C;
+                    for (int c = 0; c < C; ++c) grad_grid[off + c] += q->t.w[k] * dx[c];
+                }
+            }
+        }
+    }
+    for (int b = 0; b < BWD_BLOCK; ++b) {
+        free(rec[b]);
+        free(dxs[b]);
+        free(gms[b]);
+    }
     return err ? -1 : 0;
 }
 

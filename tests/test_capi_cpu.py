@@ -45,6 +45,10 @@ def test_validation_without_device_work():
     st = lib.atdf_project_fwd(C.c_void_p(1), 9, None, None, 4, C.byref(sp), None, None, C.byref(f), None, None)
     assert st == capi.ERR_INVALID_ARGUMENT and b"channels" in lib.atdf_last_error()
     assert lib.atdf_tree_packed_bytes(10) == 480
+    lo, hi = (C.c_float * 3)(-0.5, 0.2, -1.0), (C.c_float * 3)(0.5, 0.2, 1.0)  # empty along y
+    st = lib.atdf_tree_pack_box(lo, hi, C.c_void_p(1), None)
+    assert st == capi.ERR_INVALID_ARGUMENT and b"axis 1" in lib.atdf_last_error()
+    assert lib.atdf_tree_pack_box(None, hi, C.c_void_p(1), None) == capi.ERR_INVALID_ARGUMENT
     assert lib.atdf_scan_workspace_bytes(1024) == 8
 
 

@@ -208,44 +208,46 @@ def test_bf16_grid_forward_error(oracle, cuda):
 @pytest.mark.parametrize("C_,grid_bf16", [(16, False), (16, True), (8, True)])
 def test_bf16_decoder_matches_emulating_oracle(oracle, cuda, C_, grid_bf16):
     """BF16 MFMA decoder (FP32 accumulation) against the oracle's FP64 path
-    with the same bf16 input rounding; agreement is limited only by
-    accumulation order and by values that land on a bf16 rounding boundary."""
-    from adaptivetensordf_amd import native
+    with the same bf16 operand rounding (the mode pinned in
+    test_bf16_adjoint_cpu.py).  Forward within 2e-4 (values that land on a
+    bf16 rounding boundary); decoder gradient per parameter block (L2 and
+    max) and grid gradient (max and 90th percentile) within the tolerances
+    of bf16_emulation, which every misplaced rounding point exceeds several
+    times on these same inputs.  The grid gradient is checked for BF16 grids
+    too, through the C ABI's f32 gradient buffer."""
+    from bf16_emulation import (DEC_GRAD_L2_TOL, DEC_GRAD_MAX_TOL, GRID_GRAD_MAX_TOL, GRID_GRAD_Q90_TOL, block_errors,
+                                gpu_case_arrays, gpu_case_dLdP, grad_errors)
+    from capi_util import Fused
 
     cs = make_case(oracle, C_=C_, seed=31 + C_, jitter_mode=2)
-    o_t, d_t = rays_to_device(cs["o"], cs["d"])
-    rays = native().RayList(o_t, d_t)
-    fp = cs["fp"]
-    fp.bf16_decoder = True
+    t, act, grid_h, mlp_h, o, d, K, js = gpu_case_arrays(oracle, C_, grid_bf16)
+    assert np.array_equal(o, cs["o"]) and np.array_equal(mlp_h, cs["mlp"].numpy())
+    o_t, d_t = rays_to_device(o, d)
     g = cs["grid"].cuda()
     if grid_bf16:
         g = g.to(torch.bfloat16)
-    grid_host = g.float().cpu().numpy()
-    gt = g.float().requires_grad_() if not grid_bf16 else g
-    mt = cs["mlp"].cuda().requires_grad_()
-    P = native().ProjectRays(cs["tree"], rays, gt if not grid_bf16 else g, mt, cs["sp"], fp)
-    fmb = oracle.FusedModel(cs["t"]["node_min"], cs["t"]["node_max"], cs["act"], grid_host, cs["mlp"].numpy(), C_,
-                            K=6, jitter_mode=2, seed=1234 + 31 + C_, decoder_bf16=True)
-    Pb = fmb.project(cs["o"], cs["d"], precision=1)
-    fm64 = oracle.FusedModel(cs["t"]["node_min"], cs["t"]["node_max"], cs["act"], grid_host, cs["mlp"].numpy(), C_,
-                             K=6, jitter_mode=2, seed=1234 + 31 + C_)
-    P64 = fm64.project(cs["o"], cs["d"], precision=1)
-    Pg = P.detach().cpu().double().numpy()
-    e_emul = rel_close(Pg, Pb, 1e-4)
-    e_64 = rel_close(Pg, P64, 1e-2)
+    assert np.array_equal(g.float().cpu().numpy(), grid_h)
+    mlp = cs["mlp"].cuda()
+    fu = Fused(cs["tree"], o_t, d_t, K, g, mlp, C_, jitter_seed=js, bf16_decoder=True)
+    P = fu.forward().cpu().double().numpy()
+    fmb = oracle.FusedModel(t["node_min"], t["node_max"], act, grid_h, mlp_h, C_, K=K, jitter_mode=2, seed=js,
+                            decoder_bf16=True)
+    Pb = fmb.project(o, d, precision=1)
+    fm64 = oracle.FusedModel(t["node_min"], t["node_max"], act, grid_h, mlp_h, C_, K=K, jitter_mode=2, seed=js)
+    P64 = fm64.project(o, d, precision=1)
+    e_emul = rel_close(P, Pb, 1e-4)
+    e_64 = rel_close(P, P64, 1e-2)
     print(f"C={C_} bf16 grid={grid_bf16}: rel err vs bf16-emulating FP64 {e_emul:.2e}, vs FP64 {e_64:.2e}", flush=True)
     assert e_emul <= 2e-4 and e_64 <= 5e-2
-    dLdP = np.random.default_rng(C_).normal(size=P.shape[0])
-    wrt = [mt] if grid_bf16 else [gt, mt]
-    grads = torch.autograd.grad(P, wrt, torch.from_numpy(dLdP).float().cuda())
-    ref_g, ref_m = fmb.project_bwd(cs["o"], cs["d"], dLdP)
-    em = np.abs(grads[-1].cpu().double().numpy() - ref_m).max() / np.abs(ref_m).max()
-    print(f"  decoder grad rel err {em:.2e}", flush=True)
-    assert em <= 2e-3
-    if not grid_bf16:
-        eg = np.abs(grads[0].cpu().double().numpy() - ref_g).max() / np.abs(ref_g).max()
-        print(f"  grid grad rel err {eg:.2e}", flush=True)
-        assert eg <= 2e-3
+    dLdP = gpu_case_dLdP(C_, o.shape[0])
+    gg, gm = fu.backward(torch.from_numpy(dLdP).float().cuda())
+    ref_g, ref_m = fmb.project_bwd(o, d, dLdP)
+    b_max, b_l2 = block_errors(gm.cpu().double().numpy(), ref_m, C_)
+    e_max, e_q90 = grad_errors(gg.cpu().double().numpy(), ref_g)
+    print(f"  decoder grad per block (W1 b1 W2 b2 W3 b3): max {' '.join(f'{e:.1e}' for e in b_max)}, "
+          f"L2 {' '.join(f'{e:.1e}' for e in b_l2)}; grid grad max {e_max:.2e} q90 {e_q90:.2e}", flush=True)
+    assert all(e <= tol for e, tol in zip(b_l2, DEC_GRAD_L2_TOL)) and max(b_max) <= DEC_GRAD_MAX_TOL
+    assert e_max <= GRID_GRAD_MAX_TOL and e_q90 <= GRID_GRAD_Q90_TOL
 
 
 @pytest.mark.parametrize("bf16", [False, True])

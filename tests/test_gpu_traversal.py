@@ -274,3 +274,56 @@ def test_listed_traversal_bit_exact(oracle, cuda, case):
     assert np.array_equal(g["tmin"].view(np.uint32), r["iv_tmin"].view(np.uint32))
     assert np.array_equal(g["tmax"].view(np.uint32), r["iv_tmax"].view(np.uint32))
     assert np.array_equal(g["ray"], np.repeat(listed, r["ray_num_intervals"]))
+
+
+def test_create_samples_for_rays_area(oracle, cuda):
+    """CreateSamplesForRays_area / CreateIntervalsForRays_area on a depth-0
+    tree with a non-root ROI box: samples and intervals bit-exact with the
+    oracle's one-box traversal, node id 0 throughout; the projection over
+    the box's intervals takes its local coordinates from node 0 (the root)
+    and matches the oracle over the same intervals."""
+    from test_oracle import ROI_MAX, ROI_MIN, area_traverse
+    from adaptivetensordf_amd import native, phantom
+
+    t = oracle.tree_build(0)
+    act = np.ones(1, np.int32)
+    tree = device_tree(0, act)
+    o1, d1 = random_rays(3000, 12, spread=0.9)
+    o2, d2 = tilt_rays(2000, 13)
+    o3, d3 = edge_case_rays()
+    o, d = np.concatenate([o1, o2, o3]), np.concatenate([d1, d2, d3])
+    rays = native().RayList(*rays_to_device(o, d))
+    K = 9
+    sl = tree.CreateSamplesForRays_area(rays, K, False, list(ROI_MIN), list(ROI_MAX))
+    r = area_traverse(oracle, o, d, K)
+    print(f"area: {sl.size()} samples on {int((r['ray_num_intervals'] > 0).sum())} rays", flush=True)
+    for k in ("ray_index", "local_index_in_ray", "node_id"):
+        assert np.array_equal(getattr(sl, k).cpu().numpy(), r[k])
+    for k in ("ray_t", "weight", "global_coordinate"):
+        assert np.array_equal(getattr(sl, k).cpu().numpy().view(np.uint32), r[k].view(np.uint32))
+    assert sl.max_samples_per_ray == r["ray_num_samples"].max() <= K
+    sp = native().SamplingParams()
+    sp.max_samples_per_node = K
+    sp.jitter_mode = 2
+    sp.jitter_seed = 31
+    iv = tree.CreateIntervalsForRays_area(rays, sp, list(ROI_MIN), list(ROI_MAX))
+    rj = area_traverse(oracle, o, d, K, jitter_mode=2, seed=31)
+    assert np.array_equal(iv.ray_offset.cpu().numpy(), rj["ray_interval_offset"])
+    assert np.array_equal(iv.node.cpu().numpy(), rj["iv_node"]) and np.all(rj["iv_node"] == 0)
+    assert np.array_equal(iv.count.cpu().numpy(), rj["iv_count"])
+    assert np.array_equal(iv.tmin.cpu().numpy().view(np.uint32), rj["iv_tmin"].view(np.uint32))
+    assert np.array_equal(iv.tmax.cpu().numpy().view(np.uint32), rj["iv_tmax"].view(np.uint32))
+    C_, G = 8, 9
+    grid = phantom.init_grid(1, G, C_, std=0.4, seed=2)
+    mlp = phantom.init_mlp(C_, 64, seed=2)
+    fp = native().FieldParams()
+    fp.C, fp.G = C_, G
+    P = native().ProjectIntervals(tree, rays, iv, grid.cuda(), mlp.cuda(), sp, fp).cpu().double().numpy()
+    fm = oracle.FusedModel(t["node_min"], t["node_max"], act, grid.numpy(), mlp.numpy(), C_, K=K, jitter_mode=2,
+                           seed=31)
+    fm.set_intervals(rj["ray_interval_offset"], rj["iv_node"], rj["iv_tmin"], rj["iv_tmax"], rj["iv_count"])
+    P64 = fm.project(o, d, precision=1)
+    scale = np.maximum(np.abs(P64), 1e-3 * np.abs(P64).max())
+    err = np.max(np.abs(P - P64) / scale)
+    print(f"projection over the ROI box's intervals: max rel err {err:.2e}", flush=True)
+    assert err <= 1e-5 and np.all(P[rj["ray_num_intervals"] == 0] == 0)

@@ -437,3 +437,44 @@ def test_node_density_stats(oracle):
     mx, s1, s2, _ = oracle.node_density_stats(m2, 1, 3, rounds=2, seed=1, precision=1)
     c = math.log1p(math.exp(0.5)) / 2
     assert np.allclose(mx, c, rtol=1e-6) and np.allclose(s1 / 54, c, rtol=1e-6)
+
+
+# ------------------------------------------------ single ROI box (_area)
+ROI_MIN = np.array([-0.6, -0.3, -0.8], np.float32)
+ROI_MAX = np.array([0.5, 0.7, 0.45], np.float32)
+
+
+def area_traverse(oracle, o, d, K, **kw):
+    """ComputeRaySamples_area (structure/HyperTree.cu:651-816) in the oracle:
+    the traversal over a one-node tree whose box is the ROI."""
+    return oracle.traverse(ROI_MIN[None], ROI_MAX[None], np.array([0], np.int64), o, d, K, **kw)
+
+
+@pytest.mark.parametrize("K", [1, 7, 16])
+def test_area_traversal_closed_form(oracle, K):
+    """One interval per ray through the ROI box, node 0; its midpoint
+    weights sum to the FP64 chord through the box; its count is
+    ceil(chord / |roi_max - roi_min| * K) <= K (away from integer
+    boundaries, where float rounding may decide); samples inside the box."""
+    from conftest import edge_case_rays, random_rays
+
+    o1, d1 = random_rays(3000, 11, spread=0.9)
+    o2, d2 = edge_case_rays()
+    o, d = np.concatenate([o1, o2]), np.concatenate([d1, d2])
+    r = area_traverse(oracle, o, d, K)
+    assert np.all(r["ray_num_intervals"] <= 1) and np.all(r["iv_node"] == 0)
+    diag = float(np.linalg.norm(ROI_MAX.astype(np.float64) - ROI_MIN))
+    ws = np.zeros(o.shape[0])
+    np.add.at(ws, r["ray_index"], r["weight"].astype(np.float64))
+    hit = r["ray_num_intervals"] == 1
+    chord = np.array([_fp64_chord(ROI_MIN.astype(np.float64), ROI_MAX.astype(np.float64), o[i].astype(np.float64),
+                                  d[i].astype(np.float64)) for i in range(o.shape[0])])
+    assert hit.sum() > 1000
+    assert np.all(chord[~hit] < 1e-5)  # misses and dropped grazing hits
+    assert np.abs(ws[hit] - chord[hit]).max() <= 2e-6
+    n_exact = chord[hit] / diag * K
+    clear = np.abs(n_exact - np.round(n_exact)) > 1e-4
+    assert np.array_equal(r["ray_num_samples"][hit][clear], np.ceil(n_exact[clear]).astype(np.int32))
+    assert r["ray_num_samples"].max() <= K
+    pos = r["global_coordinate"]
+    assert np.all(pos >= ROI_MIN - 1e-6) and np.all(pos <= ROI_MAX + 1e-6)
