@@ -2,23 +2,37 @@
 """Benchmark: fwd+bwd rays/s of one training step of the reconstruction hot
 path on one MI355X.
 
-Workload (the 512^3 configuration): 200-ellipsoid phantom (seed 1), 61 tilts
-from -60 to +60 degrees, 512 x 512 detector, adaptive octree up to depth 5,
-11^3 x 16-channel node grids, batches of 2^18 rays.  One step = sampling ->
-fused forward -> MSE -> fused adjoint -> Adam on every grid and decoder
-parameter, all on the device.  Inputs (rays, targets) are resident in HBM
-before the timed region.
+Workload (the 512^3 configuration): a 512^3 FP32 volume of 200 random
+ellipsoids (seed 1) plus Gaussian noise (sigma 0.01), 61 tilts from -60 to
++60 degrees on a 512 x 512 detector, targets = line integrals of the
+trilinearly interpolated noisy volume (step 1/4 voxel, FP64 sums), batches
+of 2^18 rays, 11^3 x 16-channel node grids, the 16-64-64-1 decoder.
 
-Modes:
+The octree is grown by the method's own structure loop in an untimed
+warm-up: it starts uniform at depth 3 (512 nodes), trains the plain FP32
+field, and every 100 iterations computes the per-node FP64 error
+(AccumulateSampleLossPerNode of the per-ray MSE), the per-node max density
+(ComputeMaxDensity), and restructures (exact split/merge solve, grid
+transfer, optimizer reset) up to depth 5 and 8192 active nodes.  Every leg
+then starts from that grown state.
+
+One step = sampling -> fused forward -> MSE -> fused adjoint -> Adam on every
+grid and decoder parameter, all on the device; inputs (rays, targets) are
+resident in HBM before the timed region.
+
+Legs (--mode learned, the default, runs all four in one process):
   plain    FP32 grids and decoder, stratified samples (~128 per ray)
-  bf16     BF16 grids (FP32 masters) and BF16-MFMA decoder, stratified samples
-  learned  (default) BF16 as above with the learned proposal: 32 samples per
-           ray, stratified ~128 on the KL fallback rays, the proposal trained
-           online every 100 iterations inside the timed loop.  The plain FP32
-           leg runs in the same process; both legs train the same number of
-           iterations from the same initial field and are then judged on
-           held-out rays with the stratified ~128-sample projection against
-           the analytic targets ("projection MSE").
+  bf16     BF16 grids (FP32 masters), BF16-MFMA decoder, stratified ~128
+  strat32  BF16 as above, stratified at the learned leg's sample count (~32):
+           the ablation of the learned proposal
+  learned  BF16 with the learned proposal: 32 samples per ray, stratified
+           ~128 on the KL fallback rays; the proposal is trained before the
+           leg's first step and retrained every 100 iterations (50 Adam
+           steps on 2^15 rays); that cost is amortised into ms_per_step
+Each leg trains --train-iters iterations; every 100 the held-out projection
+MSE is judged with the stratified ~128-sample sampler (midpoints) against
+the held-out targets, each field in its own arithmetic.  `value` is the
+learned leg's rays/s; the other legs sit beside it.
 
 Prints one JSON line (see the keys below).
 """
@@ -28,6 +42,8 @@ import argparse
 import json
 import math
 import os
+import statistics
+import subprocess
 import sys
 import time
 
@@ -40,6 +56,7 @@ METRIC = "fwd+bwd rays/s at 512³, 61 tilts; learned+bf16 vs plain at equal proj
 HBM_PEAK_GBS = 8000.0
 FP32_PEAK_TFLOPS = 157.3
 BF16_PEAK_TFLOPS = 2500.0
+C, G, H, DEPTH = 16, 11, 64, 5
 
 
 def parse():
@@ -48,22 +65,28 @@ def parse():
     ap.add_argument("--steps", type=int, default=100)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--mode", choices=["plain", "bf16", "learned", "vm"], default="learned",
-                    help="vm: the tensorial (plane x line) field through the composed operator path")
-    ap.add_argument("--train-iters", type=int, default=300,
-                    help="learned mode: iterations each leg trains before the projection-MSE check")
-    ap.add_argument("--mix", type=float, default=0.3, help="learned mode: uniform mixing of the proposal pdf")
-    ap.add_argument("--kl-threshold", type=float, default=0.3, help="learned mode: fallback threshold")
+                    help="learned: all four legs; plain / bf16: that leg alone; vm: the tensorial field through "
+                         "the composed operators")
+    ap.add_argument("--train-iters", type=int, default=1500,
+                    help="iterations each leg trains (MSE judged every 100)")
+    ap.add_argument("--grow-iters", type=int, default=400, help="shared warm-up iterations growing the octree")
+    ap.add_argument("--restructure-every", type=int, default=100)
+    ap.add_argument("--mix", type=float, default=0.3, help="uniform mixing of the proposal pdf")
+    ap.add_argument("--kl-threshold", type=float, default=0.3, help="proposal fallback threshold")
     ap.add_argument("--proposal-rays", type=int, default=1 << 15, help="rays per proposal training pass")
-    ap.add_argument("--volume", type=int, default=512)
+    ap.add_argument("--volume", type=int, default=512, help="phantom and detector size (512 or 768)")
     ap.add_argument("--rays", type=int, default=1 << 18)
     ap.add_argument("--batches", type=int, default=4)
     ap.add_argument("--budget", type=int, default=8192, help="max active octree nodes")
     ap.add_argument("--samples-per-ray", type=float, default=128.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-runs", type=int, default=5, help="timed CPU baseline runs (median reported)")
     ap.add_argument("--node-major-forward", action="store_true",
                     help="training step with the node-major forward over the adjoint's sort")
-    ap.add_argument("--cpu-seconds", type=float, default=12.0)
-    ap.add_argument("--profile-steps", type=int, default=0, help="only run this many steps (profiling)")
+    ap.add_argument("--save-state", default="", help="write the grown workload state to this file and exit")
+    ap.add_argument("--load-state", default="", help="start from a grown workload state written by --save-state")
+    ap.add_argument("--profile-steps", type=int, default=0,
+                    help="only run this many steps of the plain and learned legs (profiling)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only copy a 512 MiB buffer 3 times (scale factor for HBM byte counters)")
     return ap.parse_args()
@@ -73,339 +96,290 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def main():
-    args = parse()
-    if args.gpus != 1:
-        print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
-              "run it with --gpus 1", file=sys.stderr)
-        sys.exit(2)
-
+# --------------------------------------------------------------------------
+# synthetic data: voxelised noisy phantom and its line integrals
+# --------------------------------------------------------------------------
+def voxel_phantom(ell, n, noise_sigma, noise_seed, dev):
+    """[n, n, n] float32 volume (index z, y, x) on the lattice -1 + 2 i / (n - 1)
+    (align_corners): the sum of the ellipsoid densities at each voxel plus
+    N(0, noise_sigma^2) noise."""
     import torch
 
-    if args.copy_calibration:
-        x = torch.empty(1 << 27, dtype=torch.float32, device="cuda:0").fill_(1.0)
-        y = torch.empty_like(x)
-        torch.cuda.synchronize()
-        for _ in range(3):
-            y.copy_(x)
-        torch.cuda.synchronize()
-        log("copied 3 x 512 MiB (read 512 MiB + write 512 MiB per copy)")
-        return
+    lin = torch.linspace(-1.0, 1.0, n, dtype=torch.float64, device=dev)
+    vol = torch.zeros((n, n, n), dtype=torch.float32, device=dev)
+    step = 2.0 / (n - 1)
+    for e in ell:
+        c, a, rho = e[0:3], e[3:6], float(e[6])
+        lo = [max(0, int(math.floor((c[k] - a[k] + 1.0) / step))) for k in range(3)]
+        hi = [min(n, int(math.ceil((c[k] + a[k] + 1.0) / step)) + 1) for k in range(3)]
+        x = ((lin[lo[0]:hi[0]] - c[0]) / a[0]) ** 2
+        y = ((lin[lo[1]:hi[1]] - c[1]) / a[1]) ** 2
+        z = ((lin[lo[2]:hi[2]] - c[2]) / a[2]) ** 2
+        inside = (z[:, None, None] + y[None, :, None] + x[None, None, :]) <= 1.0
+        vol[lo[2]:hi[2], lo[1]:hi[1], lo[0]:hi[0]] += inside.to(torch.float32) * rho
+    if noise_sigma > 0:
+        g = torch.Generator(device=dev).manual_seed(noise_seed)
+        vol += torch.randn((n, n, n), generator=g, device=dev, dtype=torch.float32) * noise_sigma
+    return vol
 
-    from adaptivetensordf_amd import native, phantom
 
-    N = native()
-    dev = torch.device("cuda:0")
-    torch.cuda.set_device(dev)
+def volume_projection(vol, o, d, chunk=4096):
+    """Line integrals of the trilinearly interpolated volume along the rays
+    (|d| = 1), midpoint rule with steps of at most 1/4 voxel over each
+    ray's chord through [-1, 1]^3, FP64 sums.  o, d: [R, 3] tensors."""
+    import torch
+    import torch.nn.functional as F
+
+    n = vol.shape[0]
+    h = 2.0 / (n - 1) / 4.0
+    o64, d64 = o.double(), d.double()
+    with torch.no_grad():
+        inv = torch.where(d64 == 0, torch.full_like(d64, float("inf")), 1.0 / d64)
+        t1, t2 = (-1.0 - o64) * inv, (1.0 - o64) * inv
+        par = d64 == 0
+        inside = (o64 >= -1.0) & (o64 < 1.0)
+        tlo = torch.where(par, torch.where(inside, -math.inf, math.inf), torch.minimum(t1, t2)).amax(1)
+        thi = torch.where(par, torch.where(inside, math.inf, -math.inf), torch.maximum(t1, t2)).amin(1)
+        L = torch.clamp(thi - tlo, min=0.0)
+        nsteps = torch.ceil(L / h).long()
+        out = torch.zeros(o.shape[0], dtype=torch.float64, device=o.device)
+        src = vol[None, None]
+        for s in range(0, o.shape[0], chunk):
+            e = min(s + chunk, o.shape[0])
+            ns = nsteps[s:e]
+            m = int(ns.max().item()) if e > s else 0
+            if m == 0:
+                continue
+            stp = L[s:e] / ns.clamp(min=1)
+            k = torch.arange(m, device=o.device, dtype=torch.float64)
+            t = tlo[s:e, None] + (k[None, :] + 0.5) * stp[:, None]
+            pts = (o64[s:e, None, :] + d64[s:e, None, :] * t[:, :, None]).float()
+            v = F.grid_sample(src, pts.view(1, e - s, m, 1, 3), mode="bilinear", padding_mode="zeros",
+                              align_corners=True)[0, 0, :, :, 0]
+            valid = k[None, :] < ns[:, None]
+            out[s:e] = (v.double() * valid).sum(1) * stp
+        return out
+
+
+# --------------------------------------------------------------------------
+# workload: data, grown tree, initial field
+# --------------------------------------------------------------------------
+def build_data(N, args, dev):
+    import torch
+
+    from adaptivetensordf_amd import phantom
+
     n = args.volume
-    C, G, H, depth = 16, 11, 64, 5
-
-    # ------------------------------------------------------------ workload
     t0 = time.time()
     ell = phantom.ellipsoids(200, 1)
-    tree = N.HyperTreeBase(3, depth)
-    tree.settree()
-    active = phantom.adaptive_active_mask(tree.node_position_min.numpy(), tree.node_position_max.numpy(),
-                                          tree.node_children.numpy(), tree.node_depth.numpy(), ell, args.budget, 3,
-                                          depth)
-    tree.node_active = torch.from_numpy(active)
-    tree.UpdateActive()
-    tree.to("cuda")
-    num_active = int(tree.NumActiveNodes())
+    vol = voxel_phantom(ell, n, 0.01, 11, dev)
     angles = phantom.tilt_angles(61)
 
     def make_batch(nrays, seed):
         tl, px, py = phantom.sample_pixels(nrays, 61, n, n, seed)
         o, d = phantom.rays_for_pixels(tl, px, py, n, n, angles)
         o_t, d_t = torch.from_numpy(o).to(dev), torch.from_numpy(d).to(dev)
-        tgt = phantom.analytic_projection_torch(o_t, d_t, ell).float()
-        return N.RayList(o_t, d_t), tgt.contiguous()
+        return N.RayList(o_t, d_t), volume_projection(vol, o_t, d_t).float().contiguous()
 
     batches = [make_batch(args.rays, 100 + b) for b in range(args.batches)]
-    # K so that the mean number of samples per ray is ~samples_per_ray
+    ev = make_batch(1 << 16, 999)
+    del vol
+    torch.cuda.synchronize()
+    log(f"data {time.time() - t0:.1f}s: {n}^3 noisy phantom, {args.batches} x {args.rays} training rays, "
+        f"{ev[0].size()} held-out rays")
+    return batches, ev
+
+
+def field_params(N, bf16, node_forward=False):
+    fp = N.FieldParams()
+    fp.C, fp.G, fp.H = C, G, H
+    fp.bf16_decoder = bf16
+    fp.node_forward = node_forward
+    return fp
+
+
+def grow_tree(N, args, batches, dev):
+    """The method's structure loop in the untimed warm-up: plain FP32 training
+    from a uniform depth-3 tree, restructured every --restructure-every
+    iterations by the per-node FP64 error.  Returns (tree, grid, mlp, log)."""
+    import torch
+
+    from adaptivetensordf_amd import phantom
+
+    tree = N.HyperTreeBase(3, DEPTH)
+    tree.settree()
+    tree.SetActive(3)
+    tree.to("cuda")
+    st = N.MakeTrainState(phantom.init_grid(int(tree.NumActiveNodes()), G, C, std=0.01, seed=0, device=dev),
+                          phantom.init_mlp(C, H, seed=0).to(dev), False, 1e-3, 1e-3)
+    fp = field_params(N, False)
     sp = N.SamplingParams()
+    sp.max_samples_per_node = 12
     sp.jitter_mode = 2
-    sp.jitter_seed = 7
+    tp = N.TreeOptimizerParams()
+    tp.max_active_nodes = args.budget
+    rays_e, tgt_e = batches[0]
+    sub = torch.arange(0, rays_e.size(), 4, device=dev)
+    err_rays = N.RayList(rays_e.origin.index_select(0, sub).contiguous(),
+                         rays_e.direction.index_select(0, sub).contiguous())
+    err_tgt = tgt_e.index_select(0, sub).contiguous()
+    history = []
+    t0 = time.time()
+    for it in range(args.grow_iters):
+        if it > 0 and it % args.restructure_every == 0 and it <= args.grow_iters - args.restructure_every:
+            samples = tree.CreateSamplesForRays(err_rays, 12, False)
+            with torch.no_grad():
+                P = N.ProjectRays(tree, err_rays, st.grid, st.mlp, sp, fp)
+            per_ray = ((P - err_tgt) ** 2)[None].float()
+            loss_sum, _ = N.AccumulateSampleLossPerNode(samples, per_ray, tree.NumNodes())
+            tree.SetErrorForActiveNodes(loss_sum.to(tree.node_error.device), "override")
+            N.ComputeMaxDensity(tree, st.grid, st.mlp, fp, 20)
+            before = int(tree.NumActiveNodes())
+            changed, dec = N.RestructureTrainState(tree, st, tp)
+            depth = int(tree.node_depth[tree.ActiveNodeTensor().to(tree.node_depth.device)].max().item())
+            history.append({"iteration": it, "active_before": before, "active_after": int(tree.NumActiveNodes()),
+                            "split": int(dec.num_split), "merged_groups": int(dec.num_grp), "max_depth": depth,
+                            "err_rays": err_rays.size()})
+            log(f"  restructure at {it}: {before} -> {tree.NumActiveNodes()} active nodes (split {dec.num_split}, "
+                f"merged groups {dec.num_grp}), depth {depth}")
+        sp.jitter_seed = 1000 + it
+        b = it % len(batches)
+        N.TrainStep(tree, st, batches[b][0], batches[b][1], sp, fp)
+    torch.cuda.synchronize()
+    log(f"grow {time.time() - t0:.1f}s: {args.grow_iters} iterations, {tree.NumActiveNodes()} active nodes, "
+        f"loss {float(st.loss.item()):.4e}")
+    return tree, st.grid.clone(), st.mlp.clone(), history
+
+
+def save_state(path, tree, grid, mlp, history):
+    import torch
+
+    torch.save({"node_active": tree.node_active.cpu(), "grid": grid.cpu(), "mlp": mlp.cpu(), "history": history},
+               path)
+
+
+def load_state(N, path, dev):
+    import torch
+
+    s = torch.load(path, weights_only=True)
+    tree = N.HyperTreeBase(3, DEPTH)
+    tree.settree()
+    tree.node_active = s["node_active"].to(torch.int32)
+    tree.UpdateActive()
+    tree.to("cuda")
+    return tree, s["grid"].to(dev), s["mlp"].to(dev), s["history"]
+
+
+def choose_K(N, tree, rays, target):
+    sp = N.SamplingParams()
     best = None
     for K in range(1, 65):
         sp.max_samples_per_node = K
-        s_eff = tree.CreateIntervalsForRays(batches[0][0], sp).num_samples / args.rays
-        if best is None or abs(s_eff - args.samples_per_ray) < abs(best[1] - args.samples_per_ray):
+        s_eff = tree.CreateIntervalsForRays(rays, sp).num_samples / rays.size()
+        if best is None or abs(s_eff - target) < abs(best[1] - target):
             best = (K, s_eff)
-        if s_eff > args.samples_per_ray:
+        if s_eff > target:
             break
-    K, s_eff = best
-    sp.max_samples_per_node = K
-    grid0 = phantom.init_grid(num_active, G, C, std=0.01, seed=0, device=dev)
-    mlp0 = phantom.init_mlp(C, H, seed=0).to(dev)
-    log(f"setup {time.time() - t0:.1f}s: active nodes {num_active}, K {K}, mean samples/ray {s_eff:.1f}")
-
-    def field(bf16):
-        fp = N.FieldParams()
-        fp.C, fp.G, fp.H = C, G, H
-        fp.bf16_decoder = bf16
-        fp.node_forward = args.node_major_forward
-        return fp
-
-    steps = args.profile_steps if args.profile_steps > 0 else args.steps
-
-    if args.mode == "vm":
-        vm_leg(N, tree, batches, K, s_eff, C, G, H, depth, n, args, steps)
-        return
-    if args.mode != "learned":
-        bf16 = args.mode == "bf16"
-        leg = run_leg(N, tree, batches, sp, field(bf16), grid0, mlp0, bf16, None, args, steps)
-        if args.profile_steps > 0:
-            return
-        out_leg = leg
-        kern = kernel_times(N, tree, batches[0][0], sp, field(bf16), leg["state"], bf16, None)
-        plain_summary = None
-        mse = None
-        workload = (f"{'plain FP32' if not bf16 else 'BF16 grids'} training step, {n}^3 phantom (200 ellipsoids), "
-                    f"61 tilts, {n}x{n} detector, adaptive octree depth<=5")
-        dtype = "f32" if not bf16 else "bf16 grids / f32 accumulate"
-    else:
-        plain = run_leg(N, tree, batches, sp, field(False), grid0, mlp0, False, None, args, steps)
-        pp = N.ProposalParams()
-        pp.mix = args.mix
-        pp.kl_threshold = args.kl_threshold
-        ps = N.MakeProposalState(phantom.init_proposal(0).to(dev), 3e-3, pp)
-        learned = run_leg(N, tree, batches, sp, field(True), grid0, mlp0, True, ps, args, steps)
-        if args.profile_steps > 0:
-            return
-        out_leg = learned
-        kern = kernel_times(N, tree, batches[0][0], sp, field(True), learned["state"], True, ps)
-        kern_plain = kernel_times(N, tree, batches[0][0], sp, field(False), plain["state"], False, None)
-        # projection MSE on held-out rays, judged by the stratified sampler
-        ev_rays, ev_tgt = make_batch(1 << 16, 999)
-        sp_judge = N.SamplingParams()
-        sp_judge.max_samples_per_node = K
-        iv_j = tree.CreateIntervalsForRays(ev_rays, sp_judge)
-
-        def judge(leg, fp):
-            P = N.ProjectIntervals(tree, ev_rays, iv_j, leg["state"].grid, leg["state"].mlp, sp_judge, fp)
-            return float(((P.double() - ev_tgt.double()) ** 2).mean().item())
-
-        ivl, fb = N.CreateProposalIntervals(tree, ps, ev_rays, sp_judge)
-        P_own = N.ProjectIntervals(tree, ev_rays, ivl, learned["state"].grid, learned["state"].mlp, sp_judge,
-                                   field(True))
-        mse = {
-            "judge": f"stratified K={K} (~{s_eff:.0f} samples/ray, midpoints) on {ev_rays.size()} held-out rays "
-                     f"vs analytic targets, after {args.train_iters} training iterations of each leg",
-            "plain": judge(plain, field(False)),
-            "learned_bf16": judge(learned, field(True)),
-            "learned_bf16_own_sampler": float(((P_own.double() - ev_tgt.double()) ** 2).mean().item()),
-            "learned_own_sampler_samples_per_ray": ivl.num_samples / ev_rays.size(),
-            "learned_own_sampler_fallback_frac": float(fb.float().mean().item()),
-        }
-        mse["ratio_learned_over_plain"] = mse["learned_bf16"] / mse["plain"]
-        # "equal projection MSE (within 1%)": the learned leg is no worse than 1% above plain
-        mse["learned_within_1pct_or_better"] = mse["ratio_learned_over_plain"] <= 1.01
-        log(f"projection MSE (judge): plain {mse['plain']:.5e}, learned+bf16 {mse['learned_bf16']:.5e} "
-            f"(ratio {mse['ratio_learned_over_plain']:.4f}); learned own sampler {mse['learned_bf16_own_sampler']:.5e}")
-        plain_summary = {
-            "value": plain["rays_per_s"], "ms_per_step": plain["ms_per_step"],
-            "speedup_learned_over_plain": learned["rays_per_s"] / plain["rays_per_s"],
-            "kernel_ms": kern_plain["kernel_ms"], "roofline": kern_plain["roofline"],
-            "loss_first_last": plain["loss_first_last"],
-        }
-        workload = (f"learned proposal + BF16 grids training step (proposal trained online every 100 iterations "
-                    f"inside the timed loop), {n}^3 phantom (200 ellipsoids), 61 tilts, {n}x{n} detector, "
-                    f"adaptive octree depth<=5; plain FP32 leg timed in the same run")
-        dtype = "bf16 grids / f32 accumulate"
-
-    rays_per_s = out_leg["rays_per_s"]
-    cpu = None
-    if not args.no_cpu_baseline:
-        rays0, tgt0 = batches[0]
-        cpu = cpu_baseline(tree, active, grid0, mlp0, rays0, tgt0, K, C, G, args.cpu_seconds)
-    b = 2 if args.mode != "plain" else 4
-    out = {
-        "metric": METRIC,
-        "value": rays_per_s,
-        "unit": "rays/s",
-        "n_gpus": 1,
-        "steps": steps,
-        "warmup": args.warmup,
-        "ms_per_step": out_leg["ms_per_step"],
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": dtype,
-        "data": "synthetic (analytic ellipsoid phantom projections, random-init field)",
-        "config": {
-            "workload": workload,
-            "rays_per_batch": args.rays, "stratified_samples_per_ray": round(s_eff, 2), "K": K,
-            "samples_per_ray": round(out_leg["samples_per_ray"], 2),
-            "active_nodes": num_active, "channels": C, "grid": G, "decoder": f"{C}-{H}-{H}-1",
-            "parallelism": "single GPU",
-            "step": "sampling + fused fwd + MSE + fused adjoint + Adam(grids, decoder)",
-            "loss_first_last": out_leg["loss_first_last"],
-            "step_model_bytes_per_ray": 60 + out_leg["samples_per_ray"] * (16 * C * b + 32 * C),
-            "kernel_ms": kern["kernel_ms"],
-        },
-        "roofline": kern["roofline"],
-        "cpu_baseline": cpu,
-    }
-    if args.mode == "learned":
-        out["config"]["proposal"] = learned["proposal"]
-        out["plain_fp32"] = plain_summary
-        out["projection_mse"] = mse
-    print(json.dumps(out), flush=True)
+    return best
 
 
-def vm_leg(N, tree, batches, K, s_eff, C, G, H, depth, n, args, steps):
-    """Training step of the tensorial geometry (GeometryTensorQO) through the
-    composed operators: CreateSamplesForRays, ComputeImageVM (VM gathers for
-    in-ROI nodes, explicit grids for the rest, DecodeFeatures, integration),
-    MSE, autograd backward (slot-major gather adjoints, decoder adjoint),
-    Adam on every field.  One JSON line of its own (not the headline)."""
+# --------------------------------------------------------------------------
+# legs
+# --------------------------------------------------------------------------
+class Judge:
+    """Held-out projection MSE with the stratified ~128-sample sampler."""
+
+    def __init__(self, N, tree, ev, K):
+        self.N, self.tree = N, tree
+        self.rays, self.tgt = ev
+        self.sp = N.SamplingParams()
+        self.sp.max_samples_per_node = K
+        self.iv = tree.CreateIntervalsForRays(self.rays, self.sp)
+
+    def __call__(self, st, fp):
+        P = self.N.ProjectIntervals(self.tree, self.rays, self.iv, st.grid, st.mlp, self.sp, fp)
+        return float(((P.double() - self.tgt.double()) ** 2).mean().item())
+
+
+def run_leg(N, name, tree, batches, K, fp, grid0, mlp0, bf16, ps, judge, args, steps, prop_rays=None):
+    """Train one leg for --train-iters iterations (or `steps` when
+    profiling), timing iterations [warmup, warmup + steps).  With a proposal
+    state, the proposal is trained before the first step (200 Adam steps)
+    and every 100 iterations (50 steps)."""
     import torch
 
-    dev = torch.device("cuda:0")
-    lo, hi = [-0.6, -0.6, -0.6], [0.6, 0.6, 0.6]
-    roi = tree.RoiSplit(lo, hi, 0.0)
-    n_in, n_out = int(roi.in_roi_node_index.numel()), int(roi.out_roi_node_index.numel())
-    line = (G - 1) * (1 << depth) + 1
-    f = N.VMFields()
-    gen = torch.Generator(device="cpu").manual_seed(0)
-
-    def param(*shape):
-        return (torch.randn(*shape, generator=gen) * 0.01).to(dev).requires_grad_()
-
-    f.plane_yz, f.plane_xz, f.plane_xy = (param(n_in, C, G, G) for _ in range(3))
-    f.line_x, f.line_y, f.line_z = (param(1, C, line, 1) for _ in range(3))
-    f.grid_out = param(n_out, C, G, G, G)
-    f.tensor_op = "add"
-    f.roi_min, f.roi_max = lo, hi
-    from adaptivetensordf_amd import phantom
-
-    mlp = phantom.init_mlp(C, H, seed=0).to(dev).requires_grad_()
-    fp = N.FieldParams()
-    fp.C, fp.G, fp.H = C, G, H
-    params = [f.plane_yz, f.plane_xz, f.plane_xy, f.line_x, f.line_y, f.line_z, f.grid_out, mlp]
-    moments = [(torch.zeros_like(p), torch.zeros_like(p)) for p in params]
-    state = {"it": 0, "loss": None}
-
-    def step():
-        rays, tgt = batches[state["it"] % len(batches)]
-        samples = tree.CreateSamplesForRays(rays, K, True)
-        img = N.ComputeImageVM(tree, roi, f, mlp, fp, samples, rays.size())
-        loss = (img[0] - tgt).square().mean()
-        grads = torch.autograd.grad(loss, params)
-        state["it"] += 1
-        for p, g, (m, v) in zip(params, grads, moments):
-            N.AdamStep(p.data, g.contiguous(), m, v, state["it"], 1e-3)
-        state["loss"] = loss.detach()
-
-    for _ in range(args.warmup):
-        step()
-    torch.cuda.synchronize()
-    first = float(state["loss"].item()) if state["loss"] is not None else None
-    t0 = time.perf_counter()
-    for _ in range(steps):
-        step()
-    torch.cuda.synchronize()
-    el = time.perf_counter() - t0
-    R = batches[0][0].size()
-    log(f"vm: {steps} steps in {el:.3f}s: {el * 1e3 / steps:.2f} ms/step, {R * steps / el / 1e6:.3f} M rays/s, "
-        f"{n_in} VM / {n_out} explicit nodes")
-    print(json.dumps({
-        "metric": "fwd+bwd rays/s, tensorial (VM) field through the composed operators, 512^3, 61 tilts",
-        "value": R * steps / el, "unit": "rays/s", "n_gpus": 1, "steps": steps, "warmup": args.warmup,
-        "ms_per_step": el * 1e3 / steps, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f32", "data": "synthetic (analytic ellipsoid phantom projections, random-init field)",
-        "config": {"workload": f"tensorial geometry training step, {n}^3 phantom, 61 tilts, {n}x{n} detector, "
-                               f"adaptive octree depth<=5, ROI {lo}..{hi}: {n_in} plane/line nodes, {n_out} "
-                               f"explicit-grid nodes, lines of {line}",
-                   "rays_per_batch": R, "K": K, "stratified_samples_per_ray": round(s_eff, 2), "channels": C,
-                   "grid": G, "decoder": f"{C}-{H}-{H}-1", "parallelism": "single GPU",
-                   "loss_first_last": [first, float(state["loss"].item())]},
-        "roofline": None, "cpu_baseline": None}), flush=True)
-
-
-def run_leg(N, tree, batches, sp0, fp, grid0, mlp0, bf16, ps, args, steps):
-    """Train one leg: warmup, `steps` timed steps, then (learned mode) more
-    untimed steps up to --train-iters.  With a proposal state, the proposal is
-    trained every 100 iterations (the first time at iteration 100, before
-    which the leg samples stratified)."""
-    import torch
-
-    st = N.MakeTrainState(grid0, mlp0, bf16, 1e-3, 1e-3)
-    prop_rays = {}
-    stats = {"updates": 0, "update_ms": [], "loss": [], "fallback_frac": None}
+    st = N.MakeTrainState(grid0.clone(), mlp0.clone(), bf16, 1e-3, 1e-3)
     sp = N.SamplingParams()
-    sp.max_samples_per_node = sp0.max_samples_per_node
-    sp.jitter_mode = sp0.jitter_mode
+    sp.max_samples_per_node = K
+    sp.jitter_mode = 2
+    updates = []  # (iteration, ms, proposal loss)
+    curve = [] if judge is not None else None
+    total = args.warmup + steps if args.profile_steps > 0 else max(args.train_iters, args.warmup + steps)
 
-    def sub(b):
-        if b not in prop_rays:
-            rays, _ = batches[b]
-            R = rays.size()
-            stride = max(1, R // args.proposal_rays)
-            idx = torch.arange(0, R, stride, device=rays.origin.device)[: args.proposal_rays]
-            prop_rays[b] = N.RayList(rays.origin.index_select(0, idx).contiguous(),
-                                     rays.direction.index_select(0, idx).contiguous())
-        return prop_rays[b]
+    def update(i):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        b = (i // 100) % len(batches)
+        loss = N.ProposalTrain(tree, ps, prop_rays[b], sp, st.grid, st.mlp, fp, 200 if i == 0 else 50)
+        torch.cuda.synchronize()
+        updates.append((i, (time.perf_counter() - t0) * 1e3, loss))
 
-    def step(i):
+    t_start = elapsed = None
+    in_window_update_ms = 0.0
+    for i in range(total):
+        if curve is not None and i % 100 == 0:
+            curve.append((i, judge(st, fp)))
+        if i == args.warmup:
+            torch.cuda.synchronize()
+            first_loss = float(st.loss.item()) if i > 0 else float("nan")
+            t_start = time.perf_counter()
+        if ps is not None and i % 100 == 0:
+            n_before = len(updates)
+            update(i)
+            if args.warmup <= i < args.warmup + steps:
+                in_window_update_ms += updates[n_before][1]
         b = i % len(batches)
         rays, tgt = batches[b]
         sp.jitter_seed = 7 + i
         if ps is None:
-            return N.TrainStep(tree, st, rays, tgt, sp, fp)
-        if i > 0 and i % 100 == 0:
-            t0 = time.perf_counter()
-            loss = N.ProposalTrain(tree, ps, sub(b), sp, st.grid, st.mlp, fp, 200 if stats["updates"] == 0 else 50)
-            stats["update_ms"].append((time.perf_counter() - t0) * 1e3)
-            stats["loss"].append(loss)
-            stats["updates"] += 1
-        return N.TrainStepLearned(tree, st, ps, rays, tgt, sp, fp)
-
-    # learned legs start timing once the proposal is in use (iteration 100)
-    start = 100 if ps is not None else 0
-    it = 0
-    while it < start + args.warmup:
-        step(it)
-        it += 1
+            N.TrainStep(tree, st, rays, tgt, sp, fp)
+        else:
+            N.TrainStepLearned(tree, st, ps, rays, tgt, sp, fp)
+        if i == args.warmup + steps - 1:
+            torch.cuda.synchronize()
+            elapsed = time.perf_counter() - t_start
+            last_loss = float(st.loss.item())
     torch.cuda.synchronize()
-    first_loss = float(st.loss.item())
-    torch.cuda.synchronize()
-    t_start = time.perf_counter()
-    for _ in range(steps):
-        step(it)
-        it += 1
-    torch.cuda.synchronize()
-    elapsed = time.perf_counter() - t_start
-    last_loss = float(st.loss.item())
-    ms = elapsed * 1e3 / steps
-    rps = batches[0][0].size() * steps / elapsed
-    # samples per ray of the leg's own sampler on batch 0
+    if curve is not None:
+        curve.append((total, judge(st, fp)))
+    R = batches[0][0].size()
+    raw_ms = elapsed * 1e3 / steps
+    leg = {"name": name, "state": st, "raw_ms_per_step": raw_ms, "loss_first_last": [first_loss, last_loss],
+           "iterations": total, "mse_curve": curve}
     rays0 = batches[0][0]
+    sp.jitter_seed = 7
     if ps is not None:
         iv, fb = N.CreateProposalIntervals(tree, ps, rays0, sp)
-        stats["fallback_frac"] = float(fb.float().mean().item())
+        retrain = [u[1] for u in updates if u[0] > 0]
+        amort = (statistics.mean(retrain) if retrain else updates[0][1]) / 100.0
+        ms = raw_ms - in_window_update_ms / steps + amort
+        leg["proposal"] = {
+            "mix": args.mix, "kl_threshold": args.kl_threshold, "bins": 32, "samples": 32,
+            "train_rays": args.proposal_rays, "updates": len(updates),
+            "updates_in_timed_region": sum(1 for u in updates if args.warmup <= u[0] < args.warmup + steps),
+            "initial_update_ms": updates[0][1], "retrain_ms_mean": statistics.mean(retrain) if retrain else None,
+            "amortised_ms_per_step": amort, "proposal_loss_first_last": [updates[0][2], updates[-1][2]],
+            "fallback_frac": float(fb.float().mean().item())}
     else:
         iv = tree.CreateIntervalsForRays(rays0, sp)
-    spr = iv.num_samples / rays0.size()
-    log(f"{'learned+bf16' if ps is not None else ('bf16' if bf16 else 'plain')}: {steps} steps in {elapsed:.3f}s: "
-        f"{ms:.2f} ms/step, {rps / 1e6:.3f} M rays/s, {spr:.1f} samples/ray, loss {first_loss:.5f} -> {last_loss:.5f}"
-        + (f", fallback {stats['fallback_frac']:.3f}, proposal updates {stats['updates']} "
-           f"({', '.join(f'{x:.0f}' for x in stats['update_ms'])} ms)" if ps is not None else ""))
-    if args.mode == "learned":
-        while it < args.train_iters:
-            step(it)
-            it += 1
-        torch.cuda.synchronize()
-    leg = {"state": st, "rays_per_s": rps, "ms_per_step": ms, "loss_first_last": [first_loss, last_loss],
-           "samples_per_ray": spr}
-    if ps is not None:
-        leg["proposal"] = {"mix": args.mix, "kl_threshold": args.kl_threshold, "bins": 32, "samples": 32,
-                           "updates_in_timed_region": sum(1 for j in range(start + args.warmup,
-                                                                           start + args.warmup + steps)
-                                                          if j > 0 and j % 100 == 0),
-                           "update_ms": stats["update_ms"], "proposal_loss": stats["loss"],
-                           "fallback_frac": stats["fallback_frac"], "train_rays": args.proposal_rays}
+        ms = raw_ms
+    leg["ms_per_step"] = ms
+    leg["rays_per_s"] = R / (ms * 1e-3)
+    leg["samples_per_ray"] = iv.num_samples / rays0.size()
+    log(f"{name}: {steps} timed steps {raw_ms:.2f} ms/step raw, {ms:.2f} ms/step"
+        f"{' with the proposal retraining amortised' if ps is not None else ''}, {leg['rays_per_s'] / 1e6:.3f} M rays/s, "
+        f"{leg['samples_per_ray']:.1f} samples/ray"
+        + (f", final held-out MSE {curve[-1][1]:.4e} after {total} iterations" if curve else ""))
     return leg
 
 
@@ -424,7 +398,6 @@ def kernel_times(N, tree, rays, sp, fp, st, bf16, ps):
     else:
         iv = tree.CreateIntervalsForRays(rays, sp)
     dev = rays.origin.device
-    C, G, H = fp.C, fp.G, fp.H
     S_total = iv.num_samples
     R = rays.size()
     f = capi.Field(C, H, G, capi.GRID_BF16 if bf16 else capi.GRID_F32, 2.0, 0, st.grid.data_ptr(),
@@ -433,24 +406,18 @@ def kernel_times(N, tree, rays, sp, fp, st, bf16, ps):
     csp = capi.Sampling(sp.max_samples_per_node, 256, sp.jitter_mode, 0, sp.jitter_seed, None, None)
     civ = capi.Intervals(iv.node.data_ptr(), iv.tmin.data_ptr(), iv.tmax.data_ptr(), iv.count.data_ptr(),
                          iv.first.data_ptr(), iv.ray.data_ptr(), iv.num_intervals)
-    packed = torch.empty(capi.lib().atdf_tree_packed_bytes(tree.NumNodes()), dtype=torch.uint8, device=dev)
-    tdesc = capi.Tree(tree.NumNodes(), tree.node_position_min.data_ptr(), tree.node_position_max.data_ptr(),
-                      tree.node_children.data_ptr(), tree.node_parent.data_ptr(), tree.node_active.data_ptr())
+    packed = tree.PackedTree()
     s = capi.stream_ptr()
-    capi.call("atdf_tree_pack", Ct.byref(tdesc), capi.ptr(packed), s)
     P = torch.empty(R, device=dev)
     dP = torch.randn(R, device=dev) * 1e-6
     gg = torch.zeros(st.grid.shape, dtype=torch.float32, device=dev)
     gm = torch.zeros_like(st.mlp)
     ws = torch.empty(capi.lib().atdf_project_bwd_workspace_bytes(R, iv.num_intervals, Ct.byref(f)),
                      dtype=torch.uint8, device=dev)
-
     args = (capi.ptr(packed), tree.NumNodes(), capi.ptr(rays.origin), capi.ptr(rays.direction), R, Ct.byref(csp),
             capi.ptr(iv.ray_offset), Ct.byref(civ), Ct.byref(f))
     node_fwd = fp.node_forward
 
-    # the calls the training step makes: with the node-major forward, one
-    # counting sort (prepare) serves the forward and the adjoint
     def prep():
         capi.call("atdf_project_prepare", *args, capi.ptr(ws), ws.numel(), s)
 
@@ -471,7 +438,7 @@ def kernel_times(N, tree, rays, sp, fp, st, bf16, ps):
     def timed(fn, reps=5):
         fn()
         torch.cuda.synchronize()
-        cur = torch.cuda.current_stream()
+        cur = torch.cuda.current_stream()  # the stream every call above launches on
         e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
         e0.record(cur)
         for _ in range(reps):
@@ -500,8 +467,7 @@ def kernel_times(N, tree, rays, sp, fp, st, bf16, ps):
     achieved_tflops = flop_dom / t_dom / 1e12
     mfma_peak = BF16_PEAK_TFLOPS if bf16 else FP32_PEAK_TFLOPS
     log(f"  {'project_prepare %.2f ms, ' % (t_prep * 1e3) if node_fwd else ''}"
-        f"project_fwd{'_prepared' if node_fwd else ''} {t_fwd * 1e3:.2f} ms, "
-        f"project_bwd{'_prepared' if node_fwd else ''} {t_bwd * 1e3:.2f} ms ({S_total / R:.1f} samples/ray, "
+        f"project_fwd {t_fwd * 1e3:.2f} ms, project_bwd {t_bwd * 1e3:.2f} ms ({S_total / R:.1f} samples/ray, "
         f"{iv.num_intervals} intervals); dominant {dom}: {achieved_gbs:.0f} GB/s algorithmic, "
         f"{achieved_tflops:.1f} TFLOP/s decoder")
     mode = ("learned" if ps is not None else "bf16") if bf16 else "plain"
@@ -538,41 +504,333 @@ def measured_traffic(mode, kernel):
     return None
 
 
-def cpu_baseline(tree, active, grid0, mlp0, rays, tgt, K, C, G, seconds):
-    """The oracle's FP32 training step (OpenMP over rays, dense Adam) on the
-    same workload, timed on this host for about `seconds`."""
+# --------------------------------------------------------------------------
+# CPU baseline: the oracle's FP32 training step on the host cores
+# --------------------------------------------------------------------------
+def host_cpu():
+    model = None
+    try:
+        out = subprocess.run(["lscpu"], stdout=subprocess.PIPE, stderr=subprocess.DEVNULL, text=True).stdout
+        for line in out.splitlines():
+            if line.startswith("Model name:"):
+                model = line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    affinity = len(os.sched_getaffinity(0))
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or affinity
+    return model, affinity, min(threads, affinity)
+
+
+def _median_runs(fn, runs):
+    times = []
+    for _ in range(runs):
+        t0 = time.perf_counter()
+        fn()
+        times.append(time.perf_counter() - t0)
+    return statistics.median(times), times
+
+
+def cpu_baseline(N, tree, grid0, mlp0, rays, tgt, K, runs, dev):
+    """The oracle's FP32 training step (forward, adjoint, Adam; OpenMP over
+    rays) on the host cores: at the 512^3 workload's shape (2048 rays of a
+    batch, the grown tree and field) and at the 128^3 CPU configuration's
+    own shape (uniform depth-3 octree, C = 8, 4096 rays x ~64 samples).
+    Median of `runs` timed runs after one warm-up."""
     import torch
+
+    from adaptivetensordf_amd import phantom
 
     sys.path.insert(0, os.path.join(HERE, "oracle"))
     import oracle as O
 
-    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or os.cpu_count()
+    model, affinity, threads = host_cpu()
+    # 512^3 workload shape
+    active = tree.node_active.cpu().numpy()
     nr = 2048
-    o = rays.origin[:nr].cpu().numpy()
-    d = rays.direction[:nr].cpu().numpy()
-    t = tgt[:nr].cpu().numpy()
+    o, d, t = (x[:nr].cpu().numpy() for x in (rays.origin, rays.direction, tgt))
     fm = O.FusedModel(tree.node_position_min.cpu().numpy(), tree.node_position_max.cpu().numpy(), active,
                       grid0.cpu().numpy(), mlp0.cpu().numpy(), C, K=K, jitter_mode=2, seed=7)
     tb = O.TrainBaseline(fm, lr=1e-3)
-    tb.train_step(o, d, t, threads, do_adam=False)  # warm-up
-    n_steps, t0 = 0, time.perf_counter()
-    while True:
-        tb.train_step(o, d, t, threads, do_adam=False)
-        n_steps += 1
-        el = time.perf_counter() - t0
-        if el >= seconds or n_steps >= 200:
+    tb.train_step(o, d, t, threads, do_adam=False)
+    t1, _ = _median_runs(lambda: tb.train_step(o, d, t, threads, do_adam=False), runs)
+    t_adam, _ = _median_runs(lambda: tb.train_step(o[:1], d[:1], t[:1], threads, do_adam=True), 3)
+    R = rays.size()
+    c1_value = R / (t1 / nr * R + t_adam)
+    # the 128^3 CPU configuration: 20 ellipsoids (seed 0) + noise, 128^2
+    # detector, uniform depth 3, C = 8, 4096 rays, ~64 samples per ray
+    t0 = O.tree_build(3)
+    act0 = (t0["depth"] == 3).astype(np.int32)
+    ang = phantom.tilt_angles(61)
+    tl, px, py = phantom.sample_pixels(4096, 61, 128, 128, 0)
+    o0, d0 = phantom.rays_for_pixels(tl, px, py, 128, 128, ang)
+    vol0 = voxel_phantom(phantom.ellipsoids(20, 0), 128, 0.01, 3, dev)
+    tg0 = volume_projection(vol0, torch.from_numpy(o0).to(dev), torch.from_numpy(d0).to(dev)).float().cpu().numpy()
+    K0, s0 = None, 0.0
+    for k in range(1, 40):
+        r = O.traverse(t0["node_min"], t0["node_max"], O.active_ids(act0), o0, d0, k)
+        s0 = r["weight"].shape[0] / 4096
+        if s0 >= 64:
+            K0 = k
             break
-    per_ray = el / (n_steps * nr)
-    t1 = time.perf_counter()
-    tb.train_step(o[:1], d[:1], t[:1], threads, do_adam=True)  # one dense Adam pass (+1 ray)
-    t_adam = time.perf_counter() - t1
-    R = rays.origin.shape[0]
-    value = R / (per_ray * R + t_adam)
-    return {"value": value, "unit": "rays/s", "cores": threads, "kind": "port",
-            "sample": f"{n_steps} FP32 fwd+bwd passes over {nr} rays of the same workload ({el:.1f} s; oracle "
-                      f"restatement, OpenMP over rays) plus one dense Adam pass over all "
-                      f"{grid0.numel() + mlp0.numel()} parameters ({t_adam:.2f} s), scaled to one "
-                      f"{R}-ray step"}
+    fm0 = O.FusedModel(t0["node_min"], t0["node_max"], act0, phantom.init_grid(512, G, 8, std=0.01, seed=0).numpy(),
+                       phantom.init_mlp(8, H, seed=0).numpy(), 8, K=K0, jitter_mode=2, seed=7)
+    tb0 = O.TrainBaseline(fm0, lr=1e-3)
+    tb0.train_step(o0, d0, tg0, threads)
+    t_c0, _ = _median_runs(lambda: tb0.train_step(o0, d0, tg0, threads), runs)
+    return {"value": c1_value, "unit": "rays/s", "cores": threads, "kind": "port",
+            "cpu_model": model, "affinity_cpus": affinity, "runs": runs, "statistic": "median",
+            "sample": f"512^3 workload shape: median of {runs} FP32 fwd+bwd passes over {nr} rays of a 2^18-ray batch "
+                      f"(grown tree, K={K}) = {t1:.2f} s each, plus one dense Adam pass over all "
+                      f"{grid0.numel() + mlp0.numel()} parameters ({t_adam:.2f} s), scaled to one {R}-ray step; "
+                      f"oracle restatement, OpenMP over rays on {threads} threads of {model}",
+            "c0_128cube": {"value": 4096 / t_c0, "unit": "rays/s", "s_per_iteration": t_c0,
+                           "s_for_200_iterations": 200 * t_c0, "samples_per_ray": round(s0, 1), "K": K0,
+                           "sample": f"median of {runs} full training iterations (4096 rays, fwd+bwd, Adam on "
+                                     f"all parameters) at the 128^3 CPU configuration's shape, {threads} threads"}}
+
+
+# --------------------------------------------------------------------------
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
+              "run it with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    if args.copy_calibration:
+        x = torch.empty(1 << 27, dtype=torch.float32, device="cuda:0").fill_(1.0)
+        y = torch.empty_like(x)
+        torch.cuda.synchronize()
+        for _ in range(3):
+            y.copy_(x)
+        torch.cuda.synchronize()
+        log("copied 3 x 512 MiB (read 512 MiB + write 512 MiB per copy)")
+        return
+
+    from adaptivetensordf_amd import native, phantom
+
+    N = native()
+    dev = torch.device("cuda:0")
+    torch.cuda.set_device(dev)
+    n = args.volume
+    batches, ev = build_data(N, args, dev)
+    if args.load_state:
+        tree, grid0, mlp0, grow_log = load_state(N, args.load_state, dev)
+        log(f"loaded grown state: {tree.NumActiveNodes()} active nodes")
+    else:
+        tree, grid0, mlp0, grow_log = grow_tree(N, args, batches, dev)
+    if args.save_state:
+        save_state(args.save_state, tree, grid0, mlp0, grow_log)
+        log(f"saved the grown state to {args.save_state}")
+        return
+    num_active = int(tree.NumActiveNodes())
+    K, s_eff = choose_K(N, tree, batches[0][0], args.samples_per_ray)
+    judge = Judge(N, tree, ev, K) if args.profile_steps == 0 else None
+    steps = args.profile_steps if args.profile_steps > 0 else args.steps
+    log(f"workload: {num_active} active nodes, K {K} ({s_eff:.1f} samples/ray)")
+
+    if args.mode == "vm":
+        vm_leg(N, tree, batches, K, s_eff, n, args, steps)
+        return
+
+    pp = N.ProposalParams()
+    pp.mix = args.mix
+    pp.kl_threshold = args.kl_threshold
+    prop_rays = []
+    for rays, _ in batches:
+        R = rays.size()
+        idx = torch.arange(0, R, max(1, R // args.proposal_rays), device=dev)[: args.proposal_rays]
+        prop_rays.append(N.RayList(rays.origin.index_select(0, idx).contiguous(),
+                                   rays.direction.index_select(0, idx).contiguous()))
+    legs = {}
+    if args.mode in ("plain", "learned"):
+        legs["plain"] = run_leg(N, "plain", tree, batches, K, field_params(N, False, args.node_major_forward), grid0,
+                                mlp0, False, None, judge, args, steps)
+    if args.mode == "learned":
+        ps = N.MakeProposalState(phantom.init_proposal(0).to(dev), 3e-3, pp)
+        legs["learned"] = run_leg(N, "learned+bf16", tree, batches, K, field_params(N, True, args.node_major_forward),
+                                  grid0, mlp0, True, ps, judge, args, steps, prop_rays)
+    if args.profile_steps > 0:
+        return
+    if args.mode in ("bf16", "learned"):
+        legs["bf16"] = run_leg(N, "bf16", tree, batches, K, field_params(N, True), grid0, mlp0, True, None, judge,
+                               args, steps)
+    if args.mode == "learned":
+        K32, s32 = choose_K(N, tree, batches[0][0], legs["learned"]["samples_per_ray"])
+        legs["strat32"] = run_leg(N, "bf16 stratified at the learned sample count", tree, batches, K32,
+                                  field_params(N, True), grid0, mlp0, True, None, judge, args, steps)
+        legs["strat32"]["K"] = K32
+
+    head = legs["learned" if args.mode == "learned" else args.mode]
+    bf16_head = args.mode != "plain"
+    sp_k = N.SamplingParams()
+    sp_k.max_samples_per_node = K
+    sp_k.jitter_mode = 2
+    sp_k.jitter_seed = 7
+    kern = kernel_times(N, tree, batches[0][0], sp_k, field_params(N, bf16_head, args.node_major_forward),
+                        head["state"], bf16_head, ps if args.mode == "learned" else None)
+    summary = {}
+    for key, leg in legs.items():
+        if leg is head:
+            continue
+        summary[key] = {"value": leg["rays_per_s"], "ms_per_step": leg["ms_per_step"],
+                        "samples_per_ray": leg["samples_per_ray"], "final_mse": leg["mse_curve"][-1][1],
+                        "loss_first_last": leg["loss_first_last"]}
+        if key == "strat32":
+            summary[key]["K"] = leg["K"]
+    if "plain" in legs and head is not legs["plain"]:
+        kp = kernel_times(N, tree, batches[0][0], sp_k, field_params(N, False, args.node_major_forward),
+                          legs["plain"]["state"], False, None)
+        summary["plain"].update({"kernel_ms": kp["kernel_ms"], "roofline": kp["roofline"],
+                                 "speedup_learned_over_plain": head["rays_per_s"] / legs["plain"]["rays_per_s"]})
+    mse = None
+    if args.mode == "learned":
+        final = {k: leg["mse_curve"][-1][1] for k, leg in legs.items()}
+        ivl, fb = N.CreateProposalIntervals(tree, ps, ev[0], judge.sp)
+        P_own = N.ProjectIntervals(tree, ev[0], ivl, legs["learned"]["state"].grid, legs["learned"]["state"].mlp,
+                                   judge.sp, field_params(N, True))
+        mse = {
+            "judge": f"stratified K={K} (~{s_eff:.0f} samples/ray, midpoints) on {ev[0].size()} held-out rays vs their "
+                     f"targets (noisy-volume line integrals), every 100 iterations of each leg",
+            "final": final,
+            "curve": {k: leg["mse_curve"] for k, leg in legs.items()},
+            "ratio_learned_over_plain": final["learned"] / final["plain"],
+            "learned_within_1pct_or_better": final["learned"] / final["plain"] <= 1.01,
+            "ratio_learned_over_strat32": final["learned"] / final["strat32"],
+            "learned_own_sampler": float(((P_own.double() - ev[1].double()) ** 2).mean().item()),
+            "learned_own_sampler_samples_per_ray": ivl.num_samples / ev[0].size(),
+            "learned_own_sampler_fallback_frac": float(fb.float().mean().item()),
+        }
+        log(f"final held-out MSE: " + ", ".join(f"{k} {v:.4e}" for k, v in final.items())
+            + f"; learned/plain {mse['ratio_learned_over_plain']:.4f}, learned/strat32 "
+              f"{mse['ratio_learned_over_strat32']:.4f}")
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        cpu = cpu_baseline(N, tree, grid0, mlp0, batches[0][0], batches[0][1], K, args.cpu_runs, dev)
+        log(f"cpu baseline: {cpu['value']:.0f} rays/s (512^3 shape), {cpu['c0_128cube']['value']:.0f} rays/s "
+            f"(128^3 shape) on {cpu['cores']} threads of {cpu['cpu_model']}")
+    b = 2 if bf16_head else 4
+    if args.mode == "learned":
+        pr = head["proposal"]
+        workload = (f"learned proposal + BF16 grids training step, {n}^3 noisy ellipsoid phantom (200, sigma 0.01), "
+                    f"61 tilts, {n}x{n} detector, octree grown by the error-driven structure loop to {num_active} "
+                    f"nodes (depth<=5); the proposal retraining (every 100 iterations, "
+                    f"{pr['retrain_ms_mean'] or pr['initial_update_ms']:.1f} ms each) is amortised into ms_per_step "
+                    f"({pr['updates_in_timed_region']} retrainings fell inside the timed steps); plain FP32, BF16 "
+                    f"and BF16-stratified-at-equal-samples legs trained in the same run")
+        dtype = "bf16 grids / f32 accumulate"
+    else:
+        workload = (f"{'plain FP32' if not bf16_head else 'BF16 grids'} training step, {n}^3 noisy ellipsoid phantom, "
+                    f"61 tilts, {n}x{n} detector, octree grown by the error-driven structure loop to {num_active} "
+                    f"nodes (depth<=5)")
+        dtype = "f32" if not bf16_head else "bf16 grids / f32 accumulate"
+    out = {
+        "metric": METRIC,
+        "value": head["rays_per_s"],
+        "unit": "rays/s",
+        "n_gpus": 1,
+        "steps": steps,
+        "warmup": args.warmup,
+        "ms_per_step": head["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": dtype,
+        "data": "synthetic (voxelised noisy ellipsoid phantom, its line integrals as targets; random-init field "
+                "trained in the run)",
+        "config": {
+            "workload": workload,
+            "rays_per_batch": args.rays, "stratified_samples_per_ray": round(s_eff, 2), "K": K,
+            "samples_per_ray": round(head["samples_per_ray"], 2), "raw_ms_per_step": head["raw_ms_per_step"],
+            "active_nodes": num_active, "channels": C, "grid": G, "decoder": f"{C}-{H}-{H}-1",
+            "parallelism": "single GPU",
+            "step": "sampling + fused fwd + MSE + fused adjoint + Adam(grids, decoder)",
+            "train_iterations_per_leg": head["iterations"], "grow": grow_log,
+            "loss_first_last": head["loss_first_last"],
+            "step_model_bytes_per_ray": 60 + head["samples_per_ray"] * (16 * C * b + 32 * C),
+            "kernel_ms": kern["kernel_ms"],
+        },
+        "roofline": kern["roofline"],
+        "cpu_baseline": cpu,
+    }
+    if args.mode == "learned":
+        out["config"]["proposal"] = head["proposal"]
+        out["projection_mse"] = mse
+    out["legs"] = summary
+    print(json.dumps(out), flush=True)
+
+
+def vm_leg(N, tree, batches, K, s_eff, n, args, steps):
+    """Training step of the tensorial geometry (GeometryTensorQO) through the
+    composed operators: CreateSamplesForRays, ComputeImageVM (VM gathers for
+    in-ROI nodes, explicit grids for the rest, DecodeFeatures, integration),
+    MSE, autograd backward (slot-major gather adjoints, decoder adjoint),
+    Adam on every field.  One JSON line of its own (not the headline)."""
+    import torch
+
+    from adaptivetensordf_amd import phantom
+
+    dev = torch.device("cuda:0")
+    lo, hi = [-0.6, -0.6, -0.6], [0.6, 0.6, 0.6]
+    roi = tree.RoiSplit(lo, hi, 0.0)
+    n_in, n_out = int(roi.in_roi_node_index.numel()), int(roi.out_roi_node_index.numel())
+    line = (G - 1) * (1 << DEPTH) + 1
+    f = N.VMFields()
+    gen = torch.Generator(device="cpu").manual_seed(0)
+
+    def param(*shape):
+        return (torch.randn(*shape, generator=gen) * 0.01).to(dev).requires_grad_()
+
+    f.plane_yz, f.plane_xz, f.plane_xy = (param(n_in, C, G, G) for _ in range(3))
+    f.line_x, f.line_y, f.line_z = (param(1, C, line, 1) for _ in range(3))
+    f.grid_out = param(n_out, C, G, G, G)
+    f.tensor_op = "add"
+    f.roi_min, f.roi_max = lo, hi
+    mlp = phantom.init_mlp(C, H, seed=0).to(dev).requires_grad_()
+    fp = field_params(N, False)
+    params = [f.plane_yz, f.plane_xz, f.plane_xy, f.line_x, f.line_y, f.line_z, f.grid_out, mlp]
+    moments = [(torch.zeros_like(p), torch.zeros_like(p)) for p in params]
+    state = {"it": 0, "loss": None}
+
+    def step():
+        rays, tgt = batches[state["it"] % len(batches)]
+        samples = tree.CreateSamplesForRays(rays, K, True)
+        img = N.ComputeImageVM(tree, roi, f, mlp, fp, samples, rays.size())
+        loss = (img[0] - tgt).square().mean()
+        grads = torch.autograd.grad(loss, params)
+        state["it"] += 1
+        for p, g, (m, v) in zip(params, grads, moments):
+            N.AdamStep(p.data, g.contiguous(), m, v, state["it"], 1e-3)
+        state["loss"] = loss.detach()
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    first = float(state["loss"].item()) if state["loss"] is not None else None
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        step()
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    R = batches[0][0].size()
+    log(f"vm: {steps} steps in {el:.3f}s: {el * 1e3 / steps:.2f} ms/step, {R * steps / el / 1e6:.3f} M rays/s, "
+        f"{n_in} VM / {n_out} explicit nodes")
+    print(json.dumps({
+        "metric": "fwd+bwd rays/s, tensorial (VM) field through the composed operators, 512^3, 61 tilts",
+        "value": R * steps / el, "unit": "rays/s", "n_gpus": 1, "steps": steps, "warmup": args.warmup,
+        "ms_per_step": el * 1e3 / steps, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32", "data": "synthetic (voxelised noisy ellipsoid phantom projections, random-init field)",
+        "config": {"workload": f"tensorial geometry training step, {n}^3 phantom, 61 tilts, {n}x{n} detector, "
+                               f"grown octree, ROI {lo}..{hi}: {n_in} plane/line nodes, {n_out} explicit-grid nodes, "
+                               f"lines of {line}",
+                   "rays_per_batch": R, "K": K, "stratified_samples_per_ray": round(s_eff, 2), "channels": C,
+                   "grid": G, "decoder": f"{C}-{H}-{H}-1", "parallelism": "single GPU",
+                   "loss_first_last": [first, float(state["loss"].item())]},
+        "roofline": None, "cpu_baseline": None}), flush=True)
+
 
 if __name__ == "__main__":
     main()
