@@ -127,6 +127,9 @@ _SIGNATURES = {
     "atdf_project_bwd_workspace_bytes": (_SZ, [_I64, _I64, C.POINTER(Field)]),
     "atdf_project_bwd": (_I32, [_P, _I64, _P, _P, _I64, C.POINTER(Sampling), _P, C.POINTER(Intervals),
                                 C.POINTER(Field), _P, _P, _P, _P, _SZ, _P]),
+    "atdf_project_bwd_noalloc_max_bytes": (_SZ, []),
+    "atdf_project_bwd_noalloc": (_I32, [_P, _I64, _P, _P, _I64, C.POINTER(Sampling), _P, C.POINTER(Intervals),
+                                        C.POINTER(Field), _P, _P, _P, _P]),
     "atdf_node_density_stats": (_I32, [_P, _I64, _P, _I64, C.POINTER(Field), _I32, _I32, _I32, _U64, _P, _P, _P,
                                         _P]),
     "atdf_mse_workspace_bytes": (_SZ, [_I64]),

@@ -27,6 +27,8 @@
 // (a K = 64 contraction is two 16x16x32 steps; a K = 16 one is one 16x16x16).
 #include <hip/hip_runtime.h>
 
+#include <mutex>
+
 #include <algorithm>
 
 #include "atdf_device.h"
@@ -2326,6 +2328,77 @@ atdf_status atdf_project_bwd(const void* packed_tree, int64_t num_nodes, const f
 {
     return project_bwd_impl(packed_tree, num_nodes, O, D, R, sp, iv_off, iv, f, dLdP, grad_grid, grad_mlp, nullptr,
                             workspace, workspace_bytes, false, stream);
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// The adjoint for callers that cannot allocate (HIP graph capture, serving
+// loops): its counting sort, interval records, chunk table and decoder-
+// gradient partials live in `scratch`, a fixed device array of the library
+// (each GPU the library runs on holds its own copy until the process ends).
+// In host code the array's name is only a stand-in for the symbol; its device
+// address on the current device comes from hipGetSymbolAddress, looked up
+// once per device and kept.  Every call on a GPU shares that GPU's array:
+// calls that overlap on one GPU (two streams, two host threads) race.  The
+// kernels reset everything they accumulate into (counts, counters, chunk
+// table) before use, so a call does not depend on what the previous one left.
+// ---------------------------------------------------------------------------
+namespace {
+constexpr size_t kScratchBytes = size_t(256) << 20;
+constexpr int kMaxDevices = 64;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+
+atdf_status scratch_for_current_device(void** out)
+{
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) {
+        set_error("atdf_project_bwd_noalloc: hipGetDevice: %s", hipGetErrorString(e));
+        return ATDF_ERR_HIP;
+    }
+    ATDF_REQUIRE(dev >= 0 && dev < kMaxDevices, "atdf_project_bwd_noalloc: device %d beyond %d", dev, kMaxDevices);
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) {
+            set_error("atdf_project_bwd_noalloc: hipGetSymbolAddress(scratch) on device %d: %s", dev,
+                      hipGetErrorString(e));
+            return ATDF_ERR_HIP;
+        }
+        g_scratch_ptr[dev] = p;
+    }
+    *out = g_scratch_ptr[dev];
+    return ATDF_OK;
+}
+}  // namespace
+
+extern "C" {
+
+size_t atdf_project_bwd_noalloc_max_bytes(void) { return kScratchBytes; }
+
+atdf_status atdf_project_bwd_noalloc(const void* packed_tree, int64_t num_nodes, const float* O, const float* D,
+                                     int64_t R, const atdf_sampling* sp, const int64_t* iv_off,
+                                     const atdf_intervals* iv, const atdf_field* f, const float* dLdP,
+                                     float* grad_grid, float* grad_mlp, atdf_stream_t stream)
+{
+    atdf_status st = check_project_args(packed_tree, num_nodes, O, D, R, sp, iv_off, iv, f);
+    if (st) return st;
+    if (R == 0) return ATDF_OK;
+    const size_t need = atdf_project_bwd_workspace_bytes(R, iv->total, f);
+    ATDF_REQUIRE(need <= kScratchBytes,
+                 "atdf_project_bwd_noalloc: %lld intervals over %lld grids need %zu bytes of scratch, more than the "
+                 "%zu the library holds; use atdf_project_bwd with a workspace",
+                 (long long)iv->total, (long long)f->num_grids, need, kScratchBytes);
+    void* ws = nullptr;
+    st = scratch_for_current_device(&ws);
+    if (st) return st;
+    return project_bwd_impl(packed_tree, num_nodes, O, D, R, sp, iv_off, iv, f, dLdP, grad_grid, grad_mlp, nullptr,
+                            ws, kScratchBytes, false, stream);
 }
 
 atdf_status atdf_project_bwd_adam(const void* packed_tree, int64_t num_nodes, const float* O, const float* D,

@@ -305,6 +305,23 @@ atdf_status atdf_project_bwd(const void* packed_tree, int64_t num_nodes, const f
                              const atdf_field* field, const float* dLdP, float* grad_grid, float* grad_mlp,
                              void* workspace, size_t workspace_bytes, atdf_stream_t stream);
 
+/* atdf_project_bwd without the workspace argument and without allocating:
+ * the counting sort, interval records, chunk table and decoder-gradient
+ * partials live in a fixed device array of the library, `scratch`, of
+ * atdf_project_bwd_noalloc_max_bytes() bytes (256 MiB) per GPU, held until
+ * the process ends.  No counterpart in the reference: it is for callers that
+ * cannot allocate (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  Results are those of atdf_project_bwd.
+ * A call whose atdf_project_bwd_workspace_bytes() exceeds the maximum is
+ * rejected (ATDF_ERR_INVALID_ARGUMENT) before any device work.  Every call
+ * on one GPU shares that GPU's array: calls that overlap on one GPU race. */
+size_t atdf_project_bwd_noalloc_max_bytes(void);
+atdf_status atdf_project_bwd_noalloc(const void* packed_tree, int64_t num_nodes, const float* ray_origin,
+                                     const float* ray_direction, int64_t num_rays, const atdf_sampling* sampling,
+                                     const int64_t* ray_interval_offset, const atdf_intervals* intervals,
+                                     const atdf_field* field, const float* dLdP, float* grad_grid, float* grad_mlp,
+                                     atdf_stream_t stream);
+
 /* Adam state of the node grids for atdf_project_bwd_adam. */
 typedef struct {
     float* param;          /* [num_grids * G^3 * C] f32 grids; the master copy when the field is BF16 */
