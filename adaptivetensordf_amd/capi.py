@@ -33,7 +33,8 @@ class AtdfError(RuntimeError):
 
 class Tree(C.Structure):
     _fields_ = [("num_nodes", C.c_int64), ("node_position_min", C.c_void_p), ("node_position_max", C.c_void_p),
-                ("node_children", C.c_void_p), ("node_parent", C.c_void_p), ("node_active", C.c_void_p)]
+                ("node_children", C.c_void_p), ("node_parent", C.c_void_p), ("node_active", C.c_void_p),
+                ("children_per_node", C.c_int32)]
 
 
 class Sampling(C.Structure):
@@ -71,7 +72,7 @@ class TVShape(C.Structure):
 
 class NeighborSpec(C.Structure):
     _fields_ = [("size", C.c_int32 * 3), ("epsilon", C.c_float), ("use_roi", C.c_int32), ("roi_min", C.c_float * 3),
-                ("roi_max", C.c_float * 3), ("variant", C.c_int32)]
+                ("roi_max", C.c_float * 3), ("variant", C.c_int32), ("plane", C.c_int32)]
 
 
 class Intervals(C.Structure):

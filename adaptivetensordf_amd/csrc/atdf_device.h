@@ -24,6 +24,18 @@ static_assert(sizeof(PNode) == 48, "PNode must stay 48 bytes");
 
 constexpr int kFlagActive = 1;
 constexpr int kFlagActiveBelow = 2;
+// the quad-tree representation: four children, bit 0 selects the upper half
+// in x, bit 1 in z (every node spans the same y range)
+constexpr int kFlagQuad = 4;
+
+__device__ __forceinline__ int child_count(const PNode& nd) { return (nd.flags & kFlagQuad) ? 4 : 8; }
+
+// Child-index bits of the near half along the ray, so child k ^ mask for
+// k = 0, 1, ... visits the children front to back.
+__device__ __forceinline__ int near_child_mask(const float* d, bool quad)
+{
+    return quad ? ((d[0] < 0 ? 1 : 0) | (d[2] < 0 ? 2 : 0)) : ((d[0] < 0 ? 1 : 0) | (d[1] < 0 ? 2 : 0) | (d[2] < 0 ? 4 : 0));
+}
 
 // Slab test of structure/HyperTree.h:338-380: float divisions, running
 // near/far values kept in double, the parallel-axis test asymmetric
@@ -208,11 +220,12 @@ __device__ __forceinline__ float bf16_to_f32(uint16_t h)
 // Active node holding p, or -1: from the root, step into the first child (in
 // child order) whose closed box holds p (ActiveNodeIdForGlobalPosition,
 // structure/HyperTree.cu:1339-1399).
-// nchild = 4 descends only into the first four children of every node, as
-// ActiveNodeIdForGlobalPosition with use_quad (structure/HyperTree.cu:
-// 1340-1400) does on any tree.
+// nchild = 0 descends into every child of a node (eight, or four in the
+// quad-tree representation); nchild = 4 descends only into the first four
+// children of every node, as ActiveNodeIdForGlobalPosition with use_quad
+// (structure/HyperTree.cu:1340-1400) does on any tree.
 __device__ __forceinline__ int32_t active_node_for_position(const PNode* __restrict__ nodes, const float* p,
-                                                            int nchild = 8)
+                                                            int nchild = 0)
 {
     int32_t node = 0;
     while (true) {
@@ -220,7 +233,8 @@ __device__ __forceinline__ int32_t active_node_for_position(const PNode* __restr
         if (nd.flags & kFlagActive) return node;
         if (nd.first_child < 0) return -1;
         int32_t next = -1;
-        for (int c = 0; c < nchild; ++c) {
+        const int nc = nchild > 0 ? nchild : child_count(nd);
+        for (int c = 0; c < nc; ++c) {
             const PNode ch = nodes[nd.first_child + c];
             if (ch.mn[0] <= p[0] && ch.mx[0] >= p[0] && ch.mn[1] <= p[1] && ch.mx[1] >= p[1] && ch.mn[2] <= p[2] &&
                 ch.mx[2] >= p[2]) {

@@ -879,6 +879,17 @@ struct GridAdam {
     int enabled;
 };
 
+// Waves per workgroup of the adjoint (they share one LDS node gradient).
+#ifndef ATDF_BWD_WAVES_BF16
+#define ATDF_BWD_WAVES_BF16 4
+#endif
+template <bool kDecBF16>
+__host__ __device__ constexpr int bwd_waves()
+{
+    return kDecBF16 ? ATDF_BWD_WAVES_BF16 : kWaves;
+}
+constexpr int kMaxBwdWaves = ATDF_BWD_WAVES_BF16 > kWaves ? ATDF_BWD_WAVES_BF16 : kWaves;
+
 struct NodeBwdArgs {
     ProjArgs a;
     GridAdam ad;
@@ -984,11 +995,12 @@ __device__ __forceinline__ void lds_unlock(int* lock, int lane)
 }
 
 template <int C, bool kBF16, bool kDecBF16>
-__global__ __launch_bounds__(64 * kWaves) void project_bwd_node_kernel(NodeBwdArgs b)
+__global__ __launch_bounds__(64 * bwd_waves<kDecBF16>()) void project_bwd_node_kernel(NodeBwdArgs b)
 {
     extern __shared__ __attribute__((aligned(16))) unsigned char dsm[];
     __shared__ int s_chunk;
     __shared__ int s_lock;
+    constexpr int kNW = bwd_waves<kDecBF16>();
     const ProjArgs& a = b.a;
     const float4* __restrict__ rec = b.rec;
     constexpr int CPL = C / 4;
@@ -1028,7 +1040,7 @@ __global__ __launch_bounds__(64 * kWaves) void project_bwd_node_kernel(NodeBwdAr
         const int64_t gbase = (int64_t)ch.x * gsz;
         const PNode nd = a.nodes[a.iv_node[b.sorted[ch.y]]];  // one node per chunk
         const int n_iv = ch.z - ch.y;
-        const int per = (n_iv + kWaves - 1) / kWaves;
+        const int per = (n_iv + kNW - 1) / kNW;
         const int ib = ch.y + wl * per;
         const int ie = min(ib + per, ch.z);
         // samples of this wave's intervals
@@ -1242,7 +1254,7 @@ __global__ __launch_bounds__(64 * kWaves) void project_bwd_node_kernel(NodeBwdAr
         }
         __syncthreads();
     }
-    store_partials<C>(q, b.ws + ((int64_t)blockIdx.x * kWaves + wl) * NP, g, j, lane);
+    store_partials<C>(q, b.ws + ((int64_t)blockIdx.x * kNW + wl) * NP, g, j, lane);
 }
 
 // ---------------------------------------------------------------------------
@@ -2020,7 +2032,7 @@ BwdWorkspace bwd_layout(int64_t H, int64_t NA, int np)
     w.counter = take(256);
     w.stotal = take(sizeof(unsigned long long));
     w.scan = take(atdf_scan_workspace_bytes(NA));
-    w.partials = take(sizeof(float) * (size_t)persistent_blocks() * kWaves * np);
+    w.partials = take(sizeof(float) * (size_t)persistent_blocks() * kMaxBwdWaves * np);
     w.rec = take(3 * sizeof(float4) * (size_t)(H > 0 ? H : 1));
     w.ivsum = take(sizeof(float) * (size_t)(H > 0 ? H : 1));
     w.total = o;
@@ -2036,7 +2048,7 @@ void lds_layout(int G, int& f32_off, int& bf16_off, int& tb_off, int& total)
     bf16_off = (after_f32 + 15) & ~15;
     int after_bf = bf16_off + (kDecBF16 ? BSmem<C>::kBf16Total * 2 : 0);
     tb_off = (after_bf + 15) & ~15;
-    total = tb_off + (kDecBF16 ? kWaves * 2 * kH * kTB * 2 : kWaves * 2 * kH * kTS * 4);
+    total = tb_off + (kDecBF16 ? bwd_waves<true>() * 2 * kH * kTB * 2 : kWaves * 2 * kH * kTS * 4);
 }
 
 int max_lds_per_block()
@@ -2069,7 +2081,7 @@ atdf_status launch_node_bwd(NodeBwdArgs args, hipStream_t s)
         return ATDF_ERR_UNSUPPORTED;
     }
     hipLaunchKernelGGL((project_bwd_node_kernel<C, kBF16, kDecBF16>), dim3(persistent_blocks()),
-                       dim3(64 * kWaves), total, s, args);
+                       dim3(64 * bwd_waves<kDecBF16>()), total, s, args);
     return ATDF_OK;
 }
 }  // namespace
@@ -2228,8 +2240,9 @@ static atdf_status bwd_node_impl(const Prepared& P, const void* packed_tree, con
         hipLaunchKernelGGL(grid_adam_rest_kernel, dim3((unsigned)(NA * bps)), dim3(256), 0, s, args.ad, grad_grid,
                            P.chunk_cnt(), bps, n4);
     }
+    const int waves = bd ? bwd_waves<true>() : bwd_waves<false>();
     hipLaunchKernelGGL(reduce_partials_kernel, dim3((unsigned)div_up(np, kReduceParams)), dim3(256), 0, s, partials,
-                       (int64_t)persistent_blocks() * kWaves, np, grad_mlp);
+                       (int64_t)persistent_blocks() * waves, np, grad_mlp);
     return check_launch("atdf_project_bwd");
 }
 

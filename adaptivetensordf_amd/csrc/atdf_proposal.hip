@@ -289,11 +289,15 @@ __global__ __launch_bounds__(256) void leaf_table_kernel(const PNode* __restrict
             node = -1;
             break;
         }
+        // child bit b splits axis b (octree) or axes x, z (quad tree)
+        const bool quad = (nd.flags & kFlagQuad) != 0;
         int sel = 0;
 #pragma unroll
-        for (int k = 0; k < 3; ++k) {
+        for (int b = 0; b < 3; ++b) {
+            if (quad && b == 2) break;
+            const int k = quad && b == 1 ? 2 : b;
             float mid = (nd.mn[k] + nd.mx[k]) * 0.5f;
-            if (c[k] >= mid) sel |= 1 << k;
+            if (c[k] >= mid) sel |= 1 << b;
         }
         node = nd.first_child + sel;
     }

@@ -132,6 +132,8 @@ struct NeighborArgs {
     int use_roi;
     float roi_min[3], roi_max[3];
     int quad;  // NodeNeighborSamplesQuad: sides x and z, in_roi_quad, four-child descent
+    int two_d; // NodeNeighborSamples2D (quad-tree representation) on plane `plane`
+    int plane;
 };
 
 // in_roi (structure/HyperTree.cu:1667-1690), including its mid = min + max;
@@ -149,9 +151,73 @@ __device__ __forceinline__ bool node_in_roi(const PNode& n, const float* lo_, co
     return in;
 }
 
+// NodeNeighborSamples2D (structure/HyperTree.cu:1815-1913), candidate c of
+// node a: on the face local[plane] = 0, tt = 0, 1 takes (side_a, side_b) =
+// (s1, s2) then (s2, s1) with (s1, s2) = (plane + 1, plane + 2) mod 3, back
+// / front along side_a, size[side_b] points along side_b; the four-child
+// descent; in_roi with the ROI's y range widened by eps once for every
+// candidate the reference's thread (sid % 128 of 128) visited up to this one,
+// in float steps as its loop body does.
+__device__ int neighbor_candidate_2d(const NeighborArgs& a, int64_t act, int c, int* node_out, float* g)
+{
+    constexpr int kThreads = 128;
+    const int sidx[2] = {(a.plane + 1) % 3, (a.plane + 2) % 3};
+    int tt, front, sid;
+    const int n0 = a.size[sidx[1]];  // points per side for tt = 0 (side_b = sidx[1])
+    if (c < 2 * n0) {
+        tt = 0;
+        front = c / n0;
+        sid = c % n0;
+    } else {
+        const int n1 = a.size[sidx[0]];
+        const int cc = c - 2 * n0;
+        tt = 1;
+        front = cc / n1;
+        sid = cc % n1;
+    }
+    const int sa = sidx[tt], sb = sidx[(tt + 1) % 2];
+    const int node = (int)a.active_ids[act];
+    const PNode nd = a.nodes[node];
+    float local[3];
+    local[sa] = front ? 1.f : 0.f;
+    local[sb] = (float)(sid % a.size[sb]) / ((float)a.size[sb] - 1.f);
+    local[a.plane] = 0.f;
+    float q[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const float box = nd.mx[k] - nd.mn[k];
+        g[k] = local[k] * box + nd.mn[k];
+        q[k] = g[k];
+    }
+    q[sa] = g[sa] + (front ? a.eps : -a.eps);
+    const int other = active_node_for_position(a.nodes, q, 4);
+    *node_out = node;
+    if (other < 0 || other == node) return -1;
+    if (a.use_roi) {
+        const int t = sid % kThreads;
+        int m = 0;
+        for (int tp = 0; tp <= tt; ++tp)
+            for (int fp = 0; fp < 2; ++fp) {
+                if (tp == tt && fp > front) break;
+                const int nb = a.size[sidx[(tp + 1) % 2]];
+                const int upto = (tp == tt && fp == front) ? sid : nb - 1;
+                for (int s2 = t; s2 <= upto && s2 < nb; s2 += kThreads) ++m;
+            }
+        float lo[3] = {a.roi_min[0], a.roi_min[1], a.roi_min[2]};
+        float hi[3] = {a.roi_max[0], a.roi_max[1], a.roi_max[2]};
+        for (int i = 0; i < m; ++i) {
+            lo[1] = lo[1] - a.eps;
+            hi[1] = hi[1] + a.eps;
+        }
+        if (!(node_in_roi(nd, lo, hi, 0.f) && node_in_roi(a.nodes[other], lo, hi, 0.f))) return -1;
+    }
+    return other;
+}
+
 // candidate c of node a: returns the partner node (or -1) and the point
 __device__ int neighbor_candidate(const NeighborArgs& a, int64_t act, int c, int* node_out, float* g)
 {
+    if (a.two_d) return neighbor_candidate_2d(a, act, c, node_out, g);
     int side = 0, front = 0, sid = c;
     for (side = 0; side < 3; side += a.quad ? 2 : 1) {
         const int sa = (side + 1) % 3, sb = (side + 2) % 3;
@@ -283,9 +349,13 @@ static atdf_status make_neighbor_args(const void* packed_tree, int64_t num_nodes
         a->roi_min[k] = spec->roi_min[k];
         a->roi_max[k] = spec->roi_max[k];
     }
-    ATDF_REQUIRE(spec->variant == 0 || spec->variant == 1, "%s: variant %d (0 NodeNeighborSamples, 1 "
-                 "NodeNeighborSamplesQuad)", who, spec->variant);
+    ATDF_REQUIRE(spec->variant >= 0 && spec->variant <= 2, "%s: variant %d (0 NodeNeighborSamples, 1 "
+                 "NodeNeighborSamplesQuad, 2 NodeNeighborSamples2D)", who, spec->variant);
+    ATDF_REQUIRE(spec->variant != 2 || (spec->plane >= 0 && spec->plane <= 2), "%s: plane %d outside [0, 2]", who,
+                 spec->plane);
     a->quad = spec->variant == 1;
+    a->two_d = spec->variant == 2;
+    a->plane = spec->plane;
     a->per_node = (int)atdf_neighbor_candidates(1, spec);
     a->eps = spec->epsilon;
     a->use_roi = spec->use_roi != 0;
@@ -296,6 +366,10 @@ static atdf_status make_neighbor_args(const void* packed_tree, int64_t num_nodes
 int64_t atdf_neighbor_candidates(int64_t num_active, const atdf_neighbor_spec* spec)
 {
     if (!spec || num_active <= 0) return 0;
+    if (spec->variant == 2) {
+        if (spec->plane < 0 || spec->plane > 2) return 0;
+        return num_active * 2 * ((int64_t)spec->size[(spec->plane + 1) % 3] + spec->size[(spec->plane + 2) % 3]);
+    }
     const int64_t x = (int64_t)spec->size[1] * spec->size[2], y = (int64_t)spec->size[2] * spec->size[0],
                   z = (int64_t)spec->size[0] * spec->size[1];
     return num_active * 2 * (spec->variant == 1 ? x + z : x + y + z);

@@ -39,7 +39,8 @@ void set_error(const char* fmt, ...)
 __global__ __launch_bounds__(256) void pack_nodes_kernel(const float* __restrict__ pmin, const float* __restrict__ pmax,
                                                          const int32_t* __restrict__ children,
                                                          const int32_t* __restrict__ parent,
-                                                         const int32_t* __restrict__ active, int64_t n, PNode* out)
+                                                         const int32_t* __restrict__ active, int64_t n, int ns,
+                                                         PNode* out)
 {
     int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (i >= n) return;
@@ -48,9 +49,9 @@ __global__ __launch_bounds__(256) void pack_nodes_kernel(const float* __restrict
         p.mn[k] = pmin[i * 3 + k];
         p.mx[k] = pmax[i * 3 + k];
     }
-    p.first_child = children[i * 8];
+    p.first_child = children[i * ns];
     p.parent = parent[i];
-    p.flags = active[i] ? kFlagActive : 0;
+    p.flags = (active[i] ? kFlagActive : 0) | (ns == 4 ? kFlagQuad : 0);
     p.pad[0] = p.pad[1] = p.pad[2] = 0;
     out[i] = p;
 }
@@ -121,7 +122,9 @@ __global__ __launch_bounds__(256) void traverse_kernel(TraverseArgs a)
         o[k] = a.O[r * 3 + k];
         d[k] = a.D[r * 3 + k];
     }
-    const int mask = (d[0] < 0 ? 1 : 0) | (d[1] < 0 ? 2 : 0) | (d[2] < 0 ? 4 : 0);
+    const bool quad = (a.nodes[0].flags & kFlagQuad) != 0;
+    const int mask = near_child_mask(d, quad);
+    const int last_sibling = quad ? 3 : 7;
 
     int nh = 0, ns = 0;
     float last_t = -INFINITY;
@@ -184,7 +187,7 @@ __global__ __launch_bounds__(256) void traverse_kernel(TraverseArgs a)
             if (parent < 0) goto done;
             int first = a.nodes[parent].first_child;
             int k = (cur - first) ^ mask;
-            if (k < 7) {
+            if (k < last_sibling) {
                 node = first + ((k + 1) ^ mask);
                 break;
             }
@@ -267,7 +270,9 @@ __global__ __launch_bounds__(kCoopThreads) void traverse_coop_kernel(TraverseArg
         o[k] = a.O[r * 3 + k];
         d[k] = a.D[r * 3 + k];
     }
-    const int mask = (d[0] < 0 ? 1 : 0) | (d[1] < 0 ? 2 : 0) | (d[2] < 0 ? 4 : 0);
+    const bool quad = (a.nodes[0].flags & kFlagQuad) != 0;
+    const int mask = near_child_mask(d, quad);
+    const int nchild = quad ? 4 : 8;
     int nh = 0, ns = 0;
     float last_t = -INFINITY;
     int last_node = -1;
@@ -278,19 +283,22 @@ __global__ __launch_bounds__(kCoopThreads) void traverse_coop_kernel(TraverseArg
     int32_t* bits = s_bits[gi];
     int level = -1;
 
-    // push the eight children of a node: lane gl tests the child at
-    // front-to-back position gl
+    // push the children of a node: lane gl tests the child at front-to-back
+    // position gl (lanes 4-7 idle in the quad-tree representation)
     auto push = [&](int32_t fc) {
         if (level + 1 >= kCoopLevels) {
             overflow = true;
             return;
         }
         ++level;
-        const int child = fc + (gl ^ mask);
-        const PNode c = a.nodes[child];
-        float t0, t1;
-        const bool hit = intersect_box_ray(c.mn, c.mx, o, d, t0, t1);
-        ch[level][gl] = CoopChild{t0, t1, c.first_child, c.flags | (hit ? 256 : 0)};
+        bool hit = false;
+        if (gl < nchild) {
+            const int child = fc + (gl ^ mask);
+            const PNode c = a.nodes[child];
+            float t0, t1;
+            hit = intersect_box_ray(c.mn, c.mx, o, d, t0, t1);
+            ch[level][gl] = CoopChild{t0, t1, c.first_child, c.flags | (hit ? 256 : 0)};
+        }
         int b = hit ? (1 << gl) : 0;
         b |= __shfl_xor(b, 1, 8);
         b |= __shfl_xor(b, 2, 8);
@@ -471,12 +479,15 @@ atdf_status atdf_tree_pack(const atdf_tree* t, void* packed, atdf_stream_t strea
     ATDF_REQUIRE(t->node_position_min && t->node_position_max && t->node_children && t->node_parent &&
                      t->node_active,
                  "atdf_tree_pack: null tree tensor");
+    const int ns = t->children_per_node == 0 ? 8 : t->children_per_node;
+    ATDF_REQUIRE(ns == 8 || ns == 4, "atdf_tree_pack: children_per_node %d (8: octree, 4: quad tree)",
+                 t->children_per_node);
     hipStream_t s = to_hip(stream);
     int64_t n = t->num_nodes;
     dim3 grid((unsigned)div_up(n, 256));
     PNode* out = reinterpret_cast<PNode*>(packed);
     hipLaunchKernelGGL(pack_nodes_kernel, grid, dim3(256), 0, s, t->node_position_min, t->node_position_max,
-                       t->node_children, t->node_parent, t->node_active, n, out);
+                       t->node_children, t->node_parent, t->node_active, n, ns, out);
     hipLaunchKernelGGL(mark_active_below_kernel, grid, dim3(256), 0, s, t->node_parent, t->node_active, n, out);
     return check_launch("atdf_tree_pack");
 }

@@ -61,7 +61,9 @@ PYBIND11_MODULE(TORCH_EXTENSION_NAME, m)
 
     py::class_<HyperTreeBase>(m, "HyperTreeBase")
         .def(py::init<int, int>(), py::arg("d"), py::arg("max_depth"))
-        .def("settree", &HyperTreeBase::settree)
+        .def("settree", &HyperTreeBase::settree, py::arg("use_quad") = false, py::arg("y_min") = -1.f,
+             py::arg("y_max") = 1.f)
+        .def("ChildrenPerNode", &HyperTreeBase::ChildrenPerNode)
         .def("SetActive", &HyperTreeBase::SetActive)
         .def("UpdateActive", &HyperTreeBase::UpdateActive)
         .def("to", [](HyperTreeBase& t, const std::string& dev) { t.to(torch::Device(dev)); })

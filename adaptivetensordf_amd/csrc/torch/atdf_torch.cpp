@@ -64,9 +64,10 @@ HyperTreeBase::HyperTreeBase(int d_, int max_depth_) : d(d_), max_depth(max_dept
     TORCH_CHECK(max_depth >= 0 && max_depth <= 8, "HyperTreeBase: max_depth must be in [0, 8]");
 }
 
-void HyperTreeBase::settree()
+void HyperTreeBase::settree(bool use_quad, float y_min, float y_max)
 {
-    const int NS = 8;
+    TORCH_CHECK(!use_quad || y_min < y_max, "settree: the quad-tree y range needs y_min < y_max");
+    const int NS = use_quad ? 4 : 8;
     int64_t num_nodes = 0, p = 1;
     for (int i = 0; i <= max_depth; ++i) {
         num_nodes += p;
@@ -99,7 +100,10 @@ void HyperTreeBase::settree()
         par[id] = (int32_t)parent;
         scl[id] = scale;
     };
-    const float rmin[3] = {-1.f, -1.f, -1.f}, rmax[3] = {1.f, 1.f, 1.f};
+    // the quad-tree representation (HyperTree.cpp:391-406, 461-464): children
+    // split x (bit 0) and z (bit 1) at the centre, every node spans
+    // [y_min, y_max] in y
+    const float rmin[3] = {-1.f, use_quad ? y_min : -1.f, -1.f}, rmax[3] = {1.f, use_quad ? y_max : 1.f, 1.f};
     int64_t cur = 0;
     set_node(cur++, -1, rmin, rmax, 0);
     int64_t start_layer = 0;
@@ -111,7 +115,12 @@ void HyperTreeBase::settree()
             for (int k = 0; k < 3; ++k) c[k] = (pmin[id * 3 + k] + pmax[id * 3 + k]) * 0.5f;
             for (int i = 0; i < NS; ++i) {
                 for (int k = 0; k < 3; ++k) {
-                    if (((i >> k) & 1) == 0) {
+                    mn[k] = pmin[id * 3 + k];
+                    mx[k] = pmax[id * 3 + k];
+                }
+                for (int b = 0; b < (use_quad ? 2 : 3); ++b) {
+                    const int k = use_quad && b == 1 ? 2 : b;
+                    if (((i >> b) & 1) == 0) {
                         mn[k] = pmin[id * 3 + k];
                         mx[k] = c[k];
                     } else {
@@ -193,6 +202,7 @@ void HyperTreeBase::repack()
     t.node_children = node_children.data_ptr<int32_t>();
     t.node_parent = node_parent.data_ptr<int32_t>();
     t.node_active = node_active.data_ptr<int32_t>();
+    t.children_per_node = (int32_t)node_children.size(1);
     check_status(atdf_tree_pack(&t, packed_.data_ptr(), current_stream()), "atdf_tree_pack");
     const int depth = leaf_depth();
     leaf_table_ = torch::empty({atdf_leaf_table_numel(depth)},
@@ -410,15 +420,16 @@ void HyperTreeBase::UpdateCulling()
     int32_t* cp = culled.data_ptr<int32_t>();
     const int32_t* chp = children.data_ptr<int32_t>();
     const int32_t* ap = active.data_ptr<int32_t>();
+    const int64_t NS = children.size(1);
     bool changed = true;
     while (changed) {
         changed = false;
         for (int64_t i = 0; i < culled.size(0); ++i) {
             if (cp[i]) continue;
-            if (chp[i * 8] < 0) continue;  // leaves have no children to inherit culling from
+            if (chp[i * NS] < 0) continue;  // leaves have no children to inherit culling from
             bool all = true;
-            for (int c = 0; c < 8; ++c)
-                if (!cp[chp[i * 8 + c]]) all = false;
+            for (int64_t c = 0; c < NS; ++c)
+                if (!cp[chp[i * NS + c]]) all = false;
             if (all) {
                 TORCH_CHECK(!ap[i], "UpdateCulling: active node with all children culled");
                 cp[i] = 1;

@@ -83,7 +83,10 @@ struct RoiMaps {
 class HyperTreeBase {
    public:
     HyperTreeBase(int d, int max_depth);
-    void settree();                      // HyperTree.cpp:298-525
+    // HyperTree.cpp:298-525; use_quad: the quad-tree representation (four
+    // children per node splitting x and z, every node spanning [y_min, y_max])
+    void settree(bool use_quad = false, float y_min = -1.f, float y_max = 1.f);
+    int ChildrenPerNode() const { return (int)node_children.size(1); }
     void SetActive(int depth);           // HyperTree.cpp:544-549
     void UpdateActive();                 // HyperTree.cpp:550-588
     void to(torch::Device device);
@@ -123,8 +126,9 @@ class HyperTreeBase {
     // with use_quad = false the NodeNeighborSamplesQuad kernel -- faces normal
     // to x and z only, in_roi_quad (y range widened by epsilon / 2), point
     // location over the first four children of each node, as the reference
-    // does on any tree; plane_index is unused.  use_quad = true (the
-    // quad-tree representation's NodeNeighborSamples2D) is rejected.
+    // does on any tree; plane_index is unused.  use_quad = true: the
+    // quad-tree representation's NodeNeighborSamples2D on plane plane_index
+    // (HyperTree.cu:1815-1913).
     SampleList NodeNeighborSamplesPlane2D(std::vector<int> size, double epsilon, int plane_index,
                                           std::vector<float> roi_min, std::vector<float> roi_max, bool use_roi,
                                           bool use_quad) const;
@@ -164,7 +168,7 @@ class HyperTreeBase {
     SampleList samples_for(const void* packed, int64_t num_nodes, const RayList& rays, int max_samples_per_node,
                            bool interval_jitter) const;
     SampleList neighbor_samples(std::vector<int> size, double epsilon, std::vector<float> roi_min,
-                                std::vector<float> roi_max, bool use_roi, int variant) const;
+                                std::vector<float> roi_max, bool use_roi, int variant, int plane = 0) const;
     torch::Tensor packed_;
     torch::Tensor leaf_table_;
 };

@@ -52,11 +52,11 @@ struct Option {
 
 StructureDecision SolveTreeStructure(const HyperTreeBase& tree, const TreeOptimizerParams& p)
 {
-    constexpr int NS = 8;
     auto act_t = tree.node_active.to(torch::kCPU).contiguous();
     auto cul_t = tree.node_culled.to(torch::kCPU).contiguous();
     auto par_t = tree.node_parent.to(torch::kCPU).contiguous();
     auto ch_t = tree.node_children.to(torch::kCPU).contiguous();
+    const int NS = (int)ch_t.size(1);  // 8, or 4 for the quad-tree representation
     auto err_t = tree.node_error.to(torch::kCPU).to(torch::kFloat32).contiguous();
     auto den_t = tree.node_max_density.to(torch::kCPU).to(torch::kFloat32).contiguous();
     auto dep_t = tree.node_depth.to(torch::kCPU).contiguous();
@@ -280,8 +280,9 @@ bool OptimizeTreeStructure(HyperTreeBase& tree, const TreeOptimizerParams& p, St
         const int c = dec.choice[i];
         if (c == 1) {
             act[i] = 0;
-            for (int k = 0; k < 8; ++k) {
-                const int32_t cc = ch[i * 8 + k];
+            const int64_t NS = ch_t.size(1);
+            for (int64_t k = 0; k < NS; ++k) {
+                const int32_t cc = ch[i * NS + k];
                 if (cul[cc] == 0) act[cc] = 1;
             }
         } else if (c == 2) {
@@ -363,14 +364,16 @@ SampleList HyperTreeBase::NodeNeighborSamplesPlane2D(std::vector<int> size, doub
                                                      std::vector<float> roi_min, std::vector<float> roi_max,
                                                      bool use_roi, bool use_quad) const
 {
-    (void)plane_index;  // unused by the reference's kernels as well
-    TORCH_CHECK(!use_quad, "NodeNeighborSamplesPlane2D: use_quad selects NodeNeighborSamples2D, the pairs of the "
-                "quad-tree representation, which is not built (the octree's pairs come from use_quad = false)");
-    return neighbor_samples(size, epsilon, roi_min, roi_max, use_roi, 1);
+    // use_quad: NodeNeighborSamples2D on plane plane_index (the quad-tree
+    // representation's pairs); otherwise NodeNeighborSamplesQuad, which, like
+    // the reference's kernel, ignores plane_index
+    TORCH_CHECK(!use_quad || (plane_index >= 0 && plane_index <= 2), "NodeNeighborSamplesPlane2D: plane_index ",
+                plane_index, " outside [0, 2]");
+    return neighbor_samples(size, epsilon, roi_min, roi_max, use_roi, use_quad ? 2 : 1, plane_index);
 }
 
 SampleList HyperTreeBase::neighbor_samples(std::vector<int> size, double epsilon, std::vector<float> roi_min,
-                                           std::vector<float> roi_max, bool use_roi, int variant) const
+                                           std::vector<float> roi_max, bool use_roi, int variant, int plane) const
 {
     TORCH_CHECK(node_position_min.is_cuda(), "NodeNeighborSamples: the tree must be on the GPU");
     TORCH_CHECK(size.size() == 3, "NodeNeighborSamples: size needs 3 entries");
@@ -384,6 +387,7 @@ SampleList HyperTreeBase::neighbor_samples(std::vector<int> size, double epsilon
     spec.epsilon = (float)epsilon;
     spec.use_roi = use_roi ? 1 : 0;
     spec.variant = variant;
+    spec.plane = plane;
     auto dev = node_position_min.device();
     auto ids = active_node_ids.contiguous();
     const int64_t NA = ids.size(0);

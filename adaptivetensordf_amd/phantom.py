@@ -188,7 +188,8 @@ def adaptive_active_mask(node_min: np.ndarray, node_max: np.ndarray, children: n
         if s > 0 and depth[n] < max_depth:
             heapq.heappush(heap, (-s, int(n)))
     count = int(active.sum())
-    while heap and count + 7 <= budget:
+    grow = children.shape[1] - 1  # nodes added by one split
+    while heap and count + grow <= budget:
         _, n = heapq.heappop(heap)
         active[n] = 0
         for c in children[n]:
@@ -196,7 +197,7 @@ def adaptive_active_mask(node_min: np.ndarray, node_max: np.ndarray, children: n
             s = score(c)
             if s > 0 and depth[c] < max_depth:
                 heapq.heappush(heap, (-s, int(c)))
-        count += 7
+        count += grow
     return active
 
 

@@ -41,15 +41,19 @@ const char* atdf_version(void);
 /* ======================================================================= */
 /* The reference tree tensors (structure/HyperTree.h:522-560), device side.
  * Children of a node are contiguous ids (BFS numbering of
- * structure/HyperTree.cpp:453-497); child c takes the upper half on axis k
- * iff bit k of c is set. */
+ * structure/HyperTree.cpp:453-497).  Octree: child c takes the upper half on
+ * axis k iff bit k of c is set.  Quad-tree representation
+ * (use_quad_tree_rep, HyperTree.cpp:391-406, 461-464): four children, bit 0
+ * of c selects the upper half in x and bit 1 in z; every node spans the
+ * same y range. */
 typedef struct {
     int64_t num_nodes;
     const float* node_position_min; /* [num_nodes, 3] */
     const float* node_position_max; /* [num_nodes, 3] */
-    const int32_t* node_children;   /* [num_nodes, 8], -1 = leaf */
+    const int32_t* node_children;   /* [num_nodes, children_per_node], -1 = leaf */
     const int32_t* node_parent;     /* [num_nodes], -1 = root */
     const int32_t* node_active;     /* [num_nodes], 0 / 1 */
+    int32_t children_per_node;      /* 8 (octree; 0 also means 8) or 4 (quad tree) */
 } atdf_tree;
 
 /* Packed traversal form of the tree (48 B per node), rebuilt after every
@@ -484,12 +488,19 @@ typedef struct {
     int32_t use_roi;
     float roi_min[3];
     float roi_max[3];
-    int32_t variant; /* 0 NodeNeighborSamples; 1 NodeNeighborSamplesQuad */
+    int32_t variant; /* 0 NodeNeighborSamples; 1 NodeNeighborSamplesQuad; 2 NodeNeighborSamples2D */
+    int32_t plane;   /* variant 2: the plane index (the face local[plane] = 0 of every node) */
 } atdf_neighbor_spec;
 
 /* Candidates: num_active * 2 * (size1 size2 + size2 size0 + size0 size1)
  * (variant 1: without the size2 size0 term of side 1), numbered per active
- * node as (side, back / front, lattice index). */
+ * node as (side, back / front, lattice index).  Variant 2
+ * (NodeNeighborSamples2D of the quad-tree representation,
+ * structure/HyperTree.cu:1815-1913): num_active * 2 * (size[s1] + size[s2])
+ * with (s1, s2) = (plane + 1, plane + 2) mod 3, numbered per active node as
+ * (tt, back / front, point): tt = 0 steps along s1 with points along s2,
+ * tt = 1 the reverse; the four-child descent; the ROI's y range widened by
+ * epsilon once per candidate the reference's thread has visited. */
 int64_t atdf_neighbor_candidates(int64_t num_active, const atdf_neighbor_spec* spec);
 /* flags[c] = 1 where candidate c is a pair (device int32). */
 atdf_status atdf_neighbor_flags(const void* packed_tree, int64_t num_nodes, const int64_t* active_ids,

@@ -13,27 +13,37 @@
 #endif
 
 /* ------------------------------------------------------------------------ */
-/* tree: structure/HyperTree.cpp:298-525 (octree branch, NS = 8)             */
+/* tree: structure/HyperTree.cpp:298-525.  ns = 8: the octree; ns = 4: the   */
+/* quad-tree representation (use_quad_tree_rep): children split x (bit 0)   */
+/* and z (bit 1) at the centre, every node spans [y_min, y_max] in y         */
+/* (set_node_quad, HyperTree.cpp:391-406, 461-464).                          */
 /* ------------------------------------------------------------------------ */
-int64_t atdfo_tree_num_nodes(int max_depth)
+int64_t atdfo_tree_num_nodes_ns(int max_depth, int ns)
 {
     int64_t n = 0, p = 1;
     for (int i = 0; i <= max_depth; ++i) {
         n += p;
-        p *= 8;
+        p *= ns;
     }
     return n;
 }
 
-void atdfo_tree_build(int max_depth, float* node_min, float* node_max, int32_t* children,
-                      int32_t* parent, int32_t* depth)
+int64_t atdfo_tree_num_nodes(int max_depth) { return atdfo_tree_num_nodes_ns(max_depth, 8); }
+
+void atdfo_tree_build_ns(int max_depth, int ns, float y_min, float y_max, float* node_min, float* node_max,
+                         int32_t* children, int32_t* parent, int32_t* depth)
 {
-    int64_t num_nodes = atdfo_tree_num_nodes(max_depth);
-    for (int64_t i = 0; i < num_nodes * 8; ++i) children[i] = -1;
+    const int quad = ns == 4;
+    int64_t num_nodes = atdfo_tree_num_nodes_ns(max_depth, ns);
+    for (int64_t i = 0; i < num_nodes * ns; ++i) children[i] = -1;
     int64_t cur = 0;
     for (int k = 0; k < 3; ++k) {
         node_min[k] = -1.0f;
         node_max[k] = 1.0f;
+    }
+    if (quad) {
+        node_min[1] = y_min;
+        node_max[1] = y_max;
     }
     parent[0] = -1;
     depth[0] = 0;
@@ -45,9 +55,14 @@ void atdfo_tree_build(int max_depth, float* node_min, float* node_max, int32_t* 
             int64_t id = start_layer + j;
             float c[3];
             for (int k = 0; k < 3; ++k) c[k] = (node_min[id * 3 + k] + node_max[id * 3 + k]) * 0.5f;
-            for (int i = 0; i < 8; ++i) {
+            for (int i = 0; i < ns; ++i) {
                 for (int k = 0; k < 3; ++k) {
-                    if (((i >> k) & 1) == 0) {
+                    node_min[cur * 3 + k] = node_min[id * 3 + k];
+                    node_max[cur * 3 + k] = node_max[id * 3 + k];
+                }
+                for (int b = 0; b < (quad ? 2 : 3); ++b) {
+                    int k = quad && b == 1 ? 2 : b; /* split axes: x, y, z or x, z */
+                    if (((i >> b) & 1) == 0) {
                         node_min[cur * 3 + k] = node_min[id * 3 + k];
                         node_max[cur * 3 + k] = c[k];
                     } else {
@@ -55,14 +70,24 @@ void atdfo_tree_build(int max_depth, float* node_min, float* node_max, int32_t* 
                         node_max[cur * 3 + k] = node_max[id * 3 + k];
                     }
                 }
-                children[id * 8 + i] = (int32_t)cur;
+                if (quad) {
+                    node_min[cur * 3 + 1] = y_min;
+                    node_max[cur * 3 + 1] = y_max;
+                }
+                children[id * ns + i] = (int32_t)cur;
                 parent[cur] = (int32_t)id;
                 depth[cur] = dpt;
                 cur++;
             }
         }
-        start_layer = cur - n * 8;
+        start_layer = cur - n * ns;
     }
+}
+
+void atdfo_tree_build(int max_depth, float* node_min, float* node_max, int32_t* children, int32_t* parent,
+                      int32_t* depth)
+{
+    atdfo_tree_build_ns(max_depth, 8, -1.f, 1.f, node_min, node_max, children, parent, depth);
 }
 
 /* ------------------------------------------------------------------------ */

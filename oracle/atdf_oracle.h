@@ -34,6 +34,11 @@ extern "C" {
 
 /* ---- tree ------------------------------------------------------------- */
 int64_t atdfo_tree_num_nodes(int max_depth);
+/* ns = 8 (octree) or 4 (quad-tree representation: children split x and z,
+ * every node spans [y_min, y_max] in y; children [num_nodes, 4]) */
+int64_t atdfo_tree_num_nodes_ns(int max_depth, int ns);
+void atdfo_tree_build_ns(int max_depth, int ns, float y_min, float y_max, float* node_min, float* node_max,
+                         int32_t* children, int32_t* parent, int32_t* depth);
 /* BFS octree over [-1,1]^3: node 0 = root, children of node i contiguous,
  * child c takes the upper half on axis k iff bit k of c is set. */
 void atdfo_tree_build(int max_depth, float* node_min, float* node_max, int32_t* children,
@@ -209,6 +214,8 @@ float atdfo_prop_log(float x);
 /* Active node id (or -1) of every cell of the (2^depth)^3 lattice over the
  * root box (x fastest): descend from the root through the child whose half
  * holds the cell centre until an active node or a leaf. */
+void atdfo_leaf_table_ns(const float* node_min, const float* node_max, const int32_t* children, int ns,
+                         const int32_t* active, int depth, int32_t* table);
 void atdfo_leaf_table(const float* node_min, const float* node_max, const int32_t* children,
                       const int32_t* active, int depth, int32_t* table);
 int32_t atdfo_locate(const float* root_min, const float* root_max, const int32_t* table, int depth,
@@ -251,6 +258,17 @@ double atdfo_prop_grad(const float* params, const float* feat, const float* bins
  * from the root through the first child whose closed box holds p, or -1. */
 int32_t atdfo_active_node_for_position(const float* node_min, const float* node_max, const int32_t* children,
                                        const int32_t* active, const float* p);
+/* the same over a tree of `ns` children per node (4: the quad-tree
+ * representation, children [num_nodes, 4]) */
+int32_t atdfo_active_node_for_position_ns(const float* node_min, const float* node_max, const int32_t* children,
+                                          int ns, const int32_t* active, const float* p);
+void atdfo_node_id_for_position_ns(const float* node_min, const float* node_max, const int32_t* children, int ns,
+                                   const int32_t* active, int64_t first_active, const float* pos, int64_t n,
+                                   int64_t* node_id, float* mask);
+void atdfo_grid_transfer_ns(const float* node_min, const float* node_max, const int32_t* children, int ns,
+                            const int32_t* src_active, const int32_t* src_slot, const float* src_grid,
+                            const int64_t* dst_nodes, int64_t n_dst, const int32_t* dst_slot, float* dst_grid, int C,
+                            int G, int layout);
 /* NodeIdForPositionGPU: -1 becomes first_active with mask 0. */
 void atdfo_node_id_for_position(const float* node_min, const float* node_max, const int32_t* children,
                                 const int32_t* active, int64_t first_active, const float* pos, int64_t n,
@@ -296,6 +314,10 @@ double atdfo_tv_loss(const float* x, int64_t B, int64_t C, int D, const int* n, 
                      const int64_t* ss, const float* weight, double scale, float* grad);
 /* NodeNeighborSamples pairs in candidate order; returns the pair count and
  * writes at most cap pairs (pos [2P, 3], node [2P]). */
+int64_t atdfo_neighbor_pairs_ns(const float* node_min, const float* node_max, const int32_t* children, int ns,
+                                const int32_t* active, const int64_t* active_ids, int64_t num_active, const int* size,
+                                float eps, int use_roi, const float* roi_min, const float* roi_max, int variant,
+                                int plane, float* pos, int64_t* node_out, int64_t cap);
 int64_t atdfo_neighbor_pairs(const float* node_min, const float* node_max, const int32_t* children,
                              const int32_t* active, const int64_t* active_ids, int64_t num_active, const int* size,
                              float eps, int use_roi, const float* roi_min, const float* roi_max, int variant,

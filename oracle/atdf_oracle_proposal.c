@@ -86,8 +86,8 @@ float atdfo_prop_log(float x)
 /* ------------------------------------------------------------------------ */
 /* point location                                                            */
 /* ------------------------------------------------------------------------ */
-void atdfo_leaf_table(const float* node_min, const float* node_max, const int32_t* children,
-                      const int32_t* active, int depth, int32_t* table)
+void atdfo_leaf_table_ns(const float* node_min, const float* node_max, const int32_t* children, int ns,
+                         const int32_t* active, int depth, int32_t* table)
 {
     const int res = 1 << depth;
     float cell[3];
@@ -99,20 +99,28 @@ void atdfo_leaf_table(const float* node_min, const float* node_max, const int32_
         int32_t node = 0;
         while (1) {
             if (active[node]) break;
-            int32_t first = children[(int64_t)node * 8];
+            int32_t first = children[(int64_t)node * ns];
             if (first < 0) {
                 node = -1;
                 break;
             }
+            /* child bit b splits axis b (octree) or axes x, z (quad tree) */
             int sel = 0;
-            for (int k = 0; k < 3; ++k) {
+            for (int b = 0; b < (ns == 4 ? 2 : 3); ++b) {
+                int k = ns == 4 && b == 1 ? 2 : b;
                 float mid = (node_min[(int64_t)node * 3 + k] + node_max[(int64_t)node * 3 + k]) * 0.5f;
-                if (c[k] >= mid) sel |= 1 << k;
+                if (c[k] >= mid) sel |= 1 << b;
             }
             node = first + sel;
         }
         table[idx] = node;
     }
+}
+
+void atdfo_leaf_table(const float* node_min, const float* node_max, const int32_t* children,
+                      const int32_t* active, int depth, int32_t* table)
+{
+    atdfo_leaf_table_ns(node_min, node_max, children, 8, active, depth, table);
 }
 
 int32_t atdfo_locate(const float* root_min, const float* root_max, const int32_t* table, int depth,

@@ -67,14 +67,19 @@ class Model(C.Structure):
 
 
 # ------------------------------------------------------------------ tree
-def tree_build(max_depth: int):
-    n = lib().atdfo_tree_num_nodes(max_depth)
+def tree_build(max_depth: int, ns: int = 8, y_min: float = -1.0, y_max: float = 1.0):
+    """ns = 8: the octree; ns = 4: the quad-tree representation (children
+    split x and z, every node spans [y_min, y_max] in y)."""
+    L = lib()
+    L.atdfo_tree_num_nodes_ns.restype = C.c_int64
+    n = L.atdfo_tree_num_nodes_ns(C.c_int(max_depth), C.c_int(ns))
     nmin = np.zeros((n, 3), np.float32)
     nmax = np.zeros((n, 3), np.float32)
-    ch = np.zeros((n, 8), np.int32)
+    ch = np.zeros((n, ns), np.int32)
     par = np.zeros(n, np.int32)
     dep = np.zeros(n, np.int32)
-    lib().atdfo_tree_build(C.c_int(max_depth), _p(nmin), _p(nmax), _p(ch), _p(par), _p(dep))
+    L.atdfo_tree_build_ns(C.c_int(max_depth), C.c_int(ns), C.c_float(y_min), C.c_float(y_max), _p(nmin), _p(nmax),
+                          _p(ch), _p(par), _p(dep))
     return dict(node_min=nmin, node_max=nmax, children=ch, parent=par, depth=dep)
 
 
@@ -389,8 +394,9 @@ def prop_log(x):
 def leaf_table(node_min, node_max, children, active, depth):
     res = 1 << depth
     table = np.zeros(res ** 3, np.int32)
-    lib().atdfo_leaf_table(_p(_c(node_min, np.float32)), _p(_c(node_max, np.float32)), _p(_c(children, np.int32)),
-                           _p(_c(active, np.int32)), C.c_int(depth), _p(table))
+    ch = _c(children, np.int32)
+    lib().atdfo_leaf_table_ns(_p(_c(node_min, np.float32)), _p(_c(node_max, np.float32)), _p(ch), C.c_int(ch.shape[1]),
+                              _p(_c(active, np.int32)), C.c_int(depth), _p(table))
     return table
 
 
@@ -451,9 +457,11 @@ def prop_grad(params, feat, bins, power=1.0):
 
 # ------------------------------------------------------- point location / grid transfer
 def node_for_position(node_min, node_max, children, active, p):
-    return int(lib().atdfo_active_node_for_position(_p(_c(node_min, np.float32)), _p(_c(node_max, np.float32)),
-                                                    _p(_c(children, np.int32)), _p(_c(active, np.int32)),
-                                                    _p(_c(p, np.float32))))
+    ch = _c(children, np.int32)
+    fn = lib().atdfo_active_node_for_position_ns
+    fn.restype = C.c_int32
+    return int(fn(_p(_c(node_min, np.float32)), _p(_c(node_max, np.float32)), _p(ch), C.c_int(ch.shape[1]),
+                  _p(_c(active, np.int32)), _p(_c(p, np.float32))))
 
 
 def node_id_for_position(node_min, node_max, children, active, pos):
@@ -464,9 +472,10 @@ def node_id_for_position(node_min, node_max, children, active, pos):
     node_id = np.zeros(n, np.int64)
     mask = np.zeros(n, np.float32)
     first = int(np.nonzero(active)[0][0])
-    lib().atdfo_node_id_for_position(_p(_c(node_min, np.float32)), _p(_c(node_max, np.float32)),
-                                     _p(_c(children, np.int32)), _p(active), C.c_int64(first), _p(pos),
-                                     C.c_int64(n), _p(node_id), _p(mask))
+    ch = _c(children, np.int32)
+    lib().atdfo_node_id_for_position_ns(_p(_c(node_min, np.float32)), _p(_c(node_max, np.float32)), _p(ch),
+                                        C.c_int(ch.shape[1]), _p(active), C.c_int64(first), _p(pos), C.c_int64(n),
+                                        _p(node_id), _p(mask))
     return node_id, mask
 
 
@@ -483,8 +492,10 @@ def grid_transfer(node_min, node_max, children, src_active, src_slot, src_grid, 
     dst_nodes = _c(dst_nodes, np.int64)
     ss = None if src_slot is None else _c(src_slot, np.int32)
     ds = None if dst_slot is None else _c(dst_slot, np.int32)
-    lib().atdfo_grid_transfer(_p(_c(node_min, np.float32)), _p(_c(node_max, np.float32)), _p(_c(children, np.int32)),
-                              _p(_c(src_active, np.int32)), None if ss is None else _p(ss), _p(src_grid),
+    ch = _c(children, np.int32)
+    lib().atdfo_grid_transfer_ns(_p(_c(node_min, np.float32)), _p(_c(node_max, np.float32)), _p(ch),
+                                 C.c_int(ch.shape[1]),
+                                 _p(_c(src_active, np.int32)), None if ss is None else _p(ss), _p(src_grid),
                               _p(dst_nodes), C.c_int64(dst_nodes.shape[0]), None if ds is None else _p(ds),
                               _p(dst_grid), C.c_int(Cc), C.c_int(G), C.c_int(layout))
     return dst_grid
@@ -600,15 +611,18 @@ def tv_loss(x, channels_last=False, weight=None, scale=1.0, grad=None):
     return loss, g
 
 
-def neighbor_pairs(node_min, node_max, children, active, size, eps=1e-3, roi=None, quad=False):
-    """NodeNeighborSamples (quad: NodeNeighborSamplesQuad): (pos [2P, 3],
-    node [2P]) in candidate order."""
+def neighbor_pairs(node_min, node_max, children, active, size, eps=1e-3, roi=None, quad=False, plane=None):
+    """NodeNeighborSamples (quad: NodeNeighborSamplesQuad; plane given:
+    NodeNeighborSamples2D of the quad-tree representation on that plane):
+    (pos [2P, 3], node [2P]) in candidate order."""
     ids = active_ids(np.asarray(active))
-    args = [_p(_c(node_min, np.float32)), _p(_c(node_max, np.float32)), _p(_c(children, np.int32)),
+    ch = _c(children, np.int32)
+    variant = 2 if plane is not None else (1 if quad else 0)
+    args = [_p(_c(node_min, np.float32)), _p(_c(node_max, np.float32)), _p(ch), C.c_int(ch.shape[1]),
             _p(_c(active, np.int32)), _p(ids), C.c_int64(ids.size), _p(_c(size, np.int32)), C.c_float(eps),
             C.c_int(roi is not None), _p(_c(roi[0] if roi else (0, 0, 0), np.float32)),
-            _p(_c(roi[1] if roi else (0, 0, 0), np.float32)), C.c_int(1 if quad else 0)]
-    fn = lib().atdfo_neighbor_pairs
+            _p(_c(roi[1] if roi else (0, 0, 0), np.float32)), C.c_int(variant), C.c_int(plane or 0)]
+    fn = lib().atdfo_neighbor_pairs_ns
     fn.restype = C.c_int64
     n = fn(*args, None, None, C.c_int64(0))
     pos = np.zeros((2 * n, 3), np.float32)

@@ -18,10 +18,14 @@ def adaptive_active(tree_arrays, num_ellipsoids=30, seed=0, budget=400, start_de
                                         tree_arrays["depth"], ell, budget, start_depth, md)
 
 
-def device_tree(max_depth, active, device="cuda"):
-    """libtorch HyperTreeBase with the given active mask, moved to the GPU."""
+def device_tree(max_depth, active, device="cuda", quad=None):
+    """libtorch HyperTreeBase with the given active mask, moved to the GPU;
+    quad = (y_min, y_max) builds the quad-tree representation."""
     t = native().HyperTreeBase(3, max_depth)
-    t.settree()
+    if quad is None:
+        t.settree()
+    else:
+        t.settree(True, quad[0], quad[1])
     t.node_active = torch.from_numpy(np.ascontiguousarray(active, np.int32))
     t.UpdateActive()
     t.to(device)

@@ -108,6 +108,18 @@ def test_host_tree_matches_oracle(oracle, native):
     assert np.array_equal(t.node_to_grid.numpy(), pre.astype(np.int32))
 
 
+def test_host_quad_tree_matches_oracle(oracle, native):
+    t = native.HyperTreeBase(3, 3)
+    t.settree(True, -0.3, 0.45)
+    ref = oracle.tree_build(3, 4, -0.3, 0.45)
+    assert t.ChildrenPerNode() == 4
+    for a, b in (("node_position_min", "node_min"), ("node_position_max", "node_max"), ("node_children", "children"),
+                 ("node_parent", "parent"), ("node_depth", "depth")):
+        assert np.array_equal(getattr(t, a).numpy(), ref[b]), a
+    t.SetActive(3)
+    assert np.array_equal(t.active_node_ids.numpy(), np.arange(21, 85))
+
+
 def test_host_error_and_culling_semantics(native):
     t = native.HyperTreeBase(3, 2)
     t.settree()

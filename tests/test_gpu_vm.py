@@ -293,8 +293,13 @@ def test_node_neighbor_samples_bit_exact(oracle, native, cuda, use_roi, quad):
         assert np.array_equal(s.node_id.cpu().numpy(), node)
         assert np.array_equal(s.local_index_in_ray.cpu().numpy(), np.arange(pos.shape[0]) % 2)
         if quad:
-            with pytest.raises(RuntimeError, match="quad-tree representation"):
-                tree.NodeNeighborSamplesPlane2D([4, 3, 5], 1e-3, 0, lo, hi, use_roi, True)
+            # use_quad: NodeNeighborSamples2D, which the reference runs on
+            # whatever tree it is given (here an octree, four-child descent)
+            s2 = tree.NodeNeighborSamplesPlane2D([4, 3, 5], 1e-3, 1, lo, hi, use_roi, True)
+            pos2, node2 = oracle.neighbor_pairs(t["node_min"], t["node_max"], t["children"], act, (4, 3, 5),
+                                                eps=1e-3, roi=roi, plane=1)
+            assert np.array_equal(s2.global_coordinate.cpu().numpy(), pos2)
+            assert np.array_equal(s2.node_id.cpu().numpy(), node2)
 
 
 def test_edge_loss_and_regularizer(oracle, native, cuda):

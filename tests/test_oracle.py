@@ -478,3 +478,52 @@ def test_area_traversal_closed_form(oracle, K):
     assert r["ray_num_samples"].max() <= K
     pos = r["global_coordinate"]
     assert np.all(pos >= ROI_MIN - 1e-6) and np.all(pos <= ROI_MAX + 1e-6)
+
+
+def test_quad_tree_build_closed_form(oracle):
+    """The quad-tree representation (structure/HyperTree.cpp:391-406,
+    461-464): sum 4^d nodes, BFS numbering with contiguous children, child c
+    the quadrant with bit 0 -> upper x and bit 1 -> upper z, every node
+    spanning [y_min, y_max]."""
+    y0, y1 = -0.3, 0.45
+    t = oracle.tree_build(3, 4, y0, y1)
+    n = 1 + 4 + 16 + 64
+    assert t["node_min"].shape == (n, 3) and t["children"].shape == (n, 4)
+    assert np.all(t["node_min"][:, 1] == np.float32(y0)) and np.all(t["node_max"][:, 1] == np.float32(y1))
+    for i in range(1 + 4 + 16):
+        ch = t["children"][i]
+        assert list(ch) == list(range(4 * i + 1, 4 * i + 5)) and np.all(t["parent"][ch] == i)
+        mn, mx = t["node_min"][i], t["node_max"][i]
+        mid = (mn + mx) / 2
+        for c in range(4):
+            lo = [mid[0] if c & 1 else mn[0], y0, mid[2] if c & 2 else mn[2]]
+            hi = [mx[0] if c & 1 else mid[0], y1, mx[2] if c & 2 else mid[2]]
+            assert np.allclose(t["node_min"][ch[c]], lo, atol=0) and np.allclose(t["node_max"][ch[c]], hi, atol=0)
+    leaves = np.nonzero(t["depth"] == 3)[0]
+    assert leaves.size == 64
+    side = t["node_max"][leaves] - t["node_min"][leaves]
+    assert np.all(side[:, 0] == 0.25) and np.all(side[:, 2] == 0.25)
+
+
+def test_quad_tree_neighbor_pairs_closed_form(oracle):
+    """NodeNeighborSamples2D on plane 1 over the four depth-1 quadrants: every
+    point of each node's x and z faces toward a sibling pairs, 4 (s0 + s2)
+    pairs.  With an ROI whose y range starts 1.5 eps above the nodes' upper y
+    face, only candidates whose thread (sid of 128) has widened the range at
+    least twice pass.  Thread t ran the tt = 0 loops only if t < s0, so with
+    s0 < s2: node 1 keeps s0 + s2 pairs, node 2 s0 + min(s0, s2), node 3 s2
+    and node 4 min(s0, s2), 2 (s0 + s2 + min(s0, s2)) in all."""
+    t = oracle.tree_build(1, 4, -0.4, 0.4)
+    act = (t["depth"] == 1).astype(np.int32)
+    s = (5, 3, 7)
+    pos, node = oracle.neighbor_pairs(t["node_min"], t["node_max"], t["children"], act, s, eps=1e-3, plane=1)
+    assert node.size == 2 * 4 * (s[0] + s[2])
+    assert np.all(pos[:, 1] == np.float32(-0.4))  # the face local[plane] = 0
+    assert np.all(node[0::2] != node[1::2])
+    eps = np.float32(1e-3)
+    roi = ((-2.0, float(np.float32(0.4) + np.float32(1.5) * eps), -2.0), (2.0, 2.0, 2.0))
+    pos, node = oracle.neighbor_pairs(t["node_min"], t["node_max"], t["children"], act, s, eps=1e-3, roi=roi,
+                                      plane=1)
+    assert node.size == 2 * 2 * (s[0] + s[2] + min(s[0], s[2]))
+    per = np.bincount(node[0::2], minlength=5)
+    assert list(per[1:]) == [s[0] + s[2], s[0] + min(s[0], s[2]), s[2], min(s[0], s[2])]

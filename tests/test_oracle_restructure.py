@@ -169,3 +169,21 @@ def test_transfer_matches_numpy_restatement(oracle):
                                full.copy(), 1)
     want = _transfer_numpy(t, old, None, full, inactive, None, full.copy(), 1)
     assert np.array_equal(got, want)
+
+
+def test_quad_tree_point_location_matches_halfopen_partition(oracle):
+    """The quad-tree representation (four children splitting x and z, every
+    node spanning [y_min, y_max]): the descent equals the half-open
+    partition; points outside the y range are outside every node."""
+    rng = np.random.default_rng(4)
+    for depth, seed in ((3, 0), (4, 1)):
+        t = oracle.tree_build(depth, 4, -0.4, 0.4)
+        act = adaptive_active(t, budget=90, seed=seed)
+        assert t["children"].shape[1] == 4 and act.sum() > 30
+        pts = _points(depth, rng)
+        pts[::7, 1] = np.float32(0.4)   # on the upper y face (closed)
+        pts[1::7, 1] = np.float32(-0.4)  # on the lower y face (a root face: closed)
+        got = [oracle.node_for_position(t["node_min"], t["node_max"], t["children"], act, p) for p in pts]
+        want = [_halfopen_locate(t, act, p) for p in pts]
+        assert got == want
+        assert -1 in want and sum(w >= 0 for w in want) > len(want) // 3

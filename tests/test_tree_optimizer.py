@@ -6,12 +6,9 @@ import itertools
 import numpy as np
 import pytest
 
-NS = 8
-
-
-def _tree(native, depth, active, culled, err, dens):
+def _tree(native, depth, active, culled, err, dens, quad=False):
     t = native.HyperTreeBase(3, depth)
-    t.settree()
+    t.settree(quad, -0.4, 0.4)
     import torch
 
     t.node_active = torch.from_numpy(active.astype(np.int32))
@@ -36,6 +33,7 @@ def _brute(t, p):
     cul = t.node_culled.numpy()
     par = t.node_parent.numpy()
     ch = t.node_children.numpy()
+    NS = ch.shape[1]  # 8, or 4 for the quad-tree representation
     err = t.node_error.numpy()
     den = t.node_max_density.numpy()
     mn, mx = t.node_position_min.numpy(), t.node_position_max.numpy()
@@ -126,8 +124,8 @@ def _check_decision(t, p, dec, info):
     return cost
 
 
-def _random_case(native, rng, depth, active_ids, culled_ids, saved):
-    n = sum(8 ** k for k in range(depth + 1))
+def _random_case(native, rng, depth, active_ids, culled_ids, saved, quad=False):
+    n = sum((4 if quad else 8) ** k for k in range(depth + 1))
     active = np.zeros(n, np.int32)
     active[list(active_ids)] = 1
     culled = np.zeros(n, np.int32)
@@ -136,7 +134,7 @@ def _random_case(native, rng, depth, active_ids, culled_ids, saved):
     if not saved:
         err[~active.astype(bool)] = -1
     dens = rng.uniform(0.0, 2.0, n).astype(np.float32)
-    return _tree(native, depth, active, culled, err, dens)
+    return _tree(native, depth, active, culled, err, dens, quad)
 
 
 CASES = [
@@ -147,13 +145,23 @@ CASES = [
 ]
 
 
-@pytest.mark.parametrize("case", range(len(CASES)))
+# the quad-tree representation: children 4i+1 .. 4i+4 of node i
+QUAD_CASES = [
+    (3, range(1, 5), []),                                  # one full sibling group at depth 1
+    (2, [1, 2] + list(range(13, 17)), [4]),                # a culled sibling, one split node
+    (3, [2, 3, 4, 5, 6] + list(range(21, 25)), [7, 8]),    # culled grand-children, a split child
+]
+
+
+@pytest.mark.parametrize("case", [("oct", i) for i in range(len(CASES))] + [("quad", i) for i in range(len(QUAD_CASES))])
 @pytest.mark.parametrize("saved", [True, False])
 def test_dp_equals_exhaustive_mip(native, case, saved):
-    depth, act_ids, cul_ids = CASES[case]
-    rng = np.random.default_rng(10 * case + saved)
+    kind, ci = case
+    quad = kind == "quad"
+    depth, act_ids, cul_ids = (QUAD_CASES if quad else CASES)[ci]
+    rng = np.random.default_rng(10 * ci + saved + (100 if quad else 0))
     for trial in range(4):
-        t = _random_case(native, rng, depth, act_ids, cul_ids, saved)
+        t = _random_case(native, rng, depth, act_ids, cul_ids, saved, quad)
         na = int(t.node_active.numpy().sum())
         for budget in (na - 3, na, na + 7, 200):
             p = native.TreeOptimizerParams()
@@ -168,6 +176,8 @@ def test_dp_equals_exhaustive_mip(native, case, saved):
                 continue
             assert dec.feasible
             assert abs(dec.objective - best[0]) <= 1e-12 * max(1.0, abs(best[0])), (case, trial, budget)
+            if quad:
+                assert t.ChildrenPerNode() == 4
             cost = _check_decision(t, p, dec, info)
             assert abs(cost - dec.objective) <= 1e-12 * max(1.0, abs(cost))
             assert dec.active_weight <= p.max_active_nodes
@@ -218,3 +228,35 @@ def test_node_culling(native):
     assert t.NumActiveNodes() == 64 - 3
     assert np.all(t.node_active.numpy()[low] == 0)
     assert mn.shape == (3,)
+
+
+def test_quad_tree_apply_split_and_culling(native):
+    """ApplySplit and UpdateCulling over four children per node."""
+    import torch
+
+    rng = np.random.default_rng(5)
+    t = _random_case(native, rng, 2, range(1, 5), [], True, quad=True)
+    err = t.node_error.numpy().copy()
+    err[1:5] = 1.0
+    err[5:9] = 0.0001  # the children of node 1 (5..8) are near-perfect: node 1 splits
+    t.node_error = torch.from_numpy(err)
+    t.node_max_density = torch.ones_like(t.node_max_density)
+    p = native.TreeOptimizerParams()
+    p.max_active_nodes = 7
+    changed, dec = native.OptimizeTreeStructure(t, p)
+    act = t.node_active.numpy()
+    assert changed and dec.choice[1] == 1
+    assert act[1] == 0 and np.all(act[5:9] == 1) and np.all(act[2:5] == 1) and t.NumActiveNodes() == 7
+    # culling: every child of node 2 culled -> node 2 is culled too
+    t2 = native.HyperTreeBase(3, 2)
+    t2.settree(True, -0.4, 0.4)
+    t2.SetActive(2)
+    cul = np.zeros(t2.NumNodes(), np.int32)
+    cul[9:13] = 1
+    act = t2.node_active.numpy().copy()
+    act[9:13] = 0
+    t2.node_culled = torch.from_numpy(cul)
+    t2.node_active = torch.from_numpy(act)
+    t2.UpdateActive()
+    t2.UpdateCulling()
+    assert t2.node_culled.numpy()[2] == 1 and t2.node_culled.numpy()[1] == 0
