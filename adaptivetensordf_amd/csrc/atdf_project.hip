@@ -967,25 +967,24 @@ __device__ __forceinline__ void make_sample_node(const ProjArgs& a, const PNode&
 }
 
 // Workgroup lock in LDS guarding the node gradient.  Lane 0 polls with a
-// compare-and-swap; the holder runs a bounded scatter and releases, so the
-// wait ends.  Returns 0 only if the lock was not obtained within 2^22 polls.
-constexpr int kLockPolls = 1 << 22;
-
-__device__ __forceinline__ int lds_lock(int* lock, int lane)
+// compare-and-swap until it gets the lock.  The wait ends: a holder runs a
+// fixed sequence of LDS read-add-writes (it waits on no other wave and on no
+// global memory) and releases, and the waves of a workgroup take the lock a
+// finite number of times (once per sample tile of the node's chunk), so a
+// waiting wave gets it at the latest when the other waves have finished
+// their scatters.
+__device__ __forceinline__ void lds_lock(int* lock, int lane)
 {
-    int got = 0;
     if (lane == 0) {
-        for (int it = 0; it < kLockPolls; ++it) {
+        while (true) {
             int expected = 0;
             if (__hip_atomic_compare_exchange_strong(lock, &expected, 1, __ATOMIC_ACQUIRE, __ATOMIC_RELAXED,
-                                                     __HIP_MEMORY_SCOPE_WORKGROUP)) {
-                got = 1;
+                                                     __HIP_MEMORY_SCOPE_WORKGROUP))
                 break;
-            }
             __builtin_amdgcn_s_sleep(1);
         }
     }
-    return __shfl(got, 0, 64);
+    __builtin_amdgcn_wave_barrier();
 }
 
 __device__ __forceinline__ void lds_unlock(int* lock, int lane)
@@ -1141,8 +1140,8 @@ __global__ __launch_bounds__(64 * bwd_waves<kDecBF16>()) void project_bwd_node_k
             const bool plain = live && !border && !dup && 4 * g < C;
             const bool atomic = live && (border || dup) && 4 * g < C;
             if (__any(plain || atomic)) {
-                const int got = lds_lock(&s_lock, lane);
-                if (got && plain) {
+                lds_lock(&s_lock, lane);
+                if (plain) {
 #pragma unroll
                     for (int k = 0; k < 8; ++k) {
                         float4* gp = reinterpret_cast<float4*>(lgrad + corner_cell(sc.tri, k, G) * C + 4 * g);
@@ -1156,8 +1155,7 @@ __global__ __launch_bounds__(64 * bwd_waves<kDecBF16>()) void project_bwd_node_k
                     }
                 }
                 wave_lds_sync();
-                // without the lock (never seen: 2^22 polls) every update is atomic
-                if (atomic || (!got && plain)) {
+                if (atomic) {
 #pragma unroll
                     for (int k = 0; k < 8; ++k) {
                         float* gp = lgrad + corner_cell(sc.tri, k, G) * C + 4 * g;
@@ -1166,7 +1164,7 @@ __global__ __launch_bounds__(64 * bwd_waves<kDecBF16>()) void project_bwd_node_k
                         for (int r4 = 0; r4 < 4; ++r4) atomicAdd(gp + r4, wk * dx[r4]);
                     }
                 }
-                if (got) lds_unlock(&s_lock, lane);
+                lds_unlock(&s_lock, lane);
             }
         }
         __syncthreads();

@@ -185,10 +185,18 @@ Tensor parse_npy(const unsigned char* buf, size_t size, const std::string& what)
         int64_t v;
         while (ss >> v) shape.push_back(v);
     }
-    int64_t n = 1;
-    for (auto v : shape) n *= v;
+    TORCH_CHECK(bytes >= 1 && bytes <= 8, what, ": unsupported dtype '", descr, "'");
     const size_t data_off = hoff + hlen;
-    TORCH_CHECK(data_off + (size_t)n * bytes <= size, what, ": data shorter than the shape ", n, " x ", bytes, " B");
+    // element count with overflow checks: every dimension >= 0 and the total
+    // byte count within what the file holds
+    const size_t avail = size - data_off;
+    int64_t n = 1;
+    for (auto v : shape) {
+        TORCH_CHECK(v >= 0, what, ": negative dimension ", v, " in the NPY shape");
+        TORCH_CHECK(v == 0 || n <= (int64_t)(avail / (size_t)bytes) / v, what, ": data shorter than the shape");
+        n *= v;
+    }
+    TORCH_CHECK((size_t)n * (size_t)bytes <= avail, what, ": data shorter than the shape ", n, " x ", bytes, " B");
     const unsigned char* src = buf + data_off;
     c10::ScalarType dt;
     if (kind == 'f' && bytes == 4)
@@ -268,35 +276,52 @@ Tensor LoadNumpyNpz(const std::string& path, const std::string& varname)
     if (cd_off == 0xffffffffu || entries == 0xffffu) {
         TORCH_CHECK(eocd >= 20 && rd32(&z[eocd - 20]) == 0x07064b50u, path, ": missing ZIP64 locator");
         const uint64_t z64 = rd64(&z[eocd - 20 + 8]);
-        TORCH_CHECK(z64 + 56 <= n && rd32(&z[z64]) == 0x06064b50u, path, ": bad ZIP64 end record");
+        TORCH_CHECK(z64 < n && n - z64 >= 56 && rd32(&z[z64]) == 0x06064b50u, path, ": bad ZIP64 end record");
         entries = rd64(&z[z64 + 32]);
         cd_off = rd64(&z[z64 + 48]);
     }
     const std::string want = varname + ".npy";
     size_t p = (size_t)cd_off;
+    TORCH_CHECK(cd_off <= n, path, ": central directory outside the archive");
     for (uint64_t e = 0; e < entries; ++e) {
-        TORCH_CHECK(p + 46 <= n && rd32(&z[p]) == 0x02014b50u, path, ": bad central directory entry");
+        TORCH_CHECK(p <= n && n - p >= 46 && rd32(&z[p]) == 0x02014b50u, path, ": bad central directory entry");
         const uint32_t method = rd16(&z[p + 10]);
         uint64_t csize = rd32(&z[p + 20]), usize = rd32(&z[p + 24]);
         const uint32_t name_len = rd16(&z[p + 28]), extra_len = rd16(&z[p + 30]), comment_len = rd16(&z[p + 32]);
         uint64_t local = rd32(&z[p + 42]);
+        TORCH_CHECK(n - p - 46 >= (size_t)name_len + extra_len + comment_len, path,
+                    ": central directory entry runs past the archive");
         const std::string name(reinterpret_cast<const char*>(&z[p + 46]), name_len);
         // ZIP64 extra field: the saturated sizes / offset, in that order
-        for (size_t x = p + 46 + name_len; x + 4 <= p + 46 + name_len + extra_len;) {
+        const size_t xend = p + 46 + name_len + extra_len;
+        for (size_t x = p + 46 + name_len; x + 4 <= xend;) {
             const uint32_t id = rd16(&z[x]), len = rd16(&z[x + 2]);
+            TORCH_CHECK(x + 4 + len <= xend, path, ": bad extra field in the entry of ", name);
             if (id == 0x0001) {
                 size_t q = x + 4;
-                if (usize == 0xffffffffu) usize = rd64(&z[q]), q += 8;
-                if (csize == 0xffffffffu) csize = rd64(&z[q]), q += 8;
-                if (local == 0xffffffffu) local = rd64(&z[q]);
+                const size_t qend = x + 4 + len;
+                if (usize == 0xffffffffu) {
+                    TORCH_CHECK(q + 8 <= qend, path, ": short ZIP64 extra field");
+                    usize = rd64(&z[q]), q += 8;
+                }
+                if (csize == 0xffffffffu) {
+                    TORCH_CHECK(q + 8 <= qend, path, ": short ZIP64 extra field");
+                    csize = rd64(&z[q]), q += 8;
+                }
+                if (local == 0xffffffffu) {
+                    TORCH_CHECK(q + 8 <= qend, path, ": short ZIP64 extra field");
+                    local = rd64(&z[q]);
+                }
             }
             x += 4 + len;
         }
-        p += 46 + name_len + extra_len + comment_len;
+        p += 46 + (size_t)name_len + extra_len + comment_len;
         if (name != want) continue;
-        TORCH_CHECK(local + 30 <= n && rd32(&z[local]) == 0x04034b50u, path, ": bad local header of ", name);
+        TORCH_CHECK(local < n && n - local >= 30 && rd32(&z[local]) == 0x04034b50u, path, ": bad local header of ",
+                    name);
         const size_t data = (size_t)local + 30 + rd16(&z[local + 26]) + rd16(&z[local + 28]);
-        TORCH_CHECK(data + csize <= n, path, ": member ", name, " is truncated");
+        TORCH_CHECK(data <= n && csize <= n - data, path, ": member ", name, " is truncated");
+        TORCH_CHECK(usize <= ((uint64_t)1 << 40), path, ": member ", name, " is implausibly large");
         if (method == 0) return parse_npy(&z[data], (size_t)csize, path + ":" + name);
         TORCH_CHECK(method == 8, path, ": member ", name, " uses unsupported compression method ", method);
         std::vector<unsigned char> out((size_t)usize);

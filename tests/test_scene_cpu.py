@@ -235,6 +235,59 @@ def test_npy_reader_rejects(native, tmp_path):
         native.LoadNumpy(p)
 
 
+def _npy_with_shape(shape_text):
+    """A little-endian f4 NPY whose header states `shape_text` and holds
+    16 bytes of data."""
+    hdr = "{'descr': '<f4', 'fortran_order': False, 'shape': %s, }" % shape_text
+    hdr += " " * ((64 - (10 + len(hdr) + 1) % 64) % 64) + "\n"
+    return b"\x93NUMPY\x01\x00" + len(hdr).to_bytes(2, "little") + hdr.encode() + bytes(16)
+
+
+def test_npy_reader_rejects_bad_shapes(native, tmp_path):
+    p = str(tmp_path / "s.npy")
+    open(p, "wb").write(_npy_with_shape("(2, 2)"))
+    assert tuple(native.LoadNumpy(p).shape) == (2, 2)
+    for bad, msg in (("(-1, 4)", "negative dimension"), ("(3037000500, 3037000500)", "shorter"),
+                     ("(4611686018427387904, 4)", "shorter"), ("(2, 3)", "shorter")):
+        open(p, "wb").write(_npy_with_shape(bad))
+        with pytest.raises(RuntimeError, match=msg):
+            native.LoadNumpy(p)
+
+
+def test_npz_reader_rejects_corrupt_directory(native, tmp_path):
+    p = str(tmp_path / "c.npz")
+    np.savez(p, proj=np.arange(12, dtype=np.float32))
+    good = bytearray(open(p, "rb").read())
+    cd = good.rfind(b"PK\x01\x02")
+    assert cd > 0
+    # name length running past the end of the archive
+    z = bytearray(good)
+    z[cd + 28:cd + 30] = (0xFFFF).to_bytes(2, "little")
+    open(p, "wb").write(bytes(z))
+    with pytest.raises(RuntimeError):
+        native.LoadNumpyNpz(p, "proj")
+    # extra field whose length runs past the entry's extra area
+    z = bytearray(good)
+    name_len = int.from_bytes(z[cd + 28:cd + 30], "little")
+    extra_len = int.from_bytes(z[cd + 30:cd + 32], "little")
+    if extra_len >= 4:
+        x = cd + 46 + name_len
+        z[x + 2:x + 4] = (extra_len + 100).to_bytes(2, "little")
+        open(p, "wb").write(bytes(z))
+        with pytest.raises(RuntimeError, match="extra field"):
+            native.LoadNumpyNpz(p, "proj")
+    # local header offset outside the archive
+    z = bytearray(good)
+    z[cd + 42:cd + 46] = (len(z) + 10).to_bytes(4, "little")
+    open(p, "wb").write(bytes(z))
+    with pytest.raises(RuntimeError, match="local header"):
+        native.LoadNumpyNpz(p, "proj")
+    # truncated archive
+    open(p, "wb").write(bytes(good[: len(good) // 2]))
+    with pytest.raises(RuntimeError):
+        native.LoadNumpyNpz(p, "proj")
+
+
 # ---------------------------------------------------------------------------
 # scene directory
 # ---------------------------------------------------------------------------
