@@ -966,6 +966,165 @@ __device__ __forceinline__ void make_sample_node(const ProjArgs& a, const PNode&
     sc.gbase = gbase;
 }
 
+// Software pipeline of the adjoint's tile loop.  The node gradient fills
+// most of the LDS, so a CU runs one workgroup (one wave per SIMD) and the
+// loop hides its own memory latency: the stages below split a tile into
+// (1) mapping its samples to intervals and loading their records,
+// (2) placing the samples and issuing the corner gathers, (3) the decoder,
+// its adjoint and the scatter, and the loop runs them on three consecutive
+// tiles.  Each tile's arithmetic is the same as in an unpipelined loop and
+// comes in the same order.
+
+// Wave-uniform interval cursor over the wave's sorted intervals [.., ie):
+// `cur` is the interval holding the next tile's first sample, `cum` the
+// number of the wave's samples before it, and `cnt` the sample count of
+// candidate interval cur + j (0 past ie), loaded one tile ahead.
+struct TileCursor {
+    int cur, cum, cnt;
+};
+
+// Stage 1 result for this lane's sample: its interval's records.
+struct TileMap {
+    bool valid;
+    int jj, n_s;
+    float4 A, B, D;
+};
+
+// Stage 1 of tile ti: maps the tile's 16 samples onto the intervals from
+// the cursor on (a window of one-sample intervals, as learned lists have,
+// maps sample j to candidate j directly; otherwise the counts are
+// prefix-summed across the lane group and each lane binary-searches),
+// loads the records of this lane's interval, advances the cursor and loads
+// the next window's counts.  Every lane of the wave runs it.
+__device__ __forceinline__ void map_tile(const float4* __restrict__ rec, int ie, int ti, int tot, int j, int grp,
+                                         TileCursor& tc, TileMap& m)
+{
+    const int base = ti * kTile;
+    const int t = base + j;
+    const int cand = tc.cur + j;
+    const int c_cnt = tc.cnt;
+    const bool ones = __all(cand >= ie || c_cnt == 1);
+    int c_end = c_cnt;
+    int pos, posn, s_cnt, s_end, last_end = 0;
+    if (ones) {
+        pos = j;
+        posn = min(kTile, ie - tc.cur);
+        s_cnt = 1;
+        s_end = tc.cum + j + 1;
+    } else {
+        c_end = scan16_inclusive(c_end);
+        c_end = cand < ie ? c_end + tc.cum : INT_MAX;
+        pos = group_search(c_end, t, grp);
+        posn = group_search(c_end, base + kTile, grp);
+        const int src = grp | (pos < kTile ? pos : kTile - 1);
+        s_cnt = __shfl(c_cnt, src, 64);
+        s_end = __shfl(c_end, src, 64);
+        last_end = __shfl(c_end, grp | (kTile - 1), 64);
+    }
+    m.valid = t < tot;
+    if (m.valid) {
+        int k = tc.cur + pos, n_s = s_cnt, start = s_end - s_cnt;
+        if (pos >= kTile) {
+            // only with zero-sample intervals in the list: walk on
+            k = tc.cur + kTile;
+            start = last_end;
+            n_s = __float_as_int(rec[3 * (int64_t)k].z);
+            while (start + n_s <= t) {
+                start += n_s;
+                ++k;
+                n_s = __float_as_int(rec[3 * (int64_t)k].z);
+            }
+        }
+        m.n_s = n_s;
+        m.jj = t - start;
+        m.A = rec[3 * (int64_t)k];
+        m.B = rec[3 * (int64_t)k + 1];
+        m.D = rec[3 * (int64_t)k + 2];
+    }
+    if (ones)
+        tc.cum += posn;
+    else if (posn > 0)
+        tc.cum = __shfl(c_end, grp | (posn - 1), 64);
+    tc.cur += posn;
+    const int nc = tc.cur + j;
+    tc.cnt = nc < ie ? __float_as_int(rec[3 * (int64_t)nc].z) : 0;
+}
+
+// Raw corner records of one lane's CPL channels (f32, or bf16 pairs), as
+// loaded; decoded by corners_combine.
+template <int C, bool kBF16>
+struct Corners {
+    static constexpr int CPL = C / 4;
+    static constexpr int W = kBF16 ? CPL / 2 : CPL;  // dwords per corner
+    uint32_t u[8][W];
+};
+
+// Stage 2: places this lane's sample, issues the gathers of its 8 corners
+// (channels 4g.., C = 16; 2g.., C = 8) and returns dL/dP of its ray (0 for
+// a lane without a sample).
+template <int C, bool kBF16>
+__device__ __forceinline__ float sample_stage(const ProjArgs& a, const float* __restrict__ dLdP, const PNode& nd,
+                                              int64_t gbase, const TileMap& m, int g, SampleCtx& sc,
+                                              Corners<C, kBF16>& cr)
+{
+    constexpr int CPL = C / 4;
+    constexpr int W = Corners<C, kBF16>::W;
+    sc.valid = m.valid;
+    sc.w = 0.f;
+    sc.gbase = -1;
+    if (!m.valid) return 0.f;
+    const float o[3] = {m.B.x, m.B.y, m.B.z};
+    const float d[3] = {m.B.w, m.D.x, m.D.y};
+    const int r = __float_as_int(m.D.w);
+    make_sample_node(a, nd, gbase, r, o, d, m.A.x, m.A.y, m.n_s, m.jj, __float_as_int(m.A.w) + m.jj, sc);
+    const float dp = dLdP[r];
+    const unsigned char* grid = reinterpret_cast<const unsigned char*>(a.grid);
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        const int64_t off = sc.gbase + corner_cell(sc.tri, k, a.G) * C + CPL * g;
+        const unsigned char* p = grid + off * (kBF16 ? 2 : 4);
+        if constexpr (W == 4) {
+            const uint4 v = *reinterpret_cast<const uint4*>(p);
+            cr.u[k][0] = v.x;
+            cr.u[k][1] = v.y;
+            cr.u[k][2] = v.z;
+            cr.u[k][3] = v.w;
+        } else if constexpr (W == 2) {
+            const uint2 v = *reinterpret_cast<const uint2*>(p);
+            cr.u[k][0] = v.x;
+            cr.u[k][1] = v.y;
+        } else {
+            cr.u[k][0] = *reinterpret_cast<const uint32_t*>(p);
+        }
+    }
+    return dp;
+}
+
+// Trilinear combination of the loaded corners: canonical corner order per
+// channel, accumulated as a fused multiply-add chain (as gather_channels).
+template <int C, bool kBF16>
+__device__ __forceinline__ void corners_combine(const SampleCtx& sc, const Corners<C, kBF16>& cr, float* x)
+{
+    constexpr int CPL = C / 4;
+#pragma unroll
+    for (int q = 0; q < CPL; ++q) x[q] = 0.f;
+    if (!sc.valid) return;
+#pragma unroll
+    for (int q = 0; q < CPL; ++q) {
+        float acc = 0.f;
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            float v;
+            if constexpr (kBF16)
+                v = (q & 1) ? __uint_as_float(cr.u[k][q >> 1] & 0xffff0000u) : __uint_as_float(cr.u[k][q >> 1] << 16);
+            else
+                v = __uint_as_float(cr.u[k][q]);
+            acc = fmaf(v, sc.tri.w[k], acc);
+        }
+        x[q] = acc;
+    }
+}
+
 // Workgroup lock in LDS guarding the node gradient.  Lane 0 polls with a
 // compare-and-swap until it gets the lock.  The wait ends: a holder runs a
 // fixed sequence of LDS read-add-writes (it waits on no other wave and on no
@@ -1049,80 +1208,39 @@ __global__ __launch_bounds__(64 * bwd_waves<kDecBF16>()) void project_bwd_node_k
         for (int off = 32; off > 0; off >>= 1) tot += __shfl_xor(tot, off, 64);
         const int ntiles = (tot + kTile - 1) / kTile;
 
-        // wave-uniform cursor: `cur` holds the tile's first sample, `cum`
-        // samples of this wave's list precede it
-        int cur = ib, cum = 0;
+        // Software-pipelined tile loop: while tile ti runs the decoder and
+        // the scatter, the corner gathers of tile ti+1, the interval records
+        // of tile ti+2 and the interval counts of tile ti+3 are in flight.
         const int grp = lane & 48;
+        TileCursor tc;
+        tc.cur = ib;
+        tc.cum = 0;
+        tc.cnt = ib + j < ie ? __float_as_int(rec[3 * (int64_t)(ib + j)].z) : 0;
+        TileMap m1;
+        m1.valid = false;
+        if (ntiles > 0) map_tile(rec, ie, 0, tot, j, grp, tc, m1);
+        SampleCtx sc0;
+        Corners<C, kBF16> cr0;
+        float dp0 = sample_stage<C, kBF16>(a, b.dLdP, nd, gbase, m1, g, sc0, cr0);
+        m1.valid = false;
+        if (ntiles > 1) map_tile(rec, ie, 1, tot, j, grp, tc, m1);
         for (int ti = 0; ti < ntiles; ++ti) {
-            const int base = ti * kTile;
-            const int t = base + j;
-            // the tile's samples lie in the 16 intervals from `cur` on (every
-            // interval holds at least one sample): read their counts once,
-            // prefix-sum them across the lane group, binary-search per lane
-            // (a window of one-sample intervals, as learned lists have, maps
-            // sample j to candidate j directly)
-            const int cand = cur + j;
-            const int c_cnt = cand < ie ? __float_as_int(rec[3 * (int64_t)cand].z) : 0;
-            const bool ones = __all(cand >= ie || c_cnt == 1);
-            int c_end = c_cnt;
-            int pos, posn, s_cnt, s_end, last_end = 0;
-            if (ones) {
-                pos = j;
-                posn = min(kTile, ie - cur);
-                s_cnt = 1;
-                s_end = cum + j + 1;
-            } else {
-                c_end = scan16_inclusive(c_end);
-                c_end = cand < ie ? c_end + cum : INT_MAX;
-                pos = group_search(c_end, t, grp);
-                posn = group_search(c_end, base + kTile, grp);
-                const int src = grp | (pos < kTile ? pos : kTile - 1);
-                s_cnt = __shfl(c_cnt, src, 64);
-                s_end = __shfl(c_end, src, 64);
-                last_end = __shfl(c_end, grp | (kTile - 1), 64);
-            }
-            SampleCtx sc;
-            sc.valid = t < tot;
-            sc.w = 0.f;
-            sc.gbase = -1;
-            float dp = 0.f;
-            if (sc.valid) {
-                int k = cur + pos, n_s = s_cnt, start = s_end - s_cnt;
-                if (pos >= kTile) {
-                    // only with zero-sample intervals in the list: walk on
-                    k = cur + kTile;
-                    start = last_end;
-                    n_s = __float_as_int(rec[3 * (int64_t)k].z);
-                    while (start + n_s <= t) {
-                        start += n_s;
-                        ++k;
-                        n_s = __float_as_int(rec[3 * (int64_t)k].z);
-                    }
-                }
-                const float4 A = rec[3 * (int64_t)k];
-                const float4 B = rec[3 * (int64_t)k + 1];
-                const float4 D = rec[3 * (int64_t)k + 2];
-                const float o[3] = {B.x, B.y, B.z};
-                const float d[3] = {B.w, D.x, D.y};
-                const int jj = t - start;
-                make_sample_node(a, nd, gbase, __float_as_int(D.w), o, d, A.x, A.y, n_s, jj,
-                                 __float_as_int(A.w) + jj, sc);
-                dp = b.dLdP[__float_as_int(D.w)];
-            }
-            if (ones)
-                cum += posn;
-            else if (posn > 0)
-                cum = __shfl(c_end, grp | (posn - 1), 64);
-            cur += posn;
+            SampleCtx sc1;
+            Corners<C, kBF16> cr1;
+            const float dp1 = sample_stage<C, kBF16>(a, b.dLdP, nd, gbase, m1, g, sc1, cr1);
+            m1.valid = false;
+            if (ti + 2 < ntiles) map_tile(rec, ie, ti + 2, tot, j, grp, tc, m1);
+
+            const SampleCtx& sc = sc0;
             float x[CPL];
-            gather_channels<C, kBF16>(a, sc, CPL * g, x);
+            corners_combine<C, kBF16>(sc, cr0, x);
             v4f h1[4], h2[4];
             float y;
             if constexpr (kDecBF16)
                 y = decoder_fwd_bf16<C>(sb, s, x, g, j, h1, h2);
             else
                 y = decoder_fwd<C>(s, x, g, j, h1, h2);
-            const float dy = sc.valid ? dp * sc.w * softplus_grad_f(y, a.beta) : 0.f;
+            const float dy = sc.valid ? dp0 * sc.w * softplus_grad_f(y, a.beta) : 0.f;
             v4f dx = tile_backward<C, kDecBF16>(s, sb, bufA, bufB, bufAb, bufBb, x, h1, h2, dy, g, j, q);
 
             // Scatter w_corner * dX into the LDS node gradient.  One wave at a
@@ -1166,6 +1284,9 @@ __global__ __launch_bounds__(64 * bwd_waves<kDecBF16>()) void project_bwd_node_k
                 }
                 lds_unlock(&s_lock, lane);
             }
+            sc0 = sc1;
+            cr0 = cr1;
+            dp0 = dp1;
         }
         __syncthreads();
         // flush the node gradient once, clear LDS for the next chunk (float4
