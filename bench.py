@@ -428,12 +428,10 @@ def kernel_times(N, tree, rays, sp, fp, st, bf16, ps):
             capi.call("atdf_project_fwd", *args, capi.ptr(P), s)
 
     def bwd():
-        if node_fwd:
-            capi.call("atdf_project_bwd_prepared", *args, capi.ptr(dP), capi.ptr(gg), capi.ptr(gm), None,
-                      capi.ptr(ws), ws.numel(), s)
-        else:
-            capi.call("atdf_project_bwd", *args, capi.ptr(dP), capi.ptr(gg), capi.ptr(gm), capi.ptr(ws), ws.numel(),
-                      s)
+        # the adjoint over the counting sort made by prep(): the node kernel
+        # and its decoder-partial reduction (atdf_project_bwd = prep + this)
+        capi.call("atdf_project_bwd_prepared", *args, capi.ptr(dP), capi.ptr(gg), capi.ptr(gm), None,
+                  capi.ptr(ws), ws.numel(), s)
 
     def timed(fn, reps=5):
         fn()
@@ -447,9 +445,8 @@ def kernel_times(N, tree, rays, sp, fp, st, bf16, ps):
         e1.synchronize()
         return e0.elapsed_time(e1) / reps / 1e3
 
-    t_prep = timed(prep) if node_fwd else 0.0
-    if node_fwd:
-        prep()
+    t_prep = timed(prep)
+    prep()
     t_fwd = timed(fwd)
     t_bwd = timed(bwd)
     b = 2 if bf16 else 4
@@ -466,16 +463,16 @@ def kernel_times(N, tree, rays, sp, fp, st, bf16, ps):
     achieved_gbs = bytes_dom / t_dom / 1e9
     achieved_tflops = flop_dom / t_dom / 1e12
     mfma_peak = BF16_PEAK_TFLOPS if bf16 else FP32_PEAK_TFLOPS
-    log(f"  {'project_prepare %.2f ms, ' % (t_prep * 1e3) if node_fwd else ''}"
+    log(f"  project_prepare (counting sort) {t_prep * 1e3:.2f} ms, "
         f"project_fwd {t_fwd * 1e3:.2f} ms, project_bwd {t_bwd * 1e3:.2f} ms ({S_total / R:.1f} samples/ray, "
         f"{iv.num_intervals} intervals); dominant {dom}: {achieved_gbs:.0f} GB/s algorithmic, "
         f"{achieved_tflops:.1f} TFLOP/s decoder")
     mode = ("learned" if ps is not None else "bf16") if bf16 else "plain"
     traffic = measured_traffic(mode, f"project_{dom}")
     return {
-        "kernel_ms": ({"project_prepare": t_prep * 1e3, "project_fwd_prepared": t_fwd * 1e3,
-                       "project_bwd_prepared": t_bwd * 1e3} if node_fwd else
-                      {"project_fwd": t_fwd * 1e3, "project_bwd": t_bwd * 1e3}),
+        "kernel_ms": {"project_prepare": t_prep * 1e3,
+                      ("project_fwd_prepared" if node_fwd else "project_fwd"): t_fwd * 1e3,
+                      "project_bwd_prepared": t_bwd * 1e3},
         "roofline": {
             "bound": "hbm", "kernel": f"project_{dom}", "achieved": achieved_gbs, "peak": HBM_PEAK_GBS,
             "unit": "GB/s", "frac": achieved_gbs / HBM_PEAK_GBS, "traffic": traffic,
@@ -653,6 +650,16 @@ def main():
         legs["learned"] = run_leg(N, "learned+bf16", tree, batches, K, field_params(N, True, args.node_major_forward),
                                   grid0, mlp0, True, ps, judge, args, steps, prop_rays)
     if args.profile_steps > 0:
+        # the launches the full run's roofline times: 1 + 5 of project_fwd,
+        # then 1 + 5 of project_bwd per leg, last in the run
+        sp_k = N.SamplingParams()
+        sp_k.max_samples_per_node = K
+        sp_k.jitter_mode = 2
+        sp_k.jitter_seed = 7
+        for key, leg in legs.items():
+            bf = key != "plain"
+            kernel_times(N, tree, batches[0][0], sp_k, field_params(N, bf, args.node_major_forward), leg["state"],
+                         bf, ps if key == "learned" else None)
         return
     if args.mode in ("bf16", "learned"):
         legs["bf16"] = run_leg(N, "bf16", tree, batches, K, field_params(N, True), grid0, mlp0, True, None, judge,
