@@ -64,7 +64,7 @@ def build_torch_layer() -> str:
     cflags, ldflags = _torch_flags()
     os.makedirs(OBJ_DIR, exist_ok=True)
     srcs = [os.path.join(HERE, "csrc", "torch", f) for f in ("atdf_torch.cpp", "atdf_tree_opt.cpp", "atdf_volume.cpp", "atdf_scene.cpp", "atdf_regularize.cpp",
-                                                                    "atdf_bindings.cpp")]
+                                                                    "atdf_vm_fused.cpp", "atdf_bindings.cpp")]
     objs = [os.path.join(OBJ_DIR, os.path.basename(s) + ".o") for s in srcs]
     cxx = os.environ.get("CXX", "g++")
     with ThreadPoolExecutor(max_workers=2) as ex:

@@ -91,11 +91,18 @@ class Samples(C.Structure):
                 ("ray_index", C.c_void_p), ("local_index_in_ray", C.c_void_p), ("node_id", C.c_void_p)]
 
 
+class VM(C.Structure):
+    """atdf_vm: the tensorial (plane x line) field of the in-ROI nodes."""
+    _fields_ = [("planes", C.c_void_p), ("lines", C.c_void_p), ("num_planes", C.c_int64),
+                ("node_to_plane", C.c_void_p), ("Gp", C.c_int32), ("Gl", C.c_int32), ("roi_min_x", C.c_float),
+                ("roi_max_x", C.c_float), ("grad_planes", C.c_void_p), ("grad_lines", C.c_void_p)]
+
+
 class Field(C.Structure):
     _fields_ = [("C", C.c_int32), ("H", C.c_int32), ("G", C.c_int32), ("grid_dtype", C.c_int32),
                 ("softplus_beta", C.c_float), ("rays_per_wave", C.c_int32), ("grid", C.c_void_p),
                 ("num_grids", C.c_int64), ("node_to_grid", C.c_void_p), ("mlp", C.c_void_p),
-                ("decoder_dtype", C.c_int32), ("reserved", C.c_int32)]
+                ("decoder_dtype", C.c_int32), ("reserved", C.c_int32), ("vm", C.POINTER(VM))]
 
 
 _P, _I32, _I64, _F, _SZ, _U64 = C.c_void_p, C.c_int32, C.c_int64, C.c_float, C.c_size_t, C.c_uint64

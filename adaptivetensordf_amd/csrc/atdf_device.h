@@ -176,6 +176,50 @@ __device__ __forceinline__ void tri_setup(float u, float v, float s, int Z, int 
     t.z1 = clampi(iz1, Z);
 }
 
+// Bilinear read of IndirectGridSample2D, grid [N, C, H, W]
+// (modules/IndirectGridSample2D.cu:22-162):
+// corners tnw tne tsw tse from floor(ix), floor(iy), weights from the
+// unclipped corners, indices clipped (border padding), summed in that order.
+struct Bil {
+    int x0, x1, y0, y1;
+    float w[4];
+};
+
+__device__ __forceinline__ void bil_setup(float u, float v, int H, int W, Bil& b)
+{
+    const float ix = ((u + 1) / 2) * (float)(W - 1);
+    const float iy = ((v + 1) / 2) * (float)(H - 1);
+    const int x0 = (int)floorf(ix), y0 = (int)floorf(iy);
+    const int x1 = x0 + 1, y1 = y0 + 1;
+    const float fx1 = (float)x1 - ix, fx0 = ix - (float)x0;
+    const float fy1 = (float)y1 - iy, fy0 = iy - (float)y0;
+    b.w[0] = fx1 * fy1;
+    b.w[1] = fx0 * fy1;
+    b.w[2] = fx1 * fy0;
+    b.w[3] = fx0 * fy0;
+    b.x0 = clampi(x0, W);
+    b.x1 = clampi(x1, W);
+    b.y0 = clampi(y0, H);
+    b.y1 = clampi(y1, H);
+}
+
+// 1-D read of IndirectGridSample1D (modules/IndirectGridSample1D.cu:20-140):
+// weights from the unclipped corners, indices clipped.
+struct Lin {
+    int x0, x1;
+    float w0, w1;
+};
+
+__device__ __forceinline__ void lin_setup(float u, int W, Lin& l)
+{
+    const float ix = ((u + 1) / 2) * (float)(W - 1);
+    const int x0 = (int)floorf(ix), x1 = x0 + 1;
+    l.w0 = (float)x1 - ix;
+    l.w1 = ix - (float)x0;
+    l.x0 = clampi(x0, W);
+    l.x1 = clampi(x1, W);
+}
+
 // Partial derivatives of the 8 weights w.r.t. (ix, iy, iz).
 __device__ __forceinline__ void tri_dweights(float u, float v, float s, int Z, int Y, int X, float dw[8][3])
 {

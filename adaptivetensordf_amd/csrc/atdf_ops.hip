@@ -97,32 +97,8 @@ __global__ __launch_bounds__(256) void grid_sample3d_bwd_kernel(const float* __r
     }
 }
 
-// 2-D indirect gather, grid [N, C, H, W] (modules/IndirectGridSample2D.cu:22-162):
-// corners tnw tne tsw tse from floor(ix), floor(iy), weights from the
-// unclipped corners, indices clipped (border padding), summed in that order.
-struct Bil {
-    int x0, x1, y0, y1;
-    float w[4];
-};
-
-__device__ __forceinline__ void bil_setup(float u, float v, int H, int W, Bil& b)
-{
-    const float ix = ((u + 1) / 2) * (float)(W - 1);
-    const float iy = ((v + 1) / 2) * (float)(H - 1);
-    const int x0 = (int)floorf(ix), y0 = (int)floorf(iy);
-    const int x1 = x0 + 1, y1 = y0 + 1;
-    const float fx1 = (float)x1 - ix, fx0 = ix - (float)x0;
-    const float fy1 = (float)y1 - iy, fy0 = iy - (float)y0;
-    b.w[0] = fx1 * fy1;
-    b.w[1] = fx0 * fy1;
-    b.w[2] = fx1 * fy0;
-    b.w[3] = fx0 * fy0;
-    b.x0 = clampi(x0, W);
-    b.x1 = clampi(x1, W);
-    b.y0 = clampi(y0, H);
-    b.y1 = clampi(y1, H);
-}
-
+// 2-D indirect gather, grid [N, C, H, W] (modules/IndirectGridSample2D.cu:22-162),
+// corner setup in bil_setup (atdf_device.h).
 __global__ __launch_bounds__(256) void grid_sample2d_fwd_kernel(const float* __restrict__ grid, int C, int H, int W,
                                                                 const int64_t* __restrict__ index,
                                                                 const float* __restrict__ uv, int64_t S,
@@ -198,10 +174,10 @@ __global__ __launch_bounds__(256) void grid_sample1d_fwd_kernel(const float* __r
 {
     const int64_t s = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (s >= S) return;
-    const float ix = ((uv[s] + 1) / 2) * (float)(W - 1);
-    const int x0 = (int)floorf(ix), x1 = x0 + 1;
-    const float w0 = (float)x1 - ix, w1 = ix - (float)x0;
-    const int c0 = clampi(x0, W), c1 = clampi(x1, W);
+    Lin l;
+    lin_setup(uv[s], W, l);
+    const float w0 = l.w0, w1 = l.w1;
+    const int c0 = l.x0, c1 = l.x1;
     const float* g = grid + index[s] * C * W;
     for (int c = 0; c < C; ++c) {
         float acc = 0.f;
@@ -220,10 +196,10 @@ __global__ __launch_bounds__(256) void grid_sample1d_bwd_kernel(const float* __r
 {
     const int64_t s = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (s >= S) return;
-    const float ix = ((uv[s] + 1) / 2) * (float)(W - 1);
-    const int x0 = (int)floorf(ix), x1 = x0 + 1;
-    const float w0 = (float)x1 - ix, w1 = ix - (float)x0;
-    const int c0 = clampi(x0, W), c1 = clampi(x1, W);
+    Lin l;
+    lin_setup(uv[s], W, l);
+    const float w0 = l.w0, w1 = l.w1;
+    const int c0 = l.x0, c1 = l.x1;
     const int64_t base = index[s] * C * W;
     float gx = 0.f;
     for (int c = 0; c < C; ++c) {

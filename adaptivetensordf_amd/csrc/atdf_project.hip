@@ -105,6 +105,15 @@ struct ProjArgs {
     float beta;
     int rpw;
     float* bins;  // [R, 32] mass histogram (binned forward only)
+    // tensorial (VM) field of the in-ROI nodes (atdf_vm); planes null = none
+    struct {
+        const float* planes;         // [num_planes, 3, Gp, Gp, C]
+        const float* lines;          // [3, Gl, C]
+        const int32_t* node_to_plane;
+        int64_t NG;                  // explicit grid slots; plane slot p sorts as NG + p
+        int Gp, Gl;
+        float rx0, rx1;              // ROI x range of the line x coordinate
+    } vm;
 };
 
 template <int C>
@@ -146,9 +155,14 @@ struct SampleCtx {
     float w;       // integration weight
     Tri tri;
     int64_t gbase; // element offset of the node grid, -1 if none
+    // tensorial field only: local coordinates, global x and z, plane slot
+    float loc[3];
+    float gx, gz;
+    int pslot;
 };
 
 // Recompute sample jj of a node interval (ray r, sample index s_local in the ray).
+template <bool kVM = false>
 __device__ __forceinline__ void make_sample_iv(const ProjArgs& a, int64_t r, const float* o, const float* d, int node,
                                                float tmin, float tmax, int n, int jj, int s_local, int C,
                                                SampleCtx& sc)
@@ -172,10 +186,16 @@ __device__ __forceinline__ void make_sample_iv(const ProjArgs& a, int64_t r, con
         float pos = o[k] + d[k] * t;
         float size = nd.mx[k] - nd.mn[k];
         loc[k] = ((pos - nd.mn[k]) / size) * 2 - 1;
+        if constexpr (kVM) {
+            sc.loc[k] = loc[k];
+            if (k == 0) sc.gx = pos;
+            if (k == 2) sc.gz = pos;
+        }
     }
     tri_setup(loc[0], loc[1], loc[2], a.G, a.G, a.G, sc.tri);
     const int slot = a.node_to_grid[node];
     sc.gbase = slot >= 0 ? (int64_t)slot * a.G * a.G * a.G * C : -1;
+    if constexpr (kVM) sc.pslot = slot < 0 ? a.vm.node_to_plane[node] : -1;
 }
 
 // Index (0..16) of the lane group's candidate interval holding sample t:
@@ -286,6 +306,95 @@ __device__ __forceinline__ void gather_channels(const ProjArgs& a, const SampleC
 #pragma unroll
         for (int k = 0; k < 8; ++k) acc = fmaf(v[k][q], sc.tri.w[k], acc);
         x[q] = acc;
+    }
+}
+
+// Tensorial (VM) features of an in-ROI sample, tensor_op "add"
+// (geometry/geometry_tensor_qo.cpp:279-327): planes yz (u = ly, v = lz),
+// xz (lx, lz), xy (lx, ly) and lines x (the ROI-normalised global x),
+// y (ly), z (the raw global z); F = Pyz Lx + Pxz Ly + Pxy Lz with the
+// products rounded and added in that order, each read accumulated
+// acc += value * weight from 0 in corner order (as the 2-D / 1-D gathers).
+// Channels c0 .. c0 + NCH - 1; P and L keep the reads for the adjoint.
+struct VMSetup {
+    Bil b[3];
+    Lin l[3];
+};
+
+__device__ __forceinline__ void vm_setup(const ProjArgs& a, const SampleCtx& sc, VMSetup& st)
+{
+    bil_setup(sc.loc[1], sc.loc[2], a.vm.Gp, a.vm.Gp, st.b[0]);
+    bil_setup(sc.loc[0], sc.loc[2], a.vm.Gp, a.vm.Gp, st.b[1]);
+    bil_setup(sc.loc[0], sc.loc[1], a.vm.Gp, a.vm.Gp, st.b[2]);
+    const float xr = (2.0f * (sc.gx - a.vm.rx0)) / (a.vm.rx1 - a.vm.rx0) - 1.0f;
+    lin_setup(xr, a.vm.Gl, st.l[0]);
+    lin_setup(sc.loc[1], a.vm.Gl, st.l[1]);
+    lin_setup(sc.gz, a.vm.Gl, st.l[2]);
+}
+
+template <int C, int NCH>
+__device__ __forceinline__ void load_channels(const float* p, float* v)
+{
+    if constexpr (NCH == 4) {
+        const float4 f = *reinterpret_cast<const float4*>(p);
+        v[0] = f.x;
+        v[1] = f.y;
+        v[2] = f.z;
+        v[3] = f.w;
+    } else {
+        const float2 f = *reinterpret_cast<const float2*>(p);
+        v[0] = f.x;
+        v[1] = f.y;
+    }
+}
+
+template <int C, int NCH>
+__device__ __forceinline__ void vm_reads(const ProjArgs& a, int pslot, const VMSetup& st, int c0, float (*P)[NCH],
+                                         float (*L)[NCH])
+{
+    const int Gp = a.vm.Gp;
+    const float* pl = a.vm.planes + (int64_t)pslot * 3 * Gp * Gp * C + c0;
+#pragma unroll
+    for (int p = 0; p < 3; ++p) {
+        const Bil& b = st.b[p];
+        const float* pp = pl + (int64_t)p * Gp * Gp * C;
+        const int off[4] = {(b.y0 * Gp + b.x0) * C, (b.y0 * Gp + b.x1) * C, (b.y1 * Gp + b.x0) * C,
+                            (b.y1 * Gp + b.x1) * C};
+        float acc[NCH];
+#pragma unroll
+        for (int q = 0; q < NCH; ++q) acc[q] = 0.f;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            float v[NCH];
+            load_channels<C, NCH>(pp + off[k], v);
+#pragma unroll
+            for (int q = 0; q < NCH; ++q) acc[q] += v[q] * b.w[k];
+        }
+        const Lin& l = st.l[p];
+        const float* ln = a.vm.lines + (int64_t)p * a.vm.Gl * C + c0;
+        float v0[NCH], v1[NCH];
+        load_channels<C, NCH>(ln + l.x0 * C, v0);
+        load_channels<C, NCH>(ln + l.x1 * C, v1);
+#pragma unroll
+        for (int q = 0; q < NCH; ++q) {
+            P[p][q] = acc[q];
+            float la = 0.f;
+            la += v0[q] * l.w0;
+            la += v1[q] * l.w1;
+            L[p][q] = la;
+        }
+    }
+}
+
+template <int NCH>
+__device__ __forceinline__ void vm_combine(const float (*P)[NCH], const float (*L)[NCH], float* x)
+{
+#pragma unroll
+    for (int q = 0; q < NCH; ++q) {
+        float f = P[0][q] * L[0][q];
+        f = f + P[1][q] * L[1][q];
+        f = f + P[2][q] * L[2][q];
+        x[q] = f;
     }
 }
 
@@ -458,7 +567,7 @@ __device__ __forceinline__ int sample_bin(float t, float t0, float t1)
 // Four waves per SIMD: the weight reads below go through an opaque copy of
 // the LDS pointers, so the compiler reloads them per tile instead of keeping
 // every weight in registers (about 95 VGPRs instead of 180-250).
-template <int C, bool kBF16, bool kDecBF16, bool kBins>
+template <int C, bool kBF16, bool kDecBF16, bool kBins, bool kVM = false>
 __global__ __launch_bounds__(64 * kWaves, 4) void project_fwd_kernel(ProjArgs a, float* __restrict__ P)
 {
     using SM = DecoderSmem<C, kDecBF16>;
@@ -550,18 +659,27 @@ __global__ __launch_bounds__(64 * kWaves, 4) void project_fwd_kernel(ProjArgs a,
             sc.gbase = -1;
             if (sc.valid) {
                 if (pos < kTile) {
-                    make_sample_iv(a, r, o, d, s_node, s_lo, s_hi, s_cnt, t - (s_end - s_cnt), t, C, sc);
+                    make_sample_iv<kVM>(a, r, o, d, s_node, s_lo, s_hi, s_cnt, t - (s_end - s_cnt), t, C, sc);
                 } else {
                     // only with zero-sample intervals in the list: walk on
                     int64_t q = cur + kTile - 1;
                     while (q + 1 < h1e && a.iv_first[q] + a.iv_count[q] <= t) ++q;
-                    make_sample_iv(a, r, o, d, a.iv_node[q], a.iv_tmin[q], a.iv_tmax[q], a.iv_count[q],
-                                   t - a.iv_first[q], t, C, sc);
+                    make_sample_iv<kVM>(a, r, o, d, a.iv_node[q], a.iv_tmin[q], a.iv_tmax[q], a.iv_count[q],
+                                        t - a.iv_first[q], t, C, sc);
                 }
             }
             cur += posn;
             float x[CPL];
             gather_channels<C, kBF16>(a, sc, CPL * g, x);
+            if constexpr (kVM) {
+                if (sc.valid && sc.pslot >= 0) {
+                    VMSetup st;
+                    vm_setup(a, sc, st);
+                    float Pv[3][CPL], Lv[3][CPL];
+                    vm_reads<C, CPL>(a, sc.pslot, st, CPL * g, Pv, Lv);
+                    vm_combine<CPL>(Pv, Lv, x);
+                }
+            }
             v4f h1[4], h2[4];
             float y;
             int opq = 0;
@@ -906,6 +1024,12 @@ struct NodeBwdArgs {
     int lds_f32_off;          // float offset of the weight region in dynamic LDS
     int lds_bf16_off;         // byte offset of the bf16 weight region
     int lds_tb_off;           // byte offset of the transpose buffers
+    // tensorial field: plane gradients, per-workgroup line partial rows
+    // ([blocks][3 Gl C]) and the float offset of the line partials in LDS
+    float* grad_planes;
+    float* line_part;
+    int lds_line_off;
+    int lds_grad_floats;      // LDS node-gradient region, zeroed at start
 };
 
 template <int C, bool kDecBF16>
@@ -939,6 +1063,7 @@ __device__ __forceinline__ void write_record(const RecordSrc& rs, int64_t i, int
 }
 
 // make_sample_iv for a node whose box and grid offset are already known.
+template <bool kVM = false>
 __device__ __forceinline__ void make_sample_node(const ProjArgs& a, const PNode& nd, int64_t gbase, int64_t r,
                                                  const float* o, const float* d, float tmin, float tmax, int n,
                                                  int jj, int s_local, SampleCtx& sc)
@@ -961,6 +1086,11 @@ __device__ __forceinline__ void make_sample_node(const ProjArgs& a, const PNode&
         float pos = o[k] + d[k] * t;
         float size = nd.mx[k] - nd.mn[k];
         loc[k] = ((pos - nd.mn[k]) / size) * 2 - 1;
+        if constexpr (kVM) {
+            sc.loc[k] = loc[k];
+            if (k == 0) sc.gx = pos;
+            if (k == 2) sc.gz = pos;
+        }
     }
     tri_setup(loc[0], loc[1], loc[2], a.G, a.G, a.G, sc.tri);
     sc.gbase = gbase;
@@ -1062,10 +1192,10 @@ struct Corners {
 // Stage 2: places this lane's sample, issues the gathers of its 8 corners
 // (channels 4g.., C = 16; 2g.., C = 8) and returns dL/dP of its ray (0 for
 // a lane without a sample).
-template <int C, bool kBF16>
+template <int C, bool kBF16, bool kVM = false>
 __device__ __forceinline__ float sample_stage(const ProjArgs& a, const float* __restrict__ dLdP, const PNode& nd,
                                               int64_t gbase, const TileMap& m, int g, SampleCtx& sc,
-                                              Corners<C, kBF16>& cr)
+                                              Corners<C, kBF16>& cr, bool vm_chunk = false)
 {
     constexpr int CPL = C / 4;
     constexpr int W = Corners<C, kBF16>::W;
@@ -1076,8 +1206,9 @@ __device__ __forceinline__ float sample_stage(const ProjArgs& a, const float* __
     const float o[3] = {m.B.x, m.B.y, m.B.z};
     const float d[3] = {m.B.w, m.D.x, m.D.y};
     const int r = __float_as_int(m.D.w);
-    make_sample_node(a, nd, gbase, r, o, d, m.A.x, m.A.y, m.n_s, m.jj, __float_as_int(m.A.w) + m.jj, sc);
+    make_sample_node<kVM>(a, nd, gbase, r, o, d, m.A.x, m.A.y, m.n_s, m.jj, __float_as_int(m.A.w) + m.jj, sc);
     const float dp = dLdP[r];
+    if (kVM && vm_chunk) return dp;  // tensorial reads happen in stage 3
     const unsigned char* grid = reinterpret_cast<const unsigned char*>(a.grid);
 #pragma unroll
     for (int k = 0; k < 8; ++k) {
@@ -1152,7 +1283,7 @@ __device__ __forceinline__ void lds_unlock(int* lock, int lane)
     if (lane == 0) __hip_atomic_store(lock, 0, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
 
-template <int C, bool kBF16, bool kDecBF16>
+template <int C, bool kBF16, bool kDecBF16, bool kVM = false>
 __global__ __launch_bounds__(64 * bwd_waves<kDecBF16>()) void project_bwd_node_kernel(NodeBwdArgs b)
 {
     extern __shared__ __attribute__((aligned(16))) unsigned char dsm[];
@@ -1180,8 +1311,15 @@ __global__ __launch_bounds__(64 * bwd_waves<kDecBF16>()) void project_bwd_node_k
         load_mlp_bf16_to_lds<C>(a.mlp, sb, s);
     else
         load_mlp_to_lds<C>(a.mlp, s);
-    for (int i = threadIdx.x; i < gsz; i += blockDim.x) lgrad[i] = 0.f;
+    for (int i = threadIdx.x; i < b.lds_grad_floats; i += blockDim.x) lgrad[i] = 0.f;
     if (threadIdx.x == 0) s_lock = 0;
+    // tensorial chunks (slots >= NG) come after every explicit one in the
+    // chunk table, and a workgroup takes chunks in increasing order, so once
+    // it reaches them the line partials at lds_line_off (zero until then:
+    // explicit chunks clear their gradient after each flush) stay in LDS
+    // until the workgroup ends
+    float* lline = lgrad + b.lds_line_off;
+    const int Gp = kVM ? a.vm.Gp : 0, Gl = kVM ? a.vm.Gl : 0;
 
     DecGrads q;
     zero_grads(q);
@@ -1195,7 +1333,9 @@ __global__ __launch_bounds__(64 * bwd_waves<kDecBF16>()) void project_bwd_node_k
         __syncthreads();
         if (c >= total_chunks) break;
         const int4 ch = b.chunks[c];
-        const int64_t gbase = (int64_t)ch.x * gsz;
+        const bool vmc = kVM && ch.x >= a.vm.NG;
+        const int pslot = vmc ? (int)(ch.x - a.vm.NG) : -1;
+        const int64_t gbase = vmc ? 0 : (int64_t)ch.x * gsz;
         const PNode nd = a.nodes[a.iv_node[b.sorted[ch.y]]];  // one node per chunk
         const int n_iv = ch.z - ch.y;
         const int per = (n_iv + kNW - 1) / kNW;
@@ -1221,19 +1361,31 @@ __global__ __launch_bounds__(64 * bwd_waves<kDecBF16>()) void project_bwd_node_k
         if (ntiles > 0) map_tile(rec, ie, 0, tot, j, grp, tc, m1);
         SampleCtx sc0;
         Corners<C, kBF16> cr0;
-        float dp0 = sample_stage<C, kBF16>(a, b.dLdP, nd, gbase, m1, g, sc0, cr0);
+        float dp0 = sample_stage<C, kBF16, kVM>(a, b.dLdP, nd, gbase, m1, g, sc0, cr0, vmc);
         m1.valid = false;
         if (ntiles > 1) map_tile(rec, ie, 1, tot, j, grp, tc, m1);
         for (int ti = 0; ti < ntiles; ++ti) {
             SampleCtx sc1;
             Corners<C, kBF16> cr1;
-            const float dp1 = sample_stage<C, kBF16>(a, b.dLdP, nd, gbase, m1, g, sc1, cr1);
+            const float dp1 = sample_stage<C, kBF16, kVM>(a, b.dLdP, nd, gbase, m1, g, sc1, cr1, vmc);
             m1.valid = false;
             if (ti + 2 < ntiles) map_tile(rec, ie, ti + 2, tot, j, grp, tc, m1);
 
             const SampleCtx& sc = sc0;
             float x[CPL];
-            corners_combine<C, kBF16>(sc, cr0, x);
+            VMSetup vst;
+            float Pv[3][CPL], Lv[3][CPL];
+            if (kVM && vmc) {
+#pragma unroll
+                for (int qq = 0; qq < CPL; ++qq) x[qq] = 0.f;
+                if (sc.valid) {
+                    vm_setup(a, sc, vst);
+                    vm_reads<C, CPL>(a, pslot, vst, CPL * g, Pv, Lv);
+                    vm_combine<CPL>(Pv, Lv, x);
+                }
+            } else {
+                corners_combine<C, kBF16>(sc, cr0, x);
+            }
             v4f h1[4], h2[4];
             float y;
             if constexpr (kDecBF16)
@@ -1251,6 +1403,47 @@ __global__ __launch_bounds__(64 * bwd_waves<kDecBF16>()) void project_bwd_node_k
             // unique cell in the tile use a plain float4 read-add-write and the
             // rest (shared cell, or a clamped corner at the grid border) use
             // LDS atomics.
+            if (kVM && vmc) {
+                // dL/dplane = w dX L, dL/dline = w dX P into the LDS plane
+                // gradient and the line partials, all by LDS atomics (no
+                // lock: no plain read-add-write in this chunk)
+                if (sc.valid && dy != 0.f && 4 * g < C) {
+                    float P4[3][4], L4[3][4];
+                    if constexpr (C == 16) {
+#pragma unroll
+                        for (int pp = 0; pp < 3; ++pp)
+#pragma unroll
+                            for (int qq = 0; qq < 4; ++qq) {
+                                P4[pp][qq] = Pv[pp][qq];
+                                L4[pp][qq] = Lv[pp][qq];
+                            }
+                    } else {
+                        vm_reads<C, 4>(a, pslot, vst, 4 * g, P4, L4);
+                    }
+#pragma unroll
+                    for (int pp = 0; pp < 3; ++pp) {
+                        const Bil& bb = vst.b[pp];
+                        float* gpl = lgrad + pp * Gp * Gp * C + 4 * g;
+                        const int off[4] = {(bb.y0 * Gp + bb.x0) * C, (bb.y0 * Gp + bb.x1) * C,
+                                            (bb.y1 * Gp + bb.x0) * C, (bb.y1 * Gp + bb.x1) * C};
+                        const Lin& ll = vst.l[pp];
+                        float* gl = lline + pp * Gl * C + 4 * g;
+#pragma unroll
+                        for (int qq = 0; qq < 4; ++qq) {
+                            const float dP = dx[qq] * L4[pp][qq];
+                            const float dL = dx[qq] * P4[pp][qq];
+#pragma unroll
+                            for (int k = 0; k < 4; ++k) atomicAdd(gpl + off[k] + qq, bb.w[k] * dP);
+                            atomicAdd(gl + ll.x0 * C + qq, ll.w0 * dL);
+                            atomicAdd(gl + ll.x1 * C + qq, ll.w1 * dL);
+                        }
+                    }
+                }
+                sc0 = sc1;
+                cr0 = cr1;
+                dp0 = dp1;
+                continue;
+            }
             const bool live = sc.valid && dy != 0.f;
             const bool border = sc.tri.x0 == sc.tri.x1 || sc.tri.y0 == sc.tri.y1 || sc.tri.z0 == sc.tri.z1;
             const int key = live ? (border ? -1 : (sc.tri.z0 * G + sc.tri.y0) * G + sc.tri.x0) : -2 - j;
@@ -1293,11 +1486,14 @@ __global__ __launch_bounds__(64 * bwd_waves<kDecBF16>()) void project_bwd_node_k
         // accesses, kU in flight per thread; G^3 C is a multiple of 4 for C in
         // {8, 16})
         constexpr int kU = kDecBF16 ? 16 : 4;
-        float4* gg4 = reinterpret_cast<float4*>(b.grad_grid + gbase);
+        // tensorial chunk: the node's three plane gradients (3 Gp^2 C, a
+        // multiple of 4 floats for C in {8, 16})
+        float4* gg4 = reinterpret_cast<float4*>(vmc ? b.grad_planes + (int64_t)pslot * 3 * Gp * Gp * C
+                                                    : b.grad_grid + gbase);
         float4* lg4 = reinterpret_cast<float4*>(lgrad);
-        const int n4 = gsz >> 2;
+        const int n4 = (vmc ? 3 * Gp * Gp * C : gsz) >> 2;
         const float4 z4 = make_float4(0.f, 0.f, 0.f, 0.f);
-        if (b.ad.enabled && !ch.w) {
+        if (b.ad.enabled && !ch.w && !vmc) {
             // the chunk holds all of this node's samples: its gradient is
             // complete, so apply Adam here instead of going through grad_grid
             const GridAdam& ad = b.ad;
@@ -1372,6 +1568,12 @@ __global__ __launch_bounds__(64 * bwd_waves<kDecBF16>()) void project_bwd_node_k
             }
         }
         __syncthreads();
+    }
+    if constexpr (kVM) {
+        // this workgroup's line partials (all zero if it took no tensorial
+        // chunk), summed over workgroups in a fixed order afterwards
+        float* row = b.line_part + (int64_t)blockIdx.x * 3 * Gl * C;
+        for (int i = threadIdx.x; i < 3 * Gl * C; i += blockDim.x) row[i] = lline[i];
     }
     store_partials<C>(q, b.ws + ((int64_t)blockIdx.x * kNW + wl) * NP, g, j, lane);
 }
@@ -1549,25 +1751,41 @@ __global__ __launch_bounds__(256) void fill_i32_kernel(int32_t* p, int64_t n, in
     if (i < n) p[i] = v;
 }
 
-__global__ __launch_bounds__(256) void slot_hist_kernel(const int32_t* __restrict__ iv_node,
-                                                        const int32_t* __restrict__ node_to_grid, int64_t H,
+// Sort key of a node: its explicit grid slot, else NG + its plane slot
+// (tensorial field), else -1 (a node without a field: its samples drop out).
+struct SlotMap {
+    const int32_t* n2g;
+    const int32_t* n2p;  // null without a tensorial field
+    int64_t NG;
+};
+
+__device__ __forceinline__ int slot_of(const SlotMap& m, int node)
+{
+    int s = m.n2g[node];
+    if (s < 0 && m.n2p) {
+        const int p = m.n2p[node];
+        s = p >= 0 ? (int)(m.NG + p) : -1;
+    }
+    return s;
+}
+
+__global__ __launch_bounds__(256) void slot_hist_kernel(const int32_t* __restrict__ iv_node, SlotMap sm, int64_t H,
                                                         int32_t* __restrict__ cnt)
 {
     int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (i >= H) return;
-    int slot = node_to_grid[iv_node[i]];
+    int slot = slot_of(sm, iv_node[i]);
     if (slot >= 0) atomicAdd(&cnt[slot], 1);
 }
 
-__global__ __launch_bounds__(256) void slot_scatter_kernel(const int32_t* __restrict__ iv_node,
-                                                           const int32_t* __restrict__ node_to_grid, int64_t H,
+__global__ __launch_bounds__(256) void slot_scatter_kernel(const int32_t* __restrict__ iv_node, SlotMap sm, int64_t H,
                                                            const int64_t* __restrict__ slot_off,
                                                            int32_t* __restrict__ cursor, int32_t* __restrict__ sorted,
                                                            RecordSrc rs)
 {
     int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (i >= H) return;
-    int slot = node_to_grid[iv_node[i]];
+    int slot = slot_of(sm, iv_node[i]);
     if (slot < 0) return;
     int pos = atomicAdd(&cursor[slot], 1);
     const int64_t k = slot_off[slot] + pos;
@@ -1583,50 +1801,47 @@ __global__ __launch_bounds__(256) void slot_scatter_kernel(const int32_t* __rest
 // with one atomic and ranks its intervals with LDS atomics.
 constexpr int kSortItems = 8192;
 
-__device__ __forceinline__ void block_slot_hist(const int32_t* __restrict__ iv_node,
-                                                const int32_t* __restrict__ node_to_grid, int64_t b0, int64_t e0,
-                                                int64_t NA, int32_t* lh)
+__device__ __forceinline__ void block_slot_hist(const int32_t* __restrict__ iv_node, const SlotMap& sm, int64_t b0,
+                                                int64_t e0, int64_t NA, int32_t* lh)
 {
     for (int64_t s = threadIdx.x; s < NA; s += blockDim.x) lh[s] = 0;
     __syncthreads();
     for (int64_t i = b0 + threadIdx.x; i < e0; i += blockDim.x) {
-        const int slot = node_to_grid[iv_node[i]];
+        const int slot = slot_of(sm, iv_node[i]);
         if (slot >= 0) atomicAdd(&lh[slot], 1);
     }
     __syncthreads();
 }
 
-__global__ __launch_bounds__(256) void slot_hist_lds_kernel(const int32_t* __restrict__ iv_node,
-                                                            const int32_t* __restrict__ node_to_grid, int64_t H,
-                                                            int64_t NA, int32_t* __restrict__ cnt)
+__global__ __launch_bounds__(256) void slot_hist_lds_kernel(const int32_t* __restrict__ iv_node, SlotMap sm,
+                                                            int64_t H, int64_t NA, int32_t* __restrict__ cnt)
 {
     extern __shared__ int32_t lh[];
     const int64_t b0 = blockIdx.x * (int64_t)kSortItems;
     const int64_t e0 = b0 + kSortItems < H ? b0 + kSortItems : H;
-    block_slot_hist(iv_node, node_to_grid, b0, e0, NA, lh);
+    block_slot_hist(iv_node, sm, b0, e0, NA, lh);
     for (int64_t s = threadIdx.x; s < NA; s += blockDim.x) {
         const int v = lh[s];
         if (v) atomicAdd(&cnt[s], v);
     }
 }
 
-__global__ __launch_bounds__(256) void slot_scatter_lds_kernel(const int32_t* __restrict__ iv_node,
-                                                               const int32_t* __restrict__ node_to_grid, int64_t H,
-                                                               int64_t NA, const int64_t* __restrict__ slot_off,
+__global__ __launch_bounds__(256) void slot_scatter_lds_kernel(const int32_t* __restrict__ iv_node, SlotMap sm,
+                                                               int64_t H, int64_t NA, const int64_t* __restrict__ slot_off,
                                                                int32_t* __restrict__ cursor,
                                                                int32_t* __restrict__ sorted, RecordSrc rs)
 {
     extern __shared__ int32_t lh[];
     const int64_t b0 = blockIdx.x * (int64_t)kSortItems;
     const int64_t e0 = b0 + kSortItems < H ? b0 + kSortItems : H;
-    block_slot_hist(iv_node, node_to_grid, b0, e0, NA, lh);
+    block_slot_hist(iv_node, sm, b0, e0, NA, lh);
     for (int64_t s = threadIdx.x; s < NA; s += blockDim.x) {
         const int v = lh[s];
         lh[s] = v ? (int)(slot_off[s] + atomicAdd(&cursor[s], v)) : 0;
     }
     __syncthreads();
     for (int64_t i = b0 + threadIdx.x; i < e0; i += blockDim.x) {
-        const int slot = node_to_grid[iv_node[i]];
+        const int slot = slot_of(sm, iv_node[i]);
         if (slot >= 0) {
             const int64_t k = atomicAdd(&lh[slot], 1);
             sorted[k] = (int32_t)i;
@@ -2028,7 +2243,22 @@ static atdf_status check_project_args(const void* packed_tree, int64_t num_nodes
     ATDF_REQUIRE(f->grid_dtype == ATDF_GRID_F32 || f->grid_dtype == ATDF_GRID_BF16, "project: unknown grid dtype");
     ATDF_REQUIRE(f->decoder_dtype == ATDF_DECODER_F32 || f->decoder_dtype == ATDF_DECODER_BF16,
                  "project: unknown decoder dtype");
-    ATDF_REQUIRE(f->grid && f->node_to_grid && f->mlp && f->num_grids > 0, "project: incomplete field");
+    if (f->vm) {
+        const atdf_vm* v = f->vm;
+        ATDF_REQUIRE(f->grid_dtype == ATDF_GRID_F32, "project: a tensorial field is f32 only");
+        ATDF_REQUIRE(v->planes && v->lines && v->node_to_plane && v->num_planes > 0,
+                     "project: incomplete tensorial field");
+        ATDF_REQUIRE(v->Gp >= 2 && v->Gl >= 2, "project: plane edge and line length must be >= 2");
+        ATDF_REQUIRE(v->roi_max_x > v->roi_min_x, "project: empty ROI x range");
+        ATDF_REQUIRE((int64_t)v->num_planes * 3 * v->Gp * v->Gp * f->C < ((int64_t)1 << 40),
+                     "project: tensorial field too large");
+        ATDF_REQUIRE(((uintptr_t)v->planes | (uintptr_t)v->lines) % 16 == 0,
+                     "project: planes and lines must be 16-byte aligned");
+        ATDF_REQUIRE(f->node_to_grid && f->mlp && f->num_grids >= 0 && (f->num_grids == 0 || f->grid),
+                     "project: incomplete field");
+    } else {
+        ATDF_REQUIRE(f->grid && f->node_to_grid && f->mlp && f->num_grids > 0, "project: incomplete field");
+    }
     if (R > 0) {
         ATDF_REQUIRE(O && D && iv_off && iv && iv->node && iv->tmin && iv->tmax && iv->count && iv->first,
                      "project: null ray or interval buffer");
@@ -2058,6 +2288,16 @@ static ProjArgs make_args(const void* packed_tree, const float* O, const float* 
     a.G = f->G;
     a.beta = f->softplus_beta;
     a.rpw = (int)rays_per_wave(f);
+    if (f->vm) {
+        a.vm.planes = f->vm->planes;
+        a.vm.lines = f->vm->lines;
+        a.vm.node_to_plane = f->vm->node_to_plane;
+        a.vm.NG = f->num_grids;
+        a.vm.Gp = f->vm->Gp;
+        a.vm.Gl = f->vm->Gl;
+        a.vm.rx0 = f->vm->roi_min_x;
+        a.vm.rx1 = f->vm->roi_max_x;
+    }
     return a;
 }
 
@@ -2079,6 +2319,17 @@ static atdf_status project_fwd_impl(const void* packed_tree, int64_t num_nodes, 
     hipStream_t s = to_hip(stream);
     const bool bf = f->grid_dtype == ATDF_GRID_BF16;
     const bool bd = f->decoder_dtype == ATDF_DECODER_BF16;
+    if (f->vm) {
+        ATDF_REQUIRE(!bins, "%s: the binned forward does not take a tensorial field", who);
+        if (f->C == 16) {
+            if (bd) hipLaunchKernelGGL((project_fwd_kernel<16, false, true, false, true>), grid, block, 0, s, a, P);
+            else hipLaunchKernelGGL((project_fwd_kernel<16, false, false, false, true>), grid, block, 0, s, a, P);
+        } else {
+            if (bd) hipLaunchKernelGGL((project_fwd_kernel<8, false, true, false, true>), grid, block, 0, s, a, P);
+            else hipLaunchKernelGGL((project_fwd_kernel<8, false, false, false, true>), grid, block, 0, s, a, P);
+        }
+        return check_launch(who);
+    }
 #define ATDF_FWD(CC, BF, BD)                                                                              \
     do {                                                                                                  \
         if (bins)                                                                                         \
@@ -2119,7 +2370,8 @@ atdf_status atdf_project_fwd_bins(const void* packed_tree, int64_t num_nodes, co
 
 namespace {
 struct BwdWorkspace {
-    size_t cnt, slot_off, sorted, chunk_cnt, chunk_off, desc, counter, stotal, scan, partials, rec, ivsum, total;
+    size_t cnt, slot_off, sorted, chunk_cnt, chunk_off, desc, counter, stotal, scan, partials, rec, ivsum, line_part,
+        total;
 };
 
 size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
@@ -2137,7 +2389,8 @@ int persistent_blocks()
     return n;
 }
 
-BwdWorkspace bwd_layout(int64_t H, int64_t NA, int np)
+// line_floats: 3 Gl C of a tensorial field (per-workgroup line partial rows), else 0
+BwdWorkspace bwd_layout(int64_t H, int64_t NA, int np, int64_t line_floats = 0)
 {
     BwdWorkspace w{};
     size_t o = 0;
@@ -2154,15 +2407,16 @@ BwdWorkspace bwd_layout(int64_t H, int64_t NA, int np)
     w.partials = take(sizeof(float) * (size_t)persistent_blocks() * kMaxBwdWaves * np);
     w.rec = take(3 * sizeof(float4) * (size_t)(H > 0 ? H : 1));
     w.ivsum = take(sizeof(float) * (size_t)(H > 0 ? H : 1));
+    w.line_part = take(sizeof(float) * (size_t)persistent_blocks() * (size_t)line_floats);
     w.total = o;
     return w;
 }
 
+// grad_floats: the node-gradient region at the start of the dynamic LDS
 template <int C, bool kDecBF16>
-void lds_layout(int G, int& f32_off, int& bf16_off, int& tb_off, int& total)
+void lds_layout(int grad_floats, int& f32_off, int& bf16_off, int& tb_off, int& total)
 {
-    const int gsz = G * G * G * C;
-    f32_off = (gsz + 3) & ~3;
+    f32_off = (grad_floats + 3) & ~3;
     int after_f32 = (f32_off + weights_f32_floats<C, kDecBF16>()) * 4;
     bf16_off = (after_f32 + 15) & ~15;
     int after_bf = bf16_off + (kDecBF16 ? BSmem<C>::kBf16Total * 2 : 0);
@@ -2185,11 +2439,27 @@ int max_lds_per_block()
     return max_lds;
 }
 
-template <int C, bool kBF16, bool kDecBF16>
+template <int C, bool kBF16, bool kDecBF16, bool kVM>
 atdf_status launch_node_bwd(NodeBwdArgs args, hipStream_t s)
 {
+    // node-gradient region: an explicit grid (G^3 C), or for a tensorial
+    // field the larger of that and three planes (3 Gp^2 C) plus the line
+    // partials (3 Gl C) behind them
+    const int G = args.a.G;
+    int64_t grad_floats = (int64_t)G * G * G * C;
+    args.lds_line_off = 0;
+    if (kVM) {
+        const int64_t planes = 3 * (int64_t)args.a.vm.Gp * args.a.vm.Gp * C;
+        args.lds_line_off = (int)planes;
+        grad_floats = std::max<int64_t>(grad_floats, planes + 3 * (int64_t)args.a.vm.Gl * C);
+    }
+    if (grad_floats * 4 > max_lds_per_block()) {
+        set_error("atdf_project_bwd: node gradient of %lld floats does not fit in LDS", (long long)grad_floats);
+        return ATDF_ERR_UNSUPPORTED;
+    }
+    args.lds_grad_floats = (int)grad_floats;
     int f32_off, bf16_off, tb_off, total;
-    lds_layout<C, kDecBF16>(args.a.G, f32_off, bf16_off, tb_off, total);
+    lds_layout<C, kDecBF16>((int)grad_floats, f32_off, bf16_off, tb_off, total);
     args.lds_f32_off = f32_off;
     args.lds_bf16_off = bf16_off;
     args.lds_tb_off = tb_off;
@@ -2199,7 +2469,7 @@ atdf_status launch_node_bwd(NodeBwdArgs args, hipStream_t s)
                   "allows %d", args.a.G, C, total, max_lds);
         return ATDF_ERR_UNSUPPORTED;
     }
-    hipLaunchKernelGGL((project_bwd_node_kernel<C, kBF16, kDecBF16>), dim3(persistent_blocks()),
+    hipLaunchKernelGGL((project_bwd_node_kernel<C, kBF16, kDecBF16, kVM>), dim3(persistent_blocks()),
                        dim3(64 * bwd_waves<kDecBF16>()), total, s, args);
     return ATDF_OK;
 }
@@ -2209,9 +2479,12 @@ extern "C" {
 
 size_t atdf_project_bwd_workspace_bytes(int64_t R, int64_t H, const atdf_field* f)
 {
-    if (!f || R <= 0 || H < 0 || f->num_grids <= 0) return 0;
+    if (!f || R <= 0 || H < 0 || f->num_grids < 0) return 0;
+    const int64_t NA = f->num_grids + (f->vm ? f->vm->num_planes : 0);
+    if (NA <= 0) return 0;
     int np = (int)((int64_t)f->H * f->C + f->H + (int64_t)f->H * f->H + f->H + f->H + 1);
-    return bwd_layout(H, f->num_grids, np).total;
+    const int64_t lf = f->vm ? 3 * (int64_t)f->vm->Gl * f->C : 0;
+    return bwd_layout(H, NA, np, lf).total;
 }
 
 }  // extern "C"
@@ -2223,7 +2496,7 @@ struct Prepared {
     BwdWorkspace L;
     unsigned char* w;
     int np;
-    int64_t H, NA;
+    int64_t H, NA, line_floats;
     int32_t* chunk_cnt() const { return reinterpret_cast<int32_t*>(w + L.chunk_cnt); }
     int32_t* sorted() const { return reinterpret_cast<int32_t*>(w + L.sorted); }
     int64_t* chunk_off() const { return reinterpret_cast<int64_t*>(w + L.chunk_off); }
@@ -2239,9 +2512,10 @@ static atdf_status prepared_view(const atdf_intervals* iv, const atdf_field* f, 
     ATDF_REQUIRE(iv->total < (int64_t)INT32_MAX, "%s: too many intervals", who);
     ATDF_REQUIRE(workspace_bytes >= atdf_project_bwd_workspace_bytes(R, iv->total, f), "%s: workspace too small", who);
     P.H = iv->total;
-    P.NA = f->num_grids;
+    P.NA = f->num_grids + (f->vm ? f->vm->num_planes : 0);
     P.np = f->C == 16 ? mlp_numel<16>() : mlp_numel<8>();
-    P.L = bwd_layout(P.H, P.NA, P.np);
+    P.line_floats = f->vm ? 3 * (int64_t)f->vm->Gl * f->C : 0;
+    P.L = bwd_layout(P.H, P.NA, P.np, P.line_floats);
     P.w = reinterpret_cast<unsigned char*>(workspace);
     return ATDF_OK;
 }
@@ -2261,6 +2535,7 @@ static atdf_status prepare_impl(const Prepared& P, const float* O, const float* 
     void* scan_ws = w + L.scan;
     hipStream_t s = to_hip(stream);
     const size_t scan_bytes = atdf_scan_workspace_bytes(NA);
+    const SlotMap sm{f->node_to_grid, f->vm ? f->vm->node_to_plane : nullptr, f->num_grids};
 
     hipLaunchKernelGGL(fill_i32_kernel, dim3((unsigned)div_up(NA, 256)), dim3(256), 0, s, cnt, NA, 0);
     // LDS-aggregated counting sort when a slot histogram fits in LDS
@@ -2269,11 +2544,11 @@ static atdf_status prepare_impl(const Prepared& P, const float* O, const float* 
     const unsigned sort_blocks = (unsigned)div_up(H, kSortItems);
     if (H > 0) {
         if (lds_sort)
-            hipLaunchKernelGGL(slot_hist_lds_kernel, dim3(sort_blocks), dim3(256), hist_lds, s, iv->node,
-                               f->node_to_grid, H, NA, cnt);
+            hipLaunchKernelGGL(slot_hist_lds_kernel, dim3(sort_blocks), dim3(256), hist_lds, s, iv->node, sm, H,
+                               NA, cnt);
         else
-            hipLaunchKernelGGL(slot_hist_kernel, dim3((unsigned)div_up(H, 256)), dim3(256), 0, s, iv->node,
-                               f->node_to_grid, H, cnt);
+            hipLaunchKernelGGL(slot_hist_kernel, dim3((unsigned)div_up(H, 256)), dim3(256), 0, s, iv->node, sm, H,
+                               cnt);
     }
     atdf_status st = atdf_exclusive_scan_i32(cnt, NA, slot_off, scan_ws, scan_bytes, stream);
     if (st) return st;
@@ -2281,11 +2556,11 @@ static atdf_status prepare_impl(const Prepared& P, const float* O, const float* 
     const RecordSrc rs{iv->tmin, iv->tmax, iv->count, iv->first, iv->ray, O, D, P.rec()};
     if (H > 0) {
         if (lds_sort)
-            hipLaunchKernelGGL(slot_scatter_lds_kernel, dim3(sort_blocks), dim3(256), hist_lds, s, iv->node,
-                               f->node_to_grid, H, NA, slot_off, cnt, sorted, rs);
+            hipLaunchKernelGGL(slot_scatter_lds_kernel, dim3(sort_blocks), dim3(256), hist_lds, s, iv->node, sm,
+                               H, NA, slot_off, cnt, sorted, rs);
         else
-            hipLaunchKernelGGL(slot_scatter_kernel, dim3((unsigned)div_up(H, 256)), dim3(256), 0, s, iv->node,
-                               f->node_to_grid, H, slot_off, cnt, sorted, rs);
+            hipLaunchKernelGGL(slot_scatter_kernel, dim3((unsigned)div_up(H, 256)), dim3(256), 0, s, iv->node, sm,
+                               H, slot_off, cnt, sorted, rs);
     }
     unsigned long long* stotal = reinterpret_cast<unsigned long long*>(w + L.stotal);
     hipLaunchKernelGGL(fill_i32_kernel, dim3(1), dim3(256), 0, s, reinterpret_cast<int32_t*>(stotal), 2, 0);
@@ -2343,8 +2618,20 @@ static atdf_status bwd_node_impl(const Prepared& P, const void* packed_tree, con
     atdf_status st = ATDF_OK;
     const bool bf = f->grid_dtype == ATDF_GRID_BF16;
     const bool bd = f->decoder_dtype == ATDF_DECODER_BF16;
-#define ATDF_BWD(CC, BF, BD) st = launch_node_bwd<CC, BF, BD>(args, s)
-    if (f->C == 16) {
+    if (f->vm) {
+        args.grad_planes = f->vm->grad_planes;
+        args.line_part = reinterpret_cast<float*>(P.w + P.L.line_part);
+    }
+#define ATDF_BWD(CC, BF, BD) st = launch_node_bwd<CC, BF, BD, false>(args, s)
+    if (f->vm) {
+        if (f->C == 16) {
+            if (bd) st = launch_node_bwd<16, false, true, true>(args, s);
+            else st = launch_node_bwd<16, false, false, true>(args, s);
+        } else {
+            if (bd) st = launch_node_bwd<8, false, true, true>(args, s);
+            else st = launch_node_bwd<8, false, false, true>(args, s);
+        }
+    } else if (f->C == 16) {
         if (bf) { if (bd) ATDF_BWD(16, true, true); else ATDF_BWD(16, true, false); }
         else    { if (bd) ATDF_BWD(16, false, true); else ATDF_BWD(16, false, false); }
     } else {
@@ -2362,6 +2649,11 @@ static atdf_status bwd_node_impl(const Prepared& P, const void* packed_tree, con
     const int waves = bd ? bwd_waves<true>() : bwd_waves<false>();
     hipLaunchKernelGGL(reduce_partials_kernel, dim3((unsigned)div_up(np, kReduceParams)), dim3(256), 0, s, partials,
                        (int64_t)persistent_blocks() * waves, np, grad_mlp);
+    if (f->vm) {
+        const int lf = (int)P.line_floats;
+        hipLaunchKernelGGL(reduce_partials_kernel, dim3((unsigned)div_up(lf, kReduceParams)), dim3(256), 0, s,
+                           args.line_part, (int64_t)persistent_blocks(), lf, f->vm->grad_lines);
+    }
     return check_launch("atdf_project_bwd");
 }
 
@@ -2434,13 +2726,18 @@ static atdf_status project_bwd_impl(const void* packed_tree, int64_t num_nodes, 
 {
     atdf_status st = check_project_args(packed_tree, num_nodes, O, D, R, sp, iv_off, iv, f);
     if (st) return st;
+    if (f->vm) {
+        ATDF_REQUIRE(!adam, "atdf_project_bwd: the fused grid Adam does not take a tensorial field");
+        ATDF_REQUIRE(f->vm->grad_planes && f->vm->grad_lines, "atdf_project_bwd: null tensorial gradient");
+        ATDF_REQUIRE((uintptr_t)f->vm->grad_planes % 16 == 0, "atdf_project_bwd: grad_planes must be 16-byte aligned");
+    }
     if (adam) {
         st = check_adam(f, adam, grad_grid);
         if (st) return st;
         if (R == 0) return adam_without_samples(f, adam, grad_grid, stream);
     }
     if (R == 0) return ATDF_OK;
-    ATDF_REQUIRE(dLdP && grad_grid && grad_mlp, "atdf_project_bwd: null buffer");
+    ATDF_REQUIRE(dLdP && (grad_grid || f->num_grids == 0) && grad_mlp, "atdf_project_bwd: null buffer");
     Prepared P;
     st = prepared_view(iv, f, workspace, workspace_bytes, R, P, "atdf_project_bwd");
     if (st) return st;
@@ -2564,6 +2861,7 @@ atdf_status atdf_project_fwd_prepared(const void* packed_tree, int64_t num_nodes
 {
     atdf_status st = check_project_args(packed_tree, num_nodes, O, D, R, sp, iv_off, iv, f);
     if (st) return st;
+    ATDF_REQUIRE(!f->vm, "atdf_project_fwd_prepared: a tensorial field takes atdf_project_fwd");
     if (R == 0) return ATDF_OK;
     ATDF_REQUIRE(P_out, "atdf_project_fwd_prepared: null projection");
     Prepared P;
@@ -2589,6 +2887,7 @@ atdf_status atdf_sample_volume(const void* packed_tree, int64_t num_nodes, const
                                float* valid, float* position, atdf_stream_t stream)
 {
     ATDF_REQUIRE(packed_tree && num_nodes > 0 && lattice && active_node_ids, "atdf_sample_volume: null argument");
+    ATDF_REQUIRE(!f || !f->vm, "atdf_sample_volume: a tensorial field is not supported");
     ATDF_REQUIRE(lattice->mode == 0 || lattice->mode == 1, "atdf_sample_volume: mode must be 0 or 1");
     int64_t total = 1;
     for (int k = 0; k < 3; ++k) {
@@ -2665,6 +2964,7 @@ atdf_status atdf_node_density_stats_masked(const void* packed_tree, int64_t num_
                                            double* node_sum, double* node_sumsq, atdf_stream_t stream)
 {
     ATDF_REQUIRE(packed_tree && num_nodes > 0 && f, "atdf_node_density_stats: empty tree or field");
+    ATDF_REQUIRE(!f->vm, "atdf_node_density_stats: a tensorial field is not supported");
     ATDF_REQUIRE(f->H == 64 && (f->C == 16 || f->C == 8), "atdf_node_density_stats: field shape not built");
     ATDF_REQUIRE(f->grid && f->node_to_grid && f->mlp, "atdf_node_density_stats: incomplete field");
     ATDF_REQUIRE(mode == 0 || mode == 1, "atdf_node_density_stats: mode must be 0 (lattice) or 1 (random)");

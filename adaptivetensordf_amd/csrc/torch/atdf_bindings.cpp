@@ -277,6 +277,20 @@ PYBIND11_MODULE(TORCH_EXTENSION_NAME, m)
     m.def("DecodeFeatures", &DecodeFeatures);
     m.def("SampleVolumeIndirect", &SampleVolumeIndirect);
     m.def("ComputeImageVM", &ComputeImageVM);
+    m.def("ProjectRaysVM", &ProjectRaysVM, py::keep_alive<0, 1>(), py::keep_alive<0, 2>());
+    py::class_<VMTrainState>(m, "VMTrainState")
+        .def(py::init<>())
+        .def_readwrite("planes", &VMTrainState::planes)
+        .def_readwrite("lines", &VMTrainState::lines)
+        .def_readwrite("grid", &VMTrainState::grid)
+        .def_readwrite("mlp", &VMTrainState::mlp)
+        .def_readwrite("grads", &VMTrainState::grads)
+        .def_readonly("loss", &VMTrainState::loss)
+        .def_readonly("step", &VMTrainState::step)
+        .def_readwrite("lr", &VMTrainState::lr);
+    m.def("MakeVMTrainState", &MakeVMTrainState);
+    m.def("VMTrainStateFields", &VMTrainStateFields);
+    m.def("TrainStepVM", &TrainStepVM);
     m.def("FourierRingMeans", &FourierRingMeans);
     m.def("FourierLoss", &FourierLoss);
     py::class_<RegularizerParams>(m, "RegularizerParams")

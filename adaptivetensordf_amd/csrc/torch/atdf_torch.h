@@ -284,6 +284,34 @@ torch::Tensor SampleVolumeIndirect(HyperTreeBase& tree, const RoiMaps& roi, cons
 // through the VM gathers, the decoder and the integration).
 torch::Tensor ComputeImageVM(HyperTreeBase& tree, const RoiMaps& roi, const VMFields& fields, torch::Tensor mlp,
                              const FieldParams& fp, const SampleList& samples, int64_t num_rays);
+// The tensorial geometry through the fused kernels: the projection [R] of
+// ComputeImageVM for tensor_op "add" with samples recomputed from the
+// rays (CreateIntervalsForRays with sp) inside atdf_project_fwd /
+// atdf_project_bwd instead of materialised; autograd to every plane, line,
+// grid_out and the decoder.  tree and roi must outlive the backward pass.
+torch::Tensor ProjectRaysVM(HyperTreeBase& tree, const RoiMaps& roi, const VMFields& fields, torch::Tensor mlp,
+                            const SamplingParams& sp, const FieldParams& fp, const RayList& rays);
+// Training state of the tensorial geometry in the fused (channels-last)
+// layouts: planes [N_in, 3, Gp, Gp, C], lines [3, Gl, C], explicit grids
+// [N_out, G, G, G, C], decoder; gradients and Adam moments per parameter in
+// that order (the gradients are zero between steps).
+struct VMTrainState {
+    torch::Tensor planes, lines, grid, mlp;
+    std::vector<torch::Tensor> grads, m, v;
+    torch::Tensor loss;  // [1] f64 on device
+    int64_t step = 0;
+    float lr = 1e-3f;
+    float roi_min_x = -1.f, roi_max_x = 1.f;
+};
+VMTrainState MakeVMTrainState(HyperTreeBase& tree, const RoiMaps& roi, const VMFields& fields, torch::Tensor mlp,
+                              float lr, const FieldParams& fp);
+// The state's fields in the reference layouts (tensor_op and ROI from `like`).
+VMFields VMTrainStateFields(const VMTrainState& st, const VMFields& like);
+// One step: intervals, fused forward, MSE, fused adjoint, Adam on planes,
+// lines, grids and decoder (the reference's optimizer step over the
+// tensorial geometry's parameters, geometry_tensor_qo.cpp:199-255).
+torch::Tensor TrainStepVM(HyperTreeBase& tree, const RoiMaps& roi, VMTrainState& st, const RayList& rays,
+                          torch::Tensor target, const SamplingParams& sp, const FieldParams& fp);
 // Ring means [X / 2, Y] of img [Y, X, Z] (atdf_fourier_ring_means; row 0 +inf).
 torch::Tensor FourierRingMeans(torch::Tensor img);
 // The Fourier term on densities [X, Y, Z] (geometry_tensor_qo.cpp:758-850,

@@ -268,6 +268,34 @@ atdf_status atdf_accumulate_loss_per_node(const float* per_ray_loss, int32_t C, 
  * with FP32 accumulation (v_mfma_f32_16x16x16_bf16 / 16x16x32_bf16). */
 #define ATDF_DECODER_F32 0
 #define ATDF_DECODER_BF16 1
+/* The tensorial ("VM") field of the in-ROI nodes
+ * (GeometryTensorQO::SampleVolumeIndirect_feature, tensor_op "add",
+ * geometry/geometry_tensor_qo.cpp:279-327): for a sample of node n with
+ * local coordinates (lx, ly, lz) in [-1, 1] and global position (gx, gy, gz)
+ *   F[c] = Pyz[c](ly, lz) Lx[c](x') + Pxz[c](lx, lz) Ly[c](ly) + Pxy[c](lx, ly) Lz[c](gz),
+ *   x' = 2 (gx - roi_min_x) / (roi_max_x - roi_min_x) - 1,
+ * products rounded and added in that order; P are bilinear reads (first
+ * coordinate along the plane's last axis) and L linear reads of the shared
+ * lines, both with align_corners and border clamping as IndirectGridSample2D
+ * / 1D (modules/IndirectGridSample2D.cu:22-162, IndirectGridSample1D.cu:20-140).
+ * Nodes outside the ROI read the field's explicit grids (node_to_grid).
+ * F32 only (grid_dtype ATDF_GRID_F32); either decoder dtype.  Layouts are
+ * channels-last: the reference stores each plane [N, C, Gp, Gp] and each
+ * line [1, C, Gl, 1] (geometry/tensorfield.h:9-127, 218-335).
+ * grad_planes / grad_lines are the adjoint's outputs (+=, same shapes); the
+ * forward ignores them. */
+typedef struct {
+    const float* planes;           /* [num_planes, 3, Gp, Gp, C]: yz, xz, xy; (slot, p, v, u, c) */
+    const float* lines;            /* [3, Gl, C]: x, y, z                                          */
+    int64_t num_planes;
+    const int32_t* node_to_plane;  /* [num_nodes] -> plane slot, -1 = not an in-ROI node           */
+    int32_t Gp;                    /* plane edge                                                   */
+    int32_t Gl;                    /* line length                                                  */
+    float roi_min_x, roi_max_x;    /* the ROI's x range (line x coordinate)                        */
+    float* grad_planes;            /* adjoint: += dL/dplanes                                       */
+    float* grad_lines;             /* adjoint: += dL/dlines                                        */
+} atdf_vm;
+
 typedef struct {
     int32_t C;                   /* channels; 16 and 8 are built              */
     int32_t H;                   /* hidden width; 64 is built                 */
@@ -281,6 +309,10 @@ typedef struct {
     const float* mlp;            /* packed decoder parameters (f32)          */
     int32_t decoder_dtype;       /* ATDF_DECODER_F32 / ATDF_DECODER_BF16      */
     int32_t reserved;
+    const atdf_vm* vm;           /* tensorial in-ROI field, or null. Honoured by
+                                    atdf_project_fwd, atdf_project_bwd (and its
+                                    _noalloc variant); the other entry points
+                                    reject a field with vm set */
 } atdf_field;
 
 /* Forward: projection[r] = sum over the ray's samples of

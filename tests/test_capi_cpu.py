@@ -52,6 +52,52 @@ def test_validation_without_device_work():
     assert lib.atdf_scan_workspace_bytes(1024) == 8
 
 
+def test_tensorial_field_validation_without_device_work():
+    """atdf_vm checks and the entry points that reject a tensorial field,
+    all decided before any device work (fake, aligned addresses)."""
+    import ctypes as C
+
+    from adaptivetensordf_amd import capi
+
+    lib = capi.lib()
+    A = [C.c_void_p(256 * (i + 1)) for i in range(16)]
+    vm = capi.VM(A[0], A[1], 40, A[2], 9, 17, -0.5, 0.5, A[3], A[4])
+    f = capi.Field(16, 64, 5, capi.GRID_F32, 2.0, 0, A[5], 30, A[6], A[7], capi.DECODER_F32, 0, C.pointer(vm))
+    sp = capi.Sampling(4, 256, 0, 0, 0, None)
+    iv = capi.Intervals(A[8], A[9], A[10], A[11], A[12], A[13], 100)
+    args = (A[14], 9, A[15], A[15], 0, C.byref(sp), A[8], C.byref(iv), C.byref(f))
+    assert lib.atdf_project_fwd(*args, A[0], None) == capi.OK  # no rays: checks only
+    ws_vm = lib.atdf_project_bwd_workspace_bytes(1000, 100, C.byref(f))
+    f_plain = capi.Field(16, 64, 5, capi.GRID_F32, 2.0, 0, A[5], 30 + 40, A[6], A[7], capi.DECODER_F32, 0)
+    assert ws_vm > lib.atdf_project_bwd_workspace_bytes(1000, 100, C.byref(f_plain)) > 0  # + line partial rows
+    bad = [(capi.Field(16, 64, 5, capi.GRID_BF16, 2.0, 0, A[5], 30, A[6], A[7], 0, 0, C.pointer(vm)), b"f32 only"),
+           (capi.Field(16, 64, 5, 0, 2.0, 0, A[5], 30, A[6], A[7], 0, 0,
+                       C.pointer(capi.VM(A[0], A[1], 40, A[2], 1, 17, -0.5, 0.5))), b">= 2"),
+           (capi.Field(16, 64, 5, 0, 2.0, 0, A[5], 30, A[6], A[7], 0, 0,
+                       C.pointer(capi.VM(A[0], A[1], 40, A[2], 9, 17, 0.5, 0.5))), b"ROI x range"),
+           (capi.Field(16, 64, 5, 0, 2.0, 0, A[5], 30, A[6], A[7], 0, 0,
+                       C.pointer(capi.VM(C.c_void_p(260), A[1], 40, A[2], 9, 17, -0.5, 0.5))), b"aligned"),
+           (capi.Field(16, 64, 5, 0, 2.0, 0, A[5], 30, A[6], A[7], 0, 0,
+                       C.pointer(capi.VM(A[0], None, 40, A[2], 9, 17, -0.5, 0.5))), b"incomplete tensorial")]
+    for fb, msg in bad:
+        st = lib.atdf_project_fwd(A[14], 9, A[15], A[15], 0, C.byref(sp), A[8], C.byref(iv), C.byref(fb), A[0], None)
+        assert st == capi.ERR_INVALID_ARGUMENT and msg in lib.atdf_last_error(), msg
+    # one ray: the checks pass, then these entry points refuse the field
+    args1 = (A[14], 9, A[15], A[15], 1, C.byref(sp), A[8], C.byref(iv), C.byref(f))
+    st = lib.atdf_project_fwd_prepared(*args1, A[0], A[1], 1 << 30, None)
+    assert st == capi.ERR_INVALID_ARGUMENT and b"tensorial" in lib.atdf_last_error()
+    st = lib.atdf_project_fwd_bins(*args1, A[0], A[1], None)
+    assert st == capi.ERR_INVALID_ARGUMENT and b"tensorial" in lib.atdf_last_error()
+    ga = capi.GridAdam(A[5], None, A[2], A[3], 1e-3, 0.9, 0.999, 1e-8, 1)
+    st = lib.atdf_project_bwd_adam(*args1, A[0], A[5], A[7], C.byref(ga), A[1], 1 << 30, None)
+    assert st == capi.ERR_INVALID_ARGUMENT and b"tensorial" in lib.atdf_last_error()
+    vm_nograd = capi.VM(A[0], A[1], 40, A[2], 9, 17, -0.5, 0.5, None, None)
+    f_ng = capi.Field(16, 64, 5, 0, 2.0, 0, A[5], 30, A[6], A[7], 0, 0, C.pointer(vm_nograd))
+    st = lib.atdf_project_bwd(A[14], 9, A[15], A[15], 1, C.byref(sp), A[8], C.byref(iv), C.byref(f_ng), A[0], A[5],
+                              A[7], A[1], 1 << 30, None)
+    assert st == capi.ERR_INVALID_ARGUMENT and b"tensorial gradient" in lib.atdf_last_error()
+
+
 def test_proposal_boundary_without_device_work(oracle):
     import ctypes as C
 
