@@ -287,7 +287,9 @@ PYBIND11_MODULE(TORCH_EXTENSION_NAME, m)
         .def_readwrite("grads", &VMTrainState::grads)
         .def_readonly("loss", &VMTrainState::loss)
         .def_readonly("step", &VMTrainState::step)
-        .def_readwrite("lr", &VMTrainState::lr);
+        .def_readwrite("lr", &VMTrainState::lr)
+        .def_readonly("roi_min_x", &VMTrainState::roi_min_x)
+        .def_readonly("roi_max_x", &VMTrainState::roi_max_x);
     m.def("MakeVMTrainState", &MakeVMTrainState);
     m.def("VMTrainStateFields", &VMTrainStateFields);
     m.def("TrainStepVM", &TrainStepVM);

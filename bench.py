@@ -771,11 +771,12 @@ def main():
 
 
 def vm_leg(N, tree, batches, K, s_eff, n, args, steps):
-    """Training step of the tensorial geometry (GeometryTensorQO) through the
-    composed operators: CreateSamplesForRays, ComputeImageVM (VM gathers for
-    in-ROI nodes, explicit grids for the rest, DecodeFeatures, integration),
-    MSE, autograd backward (slot-major gather adjoints, decoder adjoint),
-    Adam on every field.  One JSON line of its own (not the headline)."""
+    """Training step of the tensorial geometry (GeometryTensorQO): the fused
+    path (TrainStepVM: the plane x line and explicit-grid gathers inside
+    the projection kernels and their adjoint, MSE, Adam on every field)
+    beside the composed operators (CreateSamplesForRays, ComputeImageVM,
+    autograd, Adam) on the same fields and rays.  One JSON line of its own
+    (not the headline); `value` is the fused path."""
     import torch
 
     from adaptivetensordf_amd import phantom
@@ -798,11 +799,16 @@ def vm_leg(N, tree, batches, K, s_eff, n, args, steps):
     f.roi_min, f.roi_max = lo, hi
     mlp = phantom.init_mlp(C, H, seed=0).to(dev).requires_grad_()
     fp = field_params(N, False)
+    sp = N.SamplingParams()
+    sp.max_samples_per_node = K
+    sp.jitter_mode = 2
+
+    # composed operators
     params = [f.plane_yz, f.plane_xz, f.plane_xy, f.line_x, f.line_y, f.line_z, f.grid_out, mlp]
     moments = [(torch.zeros_like(p), torch.zeros_like(p)) for p in params]
     state = {"it": 0, "loss": None}
 
-    def step():
+    def step_composed():
         rays, tgt = batches[state["it"] % len(batches)]
         samples = tree.CreateSamplesForRays(rays, K, True)
         img = N.ComputeImageVM(tree, roi, f, mlp, fp, samples, rays.size())
@@ -813,30 +819,119 @@ def vm_leg(N, tree, batches, K, s_eff, n, args, steps):
             N.AdamStep(p.data, g.contiguous(), m, v, state["it"], 1e-3)
         state["loss"] = loss.detach()
 
-    for _ in range(args.warmup):
-        step()
-    torch.cuda.synchronize()
-    first = float(state["loss"].item()) if state["loss"] is not None else None
-    t0 = time.perf_counter()
-    for _ in range(steps):
-        step()
-    torch.cuda.synchronize()
-    el = time.perf_counter() - t0
+    # fused (state copied before the composed leg trains the fields)
+    st = N.MakeVMTrainState(tree, roi, f, mlp, 1e-3, fp)
+    fstate = {"it": 0}
+
+    def step_fused():
+        rays, tgt = batches[fstate["it"] % len(batches)]
+        sp.jitter_seed = 1000 + fstate["it"]
+        N.TrainStepVM(tree, roi, st, rays, tgt, sp, fp)
+        fstate["it"] += 1
+
+    def timed(fn):
+        for _ in range(args.warmup):
+            fn()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            fn()
+        torch.cuda.synchronize()
+        return time.perf_counter() - t0
+
+    el_c = timed(step_composed)
+    loss_c = float(state["loss"].item())
+    el_f = timed(step_fused)
+    loss_f = float(st.loss.item())
     R = batches[0][0].size()
-    log(f"vm: {steps} steps in {el:.3f}s: {el * 1e3 / steps:.2f} ms/step, {R * steps / el / 1e6:.3f} M rays/s, "
-        f"{n_in} VM / {n_out} explicit nodes")
+    kern = vm_kernel_times(N, tree, roi, st, batches[0][0], sp, fp)
+    log(f"vm composed: {el_c * 1e3 / steps:.2f} ms/step, {R * steps / el_c / 1e6:.3f} M rays/s; fused: "
+        f"{el_f * 1e3 / steps:.2f} ms/step, {R * steps / el_f / 1e6:.3f} M rays/s; {n_in} VM / {n_out} explicit nodes")
     print(json.dumps({
-        "metric": "fwd+bwd rays/s, tensorial (VM) field through the composed operators, 512^3, 61 tilts",
-        "value": R * steps / el, "unit": "rays/s", "n_gpus": 1, "steps": steps, "warmup": args.warmup,
-        "ms_per_step": el * 1e3 / steps, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "metric": "fwd+bwd rays/s, tensorial (VM) field through the fused projection kernels, 512^3, 61 tilts",
+        "value": R * steps / el_f, "unit": "rays/s", "n_gpus": 1, "steps": steps, "warmup": args.warmup,
+        "ms_per_step": el_f * 1e3 / steps, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f32", "data": "synthetic (voxelised noisy ellipsoid phantom projections, random-init field)",
         "config": {"workload": f"tensorial geometry training step, {n}^3 phantom, 61 tilts, {n}x{n} detector, "
                                f"grown octree, ROI {lo}..{hi}: {n_in} plane/line nodes, {n_out} explicit-grid nodes, "
-                               f"lines of {line}",
+                               f"planes {G}^2, lines of {line}, tensor_op add; step = sampling + fused fwd + MSE + "
+                               f"fused adjoint + Adam on planes, lines, grids and decoder",
                    "rays_per_batch": R, "K": K, "stratified_samples_per_ray": round(s_eff, 2), "channels": C,
                    "grid": G, "decoder": f"{C}-{H}-{H}-1", "parallelism": "single GPU",
-                   "loss_first_last": [first, float(state["loss"].item())]},
-        "roofline": None, "cpu_baseline": None}), flush=True)
+                   "loss_last": {"fused": loss_f, "composed": loss_c},
+                   "composed": {"ms_per_step": el_c * 1e3 / steps, "rays_per_s": R * steps / el_c},
+                   "speedup_fused_over_composed": el_c / el_f, "kernel_ms": kern["kernel_ms"],
+                   "samples_in_roi": kern["samples_in_roi"], "samples_out_roi": kern["samples_out_roi"]},
+        "roofline": kern["roofline"], "cpu_baseline": None}), flush=True)
+
+
+def vm_kernel_times(N, tree, roi, st, rays, sp, fp):
+    """HIP-event durations of atdf_project_fwd / atdf_project_bwd with the
+    tensorial field over one batch, and the adjoint's roofline.  Algorithmic
+    bytes: an in-ROI sample gathers 3 planes x 4 corners + 3 lines x 2 corners
+    of C f32 (72 C B) and the adjoint scatters as many; an explicit-grid
+    sample 8 corners x C f32 each way; 28 B per ray."""
+    import ctypes as Ct
+
+    import torch
+
+    from adaptivetensordf_amd import capi
+
+    iv = tree.CreateIntervalsForRays(rays, sp)
+    R = rays.size()
+    flag = roi.in_roi_flag.to(iv.node.device)[iv.node.long()].bool()
+    s_in = int(iv.count[flag].sum().item())
+    s_out = int(iv.count[~flag].sum().item())
+    n2p = roi.prefix_inroi.to(torch.int32).contiguous()
+    n2g = roi.prefix_outroi.to(torch.int32).contiguous()
+    gpl, gln = torch.zeros_like(st.planes), torch.zeros_like(st.lines)
+    ggr, gml = torch.zeros_like(st.grid), torch.zeros_like(st.mlp)
+    vm = capi.VM(st.planes.data_ptr(), st.lines.data_ptr(), st.planes.shape[0], n2p.data_ptr(), st.planes.shape[2],
+                 st.lines.shape[1], st.roi_min_x, st.roi_max_x, gpl.data_ptr(), gln.data_ptr())
+    f = capi.Field(C, H, G, capi.GRID_F32, 2.0, 0, st.grid.data_ptr() if st.grid.numel() else None, st.grid.shape[0],
+                   n2g.data_ptr(), st.mlp.data_ptr(), capi.DECODER_F32, 0, Ct.pointer(vm))
+    csp = capi.Sampling(sp.max_samples_per_node, 256, sp.jitter_mode, 0, sp.jitter_seed, None, None)
+    civ = capi.Intervals(iv.node.data_ptr(), iv.tmin.data_ptr(), iv.tmax.data_ptr(), iv.count.data_ptr(),
+                         iv.first.data_ptr(), iv.ray.data_ptr(), iv.num_intervals)
+    packed = tree.PackedTree()
+    s = capi.stream_ptr()
+    P = torch.empty(R, device=rays.origin.device)
+    dP = torch.randn(R, device=rays.origin.device) * 1e-6
+    ws = torch.empty(capi.lib().atdf_project_bwd_workspace_bytes(R, iv.num_intervals, Ct.byref(f)),
+                     dtype=torch.uint8, device=rays.origin.device)
+    a = (capi.ptr(packed), tree.NumNodes(), capi.ptr(rays.origin), capi.ptr(rays.direction), R, Ct.byref(csp),
+         capi.ptr(iv.ray_offset), Ct.byref(civ), Ct.byref(f))
+
+    def fwd():
+        capi.call("atdf_project_fwd", *a, capi.ptr(P), s)
+
+    def bwd():
+        capi.call("atdf_project_bwd", *a, capi.ptr(dP), capi.ptr(ggr) if ggr.numel() else None, capi.ptr(gml),
+                  capi.ptr(ws), ws.numel(), s)
+
+    def timed(fn, reps=5):
+        fn()
+        torch.cuda.synchronize()
+        cur = torch.cuda.current_stream()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(cur)
+        for _ in range(reps):
+            fn()
+        e1.record(cur)
+        e1.synchronize()
+        return e0.elapsed_time(e1) / reps / 1e3
+
+    t_fwd = timed(fwd)
+    t_bwd = timed(bwd)
+    bwd_bytes = s_in * 2 * 72 * C + s_out * 2 * 8 * C * 4 + R * 28
+    gbs = bwd_bytes / t_bwd / 1e9
+    log(f"  vm project_fwd {t_fwd * 1e3:.2f} ms, project_bwd {t_bwd * 1e3:.2f} ms (counting sort included; "
+        f"{s_in} in-ROI + {s_out} explicit samples): {gbs:.0f} GB/s algorithmic")
+    return {"kernel_ms": {"project_fwd": t_fwd * 1e3, "project_bwd": t_bwd * 1e3}, "samples_in_roi": s_in,
+            "samples_out_roi": s_out,
+            "roofline": {"bound": "hbm", "kernel": "project_bwd (tensorial)", "achieved": gbs, "peak": HBM_PEAK_GBS,
+                         "unit": "GB/s", "frac": gbs / HBM_PEAK_GBS, "traffic": None,
+                         "algorithmic_bytes_per_launch": bwd_bytes}}
 
 
 if __name__ == "__main__":
