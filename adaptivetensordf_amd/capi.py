@@ -159,6 +159,8 @@ _SIGNATURES = {
     "atdf_leaf_table": (_I32, [_P, _I64, _I32, _P, _P]),
     "atdf_proposal_features": (_I32, [_P, _I64, _P, _P, _I64, C.POINTER(Proposal), _P, _P, _P]),
     "atdf_proposal_sample": (_I32, [_P, _I64, _P, _P, _I64, C.POINTER(Proposal), _P, _P, _P, _P, _P, _P, _P]),
+    "atdf_proposal_allocate": (_I32, [_P, _I64, _P, _P, _I64, C.POINTER(Proposal), _I32, _P, C.POINTER(Intervals),
+                                      _P, _P, _P, _P]),
     "atdf_proposal_merge_counts": (_I32, [_I64, _P, _P, _P, _P, _P]),
     "atdf_proposal_write": (_I32, [_I64, _P, _P, _P, _P, _P, _P, C.POINTER(Intervals), _P]),
     "atdf_proposal_grad_workspace_bytes": (_SZ, [_I64]),

@@ -326,28 +326,19 @@ __global__ __launch_bounds__(256) void features_kernel(PropArgs a)
 
 // One ray per lane: features, MLP, pdf, KL gate, systematic allocation and
 // the one-sample intervals of a learned ray.
-__global__ __launch_bounds__(kThreads) void sample_kernel(PropArgs a, const float* __restrict__ params)
+// The mixed pdf pm[32] of ray r over its root chord [t0, t1] and the gate's
+// KL(coarse || pdf); false (kl = 0) when the ray has no root chord.  Every
+// lane of the wave runs the network (its activation columns share `buf`).
+__device__ __forceinline__ bool ray_pdf(const PropArgs& a, const float* __restrict__ params, float* buf, int lane,
+                                        const PNode& root, const float* o, const float* d, float* pm, float& t0,
+                                        float& t1, float& kl)
 {
-    extern __shared__ __attribute__((aligned(16))) float sm[];
-    float* buf = sm + (threadIdx.x >> 6) * kActFloats;
-    const int lane = threadIdx.x & 63;
-    const int64_t r = blockIdx.x * (int64_t)kThreads + threadIdx.x;
-    if (r >= a.R) return;  // no barrier below
-    const PNode root = a.nodes[0];
-    float o[3], d[3], x[NI], t0, t1;
-#pragma unroll
-    for (int k = 0; k < 3; ++k) {
-        o[k] = a.O[r * 3 + k];
-        d[k] = a.D[r * 3 + k];
-    }
+    float x[NI];
     const bool ok = ray_features(root, a.table, a.depth, a.coarse, o, d, x, t0, t1);
     float z[NB];
     mlp_forward_s(params, buf, lane, x, z);
-    a.fallback[r] = 0;
-    a.n_learned[r] = 0;
-    if (a.kl) a.kl[r] = 0.f;
-    if (!(t1 > t0) || !ok) return;
-    float pm[NB];
+    kl = 0.f;
+    if (!(t1 > t0) || !ok) return false;
     softmax32(z, pm);
     const float a1 = 1.0f - a.mix;
     const float a0 = a.mix * (1.0f / (float)NB);
@@ -356,7 +347,6 @@ __global__ __launch_bounds__(kThreads) void sample_kernel(PropArgs a, const floa
     float Q = 0.f;
 #pragma unroll
     for (int b = 0; b < NB; ++b) Q = Q + x[6 + b];
-    float kl = 0.f;
     if (Q > 0.f) {
         bool inf = false;
 #pragma unroll
@@ -373,6 +363,29 @@ __global__ __launch_bounds__(kThreads) void sample_kernel(PropArgs a, const floa
         }
         if (inf) kl = INFINITY;
     }
+    return true;
+}
+
+__global__ __launch_bounds__(kThreads) void sample_kernel(PropArgs a, const float* __restrict__ params)
+{
+    extern __shared__ __attribute__((aligned(16))) float sm[];
+    float* buf = sm + (threadIdx.x >> 6) * kActFloats;
+    const int lane = threadIdx.x & 63;
+    const int64_t r = blockIdx.x * (int64_t)kThreads + threadIdx.x;
+    if (r >= a.R) return;  // no barrier below
+    const PNode root = a.nodes[0];
+    float o[3], d[3], t0, t1, kl;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        o[k] = a.O[r * 3 + k];
+        d[k] = a.D[r * 3 + k];
+    }
+    float pm[NB];
+    const bool ok = ray_pdf(a, params, buf, lane, root, o, d, pm, t0, t1, kl);
+    a.fallback[r] = 0;
+    a.n_learned[r] = 0;
+    if (a.kl) a.kl[r] = 0.f;
+    if (!ok) return;
     if (a.kl) a.kl[r] = kl;
     if (kl > a.kl_threshold) {
         a.fallback[r] = 1;
@@ -426,6 +439,78 @@ __global__ __launch_bounds__(kThreads) void sample_kernel(PropArgs a, const floa
         if (!last) c = c + pm[b];
     }
     a.n_learned[r] = emitted;
+}
+
+// Learned allocation over the stratified node intervals of a ray list: for a
+// ray the gate keeps, interval i gets 1 + round(E F(tmax_i)) - round(E F(tmin_i))
+// samples (at least one per node interval; the extras telescope to E over
+// intervals that tile the chord), F the CDF of the mixed pdf over the root
+// chord, linear inside each of the 32 bins and clamped to [0, 1],
+// round(x) = floor(x + 1/2); `first` becomes the running sum of the ray's
+// counts.  Fallback rays and rays without a root chord keep the stratified
+// counts.  The CDF and the pdf sit in the lane's own column of the wave's
+// activation buffer (rows 0..31: F at the bin edges, 32..63: pm), read at the
+// bin of each t.
+__global__ __launch_bounds__(kThreads) void alloc_kernel(PropArgs a, const float* __restrict__ params, int extra,
+                                                        const int64_t* __restrict__ iv_off,
+                                                        const float* __restrict__ iv_tmin,
+                                                        const float* __restrict__ iv_tmax,
+                                                        int32_t* __restrict__ iv_count,
+                                                        int32_t* __restrict__ iv_first, int32_t* __restrict__ ray_ns)
+{
+    extern __shared__ __attribute__((aligned(16))) float sm[];
+    float* buf = sm + (threadIdx.x >> 6) * kActFloats;
+    const int lane = threadIdx.x & 63;
+    const int64_t r = blockIdx.x * (int64_t)kThreads + threadIdx.x;
+    if (r >= a.R) return;  // no barrier below
+    const PNode root = a.nodes[0];
+    float o[3], d[3], t0, t1, kl;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        o[k] = a.O[r * 3 + k];
+        d[k] = a.D[r * 3 + k];
+    }
+    float pm[NB];
+    const bool ok = ray_pdf(a, params, buf, lane, root, o, d, pm, t0, t1, kl);
+    const bool fall = ok && kl > a.kl_threshold;
+    a.fallback[r] = fall ? 1 : 0;
+    if (a.kl) a.kl[r] = kl;
+    const bool learn = ok && !fall;
+    if (learn) {
+        float c = 0.f;
+#pragma unroll
+        for (int b = 0; b < NB; ++b) {
+            buf[b * 64 + lane] = c;
+            buf[(NB + b) * 64 + lane] = pm[b];
+            c = c + pm[b];
+        }
+    }
+    const float len = (t1 - t0) / (float)NB;
+    const float E = (float)extra;
+    auto cdf = [&](float t) {
+        int b = (int)floorf((t - t0) / len);
+        b = b < 0 ? 0 : (b > NB - 1 ? NB - 1 : b);
+        float F = buf[b * 64 + lane] + buf[(NB + b) * 64 + lane] * ((t - bin_edge(t0, t1, len, b)) / len);
+        if (!(F >= 0.f)) F = 0.f;
+        if (!(F <= 1.f)) F = 1.f;
+        return F;
+    };
+    const int64_t h0 = iv_off[r], h1 = iv_off[r + 1];
+    int s = 0;
+    for (int64_t h = h0; h < h1; ++h) {
+        int n;
+        if (learn) {
+            const int ka = (int)floorf(E * cdf(iv_tmin[h]) + 0.5f);
+            const int kb = (int)floorf(E * cdf(iv_tmax[h]) + 0.5f);
+            n = 1 + (kb > ka ? kb - ka : 0);
+            iv_count[h] = n;
+        } else {
+            n = iv_count[h];
+        }
+        iv_first[h] = s;
+        s += n;
+    }
+    if (ray_ns) ray_ns[r] = s;
 }
 
 __global__ __launch_bounds__(256) void merge_counts_kernel(int64_t R, const int32_t* __restrict__ fallback,
@@ -815,6 +900,32 @@ atdf_status atdf_proposal_sample(const void* packed_tree, int64_t num_nodes, con
     hipLaunchKernelGGL(sample_kernel, dim3((unsigned)div_up(num_rays, kThreads)), dim3(kThreads), kSampleLds,
                        to_hip(stream), a, proposal->params);
     return check_launch("atdf_proposal_sample");
+}
+
+atdf_status atdf_proposal_allocate(const void* packed_tree, int64_t num_nodes, const float* ray_origin,
+                                   const float* ray_direction, int64_t num_rays, const atdf_proposal* proposal,
+                                   int32_t extra_samples, const int64_t* ray_interval_offset,
+                                   const atdf_intervals* intervals, int32_t* ray_fallback, int32_t* ray_num_samples,
+                                   float* ray_kl, atdf_stream_t stream)
+{
+    atdf_status st = check_proposal(packed_tree, num_nodes, proposal, "atdf_proposal_allocate");
+    if (st) return st;
+    ATDF_REQUIRE(num_rays >= 0, "atdf_proposal_allocate: negative ray count");
+    ATDF_REQUIRE(extra_samples >= 0 && extra_samples <= (1 << 20), "atdf_proposal_allocate: extra_samples out of range");
+    if (num_rays == 0) return ATDF_OK;
+    ATDF_REQUIRE(ray_origin && ray_direction && ray_interval_offset && intervals && ray_fallback,
+                 "atdf_proposal_allocate: null buffer");
+    ATDF_REQUIRE(intervals->tmin && intervals->tmax && intervals->count && intervals->first,
+                 "atdf_proposal_allocate: incomplete interval buffers");
+    ATDF_REQUIRE((int)kSampleLds <= max_lds_bytes(), "atdf_proposal_allocate: needs %d bytes of LDS per workgroup",
+                 (int)kSampleLds);
+    PropArgs a = make_args(packed_tree, ray_origin, ray_direction, num_rays, proposal);
+    a.fallback = ray_fallback;
+    a.kl = ray_kl;
+    hipLaunchKernelGGL(alloc_kernel, dim3((unsigned)div_up(num_rays, kThreads)), dim3(kThreads), kSampleLds,
+                       to_hip(stream), a, proposal->params, (int)extra_samples, ray_interval_offset, intervals->tmin,
+                       intervals->tmax, intervals->count, intervals->first, ray_num_samples);
+    return check_launch("atdf_proposal_allocate");
 }
 
 atdf_status atdf_proposal_merge_counts(int64_t num_rays, const int32_t* ray_fallback,

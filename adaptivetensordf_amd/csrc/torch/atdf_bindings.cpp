@@ -360,7 +360,8 @@ PYBIND11_MODULE(TORCH_EXTENSION_NAME, m)
         .def_readwrite("mix", &ProposalParams::mix)
         .def_readwrite("kl_threshold", &ProposalParams::kl_threshold)
         .def_readwrite("coarse_lattice", &ProposalParams::coarse_lattice)
-        .def_readwrite("target_power", &ProposalParams::target_power);
+        .def_readwrite("target_power", &ProposalParams::target_power)
+        .def_readwrite("alloc_extra", &ProposalParams::alloc_extra);
     py::class_<ProposalState>(m, "ProposalState")
         .def_readwrite("params", &ProposalState::params)
         .def_readwrite("grad", &ProposalState::grad)

@@ -1223,6 +1223,29 @@ Intervals CreateProposalIntervals(HyperTreeBase& tree, const ProposalState& ps, 
     check_rays(rays);
     TORCH_CHECK(fallback_sp.jitter_mode != 1, "CreateProposalIntervals: jitter_mode 1 needs materialised samples");
     const int64_t R = rays.size();
+    if (ps.pp.alloc_extra > 0) {
+        // allocation mode: the stratified list of every ray, then the learned
+        // counts of the rays the gate keeps
+        Intervals iv = tree.CreateIntervalsForRays(rays, fallback_sp);
+        auto dev = tree.device();
+        auto i32 = torch::TensorOptions().dtype(torch::kInt32).device(dev);
+        auto fb = torch::empty({R}, i32), ns = torch::empty({R}, i32);
+        if (R > 0) {
+            atdf_proposal p = proposal_view(tree, ps, &fallback_sp);
+            atdf_intervals w = iv.view();
+            check_status(atdf_proposal_allocate(tree.packed(), tree.NumNodes(), rays.origin.data_ptr<float>(),
+                                                rays.direction.data_ptr<float>(), R, &p, ps.pp.alloc_extra,
+                                                iv.ray_offset.data_ptr<int64_t>(), &w, fb.data_ptr<int32_t>(),
+                                                ns.data_ptr<int32_t>(), nullptr, current_stream()),
+                         "atdf_proposal_allocate");
+            auto info = torch::stack({ns.to(torch::kLong).sum(), ns.max().to(torch::kLong)}).to(torch::kCPU);
+            iv.num_samples = info[0].item<int64_t>();
+            iv.max_samples_per_ray = (int)info[1].item<int64_t>();
+        }
+        iv.ray_num_samples = ns;
+        if (ray_fallback) *ray_fallback = fb;
+        return iv;
+    }
     auto v = ProposalSample(tree, ps, rays, fallback_sp);
     Tensor fb = v[0], nl = v[1];
     atdf_sampling s = fallback_sp.view();

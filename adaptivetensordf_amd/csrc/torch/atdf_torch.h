@@ -459,6 +459,12 @@ struct ProposalParams {
     float kl_threshold = 0.5f;  // fallback when KL(coarse || pdf) exceeds it
     int coarse_lattice = 4;     // n^3 lattice per node for the coarse density
     float target_power = 0.5f;  // training target q_b ~ (binned density mass)^power
+    // 0: the proposal places 32 one-sample cells per ray.  E > 0: it allocates
+    // over the stratified traversal's node intervals instead -- every node
+    // interval keeps one sample and E more per ray follow the proposal's CDF
+    // (atdf_proposal_allocate); the traversal's K gives the fallback rays'
+    // counts.
+    int alloc_extra = 0;
 };
 struct ProposalState {
     torch::Tensor params, grad, m, v;  // [P] f32

@@ -727,6 +727,24 @@ atdf_status atdf_proposal_sample(const void* packed_tree, int64_t num_nodes, con
                                  int32_t* ray_fallback, int32_t* ray_num_learned, int32_t* learned_node,
                                  float* learned_tmin, float* learned_tmax, float* ray_kl, atdf_stream_t stream);
 
+/* Learned allocation over a stratified interval list (the proposal's second
+ * mode; no reference counterpart).  `intervals` is the list of
+ * atdf_traverse_write for every ray (any K); for each ray the gate keeps
+ * (KL(coarse || pdf) <= kl_threshold), node interval i of the ray gets
+ *   1 + round(E F(tmax_i)) - round(E F(tmin_i))
+ * samples, E = extra_samples, F the CDF of the mixed pdf over the ray's root
+ * chord (linear within each of the 32 bins, clamped to [0, 1]),
+ * round(x) = floor(x + 1/2): every node interval keeps at least one sample
+ * and the extras telescope to E over intervals that tile the chord.  count
+ * and first are rewritten in place; fallback rays and rays without a root
+ * chord keep their stratified counts.  ray_fallback [num_rays] (0/1),
+ * ray_num_samples (nullable) = the ray's new sample count, ray_kl nullable. */
+atdf_status atdf_proposal_allocate(const void* packed_tree, int64_t num_nodes, const float* ray_origin,
+                                   const float* ray_direction, int64_t num_rays, const atdf_proposal* proposal,
+                                   int32_t extra_samples, const int64_t* ray_interval_offset,
+                                   const atdf_intervals* intervals, int32_t* ray_fallback, int32_t* ray_num_samples,
+                                   float* ray_kl, atdf_stream_t stream);
+
 /* Mixed interval list: run atdf_traverse_count with sampling.ray_select =
  * ray_fallback, then this replaces the counts of the learned rays by
  * ray_num_learned (in place); after atdf_exclusive_scan_i32 and

@@ -247,6 +247,9 @@ void atdfo_prop_forward(const float* params, const float* feat, int64_t R, float
  * cells and cells whose sample lies where no active node is are dropped. */
 void atdfo_prop_sample(const atdfo_proposal* p, const float* o, const float* d, int64_t R, int32_t* fallback,
                        int32_t* n_learned, int32_t* node, float* tmin, float* tmax, float* kl);
+void atdfo_prop_allocate(const atdfo_proposal* p, const float* o, const float* d, int64_t R, int32_t extra,
+                         const int64_t* iv_off, const float* tmin, const float* tmax, int32_t* count, int32_t* first,
+                         int32_t* fallback, int32_t* ns, float* kl_out);
 /* FP64 cross-entropy of the proposal pdf against the target q_b ~
  * bins_b^power normalised per ray (bins [R, 32] the binned masses; rays with
  * zero mass skipped), divided by R; grad (overwritten, FP64) of that loss

@@ -244,6 +244,91 @@ static float cell_sample_t(float lo, float hi, float a)
     return t1 * (1 - a) + t2 * a;
 }
 
+/* The mixed pdf pm[32] of one ray over its root chord [t0, t1] and the
+ * gate's KL(coarse || pdf); 0 (kl = 0) when the ray has no root chord. */
+static int ray_pdf(const atdfo_proposal* p, const float* ro, const float* rd, float* pm, float* t0, float* t1,
+                   float* kl_out)
+{
+    const float a1 = 1.0f - p->mix;
+    const float a0 = p->mix * (1.0f / (float)NB);
+    float f[NI];
+    ray_features(p, ro, rd, f, t0, t1);
+    *kl_out = 0;
+    if (!(*t1 > *t0)) return 0;
+    float z[NB], pr[NB];
+    forward_one(p->params, f, z, pr);
+    for (int b = 0; b < NB; ++b) pm[b] = pr[b] * a1 + a0;
+    /* KL(coarse || pdf) */
+    float Q = 0;
+    for (int b = 0; b < NB; ++b) Q = Q + f[6 + b];
+    float kl = 0;
+    if (Q > 0) {
+        for (int b = 0; b < NB; ++b) {
+            float occ = f[6 + b];
+            if (!(occ > 0)) continue;
+            if (!(pm[b] > 0)) {
+                kl = INFINITY;
+                break;
+            }
+            float q = occ / Q;
+            kl = kl + q * atdfo_prop_log(q / pm[b]);
+        }
+    }
+    *kl_out = kl;
+    return 1;
+}
+
+/* Learned allocation over a stratified interval list (atdf_proposal_allocate):
+ * for a ray the gate keeps, interval i gets 1 + round(E F(tmax_i)) -
+ * round(E F(tmin_i)) samples, F(t) = cum[b] + pm[b] ((t - edge_b) / len) at
+ * b = clamp(floor((t - t0) / len), 0, 31), clamped to [0, 1], cum the running
+ * sum of pm, round(x) = floor(x + 1/2); first = the running sum of counts.
+ * Other rays keep their counts (first rewritten the same way). */
+void atdfo_prop_allocate(const atdfo_proposal* p, const float* o, const float* d, int64_t R, int32_t extra,
+                         const int64_t* iv_off, const float* tmin, const float* tmax, int32_t* count, int32_t* first,
+                         int32_t* fallback, int32_t* ns, float* kl_out)
+{
+    for (int64_t r = 0; r < R; ++r) {
+        float pm[NB], t0, t1, kl;
+        int ok = ray_pdf(p, o + r * 3, d + r * 3, pm, &t0, &t1, &kl);
+        int fall = ok && kl > p->kl_threshold;
+        fallback[r] = fall;
+        if (kl_out) kl_out[r] = kl;
+        int learn = ok && !fall;
+        float cum[NB];
+        float c = 0;
+        for (int b = 0; b < NB; ++b) {
+            cum[b] = c;
+            c = c + pm[b];
+        }
+        float len = (t1 - t0) / (float)NB;
+        float E = (float)extra;
+        int s = 0;
+        for (int64_t h = iv_off[r]; h < iv_off[r + 1]; ++h) {
+            int n;
+            if (learn) {
+                float tt[2] = {tmin[h], tmax[h]};
+                int k[2];
+                for (int e = 0; e < 2; ++e) {
+                    int b = (int)floorf((tt[e] - t0) / len);
+                    b = b < 0 ? 0 : (b > NB - 1 ? NB - 1 : b);
+                    float F = cum[b] + pm[b] * ((tt[e] - bin_edge(t0, t1, len, b)) / len);
+                    if (!(F >= 0.f)) F = 0.f;
+                    if (!(F <= 1.f)) F = 1.f;
+                    k[e] = (int)floorf(E * F + 0.5f);
+                }
+                n = 1 + (k[1] > k[0] ? k[1] - k[0] : 0);
+                count[h] = n;
+            } else {
+                n = count[h];
+            }
+            first[h] = s;
+            s += n;
+        }
+        if (ns) ns[r] = s;
+    }
+}
+
 void atdfo_prop_sample(const atdfo_proposal* p, const float* o, const float* d, int64_t R, int32_t* fallback,
                        int32_t* n_learned, int32_t* node, float* tmin, float* tmax, float* kl_out)
 {

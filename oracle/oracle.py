@@ -445,6 +445,22 @@ class Proposal:
                                 _p(out["kl"]))
         return out
 
+    def allocate(self, o, d, extra, iv_off, tmin, tmax, count):
+        """Learned allocation over a stratified interval list: (count, first,
+        fallback, ray sample counts, kl), count and first rewritten."""
+        o, d = _c(o, np.float32), _c(d, np.float32)
+        R = o.shape[0]
+        iv_off = _c(iv_off, np.int64)
+        tmin, tmax = _c(tmin, np.float32), _c(tmax, np.float32)
+        cnt = np.array(count, np.int32, copy=True)
+        first = np.zeros_like(cnt)
+        fb = np.zeros(R, np.int32)
+        ns = np.zeros(R, np.int32)
+        kl = np.zeros(R, np.float32)
+        lib().atdfo_prop_allocate(C.byref(self.s), _p(o), _p(d), C.c_int64(R), C.c_int32(extra), _p(iv_off),
+                                  _p(tmin), _p(tmax), _p(cnt), _p(first), _p(fb), _p(ns), _p(kl))
+        return cnt, first, fb, ns, kl
+
 
 def prop_grad(params, feat, bins, power=1.0):
     params = _c(params, np.float32)

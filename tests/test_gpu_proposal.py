@@ -229,6 +229,73 @@ def test_mixed_interval_list_and_projection(oracle, native, cuda, jitter_mode):
     assert e_g <= 1e-5 and e_m <= 1e-5
 
 
+@pytest.mark.parametrize("jitter_mode", [0, 2])
+def test_allocated_interval_list_and_projection(oracle, native, cuda, jitter_mode):
+    """Allocation mode (ProposalParams.alloc_extra = E): the stratified list
+    of every ray with the learned counts of the gated rays, bit-exact against
+    the oracle's allocation over the oracle's traversal; at least one sample
+    per node interval and E extras per gated ray (up to a rounding tie at an
+    end); the projection over it within 1e-5 of the FP64 oracle."""
+    from adaptivetensordf_amd import phantom
+
+    t, act, tree = _scene(oracle, native, seed=2)
+    o, d = _rays(1500, 2)
+    rays = native.RayList(*rays_to_device(o, d))
+    ps, prop, _ = _state(native, oracle, t, act, mix=0.25, tau=1e30, seed=2)
+    kl = prop.sample(o, d)["kl"]
+    tau = float(np.quantile(kl[kl > 0], 0.7))
+    E, K = 12, 3
+    ps, prop, _ = _state(native, oracle, t, act, mix=0.25, tau=tau, seed=2, jitter_mode=jitter_mode, jitter_seed=77)
+    pp = ps.pp
+    pp.alloc_extra = E
+    ps.pp = pp
+    sp = _sp(native, K, jitter_mode, 77)
+    iv, fb = native.CreateProposalIntervals(tree, ps, rays, sp)
+    tr = oracle.traverse(t["node_min"], t["node_max"], oracle.active_ids(act), o, d, K, jitter_mode=jitter_mode,
+                         seed=77)
+    off = tr["ray_interval_offset"]
+    cnt, first, fb_ref, ns, _ = prop.allocate(o, d, E, off, tr["iv_tmin"], tr["iv_tmax"], tr["iv_count"])
+    print(f"jitter {jitter_mode}: {iv.num_intervals} intervals, {int(fb.sum())} fallback rays, "
+          f"{iv.num_samples / o.shape[0]:.1f} samples per ray (stratified K={K}: {tr['iv_count'].sum() / o.shape[0]:.1f})",
+          flush=True)
+    assert 0 < fb_ref.sum() < fb_ref.size
+    assert np.array_equal(fb.cpu().numpy(), fb_ref)
+    assert np.array_equal(iv.ray_offset.cpu().numpy(), off) and np.array_equal(iv.node.cpu().numpy(), tr["iv_node"])
+    assert np.array_equal(iv.count.cpu().numpy(), cnt) and np.array_equal(iv.first.cpu().numpy(), first)
+    assert np.array_equal(iv.ray_num_samples.cpu().numpy(), ns) and iv.num_samples == int(cnt.sum())
+    assert cnt.min() >= 1
+    # the extras telescope to E on rays whose intervals tile their root chord
+    # (culled holes leave the proposal's mass there unallocated)
+    _, ch = prop.features(o, d)
+    lo_, hi_ = tr["iv_tmin"], tr["iv_tmax"]
+    extras, tiled = [], []
+    for r in np.nonzero((fb_ref == 0) & (off[1:] > off[:-1]))[0]:
+        a, b = off[r], off[r + 1]
+        L = float(ch[r, 1] - ch[r, 0])
+        tiled.append(np.array_equal(lo_[a + 1:b], hi_[a:b - 1]) and abs(lo_[a] - ch[r, 0]) <= 1e-6 * L
+                     and abs(hi_[b - 1] - ch[r, 1]) <= 1e-6 * L)
+        extras.append(int(cnt[a:b].sum()) - (b - a))
+    extras, tiled = np.array(extras), np.array(tiled)
+    print(f"gated rays: {extras.size}, tiling their chord: {int(tiled.sum())}", flush=True)
+    assert tiled.sum() > 0.5 * tiled.size
+    assert np.all(extras <= E + 1) and np.all(np.abs(extras[tiled] - E) <= 1) and np.mean(extras[tiled] == E) > 0.95
+    C_, G = 16, 7
+    na = int(act.sum())
+    grid = phantom.init_grid(na, G, C_, std=0.3, seed=4)
+    mlp = phantom.init_mlp(C_, 64, seed=4)
+    fp = native.FieldParams()
+    fp.C, fp.G = C_, G
+    P = native.ProjectIntervals(tree, rays, iv, grid.cuda(), mlp.cuda(), sp, fp).cpu().numpy().astype(np.float64)
+    fm = oracle.FusedModel(t["node_min"], t["node_max"], act, grid.numpy(), mlp.numpy(), C_, K=K,
+                           jitter_mode=jitter_mode, seed=77)
+    fm.set_intervals(off, tr["iv_node"], tr["iv_tmin"], tr["iv_tmax"], cnt)
+    P64 = fm.project(o, d, precision=1)
+    scale = np.maximum(np.abs(P64), 1e-3 * np.abs(P64).max())
+    err = np.max(np.abs(P - P64) / scale)
+    print(f"projection over the allocated list: max rel err {err:.2e}", flush=True)
+    assert err <= 1e-5
+
+
 def test_binned_forward(oracle, native, cuda):
     from adaptivetensordf_amd import phantom
 
