@@ -475,14 +475,17 @@ __global__ __launch_bounds__(kThreads) void alloc_kernel(PropArgs a, const float
     const bool fall = ok && kl > a.kl_threshold;
     a.fallback[r] = fall ? 1 : 0;
     if (a.kl) a.kl[r] = kl;
-    const bool learn = ok && !fall;
+    // a gated ray allocates along its mixed pdf, a fallback ray along the
+    // uniform one (stratified sampling at the same budget of extras)
+    const bool learn = ok;
     if (learn) {
         float c = 0.f;
 #pragma unroll
         for (int b = 0; b < NB; ++b) {
+            const float p = fall ? 1.f / (float)NB : pm[b];
             buf[b * 64 + lane] = c;
-            buf[(NB + b) * 64 + lane] = pm[b];
-            c = c + pm[b];
+            buf[(NB + b) * 64 + lane] = p;
+            c = c + p;
         }
     }
     const float len = (t1 - t0) / (float)NB;

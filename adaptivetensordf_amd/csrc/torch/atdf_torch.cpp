@@ -1224,8 +1224,8 @@ Intervals CreateProposalIntervals(HyperTreeBase& tree, const ProposalState& ps, 
     TORCH_CHECK(fallback_sp.jitter_mode != 1, "CreateProposalIntervals: jitter_mode 1 needs materialised samples");
     const int64_t R = rays.size();
     if (ps.pp.alloc_extra > 0) {
-        // allocation mode: the stratified list of every ray, then the learned
-        // counts of the rays the gate keeps
+        // allocation mode: the stratified list of every ray, then the counts
+        // along the proposal's CDF (the uniform one for the fallback rays)
         Intervals iv = tree.CreateIntervalsForRays(rays, fallback_sp);
         auto dev = tree.device();
         auto i32 = torch::TensorOptions().dtype(torch::kInt32).device(dev);

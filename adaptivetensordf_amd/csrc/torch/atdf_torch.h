@@ -462,8 +462,8 @@ struct ProposalParams {
     // 0: the proposal places 32 one-sample cells per ray.  E > 0: it allocates
     // over the stratified traversal's node intervals instead -- every node
     // interval keeps one sample and E more per ray follow the proposal's CDF
-    // (atdf_proposal_allocate); the traversal's K gives the fallback rays'
-    // counts.
+    // (atdf_proposal_allocate), fallback rays' E extras follow the uniform
+    // CDF; the traversal's K only sets the list of node intervals.
     int alloc_extra = 0;
 };
 struct ProposalState {

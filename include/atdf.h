@@ -735,10 +735,13 @@ atdf_status atdf_proposal_sample(const void* packed_tree, int64_t num_nodes, con
  * samples, E = extra_samples, F the CDF of the mixed pdf over the ray's root
  * chord (linear within each of the 32 bins, clamped to [0, 1]),
  * round(x) = floor(x + 1/2): every node interval keeps at least one sample
- * and the extras telescope to E over intervals that tile the chord.  count
- * and first are rewritten in place; fallback rays and rays without a root
- * chord keep their stratified counts.  ray_fallback [num_rays] (0/1),
- * ray_num_samples (nullable) = the ray's new sample count, ray_kl nullable. */
+ * and the extras telescope to E over intervals that tile the chord.  A
+ * fallback ray (KL above the threshold) is allocated the same way along the
+ * uniform pdf (F = the chord fraction, up to FP32 rounding): stratified
+ * sampling at the same budget.  count and first are rewritten in place; rays
+ * without a root chord keep their stratified counts.  ray_fallback
+ * [num_rays] (0/1), ray_num_samples (nullable) = the ray's new sample count,
+ * ray_kl nullable. */
 atdf_status atdf_proposal_allocate(const void* packed_tree, int64_t num_nodes, const float* ray_origin,
                                    const float* ray_direction, int64_t num_rays, const atdf_proposal* proposal,
                                    int32_t extra_samples, const int64_t* ray_interval_offset,

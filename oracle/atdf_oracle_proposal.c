@@ -283,7 +283,9 @@ static int ray_pdf(const atdfo_proposal* p, const float* ro, const float* rd, fl
  * round(E F(tmin_i)) samples, F(t) = cum[b] + pm[b] ((t - edge_b) / len) at
  * b = clamp(floor((t - t0) / len), 0, 31), clamped to [0, 1], cum the running
  * sum of pm, round(x) = floor(x + 1/2); first = the running sum of counts.
- * Other rays keep their counts (first rewritten the same way). */
+ * A fallback ray (KL above the threshold) does the same with pm = 1/32 in
+ * every bin (the uniform pdf); rays without a root chord keep their counts
+ * (first rewritten the same way). */
 void atdfo_prop_allocate(const atdfo_proposal* p, const float* o, const float* d, int64_t R, int32_t extra,
                          const int64_t* iv_off, const float* tmin, const float* tmax, int32_t* count, int32_t* first,
                          int32_t* fallback, int32_t* ns, float* kl_out)
@@ -294,7 +296,11 @@ void atdfo_prop_allocate(const atdfo_proposal* p, const float* o, const float* d
         int fall = ok && kl > p->kl_threshold;
         fallback[r] = fall;
         if (kl_out) kl_out[r] = kl;
-        int learn = ok && !fall;
+        /* fallback rays allocate along the uniform pdf: stratified sampling
+         * at the same budget of extras */
+        int learn = ok;
+        if (fall)
+            for (int b = 0; b < NB; ++b) pm[b] = 1.0f / (float)NB;
         float cum[NB];
         float c = 0;
         for (int b = 0; b < NB; ++b) {

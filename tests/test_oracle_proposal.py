@@ -360,7 +360,8 @@ def test_allocation_one_bin_pdf_and_gate(oracle):
     """A pdf concentrated in one bin (mix 0, a dominant output bias): all E
     extras go to the intervals overlapping that bin, in proportion to the
     overlap (FP64 closed form away from ties); with a finite KL threshold the
-    gated rays keep the stratified counts and first is rewritten alike."""
+    gated-out (fallback) rays are allocated along the uniform pdf: bit-exact
+    the counts of mix = 1, whose mixed pdf is 1/32 in every bin."""
     from adaptivetensordf_amd import phantom
 
     E, k = 24, 13
@@ -386,13 +387,22 @@ def test_allocation_one_bin_pdf_and_gate(oracle):
         assert np.array_equal(cnt[a:b][~ties], want[~ties].astype(np.int32)), r
         outside = (tr["iv_tmax"][a:b] <= lo) | (tr["iv_tmin"][a:b] >= hi)
         assert np.all(cnt[a:b][outside & ~ties] == 1)
-    # the gate: a finite threshold sends some rays to the fallback, which keep their counts
+    # the gate: a finite threshold sends some rays to the fallback, which are
+    # allocated along the uniform pdf, as every ray is at mix = 1
     prop2, o, d, tr, off, ch = _alloc_case(oracle, mix=0.25, tau=0.3)
     cnt, first, fb, ns, kl = prop2.allocate(o, d, E, off, tr["iv_tmin"], tr["iv_tmax"], tr["iv_count"])
     assert 0 < fb.sum() < fb.size
     _, feat_kl = prop2.sample(o, d)["fallback"], prop2.sample(o, d)["kl"]
     assert np.array_equal(fb, (feat_kl > np.float32(0.3)).astype(np.int32))
-    for r in np.nonzero(fb)[0]:
+    prop_u, *_ = _alloc_case(oracle, mix=1.0, tau=float("inf"))
+    cnt_u, first_u, fb_u, ns_u, _ = prop_u.allocate(o, d, E, off, tr["iv_tmin"], tr["iv_tmax"], tr["iv_count"])
+    assert not fb_u.any()
+    differ = 0
+    for r in range(o.shape[0]):
         a, b = off[r], off[r + 1]
-        assert np.array_equal(cnt[a:b], tr["iv_count"][a:b])
-        assert np.array_equal(first[a:b], tr["iv_first"][a:b])
+        same = np.array_equal(cnt[a:b], cnt_u[a:b])
+        if fb[r]:
+            assert same and np.array_equal(first[a:b], first_u[a:b]) and ns[r] == ns_u[r], r
+        else:
+            differ += not same
+    assert differ > 0  # the gated rays follow their own pdf
