@@ -133,6 +133,7 @@ _SIGNATURES = {
     "atdf_project_fwd": (_I32, [_P, _I64, _P, _P, _I64, C.POINTER(Sampling), _P, C.POINTER(Intervals),
                                 C.POINTER(Field), _P, _P]),
     "atdf_project_bwd_workspace_bytes": (_SZ, [_I64, _I64, C.POINTER(Field)]),
+    "atdf_project_bwd_staging_bytes": (_SZ, [_I64, _I64, C.POINTER(Field)]),
     "atdf_project_bwd": (_I32, [_P, _I64, _P, _P, _I64, C.POINTER(Sampling), _P, C.POINTER(Intervals),
                                 C.POINTER(Field), _P, _P, _P, _P, _SZ, _P]),
     "atdf_project_bwd_noalloc_max_bytes": (_SZ, []),

@@ -150,11 +150,15 @@ struct Tri {
     float w[8];
 };
 
-__device__ __forceinline__ void tri_setup(float u, float v, float s, int Z, int Y, int X, Tri& t)
+// Lattice coordinates (ix, iy, iz) of a local point in [-1, 1]^3.
+__device__ __forceinline__ float tri_index(float u, int n)
 {
-    float ix = ((u + 1) / 2) * (float)(X - 1);
-    float iy = ((v + 1) / 2) * (float)(Y - 1);
-    float iz = ((s + 1) / 2) * (float)(Z - 1);
+    return ((u + 1) / 2) * (float)(n - 1);
+}
+
+// Corners and weights from lattice coordinates (the second half of tri_setup).
+__device__ __forceinline__ void tri_from_index(float ix, float iy, float iz, int Z, int Y, int X, Tri& t)
+{
     int ix0 = (int)floorf(ix), iy0 = (int)floorf(iy), iz0 = (int)floorf(iz);
     int ix1 = ix0 + 1, iy1 = iy0 + 1, iz1 = iz0 + 1;
     float fx1 = (float)ix1 - ix, fx0 = ix - (float)ix0;
@@ -174,6 +178,11 @@ __device__ __forceinline__ void tri_setup(float u, float v, float s, int Z, int 
     t.y1 = clampi(iy1, Y);
     t.z0 = clampi(iz0, Z);
     t.z1 = clampi(iz1, Z);
+}
+
+__device__ __forceinline__ void tri_setup(float u, float v, float s, int Z, int Y, int X, Tri& t)
+{
+    tri_from_index(tri_index(u, X), tri_index(v, Y), tri_index(s, Z), Z, Y, X, t);
 }
 
 // Bilinear read of IndirectGridSample2D, grid [N, C, H, W]

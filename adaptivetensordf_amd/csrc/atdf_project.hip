@@ -128,6 +128,16 @@ struct MlpSmem {
     static constexpr int kTotal = kB3 + 4;
 };
 
+// MlpSmem plus transposed copies of W2 (W2T[i][o] = W2[o][i]) and W1 (W1T[c][h]
+// = W1[h][c], 16 rows, rows c >= C zero) for float4 operand reads in the
+// FP32 decoder backward.
+template <int C>
+struct MlpSmemT {
+    static constexpr int kW2T = MlpSmem<C>::kTotal;
+    static constexpr int kW1T = kW2T + kH * kW2S;
+    static constexpr int kTotal = kW1T + 16 * kW2S;
+};
+
 template <int C>
 __device__ __forceinline__ void load_mlp_to_lds(const float* __restrict__ mlp, float* s)
 {
@@ -148,12 +158,27 @@ __device__ __forceinline__ void load_mlp_to_lds(const float* __restrict__ mlp, f
     if (threadIdx.x == 0) s[L::kB3] = b3[0];
 }
 
+template <int C>
+__device__ __forceinline__ void load_mlp_to_lds_t(const float* __restrict__ mlp, float* s)
+{
+    using T = MlpSmemT<C>;
+    load_mlp_to_lds<C>(mlp, s);
+    const float* W1 = mlp;
+    const float* W2 = W1 + kH * C + kH;
+    for (int i = threadIdx.x; i < kH * kH; i += blockDim.x) s[T::kW2T + (i % kH) * kW2S + i / kH] = W2[i];
+    for (int i = threadIdx.x; i < 16 * kH; i += blockDim.x) {
+        const int c = i / kH, h = i % kH;
+        s[T::kW1T + c * kW2S + h] = c < C ? W1[h * C + c] : 0.f;
+    }
+}
+
 // Per-lane sample state of one tile.
 struct SampleCtx {
     bool valid;
     float t;       // ray parameter
     float w;       // integration weight
     Tri tri;
+    float li[3];   // lattice coordinates of the sample in its node grid
     int64_t gbase; // element offset of the node grid, -1 if none
     // tensorial field only: local coordinates, global x and z, plane slot
     float loc[3];
@@ -732,9 +757,23 @@ constexpr int mlp_numel()
 
 // Decoder-parameter gradient accumulators of one wave.
 struct DecGrads {
+    static constexpr bool kRS = false;
     v4f W2[4][4];
     v4f W1[4];
-    float W3[4][4], B2[4][4], B1[4][4];
+    float W3[4][4], B2[4][4], B1[4][4];  // per-lane sums over its samples
+    float B3;
+};
+
+// The same with the 64-entry gradients of b1, b2 and W3 summed over the
+// tile's 16 sample lanes at the end of every tile (row_reduce_scatter16), so
+// that each lane keeps one entry of each (row 16 (j / 4) + 4 g + j % 4):
+// 45 fewer registers for the price of about 135 lane-permute VALU
+// operations per tile.
+struct DecGradsRS {
+    static constexpr bool kRS = true;
+    v4f W2[4][4];
+    v4f W1[4];
+    float W3, B2, B1;
     float B3;
 };
 
@@ -752,31 +791,81 @@ __device__ __forceinline__ void zero_grads(DecGrads& q)
     q.B3 = 0.f;
 }
 
+__device__ __forceinline__ void zero_grads(DecGradsRS& q)
+{
+#pragma unroll
+    for (int p = 0; p < 4; ++p) {
+        q.W1[p] = (v4f){0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int r = 0; r < 4; ++r) q.W2[p][r] = (v4f){0.f, 0.f, 0.f, 0.f};
+    }
+    q.W3 = q.B2 = q.B1 = q.B3 = 0.f;
+}
+
+template <int kCtrl>
+__device__ __forceinline__ float dpp_f(float x)
+{
+    return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, x), kCtrl, 0xF, 0xF,
+                                                                 false));
+}
+
+// Sum of v[i] over the 16 lanes of each 16-lane row; lane j of the row
+// returns the sum for i = j.  A butterfly in which each step halves the
+// entries a lane keeps: the partner differs in bit 3 (row_ror:8), bit 2
+// (row_half_mirror: both lanes still hold the same entries), bit 1 and bit 0
+// (quad permutations); the lane whose bit is set keeps the upper half.
+__device__ __forceinline__ float row_reduce_scatter16(const float* v, int j)
+{
+    float u[8], w[4], x[2];
+    const bool b3 = j & 8, b2 = j & 4, b1 = j & 2, b0 = j & 1;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) u[k] = (b3 ? v[k + 8] : v[k]) + dpp_f<0x128>(b3 ? v[k] : v[k + 8]);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) w[k] = (b2 ? u[k + 4] : u[k]) + dpp_f<0x141>(b2 ? u[k] : u[k + 4]);
+#pragma unroll
+    for (int k = 0; k < 2; ++k) x[k] = (b1 ? w[k + 2] : w[k]) + dpp_f<0x4E>(b1 ? w[k] : w[k + 2]);
+    return (b0 ? x[1] : x[0]) + dpp_f<0xB1>(b0 ? x[0] : x[1]);
+}
+
 // Backward of the decoder for one tile given dL/dy (dy) of this lane's
 // sample; accumulates the parameter gradients and returns dX (channels
 // 4g .. 4g+3 of sample j, result layout).  bufA/bufB (f32) or bufAb/bufBb
 // (bf16) are this wave's transpose buffers.
-template <int C, bool kDecBF16>
+// kT (FP32 decoder): the weights carry the MlpSmemT transposed copies.
+template <int C, bool kDecBF16, class Q, bool kT = false>
 __device__ __forceinline__ v4f tile_backward(const float* s, const __bf16* sb, float* bufA, float* bufB, __bf16* bufAb,
                                              __bf16* bufBb, const float* x, const v4f* h1, const v4f* h2, float dy,
-                                             int g, int j, DecGrads& q)
+                                             int g, int j, Q& q)
 {
+    using LT = MlpSmemT<C>;
     using L = MlpSmem<C>;
     using LB = BSmem<C>;
     constexpr int CPL = C / 4;
     const float* sW3 = kDecBF16 ? s + LB::kW3 : s + L::kW3;
     if (g == 0) q.B3 += dy;
     v4f dh2[4];
+    float tw3[16];
 #pragma unroll
     for (int rb2 = 0; rb2 < 4; ++rb2)
 #pragma unroll
         for (int r4 = 0; r4 < 4; ++r4) {
             const int row = 16 * rb2 + 4 * g + r4;
-            q.W3[rb2][r4] += dy * h2[rb2][r4];
             float v = (h2[rb2][r4] > 0.f) ? dy * sW3[row] : 0.f;
             dh2[rb2][r4] = v;
-            q.B2[rb2][r4] += v;
+            if constexpr (Q::kRS) {
+                tw3[4 * rb2 + r4] = dy * h2[rb2][r4];
+            } else {
+                q.W3[rb2][r4] += dy * h2[rb2][r4];
+                q.B2[rb2][r4] += v;
+            }
         }
+    if constexpr (Q::kRS) {
+        q.W3 += row_reduce_scatter16(tw3, j);
+        float tb2[16];
+#pragma unroll
+        for (int i = 0; i < 16; ++i) tb2[i] = dh2[i >> 2][i & 3];
+        q.B2 += row_reduce_scatter16(tb2, j);
+    }
     v4f dh1[4];
     v4f dx = {0.f, 0.f, 0.f, 0.f};
     if constexpr (!kDecBF16) {
@@ -809,25 +898,44 @@ __device__ __forceinline__ v4f tile_backward(const float* s, const __bf16* sb, f
         for (int rb = 0; rb < 4; ++rb) {
             v4f acc = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-            for (int rb2 = 0; rb2 < 4; ++rb2)
+            for (int rb2 = 0; rb2 < 4; ++rb2) {
+                if constexpr (kT) {
+                    const float4 w = *reinterpret_cast<const float4*>(s + LT::kW2T + (16 * rb + j) * kW2S + 16 * rb2 +
+                                                                      4 * g);
+                    acc = mfma4(w.x, dh2[rb2][0], acc);
+                    acc = mfma4(w.y, dh2[rb2][1], acc);
+                    acc = mfma4(w.z, dh2[rb2][2], acc);
+                    acc = mfma4(w.w, dh2[rb2][3], acc);
+                } else {
 #pragma unroll
-                for (int kk = 0; kk < 4; ++kk)
-                    acc = mfma4(s[L::kW2 + (16 * rb2 + 4 * g + kk) * kW2S + 16 * rb + j], dh2[rb2][kk], acc);
+                    for (int kk = 0; kk < 4; ++kk)
+                        acc = mfma4(s[L::kW2 + (16 * rb2 + 4 * g + kk) * kW2S + 16 * rb + j], dh2[rb2][kk], acc);
+                }
+            }
 #pragma unroll
             for (int r4 = 0; r4 < 4; ++r4) {
                 float v = (h1[rb][r4] > 0.f) ? acc[r4] : 0.f;
                 dh1[rb][r4] = v;
-                q.B1[rb][r4] += v;
+                if constexpr (!Q::kRS) q.B1[rb][r4] += v;
             }
         }
         // dX = W1^T . dH1 (rows = channels, padded to 16)
 #pragma unroll
-        for (int rb = 0; rb < 4; ++rb)
+        for (int rb = 0; rb < 4; ++rb) {
+            if constexpr (kT) {
+                const float4 w = *reinterpret_cast<const float4*>(s + LT::kW1T + j * kW2S + 16 * rb + 4 * g);
+                dx = mfma4(w.x, dh1[rb][0], dx);
+                dx = mfma4(w.y, dh1[rb][1], dx);
+                dx = mfma4(w.z, dh1[rb][2], dx);
+                dx = mfma4(w.w, dh1[rb][3], dx);
+            } else {
 #pragma unroll
-            for (int kk = 0; kk < 4; ++kk) {
-                float wv = (j < C) ? s[L::kW1 + (16 * rb + 4 * g + kk) * L::W1S + j] : 0.f;
-                dx = mfma4(wv, dh1[rb][kk], dx);
+                for (int kk = 0; kk < 4; ++kk) {
+                    float wv = (j < C) ? s[L::kW1 + (16 * rb + 4 * g + kk) * L::W1S + j] : 0.f;
+                    dx = mfma4(wv, dh1[rb][kk], dx);
+                }
             }
+        }
         // dW1 += dH1 . X^T
 #pragma unroll
         for (int rb = 0; rb < 4; ++rb)
@@ -882,7 +990,7 @@ __device__ __forceinline__ v4f tile_backward(const float* s, const __bf16* sb, f
             for (int r4 = 0; r4 < 4; ++r4) {
                 float v = (h1[rb][r4] > 0.f) ? acc[r4] : 0.f;
                 dh1[rb][r4] = v;
-                q.B1[rb][r4] += v;
+                if constexpr (!Q::kRS) q.B1[rb][r4] += v;
             }
         }
         // dX = W1^T . dH1 (rows = channels, padded to 16)
@@ -907,6 +1015,12 @@ __device__ __forceinline__ v4f tile_backward(const float* s, const __bf16* sb, f
             q.W1[rb] = mfma_bf16_k16(av, bv, q.W1[rb]);
         }
         wave_lds_sync();
+    }
+    if constexpr (Q::kRS) {
+        float tb1[16];
+#pragma unroll
+        for (int i = 0; i < 16; ++i) tb1[i] = dh1[i >> 2][i & 3];
+        q.B1 += row_reduce_scatter16(tb1, j);
     }
     return dx;
 }
@@ -945,6 +1059,31 @@ __device__ __forceinline__ void store_partials(DecGrads& q, float* out, int g, i
                 out[oW3 + row] = v3;
             }
         }
+    float b3 = q.B3;
+#pragma unroll
+    for (int off = 1; off < 16; off <<= 1) b3 += __shfl_xor(b3, off, 64);
+    if (lane == 0) out[oB3] = b3;
+}
+
+template <int C>
+__device__ __forceinline__ void store_partials(DecGradsRS& q, float* out, int g, int j, int lane)
+{
+    constexpr int oW1 = 0, oB1 = kH * C, oW2 = oB1 + kH, oB2 = oW2 + kH * kH, oW3 = oB2 + kH, oB3 = oW3 + kH;
+#pragma unroll
+    for (int rb = 0; rb < 4; ++rb)
+#pragma unroll
+        for (int r4 = 0; r4 < 4; ++r4)
+            if (j < C) out[oW1 + (16 * rb + 4 * g + r4) * C + j] = q.W1[rb][r4];
+#pragma unroll
+    for (int rb2 = 0; rb2 < 4; ++rb2)
+#pragma unroll
+        for (int rb = 0; rb < 4; ++rb)
+#pragma unroll
+            for (int r4 = 0; r4 < 4; ++r4) out[oW2 + (16 * rb2 + 4 * g + r4) * kH + 16 * rb + j] = q.W2[rb2][rb][r4];
+    const int row = 16 * (j >> 2) + 4 * g + (j & 3);
+    out[oB1 + row] = q.B1;
+    out[oB2 + row] = q.B2;
+    out[oW3 + row] = q.W3;
     float b3 = q.B3;
 #pragma unroll
     for (int off = 1; off < 16; off <<= 1) b3 += __shfl_xor(b3, off, 64);
@@ -1030,6 +1169,16 @@ struct NodeBwdArgs {
     float* line_part;
     int lds_line_off;
     int lds_grad_floats;      // LDS node-gradient region, zeroed at start
+    // two-pass adjoint (the staging region of the workspace)
+    const int* mode;          // 1: the two-pass kernels run, 0: the fused kernel; null: fused
+    const int64_t* soff;      // [H + 1] first staged sample of each sorted interval
+    const int4* parts;        // decoder-pass work items (slot, begin, end, chunk)
+    const int64_t* part_off;  // [max_chunks + 1]; part_off[max_chunks] = item count
+    int64_t max_chunks;
+    float4* stage;            // per sample: lattice coordinates, then dX (C/4 float4)
+    float* ws2;               // decoder-pass gradient partial rows
+    int* counter2;            // decoder-pass item counter
+    int* counter3;            // scatter-pass chunk counter
 };
 
 template <int C, bool kDecBF16>
@@ -1052,11 +1201,13 @@ struct RecordSrc {
     const float* O;
     const float* D;
     float4* rec;
+    int32_t* scnt;  // sample count per sorted position (two-pass adjoint), or null
 };
 
 __device__ __forceinline__ void write_record(const RecordSrc& rs, int64_t i, int64_t k)
 {
     const int64_t r = rs.ray[i];
+    if (rs.scnt) rs.scnt[k] = rs.count[i];
     rs.rec[k * 3] = make_float4(rs.tmin[i], rs.tmax[i], __int_as_float(rs.count[i]), __int_as_float(rs.first[i]));
     rs.rec[k * 3 + 1] = make_float4(rs.O[r * 3], rs.O[r * 3 + 1], rs.O[r * 3 + 2], rs.D[r * 3]);
     rs.rec[k * 3 + 2] = make_float4(rs.D[r * 3 + 1], rs.D[r * 3 + 2], __int_as_float((int)i), __int_as_float((int)r));
@@ -1092,7 +1243,9 @@ __device__ __forceinline__ void make_sample_node(const ProjArgs& a, const PNode&
             if (k == 2) sc.gz = pos;
         }
     }
-    tri_setup(loc[0], loc[1], loc[2], a.G, a.G, a.G, sc.tri);
+#pragma unroll
+    for (int k = 0; k < 3; ++k) sc.li[k] = tri_index(loc[k], a.G);
+    tri_from_index(sc.li[0], sc.li[1], sc.li[2], a.G, a.G, a.G, sc.tri);
     sc.gbase = gbase;
 }
 
@@ -1256,6 +1409,45 @@ __device__ __forceinline__ void corners_combine(const SampleCtx& sc, const Corne
     }
 }
 
+// Adds w_corner * (d0, d1, d2, d3) into the LDS node gradient at channels
+// 4g .. 4g+3 of the sample's 8 corners.  `plain` lanes (a sample off the
+// grid border whose floor cell no other lane of its 16-lane row shares)
+// read-add-write float4; `atomic` lanes (a shared cell, or a clamped corner
+// at the border) add afterwards with LDS atomics.  The caller keeps other
+// waves out (workgroup lock).
+template <int C>
+__device__ __forceinline__ void scatter_corners(float* lgrad, const Tri& tri, int G, int g, float d0, float d1, float d2,
+                                                float d3, bool plain, bool atomic)
+{
+    if (plain) {
+        // corner by corner: a corner of one lane can be another corner of a
+        // lane whose floor cell is adjacent, and LDS runs a wave's operations
+        // in order, so each read sees the previous corners' writes
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            float4* gp = reinterpret_cast<float4*>(lgrad + corner_cell(tri, k, G) * C + 4 * g);
+            const float wk = tri.w[k];
+            float4 v = *gp;
+            v.x = fmaf(wk, d0, v.x);
+            v.y = fmaf(wk, d1, v.y);
+            v.z = fmaf(wk, d2, v.z);
+            v.w = fmaf(wk, d3, v.w);
+            *gp = v;
+        }
+    }
+    wave_lds_sync();
+    if (atomic) {
+        const float dv[4] = {d0, d1, d2, d3};
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            float* gp = lgrad + corner_cell(tri, k, G) * C + 4 * g;
+            const float wk = tri.w[k];
+#pragma unroll
+            for (int r4 = 0; r4 < 4; ++r4) atomicAdd(gp + r4, wk * dv[r4]);
+        }
+    }
+}
+
 // Workgroup lock in LDS guarding the node gradient.  Lane 0 polls with a
 // compare-and-swap until it gets the lock.  The wait ends: a holder runs a
 // fixed sequence of LDS read-add-writes (it waits on no other wave and on no
@@ -1283,12 +1475,118 @@ __device__ __forceinline__ void lds_unlock(int* lock, int lane)
     if (lane == 0) __hip_atomic_store(lock, 0, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
 
+// End of a chunk: adds the LDS node gradient (n4 float4) to HBM once and
+// clears it for the next chunk (float4 accesses, kU in flight per thread).
+// adam_here: the chunk holds all of the node's samples, so its gradient is
+// complete and Adam is applied to the node's grid (parameters at gbase)
+// instead of going through the gradient buffer.  split: other chunks of the
+// node add to the same gradient, through device-scope atomics.
+template <int kSwzC>
+__device__ __forceinline__ float4 take_lds4(float* lgrad, int i)
+{
+    const float4 z4 = make_float4(0.f, 0.f, 0.f, 0.f);
+    if constexpr (kSwzC == 0) {
+        float4* lg4 = reinterpret_cast<float4*>(lgrad);
+        const float4 v = lg4[i];
+        lg4[i] = z4;
+        return v;
+    } else {
+        // float4 i = channels 4q .. 4q+3 of cell i / (C/4), stored as two
+        // swizzled channel pairs (the scatter pass's layout)
+        constexpr int Q = kSwzC / 4;
+        const int cell = i / Q, q = i % Q;
+        const int swz = 2 * (cell & (kSwzC / 2 - 1));
+        float2* a = reinterpret_cast<float2*>(lgrad + cell * kSwzC + ((4 * q) ^ swz));
+        float2* b = reinterpret_cast<float2*>(lgrad + cell * kSwzC + ((4 * q + 2) ^ swz));
+        const float2 va = *a, vb = *b;
+        *a = make_float2(0.f, 0.f);
+        *b = make_float2(0.f, 0.f);
+        return make_float4(va.x, va.y, vb.x, vb.y);
+    }
+}
+
+template <int kU, int kSwzC = 0>
+__device__ __forceinline__ void flush_node_grad(const GridAdam& ad, bool adam_here, bool split, float* lgrad,
+                                                float* gdst, int64_t gbase, int n4)
+{
+    float4* gg4 = reinterpret_cast<float4*>(gdst);
+    if (adam_here) {
+        float4* p4 = reinterpret_cast<float4*>(ad.p + gbase);
+        float4* m4 = reinterpret_cast<float4*>(ad.m + gbase);
+        float4* v4 = reinterpret_cast<float4*>(ad.v + gbase);
+        constexpr int kA = 4;
+        for (int i0 = threadIdx.x; i0 < n4; i0 += kA * blockDim.x) {
+            float4 gv[kA], pv[kA], mv[kA], vv[kA];
+#pragma unroll
+            for (int u = 0; u < kA; ++u) {
+                const int i = i0 + u * blockDim.x;
+                if (i < n4) {
+                    gv[u] = take_lds4<kSwzC>(lgrad, i);
+                    pv[u] = p4[i];
+                    mv[u] = m4[i];
+                    vv[u] = v4[i];
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < kA; ++u) {
+                const int i = i0 + u * blockDim.x;
+                if (i >= n4) continue;
+                float* gp = &gv[u].x;
+                float* pp = &pv[u].x;
+                float* mp = &mv[u].x;
+                float* vp = &vv[u].x;
+#pragma unroll
+                for (int k = 0; k < 4; ++k)
+                    pp[k] = adam_update(pp[k], gp[k], mp[k], vp[k], ad.lr, ad.b1, ad.b2, ad.eps, ad.bc1, ad.bc2s);
+                p4[i] = pv[u];
+                m4[i] = mv[u];
+                v4[i] = vv[u];
+                if (ad.pb) {
+                    ushort4 h;
+                    h.x = f32_to_bf16_rne(pp[0]);
+                    h.y = f32_to_bf16_rne(pp[1]);
+                    h.z = f32_to_bf16_rne(pp[2]);
+                    h.w = f32_to_bf16_rne(pp[3]);
+                    reinterpret_cast<ushort4*>(ad.pb + gbase)[i] = h;
+                }
+            }
+        }
+        return;
+    }
+    for (int i0 = threadIdx.x; i0 < n4; i0 += kU * blockDim.x) {
+        float4 v[kU], o[kU];
+#pragma unroll
+        for (int u = 0; u < kU; ++u) {
+            const int i = i0 + u * blockDim.x;
+            if (i < n4) {
+                v[u] = take_lds4<kSwzC>(lgrad, i);
+                if (!split) o[u] = gg4[i];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < kU; ++u) {
+            const int i = i0 + u * blockDim.x;
+            if (i >= n4) continue;
+            if (split) {
+                float* dst = reinterpret_cast<float*>(gg4 + i);
+                if (v[u].x != 0.f) unsafeAtomicAdd(dst + 0, v[u].x);
+                if (v[u].y != 0.f) unsafeAtomicAdd(dst + 1, v[u].y);
+                if (v[u].z != 0.f) unsafeAtomicAdd(dst + 2, v[u].z);
+                if (v[u].w != 0.f) unsafeAtomicAdd(dst + 3, v[u].w);
+            } else {
+                gg4[i] = make_float4(o[u].x + v[u].x, o[u].y + v[u].y, o[u].z + v[u].z, o[u].w + v[u].w);
+            }
+        }
+    }
+}
+
 template <int C, bool kBF16, bool kDecBF16, bool kVM = false>
 __global__ __launch_bounds__(64 * bwd_waves<kDecBF16>()) void project_bwd_node_kernel(NodeBwdArgs b)
 {
     extern __shared__ __attribute__((aligned(16))) unsigned char dsm[];
     __shared__ int s_chunk;
     __shared__ int s_lock;
+    if (b.mode && *b.mode) return;  // the two-pass kernels run instead
     constexpr int kNW = bwd_waves<kDecBF16>();
     const ProjArgs& a = b.a;
     const float4* __restrict__ rec = b.rec;
@@ -1452,29 +1750,7 @@ __global__ __launch_bounds__(64 * bwd_waves<kDecBF16>()) void project_bwd_node_k
             const bool atomic = live && (border || dup) && 4 * g < C;
             if (__any(plain || atomic)) {
                 lds_lock(&s_lock, lane);
-                if (plain) {
-#pragma unroll
-                    for (int k = 0; k < 8; ++k) {
-                        float4* gp = reinterpret_cast<float4*>(lgrad + corner_cell(sc.tri, k, G) * C + 4 * g);
-                        const float wk = sc.tri.w[k];
-                        float4 v = *gp;
-                        v.x = fmaf(wk, dx[0], v.x);
-                        v.y = fmaf(wk, dx[1], v.y);
-                        v.z = fmaf(wk, dx[2], v.z);
-                        v.w = fmaf(wk, dx[3], v.w);
-                        *gp = v;
-                    }
-                }
-                wave_lds_sync();
-                if (atomic) {
-#pragma unroll
-                    for (int k = 0; k < 8; ++k) {
-                        float* gp = lgrad + corner_cell(sc.tri, k, G) * C + 4 * g;
-                        const float wk = sc.tri.w[k];
-#pragma unroll
-                        for (int r4 = 0; r4 < 4; ++r4) atomicAdd(gp + r4, wk * dx[r4]);
-                    }
-                }
+                scatter_corners<C>(lgrad, sc.tri, G, g, dx[0], dx[1], dx[2], dx[3], plain, atomic);
                 lds_unlock(&s_lock, lane);
             }
             sc0 = sc1;
@@ -1482,91 +1758,12 @@ __global__ __launch_bounds__(64 * bwd_waves<kDecBF16>()) void project_bwd_node_k
             dp0 = dp1;
         }
         __syncthreads();
-        // flush the node gradient once, clear LDS for the next chunk (float4
-        // accesses, kU in flight per thread; G^3 C is a multiple of 4 for C in
-        // {8, 16})
-        constexpr int kU = kDecBF16 ? 16 : 4;
-        // tensorial chunk: the node's three plane gradients (3 Gp^2 C, a
-        // multiple of 4 floats for C in {8, 16})
-        float4* gg4 = reinterpret_cast<float4*>(vmc ? b.grad_planes + (int64_t)pslot * 3 * Gp * Gp * C
-                                                    : b.grad_grid + gbase);
-        float4* lg4 = reinterpret_cast<float4*>(lgrad);
+        // flush the node gradient once, clear LDS for the next chunk (G^3 C,
+        // or a tensorial chunk's three planes 3 Gp^2 C: a multiple of 4
+        // floats for C in {8, 16})
+        float* gdst = vmc ? b.grad_planes + (int64_t)pslot * 3 * Gp * Gp * C : b.grad_grid + gbase;
         const int n4 = (vmc ? 3 * Gp * Gp * C : gsz) >> 2;
-        const float4 z4 = make_float4(0.f, 0.f, 0.f, 0.f);
-        if (b.ad.enabled && !ch.w && !vmc) {
-            // the chunk holds all of this node's samples: its gradient is
-            // complete, so apply Adam here instead of going through grad_grid
-            const GridAdam& ad = b.ad;
-            float4* p4 = reinterpret_cast<float4*>(ad.p + gbase);
-            float4* m4 = reinterpret_cast<float4*>(ad.m + gbase);
-            float4* v4 = reinterpret_cast<float4*>(ad.v + gbase);
-            constexpr int kA = 4;
-            for (int i0 = threadIdx.x; i0 < n4; i0 += kA * blockDim.x) {
-                float4 gv[kA], pv[kA], mv[kA], vv[kA];
-#pragma unroll
-                for (int u = 0; u < kA; ++u) {
-                    const int i = i0 + u * blockDim.x;
-                    if (i < n4) {
-                        gv[u] = lg4[i];
-                        lg4[i] = z4;
-                        pv[u] = p4[i];
-                        mv[u] = m4[i];
-                        vv[u] = v4[i];
-                    }
-                }
-#pragma unroll
-                for (int u = 0; u < kA; ++u) {
-                    const int i = i0 + u * blockDim.x;
-                    if (i >= n4) continue;
-                    float* gp = &gv[u].x;
-                    float* pp = &pv[u].x;
-                    float* mp = &mv[u].x;
-                    float* vp = &vv[u].x;
-#pragma unroll
-                    for (int k = 0; k < 4; ++k)
-                        pp[k] = adam_update(pp[k], gp[k], mp[k], vp[k], ad.lr, ad.b1, ad.b2, ad.eps, ad.bc1, ad.bc2s);
-                    p4[i] = pv[u];
-                    m4[i] = mv[u];
-                    v4[i] = vv[u];
-                    if (ad.pb) {
-                        ushort4 h;
-                        h.x = f32_to_bf16_rne(pp[0]);
-                        h.y = f32_to_bf16_rne(pp[1]);
-                        h.z = f32_to_bf16_rne(pp[2]);
-                        h.w = f32_to_bf16_rne(pp[3]);
-                        reinterpret_cast<ushort4*>(ad.pb + gbase)[i] = h;
-                    }
-                }
-            }
-            __syncthreads();
-            continue;
-        }
-        for (int i0 = threadIdx.x; i0 < n4; i0 += kU * blockDim.x) {
-            float4 v[kU], o[kU];
-#pragma unroll
-            for (int u = 0; u < kU; ++u) {
-                const int i = i0 + u * blockDim.x;
-                if (i < n4) {
-                    v[u] = lg4[i];
-                    lg4[i] = z4;
-                    if (!ch.w) o[u] = gg4[i];
-                }
-            }
-#pragma unroll
-            for (int u = 0; u < kU; ++u) {
-                const int i = i0 + u * blockDim.x;
-                if (i >= n4) continue;
-                if (ch.w) {
-                    float* dst = reinterpret_cast<float*>(gg4 + i);
-                    if (v[u].x != 0.f) unsafeAtomicAdd(dst + 0, v[u].x);
-                    if (v[u].y != 0.f) unsafeAtomicAdd(dst + 1, v[u].y);
-                    if (v[u].z != 0.f) unsafeAtomicAdd(dst + 2, v[u].z);
-                    if (v[u].w != 0.f) unsafeAtomicAdd(dst + 3, v[u].w);
-                } else {
-                    gg4[i] = make_float4(o[u].x + v[u].x, o[u].y + v[u].y, o[u].z + v[u].z, o[u].w + v[u].w);
-                }
-            }
-        }
+        flush_node_grad<kDecBF16 ? 16 : 4>(b.ad, b.ad.enabled && !ch.w && !vmc, ch.w != 0, lgrad, gdst, gbase, n4);
         __syncthreads();
     }
     if constexpr (kVM) {
@@ -1576,6 +1773,229 @@ __global__ __launch_bounds__(64 * bwd_waves<kDecBF16>()) void project_bwd_node_k
         for (int i = threadIdx.x; i < 3 * Gl * C; i += blockDim.x) row[i] = lline[i];
     }
     store_partials<C>(q, b.ws + ((int64_t)blockIdx.x * kNW + wl) * NP, g, j, lane);
+}
+
+// ---------------------------------------------------------------------------
+// Two-pass adjoint.  The fused kernel above keeps a node's whole gradient
+// (G^3 C f32, 85 KB at G = 11, C = 16) in LDS next to the decoder, so a CU
+// runs one workgroup and each SIMD one wave, and the decoder's dependent
+// LDS and MFMA steps run without another wave to overlap them.  With a
+// staging buffer in the workspace (atdf_project_bwd_staging_bytes) the same
+// work runs as two kernels over the same counting sort and chunk table:
+//
+// (1) project_bwd_dec_kernel: the decoder adjoint.  No node gradient in LDS
+//     (weights and per-wave transpose buffers only) and at most 256
+//     registers per wave, so a CU runs eight waves, two per SIMD.  Waves
+//     take work items of about kPartSamples samples (a run of one chunk's
+//     sorted intervals) from a global counter and write, per sample, its
+//     lattice coordinates and dX = dL/dx (the dL/dy-weighted feature
+//     gradient, f32) at the sample's position in the sorted order.
+//     Decoder-gradient partials per wave, reduced in a fixed order
+//     afterwards.
+// (2) project_bwd_scatter_kernel: per chunk, C/2 waves (two channels each)
+//     add w_corner * dX of the staged samples into the LDS node gradient,
+//     then flush it (and apply Adam) as the fused kernel does.
+//
+// Staged bytes per sample: 16 (lattice coordinates) + 4 C (dX).
+// ---------------------------------------------------------------------------
+#ifndef ATDF_PART_SAMPLES
+#define ATDF_PART_SAMPLES 256
+#endif
+constexpr int64_t kPartSamples = ATDF_PART_SAMPLES;
+constexpr int kMinPartIv = 16;    // intervals per decoder-pass work item, at least
+constexpr int kDecWaves = 8;  // one workgroup per CU, two waves per SIMD
+
+// Work items of a chunk of n_iv intervals holding `samples` samples.
+__host__ __device__ inline int parts_of(int n_iv, int64_t samples)
+{
+    if (n_iv <= 0) return 0;
+    int64_t p = (samples + kPartSamples - 1) / kPartSamples;
+    const int64_t pmax = (n_iv + kMinPartIv - 1) / kMinPartIv;
+    p = p < pmax ? p : pmax;
+    p = p > 1 ? p : 1;
+    const int per = (int)((n_iv + p - 1) / p);
+    return (n_iv + per - 1) / per;
+}
+
+template <int C, bool kBF16, bool kDecBF16>
+__global__ __launch_bounds__(64 * kDecWaves, 2) void project_bwd_dec_kernel(NodeBwdArgs b)
+{
+    extern __shared__ __attribute__((aligned(16))) unsigned char dsm[];
+    if (*b.mode == 0) return;
+    const ProjArgs& a = b.a;
+    const float4* __restrict__ rec = b.rec;
+    constexpr int CPL = C / 4;
+    constexpr int NP = mlp_numel<C>();
+    constexpr int RF = 1 + C / 4;
+    const int G = a.G;
+    const int gsz = G * G * G * C;
+    float* s = reinterpret_cast<float*>(dsm) + b.lds_f32_off;
+    __bf16* sb = reinterpret_cast<__bf16*>(dsm + b.lds_bf16_off);
+    const int lane = threadIdx.x & 63;
+    const int g = lane >> 4, j = lane & 15;
+    const int wl = threadIdx.x >> 6;
+    float* bufA = reinterpret_cast<float*>(dsm + b.lds_tb_off) + wl * 2 * kH * kTS;
+    float* bufB = bufA + kH * kTS;
+    __bf16* bufAb = reinterpret_cast<__bf16*>(dsm + b.lds_tb_off) + wl * 2 * kH * kTB;
+    __bf16* bufBb = bufAb + kH * kTB;
+    if constexpr (kDecBF16)
+        load_mlp_bf16_to_lds<C>(a.mlp, sb, s);
+    else
+        load_mlp_to_lds_t<C>(a.mlp, s);
+    __syncthreads();
+
+    DecGradsRS q;
+    zero_grads(q);
+    const int64_t n_items = b.part_off[b.max_chunks];
+    const int grp = lane & 48;
+    while (true) {
+        int item = 0;
+        if (lane == 0) item = atomicAdd(b.counter2, 1);
+        item = __builtin_amdgcn_readfirstlane(__shfl(item, 0, 64));
+        if (item >= n_items) break;
+        const int4 it = b.parts[item];
+        const PNode nd = a.nodes[a.iv_node[b.sorted[it.y]]];
+        const int64_t gbase = (int64_t)it.x * gsz;
+        const int ib = it.y, ie = it.z;
+        const int64_t s_base = b.soff[ib];
+        const int tot = (int)(b.soff[ie] - s_base);
+        const int ntiles = (tot + kTile - 1) / kTile;
+
+        // tile by tile (the other workgroup on the CU hides the load latency)
+        TileCursor tc;
+        tc.cur = ib;
+        tc.cum = 0;
+        tc.cnt = ib + j < ie ? __float_as_int(rec[3 * (int64_t)(ib + j)].z) : 0;
+        for (int ti = 0; ti < ntiles; ++ti) {
+            TileMap m;
+            m.valid = false;
+            map_tile(rec, ie, ti, tot, j, grp, tc, m);
+            SampleCtx sc;
+            Corners<C, kBF16> cr;
+            const float dp = sample_stage<C, kBF16>(a, b.dLdP, nd, gbase, m, g, sc, cr);
+            float x[CPL];
+            corners_combine<C, kBF16>(sc, cr, x);
+            v4f h1[4], h2[4];
+            float y;
+            if constexpr (kDecBF16)
+                y = decoder_fwd_bf16<C>(sb, s, x, g, j, h1, h2);
+            else
+                y = decoder_fwd<C>(s, x, g, j, h1, h2);
+            const float dy = sc.valid ? dp * sc.w * softplus_grad_f(y, a.beta) : 0.f;
+            const v4f dx =
+                tile_backward<C, kDecBF16, DecGradsRS, !kDecBF16>(s, sb, bufA, bufB, bufAb, bufBb, x, h1, h2, dy, g, j, q);
+            if (sc.valid) {
+                float4* out = b.stage + (s_base + (int64_t)ti * kTile + j) * RF;
+                if (g == 0) out[0] = make_float4(sc.li[0], sc.li[1], sc.li[2], 0.f);
+                if (4 * g < C) out[1 + g] = make_float4(dx[0], dx[1], dx[2], dx[3]);
+            }
+        }
+    }
+    store_partials<C>(q, b.ws2 + ((int64_t)blockIdx.x * kDecWaves + wl) * NP, g, j, lane);
+}
+
+// Scatter pass.  Wave w of the workgroup owns channels 2w and 2w + 1 of the
+// node gradient (C / 2 waves), so waves never touch the same LDS words and
+// need no lock.  Lane l walks its own stretch of the chunk's staged samples
+// (the chunk cut into 64 consecutive runs), so the 64 samples of one step
+// come from distant parts of the chunk; the next step's record is loaded
+// while the current one scatters.  Per step the lanes claim their floor
+// cell in a per-wave owner table (one byte per cell); a lane that finds
+// another lane's claim (two samples in one cell) adds with LDS atomics after
+// the others.  The 8 corners go one after another: a corner of one lane can
+// be another corner of a lane in an adjacent cell, and LDS runs a wave's
+// operations in issue order, so each read sees the previous corners' writes.
+//
+// LDS layout: cell-major with C floats per cell, the channel pairs of cell
+// x at (c ^ swz(x)), swz(x) = 2 (x mod C/2): one wave's 64 lanes (same
+// channel pair, different cells) then spread over all 32 banks instead of
+// the 2 that channels-last would give.
+__device__ __forceinline__ int pair_swz(int cell, int C) { return 2 * (cell & (C / 2 - 1)); }
+
+template <int C>
+__global__ __launch_bounds__(32 * C) void project_bwd_scatter_kernel(NodeBwdArgs b)
+{
+    extern __shared__ __attribute__((aligned(16))) unsigned char dsm[];
+    __shared__ int s_chunk;
+    if (*b.mode == 0) return;
+    constexpr int RF = 1 + C / 4;
+    const int G = b.a.G;
+    const int ncell = G * G * G;
+    const int gsz = ncell * C;
+    float* lgrad = reinterpret_cast<float*>(dsm);
+    const int lane = threadIdx.x & 63, wl = threadIdx.x >> 6;
+    const int c0 = 2 * wl;  // this wave's channel pair
+    unsigned char* owner = dsm + (size_t)gsz * 4 + (size_t)wl * ncell;
+    for (int i = threadIdx.x; i < gsz; i += blockDim.x) lgrad[i] = 0.f;
+    const int64_t total_chunks = b.chunk_off[b.num_slots];
+    const float* stagef = reinterpret_cast<const float*>(b.stage);
+    __syncthreads();
+    while (true) {
+        if (threadIdx.x == 0) s_chunk = atomicAdd(b.counter3, 1);
+        __syncthreads();
+        const int64_t c = s_chunk;
+        __syncthreads();
+        if (c >= total_chunks) break;
+        const int4 ch = b.chunks[c];
+        const int64_t gbase = (int64_t)ch.x * gsz;
+        const int64_t s0 = b.soff[ch.y], s1 = b.soff[ch.z];
+        const int64_t run = (s1 - s0 + 63) / 64;
+        const int64_t rb = s0 + lane * run;
+        const int64_t re = rb + run < s1 ? rb + run : s1;
+        float2 nd = make_float2(0.f, 0.f);
+        float npx = 0.f, npy = 0.f, npz = 0.f;
+        if (rb < re) {
+            const float* rec = stagef + rb * RF * 4;
+            nd = *reinterpret_cast<const float2*>(rec + 4 + c0);
+            npx = rec[0];
+            npy = rec[1];
+            npz = rec[2];
+        }
+        for (int64_t st = 0; st < run; ++st) {
+            const int64_t si = rb + st;
+            const float2 dv = nd;
+            const float px = npx, py = npy, pz = npz;
+            const bool live = si < re && (dv.x != 0.f || dv.y != 0.f);
+            if (si + 1 < re) {
+                const float* rec = stagef + (si + 1) * RF * 4;
+                nd = *reinterpret_cast<const float2*>(rec + 4 + c0);
+                npx = rec[0];
+                npy = rec[1];
+                npz = rec[2];
+            }
+            Tri t;
+            tri_from_index(px, py, pz, G, G, G, t);
+            const int cell = (t.z0 * G + t.y0) * G + t.x0;
+            if (live) owner[cell] = (unsigned char)lane;
+            wave_lds_sync();
+            const bool mine = live && owner[cell] == lane;
+#pragma unroll
+            for (int k = 0; k < 8; ++k) {
+                if (mine) {
+                    const int cc = (int)corner_cell(t, k, G);
+                    float2* gp = reinterpret_cast<float2*>(lgrad + cc * C + (c0 ^ pair_swz(cc, C)));
+                    float2 v = *gp;
+                    v.x = fmaf(t.w[k], dv.x, v.x);
+                    v.y = fmaf(t.w[k], dv.y, v.y);
+                    *gp = v;
+                }
+            }
+            wave_lds_sync();
+            if (live && !mine) {
+#pragma unroll
+                for (int k = 0; k < 8; ++k) {
+                    const int cc = (int)corner_cell(t, k, G);
+                    float* gp = lgrad + cc * C + (c0 ^ pair_swz(cc, C));
+                    atomicAdd(gp, t.w[k] * dv.x);
+                    atomicAdd(gp + 1, t.w[k] * dv.y);
+                }
+            }
+            wave_lds_sync();
+        }
+        __syncthreads();
+        flush_node_grad<4, C>(b.ad, b.ad.enabled && !ch.w, ch.w != 0, lgrad, b.grad_grid + gbase, gbase, gsz >> 2);
+        __syncthreads();
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -1893,6 +2313,50 @@ __global__ __launch_bounds__(256) void chunk_desc_kernel(const int64_t* __restri
     }
 }
 
+// Decoder-pass work items: each chunk's intervals cut into parts_of() runs.
+__global__ __launch_bounds__(256) void part_count_kernel(const int4* __restrict__ desc,
+                                                         const int64_t* __restrict__ chunk_off, int64_t NA,
+                                                         int64_t max_chunks, const int64_t* __restrict__ soff,
+                                                         int32_t* __restrict__ part_cnt)
+{
+    int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (c >= max_chunks) return;
+    if (c >= chunk_off[NA]) {
+        part_cnt[c] = 0;
+        return;
+    }
+    const int4 ch = desc[c];
+    part_cnt[c] = parts_of(ch.z - ch.y, soff[ch.z] - soff[ch.y]);
+}
+
+__global__ __launch_bounds__(256) void part_desc_kernel(const int4* __restrict__ desc,
+                                                        const int64_t* __restrict__ chunk_off, int64_t NA,
+                                                        const int64_t* __restrict__ soff,
+                                                        const int64_t* __restrict__ part_off, int4* __restrict__ parts)
+{
+    int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (c >= chunk_off[NA]) return;
+    const int4 ch = desc[c];
+    const int n_iv = ch.z - ch.y;
+    const int np = parts_of(n_iv, soff[ch.z] - soff[ch.y]);
+    if (np == 0) return;
+    const int per = (n_iv + np - 1) / np;
+    const int64_t o = part_off[c];
+    for (int p = 0; p < np; ++p) {
+        const int bb = ch.y + p * per;
+        const int ee = bb + per < ch.z ? bb + per : ch.z;
+        parts[o + p] = make_int4(ch.x, bb, ee, (int)c);
+    }
+}
+
+// mode = 1 (two-pass kernels) when the sort prepared the staging tables and
+// every sample's staged record fits in the staging capacity, else 0.
+__global__ void stage_mode_kernel(const unsigned long long* __restrict__ stotal, const int* __restrict__ prepared,
+                                  unsigned long long capacity, int rec_bytes, int* __restrict__ mode)
+{
+    if (threadIdx.x == 0) *mode = (*prepared == 1 && *stotal * (unsigned long long)rec_bytes <= capacity) ? 1 : 0;
+}
+
 // ---------------------------------------------------------------------------
 // Per-node density statistics: one workgroup per active node, tiles of 16
 // points per wave, FP64 sum / sum of squares and FP32 max reduced in a fixed
@@ -2163,8 +2627,8 @@ __global__ __launch_bounds__(64 * kWaves, 4) void volume_kernel(VolumeArgs v)
 // the 16 group sums are added in group order.
 constexpr int kReduceParams = 16;
 
-__global__ __launch_bounds__(256) void reduce_partials_kernel(const float* __restrict__ ws, int64_t nwaves, int np,
-                                                              float* __restrict__ grad)
+__device__ __forceinline__ void reduce_partials(const float* __restrict__ ws, int64_t nwaves, int np,
+                                                float* __restrict__ grad)
 {
     __shared__ float part[16][kReduceParams + 1];
     const int pl = threadIdx.x % kReduceParams, rg = threadIdx.x / kReduceParams;
@@ -2180,6 +2644,24 @@ __global__ __launch_bounds__(256) void reduce_partials_kernel(const float* __res
         for (int k = 0; k < 16; ++k) t += part[k][pl];
         grad[i] += t;
     }
+}
+
+__global__ __launch_bounds__(256) void reduce_partials_kernel(const float* __restrict__ ws, int64_t nwaves, int np,
+                                                              float* __restrict__ grad)
+{
+    reduce_partials(ws, nwaves, np, grad);
+}
+
+// reduce_partials over the rows of whichever adjoint ran (*mode: 1 = rows b).
+__global__ __launch_bounds__(256) void reduce_partials_sel_kernel(const float* __restrict__ ws_a, int64_t rows_a,
+                                                                  const float* __restrict__ ws_b, int64_t rows_b,
+                                                                  const int* __restrict__ mode, int np,
+                                                                  float* __restrict__ grad)
+{
+    if (*mode)
+        reduce_partials(ws_b, rows_b, np, grad);
+    else
+        reduce_partials(ws_a, rows_a, np, grad);
 }
 
 // Adam for the grid slots the adjoint did not update itself: nodes split over
@@ -2412,6 +2894,32 @@ BwdWorkspace bwd_layout(int64_t H, int64_t NA, int np, int64_t line_floats = 0)
     return w;
 }
 
+int64_t max_chunks_of(int64_t H, int64_t NA) { return H / kMinChunk + NA + 1; }
+
+// The two-pass adjoint's region, after the base workspace: sorted sample
+// counts and offsets, the decoder-pass work items, scan scratch, the
+// decoder-pass partial rows, then the staged samples (S x (1 + C/4) float4).
+struct StageLayout {
+    size_t scnt, soff, part_cnt, part_off, parts, scan, ws2, stage;
+};
+
+StageLayout stage_layout(size_t base, int64_t H, int64_t NA, int np)
+{
+    StageLayout st{};
+    size_t o = align256(base);
+    auto take = [&](size_t bytes) { size_t at = o; o = align256(o + bytes); return at; };
+    const int64_t mc = max_chunks_of(H, NA);
+    st.scnt = take(sizeof(int32_t) * (size_t)(H > 0 ? H : 1));
+    st.soff = take(sizeof(int64_t) * (size_t)(H + 1));
+    st.part_cnt = take(sizeof(int32_t) * (size_t)(mc + 1));
+    st.part_off = take(sizeof(int64_t) * (size_t)(mc + 1));
+    st.parts = take(sizeof(int4) * (size_t)(H / kMinPartIv + mc + 1));
+    st.scan = take(atdf_scan_workspace_bytes(H > mc ? H : mc));
+    st.ws2 = take(sizeof(float) * (size_t)persistent_blocks() * kDecWaves * np);
+    st.stage = o;
+    return st;
+}
+
 // grad_floats: the node-gradient region at the start of the dynamic LDS
 template <int C, bool kDecBF16>
 void lds_layout(int grad_floats, int& f32_off, int& bf16_off, int& tb_off, int& total)
@@ -2473,6 +2981,41 @@ atdf_status launch_node_bwd(NodeBwdArgs args, hipStream_t s)
                        dim3(64 * bwd_waves<kDecBF16>()), total, s, args);
     return ATDF_OK;
 }
+
+// The two passes of the staged adjoint; both return at once unless *args.mode.
+template <int C, bool kBF16, bool kDecBF16>
+atdf_status launch_two_pass(NodeBwdArgs args, hipStream_t s)
+{
+    const int G = args.a.G;
+    const int64_t grad_floats = (int64_t)G * G * G * C;
+    const int max_lds = max_lds_per_block();
+    if (grad_floats * 4 > max_lds) {
+        set_error("atdf_project_bwd: node gradient of %lld floats does not fit in LDS", (long long)grad_floats);
+        return ATDF_ERR_UNSUPPORTED;
+    }
+    // decoder pass: weights (with the transposed copies on the FP32 path),
+    // then kDecWaves pairs of transpose buffers
+    const int wf = kDecBF16 ? BSmem<C>::kF32Total : MlpSmemT<C>::kTotal;
+    args.lds_f32_off = 0;
+    args.lds_bf16_off = (wf * 4 + 15) & ~15;
+    args.lds_tb_off = (args.lds_bf16_off + (kDecBF16 ? BSmem<C>::kBf16Total * 2 : 0) + 15) & ~15;
+    const int total = args.lds_tb_off + (kDecBF16 ? kDecWaves * 2 * kH * kTB * 2 : kDecWaves * 2 * kH * kTS * 4);
+    if (total > max_lds) {
+        set_error("atdf_project_bwd: the decoder pass needs %d bytes of LDS, the device allows %d", total, max_lds);
+        return ATDF_ERR_UNSUPPORTED;
+    }
+    hipLaunchKernelGGL((project_bwd_dec_kernel<C, kBF16, kDecBF16>), dim3(persistent_blocks()), dim3(64 * kDecWaves),
+                       total, s, args);
+    const size_t scatter_lds = (size_t)grad_floats * 4 + (size_t)(C / 2) * G * G * G;
+    if (scatter_lds > (size_t)max_lds) {
+        set_error("atdf_project_bwd: the scatter pass needs %zu bytes of LDS, the device allows %d", scatter_lds,
+                  max_lds);
+        return ATDF_ERR_UNSUPPORTED;
+    }
+    hipLaunchKernelGGL((project_bwd_scatter_kernel<C>), dim3(persistent_blocks()), dim3(32 * C), scatter_lds, s,
+                       args);
+    return ATDF_OK;
+}
 }  // namespace
 
 extern "C" {
@@ -2487,6 +3030,16 @@ size_t atdf_project_bwd_workspace_bytes(int64_t R, int64_t H, const atdf_field* 
     return bwd_layout(H, NA, np, lf).total;
 }
 
+size_t atdf_project_bwd_staging_bytes(int64_t H, int64_t S, const atdf_field* f)
+{
+    if (!f || H < 0 || S < 0 || f->num_grids <= 0 || f->vm || (f->C != 8 && f->C != 16)) return 0;
+    const int64_t NA = f->num_grids;
+    const int np = (int)((int64_t)f->H * f->C + f->H + (int64_t)f->H * f->H + f->H + f->H + 1);
+    const size_t base = bwd_layout(H, NA, np, 0).total;
+    const StageLayout st = stage_layout(base, H, NA, np);
+    return st.stage - base + (size_t)S * (1 + f->C / 4) * sizeof(float4);
+}
+
 }  // extern "C"
 
 // The counting sort of the intervals by grid slot, their records and the
@@ -2494,9 +3047,13 @@ size_t atdf_project_bwd_workspace_bytes(int64_t R, int64_t H, const atdf_field* 
 // and the adjoint).
 struct Prepared {
     BwdWorkspace L;
+    StageLayout SL;
     unsigned char* w;
+    size_t ws_bytes;
+    bool staging;  // the workspace holds the two-pass region (explicit grids only)
     int np;
     int64_t H, NA, line_floats;
+    int* counters() const { return reinterpret_cast<int*>(w + L.counter); }  // fused, dec, scatter, mode, prepared
     int32_t* chunk_cnt() const { return reinterpret_cast<int32_t*>(w + L.chunk_cnt); }
     int32_t* sorted() const { return reinterpret_cast<int32_t*>(w + L.sorted); }
     int64_t* chunk_off() const { return reinterpret_cast<int64_t*>(w + L.chunk_off); }
@@ -2517,6 +3074,9 @@ static atdf_status prepared_view(const atdf_intervals* iv, const atdf_field* f, 
     P.line_floats = f->vm ? 3 * (int64_t)f->vm->Gl * f->C : 0;
     P.L = bwd_layout(P.H, P.NA, P.np, P.line_floats);
     P.w = reinterpret_cast<unsigned char*>(workspace);
+    P.ws_bytes = workspace_bytes;
+    P.SL = stage_layout(P.L.total, P.H, P.NA, P.np);
+    P.staging = !f->vm && (f->C == 8 || f->C == 16) && f->num_grids > 0 && workspace_bytes > P.SL.stage;
     return ATDF_OK;
 }
 
@@ -2553,7 +3113,9 @@ static atdf_status prepare_impl(const Prepared& P, const float* O, const float* 
     atdf_status st = atdf_exclusive_scan_i32(cnt, NA, slot_off, scan_ws, scan_bytes, stream);
     if (st) return st;
     hipLaunchKernelGGL(fill_i32_kernel, dim3((unsigned)div_up(NA, 256)), dim3(256), 0, s, cnt, NA, 0);
-    const RecordSrc rs{iv->tmin, iv->tmax, iv->count, iv->first, iv->ray, O, D, P.rec()};
+    int32_t* scnt = P.staging ? reinterpret_cast<int32_t*>(w + P.SL.scnt) : nullptr;
+    if (scnt && H > 0) hipLaunchKernelGGL(fill_i32_kernel, dim3((unsigned)div_up(H, 256)), dim3(256), 0, s, scnt, H, 0);
+    const RecordSrc rs{iv->tmin, iv->tmax, iv->count, iv->first, iv->ray, O, D, P.rec(), scnt};
     if (H > 0) {
         if (lds_sort)
             hipLaunchKernelGGL(slot_scatter_lds_kernel, dim3(sort_blocks), dim3(256), hist_lds, s, iv->node, sm,
@@ -2573,6 +3135,24 @@ static atdf_status prepare_impl(const Prepared& P, const float* O, const float* 
     if (st) return st;
     hipLaunchKernelGGL(chunk_desc_kernel, dim3((unsigned)div_up(NA, 256)), dim3(256), 0, s, slot_off, chunk_off, NA,
                        H, stotal, desc);
+    int* prepared_flag = P.counters() + 4;
+    if (P.staging) {
+        const int64_t mc = max_chunks_of(H, NA);
+        void* scan2 = w + P.SL.scan;
+        const size_t scan2_bytes = atdf_scan_workspace_bytes(H > mc ? H : mc);
+        int64_t* soff = reinterpret_cast<int64_t*>(w + P.SL.soff);
+        int32_t* part_cnt = reinterpret_cast<int32_t*>(w + P.SL.part_cnt);
+        int64_t* part_off = reinterpret_cast<int64_t*>(w + P.SL.part_off);
+        st = atdf_exclusive_scan_i32(scnt, H, soff, scan2, scan2_bytes, stream);
+        if (st) return st;
+        hipLaunchKernelGGL(part_count_kernel, dim3((unsigned)div_up(mc, 256)), dim3(256), 0, s, desc, chunk_off, NA,
+                           mc, soff, part_cnt);
+        st = atdf_exclusive_scan_i32(part_cnt, mc, part_off, scan2, scan2_bytes, stream);
+        if (st) return st;
+        hipLaunchKernelGGL(part_desc_kernel, dim3((unsigned)div_up(mc, 256)), dim3(256), 0, s, desc, chunk_off, NA,
+                           soff, part_off, reinterpret_cast<int4*>(w + P.SL.parts));
+    }
+    hipLaunchKernelGGL(fill_i32_kernel, dim3(1), dim3(64), 0, s, prepared_flag, 1, P.staging ? 1 : 0);
     return check_launch("atdf_project_prepare");
 }
 
@@ -2585,9 +3165,9 @@ static atdf_status bwd_node_impl(const Prepared& P, const void* packed_tree, con
     const int np = P.np;
     const int64_t n_grid = f->num_grids * (int64_t)f->G * f->G * f->G * f->C;
     hipStream_t s = to_hip(stream);
-    int* counter = reinterpret_cast<int*>(P.w + P.L.counter);
+    int* counter = P.counters();
     float* partials = reinterpret_cast<float*>(P.w + P.L.partials);
-    hipLaunchKernelGGL(fill_i32_kernel, dim3(1), dim3(256), 0, s, counter, 1, 0);
+    hipLaunchKernelGGL(fill_i32_kernel, dim3(1), dim3(256), 0, s, counter, 3, 0);
     NodeBwdArgs args{};
     args.a = make_args(packed_tree, O, D, R, sp, iv_off, iv, f);
     args.dLdP = dLdP;
@@ -2622,6 +3202,34 @@ static atdf_status bwd_node_impl(const Prepared& P, const void* packed_tree, con
         args.grad_planes = f->vm->grad_planes;
         args.line_part = reinterpret_cast<float*>(P.w + P.L.line_part);
     }
+    int* mode = counter + 3;
+    const int rf_bytes = (1 + f->C / 4) * (int)sizeof(float4);
+    if (P.staging) {
+        const unsigned char* w = P.w;
+        const unsigned long long cap = (unsigned long long)(P.ws_bytes - P.SL.stage);
+        hipLaunchKernelGGL(stage_mode_kernel, dim3(1), dim3(64), 0, s,
+                           reinterpret_cast<const unsigned long long*>(w + P.L.stotal), counter + 4, cap, rf_bytes,
+                           mode);
+        args.mode = mode;
+        args.soff = reinterpret_cast<const int64_t*>(w + P.SL.soff);
+        args.parts = reinterpret_cast<const int4*>(w + P.SL.parts);
+        args.part_off = reinterpret_cast<const int64_t*>(w + P.SL.part_off);
+        args.max_chunks = max_chunks_of(P.H, NA);
+        args.stage = reinterpret_cast<float4*>(P.w + P.SL.stage);
+        args.ws2 = reinterpret_cast<float*>(P.w + P.SL.ws2);
+        args.counter2 = counter + 1;
+        args.counter3 = counter + 2;
+#define ATDF_TP(CC, BF, BD) st = launch_two_pass<CC, BF, BD>(args, s)
+        if (f->C == 16) {
+            if (bf) { if (bd) ATDF_TP(16, true, true); else ATDF_TP(16, true, false); }
+            else    { if (bd) ATDF_TP(16, false, true); else ATDF_TP(16, false, false); }
+        } else {
+            if (bf) { if (bd) ATDF_TP(8, true, true); else ATDF_TP(8, true, false); }
+            else    { if (bd) ATDF_TP(8, false, true); else ATDF_TP(8, false, false); }
+        }
+#undef ATDF_TP
+        if (st) return st;
+    }
 #define ATDF_BWD(CC, BF, BD) st = launch_node_bwd<CC, BF, BD, false>(args, s)
     if (f->vm) {
         if (f->C == 16) {
@@ -2647,8 +3255,13 @@ static atdf_status bwd_node_impl(const Prepared& P, const void* packed_tree, con
                            P.chunk_cnt(), bps, n4);
     }
     const int waves = bd ? bwd_waves<true>() : bwd_waves<false>();
-    hipLaunchKernelGGL(reduce_partials_kernel, dim3((unsigned)div_up(np, kReduceParams)), dim3(256), 0, s, partials,
-                       (int64_t)persistent_blocks() * waves, np, grad_mlp);
+    if (P.staging)
+        hipLaunchKernelGGL(reduce_partials_sel_kernel, dim3((unsigned)div_up(np, kReduceParams)), dim3(256), 0, s,
+                           partials, (int64_t)persistent_blocks() * waves, args.ws2,
+                           (int64_t)persistent_blocks() * kDecWaves, mode, np, grad_mlp);
+    else
+        hipLaunchKernelGGL(reduce_partials_kernel, dim3((unsigned)div_up(np, kReduceParams)), dim3(256), 0, s,
+                           partials, (int64_t)persistent_blocks() * waves, np, grad_mlp);
     if (f->vm) {
         const int lf = (int)P.line_floats;
         hipLaunchKernelGGL(reduce_partials_kernel, dim3((unsigned)div_up(lf, kReduceParams)), dim3(256), 0, s,

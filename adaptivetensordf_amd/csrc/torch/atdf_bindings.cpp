@@ -57,7 +57,8 @@ PYBIND11_MODULE(TORCH_EXTENSION_NAME, m)
         .def_readwrite("softplus_beta", &FieldParams::softplus_beta)
         .def_readwrite("rays_per_wave", &FieldParams::rays_per_wave)
         .def_readwrite("bf16_decoder", &FieldParams::bf16_decoder)
-        .def_readwrite("node_forward", &FieldParams::node_forward);
+        .def_readwrite("node_forward", &FieldParams::node_forward)
+        .def_readwrite("two_pass_adjoint", &FieldParams::two_pass_adjoint);
 
     py::class_<HyperTreeBase>(m, "HyperTreeBase")
         .def(py::init<int, int>(), py::arg("d"), py::arg("max_depth"))

@@ -810,6 +810,15 @@ std::pair<Tensor, Tensor> AccumulateSampleLossPerNode(const SampleList& samples,
 // ---------------------------------------------------------------------------
 // fused projection
 // ---------------------------------------------------------------------------
+// Adjoint workspace: atdf_project_bwd's, plus the staging region of its
+// two-pass form when enabled (the list's sample count sizes it).
+static int64_t adjoint_workspace_bytes(int64_t R, const Intervals& iv, const atdf_field& f, bool two_pass)
+{
+    size_t n = atdf_project_bwd_workspace_bytes(R, iv.num_intervals, &f);
+    if (two_pass) n += atdf_project_bwd_staging_bytes(iv.num_intervals, iv.num_samples, &f);
+    return (int64_t)n;
+}
+
 atdf_field make_field(const HyperTreeBase& tree, const Tensor& grid, const Tensor& mlp, const FieldParams& fp)
 {
     TORCH_CHECK(grid.is_cuda() && grid.is_contiguous(), "grid must be a contiguous GPU tensor");
@@ -867,6 +876,7 @@ struct ProjectRaysFn : public torch::autograd::Function<ProjectRaysFn> {
         ctx->saved_data["beta"] = (double)fp.softplus_beta;
         ctx->saved_data["rpw"] = (int64_t)fp.rays_per_wave;
         ctx->saved_data["bfdec"] = (bool)fp.bf16_decoder;
+        ctx->saved_data["twopass"] = (bool)fp.two_pass_adjoint;
         ctx->saved_data["H_iv"] = iv.num_intervals;
         return P;
     }
@@ -898,13 +908,14 @@ struct ProjectRaysFn : public torch::autograd::Function<ProjectRaysFn> {
         fp.softplus_beta = (float)ctx->saved_data["beta"].toDouble();
         fp.rays_per_wave = (int)ctx->saved_data["rpw"].toInt();
         fp.bf16_decoder = ctx->saved_data["bfdec"].toBool();
+        fp.two_pass_adjoint = ctx->saved_data["twopass"].toBool();
         atdf_field f = make_field(*tree, grid, mlp, fp);
         atdf_sampling s = sp.view();
         auto dP = grad_output[0].to(torch::kFloat32).contiguous();
         const int64_t R = origin.size(0);
         auto grad_grid = torch::zeros(grid.sizes(), grid.options().dtype(torch::kFloat32));
         auto grad_mlp = torch::zeros_like(mlp);
-        auto ws = torch::empty({(int64_t)atdf_project_bwd_workspace_bytes(R, iv.num_intervals, &f)},
+        auto ws = torch::empty({adjoint_workspace_bytes(R, iv, f, fp.two_pass_adjoint)},
                                torch::TensorOptions().dtype(torch::kUInt8).device(grid.device()));
         atdf_intervals v = iv.view();
         check_status(atdf_project_bwd(tree->packed(), tree->NumNodes(), origin.data_ptr<float>(),
@@ -1019,7 +1030,7 @@ Tensor TrainStepIntervals(HyperTreeBase& tree, TrainState& st, const RayList& ra
     atdf_sampling s = sp.view();
     atdf_intervals v = iv.view();
     auto P = torch::empty({R}, rays.origin.options());
-    auto ws = torch::empty({(int64_t)atdf_project_bwd_workspace_bytes(R, iv.num_intervals, &f)},
+    auto ws = torch::empty({adjoint_workspace_bytes(R, iv, f, fp.two_pass_adjoint)},
                            torch::TensorOptions().dtype(torch::kUInt8).device(rays.origin.device()));
     const int64_t* ivo = iv.ray_offset.data_ptr<int64_t>();
     const float *O = rays.origin.data_ptr<float>(), *D = rays.direction.data_ptr<float>();
@@ -1136,7 +1147,7 @@ std::pair<Tensor, Tensor> ProjectIntervalsBackward(HyperTreeBase& tree, const Ra
     atdf_intervals v = iv.view();
     auto grad_grid = torch::zeros(grid.sizes(), grid.options().dtype(torch::kFloat32));
     auto grad_mlp = torch::zeros_like(mlp);
-    auto ws = torch::empty({(int64_t)atdf_project_bwd_workspace_bytes(R, iv.num_intervals, &f)},
+    auto ws = torch::empty({adjoint_workspace_bytes(R, iv, f, fp.two_pass_adjoint)},
                            torch::TensorOptions().dtype(torch::kUInt8).device(grid.device()));
     check_status(atdf_project_bwd(tree.packed(), tree.NumNodes(), rays.origin.data_ptr<float>(),
                                   rays.direction.data_ptr<float>(), R, &s, iv.ray_offset.data_ptr<int64_t>(), &v, &f,

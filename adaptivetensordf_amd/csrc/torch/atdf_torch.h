@@ -369,6 +369,7 @@ struct FieldParams {
     int rays_per_wave = 0;
     bool bf16_decoder = false;  // BF16 MFMA inputs, FP32 accumulation
     bool node_forward = false;  // training step: node-major forward over the adjoint's sort
+    bool two_pass_adjoint = true;  // workspace with the adjoint's staging region (atdf_project_bwd_staging_bytes)
 };
 
 // C-ABI view of a field (checks shapes and dtypes).

@@ -442,8 +442,10 @@ def kernel_times(N, tree, rays, sp, fp, st, bf16, ps):
     dP = torch.randn(R, device=dev) * 1e-6
     gg = torch.zeros(st.grid.shape, dtype=torch.float32, device=dev)
     gm = torch.zeros_like(st.mlp)
-    ws = torch.empty(capi.lib().atdf_project_bwd_workspace_bytes(R, iv.num_intervals, Ct.byref(f)),
-                     dtype=torch.uint8, device=dev)
+    ws_bytes = capi.lib().atdf_project_bwd_workspace_bytes(R, iv.num_intervals, Ct.byref(f))
+    if fp.two_pass_adjoint:
+        ws_bytes += capi.lib().atdf_project_bwd_staging_bytes(iv.num_intervals, S_total, Ct.byref(f))
+    ws = torch.empty(ws_bytes, dtype=torch.uint8, device=dev)
     args = (capi.ptr(packed), tree.NumNodes(), capi.ptr(rays.origin), capi.ptr(rays.direction), R, Ct.byref(csp),
             capi.ptr(iv.ray_offset), Ct.byref(civ), Ct.byref(f))
     node_fwd = fp.node_forward

@@ -335,6 +335,21 @@ atdf_status atdf_project_fwd(const void* packed_tree, int64_t num_nodes, const f
  * intervals->total must hold the interval count. */
 size_t atdf_project_bwd_workspace_bytes(int64_t num_rays, int64_t num_intervals, const atdf_field* field);
 
+/* Extra workspace, beyond atdf_project_bwd_workspace_bytes, that switches the
+ * adjoint (atdf_project_bwd, _adam, _prepared, and the sort of
+ * atdf_project_prepare) to its two-pass form for a list of num_samples
+ * samples: a decoder pass at two workgroups per CU writes each sample's
+ * lattice coordinates and dL/dfeature (16 + 4 C bytes) in node-sorted order,
+ * then a scatter pass adds them into the LDS node gradient.  Same results to
+ * FP32 rounding (both forms sum the scatter in a run-dependent order).  A
+ * workspace of at least the two sizes added runs the two-pass form whenever
+ * the list's samples fit it; a smaller one runs the fused kernel.  0 for a
+ * tensorial field (fused kernel only) or C outside {8, 16}.  No reference
+ * counterpart (the reference scatters per sample with global atomics,
+ * modules/IndirectGridSample2D.cu:175-393).  atdf_project_prepare and
+ * atdf_project_bwd_prepared must be given the same workspace and size. */
+size_t atdf_project_bwd_staging_bytes(int64_t num_intervals, int64_t num_samples, const atdf_field* field);
+
 atdf_status atdf_project_bwd(const void* packed_tree, int64_t num_nodes, const float* ray_origin,
                              const float* ray_direction, int64_t num_rays, const atdf_sampling* sampling,
                              const int64_t* ray_interval_offset, const atdf_intervals* intervals,

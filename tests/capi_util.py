@@ -42,16 +42,22 @@ class Fused:
         capi.call("atdf_project_fwd", *self.args(), capi.ptr(P), capi.stream_ptr())
         return P
 
-    def workspace(self):
+    def workspace(self, staging=0):
+        """The adjoint's workspace; staging > 0 adds that many bytes for its
+        two-pass form (staging_bytes() covers the whole list)."""
         n = capi.lib().atdf_project_bwd_workspace_bytes(self.R, self.iv.num_intervals, C.byref(self.field))
-        return torch.empty(n, dtype=torch.uint8, device=self.o.device)
+        return torch.empty(n + staging, dtype=torch.uint8, device=self.o.device)
 
-    def backward(self, dP):
+    def staging_bytes(self):
+        return capi.lib().atdf_project_bwd_staging_bytes(self.iv.num_intervals, self.iv.num_samples,
+                                                         C.byref(self.field))
+
+    def backward(self, dP, staging=0):
         """(grad_grid f32 [slots, G, G, G, C], grad_mlp f32) for dL/dP."""
         gshape = tuple(self.grid.shape)
         gg = torch.zeros(gshape, dtype=torch.float32, device=self.o.device)
         gm = torch.zeros_like(self.mlp)
-        ws = self.workspace()
+        ws = self.workspace(staging)
         dP = dP.contiguous().float()
         capi.call("atdf_project_bwd", *self.args(), capi.ptr(dP), capi.ptr(gg), capi.ptr(gm), capi.ptr(ws),
                   ws.numel(), capi.stream_ptr())
