@@ -2018,15 +2018,76 @@ struct NodeFwdArgs {
     int64_t num_slots;
     const float4* rec;
     float* ivsum;
+    int* counter;  // chunk counter of the LDS-staged variant (zeroed before the launch)
 };
 
-template <int C, bool kBF16, bool kDecBF16>
-__global__ __launch_bounds__(64 * kWaves, 4) void project_fwd_node_kernel(NodeFwdArgs b)
+// Trilinear gather of CPL channels from a node grid staged in LDS
+// (channels-last, f32 or bf16): the reads and fused multiply-add order of
+// gather_channels.
+template <int C, bool kBF16>
+__device__ __forceinline__ void gather_channels_lds(const unsigned char* lgrid, const SampleCtx& sc, int G, int c0,
+                                                    float* x)
+{
+    constexpr int CPL = C / 4;
+#pragma unroll
+    for (int q = 0; q < CPL; ++q) x[q] = 0.f;
+    if (!sc.valid) return;
+    float v[8][CPL];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        const int off = (int)corner_cell(sc.tri, k, G) * C + c0;
+        if constexpr (!kBF16) {
+            const float* p = reinterpret_cast<const float*>(lgrid) + off;
+            if constexpr (CPL == 4) {
+                const float4 f = *reinterpret_cast<const float4*>(p);
+                v[k][0] = f.x;
+                v[k][1] = f.y;
+                v[k][2] = f.z;
+                v[k][3] = f.w;
+            } else {
+                const float2 f = *reinterpret_cast<const float2*>(p);
+                v[k][0] = f.x;
+                v[k][1] = f.y;
+            }
+        } else {
+            const uint16_t* p = reinterpret_cast<const uint16_t*>(lgrid) + off;
+            if constexpr (CPL == 4) {
+                const uint2 u = *reinterpret_cast<const uint2*>(p);
+                v[k][0] = __uint_as_float(u.x << 16);
+                v[k][1] = __uint_as_float(u.x & 0xffff0000u);
+                v[k][2] = __uint_as_float(u.y << 16);
+                v[k][3] = __uint_as_float(u.y & 0xffff0000u);
+            } else {
+                const uint32_t u = *reinterpret_cast<const uint32_t*>(p);
+                v[k][0] = __uint_as_float(u << 16);
+                v[k][1] = __uint_as_float(u & 0xffff0000u);
+            }
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < CPL; ++q) {
+        float acc = 0.f;
+#pragma unroll
+        for (int k = 0; k < 8; ++k) acc = fmaf(v[k][q], sc.tri.w[k], acc);
+        x[q] = acc;
+    }
+}
+
+constexpr int kFwdLdsWaves = 16;
+
+// kLDS: a workgroup of kNW waves takes chunks from a global counter, copies
+// the chunk's node grid (G^3 C f32 or bf16) into LDS once and gathers from
+// there; otherwise workgroups of kWaves waves stride over the chunk table
+// and gather from HBM (the grid stays in L2 while its chunk is integrated).
+template <int C, bool kBF16, bool kDecBF16, bool kLDS = false, int kNW = kWaves>
+__global__ __launch_bounds__(64 * kNW, kLDS ? 1 : 4) void project_fwd_node_kernel(NodeFwdArgs b)
 {
     using SM = DecoderSmem<C, kDecBF16>;
     constexpr int CPL = C / 4;
     __shared__ __attribute__((aligned(16))) float s[SM::kF32];
     __shared__ __attribute__((aligned(16))) __bf16 sb[SM::kBF16];
+    extern __shared__ __attribute__((aligned(16))) unsigned char lgrid[];
+    __shared__ int s_chunk;
     const ProjArgs& a = b.a;
     const float4* __restrict__ rec = b.rec;
     if constexpr (kDecBF16)
@@ -2040,12 +2101,27 @@ __global__ __launch_bounds__(64 * kWaves, 4) void project_fwd_node_kernel(NodeFw
     const int g = lane >> 4, j = lane & 15;
     const int wl = threadIdx.x >> 6;
     const int64_t total_chunks = b.chunk_off[b.num_slots];
-    for (int64_t c = blockIdx.x; c < total_chunks; c += gridDim.x) {
+    for (int64_t c = blockIdx.x;; c += gridDim.x) {
+        if constexpr (kLDS) {
+            if (threadIdx.x == 0) s_chunk = atomicAdd(b.counter, 1);
+            __syncthreads();  // also: every wave is done with the previous grid
+            c = s_chunk;
+        }
+        if (c >= total_chunks) break;
         const int4 ch = b.chunks[c];
         const int64_t gbase = (int64_t)ch.x * gsz;
+        if constexpr (kLDS) {
+            // the node grid, 16 bytes per thread and step
+            const int n16 = gsz * (kBF16 ? 2 : 4) / 16;
+            const uint4* src = reinterpret_cast<const uint4*>(reinterpret_cast<const unsigned char*>(a.grid) +
+                                                              gbase * (kBF16 ? 2 : 4));
+            uint4* dst = reinterpret_cast<uint4*>(lgrid);
+            for (int i = threadIdx.x; i < n16; i += blockDim.x) dst[i] = src[i];
+            __syncthreads();
+        }
         const PNode nd = a.nodes[a.iv_node[b.sorted[ch.y]]];
         const int n_iv = ch.z - ch.y;
-        const int per = (n_iv + kWaves - 1) / kWaves;
+        const int per = (n_iv + kNW - 1) / kNW;
         const int ib = ch.y + wl * per;
         const int ie = min(ib + per, ch.z);
         int tot = 0;
@@ -2114,7 +2190,10 @@ __global__ __launch_bounds__(64 * kWaves, 4) void project_fwd_node_kernel(NodeFw
                 cum = __shfl(c_end, grp | (posn - 1), 64);
             cur += posn;
             float x[CPL];
-            gather_channels<C, kBF16>(a, sc, CPL * g, x);
+            if constexpr (kLDS)
+                gather_channels_lds<C, kBF16>(lgrid, sc, G, CPL * g, x);
+            else
+                gather_channels<C, kBF16>(a, sc, CPL * g, x);
             v4f h1[4], h2[4];
             float y;
             int opq = 0;
@@ -3283,14 +3362,29 @@ static atdf_status fwd_node_impl(const Prepared& P, const void* packed_tree, con
     args.num_slots = P.NA;
     args.rec = P.rec();
     args.ivsum = reinterpret_cast<float*>(P.w + P.L.ivsum);
-    // chunks are similar in work; a fixed grid strides over them
+    args.counter = P.counters() + 5;
+    const bool bf = f->grid_dtype == ATDF_GRID_BF16;
+    const bool bd = f->decoder_dtype == ATDF_DECODER_BF16;
+    // the node grid staged in LDS when it fits beside the decoder weights
+    // (the weights take at most 24 KB of static LDS)
+    const size_t grid_bytes = (size_t)f->G * f->G * f->G * f->C * (bf ? 2 : 4);
+    const bool lds = grid_bytes + 24 * 1024 <= (size_t)max_lds_per_block() && grid_bytes % 16 == 0;
+    if (lds) hipLaunchKernelGGL(fill_i32_kernel, dim3(1), dim3(64), 0, s, args.counter, 1, 0);
+    // chunks are similar in work; a fixed grid strides over them (or takes
+    // them from the counter: up to four workgroups per CU, as LDS allows)
     const int64_t max_chunks = P.H / kMinChunk + P.NA + 1;
     const unsigned blocks = (unsigned)std::max<int64_t>(1, std::min<int64_t>(max_chunks, (int64_t)persistent_blocks() * 8));
     dim3 grid(blocks), block(64 * kWaves);
-    const bool bf = f->grid_dtype == ATDF_GRID_BF16;
-    const bool bd = f->decoder_dtype == ATDF_DECODER_BF16;
+    dim3 grid_l((unsigned)persistent_blocks() * 4), block_l(64 * kFwdLdsWaves);
     if (P.H > 0) {
-#define ATDF_FN(CC, BF, BD) hipLaunchKernelGGL((project_fwd_node_kernel<CC, BF, BD>), grid, block, 0, s, args)
+#define ATDF_FN(CC, BF, BD)                                                                                        \
+    do {                                                                                                           \
+        if (lds)                                                                                                   \
+            hipLaunchKernelGGL((project_fwd_node_kernel<CC, BF, BD, true, kFwdLdsWaves>), grid_l, block_l,         \
+                               grid_bytes, s, args);                                                               \
+        else                                                                                                       \
+            hipLaunchKernelGGL((project_fwd_node_kernel<CC, BF, BD>), grid, block, 0, s, args);                    \
+    } while (0)
         if (f->C == 16) {
             if (bf) { if (bd) ATDF_FN(16, true, true); else ATDF_FN(16, true, false); }
             else    { if (bd) ATDF_FN(16, false, true); else ATDF_FN(16, false, false); }
