@@ -1179,6 +1179,7 @@ struct NodeBwdArgs {
     float* ws2;               // decoder-pass gradient partial rows
     int* counter2;            // decoder-pass item counter
     int* counter3;            // scatter-pass chunk counter
+    int lds_pl_off;           // byte offset of the decoder pass's per-wave sample tables
 };
 
 template <int C, bool kDecBF16>
@@ -1333,6 +1334,87 @@ __device__ __forceinline__ void map_tile(const float4* __restrict__ rec, int ie,
     tc.cnt = nc < ie ? __float_as_int(rec[3 * (int64_t)nc].z) : 0;
 }
 
+// Number of the wave's 64 lanes whose nondecreasing `ends` are <= t (0..64):
+// a binary search with seven cross-lane reads.
+__device__ __forceinline__ int wave_search(int ends, int t)
+{
+    int pos = 0;
+#pragma unroll
+    for (int st = 32; st > 0; st >>= 1) {
+        const int e = __shfl(ends, pos + st - 1, 64);
+        if (e <= t) pos += st;
+    }
+    const int e63 = __shfl(ends, 63, 64);
+    return e63 <= t ? 64 : pos;
+}
+
+// Inclusive prefix sum over the wave's 64 lanes.
+__device__ __forceinline__ int wave_scan_inclusive(int v, int lane)
+{
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const int u = __shfl_up(v, off, 64);
+        if (lane >= off) v += u;
+    }
+    return v;
+}
+
+// map_tile for 64 samples at once (lane = sample base + lane): a window of 64
+// candidate intervals from the cursor (every interval holds at least one
+// sample, so 64 samples span at most 64 intervals), their counts scanned
+// across the wave; the cursor advances past the intervals the 64 samples
+// finish.  tc.cnt is not used.
+__device__ __forceinline__ void map_tile64(const float4* __restrict__ rec, int ie, int base, int tot, int lane,
+                                           TileCursor& tc, TileMap& m)
+{
+    const int t = base + lane;
+    const int cand = tc.cur + lane;
+    const int c_cnt = cand < ie ? __float_as_int(rec[3 * (int64_t)cand].z) : 0;
+    const bool ones = __all(cand >= ie || c_cnt == 1);
+    int c_end = c_cnt;
+    int pos, posn, s_cnt, s_end, last_end = 0;
+    if (ones) {
+        pos = lane;
+        posn = min(64, ie - tc.cur);
+        s_cnt = 1;
+        s_end = tc.cum + lane + 1;
+    } else {
+        c_end = wave_scan_inclusive(c_end, lane);
+        c_end = cand < ie ? c_end + tc.cum : INT_MAX;
+        pos = wave_search(c_end, t);
+        posn = wave_search(c_end, base + 64);
+        const int src = pos < 64 ? pos : 63;
+        s_cnt = __shfl(c_cnt, src, 64);
+        s_end = __shfl(c_end, src, 64);
+        last_end = __shfl(c_end, 63, 64);
+    }
+    m.valid = t < tot;
+    if (m.valid) {
+        int k = tc.cur + pos, n_s = s_cnt, start = s_end - s_cnt;
+        if (pos >= 64) {
+            // only with zero-sample intervals in the list: walk on
+            k = tc.cur + 64;
+            start = last_end;
+            n_s = __float_as_int(rec[3 * (int64_t)k].z);
+            while (start + n_s <= t) {
+                start += n_s;
+                ++k;
+                n_s = __float_as_int(rec[3 * (int64_t)k].z);
+            }
+        }
+        m.n_s = n_s;
+        m.jj = t - start;
+        m.A = rec[3 * (int64_t)k];
+        m.B = rec[3 * (int64_t)k + 1];
+        m.D = rec[3 * (int64_t)k + 2];
+    }
+    if (ones)
+        tc.cum += posn;
+    else if (posn > 0)
+        tc.cum = __shfl(c_end, posn - 1, 64);
+    tc.cur += posn;
+}
+
 // Raw corner records of one lane's CPL channels (f32, or bf16 pairs), as
 // loaded; decoded by corners_combine.
 template <int C, bool kBF16>
@@ -1382,6 +1464,34 @@ __device__ __forceinline__ float sample_stage(const ProjArgs& a, const float* __
         }
     }
     return dp;
+}
+
+// The gathers of sample_stage for a placed sample: its 8 corner records at
+// channels CPL g .. (C = 16: 4g.., C = 8: 2g..).
+template <int C, bool kBF16>
+__device__ __forceinline__ void gather_corners(const ProjArgs& a, const SampleCtx& sc, int g, Corners<C, kBF16>& cr)
+{
+    constexpr int CPL = C / 4;
+    constexpr int W = Corners<C, kBF16>::W;
+    const unsigned char* grid = reinterpret_cast<const unsigned char*>(a.grid);
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        const int64_t off = sc.gbase + corner_cell(sc.tri, k, a.G) * C + CPL * g;
+        const unsigned char* p = grid + off * (kBF16 ? 2 : 4);
+        if constexpr (W == 4) {
+            const uint4 v = *reinterpret_cast<const uint4*>(p);
+            cr.u[k][0] = v.x;
+            cr.u[k][1] = v.y;
+            cr.u[k][2] = v.z;
+            cr.u[k][3] = v.w;
+        } else if constexpr (W == 2) {
+            const uint2 v = *reinterpret_cast<const uint2*>(p);
+            cr.u[k][0] = v.x;
+            cr.u[k][1] = v.y;
+        } else {
+            cr.u[k][0] = *reinterpret_cast<const uint32_t*>(p);
+        }
+    }
 }
 
 // Trilinear combination of the loaded corners: canonical corner order per
@@ -1847,7 +1957,6 @@ __global__ __launch_bounds__(64 * kDecWaves, 2) void project_bwd_dec_kernel(Node
     DecGradsRS q;
     zero_grads(q);
     const int64_t n_items = b.part_off[b.max_chunks];
-    const int grp = lane & 48;
     while (true) {
         int item = 0;
         if (lane == 0) item = atomicAdd(b.counter2, 1);
@@ -1859,36 +1968,91 @@ __global__ __launch_bounds__(64 * kDecWaves, 2) void project_bwd_dec_kernel(Node
         const int ib = it.y, ie = it.z;
         const int64_t s_base = b.soff[ib];
         const int tot = (int)(b.soff[ie] - s_base);
-        const int ntiles = (tot + kTile - 1) / kTile;
 
-        // tile by tile (the other workgroup on the CU hides the load latency)
+        // 64 samples per step.  (A) lane l maps and places sample l of the
+        // step once (the interval search, jitter hashes, divisions and
+        // trilinear set-up are done by one lane per sample instead of the four
+        // lane rows of a tile), writes its lattice coordinates to the stage
+        // and its packed corners, weights and dL/dP * w to the wave's sample
+        // table in LDS; (B) the step's four tiles of 16 read the table, gather,
+        // run the decoder and its adjoint and stage dX.
+        float4* pl = reinterpret_cast<float4*>(dsm + b.lds_pl_off) + wl * 64 * 3;
         TileCursor tc;
         tc.cur = ib;
         tc.cum = 0;
-        tc.cnt = ib + j < ie ? __float_as_int(rec[3 * (int64_t)(ib + j)].z) : 0;
-        for (int ti = 0; ti < ntiles; ++ti) {
-            TileMap m;
-            m.valid = false;
-            map_tile(rec, ie, ti, tot, j, grp, tc, m);
-            SampleCtx sc;
-            Corners<C, kBF16> cr;
-            const float dp = sample_stage<C, kBF16>(a, b.dLdP, nd, gbase, m, g, sc, cr);
-            float x[CPL];
-            corners_combine<C, kBF16>(sc, cr, x);
-            v4f h1[4], h2[4];
-            float y;
-            if constexpr (kDecBF16)
-                y = decoder_fwd_bf16<C>(sb, s, x, g, j, h1, h2);
-            else
-                y = decoder_fwd<C>(s, x, g, j, h1, h2);
-            const float dy = sc.valid ? dp * sc.w * softplus_grad_f(y, a.beta) : 0.f;
-            const v4f dx =
-                tile_backward<C, kDecBF16, DecGradsRS, !kDecBF16>(s, sb, bufA, bufB, bufAb, bufBb, x, h1, h2, dy, g, j, q);
-            if (sc.valid) {
-                float4* out = b.stage + (s_base + (int64_t)ti * kTile + j) * RF;
-                if (g == 0) out[0] = make_float4(sc.li[0], sc.li[1], sc.li[2], 0.f);
-                if (4 * g < C) out[1 + g] = make_float4(dx[0], dx[1], dx[2], dx[3]);
+        tc.cnt = 0;
+        const int nsteps = (tot + 63) / 64;
+        for (int st = 0; st < nsteps; ++st) {
+            const int base = st * 64;
+            {
+                TileMap m;
+                m.valid = false;
+                map_tile64(rec, ie, base, tot, lane, tc, m);
+                int packed = -1;
+                float dpw = 0.f;
+                SampleCtx sc;
+                sc.valid = m.valid;
+                if (m.valid) {
+                    const float o[3] = {m.B.x, m.B.y, m.B.z};
+                    const float d[3] = {m.B.w, m.D.x, m.D.y};
+                    const int r = __float_as_int(m.D.w);
+                    make_sample_node(a, nd, gbase, r, o, d, m.A.x, m.A.y, m.n_s, m.jj,
+                                     __float_as_int(m.A.w) + m.jj, sc);
+                    dpw = b.dLdP[r] * sc.w;
+                    packed = sc.tri.x0 | (sc.tri.x1 << 5) | (sc.tri.y0 << 10) | (sc.tri.y1 << 15) |
+                             (sc.tri.z0 << 20) | (sc.tri.z1 << 25);
+                    b.stage[(s_base + base + lane) * RF] = make_float4(sc.li[0], sc.li[1], sc.li[2], 0.f);
+                }
+                pl[lane * 3] = make_float4(__int_as_float(packed), sc.tri.w[0], sc.tri.w[1], sc.tri.w[2]);
+                pl[lane * 3 + 1] = make_float4(sc.tri.w[3], sc.tri.w[4], sc.tri.w[5], sc.tri.w[6]);
+                pl[lane * 3 + 2] = make_float4(sc.tri.w[7], dpw, 0.f, 0.f);
             }
+            wave_lds_sync();
+            const int ntile = min(4, (tot - base + kTile - 1) / kTile);
+            for (int q4 = 0; q4 < ntile; ++q4) {
+                const int e = q4 * kTile + j;
+                const float4 p0 = pl[e * 3], p1 = pl[e * 3 + 1], p2 = pl[e * 3 + 2];
+                const int packed = __float_as_int(p0.x);
+                SampleCtx sc;
+                sc.valid = packed >= 0;
+                sc.tri.x0 = packed & 31;
+                sc.tri.x1 = (packed >> 5) & 31;
+                sc.tri.y0 = (packed >> 10) & 31;
+                sc.tri.y1 = (packed >> 15) & 31;
+                sc.tri.z0 = (packed >> 20) & 31;
+                sc.tri.z1 = (packed >> 25) & 31;
+                sc.tri.w[0] = p0.y;
+                sc.tri.w[1] = p0.z;
+                sc.tri.w[2] = p0.w;
+                sc.tri.w[3] = p1.x;
+                sc.tri.w[4] = p1.y;
+                sc.tri.w[5] = p1.z;
+                sc.tri.w[6] = p1.w;
+                sc.tri.w[7] = p2.x;
+                sc.gbase = gbase;
+                float x[CPL];
+                if (sc.valid) {
+                    Corners<C, kBF16> cr;
+                    gather_corners<C, kBF16>(a, sc, g, cr);
+                    corners_combine<C, kBF16>(sc, cr, x);
+                } else {
+#pragma unroll
+                    for (int qq = 0; qq < CPL; ++qq) x[qq] = 0.f;
+                }
+                v4f h1[4], h2[4];
+                float y;
+                if constexpr (kDecBF16)
+                    y = decoder_fwd_bf16<C>(sb, s, x, g, j, h1, h2);
+                else
+                    y = decoder_fwd<C>(s, x, g, j, h1, h2);
+                const float dy = sc.valid ? p2.y * softplus_grad_f(y, a.beta) : 0.f;
+                const v4f dx =
+                    tile_backward<C, kDecBF16, DecGradsRS, !kDecBF16>(s, sb, bufA, bufB, bufAb, bufBb, x, h1, h2, dy, g, j,
+                                                                     q);
+                if (sc.valid && 4 * g < C)
+                    b.stage[(s_base + base + e) * RF + 1 + g] = make_float4(dx[0], dx[1], dx[2], dx[3]);
+            }
+            wave_lds_sync();
         }
     }
     store_partials<C>(q, b.ws2 + ((int64_t)blockIdx.x * kDecWaves + wl) * NP, g, j, lane);
@@ -3078,7 +3242,8 @@ atdf_status launch_two_pass(NodeBwdArgs args, hipStream_t s)
     args.lds_f32_off = 0;
     args.lds_bf16_off = (wf * 4 + 15) & ~15;
     args.lds_tb_off = (args.lds_bf16_off + (kDecBF16 ? BSmem<C>::kBf16Total * 2 : 0) + 15) & ~15;
-    const int total = args.lds_tb_off + (kDecBF16 ? kDecWaves * 2 * kH * kTB * 2 : kDecWaves * 2 * kH * kTS * 4);
+    args.lds_pl_off = args.lds_tb_off + (kDecBF16 ? kDecWaves * 2 * kH * kTB * 2 : kDecWaves * 2 * kH * kTS * 4);
+    const int total = args.lds_pl_off + kDecWaves * 64 * 3 * (int)sizeof(float4);
     if (total > max_lds) {
         set_error("atdf_project_bwd: the decoder pass needs %d bytes of LDS, the device allows %d", total, max_lds);
         return ATDF_ERR_UNSUPPORTED;
@@ -3111,7 +3276,7 @@ size_t atdf_project_bwd_workspace_bytes(int64_t R, int64_t H, const atdf_field* 
 
 size_t atdf_project_bwd_staging_bytes(int64_t H, int64_t S, const atdf_field* f)
 {
-    if (!f || H < 0 || S < 0 || f->num_grids <= 0 || f->vm || (f->C != 8 && f->C != 16)) return 0;
+    if (!f || H < 0 || S < 0 || f->num_grids <= 0 || f->vm || (f->C != 8 && f->C != 16) || f->G > 32) return 0;
     const int64_t NA = f->num_grids;
     const int np = (int)((int64_t)f->H * f->C + f->H + (int64_t)f->H * f->H + f->H + f->H + 1);
     const size_t base = bwd_layout(H, NA, np, 0).total;
@@ -3155,7 +3320,9 @@ static atdf_status prepared_view(const atdf_intervals* iv, const atdf_field* f, 
     P.w = reinterpret_cast<unsigned char*>(workspace);
     P.ws_bytes = workspace_bytes;
     P.SL = stage_layout(P.L.total, P.H, P.NA, P.np);
-    P.staging = !f->vm && (f->C == 8 || f->C == 16) && f->num_grids > 0 && workspace_bytes > P.SL.stage;
+    // (the decoder pass packs a sample's corner indices in 5 bits each: G <= 32)
+    P.staging = !f->vm && (f->C == 8 || f->C == 16) && f->G <= 32 && f->num_grids > 0 &&
+                workspace_bytes > P.SL.stage;
     return ATDF_OK;
 }
 

@@ -442,9 +442,9 @@ __global__ __launch_bounds__(kThreads) void sample_kernel(PropArgs a, const floa
 }
 
 // Learned allocation over the stratified node intervals of a ray list: for a
-// ray the gate keeps, interval i gets 1 + round(E F(tmax_i)) - round(E F(tmin_i))
-// samples (at least one per node interval; the extras telescope to E over
-// intervals that tile the chord), F the CDF of the mixed pdf over the root
+// ray the gate keeps, interval i gets its stratified count (at least one) plus
+// round(E F(tmax_i)) - round(E F(tmin_i)) samples (the extras telescope to E
+// over intervals that tile the chord), F the CDF of the mixed pdf over the root
 // chord, linear inside each of the 32 bins and clamped to [0, 1],
 // round(x) = floor(x + 1/2); `first` becomes the running sum of the ray's
 // counts.  Fallback rays and rays without a root chord keep the stratified
@@ -505,7 +505,7 @@ __global__ __launch_bounds__(kThreads) void alloc_kernel(PropArgs a, const float
         if (learn) {
             const int ka = (int)floorf(E * cdf(iv_tmin[h]) + 0.5f);
             const int kb = (int)floorf(E * cdf(iv_tmax[h]) + 0.5f);
-            n = 1 + (kb > ka ? kb - ka : 0);
+            n = iv_count[h] + (kb > ka ? kb - ka : 0);
             iv_count[h] = n;
         } else {
             n = iv_count[h];

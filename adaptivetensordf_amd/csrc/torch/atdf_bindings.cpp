@@ -362,6 +362,7 @@ PYBIND11_MODULE(TORCH_EXTENSION_NAME, m)
         .def_readwrite("kl_threshold", &ProposalParams::kl_threshold)
         .def_readwrite("coarse_lattice", &ProposalParams::coarse_lattice)
         .def_readwrite("target_power", &ProposalParams::target_power)
+        .def_readwrite("target_mode", &ProposalParams::target_mode)
         .def_readwrite("alloc_extra", &ProposalParams::alloc_extra);
     py::class_<ProposalState>(m, "ProposalState")
         .def_readwrite("params", &ProposalState::params)
@@ -375,6 +376,7 @@ PYBIND11_MODULE(TORCH_EXTENSION_NAME, m)
     m.def("MakeProposalState", &MakeProposalState);
     m.def("ProposalRefreshCoarse", &ProposalRefreshCoarse);
     m.def("ProposalFeatures", &ProposalFeatures);
+    m.def("BinVariation", &BinVariation);
     m.def("ProposalSample", &ProposalSample);
     m.def("CreateProposalIntervals",
           [](HyperTreeBase& tree, const ProposalState& ps, const RayList& rays, const SamplingParams& sp) {

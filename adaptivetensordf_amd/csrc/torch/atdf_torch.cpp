@@ -1345,12 +1345,26 @@ Tensor ProposalGrad(ProposalState& ps, Tensor features, Tensor target_bins)
     return ps.loss;
 }
 
+Tensor BinVariation(const Tensor& bins)
+{
+    TORCH_CHECK(bins.dim() == 2 && bins.size(1) >= 2, "BinVariation: bins must be [R, B] with B >= 2");
+    const int64_t B = bins.size(1);
+    auto v = torch::empty_like(bins);
+    using torch::indexing::Slice;
+    v.index_put_({Slice(), Slice(1, B - 1)},
+                 (bins.index({Slice(), Slice(2, B)}) - bins.index({Slice(), Slice(0, B - 2)})).abs() * 0.5f);
+    v.index_put_({Slice(), 0}, (bins.index({Slice(), 1}) - bins.index({Slice(), 0})).abs());
+    v.index_put_({Slice(), B - 1}, (bins.index({Slice(), B - 1}) - bins.index({Slice(), B - 2})).abs());
+    return v;
+}
+
 double ProposalTrain(HyperTreeBase& tree, ProposalState& ps, const RayList& rays, const SamplingParams& sp,
                      Tensor grid, Tensor mlp, const FieldParams& fp, int steps)
 {
     TORCH_CHECK(steps >= 1, "ProposalTrain: steps must be >= 1");
     ProposalRefreshCoarse(tree, ps, grid, mlp, fp);
     auto pb = ProjectRaysBins(tree, rays, grid, mlp, sp, fp);
+    if (ps.pp.target_mode == 1) pb.second = BinVariation(pb.second);
     auto feat = ProposalFeatures(tree, ps, rays).first;
     const int64_t R = rays.size();
     auto ws = torch::empty({(int64_t)atdf_proposal_grad_workspace_bytes(R)},

@@ -459,7 +459,11 @@ struct ProposalParams {
     float mix = 0.25f;          // uniform mixing weight of the sampling pdf
     float kl_threshold = 0.5f;  // fallback when KL(coarse || pdf) exceeds it
     int coarse_lattice = 4;     // n^3 lattice per node for the coarse density
-    float target_power = 0.5f;  // training target q_b ~ (binned density mass)^power
+    float target_power = 0.5f;  // training target q_b ~ (binned quantity)^power
+    // the binned quantity: 0 the density mass of bin b; 1 the change of the
+    // bin-mean density across the bin, |m[b+1] - m[b-1]| / 2 (one-sided at
+    // the ends), which is where a stratified estimate's variance sits
+    int target_mode = 0;
     // 0: the proposal places 32 one-sample cells per ray.  E > 0: it allocates
     // over the stratified traversal's node intervals instead -- every node
     // interval keeps one sample and E more per ray follow the proposal's CDF
@@ -496,6 +500,11 @@ torch::Tensor ProposalGrad(ProposalState& ps, torch::Tensor features, torch::Ten
 // Online training on densities of the current reconstruction: target masses
 // from the stratified binned forward over `rays`, then `steps` Adam steps.
 // Refreshes the coarse densities first.  Returns the last loss.
+// Proposal target of target_mode 1 from binned masses [R, B]: per bin the
+// change of the bin-mean density across it, |m[b+1] - m[b-1]| / 2, and
+// |m[1] - m[0]|, |m[B-1] - m[B-2]| at the ends (equal-length bins: the mass
+// is the mean times a per-ray constant).
+torch::Tensor BinVariation(const torch::Tensor& bins);
 double ProposalTrain(HyperTreeBase& tree, ProposalState& ps, const RayList& rays, const SamplingParams& sp,
                      torch::Tensor grid, torch::Tensor mlp, const FieldParams& fp, int steps);
 // Training step sampling with the proposal once it is ready (stratified before).

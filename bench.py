@@ -84,10 +84,13 @@ def parse():
     ap.add_argument("--kl-threshold", type=float, default=0.3, help="proposal fallback threshold")
     ap.add_argument("--proposal-rays", type=int, default=1 << 15, help="rays per proposal training pass")
     ap.add_argument("--proposal-mode", choices=["alloc", "cells"], default="alloc",
-                    help="alloc: one sample per node interval of the K=--alloc-k stratified list plus --alloc-extra "
+                    help="alloc: the samples of the K=--alloc-k stratified list plus --alloc-extra "
                          "samples along the proposal's CDF; cells: 32 one-sample cells per ray")
-    ap.add_argument("--alloc-extra", type=int, default=16, help="learned samples per ray beyond one per node interval")
-    ap.add_argument("--alloc-k", type=int, default=1, help="stratified K of the allocation's base list (and fallback)")
+    ap.add_argument("--alloc-extra", type=int, default=9, help="learned samples per ray beyond the stratified list's")
+    ap.add_argument("--proposal-target", choices=["mass", "variation"], default="mass",
+                    help="what the proposal learns: the binned density mass, or its change across each bin")
+    ap.add_argument("--target-power", type=float, default=0.5, help="proposal target = (binned quantity)^power")
+    ap.add_argument("--alloc-k", type=int, default=2, help="stratified K of the allocation's base list (and fallback)")
     ap.add_argument("--seed", type=int, default=0,
                     help="offset of the per-iteration jitter seeds (repeat runs of the same workload)")
     ap.add_argument("--volume", type=int, default=512, help="phantom and detector size (512 or 768)")
@@ -674,6 +677,8 @@ def main():
     pp.mix = args.mix
     pp.kl_threshold = args.kl_threshold
     pp.alloc_extra = args.alloc_extra if args.proposal_mode == "alloc" else 0
+    pp.target_mode = 1 if args.proposal_target == "variation" else 0
+    pp.target_power = args.target_power
     K_learned = args.alloc_k if args.proposal_mode == "alloc" else K
     prop_rays = []
     for rays, _ in batches:
@@ -769,7 +774,7 @@ def main():
     b = 2 if bf16_head else 4
     if args.mode == "learned":
         pr = head["proposal"]
-        how = (f"allocation: one sample per node interval of the K={pr['base_K']} stratified list plus "
+        how = (f"allocation: the samples of the K={pr['base_K']} stratified list plus "
                f"{pr['extra_samples']} along the proposal's CDF" if pr["mode"] == "alloc" else "32 learned cells")
         workload = (f"learned proposal ({how}) + BF16 grids training step, {n}^3 noisy ellipsoid phantom "
                     f"(200, sigma 0.01), "

@@ -746,10 +746,11 @@ atdf_status atdf_proposal_sample(const void* packed_tree, int64_t num_nodes, con
  * mode; no reference counterpart).  `intervals` is the list of
  * atdf_traverse_write for every ray (any K); for each ray the gate keeps
  * (KL(coarse || pdf) <= kl_threshold), node interval i of the ray gets
- *   1 + round(E F(tmax_i)) - round(E F(tmin_i))
- * samples, E = extra_samples, F the CDF of the mixed pdf over the ray's root
- * chord (linear within each of the 32 bins, clamped to [0, 1]),
- * round(x) = floor(x + 1/2): every node interval keeps at least one sample
+ *   count_i + round(E F(tmax_i)) - round(E F(tmin_i))
+ * samples, count_i its stratified count in the list (at least one),
+ * E = extra_samples, F the CDF of the mixed pdf over the ray's root chord
+ * (linear within each of the 32 bins, clamped to [0, 1]),
+ * round(x) = floor(x + 1/2): every node interval keeps its stratified samples
  * and the extras telescope to E over intervals that tile the chord.  A
  * fallback ray (KL above the threshold) is allocated the same way along the
  * uniform pdf (F = the chord fraction, up to FP32 rounding): stratified

@@ -279,8 +279,8 @@ static int ray_pdf(const atdfo_proposal* p, const float* ro, const float* rd, fl
 }
 
 /* Learned allocation over a stratified interval list (atdf_proposal_allocate):
- * for a ray the gate keeps, interval i gets 1 + round(E F(tmax_i)) -
- * round(E F(tmin_i)) samples, F(t) = cum[b] + pm[b] ((t - edge_b) / len) at
+ * for a ray the gate keeps, interval i gets its stratified count plus
+ * round(E F(tmax_i)) - round(E F(tmin_i)) samples, F(t) = cum[b] + pm[b] ((t - edge_b) / len) at
  * b = clamp(floor((t - t0) / len), 0, 31), clamped to [0, 1], cum the running
  * sum of pm, round(x) = floor(x + 1/2); first = the running sum of counts.
  * A fallback ray (KL above the threshold) does the same with pm = 1/32 in
@@ -323,7 +323,7 @@ void atdfo_prop_allocate(const atdfo_proposal* p, const float* o, const float* d
                     if (!(F <= 1.f)) F = 1.f;
                     k[e] = (int)floorf(E * F + 0.5f);
                 }
-                n = 1 + (k[1] > k[0] ? k[1] - k[0] : 0);
+                n = count[h] + (k[1] > k[0] ? k[1] - k[0] : 0);
                 count[h] = n;
             } else {
                 n = count[h];

@@ -264,7 +264,7 @@ def test_allocated_interval_list_and_projection(oracle, native, cuda, jitter_mod
     assert np.array_equal(iv.ray_offset.cpu().numpy(), off) and np.array_equal(iv.node.cpu().numpy(), tr["iv_node"])
     assert np.array_equal(iv.count.cpu().numpy(), cnt) and np.array_equal(iv.first.cpu().numpy(), first)
     assert np.array_equal(iv.ray_num_samples.cpu().numpy(), ns) and iv.num_samples == int(cnt.sum())
-    assert cnt.min() >= 1
+    assert cnt.min() >= 1 and np.all(cnt >= tr["iv_count"])
     # the extras telescope to E on rays whose intervals tile their root chord
     # (culled holes leave the pdf's mass there unallocated), gated or not
     _, ch = prop.features(o, d)
@@ -275,7 +275,7 @@ def test_allocated_interval_list_and_projection(oracle, native, cuda, jitter_mod
         L = float(ch[r, 1] - ch[r, 0])
         tiled.append(np.array_equal(lo_[a + 1:b], hi_[a:b - 1]) and abs(lo_[a] - ch[r, 0]) <= 1e-6 * L
                      and abs(hi_[b - 1] - ch[r, 1]) <= 1e-6 * L)
-        extras.append(int(cnt[a:b].sum()) - (b - a))
+        extras.append(int(cnt[a:b].sum()) - int(tr["iv_count"][a:b].sum()))
     extras, tiled = np.array(extras), np.array(tiled)
     print(f"gated rays: {extras.size}, tiling their chord: {int(tiled.sum())}", flush=True)
     assert tiled.sum() > 0.5 * tiled.size

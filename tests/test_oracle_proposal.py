@@ -330,9 +330,9 @@ def _alloc_case(oracle, mix, tau, params=None, depth=3, K=3, n=200, seed=9):
 
 def test_allocation_uniform_pdf_closed_form(oracle):
     """mix = 1 (uniform pdf): F is the chord fraction, so interval i of a ray
-    gets 1 + round(E (tmax - t0) / L) - round(E (tmin - t0) / L) samples (FP64,
-    away from rounding ties); the extras telescope to E; fallback rays and
-    `first` follow."""
+    gets its stratified count + round(E (tmax - t0) / L) - round(E (tmin - t0)
+    / L) samples (FP64, away from rounding ties); the extras telescope to E;
+    fallback rays and `first` follow."""
     E = 16
     prop, o, d, tr, off, ch = _alloc_case(oracle, mix=1.0, tau=float("inf"))
     cnt, first, fb, ns, kl = prop.allocate(o, d, E, off, tr["iv_tmin"], tr["iv_tmax"], tr["iv_count"])
@@ -347,11 +347,12 @@ def test_allocation_uniform_pdf_closed_form(oracle):
         fa = E * (tr["iv_tmin"][a:b].astype(np.float64) - t0) / L
         fb_ = E * (tr["iv_tmax"][a:b].astype(np.float64) - t0) / L
         ties = np.minimum(np.abs(fa + 0.5 - np.round(fa + 0.5)), np.abs(fb_ + 0.5 - np.round(fb_ + 0.5))) < 1e-4
-        want = 1 + np.maximum(np.floor(fb_ + 0.5) - np.floor(fa + 0.5), 0)
+        base = tr["iv_count"][a:b]
+        want = base + np.maximum(np.floor(fb_ + 0.5) - np.floor(fa + 0.5), 0)
         assert np.array_equal(cnt[a:b][~ties], want[~ties].astype(np.int32)), r
         assert np.array_equal(first[a:b], np.concatenate([[0], np.cumsum(cnt[a:b])[:-1]]))
         assert ns[r] == cnt[a:b].sum()
-        assert abs(int(cnt[a:b].sum()) - (b - a) - E) <= 1  # the extras telescope (up to a tie at an end)
+        assert abs(int(cnt[a:b].sum()) - int(base.sum()) - E) <= 1  # the extras telescope (up to a tie at an end)
         checked += 1
     assert checked > 100
 
@@ -383,10 +384,11 @@ def test_allocation_one_bin_pdf_and_gate(oracle):
         fa = E * F(tr["iv_tmin"][a:b].astype(np.float64))
         fz = E * F(tr["iv_tmax"][a:b].astype(np.float64))
         ties = np.minimum(np.abs(fa + 0.5 - np.round(fa + 0.5)), np.abs(fz + 0.5 - np.round(fz + 0.5))) < 1e-3
-        want = 1 + np.maximum(np.floor(fz + 0.5) - np.floor(fa + 0.5), 0)
+        base = tr["iv_count"][a:b]
+        want = base + np.maximum(np.floor(fz + 0.5) - np.floor(fa + 0.5), 0)
         assert np.array_equal(cnt[a:b][~ties], want[~ties].astype(np.int32)), r
         outside = (tr["iv_tmax"][a:b] <= lo) | (tr["iv_tmin"][a:b] >= hi)
-        assert np.all(cnt[a:b][outside & ~ties] == 1)
+        assert np.array_equal(cnt[a:b][outside & ~ties], base[outside & ~ties])
     # the gate: a finite threshold sends some rays to the fallback, which are
     # allocated along the uniform pdf, as every ray is at mix = 1
     prop2, o, d, tr, off, ch = _alloc_case(oracle, mix=0.25, tau=0.3)
