@@ -1913,6 +1913,9 @@ __global__ __launch_bounds__(64 * bwd_waves<kDecBF16>()) void project_bwd_node_k
 #endif
 constexpr int64_t kPartSamples = ATDF_PART_SAMPLES;
 constexpr int kMinPartIv = 16;    // intervals per decoder-pass work item, at least
+// Staged records past the list's last sample: the scatter pass reads up to
+// 64 + 2 records beyond a chunk's end without a branch.
+constexpr int64_t kStagePad = 128;
 constexpr int kDecWaves = 8;  // one workgroup per CU, two waves per SIMD
 
 // Work items of a chunk of n_iv intervals holding `samples` samples.
@@ -2106,55 +2109,63 @@ __global__ __launch_bounds__(32 * C) void project_bwd_scatter_kernel(NodeBwdArgs
         const int64_t run = (s1 - s0 + 63) / 64;
         const int64_t rb = s0 + lane * run;
         const int64_t re = rb + run < s1 ? rb + run : s1;
-        float2 nd = make_float2(0.f, 0.f);
-        float npx = 0.f, npy = 0.f, npz = 0.f;
-        if (rb < re) {
-            const float* rec = stagef + rb * RF * 4;
-            nd = *reinterpret_cast<const float2*>(rec + 4 + c0);
-            npx = rec[0];
-            npy = rec[1];
-            npz = rec[2];
-        }
-        for (int64_t st = 0; st < run; ++st) {
-            const int64_t si = rb + st;
-            const float2 dv = nd;
-            const float px = npx, py = npy, pz = npz;
-            const bool live = si < re && (dv.x != 0.f || dv.y != 0.f);
-            if (si + 1 < re) {
-                const float* rec = stagef + (si + 1) * RF * 4;
-                nd = *reinterpret_cast<const float2*>(rec + 4 + c0);
-                npx = rec[0];
-                npy = rec[1];
-                npz = rec[2];
-            }
-            Tri t;
-            tri_from_index(px, py, pz, G, G, G, t);
-            const int cell = (t.z0 * G + t.y0) * G + t.x0;
-            if (live) owner[cell] = (unsigned char)lane;
-            wave_lds_sync();
-            const bool mine = live && owner[cell] == lane;
+        // three register sets, one per step modulo 3, so that the records of
+        // the next two steps load while a step runs (the loop is unrolled by
+        // three: no copies between sets, no wait on the later loads)
+        float2 nd[3];
+        float np_[3][3];
+        // loads without a branch: a lane past its run reads a record of the
+        // next chunk or of the stage's kStagePad padding and ignores it (it
+        // is not live; its clamped cell indices stay in range), so the
+        // compiler's wait counts stay exact across the unrolled steps
+        auto load = [&](int u, int64_t si) {
+            const float* rec = stagef + si * RF * 4;
+            nd[u] = *reinterpret_cast<const float2*>(rec + 4 + c0);
+            np_[u][0] = rec[0];
+            np_[u][1] = rec[1];
+            np_[u][2] = rec[2];
+        };
+        load(0, rb);
+        load(1, rb + 1);
+        for (int64_t st0 = 0; st0 < run; st0 += 3) {
 #pragma unroll
-            for (int k = 0; k < 8; ++k) {
-                if (mine) {
-                    const int cc = (int)corner_cell(t, k, G);
-                    float2* gp = reinterpret_cast<float2*>(lgrad + cc * C + (c0 ^ pair_swz(cc, C)));
-                    float2 v = *gp;
-                    v.x = fmaf(t.w[k], dv.x, v.x);
-                    v.y = fmaf(t.w[k], dv.y, v.y);
-                    *gp = v;
-                }
-            }
-            wave_lds_sync();
-            if (live && !mine) {
+            for (int u = 0; u < 3; ++u) {
+                const int64_t st = st0 + u;
+                if (st >= run) break;
+                const int64_t si = rb + st;
+                const float2 dv = nd[u];
+                const float px = np_[u][0], py = np_[u][1], pz = np_[u][2];
+                load((u + 2) % 3, si + 2);
+                const bool live = (si < re) & ((dv.x != 0.f) | (dv.y != 0.f));
+                Tri t;
+                tri_from_index(px, py, pz, G, G, G, t);
+                const int cell = (t.z0 * G + t.y0) * G + t.x0;
+                if (live) owner[cell] = (unsigned char)lane;
+                wave_lds_sync();
+                const bool mine = live && owner[cell] == lane;
 #pragma unroll
                 for (int k = 0; k < 8; ++k) {
-                    const int cc = (int)corner_cell(t, k, G);
-                    float* gp = lgrad + cc * C + (c0 ^ pair_swz(cc, C));
-                    atomicAdd(gp, t.w[k] * dv.x);
-                    atomicAdd(gp + 1, t.w[k] * dv.y);
+                    if (mine) {
+                        const int cc = (int)corner_cell(t, k, G);
+                        float2* gp = reinterpret_cast<float2*>(lgrad + cc * C + (c0 ^ pair_swz(cc, C)));
+                        float2 v = *gp;
+                        v.x = fmaf(t.w[k], dv.x, v.x);
+                        v.y = fmaf(t.w[k], dv.y, v.y);
+                        *gp = v;
+                    }
                 }
+                wave_lds_sync();
+                if (live && !mine) {
+#pragma unroll
+                    for (int k = 0; k < 8; ++k) {
+                        const int cc = (int)corner_cell(t, k, G);
+                        float* gp = lgrad + cc * C + (c0 ^ pair_swz(cc, C));
+                        atomicAdd(gp, t.w[k] * dv.x);
+                        atomicAdd(gp + 1, t.w[k] * dv.y);
+                    }
+                }
+                wave_lds_sync();
             }
-            wave_lds_sync();
         }
         __syncthreads();
         flush_node_grad<4, C>(b.ad, b.ad.enabled && !ch.w, ch.w != 0, lgrad, b.grad_grid + gbase, gbase, gsz >> 2);
@@ -2182,7 +2193,12 @@ struct NodeFwdArgs {
     int64_t num_slots;
     const float4* rec;
     float* ivsum;
-    int* counter;  // chunk counter of the LDS-staged variant (zeroed before the launch)
+    int* counter;  // chunk / item counter (zeroed before the launch)
+    // work items of the two-pass adjoint's part table (project_fwd_part_kernel)
+    const int64_t* soff;
+    const int4* parts;
+    const int64_t* part_off;
+    int64_t max_chunks;
 };
 
 // Trilinear gather of CPL channels from a node grid staged in LDS
@@ -2394,6 +2410,153 @@ __global__ __launch_bounds__(64 * kNW, kLDS ? 1 : 4) void project_fwd_node_kerne
     }
 }
 
+// Node-major forward over the part table of the adjoint's sort (a workspace
+// with the staging region): waves take work items of ~kPartSamples samples
+// (a run of one chunk's intervals, one node) from a counter and run them in
+// steps of 64 samples.  (A) lane l maps and places sample l of the step once
+// (map_tile64) and writes its packed corners, trilinear weights, integration
+// weight and interval id to the wave's sample table in LDS; (B) the step's
+// four tiles of 16 read the table, gather their lane's channels, run the
+// decoder and sum w * density per interval with the segmented scan of
+// project_fwd_node_kernel (an item holds whole intervals, so no sum crosses
+// items).  ray_sum_kernel adds a ray's interval sums in list order.
+constexpr int kFwdPartWaves = 4;
+
+template <int C, bool kBF16, bool kDecBF16>
+__global__ __launch_bounds__(64 * kFwdPartWaves, 4) void project_fwd_part_kernel(NodeFwdArgs b)
+{
+    using SM = DecoderSmem<C, kDecBF16>;
+    constexpr int CPL = C / 4;
+    __shared__ __attribute__((aligned(16))) float s[SM::kF32];
+    __shared__ __attribute__((aligned(16))) __bf16 sb[SM::kBF16];
+    __shared__ float4 s_pl[kFwdPartWaves][64 * 3];
+    const ProjArgs& a = b.a;
+    const float4* __restrict__ rec = b.rec;
+    if constexpr (kDecBF16)
+        load_mlp_bf16_to_lds<C>(a.mlp, sb, s);
+    else
+        load_mlp_to_lds<C>(a.mlp, s);
+    __syncthreads();
+    const int G = a.G;
+    const int gsz = G * G * G * C;
+    const int lane = threadIdx.x & 63;
+    const int g = lane >> 4, j = lane & 15;
+    const int wl = threadIdx.x >> 6;
+    float4* pl = s_pl[wl];
+    const int64_t n_items = b.part_off[b.max_chunks];
+    while (true) {
+        int item = 0;
+        if (lane == 0) item = atomicAdd(b.counter, 1);
+        item = __builtin_amdgcn_readfirstlane(__shfl(item, 0, 64));
+        if (item >= n_items) break;
+        const int4 it = b.parts[item];
+        const PNode nd = a.nodes[a.iv_node[b.sorted[it.y]]];
+        const int64_t gbase = (int64_t)it.x * gsz;
+        const int ib = it.y, ie = it.z;
+        const int tot = (int)(b.soff[ie] - b.soff[ib]);
+        TileCursor tc;
+        tc.cur = ib;
+        tc.cum = 0;
+        tc.cnt = 0;
+        // the wave's open interval (continuing into the next tile) and its sum
+        int carry_id = -1;
+        float carry = 0.f;
+        const int nsteps = (tot + 63) / 64;
+        for (int st = 0; st < nsteps; ++st) {
+            const int base = st * 64;
+            {
+                TileMap m;
+                m.valid = false;
+                map_tile64(rec, ie, base, tot, lane, tc, m);
+                int packed = -1, iv_id = -1;
+                SampleCtx sc;
+                sc.valid = m.valid;
+                sc.w = 0.f;
+                if (m.valid) {
+                    const float o[3] = {m.B.x, m.B.y, m.B.z};
+                    const float d[3] = {m.B.w, m.D.x, m.D.y};
+                    make_sample_node(a, nd, gbase, __float_as_int(m.D.w), o, d, m.A.x, m.A.y, m.n_s, m.jj,
+                                     __float_as_int(m.A.w) + m.jj, sc);
+                    packed = sc.tri.x0 | (sc.tri.x1 << 5) | (sc.tri.y0 << 10) | (sc.tri.y1 << 15) |
+                             (sc.tri.z0 << 20) | (sc.tri.z1 << 25);
+                    iv_id = __float_as_int(m.D.z);
+                }
+                pl[lane * 3] = make_float4(__int_as_float(packed), sc.tri.w[0], sc.tri.w[1], sc.tri.w[2]);
+                pl[lane * 3 + 1] = make_float4(sc.tri.w[3], sc.tri.w[4], sc.tri.w[5], sc.tri.w[6]);
+                pl[lane * 3 + 2] = make_float4(sc.tri.w[7], sc.w, __int_as_float(iv_id), 0.f);
+            }
+            wave_lds_sync();
+            const int ntile = min(4, (tot - base + kTile - 1) / kTile);
+            for (int q4 = 0; q4 < ntile; ++q4) {
+                const int tb = base + q4 * kTile;
+                const int e = q4 * kTile + j;
+                const float4 p0 = pl[e * 3], p1 = pl[e * 3 + 1], p2 = pl[e * 3 + 2];
+                const int packed = __float_as_int(p0.x);
+                SampleCtx sc;
+                sc.valid = packed >= 0;
+                sc.tri.x0 = packed & 31;
+                sc.tri.x1 = (packed >> 5) & 31;
+                sc.tri.y0 = (packed >> 10) & 31;
+                sc.tri.y1 = (packed >> 15) & 31;
+                sc.tri.z0 = (packed >> 20) & 31;
+                sc.tri.z1 = (packed >> 25) & 31;
+                sc.tri.w[0] = p0.y;
+                sc.tri.w[1] = p0.z;
+                sc.tri.w[2] = p0.w;
+                sc.tri.w[3] = p1.x;
+                sc.tri.w[4] = p1.y;
+                sc.tri.w[5] = p1.z;
+                sc.tri.w[6] = p1.w;
+                sc.tri.w[7] = p2.x;
+                sc.gbase = gbase;
+                const int iv_id = __float_as_int(p2.z);
+                float x[CPL];
+                if (sc.valid) {
+                    Corners<C, kBF16> cr;
+                    gather_corners<C, kBF16>(a, sc, g, cr);
+                    corners_combine<C, kBF16>(sc, cr, x);
+                } else {
+#pragma unroll
+                    for (int qq = 0; qq < CPL; ++qq) x[qq] = 0.f;
+                }
+                v4f h1[4], h2[4];
+                float y;
+                int opq = 0;
+                asm volatile("" : "+v"(opq));  // no hoisting of the weight reads
+                const float* so = s + opq;
+                const __bf16* sbo = sb + opq;
+                if constexpr (kDecBF16)
+                    y = decoder_fwd_bf16<C>(sbo, so, x, g, j, h1, h2);
+                else
+                    y = decoder_fwd<C>(so, x, g, j, h1, h2);
+                // segmented inclusive scan of w * density over the tile (lanes 0..15)
+                float v = sc.valid ? softplus_f(y, a.beta) * p2.y : 0.f;
+#pragma unroll
+                for (int off = 1; off < kTile; off <<= 1) {
+                    const float ov = __shfl_up(v, off, kTile);
+                    const int oid = __shfl_up(iv_id, off, kTile);
+                    if (j >= off && oid == iv_id) v += ov;
+                }
+                const int next_id = __shfl_down(iv_id, 1, kTile);
+                const bool seg_end = sc.valid && (j == kTile - 1 || next_id != iv_id);
+                // the tile's first interval may continue the previous tile's last one
+                const int first_id = __shfl(iv_id, 0, 64);
+                if (carry_id >= 0 && carry_id != first_id && lane == 0) b.ivsum[carry_id] = carry;
+                if (seg_end && iv_id == carry_id) v += carry;
+                // the last valid segment stays open unless this is the item's last tile
+                const int last_valid = min(kTile, tot - tb) - 1;
+                const int last_id = __shfl(iv_id, last_valid, 64);
+                const float last_v = __shfl(v, last_valid, 64);
+                const bool final_tile = tb + kTile >= tot;
+                if (g == 0 && seg_end && (iv_id != last_id || final_tile)) b.ivsum[iv_id] = v;
+                carry_id = final_tile ? -1 : last_id;
+                carry = last_v;
+            }
+            wave_lds_sync();
+        }
+    }
+}
+
 __global__ __launch_bounds__(256) void ray_sum_kernel(const int64_t* __restrict__ iv_off,
                                                       const int32_t* __restrict__ count,
                                                       const float* __restrict__ ivsum, int64_t R,
@@ -2597,7 +2760,8 @@ __global__ __launch_bounds__(256) void part_desc_kernel(const int4* __restrict__
 __global__ void stage_mode_kernel(const unsigned long long* __restrict__ stotal, const int* __restrict__ prepared,
                                   unsigned long long capacity, int rec_bytes, int* __restrict__ mode)
 {
-    if (threadIdx.x == 0) *mode = (*prepared == 1 && *stotal * (unsigned long long)rec_bytes <= capacity) ? 1 : 0;
+    if (threadIdx.x == 0)
+        *mode = (*prepared == 1 && (*stotal + kStagePad) * (unsigned long long)rec_bytes <= capacity) ? 1 : 0;
 }
 
 // ---------------------------------------------------------------------------
@@ -3281,7 +3445,7 @@ size_t atdf_project_bwd_staging_bytes(int64_t H, int64_t S, const atdf_field* f)
     const int np = (int)((int64_t)f->H * f->C + f->H + (int64_t)f->H * f->H + f->H + f->H + 1);
     const size_t base = bwd_layout(H, NA, np, 0).total;
     const StageLayout st = stage_layout(base, H, NA, np);
-    return st.stage - base + (size_t)S * (1 + f->C / 4) * sizeof(float4);
+    return st.stage - base + (size_t)(S + kStagePad) * (1 + f->C / 4) * sizeof(float4);
 }
 
 }  // extern "C"
@@ -3536,7 +3700,28 @@ static atdf_status fwd_node_impl(const Prepared& P, const void* packed_tree, con
     // (the weights take at most 24 KB of static LDS)
     const size_t grid_bytes = (size_t)f->G * f->G * f->G * f->C * (bf ? 2 : 4);
     const bool lds = grid_bytes + 24 * 1024 <= (size_t)max_lds_per_block() && grid_bytes % 16 == 0;
-    if (lds) hipLaunchKernelGGL(fill_i32_kernel, dim3(1), dim3(64), 0, s, args.counter, 1, 0);
+    if (lds || P.staging) hipLaunchKernelGGL(fill_i32_kernel, dim3(1), dim3(64), 0, s, args.counter, 1, 0);
+    if (P.staging && P.H > 0) {
+        // the part table of the adjoint's two-pass form is in the workspace:
+        // work items of ~kPartSamples samples, each sample placed once
+        args.soff = reinterpret_cast<const int64_t*>(P.w + P.SL.soff);
+        args.parts = reinterpret_cast<const int4*>(P.w + P.SL.parts);
+        args.part_off = reinterpret_cast<const int64_t*>(P.w + P.SL.part_off);
+        args.max_chunks = max_chunks_of(P.H, P.NA);
+        dim3 grid_p((unsigned)persistent_blocks() * 4), block_p(64 * kFwdPartWaves);
+#define ATDF_FP(CC, BF, BD) hipLaunchKernelGGL((project_fwd_part_kernel<CC, BF, BD>), grid_p, block_p, 0, s, args)
+        if (f->C == 16) {
+            if (bf) { if (bd) ATDF_FP(16, true, true); else ATDF_FP(16, true, false); }
+            else    { if (bd) ATDF_FP(16, false, true); else ATDF_FP(16, false, false); }
+        } else {
+            if (bf) { if (bd) ATDF_FP(8, true, true); else ATDF_FP(8, true, false); }
+            else    { if (bd) ATDF_FP(8, false, true); else ATDF_FP(8, false, false); }
+        }
+#undef ATDF_FP
+        hipLaunchKernelGGL(ray_sum_kernel, dim3((unsigned)div_up(R, 256)), dim3(256), 0, s, iv_off, iv->count,
+                           args.ivsum, R, out);
+        return check_launch("atdf_project_fwd_prepared");
+    }
     // chunks are similar in work; a fixed grid strides over them (or takes
     // them from the counter: up to four workgroups per CU, as LDS allows)
     const int64_t max_chunks = P.H / kMinChunk + P.NA + 1;

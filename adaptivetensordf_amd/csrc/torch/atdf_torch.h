@@ -368,7 +368,7 @@ struct FieldParams {
     float softplus_beta = 2.0f;
     int rays_per_wave = 0;
     bool bf16_decoder = false;  // BF16 MFMA inputs, FP32 accumulation
-    bool node_forward = false;  // training step: node-major forward over the adjoint's sort
+    bool node_forward = true;   // training step: node-major forward over the adjoint's sort
     bool two_pass_adjoint = true;  // workspace with the adjoint's staging region (atdf_project_bwd_staging_bytes)
 };
 

@@ -101,8 +101,9 @@ def parse():
     ap.add_argument("--samples-per-ray", type=float, default=128.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-runs", type=int, default=5, help="timed CPU baseline runs (median reported)")
-    ap.add_argument("--node-major-forward", action="store_true",
-                    help="training step with the node-major forward over the adjoint's sort")
+    ap.add_argument("--ray-major-forward", action="store_true",
+                    help="training step with the ray-major forward (default: the node-major forward over the "
+                         "adjoint's sort)")
     ap.add_argument("--save-state", default="", help="write the grown workload state to this file and exit")
     ap.add_argument("--load-state", default="", help="start from a grown workload state written by --save-state")
     ap.add_argument("--profile-steps", type=int, default=0,
@@ -210,7 +211,7 @@ def build_data(N, args, dev):
     return batches, ev
 
 
-def field_params(N, bf16, node_forward=False):
+def field_params(N, bf16, node_forward=True):
     fp = N.FieldParams()
     fp.C, fp.G, fp.H = C, G, H
     fp.bf16_decoder = bf16
@@ -688,34 +689,36 @@ def main():
                                    rays.direction.index_select(0, idx).contiguous()))
     legs = {}
     if args.mode in ("plain", "learned"):
-        legs["plain"] = run_leg(N, "plain", tree, batches, K, field_params(N, False, args.node_major_forward), grid0,
+        legs["plain"] = run_leg(N, "plain", tree, batches, K, field_params(N, False, not args.ray_major_forward), grid0,
                                 mlp0, False, None, judge, args, steps)
     if args.mode == "learned":
         ps = N.MakeProposalState(phantom.init_proposal(0).to(dev), 3e-3, pp)
         legs["learned"] = run_leg(N, "learned+bf16", tree, batches, K_learned,
-                                  field_params(N, True, args.node_major_forward), grid0, mlp0, True, ps, judge, args,
+                                  field_params(N, True, not args.ray_major_forward), grid0, mlp0, True, ps, judge, args,
                                   steps, prop_rays, K_train=K)
     if args.profile_steps > 0:
         # the launches the full run's roofline times: 1 + 5 of project_fwd,
         # then 1 + 5 of project_bwd per leg, last in the run
         for key, leg in legs.items():
             bf = key != "plain"
-            kernel_times(N, tree, batches[0][0], leg_sampling(N, leg["K"]), field_params(N, bf, args.node_major_forward),
+            kernel_times(N, tree, batches[0][0], leg_sampling(N, leg["K"]), field_params(N, bf, not args.ray_major_forward),
                          leg["state"], bf, ps if key == "learned" else None)
         return
     if args.mode in ("bf16", "learned"):
-        legs["bf16"] = run_leg(N, "bf16", tree, batches, K, field_params(N, True), grid0, mlp0, True, None, judge,
+        legs["bf16"] = run_leg(N, "bf16", tree, batches, K, field_params(N, True, not args.ray_major_forward), grid0,
+                               mlp0, True, None, judge,
                                args, steps)
     if args.mode == "learned":
         K32, s32 = choose_K(N, tree, batches[0][0], legs["learned"]["samples_per_ray"])
         legs["strat32"] = run_leg(N, "bf16 stratified at the learned sample count", tree, batches, K32,
-                                  field_params(N, True), grid0, mlp0, True, None, judge, args, steps)
+                                  field_params(N, True, not args.ray_major_forward), grid0, mlp0, True, None, judge,
+                                  args, steps)
         legs["strat32"]["K"] = K32
 
     head = legs["learned" if args.mode == "learned" else args.mode]
     bf16_head = args.mode != "plain"
     kern = kernel_times(N, tree, batches[0][0], leg_sampling(N, head["K"]),
-                        field_params(N, bf16_head, args.node_major_forward), head["state"], bf16_head,
+                        field_params(N, bf16_head, not args.ray_major_forward), head["state"], bf16_head,
                         ps if args.mode == "learned" else None)
     summary = {}
     for key, leg in legs.items():
@@ -728,7 +731,7 @@ def main():
             summary[key]["K"] = leg["K"]
     if "plain" in legs and head is not legs["plain"]:
         kp = kernel_times(N, tree, batches[0][0], leg_sampling(N, legs["plain"]["K"]),
-                          field_params(N, False, args.node_major_forward), legs["plain"]["state"], False, None)
+                          field_params(N, False, not args.ray_major_forward), legs["plain"]["state"], False, None)
         summary["plain"].update({"kernel_ms": kp["kernel_ms"], "roofline": kp["roofline"],
                                  "speedup_learned_over_plain": head["rays_per_s"] / legs["plain"]["rays_per_s"]})
     mse = None

@@ -365,14 +365,17 @@ def test_fused_adjoint_adam_matches_separate(oracle, cuda, bf16):
         assert torch.equal(pb1, p1.to(torch.bfloat16))
 
 
-@pytest.mark.parametrize("C_,bf16", [(16, False), (16, True), (8, False)])
-def test_node_forward_prepared(oracle, cuda, C_, bf16):
+@pytest.mark.parametrize("C_,bf16,staged", [(16, False, False), (16, True, False), (8, False, False),
+                                             (16, False, True), (16, True, True), (8, False, True)])
+def test_node_forward_prepared(oracle, cuda, C_, bf16, staged):
     """atdf_project_prepare + atdf_project_fwd_prepared (node-major) against
     the ray-major atdf_project_fwd on the same intervals: per-interval sums
     change the rounding order only (<= 2e-6 relative to the largest
     projection), zero for rays without samples; within 1e-5 of the oracle;
     atdf_project_bwd_prepared after the forward equals atdf_project_bwd_adam
-    to the rounding of the adjoint's LDS accumulation order."""
+    to the rounding of the adjoint's LDS accumulation order.  `staged`: the
+    workspace carries the two-pass region, so the forward runs over its part
+    table (project_fwd_part_kernel) and the adjoints in their two-pass form."""
     from conftest import edge_case_rays
     from adaptivetensordf_amd import capi, native, phantom
 
@@ -399,8 +402,10 @@ def test_node_forward_prepared(oracle, cuda, C_, bf16):
             capi.ptr(iv.ray_offset), C.byref(civ), C.byref(f))
     P0 = torch.empty(R, device=cuda)
     capi.call("atdf_project_fwd", *args, capi.ptr(P0), capi.stream_ptr())
-    ws = torch.empty(capi.lib().atdf_project_bwd_workspace_bytes(R, iv.num_intervals, C.byref(f)), dtype=torch.uint8,
-                     device=cuda)
+    ws_bytes = capi.lib().atdf_project_bwd_workspace_bytes(R, iv.num_intervals, C.byref(f))
+    if staged:
+        ws_bytes += capi.lib().atdf_project_bwd_staging_bytes(iv.num_intervals, iv.num_samples, C.byref(f))
+    ws = torch.empty(ws_bytes, dtype=torch.uint8, device=cuda)
     P1 = torch.full((R,), float("nan"), device=cuda)
     print(f"C={C_} bf16={bf16}: {iv.num_intervals} intervals ({zero_iv} without samples)", flush=True)
     capi.call("atdf_project_prepare", *args, capi.ptr(ws), ws.numel(), capi.stream_ptr())
