@@ -411,6 +411,53 @@ __device__ __forceinline__ void vm_reads(const ProjArgs& a, int pslot, const VMS
     }
 }
 
+// vm_setup + vm_reads for plane p and line p alone (the same arithmetic):
+// a sample's tensorial reads one pair at a time, in fewer registers.
+template <int C, int NCH>
+__device__ __forceinline__ void vm_pair_read(const ProjArgs& a, int pslot, const SampleCtx& sc, int p, int c0,
+                                             float* P, float* L)
+{
+    const int Gp = a.vm.Gp;
+    Bil b;
+    Lin l;
+    if (p == 0) {
+        bil_setup(sc.loc[1], sc.loc[2], Gp, Gp, b);
+        const float xr = (2.0f * (sc.gx - a.vm.rx0)) / (a.vm.rx1 - a.vm.rx0) - 1.0f;
+        lin_setup(xr, a.vm.Gl, l);
+    } else if (p == 1) {
+        bil_setup(sc.loc[0], sc.loc[2], Gp, Gp, b);
+        lin_setup(sc.loc[1], a.vm.Gl, l);
+    } else {
+        bil_setup(sc.loc[0], sc.loc[1], Gp, Gp, b);
+        lin_setup(sc.gz, a.vm.Gl, l);
+    }
+    const float* pp = a.vm.planes + ((int64_t)pslot * 3 + p) * Gp * Gp * C + c0;
+    const int off[4] = {(b.y0 * Gp + b.x0) * C, (b.y0 * Gp + b.x1) * C, (b.y1 * Gp + b.x0) * C,
+                        (b.y1 * Gp + b.x1) * C};
+    float acc[NCH];
+#pragma unroll
+    for (int q = 0; q < NCH; ++q) acc[q] = 0.f;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        float v[NCH];
+        load_channels<C, NCH>(pp + off[k], v);
+#pragma unroll
+        for (int q = 0; q < NCH; ++q) acc[q] += v[q] * b.w[k];
+    }
+    const float* ln = a.vm.lines + (int64_t)p * a.vm.Gl * C + c0;
+    float v0[NCH], v1[NCH];
+    load_channels<C, NCH>(ln + l.x0 * C, v0);
+    load_channels<C, NCH>(ln + l.x1 * C, v1);
+#pragma unroll
+    for (int q = 0; q < NCH; ++q) {
+        P[q] = acc[q];
+        float la = 0.f;
+        la += v0[q] * l.w0;
+        la += v1[q] * l.w1;
+        L[q] = la;
+    }
+}
+
 template <int NCH>
 __device__ __forceinline__ void vm_combine(const float (*P)[NCH], const float (*L)[NCH], float* x)
 {
@@ -1065,29 +1112,31 @@ __device__ __forceinline__ void store_partials(DecGrads& q, float* out, int g, i
     if (lane == 0) out[oB3] = b3;
 }
 
-template <int C>
+// kAdd: add to the row (written by an earlier kernel on the same stream)
+template <int C, bool kAdd = false>
 __device__ __forceinline__ void store_partials(DecGradsRS& q, float* out, int g, int j, int lane)
 {
     constexpr int oW1 = 0, oB1 = kH * C, oW2 = oB1 + kH, oB2 = oW2 + kH * kH, oW3 = oB2 + kH, oB3 = oW3 + kH;
+    auto put = [&](int i, float v) { out[i] = kAdd ? out[i] + v : v; };
 #pragma unroll
     for (int rb = 0; rb < 4; ++rb)
 #pragma unroll
         for (int r4 = 0; r4 < 4; ++r4)
-            if (j < C) out[oW1 + (16 * rb + 4 * g + r4) * C + j] = q.W1[rb][r4];
+            if (j < C) put(oW1 + (16 * rb + 4 * g + r4) * C + j, q.W1[rb][r4]);
 #pragma unroll
     for (int rb2 = 0; rb2 < 4; ++rb2)
 #pragma unroll
         for (int rb = 0; rb < 4; ++rb)
 #pragma unroll
-            for (int r4 = 0; r4 < 4; ++r4) out[oW2 + (16 * rb2 + 4 * g + r4) * kH + 16 * rb + j] = q.W2[rb2][rb][r4];
+            for (int r4 = 0; r4 < 4; ++r4) put(oW2 + (16 * rb2 + 4 * g + r4) * kH + 16 * rb + j, q.W2[rb2][rb][r4]);
     const int row = 16 * (j >> 2) + 4 * g + (j & 3);
-    out[oB1 + row] = q.B1;
-    out[oB2 + row] = q.B2;
-    out[oW3 + row] = q.W3;
+    put(oB1 + row, q.B1);
+    put(oB2 + row, q.B2);
+    put(oW3 + row, q.W3);
     float b3 = q.B3;
 #pragma unroll
     for (int off = 1; off < 16; off <<= 1) b3 += __shfl_xor(b3, off, 64);
-    if (lane == 0) out[oB3] = b3;
+    if (lane == 0) put(oB3, b3);
 }
 
 // ---------------------------------------------------------------------------
@@ -1180,6 +1229,15 @@ struct NodeBwdArgs {
     int* counter2;            // decoder-pass item counter
     int* counter3;            // scatter-pass chunk counter
     int lds_pl_off;           // byte offset of the decoder pass's per-wave sample tables
+    // tensorial field in the two-pass form: slots [0, exp_slots) are explicit
+    // grids; the in-ROI samples (sorted after every explicit one, from
+    // soff[slot_off[exp_slots]]) are staged in records of their own
+    // (vm_record_f4) after the explicit region and its padding
+    const int64_t* slot_off;  // [num_slots + 1] first sorted interval of each slot
+    int64_t exp_slots;
+    int* counter4;            // tensorial plane-pass chunk counter
+    int* counter5;            // tensorial decoder-pass item counter
+    float* line_rows;         // line partial rows of the tensorial line pass
 };
 
 template <int C, bool kDecBF16>
@@ -1214,11 +1272,12 @@ __device__ __forceinline__ void write_record(const RecordSrc& rs, int64_t i, int
     rs.rec[k * 3 + 2] = make_float4(rs.D[r * 3 + 1], rs.D[r * 3 + 2], __int_as_float((int)i), __int_as_float((int)r));
 }
 
-// make_sample_iv for a node whose box and grid offset are already known.
+// make_sample_iv for a node whose box and grid offset are already known
+// (with_tri false: an in-ROI sample of a tensorial field, no grid corners).
 template <bool kVM = false>
 __device__ __forceinline__ void make_sample_node(const ProjArgs& a, const PNode& nd, int64_t gbase, int64_t r,
                                                  const float* o, const float* d, float tmin, float tmax, int n,
-                                                 int jj, int s_local, SampleCtx& sc)
+                                                 int jj, int s_local, SampleCtx& sc, bool with_tri = true)
 {
     const float step = (tmax - tmin) / (float)n;
     float aj = 0.5f, ap = 0.5f, an = 0.5f;
@@ -1244,10 +1303,11 @@ __device__ __forceinline__ void make_sample_node(const ProjArgs& a, const PNode&
             if (k == 2) sc.gz = pos;
         }
     }
+    sc.gbase = gbase;
+    if (!with_tri) return;
 #pragma unroll
     for (int k = 0; k < 3; ++k) sc.li[k] = tri_index(loc[k], a.G);
     tri_from_index(sc.li[0], sc.li[1], sc.li[2], a.G, a.G, a.G, sc.tri);
-    sc.gbase = gbase;
 }
 
 // Software pipeline of the adjoint's tile loop.  The node gradient fills
@@ -1930,7 +1990,49 @@ __host__ __device__ inline int parts_of(int n_iv, int64_t samples)
     return (n_iv + per - 1) / per;
 }
 
-template <int C, bool kBF16, bool kDecBF16>
+// Staged records of an in-ROI sample of a tensorial field (float4 each):
+// a plane record {local x, y, z, 0} + dL/dplane_p / w = dX * L_p for the
+// planes yz, xz, xy (C floats each), in node-sorted order, and a line record
+// {ROI-normalised x, local y, global z, 0} + dL/dline_p / w = dX * P_p for
+// the lines x, y, z, in the line pass's interleaved order: with n in-ROI
+// samples and span = ceil(n / 64), sample v sits at (v mod span) 64 +
+// v / span, so that the 64 lanes of a line-pass step (lane l walking
+// samples [l span, (l + 1) span)) read 64 consecutive records.
+template <int C>
+__host__ __device__ constexpr int vm_record_f4()
+{
+    return 1 + 3 * C / 4;
+}
+
+struct VMStage {
+    float4* plane;  // [n + kStagePad] plane records
+    float4* line;   // [64 span + 2 kStagePad] line records
+    int64_t s_vm0;  // first in-ROI sample in the sorted order
+    int64_t n, span;
+};
+
+// The in-ROI regions follow the explicit records (and their padding); the
+// line region follows the plane region (and its padding).
+template <int C>
+__device__ __forceinline__ VMStage vm_stage(const NodeBwdArgs& b)
+{
+    VMStage v;
+    v.s_vm0 = b.soff[b.slot_off[b.exp_slots]];
+    v.n = b.soff[b.slot_off[b.num_slots]] - v.s_vm0;
+    v.span = (v.n + 63) / 64;
+    v.plane = b.stage + (v.s_vm0 + kStagePad) * (1 + C / 4);
+    v.line = v.plane + (v.n + kStagePad) * vm_record_f4<C>();
+    return v;
+}
+
+// (32-bit division: a list holds fewer than 2^31 samples)
+__device__ __forceinline__ int64_t vm_line_pos(int64_t v, int64_t span)
+{
+    const unsigned q = (unsigned)v / (unsigned)span, r = (unsigned)v - q * (unsigned)span;
+    return (int64_t)r * 64 + q;
+}
+
+template <int C, bool kBF16, bool kDecBF16, bool kVM = false>
 __global__ __launch_bounds__(64 * kDecWaves, 2) void project_bwd_dec_kernel(NodeBwdArgs b)
 {
     extern __shared__ __attribute__((aligned(16))) unsigned char dsm[];
@@ -1940,6 +2042,7 @@ __global__ __launch_bounds__(64 * kDecWaves, 2) void project_bwd_dec_kernel(Node
     constexpr int CPL = C / 4;
     constexpr int NP = mlp_numel<C>();
     constexpr int RF = 1 + C / 4;
+    constexpr int RFV = vm_record_f4<C>();
     const int G = a.G;
     const int gsz = G * G * G * C;
     float* s = reinterpret_cast<float*>(dsm) + b.lds_f32_off;
@@ -1955,22 +2058,32 @@ __global__ __launch_bounds__(64 * kDecWaves, 2) void project_bwd_dec_kernel(Node
         load_mlp_bf16_to_lds<C>(a.mlp, sb, s);
     else
         load_mlp_to_lds_t<C>(a.mlp, s);
+    VMStage vs{};
+    if constexpr (kVM) vs = vm_stage<C>(b);
     __syncthreads();
 
     DecGradsRS q;
     zero_grads(q);
-    const int64_t n_items = b.part_off[b.max_chunks];
+    // the explicit-grid items come first in the part table, the in-ROI
+    // items of a tensorial field after them; each instance takes its own
+    const int64_t split_item = b.part_off[b.chunk_off[b.exp_slots]];
+    const int64_t i0 = kVM ? split_item : 0;
+    const int64_t i1 = kVM ? b.part_off[b.max_chunks] : split_item;
+    int* const counter = kVM ? b.counter5 : b.counter2;
     while (true) {
         int item = 0;
-        if (lane == 0) item = atomicAdd(b.counter2, 1);
+        if (lane == 0) item = (int)i0 + atomicAdd(counter, 1);
         item = __builtin_amdgcn_readfirstlane(__shfl(item, 0, 64));
-        if (item >= n_items) break;
+        if (item >= i1) break;
         const int4 it = b.parts[item];
         const PNode nd = a.nodes[a.iv_node[b.sorted[it.y]]];
-        const int64_t gbase = (int64_t)it.x * gsz;
+        constexpr bool vmc = kVM;
+        const int pslot = vmc ? it.x - a.vm.NG : -1;
+        const int64_t gbase = vmc ? 0 : (int64_t)it.x * gsz;
         const int ib = it.y, ie = it.z;
         const int64_t s_base = b.soff[ib];
         const int tot = (int)(b.soff[ie] - s_base);
+        const int64_t v_base = s_base - vs.s_vm0;  // in-ROI index of the item's first sample
 
         // 64 samples per step.  (A) lane l maps and places sample l of the
         // step once (the interval search, jitter hashes, divisions and
@@ -1978,7 +2091,10 @@ __global__ __launch_bounds__(64 * kDecWaves, 2) void project_bwd_dec_kernel(Node
         // lane rows of a tile), writes its lattice coordinates to the stage
         // and its packed corners, weights and dL/dP * w to the wave's sample
         // table in LDS; (B) the step's four tiles of 16 read the table, gather,
-        // run the decoder and its adjoint and stage dX.
+        // run the decoder and its adjoint and stage dX.  An in-ROI sample of a
+        // tensorial field keeps its local and global coordinates in the table
+        // instead, gathers its planes and lines, and stages dX * L_p and
+        // dX * P_p (the plane and line gradients before the corner weights).
         float4* pl = reinterpret_cast<float4*>(dsm + b.lds_pl_off) + wl * 64 * 3;
         TileCursor tc;
         tc.cur = ib;
@@ -1995,23 +2111,99 @@ __global__ __launch_bounds__(64 * kDecWaves, 2) void project_bwd_dec_kernel(Node
                 float dpw = 0.f;
                 SampleCtx sc;
                 sc.valid = m.valid;
-                if (m.valid) {
-                    const float o[3] = {m.B.x, m.B.y, m.B.z};
-                    const float d[3] = {m.B.w, m.D.x, m.D.y};
-                    const int r = __float_as_int(m.D.w);
-                    make_sample_node(a, nd, gbase, r, o, d, m.A.x, m.A.y, m.n_s, m.jj,
-                                     __float_as_int(m.A.w) + m.jj, sc);
-                    dpw = b.dLdP[r] * sc.w;
-                    packed = sc.tri.x0 | (sc.tri.x1 << 5) | (sc.tri.y0 << 10) | (sc.tri.y1 << 15) |
-                             (sc.tri.z0 << 20) | (sc.tri.z1 << 25);
-                    b.stage[(s_base + base + lane) * RF] = make_float4(sc.li[0], sc.li[1], sc.li[2], 0.f);
+                if (kVM && vmc) {
+                    sc.loc[0] = sc.loc[1] = sc.loc[2] = sc.gx = sc.gz = 0.f;
+                    if (m.valid) {
+                        const float o[3] = {m.B.x, m.B.y, m.B.z};
+                        const float d[3] = {m.B.w, m.D.x, m.D.y};
+                        const int r = __float_as_int(m.D.w);
+                        make_sample_node<true>(a, nd, 0, r, o, d, m.A.x, m.A.y, m.n_s, m.jj,
+                                               __float_as_int(m.A.w) + m.jj, sc, false);
+                        dpw = b.dLdP[r] * sc.w;
+                        // (the line x coordinate as vm_setup forms it)
+                        const float xr = (2.0f * (sc.gx - a.vm.rx0)) / (a.vm.rx1 - a.vm.rx0) - 1.0f;
+                        const int64_t v = v_base + base + lane;
+                        vs.plane[v * RFV] = make_float4(sc.loc[0], sc.loc[1], sc.loc[2], 0.f);
+                        vs.line[vm_line_pos(v, vs.span) * RFV] = make_float4(xr, sc.loc[1], sc.gz, 0.f);
+                    }
+                    pl[lane * 3] = make_float4(sc.loc[0], sc.loc[1], sc.loc[2], sc.gx);
+                    pl[lane * 3 + 1] = make_float4(sc.gz, dpw, m.valid ? 1.f : 0.f, 0.f);
+                } else {
+                    if (m.valid) {
+                        const float o[3] = {m.B.x, m.B.y, m.B.z};
+                        const float d[3] = {m.B.w, m.D.x, m.D.y};
+                        const int r = __float_as_int(m.D.w);
+                        make_sample_node(a, nd, gbase, r, o, d, m.A.x, m.A.y, m.n_s, m.jj,
+                                         __float_as_int(m.A.w) + m.jj, sc);
+                        dpw = b.dLdP[r] * sc.w;
+                        packed = sc.tri.x0 | (sc.tri.x1 << 5) | (sc.tri.y0 << 10) | (sc.tri.y1 << 15) |
+                                 (sc.tri.z0 << 20) | (sc.tri.z1 << 25);
+                        b.stage[(s_base + base + lane) * RF] = make_float4(sc.li[0], sc.li[1], sc.li[2], 0.f);
+                    }
+                    pl[lane * 3] = make_float4(__int_as_float(packed), sc.tri.w[0], sc.tri.w[1], sc.tri.w[2]);
+                    pl[lane * 3 + 1] = make_float4(sc.tri.w[3], sc.tri.w[4], sc.tri.w[5], sc.tri.w[6]);
+                    pl[lane * 3 + 2] = make_float4(sc.tri.w[7], dpw, 0.f, 0.f);
                 }
-                pl[lane * 3] = make_float4(__int_as_float(packed), sc.tri.w[0], sc.tri.w[1], sc.tri.w[2]);
-                pl[lane * 3 + 1] = make_float4(sc.tri.w[3], sc.tri.w[4], sc.tri.w[5], sc.tri.w[6]);
-                pl[lane * 3 + 2] = make_float4(sc.tri.w[7], dpw, 0.f, 0.f);
             }
             wave_lds_sync();
             const int ntile = min(4, (tot - base + kTile - 1) / kTile);
+            if (kVM && vmc) {
+                for (int q4 = 0; q4 < ntile; ++q4) {
+                    const int e = q4 * kTile + j;
+                    SampleCtx sc;
+                    float dpw;
+                    {
+                        const float4 p0 = pl[e * 3], p1 = pl[e * 3 + 1];
+                        sc.valid = p1.z != 0.f;
+                        sc.loc[0] = p0.x;
+                        sc.loc[1] = p0.y;
+                        sc.loc[2] = p0.z;
+                        sc.gx = p0.w;
+                        sc.gz = p1.x;
+                        dpw = p1.y;
+                    }
+                    float x[CPL];
+#pragma unroll
+                    for (int qq = 0; qq < CPL; ++qq) x[qq] = 0.f;
+                    if (sc.valid) {
+                        // F = Pyz Lx + Pxz Ly + Pxy Lz, one pair at a time
+#pragma unroll
+                        for (int pp = 0; pp < 3; ++pp) {
+                            float Pv[CPL], Lv[CPL];
+                            vm_pair_read<C, CPL>(a, pslot, sc, pp, CPL * g, Pv, Lv);
+#pragma unroll
+                            for (int qq = 0; qq < CPL; ++qq) x[qq] = pp == 0 ? Pv[qq] * Lv[qq] : x[qq] + Pv[qq] * Lv[qq];
+                        }
+                    }
+                    v4f h1[4], h2[4];
+                    float y;
+                    if constexpr (kDecBF16)
+                        y = decoder_fwd_bf16<C>(sb, s, x, g, j, h1, h2);
+                    else
+                        y = decoder_fwd<C>(s, x, g, j, h1, h2);
+                    const float dy = sc.valid ? dpw * softplus_grad_f(y, a.beta) : 0.f;
+                    const v4f dx = tile_backward<C, kDecBF16, DecGradsRS, !kDecBF16>(s, sb, bufA, bufB, bufAb, bufBb,
+                                                                                    x, h1, h2, dy, g, j, q);
+                    if (sc.valid && 4 * g < C) {
+                        // the plane and line reads again (cache hits) rather
+                        // than kept live through the decoder
+                        const int64_t v = v_base + base + e;
+                        float4* rp = vs.plane + v * RFV + 1;
+                        float4* rl = vs.line + vm_line_pos(v, vs.span) * RFV + 1;
+#pragma unroll
+                        for (int pp = 0; pp < 3; ++pp) {
+                            float P4[4], L4[4];
+                            vm_pair_read<C, 4>(a, pslot, sc, pp, 4 * g, P4, L4);
+                            rp[pp * (C / 4) + g] =
+                                make_float4(dx[0] * L4[0], dx[1] * L4[1], dx[2] * L4[2], dx[3] * L4[3]);
+                            rl[pp * (C / 4) + g] =
+                                make_float4(dx[0] * P4[0], dx[1] * P4[1], dx[2] * P4[2], dx[3] * P4[3]);
+                        }
+                    }
+                }
+                wave_lds_sync();
+                continue;
+            }
             for (int q4 = 0; q4 < ntile; ++q4) {
                 const int e = q4 * kTile + j;
                 const float4 p0 = pl[e * 3], p1 = pl[e * 3 + 1], p2 = pl[e * 3 + 2];
@@ -2058,7 +2250,8 @@ __global__ __launch_bounds__(64 * kDecWaves, 2) void project_bwd_dec_kernel(Node
             wave_lds_sync();
         }
     }
-    store_partials<C>(q, b.ws2 + ((int64_t)blockIdx.x * kDecWaves + wl) * NP, g, j, lane);
+    // the tensorial instance runs after the explicit one and adds to its rows
+    store_partials<C, kVM>(q, b.ws2 + ((int64_t)blockIdx.x * kDecWaves + wl) * NP, g, j, lane);
 }
 
 // Scatter pass.  Wave w of the workgroup owns channels 2w and 2w + 1 of the
@@ -2094,7 +2287,7 @@ __global__ __launch_bounds__(32 * C) void project_bwd_scatter_kernel(NodeBwdArgs
     const int c0 = 2 * wl;  // this wave's channel pair
     unsigned char* owner = dsm + (size_t)gsz * 4 + (size_t)wl * ncell;
     for (int i = threadIdx.x; i < gsz; i += blockDim.x) lgrad[i] = 0.f;
-    const int64_t total_chunks = b.chunk_off[b.num_slots];
+    const int64_t total_chunks = b.chunk_off[b.exp_slots];  // explicit-grid chunks come first
     const float* stagef = reinterpret_cast<const float*>(b.stage);
     __syncthreads();
     while (true) {
@@ -2171,6 +2364,220 @@ __global__ __launch_bounds__(32 * C) void project_bwd_scatter_kernel(NodeBwdArgs
         flush_node_grad<4, C>(b.ad, b.ad.enabled && !ch.w, ch.w != 0, lgrad, b.grad_grid + gbase, gbase, gsz >> 2);
         __syncthreads();
     }
+}
+
+// Scatter passes of the in-ROI samples of a tensorial field, in the
+// explicit scatter pass's scheme: wave w owns channels 2w and 2w + 1 (LDS
+// words no other wave touches) and each lane adds its own samples.  The
+// three planes (or lines) of a sample are three disjoint LDS regions, so a
+// step handles them together: per region a per-wave owner table (one byte
+// per cell) picks one lane per floor cell, the owners add corner k of all
+// three regions with plain float2 read-add-writes (corner after corner: a
+// corner of one lane can be another corner of a lane in an adjacent cell),
+// and the lanes that lost a claim, or whose corners are clamped together at
+// the border, add with LDS atomics afterwards.  Channel pairs are swizzled
+// as in the explicit pass (pair_swz of the cell index within the LDS array).
+// Records are loaded two steps ahead, without a branch (the staged regions
+// are padded).
+//
+// project_bwd_vm_plane_kernel: per in-ROI chunk, the node's three plane
+// gradients (3 Gp^2 C) in LDS, w_corner * (dX L_p) into 4 corners of each
+// plane; lane l walks the l-th 64th of the chunk's records; the planes are
+// added to HBM once per chunk.
+// project_bwd_vm_line_kernel: the three lines are shared by every node, so
+// this pass is not node-major: lane l walks the l-th 64th of the whole
+// in-ROI list (64 distant nodes per step: the x and z cells, which one node
+// confines to a narrow window of the line, spread over the line), reading
+// the interleaved line records, and each workgroup adds w_corner * (dX P_p)
+// into its own LDS partials of the lines, written to its row of line
+// partials at the end (rows summed in a fixed order afterwards).
+template <int C, int NK>
+__device__ __forceinline__ void vm_scatter3(float* lg, unsigned char* owner, const int (*cell)[NK],
+                                            const float (*w)[NK], const int* f0, const int* tsz, const float2* dv,
+                                            bool live, const bool* border, int c0, int lane)
+{
+    bool lv[3], mine[3];
+#pragma unroll
+    for (int p = 0; p < 3; ++p) {
+        lv[p] = live & ((dv[p].x != 0.f) | (dv[p].y != 0.f));
+        if (lv[p] && !border[p]) owner[tsz[p] + f0[p]] = (unsigned char)lane;
+    }
+    wave_lds_sync();
+#pragma unroll
+    for (int p = 0; p < 3; ++p) mine[p] = lv[p] && !border[p] && owner[tsz[p] + f0[p]] == lane;
+#pragma unroll
+    for (int k = 0; k < NK; ++k) {
+        float2 v[3];
+#pragma unroll
+        for (int p = 0; p < 3; ++p)
+            if (mine[p]) v[p] = *reinterpret_cast<float2*>(lg + cell[p][k] * C + (c0 ^ pair_swz(cell[p][k], C)));
+#pragma unroll
+        for (int p = 0; p < 3; ++p)
+            if (mine[p]) {
+                v[p].x = fmaf(w[p][k], dv[p].x, v[p].x);
+                v[p].y = fmaf(w[p][k], dv[p].y, v[p].y);
+                *reinterpret_cast<float2*>(lg + cell[p][k] * C + (c0 ^ pair_swz(cell[p][k], C))) = v[p];
+            }
+    }
+    wave_lds_sync();
+#pragma unroll
+    for (int p = 0; p < 3; ++p)
+        if (lv[p] && !mine[p]) {
+#pragma unroll
+            for (int k = 0; k < NK; ++k) {
+                float* gp = lg + cell[p][k] * C + (c0 ^ pair_swz(cell[p][k], C));
+                atomicAdd(gp, w[p][k] * dv[p].x);
+                atomicAdd(gp + 1, w[p][k] * dv[p].y);
+            }
+        }
+    wave_lds_sync();
+}
+
+template <int C>
+__global__ __launch_bounds__(32 * C) void project_bwd_vm_plane_kernel(NodeBwdArgs b)
+{
+    extern __shared__ __attribute__((aligned(16))) unsigned char dsm[];
+    __shared__ int s_chunk;
+    if (*b.mode == 0) return;
+    constexpr int RFV = vm_record_f4<C>();
+    const ProjArgs& a = b.a;
+    const int Gp = a.vm.Gp;
+    const int pc = Gp * Gp;
+    const int pf = 3 * pc * C;
+    float* lpl = reinterpret_cast<float*>(dsm);
+    const int lane = threadIdx.x & 63, wl = threadIdx.x >> 6;
+    const int c0 = 2 * wl;
+    unsigned char* owner = dsm + (size_t)pf * 4 + (size_t)wl * 3 * pc;
+    for (int i = threadIdx.x; i < pf; i += blockDim.x) lpl[i] = 0.f;
+    const int64_t first_chunk = b.chunk_off[b.exp_slots];
+    const int64_t total_chunks = b.chunk_off[b.num_slots];
+    const VMStage vs = vm_stage<C>(b);
+    const float* rec = reinterpret_cast<const float*>(vs.plane);
+    const int tsz[3] = {0, pc, 2 * pc};
+    __syncthreads();
+    while (true) {
+        if (threadIdx.x == 0) s_chunk = atomicAdd(b.counter4, 1);
+        __syncthreads();
+        const int64_t c = first_chunk + s_chunk;
+        __syncthreads();
+        if (c >= total_chunks) break;
+        const int4 ch = b.chunks[c];
+        const int pslot = ch.x - a.vm.NG;
+        const int64_t s0 = b.soff[ch.y] - vs.s_vm0, s1 = b.soff[ch.z] - vs.s_vm0;
+        const int64_t run = (s1 - s0 + 63) / 64;
+        const int64_t rb = s0 + lane * run;
+        const int64_t re = rb + run < s1 ? rb + run : s1;
+        // three register sets (the loop is unrolled by three): the records
+        // of the next two steps load while a step runs; a lane past its run
+        // reads a later record (of the next chunk, or the padding) and
+        // ignores it
+        float4 co[3];
+        float2 dq[3][3];
+        auto load = [&](int u, int64_t si) {
+            const float* r = rec + si * RFV * 4;
+            co[u] = *reinterpret_cast<const float4*>(r);
+#pragma unroll
+            for (int p = 0; p < 3; ++p) dq[u][p] = *reinterpret_cast<const float2*>(r + 4 + p * C + c0);
+        };
+        load(0, rb);
+        load(1, rb + 1);
+        for (int64_t st0 = 0; st0 < run; st0 += 3) {
+#pragma unroll
+            for (int u = 0; u < 3; ++u) {
+                const int64_t st = st0 + u;
+                if (st >= run) break;
+                const int64_t si = rb + st;
+                const float4 A = co[u];
+                float2 dv[3] = {dq[u][0], dq[u][1], dq[u][2]};
+                load((u + 2) % 3, si + 2);
+                const float uu[3] = {A.y, A.x, A.x}, vv[3] = {A.z, A.z, A.y};
+                int cell[3][4], f0[3];
+                float w[3][4];
+                bool border[3];
+#pragma unroll
+                for (int p = 0; p < 3; ++p) {
+                    Bil bb;
+                    bil_setup(uu[p], vv[p], Gp, Gp, bb);
+                    border[p] = bb.x0 == bb.x1 || bb.y0 == bb.y1;
+                    f0[p] = bb.y0 * Gp + bb.x0;
+                    cell[p][0] = p * pc + f0[p];
+                    cell[p][1] = p * pc + bb.y0 * Gp + bb.x1;
+                    cell[p][2] = p * pc + bb.y1 * Gp + bb.x0;
+                    cell[p][3] = p * pc + bb.y1 * Gp + bb.x1;
+#pragma unroll
+                    for (int k = 0; k < 4; ++k) w[p][k] = bb.w[k];
+                }
+                vm_scatter3<C, 4>(lpl, owner, cell, w, f0, tsz, dv, si < re, border, c0, lane);
+            }
+        }
+        __syncthreads();
+        flush_node_grad<4, C>(b.ad, false, ch.w != 0, lpl, b.grad_planes + (int64_t)pslot * pf, 0, pf >> 2);
+        __syncthreads();
+    }
+}
+
+template <int C>
+__global__ __launch_bounds__(32 * C) void project_bwd_vm_line_kernel(NodeBwdArgs b)
+{
+    extern __shared__ __attribute__((aligned(16))) unsigned char dsm[];
+    if (*b.mode == 0) return;  // the fused kernel writes the line partial rows
+    constexpr int RFV = vm_record_f4<C>();
+    const ProjArgs& a = b.a;
+    const int Gl = a.vm.Gl;
+    const int lf = 3 * Gl * C;
+    float* lln = reinterpret_cast<float*>(dsm);
+    const int lane = threadIdx.x & 63, wl = threadIdx.x >> 6;
+    const int c0 = 2 * wl;
+    unsigned char* owner = dsm + (size_t)lf * 4 + (size_t)wl * 3 * Gl;
+    for (int i = threadIdx.x; i < lf; i += blockDim.x) lln[i] = 0.f;
+    const VMStage vs = vm_stage<C>(b);
+    const float* rec = reinterpret_cast<const float*>(vs.line);
+    const int64_t span = vs.span;
+    const int64_t jb = span * blockIdx.x / gridDim.x, je = span * (blockIdx.x + 1) / gridDim.x;
+    const int tsz[3] = {0, Gl, 2 * Gl};
+    __syncthreads();
+    float4 co[3];
+    float2 dq[3][3];
+    // step jj: lane l's sample l span + jj at record jj 64 + l (the records
+    // past the list's end are padding, read and ignored)
+    auto load = [&](int u, int64_t jj) {
+        const float* r = rec + (jj * 64 + lane) * RFV * 4;
+        co[u] = *reinterpret_cast<const float4*>(r);
+#pragma unroll
+        for (int p = 0; p < 3; ++p) dq[u][p] = *reinterpret_cast<const float2*>(r + 4 + p * C + c0);
+    };
+    load(0, jb);
+    load(1, jb + 1);
+    for (int64_t j0 = jb; j0 < je; j0 += 3) {
+#pragma unroll
+        for (int u = 0; u < 3; ++u) {
+            const int64_t jj = j0 + u;
+            if (jj >= je) break;
+            const float4 A = co[u];
+            float2 dv[3] = {dq[u][0], dq[u][1], dq[u][2]};
+            load((u + 2) % 3, jj + 2);
+            const float uu[3] = {A.x, A.y, A.z};  // ROI-normalised x, local y, global z
+            int cell[3][2], f0[3];
+            float w[3][2];
+            bool border[3];
+#pragma unroll
+            for (int p = 0; p < 3; ++p) {
+                Lin ll;
+                lin_setup(uu[p], Gl, ll);
+                border[p] = ll.x0 == ll.x1;
+                f0[p] = ll.x0;
+                cell[p][0] = p * Gl + ll.x0;
+                cell[p][1] = p * Gl + ll.x1;
+                w[p][0] = ll.w0;
+                w[p][1] = ll.w1;
+            }
+            vm_scatter3<C, 2>(lln, owner, cell, w, f0, tsz, dv, lane * span + jj < vs.n, border, c0, lane);
+        }
+    }
+    __syncthreads();
+    // this workgroup's line partials (zero if it took no sample)
+    float* row = b.line_rows + (int64_t)blockIdx.x * lf;
+    for (int i = threadIdx.x; i < lf / 4; i += blockDim.x) reinterpret_cast<float4*>(row)[i] = take_lds4<C>(lln, i);
 }
 
 // ---------------------------------------------------------------------------
@@ -2756,12 +3163,31 @@ __global__ __launch_bounds__(256) void part_desc_kernel(const int4* __restrict__
 }
 
 // mode = 1 (two-pass kernels) when the sort prepared the staging tables and
-// every sample's staged record fits in the staging capacity, else 0.
+// every sample's staged record fits in the staging capacity, else 0.  With a
+// tensorial field (vm_rec_bytes > 0) the in-ROI samples, sorted after the
+// explicit ones from soff[slot_off[exp_slots]], take records of their own
+// after the explicit region and its padding.
 __global__ void stage_mode_kernel(const unsigned long long* __restrict__ stotal, const int* __restrict__ prepared,
-                                  unsigned long long capacity, int rec_bytes, int* __restrict__ mode)
+                                  unsigned long long capacity, int rec_bytes, const int64_t* __restrict__ soff,
+                                  const int64_t* __restrict__ slot_off, int64_t exp_slots, int vm_rec_bytes,
+                                  int* __restrict__ mode)
 {
-    if (threadIdx.x == 0)
-        *mode = (*prepared == 1 && (*stotal + kStagePad) * (unsigned long long)rec_bytes <= capacity) ? 1 : 0;
+    if (threadIdx.x != 0) return;
+    if (*prepared != 1) {
+        *mode = 0;
+        return;
+    }
+    const unsigned long long total = *stotal;
+    unsigned long long need = (total + kStagePad) * (unsigned long long)rec_bytes;
+    if (vm_rec_bytes > 0) {
+        // explicit records, then the in-ROI plane records and the line
+        // records (64 span of them), each region padded (vm_stage)
+        const unsigned long long s_exp = (unsigned long long)soff[slot_off[exp_slots]];
+        const unsigned long long n = total - s_exp, span = (n + 63) / 64;
+        need = (s_exp + kStagePad) * (unsigned long long)rec_bytes +
+               (n + kStagePad + 64 * span + 2 * kStagePad) * (unsigned long long)vm_rec_bytes;
+    }
+    *mode = need <= capacity ? 1 : 0;
 }
 
 // ---------------------------------------------------------------------------
@@ -3278,6 +3704,9 @@ int persistent_blocks()
     return n;
 }
 
+// workgroups (and line partial rows) of the tensorial line pass
+int vm_line_blocks() { return 2 * persistent_blocks(); }
+
 // line_floats: 3 Gl C of a tensorial field (per-workgroup line partial rows), else 0
 BwdWorkspace bwd_layout(int64_t H, int64_t NA, int np, int64_t line_floats = 0)
 {
@@ -3307,10 +3736,11 @@ int64_t max_chunks_of(int64_t H, int64_t NA) { return H / kMinChunk + NA + 1; }
 // counts and offsets, the decoder-pass work items, scan scratch, the
 // decoder-pass partial rows, then the staged samples (S x (1 + C/4) float4).
 struct StageLayout {
-    size_t scnt, soff, part_cnt, part_off, parts, scan, ws2, stage;
+    size_t scnt, soff, part_cnt, part_off, parts, scan, ws2, line_rows, stage;
 };
 
-StageLayout stage_layout(size_t base, int64_t H, int64_t NA, int np)
+// line_floats: 3 Gl C of a tensorial field (the line pass's partial rows), else 0
+StageLayout stage_layout(size_t base, int64_t H, int64_t NA, int np, int64_t line_floats)
 {
     StageLayout st{};
     size_t o = align256(base);
@@ -3323,6 +3753,7 @@ StageLayout stage_layout(size_t base, int64_t H, int64_t NA, int np)
     st.parts = take(sizeof(int4) * (size_t)(H / kMinPartIv + mc + 1));
     st.scan = take(atdf_scan_workspace_bytes(H > mc ? H : mc));
     st.ws2 = take(sizeof(float) * (size_t)persistent_blocks() * kDecWaves * np);
+    st.line_rows = take(sizeof(float) * (size_t)vm_line_blocks() * (size_t)line_floats);
     st.stage = o;
     return st;
 }
@@ -3389,8 +3820,18 @@ atdf_status launch_node_bwd(NodeBwdArgs args, hipStream_t s)
     return ATDF_OK;
 }
 
-// The two passes of the staged adjoint; both return at once unless *args.mode.
-template <int C, bool kBF16, bool kDecBF16>
+// LDS of the tensorial scatter passes: three plane gradients or the three
+// lines' partials, and a byte per cell for each of the C/2 waves' three
+// owner tables.
+size_t vm_plane_lds(int C, int Gp)
+{
+    return (size_t)3 * Gp * Gp * C * sizeof(float) + (size_t)(C / 2) * 3 * Gp * Gp;
+}
+size_t vm_line_lds(int C, int Gl) { return (size_t)3 * Gl * C * sizeof(float) + (size_t)(C / 2) * 3 * Gl; }
+
+
+// The passes of the staged adjoint; all return at once unless *args.mode.
+template <int C, bool kBF16, bool kDecBF16, bool kVM = false>
 atdf_status launch_two_pass(NodeBwdArgs args, hipStream_t s)
 {
     const int G = args.a.G;
@@ -3412,19 +3853,47 @@ atdf_status launch_two_pass(NodeBwdArgs args, hipStream_t s)
         set_error("atdf_project_bwd: the decoder pass needs %d bytes of LDS, the device allows %d", total, max_lds);
         return ATDF_ERR_UNSUPPORTED;
     }
-    hipLaunchKernelGGL((project_bwd_dec_kernel<C, kBF16, kDecBF16>), dim3(persistent_blocks()), dim3(64 * kDecWaves),
-                       total, s, args);
     const size_t scatter_lds = (size_t)grad_floats * 4 + (size_t)(C / 2) * G * G * G;
-    if (scatter_lds > (size_t)max_lds) {
+    const bool explicit_pass = args.exp_slots > 0;
+    if (explicit_pass && scatter_lds > (size_t)max_lds) {
         set_error("atdf_project_bwd: the scatter pass needs %zu bytes of LDS, the device allows %d", scatter_lds,
                   max_lds);
         return ATDF_ERR_UNSUPPORTED;
     }
-    hipLaunchKernelGGL((project_bwd_scatter_kernel<C>), dim3(persistent_blocks()), dim3(32 * C), scatter_lds, s,
-                       args);
+    const size_t plane_lds = kVM ? vm_plane_lds(C, args.a.vm.Gp) : 0;
+    const size_t line_lds = kVM ? vm_line_lds(C, args.a.vm.Gl) : 0;
+    if (plane_lds > (size_t)max_lds || line_lds > (size_t)max_lds) {
+        set_error("atdf_project_bwd: the tensorial scatter passes need %zu and %zu bytes of LDS, the device allows %d",
+                  plane_lds, line_lds, max_lds);
+        return ATDF_ERR_UNSUPPORTED;
+    }
+    hipLaunchKernelGGL((project_bwd_dec_kernel<C, kBF16, kDecBF16>), dim3(persistent_blocks()),
+                       dim3(64 * kDecWaves), total, s, args);
+    if constexpr (kVM)
+        hipLaunchKernelGGL((project_bwd_dec_kernel<C, kBF16, kDecBF16, true>), dim3(persistent_blocks()),
+                           dim3(64 * kDecWaves), total, s, args);
+    if (explicit_pass)
+        hipLaunchKernelGGL((project_bwd_scatter_kernel<C>), dim3(persistent_blocks()), dim3(32 * C), scatter_lds, s,
+                           args);
+    if constexpr (kVM) {
+        hipLaunchKernelGGL((project_bwd_vm_plane_kernel<C>), dim3(persistent_blocks() * 4), dim3(32 * C), plane_lds,
+                           s, args);
+        // one workgroup per line partial row
+        hipLaunchKernelGGL((project_bwd_vm_line_kernel<C>), dim3(vm_line_blocks()), dim3(32 * C), line_lds, s, args);
+    }
     return ATDF_OK;
 }
 }  // namespace
+
+// A tensorial field runs the two-pass adjoint when its explicit grids fit
+// the decoder pass's 5-bit corner packing and its plane and line gradients
+// fit the tensorial scatter pass's LDS.
+bool two_pass_vm_ok(const atdf_field* f)
+{
+    const size_t max_lds = (size_t)max_lds_per_block();
+    return f->vm && (f->num_grids == 0 || f->G <= 32) && vm_plane_lds(f->C, f->vm->Gp) <= max_lds &&
+           vm_line_lds(f->C, f->vm->Gl) <= max_lds;
+}
 
 extern "C" {
 
@@ -3440,11 +3909,17 @@ size_t atdf_project_bwd_workspace_bytes(int64_t R, int64_t H, const atdf_field* 
 
 size_t atdf_project_bwd_staging_bytes(int64_t H, int64_t S, const atdf_field* f)
 {
-    if (!f || H < 0 || S < 0 || f->num_grids <= 0 || f->vm || (f->C != 8 && f->C != 16) || f->G > 32) return 0;
-    const int64_t NA = f->num_grids;
+    if (!f || H < 0 || S < 0 || (f->C != 8 && f->C != 16)) return 0;
+    if (!(f->vm ? two_pass_vm_ok(f) : (f->num_grids > 0 && f->G <= 32))) return 0;
+    const int64_t NA = f->num_grids + (f->vm ? f->vm->num_planes : 0);
     const int np = (int)((int64_t)f->H * f->C + f->H + (int64_t)f->H * f->H + f->H + f->H + 1);
-    const size_t base = bwd_layout(H, NA, np, 0).total;
-    const StageLayout st = stage_layout(base, H, NA, np);
+    const int64_t lf = f->vm ? 3 * (int64_t)f->vm->Gl * f->C : 0;
+    const size_t base = bwd_layout(H, NA, np, lf).total;
+    const StageLayout st = stage_layout(base, H, NA, np, lf);
+    if (f->vm) {  // every sample at the in-ROI records' size (two), bounding any split
+        const size_t rf = (size_t)std::max(1 + f->C / 4, 2 * (f->C == 16 ? vm_record_f4<16>() : vm_record_f4<8>()));
+        return st.stage - base + (size_t)(S + 4 * kStagePad) * rf * sizeof(float4);
+    }
     return st.stage - base + (size_t)(S + kStagePad) * (1 + f->C / 4) * sizeof(float4);
 }
 
@@ -3458,10 +3933,12 @@ struct Prepared {
     StageLayout SL;
     unsigned char* w;
     size_t ws_bytes;
-    bool staging;  // the workspace holds the two-pass region (explicit grids only)
+    bool staging;  // the workspace holds the two-pass region
     int np;
     int64_t H, NA, line_floats;
-    int* counters() const { return reinterpret_cast<int*>(w + L.counter); }  // fused, dec, scatter, mode, prepared
+    // ints: fused, decoder-pass, scatter-pass counters, mode, prepared flag,
+    // forward counter, tensorial scatter-pass and decoder-pass counters
+    int* counters() const { return reinterpret_cast<int*>(w + L.counter); }
     int32_t* chunk_cnt() const { return reinterpret_cast<int32_t*>(w + L.chunk_cnt); }
     int32_t* sorted() const { return reinterpret_cast<int32_t*>(w + L.sorted); }
     int64_t* chunk_off() const { return reinterpret_cast<int64_t*>(w + L.chunk_off); }
@@ -3483,9 +3960,9 @@ static atdf_status prepared_view(const atdf_intervals* iv, const atdf_field* f, 
     P.L = bwd_layout(P.H, P.NA, P.np, P.line_floats);
     P.w = reinterpret_cast<unsigned char*>(workspace);
     P.ws_bytes = workspace_bytes;
-    P.SL = stage_layout(P.L.total, P.H, P.NA, P.np);
+    P.SL = stage_layout(P.L.total, P.H, P.NA, P.np, P.line_floats);
     // (the decoder pass packs a sample's corner indices in 5 bits each: G <= 32)
-    P.staging = !f->vm && (f->C == 8 || f->C == 16) && f->G <= 32 && f->num_grids > 0 &&
+    P.staging = (f->C == 8 || f->C == 16) && (f->vm ? two_pass_vm_ok(f) : (f->G <= 32 && f->num_grids > 0)) &&
                 workspace_bytes > P.SL.stage;
     return ATDF_OK;
 }
@@ -3614,14 +4091,22 @@ static atdf_status bwd_node_impl(const Prepared& P, const void* packed_tree, con
     }
     int* mode = counter + 3;
     const int rf_bytes = (1 + f->C / 4) * (int)sizeof(float4);
+    const int vm_rf_bytes =
+        f->vm ? (f->C == 16 ? vm_record_f4<16>() : vm_record_f4<8>()) * (int)sizeof(float4) : 0;
+    args.exp_slots = f->num_grids;
     if (P.staging) {
         const unsigned char* w = P.w;
         const unsigned long long cap = (unsigned long long)(P.ws_bytes - P.SL.stage);
+        args.slot_off = reinterpret_cast<const int64_t*>(w + P.L.slot_off);
+        args.soff = reinterpret_cast<const int64_t*>(w + P.SL.soff);
+        args.counter4 = counter + 6;
+        args.counter5 = counter + 7;
+        args.line_rows = reinterpret_cast<float*>(P.w + P.SL.line_rows);
+        if (f->vm) hipLaunchKernelGGL(fill_i32_kernel, dim3(1), dim3(64), 0, s, args.counter4, 2, 0);
         hipLaunchKernelGGL(stage_mode_kernel, dim3(1), dim3(64), 0, s,
                            reinterpret_cast<const unsigned long long*>(w + P.L.stotal), counter + 4, cap, rf_bytes,
-                           mode);
+                           args.soff, args.slot_off, args.exp_slots, vm_rf_bytes, mode);
         args.mode = mode;
-        args.soff = reinterpret_cast<const int64_t*>(w + P.SL.soff);
         args.parts = reinterpret_cast<const int4*>(w + P.SL.parts);
         args.part_off = reinterpret_cast<const int64_t*>(w + P.SL.part_off);
         args.max_chunks = max_chunks_of(P.H, NA);
@@ -3630,7 +4115,15 @@ static atdf_status bwd_node_impl(const Prepared& P, const void* packed_tree, con
         args.counter2 = counter + 1;
         args.counter3 = counter + 2;
 #define ATDF_TP(CC, BF, BD) st = launch_two_pass<CC, BF, BD>(args, s)
-        if (f->C == 16) {
+        if (f->vm) {
+            if (f->C == 16) {
+                if (bd) st = launch_two_pass<16, false, true, true>(args, s);
+                else st = launch_two_pass<16, false, false, true>(args, s);
+            } else {
+                if (bd) st = launch_two_pass<8, false, true, true>(args, s);
+                else st = launch_two_pass<8, false, false, true>(args, s);
+            }
+        } else if (f->C == 16) {
             if (bf) { if (bd) ATDF_TP(16, true, true); else ATDF_TP(16, true, false); }
             else    { if (bd) ATDF_TP(16, false, true); else ATDF_TP(16, false, false); }
         } else {
@@ -3674,8 +4167,13 @@ static atdf_status bwd_node_impl(const Prepared& P, const void* packed_tree, con
                            partials, (int64_t)persistent_blocks() * waves, np, grad_mlp);
     if (f->vm) {
         const int lf = (int)P.line_floats;
-        hipLaunchKernelGGL(reduce_partials_kernel, dim3((unsigned)div_up(lf, kReduceParams)), dim3(256), 0, s,
-                           args.line_part, (int64_t)persistent_blocks(), lf, f->vm->grad_lines);
+        if (P.staging)
+            hipLaunchKernelGGL(reduce_partials_sel_kernel, dim3((unsigned)div_up(lf, kReduceParams)), dim3(256), 0, s,
+                               args.line_part, (int64_t)persistent_blocks(), args.line_rows,
+                               (int64_t)vm_line_blocks(), mode, lf, f->vm->grad_lines);
+        else
+            hipLaunchKernelGGL(reduce_partials_kernel, dim3((unsigned)div_up(lf, kReduceParams)), dim3(256), 0, s,
+                               args.line_part, (int64_t)persistent_blocks(), lf, f->vm->grad_lines);
     }
     return check_launch("atdf_project_bwd");
 }

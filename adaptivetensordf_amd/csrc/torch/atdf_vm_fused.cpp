@@ -123,6 +123,15 @@ void make_vm_view(const HyperTreeBase& tree, const RoiMaps& roi, const Tensor& p
     v.f.vm = &v.vm;
 }
 
+// The adjoint's workspace, with the staging region of its two-pass form
+// when asked for (atdf_project_bwd picks the two-pass kernels when it fits).
+int64_t vm_workspace_bytes(int64_t R, const Intervals& iv, const atdf_field& f, bool two_pass)
+{
+    size_t n = atdf_project_bwd_workspace_bytes(R, iv.num_intervals, &f);
+    if (two_pass) n += atdf_project_bwd_staging_bytes(iv.num_intervals, iv.num_samples, &f);
+    return (int64_t)n;
+}
+
 struct ProjectVMFn : public torch::autograd::Function<ProjectVMFn> {
     static Tensor forward(torch::autograd::AutogradContext* ctx, Tensor yz, Tensor xz, Tensor xy, Tensor lx,
                           Tensor ly, Tensor lz, Tensor grid_out, Tensor mlp, Tensor origin, Tensor direction,
@@ -159,6 +168,8 @@ struct ProjectVMFn : public torch::autograd::Function<ProjectVMFn> {
         ctx->saved_data["rpw"] = (int64_t)fp.rays_per_wave;
         ctx->saved_data["bfdec"] = (bool)fp.bf16_decoder;
         ctx->saved_data["H_iv"] = iv.num_intervals;
+        ctx->saved_data["S_iv"] = iv.num_samples;
+        ctx->saved_data["twopass"] = (bool)fp.two_pass_adjoint;
         ctx->saved_data["rx0"] = rx0;
         ctx->saved_data["rx1"] = rx1;
         ctx->saved_data["has_grid"] = grid.defined();
@@ -181,6 +192,7 @@ struct ProjectVMFn : public torch::autograd::Function<ProjectVMFn> {
         iv.first = sv[11];
         iv.ray = sv[12];
         iv.num_intervals = ctx->saved_data["H_iv"].toInt();
+        iv.num_samples = ctx->saved_data["S_iv"].toInt();
         auto* tree = reinterpret_cast<HyperTreeBase*>((intptr_t)ctx->saved_data["tree"].toInt());
         auto* roi = reinterpret_cast<const RoiMaps*>((intptr_t)ctx->saved_data["roi"].toInt());
         SamplingParams sp;
@@ -195,6 +207,7 @@ struct ProjectVMFn : public torch::autograd::Function<ProjectVMFn> {
         fp.softplus_beta = (float)ctx->saved_data["beta"].toDouble();
         fp.rays_per_wave = (int)ctx->saved_data["rpw"].toInt();
         fp.bf16_decoder = ctx->saved_data["bfdec"].toBool();
+        fp.two_pass_adjoint = ctx->saved_data["twopass"].toBool();
         VMView v;
         make_vm_view(*tree, *roi, planes, lines, grid, mlp, fp, (float)ctx->saved_data["rx0"].toDouble(),
                      (float)ctx->saved_data["rx1"].toDouble(), v);
@@ -208,7 +221,7 @@ struct ProjectVMFn : public torch::autograd::Function<ProjectVMFn> {
         atdf_intervals av = iv.view();
         auto dP = grad_output[0].to(torch::kFloat32).contiguous();
         const int64_t R = origin.size(0);
-        auto ws = torch::empty({(int64_t)atdf_project_bwd_workspace_bytes(R, iv.num_intervals, &v.f)},
+        auto ws = torch::empty({vm_workspace_bytes(R, iv, v.f, fp.two_pass_adjoint)},
                                torch::TensorOptions().dtype(torch::kUInt8).device(mlp.device()));
         check_status(atdf_project_bwd(tree->packed(), tree->NumNodes(), origin.data_ptr<float>(),
                                       direction.data_ptr<float>(), R, &s, iv.ray_offset.data_ptr<int64_t>(), &av,
@@ -306,7 +319,7 @@ Tensor TrainStepVM(HyperTreeBase& tree, const RoiMaps& roi, VMTrainState& st, co
     // each Adam pass below clears its gradient (zero_grad)
     v.vm.grad_planes = st.grads[0].data_ptr<float>();
     v.vm.grad_lines = st.grads[1].data_ptr<float>();
-    auto ws = torch::empty({(int64_t)atdf_project_bwd_workspace_bytes(R, iv.num_intervals, &v.f)},
+    auto ws = torch::empty({vm_workspace_bytes(R, iv, v.f, fp.two_pass_adjoint)},
                            torch::TensorOptions().dtype(torch::kUInt8).device(rays.origin.device()));
     check_status(atdf_project_bwd(tree.packed(), tree.NumNodes(), O, D, R, &s, ivo, &av, &v.f, dP.data_ptr<float>(),
                                   st.grid.size(0) > 0 ? st.grads[2].data_ptr<float>() : nullptr,

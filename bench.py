@@ -920,7 +920,8 @@ def vm_leg(N, tree, batches, K, s_eff, n, args, steps):
                    "loss_last": {"fused": loss_f, "composed": loss_c},
                    "composed": {"ms_per_step": el_c * 1e3 / steps, "rays_per_s": R * steps / el_c},
                    "speedup_fused_over_composed": el_c / el_f, "kernel_ms": kern["kernel_ms"],
-                   "samples_in_roi": kern["samples_in_roi"], "samples_out_roi": kern["samples_out_roi"]},
+                   "samples_in_roi": kern["samples_in_roi"], "samples_out_roi": kern["samples_out_roi"],
+                   "adjoint_form": kern["adjoint_form"]},
         "roofline": kern["roofline"], "cpu_baseline": None}), flush=True)
 
 
@@ -956,8 +957,10 @@ def vm_kernel_times(N, tree, roi, st, rays, sp, fp):
     s = capi.stream_ptr()
     P = torch.empty(R, device=rays.origin.device)
     dP = torch.randn(R, device=rays.origin.device) * 1e-6
-    ws = torch.empty(capi.lib().atdf_project_bwd_workspace_bytes(R, iv.num_intervals, Ct.byref(f)),
-                     dtype=torch.uint8, device=rays.origin.device)
+    ws_bytes = capi.lib().atdf_project_bwd_workspace_bytes(R, iv.num_intervals, Ct.byref(f))
+    staging = capi.lib().atdf_project_bwd_staging_bytes(iv.num_intervals, iv.num_samples, Ct.byref(f)) \
+        if fp.two_pass_adjoint else 0
+    ws = torch.empty(ws_bytes + staging, dtype=torch.uint8, device=rays.origin.device)
     a = (capi.ptr(packed), tree.NumNodes(), capi.ptr(rays.origin), capi.ptr(rays.direction), R, Ct.byref(csp),
          capi.ptr(iv.ray_offset), Ct.byref(civ), Ct.byref(f))
 
@@ -984,10 +987,12 @@ def vm_kernel_times(N, tree, roi, st, rays, sp, fp):
     t_bwd = timed(bwd)
     bwd_bytes = s_in * 2 * 72 * C + s_out * 2 * 8 * C * 4 + R * 28
     gbs = bwd_bytes / t_bwd / 1e9
-    log(f"  vm project_fwd {t_fwd * 1e3:.2f} ms, project_bwd {t_bwd * 1e3:.2f} ms (counting sort included; "
+    log(f"  vm project_fwd {t_fwd * 1e3:.2f} ms, project_bwd {t_bwd * 1e3:.2f} ms ("
+        f"{'two-pass' if staging else 'fused'}, counting sort included; "
         f"{s_in} in-ROI + {s_out} explicit samples): {gbs:.0f} GB/s algorithmic")
     return {"kernel_ms": {"project_fwd": t_fwd * 1e3, "project_bwd": t_bwd * 1e3}, "samples_in_roi": s_in,
             "samples_out_roi": s_out,
+            "adjoint_form": "two-pass" if staging else "fused",
             "roofline": {"bound": "hbm", "kernel": "project_bwd (tensorial)", "achieved": gbs, "peak": HBM_PEAK_GBS,
                          "unit": "GB/s", "frac": gbs / HBM_PEAK_GBS, "traffic": None,
                          "algorithmic_bytes_per_launch": bwd_bytes}}

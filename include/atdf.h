@@ -343,8 +343,13 @@ size_t atdf_project_bwd_workspace_bytes(int64_t num_rays, int64_t num_intervals,
  * then a scatter pass adds them into the LDS node gradient.  Same results to
  * FP32 rounding (both forms sum the scatter in a run-dependent order).  A
  * workspace of at least the two sizes added runs the two-pass form whenever
- * the list's samples fit it; a smaller one runs the fused kernel.  0 for a
- * tensorial field (fused kernel only) or C outside {8, 16}.  No reference
+ * the list's samples fit it; a smaller one runs the fused kernel.  With a
+ * tensorial field the in-ROI samples stage their coordinates and dL/dplane,
+ * dL/dline before the corner weights (32 + 24 C bytes) and a third pass adds
+ * them into LDS plane gradients and line partials; the size then bounds
+ * every sample at that record size.  0 for C outside {8, 16}, explicit grids
+ * with G > 32, or planes and lines whose gradients exceed the LDS of one
+ * workgroup.  No reference
  * counterpart (the reference scatters per sample with global atomics,
  * modules/IndirectGridSample2D.cu:175-393).  atdf_project_prepare and
  * atdf_project_bwd_prepared must be given the same workspace and size. */

@@ -100,7 +100,7 @@ def _oracle(oracle, t, act, roi, arr, mlp, o, d, K, jm, seed, C_, lohi, dLdP=Non
     return proj, grads, info
 
 
-def _gpu(native, tree, roi, f, mlp, o, d, K, jm, seed, C_, Go, bf=False):
+def _gpu(native, tree, roi, f, mlp, o, d, K, jm, seed, C_, Go, bf=False, two_pass=True):
     sp = native.SamplingParams()
     sp.max_samples_per_node = K
     sp.jitter_mode = jm
@@ -108,21 +108,25 @@ def _gpu(native, tree, roi, f, mlp, o, d, K, jm, seed, C_, Go, bf=False):
     fp = native.FieldParams()
     fp.C, fp.G, fp.H = C_, Go, 64
     fp.bf16_decoder = bf
+    fp.two_pass_adjoint = two_pass
     m = torch.from_numpy(mlp).cuda().requires_grad_()
     rays = native.RayList(torch.from_numpy(o).cuda(), torch.from_numpy(d).cuda())
     return native.ProjectRaysVM(tree, roi, f, m, sp, fp, rays), m, sp, fp, rays
 
 
+@pytest.mark.parametrize("two_pass", [True, False])
 @pytest.mark.parametrize("C_,Gp,Gl,Go,jm", [(16, 9, 17, 5, 0), (16, 11, 33, 11, 2), (8, 9, 17, 5, 2)])
-def test_fused_vm_projection_and_adjoint(oracle, native, cuda, C_, Gp, Gl, Go, jm):
+def test_fused_vm_projection_and_adjoint(oracle, native, cuda, C_, Gp, Gl, Go, jm, two_pass):
+    """Both forms of the adjoint: the two-pass kernels (decoder pass, staged
+    plane / line gradients, LDS scatter passes) and the fused kernel."""
     lo, hi = [-0.55, -0.45, -0.6], [0.5, 0.6, 0.4]
     t, act, tree, roi, f, arr, mlp, o, d = _setup(native, oracle, C_, Gp, Gl, Go, lo, hi)
     K, seed = 4, 11
-    P, m, *_ = _gpu(native, tree, roi, f, mlp, o, d, K, jm, seed, C_, Go)
+    P, m, *_ = _gpu(native, tree, roi, f, mlp, o, d, K, jm, seed, C_, Go, two_pass=two_pass)
     rng = np.random.default_rng(3)
     dLdP = rng.normal(size=o.shape[0]).astype(np.float32)
     ref, rgrads, info = _oracle(oracle, t, act, roi, arr, mlp, o, d, K, jm, seed, C_, (lo, hi), dLdP)
-    print(f"C={C_} Gp={Gp} Gl={Gl} Go={Go} jitter={jm}: {info}", flush=True)
+    print(f"C={C_} Gp={Gp} Gl={Gl} Go={Go} jitter={jm} two-pass={two_pass}: {info}", flush=True)
     assert info["n_in_samples"] > 0 and info["n_out_samples"] > 0
     Pn = P.detach().cpu().double().numpy()
     err = np.abs(Pn - ref).max() / np.abs(ref).max()
@@ -136,13 +140,14 @@ def test_fused_vm_projection_and_adjoint(oracle, native, cuda, C_, Gp, Gl, Go, j
         assert e <= 1e-5, k
 
 
-def test_fused_vm_all_nodes_in_roi(oracle, native, cuda):
+@pytest.mark.parametrize("two_pass", [True, False])
+def test_fused_vm_all_nodes_in_roi(oracle, native, cuda, two_pass):
     """An ROI holding every node: no explicit grids at all (num_grids 0)."""
     lo, hi = [-1.0, -1.0, -1.0], [1.0, 1.0, 1.0]
     C_, Gp, Gl, Go = 16, 7, 13, 5
     t, act, tree, roi, f, arr, mlp, o, d = _setup(native, oracle, C_, Gp, Gl, Go, lo, hi, seed=2)
     assert int(roi.out_roi_node_index.numel()) == 0
-    P, m, *_ = _gpu(native, tree, roi, f, mlp, o, d, 3, 2, 5, C_, Go)
+    P, m, *_ = _gpu(native, tree, roi, f, mlp, o, d, 3, 2, 5, C_, Go, two_pass=two_pass)
     dLdP = np.random.default_rng(4).normal(size=o.shape[0]).astype(np.float32)
     ref, rgrads, info = _oracle(oracle, t, act, roi, arr, mlp, o, d, 3, 2, 5, C_, (lo, hi), dLdP)
     assert info["n_out_samples"] == 0
@@ -254,3 +259,39 @@ def test_train_step_vm(oracle, native, cuda):
               flush=True)
         assert torch.allclose(step, want, rtol=0, atol=2e-3 * lr), k
     assert all(float(gr.abs().max()) == 0.0 for gr in st.grads)  # cleared by the Adam passes
+
+
+def _adjoint(native, tree, roi, f, mlp, o, d, K, seed, C_, Go, bf, two_pass, dLdP):
+    P, m, *_ = _gpu(native, tree, roi, f, mlp, o, d, K, 2, seed, C_, Go, bf=bf, two_pass=two_pass)
+    names = NAMES if int(roi.out_roi_node_index.numel()) > 0 else NAMES[:6]
+    grads = torch.autograd.grad(P, [getattr(f, k) for k in names] + [m], torch.from_numpy(dLdP).cuda())
+    return dict(zip(names + ["mlp"], [g.detach().cpu().double().numpy() for g in grads]))
+
+
+@pytest.mark.parametrize("C_,bf,depth,budget,n_rays,K,lohi", [
+    (16, False, 4, 400, 3000, 6, ([-0.55, -0.45, -0.6], [0.5, 0.6, 0.4])),
+    (16, True, 4, 400, 3000, 6, ([-0.55, -0.45, -0.6], [0.5, 0.6, 0.4])),
+    (8, False, 4, 400, 3000, 6, ([-0.55, -0.45, -0.6], [0.5, 0.6, 0.4])),
+    (8, True, 4, 400, 2000, 6, ([-0.55, -0.45, -0.6], [0.5, 0.6, 0.4])),
+    # few large nodes, many samples each: nodes split over several chunks
+    (16, False, 2, 20, 6000, 8, ([-0.6, -0.6, -0.6], [0.6, 0.6, 0.6])),
+])
+def test_vm_two_pass_matches_fused(oracle, native, cuda, C_, bf, depth, budget, n_rays, K, lohi):
+    """The two-pass adjoint of a tensorial field against its fused kernel on
+    larger lists (FP32 and BF16-MFMA decoder, C = 16 and 8, nodes split over
+    chunks): every plane, line, explicit-grid and decoder gradient within
+    1e-5 (both sum in run-dependent orders)."""
+    lo, hi = lohi
+    Gp, Gl, Go = 11, 65, 11
+    t, act, tree, roi, f, arr, mlp, o, d = _setup(native, oracle, C_, Gp, Gl, Go, lo, hi, seed=6, depth=depth,
+                                                  budget=budget, n_rays=n_rays)
+    dLdP = np.random.default_rng(12).normal(size=o.shape[0]).astype(np.float32)
+    ga = _adjoint(native, tree, roi, f, mlp, o, d, K, 17, C_, Go, bf, True, dLdP)
+    gb = _adjoint(native, tree, roi, f, mlp, o, d, K, 17, C_, Go, bf, False, dLdP)
+    print(f"C={C_} bf16 decoder={bf} depth={depth}: {int(roi.in_roi_node_index.numel())} in-ROI / "
+          f"{int(roi.out_roi_node_index.numel())} explicit nodes, {o.shape[0]} rays", flush=True)
+    for k in ga:
+        e = _rel(ga[k], gb[k])
+        print(f"  two-pass vs fused grad {k}: {e:.2e}", flush=True)
+        assert np.abs(gb[k]).max() > 0, k
+        assert e <= 1e-5, k
