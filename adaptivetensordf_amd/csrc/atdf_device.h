@@ -18,7 +18,8 @@ struct alignas(16) PNode {
     float mx[3];
     int32_t parent;       // -1 for the root
     int32_t flags;        // bit 0: active, bit 1: has an active descendant
-    int32_t pad[3];
+    int32_t first_sib;    // first child of the parent (0 for the root)
+    int32_t pad[2];
 };
 static_assert(sizeof(PNode) == 48, "PNode must stay 48 bytes");
 

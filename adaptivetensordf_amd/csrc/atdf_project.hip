@@ -1651,6 +1651,18 @@ __device__ __forceinline__ void lds_unlock(int* lock, int lane)
 // complete and Adam is applied to the node's grid (parameters at gbase)
 // instead of going through the gradient buffer.  split: other chunks of the
 // node add to the same gradient, through device-scope atomics.
+// Channel-pair swizzle of the scatter passes' LDS gradients (cell-major, C
+// floats per cell): channels c, c + 1 of cell x sit at c ^ pair_swz(x).  The
+// low bits of x already move a cell across bank rows (64 / C cells fill the
+// 64 banks of one row), so the swizzle takes the bits above them: lanes of
+// one float2 access at random cells then reach every bank pair with 32 or
+// with 64 banks, where 2 (x mod C/2) reached half of them.
+__device__ __forceinline__ int pair_swz(int cell, int C)
+{
+    const int sh = C >= 16 ? 2 : (C == 8 ? 3 : 4);
+    return 2 * ((cell >> sh) & (C / 2 - 1));
+}
+
 template <int kSwzC>
 __device__ __forceinline__ float4 take_lds4(float* lgrad, int i)
 {
@@ -1665,7 +1677,7 @@ __device__ __forceinline__ float4 take_lds4(float* lgrad, int i)
         // swizzled channel pairs (the scatter pass's layout)
         constexpr int Q = kSwzC / 4;
         const int cell = i / Q, q = i % Q;
-        const int swz = 2 * (cell & (kSwzC / 2 - 1));
+        const int swz = pair_swz(cell, kSwzC);
         float2* a = reinterpret_cast<float2*>(lgrad + cell * kSwzC + ((4 * q) ^ swz));
         float2* b = reinterpret_cast<float2*>(lgrad + cell * kSwzC + ((4 * q + 2) ^ swz));
         const float2 va = *a, vb = *b;
@@ -2267,10 +2279,9 @@ __global__ __launch_bounds__(64 * kDecWaves, 2) void project_bwd_dec_kernel(Node
 // operations in issue order, so each read sees the previous corners' writes.
 //
 // LDS layout: cell-major with C floats per cell, the channel pairs of cell
-// x at (c ^ swz(x)), swz(x) = 2 (x mod C/2): one wave's 64 lanes (same
-// channel pair, different cells) then spread over all 32 banks instead of
+// x at (c ^ pair_swz(x)) (defined with the flush above): one wave's 64 lanes
+// (same channel pair, different cells) then spread over all banks instead of
 // the 2 that channels-last would give.
-__device__ __forceinline__ int pair_swz(int cell, int C) { return 2 * (cell & (C / 2 - 1)); }
 
 template <int C>
 __global__ __launch_bounds__(32 * C) void project_bwd_scatter_kernel(NodeBwdArgs b)

@@ -52,7 +52,8 @@ __global__ __launch_bounds__(256) void pack_nodes_kernel(const float* __restrict
     p.first_child = children[i * ns];
     p.parent = parent[i];
     p.flags = (active[i] ? kFlagActive : 0) | (ns == 4 ? kFlagQuad : 0);
-    p.pad[0] = p.pad[1] = p.pad[2] = 0;
+    p.first_sib = p.parent >= 0 ? children[(int64_t)p.parent * ns] : 0;
+    p.pad[0] = p.pad[1] = 0;
     out[i] = p;
 }
 
@@ -181,18 +182,22 @@ __global__ __launch_bounds__(256) void traverse_kernel(TraverseArgs a)
             continue;
         }
         // next sibling in front-to-back order, ascending while exhausted
+        // (a node carries its parent's first child, so a sibling step needs
+        // no load; each level ascended loads the parent's record once)
         int cur = node;
         int parent = nd.parent;
+        int first = nd.first_sib;
         while (true) {
             if (parent < 0) goto done;
-            int first = a.nodes[parent].first_child;
             int k = (cur - first) ^ mask;
             if (k < last_sibling) {
                 node = first + ((k + 1) ^ mask);
                 break;
             }
             cur = parent;
-            parent = a.nodes[parent].parent;
+            const PNode& pn = a.nodes[parent];
+            first = pn.first_sib;
+            parent = pn.parent;
         }
     }
 done:
@@ -505,7 +510,8 @@ __global__ void pack_box_kernel(float x0, float y0, float z0, float x1, float y1
     p.first_child = -1;
     p.parent = -1;
     p.flags = kFlagActive;
-    p.pad[0] = p.pad[1] = p.pad[2] = 0;
+    p.first_sib = 0;
+    p.pad[0] = p.pad[1] = 0;
     out[0] = p;
 }
 

@@ -77,16 +77,34 @@ def rays_for_pixels(tilt: np.ndarray, px: np.ndarray, py: np.ndarray, W: int, H:
     return o.astype(np.float32), d.astype(np.float32)
 
 
-def sample_pixels(num_rays: int, num_tilts: int, W: int, H: int, seed: int):
+def _interleave_bits(v: np.ndarray, bits: int) -> np.ndarray:
+    out = np.zeros_like(v)
+    for b in range(bits):
+        out |= ((v >> b) & 1) << (2 * b)
+    return out
+
+
+def sample_pixels(num_rays: int, num_tilts: int, W: int, H: int, seed: int, order: str = "row"):
     """A random batch of detector pixels (like the reference's random-pixel
-    loader), returned sorted by (tilt, y, x) so adjacent rays are adjacent
-    pixels.  Sampling is with replacement."""
+    loader).  Sampling is with replacement.  order "row": sorted by (tilt,
+    y, x), so adjacent rays are adjacent pixels of a row; "tile": sorted by
+    tilt, then by the Z-order (bit-interleaved y, x) of the pixel, so the 64
+    rays of one wave cover a compact patch of the detector and walk mostly
+    the same octree nodes.  The same pixels are drawn in either order."""
     rng = np.random.Generator(np.random.PCG64(seed))
     lin = rng.integers(0, num_tilts * H * W, size=num_rays, dtype=np.int64)
     lin.sort()
     tilt = lin // (H * W)
     rem = lin % (H * W)
-    return tilt, rem % W, rem // W
+    px, py = rem % W, rem // W
+    if order == "tile":
+        bits = int(max(W, H) - 1).bit_length()
+        key = (tilt << (2 * bits)) | (_interleave_bits(py, bits) << 1) | _interleave_bits(px, bits)
+        idx = np.argsort(key, kind="stable")
+        tilt, px, py = tilt[idx], px[idx], py[idx]
+    elif order != "row":
+        raise ValueError(f"unknown pixel order {order!r}")
+    return tilt, px, py
 
 
 def analytic_projection(o: np.ndarray, d: np.ndarray, ell: np.ndarray) -> np.ndarray:

@@ -98,6 +98,8 @@ def parse():
     ap.add_argument("--batches", type=int, default=32,
                     help="distinct training batches (32 x 2^18 rays = half of the 61 x 512^2 detector pixels)")
     ap.add_argument("--budget", type=int, default=8192, help="max active octree nodes")
+    ap.add_argument("--ray-order", choices=["row", "tile"], default="tile",
+                    help="order of a batch's rays: by (tilt, row, column), or by tilt and the Z-order of the pixel")
     ap.add_argument("--samples-per-ray", type=float, default=128.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-runs", type=int, default=5, help="timed CPU baseline runs (median reported)")
@@ -197,7 +199,7 @@ def build_data(N, args, dev):
     angles = phantom.tilt_angles(61)
 
     def make_batch(nrays, seed):
-        tl, px, py = phantom.sample_pixels(nrays, 61, n, n, seed)
+        tl, px, py = phantom.sample_pixels(nrays, 61, n, n, seed, args.ray_order)
         o, d = phantom.rays_for_pixels(tl, px, py, n, n, angles)
         o_t, d_t = torch.from_numpy(o).to(dev), torch.from_numpy(d).to(dev)
         return N.RayList(o_t, d_t), volume_projection(vol, o_t, d_t).float().contiguous()
