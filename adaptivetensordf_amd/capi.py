@@ -119,6 +119,10 @@ _SIGNATURES = {
     "atdf_exclusive_scan_i32": (_I32, [_P, _I64, _P, _P, _SZ, _P]),
     "atdf_traverse_write": (_I32, [_P, _I64, _P, _P, _I64, C.POINTER(Sampling), _P, _P, C.POINTER(Intervals),
                                    C.POINTER(Samples), _P]),
+    "atdf_traverse_cache_bytes": (_SZ, [_I64, _I32]),
+    "atdf_traverse_count_cached": (_I32, [_P, _I64, _P, _P, _I64, C.POINTER(Sampling), _P, _P, _P, _P, _I32, _P]),
+    "atdf_traverse_write_cached": (_I32, [_P, _I64, _P, _P, _I64, C.POINTER(Sampling), _P, _P,
+                                          C.POINTER(Intervals), C.POINTER(Samples), _P, _I32, _P]),
     "atdf_local_samples": (_I32, [_P, _P, _P, _P, _I64, _P, _P]),
     "atdf_grid_sample3d_fwd": (_I32, [_P, _I64, _I32, _I32, _I32, _I32, _P, _P, _I64, _P, _P]),
     "atdf_grid_sample3d_bwd": (_I32, [_P, _I64, _I32, _I32, _I32, _I32, _P, _P, _I64, _P, _P, _P, _P]),

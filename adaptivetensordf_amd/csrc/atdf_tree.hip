@@ -93,6 +93,13 @@ struct TraverseArgs {
     const int64_t* smp_off;
     atdf_intervals iv;
     atdf_samples smp;
+    // hit cache (dense ray lists only): the count pass records each ray's
+    // hit count and its first cache_slots hits (slot-major, so a wave's
+    // records are contiguous); the write pass writes a ray whose hits all
+    // fit from the cache instead of walking the tree again
+    int32_t* cache_n;  // [R]
+    int4* cache;       // [cache_slots][R]: node, tmin bits, tmax bits, samples
+    int cache_slots;
 };
 
 __device__ __forceinline__ float jitter_of(const TraverseArgs& a, int64_t slot, int64_t ray, int64_t loc)
@@ -100,6 +107,41 @@ __device__ __forceinline__ float jitter_of(const TraverseArgs& a, int64_t slot, 
     if (a.jitter_mode == 1) return a.jitter[slot];
     if (a.jitter_mode == 2) return hash_jitter(a.seed, ray, loc);
     return 0.5f;
+}
+
+// The write pass's outputs for one hit node (interval and, when asked, its
+// samples), the nh-th hit of ray r whose samples start at ns.
+__device__ __forceinline__ void write_hit(const TraverseArgs& a, int64_t r, const float* o, const float* d, int node,
+                                          float tmin, float tmax, int n, int64_t ivb, int64_t sb, int nh, int ns)
+{
+    if (a.iv.node) {
+        int64_t q = ivb + nh;
+        a.iv.node[q] = node;
+        a.iv.tmin[q] = tmin;
+        a.iv.tmax[q] = tmax;
+        a.iv.count[q] = n;
+        a.iv.first[q] = ns;
+        a.iv.ray[q] = (int32_t)r;
+    }
+    if (a.smp.weight && n > 0) {
+        float step = (tmax - tmin) / (float)n;
+        int64_t slot0 = sb + ns;
+        float t_prev = 0.f;
+        float t_cur = sample_t(tmin, step, 0, jitter_of(a, slot0, r, ns));
+        for (int j = 0; j < n; ++j) {
+            float t_next = 0.f;
+            if (j < n - 1) t_next = sample_t(tmin, step, j + 1, jitter_of(a, slot0 + j + 1, r, ns + j + 1));
+            int64_t slot = slot0 + j;
+            a.smp.ray_t[slot] = t_cur;
+            a.smp.weight[slot] = sample_weight(tmin, tmax, n, j, t_prev, t_cur, t_next);
+            for (int k = 0; k < 3; ++k) a.smp.global_coordinate[slot * 3 + k] = o[k] + d[k] * t_cur;
+            a.smp.ray_index[slot] = r;
+            a.smp.local_index_in_ray[slot] = ns + j;
+            a.smp.node_id[slot] = node;
+            t_prev = t_cur;
+            t_cur = t_next;
+        }
+    }
 }
 
 template <bool kWrite>
@@ -133,6 +175,18 @@ __global__ __launch_bounds__(256) void traverse_kernel(TraverseArgs a)
     bool order_bad = false;
     const int64_t ivb = kWrite ? a.iv_off[r] : 0;
     const int64_t sb = kWrite ? a.smp_off[r] : 0;
+    if (kWrite && a.cache) {
+        const int nc = a.cache_n[r];
+        if (nc <= a.cache_slots) {
+            for (int q = 0; q < nc; ++q) {
+                const int4 e = a.cache[(int64_t)q * a.R + r];
+                write_hit(a, r, o, d, e.x, __int_as_float(e.y), __int_as_float(e.z), e.w, ivb, sb, nh, ns);
+                nh++;
+                ns += e.w;
+            }
+            return;
+        }
+    }
 
     int node = 0;
     while (true) {
@@ -144,36 +198,9 @@ __global__ __launch_bounds__(256) void traverse_kernel(TraverseArgs a)
             if (tmin < last_t || (tmin == last_t && node < last_node)) order_bad = true;
             last_t = tmin;
             last_node = node;
-            if (kWrite) {
-                if (a.iv.node) {
-                    int64_t q = ivb + nh;
-                    a.iv.node[q] = node;
-                    a.iv.tmin[q] = tmin;
-                    a.iv.tmax[q] = tmax;
-                    a.iv.count[q] = n;
-                    a.iv.first[q] = ns;
-                    a.iv.ray[q] = (int32_t)r;
-                }
-                if (a.smp.weight && n > 0) {
-                    float step = (tmax - tmin) / (float)n;
-                    int64_t slot0 = sb + ns;
-                    float t_prev = 0.f;
-                    float t_cur = sample_t(tmin, step, 0, jitter_of(a, slot0, r, ns));
-                    for (int j = 0; j < n; ++j) {
-                        float t_next = 0.f;
-                        if (j < n - 1) t_next = sample_t(tmin, step, j + 1, jitter_of(a, slot0 + j + 1, r, ns + j + 1));
-                        int64_t slot = slot0 + j;
-                        a.smp.ray_t[slot] = t_cur;
-                        a.smp.weight[slot] = sample_weight(tmin, tmax, n, j, t_prev, t_cur, t_next);
-                        for (int k = 0; k < 3; ++k) a.smp.global_coordinate[slot * 3 + k] = o[k] + d[k] * t_cur;
-                        a.smp.ray_index[slot] = r;
-                        a.smp.local_index_in_ray[slot] = ns + j;
-                        a.smp.node_id[slot] = node;
-                        t_prev = t_cur;
-                        t_cur = t_next;
-                    }
-                }
-            }
+            if (kWrite) write_hit(a, r, o, d, node, tmin, tmax, n, ivb, sb, nh, ns);
+            else if (a.cache && nh < a.cache_slots)
+                a.cache[(int64_t)nh * a.R + r] = make_int4(node, __float_as_int(tmin), __float_as_int(tmax), n);
             nh++;
             ns += n;
         }
@@ -204,6 +231,7 @@ done:
     if (!kWrite) {
         a.ray_num_intervals[r] = nh;
         a.ray_num_samples[r] = ns;
+        if (a.cache) a.cache_n[r] = nh;
         if (nh > a.max_inter) atomicAdd(&a.status[0], 1);
         if (order_bad) atomicAdd(&a.status[1], 1);
         atomicMax(&a.status[2], ns);
@@ -536,10 +564,17 @@ static atdf_status check_sampling(const atdf_sampling* sp)
     return ATDF_OK;
 }
 
-atdf_status atdf_traverse_count(const void* packed_tree, int64_t num_nodes, const float* ray_origin,
-                                const float* ray_direction, int64_t num_rays, const atdf_sampling* sampling,
-                                int32_t* ray_num_intervals, int32_t* ray_num_samples, int32_t* status,
-                                atdf_stream_t stream)
+// Hit cache layout: [R] hit counts (padded to 16 B), then [slots][R] records.
+static void cache_views(void* cache, int64_t R, int32_t*& n, int4*& rec)
+{
+    n = reinterpret_cast<int32_t*>(cache);
+    rec = reinterpret_cast<int4*>(reinterpret_cast<unsigned char*>(cache) + ((size_t)R * 4 + 15) / 16 * 16);
+}
+
+static atdf_status traverse_count_impl(const void* packed_tree, int64_t num_nodes, const float* ray_origin,
+                                       const float* ray_direction, int64_t num_rays, const atdf_sampling* sampling,
+                                       int32_t* ray_num_intervals, int32_t* ray_num_samples, int32_t* status,
+                                       void* cache, int32_t cache_slots, atdf_stream_t stream)
 {
     atdf_status st = check_sampling(sampling);
     if (st) return st;
@@ -563,6 +598,10 @@ atdf_status atdf_traverse_count(const void* packed_tree, int64_t num_nodes, cons
     a.ray_num_intervals = ray_num_intervals;
     a.ray_num_samples = ray_num_samples;
     a.status = status;
+    if (cache && !a.list) {
+        cache_views(cache, num_rays, a.cache_n, a.cache);
+        a.cache_slots = cache_slots;
+    }
     if (a.list)  // a sparse selection: eight lanes per listed ray
         hipLaunchKernelGGL(traverse_coop_kernel<false>, dim3((unsigned)div_up(num_rays * 8, kCoopThreads)),
                            dim3(kCoopThreads), 0, to_hip(stream), a);
@@ -572,11 +611,11 @@ atdf_status atdf_traverse_count(const void* packed_tree, int64_t num_nodes, cons
     return check_launch("atdf_traverse_count");
 }
 
-atdf_status atdf_traverse_write(const void* packed_tree, int64_t num_nodes, const float* ray_origin,
-                                const float* ray_direction, int64_t num_rays, const atdf_sampling* sampling,
-                                const int64_t* ray_interval_offset, const int64_t* ray_sample_offset,
-                                const atdf_intervals* intervals, const atdf_samples* samples,
-                                atdf_stream_t stream)
+static atdf_status traverse_write_impl(const void* packed_tree, int64_t num_nodes, const float* ray_origin,
+                                       const float* ray_direction, int64_t num_rays, const atdf_sampling* sampling,
+                                       const int64_t* ray_interval_offset, const int64_t* ray_sample_offset,
+                                       const atdf_intervals* intervals, const atdf_samples* samples,
+                                       const void* cache, int32_t cache_slots, atdf_stream_t stream)
 {
     atdf_status st = check_sampling(sampling);
     if (st) return st;
@@ -612,6 +651,10 @@ atdf_status atdf_traverse_write(const void* packed_tree, int64_t num_nodes, cons
     a.smp_off = samples ? ray_sample_offset : ray_interval_offset;
     if (intervals) a.iv = *intervals;
     if (samples) a.smp = *samples;
+    if (cache && !a.list) {
+        cache_views(const_cast<void*>(cache), num_rays, a.cache_n, a.cache);
+        a.cache_slots = cache_slots;
+    }
     if (a.list && !samples)  // intervals of a sparse selection: eight lanes per listed ray
         hipLaunchKernelGGL(traverse_coop_kernel<true>, dim3((unsigned)div_up(num_rays * 8, kCoopThreads)),
                            dim3(kCoopThreads), 0, to_hip(stream), a);
@@ -619,6 +662,53 @@ atdf_status atdf_traverse_write(const void* packed_tree, int64_t num_nodes, cons
         hipLaunchKernelGGL(traverse_kernel<true>, dim3((unsigned)div_up(num_rays, 256)), dim3(256), 0,
                            to_hip(stream), a);
     return check_launch("atdf_traverse_write");
+}
+
+atdf_status atdf_traverse_count(const void* packed_tree, int64_t num_nodes, const float* ray_origin,
+                                const float* ray_direction, int64_t num_rays, const atdf_sampling* sampling,
+                                int32_t* ray_num_intervals, int32_t* ray_num_samples, int32_t* status,
+                                atdf_stream_t stream)
+{
+    return traverse_count_impl(packed_tree, num_nodes, ray_origin, ray_direction, num_rays, sampling,
+                               ray_num_intervals, ray_num_samples, status, nullptr, 0, stream);
+}
+
+atdf_status atdf_traverse_write(const void* packed_tree, int64_t num_nodes, const float* ray_origin,
+                                const float* ray_direction, int64_t num_rays, const atdf_sampling* sampling,
+                                const int64_t* ray_interval_offset, const int64_t* ray_sample_offset,
+                                const atdf_intervals* intervals, const atdf_samples* samples,
+                                atdf_stream_t stream)
+{
+    return traverse_write_impl(packed_tree, num_nodes, ray_origin, ray_direction, num_rays, sampling,
+                               ray_interval_offset, ray_sample_offset, intervals, samples, nullptr, 0, stream);
+}
+
+size_t atdf_traverse_cache_bytes(int64_t num_rays, int32_t cache_slots)
+{
+    if (num_rays < 0 || cache_slots < 0) return 0;
+    return ((size_t)num_rays * 4 + 15) / 16 * 16 + (size_t)cache_slots * (size_t)num_rays * sizeof(int4);
+}
+
+atdf_status atdf_traverse_count_cached(const void* packed_tree, int64_t num_nodes, const float* ray_origin,
+                                       const float* ray_direction, int64_t num_rays, const atdf_sampling* sampling,
+                                       int32_t* ray_num_intervals, int32_t* ray_num_samples, int32_t* status,
+                                       void* cache, int32_t cache_slots, atdf_stream_t stream)
+{
+    ATDF_REQUIRE(cache && cache_slots > 0, "atdf_traverse_count_cached: null cache or no slots");
+    return traverse_count_impl(packed_tree, num_nodes, ray_origin, ray_direction, num_rays, sampling,
+                               ray_num_intervals, ray_num_samples, status, cache, cache_slots, stream);
+}
+
+atdf_status atdf_traverse_write_cached(const void* packed_tree, int64_t num_nodes, const float* ray_origin,
+                                       const float* ray_direction, int64_t num_rays, const atdf_sampling* sampling,
+                                       const int64_t* ray_interval_offset, const int64_t* ray_sample_offset,
+                                       const atdf_intervals* intervals, const atdf_samples* samples,
+                                       const void* cache, int32_t cache_slots, atdf_stream_t stream)
+{
+    ATDF_REQUIRE(cache && cache_slots > 0, "atdf_traverse_write_cached: null cache or no slots");
+    return traverse_write_impl(packed_tree, num_nodes, ray_origin, ray_direction, num_rays, sampling,
+                               ray_interval_offset, ray_sample_offset, intervals, samples, cache, cache_slots,
+                               stream);
 }
 
 size_t atdf_scan_workspace_bytes(int64_t n)

@@ -214,9 +214,14 @@ void HyperTreeBase::repack()
 namespace {
 struct TraversalCounts {
     Tensor num_iv, num_s, iv_off, smp_off;
+    Tensor cache;  // the count pass's hits, replayed by the write pass (dense ray lists)
     int64_t total_iv = 0, total_s = 0;
     int max_per_ray = 0;
 };
+
+// Hit-cache slots per ray: rays with more hits are walked again by the write
+// pass (the adaptive 512^3 octrees give about 22 per ray).
+constexpr int32_t kTraverseCacheSlots = 64;
 
 TraversalCounts count_and_scan(const void* packed, int64_t num_nodes, torch::Device dev, const RayList& rays,
                                const atdf_sampling& s)
@@ -231,7 +236,15 @@ TraversalCounts count_and_scan(const void* packed, int64_t num_nodes, torch::Dev
     tc.smp_off = torch::empty({R + 1}, i64);
     auto status = torch::zeros({4}, i32);
     auto st = current_stream();
-    if (R > 0) {
+    if (R > 0 && !s.ray_list) {
+        tc.cache = torch::empty({(int64_t)atdf_traverse_cache_bytes(R, kTraverseCacheSlots)},
+                                torch::TensorOptions().dtype(torch::kUInt8).device(dev));
+        check_status(atdf_traverse_count_cached(packed, num_nodes, rays.origin.data_ptr<float>(),
+                                                rays.direction.data_ptr<float>(), R, &s, tc.num_iv.data_ptr<int32_t>(),
+                                                tc.num_s.data_ptr<int32_t>(), status.data_ptr<int32_t>(),
+                                                tc.cache.data_ptr(), kTraverseCacheSlots, st),
+                     "atdf_traverse_count_cached");
+    } else if (R > 0) {
         check_status(atdf_traverse_count(packed, num_nodes, rays.origin.data_ptr<float>(),
                                          rays.direction.data_ptr<float>(), R, &s, tc.num_iv.data_ptr<int32_t>(),
                                          tc.num_s.data_ptr<int32_t>(), status.data_ptr<int32_t>(), st),
@@ -257,6 +270,25 @@ TraversalCounts count_and_scan(const void* packed, int64_t num_nodes, torch::Dev
     tc.total_iv = ip[4];
     tc.total_s = ip[5];
     return tc;
+}
+
+// The write pass over the rays count_and_scan counted (from its hit cache
+// when it kept one).
+void traverse_write(const void* packed, int64_t num_nodes, const RayList& rays, const atdf_sampling& s,
+                    const TraversalCounts& tc, const atdf_intervals* iv, const atdf_samples* smp)
+{
+    if (tc.cache.defined())
+        check_status(atdf_traverse_write_cached(packed, num_nodes, rays.origin.data_ptr<float>(),
+                                                rays.direction.data_ptr<float>(), rays.size(), &s,
+                                                tc.iv_off.data_ptr<int64_t>(), tc.smp_off.data_ptr<int64_t>(), iv, smp,
+                                                tc.cache.data_ptr(), kTraverseCacheSlots, current_stream()),
+                     "atdf_traverse_write_cached");
+    else
+        check_status(atdf_traverse_write(packed, num_nodes, rays.origin.data_ptr<float>(),
+                                         rays.direction.data_ptr<float>(), rays.size(), &s,
+                                         tc.iv_off.data_ptr<int64_t>(), tc.smp_off.data_ptr<int64_t>(), iv, smp,
+                                         current_stream()),
+                     "atdf_traverse_write");
 }
 
 void check_rays(const RayList& rays)
@@ -293,11 +325,7 @@ Intervals HyperTreeBase::intervals_for(const void* packed, int64_t num_nodes, co
     iv.ray = torch::empty({H}, i32);
     if (H > 0) {
         atdf_intervals v = iv.view();
-        check_status(atdf_traverse_write(packed, num_nodes, rays.origin.data_ptr<float>(),
-                                         rays.direction.data_ptr<float>(), rays.size(), &s,
-                                         iv.ray_offset.data_ptr<int64_t>(), tc.smp_off.data_ptr<int64_t>(), &v,
-                                         nullptr, current_stream()),
-                     "atdf_traverse_write");
+        traverse_write(packed, num_nodes, rays, s, tc, &v, nullptr);
     }
     return iv;
 }
@@ -358,10 +386,7 @@ SampleList HyperTreeBase::samples_for(const void* packed, int64_t num_nodes, con
         atdf_samples v{list.global_coordinate.data_ptr<float>(), list.weight.data_ptr<float>(),
                        list.ray_t.data_ptr<float>(),              list.ray_index.data_ptr<int64_t>(),
                        list.local_index_in_ray.data_ptr<int64_t>(), list.node_id.data_ptr<int64_t>()};
-        check_status(atdf_traverse_write(packed, num_nodes, rays.origin.data_ptr<float>(),
-                                         rays.direction.data_ptr<float>(), rays.size(), &s, tc.iv_off.data_ptr<int64_t>(),
-                                         tc.smp_off.data_ptr<int64_t>(), nullptr, &v, current_stream()),
-                     "atdf_traverse_write");
+        traverse_write(packed, num_nodes, rays, s, tc, nullptr, &v);
     }
     return list;
 }

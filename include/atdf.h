@@ -146,6 +146,27 @@ atdf_status atdf_traverse_write(const void* packed_tree, int64_t num_nodes, cons
                                 const atdf_intervals* intervals, const atdf_samples* samples,
                                 atdf_stream_t stream);
 
+/* The same two passes with a hit cache, for the reference's count-then-fill
+ * CreateSamplesForRays (structure/HyperTree.cu:516-610) when both passes run
+ * on one ray list: the count pass records each ray's hit count and its
+ * first cache_slots hits (node, tmin, tmax, sample count: 16 B each,
+ * slot-major in the caller's cache of atdf_traverse_cache_bytes bytes), and
+ * the write pass writes a ray whose hits all fit from the cache instead of
+ * walking the tree again (a ray with more hits is walked).  Outputs,
+ * statuses and errors are those of atdf_traverse_count / atdf_traverse_write;
+ * the write pass must see the sampling and rays its count pass saw.  With a
+ * ray_list the cache is not used. */
+size_t atdf_traverse_cache_bytes(int64_t num_rays, int32_t cache_slots);
+atdf_status atdf_traverse_count_cached(const void* packed_tree, int64_t num_nodes, const float* ray_origin,
+                                       const float* ray_direction, int64_t num_rays, const atdf_sampling* sampling,
+                                       int32_t* ray_num_intervals, int32_t* ray_num_samples, int32_t* status,
+                                       void* cache, int32_t cache_slots, atdf_stream_t stream);
+atdf_status atdf_traverse_write_cached(const void* packed_tree, int64_t num_nodes, const float* ray_origin,
+                                       const float* ray_direction, int64_t num_rays, const atdf_sampling* sampling,
+                                       const int64_t* ray_interval_offset, const int64_t* ray_sample_offset,
+                                       const atdf_intervals* intervals, const atdf_samples* samples,
+                                       const void* cache, int32_t cache_slots, atdf_stream_t stream);
+
 /* ======================================================================= */
 /* Point location and grid transfer across a change of the active set      */
 /* ======================================================================= */

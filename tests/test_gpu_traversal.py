@@ -13,8 +13,10 @@ from scene_util import adaptive_active, device_tree, rays_to_device, tilt_rays, 
 pytestmark = pytest.mark.gpu
 
 
-def capi_traverse(tree, o_t, d_t, K, max_inter=256, jitter_mode=0, seed=0, jitter=None):
-    """Count -> scan -> write through the raw C ABI; returns numpy arrays."""
+def capi_traverse(tree, o_t, d_t, K, max_inter=256, jitter_mode=0, seed=0, jitter=None, cache_slots=0):
+    """Count -> scan -> write through the raw C ABI; returns numpy arrays.
+    cache_slots > 0: the cached entry points (the write pass replays the
+    count pass's hits of every ray with at most cache_slots of them)."""
     from adaptivetensordf_amd import capi
 
     dev = o_t.device
@@ -28,8 +30,17 @@ def capi_traverse(tree, o_t, d_t, K, max_inter=256, jitter_mode=0, seed=0, jitte
     niv = torch.empty(R, dtype=torch.int32, device=dev)
     ns = torch.empty(R, dtype=torch.int32, device=dev)
     status = torch.zeros(4, dtype=torch.int32, device=dev)
-    capi.call("atdf_traverse_count", capi.ptr(packed), tree.NumNodes(), capi.ptr(o_t), capi.ptr(d_t), R,
-              C.byref(sp), capi.ptr(niv), capi.ptr(ns), capi.ptr(status), s)
+    cache = None
+    if cache_slots > 0:
+        # poisoned, so a record the write pass reads without the count pass
+        # having written it shows up as a wrong interval
+        cache = torch.full((capi.lib().atdf_traverse_cache_bytes(R, cache_slots),), 0x7F, dtype=torch.uint8,
+                           device=dev)
+        capi.call("atdf_traverse_count_cached", capi.ptr(packed), tree.NumNodes(), capi.ptr(o_t), capi.ptr(d_t), R,
+                  C.byref(sp), capi.ptr(niv), capi.ptr(ns), capi.ptr(status), capi.ptr(cache), cache_slots, s)
+    else:
+        capi.call("atdf_traverse_count", capi.ptr(packed), tree.NumNodes(), capi.ptr(o_t), capi.ptr(d_t), R,
+                  C.byref(sp), capi.ptr(niv), capi.ptr(ns), capi.ptr(status), s)
     ws = torch.empty(max(8, capi.lib().atdf_scan_workspace_bytes(R)), dtype=torch.uint8, device=dev)
     ivo = torch.empty(R + 1, dtype=torch.int64, device=dev)
     so = torch.empty(R + 1, dtype=torch.int64, device=dev)
@@ -46,7 +57,11 @@ def capi_traverse(tree, o_t, d_t, K, max_inter=256, jitter_mode=0, seed=0, jitte
     civ = capi.Intervals(*[iv[k].data_ptr() for k in ("node", "tmin", "tmax", "count", "first", "ray")])
     csm = capi.Samples(*[sm[k].data_ptr() for k in ("global_coordinate", "weight", "ray_t", "ray_index",
                                                     "local_index_in_ray", "node_id")])
-    if H > 0:  # the write pass needs real buffers; with no hits there is nothing to write
+    if H > 0 and cache is not None:
+        capi.call("atdf_traverse_write_cached", capi.ptr(packed), tree.NumNodes(), capi.ptr(o_t), capi.ptr(d_t), R,
+                  C.byref(sp), capi.ptr(ivo), capi.ptr(so), C.byref(civ), C.byref(csm), capi.ptr(cache), cache_slots,
+                  s)
+    elif H > 0:  # the write pass needs real buffers; with no hits there is nothing to write
         capi.call("atdf_traverse_write", capi.ptr(packed), tree.NumNodes(), capi.ptr(o_t), capi.ptr(d_t), R,
                   C.byref(sp), capi.ptr(ivo), capi.ptr(so), C.byref(civ), C.byref(csm), s)
     torch.cuda.synchronize()
@@ -91,6 +106,31 @@ def test_capi_traversal_bit_exact(oracle, cuda, case, jitter_mode):
     assert g["status"][2] == r["ray_num_samples"].max()
     assert_same_traversal(g, r)
     assert np.array_equal(g["ray"], np.repeat(np.arange(o.shape[0]), r["ray_num_intervals"]))
+
+
+@pytest.mark.parametrize("cache_slots", [64, 3, 1])
+@pytest.mark.parametrize("jitter_mode", [0, 2])
+def test_capi_traversal_hit_cache(oracle, cuda, cache_slots, jitter_mode):
+    """The cached count / write passes: bit-exact against the oracle whether a
+    ray's hits come from the cache (at most cache_slots of them) or from a
+    second walk (more)."""
+    t = oracle.tree_build(4)
+    act = adaptive_active(t, budget=600)
+    tree = device_tree(4, act)
+    o, d = random_rays(700, 23)
+    eo, ed = edge_case_rays()
+    to_, td = tilt_rays(400, 5)
+    o, d = np.concatenate([o, eo, to_]), np.concatenate([d, ed, td])
+    print(f"cache_slots={cache_slots} jitter={jitter_mode} R={o.shape[0]}", flush=True)
+    g = capi_traverse(tree, *rays_to_device(o, d), 6, jitter_mode=jitter_mode, seed=7, cache_slots=cache_slots)
+    r = oracle.traverse(t["node_min"], t["node_max"], oracle.active_ids(act), o, d, 6, jitter_mode=jitter_mode,
+                        seed=7)
+    n = r["ray_num_intervals"]
+    # both paths are exercised: rays within the slots and rays beyond them
+    assert (n <= cache_slots).any() and (cache_slots == 64 or (n > cache_slots).any())
+    assert g["status"][0] == 0 and g["status"][1] == 0
+    assert_same_traversal(g, r)
+    assert np.array_equal(g["ray"], np.repeat(np.arange(o.shape[0]), n))
 
 
 def test_capi_traversal_explicit_jitter_array(oracle, cuda):
