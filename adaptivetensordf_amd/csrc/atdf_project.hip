@@ -1163,7 +1163,7 @@ __device__ __forceinline__ void store_partials(DecGradsRS& q, float* out, int g,
 // samples than that.
 // ---------------------------------------------------------------------------
 #ifndef ATDF_NODE_CHUNK_SAMPLES
-#define ATDF_NODE_CHUNK_SAMPLES 3072
+#define ATDF_NODE_CHUNK_SAMPLES 8192
 #endif
 constexpr int64_t kChunkSamples = ATDF_NODE_CHUNK_SAMPLES;
 constexpr int kMinChunk = 64;     // intervals

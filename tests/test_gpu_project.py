@@ -284,7 +284,7 @@ def test_node_density_stats_match_oracle(oracle, cuda, bf16):
 @pytest.mark.parametrize("bf16", [False, True])
 def test_fused_adjoint_adam_matches_separate(oracle, cuda, bf16):
     """atdf_project_bwd_adam against atdf_project_bwd + atdf_adam_step[_bf16]
-    on a list where some nodes split over chunks (> 3072 samples), some fit
+    on a list where some nodes split over chunks (> 8192 samples), some fit
     one chunk and most see no sample: the same parameters, Adam moments and
     BF16 copy (to the rounding of the LDS accumulation order), grad_grid
     left at zero, the same decoder gradient."""
@@ -310,7 +310,7 @@ def test_fused_adjoint_adam_matches_separate(oracle, cuda, bf16):
     used = counts[act == 1]
     print(f"{iv.num_samples} samples; nodes with samples: {(used > 0).sum()} of {used.size}, max {used.max():.0f}, "
           f"min>0 {used[used > 0].min():.0f}", flush=True)
-    assert (used > 3072 * 2).any() and ((used > 0) & (used < 3072)).any() and (used == 0).any()
+    assert (used > 8192 * 2).any() and ((used > 0) & (used < 8192)).any() and (used == 0).any()
     na = int(act.sum())
     C_ = 16
     master = (0.3 * torch.randn(na, 11, 11, 11, C_, generator=torch.Generator().manual_seed(1))).cuda()
