@@ -97,8 +97,8 @@ struct TraverseArgs {
     // hit count and its first cache_slots hits (slot-major, so a wave's
     // records are contiguous); the write pass writes a ray whose hits all
     // fit from the cache instead of walking the tree again
-    int32_t* cache_n;  // [R]
-    int4* cache;       // [cache_slots][R]: node, tmin bits, tmax bits, samples
+    int32_t* cache_n;  // [R]; more than cache_slots: walk the ray again
+    int4* cache;       // [cache_slots][R]: node, tmin bits, tmax bits, samples | first sample << 12
     int cache_slots;
 };
 
@@ -144,6 +144,11 @@ __device__ __forceinline__ void write_hit(const TraverseArgs& a, int64_t r, cons
     }
 }
 
+// A cache record packs a hit's sample count (12 bits) and its first sample
+// within the ray (20 bits); a ray with a hit beyond either is walked again.
+constexpr int kCacheMaxCount = 1 << 12;
+constexpr int kCacheMaxFirst = 1 << 20;
+
 template <bool kWrite>
 __global__ __launch_bounds__(256) void traverse_kernel(TraverseArgs a)
 {
@@ -172,7 +177,7 @@ __global__ __launch_bounds__(256) void traverse_kernel(TraverseArgs a)
     int nh = 0, ns = 0;
     float last_t = -INFINITY;
     int last_node = -1;
-    bool order_bad = false;
+    bool order_bad = false, cache_miss = false;
     const int64_t ivb = kWrite ? a.iv_off[r] : 0;
     const int64_t sb = kWrite ? a.smp_off[r] : 0;
     if (kWrite && a.cache) {
@@ -180,9 +185,10 @@ __global__ __launch_bounds__(256) void traverse_kernel(TraverseArgs a)
         if (nc <= a.cache_slots) {
             for (int q = 0; q < nc; ++q) {
                 const int4 e = a.cache[(int64_t)q * a.R + r];
-                write_hit(a, r, o, d, e.x, __int_as_float(e.y), __int_as_float(e.z), e.w, ivb, sb, nh, ns);
+                const int n = e.w & (kCacheMaxCount - 1);
+                write_hit(a, r, o, d, e.x, __int_as_float(e.y), __int_as_float(e.z), n, ivb, sb, nh, ns);
                 nh++;
-                ns += e.w;
+                ns += n;
             }
             return;
         }
@@ -199,8 +205,13 @@ __global__ __launch_bounds__(256) void traverse_kernel(TraverseArgs a)
             last_t = tmin;
             last_node = node;
             if (kWrite) write_hit(a, r, o, d, node, tmin, tmax, n, ivb, sb, nh, ns);
-            else if (a.cache && nh < a.cache_slots)
-                a.cache[(int64_t)nh * a.R + r] = make_int4(node, __float_as_int(tmin), __float_as_int(tmax), n);
+            else if (a.cache && nh < a.cache_slots) {
+                if (n < kCacheMaxCount && ns < kCacheMaxFirst)
+                    a.cache[(int64_t)nh * a.R + r] =
+                        make_int4(node, __float_as_int(tmin), __float_as_int(tmax), n | (ns << 12));
+                else
+                    cache_miss = true;
+            }
             nh++;
             ns += n;
         }
@@ -231,10 +242,95 @@ done:
     if (!kWrite) {
         a.ray_num_intervals[r] = nh;
         a.ray_num_samples[r] = ns;
-        if (a.cache) a.cache_n[r] = nh;
+        if (a.cache) a.cache_n[r] = cache_miss ? a.cache_slots + 1 : nh;
         if (nh > a.max_inter) atomicAdd(&a.status[0], 1);
         if (order_bad) atomicAdd(&a.status[1], 1);
         atomicMax(&a.status[2], ns);
+    }
+}
+
+// Write pass of an interval list from the hit cache, coalesced: a wave owns
+// 64 consecutive rays, whose intervals form one contiguous output range, and
+// its lanes write that range in order (lane l the l-th of each 64 outputs),
+// each finding its output's ray by a binary search over the rays' offsets.
+// A ray that was not cached is walked by its own lane afterwards.  Same
+// values as traverse_kernel<true>.
+__global__ __launch_bounds__(256) void traverse_write_cached_kernel(TraverseArgs a)
+{
+    const int lane = threadIdx.x & 63;
+    const int64_t r0 = blockIdx.x * (int64_t)blockDim.x + (threadIdx.x - lane);
+    if (r0 >= a.R) return;  // whole waves
+    const int64_t r = r0 + lane;
+    const bool valid = r < a.R;
+    const int64_t beg = a.iv_off[valid ? r : a.R];
+    const int64_t end = valid ? a.iv_off[r + 1] : beg;
+    const bool selected = valid && !(a.select && a.select[r] == 0);
+    const bool walk = selected && end > beg && a.cache_n[r] > a.cache_slots;
+    const int64_t wbeg = __shfl(beg, 0, 64);
+    const int64_t wend = a.iv_off[r0 + 64 < a.R ? r0 + 64 : a.R];
+    const unsigned long long walkers = __ballot(walk);
+    for (int64_t q0 = wbeg; q0 < wend; q0 += 64) {
+        const int64_t q = q0 + lane;
+        // the last lane whose range starts at or before q (empty ranges
+        // share their start with the next range, so this one holds q)
+        int L = 0;
+#pragma unroll
+        for (int step = 32; step > 0; step >>= 1) {
+            const int64_t b = __shfl(beg, L + step, 64);
+            if (b <= q) L += step;
+        }
+        const int64_t bL = __shfl(beg, L, 64);
+        if (q < wend && !((walkers >> L) & 1ull)) {
+            const int4 e = a.cache[(int64_t)(q - bL) * a.R + r0 + L];
+            a.iv.node[q] = e.x;
+            a.iv.tmin[q] = __int_as_float(e.y);
+            a.iv.tmax[q] = __int_as_float(e.z);
+            a.iv.count[q] = e.w & (kCacheMaxCount - 1);
+            a.iv.first[q] = (int32_t)((uint32_t)e.w >> 12);
+            a.iv.ray[q] = (int32_t)(r0 + L);
+        }
+    }
+    if (!walk) return;
+    // a ray whose hits did not fit the cache: the walk of traverse_kernel
+    float o[3], d[3];
+    for (int k = 0; k < 3; ++k) {
+        o[k] = a.O[r * 3 + k];
+        d[k] = a.D[r * 3 + k];
+    }
+    const bool quad = (a.nodes[0].flags & kFlagQuad) != 0;
+    const int mask = near_child_mask(d, quad);
+    const int last_sibling = quad ? 3 : 7;
+    int nh = 0, ns = 0;
+    int node = 0;
+    while (true) {
+        const PNode nd = a.nodes[node];
+        float tmin, tmax;
+        bool hit = intersect_box_ray(nd.mn, nd.mx, o, d, tmin, tmax);
+        if (hit && (nd.flags & kFlagActive) && !(tmax - tmin < 1e-5)) {
+            const int n = node_sample_count(nd.mn, nd.mx, tmin, tmax, a.K);
+            write_hit(a, r, o, d, node, tmin, tmax, n, beg, 0, nh, ns);
+            nh++;
+            ns += n;
+        }
+        if (hit && (nd.flags & kFlagActiveBelow) && nd.first_child >= 0) {
+            node = nd.first_child + mask;
+            continue;
+        }
+        int cur = node;
+        int parent = nd.parent;
+        int first = nd.first_sib;
+        while (true) {
+            if (parent < 0) return;
+            int k = (cur - first) ^ mask;
+            if (k < last_sibling) {
+                node = first + ((k + 1) ^ mask);
+                break;
+            }
+            cur = parent;
+            const PNode& pn = a.nodes[parent];
+            first = pn.first_sib;
+            parent = pn.parent;
+        }
     }
 }
 
@@ -655,7 +751,10 @@ static atdf_status traverse_write_impl(const void* packed_tree, int64_t num_node
         cache_views(const_cast<void*>(cache), num_rays, a.cache_n, a.cache);
         a.cache_slots = cache_slots;
     }
-    if (a.list && !samples)  // intervals of a sparse selection: eight lanes per listed ray
+    if (a.cache && !samples)  // an interval list from the hit cache, written coalesced
+        hipLaunchKernelGGL(traverse_write_cached_kernel, dim3((unsigned)div_up(num_rays, 256)), dim3(256), 0,
+                           to_hip(stream), a);
+    else if (a.list && !samples)  // intervals of a sparse selection: eight lanes per listed ray
         hipLaunchKernelGGL(traverse_coop_kernel<true>, dim3((unsigned)div_up(num_rays * 8, kCoopThreads)),
                            dim3(kCoopThreads), 0, to_hip(stream), a);
     else

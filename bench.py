@@ -699,6 +699,9 @@ def main():
                                   field_params(N, True, not args.ray_major_forward), grid0, mlp0, True, ps, judge, args,
                                   steps, prop_rays, K_train=K)
     if args.profile_steps > 0:
+        if args.mode == "learned":
+            legs["bf16"] = run_leg(N, "bf16", tree, batches, K, field_params(N, True, not args.ray_major_forward),
+                                   grid0, mlp0, True, None, judge, args, steps)
         # the launches the full run's roofline times: 1 + 5 of project_fwd,
         # then 1 + 5 of project_bwd per leg, last in the run
         for key, leg in legs.items():

@@ -13,16 +13,19 @@ from scene_util import adaptive_active, device_tree, rays_to_device, tilt_rays, 
 pytestmark = pytest.mark.gpu
 
 
-def capi_traverse(tree, o_t, d_t, K, max_inter=256, jitter_mode=0, seed=0, jitter=None, cache_slots=0):
+def capi_traverse(tree, o_t, d_t, K, max_inter=256, jitter_mode=0, seed=0, jitter=None, cache_slots=0,
+                  samples=True, select=None):
     """Count -> scan -> write through the raw C ABI; returns numpy arrays.
     cache_slots > 0: the cached entry points (the write pass replays the
-    count pass's hits of every ray with at most cache_slots of them)."""
+    count pass's hits of every ray with at most cache_slots of them).
+    samples=False: the write pass writes the interval list alone."""
     from adaptivetensordf_amd import capi
 
     dev = o_t.device
     R = o_t.shape[0]
     s = capi.stream_ptr()
-    sp = capi.Sampling(K, max_inter, jitter_mode, 0, seed, None if jitter is None else jitter.data_ptr())
+    sp = capi.Sampling(K, max_inter, jitter_mode, 0, seed, None if jitter is None else jitter.data_ptr(),
+                       None if select is None else select.data_ptr())
     packed = torch.empty(capi.lib().atdf_tree_packed_bytes(tree.NumNodes()), dtype=torch.uint8, device=dev)
     tdesc = capi.Tree(tree.NumNodes(), tree.node_position_min.data_ptr(), tree.node_position_max.data_ptr(),
                       tree.node_children.data_ptr(), tree.node_parent.data_ptr(), tree.node_active.data_ptr())
@@ -57,13 +60,13 @@ def capi_traverse(tree, o_t, d_t, K, max_inter=256, jitter_mode=0, seed=0, jitte
     civ = capi.Intervals(*[iv[k].data_ptr() for k in ("node", "tmin", "tmax", "count", "first", "ray")])
     csm = capi.Samples(*[sm[k].data_ptr() for k in ("global_coordinate", "weight", "ray_t", "ray_index",
                                                     "local_index_in_ray", "node_id")])
+    psm = C.byref(csm) if samples else None
     if H > 0 and cache is not None:
         capi.call("atdf_traverse_write_cached", capi.ptr(packed), tree.NumNodes(), capi.ptr(o_t), capi.ptr(d_t), R,
-                  C.byref(sp), capi.ptr(ivo), capi.ptr(so), C.byref(civ), C.byref(csm), capi.ptr(cache), cache_slots,
-                  s)
+                  C.byref(sp), capi.ptr(ivo), capi.ptr(so), C.byref(civ), psm, capi.ptr(cache), cache_slots, s)
     elif H > 0:  # the write pass needs real buffers; with no hits there is nothing to write
         capi.call("atdf_traverse_write", capi.ptr(packed), tree.NumNodes(), capi.ptr(o_t), capi.ptr(d_t), R,
-                  C.byref(sp), capi.ptr(ivo), capi.ptr(so), C.byref(civ), C.byref(csm), s)
+                  C.byref(sp), capi.ptr(ivo), capi.ptr(so), C.byref(civ), psm, s)
     torch.cuda.synchronize()
     out = {k: v.cpu().numpy() for k, v in {**iv, **sm}.items()}
     out.update(status=status.cpu().numpy(), ray_num_intervals=niv.cpu().numpy(), ray_num_samples=ns.cpu().numpy(),
@@ -108,12 +111,14 @@ def test_capi_traversal_bit_exact(oracle, cuda, case, jitter_mode):
     assert np.array_equal(g["ray"], np.repeat(np.arange(o.shape[0]), r["ray_num_intervals"]))
 
 
+@pytest.mark.parametrize("samples", [True, False])
 @pytest.mark.parametrize("cache_slots", [64, 3, 1])
 @pytest.mark.parametrize("jitter_mode", [0, 2])
-def test_capi_traversal_hit_cache(oracle, cuda, cache_slots, jitter_mode):
+def test_capi_traversal_hit_cache(oracle, cuda, cache_slots, jitter_mode, samples):
     """The cached count / write passes: bit-exact against the oracle whether a
     ray's hits come from the cache (at most cache_slots of them) or from a
-    second walk (more)."""
+    second walk (more); with samples (per-ray replay) and for an interval
+    list alone (the coalesced wave-cooperative write)."""
     t = oracle.tree_build(4)
     act = adaptive_active(t, budget=600)
     tree = device_tree(4, act)
@@ -122,15 +127,44 @@ def test_capi_traversal_hit_cache(oracle, cuda, cache_slots, jitter_mode):
     to_, td = tilt_rays(400, 5)
     o, d = np.concatenate([o, eo, to_]), np.concatenate([d, ed, td])
     print(f"cache_slots={cache_slots} jitter={jitter_mode} R={o.shape[0]}", flush=True)
-    g = capi_traverse(tree, *rays_to_device(o, d), 6, jitter_mode=jitter_mode, seed=7, cache_slots=cache_slots)
+    g = capi_traverse(tree, *rays_to_device(o, d), 6, jitter_mode=jitter_mode, seed=7, cache_slots=cache_slots,
+                      samples=samples)
     r = oracle.traverse(t["node_min"], t["node_max"], oracle.active_ids(act), o, d, 6, jitter_mode=jitter_mode,
                         seed=7)
     n = r["ray_num_intervals"]
     # both paths are exercised: rays within the slots and rays beyond them
     assert (n <= cache_slots).any() and (cache_slots == 64 or (n > cache_slots).any())
     assert g["status"][0] == 0 and g["status"][1] == 0
-    assert_same_traversal(g, r)
+    if samples:
+        assert_same_traversal(g, r)
+    else:
+        assert np.array_equal(g["ray_interval_offset"], r["ray_interval_offset"])
+        assert np.array_equal(g["node"], r["iv_node"])
+        assert np.array_equal(g["count"], r["iv_count"])
+        assert np.array_equal(g["first"], r["iv_first"])
+        assert np.array_equal(g["tmin"].view(np.uint32), r["iv_tmin"].view(np.uint32))
+        assert np.array_equal(g["tmax"].view(np.uint32), r["iv_tmax"].view(np.uint32))
     assert np.array_equal(g["ray"], np.repeat(np.arange(o.shape[0]), n))
+
+
+def test_capi_traversal_hit_cache_with_selection(oracle, cuda):
+    """Rays switched off by ray_select (the proposal's fallback mask) get no
+    intervals on the cached path either: cached and uncached interval lists
+    are identical."""
+    t = oracle.tree_build(4)
+    act = adaptive_active(t, budget=600)
+    tree = device_tree(4, act)
+    o, d = random_rays(1500, 29)
+    sel = torch.from_numpy((np.random.default_rng(3).random(o.shape[0]) < 0.6).astype(np.int32)).cuda()
+    args = (tree, *rays_to_device(o, d), 6)
+    a = capi_traverse(*args, jitter_mode=2, seed=1, samples=False, select=sel)
+    for slots in (64, 2):
+        b = capi_traverse(*args, jitter_mode=2, seed=1, samples=False, select=sel, cache_slots=slots)
+        for k in ("ray_interval_offset", "node", "count", "first", "ray"):
+            assert np.array_equal(a[k], b[k]), (slots, k)
+        for k in ("tmin", "tmax"):
+            assert np.array_equal(a[k].view(np.uint32), b[k].view(np.uint32)), (slots, k)
+    assert (a["ray_num_intervals"][sel.cpu().numpy() == 0] == 0).all()
 
 
 def test_capi_traversal_explicit_jitter_array(oracle, cuda):
