@@ -356,8 +356,18 @@ def run_leg(N, name, tree, batches, K, fp, grid0, mlp0, bf16, ps, judge, args, s
     t_start = elapsed = None
     in_window_update_ms = 0.0
     for i in range(total):
-        if curve is not None and i % 100 == 0:
+        # judged every 100 iterations, and every 25 over the last 500 (the
+        # points the equal-MSE comparison takes its median over); a judgement
+        # inside the timed steps is taken out of their time
+        if curve is not None and (i % 100 == 0 or (i >= total - 500 and i % 25 == 0)):
+            timed = args.warmup <= i < args.warmup + steps
+            if timed:
+                torch.cuda.synchronize()
+                tj = time.perf_counter()
             curve.append((i, judge(st, fp)))
+            if timed:
+                torch.cuda.synchronize()
+                in_window_update_ms += (time.perf_counter() - tj) * 1e3
         if i == args.warmup:
             torch.cuda.synchronize()
             first_loss = float(st.loss.item()) if i > 0 else float("nan")
@@ -742,9 +752,10 @@ def main():
     mse = None
     if args.mode == "learned":
         final = {k: leg["mse_curve"][-1][1] for k, leg in legs.items()}
-        # the median of the last five judge points (iterations 1100..1500 by
+        # the median of the judge points over the last 500 iterations (21 by
         # default): single points carry transient spikes of the training noise
-        med5 = {k: statistics.median(v for _, v in leg["mse_curve"][-5:]) for k, leg in legs.items()}
+        med5 = {k: statistics.median(v for it, v in leg["mse_curve"] if it >= leg["iterations"] - 500)
+                for k, leg in legs.items()}
         sp_own = N.SamplingParams()
         sp_own.max_samples_per_node = legs["learned"]["K"]
         ivl, fb = N.CreateProposalIntervals(tree, ps, ev[0], sp_own)
@@ -754,7 +765,7 @@ def main():
             "judge": f"stratified K={K} (~{s_eff:.0f} samples/ray, midpoints) on {ev[0].size()} held-out rays vs their "
                      f"targets (noisy-volume line integrals), every 100 iterations of each leg",
             "final": final,
-            "median_last5": med5,
+            "median_last500": med5,
             "curve": {k: leg["mse_curve"] for k, leg in legs.items()},
             "ratio_learned_over_plain": med5["learned"] / med5["plain"],
             "ratio_learned_over_plain_final_point": final["learned"] / final["plain"],
@@ -765,14 +776,24 @@ def main():
             "learned_own_sampler_samples_per_ray": ivl.num_samples / ev[0].size(),
             "learned_own_sampler_fallback_frac": float(fb.float().mean().item()),
             # legs whose field is as good as the plain leg's (median of the
-            # last five judge points within 1 %, or below), with their speed
+            # judge points of the last 500 iterations within 1 %, or below), with their speed
             "equal_mse": {k: {"mse_over_plain": med5[k] / med5["plain"],
                               "speedup_over_plain": legs[k]["rays_per_s"] / legs["plain"]["rays_per_s"]}
                           for k in legs if k != "plain" and med5[k] / med5["plain"] <= 1.01},
         }
-        log(f"held-out MSE (median of the last 5 judge points): " + ", ".join(f"{k} {v:.4e}" for k, v in med5.items())
-            + f"; learned/plain {mse['ratio_learned_over_plain']:.4f}, learned/strat32 "
-              f"{mse['ratio_learned_over_strat32']:.4f}")
+        eq = mse["equal_mse"]
+        best = max(eq, key=lambda k: eq[k]["speedup_over_plain"]) if eq else None
+        r_ls = mse["ratio_learned_over_strat32"]
+        mse["finding"] = (
+            (f"at equal projection MSE (within 1 %) the fastest leg is {best}, "
+             f"{eq[best]['speedup_over_plain']:.2f}x the plain FP32 leg's rays/s" if best
+             else "no leg reaches the plain FP32 leg's projection MSE within 1 %")
+            + f"; the learned leg's MSE is {mse['ratio_learned_over_plain']:.2f}x the plain leg's and {r_ls:.3f}x "
+              f"that of BF16 stratified sampling at the same sample count ("
+            + ("the learned allocation gains over stratified sampling)" if r_ls < 0.99
+               else "the learned allocation gains nothing over stratified sampling)"))
+        log(f"held-out MSE (median of the judge points of the last 500 iterations): "
+            + ", ".join(f"{k} {v:.4e}" for k, v in med5.items()) + f"; {mse['finding']}")
 
     cpu = None
     if not args.no_cpu_baseline:
