@@ -544,7 +544,7 @@ def measured_traffic(mode, kernel):
     WRITE_SIZE, copy-calibrated), or None when that combination was not
     measured."""
     path = os.path.join(HERE, "profiles", "traffic_512cube.json")
-    if mode not in ("plain", "learned") or not os.path.exists(path):
+    if mode not in ("plain", "learned", "bf16") or not os.path.exists(path):
         return None
     with open(path) as fh:
         data = json.load(fh)
