@@ -462,58 +462,103 @@ __global__ __launch_bounds__(kThreads) void alloc_kernel(PropArgs a, const float
     float* buf = sm + (threadIdx.x >> 6) * kActFloats;
     const int lane = threadIdx.x & 63;
     const int64_t r = blockIdx.x * (int64_t)kThreads + threadIdx.x;
-    if (r >= a.R) return;  // no barrier below
-    const PNode root = a.nodes[0];
-    float o[3], d[3], t0, t1, kl;
+    const int64_t r0 = r - lane;  // the wave's first ray
+    if (r0 >= a.R) return;        // whole waves: the interval pass below needs every lane
+    const bool valid = r < a.R;
+    float t0 = 0.f, t1 = 0.f;
+    bool learn = false;
+    if (valid) {
+        const PNode root = a.nodes[0];
+        float o[3], d[3], kl;
 #pragma unroll
-    for (int k = 0; k < 3; ++k) {
-        o[k] = a.O[r * 3 + k];
-        d[k] = a.D[r * 3 + k];
-    }
-    float pm[NB];
-    const bool ok = ray_pdf(a, params, buf, lane, root, o, d, pm, t0, t1, kl);
-    const bool fall = ok && kl > a.kl_threshold;
-    a.fallback[r] = fall ? 1 : 0;
-    if (a.kl) a.kl[r] = kl;
-    // a gated ray allocates along its mixed pdf, a fallback ray along the
-    // uniform one (stratified sampling at the same budget of extras)
-    const bool learn = ok;
-    if (learn) {
-        float c = 0.f;
-#pragma unroll
-        for (int b = 0; b < NB; ++b) {
-            const float p = fall ? 1.f / (float)NB : pm[b];
-            buf[b * 64 + lane] = c;
-            buf[(NB + b) * 64 + lane] = p;
-            c = c + p;
+        for (int k = 0; k < 3; ++k) {
+            o[k] = a.O[r * 3 + k];
+            d[k] = a.D[r * 3 + k];
         }
-    }
-    const float len = (t1 - t0) / (float)NB;
-    const float E = (float)extra;
-    auto cdf = [&](float t) {
-        int b = (int)floorf((t - t0) / len);
-        b = b < 0 ? 0 : (b > NB - 1 ? NB - 1 : b);
-        float F = buf[b * 64 + lane] + buf[(NB + b) * 64 + lane] * ((t - bin_edge(t0, t1, len, b)) / len);
-        if (!(F >= 0.f)) F = 0.f;
-        if (!(F <= 1.f)) F = 1.f;
-        return F;
-    };
-    const int64_t h0 = iv_off[r], h1 = iv_off[r + 1];
-    int s = 0;
-    for (int64_t h = h0; h < h1; ++h) {
-        int n;
+        float pm[NB];
+        const bool ok = ray_pdf(a, params, buf, lane, root, o, d, pm, t0, t1, kl);
+        const bool fall = ok && kl > a.kl_threshold;
+        a.fallback[r] = fall ? 1 : 0;
+        if (a.kl) a.kl[r] = kl;
+        // a gated ray allocates along its mixed pdf, a fallback ray along the
+        // uniform one (stratified sampling at the same budget of extras); the
+        // CDF table goes to the ray's column of the wave's buffer
+        learn = ok;
         if (learn) {
-            const int ka = (int)floorf(E * cdf(iv_tmin[h]) + 0.5f);
-            const int kb = (int)floorf(E * cdf(iv_tmax[h]) + 0.5f);
-            n = iv_count[h] + (kb > ka ? kb - ka : 0);
-            iv_count[h] = n;
-        } else {
-            n = iv_count[h];
+            float c = 0.f;
+#pragma unroll
+            for (int b = 0; b < NB; ++b) {
+                const float p = fall ? 1.f / (float)NB : pm[b];
+                buf[b * 64 + lane] = c;
+                buf[(NB + b) * 64 + lane] = p;
+                c = c + p;
+            }
         }
-        iv_first[h] = s;
-        s += n;
     }
-    if (ray_ns) ray_ns[r] = s;
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    // The interval pass: the wave's 64 rays own one contiguous range of the
+    // interval list; lane l takes the l-th of each 64 intervals (coalesced),
+    // finds its ray L by a binary search over the rays' first intervals, and
+    // adds ray L's extras round(E F(tmax)) - round(E F(tmin)) with ray L's
+    // CDF column.  first = the exclusive sum of the counts along the ray: a
+    // segmented scan over the 64 intervals, carried into the next 64 for the
+    // ray that continues.
+    const float E = (float)extra;
+    const int64_t h0 = iv_off[valid ? r : a.R];
+    if (valid && ray_ns && iv_off[r + 1] == h0) ray_ns[r] = 0;
+    const int64_t wbeg = __shfl(h0, 0, 64);
+    const int64_t wend = iv_off[r0 + 64 < a.R ? r0 + 64 : a.R];
+    int carry = 0;
+    for (int64_t q0 = wbeg; q0 < wend; q0 += 64) {
+        const int64_t q = q0 + lane;
+        int L = 0;
+#pragma unroll
+        for (int step = 32; step > 0; step >>= 1)
+            if (__shfl(h0, L + step, 64) <= q) L += step;
+        const int64_t hL = __shfl(h0, L, 64);
+        // end of ray L's intervals (every lane takes part in the shuffle: a
+        // lane reading from an inactive lane would get 0)
+        const int64_t hN = __shfl(h0, L < 63 ? L + 1 : 63, 64);
+        const int64_t hE = L < 63 ? hN : wend;
+        const float t0L = __shfl(t0, L, 64), t1L = __shfl(t1, L, 64);
+        const int learnL = __shfl(learn ? 1 : 0, L, 64);
+        const bool act = q < wend;
+        int n = 0;
+        if (act) {
+            n = iv_count[q];
+            if (learnL) {
+                const float len = (t1L - t0L) / (float)NB;
+                auto cdf = [&](float t) {
+                    int b = (int)floorf((t - t0L) / len);
+                    b = b < 0 ? 0 : (b > NB - 1 ? NB - 1 : b);
+                    float F = buf[b * 64 + L] + buf[(NB + b) * 64 + L] * ((t - bin_edge(t0L, t1L, len, b)) / len);
+                    if (!(F >= 0.f)) F = 0.f;
+                    if (!(F <= 1.f)) F = 1.f;
+                    return F;
+                };
+                const int ka = (int)floorf(E * cdf(iv_tmin[q]) + 0.5f);
+                const int kb = (int)floorf(E * cdf(iv_tmax[q]) + 0.5f);
+                n += kb > ka ? kb - ka : 0;
+                iv_count[q] = n;
+            }
+        }
+        int x = n, f = (act && q == hL) ? 1 : 0;
+#pragma unroll
+        for (int off = 1; off < 64; off <<= 1) {
+            const int up = __shfl_up(x, off, 64), fu = __shfl_up(f, off, 64);
+            if (lane >= off) {
+                if (!f) x += up;
+                f |= fu;
+            }
+        }
+        if (!f) x += carry;  // the ray continuing from the previous 64 intervals
+        if (act) {
+            iv_first[q] = x - n;
+            if (ray_ns && q == hE - 1) ray_ns[r0 + L] = x;
+        }
+        carry = __shfl(x, 63, 64);
+    }
 }
 
 __global__ __launch_bounds__(256) void merge_counts_kernel(int64_t R, const int32_t* __restrict__ fallback,
