@@ -2840,7 +2840,7 @@ __global__ __launch_bounds__(64 * kNW, kLDS ? 1 : 4) void project_fwd_node_kerne
 // items).  ray_sum_kernel adds a ray's interval sums in list order.
 constexpr int kFwdPartWaves = 4;
 
-template <int C, bool kBF16, bool kDecBF16>
+template <int C, bool kBF16, bool kDecBF16, bool kVM = false>
 __global__ __launch_bounds__(64 * kFwdPartWaves, 4) void project_fwd_part_kernel(NodeFwdArgs b)
 {
     using SM = DecoderSmem<C, kDecBF16>;
@@ -2869,7 +2869,11 @@ __global__ __launch_bounds__(64 * kFwdPartWaves, 4) void project_fwd_part_kernel
         if (item >= n_items) break;
         const int4 it = b.parts[item];
         const PNode nd = a.nodes[a.iv_node[b.sorted[it.y]]];
-        const int64_t gbase = (int64_t)it.x * gsz;
+        // a tensorial field's in-ROI items follow the explicit-grid ones
+        // (slot = explicit grids + plane slot)
+        const int pslot = kVM ? it.x - a.vm.NG : -1;
+        const bool vmc = kVM && pslot >= 0;
+        const int64_t gbase = vmc ? 0 : (int64_t)it.x * gsz;
         const int ib = it.y, ie = it.z;
         const int tot = (int)(b.soff[ie] - b.soff[ib]);
         TileCursor tc;
@@ -2890,18 +2894,33 @@ __global__ __launch_bounds__(64 * kFwdPartWaves, 4) void project_fwd_part_kernel
                 SampleCtx sc;
                 sc.valid = m.valid;
                 sc.w = 0.f;
-                if (m.valid) {
-                    const float o[3] = {m.B.x, m.B.y, m.B.z};
-                    const float d[3] = {m.B.w, m.D.x, m.D.y};
-                    make_sample_node(a, nd, gbase, __float_as_int(m.D.w), o, d, m.A.x, m.A.y, m.n_s, m.jj,
-                                     __float_as_int(m.A.w) + m.jj, sc);
-                    packed = sc.tri.x0 | (sc.tri.x1 << 5) | (sc.tri.y0 << 10) | (sc.tri.y1 << 15) |
-                             (sc.tri.z0 << 20) | (sc.tri.z1 << 25);
-                    iv_id = __float_as_int(m.D.z);
+                if (kVM && vmc) {
+                    // an in-ROI sample keeps its local and global coordinates
+                    sc.loc[0] = sc.loc[1] = sc.loc[2] = sc.gx = sc.gz = 0.f;
+                    if (m.valid) {
+                        const float o[3] = {m.B.x, m.B.y, m.B.z};
+                        const float d[3] = {m.B.w, m.D.x, m.D.y};
+                        make_sample_node<true>(a, nd, 0, __float_as_int(m.D.w), o, d, m.A.x, m.A.y, m.n_s, m.jj,
+                                               __float_as_int(m.A.w) + m.jj, sc, false);
+                        iv_id = __float_as_int(m.D.z);
+                    }
+                    pl[lane * 3] = make_float4(sc.loc[0], sc.loc[1], sc.loc[2], sc.gx);
+                    pl[lane * 3 + 1] = make_float4(sc.gz, m.valid ? 1.f : 0.f, 0.f, 0.f);
+                    pl[lane * 3 + 2] = make_float4(0.f, sc.w, __int_as_float(iv_id), 0.f);
+                } else {
+                    if (m.valid) {
+                        const float o[3] = {m.B.x, m.B.y, m.B.z};
+                        const float d[3] = {m.B.w, m.D.x, m.D.y};
+                        make_sample_node(a, nd, gbase, __float_as_int(m.D.w), o, d, m.A.x, m.A.y, m.n_s, m.jj,
+                                         __float_as_int(m.A.w) + m.jj, sc);
+                        packed = sc.tri.x0 | (sc.tri.x1 << 5) | (sc.tri.y0 << 10) | (sc.tri.y1 << 15) |
+                                 (sc.tri.z0 << 20) | (sc.tri.z1 << 25);
+                        iv_id = __float_as_int(m.D.z);
+                    }
+                    pl[lane * 3] = make_float4(__int_as_float(packed), sc.tri.w[0], sc.tri.w[1], sc.tri.w[2]);
+                    pl[lane * 3 + 1] = make_float4(sc.tri.w[3], sc.tri.w[4], sc.tri.w[5], sc.tri.w[6]);
+                    pl[lane * 3 + 2] = make_float4(sc.tri.w[7], sc.w, __int_as_float(iv_id), 0.f);
                 }
-                pl[lane * 3] = make_float4(__int_as_float(packed), sc.tri.w[0], sc.tri.w[1], sc.tri.w[2]);
-                pl[lane * 3 + 1] = make_float4(sc.tri.w[3], sc.tri.w[4], sc.tri.w[5], sc.tri.w[6]);
-                pl[lane * 3 + 2] = make_float4(sc.tri.w[7], sc.w, __int_as_float(iv_id), 0.f);
             }
             wave_lds_sync();
             const int ntile = min(4, (tot - base + kTile - 1) / kTile);
@@ -2911,7 +2930,7 @@ __global__ __launch_bounds__(64 * kFwdPartWaves, 4) void project_fwd_part_kernel
                 const float4 p0 = pl[e * 3], p1 = pl[e * 3 + 1], p2 = pl[e * 3 + 2];
                 const int packed = __float_as_int(p0.x);
                 SampleCtx sc;
-                sc.valid = packed >= 0;
+                sc.valid = (kVM && vmc) ? p1.y != 0.f : packed >= 0;
                 sc.tri.x0 = packed & 31;
                 sc.tri.x1 = (packed >> 5) & 31;
                 sc.tri.y0 = (packed >> 10) & 31;
@@ -2929,7 +2948,26 @@ __global__ __launch_bounds__(64 * kFwdPartWaves, 4) void project_fwd_part_kernel
                 sc.gbase = gbase;
                 const int iv_id = __float_as_int(p2.z);
                 float x[CPL];
-                if (sc.valid) {
+                if (kVM && vmc) {
+#pragma unroll
+                    for (int qq = 0; qq < CPL; ++qq) x[qq] = 0.f;
+                    if (sc.valid) {
+                        sc.loc[0] = p0.x;
+                        sc.loc[1] = p0.y;
+                        sc.loc[2] = p0.z;
+                        sc.gx = p0.w;
+                        sc.gz = p1.x;
+                        // F = Pyz Lx + Pxz Ly + Pxy Lz, one pair at a time
+#pragma unroll
+                        for (int pp = 0; pp < 3; ++pp) {
+                            float Pv[CPL], Lv[CPL];
+                            vm_pair_read<C, CPL>(a, pslot, sc, pp, CPL * g, Pv, Lv);
+#pragma unroll
+                            for (int qq = 0; qq < CPL; ++qq)
+                                x[qq] = pp == 0 ? Pv[qq] * Lv[qq] : x[qq] + Pv[qq] * Lv[qq];
+                        }
+                    }
+                } else if (sc.valid) {
                     Corners<C, kBF16> cr;
                     gather_corners<C, kBF16>(a, sc, g, cr);
                     corners_combine<C, kBF16>(sc, cr, x);
@@ -4219,7 +4257,11 @@ static atdf_status fwd_node_impl(const Prepared& P, const void* packed_tree, con
         args.max_chunks = max_chunks_of(P.H, P.NA);
         dim3 grid_p((unsigned)persistent_blocks() * 4), block_p(64 * kFwdPartWaves);
 #define ATDF_FP(CC, BF, BD) hipLaunchKernelGGL((project_fwd_part_kernel<CC, BF, BD>), grid_p, block_p, 0, s, args)
-        if (f->C == 16) {
+#define ATDF_FPV(CC, BD) hipLaunchKernelGGL((project_fwd_part_kernel<CC, false, BD, true>), grid_p, block_p, 0, s, args)
+        if (f->vm) {  // explicit grids are f32 beside a tensorial field
+            if (f->C == 16) { if (bd) ATDF_FPV(16, true); else ATDF_FPV(16, false); }
+            else            { if (bd) ATDF_FPV(8, true); else ATDF_FPV(8, false); }
+        } else if (f->C == 16) {
             if (bf) { if (bd) ATDF_FP(16, true, true); else ATDF_FP(16, true, false); }
             else    { if (bd) ATDF_FP(16, false, true); else ATDF_FP(16, false, false); }
         } else {
@@ -4227,6 +4269,7 @@ static atdf_status fwd_node_impl(const Prepared& P, const void* packed_tree, con
             else    { if (bd) ATDF_FP(8, false, true); else ATDF_FP(8, false, false); }
         }
 #undef ATDF_FP
+#undef ATDF_FPV
         hipLaunchKernelGGL(ray_sum_kernel, dim3((unsigned)div_up(R, 256)), dim3(256), 0, s, iv_off, iv->count,
                            args.ivsum, R, out);
         return check_launch("atdf_project_fwd_prepared");
@@ -4429,12 +4472,13 @@ atdf_status atdf_project_fwd_prepared(const void* packed_tree, int64_t num_nodes
 {
     atdf_status st = check_project_args(packed_tree, num_nodes, O, D, R, sp, iv_off, iv, f);
     if (st) return st;
-    ATDF_REQUIRE(!f->vm, "atdf_project_fwd_prepared: a tensorial field takes atdf_project_fwd");
     if (R == 0) return ATDF_OK;
     ATDF_REQUIRE(P_out, "atdf_project_fwd_prepared: null projection");
     Prepared P;
     st = prepared_view(iv, f, workspace, workspace_bytes, R, P, "atdf_project_fwd_prepared");
     if (st) return st;
+    if (f->vm && !(P.staging && P.H > 0))  // the node-major tensorial forward needs the part table
+        return atdf_project_fwd(packed_tree, num_nodes, O, D, R, sp, iv_off, iv, f, P_out, stream);
     return fwd_node_impl(P, packed_tree, O, D, R, sp, iv_off, iv, f, P_out, stream);
 }
 

@@ -306,25 +306,42 @@ Tensor TrainStepVM(HyperTreeBase& tree, const RoiMaps& roi, VMTrainState& st, co
     const int64_t* ivo = iv.ray_offset.data_ptr<int64_t>();
     const float *O = rays.origin.data_ptr<float>(), *D = rays.direction.data_ptr<float>();
     auto P = torch::empty({R}, rays.origin.options());
-    check_status(atdf_project_fwd(tree.packed(), tree.NumNodes(), O, D, R, &s, ivo, &av, &v.f, P.data_ptr<float>(),
-                                  stream),
-                 "atdf_project_fwd");
-    auto dP = torch::empty({R}, rays.origin.options());
-    auto mse_ws = torch::empty({(int64_t)atdf_mse_workspace_bytes(R)},
-                               torch::TensorOptions().dtype(torch::kUInt8).device(rays.origin.device()));
-    check_status(atdf_mse_loss(P.data_ptr<float>(), target.data_ptr<float>(), R, 1.0f, dP.data_ptr<float>(),
-                               st.loss.data_ptr<double>(), mse_ws.data_ptr(), mse_ws.numel(), stream),
-                 "atdf_mse_loss");
     // the gradients are zero here: MakeVMTrainState allocates them zeroed and
     // each Adam pass below clears its gradient (zero_grad)
     v.vm.grad_planes = st.grads[0].data_ptr<float>();
     v.vm.grad_lines = st.grads[1].data_ptr<float>();
     auto ws = torch::empty({vm_workspace_bytes(R, iv, v.f, fp.two_pass_adjoint)},
                            torch::TensorOptions().dtype(torch::kUInt8).device(rays.origin.device()));
-    check_status(atdf_project_bwd(tree.packed(), tree.NumNodes(), O, D, R, &s, ivo, &av, &v.f, dP.data_ptr<float>(),
-                                  st.grid.size(0) > 0 ? st.grads[2].data_ptr<float>() : nullptr,
-                                  st.grads[3].data_ptr<float>(), ws.data_ptr(), ws.numel(), stream),
-                 "atdf_project_bwd");
+    if (fp.node_forward) {
+        // one counting sort for both directions; node-major forward over it
+        check_status(atdf_project_prepare(tree.packed(), tree.NumNodes(), O, D, R, &s, ivo, &av, &v.f, ws.data_ptr(),
+                                          ws.numel(), stream),
+                     "atdf_project_prepare");
+        check_status(atdf_project_fwd_prepared(tree.packed(), tree.NumNodes(), O, D, R, &s, ivo, &av, &v.f,
+                                               P.data_ptr<float>(), ws.data_ptr(), ws.numel(), stream),
+                     "atdf_project_fwd_prepared");
+    } else {
+        check_status(atdf_project_fwd(tree.packed(), tree.NumNodes(), O, D, R, &s, ivo, &av, &v.f,
+                                      P.data_ptr<float>(), stream),
+                     "atdf_project_fwd");
+    }
+    auto dP = torch::empty({R}, rays.origin.options());
+    auto mse_ws = torch::empty({(int64_t)atdf_mse_workspace_bytes(R)},
+                               torch::TensorOptions().dtype(torch::kUInt8).device(rays.origin.device()));
+    check_status(atdf_mse_loss(P.data_ptr<float>(), target.data_ptr<float>(), R, 1.0f, dP.data_ptr<float>(),
+                               st.loss.data_ptr<double>(), mse_ws.data_ptr(), mse_ws.numel(), stream),
+                 "atdf_mse_loss");
+    float* gg = st.grid.size(0) > 0 ? st.grads[2].data_ptr<float>() : nullptr;
+    if (fp.node_forward)
+        check_status(atdf_project_bwd_prepared(tree.packed(), tree.NumNodes(), O, D, R, &s, ivo, &av, &v.f,
+                                               dP.data_ptr<float>(), gg, st.grads[3].data_ptr<float>(), nullptr,
+                                               ws.data_ptr(), ws.numel(), stream),
+                     "atdf_project_bwd_prepared");
+    else
+        check_status(atdf_project_bwd(tree.packed(), tree.NumNodes(), O, D, R, &s, ivo, &av, &v.f,
+                                      dP.data_ptr<float>(), gg, st.grads[3].data_ptr<float>(), ws.data_ptr(),
+                                      ws.numel(), stream),
+                     "atdf_project_bwd");
     st.step += 1;
     Tensor* params[4] = {&st.planes, &st.lines, &st.grid, &st.mlp};
     for (int i = 0; i < 4; ++i) {

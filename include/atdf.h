@@ -435,8 +435,12 @@ atdf_status atdf_project_bwd_adam(const void* packed_tree, int64_t num_nodes, co
  * atdf_project_bwd_prepared is atdf_project_bwd_adam (adam non-null) or
  * atdf_project_bwd (adam null) without sorting again.  The three calls must
  * see the same rays, intervals and slot map; the grid and decoder may
- * change in between.  No reference counterpart: the reference gathers
- * ray-major in both directions (geometry/geometry.cpp:828-914). */
+ * change in between.  A tensorial field (field->vm) takes the same calls:
+ * its in-ROI items are integrated node-major too when the workspace holds
+ * the two-pass staging region (atdf_project_bwd_staging_bytes), and
+ * atdf_project_fwd_prepared falls back to atdf_project_fwd without it.  No
+ * reference counterpart: the reference gathers ray-major in both directions
+ * (geometry/geometry.cpp:828-914). */
 atdf_status atdf_project_prepare(const void* packed_tree, int64_t num_nodes, const float* ray_origin,
                                  const float* ray_direction, int64_t num_rays, const atdf_sampling* sampling,
                                  const int64_t* ray_interval_offset, const atdf_intervals* intervals,
