@@ -3013,17 +3013,36 @@ __global__ __launch_bounds__(64 * kFwdPartWaves, 4) void project_fwd_part_kernel
     }
 }
 
+// A wave's 64 rays own one contiguous range of interval sums: windows of it
+// are loaded coalesced into LDS, and each lane adds its own ray's sums from
+// there in list order (the order of a per-ray loop over global memory, so the
+// result is the same bit for bit; intervals without samples hold no sum and
+// add nothing: the sums are non-negative, so adding +0 changes none).
+constexpr int kRaySumWin = 512;
+
 __global__ __launch_bounds__(256) void ray_sum_kernel(const int64_t* __restrict__ iv_off,
                                                       const int32_t* __restrict__ count,
                                                       const float* __restrict__ ivsum, int64_t R,
                                                       float* __restrict__ P)
 {
+    __shared__ float s_v[4][kRaySumWin];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    if (r >= R) return;
+    const int64_t r0 = r - lane;
+    if (r0 >= R) return;  // whole waves
+    const bool valid = r < R;
+    const int64_t hb = valid ? iv_off[r] : 0, he = valid ? iv_off[r + 1] : 0;
+    const int64_t wb = iv_off[r0], we = iv_off[r0 + 64 < R ? r0 + 64 : R];
     float acc = 0.f;
-    for (int64_t h = iv_off[r]; h < iv_off[r + 1]; ++h)
-        if (count[h] > 0) acc += ivsum[h];
-    P[r] = acc;
+    for (int64_t x0 = wb; x0 < we; x0 += kRaySumWin) {
+        const int64_t x1 = x0 + kRaySumWin < we ? x0 + kRaySumWin : we;
+        for (int64_t h = x0 + lane; h < x1; h += 64) s_v[w][h - x0] = count[h] > 0 ? ivsum[h] : 0.f;
+        wave_lds_sync();
+        const int64_t a = hb > x0 ? hb : x0, b = he < x1 ? he : x1;
+        for (int64_t h = a; h < b; ++h) acc += s_v[w][h - x0];
+        wave_lds_sync();
+    }
+    if (valid) P[r] = acc;
 }
 
 // ---- counting sort of intervals by grid slot, chunk table ----
